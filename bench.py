@@ -1,0 +1,196 @@
+"""Benchmark: MG-PCG time to a 1e-8 relative residual and CSR SpMV HBM GB/s on
+the 3D Q1 Poisson problem, level 7 (128^3 cells, 2,146,689 dofs, 57,066,625
+nonzeros), on one MI355X.
+
+One step = one complete PCG(MultiGrid-V(4,4), Richardson(Jacobi 0.5)) solve of
+A x = b from x = 0 to |r| <= 1e-8 |r0| (coarse level 1, exact coarse solve),
+inputs resident in HBM.  `value` is the time to solution in seconds (lower is
+better).  The roofline object is for the CSR SpMV kernel r = A*x on the same
+matrix, timed with HIP events on the stream it is launched on.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+"""
+import argparse
+import ctypes as C
+import importlib.util
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+DIM, LF, LC = 3, 7, 1
+SEED = 28054777172512
+PEAK_HBM_GBS = 8000.0
+
+
+def load_feat():
+    spec = importlib.util.spec_from_file_location(
+        "feat3amd", os.path.join(ROOT, "feat3.doxytest_amd", "__init__.py"))
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["feat3amd"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def spmv_bytes(n, nnz, idx_bytes=8, val_bytes=8):
+    """Algorithmic bytes of r = A*x: val + col + row_ptr + x + r, each once."""
+    return nnz * (val_bytes + idx_bytes) + (n + 1) * idx_bytes + n * val_bytes + n * val_bytes
+
+
+def cpu_baseline(feat, b, iters_gpu):
+    """The reference's own generic code (oracle/_ref) on one host core, on the
+    same matrices; falls back to the C oracle where _ref is absent."""
+    from oracle.pyoracle import Oracle, Reference, LIB_REF, Csr, csr_transpose, COARSE_JACOBI, COARSE_DIRECT
+    levels = []
+    for lvl in range(LF, LC - 1, -1):
+        q = feat.q1_problem(DIM, lvl)
+        lv = dict(A=Csr(q["n"], q["n"], q["val"], q["col"], q["rowptr"]), filt=q["filt"])
+        if lvl > LC:
+            P = Csr(q["n"], q["pcols"], q["pval"], q["pcol"], q["prowptr"])
+            lv["P"], lv["R"] = P, csr_transpose(P)
+        levels.append(lv)
+    n = levels[0]["A"].rows
+    if os.path.exists(LIB_REF):
+        kind = "reference"
+        t0 = time.perf_counter()
+        r = Reference().mg_pcg_solve(levels, np.zeros(n), b, coarse=COARSE_JACOBI)
+        secs = r["seconds"] if r.get("seconds") else time.perf_counter() - t0
+    else:
+        kind = "port"
+        t0 = time.perf_counter()
+        r = Oracle().mg_pcg_solve(levels, np.zeros(n), b, coarse=COARSE_DIRECT)
+        secs = time.perf_counter() - t0
+    return dict(value=secs, unit="s", cores=1, kind=kind,
+                sample=f"one full solve of the same workload ({r['iters']} PCG iterations, "
+                       f"GPU took {iters_gpu}), single thread, {platform.processor() or 'x86_64'} "
+                       f"host of {os.cpu_count()} logical CPUs",
+                iters=r["iters"])
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--spmv-reps", type=int, default=50)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
+              "domain decomposition is design-only in this build.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    feat = load_feat()
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    stream = torch.cuda.current_stream()
+
+    # ---- setup (host assembly + upload; not timed)
+    t0 = time.perf_counter()
+    cfg = feat.mgpcg_cfg(nu_pre=4, nu_post=4, omega_jac=0.5, omega_rich=1.0, coarse="direct",
+                         tol_rel=1e-8, max_iter=100)
+    solver = feat.MgPcg(cfg, q1=(DIM, LF, LC))
+    A = solver.matrix(0)
+    n, nnz = int(A.rows), int(A.nnz)
+    b_host = feat.rhs_random(SEED, n, -1.0, 1.0)
+    m = 2 ** LF + 1
+    idx = np.arange(n)
+    c = np.stack([idx % m, (idx // m) % m, idx // (m * m)])
+    bnd = np.any((c == 0) | (c == m - 1), axis=0)
+    b_host[bnd] = 0.0                        # filter_rhs of the homogeneous Dirichlet filter
+    b = torch.from_numpy(b_host).to(dev)
+    x = torch.zeros(n, dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+    log(f"setup {time.perf_counter() - t0:.1f} s: n={n} nnz={nnz}")
+
+    # ---- MG-PCG time to solution
+    res = None
+    for _ in range(args.warmup):
+        x.zero_()
+        torch.cuda.synchronize()
+        res = solver.solve(x, b)
+    times = []
+    for k in range(args.steps):
+        x.zero_()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        res = solver.solve(x, b)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t)
+        log(f"solve {k}: {times[-1] * 1e3:.2f} ms, {res['iters']} iterations, "
+            f"rel. defect {res['def_final'] / res['def_init']:.3e}")
+    assert res["status"] == "success", res
+    t_mean = float(np.mean(times))
+
+    # ---- SpMV r = A*x roofline, HIP events on the launch stream
+    k, _ = feat.libs()
+    h = feat.Handle(0, stream=stream.cuda_stream)
+    xs = torch.rand(n, dtype=torch.float64, device=dev)
+    r = torch.empty(n, dtype=torch.float64, device=dev)
+    for _ in range(5):
+        feat.check(k.feat_csr_apply_f64_u64(h.h, feat.ptr(r), 1.0, feat.ptr(xs), 0.0, None, C.c_void_p(A.val),
+                                            C.c_void_p(A.col), C.c_void_p(A.rowptr), n, n, nnz, 0), "spmv")
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(args.spmv_reps):
+        k.feat_csr_apply_f64_u64(h.h, feat.ptr(r), 1.0, feat.ptr(xs), 0.0, None, C.c_void_p(A.val),
+                                 C.c_void_p(A.col), C.c_void_p(A.rowptr), n, n, nnz, 0)
+    e1.record(stream)
+    e1.synchronize()
+    spmv_ms = e0.elapsed_time(e1) / args.spmv_reps
+    alg = spmv_bytes(n, nnz)
+    ref_model = nnz * 16 + n * 8             # product_matvec-bench.cpp:41-44 model
+    achieved = alg / (spmv_ms * 1e-3) / 1e9
+    log(f"spmv: {spmv_ms * 1e3:.1f} us, {achieved:.0f} GB/s algorithmic "
+        f"({ref_model / (spmv_ms * 1e-3) / 1024 ** 3:.0f} GiB/s in the reference's model)")
+    h.close()
+
+    out = {
+        "metric": "MG-PCG time-to-1e-8 residual (s) + SpMV HBM GB/s, 3D Q1 Poisson 2.1M dofs",
+        "value": t_mean,
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t_mean * 1e3,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "3D Q1 Poisson unit cube level 7 (128^3 cells, 2,146,689 dofs, 57,066,625 nnz "
+                        "27-point CSR f64/u64), rhs uniform[-1,1] seed 28054777172512, PCG + MG V(4,4) "
+                        "Richardson(Jacobi 0.5), levels 7..1, exact coarse solve, tol 1e-8",
+            "dofs": n, "nnz": nnz, "levels": LF - LC + 1, "iters": res["iters"],
+            "final_rel_defect": res["def_final"] / res["def_init"],
+            "parallelism": "single GPU",
+        },
+        "spmv": {"kernel": "k_csr_rows<double,unsigned long,EpiApply<double>>", "us_per_launch": spmv_ms * 1e3,
+                 "algorithmic_bytes": alg, "gbps_algorithmic": achieved,
+                 "gibps_reference_model": ref_model / (spmv_ms * 1e-3) / 1024 ** 3},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": achieved / PEAK_HBM_GBS, "traffic": None},
+    }
+    if not args.no_cpu_baseline:
+        log("cpu baseline: reference MG-PCG on one host core ...")
+        cb = cpu_baseline(feat, b_host, res["iters"])
+        out["cpu_baseline"] = {kk: cb[kk] for kk in ("value", "unit", "cores", "kind", "sample")}
+        log(f"cpu baseline: {cb['value']:.2f} s, {cb['iters']} iterations")
+    solver.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

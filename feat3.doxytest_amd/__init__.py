@@ -1,0 +1,325 @@
+"""Python binding of the MI355X LAFEM + Solver hot path.
+
+Loads the in-tree libraries built by build.py:
+  lib/libfeat3amd.so       C-ABI of include/feat3amd.h (HIP kernels, gfx950)
+  lib/libfeat3amd_host.so  C++ host layer, include/feat3amd_host.h
+and declares their signatures for ctypes.  Device buffers are torch tensors
+(plumbing only); every computation runs in the HIP library.  There is no
+fallback: if the libraries are missing, loading fails.
+
+The directory name contains a dot, so import it by path:
+    spec = importlib.util.spec_from_file_location("feat3amd", ".../feat3.doxytest_amd/__init__.py")
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_DIR = os.path.join(PKG_DIR, "lib")
+KERNEL_LIB = os.path.join(LIB_DIR, "libfeat3amd.so")
+HOST_LIB = os.path.join(LIB_DIR, "libfeat3amd_host.so")
+
+FEAT_OK = 0
+STATUS = {0: "undefined", 1: "progress", 2: "success", 3: "aborted", 4: "diverged",
+          5: "max_iter", 6: "stagnated"}
+COARSE = {"none": 0, "smoother": 1, "jacobi": 2, "direct": 3}
+
+_vp = C.c_void_p
+_u64 = C.c_uint64
+_d = C.c_double
+_f = C.c_float
+_i = C.c_int
+
+
+class FeatError(RuntimeError):
+    pass
+
+
+class CsrF64(C.Structure):
+    _fields_ = [("rows", _u64), ("cols", _u64), ("nnz", _u64), ("val", _vp), ("col", _vp),
+                ("rowptr", _vp)]
+
+
+class CsrF32(C.Structure):
+    _fields_ = [("rows", _u64), ("cols", _u64), ("nnz", _u64), ("val", _vp), ("col", _vp),
+                ("rowptr", _vp)]
+
+
+class ReduceWs(C.Structure):
+    _fields_ = [("partials", _vp), ("counter", _vp)]
+
+
+class MgPcgCfg(C.Structure):
+    _fields_ = [("nu_pre", _i), ("nu_post", _i), ("omega_jac", _d), ("omega_rich", _d),
+                ("coarse", _i), ("coarse_omega", _d), ("tol_rel", _d), ("max_iter", _u64),
+                ("force_generic", _i)]
+
+
+class LevelHost(C.Structure):
+    _fields_ = [("rows", _u64), ("val", _vp), ("col", _vp), ("rowptr", _vp), ("filt", _vp),
+                ("nfilt", _u64), ("pcols", _u64), ("pval", _vp), ("pcol", _vp), ("prowptr", _vp)]
+
+
+class SolveResult(C.Structure):
+    _fields_ = [("status", _i), ("iters", _u64), ("def_init", _d), ("def_final", _d)]
+
+
+# (name, restype, argtypes) of every entry point of include/feat3amd.h
+KERNEL_API = [
+    ("feat_last_error", C.c_char_p, []),
+    ("feat_handle_create", _i, [C.POINTER(_vp), _i, _vp]),
+    ("feat_handle_destroy", _i, [_vp]),
+    ("feat_handle_stream", _i, [_vp, C.POINTER(_vp)]),
+    ("feat_sync", _i, [_vp]),
+    ("feat_malloc", _i, [C.POINTER(_vp), C.c_size_t]),
+    ("feat_free", _i, [_vp]),
+    ("feat_host_malloc", _i, [C.POINTER(_vp), C.c_size_t]),
+    ("feat_host_free", _i, [_vp]),
+    ("feat_copy", _i, [_vp, _vp, _vp, C.c_size_t]),
+    ("feat_set_memory_f64", _i, [_vp, _vp, _d, _u64]),
+    ("feat_set_memory_f32", _i, [_vp, _vp, _f, _u64]),
+    ("feat_convert_f64_f32", _i, [_vp, _vp, _vp, _u64]),
+    ("feat_convert_f32_f64", _i, [_vp, _vp, _vp, _u64]),
+    ("feat_convert_f64_f16", _i, [_vp, _vp, _vp, _u64]),
+    ("feat_convert_f16_f64", _i, [_vp, _vp, _vp, _u64]),
+    ("feat_convert_u64_u32", _i, [_vp, _vp, _vp, _u64]),
+    ("feat_csr_apply_f64_u64", _i, [_vp, _vp, _d, _vp, _d, _vp, _vp, _vp, _vp, _u64, _u64, _u64, _i]),
+    ("feat_csr_apply_f64_u32", _i, [_vp, _vp, _d, _vp, _d, _vp, _vp, _vp, _vp, _u64, _u64, _u64, _i]),
+    ("feat_csr_apply_f32_u32", _i, [_vp, _vp, _f, _vp, _f, _vp, _vp, _vp, _vp, _u64, _u64, _u64, _i]),
+    ("feat_csr_apply_f32_u64", _i, [_vp, _vp, _f, _vp, _f, _vp, _vp, _vp, _vp, _u64, _u64, _u64, _i]),
+    ("feat_bcsr_apply_f64_u64", _i, [_vp, _i, _i, _vp, _d, _vp, _d, _vp, _vp, _vp, _vp, _u64, _u64, _u64]),
+    ("feat_axpy_f64", _i, [_vp, _vp, _d, _vp, _vp, _u64]),
+    ("feat_axpy_f32", _i, [_vp, _vp, _f, _vp, _vp, _u64]),
+    ("feat_dot_f64", _i, [_vp, _vp, _vp, _vp, _u64]),
+    ("feat_dot_f64_host", _i, [_vp, C.POINTER(_d), _vp, _vp, _u64]),
+    ("feat_triple_dot_f64_host", _i, [_vp, C.POINTER(_d), _vp, _vp, _vp, _u64]),
+    ("feat_norm2_f64", _i, [_vp, _vp, _vp, _u64]),
+    ("feat_norm2_f64_host", _i, [_vp, C.POINTER(_d), _vp, _u64]),
+    ("feat_scale_f64", _i, [_vp, _vp, _vp, _d, _u64]),
+    ("feat_component_product_f64", _i, [_vp, _vp, _vp, _vp, _u64]),
+    ("feat_component_invert_f64", _i, [_vp, _vp, _vp, _d, _u64]),
+    ("feat_csr_diag_index_u64", _i, [_vp, _vp, _vp, _vp, _u64]),
+    ("feat_csr_extract_diag_f64_u64", _i, [_vp, _vp, _vp, _vp, _vp, _u64]),
+    ("feat_unit_filter_rhs_f64_u64", _i, [_vp, _vp, _vp, _vp, _u64]),
+    ("feat_unit_filter_def_f64_u64", _i, [_vp, _vp, _vp, _u64]),
+    ("feat_unit_filter_mask_u64", _i, [_vp, _vp, _u64, _vp, _u64]),
+    ("feat_unit_filter_blocked_def_f64_u64", _i, [_vp, _i, _vp, _vp, _u64]),
+    ("feat_mean_filter_f64", _i, [_vp, _vp, _vp, _vp, _d, _u64]),
+    ("feat_csr_defect_f64", _i, [_vp, C.POINTER(CsrF64), _vp, _vp, _vp, _vp, _vp, C.POINTER(ReduceWs)]),
+    ("feat_csr_apply_masked_f64", _i, [_vp, C.POINTER(CsrF64), _vp, _vp, _vp, _vp, _vp, _vp, C.POINTER(ReduceWs)]),
+    ("feat_csr_prol_add_f64", _i, [_vp, C.POINTER(CsrF64), _vp, _vp, _vp]),
+    ("feat_rj_smooth_f64", _i, [_vp, C.POINTER(CsrF64), _vp, _vp, _i, _d, _vp, _vp, _vp, _vp, _vp, _vp,
+                                _vp, _vp, C.POINTER(ReduceWs)]),
+    ("feat_dense_gemv_f64", _i, [_vp, _i, _vp, _vp, _vp, _vp]),
+    ("feat_pcg_update_f64", _i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _u64, C.POINTER(ReduceWs)]),
+    ("feat_pcg_direction_f64", _i, [_vp, _vp, _vp, _vp, _vp, _u64]),
+    ("feat_read_scalar_f64", _i, [_vp, _vp, C.POINTER(_d)]),
+    ("feat_csr_apply_masked_f32", _i, [_vp, C.POINTER(CsrF32), _vp, _vp, _vp, _vp, C.POINTER(ReduceWs)]),
+    ("feat_csr_defect_f32", _i, [_vp, C.POINTER(CsrF32), _vp, _vp, _vp, _vp, _vp, C.POINTER(ReduceWs)]),
+    ("feat_csr_prol_add_f32", _i, [_vp, C.POINTER(CsrF32), _vp, _vp, _vp]),
+    ("feat_rj_smooth_f32", _i, [_vp, C.POINTER(CsrF32), _vp, _vp, _i, _f, _vp, _vp, _vp, _vp, _vp, _vp,
+                                C.POINTER(ReduceWs)]),
+    ("feat_dense_gemv_f32", _i, [_vp, _i, _vp, _vp, _vp, _vp]),
+]
+
+HOST_API = [
+    ("feat_host_last_error", C.c_char_p, []),
+    ("feat_q1_sizes", _i, [_i, _i, _vp]),
+    ("feat_q1_assemble", _i, [_i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    ("feat_q1_rhs_sine", _i, [_i, _i, _vp]),
+    ("feat_rhs_random", _i, [_u64, _u64, _d, _d, _vp]),
+    ("feat_mgpcg_create", _i, [C.POINTER(_vp), C.POINTER(LevelHost), _i, C.POINTER(MgPcgCfg)]),
+    ("feat_mgpcg_create_q1", _i, [C.POINTER(_vp), _i, _i, _i, C.POINTER(MgPcgCfg)]),
+    ("feat_mgpcg_solve", _i, [_vp, _vp, _vp, C.POINTER(SolveResult), _vp, _u64]),
+    ("feat_mgpcg_matrix", _i, [_vp, _i, C.POINTER(_u64), C.POINTER(_u64), C.POINTER(_vp), C.POINTER(_vp),
+                               C.POINTER(_vp)]),
+    ("feat_mgpcg_destroy", _i, [_vp]),
+    ("feat_runtime_init", _i, [_i, _vp]),
+]
+
+
+def _declare(lib, api):
+    for name, res, args in api:
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+
+
+_LIBS = {}
+
+
+def libs():
+    """(kernel_lib, host_lib); raises if the HIP library is not built."""
+    if not _LIBS:
+        if not os.path.exists(KERNEL_LIB) or not os.path.exists(HOST_LIB):
+            raise FeatError(f"HIP extension not built: run python {PKG_DIR}/build.py")
+        k = C.CDLL(KERNEL_LIB, mode=C.RTLD_GLOBAL)
+        _declare(k, KERNEL_API)
+        h = C.CDLL(HOST_LIB)
+        _declare(h, HOST_API)
+        _LIBS["k"], _LIBS["h"] = k, h
+    return _LIBS["k"], _LIBS["h"]
+
+
+def check(rc, what=""):
+    if rc != FEAT_OK:
+        k, h = libs()
+        msg = (k.feat_last_error() or b"").decode() or (h.feat_host_last_error() or b"").decode()
+        raise FeatError(f"{what} failed ({rc}): {msg}")
+
+
+def ptr(t):
+    """Device pointer of a torch tensor (or None)."""
+    return None if t is None else C.c_void_p(t.data_ptr())
+
+
+class Handle:
+    """feat_handle_t on the current torch stream of `device`."""
+
+    def __init__(self, device=0, stream=None):
+        k, _ = libs()
+        self.k = k
+        self.h = C.c_void_p()
+        check(k.feat_handle_create(C.byref(self.h), device, stream), "feat_handle_create")
+
+    def sync(self):
+        check(self.k.feat_sync(self.h), "feat_sync")
+
+    def stream(self):
+        s = C.c_void_p()
+        check(self.k.feat_handle_stream(self.h, C.byref(s)), "feat_handle_stream")
+        return s.value
+
+    def close(self):
+        if self.h:
+            self.k.feat_handle_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def csr_desc(val, col, rowptr, rows, cols):
+    cls = CsrF64 if val.dtype.itemsize == 8 else CsrF32
+    return cls(rows, cols, val.numel(), val.data_ptr(), col.data_ptr(), rowptr.data_ptr())
+
+
+def reduce_ws(torch_mod, device):
+    """A reduction workspace: (keep-alive tensors, ReduceWs)."""
+    part = torch_mod.zeros(2 * 4096, dtype=torch_mod.float64, device=device)
+    cnt = torch_mod.zeros(16, dtype=torch_mod.int32, device=device)
+    return (part, cnt), ReduceWs(part.data_ptr(), cnt.data_ptr())
+
+
+# ---------------------------------------------------------------- host layer
+def q1_problem(dim, level):
+    """Structured Q1 Poisson level (lexicographic numbering), host numpy arrays."""
+    _, h = libs()
+    s = np.zeros(5, np.uint64)
+    check(h.feat_q1_sizes(dim, level, s.ctypes.data), "feat_q1_sizes")
+    n, nnz, nb, pnnz, pcols = (int(v) for v in s)
+    val = np.empty(nnz)
+    col = np.empty(nnz, np.uint64)
+    rp = np.empty(n + 1, np.uint64)
+    bnd = np.empty(nb, np.uint64)
+    pval = np.empty(max(pnnz, 1))
+    pcol = np.empty(max(pnnz, 1), np.uint64)
+    prp = np.empty(n + 1, np.uint64)
+    check(h.feat_q1_assemble(dim, level, val.ctypes.data, col.ctypes.data, rp.ctypes.data,
+                             bnd.ctypes.data, pval.ctypes.data, pcol.ctypes.data, prp.ctypes.data),
+          "feat_q1_assemble")
+    out = dict(n=n, val=val, col=col, rowptr=rp, filt=bnd)
+    if level > 0:
+        out.update(pval=pval[:pnnz], pcol=pcol[:pnnz], prowptr=prp, pcols=pcols)
+    return out
+
+
+def q1_rhs_sine(dim, level, n):
+    _, h = libs()
+    b = np.empty(n)
+    check(h.feat_q1_rhs_sine(dim, level, b.ctypes.data), "feat_q1_rhs_sine")
+    return b
+
+
+def rhs_random(seed, n, a=-1.0, b=1.0):
+    _, h = libs()
+    v = np.empty(n)
+    check(h.feat_rhs_random(seed, n, a, b, v.ctypes.data), "feat_rhs_random")
+    return v
+
+
+def mgpcg_cfg(nu_pre=4, nu_post=4, omega_jac=0.5, omega_rich=1.0, coarse="direct",
+              coarse_omega=1.0, tol_rel=1e-8, max_iter=100, force_generic=False):
+    return MgPcgCfg(nu_pre, nu_post, omega_jac, omega_rich, COARSE[coarse], coarse_omega, tol_rel,
+                    max_iter, int(force_generic))
+
+
+def runtime_init(device=0, stream=None):
+    """Binds the host layer to a HIP stream (default: torch's current stream)."""
+    _, h = libs()
+    if stream is None:
+        import torch
+        stream = torch.cuda.current_stream(device).cuda_stream
+    check(h.feat_runtime_init(device, stream), "feat_runtime_init")
+
+
+class MgPcg:
+    """PCG(MultiGrid-V(Richardson(Jacobi))) of the C++ host layer."""
+
+    def __init__(self, cfg, levels=None, q1=None):
+        _, self.hl = libs()
+        runtime_init()
+        self.s = C.c_void_p()
+        self.cfg = cfg
+        if q1 is not None:
+            dim, lf, lc = q1
+            check(self.hl.feat_mgpcg_create_q1(C.byref(self.s), dim, lf, lc, C.byref(cfg)),
+                  "feat_mgpcg_create_q1")
+        else:
+            arr = (LevelHost * len(levels))()
+            keep = []
+            for i, lv in enumerate(levels):
+                a = {k: np.ascontiguousarray(v) for k, v in lv.items() if isinstance(v, np.ndarray)}
+                keep.append(a)
+                hasp = "pval" in a and i + 1 < len(levels)
+                arr[i] = LevelHost(lv["n"], a["val"].ctypes.data, a["col"].ctypes.data,
+                                   a["rowptr"].ctypes.data,
+                                   a["filt"].ctypes.data if a["filt"].size else None, a["filt"].size,
+                                   lv.get("pcols", 0) if hasp else 0,
+                                   a["pval"].ctypes.data if hasp else None,
+                                   a["pcol"].ctypes.data if hasp else None,
+                                   a["prowptr"].ctypes.data if hasp else None)
+            check(self.hl.feat_mgpcg_create(C.byref(self.s), arr, len(levels), C.byref(cfg)),
+                  "feat_mgpcg_create")
+
+    def solve(self, x, b, hist_cap=1000):
+        """x (initial guess, overwritten) and b: torch float64 device tensors."""
+        res = SolveResult()
+        hist = np.full(hist_cap, np.nan)
+        check(self.hl.feat_mgpcg_solve(self.s, ptr(x), ptr(b), C.byref(res), hist.ctypes.data,
+                                       hist_cap), "feat_mgpcg_solve")
+        it = int(res.iters)
+        return dict(status=STATUS[res.status], iters=it, def_init=res.def_init,
+                    def_final=res.def_final, hist=hist[:it + 1])
+
+    def matrix(self, lvl=0):
+        rows, nnz = _u64(), _u64()
+        v, c, r = _vp(), _vp(), _vp()
+        check(self.hl.feat_mgpcg_matrix(self.s, lvl, C.byref(rows), C.byref(nnz), C.byref(v),
+                                        C.byref(c), C.byref(r)), "feat_mgpcg_matrix")
+        return CsrF64(rows.value, rows.value, nnz.value, v.value, c.value, r.value)
+
+    def close(self):
+        if self.s:
+            self.hl.feat_mgpcg_destroy(self.s)
+            self.s = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

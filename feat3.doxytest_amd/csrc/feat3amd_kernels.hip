@@ -1,0 +1,1304 @@
+// feat3amd_kernels.hip -- gfx950 kernels and the C-ABI of include/feat3amd.h.
+//
+// Design (MI355X / CDNA4):
+//  * CSR SpMV: rows are processed in row-blocks of kRowsPerBlock.  A 256-thread
+//    workgroup streams the contiguous nonzero range of its row-block through
+//    LDS (val and the gathered x[col] side by side, fully coalesced 64-wide
+//    loads), then one lane per row sums its entries from LDS in storage order
+//    with one fma per entry.  The per-row arithmetic is therefore exactly the
+//    reference's sequential loop (arch/apply_generic.hpp:57-66), bit for bit,
+//    while HBM sees only coalesced streams.  Fused epilogues apply the unit
+//    filter (bit mask), the defect/Richardson/Jacobi updates and reductions.
+//  * Grids are capped (kMaxGrid) and row-blocks are assigned per XCD in
+//    contiguous ranges: a workgroup with blockIdx b runs on XCD b % 8, so the
+//    x-gathers of neighbouring row-blocks share that XCD's L2.
+//  * Reductions are deterministic: per-workgroup partials (fixed mapping) and a
+//    last-workgroup-done finish that sums the partials in index order.
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <type_traits>
+
+#include "feat3amd.h"
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kXcd = 8;
+constexpr int kMaxGrid = FEAT_REDUCE_MAX_BLOCKS;
+constexpr int kRowsPerBlock = 64;   // rows per CSR row-block
+constexpr int kChunk = 2048;        // nonzeros staged in LDS per pass
+
+thread_local std::string g_err;
+
+int fail(int code, const char* what)
+{
+  g_err = what;
+  return code;
+}
+
+int check_hip(hipError_t e, const char* where)
+{
+  if(e == hipSuccess) return FEAT_OK;
+  g_err = std::string(where) + ": " + hipGetErrorString(e);
+  return FEAT_ERR_HIP;
+}
+
+int check_launch(const char* where)
+{
+  return check_hip(hipGetLastError(), where);
+}
+
+inline int grid_for(uint64_t work, int per_thread = 1)
+{
+  uint64_t b = (work + uint64_t(kThreads) * per_thread - 1) / (uint64_t(kThreads) * per_thread);
+  if(b < 1) b = 1;
+  if(b > uint64_t(kMaxGrid)) b = kMaxGrid;
+  return int(b);
+}
+
+// ---------------------------------------------------------------------------
+// device helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool is_masked(const uint32_t* mask, uint64_t i)
+{
+  return mask != nullptr && ((mask[i >> 5] >> (i & 31u)) & 1u);
+}
+
+__device__ __forceinline__ double dfma(double a, double b, double c) { return __fma_rn(a, b, c); }
+__device__ __forceinline__ float dfma(float a, float b, float c) { return __fmaf_rn(a, b, c); }
+
+// Skip rule of IterativeSolver::_set_initial_defect (iterative.hpp:761-766)
+// for a smoother whose initial defect norm is sqrt(nrm2).
+template<typename DT>
+__device__ __forceinline__ bool smoother_skips(const double* nrm2)
+{
+  if(nrm2 == nullptr) return false;
+  const double d = sqrt(*nrm2);
+  const double eps = std::is_same<DT, float>::value ? double(FLT_EPSILON) : DBL_EPSILON;
+  return !isfinite(d) || d <= eps * eps;
+}
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+  for(int off = 32; off > 0; off >>= 1)
+    v += __shfl_down(v, off, 64);
+  return v;
+}
+
+// Sum over the workgroup; the result is valid in thread 0.
+__device__ double block_sum(double v, double* red /* >= kThreads/64 */)
+{
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  if(lane == 0) red[w] = v;
+  __syncthreads();
+  double t = 0.0;
+  if(threadIdx.x == 0)
+  {
+    t = red[0];
+    for(int i = 1; i < int(blockDim.x >> 6); ++i) t += red[i];
+  }
+  __syncthreads();
+  return t;
+}
+
+// Deterministic grid reduction of up to two values: partial per workgroup,
+// the last workgroup to arrive sums the partials in index order.
+struct GridReduce
+{
+  double* partials;   // 2 * kMaxGrid
+  uint32_t* counter;
+  double* out0;       // may be null
+  double* out1;       // may be null
+};
+
+__device__ void grid_reduce_finish(const GridReduce& g, double v0, double v1)
+{
+  __shared__ double red[kThreads / 64];
+  __shared__ int last;
+  const bool r0 = g.out0 != nullptr, r1 = g.out1 != nullptr;
+  if(!r0 && !r1) return;
+  double b0 = r0 ? block_sum(v0, red) : 0.0;
+  double b1 = r1 ? block_sum(v1, red) : 0.0;
+  if(threadIdx.x == 0)
+  {
+    if(r0) __hip_atomic_store(&g.partials[blockIdx.x], b0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if(r1) __hip_atomic_store(&g.partials[kMaxGrid + blockIdx.x], b1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __threadfence();
+    const uint32_t prev = atomicAdd(g.counter, 1u);
+    last = (prev == gridDim.x - 1) ? 1 : 0;
+  }
+  __syncthreads();
+  if(!last) return;
+  __threadfence();
+  double s0 = 0.0, s1 = 0.0;
+  for(uint32_t i = threadIdx.x; i < gridDim.x; i += blockDim.x)
+  {
+    if(r0) s0 += __hip_atomic_load(&g.partials[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if(r1) s1 += __hip_atomic_load(&g.partials[kMaxGrid + i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  if(r0) s0 = block_sum(s0, red);
+  if(r1) s1 = block_sum(s1, red);
+  if(threadIdx.x == 0)
+  {
+    if(r0) *g.out0 = s0;
+    if(r1) *g.out1 = s1;
+    __hip_atomic_store(g.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+// Contiguous range of row-blocks for this workgroup's XCD; the workgroups of
+// one XCD stride through it.
+struct XcdRange
+{
+  uint64_t first, end, stride;
+};
+
+__device__ __forceinline__ XcdRange xcd_range(uint64_t nitems)
+{
+  const uint32_t g = gridDim.x, b = blockIdx.x;
+  const uint32_t x = b % kXcd, j = b / kXcd;
+  const uint32_t q = g / kXcd, rem = g % kXcd;
+  const uint32_t gx = q + (x < rem ? 1u : 0u);
+  const uint64_t pre = uint64_t(x) * q + (x < rem ? x : rem);
+  XcdRange r;
+  const uint64_t begin = nitems * pre / g;
+  r.end = nitems * (pre + gx) / g;
+  r.first = begin + j;
+  r.stride = gx;
+  return r;
+}
+
+// ---------------------------------------------------------------------------
+// CSR row-block kernel with a fused epilogue.
+//   Epi::row(row, sum, acc0, acc1) is called once per row by its lane.
+// ---------------------------------------------------------------------------
+template<typename DT, typename IT, class Epi>
+__global__ __launch_bounds__(kThreads) void k_csr_rows(const DT* __restrict__ val,
+                                                       const IT* __restrict__ col,
+                                                       const IT* __restrict__ rowptr,
+                                                       const DT* __restrict__ x, uint64_t rows,
+                                                       Epi epi, GridReduce gr)
+{
+  __shared__ DT s_val[kChunk];
+  __shared__ DT s_x[kChunk];
+  const uint64_t nrb = (rows + kRowsPerBlock - 1) / kRowsPerBlock;
+  const XcdRange xr = xcd_range(nrb);
+  const int t = threadIdx.x;
+  double acc0 = 0.0, acc1 = 0.0;
+  epi.begin();
+  for(uint64_t rb = xr.first; rb < xr.end; rb += xr.stride)
+  {
+    const uint64_t r0 = rb * kRowsPerBlock;
+    const uint64_t nr = (rows - r0) < uint64_t(kRowsPerBlock) ? (rows - r0) : uint64_t(kRowsPerBlock);
+    const uint64_t p0 = uint64_t(rowptr[r0]);
+    const uint64_t p1 = uint64_t(rowptr[r0 + nr]);
+    uint64_t rs = 0, re = 0;
+    if(uint64_t(t) < nr)
+    {
+      rs = uint64_t(rowptr[r0 + t]);
+      re = uint64_t(rowptr[r0 + t + 1]);
+    }
+    DT sum = DT(0);
+    for(uint64_t c0 = p0; c0 < p1; c0 += kChunk)
+    {
+      const uint64_t cn = (p1 - c0) < uint64_t(kChunk) ? (p1 - c0) : uint64_t(kChunk);
+      for(uint64_t j = t; j < cn; j += kThreads)
+      {
+        s_val[j] = val[c0 + j];
+        s_x[j] = x[uint64_t(col[c0 + j])];
+      }
+      __syncthreads();
+      if(uint64_t(t) < nr)
+      {
+        const uint64_t kb = rs > c0 ? rs : c0;
+        const uint64_t ke = re < c0 + cn ? re : c0 + cn;
+        for(uint64_t k = kb; k < ke; ++k)
+          sum = dfma(s_val[k - c0], s_x[k - c0], sum);
+      }
+      __syncthreads();
+    }
+    if(uint64_t(t) < nr)
+      epi.row(r0 + t, sum, acc0, acc1);
+  }
+  grid_reduce_finish(gr, acc0, acc1);
+}
+
+// r = a*sum + b*y  (y not read when |b| < eps)          [arch/apply_generic.hpp:30-66]
+template<typename DT>
+struct EpiApply
+{
+  DT* r;
+  const DT* y;
+  DT a, b;
+  bool bzero;
+  __device__ void begin() {}
+  __device__ __forceinline__ void row(uint64_t i, DT sum, double&, double&) const
+  {
+    const DT yv = bzero ? DT(0) : y[i];
+    r[i] = dfma(sum, a, b * yv);
+  }
+};
+
+// r = filter(A*x) with optional |r|^2 and r.w       [transfer.hpp:247-251 + unit_filter]
+template<typename DT>
+struct EpiMasked
+{
+  DT* r;
+  const uint32_t* mask;
+  const DT* dotw;
+  bool nrm, dot;
+  __device__ void begin() {}
+  __device__ __forceinline__ void row(uint64_t i, DT sum, double& a0, double& a1) const
+  {
+    DT v = dfma(sum, DT(1), DT(0));
+    if(is_masked(mask, i)) v = DT(0);
+    r[i] = v;
+    if(nrm) a0 = fma(double(v), double(v), a0);
+    if(dot) a1 = fma(double(v), double(dotw[i]), a1);
+  }
+};
+
+// d = filter(rhs - A*x) with optional |d|^2          [sparse_matrix_csr.hpp:1807-1843]
+template<typename DT>
+struct EpiDefect
+{
+  DT* d;
+  const DT* rhs;
+  const uint32_t* mask;
+  bool nrm;
+  __device__ void begin() {}
+  __device__ __forceinline__ void row(uint64_t i, DT sum, double& a0, double&) const
+  {
+    DT v = dfma(sum, DT(-1), rhs[i]);
+    if(is_masked(mask, i)) v = DT(0);
+    d[i] = v;
+    if(nrm) a0 = fma(double(v), double(v), a0);
+  }
+};
+
+// sol += filter(P*xc)                                  [multigrid.hpp:1885-1930]
+template<typename DT>
+struct EpiProlAdd
+{
+  DT* sol;
+  const uint32_t* mask;
+  __device__ void begin() {}
+  __device__ __forceinline__ void row(uint64_t i, DT sum, double&, double&) const
+  {
+    DT c = dfma(sum, DT(1), DT(0));
+    if(is_masked(mask, i)) c = DT(0);
+    sol[i] = dfma(DT(1), c, sol[i]);
+  }
+};
+
+// One Richardson(Jacobi) step, fused (richardson.hpp:205-221, jacobi_precond.hpp:159-164):
+//   d = filter(rhs - A*x_k)
+//   mode 0 (next):  x_{k+1} = x_k + omega * filter(inv∘d)
+//   mode 1 (defect): dout = d
+//   mode 2 (acc):    acc += x_k + omega * filter(inv∘d);   optional acc.w
+template<typename DT>
+struct EpiSmooth
+{
+  const DT* rhs;
+  const DT* inv;
+  const uint32_t* mask;
+  const DT* xk;
+  DT* out;          // x_{k+1} (mode 0), d (mode 1), acc (mode 2)
+  const DT* dotw;
+  const double* rhs_nrm2;
+  DT omega;
+  int mode;
+  bool skip;
+  __device__ void begin() { skip = smoother_skips<DT>(rhs_nrm2); }
+  __device__ __forceinline__ void row(uint64_t i, DT sum, double&, double& a1) const
+  {
+    const bool m = is_masked(mask, i);
+    DT d = dfma(sum, DT(-1), rhs[i]);
+    if(m) d = DT(0);
+    if(mode == 1)
+    {
+      out[i] = d;
+      return;
+    }
+    DT c = inv[i] * d;
+    if(m) c = DT(0);
+    DT xn = dfma(omega, c, xk[i]);
+    if(skip) xn = DT(0);
+    if(mode == 0)
+      out[i] = xn;
+    else
+    {
+      const DT a = dfma(DT(1), xn, out[i]);
+      out[i] = a;
+      if(dotw) a1 = a1 + double(a) * double(dotw[i]);
+    }
+  }
+};
+
+template<typename DT, typename IT, class Epi>
+int launch_rows(hipStream_t s, const DT* val, const IT* col, const IT* rowptr, const DT* x,
+                uint64_t rows, const Epi& epi, const GridReduce& gr, const char* name)
+{
+  if(rows == 0) return FEAT_OK;
+  const uint64_t nrb = (rows + kRowsPerBlock - 1) / kRowsPerBlock;
+  const int grid = int(nrb < uint64_t(kMaxGrid) ? nrb : uint64_t(kMaxGrid));
+  hipLaunchKernelGGL((k_csr_rows<DT, IT, Epi>), dim3(grid), dim3(kThreads), 0, s, val, col, rowptr,
+                     x, rows, epi, gr);
+  return check_launch(name);
+}
+
+// ---------------------------------------------------------------------------
+// elementwise kernels
+// ---------------------------------------------------------------------------
+template<typename DT>
+__global__ void k_fill(DT* p, DT v, uint64_t n)
+{
+  for(uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
+    p[i] = v;
+}
+
+template<typename DO, typename DI>
+__global__ void k_convert(DO* o, const DI* in, uint64_t n)
+{
+  for(uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
+    o[i] = DO(in[i]);
+}
+
+__global__ void k_convert_f64_f16(uint16_t* o, const double* in, uint64_t n)
+{
+  for(uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
+  {
+    _Float16 h = _Float16(in[i]);
+    o[i] = __builtin_bit_cast(uint16_t, h);
+  }
+}
+
+__global__ void k_convert_f16_f64(double* o, const uint16_t* in, uint64_t n)
+{
+  for(uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
+    o[i] = double(__builtin_bit_cast(_Float16, in[i]));
+}
+
+// axpy: r == x (and not y) rounds a*x first, like `r[i] *= a; r[i] += y[i];`
+template<typename DT>
+__global__ void k_axpy(DT* r, DT a, const DT* x, const DT* y, uint64_t n, int two_round)
+{
+  for(uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
+  {
+    if(two_round)
+    {
+      const DT t = x[i] * a;
+      r[i] = t + y[i];
+    }
+    else
+      r[i] = dfma(a, x[i], y[i]);
+  }
+}
+
+__global__ void k_scale(double* r, const double* x, double s, uint64_t n)
+{
+  for(uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
+    r[i] = x[i] * s;
+}
+
+__global__ void k_cprod(double* r, const double* x, const double* y, uint64_t n)
+{
+  for(uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
+    r[i] = x[i] * y[i];
+}
+
+__global__ void k_cinv(double* r, const double* x, double s, uint64_t n)
+{
+  for(uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
+    r[i] = s / x[i];
+}
+
+// kind 0: sum x*y; 1: sum x*x (fma); 2: sum x*y*z
+__global__ __launch_bounds__(kThreads) void k_reduce(const double* x, const double* y, const double* z,
+                                                     uint64_t n, int kind, GridReduce gr)
+{
+  double acc = 0.0;
+  for(uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
+  {
+    if(kind == 0)
+      acc = acc + x[i] * y[i];
+    else if(kind == 1)
+      acc = fma(x[i], x[i], acc);
+    else
+      acc = fma(x[i] * y[i], z[i], acc);
+  }
+  grid_reduce_finish(gr, acc, 0.0);
+}
+
+__global__ void k_sqrt_inplace(double* p)
+{
+  if(threadIdx.x == 0 && blockIdx.x == 0) *p = sqrt(*p);
+}
+
+template<typename IT>
+__global__ void k_diag_index(IT* diag, const IT* col, const IT* rowptr, uint64_t rows)
+{
+  for(uint64_t r = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; r < rows; r += uint64_t(gridDim.x) * blockDim.x)
+  {
+    IT d = rowptr[rows];
+    for(IT i = rowptr[r]; i < rowptr[r + 1]; ++i)
+      if(uint64_t(col[i]) == r)
+      {
+        d = i;
+        break;
+      }
+    diag[r] = d;
+  }
+}
+
+__global__ void k_extract_diag(double* diag, const double* val, const uint64_t* col,
+                               const uint64_t* rowptr, uint64_t rows)
+{
+  for(uint64_t r = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; r < rows; r += uint64_t(gridDim.x) * blockDim.x)
+  {
+    double d = 0.0;
+    for(uint64_t i = rowptr[r]; i < rowptr[r + 1]; ++i)
+      if(col[i] == r)
+      {
+        d = val[i];
+        break;
+      }
+    diag[r] = d;
+  }
+}
+
+__global__ void k_filter_rhs(double* v, const double* sv, const uint64_t* idx, uint64_t ue)
+{
+  for(uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < ue; i += uint64_t(gridDim.x) * blockDim.x)
+    v[idx[i]] = sv[i];
+}
+
+__global__ void k_filter_def(double* v, const uint64_t* idx, uint64_t ue)
+{
+  for(uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < ue; i += uint64_t(gridDim.x) * blockDim.x)
+    v[idx[i]] = 0.0;
+}
+
+__global__ void k_filter_blocked_def(double* v, int bs, const uint64_t* idx, uint64_t ue)
+{
+  for(uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < ue; i += uint64_t(gridDim.x) * blockDim.x)
+    for(int k = 0; k < bs; ++k) v[idx[i] * bs + k] = 0.0;
+}
+
+__global__ void k_mask_zero(uint32_t* m, uint64_t nw)
+{
+  for(uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < nw; i += uint64_t(gridDim.x) * blockDim.x)
+    m[i] = 0u;
+}
+
+__global__ void k_mask_set(uint32_t* m, const uint64_t* idx, uint64_t ue)
+{
+  for(uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < ue; i += uint64_t(gridDim.x) * blockDim.x)
+    atomicOr(&m[idx[i] >> 5], 1u << (idx[i] & 31u));
+}
+
+// mean filter: v += (-(v.w_dot)/vol) * w_add; alpha computed from the device scalar
+__global__ void k_mean_apply(double* v, const double* w_add, const double* dotv, double vol, uint64_t n)
+{
+  const double alpha = -(*dotv) / vol;
+  for(uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
+    v[i] = fma(alpha, w_add[i], v[i]);
+}
+
+// transposed SpMV helpers: r = (b/a)*y (or 0); r[col] += val*x[row]; r *= a
+__global__ void k_tr_init(double* r, const double* y, double ba, int bzero, uint64_t n)
+{
+  for(uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
+    r[i] = bzero ? 0.0 : ba * y[i];
+}
+
+template<typename DT, typename IT>
+__global__ void k_tr_scatter(DT* r, const DT* x, const DT* val, const IT* col, const IT* rowptr, uint64_t rows)
+{
+  for(uint64_t row = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; row < rows; row += uint64_t(gridDim.x) * blockDim.x)
+    for(IT i = rowptr[row]; i < rowptr[row + 1]; ++i)
+      atomicAdd(&r[col[i]], val[i] * x[row]);
+}
+
+template<typename DT>
+__global__ void k_tr_scale(DT* r, DT a, uint64_t n)
+{
+  for(uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
+    r[i] = a * r[i];
+}
+
+// BCSR: one lane per block row, sequential over blocks, h, w (arch/apply_generic.hpp:121-154)
+template<int BH, int BW>
+__global__ void k_bcsr(double* r, double a, const double* x, double b, const double* y,
+                       const double* val, const uint64_t* col, const uint64_t* rowptr,
+                       uint64_t rows, int bzero)
+{
+  for(uint64_t row = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; row < rows; row += uint64_t(gridDim.x) * blockDim.x)
+  {
+    double bs[BH];
+#pragma unroll
+    for(int h = 0; h < BH; ++h) bs[h] = 0.0;
+    for(uint64_t i = rowptr[row]; i < rowptr[row + 1]; ++i)
+    {
+      const double* blk = val + i * (BH * BW);
+      const double* bx = x + col[i] * BW;
+#pragma unroll
+      for(int h = 0; h < BH; ++h)
+#pragma unroll
+        for(int w = 0; w < BW; ++w)
+          bs[h] = __fma_rn(blk[h * BW + w], bx[w], bs[h]);
+    }
+#pragma unroll
+    for(int h = 0; h < BH; ++h)
+    {
+      const double yv = bzero ? 0.0 : y[row * BH + h];
+      r[row * BH + h] = __fma_rn(bs[h], a, b * yv);
+    }
+  }
+}
+
+// Richardson(Jacobi) first step from x = 0: x_1 = omega*filter(inv∘rhs); optional acc += x_1
+template<typename DT>
+__global__ void k_smooth_first(DT* x1, const DT* inv, const DT* rhs, const uint32_t* mask, DT omega,
+                               const double* rhs_nrm2, DT* acc, const DT* dotw, uint64_t n,
+                               GridReduce gr)
+{
+  const bool skip = smoother_skips<DT>(rhs_nrm2);
+  double a1 = 0.0;
+  for(uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
+  {
+    DT c = inv[i] * rhs[i];
+    if(is_masked(mask, i)) c = DT(0);
+    DT xn = dfma(omega, c, DT(0));
+    if(skip) xn = DT(0);
+    if(acc)
+    {
+      const DT a = dfma(DT(1), xn, acc[i]);
+      acc[i] = a;
+      if(dotw) a1 = a1 + double(a) * double(dotw[i]);
+    }
+    else
+      x1[i] = xn;
+  }
+  grid_reduce_finish(gr, 0.0, a1);
+}
+
+template<typename DT>
+__global__ void k_dense_gemv(int n, const DT* ainv, const DT* b, DT* x, const uint32_t* mask)
+{
+  for(int r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x)
+  {
+    DT s = DT(0);
+    for(int k = 0; k < n; ++k) s = dfma(ainv[size_t(r) * n + k], b[k], s);
+    if(is_masked(mask, uint64_t(r))) s = DT(0);
+    x[r] = s;
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void k_pcg_update(double* x, double* r, const double* p, const double* q,
+                                                         const double* gamma, const double* qp, uint64_t n,
+                                                         GridReduce gr)
+{
+  const double alpha = (*gamma) / (*qp);
+  double acc = 0.0;
+  for(uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
+  {
+    x[i] = fma(alpha, p[i], x[i]);
+    const double rn = fma(-alpha, q[i], r[i]);
+    r[i] = rn;
+    acc = fma(rn, rn, acc);
+  }
+  grid_reduce_finish(gr, acc, 0.0);
+}
+
+__global__ void k_pcg_direction(double* p, const double* z, const double* gamma_new, double* gamma_old, uint64_t n)
+{
+  const double beta = (*gamma_new) / (*gamma_old);
+  for(uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
+  {
+    const double t = p[i] * beta;
+    p[i] = t + z[i];
+  }
+}
+
+__global__ void k_copy_scalar(double* dst, const double* src)
+{
+  if(threadIdx.x == 0 && blockIdx.x == 0) *dst = *src;
+}
+
+} // namespace
+
+// ===========================================================================
+// handle
+// ===========================================================================
+struct feat_handle_s
+{
+  int device;
+  hipStream_t stream;
+  bool own_stream;
+  double* partials;   // 2 * kMaxGrid
+  uint32_t* counter;
+  double* scal;       // scratch device scalars
+  double* host;       // pinned host scalars
+};
+
+namespace {
+GridReduce handle_reduce(feat_handle_t h, double* out0, double* out1)
+{
+  GridReduce g;
+  g.partials = h->partials;
+  g.counter = h->counter;
+  g.out0 = out0;
+  g.out1 = out1;
+  return g;
+}
+
+GridReduce ws_reduce(const feat_reduce_ws_t* ws, double* out0, double* out1)
+{
+  GridReduce g;
+  g.partials = ws ? ws->partials : nullptr;
+  g.counter = ws ? ws->counter : nullptr;
+  g.out0 = out0;
+  g.out1 = out1;
+  return g;
+}
+
+bool need_ws(const feat_reduce_ws_t* ws, const void* a, const void* b = nullptr)
+{
+  return (a != nullptr || b != nullptr) && (ws == nullptr || ws->partials == nullptr || ws->counter == nullptr);
+}
+} // namespace
+
+extern "C" {
+
+const char* feat_last_error(void) { return g_err.c_str(); }
+
+int feat_handle_create(feat_handle_t* out, int device, feat_stream_t stream)
+{
+  if(!out) return fail(FEAT_ERR_INVALID, "feat_handle_create: null handle pointer");
+  int ndev = 0;
+  if(hipGetDeviceCount(&ndev) != hipSuccess || ndev <= device || device < 0)
+    return fail(FEAT_ERR_NO_DEVICE, "feat_handle_create: no such HIP device");
+  int rc = check_hip(hipSetDevice(device), "hipSetDevice");
+  if(rc) return rc;
+  feat_handle_s* h = new feat_handle_s();
+  h->device = device;
+  // NULL is the device's null stream (ordered with every blocking stream,
+  // torch's default stream included); the handle never creates a stream.
+  h->own_stream = false;
+  h->stream = hipStream_t(stream);
+  rc = check_hip(hipMalloc(&h->partials, sizeof(double) * 2 * kMaxGrid), "hipMalloc(partials)");
+  if(!rc) rc = check_hip(hipMalloc(&h->counter, 64), "hipMalloc(counter)");
+  if(!rc) rc = check_hip(hipMalloc(&h->scal, sizeof(double) * 16), "hipMalloc(scal)");
+  if(!rc) rc = check_hip(hipHostMalloc(&h->host, sizeof(double) * 16, 0), "hipHostMalloc");
+  if(!rc) rc = check_hip(hipMemsetAsync(h->counter, 0, 64, h->stream), "hipMemset");
+  if(!rc) rc = check_hip(hipStreamSynchronize(h->stream), "hipStreamSynchronize");
+  if(rc)
+  {
+    feat_handle_destroy(h);
+    return rc;
+  }
+  *out = h;
+  return FEAT_OK;
+}
+
+int feat_handle_destroy(feat_handle_t h)
+{
+  if(!h) return FEAT_OK;
+  (void)hipSetDevice(h->device);
+  (void)hipStreamSynchronize(h->stream);
+  if(h->partials) (void)hipFree(h->partials);
+  if(h->counter) (void)hipFree(h->counter);
+  if(h->scal) (void)hipFree(h->scal);
+  if(h->host) (void)hipHostFree(h->host);
+  if(h->own_stream && h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+  return FEAT_OK;
+}
+
+int feat_handle_stream(feat_handle_t h, feat_stream_t* s)
+{
+  if(!h || !s) return fail(FEAT_ERR_INVALID, "feat_handle_stream: null argument");
+  *s = h->stream;
+  return FEAT_OK;
+}
+
+int feat_sync(feat_handle_t h)
+{
+  if(!h) return fail(FEAT_ERR_INVALID, "feat_sync: null handle");
+  return check_hip(hipStreamSynchronize(h->stream), "hipStreamSynchronize");
+}
+
+int feat_malloc(void** p, size_t bytes)
+{
+  if(!p) return fail(FEAT_ERR_INVALID, "feat_malloc: null pointer");
+  if(bytes == 0) bytes = 8;
+  hipError_t e = hipMalloc(p, bytes);
+  if(e != hipSuccess) return fail(FEAT_ERR_NOMEM, "feat_malloc: hipMalloc failed");
+  return FEAT_OK;
+}
+
+int feat_free(void* p) { return p ? check_hip(hipFree(p), "hipFree") : FEAT_OK; }
+
+int feat_host_malloc(void** p, size_t bytes)
+{
+  if(!p) return fail(FEAT_ERR_INVALID, "feat_host_malloc: null pointer");
+  hipError_t e = hipHostMalloc(p, bytes ? bytes : 8, 0);
+  if(e != hipSuccess) return fail(FEAT_ERR_NOMEM, "feat_host_malloc: hipHostMalloc failed");
+  return FEAT_OK;
+}
+
+int feat_host_free(void* p) { return p ? check_hip(hipHostFree(p), "hipHostFree") : FEAT_OK; }
+
+int feat_copy(feat_handle_t h, void* dst, const void* src, size_t bytes)
+{
+  if(!h || (bytes && (!dst || !src))) return fail(FEAT_ERR_INVALID, "feat_copy: null argument");
+  if(!bytes || dst == src) return FEAT_OK;
+  return check_hip(hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, h->stream), "hipMemcpyAsync");
+}
+
+int feat_set_memory_f64(feat_handle_t h, double* p, double v, uint64_t n)
+{
+  if(!h || (n && !p)) return fail(FEAT_ERR_INVALID, "feat_set_memory_f64: null argument");
+  if(!n) return FEAT_OK;
+  hipLaunchKernelGGL(k_fill<double>, dim3(grid_for(n, 4)), dim3(kThreads), 0, h->stream, p, v, n);
+  return check_launch("k_fill");
+}
+
+int feat_set_memory_f32(feat_handle_t h, float* p, float v, uint64_t n)
+{
+  if(!h || (n && !p)) return fail(FEAT_ERR_INVALID, "feat_set_memory_f32: null argument");
+  if(!n) return FEAT_OK;
+  hipLaunchKernelGGL(k_fill<float>, dim3(grid_for(n, 4)), dim3(kThreads), 0, h->stream, p, v, n);
+  return check_launch("k_fill");
+}
+
+#define CONVERT_FN(NAME, DO, DI)                                                                  \
+  int NAME(feat_handle_t h, DO* dst, const DI* src, uint64_t n)                                  \
+  {                                                                                               \
+    if(!h || (n && (!dst || !src))) return fail(FEAT_ERR_INVALID, #NAME ": null argument");     \
+    if(!n) return FEAT_OK;                                                                        \
+    hipLaunchKernelGGL((k_convert<DO, DI>), dim3(grid_for(n, 4)), dim3(kThreads), 0, h->stream, \
+                       dst, src, n);                                                              \
+    return check_launch(#NAME);                                                                   \
+  }
+CONVERT_FN(feat_convert_f64_f32, float, double)
+CONVERT_FN(feat_convert_f32_f64, double, float)
+CONVERT_FN(feat_convert_u64_u32, uint32_t, uint64_t)
+#undef CONVERT_FN
+
+int feat_convert_f64_f16(feat_handle_t h, uint16_t* dst, const double* src, uint64_t n)
+{
+  if(!h || (n && (!dst || !src))) return fail(FEAT_ERR_INVALID, "feat_convert_f64_f16: null argument");
+  if(!n) return FEAT_OK;
+  hipLaunchKernelGGL(k_convert_f64_f16, dim3(grid_for(n, 4)), dim3(kThreads), 0, h->stream, dst, src, n);
+  return check_launch("k_convert_f64_f16");
+}
+
+int feat_convert_f16_f64(feat_handle_t h, double* dst, const uint16_t* src, uint64_t n)
+{
+  if(!h || (n && (!dst || !src))) return fail(FEAT_ERR_INVALID, "feat_convert_f16_f64: null argument");
+  if(!n) return FEAT_OK;
+  hipLaunchKernelGGL(k_convert_f16_f64, dim3(grid_for(n, 4)), dim3(kThreads), 0, h->stream, dst, src, n);
+  return check_launch("k_convert_f16_f64");
+}
+
+// ---------------------------------------------------------------------------
+// SpMV
+// ---------------------------------------------------------------------------
+} // extern "C"
+
+namespace {
+template<typename DT, typename IT>
+int csr_apply_impl(feat_handle_t h, DT* r, DT a, const DT* x, DT b, const DT* y, const DT* val,
+                   const IT* col, const IT* rowptr, uint64_t rows, uint64_t cols, uint64_t nnz,
+                   int transposed, const char* name)
+{
+  if(!h) return fail(FEAT_ERR_INVALID, "csr_apply: null handle");
+  const DT eps = std::is_same<DT, float>::value ? DT(FLT_EPSILON) : DT(DBL_EPSILON);
+  const bool bzero = (b < 0 ? -b : b) < eps;
+  const uint64_t nr = transposed ? cols : rows;
+  const uint64_t nx = transposed ? rows : cols;
+  if((nr && !r) || (nx && !x) || (nnz && (!val || !col)) || !rowptr || (!bzero && nr && !y))
+    return fail(FEAT_ERR_INVALID, "csr_apply: null argument");
+  if(r == x && nr) return fail(FEAT_ERR_ALIAS, "csr_apply: r and x must not alias");
+  if(!transposed)
+  {
+    EpiApply<DT> e{r, y, a, b, bzero};
+    GridReduce g{nullptr, nullptr, nullptr, nullptr};
+    return launch_rows<DT, IT>(h->stream, val, col, rowptr, x, rows, e, g, name);
+  }
+  if(!std::is_same<DT, double>::value)
+    return fail(FEAT_ERR_UNSUPPORTED, "csr_apply: transposed apply is FP64 only");
+  // r = (b/a)*y ; r[col] += val*x[row] ; r = a*r    (arch/apply_generic.hpp:37-53)
+  double* rd = reinterpret_cast<double*>(r);
+  if(nr)
+    hipLaunchKernelGGL(k_tr_init, dim3(grid_for(nr, 4)), dim3(kThreads), 0, h->stream, rd,
+                       reinterpret_cast<const double*>(y), double(b) / double(a), int(bzero), nr);
+  if(rows)
+    hipLaunchKernelGGL((k_tr_scatter<DT, IT>), dim3(grid_for(rows)), dim3(kThreads), 0, h->stream,
+                       r, x, val, col, rowptr, rows);
+  if(nr)
+    hipLaunchKernelGGL((k_tr_scale<DT>), dim3(grid_for(nr, 4)), dim3(kThreads), 0, h->stream, r, a, nr);
+  return check_launch(name);
+}
+} // namespace
+
+extern "C" {
+
+int feat_csr_apply_f64_u64(feat_handle_t h, double* r, double a, const double* x, double b,
+                           const double* y, const double* val, const uint64_t* col,
+                           const uint64_t* rowptr, uint64_t rows, uint64_t cols, uint64_t nnz,
+                           int transposed)
+{
+  return csr_apply_impl<double, uint64_t>(h, r, a, x, b, y, val, col, rowptr, rows, cols, nnz,
+                                          transposed, "feat_csr_apply_f64_u64");
+}
+
+int feat_csr_apply_f64_u32(feat_handle_t h, double* r, double a, const double* x, double b,
+                           const double* y, const double* val, const uint32_t* col,
+                           const uint32_t* rowptr, uint64_t rows, uint64_t cols, uint64_t nnz,
+                           int transposed)
+{
+  return csr_apply_impl<double, uint32_t>(h, r, a, x, b, y, val, col, rowptr, rows, cols, nnz,
+                                          transposed, "feat_csr_apply_f64_u32");
+}
+
+int feat_csr_apply_f32_u32(feat_handle_t h, float* r, float a, const float* x, float b,
+                           const float* y, const float* val, const uint32_t* col,
+                           const uint32_t* rowptr, uint64_t rows, uint64_t cols, uint64_t nnz,
+                           int transposed)
+{
+  return csr_apply_impl<float, uint32_t>(h, r, a, x, b, y, val, col, rowptr, rows, cols, nnz,
+                                         transposed, "feat_csr_apply_f32_u32");
+}
+
+int feat_csr_apply_f32_u64(feat_handle_t h, float* r, float a, const float* x, float b,
+                           const float* y, const float* val, const uint64_t* col,
+                           const uint64_t* rowptr, uint64_t rows, uint64_t cols, uint64_t nnz,
+                           int transposed)
+{
+  return csr_apply_impl<float, uint64_t>(h, r, a, x, b, y, val, col, rowptr, rows, cols, nnz,
+                                         transposed, "feat_csr_apply_f32_u64");
+}
+
+int feat_bcsr_apply_f64_u64(feat_handle_t h, int bh, int bw, double* r, double a,
+                            const double* x, double b, const double* y, const double* val,
+                            const uint64_t* col, const uint64_t* rowptr, uint64_t rows,
+                            uint64_t cols, uint64_t nnz)
+{
+  if(!h) return fail(FEAT_ERR_INVALID, "feat_bcsr_apply: null handle");
+  const bool bzero = std::fabs(b) < DBL_EPSILON;
+  if((rows && !r) || (cols && !x) || (nnz && (!val || !col)) || !rowptr || (!bzero && rows && !y))
+    return fail(FEAT_ERR_INVALID, "feat_bcsr_apply: null argument");
+  if(r == x && rows) return fail(FEAT_ERR_ALIAS, "feat_bcsr_apply: r and x must not alias");
+  if(!rows) return FEAT_OK;
+  dim3 g(grid_for(rows)), t(kThreads);
+  if(bh == 2 && bw == 2)
+    hipLaunchKernelGGL((k_bcsr<2, 2>), g, t, 0, h->stream, r, a, x, b, y, val, col, rowptr, rows, int(bzero));
+  else if(bh == 2 && bw == 1)
+    hipLaunchKernelGGL((k_bcsr<2, 1>), g, t, 0, h->stream, r, a, x, b, y, val, col, rowptr, rows, int(bzero));
+  else if(bh == 1 && bw == 2)
+    hipLaunchKernelGGL((k_bcsr<1, 2>), g, t, 0, h->stream, r, a, x, b, y, val, col, rowptr, rows, int(bzero));
+  else if(bh == 3 && bw == 3)
+    hipLaunchKernelGGL((k_bcsr<3, 3>), g, t, 0, h->stream, r, a, x, b, y, val, col, rowptr, rows, int(bzero));
+  else
+    return fail(FEAT_ERR_UNSUPPORTED, "feat_bcsr_apply: block shape not provided");
+  return check_launch("feat_bcsr_apply_f64_u64");
+}
+
+// ---------------------------------------------------------------------------
+// BLAS-1
+// ---------------------------------------------------------------------------
+int feat_axpy_f64(feat_handle_t h, double* r, double a, const double* x, const double* y, uint64_t n)
+{
+  if(!h || (n && (!r || !x || !y))) return fail(FEAT_ERR_INVALID, "feat_axpy_f64: null argument");
+  if(!n) return FEAT_OK;
+  const int two = (r == x && r != y) ? 1 : 0;
+  hipLaunchKernelGGL(k_axpy<double>, dim3(grid_for(n, 4)), dim3(kThreads), 0, h->stream, r, a, x, y, n, two);
+  return check_launch("feat_axpy_f64");
+}
+
+int feat_axpy_f32(feat_handle_t h, float* r, float a, const float* x, const float* y, uint64_t n)
+{
+  if(!h || (n && (!r || !x || !y))) return fail(FEAT_ERR_INVALID, "feat_axpy_f32: null argument");
+  if(!n) return FEAT_OK;
+  const int two = (r == x && r != y) ? 1 : 0;
+  hipLaunchKernelGGL(k_axpy<float>, dim3(grid_for(n, 4)), dim3(kThreads), 0, h->stream, r, a, x, y, n, two);
+  return check_launch("feat_axpy_f32");
+}
+
+} // extern "C"
+
+namespace {
+int reduce_impl(feat_handle_t h, double* res, const double* x, const double* y, const double* z,
+                uint64_t n, int kind, bool take_sqrt, const char* name)
+{
+  if(!h || !res || (n && (!x || !y || (kind == 2 && !z)))) return fail(FEAT_ERR_INVALID, name);
+  if(!n)
+    return feat_set_memory_f64(h, res, 0.0, 1);
+  hipLaunchKernelGGL(k_reduce, dim3(grid_for(n, 8)), dim3(kThreads), 0, h->stream, x, y, z, n, kind,
+                     handle_reduce(h, res, nullptr));
+  if(take_sqrt)
+    hipLaunchKernelGGL(k_sqrt_inplace, dim3(1), dim3(1), 0, h->stream, res);
+  return check_launch(name);
+}
+
+int to_host(feat_handle_t h, const double* dev, double* host)
+{
+  if(!host) return fail(FEAT_ERR_INVALID, "null host result");
+  int rc = check_hip(hipMemcpyAsync(h->host, dev, sizeof(double), hipMemcpyDeviceToHost, h->stream), "hipMemcpyAsync");
+  if(!rc) rc = check_hip(hipStreamSynchronize(h->stream), "hipStreamSynchronize");
+  if(!rc) *host = h->host[0];
+  return rc;
+}
+} // namespace
+
+extern "C" {
+
+int feat_dot_f64(feat_handle_t h, double* res, const double* x, const double* y, uint64_t n)
+{
+  const int kind = (x == y) ? 1 : 0;
+  return reduce_impl(h, res, x, y, nullptr, n, kind, false, "feat_dot_f64");
+}
+
+int feat_dot_f64_host(feat_handle_t h, double* res_host, const double* x, const double* y, uint64_t n)
+{
+  if(!h) return fail(FEAT_ERR_INVALID, "feat_dot_f64_host: null handle");
+  int rc = feat_dot_f64(h, h->scal, x, y, n);
+  return rc ? rc : to_host(h, h->scal, res_host);
+}
+
+int feat_triple_dot_f64_host(feat_handle_t h, double* res_host, const double* x, const double* y,
+                             const double* z, uint64_t n)
+{
+  if(!h) return fail(FEAT_ERR_INVALID, "feat_triple_dot_f64_host: null handle");
+  int rc = reduce_impl(h, h->scal, x, y, z, n, 2, false, "feat_triple_dot_f64_host");
+  return rc ? rc : to_host(h, h->scal, res_host);
+}
+
+int feat_norm2_f64(feat_handle_t h, double* res, const double* x, uint64_t n)
+{
+  return reduce_impl(h, res, x, x, nullptr, n, 1, true, "feat_norm2_f64");
+}
+
+int feat_norm2_f64_host(feat_handle_t h, double* res_host, const double* x, uint64_t n)
+{
+  if(!h) return fail(FEAT_ERR_INVALID, "feat_norm2_f64_host: null handle");
+  int rc = feat_norm2_f64(h, h->scal, x, n);
+  return rc ? rc : to_host(h, h->scal, res_host);
+}
+
+int feat_scale_f64(feat_handle_t h, double* r, const double* x, double s, uint64_t n)
+{
+  if(!h || (n && (!r || !x))) return fail(FEAT_ERR_INVALID, "feat_scale_f64: null argument");
+  if(!n) return FEAT_OK;
+  hipLaunchKernelGGL(k_scale, dim3(grid_for(n, 4)), dim3(kThreads), 0, h->stream, r, x, s, n);
+  return check_launch("feat_scale_f64");
+}
+
+int feat_component_product_f64(feat_handle_t h, double* r, const double* x, const double* y, uint64_t n)
+{
+  if(!h || (n && (!r || !x || !y))) return fail(FEAT_ERR_INVALID, "feat_component_product_f64: null argument");
+  if(!n) return FEAT_OK;
+  hipLaunchKernelGGL(k_cprod, dim3(grid_for(n, 4)), dim3(kThreads), 0, h->stream, r, x, y, n);
+  return check_launch("feat_component_product_f64");
+}
+
+int feat_component_invert_f64(feat_handle_t h, double* r, const double* x, double s, uint64_t n)
+{
+  if(!h || (n && (!r || !x))) return fail(FEAT_ERR_INVALID, "feat_component_invert_f64: null argument");
+  if(!n) return FEAT_OK;
+  hipLaunchKernelGGL(k_cinv, dim3(grid_for(n, 4)), dim3(kThreads), 0, h->stream, r, x, s, n);
+  return check_launch("feat_component_invert_f64");
+}
+
+int feat_csr_diag_index_u64(feat_handle_t h, uint64_t* diag, const uint64_t* col, const uint64_t* rowptr,
+                            uint64_t rows)
+{
+  if(!h || !rowptr || (rows && (!diag || !col))) return fail(FEAT_ERR_INVALID, "feat_csr_diag_index_u64: null argument");
+  if(!rows) return FEAT_OK;
+  hipLaunchKernelGGL(k_diag_index<uint64_t>, dim3(grid_for(rows)), dim3(kThreads), 0, h->stream, diag, col, rowptr, rows);
+  return check_launch("feat_csr_diag_index_u64");
+}
+
+int feat_csr_extract_diag_f64_u64(feat_handle_t h, double* diag, const double* val, const uint64_t* col,
+                                  const uint64_t* rowptr, uint64_t rows)
+{
+  if(!h || !rowptr || (rows && (!diag || !col || !val))) return fail(FEAT_ERR_INVALID, "feat_csr_extract_diag: null argument");
+  if(!rows) return FEAT_OK;
+  hipLaunchKernelGGL(k_extract_diag, dim3(grid_for(rows)), dim3(kThreads), 0, h->stream, diag, val, col, rowptr, rows);
+  return check_launch("feat_csr_extract_diag_f64_u64");
+}
+
+// ---------------------------------------------------------------------------
+// filters
+// ---------------------------------------------------------------------------
+int feat_unit_filter_rhs_f64_u64(feat_handle_t h, double* v, const double* sv_val, const uint64_t* sv_idx, uint64_t ue)
+{
+  if(!h || (ue && (!v || !sv_val || !sv_idx))) return fail(FEAT_ERR_INVALID, "feat_unit_filter_rhs: null argument");
+  if(!ue) return FEAT_OK;
+  hipLaunchKernelGGL(k_filter_rhs, dim3(grid_for(ue)), dim3(kThreads), 0, h->stream, v, sv_val, sv_idx, ue);
+  return check_launch("feat_unit_filter_rhs_f64_u64");
+}
+
+int feat_unit_filter_def_f64_u64(feat_handle_t h, double* v, const uint64_t* sv_idx, uint64_t ue)
+{
+  if(!h || (ue && (!v || !sv_idx))) return fail(FEAT_ERR_INVALID, "feat_unit_filter_def: null argument");
+  if(!ue) return FEAT_OK;
+  hipLaunchKernelGGL(k_filter_def, dim3(grid_for(ue)), dim3(kThreads), 0, h->stream, v, sv_idx, ue);
+  return check_launch("feat_unit_filter_def_f64_u64");
+}
+
+int feat_unit_filter_mask_u64(feat_handle_t h, uint32_t* mask, uint64_t n, const uint64_t* sv_idx, uint64_t ue)
+{
+  const uint64_t nw = (n + 31) / 32;
+  if(!h || (nw && !mask) || (ue && !sv_idx)) return fail(FEAT_ERR_INVALID, "feat_unit_filter_mask: null argument");
+  if(nw)
+    hipLaunchKernelGGL(k_mask_zero, dim3(grid_for(nw)), dim3(kThreads), 0, h->stream, mask, nw);
+  if(ue)
+    hipLaunchKernelGGL(k_mask_set, dim3(grid_for(ue)), dim3(kThreads), 0, h->stream, mask, sv_idx, ue);
+  return check_launch("feat_unit_filter_mask_u64");
+}
+
+int feat_unit_filter_blocked_def_f64_u64(feat_handle_t h, int bs, double* v, const uint64_t* sv_idx, uint64_t ue)
+{
+  if(!h || bs <= 0 || (ue && (!v || !sv_idx))) return fail(FEAT_ERR_INVALID, "feat_unit_filter_blocked_def: bad argument");
+  if(!ue) return FEAT_OK;
+  hipLaunchKernelGGL(k_filter_blocked_def, dim3(grid_for(ue)), dim3(kThreads), 0, h->stream, v, bs, sv_idx, ue);
+  return check_launch("feat_unit_filter_blocked_def_f64_u64");
+}
+
+int feat_mean_filter_f64(feat_handle_t h, double* v, const double* w_dot, const double* w_add, double volume, uint64_t n)
+{
+  if(!h || (n && (!v || !w_dot || !w_add))) return fail(FEAT_ERR_INVALID, "feat_mean_filter_f64: null argument");
+  if(!n) return FEAT_OK;
+  int rc = reduce_impl(h, h->scal + 1, v, w_dot, nullptr, n, (v == w_dot) ? 1 : 0, false, "feat_mean_filter_f64");
+  if(rc) return rc;
+  hipLaunchKernelGGL(k_mean_apply, dim3(grid_for(n, 4)), dim3(kThreads), 0, h->stream, v, w_add, h->scal + 1, volume, n);
+  return check_launch("feat_mean_filter_f64");
+}
+
+// ---------------------------------------------------------------------------
+// fused solver kernels
+// ---------------------------------------------------------------------------
+} // extern "C"
+
+namespace {
+template<typename DT, typename IT, typename M>
+bool csr_ok(const M* A)
+{
+  return A && A->rowptr && (A->nnz == 0 || (A->val && A->col));
+}
+
+template<typename DT, typename IT, typename M>
+int defect_impl(feat_handle_t h, const M* A, DT* d, const DT* rhs, const DT* x, const uint32_t* mask,
+                double* nrm2, const feat_reduce_ws_t* ws, const char* name)
+{
+  if(!h || !csr_ok<DT, IT>(A) || (A->rows && (!d || !rhs || !x))) return fail(FEAT_ERR_INVALID, name);
+  if(d == x) return fail(FEAT_ERR_ALIAS, name);
+  if(need_ws(ws, nrm2)) return fail(FEAT_ERR_INVALID, "reduction workspace missing");
+  EpiDefect<DT> e{d, rhs, mask, nrm2 != nullptr};
+  return launch_rows<DT, IT>(h->stream, A->val, A->col, A->rowptr, x, A->rows, e, ws_reduce(ws, nrm2, nullptr), name);
+}
+
+template<typename DT, typename IT, typename M>
+int masked_impl(feat_handle_t h, const M* A, DT* r, const DT* x, const uint32_t* mask, double* nrm2,
+                const DT* dotw, double* dot, const feat_reduce_ws_t* ws, const char* name)
+{
+  if(!h || !csr_ok<DT, IT>(A) || (A->rows && !r) || (A->cols && !x) || (dot && !dotw)) return fail(FEAT_ERR_INVALID, name);
+  if(r == x) return fail(FEAT_ERR_ALIAS, name);
+  if(need_ws(ws, nrm2, dot)) return fail(FEAT_ERR_INVALID, "reduction workspace missing");
+  EpiMasked<DT> e{r, mask, dotw, nrm2 != nullptr, dot != nullptr};
+  return launch_rows<DT, IT>(h->stream, A->val, A->col, A->rowptr, x, A->rows, e, ws_reduce(ws, nrm2, dot), name);
+}
+
+template<typename DT, typename IT, typename M>
+int prol_impl(feat_handle_t h, const M* P, DT* sol, const DT* xc, const uint32_t* mask, const char* name)
+{
+  if(!h || !csr_ok<DT, IT>(P) || (P->rows && !sol) || (P->cols && !xc)) return fail(FEAT_ERR_INVALID, name);
+  if(sol == xc) return fail(FEAT_ERR_ALIAS, name);
+  EpiProlAdd<DT> e{sol, mask};
+  GridReduce g{nullptr, nullptr, nullptr, nullptr};
+  return launch_rows<DT, IT>(h->stream, P->val, P->col, P->rowptr, xc, P->rows, e, g, name);
+}
+
+// Buffer holding x_k: the final iterate lands in `x` (pre/plain modes).
+template<typename DT>
+int smooth_impl_generic(feat_handle_t h, const DT* val, const void* colv, const void* rpv, bool u32,
+                        uint64_t n, const DT* inv, const uint32_t* mask, int steps, DT omega,
+                        const DT* rhs, const double* rhs_nrm2, DT* x, DT* tmp, DT* d_out, DT* acc,
+                        const DT* dotw, double* dot, const feat_reduce_ws_t* ws, const char* name)
+{
+  if(!h || steps < 1 || (n && (!inv || !rhs || !tmp || !x))) return fail(FEAT_ERR_INVALID, name);
+  if(x == tmp || (acc && (acc == x || acc == tmp)) || (d_out && (d_out == x || d_out == tmp)))
+    return fail(FEAT_ERR_ALIAS, name);
+  if(need_ws(ws, dot)) return fail(FEAT_ERR_INVALID, "reduction workspace missing");
+  if(d_out && acc) return fail(FEAT_ERR_INVALID, "rj_smooth: d_out and acc are exclusive");
+  if(!n) return FEAT_OK;
+  DT* buf[2] = {x, tmp};
+  // number of SpMV steps: pre (d_out): steps; acc: steps-1; plain: steps-1
+  const int nsp = d_out ? steps : steps - 1;
+  // x_1 goes to the buffer such that the last stored iterate x_{steps} lands in x:
+  // iterate k (1-based) lives in buf[(steps - k) & 1]
+  auto B = [&](int k) { return buf[(steps - k) & 1]; };
+  GridReduce none{nullptr, nullptr, nullptr, nullptr};
+  const bool acc_first = (acc != nullptr && steps == 1);
+  {
+    GridReduce g = acc_first ? ws_reduce(ws, nullptr, dot) : none;
+    hipLaunchKernelGGL((k_smooth_first<DT>), dim3(grid_for(n, 4)), dim3(kThreads), 0, h->stream, B(1), inv, rhs, mask,
+                       omega, rhs_nrm2, acc_first ? acc : (DT*)nullptr, acc_first ? dotw : (const DT*)nullptr, n, g);
+    int rc = check_launch(name);
+    if(rc) return rc;
+  }
+  for(int k = 1; k <= nsp; ++k)
+  {
+    EpiSmooth<DT> e;
+    e.rhs = rhs;
+    e.inv = inv;
+    e.mask = mask;
+    e.xk = B(k);
+    e.dotw = nullptr;
+    e.rhs_nrm2 = rhs_nrm2;
+    e.omega = omega;
+    e.skip = false;
+    GridReduce g = none;
+    if(d_out && k == steps)
+    {
+      e.mode = 1;
+      e.out = d_out;
+    }
+    else if(acc && k == steps - 1)
+    {
+      e.mode = 2;
+      e.out = acc;
+      e.dotw = dotw;
+      g = ws_reduce(ws, nullptr, dot);
+    }
+    else
+    {
+      e.mode = 0;
+      e.out = B(k + 1);
+    }
+    int rc;
+    if(u32)
+      rc = launch_rows<DT, uint32_t>(h->stream, val, (const uint32_t*)colv, (const uint32_t*)rpv, e.xk, n, e, g, name);
+    else
+      rc = launch_rows<DT, uint64_t>(h->stream, val, (const uint64_t*)colv, (const uint64_t*)rpv, e.xk, n, e, g, name);
+    if(rc) return rc;
+  }
+  return FEAT_OK;
+}
+} // namespace
+
+extern "C" {
+
+int feat_csr_defect_f64(feat_handle_t h, const feat_csr_f64_t* A, double* d, const double* rhs, const double* x,
+                        const uint32_t* mask, double* nrm2, const feat_reduce_ws_t* ws)
+{
+  return defect_impl<double, uint64_t>(h, A, d, rhs, x, mask, nrm2, ws, "feat_csr_defect_f64");
+}
+
+int feat_csr_apply_masked_f64(feat_handle_t h, const feat_csr_f64_t* A, double* r, const double* x,
+                              const uint32_t* mask, double* nrm2, const double* dotw, double* dot,
+                              const feat_reduce_ws_t* ws)
+{
+  return masked_impl<double, uint64_t>(h, A, r, x, mask, nrm2, dotw, dot, ws, "feat_csr_apply_masked_f64");
+}
+
+int feat_csr_prol_add_f64(feat_handle_t h, const feat_csr_f64_t* P, double* sol, const double* xc, const uint32_t* mask)
+{
+  return prol_impl<double, uint64_t>(h, P, sol, xc, mask, "feat_csr_prol_add_f64");
+}
+
+int feat_rj_smooth_f64(feat_handle_t h, const feat_csr_f64_t* A, const double* inv_diag, const uint32_t* mask,
+                       int steps, double omega, const double* rhs, const double* rhs_nrm2, double* x, double* tmp,
+                       double* d_out, double* acc, const double* dotw, double* dot, const feat_reduce_ws_t* ws)
+{
+  if(!A || !csr_ok<double, uint64_t>(A)) return fail(FEAT_ERR_INVALID, "feat_rj_smooth_f64: bad matrix");
+  if(A->rows != A->cols) return fail(FEAT_ERR_INVALID, "feat_rj_smooth_f64: matrix not square");
+  return smooth_impl_generic<double>(h, A->val, A->col, A->rowptr, false, A->rows, inv_diag, mask, steps, omega, rhs,
+                                     rhs_nrm2, x, tmp, d_out, acc, dotw, dot, ws, "feat_rj_smooth_f64");
+}
+
+int feat_dense_gemv_f64(feat_handle_t h, int n, const double* ainv, const double* b, double* x, const uint32_t* mask)
+{
+  if(!h || n < 0 || (n && (!ainv || !b || !x))) return fail(FEAT_ERR_INVALID, "feat_dense_gemv_f64: bad argument");
+  if(b == x && n) return fail(FEAT_ERR_ALIAS, "feat_dense_gemv_f64: b and x must not alias");
+  if(!n) return FEAT_OK;
+  hipLaunchKernelGGL(k_dense_gemv<double>, dim3((n + kThreads - 1) / kThreads), dim3(kThreads), 0, h->stream, n, ainv, b, x, mask);
+  return check_launch("feat_dense_gemv_f64");
+}
+
+int feat_pcg_update_f64(feat_handle_t h, double* x, double* r, const double* p, const double* q,
+                        const double* gamma, const double* qp, double* rnrm2, uint64_t n,
+                        const feat_reduce_ws_t* ws)
+{
+  if(!h || !gamma || !qp || !rnrm2 || (n && (!x || !r || !p || !q))) return fail(FEAT_ERR_INVALID, "feat_pcg_update_f64: null argument");
+  if(need_ws(ws, rnrm2)) return fail(FEAT_ERR_INVALID, "reduction workspace missing");
+  hipLaunchKernelGGL(k_pcg_update, dim3(grid_for(n, 4)), dim3(kThreads), 0, h->stream, x, r, p, q, gamma, qp, n,
+                     ws_reduce(ws, rnrm2, nullptr));
+  return check_launch("feat_pcg_update_f64");
+}
+
+int feat_pcg_direction_f64(feat_handle_t h, double* p, const double* z, const double* gamma_new, double* gamma_old, uint64_t n)
+{
+  if(!h || !gamma_new || !gamma_old || (n && (!p || !z))) return fail(FEAT_ERR_INVALID, "feat_pcg_direction_f64: null argument");
+  if(n)
+    hipLaunchKernelGGL(k_pcg_direction, dim3(grid_for(n, 4)), dim3(kThreads), 0, h->stream, p, z, gamma_new, gamma_old, n);
+  hipLaunchKernelGGL(k_copy_scalar, dim3(1), dim3(1), 0, h->stream, gamma_old, gamma_new);
+  return check_launch("feat_pcg_direction_f64");
+}
+
+int feat_read_scalar_f64(feat_handle_t h, const double* dev, double* host)
+{
+  if(!h || !dev || !host) return fail(FEAT_ERR_INVALID, "feat_read_scalar_f64: null argument");
+  return to_host(h, dev, host);
+}
+
+// ---------------- FP32 inner solver kernels ----------------
+int feat_csr_apply_masked_f32(feat_handle_t h, const feat_csr_f32_t* A, float* r, const float* x,
+                              const uint32_t* mask, double* nrm2, const feat_reduce_ws_t* ws)
+{
+  return masked_impl<float, uint32_t>(h, A, r, x, mask, nrm2, nullptr, nullptr, ws, "feat_csr_apply_masked_f32");
+}
+
+int feat_csr_defect_f32(feat_handle_t h, const feat_csr_f32_t* A, float* d, const float* rhs, const float* x,
+                        const uint32_t* mask, double* nrm2, const feat_reduce_ws_t* ws)
+{
+  return defect_impl<float, uint32_t>(h, A, d, rhs, x, mask, nrm2, ws, "feat_csr_defect_f32");
+}
+
+int feat_csr_prol_add_f32(feat_handle_t h, const feat_csr_f32_t* P, float* sol, const float* xc, const uint32_t* mask)
+{
+  return prol_impl<float, uint32_t>(h, P, sol, xc, mask, "feat_csr_prol_add_f32");
+}
+
+int feat_rj_smooth_f32(feat_handle_t h, const feat_csr_f32_t* A, const float* inv_diag, const uint32_t* mask,
+                       int steps, float omega, const float* rhs, const double* rhs_nrm2, float* x, float* tmp,
+                       float* d_out, float* acc, const feat_reduce_ws_t* ws)
+{
+  if(!A || !csr_ok<float, uint32_t>(A)) return fail(FEAT_ERR_INVALID, "feat_rj_smooth_f32: bad matrix");
+  if(A->rows != A->cols) return fail(FEAT_ERR_INVALID, "feat_rj_smooth_f32: matrix not square");
+  return smooth_impl_generic<float>(h, A->val, A->col, A->rowptr, true, A->rows, inv_diag, mask, steps, omega, rhs,
+                                    rhs_nrm2, x, tmp, d_out, acc, nullptr, nullptr, ws, "feat_rj_smooth_f32");
+}
+
+int feat_dense_gemv_f32(feat_handle_t h, int n, const float* ainv, const float* b, float* x, const uint32_t* mask)
+{
+  if(!h || n < 0 || (n && (!ainv || !b || !x))) return fail(FEAT_ERR_INVALID, "feat_dense_gemv_f32: bad argument");
+  if(b == x && n) return fail(FEAT_ERR_ALIAS, "feat_dense_gemv_f32: b and x must not alias");
+  if(!n) return FEAT_OK;
+  hipLaunchKernelGGL(k_dense_gemv<float>, dim3((n + kThreads - 1) / kThreads), dim3(kThreads), 0, h->stream, n, ainv, b, x, mask);
+  return check_launch("feat_dense_gemv_f32");
+}
+
+} // extern "C"

@@ -1,0 +1,279 @@
+// host_capi.cpp -- extern "C" wrappers of the C++ host layer.
+// Built by g++ (no HIP headers) into libfeat3amd_host.so, linked against the
+// kernel library libfeat3amd.so; every device operation goes through feat3amd.h.
+#include <feat3amd/assembly.hpp>
+#include <feat3amd/lafem.hpp>
+#include <feat3amd/solver.hpp>
+#include <feat3amd_host.h>
+
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+using namespace feat3amd;
+typedef LAFEM::DenseVector<double, Index> Vec;
+typedef LAFEM::SparseMatrixCSR<double, Index> Csr;
+typedef LAFEM::UnitFilter<double, Index> Filt;
+typedef LAFEM::Transfer<Csr> Trans;
+typedef Solver::MultiGridHierarchy<Csr, Filt, Trans> Hier;
+typedef Solver::MultiGrid<Csr, Filt, Trans> MG;
+
+namespace
+{
+  thread_local std::string g_host_err;
+
+  template<typename F_>
+  int guarded(F_&& f)
+  {
+    try
+    {
+      f();
+      return FEAT_OK;
+    }
+    catch(const std::exception& e)
+    {
+      g_host_err = e.what();
+      return FEAT_ERR_INVALID;
+    }
+  }
+
+  struct LevelData
+  {
+    Csr A;
+    Filt F;
+    Trans T;
+    std::vector<double> hval;
+    std::vector<std::uint64_t> hcol, hrp, hfilt;
+  };
+}
+
+struct feat_mgpcg_s
+{
+  std::vector<std::unique_ptr<LevelData>> levels;
+  std::shared_ptr<Hier> hier;
+  std::shared_ptr<MG> mg;
+  std::shared_ptr<Solver::PCG<Csr, Filt>> pcg;
+};
+
+extern "C" {
+
+const char* feat_host_last_error(void) { return g_host_err.c_str(); }
+
+int feat_runtime_init(int device, void* stream)
+{
+  return guarded([&]() { Runtime::initialize(device, stream); });
+}
+
+int feat_q1_sizes(int dim, int level, std::uint64_t* sizes)
+{
+  return guarded([&]() {
+    if(dim != 2 && dim != 3) throw Error("feat_q1_sizes: dim must be 2 or 3");
+    const auto A = Assembly::q1_laplace(dim, level);
+    sizes[0] = A.rows;
+    sizes[1] = A.rowptr[A.rows];
+    sizes[2] = Assembly::q1_boundary(dim, level).size();
+    if(level > 0)
+    {
+      const auto P = Assembly::q1_prolongation(dim, level);
+      sizes[3] = P.rowptr[P.rows];
+      sizes[4] = P.cols;
+    }
+    else
+      sizes[3] = sizes[4] = 0;
+  });
+}
+
+int feat_q1_assemble(int dim, int level, double* val, std::uint64_t* col, std::uint64_t* rowptr, std::uint64_t* bnd,
+                     double* pval, std::uint64_t* pcol, std::uint64_t* prowptr)
+{
+  return guarded([&]() {
+    const auto A = Assembly::q1_laplace(dim, level);
+    std::memcpy(val, A.val.data(), sizeof(double) * A.val.size());
+    std::memcpy(col, A.col.data(), sizeof(std::uint64_t) * A.col.size());
+    std::memcpy(rowptr, A.rowptr.data(), sizeof(std::uint64_t) * A.rowptr.size());
+    const auto b = Assembly::q1_boundary(dim, level);
+    std::memcpy(bnd, b.data(), sizeof(std::uint64_t) * b.size());
+    if(level > 0 && pval)
+    {
+      const auto P = Assembly::q1_prolongation(dim, level);
+      std::memcpy(pval, P.val.data(), sizeof(double) * P.val.size());
+      std::memcpy(pcol, P.col.data(), sizeof(std::uint64_t) * P.col.size());
+      std::memcpy(prowptr, P.rowptr.data(), sizeof(std::uint64_t) * P.rowptr.size());
+    }
+  });
+}
+
+int feat_q1_rhs_sine(int dim, int level, double* b)
+{
+  return guarded([&]() {
+    const auto v = Assembly::q1_rhs_sine(dim, level);
+    std::memcpy(b, v.data(), sizeof(double) * v.size());
+  });
+}
+
+int feat_rhs_random(std::uint64_t seed, std::uint64_t n, double a, double b, double* out)
+{
+  return guarded([&]() { Assembly::random_fill(seed, out, n, a, b); });
+}
+
+static void build_solver(feat_mgpcg_s* s, const feat_mgpcg_cfg_t* cfg)
+{
+  const int nl = int(s->levels.size());
+  s->hier = std::make_shared<Hier>();
+  for(int i = 0; i < nl; ++i)
+  {
+    LevelData& L = *s->levels[i];
+    auto jac = Solver::new_jacobi_precond(L.A, L.F, cfg->omega_jac);
+    if(i + 1 < nl)
+    {
+      auto pre = Solver::new_richardson(L.A, L.F, cfg->omega_rich, jac);
+      pre->set_min_iter(cfg->nu_pre);
+      pre->set_max_iter(cfg->nu_pre);
+      auto post = Solver::new_richardson(L.A, L.F, cfg->omega_rich, jac);
+      post->set_min_iter(cfg->nu_post);
+      post->set_max_iter(cfg->nu_post);
+      s->hier->push_level(L.A, L.F, L.T, pre, post, pre);
+    }
+    else
+    {
+      std::shared_ptr<Solver::SolverBase<Vec>> cs;
+      if(cfg->coarse == FEAT_COARSE_SMOOTHER)
+      {
+        auto r = Solver::new_richardson(L.A, L.F, cfg->omega_rich, jac);
+        r->set_min_iter(cfg->nu_pre);
+        r->set_max_iter(cfg->nu_pre);
+        cs = r;
+      }
+      else if(cfg->coarse == FEAT_COARSE_JACOBI)
+        cs = Solver::new_jacobi_precond(L.A, L.F, cfg->coarse_omega);
+      else if(cfg->coarse == FEAT_COARSE_DIRECT)
+        cs = std::make_shared<Solver::DirectCoarseSolver<Csr, Filt>>(L.A, L.F, L.hval, L.hcol, L.hrp, L.hfilt);
+      s->hier->push_level(L.A, L.F, cs);
+    }
+  }
+  s->mg = std::make_shared<MG>(s->hier);
+  s->pcg = Solver::new_pcg(s->levels[0]->A, s->levels[0]->F, s->mg);
+  s->pcg->set_tol_rel(cfg->tol_rel);
+  s->pcg->set_max_iter(cfg->max_iter);
+  s->pcg->set_force_generic(cfg->force_generic != 0);
+  s->hier->init();
+  s->pcg->init();
+}
+
+static std::unique_ptr<LevelData> make_level(const Assembly::CsrHost& A, const std::vector<std::uint64_t>& filt,
+                                             const Assembly::CsrHost* P, bool keep_host)
+{
+  auto L = std::make_unique<LevelData>();
+  L->A = Csr::from_host(A.rows, A.cols, A.val.data(), A.col.data(), A.rowptr.data());
+  L->F = Filt(A.rows, filt.data(), nullptr, filt.size());
+  if(P)
+  {
+    const auto R = Assembly::transpose(*P);
+    L->T = Trans(Csr::from_host(P->rows, P->cols, P->val.data(), P->col.data(), P->rowptr.data()),
+                 Csr::from_host(R.rows, R.cols, R.val.data(), R.col.data(), R.rowptr.data()));
+  }
+  if(keep_host)
+  {
+    L->hval = A.val;
+    L->hcol = A.col;
+    L->hrp = A.rowptr;
+    L->hfilt = filt;
+  }
+  return L;
+}
+
+int feat_mgpcg_create(feat_mgpcg_t* out, const feat_level_host_t* lv, int nl, const feat_mgpcg_cfg_t* cfg)
+{
+  return guarded([&]() {
+    if(!out || !lv || nl < 1 || !cfg) throw Error("feat_mgpcg_create: bad argument");
+    auto s = std::make_unique<feat_mgpcg_s>();
+    for(int i = 0; i < nl; ++i)
+    {
+      Assembly::CsrHost A;
+      A.rows = A.cols = lv[i].rows;
+      A.rowptr.assign(lv[i].rowptr, lv[i].rowptr + lv[i].rows + 1);
+      A.val.assign(lv[i].val, lv[i].val + A.rowptr[A.rows]);
+      A.col.assign(lv[i].col, lv[i].col + A.rowptr[A.rows]);
+      std::vector<std::uint64_t> f(lv[i].filt, lv[i].filt + lv[i].nfilt);
+      Assembly::CsrHost P;
+      const bool hasP = (i + 1 < nl);
+      if(hasP)
+      {
+        P.rows = lv[i].rows;
+        P.cols = lv[i].pcols;
+        P.rowptr.assign(lv[i].prowptr, lv[i].prowptr + P.rows + 1);
+        P.val.assign(lv[i].pval, lv[i].pval + P.rowptr[P.rows]);
+        P.col.assign(lv[i].pcol, lv[i].pcol + P.rowptr[P.rows]);
+      }
+      s->levels.push_back(make_level(A, f, hasP ? &P : nullptr, i + 1 == nl));
+    }
+    build_solver(s.get(), cfg);
+    *out = s.release();
+  });
+}
+
+int feat_mgpcg_create_q1(feat_mgpcg_t* out, int dim, int lf, int lc, const feat_mgpcg_cfg_t* cfg)
+{
+  return guarded([&]() {
+    if(!out || !cfg || lf < lc || lc < 0 || (dim != 2 && dim != 3)) throw Error("feat_mgpcg_create_q1: bad argument");
+    auto s = std::make_unique<feat_mgpcg_s>();
+    for(int l = lf; l >= lc; --l)
+    {
+      const auto A = Assembly::q1_laplace(dim, l);
+      const auto f = Assembly::q1_boundary(dim, l);
+      if(l > lc)
+      {
+        const auto P = Assembly::q1_prolongation(dim, l);
+        s->levels.push_back(make_level(A, f, &P, false));
+      }
+      else
+        s->levels.push_back(make_level(A, f, nullptr, true));
+    }
+    build_solver(s.get(), cfg);
+    *out = s.release();
+  });
+}
+
+int feat_mgpcg_solve(feat_mgpcg_t s, double* x, const double* b, feat_solve_result_t* res, double* hist,
+                     std::uint64_t hist_cap)
+{
+  return guarded([&]() {
+    if(!s || !x || !b || !res) throw Error("feat_mgpcg_solve: bad argument");
+    const Index n = s->levels[0]->A.rows();
+    Vec vx = Vec::wrap(x, n);
+    Vec vb = Vec::wrap(const_cast<double*>(b), n);
+    Solver::Status st = s->pcg->correct(vx, vb);
+    Runtime::synchronize();
+    res->status = int(st);
+    res->iters = s->pcg->get_num_iter();
+    res->def_init = s->pcg->get_def_initial();
+    res->def_final = s->pcg->get_def_final();
+    if(hist)
+    {
+      const auto& h = s->pcg->history();
+      for(std::size_t i = 0; i < h.size() && i < hist_cap; ++i) hist[i] = h[i];
+    }
+  });
+}
+
+int feat_mgpcg_matrix(feat_mgpcg_t s, int lvl, std::uint64_t* rows, std::uint64_t* nnz, const double** val,
+                      const std::uint64_t** col, const std::uint64_t** rowptr)
+{
+  return guarded([&]() {
+    if(!s || lvl < 0 || lvl >= int(s->levels.size())) throw Error("feat_mgpcg_matrix: bad level");
+    const Csr& A = s->levels[lvl]->A;
+    *rows = A.rows();
+    *nnz = A.used_elements();
+    *val = A.val();
+    *col = A.col_ind();
+    *rowptr = A.row_ptr();
+  });
+}
+
+int feat_mgpcg_destroy(feat_mgpcg_t s)
+{
+  return guarded([&]() { delete s; });
+}
+
+} // extern "C"

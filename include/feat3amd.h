@@ -1,0 +1,258 @@
+/*
+ * feat3amd.h -- C-ABI of the MI355X (gfx950) LAFEM + Solver hot path.
+ *
+ * This is the drop-in boundary.  Each entry point replaces one raw-pointer
+ * kernel of the reference's LAFEM::Arch layer (kernel/lafem/arch/<op>.hpp,
+ * dispatched by BACKEND_SKELETON_* in kernel/backend.hpp:55-120) or one fused
+ * group of them used by the solver loops (kernel/solver/<solver>.hpp).  The citation
+ * above each declaration names the reference interface it replaces.
+ *
+ * Conventions
+ *  - All array arguments are DEVICE pointers (hipMalloc or feat_malloc) unless
+ *    the name ends in _host.  The callee never keeps or frees them.
+ *  - Every call is stream-ordered on the handle's stream and returns an int
+ *    status (FEAT_OK = 0).  Reductions write a DEVICE scalar; the *_host
+ *    variants synchronise the stream and return a host value, matching the
+ *    reference's `DT_ value(...)` signatures.
+ *  - Aliasing rules follow the reference: r == y allowed in SpMV
+ *    (lafem/sparse_matrix_csr.hpp:1799), r == x forbidden (:1750,1837);
+ *    axpy allows r == x and r == y (arch/axpy_generic.hpp:25-50).
+ *  - Index types: u64 (reference default `Index`) and u32 (FEAT_INDEX_U32).
+ *  - No entry point allocates device memory in the hot path: reductions use
+ *    the handle's preallocated workspace; fused solver kernels take their
+ *    workspace (feat_reduce_ws_t) from the caller.
+ */
+#ifndef FEAT3AMD_H
+#define FEAT3AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------- status codes ---------------- */
+#define FEAT_OK 0
+#define FEAT_ERR_INVALID 1     /* bad argument (NULL pointer, size mismatch)   */
+#define FEAT_ERR_HIP 2         /* HIP runtime error                            */
+#define FEAT_ERR_ALIAS 3       /* forbidden aliasing (e.g. SpMV r == x)        */
+#define FEAT_ERR_NOMEM 4       /* allocation failed (setup calls only)         */
+#define FEAT_ERR_UNSUPPORTED 5 /* variant not provided                         */
+#define FEAT_ERR_NO_DEVICE 6   /* no gfx950 device / library not initialised    */
+
+typedef struct feat_handle_s* feat_handle_t;
+typedef void* feat_stream_t; /* a hipStream_t */
+
+/* ---------------- runtime (kernel/runtime.cpp:37-148, util/cuda_util.cu:100-136) ---------------- */
+/* Thread-local message of the last failing call. */
+const char* feat_last_error(void);
+/* Creates a handle on `device` that launches on `stream` (NULL: the device's
+ * null stream).  Allocates the handle's reduction workspace. */
+int feat_handle_create(feat_handle_t* h, int device, feat_stream_t stream);
+int feat_handle_destroy(feat_handle_t h);
+int feat_handle_stream(feat_handle_t h, feat_stream_t* stream);
+int feat_sync(feat_handle_t h);
+/* device memory (MemoryPool::allocate_memory/release_memory, util/memory_pool.hpp:79-147) */
+int feat_malloc(void** p, size_t bytes);
+int feat_free(void* p);
+int feat_host_malloc(void** p, size_t bytes); /* pinned host memory */
+int feat_host_free(void* p);
+/* MemoryPool::copy / set_memory (util/memory_pool.hpp:181-251) */
+int feat_copy(feat_handle_t h, void* dst, const void* src, size_t bytes);      /* any direction */
+int feat_set_memory_f64(feat_handle_t h, double* p, double v, uint64_t n);
+int feat_set_memory_f32(feat_handle_t h, float* p, float v, uint64_t n);
+/* MemoryPool::convert (util/memory_pool.hpp:231-251, util/cuda_util.cu:226-244) */
+int feat_convert_f64_f32(feat_handle_t h, float* dst, const double* src, uint64_t n);
+int feat_convert_f32_f64(feat_handle_t h, double* dst, const float* src, uint64_t n);
+int feat_convert_f64_f16(feat_handle_t h, uint16_t* dst, const double* src, uint64_t n);
+int feat_convert_f16_f64(feat_handle_t h, double* dst, const uint16_t* src, uint64_t n);
+int feat_convert_u64_u32(feat_handle_t h, uint32_t* dst, const uint64_t* src, uint64_t n);
+
+/* ---------------- SpMV (arch/apply.hpp:51-79, generic arch/apply_generic.hpp:25-69) ----------------
+ * r = a*A*x + b*y, or r = a*A^T*x + b*y when transposed != 0.  |b| < eps
+ * means y is not read (r is treated as 0), as in the generic code.  The
+ * non-transposed form sums each row in storage order with one fma per entry
+ * (bit-identical to the sequential CPU loop); the transposed form scatters
+ * with atomics (rounding order not fixed). */
+int feat_csr_apply_f64_u64(feat_handle_t h, double* r, double a, const double* x, double b,
+                           const double* y, const double* val, const uint64_t* col,
+                           const uint64_t* rowptr, uint64_t rows, uint64_t cols, uint64_t nnz,
+                           int transposed);
+int feat_csr_apply_f64_u32(feat_handle_t h, double* r, double a, const double* x, double b,
+                           const double* y, const double* val, const uint32_t* col,
+                           const uint32_t* rowptr, uint64_t rows, uint64_t cols, uint64_t nnz,
+                           int transposed);
+int feat_csr_apply_f32_u32(feat_handle_t h, float* r, float a, const float* x, float b,
+                           const float* y, const float* val, const uint32_t* col,
+                           const uint32_t* rowptr, uint64_t rows, uint64_t cols, uint64_t nnz,
+                           int transposed);
+int feat_csr_apply_f32_u64(feat_handle_t h, float* r, float a, const float* x, float b,
+                           const float* y, const float* val, const uint64_t* col,
+                           const uint64_t* rowptr, uint64_t rows, uint64_t cols, uint64_t nnz,
+                           int transposed);
+
+/* BCSR with bh x bw row-major blocks, (bh,bw) in {(2,2),(2,1),(1,2),(3,3)}
+ * (arch/apply.hpp:108-133, generic arch/apply_generic.hpp:121-154). */
+int feat_bcsr_apply_f64_u64(feat_handle_t h, int bh, int bw, double* r, double a,
+                            const double* x, double b, const double* y, const double* val,
+                            const uint64_t* col, const uint64_t* rowptr, uint64_t rows,
+                            uint64_t cols, uint64_t nnz);
+
+/* ---------------- BLAS-1 (arch/{axpy,dot_product,norm,scale,component_product,component_invert}.hpp) ---------------- */
+/* Axpy::value: r = a*x + y (arch/axpy.hpp; generic arch/axpy_generic.hpp:25-50).
+ * r == x rounds a*x before adding y, like the generic two-statement branch. */
+int feat_axpy_f64(feat_handle_t h, double* r, double a, const double* x, const double* y,
+                  uint64_t n);
+int feat_axpy_f32(feat_handle_t h, float* r, float a, const float* x, const float* y,
+                  uint64_t n);
+/* DotProduct::value (arch/dot_product.hpp; generic :21-40): deterministic
+ * two-stage reduction; result written to the device scalar *res. */
+int feat_dot_f64(feat_handle_t h, double* res, const double* x, const double* y, uint64_t n);
+int feat_dot_f64_host(feat_handle_t h, double* res_host, const double* x, const double* y,
+                      uint64_t n);
+/* TripleDotProduct::value (arch/dot_product_generic.hpp:44-70): sum x*y*z */
+int feat_triple_dot_f64_host(feat_handle_t h, double* res_host, const double* x,
+                             const double* y, const double* z, uint64_t n);
+/* Norm2::value (arch/norm.hpp; generic arch/norm_generic.hpp:24-33): sqrt(sum x^2) */
+int feat_norm2_f64(feat_handle_t h, double* res, const double* x, uint64_t n);
+int feat_norm2_f64_host(feat_handle_t h, double* res_host, const double* x, uint64_t n);
+/* Scale::value (arch/scale_generic.hpp:22-38): r = x*s */
+int feat_scale_f64(feat_handle_t h, double* r, const double* x, double s, uint64_t n);
+/* ComponentProduct::value (arch/component_product_generic.hpp:22-52): r = x∘y (FP64
+ * throughout; the reference's CUDA kernel rounds through float, component_product.cu:27) */
+int feat_component_product_f64(feat_handle_t h, double* r, const double* x, const double* y,
+                               uint64_t n);
+/* ComponentInvert::value (arch/component_invert_generic.hpp:21-36): r = s/x */
+int feat_component_invert_f64(feat_handle_t h, double* r, const double* x, double s,
+                              uint64_t n);
+/* Diagonal::csr (arch/diagonal_generic.hpp:21-38): index of the diagonal
+ * entry of each row, nnz where absent */
+int feat_csr_diag_index_u64(feat_handle_t h, uint64_t* diag, const uint64_t* col,
+                            const uint64_t* rowptr, uint64_t rows);
+/* SparseMatrixCSR::extract_diag (lafem/sparse_matrix_csr.hpp:2277-2303): diag values, 0 where absent */
+int feat_csr_extract_diag_f64_u64(feat_handle_t h, double* diag, const double* val,
+                                  const uint64_t* col, const uint64_t* rowptr, uint64_t rows);
+
+/* ---------------- filters ---------------- */
+/* UnitFilter::filter_rhs/filter_sol (lafem/unit_filter.hpp:376-395,
+ * arch/unit_filter_generic.hpp:22-28): v[idx[i]] = sv[i] */
+int feat_unit_filter_rhs_f64_u64(feat_handle_t h, double* v, const double* sv_val,
+                                 const uint64_t* sv_idx, uint64_t ue);
+/* UnitFilter::filter_def/filter_cor (unit_filter.hpp:402-420,
+ * arch/unit_filter_generic.hpp:31-38): v[idx[i]] = 0 */
+int feat_unit_filter_def_f64_u64(feat_handle_t h, double* v, const uint64_t* sv_idx,
+                                 uint64_t ue);
+/* Bit mask of the unit filter (1 bit per row, 32-bit words, ceil(n/32) words,
+ * zeroed by the caller-visible call itself).  Used by the fused kernels to
+ * apply filter_def/filter_cor in their epilogues. */
+int feat_unit_filter_mask_u64(feat_handle_t h, uint32_t* mask, uint64_t n,
+                              const uint64_t* sv_idx, uint64_t ue);
+/* UnitFilterBlocked::filter_def (lafem/unit_filter_blocked.hpp:449-495): zero
+ * whole blocks of size bs at the given block indices */
+int feat_unit_filter_blocked_def_f64_u64(feat_handle_t h, int bs, double* v,
+                                         const uint64_t* sv_idx, uint64_t ue);
+/* MeanFilter::filter_def / filter_cor (lafem/mean_filter.hpp:214-264):
+ * v += (-(v . w_dot) / volume) * w_add, all on device */
+int feat_mean_filter_f64(feat_handle_t h, double* v, const double* w_dot, const double* w_add,
+                         double volume, uint64_t n);
+
+/* ---------------- fused solver kernels ----------------
+ * These replace op sequences of solver/{pcg,richardson,jacobi_precond,multigrid}.hpp
+ * with fewer HBM passes.  Each keeps the reference's per-element arithmetic
+ * (same roundings, same per-row summation order); only the order of the
+ * global sums inside dot/norm differs (deterministic tree).
+ *
+ * A reduction workspace: `partials` holds 2*FEAT_REDUCE_MAX_BLOCKS doubles,
+ * `counter` one zero-initialised uint32 (left at zero after every call). */
+#define FEAT_REDUCE_MAX_BLOCKS 4096
+typedef struct {
+  double* partials;
+  uint32_t* counter;
+} feat_reduce_ws_t;
+
+typedef struct {            /* a CSR matrix on the device */
+  uint64_t rows, cols, nnz;
+  const double* val;
+  const uint64_t* col;
+  const uint64_t* rowptr;
+} feat_csr_f64_t;
+
+typedef struct {            /* FP32 values with 32-bit indices (mixed precision) */
+  uint64_t rows, cols, nnz;
+  const float* val;
+  const uint32_t* col;
+  const uint32_t* rowptr;
+} feat_csr_f32_t;
+
+/* Defect d = rhs - A*x (SparseMatrixCSR::apply(r,x,y,-1), sparse_matrix_csr.hpp:1807-1843)
+ * followed by filter_def through `mask` (may be NULL) and, when nrm2 != NULL,
+ * the squared norm of the filtered defect into the device scalar *nrm2. */
+int feat_csr_defect_f64(feat_handle_t h, const feat_csr_f64_t* A, double* d, const double* rhs,
+                        const double* x, const uint32_t* mask, double* nrm2,
+                        const feat_reduce_ws_t* ws);
+/* r = A*x (b = 0) with filter mask and optional squared norm (Transfer::rest + filter_def,
+ * multigrid.hpp:1801-1805).  `dotw` != NULL adds the dot of the result with dotw into *dot. */
+int feat_csr_apply_masked_f64(feat_handle_t h, const feat_csr_f64_t* A, double* r,
+                              const double* x, const uint32_t* mask, double* nrm2,
+                              const double* dotw, double* dot, const feat_reduce_ws_t* ws);
+/* Prolongation + correction: sol += filter_cor(P*xc) (Transfer::prol, filter_cor, axpy;
+ * multigrid.hpp:1885-1930). */
+int feat_csr_prol_add_f64(feat_handle_t h, const feat_csr_f64_t* P, double* sol,
+                          const double* xc, const uint32_t* mask);
+
+/* Richardson(JacobiPrecond) smoother with min_iter == max_iter == steps
+ * (richardson.hpp:160-226, jacobi_precond.hpp:159-164): starting from x = 0,
+ * `steps` times { c = filter_cor(inv_diag∘d); x += omega*c; d = rhs - A*x; filter_def(d) }.
+ * The smoother is skipped (x = 0, d = rhs) when sqrt(*rhs_nrm2) is not finite or
+ * <= eps^2 (iterative.hpp:748-770); rhs_nrm2 is a device scalar.
+ *   x        out: the smoother result (n); scratch in acc mode.   tmp: scratch (n)
+ *   d_out    optional: the last filtered defect (n) -- equals the defect the multigrid
+ *            recomputes after pre-smoothing (multigrid.hpp:1754-1774)
+ *   acc      optional: if set, the result is added to it instead of stored in x
+ *            (post-smoothing `sol += cor`, multigrid.hpp:1977); the last defect is
+ *            then not formed (it is unused by the reference)
+ *   dotw/dot optional: dot(acc_after, dotw) into the device scalar *dot. */
+int feat_rj_smooth_f64(feat_handle_t h, const feat_csr_f64_t* A, const double* inv_diag,
+                       const uint32_t* mask, int steps, double omega, const double* rhs,
+                       const double* rhs_nrm2, double* x, double* tmp, double* d_out,
+                       double* acc, const double* dotw, double* dot,
+                       const feat_reduce_ws_t* ws);
+
+/* Dense coarse solve x = filter_cor(Ainv * b) with a row-major n x n inverse. */
+int feat_dense_gemv_f64(feat_handle_t h, int n, const double* ainv, const double* b, double* x,
+                        const uint32_t* mask);
+
+/* PCG updates (pcg.hpp:238-282) on device scalars:
+ *   feat_pcg_update: alpha = (*gamma)/(*qp); x += alpha*p; r -= alpha*q; *rnrm2 = r.r
+ *   feat_pcg_direction: beta = (*gamma_new)/(*gamma_old); p = beta*p + z (two roundings,
+ *                       the axpy r == x branch); *gamma_old = *gamma_new */
+int feat_pcg_update_f64(feat_handle_t h, double* x, double* r, const double* p, const double* q,
+                        const double* gamma, const double* qp, double* rnrm2, uint64_t n,
+                        const feat_reduce_ws_t* ws);
+int feat_pcg_direction_f64(feat_handle_t h, double* p, const double* z, const double* gamma_new,
+                           double* gamma_old, uint64_t n);
+
+/* Reads a device scalar into host memory after synchronising the stream. */
+int feat_read_scalar_f64(feat_handle_t h, const double* dev, double* host);
+
+/* ---------------- mixed precision (solver/convert_precond.hpp:100-119) ---------------- */
+int feat_csr_apply_masked_f32(feat_handle_t h, const feat_csr_f32_t* A, float* r,
+                              const float* x, const uint32_t* mask, double* nrm2,
+                              const feat_reduce_ws_t* ws);
+int feat_csr_defect_f32(feat_handle_t h, const feat_csr_f32_t* A, float* d, const float* rhs,
+                        const float* x, const uint32_t* mask, double* nrm2,
+                        const feat_reduce_ws_t* ws);
+int feat_csr_prol_add_f32(feat_handle_t h, const feat_csr_f32_t* P, float* sol,
+                          const float* xc, const uint32_t* mask);
+int feat_rj_smooth_f32(feat_handle_t h, const feat_csr_f32_t* A, const float* inv_diag,
+                       const uint32_t* mask, int steps, float omega, const float* rhs,
+                       const double* rhs_nrm2, float* x, float* tmp, float* d_out, float* acc,
+                       const feat_reduce_ws_t* ws);
+int feat_dense_gemv_f32(feat_handle_t h, int n, const float* ainv, const float* b, float* x,
+                        const uint32_t* mask);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,811 @@
+// feat3amd/solver.hpp -- Solver::SolverBase lifecycle and the iterative
+// solvers of the hot path, on device containers.
+//
+// Mirrors kernel/solver/{base,iterative,jacobi_precond,richardson,pcg,
+// multigrid}.hpp: init_symbolic/init_numeric/apply/done, the same status
+// codes, defaults, stopping rules and op order.  Two execution forms:
+//  * the generic form composes container ops exactly like the reference
+//    (host scalars, one synchronisation per reduction);
+//  * the fused form is taken by PCG(MultiGrid(Richardson(Jacobi))) and keeps
+//    every scalar on the device (one synchronisation per PCG iteration for the
+//    convergence test) and uses the fused kernels of feat3amd.h.  It computes
+//    the same per-element arithmetic; only the global-sum order of dots and
+//    norms differs.
+#pragma once
+
+#include <feat3amd/lafem.hpp>
+
+#include <cfloat>
+#include <cmath>
+#include <functional>
+#include <limits>
+#include <memory>
+#include <string>
+#include <vector>
+
+namespace feat3amd
+{
+  namespace Solver
+  {
+    /// solver/base.hpp:48-64
+    enum class Status { undefined = 0, progress, success, aborted, diverged, max_iter, stagnated };
+
+    inline bool status_success(Status s)
+    {
+      return s == Status::success || s == Status::max_iter || s == Status::stagnated;
+    }
+
+    /// Device workspace shared by the fused kernels of one solver tree.
+    struct FusedWorkspace
+    {
+      DeviceArray<double> partials;
+      DeviceArray<std::uint32_t> counter;
+      DeviceArray<double> scalars;
+      feat_reduce_ws_t ws{};
+      void create(Index nscal)
+      {
+        if(!partials.empty()) return;
+        partials = DeviceArray<double>(2 * FEAT_REDUCE_MAX_BLOCKS);
+        counter = DeviceArray<std::uint32_t>(16);
+        scalars = DeviceArray<double>(nscal);
+        std::vector<std::uint32_t> z(16, 0u);
+        check(feat_copy(Runtime::handle(), counter.get(), z.data(), sizeof(std::uint32_t) * 16), "workspace");
+        ws.partials = partials.get();
+        ws.counter = counter.get();
+      }
+      double* scalar(Index i) const { return scalars.get() + i; }
+    };
+
+    /// SolverBase<Vector_> (solver/base.hpp)
+    template<typename Vector_>
+    class SolverBase
+    {
+    public:
+      typedef Vector_ VectorType;
+      typedef typename Vector_::DataType DataType;
+      virtual ~SolverBase() {}
+      virtual std::string name() const = 0;
+      virtual void init_symbolic() {}
+      virtual void init_numeric() {}
+      virtual void done_numeric() {}
+      virtual void done_symbolic() {}
+      void init()
+      {
+        init_symbolic();
+        init_numeric();
+      }
+      void done()
+      {
+        done_numeric();
+        done_symbolic();
+      }
+      virtual Status apply(Vector_& vec_cor, const Vector_& vec_def) = 0;
+    };
+
+    /// IterativeSolver bookkeeping (solver/iterative.hpp:117-885)
+    template<typename Vector_>
+    class IterativeSolver : public SolverBase<Vector_>
+    {
+    public:
+      typedef typename Vector_::DataType DataType;
+
+    protected:
+      // defaults of iterative.hpp:213-236
+      DataType _tol_rel = std::sqrt(std::numeric_limits<DataType>::epsilon());
+      DataType _tol_abs = DataType(1) / (std::numeric_limits<DataType>::epsilon() * std::numeric_limits<DataType>::epsilon());
+      DataType _tol_abs_low = DataType(0);
+      DataType _div_rel = DataType(1) / std::numeric_limits<DataType>::epsilon();
+      DataType _div_abs = DataType(1) / (std::numeric_limits<DataType>::epsilon() * std::numeric_limits<DataType>::epsilon());
+      DataType _stag_rate = DataType(0.95);
+      Index _min_iter = 0, _max_iter = 100, _num_iter = 0, _min_stag_iter = 0, _num_stag_iter = 0;
+      DataType _def_init = 0, _def_cur = 0, _def_prev = 0;
+      Status _status = Status::undefined;
+      std::vector<DataType> _hist;
+
+    public:
+      void set_tol_rel(DataType t) { _tol_rel = t; }
+      void set_tol_abs(DataType t) { _tol_abs = t; }
+      void set_div_rel(DataType t) { _div_rel = t; }
+      void set_div_abs(DataType t) { _div_abs = t; }
+      void set_min_iter(Index n) { _min_iter = n; }
+      void set_max_iter(Index n) { _max_iter = n; }
+      void set_min_stag_iter(Index n) { _min_stag_iter = n; }
+      void set_stag_rate(DataType r) { _stag_rate = r; }
+      Index get_num_iter() const { return _num_iter; }
+      Index get_min_iter() const { return _min_iter; }
+      Index get_max_iter() const { return _max_iter; }
+      DataType get_def_initial() const { return _def_init; }
+      DataType get_def_final() const { return _def_cur; }
+      Status get_status() const { return _status; }
+      const std::vector<DataType>& history() const { return _hist; }
+      /// defect of a smoother is never computed (iterative.hpp:862-871)
+      bool skips_defect() const { return !(_min_iter < _max_iter) && _min_stag_iter == 0; }
+
+      virtual Status correct(Vector_& vec_sol, const Vector_& vec_rhs) = 0;
+
+    protected:
+      /// iterative.hpp:748-770
+      Status _set_initial_defect(DataType def)
+      {
+        _def_init = _def_cur = _def_prev = def;
+        _num_iter = 0;
+        _num_stag_iter = 0;
+        _hist.assign(1, def);
+        if(!std::isfinite(def)) return Status::aborted;
+        const DataType e = std::numeric_limits<DataType>::epsilon();
+        if(def <= e * e) return Status::success;
+        return Status::progress;
+      }
+
+      /// iterative.hpp:795-836
+      Status _analyse_defect(Index it, DataType cur, DataType prev, bool check_stag)
+      {
+        if(!std::isfinite(cur)) return Status::aborted;
+        if(cur > _div_abs || cur > _div_rel * _def_init) return Status::diverged;
+        if(it < _min_iter) return Status::progress;
+        if(cur <= _tol_abs && (cur <= _tol_rel * _def_init || cur <= _tol_abs_low)) return Status::success;
+        if(it >= _max_iter) return Status::max_iter;
+        if(check_stag && _min_stag_iter > 0)
+        {
+          if(cur >= _stag_rate * prev)
+          {
+            if(++_num_stag_iter >= _min_stag_iter) return Status::stagnated;
+          }
+          else
+            _num_stag_iter = 0;
+        }
+        return Status::progress;
+      }
+
+      /// iterative.hpp:853-885; `norm` is evaluated only when the reference computes it
+      Status _set_new_defect(const std::function<DataType()>& norm)
+      {
+        ++_num_iter;
+        _def_prev = _def_cur;
+        if(!skips_defect())
+        {
+          _def_cur = norm();
+          _hist.push_back(_def_cur);
+        }
+        return _analyse_defect(_num_iter, _def_cur, _def_prev, true);
+      }
+    };
+
+    /// JacobiPrecond (solver/jacobi_precond.hpp:136-164)
+    template<typename Matrix_, typename Filter_>
+    class JacobiPrecond : public SolverBase<typename Matrix_::VectorTypeL>
+    {
+    public:
+      typedef typename Matrix_::VectorTypeL VectorType;
+      typedef typename Matrix_::DataType DataType;
+
+    private:
+      const Matrix_& _matrix;
+      const Filter_& _filter;
+      DataType _omega;
+      VectorType _inv_diag;
+
+    public:
+      JacobiPrecond(const Matrix_& m, const Filter_& f, DataType omega = DataType(1)) : _matrix(m), _filter(f), _omega(omega) {}
+      std::string name() const override { return "Jacobi"; }
+      void init_symbolic() override { _inv_diag = _matrix.create_vector_r(); }
+      void init_numeric() override
+      {
+        _matrix.extract_diag(_inv_diag);
+        _inv_diag.component_invert(_inv_diag, _omega);
+      }
+      Status apply(VectorType& cor, const VectorType& def) override
+      {
+        cor.component_product(_inv_diag, def);
+        _filter.filter_cor(cor);
+        return Status::success;
+      }
+      const VectorType& inv_diag() const { return _inv_diag; }
+      const Matrix_& matrix() const { return _matrix; }
+      const Filter_& filter() const { return _filter; }
+      DataType omega() const { return _omega; }
+    };
+
+    template<typename Matrix_, typename Filter_>
+    std::shared_ptr<JacobiPrecond<Matrix_, Filter_>> new_jacobi_precond(const Matrix_& m, const Filter_& f,
+                                                                        typename Matrix_::DataType omega = 1)
+    {
+      return std::make_shared<JacobiPrecond<Matrix_, Filter_>>(m, f, omega);
+    }
+
+    /// PreconditionedIterativeSolver::_apply_precond (iterative.hpp:1067-1080)
+    template<typename Vector_, typename Filter_>
+    bool apply_precond(SolverBase<Vector_>* precond, Vector_& cor, const Vector_& def, const Filter_& filter)
+    {
+      if(precond) return status_success(precond->apply(cor, def));
+      cor.copy(def);
+      filter.filter_cor(cor);
+      return true;
+    }
+
+    /// Richardson (solver/richardson.hpp:160-226)
+    template<typename Matrix_, typename Filter_>
+    class Richardson : public IterativeSolver<typename Matrix_::VectorTypeL>
+    {
+    public:
+      typedef typename Matrix_::VectorTypeL VectorType;
+      typedef typename Matrix_::DataType DataType;
+      typedef SolverBase<VectorType> PrecondType;
+
+    private:
+      const Matrix_& _matrix;
+      const Filter_& _filter;
+      DataType _omega;
+      std::shared_ptr<PrecondType> _precond;
+      VectorType _vec_def, _vec_cor;
+
+    public:
+      Richardson(const Matrix_& m, const Filter_& f, DataType omega, std::shared_ptr<PrecondType> p = nullptr)
+        : _matrix(m), _filter(f), _omega(omega), _precond(p) {}
+      std::string name() const override { return "Richardson"; }
+      void init_symbolic() override
+      {
+        if(_precond) _precond->init_symbolic();
+        _vec_def = _matrix.create_vector_r();
+        _vec_cor = _matrix.create_vector_r();
+      }
+      void init_numeric() override
+      {
+        if(_precond) _precond->init_numeric();
+      }
+      DataType omega() const { return _omega; }
+      const Matrix_& matrix() const { return _matrix; }
+      const Filter_& filter() const { return _filter; }
+
+      /// The Jacobi preconditioner when this smoother can run fused
+      /// (min_iter == max_iter, no stagnation test), else nullptr.
+      JacobiPrecond<Matrix_, Filter_>* fusable_jacobi() const
+      {
+        if(!this->skips_defect() || this->_max_iter < 1) return nullptr;
+        return dynamic_cast<JacobiPrecond<Matrix_, Filter_>*>(_precond.get());
+      }
+
+      Status apply(VectorType& vec_cor, const VectorType& vec_def) override
+      {
+        _vec_def.copy(vec_def);
+        vec_cor.format();
+        this->_status = _apply_intern(vec_cor, vec_def);
+        return this->_status;
+      }
+
+      Status correct(VectorType& vec_sol, const VectorType& vec_rhs) override
+      {
+        _matrix.apply(_vec_def, vec_sol, vec_rhs, -DataType(1));
+        _filter.filter_def(_vec_def);
+        this->_status = _apply_intern(vec_sol, vec_rhs);
+        return this->_status;
+      }
+
+    private:
+      Status _apply_intern(VectorType& vec_sol, const VectorType& vec_rhs)
+      {
+        Status st = this->_set_initial_defect(_vec_def.norm2());
+        while(st == Status::progress)
+        {
+          if(!apply_precond(_precond.get(), _vec_cor, _vec_def, _filter)) return Status::aborted;
+          vec_sol.axpy(_vec_cor, vec_sol, _omega);
+          _matrix.apply(_vec_def, vec_sol, vec_rhs, -DataType(1));
+          _filter.filter_def(_vec_def);
+          st = this->_set_new_defect([&]() { return _vec_def.norm2(); });
+        }
+        return st;
+      }
+    };
+
+    template<typename Matrix_, typename Filter_>
+    std::shared_ptr<Richardson<Matrix_, Filter_>> new_richardson(
+      const Matrix_& m, const Filter_& f, typename Matrix_::DataType omega,
+      std::shared_ptr<SolverBase<typename Matrix_::VectorTypeL>> p = nullptr)
+    {
+      return std::make_shared<Richardson<Matrix_, Filter_>>(m, f, omega, p);
+    }
+
+    /// Exact coarse solver: inverse of the filtered coarse matrix (Dirichlet
+    /// rows/columns replaced by identity), inverted once on the host in FP64
+    /// by Gauss-Jordan with diagonal pivoting (the scheme of util/math.hpp:1169)
+    /// and applied on the device as a dense GEMV followed by filter_cor.
+    template<typename Matrix_, typename Filter_>
+    class DirectCoarseSolver : public SolverBase<typename Matrix_::VectorTypeL>
+    {
+    public:
+      typedef typename Matrix_::VectorTypeL VectorType;
+
+    private:
+      const Matrix_& _matrix;
+      const Filter_& _filter;
+      std::vector<double> _host_a, _host_val;
+      std::vector<std::uint64_t> _host_col, _host_rp, _host_filt;
+      DeviceArray<double> _inv;
+
+    public:
+      DirectCoarseSolver(const Matrix_& m, const Filter_& f, std::vector<double> val, std::vector<std::uint64_t> col,
+                         std::vector<std::uint64_t> rp, std::vector<std::uint64_t> filt)
+        : _matrix(m), _filter(f), _host_val(std::move(val)), _host_col(std::move(col)), _host_rp(std::move(rp)),
+          _host_filt(std::move(filt)) {}
+      std::string name() const override { return "DirectCoarse"; }
+      void init_numeric() override
+      {
+        const Index n = _matrix.rows();
+        if(n > 4096) throw Error("DirectCoarseSolver: coarse matrix too large for a dense inverse");
+        std::vector<char> f(n, 0);
+        for(auto i : _host_filt) f[i] = 1;
+        std::vector<double> a(n * n, 0.0);
+        for(Index r = 0; r < n; ++r)
+        {
+          if(f[r])
+          {
+            a[r * n + r] = 1.0;
+            continue;
+          }
+          for(Index i = _host_rp[r]; i < _host_rp[r + 1]; ++i)
+            if(!f[_host_col[i]]) a[r * n + _host_col[i]] = _host_val[i];
+        }
+        invert(int(n), a.data());
+        _inv = DeviceArray<double>::from_host(a.data(), n * n);
+      }
+      Status apply(VectorType& cor, const VectorType& def) override
+      {
+        check(feat_dense_gemv_f64(Runtime::handle(), int(_matrix.rows()), _inv.get(), (const double*)def.elements(),
+                                  (double*)cor.elements(), _filter.mask()),
+              "DirectCoarseSolver::apply");
+        return Status::success;
+      }
+      const double* inverse() const { return _inv.get(); }
+
+      /// Gauss-Jordan with diagonal pivoting; each update is one fused multiply-add.
+      static void invert(int n, double* a)
+      {
+        if(n == 1)
+        {
+          a[0] = 1.0 / a[0];
+          return;
+        }
+        std::vector<int> p(n);
+        for(int i = 0; i < n; ++i) p[i] = i;
+        for(int k = 0; k < n; ++k)
+        {
+          double piv = std::fabs(a[p[k] * n + p[k]]);
+          int i = k;
+          for(int j = k + 1; j < n; ++j)
+          {
+            const double av = std::fabs(a[p[j] * n + p[j]]);
+            if(av > piv) { piv = av; i = j; }
+          }
+          if(i > k) std::swap(p[k], p[i]);
+          const int pk = p[k] * n;
+          const double inv = 1.0 / a[pk + p[k]];
+          a[pk + p[k]] = 1.0;
+          for(int j = 0; j < n; ++j) a[pk + j] *= inv;
+          for(int r = 0; r < n; ++r)
+          {
+            if(r == p[k]) continue;
+            const int ro = r * n;
+            const double fac = a[ro + p[k]];
+            a[ro + p[k]] = 0.0;
+            for(int j = 0; j < n; ++j) a[ro + j] = std::fma(-a[pk + j], fac, a[ro + j]);
+          }
+        }
+      }
+    };
+
+    enum class MultiGridCycle { V };
+
+    /// MultiGridHierarchy (solver/multigrid.hpp:419-957); level 0 is the finest.
+    template<typename Matrix_, typename Filter_, typename Transfer_>
+    class MultiGridHierarchy
+    {
+    public:
+      typedef typename Matrix_::VectorTypeL VectorType;
+      typedef SolverBase<VectorType> SolverType;
+      struct Level
+      {
+        const Matrix_* matrix = nullptr;
+        const Filter_* filter = nullptr;
+        const Transfer_* transfer = nullptr;
+        std::shared_ptr<SolverType> pre, post, coarse;
+        VectorType rhs, sol, def, cor, tmp;
+      };
+      std::vector<Level> levels;
+
+      void push_level(const Matrix_& m, const Filter_& f, const Transfer_& t, std::shared_ptr<SolverType> pre,
+                      std::shared_ptr<SolverType> post, std::shared_ptr<SolverType> /*peak*/ = nullptr)
+      {
+        Level l;
+        l.matrix = &m;
+        l.filter = &f;
+        l.transfer = &t;
+        l.pre = pre;
+        l.post = post;
+        levels.push_back(l);
+      }
+      void push_level(const Matrix_& m, const Filter_& f, std::shared_ptr<SolverType> coarse)
+      {
+        Level l;
+        l.matrix = &m;
+        l.filter = &f;
+        l.coarse = coarse;
+        levels.push_back(l);
+      }
+      Index size() const { return levels.size(); }
+
+      void init_symbolic()
+      {
+        for(auto& l : levels)
+        {
+          l.rhs = l.matrix->create_vector_l();
+          l.sol = l.matrix->create_vector_l();
+          l.def = l.matrix->create_vector_l();
+          l.cor = l.matrix->create_vector_l();
+          l.tmp = l.matrix->create_vector_l();
+          for(auto* s : {l.pre.get(), l.post.get(), l.coarse.get()})
+            if(s) s->init_symbolic();
+        }
+      }
+      void init_numeric()
+      {
+        for(auto& l : levels)
+          for(auto* s : {l.pre.get(), l.post.get(), l.coarse.get()})
+            if(s) s->init_numeric();
+      }
+      void init()
+      {
+        init_symbolic();
+        init_numeric();
+      }
+      void done() {}
+    };
+
+    /// MultiGrid (solver/multigrid.hpp:984-2000), V-cycle with fixed coarse-grid correction.
+    template<typename Matrix_, typename Filter_, typename Transfer_>
+    class MultiGrid : public SolverBase<typename Matrix_::VectorTypeL>
+    {
+    public:
+      typedef typename Matrix_::VectorTypeL VectorType;
+      typedef MultiGridHierarchy<Matrix_, Filter_, Transfer_> HierarchyType;
+      typedef Richardson<Matrix_, Filter_> RichType;
+      typedef JacobiPrecond<Matrix_, Filter_> JacType;
+
+    private:
+      std::shared_ptr<HierarchyType> _h;
+      FusedWorkspace _fw;
+      bool _fusable = false;
+      enum class Coarse { none, direct, jacobi, richardson, other } _coarse_kind = Coarse::other;
+
+    public:
+      explicit MultiGrid(std::shared_ptr<HierarchyType> h, MultiGridCycle = MultiGridCycle::V) : _h(h) {}
+      std::string name() const override { return "MultiGrid-V"; }
+      HierarchyType& hierarchy() { return *_h; }
+
+      void init_symbolic() override
+      {
+        const Index L = _h->size();
+        _fw.create(2 * L + 8);
+        _fusable = true;
+        for(Index i = 0; i + 1 < L; ++i)
+        {
+          auto* pre = dynamic_cast<RichType*>(_h->levels[i].pre.get());
+          auto* post = dynamic_cast<RichType*>(_h->levels[i].post.get());
+          if(!pre || !post || !pre->fusable_jacobi() || !post->fusable_jacobi()) _fusable = false;
+        }
+        auto* c = _h->levels[L - 1].coarse.get();
+        if(!c)
+          _coarse_kind = Coarse::none;
+        else if(dynamic_cast<DirectCoarseSolver<Matrix_, Filter_>*>(c))
+          _coarse_kind = Coarse::direct;
+        else if(dynamic_cast<JacType*>(c))
+          _coarse_kind = Coarse::jacobi;
+        else if(auto* r = dynamic_cast<RichType*>(c))
+          _coarse_kind = r->fusable_jacobi() ? Coarse::richardson : Coarse::other;
+        else
+          _coarse_kind = Coarse::other;
+        if(_coarse_kind == Coarse::other) _fusable = false;
+      }
+      bool fusable() const { return _fusable; }
+      const FusedWorkspace& workspace() const { return _fw; }
+
+      /// MultiGrid::apply (multigrid.hpp:1270-1330)
+      Status apply(VectorType& vec_cor, const VectorType& vec_def) override
+      {
+        if(_fusable)
+        {
+          double* nrm = _fw.scalar(0);
+          check(feat_dot_f64(Runtime::handle(), nrm, (const double*)vec_def.elements(), (const double*)vec_def.elements(),
+                             vec_def.size()),
+                "MultiGrid::apply norm");
+          apply_fused(vec_cor, vec_def, nrm, nullptr);
+          return Status::success;
+        }
+        auto& top = _h->levels[0];
+        top.rhs.copy(vec_def);
+        _apply_generic();
+        vec_cor.copy(top.sol);
+        return Status::success;
+      }
+
+      /// Fused V-cycle: cor = MG(def) where *def_nrm2 (device) holds |def|^2;
+      /// when gamma != nullptr, *gamma = cor . def (device).
+      void apply_fused(VectorType& cor, const VectorType& def, const double* def_nrm2, double* gamma)
+      {
+        const Index L = _h->size();
+        const Index c = L - 1;
+        feat_handle_t hd = Runtime::handle();
+        const feat_reduce_ws_t* ws = &_fw.ws;
+        auto rhs_of = [&](Index i) -> const double* {
+          return i == 0 ? (const double*)def.elements() : (const double*)_h->levels[i].rhs.elements();
+        };
+        auto sol_of = [&](Index i) -> double* {
+          return i == 0 ? (double*)cor.elements() : (double*)_h->levels[i].sol.elements();
+        };
+        auto nrm_of = [&](Index i) -> const double* { return i == 0 ? def_nrm2 : _fw.scalar(2 + i); };
+        // restriction leg (multigrid.hpp:1713-1840)
+        for(Index i = 0; i < c; ++i)
+        {
+          auto& l = _h->levels[i];
+          auto* pre = static_cast<RichType*>(l.pre.get());
+          auto* jac = pre->fusable_jacobi();
+          const feat_csr_f64_t A = l.matrix->desc();
+          check(feat_rj_smooth_f64(hd, &A, (const double*)jac->inv_diag().elements(), l.filter->mask(),
+                                   int(pre->get_max_iter()), pre->omega(), rhs_of(i), nrm_of(i), sol_of(i),
+                                   (double*)l.tmp.elements(), (double*)l.def.elements(), nullptr, nullptr, nullptr, ws),
+                "MultiGrid pre-smoothing");
+          const feat_csr_f64_t R = l.transfer->get_mat_rest().desc();
+          auto& lc = _h->levels[i + 1];
+          check(feat_csr_apply_masked_f64(hd, &R, (double*)lc.rhs.elements(), (const double*)l.def.elements(),
+                                          lc.filter->mask(), _fw.scalar(2 + i + 1), nullptr, nullptr, ws),
+                "MultiGrid restriction");
+        }
+        // coarse solve (multigrid.hpp:1526-1589)
+        {
+          auto& l = _h->levels[c];
+          double* xs = sol_of(c);
+          switch(_coarse_kind)
+          {
+          case Coarse::direct:
+          {
+            auto* d = static_cast<DirectCoarseSolver<Matrix_, Filter_>*>(l.coarse.get());
+            check(feat_dense_gemv_f64(hd, int(l.matrix->rows()), d->inverse(), rhs_of(c), xs, l.filter->mask()),
+                  "MultiGrid coarse direct");
+            break;
+          }
+          case Coarse::jacobi:
+          {
+            // cor = filter_cor(inv_diag∘rhs): one step from zero with omega 1 is exact (fma(1,c,0) = c)
+            auto* j = static_cast<JacType*>(l.coarse.get());
+            const feat_csr_f64_t A = l.matrix->desc();
+            check(feat_rj_smooth_f64(hd, &A, (const double*)j->inv_diag().elements(), l.filter->mask(), 1, 1.0,
+                                     rhs_of(c), nullptr, xs, (double*)l.tmp.elements(), nullptr, nullptr, nullptr,
+                                     nullptr, ws),
+                  "MultiGrid coarse Jacobi");
+            break;
+          }
+          case Coarse::richardson:
+          {
+            auto* r = static_cast<RichType*>(l.coarse.get());
+            const feat_csr_f64_t A = l.matrix->desc();
+            check(feat_rj_smooth_f64(hd, &A, (const double*)r->fusable_jacobi()->inv_diag().elements(), l.filter->mask(),
+                                     int(r->get_max_iter()), r->omega(), rhs_of(c), nrm_of(c), xs,
+                                     (double*)l.tmp.elements(), nullptr, nullptr, nullptr, nullptr, ws),
+                  "MultiGrid coarse Richardson");
+            break;
+          }
+          default:
+            check(feat_copy(hd, xs, rhs_of(c), sizeof(double) * l.matrix->rows()), "MultiGrid coarse copy");
+            if(l.filter->used_elements())
+              check(feat_unit_filter_def_f64_u64(hd, xs, l.filter->indices(), l.filter->used_elements()),
+                    "MultiGrid coarse filter");
+            break;
+          }
+        }
+        // prolongation leg (multigrid.hpp:1845-2000)
+        for(Index k = c; k-- > 0;)
+        {
+          auto& l = _h->levels[k];
+          auto* post = static_cast<RichType*>(l.post.get());
+          auto* jac = post->fusable_jacobi();
+          const feat_csr_f64_t P = l.transfer->get_mat_prol().desc();
+          check(feat_csr_prol_add_f64(hd, &P, sol_of(k), sol_of(k + 1), l.filter->mask()), "MultiGrid prolongation");
+          const feat_csr_f64_t A = l.matrix->desc();
+          double* dn = _fw.scalar(2 + L + k);
+          check(feat_csr_defect_f64(hd, &A, (double*)l.def.elements(), rhs_of(k), sol_of(k), l.filter->mask(), dn, ws),
+                "MultiGrid defect");
+          const bool top = (k == 0);
+          check(feat_rj_smooth_f64(hd, &A, (const double*)jac->inv_diag().elements(), l.filter->mask(),
+                                   int(post->get_max_iter()), post->omega(), (const double*)l.def.elements(), dn,
+                                   (double*)l.cor.elements(), (double*)l.tmp.elements(), nullptr, sol_of(k),
+                                   top && gamma ? (const double*)def.elements() : nullptr, top ? gamma : nullptr, ws),
+                "MultiGrid post-smoothing");
+        }
+      }
+
+    private:
+      /// Generic V-cycle, op for op as multigrid.hpp:1335-2000.
+      void _apply_generic()
+      {
+        const Index L = _h->size();
+        const Index c = L - 1;
+        for(Index i = 0; i < c; ++i)
+        {
+          auto& l = _h->levels[i];
+          auto& lc = _h->levels[i + 1];
+          l.pre->apply(l.sol, l.rhs);
+          l.matrix->apply(l.def, l.sol, l.rhs, -1.0);
+          l.filter->filter_def(l.def);
+          l.transfer->rest(l.def, lc.rhs);
+          lc.filter->filter_def(lc.rhs);
+        }
+        auto& lc = _h->levels[c];
+        if(lc.coarse)
+          lc.coarse->apply(lc.sol, lc.rhs);
+        else
+        {
+          lc.sol.copy(lc.rhs);
+          lc.filter->filter_cor(lc.sol);
+        }
+        for(Index k = c; k-- > 0;)
+        {
+          auto& l = _h->levels[k];
+          l.transfer->prol(l.cor, _h->levels[k + 1].sol);
+          l.filter->filter_cor(l.cor);
+          l.sol.axpy(l.cor, l.sol, 1.0);
+          l.matrix->apply(l.def, l.sol, l.rhs, -1.0);
+          l.filter->filter_def(l.def);
+          l.post->apply(l.cor, l.def);
+          l.sol.axpy(l.cor, l.sol);
+        }
+      }
+    };
+
+    /// PCG (solver/pcg.hpp:33-288)
+    template<typename Matrix_, typename Filter_>
+    class PCG : public IterativeSolver<typename Matrix_::VectorTypeL>
+    {
+    public:
+      typedef typename Matrix_::VectorTypeL VectorType;
+      typedef typename Matrix_::DataType DataType;
+      typedef SolverBase<VectorType> PrecondType;
+      typedef MultiGrid<Matrix_, Filter_, LAFEM::Transfer<Matrix_>> MGType;
+
+    private:
+      const Matrix_& _matrix;
+      const Filter_& _filter;
+      std::shared_ptr<PrecondType> _precond;
+      VectorType _vec_r, _vec_p, _vec_t;
+      FusedWorkspace _fw;
+      bool _force_generic = false;
+
+    public:
+      PCG(const Matrix_& m, const Filter_& f, std::shared_ptr<PrecondType> p = nullptr)
+        : _matrix(m), _filter(f), _precond(p) {}
+      std::string name() const override { return "PCG"; }
+      void set_force_generic(bool g) { _force_generic = g; }
+      void init_symbolic() override
+      {
+        if(_precond) _precond->init_symbolic();
+        _vec_r = _matrix.create_vector_r();
+        _vec_p = _matrix.create_vector_r();
+        _vec_t = _matrix.create_vector_r();
+        _fw.create(8);
+      }
+      void init_numeric() override
+      {
+        if(_precond) _precond->init_numeric();
+      }
+
+      /// pcg.hpp:140-156
+      Status apply(VectorType& vec_cor, const VectorType& vec_def) override
+      {
+        _vec_r.copy(vec_def);
+        vec_cor.format();
+        this->_status = _run(vec_cor, false, nullptr);
+        return this->_status;
+      }
+
+      /// pcg.hpp:159-173 (Solver::solve -> correct, iterative.hpp:930-938)
+      Status correct(VectorType& vec_sol, const VectorType& vec_rhs) override
+      {
+        this->_status = _run(vec_sol, true, &vec_rhs);
+        return this->_status;
+      }
+
+    private:
+      MGType* _fused_mg() const
+      {
+        if(_force_generic) return nullptr;
+        auto* mg = dynamic_cast<MGType*>(_precond.get());
+        return (mg && mg->fusable()) ? mg : nullptr;
+      }
+
+      Status _run(VectorType& x, bool correct, const VectorType* rhs)
+      {
+        if(MGType* mg = _fused_mg()) return _run_fused(*mg, x, correct, rhs);
+        if(correct)
+        {
+          _matrix.apply(_vec_r, x, *rhs, -DataType(1));
+          _filter.filter_def(_vec_r);
+        }
+        return _apply_intern_generic(x);
+      }
+
+      /// pcg.hpp:188-288, op for op.
+      Status _apply_intern_generic(VectorType& vec_sol)
+      {
+        VectorType& r = _vec_r;
+        VectorType& p = _vec_p;
+        VectorType& q = _vec_t;
+        VectorType& z = _vec_t;
+        Status st = this->_set_initial_defect(r.norm2());
+        if(st != Status::progress) return st;
+        if(!apply_precond(_precond.get(), p, r, _filter)) return Status::aborted;
+        DataType gamma = r.dot(p);
+        while(st == Status::progress)
+        {
+          _matrix.apply(q, p);
+          _filter.filter_def(q);
+          DataType alpha = gamma / q.dot(p);
+          vec_sol.axpy(p, vec_sol, alpha);
+          r.axpy(q, r, -alpha);
+          st = this->_set_new_defect([&]() { return r.norm2(); });
+          if(st != Status::progress) return st;
+          if(!apply_precond(_precond.get(), z, r, _filter)) return Status::aborted;
+          DataType gamma2 = gamma;
+          gamma = r.dot(z);
+          DataType beta = gamma / gamma2;
+          p.axpy(p, z, beta);
+        }
+        return st;
+      }
+
+      /// Device-scalar form of the same loop with the fused kernels.
+      Status _run_fused(MGType& mg, VectorType& x, bool correct, const VectorType* rhs)
+      {
+        feat_handle_t hd = Runtime::handle();
+        const feat_reduce_ws_t* ws = &_fw.ws;
+        double* s_rr = _fw.scalar(0);
+        double* s_g = _fw.scalar(1);
+        double* s_g2 = _fw.scalar(2);
+        double* s_qp = _fw.scalar(3);
+        const feat_csr_f64_t A = _matrix.desc();
+        const Index n = _matrix.rows();
+        double* r = (double*)_vec_r.elements();
+        double* p = (double*)_vec_p.elements();
+        double* t = (double*)_vec_t.elements();
+        if(correct)
+          check(feat_csr_defect_f64(hd, &A, r, (const double*)rhs->elements(), (const double*)x.elements(), _filter.mask(),
+                                    s_rr, ws),
+                "PCG defect");
+        else
+          check(feat_dot_f64(hd, s_rr, r, r, n), "PCG initial norm");
+        double rr = 0.0;
+        check(feat_read_scalar_f64(hd, s_rr, &rr), "PCG read");
+        Status st = this->_set_initial_defect(std::sqrt(rr));
+        if(st != Status::progress) return st;
+        mg.apply_fused(_vec_p, _vec_r, s_rr, s_g);
+        while(st == Status::progress)
+        {
+          check(feat_csr_apply_masked_f64(hd, &A, t, p, _filter.mask(), nullptr, p, s_qp, ws), "PCG A*p");
+          check(feat_pcg_update_f64(hd, (double*)x.elements(), r, p, t, s_g, s_qp, s_rr, n, ws), "PCG update");
+          st = this->_set_new_defect([&]() {
+            check(feat_read_scalar_f64(hd, s_rr, &rr), "PCG read");
+            return std::sqrt(rr);
+          });
+          if(st != Status::progress) return st;
+          mg.apply_fused(_vec_t, _vec_r, s_rr, s_g2);
+          check(feat_pcg_direction_f64(hd, p, t, s_g2, s_g, n), "PCG direction");
+        }
+        return st;
+      }
+    };
+
+    template<typename Matrix_, typename Filter_>
+    std::shared_ptr<PCG<Matrix_, Filter_>> new_pcg(const Matrix_& m, const Filter_& f,
+                                                   std::shared_ptr<SolverBase<typename Matrix_::VectorTypeL>> p = nullptr)
+    {
+      return std::make_shared<PCG<Matrix_, Filter_>>(m, f, p);
+    }
+  } // namespace Solver
+} // namespace feat3amd

@@ -1,0 +1,86 @@
+/*
+ * feat3amd_host.h -- C entry points of the C++ host layer (libfeat3amd_host.so).
+ *
+ * The host layer (include/feat3amd/{lafem,solver,assembly}.hpp) keeps the
+ * reference's container and Solver::SolverBase API in C++; these functions
+ * expose a solver tree and the structured problem setup to callers that are
+ * not C++ (tests, benchmarks, FFI).  Device pointers as in feat3amd.h.
+ */
+#ifndef FEAT3AMD_HOST_H
+#define FEAT3AMD_HOST_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- structured Q1 Poisson setup (host arrays; SymbolicAssembler + LaplaceOperator,
+ * GridTransfer::assemble_prolongation, UnitFilterAssembler; see assembly.hpp) ----
+ * sizes[0..4] = n, nnz(A), #boundary, nnz(P from level-1), cols(P) */
+int feat_q1_sizes(int dim, int level, uint64_t* sizes);
+int feat_q1_assemble(int dim, int level, double* val, uint64_t* col, uint64_t* rowptr,
+                     uint64_t* bnd, double* pval, uint64_t* pcol, uint64_t* prowptr);
+int feat_q1_rhs_sine(int dim, int level, double* b);
+int feat_rhs_random(uint64_t seed, uint64_t n, double a, double b, double* out);
+
+/* ---- PCG(MultiGrid-V(Richardson(Jacobi))) solver tree ----
+ * solver/pcg.hpp + solver/multigrid.hpp + solver/richardson.hpp + solver/jacobi_precond.hpp */
+typedef struct feat_mgpcg_s* feat_mgpcg_t;
+
+#define FEAT_COARSE_NONE 0      /* copy + filter_cor                       */
+#define FEAT_COARSE_SMOOTHER 1  /* Richardson-Jacobi, nu_pre steps          */
+#define FEAT_COARSE_JACOBI 2    /* one Jacobi application (coarse_omega)    */
+#define FEAT_COARSE_DIRECT 3    /* dense inverse of the filtered matrix     */
+
+typedef struct {
+  int nu_pre, nu_post;
+  double omega_jac, omega_rich;
+  int coarse;
+  double coarse_omega;
+  double tol_rel;
+  uint64_t max_iter;
+  int force_generic;      /* 1: reference op-by-op path instead of the fused one */
+} feat_mgpcg_cfg_t;
+
+typedef struct {          /* one level, host arrays; P maps the next coarser level onto this one */
+  uint64_t rows;
+  const double* val;
+  const uint64_t* col;
+  const uint64_t* rowptr;
+  const uint64_t* filt;
+  uint64_t nfilt;
+  uint64_t pcols;
+  const double* pval;
+  const uint64_t* pcol;
+  const uint64_t* prowptr;
+} feat_level_host_t;
+
+typedef struct {
+  int status;             /* Solver::Status (solver/base.hpp:48-64) */
+  uint64_t iters;
+  double def_init, def_final;
+} feat_solve_result_t;
+
+int feat_mgpcg_create(feat_mgpcg_t* s, const feat_level_host_t* levels, int nlevels,
+                      const feat_mgpcg_cfg_t* cfg);
+/* builds the structured Q1 hierarchy levels lf (finest) .. lc (coarsest) itself */
+int feat_mgpcg_create_q1(feat_mgpcg_t* s, int dim, int lf, int lc, const feat_mgpcg_cfg_t* cfg);
+/* Solver::solve(pcg, x, b, A, F) -> PCG::correct: x is the initial guess (device), b device */
+int feat_mgpcg_solve(feat_mgpcg_t s, double* x, const double* b, feat_solve_result_t* res,
+                     double* hist, uint64_t hist_cap);
+/* device descriptor of level `lvl`'s system matrix (0 = finest) */
+int feat_mgpcg_matrix(feat_mgpcg_t s, int lvl, uint64_t* rows, uint64_t* nnz, const double** val,
+                      const uint64_t** col, const uint64_t** rowptr);
+int feat_mgpcg_destroy(feat_mgpcg_t s);
+const char* feat_host_last_error(void);
+/* Binds the host layer's runtime (Runtime::initialize, kernel/runtime.cpp:37-80)
+ * to `device` and `stream` (NULL: the null stream).  Call before any other
+ * host-layer function; later calls are ignored once initialised. */
+int feat_runtime_init(int device, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,144 @@
+/*
+ * feat_oracle.h -- CPU oracle for the LAFEM + Solver hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a plain-C restatement of the reference's
+ * generic (CPU) backend, used exclusively by tests/, __graft_entry__.smoke()
+ * and the cpu_baseline leg of bench.py as the checker.  The product library
+ * never links, loads or calls it.
+ *
+ * Arithmetic convention: the reference's optimised build compiles with
+ * -O3 -march=<cpu> (build_system/configure_gcc.py:109-114), so GCC contracts
+ * every `s += a*b` and `a*x + y` into one fused multiply-add on FMA hardware.
+ * The oracle states each such contraction explicitly with fma() and is
+ * itself compiled with -ffp-contract=off, so its rounding is defined by the
+ * source and not by compiler flags.  Sums are sequential in index order, as
+ * in the generic loops.
+ *
+ * Index type is uint64_t (reference `Index`, kernel/base_header.hpp).
+ */
+#ifndef FEAT_ORACLE_H
+#define FEAT_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- seeded RNG: xorshift64* (kernel/util/random.hpp:84-110, 259-280) ---- */
+uint64_t orc_rng_next(uint64_t* state);
+/* fills v[0..n) with uniform values in [a,b] exactly as
+ * DenseVector(Random&, n, a, b) (lafem/dense_vector.hpp:322-332, 860-869) */
+void orc_rng_fill_f64(uint64_t* state, double* v, uint64_t n, double a, double b);
+
+/* ---- CSR (lafem/arch/apply_generic.hpp:25-69) ---- */
+void orc_csr_apply_f64(double* r, double a, const double* x, double b, const double* y,
+                       const double* val, const uint64_t* col, const uint64_t* rowptr,
+                       uint64_t rows, uint64_t cols, int transposed);
+void orc_csr_apply_f32(float* r, float a, const float* x, float b, const float* y,
+                       const float* val, const uint64_t* col, const uint64_t* rowptr,
+                       uint64_t rows, uint64_t cols, int transposed);
+/* ---- BCSR bh x bw blocks, row-major (lafem/arch/apply_generic.hpp:121-154) ---- */
+void orc_bcsr_apply_f64(int bh, int bw, double* r, double a, const double* x, double b,
+                        const double* y, const double* val, const uint64_t* col,
+                        const uint64_t* rowptr, uint64_t rows);
+
+/* ---- BLAS-1 (lafem/arch/*_generic.hpp) ---- */
+void   orc_axpy_f64(double* r, double a, const double* x, const double* y, uint64_t n);
+double orc_dot_f64(const double* x, const double* y, uint64_t n);
+double orc_triple_dot_f64(const double* x, const double* y, const double* z, uint64_t n);
+double orc_norm2_f64(const double* x, uint64_t n);
+void   orc_scale_f64(double* r, const double* x, double s, uint64_t n);
+void   orc_component_product_f64(double* r, const double* x, const double* y, uint64_t n);
+void   orc_component_invert_f64(double* r, const double* x, double s, uint64_t n);
+void   orc_diag_index(uint64_t* diag, const uint64_t* col, const uint64_t* rowptr, uint64_t rows);
+void   orc_extract_diag_f64(double* diag, const double* val, const uint64_t* col,
+                            const uint64_t* rowptr, uint64_t rows);
+void   orc_unit_filter_rhs_f64(double* v, const double* sv_val, const uint64_t* sv_idx, uint64_t ue);
+void   orc_unit_filter_def_f64(double* v, const uint64_t* sv_idx, uint64_t ue);
+void   orc_mean_filter_f64(double* v, const double* vec_prim, const double* vec_dual,
+                           double volume, uint64_t n);
+
+/* ---- dense Gauss-Jordan inverse with diagonal pivoting (util/math.hpp:1169-1274) ---- */
+double orc_invert_matrix_f64(int n, int stride, double* a, int* p);
+
+/* ---- solver configuration (solver/iterative.hpp:213-236 defaults) ---- */
+typedef struct {
+  uint64_t rows, cols, nnz;
+  const double* val;
+  const uint64_t* col;
+  const uint64_t* rowptr;
+} orc_csr_t;
+
+typedef struct {
+  double tol_rel, tol_abs, tol_abs_low, div_rel, div_abs, stag_rate;
+  uint64_t min_iter, max_iter, min_stag_iter;
+} orc_iter_cfg_t;
+
+/* Status codes of solver/base.hpp:48-64 */
+enum { ORC_UNDEFINED = 0, ORC_PROGRESS, ORC_SUCCESS, ORC_ABORTED, ORC_DIVERGED,
+       ORC_MAX_ITER, ORC_STAGNATED };
+
+typedef struct {
+  int status;
+  uint64_t num_iter;
+  double def_init, def_final;
+} orc_result_t;
+
+void orc_iter_cfg_default(orc_iter_cfg_t* c);
+
+/* coarse solver variants for the multigrid coarsest level */
+enum {
+  ORC_COARSE_NONE = 0,      /* copy rhs + filter_cor (multigrid.hpp:1560-1566)            */
+  ORC_COARSE_SMOOTHER = 1,  /* Richardson-Jacobi with the level's smoother settings        */
+  ORC_COARSE_JACOBI = 2,    /* one Jacobi application, omega = coarse_omega                */
+  ORC_COARSE_DIRECT = 3     /* dense inverse of the filtered coarse matrix, applied as GEMV */
+};
+
+typedef struct {
+  orc_csr_t A;              /* system matrix of this level                                  */
+  const uint64_t* filt;     /* homogeneous unit-filter indices                              */
+  uint64_t nfilt;
+  orc_csr_t P;              /* prolongation from the next coarser level (rows = A.rows)     */
+  orc_csr_t R;              /* restriction to the next coarser level                        */
+} orc_mg_level_t;
+
+typedef struct {
+  int nu_pre, nu_post;
+  double omega_jac;         /* JacobiPrecond omega (inv_diag = omega / a_ii)                */
+  double omega_rich;        /* Richardson damping                                           */
+  int coarse;               /* ORC_COARSE_*                                                  */
+  double coarse_omega;      /* for ORC_COARSE_JACOBI                                         */
+} orc_mg_cfg_t;
+
+/* PCG (solver/pcg.hpp:159-288) preconditioned by a V-cycle multigrid
+ * (solver/multigrid.hpp:1270-2000) with Richardson(Jacobi) smoothers
+ * (solver/richardson.hpp:160-226, solver/jacobi_precond.hpp:136-164).
+ * x: initial guess on input, solution on output.  def_hist (may be NULL) gets
+ * def_init and the defect norm after every iteration (max_iter+1 entries). */
+int orc_mg_pcg_solve(const orc_mg_level_t* levels, int nlevels, const orc_mg_cfg_t* mg,
+                     const orc_iter_cfg_t* it, double* x, const double* b,
+                     double* def_hist, orc_result_t* res);
+
+/* Stand-alone iterative solvers on one CSR matrix with a unit filter.
+ * precond: 0 = none, 1 = Jacobi(omega).  Variant of Solver::solve
+ * (solver/iterative.hpp:930-938 -> correct()). */
+int orc_pcg_solve(const orc_csr_t* A, const uint64_t* filt, uint64_t nfilt, int precond,
+                  double omega, const orc_iter_cfg_t* it, double* x, const double* b,
+                  double* def_hist, orc_result_t* res);
+int orc_richardson_solve(const orc_csr_t* A, const uint64_t* filt, uint64_t nfilt, int precond,
+                         double omega_jac, double omega_rich, const orc_iter_cfg_t* it,
+                         double* x, const double* b, double* def_hist, orc_result_t* res);
+
+/* Solver::apply() variants (x formatted to zero, b is the defect). */
+int orc_pcg_apply(const orc_csr_t* A, const uint64_t* filt, uint64_t nfilt, int precond,
+                  double omega, const orc_iter_cfg_t* it, double* x, const double* b,
+                  double* def_hist, orc_result_t* res);
+int orc_richardson_apply(const orc_csr_t* A, const uint64_t* filt, uint64_t nfilt, int precond,
+                         double omega_jac, double omega_rich, const orc_iter_cfg_t* it,
+                         double* x, const double* b, double* def_hist, orc_result_t* res);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,507 @@
+"""ctypes bindings of the CPU oracle (liboracle.so) and of the reference build
+(_ref/libfeatref.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py as the checker; the product never imports it.
+"""
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_ORACLE = os.path.join(HERE, "liboracle.so")
+LIB_REF = os.path.join(HERE, "_ref", "libfeatref.so")
+
+_u64p = np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+_f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+_i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+_vp = C.c_void_p
+
+# solver status codes (solver/base.hpp:48-64)
+STATUS = {0: "undefined", 1: "progress", 2: "success", 3: "aborted", 4: "diverged",
+          5: "max_iter", 6: "stagnated"}
+COARSE_NONE, COARSE_SMOOTHER, COARSE_JACOBI, COARSE_DIRECT = 0, 1, 2, 3
+
+
+@dataclass
+class Csr:
+    """CSR matrix with uint64 indices (reference SparseMatrixCSR<DT, Index>)."""
+    rows: int
+    cols: int
+    val: np.ndarray
+    col: np.ndarray
+    rowptr: np.ndarray
+
+    @property
+    def nnz(self):
+        return int(self.val.size)
+
+    def todense(self):
+        d = np.zeros((self.rows, self.cols), dtype=np.float64)
+        for r in range(self.rows):
+            for i in range(int(self.rowptr[r]), int(self.rowptr[r + 1])):
+                d[r, int(self.col[i])] += float(self.val[i])
+        return d
+
+
+class _CsrS(C.Structure):
+    _fields_ = [("rows", C.c_uint64), ("cols", C.c_uint64), ("nnz", C.c_uint64),
+                ("val", _vp), ("col", _vp), ("rowptr", _vp)]
+
+
+class _LevelS(C.Structure):
+    _fields_ = [("A", _CsrS), ("filt", _vp), ("nfilt", C.c_uint64), ("P", _CsrS), ("R", _CsrS)]
+
+
+class _IterCfgS(C.Structure):
+    _fields_ = [("tol_rel", C.c_double), ("tol_abs", C.c_double), ("tol_abs_low", C.c_double),
+                ("div_rel", C.c_double), ("div_abs", C.c_double), ("stag_rate", C.c_double),
+                ("min_iter", C.c_uint64), ("max_iter", C.c_uint64),
+                ("min_stag_iter", C.c_uint64)]
+
+
+class _MgCfgS(C.Structure):
+    _fields_ = [("nu_pre", C.c_int), ("nu_post", C.c_int), ("omega_jac", C.c_double),
+                ("omega_rich", C.c_double), ("coarse", C.c_int), ("coarse_omega", C.c_double)]
+
+
+class _ResultS(C.Structure):
+    _fields_ = [("status", C.c_int), ("num_iter", C.c_uint64), ("def_init", C.c_double),
+                ("def_final", C.c_double)]
+
+
+def _csr_s(m, keep):
+    if m is None:
+        return _CsrS(0, 0, 0, None, None, None)
+    val = np.ascontiguousarray(m.val, dtype=np.float64)
+    col = np.ascontiguousarray(m.col, dtype=np.uint64)
+    rp = np.ascontiguousarray(m.rowptr, dtype=np.uint64)
+    keep += [val, col, rp]
+    return _CsrS(m.rows, m.cols, val.size, val.ctypes.data, col.ctypes.data, rp.ctypes.data)
+
+
+def _levels_s(levels):
+    """levels: list of dicts {A, filt, P, R} with level 0 the finest."""
+    keep = []
+    arr = (_LevelS * len(levels))()
+    for i, lv in enumerate(levels):
+        f = np.ascontiguousarray(lv.get("filt", np.zeros(0, np.uint64)), dtype=np.uint64)
+        keep.append(f)
+        arr[i] = _LevelS(_csr_s(lv["A"], keep), f.ctypes.data if f.size else None, f.size,
+                         _csr_s(lv.get("P"), keep), _csr_s(lv.get("R"), keep))
+    return arr, keep
+
+
+class Oracle:
+    """The C oracle."""
+
+    def __init__(self, path=LIB_ORACLE):
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"{path} not built (make -C oracle)")
+        L = C.CDLL(path)
+        self.L = L
+        L.orc_rng_next.restype = C.c_uint64
+        L.orc_rng_next.argtypes = [C.POINTER(C.c_uint64)]
+        L.orc_rng_fill_f64.argtypes = [C.POINTER(C.c_uint64), _f64p, C.c_uint64, C.c_double,
+                                       C.c_double]
+        L.orc_csr_apply_f64.argtypes = [_f64p, C.c_double, _f64p, C.c_double, _f64p, _f64p,
+                                        _u64p, _u64p, C.c_uint64, C.c_uint64, C.c_int]
+        L.orc_csr_apply_f32.argtypes = [_f32p, C.c_float, _f32p, C.c_float, _f32p, _f32p,
+                                        _u64p, _u64p, C.c_uint64, C.c_uint64, C.c_int]
+        L.orc_bcsr_apply_f64.argtypes = [C.c_int, C.c_int, _f64p, C.c_double, _f64p, C.c_double,
+                                         _f64p, _f64p, _u64p, _u64p, C.c_uint64]
+        L.orc_axpy_f64.argtypes = [_f64p, C.c_double, _f64p, _f64p, C.c_uint64]
+        for fn in ("orc_dot_f64",):
+            getattr(L, fn).restype = C.c_double
+            getattr(L, fn).argtypes = [_f64p, _f64p, C.c_uint64]
+        L.orc_triple_dot_f64.restype = C.c_double
+        L.orc_triple_dot_f64.argtypes = [_f64p, _f64p, _f64p, C.c_uint64]
+        L.orc_norm2_f64.restype = C.c_double
+        L.orc_norm2_f64.argtypes = [_f64p, C.c_uint64]
+        L.orc_scale_f64.argtypes = [_f64p, _f64p, C.c_double, C.c_uint64]
+        L.orc_component_product_f64.argtypes = [_f64p, _f64p, _f64p, C.c_uint64]
+        L.orc_component_invert_f64.argtypes = [_f64p, _f64p, C.c_double, C.c_uint64]
+        L.orc_diag_index.argtypes = [_u64p, _u64p, _u64p, C.c_uint64]
+        L.orc_extract_diag_f64.argtypes = [_f64p, _f64p, _u64p, _u64p, C.c_uint64]
+        L.orc_unit_filter_rhs_f64.argtypes = [_f64p, _f64p, _u64p, C.c_uint64]
+        L.orc_unit_filter_def_f64.argtypes = [_f64p, _u64p, C.c_uint64]
+        L.orc_mean_filter_f64.argtypes = [_f64p, _f64p, _f64p, C.c_double, C.c_uint64]
+        L.orc_invert_matrix_f64.restype = C.c_double
+        L.orc_invert_matrix_f64.argtypes = [C.c_int, C.c_int, _f64p, _i32p]
+        L.orc_iter_cfg_default.argtypes = [C.POINTER(_IterCfgS)]
+        L.orc_mg_pcg_solve.restype = C.c_int
+        L.orc_mg_pcg_solve.argtypes = [C.POINTER(_LevelS), C.c_int, C.POINTER(_MgCfgS),
+                                       C.POINTER(_IterCfgS), _f64p, _f64p, _vp,
+                                       C.POINTER(_ResultS)]
+        for fn in ("orc_pcg_solve", "orc_pcg_apply"):
+            f = getattr(L, fn)
+            f.restype = C.c_int
+            f.argtypes = [C.POINTER(_CsrS), _vp, C.c_uint64, C.c_int, C.c_double,
+                          C.POINTER(_IterCfgS), _f64p, _f64p, _vp, C.POINTER(_ResultS)]
+        for fn in ("orc_richardson_solve", "orc_richardson_apply"):
+            f = getattr(L, fn)
+            f.restype = C.c_int
+            f.argtypes = [C.POINTER(_CsrS), _vp, C.c_uint64, C.c_int, C.c_double, C.c_double,
+                          C.POINTER(_IterCfgS), _f64p, _f64p, _vp, C.POINTER(_ResultS)]
+
+    # -- RNG
+    def rng_fill(self, seed, n, a, b):
+        s = C.c_uint64(seed)
+        v = np.empty(n, dtype=np.float64)
+        self.L.orc_rng_fill_f64(C.byref(s), v, n, a, b)
+        return v
+
+    def rng_next(self, seed, k):
+        s = C.c_uint64(seed)
+        return np.array([self.L.orc_rng_next(C.byref(s)) for _ in range(k)], dtype=np.uint64)
+
+    # -- kernels (all return new arrays; inputs untouched)
+    def csr_apply(self, A, x, y=None, a=1.0, b=0.0, transposed=False, r=None):
+        nr = A.cols if transposed else A.rows
+        if y is None:
+            y = np.zeros(nr)
+        if r is None:
+            r = np.empty(nr)
+        self.L.orc_csr_apply_f64(r, a, np.ascontiguousarray(x, np.float64), b,
+                                 np.ascontiguousarray(y, np.float64), A.val, A.col, A.rowptr,
+                                 A.rows, A.cols, int(transposed))
+        return r
+
+    def csr_apply_f32(self, A, x, y=None, a=1.0, b=0.0, transposed=False):
+        nr = A.cols if transposed else A.rows
+        y = np.zeros(nr, np.float32) if y is None else np.ascontiguousarray(y, np.float32)
+        r = np.empty(nr, np.float32)
+        self.L.orc_csr_apply_f32(r, a, np.ascontiguousarray(x, np.float32), b, y,
+                                 np.ascontiguousarray(A.val, np.float32), A.col, A.rowptr,
+                                 A.rows, A.cols, int(transposed))
+        return r
+
+    def bcsr_apply(self, bh, bw, rows, val, col, rowptr, x, y=None, a=1.0, b=0.0):
+        y = np.zeros(rows * bh) if y is None else y
+        r = np.empty(rows * bh)
+        self.L.orc_bcsr_apply_f64(bh, bw, r, a, np.ascontiguousarray(x, np.float64), b,
+                                  np.ascontiguousarray(y, np.float64),
+                                  np.ascontiguousarray(val, np.float64),
+                                  np.ascontiguousarray(col, np.uint64),
+                                  np.ascontiguousarray(rowptr, np.uint64), rows)
+        return r
+
+    def axpy(self, a, x, y, alias=None):
+        """r = a*x + y; alias 'x' or 'y' makes r share that operand's storage."""
+        x = np.ascontiguousarray(x, np.float64).copy()
+        y = np.ascontiguousarray(y, np.float64).copy()
+        raw = self.L["orc_axpy_f64"]
+        raw.argtypes = [_vp, C.c_double, _vp, _vp, C.c_uint64]
+        r = x if alias == "x" else (y if alias == "y" else np.empty_like(y))
+        raw(r.ctypes.data, a, x.ctypes.data, y.ctypes.data, y.size)
+        return r
+
+    def dot(self, x, y):
+        return self.L.orc_dot_f64(x, y, x.size)
+
+    def triple_dot(self, x, y, z):
+        return self.L.orc_triple_dot_f64(x, y, z, x.size)
+
+    def norm2(self, x):
+        return self.L.orc_norm2_f64(x, x.size)
+
+    def scale(self, x, s):
+        r = np.empty_like(x)
+        self.L.orc_scale_f64(r, x, s, x.size)
+        return r
+
+    def component_product(self, x, y):
+        r = np.empty_like(x)
+        self.L.orc_component_product_f64(r, x, y, x.size)
+        return r
+
+    def component_invert(self, x, s=1.0):
+        r = np.empty_like(x)
+        self.L.orc_component_invert_f64(r, x, s, x.size)
+        return r
+
+    def diag_index(self, A):
+        d = np.empty(A.rows, np.uint64)
+        self.L.orc_diag_index(d, A.col, A.rowptr, A.rows)
+        return d
+
+    def extract_diag(self, A):
+        d = np.empty(A.rows)
+        self.L.orc_extract_diag_f64(d, A.val, A.col, A.rowptr, A.rows)
+        return d
+
+    def unit_filter_def(self, v, idx):
+        v = v.copy()
+        idx = np.ascontiguousarray(idx, np.uint64)
+        self.L.orc_unit_filter_def_f64(v, idx, idx.size)
+        return v
+
+    def unit_filter_rhs(self, v, vals, idx):
+        v = v.copy()
+        idx = np.ascontiguousarray(idx, np.uint64)
+        self.L.orc_unit_filter_rhs_f64(v, np.ascontiguousarray(vals, np.float64), idx, idx.size)
+        return v
+
+    def mean_filter(self, v, w_dot, w_add, volume):
+        v = v.copy()
+        self.L.orc_mean_filter_f64(v, w_dot, w_add, volume, v.size)
+        return v
+
+    def invert_matrix(self, a, stride=None):
+        a = np.ascontiguousarray(a, np.float64).copy()
+        n = a.shape[0]
+        stride = stride or a.shape[1]
+        p = np.zeros(n, np.int32)
+        det = self.L.orc_invert_matrix_f64(n, stride, a.reshape(-1), p)
+        return a, det
+
+    # -- solvers
+    def iter_cfg(self, **kw):
+        c = _IterCfgS()
+        self.L.orc_iter_cfg_default(C.byref(c))
+        for k, v in kw.items():
+            setattr(c, k, v)
+        return c
+
+    def mg_pcg_solve(self, levels, x0, b, nu_pre=4, nu_post=4, omega_jac=0.5, omega_rich=1.0,
+                     coarse=COARSE_DIRECT, coarse_omega=1.0, tol_rel=1e-8, max_iter=100, **kw):
+        arr, keep = _levels_s(levels)
+        mg = _MgCfgS(nu_pre, nu_post, omega_jac, omega_rich, coarse, coarse_omega)
+        it = self.iter_cfg(tol_rel=tol_rel, max_iter=max_iter, **kw)
+        x = np.ascontiguousarray(x0, np.float64).copy()
+        hist = np.full(max_iter + 1, np.nan)
+        res = _ResultS()
+        st = self.L.orc_mg_pcg_solve(arr, len(levels), C.byref(mg), C.byref(it), x,
+                                     np.ascontiguousarray(b, np.float64), hist.ctypes.data,
+                                     C.byref(res))
+        return dict(x=x, status=STATUS[st], iters=int(res.num_iter), def_init=res.def_init,
+                    def_final=res.def_final, hist=hist[:int(res.num_iter) + 1])
+
+    def _single(self, fn, A, filt, args, x0, b, max_iter, **kw):
+        keep = []
+        a = _csr_s(A, keep)
+        f = np.ascontiguousarray(filt if filt is not None else np.zeros(0), np.uint64)
+        it = self.iter_cfg(max_iter=max_iter, **kw)
+        x = np.ascontiguousarray(x0, np.float64).copy()
+        hist = np.full(max_iter + 1, np.nan)
+        res = _ResultS()
+        st = fn(C.byref(a), f.ctypes.data if f.size else None, f.size, *args, C.byref(it), x,
+                np.ascontiguousarray(b, np.float64), hist.ctypes.data, C.byref(res))
+        return dict(x=x, status=STATUS[st], iters=int(res.num_iter), def_init=res.def_init,
+                    def_final=res.def_final, hist=hist[:int(res.num_iter) + 1])
+
+    def pcg(self, A, x0, b, filt=None, precond=0, omega=1.0, apply=False, max_iter=100, **kw):
+        fn = self.L.orc_pcg_apply if apply else self.L.orc_pcg_solve
+        return self._single(fn, A, filt, (precond, omega), x0, b, max_iter, **kw)
+
+    def richardson(self, A, x0, b, filt=None, precond=1, omega_jac=1.0, omega_rich=1.0,
+                   apply=False, max_iter=100, **kw):
+        fn = self.L.orc_richardson_apply if apply else self.L.orc_richardson_solve
+        return self._single(fn, A, filt, (precond, omega_jac, omega_rich), x0, b, max_iter, **kw)
+
+
+class Reference:
+    """The reference's own generic code, built by oracle/Makefile into _ref/."""
+
+    def __init__(self, path=LIB_REF):
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"{path} not built (make -C oracle ref)")
+        L = C.CDLL(path)
+        self.L = L
+        L.ref_init()
+        L.ref_csr_apply_f64.argtypes = [_f64p, C.c_double, _f64p, C.c_double, _f64p, _f64p,
+                                        _u64p, _u64p, C.c_uint64, C.c_uint64, C.c_uint64, C.c_int]
+        L.ref_bcsr_apply_f64.argtypes = [C.c_int, C.c_int, _f64p, C.c_double, _f64p, C.c_double,
+                                         _f64p, _f64p, _u64p, _u64p, C.c_uint64, C.c_uint64,
+                                         C.c_uint64]
+        L.ref_axpy_f64.argtypes = [_vp, C.c_double, _vp, _vp, C.c_uint64]
+        L.ref_dot_f64.restype = C.c_double
+        L.ref_dot_f64.argtypes = [_f64p, _f64p, C.c_uint64]
+        L.ref_triple_dot_f64.restype = C.c_double
+        L.ref_triple_dot_f64.argtypes = [_f64p, _f64p, _f64p, C.c_uint64]
+        L.ref_norm2_f64.restype = C.c_double
+        L.ref_norm2_f64.argtypes = [_f64p, C.c_uint64]
+        L.ref_scale_f64.argtypes = [_f64p, _f64p, C.c_double, C.c_uint64]
+        L.ref_component_product_f64.argtypes = [_f64p, _f64p, _f64p, C.c_uint64]
+        L.ref_component_invert_f64.argtypes = [_f64p, _f64p, C.c_double, C.c_uint64]
+        L.ref_diag_index.argtypes = [_u64p, _u64p, _u64p, C.c_uint64]
+        L.ref_invert_matrix_f64.restype = C.c_double
+        L.ref_invert_matrix_f64.argtypes = [C.c_int, C.c_int, _f64p, _i32p]
+        L.ref_rng_fill_f64.argtypes = [C.c_uint64, _f64p, C.c_uint64, C.c_double, C.c_double]
+        L.ref_pointstar_fd.argtypes = [C.c_uint64, C.c_uint64, C.POINTER(C.c_uint64),
+                                       C.POINTER(C.c_uint64), _vp, _vp, _vp, _vp]
+        L.ref_mg_pcg_solve.restype = C.c_int
+        L.ref_mg_pcg_solve.argtypes = [C.POINTER(_LevelS), C.c_int, C.c_int, C.c_int, C.c_double,
+                                       C.c_double, C.c_int, C.c_double, C.c_double, C.c_uint64,
+                                       _f64p, _f64p, _vp, C.c_uint64, C.POINTER(C.c_uint64),
+                                       C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.ref_single_solve.restype = C.c_int
+        L.ref_single_solve.argtypes = [C.c_int, C.POINTER(_CsrS), _vp, C.c_uint64, C.c_int,
+                                       C.c_double, C.c_double, C.c_double, C.c_uint64, C.c_int,
+                                       _f64p, _f64p, _vp, C.c_uint64, C.POINTER(C.c_uint64),
+                                       C.POINTER(C.c_double)]
+        L.ref_q1_create.restype = _vp
+        L.ref_q1_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_char_p]
+        L.ref_q1_sizes.argtypes = [_vp, _u64p]
+        L.ref_q1_copy.argtypes = [_vp, _f64p, _u64p, _u64p, _u64p, _f64p, _f64p, _vp, _vp, _vp]
+        L.ref_q1_free.argtypes = [_vp]
+
+    def csr_apply(self, A, x, y=None, a=1.0, b=0.0, transposed=False):
+        nr = A.cols if transposed else A.rows
+        y = np.zeros(nr) if y is None else np.ascontiguousarray(y, np.float64)
+        r = np.empty(nr)
+        self.L.ref_csr_apply_f64(r, a, np.ascontiguousarray(x, np.float64), b, y, A.val, A.col,
+                                 A.rowptr, A.rows, A.cols, A.nnz, int(transposed))
+        return r
+
+    def bcsr_apply(self, bh, bw, rows, cols, val, col, rowptr, x, y=None, a=1.0, b=0.0):
+        y = np.zeros(rows * bh) if y is None else np.ascontiguousarray(y, np.float64)
+        r = np.empty(rows * bh)
+        self.L.ref_bcsr_apply_f64(bh, bw, r, a, np.ascontiguousarray(x, np.float64), b, y,
+                                  np.ascontiguousarray(val, np.float64),
+                                  np.ascontiguousarray(col, np.uint64),
+                                  np.ascontiguousarray(rowptr, np.uint64), rows, cols, col.size)
+        return r
+
+    def axpy(self, a, x, y, alias=None):
+        """alias: None (r distinct), 'x' (r is x), 'y' (r is y)."""
+        x = x.copy()
+        y = y.copy()
+        if alias == "x":
+            self.L.ref_axpy_f64(x.ctypes.data, a, x.ctypes.data, y.ctypes.data, x.size)
+            return x
+        if alias == "y":
+            self.L.ref_axpy_f64(y.ctypes.data, a, x.ctypes.data, y.ctypes.data, x.size)
+            return y
+        r = np.empty_like(x)
+        self.L.ref_axpy_f64(r.ctypes.data, a, x.ctypes.data, y.ctypes.data, x.size)
+        return r
+
+    def dot(self, x, y):
+        return self.L.ref_dot_f64(x, y, x.size)
+
+    def triple_dot(self, x, y, z):
+        return self.L.ref_triple_dot_f64(x, y, z, x.size)
+
+    def norm2(self, x):
+        return self.L.ref_norm2_f64(x, x.size)
+
+    def scale(self, x, s):
+        r = np.empty_like(x)
+        self.L.ref_scale_f64(r, x, s, x.size)
+        return r
+
+    def component_product(self, x, y):
+        r = np.empty_like(x)
+        self.L.ref_component_product_f64(r, x, y, x.size)
+        return r
+
+    def component_invert(self, x, s=1.0):
+        r = np.empty_like(x)
+        self.L.ref_component_invert_f64(r, x, s, x.size)
+        return r
+
+    def diag_index(self, A):
+        d = np.empty(A.rows, np.uint64)
+        self.L.ref_diag_index(d, A.col, A.rowptr, A.rows)
+        return d
+
+    def invert_matrix(self, a):
+        a = np.ascontiguousarray(a, np.float64).copy()
+        n = a.shape[0]
+        p = np.zeros(n, np.int32)
+        det = self.L.ref_invert_matrix_f64(n, a.shape[1], a.reshape(-1), p)
+        return a, det
+
+    def rng_fill(self, seed, n, a, b):
+        v = np.empty(n)
+        self.L.ref_rng_fill_f64(seed, v, n, a, b)
+        return v
+
+    def pointstar_fd(self, m, d):
+        n = C.c_uint64()
+        nnz = C.c_uint64()
+        self.L.ref_pointstar_fd(m, d, C.byref(n), C.byref(nnz), None, None, None, None)
+        val = np.empty(nnz.value)
+        col = np.empty(nnz.value, np.uint64)
+        rp = np.empty(n.value + 1, np.uint64)
+        q = np.empty(n.value)
+        self.L.ref_pointstar_fd(m, d, C.byref(n), C.byref(nnz), val.ctypes.data, col.ctypes.data,
+                                rp.ctypes.data, q.ctypes.data)
+        return Csr(n.value, n.value, val, col, rp), q
+
+    def mg_pcg_solve(self, levels, x0, b, nu_pre=4, nu_post=4, omega_jac=0.5, omega_rich=1.0,
+                     coarse=COARSE_JACOBI, coarse_omega=1.0, tol_rel=1e-8, max_iter=100):
+        if coarse == COARSE_DIRECT:
+            raise ValueError("the reference's generic path has no dense direct coarse solver")
+        ref_coarse = {COARSE_NONE: 0, COARSE_SMOOTHER: 1, COARSE_JACOBI: 2}[coarse]
+        arr, keep = _levels_s(levels)
+        x = np.ascontiguousarray(x0, np.float64).copy()
+        hist = np.full(max_iter + 2, np.nan)
+        iters = C.c_uint64()
+        dfin = C.c_double()
+        secs = C.c_double()
+        st = self.L.ref_mg_pcg_solve(arr, len(levels), nu_pre, nu_post, omega_jac, omega_rich,
+                                     ref_coarse, coarse_omega, tol_rel, max_iter, x,
+                                     np.ascontiguousarray(b, np.float64), hist.ctypes.data,
+                                     hist.size, C.byref(iters), C.byref(dfin), C.byref(secs))
+        return dict(x=x, status=STATUS[st], iters=int(iters.value), def_final=dfin.value,
+                    hist=hist[:int(iters.value) + 1], seconds=secs.value)
+
+    def single_solve(self, kind, A, x0, b, filt=None, precond=1, omega_jac=1.0, omega_rich=1.0,
+                     tol_rel=np.sqrt(np.finfo(float).eps), max_iter=100, apply=False):
+        keep = []
+        a = _csr_s(A, keep)
+        f = np.ascontiguousarray(filt if filt is not None else np.zeros(0), np.uint64)
+        x = np.ascontiguousarray(x0, np.float64).copy()
+        hist = np.full(max_iter + 2, np.nan)
+        iters = C.c_uint64()
+        dfin = C.c_double()
+        st = self.L.ref_single_solve({"pcg": 0, "richardson": 1}[kind], C.byref(a),
+                                     f.ctypes.data if f.size else None, f.size, precond,
+                                     omega_jac, omega_rich, tol_rel, max_iter, int(apply), x,
+                                     np.ascontiguousarray(b, np.float64), hist.ctypes.data,
+                                     hist.size, C.byref(iters), C.byref(dfin))
+        return dict(x=x, status=STATUS[st], iters=int(iters.value), def_final=dfin.value,
+                    hist=hist[:int(iters.value) + 1])
+
+    def q1_poisson(self, dim, level, with_prol=True, cubature="gauss-legendre:2"):
+        """FE-assembled Q1 Laplace matrix, unit filter, vertex coordinates, sine-bubble
+        load vector and (optionally) the prolongation from level-1, all in the
+        reference's own DOF numbering."""
+        h = self.L.ref_q1_create(dim, level, int(with_prol), cubature.encode())
+        try:
+            s = np.zeros(5, np.uint64)
+            self.L.ref_q1_sizes(h, s)
+            n, nnz, nf, pn, pnnz = (int(v) for v in s)
+            val = np.empty(nnz)
+            col = np.empty(nnz, np.uint64)
+            rp = np.empty(n + 1, np.uint64)
+            filt = np.empty(nf, np.uint64)
+            coords = np.empty(n * dim)
+            rhs = np.empty(n)
+            pval = np.empty(max(pnnz, 1))
+            pcol = np.empty(max(pnnz, 1), np.uint64)
+            prp = np.empty(n + 1, np.uint64)
+            self.L.ref_q1_copy(h, val, col, rp, filt, coords, rhs, pval.ctypes.data,
+                               pcol.ctypes.data, prp.ctypes.data)
+        finally:
+            self.L.ref_q1_free(h)
+        out = dict(A=Csr(n, n, val, col, rp), filt=filt, coords=coords.reshape(n, dim), rhs=rhs)
+        if pnnz:
+            out["P"] = Csr(n, pn, pval[:pnnz], pcol[:pnnz], prp)
+        return out
+
+
+def csr_transpose(A):
+    """Explicit CSR transpose with column-sorted rows (SparseMatrixCSR::transpose)."""
+    counts = np.bincount(A.col.astype(np.int64), minlength=A.cols)
+    rp = np.zeros(A.cols + 1, np.uint64)
+    rp[1:] = np.cumsum(counts)
+    rows = np.repeat(np.arange(A.rows, dtype=np.uint64), np.diff(A.rowptr.astype(np.int64)))
+    # stable sort by column keeps the row order inside each new row
+    order = np.argsort(A.col, kind="stable")
+    return Csr(A.cols, A.rows, A.val[order].copy(), rows[order].copy(), rp)

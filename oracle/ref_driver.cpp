@@ -1,0 +1,527 @@
+// ref_driver.cpp -- C entry points into the reference's own generic (CPU) code.
+//
+// TEST INFRASTRUCTURE ONLY.  oracle/Makefile compiles this file together with
+// the reference's sources where they lie (the reference checkout's kernel/),
+// using the reference's own FEAT_NO_CONFIG switch (kernel/base_header.hpp:19),
+// into oracle/_ref/libfeatref.so.  Nothing of the reference is copied into
+// this repository; this file only calls its public API.  Tests use the
+// library to pin the C oracle (oracle/feat_oracle.c) against the real
+// reference, and bench.py may time it as the CPU baseline.
+#include <kernel/runtime.hpp>
+#include <kernel/util/random.hpp>
+#include <kernel/util/math.hpp>
+#include <kernel/lafem/dense_vector.hpp>
+#include <kernel/lafem/sparse_matrix_csr.hpp>
+#include <kernel/lafem/sparse_matrix_bcsr.hpp>
+#include <kernel/lafem/unit_filter.hpp>
+#include <kernel/lafem/none_filter.hpp>
+#include <kernel/lafem/transfer.hpp>
+#include <kernel/lafem/pointstar_factory.hpp>
+#include <kernel/lafem/arch/apply.hpp>
+#include <kernel/lafem/arch/axpy.hpp>
+#include <kernel/lafem/arch/dot_product.hpp>
+#include <kernel/lafem/arch/norm.hpp>
+#include <kernel/lafem/arch/scale.hpp>
+#include <kernel/lafem/arch/component_product.hpp>
+#include <kernel/lafem/arch/component_invert.hpp>
+#include <kernel/lafem/arch/diagonal.hpp>
+#include <kernel/solver/pcg.hpp>
+#include <kernel/solver/richardson.hpp>
+#include <kernel/solver/jacobi_precond.hpp>
+#include <kernel/solver/multigrid.hpp>
+#include <kernel/geometry/boundary_factory.hpp>
+#include <kernel/geometry/conformal_mesh.hpp>
+#include <kernel/geometry/common_factories.hpp>
+#include <kernel/geometry/mesh_part.hpp>
+#include <kernel/trafo/standard/mapping.hpp>
+#include <kernel/space/lagrange1/element.hpp>
+#include <kernel/cubature/dynamic_factory.hpp>
+#include <kernel/analytic/common.hpp>
+#include <kernel/assembly/symbolic_assembler.hpp>
+#include <kernel/assembly/unit_filter_assembler.hpp>
+#include <kernel/assembly/domain_assembler.hpp>
+#include <kernel/assembly/domain_assembler_helpers.hpp>
+#include <kernel/assembly/common_operators.hpp>
+#include <kernel/assembly/common_functionals.hpp>
+#include <kernel/assembly/grid_transfer.hpp>
+
+#include <chrono>
+#include <cstdint>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+using namespace FEAT;
+typedef double DT;
+typedef Index IT;
+typedef LAFEM::DenseVector<DT, IT> Vec;
+typedef LAFEM::DenseVector<IT, IT> IVec;
+typedef LAFEM::SparseMatrixCSR<DT, IT> Csr;
+typedef LAFEM::UnitFilter<DT, IT> UFilter;
+typedef LAFEM::Transfer<Csr> Trans;
+
+namespace
+{
+  // The reference runtime must be initialised once per process.
+  struct RuntimeInit
+  {
+    RuntimeInit()
+    {
+      static int argc = 1;
+      static char name[] = "featref";
+      static char* argv_data[] = {name, nullptr};
+      static char** argv = argv_data;
+      Runtime::initialize(argc, argv);
+    }
+  };
+  void ensure_runtime()
+  {
+    static RuntimeInit init;
+    (void)init;
+  }
+
+  Vec make_vec(const double* p, Index n)
+  {
+    Vec v(n);
+    if(n) std::memcpy(v.elements(), p, sizeof(double) * n);
+    return v;
+  }
+
+  IVec make_ivec(const std::uint64_t* p, Index n)
+  {
+    IVec v(n);
+    for(Index i = 0; i < n; ++i) v.elements()[i] = IT(p[i]);
+    return v;
+  }
+}
+
+extern "C" {
+
+// Mirrors orc_csr_t / orc_mg_level_t of oracle/feat_oracle.h.
+struct ref_csr_t
+{
+  std::uint64_t rows, cols, nnz;
+  const double* val;
+  const std::uint64_t* col;
+  const std::uint64_t* rowptr;
+};
+
+struct ref_level_t
+{
+  ref_csr_t A;
+  const std::uint64_t* filt;
+  std::uint64_t nfilt;
+  ref_csr_t P;
+  ref_csr_t R;
+};
+
+static Csr make_csr(const ref_csr_t& m)
+{
+  IVec ci = make_ivec(m.col, m.nnz);
+  Vec va = make_vec(m.val, m.nnz);
+  IVec rp = make_ivec(m.rowptr, m.rows + 1);
+  return Csr(m.rows, m.cols, ci, va, rp);
+}
+
+static UFilter make_filter(Index n, const std::uint64_t* idx, std::uint64_t nf)
+{
+  UFilter f(n);
+  for(std::uint64_t i = 0; i < nf; ++i)
+    f.add(IT(idx[i]), DT(0));
+  return f;
+}
+
+int ref_init() { ensure_runtime(); return 1; }
+
+// ---------------- Arch kernels (generic backend) ----------------
+void ref_csr_apply_f64(double* r, double a, const double* x, double b, const double* y,
+                       const double* val, const std::uint64_t* col, const std::uint64_t* rowptr,
+                       std::uint64_t rows, std::uint64_t cols, std::uint64_t nnz, int transposed)
+{
+  ensure_runtime();
+  LAFEM::Arch::Apply::csr_generic(r, a, x, b, y, val, (const IT*)col, (const IT*)rowptr,
+                                  Index(rows), Index(cols), Index(nnz), transposed != 0);
+}
+
+void ref_bcsr_apply_f64(int bh, int bw, double* r, double a, const double* x, double b,
+                        const double* y, const double* val, const std::uint64_t* col,
+                        const std::uint64_t* rowptr, std::uint64_t rows, std::uint64_t cols,
+                        std::uint64_t nnz)
+{
+  ensure_runtime();
+  const IT* c = (const IT*)col;
+  const IT* rp = (const IT*)rowptr;
+  if(bh == 2 && bw == 2)
+    LAFEM::Arch::Apply::bcsr_generic<2, 2>(r, a, x, b, y, val, c, rp, rows, cols, nnz);
+  else if(bh == 2 && bw == 1)
+    LAFEM::Arch::Apply::bcsr_generic<2, 1>(r, a, x, b, y, val, c, rp, rows, cols, nnz);
+  else if(bh == 1 && bw == 2)
+    LAFEM::Arch::Apply::bcsr_generic<1, 2>(r, a, x, b, y, val, c, rp, rows, cols, nnz);
+  else if(bh == 3 && bw == 3)
+    LAFEM::Arch::Apply::bcsr_generic<3, 3>(r, a, x, b, y, val, c, rp, rows, cols, nnz);
+}
+
+void ref_axpy_f64(double* r, double a, const double* x, const double* y, std::uint64_t n)
+{
+  LAFEM::Arch::Axpy::value_generic(r, a, x, y, Index(n));
+}
+
+double ref_dot_f64(const double* x, const double* y, std::uint64_t n)
+{
+  return LAFEM::Arch::DotProduct::value_generic(x, y, Index(n));
+}
+
+double ref_triple_dot_f64(const double* x, const double* y, const double* z, std::uint64_t n)
+{
+  return LAFEM::Arch::TripleDotProduct::value_generic(x, y, z, Index(n));
+}
+
+double ref_norm2_f64(const double* x, std::uint64_t n)
+{
+  return LAFEM::Arch::Norm2::value_generic(x, Index(n));
+}
+
+void ref_scale_f64(double* r, const double* x, double s, std::uint64_t n)
+{
+  LAFEM::Arch::Scale::value_generic(r, x, s, Index(n));
+}
+
+void ref_component_product_f64(double* r, const double* x, const double* y, std::uint64_t n)
+{
+  LAFEM::Arch::ComponentProduct::value_generic(r, x, y, Index(n));
+}
+
+void ref_component_invert_f64(double* r, const double* x, double s, std::uint64_t n)
+{
+  LAFEM::Arch::ComponentInvert::value_generic(r, x, s, Index(n));
+}
+
+void ref_diag_index(std::uint64_t* diag, const std::uint64_t* col, const std::uint64_t* rowptr,
+                    std::uint64_t rows)
+{
+  LAFEM::Arch::Diagonal::csr_generic((IT*)diag, (const IT*)col, (const IT*)rowptr, Index(rows));
+}
+
+double ref_invert_matrix_f64(int n, int stride, double* a, int* p)
+{
+  return Math::invert_matrix(n, stride, a, p);
+}
+
+void ref_rng_fill_f64(std::uint64_t seed, double* v, std::uint64_t n, double a, double b)
+{
+  ensure_runtime();
+  Random rng(seed);
+  Vec vec(rng, Index(n), a, b);
+  std::memcpy(v, vec.elements(), sizeof(double) * n);
+}
+
+// ---------------- FD pointstar fixture (lafem/pointstar_factory.hpp) ----------------
+// Call once with val == nullptr to get the sizes.
+int ref_pointstar_fd(std::uint64_t m, std::uint64_t d, std::uint64_t* n_out,
+                     std::uint64_t* nnz_out, double* val, std::uint64_t* col,
+                     std::uint64_t* rowptr, double* q2bubble)
+{
+  ensure_runtime();
+  LAFEM::PointstarFactoryFD<DT, IT> psf(m, d);
+  Csr a = psf.matrix_csr();
+  *n_out = a.rows();
+  *nnz_out = a.used_elements();
+  if(val)
+  {
+    std::memcpy(val, a.val(), sizeof(double) * a.used_elements());
+    for(Index i = 0; i < a.used_elements(); ++i) col[i] = a.col_ind()[i];
+    for(Index i = 0; i <= a.rows(); ++i) rowptr[i] = a.row_ptr()[i];
+    Vec q = psf.vector_q2_bubble();
+    std::memcpy(q2bubble, q.elements(), sizeof(double) * q.size());
+  }
+  return 0;
+}
+
+// ---------------- solvers on caller-given matrices ----------------
+} // extern "C"
+
+// Records the defect norm of every iteration (IterativeSolver::_set_new_defect
+// is virtual, solver/iterative.hpp:853).
+template<typename Base_>
+class Recorder : public Base_
+{
+public:
+  std::vector<double> hist;
+  template<typename... A_>
+  explicit Recorder(A_&&... args) : Base_(std::forward<A_>(args)...) {}
+  virtual Solver::Status _set_initial_defect(const Vec& d, const Vec& s) override
+  {
+    Solver::Status st = Base_::_set_initial_defect(d, s);
+    hist.assign(1, this->_def_init);
+    return st;
+  }
+  virtual Solver::Status _set_new_defect(const Vec& d, const Vec& s) override
+  {
+    Solver::Status st = Base_::_set_new_defect(d, s);
+    hist.push_back(this->_def_cur);
+    return st;
+  }
+};
+
+typedef Solver::PreconditionedIterativeSolver<Vec> PISolver;
+
+extern "C" {
+
+static void copy_hist(const std::vector<double>& h, double* out, std::uint64_t cap)
+{
+  if(!out) return;
+  for(std::size_t i = 0; i < h.size() && i < cap; ++i) out[i] = h[i];
+}
+
+// coarse: 0 none (identity + filter), 1 Richardson-Jacobi smoother, 2 Jacobi(coarse_omega)
+int ref_mg_pcg_solve(const ref_level_t* L, int nl, int nu_pre, int nu_post, double omega_jac,
+                     double omega_rich, int coarse, double coarse_omega, double tol_rel,
+                     std::uint64_t max_iter, double* x, const double* b, double* hist,
+                     std::uint64_t hist_cap, std::uint64_t* iters, double* def_final,
+                     double* solve_seconds)
+{
+  ensure_runtime();
+  std::vector<Csr> A(nl), P(nl), R(nl);
+  std::vector<UFilter> F;
+  std::vector<Trans> T(nl);
+  F.reserve(nl);
+  for(int i = 0; i < nl; ++i)
+  {
+    A[i] = make_csr(L[i].A);
+    F.push_back(make_filter(L[i].A.rows, L[i].filt, L[i].nfilt));
+    if(i + 1 < nl)
+      T[i] = Trans(make_csr(L[i].P), make_csr(L[i].R));
+  }
+  auto hierarchy = std::make_shared<Solver::MultiGridHierarchy<Csr, UFilter, Trans>>(nl);
+  for(int i = 0; i < nl; ++i)
+  {
+    auto jac = Solver::new_jacobi_precond(A[i], F[i], omega_jac);
+    if(i + 1 < nl)
+    {
+      auto pre = Solver::new_richardson(A[i], F[i], omega_rich, jac);
+      pre->set_min_iter(nu_pre);
+      pre->set_max_iter(nu_pre);
+      auto post = Solver::new_richardson(A[i], F[i], omega_rich, jac);
+      post->set_min_iter(nu_post);
+      post->set_max_iter(nu_post);
+      hierarchy->push_level(A[i], F[i], T[i], pre, post, pre);
+    }
+    else
+    {
+      std::shared_ptr<Solver::SolverBase<Vec>> cs;
+      if(coarse == 1)
+      {
+        auto s = Solver::new_richardson(A[i], F[i], omega_rich, jac);
+        s->set_min_iter(nu_pre);
+        s->set_max_iter(nu_pre);
+        cs = s;
+      }
+      else if(coarse == 2)
+        cs = Solver::new_jacobi_precond(A[i], F[i], coarse_omega);
+      hierarchy->push_level(A[i], F[i], cs);
+    }
+  }
+  auto mgv = Solver::new_multigrid(hierarchy, Solver::MultiGridCycle::V);
+  auto pcg = std::make_shared<Recorder<Solver::PCG<Csr, UFilter>>>(A[0], F[0], mgv);
+  pcg->set_tol_rel(tol_rel);
+  pcg->set_max_iter(max_iter);
+  hierarchy->init();
+  pcg->init();
+  Vec vx = make_vec(x, L[0].A.rows);
+  Vec vb = make_vec(b, L[0].A.rows);
+  const auto t0 = std::chrono::steady_clock::now();
+  Solver::Status st = Solver::solve(*pcg, vx, vb, A[0], F[0]);
+  const auto t1 = std::chrono::steady_clock::now();
+  if(solve_seconds) *solve_seconds = std::chrono::duration<double>(t1 - t0).count();
+  std::memcpy(x, vx.elements(), sizeof(double) * L[0].A.rows);
+  copy_hist(pcg->hist, hist, hist_cap);
+  *iters = pcg->get_num_iter();
+  *def_final = pcg->get_def_final();
+  pcg->done();
+  hierarchy->done();
+  return int(st);
+}
+
+// kind: 0 = PCG, 1 = Richardson;  precond: 0 none, 1 Jacobi(omega_jac)
+// use_apply: 0 -> Solver::solve (correct), 1 -> solver.apply (x formatted)
+int ref_single_solve(int kind, const ref_csr_t* m, const std::uint64_t* filt, std::uint64_t nfilt,
+                     int precond, double omega_jac, double omega_rich, double tol_rel,
+                     std::uint64_t max_iter, int use_apply, double* x, const double* b,
+                     double* hist, std::uint64_t hist_cap, std::uint64_t* iters,
+                     double* def_final)
+{
+  ensure_runtime();
+  Csr A = make_csr(*m);
+  UFilter F = make_filter(m->rows, filt, nfilt);
+  std::shared_ptr<Solver::SolverBase<Vec>> pre;
+  if(precond) pre = Solver::new_jacobi_precond(A, F, omega_jac);
+  std::shared_ptr<PISolver> s;
+  std::vector<double>* h = nullptr;
+  if(kind == 0)
+  {
+    auto p = std::make_shared<Recorder<Solver::PCG<Csr, UFilter>>>(A, F, pre);
+    h = &p->hist;
+    s = p;
+  }
+  else
+  {
+    auto p = std::make_shared<Recorder<Solver::Richardson<Csr, UFilter>>>(A, F, omega_rich, pre);
+    h = &p->hist;
+    s = p;
+  }
+  s->set_tol_rel(tol_rel);
+  s->set_max_iter(max_iter);
+  s->init();
+  Vec vx = make_vec(x, m->rows);
+  Vec vb = make_vec(b, m->rows);
+  Solver::Status st;
+  if(use_apply)
+    st = s->apply(vx, vb);
+  else
+    st = Solver::solve(*s, vx, vb, A, F);
+  std::memcpy(x, vx.elements(), sizeof(double) * m->rows);
+  copy_hist(*h, hist, hist_cap);
+  *iters = s->get_num_iter();
+  *def_final = s->get_def_final();
+  s->done();
+  return int(st);
+}
+
+} // extern "C"
+
+// ---------------- Q1 Poisson assembly on the refined unit square/cube ----------------
+struct ref_q1_t
+{
+  int dim;
+  Index n, nnz, nfilt, pn, pnnz;
+  std::vector<double> val, coords, rhs, pval;
+  std::vector<std::uint64_t> col, rowptr, filt, pcol, prowptr;
+};
+
+template<int dim_>
+static ref_q1_t* q1_build(int level, int with_prol, const char* cubature)
+{
+  typedef Shape::Hypercube<dim_> ShapeType;
+  typedef Geometry::ConformalMesh<ShapeType> MeshType;
+  typedef Geometry::MeshPart<MeshType> MeshPartType;
+  typedef Trafo::Standard::Mapping<MeshType> TrafoType;
+  typedef Space::Lagrange1::Element<TrafoType> SpaceType;
+
+  // coarse mesh on level-1 (or level 0), refined once with the StandardRefinery
+  const int lc = (with_prol && level > 0) ? level - 1 : level;
+  const Index lci = Index(lc);
+  Geometry::RefinedUnitCubeFactory<MeshType> fac_c(lci);
+  MeshType mesh_c(fac_c);
+  std::unique_ptr<MeshType> mesh_f;
+  if(lc != level)
+  {
+    Geometry::StandardRefinery<MeshType> ref(mesh_c);
+    mesh_f.reset(new MeshType(ref));
+  }
+  MeshType& mesh = mesh_f ? *mesh_f : mesh_c;
+  TrafoType trafo(mesh);
+  SpaceType space(trafo);
+  Assembly::DomainAssembler<TrafoType> dasm(trafo);
+  dasm.compile_all_elements();
+
+  Csr mat;
+  Assembly::SymbolicAssembler::assemble_matrix_std1(mat, space);
+  mat.format();
+  Assembly::Common::LaplaceOperator lap;
+  Assembly::assemble_bilinear_operator_matrix_1(dasm, mat, lap, space, cubature);
+
+  Geometry::BoundaryFactory<MeshType> bfac(mesh);
+  MeshPartType boundary(bfac);
+  Assembly::UnitFilterAssembler<MeshType> uasm;
+  uasm.add_mesh_part(boundary);
+  UFilter filter;
+  uasm.assemble(filter, space);
+
+  Vec rhs = mat.create_vector_r();
+  rhs.format();
+  Analytic::Common::SineBubbleFunction<dim_> sol_fn;
+  Assembly::Common::LaplaceFunctional<decltype(sol_fn)> force(sol_fn);
+  Assembly::assemble_linear_functional_vector(dasm, rhs, force, space, cubature);
+  filter.filter_rhs(rhs);
+
+  ref_q1_t* q = new ref_q1_t();
+  q->dim = dim_;
+  q->n = mat.rows();
+  q->nnz = mat.used_elements();
+  q->val.assign(mat.val(), mat.val() + q->nnz);
+  q->col.assign(mat.col_ind(), mat.col_ind() + q->nnz);
+  q->rowptr.assign(mat.row_ptr(), mat.row_ptr() + q->n + 1);
+  q->rhs.assign(rhs.elements(), rhs.elements() + q->n);
+  q->nfilt = filter.used_elements();
+  q->filt.assign(filter.get_indices(), filter.get_indices() + q->nfilt);
+  const auto& vtx = mesh.get_vertex_set();
+  q->coords.resize(std::size_t(q->n) * dim_);
+  for(Index i = 0; i < q->n; ++i)
+    for(int k = 0; k < dim_; ++k)
+      q->coords[std::size_t(i) * dim_ + k] = double(vtx[i][k]);
+  q->pn = 0;
+  q->pnnz = 0;
+  if(lc != level)
+  {
+    // prolongation as in control/scalar_basic.hpp:282-331
+    TrafoType trafo_c(mesh_c);
+    SpaceType space_c(trafo_c);
+    Csr prol;
+    Assembly::SymbolicAssembler::assemble_matrix_2lvl(prol, space, space_c);
+    Vec weight = prol.create_vector_l();
+    prol.format();
+    weight.format();
+    Assembly::GridTransfer::assemble_prolongation(prol, weight, space, space_c, cubature);
+    weight.component_invert(weight);
+    prol.scale_rows(prol, weight);
+    q->pn = prol.columns();
+    q->pnnz = prol.used_elements();
+    q->pval.assign(prol.val(), prol.val() + q->pnnz);
+    q->pcol.assign(prol.col_ind(), prol.col_ind() + q->pnnz);
+    q->prowptr.assign(prol.row_ptr(), prol.row_ptr() + q->n + 1);
+  }
+  return q;
+}
+
+extern "C" {
+
+void* ref_q1_create(int dim, int level, int with_prol, const char* cubature)
+{
+  ensure_runtime();
+  if(dim == 2) return q1_build<2>(level, with_prol, cubature);
+  if(dim == 3) return q1_build<3>(level, with_prol, cubature);
+  return nullptr;
+}
+
+void ref_q1_sizes(void* h, std::uint64_t* out)
+{
+  ref_q1_t* q = (ref_q1_t*)h;
+  out[0] = q->n;
+  out[1] = q->nnz;
+  out[2] = q->nfilt;
+  out[3] = q->pn;
+  out[4] = q->pnnz;
+}
+
+void ref_q1_copy(void* h, double* val, std::uint64_t* col, std::uint64_t* rowptr,
+                 std::uint64_t* filt, double* coords, double* rhs, double* pval,
+                 std::uint64_t* pcol, std::uint64_t* prowptr)
+{
+  ref_q1_t* q = (ref_q1_t*)h;
+  std::memcpy(val, q->val.data(), sizeof(double) * q->nnz);
+  std::memcpy(col, q->col.data(), sizeof(std::uint64_t) * q->nnz);
+  std::memcpy(rowptr, q->rowptr.data(), sizeof(std::uint64_t) * (q->n + 1));
+  std::memcpy(filt, q->filt.data(), sizeof(std::uint64_t) * q->nfilt);
+  std::memcpy(coords, q->coords.data(), sizeof(double) * q->coords.size());
+  std::memcpy(rhs, q->rhs.data(), sizeof(double) * q->n);
+  if(q->pnnz && pval)
+  {
+    std::memcpy(pval, q->pval.data(), sizeof(double) * q->pnnz);
+    std::memcpy(pcol, q->pcol.data(), sizeof(std::uint64_t) * q->pnnz);
+    std::memcpy(prowptr, q->prowptr.data(), sizeof(std::uint64_t) * (q->n + 1));
+  }
+}
+
+void ref_q1_free(void* h) { delete (ref_q1_t*)h; }
+
+} // extern "C"
