@@ -133,28 +133,43 @@ def main():
     assert res["status"] == "success", res
     t_mean = float(np.mean(times))
 
-    # ---- SpMV r = A*x roofline, HIP events on the launch stream
+    # ---- SpMV r = A*x on the same matrix, HIP events on the launch stream:
+    # the caller's CSR (u64) and the SELL-64 plan the solver runs on
     k, _ = feat.libs()
     h = feat.Handle(0, stream=stream.cuda_stream)
     xs = torch.rand(n, dtype=torch.float64, device=dev)
     r = torch.empty(n, dtype=torch.float64, device=dev)
-    for _ in range(5):
-        feat.check(k.feat_csr_apply_f64_u64(h.h, feat.ptr(r), 1.0, feat.ptr(xs), 0.0, None, C.c_void_p(A.val),
-                                            C.c_void_p(A.col), C.c_void_p(A.rowptr), n, n, nnz, 0), "spmv")
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record(stream)
-    for _ in range(args.spmv_reps):
-        k.feat_csr_apply_f64_u64(h.h, feat.ptr(r), 1.0, feat.ptr(xs), 0.0, None, C.c_void_p(A.val),
-                                 C.c_void_p(A.col), C.c_void_p(A.rowptr), n, n, nnz, 0)
-    e1.record(stream)
-    e1.synchronize()
-    spmv_ms = e0.elapsed_time(e1) / args.spmv_reps
-    alg = spmv_bytes(n, nnz)
-    ref_model = nnz * 16 + n * 8             # product_matvec-bench.cpp:41-44 model
-    achieved = alg / (spmv_ms * 1e-3) / 1e9
-    log(f"spmv: {spmv_ms * 1e3:.1f} us, {achieved:.0f} GB/s algorithmic "
-        f"({ref_model / (spmv_ms * 1e-3) / 1024 ** 3:.0f} GiB/s in the reference's model)")
+    csr = feat.Matrix(feat.FMT_CSR, feat.IDX_U64, n, n, nnz, A.val, A.col, A.rowptr, None)
+    plan = feat.Matrix()
+    feat.check(k.feat_sell_create_f64(h.h, C.byref(A), C.byref(plan)), "feat_sell_create_f64")
+
+    def time_apply(m):
+        for _ in range(5):
+            feat.check(k.feat_matrix_apply_f64(h.h, C.byref(m), feat.ptr(r), 1.0, feat.ptr(xs), 0.0, None), "spmv")
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(args.spmv_reps):
+            k.feat_matrix_apply_f64(h.h, C.byref(m), feat.ptr(r), 1.0, feat.ptr(xs), 0.0, None)
+        e1.record(stream)
+        e1.synchronize()
+        return e0.elapsed_time(e1) / args.spmv_reps
+
+    csr_ms = time_apply(csr)
+    sell_ms = time_apply(plan)
+    idx_bytes = {feat.IDX_D16: 2, feat.IDX_U32: 4}[plan.index]
+    nslices = (n + 63) // 64
+    # bytes each kernel must move: its own arrays once, x once, r once
+    csr_bytes = spmv_bytes(n, nnz)
+    sell_bytes = int(plan.nnz) * (8 + idx_bytes) + n * 2 + (nslices + 1) * 8 + 2 * n * 8
+    csr_gbs = csr_bytes / (csr_ms * 1e-3) / 1e9
+    sell_gbs = sell_bytes / (sell_ms * 1e-3) / 1e9
+    ref_model = nnz * 16 + n * 8             # product_matvec-bench.cpp:41-44 model (GiB = 1024^3)
+    log(f"spmv csr-u64: {csr_ms * 1e3:.1f} us, {csr_gbs:.0f} GB/s | sell-64 ({'i16 delta' if idx_bytes == 2 else 'u32'}): "
+        f"{sell_ms * 1e3:.1f} us, {sell_gbs:.0f} GB/s own bytes, {csr_bytes / (sell_ms * 1e-3) / 1e9:.0f} GB/s CSR-equivalent")
+    padded = int(plan.nnz)
+    feat.check(k.feat_sell_destroy_f64(C.byref(plan)), "feat_sell_destroy_f64")
     h.close()
+    achieved = sell_gbs
 
     out = {
         "metric": "MG-PCG time-to-1e-8 residual (s) + SpMV HBM GB/s, 3D Q1 Poisson 2.1M dofs",
@@ -177,11 +192,18 @@ def main():
             "final_rel_defect": res["def_final"] / res["def_init"],
             "parallelism": "single GPU",
         },
-        "spmv": {"kernel": "k_csr_rows<double,unsigned long,EpiApply<double>>", "us_per_launch": spmv_ms * 1e3,
-                 "algorithmic_bytes": alg, "gbps_algorithmic": achieved,
-                 "gibps_reference_model": ref_model / (spmv_ms * 1e-3) / 1024 ** 3},
+        "spmv": {
+            "sell64": {"kernel": "k_sell_rows<double,short,true,EpiApply<double>>", "us_per_launch": sell_ms * 1e3,
+                       "bytes_per_launch": sell_bytes, "gbps": sell_gbs,
+                       "csr_equivalent_gbps": csr_bytes / (sell_ms * 1e-3) / 1e9,
+                       "index": "i16 delta" if idx_bytes == 2 else "u32", "padded_entries": padded},
+            "csr_u64": {"kernel": "k_csr_rows<double,unsigned long,EpiApply<double>>", "us_per_launch": csr_ms * 1e3,
+                        "bytes_per_launch": csr_bytes, "gbps": csr_gbs,
+                        "gibps_reference_model": ref_model / (csr_ms * 1e-3) / 1024 ** 3},
+        },
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": achieved / PEAK_HBM_GBS, "traffic": None},
+                     "frac": achieved / PEAK_HBM_GBS, "traffic": None,
+                     "kernel": "SELL-64 SpMV r = A*x (the solver's matrix format), bytes = its arrays + x + r"},
     }
     if not args.no_cpu_baseline:
         log("cpu baseline: reference MG-PCG on one host core ...")

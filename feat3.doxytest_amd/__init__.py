@@ -46,6 +46,16 @@ class CsrF32(C.Structure):
                 ("rowptr", _vp)]
 
 
+class Matrix(C.Structure):
+    """feat_matrix_f64_t / feat_matrix_f32_t (same layout)."""
+    _fields_ = [("format", _i), ("index", _i), ("rows", _u64), ("cols", _u64), ("nnz", _u64),
+                ("val", _vp), ("col", _vp), ("ptr", _vp), ("rowlen", _vp)]
+
+
+FMT_CSR, FMT_SELL = 0, 1
+IDX_U64, IDX_U32, IDX_D16 = 0, 1, 2
+
+
 class ReduceWs(C.Structure):
     _fields_ = [("partials", _vp), ("counter", _vp)]
 
@@ -53,7 +63,7 @@ class ReduceWs(C.Structure):
 class MgPcgCfg(C.Structure):
     _fields_ = [("nu_pre", _i), ("nu_post", _i), ("omega_jac", _d), ("omega_rich", _d),
                 ("coarse", _i), ("coarse_omega", _d), ("tol_rel", _d), ("max_iter", _u64),
-                ("force_generic", _i)]
+                ("force_generic", _i), ("csr_only", _i)]
 
 
 class LevelHost(C.Structure):
@@ -106,21 +116,29 @@ KERNEL_API = [
     ("feat_unit_filter_mask_u64", _i, [_vp, _vp, _u64, _vp, _u64]),
     ("feat_unit_filter_blocked_def_f64_u64", _i, [_vp, _i, _vp, _vp, _u64]),
     ("feat_mean_filter_f64", _i, [_vp, _vp, _vp, _vp, _d, _u64]),
-    ("feat_csr_defect_f64", _i, [_vp, C.POINTER(CsrF64), _vp, _vp, _vp, _vp, _vp, C.POINTER(ReduceWs)]),
-    ("feat_csr_apply_masked_f64", _i, [_vp, C.POINTER(CsrF64), _vp, _vp, _vp, _vp, _vp, _vp, C.POINTER(ReduceWs)]),
-    ("feat_csr_prol_add_f64", _i, [_vp, C.POINTER(CsrF64), _vp, _vp, _vp]),
-    ("feat_rj_smooth_f64", _i, [_vp, C.POINTER(CsrF64), _vp, _vp, _i, _d, _vp, _vp, _vp, _vp, _vp, _vp,
+    ("feat_matrix_from_csr_f64", _i, [C.POINTER(CsrF64), C.POINTER(Matrix)]),
+    ("feat_sell_create_f64", _i, [_vp, C.POINTER(CsrF64), C.POINTER(Matrix)]),
+    ("feat_sell_create_f32", _i, [_vp, C.POINTER(CsrF64), C.POINTER(Matrix)]),
+    ("feat_sell_destroy_f64", _i, [C.POINTER(Matrix)]),
+    ("feat_sell_destroy_f32", _i, [C.POINTER(Matrix)]),
+    ("feat_matrix_apply_f64", _i, [_vp, C.POINTER(Matrix), _vp, _d, _vp, _d, _vp]),
+    ("feat_defect_f64", _i, [_vp, C.POINTER(Matrix), _vp, _vp, _vp, _vp, _vp, C.POINTER(ReduceWs)]),
+    ("feat_apply_masked_f64", _i, [_vp, C.POINTER(Matrix), _vp, _vp, _vp, _vp, _vp, _vp, C.POINTER(ReduceWs)]),
+    ("feat_prol_add_f64", _i, [_vp, C.POINTER(Matrix), _vp, _vp, _vp]),
+    ("feat_rj_smooth_f64", _i, [_vp, C.POINTER(Matrix), _vp, _vp, _i, _d, _vp, _vp, _vp, _vp, _vp, _vp,
                                 _vp, _vp, C.POINTER(ReduceWs)]),
     ("feat_dense_gemv_f64", _i, [_vp, _i, _vp, _vp, _vp, _vp]),
     ("feat_pcg_update_f64", _i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _u64, C.POINTER(ReduceWs)]),
     ("feat_pcg_direction_f64", _i, [_vp, _vp, _vp, _vp, _vp, _u64]),
     ("feat_read_scalar_f64", _i, [_vp, _vp, C.POINTER(_d)]),
-    ("feat_csr_apply_masked_f32", _i, [_vp, C.POINTER(CsrF32), _vp, _vp, _vp, _vp, C.POINTER(ReduceWs)]),
-    ("feat_csr_defect_f32", _i, [_vp, C.POINTER(CsrF32), _vp, _vp, _vp, _vp, _vp, C.POINTER(ReduceWs)]),
-    ("feat_csr_prol_add_f32", _i, [_vp, C.POINTER(CsrF32), _vp, _vp, _vp]),
-    ("feat_rj_smooth_f32", _i, [_vp, C.POINTER(CsrF32), _vp, _vp, _i, _f, _vp, _vp, _vp, _vp, _vp, _vp,
+    ("feat_apply_masked_f32", _i, [_vp, C.POINTER(Matrix), _vp, _vp, _vp, _vp, C.POINTER(ReduceWs)]),
+    ("feat_defect_f32", _i, [_vp, C.POINTER(Matrix), _vp, _vp, _vp, _vp, _vp, C.POINTER(ReduceWs)]),
+    ("feat_prol_add_f32", _i, [_vp, C.POINTER(Matrix), _vp, _vp, _vp]),
+    ("feat_rj_smooth_f32", _i, [_vp, C.POINTER(Matrix), _vp, _vp, _i, _f, _vp, _vp, _vp, _vp, _vp, _vp,
                                 C.POINTER(ReduceWs)]),
     ("feat_dense_gemv_f32", _i, [_vp, _i, _vp, _vp, _vp, _vp]),
+    ("feat_convert_dot_f32_f64", _i, [_vp, _vp, _vp, _vp, _u64, _vp, C.POINTER(ReduceWs)]),
+    ("feat_dense_gemv_mixed", _i, [_vp, _i, _vp, _vp, _vp, _vp]),
 ]
 
 HOST_API = [
@@ -253,9 +271,46 @@ def rhs_random(seed, n, a=-1.0, b=1.0):
 
 
 def mgpcg_cfg(nu_pre=4, nu_post=4, omega_jac=0.5, omega_rich=1.0, coarse="direct",
-              coarse_omega=1.0, tol_rel=1e-8, max_iter=100, force_generic=False):
+              coarse_omega=1.0, tol_rel=1e-8, max_iter=100, force_generic=False, csr_only=False):
     return MgPcgCfg(nu_pre, nu_post, omega_jac, omega_rich, COARSE[coarse], coarse_omega, tol_rel,
-                    max_iter, int(force_generic))
+                    max_iter, int(force_generic), int(csr_only))
+
+
+def csr_matrix(val, col, rowptr, rows, cols):
+    """CSR descriptor (Matrix) of device tensors."""
+    k, _ = libs()
+    c = csr_desc(val, col, rowptr, rows, cols)
+    m = Matrix()
+    if val.dtype.itemsize == 8 and col.dtype.itemsize == 8:
+        check(k.feat_matrix_from_csr_f64(C.byref(c), C.byref(m)), "feat_matrix_from_csr_f64")
+    else:
+        idx = IDX_U64 if col.dtype.itemsize == 8 else IDX_U32
+        m = Matrix(FMT_CSR, idx, rows, cols, val.numel(), val.data_ptr(), col.data_ptr(),
+                   rowptr.data_ptr(), None)
+    return m
+
+
+class SellPlan:
+    """SELL-64 plan of a device CSR (f64 values with u64 indices); fp='f32' rounds values."""
+
+    def __init__(self, handle, val, col, rowptr, rows, cols, fp="f64"):
+        self.k, _ = libs()
+        self.fp = fp
+        c = csr_desc(val, col, rowptr, rows, cols)
+        self.m = Matrix()
+        fn = self.k.feat_sell_create_f64 if fp == "f64" else self.k.feat_sell_create_f32
+        check(fn(handle.h, C.byref(c), C.byref(self.m)), fn.__name__)
+
+    def close(self):
+        if self.m.format == FMT_SELL:
+            fn = self.k.feat_sell_destroy_f64 if self.fp == "f64" else self.k.feat_sell_destroy_f32
+            fn(C.byref(self.m))
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def runtime_init(device=0, stream=None):

@@ -129,15 +129,21 @@ __device__ void grid_reduce_finish(const GridReduce& g, double v0, double v1)
   double b1 = r1 ? block_sum(v1, red) : 0.0;
   if(threadIdx.x == 0)
   {
-    if(r0) __hip_atomic_store(&g.partials[blockIdx.x], b0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if(r1) __hip_atomic_store(&g.partials[kMaxGrid + blockIdx.x], b1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __threadfence();
-    const uint32_t prev = atomicAdd(g.counter, 1u);
+    // Publish the partials with agent-scope atomic exchanges: they are performed
+    // at the device's coherence point, so no cache writeback (an agent-scope
+    // release fence flushes the XCD's L2 and costs ~50 ns per workgroup).  The
+    // returned values are kept live so the exchanges are issued with return,
+    // and s_waitcnt(0) waits for them before this workgroup is counted in.
+    double o0 = 0.0, o1 = 0.0;
+    if(r0) o0 = __hip_atomic_exchange(&g.partials[blockIdx.x], b0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if(r1) o1 = __hip_atomic_exchange(&g.partials[kMaxGrid + blockIdx.x], b1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __builtin_amdgcn_s_waitcnt(0);
+    asm volatile("" ::"v"(o0), "v"(o1));
+    const uint32_t prev = __hip_atomic_fetch_add(g.counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     last = (prev == gridDim.x - 1) ? 1 : 0;
   }
   __syncthreads();
   if(!last) return;
-  __threadfence();
   double s0 = 0.0, s1 = 0.0;
   for(uint32_t i = threadIdx.x; i < gridDim.x; i += blockDim.x)
   {
@@ -353,6 +359,141 @@ int launch_rows(hipStream_t s, const DT* val, const IT* col, const IT* rowptr, c
   hipLaunchKernelGGL((k_csr_rows<DT, IT, Epi>), dim3(grid), dim3(kThreads), 0, s, val, col, rowptr,
                      x, rows, epi, gr);
   return check_launch(name);
+}
+
+// ---------------------------------------------------------------------------
+// SELL-64 engine: one lane per row, coalesced 64-wide loads of val/col, the
+// row's entries summed in CSR order (4 entries per step in flight).
+// ---------------------------------------------------------------------------
+template<bool DELTA, typename CT>
+__device__ __forceinline__ uint64_t sell_col(CT c, uint64_t row)
+{
+  if(DELTA)
+    return uint64_t(std::int64_t(row) + std::int64_t(c));
+  return uint64_t(c);
+}
+
+template<typename DT, typename CT, bool DELTA, class Epi>
+__global__ __launch_bounds__(kThreads) void k_sell_rows(const DT* __restrict__ val,
+                                                        const CT* __restrict__ col,
+                                                        const uint64_t* __restrict__ soff,
+                                                        const uint16_t* __restrict__ rlen,
+                                                        const DT* __restrict__ x, uint64_t rows,
+                                                        Epi epi, GridReduce gr)
+{
+  const uint64_t nch = (rows + kThreads - 1) / kThreads;
+  const XcdRange xr = xcd_range(nch);
+  const int lane = threadIdx.x & 63;
+  double acc0 = 0.0, acc1 = 0.0;
+  epi.begin();
+  for(uint64_t ch = xr.first; ch < xr.end; ch += xr.stride)
+  {
+    const uint64_t row = ch * kThreads + threadIdx.x;
+    if(row < rows)
+    {
+      const uint64_t off = soff[row >> 6];
+      const int len = rlen[row];
+      const DT* v = val + off + lane;
+      const CT* c = col + off + lane;
+      DT sum = DT(0);
+      int k = 0;
+      for(; k + 4 <= len; k += 4)
+      {
+        const DT v0 = v[(k + 0) * 64], v1 = v[(k + 1) * 64], v2 = v[(k + 2) * 64], v3 = v[(k + 3) * 64];
+        const uint64_t c0 = sell_col<DELTA>(c[(k + 0) * 64], row), c1 = sell_col<DELTA>(c[(k + 1) * 64], row);
+        const uint64_t c2 = sell_col<DELTA>(c[(k + 2) * 64], row), c3 = sell_col<DELTA>(c[(k + 3) * 64], row);
+        const DT x0 = x[c0], x1 = x[c1], x2 = x[c2], x3 = x[c3];
+        sum = dfma(v0, x0, sum);
+        sum = dfma(v1, x1, sum);
+        sum = dfma(v2, x2, sum);
+        sum = dfma(v3, x3, sum);
+      }
+      for(; k < len; ++k)
+        sum = dfma(v[k * 64], x[sell_col<DELTA>(c[k * 64], row)], sum);
+      epi.row(row, sum, acc0, acc1);
+    }
+  }
+  grid_reduce_finish(gr, acc0, acc1);
+}
+
+// Runs Epi over the rows of a descriptor (CSR or SELL-64).
+template<typename DT, class Epi, class M>
+int launch_mat(hipStream_t s, const M* A, const DT* x, const Epi& epi, const GridReduce& gr, const char* name)
+{
+  if(A->rows == 0) return FEAT_OK;
+  if(A->format == FEAT_FMT_CSR)
+  {
+    if(A->index == FEAT_IDX_U64)
+      return launch_rows<DT, uint64_t>(s, A->val, (const uint64_t*)A->col, (const uint64_t*)A->ptr, x, A->rows, epi, gr, name);
+    if(A->index == FEAT_IDX_U32)
+      return launch_rows<DT, uint32_t>(s, A->val, (const uint32_t*)A->col, (const uint32_t*)A->ptr, x, A->rows, epi, gr, name);
+  }
+  else if(A->format == FEAT_FMT_SELL)
+  {
+    const uint64_t nch = (A->rows + kThreads - 1) / kThreads;
+    const int grid = int(nch < uint64_t(kMaxGrid) ? nch : uint64_t(kMaxGrid));
+    const uint64_t* soff = (const uint64_t*)A->ptr;
+    if(A->index == FEAT_IDX_U32)
+    {
+      hipLaunchKernelGGL((k_sell_rows<DT, uint32_t, false, Epi>), dim3(grid), dim3(kThreads), 0, s, A->val,
+                         (const uint32_t*)A->col, soff, A->rowlen, x, A->rows, epi, gr);
+      return check_launch(name);
+    }
+    if(A->index == FEAT_IDX_D16)
+    {
+      hipLaunchKernelGGL((k_sell_rows<DT, int16_t, true, Epi>), dim3(grid), dim3(kThreads), 0, s, A->val,
+                         (const int16_t*)A->col, soff, A->rowlen, x, A->rows, epi, gr);
+      return check_launch(name);
+    }
+  }
+  return fail(FEAT_ERR_UNSUPPORTED, "matrix descriptor: unknown format/index combination");
+}
+
+// SELL-64 plan construction from a device CSR (u64).
+__global__ void k_sell_scan_rows(const uint64_t* rp, const uint64_t* col, uint64_t rows, uint16_t* rlen,
+                                 uint32_t* slmax, uint32_t* stats /* [0] max |col-row| (saturated), [1] long-row flag */)
+{
+  for(uint64_t r = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; r < rows; r += uint64_t(gridDim.x) * blockDim.x)
+  {
+    const uint64_t b = rp[r], e = rp[r + 1];
+    const uint64_t len = e - b;
+    uint64_t md = 0;
+    for(uint64_t i = b; i < e; ++i)
+    {
+      const uint64_t d = col[i] > r ? col[i] - r : r - col[i];
+      md = d > md ? d : md;
+    }
+    if(len > 65535) atomicMax(&stats[1], 1u);
+    rlen[r] = uint16_t(len > 65535 ? 65535 : len);
+    atomicMax(&slmax[r >> 6], uint32_t(len > 0xFFFFFFFFull ? 0xFFFFFFFFull : len));
+    atomicMax(&stats[0], uint32_t(md > 0xFFFFFFFFull ? 0xFFFFFFFFull : md));
+  }
+}
+
+template<typename DT, typename CT, bool DELTA>
+__global__ void k_sell_fill(const double* val, const uint64_t* col, const uint64_t* rp, uint64_t rows,
+                            const uint64_t* soff, DT* sval, CT* scol)
+{
+  for(uint64_t r = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; r < rows; r += uint64_t(gridDim.x) * blockDim.x)
+  {
+    const uint64_t s = r >> 6, lane = r & 63;
+    const uint64_t off = soff[s], smax = (soff[s + 1] - soff[s]) / 64;
+    const uint64_t b = rp[r], len = rp[r + 1] - b;
+    for(uint64_t k = 0; k < smax; ++k)
+    {
+      const uint64_t p = off + k * 64 + lane;
+      if(k < len)
+      {
+        sval[p] = DT(val[b + k]);
+        scol[p] = DELTA ? CT(std::int64_t(col[b + k]) - std::int64_t(r)) : CT(col[b + k]);
+      }
+      else
+      {
+        sval[p] = DT(0);
+        scol[p] = DELTA ? CT(0) : CT(r);
+      }
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1088,68 +1229,170 @@ int feat_mean_filter_f64(feat_handle_t h, double* v, const double* w_dot, const 
 }
 
 // ---------------------------------------------------------------------------
-// fused solver kernels
+// matrix descriptors and SELL-64 plans
 // ---------------------------------------------------------------------------
+int feat_matrix_from_csr_f64(const feat_csr_f64_t* A, feat_matrix_f64_t* out)
+{
+  if(!A || !out) return fail(FEAT_ERR_INVALID, "feat_matrix_from_csr_f64: null argument");
+  *out = feat_matrix_f64_t{FEAT_FMT_CSR, FEAT_IDX_U64, A->rows, A->cols, A->nnz, A->val, A->col, A->rowptr, nullptr};
+  return FEAT_OK;
+}
+
 } // extern "C"
 
 namespace {
-template<typename DT, typename IT, typename M>
-bool csr_ok(const M* A)
+template<typename DT, typename M>
+int sell_create_impl(feat_handle_t h, const feat_csr_f64_t* A, M* out, const char* name)
 {
-  return A && A->rowptr && (A->nnz == 0 || (A->val && A->col));
+  if(!h || !A || !out || !A->rowptr || (A->nnz && (!A->val || !A->col))) return fail(FEAT_ERR_INVALID, name);
+  std::memset(out, 0, sizeof(M));
+  const uint64_t n = A->rows, ns = (n + 63) / 64;
+  uint16_t* rlen = nullptr;
+  uint32_t* slmax = nullptr;
+  uint32_t* stats = nullptr;
+  uint64_t* soff = nullptr;
+  DT* sval = nullptr;
+  void* scol = nullptr;
+  auto cleanup = [&]() {
+    if(rlen) (void)hipFree(rlen);
+    if(slmax) (void)hipFree(slmax);
+    if(stats) (void)hipFree(stats);
+    if(soff) (void)hipFree(soff);
+    if(sval) (void)hipFree(sval);
+    if(scol) (void)hipFree(scol);
+  };
+  int rc = check_hip(hipMalloc(&rlen, sizeof(uint16_t) * (n ? n : 1)), "sell rlen");
+  if(!rc) rc = check_hip(hipMalloc(&slmax, sizeof(uint32_t) * (ns ? ns : 1)), "sell slmax");
+  if(!rc) rc = check_hip(hipMalloc(&stats, sizeof(uint32_t) * 2), "sell stats");
+  if(!rc) rc = check_hip(hipMalloc(&soff, sizeof(uint64_t) * (ns + 1)), "sell soff");
+  if(!rc) rc = check_hip(hipMemsetAsync(slmax, 0, sizeof(uint32_t) * (ns ? ns : 1), h->stream), "memset");
+  if(!rc) rc = check_hip(hipMemsetAsync(stats, 0, sizeof(uint32_t) * 2, h->stream), "memset");
+  if(rc) { cleanup(); return rc; }
+  if(n)
+    hipLaunchKernelGGL(k_sell_scan_rows, dim3(grid_for(n)), dim3(kThreads), 0, h->stream, A->rowptr, A->col, n, rlen, slmax, stats);
+  std::vector<uint32_t> hmax(ns);
+  uint32_t hst[2] = {0, 0};
+  rc = check_launch(name);
+  if(!rc && ns) rc = check_hip(hipMemcpyAsync(hmax.data(), slmax, sizeof(uint32_t) * ns, hipMemcpyDeviceToHost, h->stream), "copy");
+  if(!rc) rc = check_hip(hipMemcpyAsync(hst, stats, sizeof(hst), hipMemcpyDeviceToHost, h->stream), "copy");
+  if(!rc) rc = check_hip(hipStreamSynchronize(h->stream), "sync");
+  if(rc) { cleanup(); return rc; }
+  std::vector<uint64_t> hoff(ns + 1, 0);
+  for(uint64_t i = 0; i < ns; ++i) hoff[i + 1] = hoff[i] + uint64_t(hmax[i]) * 64;
+  const uint64_t total = hoff[ns];
+  if(hst[1] || total > 2 * A->nnz + 64 * ns)
+  {
+    cleanup();
+    return fail(FEAT_ERR_UNSUPPORTED, "SELL-64 plan: row too long or padding too large; keep CSR");
+  }
+  const bool d16 = hst[0] <= 32767u;
+  if(!d16 && A->cols > 0xFFFFFFFFull)
+  {
+    cleanup();
+    return fail(FEAT_ERR_UNSUPPORTED, "SELL-64 plan: columns exceed 32 bits");
+  }
+  const size_t cb = d16 ? sizeof(int16_t) : sizeof(uint32_t);
+  rc = check_hip(hipMalloc(&sval, sizeof(DT) * (total ? total : 1)), "sell val");
+  if(!rc) rc = check_hip(hipMalloc(&scol, cb * (total ? total : 1)), "sell col");
+  if(!rc) rc = check_hip(hipMemcpyAsync(soff, hoff.data(), sizeof(uint64_t) * (ns + 1), hipMemcpyHostToDevice, h->stream), "copy");
+  if(rc) { cleanup(); return rc; }
+  if(n)
+  {
+    if(d16)
+      hipLaunchKernelGGL((k_sell_fill<DT, int16_t, true>), dim3(grid_for(n)), dim3(kThreads), 0, h->stream, A->val, A->col,
+                         A->rowptr, n, soff, sval, (int16_t*)scol);
+    else
+      hipLaunchKernelGGL((k_sell_fill<DT, uint32_t, false>), dim3(grid_for(n)), dim3(kThreads), 0, h->stream, A->val, A->col,
+                         A->rowptr, n, soff, sval, (uint32_t*)scol);
+  }
+  rc = check_launch(name);
+  if(!rc) rc = check_hip(hipStreamSynchronize(h->stream), "sync");
+  (void)hipFree(slmax);
+  (void)hipFree(stats);
+  slmax = nullptr;
+  stats = nullptr;
+  if(rc) { cleanup(); return rc; }
+  out->format = FEAT_FMT_SELL;
+  out->index = d16 ? FEAT_IDX_D16 : FEAT_IDX_U32;
+  out->rows = n;
+  out->cols = A->cols;
+  out->nnz = total;
+  out->val = sval;
+  out->col = scol;
+  out->ptr = soff;
+  out->rowlen = rlen;
+  return FEAT_OK;
 }
 
-template<typename DT, typename IT, typename M>
+template<typename M>
+int sell_destroy_impl(M* m)
+{
+  if(!m) return FEAT_OK;
+  if(m->format != FEAT_FMT_SELL) return fail(FEAT_ERR_INVALID, "feat_sell_destroy: not a SELL plan");
+  if(m->val) (void)hipFree(const_cast<void*>(static_cast<const void*>(m->val)));
+  if(m->col) (void)hipFree(const_cast<void*>(m->col));
+  if(m->ptr) (void)hipFree(const_cast<void*>(m->ptr));
+  if(m->rowlen) (void)hipFree(const_cast<uint16_t*>(m->rowlen));
+  std::memset(m, 0, sizeof(M));
+  return FEAT_OK;
+}
+
+template<typename M>
+bool mat_ok(const M* A)
+{
+  return A && A->ptr && (A->nnz == 0 || (A->val && A->col)) && (A->format != FEAT_FMT_SELL || A->rowlen);
+}
+
+template<typename DT, typename M>
 int defect_impl(feat_handle_t h, const M* A, DT* d, const DT* rhs, const DT* x, const uint32_t* mask,
                 double* nrm2, const feat_reduce_ws_t* ws, const char* name)
 {
-  if(!h || !csr_ok<DT, IT>(A) || (A->rows && (!d || !rhs || !x))) return fail(FEAT_ERR_INVALID, name);
+  if(!h || !mat_ok(A) || (A->rows && (!d || !rhs || !x))) return fail(FEAT_ERR_INVALID, name);
   if(d == x) return fail(FEAT_ERR_ALIAS, name);
   if(need_ws(ws, nrm2)) return fail(FEAT_ERR_INVALID, "reduction workspace missing");
   EpiDefect<DT> e{d, rhs, mask, nrm2 != nullptr};
-  return launch_rows<DT, IT>(h->stream, A->val, A->col, A->rowptr, x, A->rows, e, ws_reduce(ws, nrm2, nullptr), name);
+  return launch_mat<DT>(h->stream, A, x, e, ws_reduce(ws, nrm2, nullptr), name);
 }
 
-template<typename DT, typename IT, typename M>
+template<typename DT, typename M>
 int masked_impl(feat_handle_t h, const M* A, DT* r, const DT* x, const uint32_t* mask, double* nrm2,
                 const DT* dotw, double* dot, const feat_reduce_ws_t* ws, const char* name)
 {
-  if(!h || !csr_ok<DT, IT>(A) || (A->rows && !r) || (A->cols && !x) || (dot && !dotw)) return fail(FEAT_ERR_INVALID, name);
+  if(!h || !mat_ok(A) || (A->rows && !r) || (A->cols && !x) || (dot && !dotw)) return fail(FEAT_ERR_INVALID, name);
   if(r == x) return fail(FEAT_ERR_ALIAS, name);
   if(need_ws(ws, nrm2, dot)) return fail(FEAT_ERR_INVALID, "reduction workspace missing");
   EpiMasked<DT> e{r, mask, dotw, nrm2 != nullptr, dot != nullptr};
-  return launch_rows<DT, IT>(h->stream, A->val, A->col, A->rowptr, x, A->rows, e, ws_reduce(ws, nrm2, dot), name);
+  return launch_mat<DT>(h->stream, A, x, e, ws_reduce(ws, nrm2, dot), name);
 }
 
-template<typename DT, typename IT, typename M>
+template<typename DT, typename M>
 int prol_impl(feat_handle_t h, const M* P, DT* sol, const DT* xc, const uint32_t* mask, const char* name)
 {
-  if(!h || !csr_ok<DT, IT>(P) || (P->rows && !sol) || (P->cols && !xc)) return fail(FEAT_ERR_INVALID, name);
+  if(!h || !mat_ok(P) || (P->rows && !sol) || (P->cols && !xc)) return fail(FEAT_ERR_INVALID, name);
   if(sol == xc) return fail(FEAT_ERR_ALIAS, name);
   EpiProlAdd<DT> e{sol, mask};
   GridReduce g{nullptr, nullptr, nullptr, nullptr};
-  return launch_rows<DT, IT>(h->stream, P->val, P->col, P->rowptr, xc, P->rows, e, g, name);
+  return launch_mat<DT>(h->stream, P, xc, e, g, name);
 }
 
-// Buffer holding x_k: the final iterate lands in `x` (pre/plain modes).
-template<typename DT>
-int smooth_impl_generic(feat_handle_t h, const DT* val, const void* colv, const void* rpv, bool u32,
-                        uint64_t n, const DT* inv, const uint32_t* mask, int steps, DT omega,
-                        const DT* rhs, const double* rhs_nrm2, DT* x, DT* tmp, DT* d_out, DT* acc,
-                        const DT* dotw, double* dot, const feat_reduce_ws_t* ws, const char* name)
+// iterate k (1-based) of the smoother lives in buf[(steps - k) & 1], so the last
+// stored iterate lands in x
+template<typename DT, typename M>
+int smooth_impl(feat_handle_t h, const M* A, const DT* inv, const uint32_t* mask, int steps, DT omega,
+                const DT* rhs, const double* rhs_nrm2, DT* x, DT* tmp, DT* d_out, DT* acc, const DT* dotw,
+                double* dot, const feat_reduce_ws_t* ws, const char* name)
 {
-  if(!h || steps < 1 || (n && (!inv || !rhs || !tmp || !x))) return fail(FEAT_ERR_INVALID, name);
+  if(!h || !mat_ok(A) || A->rows != A->cols || steps < 1) return fail(FEAT_ERR_INVALID, name);
+  const uint64_t n = A->rows;
+  if(n && (!inv || !rhs || !tmp || !x)) return fail(FEAT_ERR_INVALID, name);
   if(x == tmp || (acc && (acc == x || acc == tmp)) || (d_out && (d_out == x || d_out == tmp)))
     return fail(FEAT_ERR_ALIAS, name);
   if(need_ws(ws, dot)) return fail(FEAT_ERR_INVALID, "reduction workspace missing");
   if(d_out && acc) return fail(FEAT_ERR_INVALID, "rj_smooth: d_out and acc are exclusive");
   if(!n) return FEAT_OK;
   DT* buf[2] = {x, tmp};
-  // number of SpMV steps: pre (d_out): steps; acc: steps-1; plain: steps-1
-  const int nsp = d_out ? steps : steps - 1;
-  // x_1 goes to the buffer such that the last stored iterate x_{steps} lands in x:
-  // iterate k (1-based) lives in buf[(steps - k) & 1]
   auto B = [&](int k) { return buf[(steps - k) & 1]; };
+  const int nsp = d_out ? steps : steps - 1;
   GridReduce none{nullptr, nullptr, nullptr, nullptr};
   const bool acc_first = (acc != nullptr && steps == 1);
   {
@@ -1188,45 +1431,87 @@ int smooth_impl_generic(feat_handle_t h, const DT* val, const void* colv, const 
       e.mode = 0;
       e.out = B(k + 1);
     }
-    int rc;
-    if(u32)
-      rc = launch_rows<DT, uint32_t>(h->stream, val, (const uint32_t*)colv, (const uint32_t*)rpv, e.xk, n, e, g, name);
-    else
-      rc = launch_rows<DT, uint64_t>(h->stream, val, (const uint64_t*)colv, (const uint64_t*)rpv, e.xk, n, e, g, name);
+    int rc = launch_mat<DT>(h->stream, A, e.xk, e, g, name);
     if(rc) return rc;
   }
   return FEAT_OK;
+}
+
+__global__ __launch_bounds__(kThreads) void k_convert_dot(double* z, const float* z32, const double* w, uint64_t n, GridReduce gr)
+{
+  double acc = 0.0;
+  for(uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
+  {
+    const double v = double(z32[i]);
+    z[i] = v;
+    if(w) acc = acc + v * w[i];
+  }
+  grid_reduce_finish(gr, acc, 0.0);
+}
+
+__global__ void k_dense_gemv_mixed(int n, const double* ainv, const float* b, float* x, const uint32_t* mask)
+{
+  for(int r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x)
+  {
+    double s = 0.0;
+    for(int k = 0; k < n; ++k) s = __fma_rn(ainv[size_t(r) * n + k], double(b[k]), s);
+    if(is_masked(mask, uint64_t(r))) s = 0.0;
+    x[r] = float(s);
+  }
 }
 } // namespace
 
 extern "C" {
 
-int feat_csr_defect_f64(feat_handle_t h, const feat_csr_f64_t* A, double* d, const double* rhs, const double* x,
-                        const uint32_t* mask, double* nrm2, const feat_reduce_ws_t* ws)
+int feat_sell_create_f64(feat_handle_t h, const feat_csr_f64_t* A, feat_matrix_f64_t* out)
 {
-  return defect_impl<double, uint64_t>(h, A, d, rhs, x, mask, nrm2, ws, "feat_csr_defect_f64");
+  return sell_create_impl<double>(h, A, out, "feat_sell_create_f64");
 }
 
-int feat_csr_apply_masked_f64(feat_handle_t h, const feat_csr_f64_t* A, double* r, const double* x,
-                              const uint32_t* mask, double* nrm2, const double* dotw, double* dot,
-                              const feat_reduce_ws_t* ws)
+int feat_sell_create_f32(feat_handle_t h, const feat_csr_f64_t* A, feat_matrix_f32_t* out)
 {
-  return masked_impl<double, uint64_t>(h, A, r, x, mask, nrm2, dotw, dot, ws, "feat_csr_apply_masked_f64");
+  return sell_create_impl<float>(h, A, out, "feat_sell_create_f32");
 }
 
-int feat_csr_prol_add_f64(feat_handle_t h, const feat_csr_f64_t* P, double* sol, const double* xc, const uint32_t* mask)
+int feat_sell_destroy_f64(feat_matrix_f64_t* m) { return sell_destroy_impl(m); }
+int feat_sell_destroy_f32(feat_matrix_f32_t* m) { return sell_destroy_impl(m); }
+
+int feat_matrix_apply_f64(feat_handle_t h, const feat_matrix_f64_t* A, double* r, double a, const double* x, double b,
+                          const double* y)
 {
-  return prol_impl<double, uint64_t>(h, P, sol, xc, mask, "feat_csr_prol_add_f64");
+  if(!h || !mat_ok(A)) return fail(FEAT_ERR_INVALID, "feat_matrix_apply_f64: bad argument");
+  const bool bzero = std::fabs(b) < DBL_EPSILON;
+  if((A->rows && !r) || (A->cols && !x) || (!bzero && A->rows && !y)) return fail(FEAT_ERR_INVALID, "feat_matrix_apply_f64: null vector");
+  if(r == x && A->rows) return fail(FEAT_ERR_ALIAS, "feat_matrix_apply_f64: r and x must not alias");
+  EpiApply<double> e{r, y, a, b, bzero};
+  GridReduce g{nullptr, nullptr, nullptr, nullptr};
+  return launch_mat<double>(h->stream, A, x, e, g, "feat_matrix_apply_f64");
 }
 
-int feat_rj_smooth_f64(feat_handle_t h, const feat_csr_f64_t* A, const double* inv_diag, const uint32_t* mask,
+int feat_defect_f64(feat_handle_t h, const feat_matrix_f64_t* A, double* d, const double* rhs, const double* x,
+                    const uint32_t* mask, double* nrm2, const feat_reduce_ws_t* ws)
+{
+  return defect_impl<double>(h, A, d, rhs, x, mask, nrm2, ws, "feat_defect_f64");
+}
+
+int feat_apply_masked_f64(feat_handle_t h, const feat_matrix_f64_t* A, double* r, const double* x,
+                          const uint32_t* mask, double* nrm2, const double* dotw, double* dot,
+                          const feat_reduce_ws_t* ws)
+{
+  return masked_impl<double>(h, A, r, x, mask, nrm2, dotw, dot, ws, "feat_apply_masked_f64");
+}
+
+int feat_prol_add_f64(feat_handle_t h, const feat_matrix_f64_t* P, double* sol, const double* xc, const uint32_t* mask)
+{
+  return prol_impl<double>(h, P, sol, xc, mask, "feat_prol_add_f64");
+}
+
+int feat_rj_smooth_f64(feat_handle_t h, const feat_matrix_f64_t* A, const double* inv_diag, const uint32_t* mask,
                        int steps, double omega, const double* rhs, const double* rhs_nrm2, double* x, double* tmp,
                        double* d_out, double* acc, const double* dotw, double* dot, const feat_reduce_ws_t* ws)
 {
-  if(!A || !csr_ok<double, uint64_t>(A)) return fail(FEAT_ERR_INVALID, "feat_rj_smooth_f64: bad matrix");
-  if(A->rows != A->cols) return fail(FEAT_ERR_INVALID, "feat_rj_smooth_f64: matrix not square");
-  return smooth_impl_generic<double>(h, A->val, A->col, A->rowptr, false, A->rows, inv_diag, mask, steps, omega, rhs,
-                                     rhs_nrm2, x, tmp, d_out, acc, dotw, dot, ws, "feat_rj_smooth_f64");
+  return smooth_impl<double>(h, A, inv_diag, mask, steps, omega, rhs, rhs_nrm2, x, tmp, d_out, acc, dotw, dot, ws,
+                             "feat_rj_smooth_f64");
 }
 
 int feat_dense_gemv_f64(feat_handle_t h, int n, const double* ainv, const double* b, double* x, const uint32_t* mask)
@@ -1265,31 +1550,29 @@ int feat_read_scalar_f64(feat_handle_t h, const double* dev, double* host)
 }
 
 // ---------------- FP32 inner solver kernels ----------------
-int feat_csr_apply_masked_f32(feat_handle_t h, const feat_csr_f32_t* A, float* r, const float* x,
-                              const uint32_t* mask, double* nrm2, const feat_reduce_ws_t* ws)
+int feat_apply_masked_f32(feat_handle_t h, const feat_matrix_f32_t* A, float* r, const float* x,
+                          const uint32_t* mask, double* nrm2, const feat_reduce_ws_t* ws)
 {
-  return masked_impl<float, uint32_t>(h, A, r, x, mask, nrm2, nullptr, nullptr, ws, "feat_csr_apply_masked_f32");
+  return masked_impl<float>(h, A, r, x, mask, nrm2, nullptr, nullptr, ws, "feat_apply_masked_f32");
 }
 
-int feat_csr_defect_f32(feat_handle_t h, const feat_csr_f32_t* A, float* d, const float* rhs, const float* x,
-                        const uint32_t* mask, double* nrm2, const feat_reduce_ws_t* ws)
+int feat_defect_f32(feat_handle_t h, const feat_matrix_f32_t* A, float* d, const float* rhs, const float* x,
+                    const uint32_t* mask, double* nrm2, const feat_reduce_ws_t* ws)
 {
-  return defect_impl<float, uint32_t>(h, A, d, rhs, x, mask, nrm2, ws, "feat_csr_defect_f32");
+  return defect_impl<float>(h, A, d, rhs, x, mask, nrm2, ws, "feat_defect_f32");
 }
 
-int feat_csr_prol_add_f32(feat_handle_t h, const feat_csr_f32_t* P, float* sol, const float* xc, const uint32_t* mask)
+int feat_prol_add_f32(feat_handle_t h, const feat_matrix_f32_t* P, float* sol, const float* xc, const uint32_t* mask)
 {
-  return prol_impl<float, uint32_t>(h, P, sol, xc, mask, "feat_csr_prol_add_f32");
+  return prol_impl<float>(h, P, sol, xc, mask, "feat_prol_add_f32");
 }
 
-int feat_rj_smooth_f32(feat_handle_t h, const feat_csr_f32_t* A, const float* inv_diag, const uint32_t* mask,
+int feat_rj_smooth_f32(feat_handle_t h, const feat_matrix_f32_t* A, const float* inv_diag, const uint32_t* mask,
                        int steps, float omega, const float* rhs, const double* rhs_nrm2, float* x, float* tmp,
                        float* d_out, float* acc, const feat_reduce_ws_t* ws)
 {
-  if(!A || !csr_ok<float, uint32_t>(A)) return fail(FEAT_ERR_INVALID, "feat_rj_smooth_f32: bad matrix");
-  if(A->rows != A->cols) return fail(FEAT_ERR_INVALID, "feat_rj_smooth_f32: matrix not square");
-  return smooth_impl_generic<float>(h, A->val, A->col, A->rowptr, true, A->rows, inv_diag, mask, steps, omega, rhs,
-                                    rhs_nrm2, x, tmp, d_out, acc, nullptr, nullptr, ws, "feat_rj_smooth_f32");
+  return smooth_impl<float>(h, A, inv_diag, mask, steps, omega, rhs, rhs_nrm2, x, tmp, d_out, acc, nullptr, nullptr,
+                            ws, "feat_rj_smooth_f32");
 }
 
 int feat_dense_gemv_f32(feat_handle_t h, int n, const float* ainv, const float* b, float* x, const uint32_t* mask)
@@ -1299,6 +1582,24 @@ int feat_dense_gemv_f32(feat_handle_t h, int n, const float* ainv, const float* 
   if(!n) return FEAT_OK;
   hipLaunchKernelGGL(k_dense_gemv<float>, dim3((n + kThreads - 1) / kThreads), dim3(kThreads), 0, h->stream, n, ainv, b, x, mask);
   return check_launch("feat_dense_gemv_f32");
+}
+
+int feat_convert_dot_f32_f64(feat_handle_t h, double* z, const float* z32, const double* w, uint64_t n, double* dot,
+                             const feat_reduce_ws_t* ws)
+{
+  if(!h || (n && (!z || !z32)) || (dot && !w)) return fail(FEAT_ERR_INVALID, "feat_convert_dot_f32_f64: null argument");
+  if(need_ws(ws, dot)) return fail(FEAT_ERR_INVALID, "reduction workspace missing");
+  hipLaunchKernelGGL(k_convert_dot, dim3(grid_for(n, 4)), dim3(kThreads), 0, h->stream, z, z32, dot ? w : nullptr, n,
+                     ws_reduce(ws, dot, nullptr));
+  return check_launch("feat_convert_dot_f32_f64");
+}
+
+int feat_dense_gemv_mixed(feat_handle_t h, int n, const double* ainv, const float* b, float* x, const uint32_t* mask)
+{
+  if(!h || n < 0 || (n && (!ainv || !b || !x))) return fail(FEAT_ERR_INVALID, "feat_dense_gemv_mixed: bad argument");
+  if(!n) return FEAT_OK;
+  hipLaunchKernelGGL(k_dense_gemv_mixed, dim3((n + kThreads - 1) / kThreads), dim3(kThreads), 0, h->stream, n, ainv, b, x, mask);
+  return check_launch("feat_dense_gemv_mixed");
 }
 
 } // extern "C"

@@ -120,6 +120,16 @@ int feat_rhs_random(std::uint64_t seed, std::uint64_t n, double a, double b, dou
 static void build_solver(feat_mgpcg_s* s, const feat_mgpcg_cfg_t* cfg)
 {
   const int nl = int(s->levels.size());
+  if(cfg->csr_only)
+    for(auto& L : s->levels)
+    {
+      L->A.set_sell(false);
+      if(!L->T.get_mat_prol().empty())
+      {
+        L->T.get_mat_prol().set_sell(false);
+        L->T.get_mat_rest().set_sell(false);
+      }
+    }
   s->hier = std::make_shared<Hier>();
   for(int i = 0; i < nl; ++i)
   {

@@ -171,41 +171,79 @@ typedef struct {
   uint32_t* counter;
 } feat_reduce_ws_t;
 
-typedef struct {            /* a CSR matrix on the device */
+typedef struct {            /* a CSR matrix on the device (u64 indices) */
   uint64_t rows, cols, nnz;
   const double* val;
   const uint64_t* col;
   const uint64_t* rowptr;
 } feat_csr_f64_t;
 
-typedef struct {            /* FP32 values with 32-bit indices (mixed precision) */
+/* Matrix descriptor of the fused kernels: the caller's CSR, or a SELL-64
+ * execution plan of it.  SELL-64 stores each slice of 64 consecutive rows
+ * column-major (entry k of row 64s+l at slice_off[s] + 64k + l), so one
+ * wavefront reads 64 consecutive values per load; every row keeps its CSR
+ * entry order, so each row sum is bit-identical to the CSR loop.  Column
+ * indices are u32 (absolute) or i16 (col - row) when the bandwidth allows. */
+#define FEAT_FMT_CSR 0
+#define FEAT_FMT_SELL 1
+#define FEAT_IDX_U64 0   /* CSR: u64 col/rowptr                              */
+#define FEAT_IDX_U32 1   /* CSR: u32 col/rowptr; SELL: u32 absolute columns  */
+#define FEAT_IDX_D16 2   /* SELL: int16 column minus row                      */
+typedef struct {
+  int format, index;
+  uint64_t rows, cols;
+  uint64_t nnz;              /* CSR nonzeros; SELL: stored (padded) entries   */
+  const double* val;
+  const void* col;
+  const void* ptr;           /* CSR row pointer / SELL slice offsets (u64)    */
+  const uint16_t* rowlen;    /* SELL row lengths                             */
+} feat_matrix_f64_t;
+
+typedef struct {             /* FP32 values (mixed-precision inner solver)    */
+  int format, index;
   uint64_t rows, cols, nnz;
   const float* val;
-  const uint32_t* col;
-  const uint32_t* rowptr;
-} feat_csr_f32_t;
+  const void* col;
+  const void* ptr;
+  const uint16_t* rowlen;
+} feat_matrix_f32_t;
+
+/* A CSR descriptor (no copy) of the caller's arrays. */
+int feat_matrix_from_csr_f64(const feat_csr_f64_t* A, feat_matrix_f64_t* out);
+/* Builds the SELL-64 plan of a CSR matrix on the device (setup time; allocates).
+ * Fails with FEAT_ERR_UNSUPPORTED when a row has more than 65535 entries or the
+ * slice padding would exceed the CSR size; the caller then keeps the CSR. */
+int feat_sell_create_f64(feat_handle_t h, const feat_csr_f64_t* A, feat_matrix_f64_t* out);
+/* Same plan with values rounded to FP32 (ConvertPrecond, convert_precond.hpp:100-119). */
+int feat_sell_create_f32(feat_handle_t h, const feat_csr_f64_t* A, feat_matrix_f32_t* out);
+int feat_sell_destroy_f64(feat_matrix_f64_t* m);
+int feat_sell_destroy_f32(feat_matrix_f32_t* m);
+
+/* r = a*A*x + b*y on a descriptor (same semantics as feat_csr_apply_f64_u64). */
+int feat_matrix_apply_f64(feat_handle_t h, const feat_matrix_f64_t* A, double* r, double a,
+                          const double* x, double b, const double* y);
 
 /* Defect d = rhs - A*x (SparseMatrixCSR::apply(r,x,y,-1), sparse_matrix_csr.hpp:1807-1843)
  * followed by filter_def through `mask` (may be NULL) and, when nrm2 != NULL,
  * the squared norm of the filtered defect into the device scalar *nrm2. */
-int feat_csr_defect_f64(feat_handle_t h, const feat_csr_f64_t* A, double* d, const double* rhs,
-                        const double* x, const uint32_t* mask, double* nrm2,
-                        const feat_reduce_ws_t* ws);
+int feat_defect_f64(feat_handle_t h, const feat_matrix_f64_t* A, double* d, const double* rhs,
+                    const double* x, const uint32_t* mask, double* nrm2,
+                    const feat_reduce_ws_t* ws);
 /* r = A*x (b = 0) with filter mask and optional squared norm (Transfer::rest + filter_def,
  * multigrid.hpp:1801-1805).  `dotw` != NULL adds the dot of the result with dotw into *dot. */
-int feat_csr_apply_masked_f64(feat_handle_t h, const feat_csr_f64_t* A, double* r,
-                              const double* x, const uint32_t* mask, double* nrm2,
-                              const double* dotw, double* dot, const feat_reduce_ws_t* ws);
+int feat_apply_masked_f64(feat_handle_t h, const feat_matrix_f64_t* A, double* r,
+                          const double* x, const uint32_t* mask, double* nrm2,
+                          const double* dotw, double* dot, const feat_reduce_ws_t* ws);
 /* Prolongation + correction: sol += filter_cor(P*xc) (Transfer::prol, filter_cor, axpy;
  * multigrid.hpp:1885-1930). */
-int feat_csr_prol_add_f64(feat_handle_t h, const feat_csr_f64_t* P, double* sol,
-                          const double* xc, const uint32_t* mask);
+int feat_prol_add_f64(feat_handle_t h, const feat_matrix_f64_t* P, double* sol,
+                      const double* xc, const uint32_t* mask);
 
 /* Richardson(JacobiPrecond) smoother with min_iter == max_iter == steps
  * (richardson.hpp:160-226, jacobi_precond.hpp:159-164): starting from x = 0,
  * `steps` times { c = filter_cor(inv_diag∘d); x += omega*c; d = rhs - A*x; filter_def(d) }.
  * The smoother is skipped (x = 0, d = rhs) when sqrt(*rhs_nrm2) is not finite or
- * <= eps^2 (iterative.hpp:748-770); rhs_nrm2 is a device scalar.
+ * <= eps^2 (iterative.hpp:748-770); rhs_nrm2 is a device scalar (NULL: never skip).
  *   x        out: the smoother result (n); scratch in acc mode.   tmp: scratch (n)
  *   d_out    optional: the last filtered defect (n) -- equals the defect the multigrid
  *            recomputes after pre-smoothing (multigrid.hpp:1754-1774)
@@ -213,7 +251,7 @@ int feat_csr_prol_add_f64(feat_handle_t h, const feat_csr_f64_t* P, double* sol,
  *            (post-smoothing `sol += cor`, multigrid.hpp:1977); the last defect is
  *            then not formed (it is unused by the reference)
  *   dotw/dot optional: dot(acc_after, dotw) into the device scalar *dot. */
-int feat_rj_smooth_f64(feat_handle_t h, const feat_csr_f64_t* A, const double* inv_diag,
+int feat_rj_smooth_f64(feat_handle_t h, const feat_matrix_f64_t* A, const double* inv_diag,
                        const uint32_t* mask, int steps, double omega, const double* rhs,
                        const double* rhs_nrm2, double* x, double* tmp, double* d_out,
                        double* acc, const double* dotw, double* dot,
@@ -236,21 +274,28 @@ int feat_pcg_direction_f64(feat_handle_t h, double* p, const double* z, const do
 /* Reads a device scalar into host memory after synchronising the stream. */
 int feat_read_scalar_f64(feat_handle_t h, const double* dev, double* host);
 
-/* ---------------- mixed precision (solver/convert_precond.hpp:100-119) ---------------- */
-int feat_csr_apply_masked_f32(feat_handle_t h, const feat_csr_f32_t* A, float* r,
-                              const float* x, const uint32_t* mask, double* nrm2,
-                              const feat_reduce_ws_t* ws);
-int feat_csr_defect_f32(feat_handle_t h, const feat_csr_f32_t* A, float* d, const float* rhs,
-                        const float* x, const uint32_t* mask, double* nrm2,
-                        const feat_reduce_ws_t* ws);
-int feat_csr_prol_add_f32(feat_handle_t h, const feat_csr_f32_t* P, float* sol,
-                          const float* xc, const uint32_t* mask);
-int feat_rj_smooth_f32(feat_handle_t h, const feat_csr_f32_t* A, const float* inv_diag,
+/* ---------------- mixed precision (solver/convert_precond.hpp:100-119) ----------------
+ * The same fused kernels on FP32 values (row sums with fmaf in CSR order; norms and
+ * dots accumulated in FP64). */
+int feat_apply_masked_f32(feat_handle_t h, const feat_matrix_f32_t* A, float* r, const float* x,
+                          const uint32_t* mask, double* nrm2, const feat_reduce_ws_t* ws);
+int feat_defect_f32(feat_handle_t h, const feat_matrix_f32_t* A, float* d, const float* rhs,
+                    const float* x, const uint32_t* mask, double* nrm2,
+                    const feat_reduce_ws_t* ws);
+int feat_prol_add_f32(feat_handle_t h, const feat_matrix_f32_t* P, float* sol, const float* xc,
+                      const uint32_t* mask);
+int feat_rj_smooth_f32(feat_handle_t h, const feat_matrix_f32_t* A, const float* inv_diag,
                        const uint32_t* mask, int steps, float omega, const float* rhs,
                        const double* rhs_nrm2, float* x, float* tmp, float* d_out, float* acc,
                        const feat_reduce_ws_t* ws);
 int feat_dense_gemv_f32(feat_handle_t h, int n, const float* ainv, const float* b, float* x,
                         const uint32_t* mask);
+/* ConvertPrecond's way back: z = double(z32) and *dot = z . w in FP64. */
+int feat_convert_dot_f32_f64(feat_handle_t h, double* z, const float* z32, const double* w,
+                             uint64_t n, double* dot, const feat_reduce_ws_t* ws);
+/* FP64 coarse solve inside the FP32 cycle: x = float(filter_cor(Ainv * double(b))). */
+int feat_dense_gemv_mixed(feat_handle_t h, int n, const double* ainv, const float* b, float* x,
+                          const uint32_t* mask);
 
 #ifdef __cplusplus
 }

@@ -234,6 +234,9 @@ namespace feat3amd
       Index _rows = 0, _cols = 0, _nnz = 0;
       DeviceArray<DT_> _val;
       DeviceArray<IT_> _col, _rowptr;
+      mutable std::shared_ptr<feat_matrix_f64_t> _plan;
+      mutable std::shared_ptr<feat_matrix_f32_t> _plan32;
+      bool _use_sell = true;
 
     public:
       typedef DT_ DataType;
@@ -279,6 +282,48 @@ namespace feat3amd
         static_assert(std::is_same<DT_, double>::value && std::is_same<IT_, std::uint64_t>::value, "desc: f64/u64");
         return feat_csr_f64_t{_rows, _cols, _nnz, (const double*)val(), (const std::uint64_t*)col_ind(),
                               (const std::uint64_t*)row_ptr()};
+      }
+
+      /// Execution descriptor for the fused kernels: the SELL-64 plan of this
+      /// matrix (built on first use, kept until the matrix is destroyed), or the
+      /// CSR arrays themselves when no plan can be built.
+      const feat_matrix_f64_t& exec() const
+      {
+        if(!_plan)
+        {
+          auto p = std::shared_ptr<feat_matrix_f64_t>(new feat_matrix_f64_t(), [](feat_matrix_f64_t* m) {
+            if(m->format == FEAT_FMT_SELL) feat_sell_destroy_f64(m);
+            delete m;
+          });
+          const feat_csr_f64_t c = desc();
+          if(!_use_sell || feat_sell_create_f64(Runtime::handle(), &c, p.get()) != FEAT_OK)
+            check(feat_matrix_from_csr_f64(&c, p.get()), "exec");
+          _plan = p;
+        }
+        return *_plan;
+      }
+
+      /// FP32 SELL-64 plan (values rounded once) for mixed-precision inner solvers.
+      const feat_matrix_f32_t& exec32() const
+      {
+        if(!_plan32)
+        {
+          auto p = std::shared_ptr<feat_matrix_f32_t>(new feat_matrix_f32_t(), [](feat_matrix_f32_t* m) {
+            if(m->format == FEAT_FMT_SELL) feat_sell_destroy_f32(m);
+            delete m;
+          });
+          const feat_csr_f64_t c = desc();
+          check(feat_sell_create_f32(Runtime::handle(), &c, p.get()), "exec32");
+          _plan32 = p;
+        }
+        return *_plan32;
+      }
+
+      /// Selects CSR (false) or the SELL-64 plan (true, default) for exec().
+      void set_sell(bool on)
+      {
+        _use_sell = on;
+        _plan.reset();
       }
 
       /// r = A*x  (sparse_matrix_csr.hpp:1728-1756)
@@ -376,6 +421,8 @@ namespace feat3amd
       Transfer(Matrix_&& prol, Matrix_&& rest) : _prol(std::move(prol)), _rest(std::move(rest)) {}
       const Matrix_& get_mat_prol() const { return _prol; }
       const Matrix_& get_mat_rest() const { return _rest; }
+      Matrix_& get_mat_prol() { return _prol; }
+      Matrix_& get_mat_rest() { return _rest; }
       bool rest(const VectorType& fine, VectorType& coarse) const
       {
         _rest.apply(coarse, fine);

@@ -548,15 +548,15 @@ namespace feat3amd
           auto& l = _h->levels[i];
           auto* pre = static_cast<RichType*>(l.pre.get());
           auto* jac = pre->fusable_jacobi();
-          const feat_csr_f64_t A = l.matrix->desc();
+          const feat_matrix_f64_t& A = l.matrix->exec();
           check(feat_rj_smooth_f64(hd, &A, (const double*)jac->inv_diag().elements(), l.filter->mask(),
                                    int(pre->get_max_iter()), pre->omega(), rhs_of(i), nrm_of(i), sol_of(i),
                                    (double*)l.tmp.elements(), (double*)l.def.elements(), nullptr, nullptr, nullptr, ws),
                 "MultiGrid pre-smoothing");
-          const feat_csr_f64_t R = l.transfer->get_mat_rest().desc();
+          const feat_matrix_f64_t& R = l.transfer->get_mat_rest().exec();
           auto& lc = _h->levels[i + 1];
-          check(feat_csr_apply_masked_f64(hd, &R, (double*)lc.rhs.elements(), (const double*)l.def.elements(),
-                                          lc.filter->mask(), _fw.scalar(2 + i + 1), nullptr, nullptr, ws),
+          check(feat_apply_masked_f64(hd, &R, (double*)lc.rhs.elements(), (const double*)l.def.elements(),
+                                      lc.filter->mask(), _fw.scalar(2 + i + 1), nullptr, nullptr, ws),
                 "MultiGrid restriction");
         }
         // coarse solve (multigrid.hpp:1526-1589)
@@ -576,7 +576,7 @@ namespace feat3amd
           {
             // cor = filter_cor(inv_diag∘rhs): one step from zero with omega 1 is exact (fma(1,c,0) = c)
             auto* j = static_cast<JacType*>(l.coarse.get());
-            const feat_csr_f64_t A = l.matrix->desc();
+            const feat_matrix_f64_t& A = l.matrix->exec();
             check(feat_rj_smooth_f64(hd, &A, (const double*)j->inv_diag().elements(), l.filter->mask(), 1, 1.0,
                                      rhs_of(c), nullptr, xs, (double*)l.tmp.elements(), nullptr, nullptr, nullptr,
                                      nullptr, ws),
@@ -586,7 +586,7 @@ namespace feat3amd
           case Coarse::richardson:
           {
             auto* r = static_cast<RichType*>(l.coarse.get());
-            const feat_csr_f64_t A = l.matrix->desc();
+            const feat_matrix_f64_t& A = l.matrix->exec();
             check(feat_rj_smooth_f64(hd, &A, (const double*)r->fusable_jacobi()->inv_diag().elements(), l.filter->mask(),
                                      int(r->get_max_iter()), r->omega(), rhs_of(c), nrm_of(c), xs,
                                      (double*)l.tmp.elements(), nullptr, nullptr, nullptr, nullptr, ws),
@@ -607,11 +607,11 @@ namespace feat3amd
           auto& l = _h->levels[k];
           auto* post = static_cast<RichType*>(l.post.get());
           auto* jac = post->fusable_jacobi();
-          const feat_csr_f64_t P = l.transfer->get_mat_prol().desc();
-          check(feat_csr_prol_add_f64(hd, &P, sol_of(k), sol_of(k + 1), l.filter->mask()), "MultiGrid prolongation");
-          const feat_csr_f64_t A = l.matrix->desc();
+          const feat_matrix_f64_t& P = l.transfer->get_mat_prol().exec();
+          check(feat_prol_add_f64(hd, &P, sol_of(k), sol_of(k + 1), l.filter->mask()), "MultiGrid prolongation");
+          const feat_matrix_f64_t& A = l.matrix->exec();
           double* dn = _fw.scalar(2 + L + k);
-          check(feat_csr_defect_f64(hd, &A, (double*)l.def.elements(), rhs_of(k), sol_of(k), l.filter->mask(), dn, ws),
+          check(feat_defect_f64(hd, &A, (double*)l.def.elements(), rhs_of(k), sol_of(k), l.filter->mask(), dn, ws),
                 "MultiGrid defect");
           const bool top = (k == 0);
           check(feat_rj_smooth_f64(hd, &A, (const double*)jac->inv_diag().elements(), l.filter->mask(),
@@ -769,14 +769,14 @@ namespace feat3amd
         double* s_g = _fw.scalar(1);
         double* s_g2 = _fw.scalar(2);
         double* s_qp = _fw.scalar(3);
-        const feat_csr_f64_t A = _matrix.desc();
+        const feat_matrix_f64_t& A = _matrix.exec();
         const Index n = _matrix.rows();
         double* r = (double*)_vec_r.elements();
         double* p = (double*)_vec_p.elements();
         double* t = (double*)_vec_t.elements();
         if(correct)
-          check(feat_csr_defect_f64(hd, &A, r, (const double*)rhs->elements(), (const double*)x.elements(), _filter.mask(),
-                                    s_rr, ws),
+          check(feat_defect_f64(hd, &A, r, (const double*)rhs->elements(), (const double*)x.elements(), _filter.mask(),
+                                s_rr, ws),
                 "PCG defect");
         else
           check(feat_dot_f64(hd, s_rr, r, r, n), "PCG initial norm");
@@ -787,7 +787,7 @@ namespace feat3amd
         mg.apply_fused(_vec_p, _vec_r, s_rr, s_g);
         while(st == Status::progress)
         {
-          check(feat_csr_apply_masked_f64(hd, &A, t, p, _filter.mask(), nullptr, p, s_qp, ws), "PCG A*p");
+          check(feat_apply_masked_f64(hd, &A, t, p, _filter.mask(), nullptr, p, s_qp, ws), "PCG A*p");
           check(feat_pcg_update_f64(hd, (double*)x.elements(), r, p, t, s_g, s_qp, s_rr, n, ws), "PCG update");
           st = this->_set_new_defect([&]() {
             check(feat_read_scalar_f64(hd, s_rr, &rr), "PCG read");

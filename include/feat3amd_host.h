@@ -42,6 +42,7 @@ typedef struct {
   double tol_rel;
   uint64_t max_iter;
   int force_generic;      /* 1: reference op-by-op path instead of the fused one */
+  int csr_only;           /* 1: fused kernels on the CSR arrays instead of SELL-64 plans */
 } feat_mgpcg_cfg_t;
 
 typedef struct {          /* one level, host arrays; P maps the next coarser level onto this one */

@@ -30,7 +30,17 @@ def q1_level(feat, dim, lvl):
     return q, Csr(q["n"], q["n"], q["val"], q["col"], q["rowptr"])
 
 
-def test_fused_defect_masked_prol(feat, handle, torch_gpu, oracle):
+def exec_desc(feat, handle, torch, A, fmt):
+    """(keep-alive, descriptor) of A on the device: its CSR arrays or a SELL-64 plan."""
+    v, c, r = upload_csr(torch, A)
+    if fmt == "csr":
+        return (v, c, r), feat.csr_matrix(v, c, r, A.rows, A.cols)
+    plan = feat.SellPlan(handle, v, c, r, A.rows, A.cols)
+    return (v, c, r, plan), plan.m
+
+
+@pytest.mark.parametrize("fmt", ["csr", "sell"])
+def test_fused_defect_masked_prol(feat, handle, torch_gpu, oracle, fmt):
     torch = torch_gpu
     k, _ = feat.libs()
     P = feat.ptr
@@ -39,19 +49,18 @@ def test_fused_defect_masked_prol(feat, handle, torch_gpu, oracle):
     n = q["n"]
     rng = np.random.default_rng(21)
     x, rhs, w = rng.standard_normal((3, n))
-    v, c, r = upload_csr(torch, A)
-    desc = feat.csr_desc(v, c, r, n, n)
+    keepA, desc = exec_desc(feat, handle, torch, A, fmt)
     m = mask_of(torch, feat, handle, n, q["filt"])
     scal = torch.zeros(4, dtype=torch.float64, device="cuda")
     d = torch.empty(n, dtype=torch.float64, device="cuda")
     xd, rd, wd = dev(torch, x), dev(torch, rhs), dev(torch, w)
-    feat.check(k.feat_csr_defect_f64(handle.h, desc, P(d), P(rd), P(xd), P(m), P(scal), ws), "defect")
+    feat.check(k.feat_defect_f64(handle.h, desc, P(d), P(rd), P(xd), P(m), P(scal), ws), "defect")
     handle.sync()
     ref = oracle.unit_filter_def(oracle.csr_apply(A, x, rhs, -1.0, 1.0), q["filt"])
     assert np.array_equal(host(d), ref)
     assert abs(host(scal)[0] - ref @ ref) <= 1e-13 * (ref @ ref)
     # masked apply with norm and dot
-    feat.check(k.feat_csr_apply_masked_f64(handle.h, desc, P(d), P(xd), P(m), P(scal[1:]), P(wd), P(scal[2:]), ws), "masked")
+    feat.check(k.feat_apply_masked_f64(handle.h, desc, P(d), P(xd), P(m), P(scal[1:]), P(wd), P(scal[2:]), ws), "masked")
     handle.sync()
     ref = oracle.unit_filter_def(oracle.csr_apply(A, x), q["filt"])
     assert np.array_equal(host(d), ref)
@@ -60,19 +69,19 @@ def test_fused_defect_masked_prol(feat, handle, torch_gpu, oracle):
     assert abs(s[2] - ref @ w) <= 1e-13 * np.abs(ref * w).sum()
     # prolongation + correction
     Pm = Csr(n, q["pcols"], q["pval"], q["pcol"], q["prowptr"])
-    pv, pc, pr = upload_csr(torch, Pm)
-    pdesc = feat.csr_desc(pv, pc, pr, n, q["pcols"])
+    keepP, pdesc = exec_desc(feat, handle, torch, Pm, fmt)
     xc = rng.standard_normal(q["pcols"])
     sol = dev(torch, x)
     xcd = dev(torch, xc)
-    feat.check(k.feat_csr_prol_add_f64(handle.h, pdesc, P(sol), P(xcd), P(m)), "prol")
+    feat.check(k.feat_prol_add_f64(handle.h, pdesc, P(sol), P(xcd), P(m)), "prol")
     handle.sync()
     cor = oracle.unit_filter_def(oracle.csr_apply(Pm, xc), q["filt"])
     assert np.array_equal(host(sol), oracle.axpy(1.0, cor, x))
 
 
+@pytest.mark.parametrize("fmt", ["csr", "sell"])
 @pytest.mark.parametrize("steps", [1, 2, 4])
-def test_fused_richardson_jacobi(feat, handle, torch_gpu, oracle, steps):
+def test_fused_richardson_jacobi(feat, handle, torch_gpu, oracle, steps, fmt):
     torch = torch_gpu
     k, _ = feat.libs()
     P = feat.ptr
@@ -81,8 +90,8 @@ def test_fused_richardson_jacobi(feat, handle, torch_gpu, oracle, steps):
     n = q["n"]
     rng = np.random.default_rng(22)
     rhs = oracle.unit_filter_def(rng.standard_normal(n), q["filt"])
-    v, c, r = upload_csr(torch, A)
-    desc = feat.csr_desc(v, c, r, n, n)
+    keepA, desc = exec_desc(feat, handle, torch, A, fmt)
+    v, c, r = keepA[:3]
     m = mask_of(torch, feat, handle, n, q["filt"])
     omega_j, omega_r = 0.5, 0.9
     inv = torch.empty(n, dtype=torch.float64, device="cuda")
@@ -131,6 +140,65 @@ def test_fused_richardson_jacobi(feat, handle, torch_gpu, oracle, steps):
     assert not host(x).any() and not host(dout).any()
 
 
+@pytest.mark.parametrize("seed,rows,cols,maxlen,band", [
+    (31, 5000, 5000, 30, 100),       # narrow band -> i16 column deltas
+    (32, 3000, 70000, 25, None),     # wide columns -> u32 absolute indices
+    (33, 700, 700, 0, None),         # empty rows only
+    (34, 129, 129, 40, 20),          # last slice holds one row
+])
+def test_sell_plan_apply_bitwise(feat, handle, torch_gpu, oracle, seed, rows, cols, maxlen, band):
+    torch = torch_gpu
+    k, _ = feat.libs()
+    P = feat.ptr
+    rng = np.random.default_rng(seed)
+    from tests.helpers import random_csr
+    A = random_csr(rng, rows, cols, maxlen // 2, maxlen)
+    if band is not None:
+        # keep columns within +-band of the row (clipped), still sorted per row
+        rr = np.repeat(np.arange(rows), np.diff(A.rowptr.astype(np.int64)))
+        A.col = np.clip(rr + (A.col.astype(np.int64) % (2 * band + 1)) - band, 0, cols - 1).astype(np.uint64)
+        for r in range(rows):
+            a, b = int(A.rowptr[r]), int(A.rowptr[r + 1])
+            A.col[a:b] = np.sort(A.col[a:b])
+    x = rng.standard_normal(cols)
+    y = rng.standard_normal(rows)
+    keep, desc = exec_desc(feat, handle, torch, A, "sell")
+    if rows and A.nnz:
+        assert desc.format == feat.FMT_SELL
+        assert desc.index == (feat.IDX_D16 if band is not None else feat.IDX_U32)
+    xd, yd = dev(torch, x), dev(torch, y)
+    rd = torch.empty(rows, dtype=torch.float64, device="cuda")
+    for a, b in [(1.0, 0.0), (-1.0, 1.0), (2.5, -0.7)]:
+        feat.check(k.feat_matrix_apply_f64(handle.h, desc, P(rd), a, P(xd), b, P(yd)), "sell apply")
+        handle.sync()
+        assert np.array_equal(host(rd), oracle.csr_apply(A, x, y, a, b)), (a, b)
+    # FP32 plan: values rounded once, fmaf in CSR order (the oracle's f32 apply)
+    v, c, r = keep[:3]
+    p32 = feat.SellPlan(handle, v, c, r, rows, cols, fp="f32")
+    x32 = dev(torch, x.astype(np.float32))
+    r32 = torch.empty(rows, dtype=torch.float32, device="cuda")
+    keep_ws, ws = feat.reduce_ws(torch, "cuda")
+    feat.check(k.feat_apply_masked_f32(handle.h, p32.m, P(r32), P(x32), None, None, ws), "sell f32")
+    handle.sync()
+    ref = oracle.csr_apply_f32(A, x.astype(np.float32))
+    assert np.array_equal(host(r32), ref)
+    p32.close()
+
+
+def test_sell_plan_rejects_long_rows(feat, handle, torch_gpu):
+    # a 70000-entry row exceeds the 16-bit row length; the plan refuses and the CSR stays usable
+    torch = torch_gpu
+    from oracle.pyoracle import Csr
+    n = 70001
+    lens = np.r_[70000, np.ones(n - 1, np.int64)]
+    rp = np.concatenate([[0], np.cumsum(lens)]).astype(np.uint64)
+    col = np.concatenate([np.arange(70000), np.arange(1, n)]).astype(np.uint64)
+    A = Csr(n, n, np.ones(col.size), col, rp)
+    v, c, r = upload_csr(torch, A)
+    with pytest.raises(feat.FeatError):
+        feat.SellPlan(handle, v, c, r, n, n)
+
+
 def c1_problem(feat):
     levels = q1_levels(feat, 2, 7, 1)
     n = levels[0]["n"]
@@ -157,14 +225,15 @@ def test_mgpcg_c1_vs_reference_golden(feat, torch_gpu, golden, generic):
     assert np.allclose(r["hist"][:ref_it + 1], golden["c1_hist"][:r["iters"] + 1], rtol=1e-9)
 
 
-@pytest.mark.parametrize("dim,lf,lc,coarse", [
-    (2, 7, 1, "direct"),
-    (2, 5, 0, "smoother"),
-    (3, 5, 1, "direct"),
-    (3, 5, 1, "jacobi"),
-    (3, 6, 2, "direct"),
+@pytest.mark.parametrize("dim,lf,lc,coarse,csr_only", [
+    (2, 7, 1, "direct", False),
+    (2, 5, 0, "smoother", False),
+    (3, 5, 1, "direct", False),
+    (3, 5, 1, "jacobi", True),
+    (3, 6, 2, "direct", False),
+    (3, 6, 1, "direct", True),
 ])
-def test_mgpcg_vs_oracle(feat, torch_gpu, oracle, dim, lf, lc, coarse):
+def test_mgpcg_vs_oracle(feat, torch_gpu, oracle, dim, lf, lc, coarse, csr_only):
     torch = torch_gpu
     levels = q1_levels(feat, dim, lf, lc)
     n = levels[0]["n"]
@@ -173,7 +242,7 @@ def test_mgpcg_vs_oracle(feat, torch_gpu, oracle, dim, lf, lc, coarse):
     ocoarse = {"direct": COARSE_DIRECT, "jacobi": COARSE_JACOBI, "smoother": COARSE_SMOOTHER}[coarse]
     o = oracle.mg_pcg_solve(levels, np.zeros(n), b, coarse=ocoarse)
     for generic in (False, True):
-        s = feat.MgPcg(feat.mgpcg_cfg(coarse=coarse, force_generic=generic), levels=levels)
+        s = feat.MgPcg(feat.mgpcg_cfg(coarse=coarse, force_generic=generic, csr_only=csr_only), levels=levels)
         x = torch.zeros(n, dtype=torch.float64, device="cuda")
         r = s.solve(x, dev(torch, b))
         s.close()
