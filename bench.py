@@ -133,13 +133,53 @@ def main():
     assert res["status"] == "success", res
     t_mean = float(np.mean(times))
 
+    # ---- the mixed-precision forms: FP32 V-cycle (FP32 or FP16 matrix values)
+    # inside the FP64 PCG; outer residual, update, convergence test and coarse solve FP64
+    kk, _ = feat.libs()
+    hh = feat.Handle(0, stream=stream.cuda_stream)
+    bnd_t = torch.from_numpy(bnd).to(dev)
+    mixed = {}
+    for prec in ("f32", "f16"):
+        cfgm = feat.mgpcg_cfg(nu_pre=4, nu_post=4, omega_jac=0.5, omega_rich=1.0, coarse="direct",
+                              tol_rel=1e-8, max_iter=100, precision=prec)
+        sm = feat.MgPcg(cfgm, q1=(DIM, LF, LC))
+        xm = torch.zeros(n, dtype=torch.float64, device=dev)
+        for _ in range(args.warmup):
+            xm.zero_()
+            torch.cuda.synchronize()
+            rm = sm.solve(xm, b)
+        tm = []
+        for k in range(args.steps):
+            xm.zero_()
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            rm = sm.solve(xm, b)
+            torch.cuda.synchronize()
+            tm.append(time.perf_counter() - t)
+        assert rm["status"] == "success", rm
+        # FP64 verification: true residual with the FP64 CSR, error vs the FP64 solve
+        rr = torch.empty(n, dtype=torch.float64, device=dev)
+        feat.check(kk.feat_csr_apply_f64_u64(hh.h, feat.ptr(rr), -1.0, feat.ptr(xm), 1.0, feat.ptr(b),
+                                             C.c_void_p(A.val), C.c_void_p(A.col), C.c_void_p(A.rowptr), n, n,
+                                             nnz, 0), "residual")
+        rr[bnd_t] = 0.0
+        tmean = float(np.mean(tm))
+        mixed[prec] = {"s_per_solve": tmean, "iters": rm["iters"], "speedup_vs_fp64": t_mean / tmean,
+                       "true_rel_residual_fp64": float(torch.linalg.norm(rr) / torch.linalg.norm(b)),
+                       "rel_error_vs_fp64_solution": float(torch.linalg.norm(xm - x) / torch.linalg.norm(x))}
+        sm.close()
+        log(f"mixed {prec}: {tmean * 1e3:.2f} ms, {rm['iters']} iterations, true rel. residual "
+            f"{mixed[prec]['true_rel_residual_fp64']:.3e}, |x-x64|/|x64| {mixed[prec]['rel_error_vs_fp64_solution']:.3e}, "
+            f"speedup {t_mean / tmean:.2f}x")
+    hh.close()
+
     # ---- SpMV r = A*x on the same matrix, HIP events on the launch stream:
     # the caller's CSR (u64) and the SELL-64 plan the solver runs on
     k, _ = feat.libs()
     h = feat.Handle(0, stream=stream.cuda_stream)
     xs = torch.rand(n, dtype=torch.float64, device=dev)
     r = torch.empty(n, dtype=torch.float64, device=dev)
-    csr = feat.Matrix(feat.FMT_CSR, feat.IDX_U64, n, n, nnz, A.val, A.col, A.rowptr, None)
+    csr = feat.Matrix(feat.FMT_CSR, feat.IDX_U64, n, n, nnz, A.val, A.col, A.rowptr, None, 0, 1.0)
     plan = feat.Matrix()
     feat.check(k.feat_sell_create_f64(h.h, C.byref(A), C.byref(plan)), "feat_sell_create_f64")
 
@@ -192,6 +232,8 @@ def main():
             "final_rel_defect": res["def_final"] / res["def_init"],
             "parallelism": "single GPU",
         },
+        "mixed_precision": {"form": "FP32 V-cycle on FP32 / FP16-valued SELL-64 plans (FP64 coarse solve) "
+                                    "inside FP64 PCG", "fp32": mixed["f32"], "fp16_values": mixed["f16"]},
         "spmv": {
             "sell64": {"kernel": "k_sell_rows<double,short,true,EpiApply<double>>", "us_per_launch": sell_ms * 1e3,
                        "bytes_per_launch": sell_bytes, "gbps": sell_gbs,

@@ -49,7 +49,8 @@ class CsrF32(C.Structure):
 class Matrix(C.Structure):
     """feat_matrix_f64_t / feat_matrix_f32_t (same layout)."""
     _fields_ = [("format", _i), ("index", _i), ("rows", _u64), ("cols", _u64), ("nnz", _u64),
-                ("val", _vp), ("col", _vp), ("ptr", _vp), ("rowlen", _vp)]
+                ("val", _vp), ("col", _vp), ("ptr", _vp), ("rowlen", _vp), ("value_kind", _i),
+                ("scale", _f)]
 
 
 FMT_CSR, FMT_SELL = 0, 1
@@ -63,7 +64,7 @@ class ReduceWs(C.Structure):
 class MgPcgCfg(C.Structure):
     _fields_ = [("nu_pre", _i), ("nu_post", _i), ("omega_jac", _d), ("omega_rich", _d),
                 ("coarse", _i), ("coarse_omega", _d), ("tol_rel", _d), ("max_iter", _u64),
-                ("force_generic", _i), ("csr_only", _i)]
+                ("force_generic", _i), ("csr_only", _i), ("precision", _i)]
 
 
 class LevelHost(C.Structure):
@@ -119,6 +120,7 @@ KERNEL_API = [
     ("feat_matrix_from_csr_f64", _i, [C.POINTER(CsrF64), C.POINTER(Matrix)]),
     ("feat_sell_create_f64", _i, [_vp, C.POINTER(CsrF64), C.POINTER(Matrix)]),
     ("feat_sell_create_f32", _i, [_vp, C.POINTER(CsrF64), C.POINTER(Matrix)]),
+    ("feat_sell_create_f16", _i, [_vp, C.POINTER(CsrF64), C.POINTER(Matrix)]),
     ("feat_sell_destroy_f64", _i, [C.POINTER(Matrix)]),
     ("feat_sell_destroy_f32", _i, [C.POINTER(Matrix)]),
     ("feat_matrix_apply_f64", _i, [_vp, C.POINTER(Matrix), _vp, _d, _vp, _d, _vp]),
@@ -228,7 +230,7 @@ def csr_desc(val, col, rowptr, rows, cols):
 
 def reduce_ws(torch_mod, device):
     """A reduction workspace: (keep-alive tensors, ReduceWs)."""
-    part = torch_mod.zeros(2 * 4096, dtype=torch_mod.float64, device=device)
+    part = torch_mod.zeros(2 * 16384, dtype=torch_mod.float64, device=device)
     cnt = torch_mod.zeros(16, dtype=torch_mod.int32, device=device)
     return (part, cnt), ReduceWs(part.data_ptr(), cnt.data_ptr())
 
@@ -271,9 +273,10 @@ def rhs_random(seed, n, a=-1.0, b=1.0):
 
 
 def mgpcg_cfg(nu_pre=4, nu_post=4, omega_jac=0.5, omega_rich=1.0, coarse="direct",
-              coarse_omega=1.0, tol_rel=1e-8, max_iter=100, force_generic=False, csr_only=False):
+              coarse_omega=1.0, tol_rel=1e-8, max_iter=100, force_generic=False, csr_only=False,
+              precision="f64"):
     return MgPcgCfg(nu_pre, nu_post, omega_jac, omega_rich, COARSE[coarse], coarse_omega, tol_rel,
-                    max_iter, int(force_generic), int(csr_only))
+                    max_iter, int(force_generic), int(csr_only), {"f64": 0, "f32": 1, "f16": 2}[precision])
 
 
 def csr_matrix(val, col, rowptr, rows, cols):
@@ -286,7 +289,7 @@ def csr_matrix(val, col, rowptr, rows, cols):
     else:
         idx = IDX_U64 if col.dtype.itemsize == 8 else IDX_U32
         m = Matrix(FMT_CSR, idx, rows, cols, val.numel(), val.data_ptr(), col.data_ptr(),
-                   rowptr.data_ptr(), None)
+                   rowptr.data_ptr(), None, 0, 1.0)
     return m
 
 
@@ -298,7 +301,8 @@ class SellPlan:
         self.fp = fp
         c = csr_desc(val, col, rowptr, rows, cols)
         self.m = Matrix()
-        fn = self.k.feat_sell_create_f64 if fp == "f64" else self.k.feat_sell_create_f32
+        fn = {"f64": self.k.feat_sell_create_f64, "f32": self.k.feat_sell_create_f32,
+              "f16": self.k.feat_sell_create_f16}[fp]
         check(fn(handle.h, C.byref(c), C.byref(self.m)), fn.__name__)
 
     def close(self):

@@ -33,6 +33,7 @@ constexpr int kXcd = 8;
 constexpr int kMaxGrid = FEAT_REDUCE_MAX_BLOCKS;
 constexpr int kRowsPerBlock = 64;   // rows per CSR row-block
 constexpr int kChunk = 2048;        // nonzeros staged in LDS per pass
+constexpr int kSellUnroll = 8;      // SELL entries per row in flight
 
 thread_local std::string g_err;
 
@@ -373,13 +374,25 @@ __device__ __forceinline__ uint64_t sell_col(CT c, uint64_t row)
   return uint64_t(c);
 }
 
-template<typename DT, typename CT, bool DELTA, class Epi>
-__global__ __launch_bounds__(kThreads) void k_sell_rows(const DT* __restrict__ val,
+// stored value -> arithmetic type (uint16_t carries FP16 bits)
+template<typename DT, typename VT>
+__device__ __forceinline__ DT ld_val(const VT* p)
+{
+  return DT(__builtin_nontemporal_load(p));
+}
+template<>
+__device__ __forceinline__ float ld_val<float, uint16_t>(const uint16_t* p)
+{
+  return float(__builtin_bit_cast(_Float16, __builtin_nontemporal_load(p)));
+}
+
+template<typename DT, typename VT, typename CT, bool DELTA, class Epi>
+__global__ __launch_bounds__(kThreads) void k_sell_rows(const VT* __restrict__ val,
                                                         const CT* __restrict__ col,
                                                         const uint64_t* __restrict__ soff,
                                                         const uint16_t* __restrict__ rlen,
                                                         const DT* __restrict__ x, uint64_t rows,
-                                                        Epi epi, GridReduce gr)
+                                                        DT scale, Epi epi, GridReduce gr)
 {
   const uint64_t nch = (rows + kThreads - 1) / kThreads;
   const XcdRange xr = xcd_range(nch);
@@ -393,23 +406,30 @@ __global__ __launch_bounds__(kThreads) void k_sell_rows(const DT* __restrict__ v
     {
       const uint64_t off = soff[row >> 6];
       const int len = rlen[row];
-      const DT* v = val + off + lane;
+      const VT* v = val + off + lane;
       const CT* c = col + off + lane;
       DT sum = DT(0);
       int k = 0;
-      for(; k + 4 <= len; k += 4)
+      // matrix arrays are streamed once: nontemporal loads keep x resident in L2
+      for(; k + kSellUnroll <= len; k += kSellUnroll)
       {
-        const DT v0 = v[(k + 0) * 64], v1 = v[(k + 1) * 64], v2 = v[(k + 2) * 64], v3 = v[(k + 3) * 64];
-        const uint64_t c0 = sell_col<DELTA>(c[(k + 0) * 64], row), c1 = sell_col<DELTA>(c[(k + 1) * 64], row);
-        const uint64_t c2 = sell_col<DELTA>(c[(k + 2) * 64], row), c3 = sell_col<DELTA>(c[(k + 3) * 64], row);
-        const DT x0 = x[c0], x1 = x[c1], x2 = x[c2], x3 = x[c3];
-        sum = dfma(v0, x0, sum);
-        sum = dfma(v1, x1, sum);
-        sum = dfma(v2, x2, sum);
-        sum = dfma(v3, x3, sum);
+        DT vv[kSellUnroll];
+        uint64_t cc[kSellUnroll];
+        DT xv[kSellUnroll];
+#pragma unroll
+        for(int u = 0; u < kSellUnroll; ++u)
+        {
+          vv[u] = ld_val<DT, VT>(v + (k + u) * 64);
+          cc[u] = sell_col<DELTA>(__builtin_nontemporal_load(c + (k + u) * 64), row);
+        }
+#pragma unroll
+        for(int u = 0; u < kSellUnroll; ++u) xv[u] = x[cc[u]];
+#pragma unroll
+        for(int u = 0; u < kSellUnroll; ++u) sum = dfma(vv[u], xv[u], sum);
       }
       for(; k < len; ++k)
-        sum = dfma(v[k * 64], x[sell_col<DELTA>(c[k * 64], row)], sum);
+        sum = dfma(ld_val<DT, VT>(v + k * 64), x[sell_col<DELTA>(__builtin_nontemporal_load(c + k * 64), row)], sum);
+      if(!std::is_same<VT, DT>::value) sum = sum * scale;  // exact: a power of two
       epi.row(row, sum, acc0, acc1);
     }
   }
@@ -430,19 +450,36 @@ int launch_mat(hipStream_t s, const M* A, const DT* x, const Epi& epi, const Gri
   }
   else if(A->format == FEAT_FMT_SELL)
   {
+    // one workgroup per 256-row chunk up to the reduction-partials limit, so the
+    // hardware dispatcher balances the load; larger matrices stride
     const uint64_t nch = (A->rows + kThreads - 1) / kThreads;
     const int grid = int(nch < uint64_t(kMaxGrid) ? nch : uint64_t(kMaxGrid));
     const uint64_t* soff = (const uint64_t*)A->ptr;
+    const DT sc = DT(A->scale);
+    if(A->value_kind == FEAT_VAL_F16)
+    {
+      if(!std::is_same<DT, float>::value) return fail(FEAT_ERR_UNSUPPORTED, "FP16 values need FP32 arithmetic");
+      const uint16_t* vh = (const uint16_t*)(const void*)A->val;
+      if(A->index == FEAT_IDX_U32)
+        hipLaunchKernelGGL((k_sell_rows<DT, uint16_t, uint32_t, false, Epi>), dim3(grid), dim3(kThreads), 0, s, vh,
+                           (const uint32_t*)A->col, soff, A->rowlen, x, A->rows, sc, epi, gr);
+      else if(A->index == FEAT_IDX_D16)
+        hipLaunchKernelGGL((k_sell_rows<DT, uint16_t, int16_t, true, Epi>), dim3(grid), dim3(kThreads), 0, s, vh,
+                           (const int16_t*)A->col, soff, A->rowlen, x, A->rows, sc, epi, gr);
+      else
+        return fail(FEAT_ERR_UNSUPPORTED, "SELL index kind");
+      return check_launch(name);
+    }
     if(A->index == FEAT_IDX_U32)
     {
-      hipLaunchKernelGGL((k_sell_rows<DT, uint32_t, false, Epi>), dim3(grid), dim3(kThreads), 0, s, A->val,
-                         (const uint32_t*)A->col, soff, A->rowlen, x, A->rows, epi, gr);
+      hipLaunchKernelGGL((k_sell_rows<DT, DT, uint32_t, false, Epi>), dim3(grid), dim3(kThreads), 0, s, A->val,
+                         (const uint32_t*)A->col, soff, A->rowlen, x, A->rows, sc, epi, gr);
       return check_launch(name);
     }
     if(A->index == FEAT_IDX_D16)
     {
-      hipLaunchKernelGGL((k_sell_rows<DT, int16_t, true, Epi>), dim3(grid), dim3(kThreads), 0, s, A->val,
-                         (const int16_t*)A->col, soff, A->rowlen, x, A->rows, epi, gr);
+      hipLaunchKernelGGL((k_sell_rows<DT, DT, int16_t, true, Epi>), dim3(grid), dim3(kThreads), 0, s, A->val,
+                         (const int16_t*)A->col, soff, A->rowlen, x, A->rows, sc, epi, gr);
       return check_launch(name);
     }
   }
@@ -470,9 +507,33 @@ __global__ void k_sell_scan_rows(const uint64_t* rp, const uint64_t* col, uint64
   }
 }
 
+template<typename VT>
+__device__ __forceinline__ VT st_val(double a, float inv_scale)
+{
+  return VT(a);
+}
+template<>
+__device__ __forceinline__ uint16_t st_val<uint16_t>(double a, float inv_scale)
+{
+  // FP16 bits of half(float(a) * 2^-e): round to FP32, scale exactly, round to FP16
+  return __builtin_bit_cast(uint16_t, _Float16(float(a) * inv_scale));
+}
+
+__global__ void k_absmax(const double* val, uint64_t n, uint32_t* out)
+{
+  uint32_t m = 0;
+  for(uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
+  {
+    const float a = std::fabs(float(val[i]));
+    const uint32_t b = __builtin_bit_cast(uint32_t, a);
+    m = b > m ? b : m;
+  }
+  atomicMax(out, m);
+}
+
 template<typename DT, typename CT, bool DELTA>
 __global__ void k_sell_fill(const double* val, const uint64_t* col, const uint64_t* rp, uint64_t rows,
-                            const uint64_t* soff, DT* sval, CT* scol)
+                            const uint64_t* soff, DT* sval, CT* scol, float inv_scale)
 {
   for(uint64_t r = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; r < rows; r += uint64_t(gridDim.x) * blockDim.x)
   {
@@ -484,7 +545,7 @@ __global__ void k_sell_fill(const double* val, const uint64_t* col, const uint64
       const uint64_t p = off + k * 64 + lane;
       if(k < len)
       {
-        sval[p] = DT(val[b + k]);
+        sval[p] = st_val<DT>(val[b + k], inv_scale);
         scol[p] = DELTA ? CT(std::int64_t(col[b + k]) - std::int64_t(r)) : CT(col[b + k]);
       }
       else
@@ -1234,7 +1295,8 @@ int feat_mean_filter_f64(feat_handle_t h, double* v, const double* w_dot, const 
 int feat_matrix_from_csr_f64(const feat_csr_f64_t* A, feat_matrix_f64_t* out)
 {
   if(!A || !out) return fail(FEAT_ERR_INVALID, "feat_matrix_from_csr_f64: null argument");
-  *out = feat_matrix_f64_t{FEAT_FMT_CSR, FEAT_IDX_U64, A->rows, A->cols, A->nnz, A->val, A->col, A->rowptr, nullptr};
+  *out = feat_matrix_f64_t{FEAT_FMT_CSR, FEAT_IDX_U64, A->rows, A->cols, A->nnz, A->val, A->col, A->rowptr, nullptr,
+                           FEAT_VAL_NATIVE, 1.0f};
   return FEAT_OK;
 }
 
@@ -1244,6 +1306,7 @@ namespace {
 template<typename DT, typename M>
 int sell_create_impl(feat_handle_t h, const feat_csr_f64_t* A, M* out, const char* name)
 {
+  // DT is the stored value type: double, float, or uint16_t (FP16 bits)
   if(!h || !A || !out || !A->rowptr || (A->nnz && (!A->val || !A->col))) return fail(FEAT_ERR_INVALID, name);
   std::memset(out, 0, sizeof(M));
   const uint64_t n = A->rows, ns = (n + 63) / 64;
@@ -1280,7 +1343,9 @@ int sell_create_impl(feat_handle_t h, const feat_csr_f64_t* A, M* out, const cha
   std::vector<uint64_t> hoff(ns + 1, 0);
   for(uint64_t i = 0; i < ns; ++i) hoff[i + 1] = hoff[i] + uint64_t(hmax[i]) * 64;
   const uint64_t total = hoff[ns];
-  if(hst[1] || total > 2 * A->nnz + 64 * ns)
+  // reject skewed row lengths (padding beyond twice the nonzeros) on matrices
+  // where it matters; small matrices (partial slices) are always accepted
+  if(hst[1] || (total > 2 * A->nnz + 64 * ns && total > (uint64_t(1) << 16)))
   {
     cleanup();
     return fail(FEAT_ERR_UNSUPPORTED, "SELL-64 plan: row too long or padding too large; keep CSR");
@@ -1292,6 +1357,28 @@ int sell_create_impl(feat_handle_t h, const feat_csr_f64_t* A, M* out, const cha
     return fail(FEAT_ERR_UNSUPPORTED, "SELL-64 plan: columns exceed 32 bits");
   }
   const size_t cb = d16 ? sizeof(int16_t) : sizeof(uint32_t);
+  // FP16: scale 2^e with 2^(e-1) <= max|a| < 2^e, so stored values lie in (-1, 1)
+  float scale = 1.0f, inv_scale = 1.0f;
+  if(std::is_same<DT, uint16_t>::value && A->nnz)
+  {
+    uint32_t* am = nullptr;
+    uint32_t hm = 0;
+    rc = check_hip(hipMalloc(&am, sizeof(uint32_t)), "absmax");
+    if(!rc) rc = check_hip(hipMemsetAsync(am, 0, sizeof(uint32_t), h->stream), "memset");
+    if(!rc) hipLaunchKernelGGL(k_absmax, dim3(grid_for(A->nnz, 8)), dim3(kThreads), 0, h->stream, A->val, A->nnz, am);
+    if(!rc) rc = check_hip(hipMemcpyAsync(&hm, am, sizeof(uint32_t), hipMemcpyDeviceToHost, h->stream), "copy");
+    if(!rc) rc = check_hip(hipStreamSynchronize(h->stream), "sync");
+    if(am) (void)hipFree(am);
+    if(rc) { cleanup(); return rc; }
+    const float mx = __builtin_bit_cast(float, hm);
+    if(mx > 0.0f && std::isfinite(mx))
+    {
+      int e = 0;
+      (void)std::frexp(mx, &e);
+      scale = std::ldexp(1.0f, e);
+      inv_scale = std::ldexp(1.0f, -e);
+    }
+  }
   rc = check_hip(hipMalloc(&sval, sizeof(DT) * (total ? total : 1)), "sell val");
   if(!rc) rc = check_hip(hipMalloc(&scol, cb * (total ? total : 1)), "sell col");
   if(!rc) rc = check_hip(hipMemcpyAsync(soff, hoff.data(), sizeof(uint64_t) * (ns + 1), hipMemcpyHostToDevice, h->stream), "copy");
@@ -1300,10 +1387,10 @@ int sell_create_impl(feat_handle_t h, const feat_csr_f64_t* A, M* out, const cha
   {
     if(d16)
       hipLaunchKernelGGL((k_sell_fill<DT, int16_t, true>), dim3(grid_for(n)), dim3(kThreads), 0, h->stream, A->val, A->col,
-                         A->rowptr, n, soff, sval, (int16_t*)scol);
+                         A->rowptr, n, soff, sval, (int16_t*)scol, inv_scale);
     else
       hipLaunchKernelGGL((k_sell_fill<DT, uint32_t, false>), dim3(grid_for(n)), dim3(kThreads), 0, h->stream, A->val, A->col,
-                         A->rowptr, n, soff, sval, (uint32_t*)scol);
+                         A->rowptr, n, soff, sval, (uint32_t*)scol, inv_scale);
   }
   rc = check_launch(name);
   if(!rc) rc = check_hip(hipStreamSynchronize(h->stream), "sync");
@@ -1317,7 +1404,9 @@ int sell_create_impl(feat_handle_t h, const feat_csr_f64_t* A, M* out, const cha
   out->rows = n;
   out->cols = A->cols;
   out->nnz = total;
-  out->val = sval;
+  out->val = (decltype(out->val))(void*)sval;
+  out->value_kind = std::is_same<DT, uint16_t>::value ? FEAT_VAL_F16 : FEAT_VAL_NATIVE;
+  out->scale = scale;
   out->col = scol;
   out->ptr = soff;
   out->rowlen = rlen;
@@ -1471,6 +1560,11 @@ int feat_sell_create_f64(feat_handle_t h, const feat_csr_f64_t* A, feat_matrix_f
 int feat_sell_create_f32(feat_handle_t h, const feat_csr_f64_t* A, feat_matrix_f32_t* out)
 {
   return sell_create_impl<float>(h, A, out, "feat_sell_create_f32");
+}
+
+int feat_sell_create_f16(feat_handle_t h, const feat_csr_f64_t* A, feat_matrix_f32_t* out)
+{
+  return sell_create_impl<uint16_t>(h, A, out, "feat_sell_create_f16");
 }
 
 int feat_sell_destroy_f64(feat_matrix_f64_t* m) { return sell_destroy_impl(m); }

@@ -3,6 +3,7 @@
 // kernel library libfeat3amd.so; every device operation goes through feat3amd.h.
 #include <feat3amd/assembly.hpp>
 #include <feat3amd/lafem.hpp>
+#include <feat3amd/mixed.hpp>
 #include <feat3amd/solver.hpp>
 #include <feat3amd_host.h>
 
@@ -18,6 +19,7 @@ typedef LAFEM::UnitFilter<double, Index> Filt;
 typedef LAFEM::Transfer<Csr> Trans;
 typedef Solver::MultiGridHierarchy<Csr, Filt, Trans> Hier;
 typedef Solver::MultiGrid<Csr, Filt, Trans> MG;
+typedef Solver::MixedMultiGrid<Csr, Filt, Trans> MG32;
 
 namespace
 {
@@ -53,6 +55,7 @@ struct feat_mgpcg_s
   std::vector<std::unique_ptr<LevelData>> levels;
   std::shared_ptr<Hier> hier;
   std::shared_ptr<MG> mg;
+  std::shared_ptr<MG32> mg32;
   std::shared_ptr<Solver::PCG<Csr, Filt>> pcg;
 };
 
@@ -162,8 +165,16 @@ static void build_solver(feat_mgpcg_s* s, const feat_mgpcg_cfg_t* cfg)
       s->hier->push_level(L.A, L.F, cs);
     }
   }
-  s->mg = std::make_shared<MG>(s->hier);
-  s->pcg = Solver::new_pcg(s->levels[0]->A, s->levels[0]->F, s->mg);
+  if(cfg->precision == 1 || cfg->precision == 2)
+  {
+    s->mg32 = std::make_shared<MG32>(s->hier, cfg->precision == 2);
+    s->pcg = Solver::new_pcg(s->levels[0]->A, s->levels[0]->F, s->mg32);
+  }
+  else
+  {
+    s->mg = std::make_shared<MG>(s->hier);
+    s->pcg = Solver::new_pcg(s->levels[0]->A, s->levels[0]->F, s->mg);
+  }
   s->pcg->set_tol_rel(cfg->tol_rel);
   s->pcg->set_max_iter(cfg->max_iter);
   s->pcg->set_force_generic(cfg->force_generic != 0);

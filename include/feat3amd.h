@@ -165,7 +165,7 @@ int feat_mean_filter_f64(feat_handle_t h, double* v, const double* w_dot, const 
  *
  * A reduction workspace: `partials` holds 2*FEAT_REDUCE_MAX_BLOCKS doubles,
  * `counter` one zero-initialised uint32 (left at zero after every call). */
-#define FEAT_REDUCE_MAX_BLOCKS 4096
+#define FEAT_REDUCE_MAX_BLOCKS 16384
 typedef struct {
   double* partials;
   uint32_t* counter;
@@ -189,6 +189,8 @@ typedef struct {            /* a CSR matrix on the device (u64 indices) */
 #define FEAT_IDX_U64 0   /* CSR: u64 col/rowptr                              */
 #define FEAT_IDX_U32 1   /* CSR: u32 col/rowptr; SELL: u32 absolute columns  */
 #define FEAT_IDX_D16 2   /* SELL: int16 column minus row                      */
+#define FEAT_VAL_NATIVE 0 /* values in the descriptor's arithmetic type     */
+#define FEAT_VAL_F16 1    /* FP32 arithmetic on FP16 values times `scale`   */
 typedef struct {
   int format, index;
   uint64_t rows, cols;
@@ -197,15 +199,19 @@ typedef struct {
   const void* col;
   const void* ptr;           /* CSR row pointer / SELL slice offsets (u64)    */
   const uint16_t* rowlen;    /* SELL row lengths                             */
+  int value_kind;            /* FEAT_VAL_NATIVE                               */
+  float scale;               /* 1                                             */
 } feat_matrix_f64_t;
 
-typedef struct {             /* FP32 values (mixed-precision inner solver)    */
+typedef struct {             /* FP32 arithmetic (mixed-precision inner solver) */
   int format, index;
   uint64_t rows, cols, nnz;
-  const float* val;
+  const float* val;          /* FP32 values, or FP16 values (value_kind F16)  */
   const void* col;
   const void* ptr;
   const uint16_t* rowlen;
+  int value_kind;            /* FEAT_VAL_NATIVE or FEAT_VAL_F16 (SELL only)    */
+  float scale;               /* row sums are multiplied by this power of two  */
 } feat_matrix_f32_t;
 
 /* A CSR descriptor (no copy) of the caller's arrays. */
@@ -216,6 +222,9 @@ int feat_matrix_from_csr_f64(const feat_csr_f64_t* A, feat_matrix_f64_t* out);
 int feat_sell_create_f64(feat_handle_t h, const feat_csr_f64_t* A, feat_matrix_f64_t* out);
 /* Same plan with values rounded to FP32 (ConvertPrecond, convert_precond.hpp:100-119). */
 int feat_sell_create_f32(feat_handle_t h, const feat_csr_f64_t* A, feat_matrix_f32_t* out);
+/* Same plan with FP16 values: v16 = half(float(a) * 2^-e) with 2^(e-1) <= max|a| < 2^e;
+ * row sums sum(fmaf(float(v16), x, s)) are multiplied by scale = 2^e. */
+int feat_sell_create_f16(feat_handle_t h, const feat_csr_f64_t* A, feat_matrix_f32_t* out);
 int feat_sell_destroy_f64(feat_matrix_f64_t* m);
 int feat_sell_destroy_f32(feat_matrix_f32_t* m);
 

@@ -235,7 +235,9 @@ namespace feat3amd
       DeviceArray<DT_> _val;
       DeviceArray<IT_> _col, _rowptr;
       mutable std::shared_ptr<feat_matrix_f64_t> _plan;
-      mutable std::shared_ptr<feat_matrix_f32_t> _plan32;
+      mutable std::shared_ptr<feat_matrix_f32_t> _plan32, _plan16;
+      mutable DeviceArray<float> _csr32_val;
+      mutable DeviceArray<std::uint32_t> _csr32_col, _csr32_rp;
       bool _use_sell = true;
 
     public:
@@ -313,10 +315,44 @@ namespace feat3amd
             delete m;
           });
           const feat_csr_f64_t c = desc();
-          check(feat_sell_create_f32(Runtime::handle(), &c, p.get()), "exec32");
+          if(feat_sell_create_f32(Runtime::handle(), &c, p.get()) != FEAT_OK)
+          {
+            // FP32 CSR copy with 32-bit indices (rows too long for a SELL-64 plan)
+            if(_nnz >= (Index(1) << 32) || _cols >= (Index(1) << 32)) throw Error("exec32: matrix too large for u32 indices");
+            _csr32_val = DeviceArray<float>(_nnz);
+            _csr32_col = DeviceArray<std::uint32_t>(_nnz);
+            _csr32_rp = DeviceArray<std::uint32_t>(_rows + 1);
+            feat_handle_t h = Runtime::handle();
+            check(feat_convert_f64_f32(h, _csr32_val.get(), (const double*)val(), _nnz), "exec32 val");
+            check(feat_convert_u64_u32(h, _csr32_col.get(), (const std::uint64_t*)col_ind(), _nnz), "exec32 col");
+            check(feat_convert_u64_u32(h, _csr32_rp.get(), (const std::uint64_t*)row_ptr(), _rows + 1), "exec32 rowptr");
+            *p = feat_matrix_f32_t{FEAT_FMT_CSR, FEAT_IDX_U32, _rows, _cols, _nnz, _csr32_val.get(), _csr32_col.get(),
+                                   _csr32_rp.get(), nullptr, FEAT_VAL_NATIVE, 1.0f};
+          }
           _plan32 = p;
         }
         return *_plan32;
+      }
+
+      /// FP16-value SELL-64 plan (FP32 arithmetic, exact power-of-two scale); falls
+      /// back to exec32() when no SELL-64 plan can be built.
+      const feat_matrix_f32_t& exec16() const
+      {
+        if(!_plan16)
+        {
+          auto p = std::shared_ptr<feat_matrix_f32_t>(new feat_matrix_f32_t(), [](feat_matrix_f32_t* m) {
+            if(m->format == FEAT_FMT_SELL) feat_sell_destroy_f32(m);
+            delete m;
+          });
+          const feat_csr_f64_t c = desc();
+          if(feat_sell_create_f16(Runtime::handle(), &c, p.get()) != FEAT_OK)
+          {
+            _plan16 = std::shared_ptr<feat_matrix_f32_t>(std::shared_ptr<feat_matrix_f32_t>(), const_cast<feat_matrix_f32_t*>(&exec32()));
+            return *_plan16;
+          }
+          _plan16 = p;
+        }
+        return *_plan16;
       }
 
       /// Selects CSR (false) or the SELL-64 plan (true, default) for exec().

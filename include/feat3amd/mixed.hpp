@@ -1,0 +1,217 @@
+// feat3amd/mixed.hpp -- mixed-precision multigrid preconditioner.
+//
+// The ConvertPrecond pattern (solver/convert_precond.hpp:100-119): the FP64
+// defect is converted to FP32, an FP32 V-cycle is applied and the correction
+// is converted back.  Here the V-cycle runs on FP32 SELL-64 plans of the same
+// matrices (values rounded once), FP32 Jacobi diagonals (omega/a_ii computed
+// in FP64, rounded once) and FP32 vectors, while the coarse-grid solve stays
+// FP64 (the dense inverse of DirectCoarseSolver applied in FP64).  Used as the
+// preconditioner of an FP64 PCG, the outer residual, solution update and
+// convergence test stay FP64: iterative refinement with Krylov acceleration.
+#pragma once
+
+#include <feat3amd/solver.hpp>
+
+namespace feat3amd
+{
+  namespace Solver
+  {
+    template<typename Matrix_, typename Filter_, typename Transfer_>
+    class MixedMultiGrid : public SolverBase<typename Matrix_::VectorTypeL>,
+                           public FusedPrecond<typename Matrix_::VectorTypeL>
+    {
+    public:
+      typedef typename Matrix_::VectorTypeL VectorType;
+      typedef LAFEM::DenseVector<float, Index> Vector32;
+      typedef MultiGridHierarchy<Matrix_, Filter_, Transfer_> HierarchyType;
+      typedef Richardson<Matrix_, Filter_> RichType;
+      typedef JacobiPrecond<Matrix_, Filter_> JacType;
+      typedef DirectCoarseSolver<Matrix_, Filter_> DirectType;
+
+    private:
+      struct Level32
+      {
+        Vector32 rhs, sol, def, cor, tmp, inv_pre, inv_post;
+        int pre_steps = 0, post_steps = 0;
+        float omega_pre = 1.0f, omega_post = 1.0f;
+      };
+      std::shared_ptr<HierarchyType> _h;
+      std::vector<Level32> _l;
+      FusedWorkspace _fw;
+      enum class Coarse { direct, jacobi, richardson, none } _coarse = Coarse::none;
+      bool _ok = false;
+      bool _half = false;
+
+      // execution plan of a level's matrix: FP32 values or FP16 values (FP32 arithmetic)
+      const feat_matrix_f32_t& plan(const Matrix_& m) const { return _half ? m.exec16() : m.exec32(); }
+
+      static Vector32 to32(const VectorType& v)
+      {
+        Vector32 r(v.size());
+        check(feat_convert_f64_f32(Runtime::handle(), r.elements(), (const double*)v.elements(), v.size()), "to32");
+        return r;
+      }
+
+    public:
+      /// half = true stores the matrices of the cycle as FP16 (FP32 arithmetic)
+      explicit MixedMultiGrid(std::shared_ptr<HierarchyType> h, bool half = false) : _h(h), _half(half) {}
+      std::string name() const override { return _half ? "MultiGrid-V(FP16/FP32)" : "MultiGrid-V(FP32)"; }
+      bool fusable() const override { return _ok; }
+
+      void init_symbolic() override
+      {
+        const Index L = _h->size();
+        _fw.create(2 * L + 8);
+        _ok = true;
+        _l.assign(L, Level32());
+        for(Index i = 0; i < L; ++i)
+        {
+          auto& lv = _h->levels[i];
+          const Index n = lv.matrix->rows();
+          _l[i].rhs = Vector32(n);
+          _l[i].sol = Vector32(n);
+          _l[i].def = Vector32(n);
+          _l[i].cor = Vector32(n);
+          _l[i].tmp = Vector32(n);
+          if(i + 1 < L)
+          {
+            auto* pre = dynamic_cast<RichType*>(lv.pre.get());
+            auto* post = dynamic_cast<RichType*>(lv.post.get());
+            if(!pre || !post || !pre->fusable_jacobi() || !post->fusable_jacobi()) _ok = false;
+          }
+        }
+        auto* c = _h->levels[L - 1].coarse.get();
+        if(!c)
+          _coarse = Coarse::none;
+        else if(dynamic_cast<DirectType*>(c))
+          _coarse = Coarse::direct;
+        else if(dynamic_cast<JacType*>(c))
+          _coarse = Coarse::jacobi;
+        else if(auto* r = dynamic_cast<RichType*>(c))
+        {
+          _coarse = Coarse::richardson;
+          if(!r->fusable_jacobi()) _ok = false;
+        }
+        else
+          _ok = false;
+        if(!_ok) throw Error("MixedMultiGrid: needs Richardson(Jacobi) smoothers with fixed step counts");
+      }
+
+      /// Needs the hierarchy's init_numeric (the FP64 Jacobi diagonals) first.
+      void init_numeric() override
+      {
+        const Index L = _h->size();
+        for(Index i = 0; i < L; ++i)
+        {
+          auto& lv = _h->levels[i];
+          // FP32 execution plans of the level's matrices (values rounded once)
+          (void)plan(*lv.matrix);
+          if(i + 1 < L)
+          {
+            (void)plan(lv.transfer->get_mat_prol());
+            (void)plan(lv.transfer->get_mat_rest());
+            auto* pre = static_cast<RichType*>(lv.pre.get());
+            auto* post = static_cast<RichType*>(lv.post.get());
+            _l[i].inv_pre = to32(pre->fusable_jacobi()->inv_diag());
+            _l[i].inv_post = to32(post->fusable_jacobi()->inv_diag());
+            _l[i].pre_steps = int(pre->get_max_iter());
+            _l[i].post_steps = int(post->get_max_iter());
+            _l[i].omega_pre = float(pre->omega());
+            _l[i].omega_post = float(post->omega());
+          }
+        }
+        auto& lc = _h->levels[L - 1];
+        if(_coarse == Coarse::jacobi)
+          _l[L - 1].inv_pre = to32(static_cast<JacType*>(lc.coarse.get())->inv_diag());
+        else if(_coarse == Coarse::richardson)
+        {
+          auto* r = static_cast<RichType*>(lc.coarse.get());
+          _l[L - 1].inv_pre = to32(r->fusable_jacobi()->inv_diag());
+          _l[L - 1].pre_steps = int(r->get_max_iter());
+          _l[L - 1].omega_pre = float(r->omega());
+        }
+      }
+
+      Status apply(VectorType& cor, const VectorType& def) override
+      {
+        double* nrm = _fw.scalar(0);
+        check(feat_dot_f64(Runtime::handle(), nrm, (const double*)def.elements(), (const double*)def.elements(), def.size()),
+              "MixedMultiGrid::apply norm");
+        apply_fused(cor, def, nrm, nullptr);
+        return Status::success;
+      }
+
+      void apply_fused(VectorType& cor, const VectorType& def, const double* def_nrm2, double* gamma) override
+      {
+        const Index L = _h->size();
+        const Index c = L - 1;
+        feat_handle_t hd = Runtime::handle();
+        const feat_reduce_ws_t* ws = &_fw.ws;
+        auto nrm_of = [&](Index i) -> const double* { return i == 0 ? def_nrm2 : _fw.scalar(2 + i); };
+        check(feat_convert_f64_f32(hd, _l[0].rhs.elements(), (const double*)def.elements(), def.size()), "to FP32");
+        for(Index i = 0; i < c; ++i)
+        {
+          auto& lv = _h->levels[i];
+          Level32& l = _l[i];
+          const feat_matrix_f32_t& A = plan(*lv.matrix);
+          check(feat_rj_smooth_f32(hd, &A, l.inv_pre.elements(), lv.filter->mask(), l.pre_steps, l.omega_pre,
+                                   l.rhs.elements(), nrm_of(i), l.sol.elements(), l.tmp.elements(), l.def.elements(),
+                                   nullptr, ws),
+                "FP32 pre-smoothing");
+          const feat_matrix_f32_t& R = plan(lv.transfer->get_mat_rest());
+          check(feat_apply_masked_f32(hd, &R, _l[i + 1].rhs.elements(), l.def.elements(), _h->levels[i + 1].filter->mask(),
+                                      _fw.scalar(2 + i + 1), ws),
+                "FP32 restriction");
+        }
+        {
+          auto& lv = _h->levels[c];
+          Level32& l = _l[c];
+          const int n = int(lv.matrix->rows());
+          switch(_coarse)
+          {
+          case Coarse::direct:
+            check(feat_dense_gemv_mixed(hd, n, static_cast<DirectType*>(lv.coarse.get())->inverse(), l.rhs.elements(),
+                                        l.sol.elements(), lv.filter->mask()),
+                  "FP64 coarse solve");
+            break;
+          case Coarse::jacobi:
+          {
+            const feat_matrix_f32_t& A = plan(*lv.matrix);
+            check(feat_rj_smooth_f32(hd, &A, l.inv_pre.elements(), lv.filter->mask(), 1, 1.0f, l.rhs.elements(), nullptr,
+                                     l.sol.elements(), l.tmp.elements(), nullptr, nullptr, ws),
+                  "FP32 coarse Jacobi");
+            break;
+          }
+          case Coarse::richardson:
+          {
+            const feat_matrix_f32_t& A = plan(*lv.matrix);
+            check(feat_rj_smooth_f32(hd, &A, l.inv_pre.elements(), lv.filter->mask(), l.pre_steps, l.omega_pre,
+                                     l.rhs.elements(), nrm_of(c), l.sol.elements(), l.tmp.elements(), nullptr, nullptr, ws),
+                  "FP32 coarse Richardson");
+            break;
+          }
+          default:
+            throw Error("MixedMultiGrid: coarse solver not supported");
+          }
+        }
+        for(Index k = c; k-- > 0;)
+        {
+          auto& lv = _h->levels[k];
+          Level32& l = _l[k];
+          const feat_matrix_f32_t& P = plan(lv.transfer->get_mat_prol());
+          check(feat_prol_add_f32(hd, &P, l.sol.elements(), _l[k + 1].sol.elements(), lv.filter->mask()), "FP32 prolongation");
+          const feat_matrix_f32_t& A = plan(*lv.matrix);
+          double* dn = _fw.scalar(2 + L + k);
+          check(feat_defect_f32(hd, &A, l.def.elements(), l.rhs.elements(), l.sol.elements(), lv.filter->mask(), dn, ws),
+                "FP32 defect");
+          check(feat_rj_smooth_f32(hd, &A, l.inv_post.elements(), lv.filter->mask(), l.post_steps, l.omega_post,
+                                   l.def.elements(), dn, l.cor.elements(), l.tmp.elements(), nullptr, l.sol.elements(), ws),
+                "FP32 post-smoothing");
+        }
+        check(feat_convert_dot_f32_f64(hd, (double*)cor.elements(), _l[0].sol.elements(), (const double*)def.elements(),
+                                       cor.size(), gamma, ws),
+              "to FP64");
+      }
+    };
+  } // namespace Solver
+} // namespace feat3amd

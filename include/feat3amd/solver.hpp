@@ -460,9 +460,23 @@ namespace feat3amd
       void done() {}
     };
 
+    /// A preconditioner that runs on device scalars: cor = M^-1 def, where
+    /// *def_nrm2 (device) holds |def|^2 and, when gamma != nullptr, the device
+    /// scalar *gamma receives cor . def.  PCG uses it to keep its loop free of
+    /// host round trips.
+    template<typename Vector_>
+    class FusedPrecond
+    {
+    public:
+      virtual ~FusedPrecond() {}
+      virtual bool fusable() const = 0;
+      virtual void apply_fused(Vector_& cor, const Vector_& def, const double* def_nrm2, double* gamma) = 0;
+    };
+
     /// MultiGrid (solver/multigrid.hpp:984-2000), V-cycle with fixed coarse-grid correction.
     template<typename Matrix_, typename Filter_, typename Transfer_>
-    class MultiGrid : public SolverBase<typename Matrix_::VectorTypeL>
+    class MultiGrid : public SolverBase<typename Matrix_::VectorTypeL>,
+                      public FusedPrecond<typename Matrix_::VectorTypeL>
     {
     public:
       typedef typename Matrix_::VectorTypeL VectorType;
@@ -505,7 +519,7 @@ namespace feat3amd
           _coarse_kind = Coarse::other;
         if(_coarse_kind == Coarse::other) _fusable = false;
       }
-      bool fusable() const { return _fusable; }
+      bool fusable() const override { return _fusable; }
       const FusedWorkspace& workspace() const { return _fw; }
 
       /// MultiGrid::apply (multigrid.hpp:1270-1330)
@@ -529,7 +543,7 @@ namespace feat3amd
 
       /// Fused V-cycle: cor = MG(def) where *def_nrm2 (device) holds |def|^2;
       /// when gamma != nullptr, *gamma = cor . def (device).
-      void apply_fused(VectorType& cor, const VectorType& def, const double* def_nrm2, double* gamma)
+      void apply_fused(VectorType& cor, const VectorType& def, const double* def_nrm2, double* gamma) override
       {
         const Index L = _h->size();
         const Index c = L - 1;
@@ -668,7 +682,7 @@ namespace feat3amd
       typedef typename Matrix_::VectorTypeL VectorType;
       typedef typename Matrix_::DataType DataType;
       typedef SolverBase<VectorType> PrecondType;
-      typedef MultiGrid<Matrix_, Filter_, LAFEM::Transfer<Matrix_>> MGType;
+      typedef FusedPrecond<VectorType> MGType;
 
     private:
       const Matrix_& _matrix;

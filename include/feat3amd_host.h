@@ -43,6 +43,9 @@ typedef struct {
   uint64_t max_iter;
   int force_generic;      /* 1: reference op-by-op path instead of the fused one */
   int csr_only;           /* 1: fused kernels on the CSR arrays instead of SELL-64 plans */
+  int precision;          /* 0: FP64 V-cycle; 1: FP32 V-cycle inside the FP64 PCG
+                             (ConvertPrecond, solver/convert_precond.hpp:100-119);
+                             2: the FP32 cycle on FP16-valued matrices */
 } feat_mgpcg_cfg_t;
 
 typedef struct {          /* one level, host arrays; P maps the next coarser level onto this one */

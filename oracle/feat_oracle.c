@@ -13,6 +13,7 @@
 #include "feat_oracle.h"
 
 #include <float.h>
+#include <immintrin.h>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -743,4 +744,261 @@ int orc_richardson_apply(const orc_csr_t* A, const uint64_t* filt, uint64_t nfil
                          double* x, const double* b, double* hist, orc_result_t* res)
 {
   return richardson_generic(A, filt, nfilt, precond, omega_jac, omega_rich, it, x, b, hist, res, 0);
+}
+
+/* ------------------------------------------------------------------------- */
+/* Mixed precision: FP64 PCG preconditioned by an FP32 V-cycle                */
+/* (the ConvertPrecond pattern, solver/convert_precond.hpp:100-119).  The     */
+/* FP32 cycle keeps the op order of the FP64 one above; matrices and the      */
+/* Jacobi diagonals (omega/a_ii in FP64) are rounded to FP32 once; the coarse */
+/* solve applies the FP64 dense inverse to the FP32 right-hand side.          */
+/* ------------------------------------------------------------------------- */
+typedef struct {
+  uint64_t rows, cols;
+  float* val;
+  const uint64_t* col;
+  const uint64_t* rowptr;
+  float scale;
+} csr32_t;
+
+typedef struct {
+  csr32_t A, P, R;
+  const uint64_t* filt;
+  uint64_t nfilt;
+  char* mask;
+  float *inv, *rhs, *sol, *def, *cor, *xa, *xb;
+  double* cinv;   /* coarse: FP64 dense inverse */
+} lvl32_t;
+
+/* half = 0: values rounded to FP32.  half = 1: FP16 values half(float(a)*2^-e)
+ * with 2^(e-1) <= max|float(a)| < 2^e, kept here as the FP32 value of the FP16
+ * number; row sums are multiplied by 2^e (F16C conversions, round to nearest even). */
+static csr32_t csr_to32(const orc_csr_t* m, int half)
+{
+  csr32_t r;
+  r.rows = m->rows;
+  r.cols = m->cols;
+  r.col = m->col;
+  r.rowptr = m->rowptr;
+  r.scale = 1.0f;
+  r.val = (float*)malloc(sizeof(float) * (m->nnz ? m->nnz : 1));
+  float inv = 1.0f;
+  if(half)
+  {
+    float mx = 0.0f;
+    for(uint64_t i = 0; i < m->nnz; ++i)
+    {
+      const float a = fabsf((float)m->val[i]);
+      if(a > mx) mx = a;
+    }
+    if(mx > 0.0f && isfinite(mx))
+    {
+      int e = 0;
+      (void)frexpf(mx, &e);
+      r.scale = ldexpf(1.0f, e);
+      inv = ldexpf(1.0f, -e);
+    }
+  }
+  for(uint64_t i = 0; i < m->nnz; ++i)
+  {
+    const float f = (float)m->val[i];
+    r.val[i] = half ? _cvtsh_ss(_cvtss_sh(f * inv, 0)) : f;
+  }
+  return r;
+}
+
+static float row_sum32(const csr32_t* A, uint64_t row, const float* x)
+{
+  float s = 0.0f;
+  for(uint64_t i = A->rowptr[row]; i < A->rowptr[row + 1]; ++i)
+    s = fmaf(A->val[i], x[A->col[i]], s);
+  return s * A->scale;
+}
+
+static double nrm2sq32(const float* v, uint64_t n)
+{
+  double s = 0.0;
+  for(uint64_t i = 0; i < n; ++i) s = fma((double)v[i], (double)v[i], s);
+  return s;
+}
+
+static int skips32(double nrm2)
+{
+  const double d = sqrt(nrm2);
+  return !isfinite(d) || d <= (double)FLT_EPSILON * (double)FLT_EPSILON;
+}
+
+/* Richardson(Jacobi) in FP32 from x = 0 (the fused smoother's semantics):
+ * mode 0: x = result; mode 1: also d_out = last filtered defect; mode 2: acc += result */
+static void smooth32(lvl32_t* L, int steps, float omega, const float* rhs, double rhs_nrm2, int use_skip,
+                     float* x, float* d_out, float* acc)
+{
+  const uint64_t n = L->A.rows;
+  const int skip = use_skip ? skips32(rhs_nrm2) : 0;
+  float* cur = L->xa;
+  float* nxt = L->xb;
+  for(uint64_t i = 0; i < n; ++i)
+  {
+    float c = L->inv[i] * rhs[i];
+    if(L->mask[i]) c = 0.0f;
+    cur[i] = skip ? 0.0f : fmaf(omega, c, 0.0f);
+  }
+  for(int k = 1; k < steps; ++k)
+  {
+    for(uint64_t i = 0; i < n; ++i)
+    {
+      float d = fmaf(row_sum32(&L->A, i, cur), -1.0f, rhs[i]);
+      if(L->mask[i]) d = 0.0f;
+      float c = L->inv[i] * d;
+      if(L->mask[i]) c = 0.0f;
+      nxt[i] = skip ? 0.0f : fmaf(omega, c, cur[i]);
+    }
+    float* t = cur; cur = nxt; nxt = t;
+  }
+  if(d_out)
+    for(uint64_t i = 0; i < n; ++i)
+    {
+      float d = fmaf(row_sum32(&L->A, i, cur), -1.0f, rhs[i]);
+      d_out[i] = L->mask[i] ? 0.0f : d;
+    }
+  for(uint64_t i = 0; i < n; ++i)
+  {
+    if(acc) acc[i] = fmaf(1.0f, cur[i], acc[i]);
+    else x[i] = cur[i];
+  }
+}
+
+typedef struct {
+  lvl32_t* L;
+  int nl;
+  const orc_mg_cfg_t* cfg;
+} mg32_t;
+
+static void mg32_init(mg32_t* m, const orc_mg_level_t* lv, int nl, const orc_mg_cfg_t* cfg, int half)
+{
+  m->nl = nl;
+  m->cfg = cfg;
+  m->L = (lvl32_t*)calloc((size_t)nl, sizeof(lvl32_t));
+  for(int i = 0; i < nl; ++i)
+  {
+    lvl32_t* L = &m->L[i];
+    const uint64_t n = lv[i].A.rows;
+    L->A = csr_to32(&lv[i].A, half);
+    if(i + 1 < nl)
+    {
+      L->P = csr_to32(&lv[i].P, half);
+      L->R = csr_to32(&lv[i].R, half);
+    }
+    L->filt = lv[i].filt;
+    L->nfilt = lv[i].nfilt;
+    L->mask = (char*)calloc(n ? n : 1, 1);
+    for(uint64_t k = 0; k < lv[i].nfilt; ++k) L->mask[lv[i].filt[k]] = 1;
+    const double om = (i + 1 == nl && cfg->coarse == ORC_COARSE_JACOBI) ? cfg->coarse_omega : cfg->omega_jac;
+    double* inv64 = jacobi_init(&lv[i].A, om);
+    L->inv = (float*)malloc(sizeof(float) * (n ? n : 1));
+    for(uint64_t k = 0; k < n; ++k) L->inv[k] = (float)inv64[k];
+    free(inv64);
+    float** vecs[6] = {&L->rhs, &L->sol, &L->def, &L->cor, &L->xa, &L->xb};
+    for(int k = 0; k < 6; ++k) *vecs[k] = (float*)calloc(n ? n : 1, sizeof(float));
+    if(i + 1 == nl && cfg->coarse == ORC_COARSE_DIRECT)
+      L->cinv = dense_filtered_inverse(&lv[i].A, lv[i].filt, lv[i].nfilt);
+  }
+}
+
+static void mg32_free(mg32_t* m)
+{
+  for(int i = 0; i < m->nl; ++i)
+  {
+    lvl32_t* L = &m->L[i];
+    free(L->A.val);
+    if(i + 1 < m->nl) { free(L->P.val); free(L->R.val); }
+    free(L->mask); free(L->inv); free(L->rhs); free(L->sol); free(L->def); free(L->cor);
+    free(L->xa); free(L->xb); free(L->cinv);
+  }
+  free(m->L);
+}
+
+/* cor = MG32(def): the FP64 defect is rounded to FP32, the FP32 cycle runs,
+ * the correction is widened back to FP64.  def_nrm2 is |def|^2 (FP64). */
+static void mg32_apply(mg32_t* m, double* cor, const double* def, double def_nrm2)
+{
+  const int nl = m->nl, c = nl - 1;
+  const orc_mg_cfg_t* cfg = m->cfg;
+  const float om = (float)cfg->omega_rich;
+  lvl32_t* L0 = &m->L[0];
+  for(uint64_t i = 0; i < L0->A.rows; ++i) L0->rhs[i] = (float)def[i];
+  double nrm = def_nrm2;
+  for(int i = 0; i < c; ++i)
+  {
+    lvl32_t* L = &m->L[i];
+    lvl32_t* Lc = &m->L[i + 1];
+    smooth32(L, cfg->nu_pre, om, L->rhs, nrm, 1, L->sol, L->def, NULL);
+    for(uint64_t r = 0; r < L->R.rows; ++r)
+    {
+      float v = fmaf(row_sum32(&L->R, r, L->def), 1.0f, 0.0f);
+      Lc->rhs[r] = Lc->mask[r] ? 0.0f : v;
+    }
+    nrm = nrm2sq32(Lc->rhs, Lc->A.rows);
+  }
+  lvl32_t* Lc = &m->L[c];
+  const uint64_t nc = Lc->A.rows;
+  if(cfg->coarse == ORC_COARSE_DIRECT)
+  {
+    for(uint64_t r = 0; r < nc; ++r)
+    {
+      double s = 0.0;
+      for(uint64_t k = 0; k < nc; ++k) s = fma(Lc->cinv[r * nc + k], (double)Lc->rhs[k], s);
+      Lc->sol[r] = Lc->mask[r] ? 0.0f : (float)s;
+    }
+  }
+  else if(cfg->coarse == ORC_COARSE_JACOBI)
+    smooth32(Lc, 1, 1.0f, Lc->rhs, 0.0, 0, Lc->sol, NULL, NULL);
+  else
+    smooth32(Lc, cfg->nu_pre, om, Lc->rhs, nrm, 1, Lc->sol, NULL, NULL);
+  for(int k = c - 1; k >= 0; --k)
+  {
+    lvl32_t* L = &m->L[k];
+    const uint64_t n = L->A.rows;
+    for(uint64_t r = 0; r < n; ++r)
+    {
+      float v = fmaf(row_sum32(&L->P, r, m->L[k + 1].sol), 1.0f, 0.0f);
+      if(L->mask[r]) v = 0.0f;
+      L->sol[r] = fmaf(1.0f, v, L->sol[r]);
+    }
+    for(uint64_t r = 0; r < n; ++r)
+    {
+      float d = fmaf(row_sum32(&L->A, r, L->sol), -1.0f, L->rhs[r]);
+      L->def[r] = L->mask[r] ? 0.0f : d;
+    }
+    smooth32(L, cfg->nu_post, om, L->def, nrm2sq32(L->def, n), 1, NULL, NULL, L->sol);
+  }
+  for(uint64_t i = 0; i < L0->A.rows; ++i) cor[i] = (double)L0->sol[i];
+}
+
+typedef struct {
+  mg32_t mg;
+} mg32ctx_t;
+
+static void mg32_precond(void* ctx, double* cor, const double* def)
+{
+  mg32_t* m = &((mg32ctx_t*)ctx)->mg;
+  const uint64_t n = m->L[0].A.rows;
+  mg32_apply(m, cor, def, orc_dot_f64(def, def, n));
+}
+
+int orc_mg_pcg_solve_mixed(const orc_mg_level_t* levels, int nlevels, const orc_mg_cfg_t* cfg,
+                           const orc_iter_cfg_t* it, int half, double* x, const double* b,
+                           double* hist, orc_result_t* res)
+{
+  const orc_csr_t* A = &levels[0].A;
+  const uint64_t n = A->rows;
+  mg32ctx_t ctx;
+  mg32_init(&ctx.mg, levels, nlevels, cfg, half);
+  double* r = (double*)malloc(sizeof(double) * (n ? n : 1));
+  csr_apply(A, r, -1.0, x, 1.0, b);
+  orc_unit_filter_def_f64(r, levels[0].filt, levels[0].nfilt);
+  int st = pcg_intern(A, levels[0].filt, levels[0].nfilt, mg32_precond, &ctx, it, x, r, hist, res);
+  free(r);
+  mg32_free(&ctx.mg);
+  return st;
 }

@@ -43,7 +43,7 @@ void orc_bcsr_apply_f64(int bh, int bw, double* r, double a, const double* x, do
                         const double* y, const double* val, const uint64_t* col,
                         const uint64_t* rowptr, uint64_t rows);
 
-/* ---- BLAS-1 (lafem/arch/*_generic.hpp) ---- */
+/* ---- BLAS-1 (lafem/arch/<op>_generic.hpp) ---- */
 void   orc_axpy_f64(double* r, double a, const double* x, const double* y, uint64_t n);
 double orc_dot_f64(const double* x, const double* y, uint64_t n);
 double orc_triple_dot_f64(const double* x, const double* y, const double* z, uint64_t n);
@@ -119,6 +119,15 @@ typedef struct {
 int orc_mg_pcg_solve(const orc_mg_level_t* levels, int nlevels, const orc_mg_cfg_t* mg,
                      const orc_iter_cfg_t* it, double* x, const double* b,
                      double* def_hist, orc_result_t* res);
+
+/* The same PCG with an FP32 V-cycle as preconditioner (the ConvertPrecond
+ * pattern, solver/convert_precond.hpp:100-119): FP64 outer residual, update and
+ * convergence test; FP32 matrices (half != 0: FP16 values with a power-of-two
+ * scale, FP32 arithmetic), Jacobi diagonals, smoothing and transfers; FP64
+ * coarse solve. */
+int orc_mg_pcg_solve_mixed(const orc_mg_level_t* levels, int nlevels, const orc_mg_cfg_t* mg,
+                           const orc_iter_cfg_t* it, int half, double* x, const double* b,
+                           double* def_hist, orc_result_t* res);
 
 /* Stand-alone iterative solvers on one CSR matrix with a unit filter.
  * precond: 0 = none, 1 = Jacobi(omega).  Variant of Solver::solve

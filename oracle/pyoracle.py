@@ -134,8 +134,11 @@ class Oracle:
         L.orc_iter_cfg_default.argtypes = [C.POINTER(_IterCfgS)]
         L.orc_mg_pcg_solve.restype = C.c_int
         L.orc_mg_pcg_solve.argtypes = [C.POINTER(_LevelS), C.c_int, C.POINTER(_MgCfgS),
-                                       C.POINTER(_IterCfgS), _f64p, _f64p, _vp,
-                                       C.POINTER(_ResultS)]
+                                       C.POINTER(_IterCfgS), _f64p, _f64p, _vp, C.POINTER(_ResultS)]
+        L.orc_mg_pcg_solve_mixed.restype = C.c_int
+        L.orc_mg_pcg_solve_mixed.argtypes = [C.POINTER(_LevelS), C.c_int, C.POINTER(_MgCfgS),
+                                             C.POINTER(_IterCfgS), C.c_int, _f64p, _f64p, _vp,
+                                             C.POINTER(_ResultS)]
         for fn in ("orc_pcg_solve", "orc_pcg_apply"):
             f = getattr(L, fn)
             f.restype = C.c_int
@@ -267,16 +270,21 @@ class Oracle:
         return c
 
     def mg_pcg_solve(self, levels, x0, b, nu_pre=4, nu_post=4, omega_jac=0.5, omega_rich=1.0,
-                     coarse=COARSE_DIRECT, coarse_omega=1.0, tol_rel=1e-8, max_iter=100, **kw):
+                     coarse=COARSE_DIRECT, coarse_omega=1.0, tol_rel=1e-8, max_iter=100,
+                     precision="f64", **kw):
         arr, keep = _levels_s(levels)
+        if precision == "f64":
+            fn = self.L.orc_mg_pcg_solve
+        else:
+            half = 1 if precision == "f16" else 0
+            fn = lambda *a: self.L.orc_mg_pcg_solve_mixed(*a[:4], half, *a[4:])
         mg = _MgCfgS(nu_pre, nu_post, omega_jac, omega_rich, coarse, coarse_omega)
         it = self.iter_cfg(tol_rel=tol_rel, max_iter=max_iter, **kw)
         x = np.ascontiguousarray(x0, np.float64).copy()
         hist = np.full(max_iter + 1, np.nan)
         res = _ResultS()
-        st = self.L.orc_mg_pcg_solve(arr, len(levels), C.byref(mg), C.byref(it), x,
-                                     np.ascontiguousarray(b, np.float64), hist.ctypes.data,
-                                     C.byref(res))
+        st = fn(arr, len(levels), C.byref(mg), C.byref(it), x, np.ascontiguousarray(b, np.float64),
+                hist.ctypes.data, C.byref(res))
         return dict(x=x, status=STATUS[st], iters=int(res.num_iter), def_init=res.def_init,
                     def_final=res.def_final, hist=hist[:int(res.num_iter) + 1])
 

@@ -282,3 +282,34 @@ def test_mgpcg_failure_status(feat, torch_gpu):
     r = s.solve(x, b)
     s.close()
     assert r["status"] == "max_iter" and r["iters"] == 2
+
+
+@pytest.mark.parametrize("precision", ["f32", "f16"])
+@pytest.mark.parametrize("dim,lf,lc,coarse", [(2, 7, 1, "direct"), (3, 5, 1, "direct"), (3, 6, 2, "jacobi"),
+                                              (2, 6, 0, "smoother")])
+def test_mixed_precision_vs_oracle(feat, torch_gpu, oracle, dim, lf, lc, coarse, precision):
+    """FP32 V-cycle (FP32 or FP16 matrix values) inside FP64 PCG: iterations within
+    +-1 of the oracle's same method, solution within 1e-10 (FP32) / 1e-9 (FP16) of the
+    FP64 oracle solution, FP64-verified relative residual below the tolerance."""
+    torch = torch_gpu
+    levels = q1_levels(feat, dim, lf, lc)
+    n = levels[0]["n"]
+    b = feat.rhs_random(28054777172512, n)
+    filt = levels[0]["filt"].astype(np.int64)
+    b[filt] = 0.0
+    oc = {"direct": COARSE_DIRECT, "jacobi": COARSE_JACOBI, "smoother": COARSE_SMOOTHER}[coarse]
+    o64 = oracle.mg_pcg_solve(levels, np.zeros(n), b, coarse=oc)
+    o32 = oracle.mg_pcg_solve(levels, np.zeros(n), b, coarse=oc, precision=precision)
+    s = feat.MgPcg(feat.mgpcg_cfg(coarse=coarse, precision=precision), levels=levels)
+    x = torch.zeros(n, dtype=torch.float64, device="cuda")
+    r = s.solve(x, dev(torch, b))
+    s.close()
+    xg = host(x)
+    assert r["status"] == "success"
+    assert abs(r["iters"] - o32["iters"]) <= 1
+    tol = 1e-10 if precision == "f32" else 1e-9
+    assert np.linalg.norm(xg - o32["x"]) <= tol * np.linalg.norm(o32["x"])
+    assert np.linalg.norm(xg - o64["x"]) <= tol * np.linalg.norm(o64["x"])
+    res = b - oracle.csr_apply(levels[0]["A"], xg)
+    res[filt] = 0.0
+    assert np.linalg.norm(res) <= 1.01e-8 * np.linalg.norm(b)
