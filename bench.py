@@ -133,6 +133,24 @@ def main():
     assert res["status"] == "success", res
     t_mean = float(np.mean(times))
 
+    # ---- the same solver launching every kernel eagerly (no recorded graphs)
+    se = feat.MgPcg(feat.mgpcg_cfg(nu_pre=4, nu_post=4, omega_jac=0.5, omega_rich=1.0, coarse="direct",
+                                   tol_rel=1e-8, max_iter=100, graphs=False), q1=(DIM, LF, LC))
+    xe = torch.zeros(n, dtype=torch.float64, device=dev)
+    te = []
+    for k in range(args.warmup + args.steps):
+        xe.zero_()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        re_ = se.solve(xe, b)
+        torch.cuda.synchronize()
+        if k >= args.warmup:
+            te.append(time.perf_counter() - t)
+    se.close()
+    assert re_["iters"] == res["iters"] and bool(torch.equal(xe, x)), "graph and eager solves differ"
+    t_eager = float(np.mean(te))
+    log(f"eager launches: {t_eager * 1e3:.2f} ms per solve (graphs {t_mean * 1e3:.2f} ms), identical result")
+
     # ---- the mixed-precision forms: FP32 V-cycle (FP32 or FP16 matrix values)
     # inside the FP64 PCG; outer residual, update, convergence test and coarse solve FP64
     kk, _ = feat.libs()
@@ -230,6 +248,7 @@ def main():
                         "Richardson(Jacobi 0.5), levels 7..1, exact coarse solve, tol 1e-8",
             "dofs": n, "nnz": nnz, "levels": LF - LC + 1, "iters": res["iters"],
             "final_rel_defect": res["def_final"] / res["def_init"],
+            "eager_launch_s_per_solve": t_eager,
             "parallelism": "single GPU",
         },
         "mixed_precision": {"form": "FP32 V-cycle on FP32 / FP16-valued SELL-64 plans (FP64 coarse solve) "

@@ -64,7 +64,7 @@ class ReduceWs(C.Structure):
 class MgPcgCfg(C.Structure):
     _fields_ = [("nu_pre", _i), ("nu_post", _i), ("omega_jac", _d), ("omega_rich", _d),
                 ("coarse", _i), ("coarse_omega", _d), ("tol_rel", _d), ("max_iter", _u64),
-                ("force_generic", _i), ("csr_only", _i), ("precision", _i)]
+                ("force_generic", _i), ("csr_only", _i), ("precision", _i), ("no_graphs", _i)]
 
 
 class LevelHost(C.Structure):
@@ -83,6 +83,14 @@ KERNEL_API = [
     ("feat_handle_destroy", _i, [_vp]),
     ("feat_handle_stream", _i, [_vp, C.POINTER(_vp)]),
     ("feat_sync", _i, [_vp]),
+    ("feat_stream_create", _i, [C.POINTER(_vp)]),
+    ("feat_stream_destroy", _i, [_vp]),
+    ("feat_handle_wait", _i, [_vp, _vp]),
+    ("feat_graph_begin", _i, [_vp]),
+    ("feat_graph_end", _i, [_vp, C.POINTER(_vp)]),
+    ("feat_graph_launch", _i, [_vp, _vp]),
+    ("feat_graph_destroy", _i, [_vp]),
+    ("feat_copy_scalar_to_host", _i, [_vp, _vp, _vp]),
     ("feat_malloc", _i, [C.POINTER(_vp), C.c_size_t]),
     ("feat_free", _i, [_vp]),
     ("feat_host_malloc", _i, [C.POINTER(_vp), C.c_size_t]),
@@ -274,9 +282,10 @@ def rhs_random(seed, n, a=-1.0, b=1.0):
 
 def mgpcg_cfg(nu_pre=4, nu_post=4, omega_jac=0.5, omega_rich=1.0, coarse="direct",
               coarse_omega=1.0, tol_rel=1e-8, max_iter=100, force_generic=False, csr_only=False,
-              precision="f64"):
+              precision="f64", graphs=True):
     return MgPcgCfg(nu_pre, nu_post, omega_jac, omega_rich, COARSE[coarse], coarse_omega, tol_rel,
-                    max_iter, int(force_generic), int(csr_only), {"f64": 0, "f32": 1, "f16": 2}[precision])
+                    max_iter, int(force_generic), int(csr_only), {"f64": 0, "f32": 1, "f16": 2}[precision],
+                    int(not graphs))
 
 
 def csr_matrix(val, col, rowptr, rows, cols):

@@ -841,11 +841,18 @@ __global__ void k_copy_scalar(double* dst, const double* src)
 // ===========================================================================
 // handle
 // ===========================================================================
+struct feat_graph_s
+{
+  hipGraph_t graph;
+  hipGraphExec_t exec;
+};
+
 struct feat_handle_s
 {
   int device;
   hipStream_t stream;
   bool own_stream;
+  hipEvent_t event;   // for feat_handle_wait
   double* partials;   // 2 * kMaxGrid
   uint32_t* counter;
   double* scal;       // scratch device scalars
@@ -901,6 +908,7 @@ int feat_handle_create(feat_handle_t* out, int device, feat_stream_t stream)
   if(!rc) rc = check_hip(hipMalloc(&h->counter, 64), "hipMalloc(counter)");
   if(!rc) rc = check_hip(hipMalloc(&h->scal, sizeof(double) * 16), "hipMalloc(scal)");
   if(!rc) rc = check_hip(hipHostMalloc(&h->host, sizeof(double) * 16, 0), "hipHostMalloc");
+  if(!rc) rc = check_hip(hipEventCreateWithFlags(&h->event, hipEventDisableTiming), "hipEventCreate");
   if(!rc) rc = check_hip(hipMemsetAsync(h->counter, 0, 64, h->stream), "hipMemset");
   if(!rc) rc = check_hip(hipStreamSynchronize(h->stream), "hipStreamSynchronize");
   if(rc)
@@ -921,9 +929,78 @@ int feat_handle_destroy(feat_handle_t h)
   if(h->counter) (void)hipFree(h->counter);
   if(h->scal) (void)hipFree(h->scal);
   if(h->host) (void)hipHostFree(h->host);
+  if(h->event) (void)hipEventDestroy(h->event);
   if(h->own_stream && h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
   return FEAT_OK;
+}
+
+int feat_stream_create(feat_stream_t* s)
+{
+  if(!s) return fail(FEAT_ERR_INVALID, "feat_stream_create: null pointer");
+  hipStream_t st = nullptr;
+  int rc = check_hip(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "hipStreamCreateWithFlags");
+  if(!rc) *s = st;
+  return rc;
+}
+
+int feat_stream_destroy(feat_stream_t s)
+{
+  return s ? check_hip(hipStreamDestroy(hipStream_t(s)), "hipStreamDestroy") : FEAT_OK;
+}
+
+int feat_handle_wait(feat_handle_t waiter, feat_handle_t signaler)
+{
+  if(!waiter || !signaler) return fail(FEAT_ERR_INVALID, "feat_handle_wait: null handle");
+  if(waiter->stream == signaler->stream) return FEAT_OK;
+  int rc = check_hip(hipEventRecord(signaler->event, signaler->stream), "hipEventRecord");
+  if(!rc) rc = check_hip(hipStreamWaitEvent(waiter->stream, signaler->event, 0), "hipStreamWaitEvent");
+  return rc;
+}
+
+int feat_graph_begin(feat_handle_t h)
+{
+  if(!h || !h->stream) return fail(FEAT_ERR_INVALID, "feat_graph_begin: needs a handle on a non-null stream");
+  return check_hip(hipStreamBeginCapture(h->stream, hipStreamCaptureModeRelaxed), "hipStreamBeginCapture");
+}
+
+int feat_graph_end(feat_handle_t h, feat_graph_t* g)
+{
+  if(!h || !g) return fail(FEAT_ERR_INVALID, "feat_graph_end: null argument");
+  *g = nullptr;
+  hipGraph_t graph = nullptr;
+  int rc = check_hip(hipStreamEndCapture(h->stream, &graph), "hipStreamEndCapture");
+  if(rc) return rc;
+  hipGraphExec_t exec = nullptr;
+  rc = check_hip(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+  if(rc)
+  {
+    (void)hipGraphDestroy(graph);
+    return rc;
+  }
+  *g = new feat_graph_s{graph, exec};
+  return FEAT_OK;
+}
+
+int feat_graph_launch(feat_handle_t h, feat_graph_t g)
+{
+  if(!h || !g) return fail(FEAT_ERR_INVALID, "feat_graph_launch: null argument");
+  return check_hip(hipGraphLaunch(g->exec, h->stream), "hipGraphLaunch");
+}
+
+int feat_graph_destroy(feat_graph_t g)
+{
+  if(!g) return FEAT_OK;
+  (void)hipGraphExecDestroy(g->exec);
+  (void)hipGraphDestroy(g->graph);
+  delete g;
+  return FEAT_OK;
+}
+
+int feat_copy_scalar_to_host(feat_handle_t h, const double* dev, double* host_pinned)
+{
+  if(!h || !dev || !host_pinned) return fail(FEAT_ERR_INVALID, "feat_copy_scalar_to_host: null argument");
+  return check_hip(hipMemcpyAsync(host_pinned, dev, sizeof(double), hipMemcpyDeviceToHost, h->stream), "hipMemcpyAsync");
 }
 
 int feat_handle_stream(feat_handle_t h, feat_stream_t* s)

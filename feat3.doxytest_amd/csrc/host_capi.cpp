@@ -178,6 +178,7 @@ static void build_solver(feat_mgpcg_s* s, const feat_mgpcg_cfg_t* cfg)
   s->pcg->set_tol_rel(cfg->tol_rel);
   s->pcg->set_max_iter(cfg->max_iter);
   s->pcg->set_force_generic(cfg->force_generic != 0);
+  s->pcg->set_use_graphs(cfg->no_graphs == 0);
   s->hier->init();
   s->pcg->init();
 }

@@ -53,6 +53,22 @@ int feat_handle_create(feat_handle_t* h, int device, feat_stream_t stream);
 int feat_handle_destroy(feat_handle_t h);
 int feat_handle_stream(feat_handle_t h, feat_stream_t* stream);
 int feat_sync(feat_handle_t h);
+/* Streams, ordering and graphs (no reference counterpart: the reference's kernels
+ * are synchronous).  A handle on a non-null stream can record a HIP graph of the
+ * calls made on it between feat_graph_begin and feat_graph_end (setup time), and
+ * replay it with feat_graph_launch.  Calls made while recording must not
+ * synchronise or allocate. */
+typedef struct feat_graph_s* feat_graph_t;
+int feat_stream_create(feat_stream_t* s);    /* non-blocking, default priority */
+int feat_stream_destroy(feat_stream_t s);
+/* `waiter`'s stream waits for the work issued so far on `signaler`'s stream */
+int feat_handle_wait(feat_handle_t waiter, feat_handle_t signaler);
+int feat_graph_begin(feat_handle_t h);
+int feat_graph_end(feat_handle_t h, feat_graph_t* g);
+int feat_graph_launch(feat_handle_t h, feat_graph_t g);
+int feat_graph_destroy(feat_graph_t g);
+/* stream-ordered copy of one device scalar into pinned host memory (recordable) */
+int feat_copy_scalar_to_host(feat_handle_t h, const double* dev, double* host_pinned);
 /* device memory (MemoryPool::allocate_memory/release_memory, util/memory_pool.hpp:79-147) */
 int feat_malloc(void** p, size_t bytes);
 int feat_free(void* p);

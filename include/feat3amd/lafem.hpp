@@ -56,11 +56,29 @@ namespace feat3amd
         slot() = nullptr;
       }
     }
+    /// handle every container/solver call of this thread currently launches on
     static feat_handle_t handle()
     {
+      if(override_slot()) return override_slot();
       if(!slot()) initialize(0);
       return slot();
     }
+    static feat_handle_t& override_slot()
+    {
+      static thread_local feat_handle_t h = nullptr;
+      return h;
+    }
+    /// Scope in which this thread's calls launch on `h` (graph recording on a solver stream).
+    class Use
+    {
+      feat_handle_t _prev;
+
+    public:
+      explicit Use(feat_handle_t h) : _prev(override_slot()) { override_slot() = h; }
+      ~Use() { override_slot() = _prev; }
+      Use(const Use&) = delete;
+      Use& operator=(const Use&) = delete;
+    };
     static void synchronize() { check(feat_sync(handle()), "Runtime::synchronize"); }
   };
 

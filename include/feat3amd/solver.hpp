@@ -518,6 +518,16 @@ namespace feat3amd
         else
           _coarse_kind = Coarse::other;
         if(_coarse_kind == Coarse::other) _fusable = false;
+        // execution plans are built here (they allocate), never inside a recorded graph
+        for(Index i = 0; i < L; ++i)
+        {
+          (void)_h->levels[i].matrix->exec();
+          if(i + 1 < L)
+          {
+            (void)_h->levels[i].transfer->get_mat_prol().exec();
+            (void)_h->levels[i].transfer->get_mat_rest().exec();
+          }
+        }
       }
       bool fusable() const override { return _fusable; }
       const FusedWorkspace& workspace() const { return _fw; }
@@ -691,12 +701,43 @@ namespace feat3amd
       VectorType _vec_r, _vec_p, _vec_t;
       FusedWorkspace _fw;
       bool _force_generic = false;
+      bool _use_graphs = true;
+
+      /// Solver-owned stream, handle and recorded graphs of the fused loop: the
+      /// first iteration (MG(p <- r), A*p, update, defect read-back) and every
+      /// later one (MG(t <- r), direction, A*p, update, read-back).
+      struct Graphs
+      {
+        feat_stream_t stream = nullptr;
+        feat_handle_t h = nullptr;
+        double* host = nullptr;
+        feat_graph_t first = nullptr, loop = nullptr;
+        const void* key_x = nullptr;
+        const void* key_b = nullptr;
+        bool failed = false;
+        void drop()
+        {
+          feat_graph_destroy(first);
+          feat_graph_destroy(loop);
+          first = loop = nullptr;
+        }
+        ~Graphs()
+        {
+          drop();
+          if(h) feat_handle_destroy(h);
+          if(stream) feat_stream_destroy(stream);
+          if(host) feat_host_free(host);
+        }
+      };
+      std::unique_ptr<Graphs> _graphs;
 
     public:
       PCG(const Matrix_& m, const Filter_& f, std::shared_ptr<PrecondType> p = nullptr)
         : _matrix(m), _filter(f), _precond(p) {}
       std::string name() const override { return "PCG"; }
       void set_force_generic(bool g) { _force_generic = g; }
+      /// Record the fused loop as HIP graphs (default) or launch every kernel eagerly.
+      void set_use_graphs(bool g) { _use_graphs = g; }
       void init_symbolic() override
       {
         if(_precond) _precond->init_symbolic();
@@ -704,10 +745,28 @@ namespace feat3amd
         _vec_p = _matrix.create_vector_r();
         _vec_t = _matrix.create_vector_r();
         _fw.create(8);
+        (void)_matrix.exec();
+        if(_use_graphs && !_graphs)
+        {
+          auto g = std::make_unique<Graphs>();
+          void* hp = nullptr;
+          check(feat_stream_create(&g->stream), "PCG graphs: stream");
+          check(feat_handle_create(&g->h, 0, g->stream), "PCG graphs: handle");
+          check(feat_host_malloc(&hp, 8 * sizeof(double)), "PCG graphs: pinned scalar");
+          g->host = static_cast<double*>(hp);
+          _graphs = std::move(g);
+        }
       }
       void init_numeric() override
       {
         if(_precond) _precond->init_numeric();
+        // recorded graphs hold the numeric data's addresses: record again on next use
+        if(_graphs) _graphs->drop();
+      }
+      void done_symbolic() override
+      {
+        _graphs.reset();
+        if(_precond) _precond->done_symbolic();
       }
 
       /// pcg.hpp:140-156
@@ -736,7 +795,11 @@ namespace feat3amd
 
       Status _run(VectorType& x, bool correct, const VectorType* rhs)
       {
-        if(MGType* mg = _fused_mg()) return _run_fused(*mg, x, correct, rhs);
+        if(MGType* mg = _fused_mg())
+        {
+          if(correct && _use_graphs && _graphs && !_graphs->failed) return _run_graphs(*mg, x, *rhs);
+          return _run_fused(*mg, x, correct, rhs);
+        }
         if(correct)
         {
           _matrix.apply(_vec_r, x, *rhs, -DataType(1));
@@ -771,6 +834,94 @@ namespace feat3amd
           DataType beta = gamma / gamma2;
           p.axpy(p, z, beta);
         }
+        return st;
+      }
+
+      /// The fused loop recorded as two HIP graphs and replayed once per iteration
+      /// on the solver's own stream (ordered after and before the caller's stream).
+      /// Same kernels in the same order as _run_fused, so the results are identical.
+      Status _run_graphs(MGType& mg, VectorType& x, const VectorType& rhs)
+      {
+        Graphs& g = *_graphs;
+        feat_handle_t caller = Runtime::handle();
+        if(g.key_x != x.elements() || g.key_b != rhs.elements())
+        {
+          g.drop();
+          g.key_x = x.elements();
+          g.key_b = rhs.elements();
+        }
+        const feat_reduce_ws_t* ws = &_fw.ws;
+        double* s_rr = _fw.scalar(0);
+        double* s_g = _fw.scalar(1);
+        double* s_g2 = _fw.scalar(2);
+        double* s_qp = _fw.scalar(3);
+        const feat_matrix_f64_t& A = _matrix.exec();
+        const Index n = _matrix.rows();
+        double* r = (double*)_vec_r.elements();
+        double* p = (double*)_vec_p.elements();
+        double* t = (double*)_vec_t.elements();
+        double* xs = (double*)x.elements();
+        auto body_tail = [&]() {
+          check(feat_apply_masked_f64(g.h, &A, t, p, _filter.mask(), nullptr, p, s_qp, ws), "PCG A*p");
+          check(feat_pcg_update_f64(g.h, xs, r, p, t, s_g, s_qp, s_rr, n, ws), "PCG update");
+          check(feat_copy_scalar_to_host(g.h, s_rr, g.host), "PCG read-back");
+        };
+        auto record = [&](feat_graph_t& slot, const std::function<void()>& body) -> bool {
+          if(slot) return true;
+          Runtime::Use use(g.h);
+          if(feat_graph_begin(g.h) != FEAT_OK) return false;
+          bool ok = true;
+          try
+          {
+            body();
+          }
+          catch(const Error&)
+          {
+            ok = false;
+          }
+          feat_graph_t gr = nullptr;
+          if(feat_graph_end(g.h, &gr) != FEAT_OK) ok = false;
+          if(!ok)
+          {
+            feat_graph_destroy(gr);
+            return false;
+          }
+          slot = gr;
+          return true;
+        };
+        check(feat_handle_wait(g.h, caller), "PCG: order after caller");
+        check(feat_defect_f64(g.h, &A, r, (const double*)rhs.elements(), xs, _filter.mask(), s_rr, ws), "PCG defect");
+        check(feat_copy_scalar_to_host(g.h, s_rr, g.host), "PCG read-back");
+        check(feat_sync(g.h), "PCG sync");
+        Status st = this->_set_initial_defect(std::sqrt(g.host[0]));
+        if(st == Status::progress)
+        {
+          const bool ok = record(g.first, [&]() {
+            mg.apply_fused(_vec_p, _vec_r, s_rr, s_g);
+            body_tail();
+          }) && record(g.loop, [&]() {
+            mg.apply_fused(_vec_t, _vec_r, s_rr, s_g2);
+            check(feat_pcg_direction_f64(g.h, p, t, s_g2, s_g, n), "PCG direction");
+            body_tail();
+          });
+          if(!ok)
+          {
+            // recording is not available: run this and later solves eagerly
+            g.failed = true;
+            g.drop();
+            check(feat_handle_wait(caller, g.h), "PCG: order before caller");
+            return _run_fused(mg, x, true, &rhs);
+          }
+          feat_graph_t cur = g.first;
+          while(st == Status::progress)
+          {
+            check(feat_graph_launch(g.h, cur), "PCG graph");
+            check(feat_sync(g.h), "PCG sync");
+            cur = g.loop;
+            st = this->_set_new_defect([&]() { return std::sqrt(g.host[0]); });
+          }
+        }
+        check(feat_handle_wait(caller, g.h), "PCG: order before caller");
         return st;
       }
 

@@ -46,6 +46,7 @@ typedef struct {
   int precision;          /* 0: FP64 V-cycle; 1: FP32 V-cycle inside the FP64 PCG
                              (ConvertPrecond, solver/convert_precond.hpp:100-119);
                              2: the FP32 cycle on FP16-valued matrices */
+  int no_graphs;          /* 1: launch every kernel eagerly instead of replaying HIP graphs */
 } feat_mgpcg_cfg_t;
 
 typedef struct {          /* one level, host arrays; P maps the next coarser level onto this one */
