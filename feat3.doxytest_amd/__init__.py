@@ -157,6 +157,8 @@ HOST_API = [
     ("feat_q1_assemble", _i, [_i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     ("feat_q1_rhs_sine", _i, [_i, _i, _vp]),
     ("feat_rhs_random", _i, [_u64, _u64, _d, _d, _vp]),
+    ("feat_stokes_sizes", _i, [_i, _vp]),
+    ("feat_stokes_assemble", _i, [_i, _vp]),
     ("feat_mgpcg_create", _i, [C.POINTER(_vp), C.POINTER(LevelHost), _i, C.POINTER(MgPcgCfg)]),
     ("feat_mgpcg_create_q1", _i, [C.POINTER(_vp), _i, _i, _i, C.POINTER(MgPcgCfg)]),
     ("feat_mgpcg_solve", _i, [_vp, _vp, _vp, C.POINTER(SolveResult), _vp, _u64]),
@@ -244,6 +246,42 @@ def reduce_ws(torch_mod, device):
 
 
 # ---------------------------------------------------------------- host layer
+class StokesArrays(C.Structure):
+    _fields_ = [(n, _vp) for n in ("a_val", "a_col", "a_ptr", "b_val", "b_col", "b_ptr", "d_val", "d_col",
+                                   "d_ptr", "filt", "filt_val", "prim", "dual", "pv_val", "pv_col", "pv_ptr",
+                                   "pp_val", "pp_col", "pp_ptr")]
+
+
+def stokes_problem(level):
+    """Q2/P1dc lid-driven cavity level (lexicographic numbering), host numpy arrays:
+    nv, np, A/B/D as (val, col, rowptr) with 2x2 / 2x1 / 1x2 row-major blocks, the
+    velocity filter (nodes, 2 values each), the mean filter vectors and the
+    prolongations from level-1 (CSR)."""
+    _, h = libs()
+    s = np.zeros(10, np.uint64)
+    check(h.feat_stokes_sizes(level, s.ctypes.data), "feat_stokes_sizes")
+    nv, np_, na, nb, nd, nf, npv, npp, nvc, npc = (int(v) for v in s)
+    out = dict(nv=nv, np=np_,
+               A=(np.empty(4 * na), np.empty(na, np.uint64), np.empty(nv + 1, np.uint64)),
+               B=(np.empty(2 * nb), np.empty(nb, np.uint64), np.empty(nv + 1, np.uint64)),
+               D=(np.empty(2 * nd), np.empty(nd, np.uint64), np.empty(np_ + 1, np.uint64)),
+               filt=np.empty(nf, np.uint64), filt_val=np.empty(2 * nf), prim=np.empty(np_), dual=np.empty(np_))
+    if level > 0:
+        out["Pv"] = (np.empty(npv), np.empty(npv, np.uint64), np.empty(nv + 1, np.uint64), nvc)
+        out["Pp"] = (np.empty(npp), np.empty(npp, np.uint64), np.empty(np_ + 1, np.uint64), npc)
+    a = StokesArrays()
+    for key, names in (("A", ("a_val", "a_col", "a_ptr")), ("B", ("b_val", "b_col", "b_ptr")),
+                       ("D", ("d_val", "d_col", "d_ptr")), ("Pv", ("pv_val", "pv_col", "pv_ptr")),
+                       ("Pp", ("pp_val", "pp_col", "pp_ptr"))):
+        if key in out:
+            for nm, arr in zip(names, out[key][:3]):
+                setattr(a, nm, arr.ctypes.data)
+    for nm in ("filt", "filt_val", "prim", "dual"):
+        setattr(a, nm, out[nm].ctypes.data)
+    check(h.feat_stokes_assemble(level, C.byref(a)), "feat_stokes_assemble")
+    return out
+
+
 def q1_problem(dim, level):
     """Structured Q1 Poisson level (lexicographic numbering), host numpy arrays."""
     _, h = libs()

@@ -5,6 +5,7 @@
 #include <feat3amd/lafem.hpp>
 #include <feat3amd/mixed.hpp>
 #include <feat3amd/solver.hpp>
+#include <feat3amd/stokes_assembly.hpp>
 #include <feat3amd_host.h>
 
 #include <cstring>
@@ -112,6 +113,45 @@ int feat_q1_rhs_sine(int dim, int level, double* b)
   return guarded([&]() {
     const auto v = Assembly::q1_rhs_sine(dim, level);
     std::memcpy(b, v.data(), sizeof(double) * v.size());
+  });
+}
+
+int feat_stokes_sizes(int level, std::uint64_t* sizes)
+{
+  return guarded([&]() {
+    if(level < 0 || level > 12) throw Error("feat_stokes_sizes: level out of range");
+    const auto S = Assembly::stokes_q2p1dc(level);
+    sizes[0] = S.nv;
+    sizes[1] = S.np;
+    sizes[2] = S.A.rowptr[S.A.rows];
+    sizes[3] = S.B.rowptr[S.B.rows];
+    sizes[4] = S.D.rowptr[S.D.rows];
+    sizes[5] = S.filt.size();
+    sizes[6] = level > 0 ? S.Pv.rowptr[S.Pv.rows] : 0;
+    sizes[7] = level > 0 ? S.Pp.rowptr[S.Pp.rows] : 0;
+    sizes[8] = level > 0 ? S.Pv.cols : 0;
+    sizes[9] = level > 0 ? S.Pp.cols : 0;
+  });
+}
+
+int feat_stokes_assemble(int level, const feat_stokes_arrays_t* o)
+{
+  return guarded([&]() {
+    if(!o) throw Error("feat_stokes_assemble: null output");
+    const auto S = Assembly::stokes_q2p1dc(level);
+    auto put = [](auto* dst, const auto& v) {
+      if(dst) std::memcpy(dst, v.data(), sizeof(v[0]) * v.size());
+    };
+    put(o->a_val, S.A.val); put(o->a_col, S.A.col); put(o->a_ptr, S.A.rowptr);
+    put(o->b_val, S.B.val); put(o->b_col, S.B.col); put(o->b_ptr, S.B.rowptr);
+    put(o->d_val, S.D.val); put(o->d_col, S.D.col); put(o->d_ptr, S.D.rowptr);
+    put(o->filt, S.filt); put(o->filt_val, S.filt_val);
+    put(o->prim, S.prim); put(o->dual, S.dual);
+    if(level > 0)
+    {
+      put(o->pv_val, S.Pv.val); put(o->pv_col, S.Pv.col); put(o->pv_ptr, S.Pv.rowptr);
+      put(o->pp_val, S.Pp.val); put(o->pp_col, S.Pp.col); put(o->pp_ptr, S.Pp.rowptr);
+    }
   });
 }
 

@@ -25,6 +25,23 @@ int feat_q1_assemble(int dim, int level, double* val, uint64_t* col, uint64_t* r
 int feat_q1_rhs_sine(int dim, int level, double* b);
 int feat_rhs_random(uint64_t seed, uint64_t n, double a, double b, double* out);
 
+/* ---- Q2/P1dc Stokes levels on the unit square (lid-driven cavity) ----
+ * Problem setup of benchmarks/stokes_multigrid-bench.cpp:576-697 (velocity
+ * blocks A = DuDv, B = -grad, D = B^T, UnitFilterBlocked velocity boundary,
+ * pressure MeanFilter, Q2 / P1dc prolongations), lexicographic numbering.
+ * sizes: {nv, np, nnz A, nnz B, nnz D, nfilt, nnz Pv, nnz Pp, nv coarse, np coarse} */
+int feat_stokes_sizes(int level, uint64_t* sizes);
+typedef struct {
+  double* a_val; uint64_t* a_col; uint64_t* a_ptr;   /* 2x2 blocks */
+  double* b_val; uint64_t* b_col; uint64_t* b_ptr;   /* 2x1 blocks */
+  double* d_val; uint64_t* d_col; uint64_t* d_ptr;   /* 1x2 blocks */
+  uint64_t* filt; double* filt_val;                  /* 2 values per node */
+  double* prim; double* dual;
+  double* pv_val; uint64_t* pv_col; uint64_t* pv_ptr; /* may be NULL on level 0 */
+  double* pp_val; uint64_t* pp_col; uint64_t* pp_ptr;
+} feat_stokes_arrays_t;
+int feat_stokes_assemble(int level, const feat_stokes_arrays_t* out);
+
 /* ---- PCG(MultiGrid-V(Richardson(Jacobi))) solver tree ----
  * solver/pcg.hpp + solver/multigrid.hpp + solver/richardson.hpp + solver/jacobi_precond.hpp */
 typedef struct feat_mgpcg_s* feat_mgpcg_t;

@@ -11,6 +11,7 @@
  * reference lines it restates.
  */
 #include "feat_oracle.h"
+#include "orc_internal.h"
 
 #include <float.h>
 #include <immintrin.h>
@@ -305,15 +306,9 @@ void orc_iter_cfg_default(orc_iter_cfg_t* c)
   c->min_stag_iter = 0;
 }
 
-typedef struct {
-  const orc_iter_cfg_t* c;
-  double def_init, def_cur, def_prev;
-  uint64_t num_iter, num_stag_iter;
-  double* hist;
-} itstate_t;
 
 /* iterative.hpp:748-770 */
-static int set_initial_defect(itstate_t* s, double def)
+int set_initial_defect(itstate_t* s, double def)
 {
   s->def_init = s->def_cur = s->def_prev = def;
   s->num_iter = 0;
@@ -325,7 +320,7 @@ static int set_initial_defect(itstate_t* s, double def)
 }
 
 /* iterative.hpp:492-507, 795-836 */
-static int analyse_defect(itstate_t* s, uint64_t it, double cur, double prev, int check_stag)
+int analyse_defect(itstate_t* s, uint64_t it, double cur, double prev, int check_stag)
 {
   const orc_iter_cfg_t* c = s->c;
   if(!isfinite(cur)) return ORC_ABORTED;
@@ -348,7 +343,7 @@ static int analyse_defect(itstate_t* s, uint64_t it, double cur, double prev, in
 
 /* iterative.hpp:853-885: the norm is skipped when min_iter == max_iter
  * (no plot, no stagnation check); the stale defect is analysed then. */
-static int set_new_defect(itstate_t* s, const double* def, uint64_t n)
+int set_new_defect(itstate_t* s, const double* def, uint64_t n)
 {
   ++s->num_iter;
   s->def_prev = s->def_cur;
@@ -359,7 +354,7 @@ static int set_new_defect(itstate_t* s, const double* def, uint64_t n)
   return analyse_defect(s, s->num_iter, s->def_cur, s->def_prev, 1);
 }
 
-static int status_success(int st)
+int status_success(int st)
 {
   return st == ORC_SUCCESS || st == ORC_MAX_ITER || st == ORC_STAGNATED;
 }

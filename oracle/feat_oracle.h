@@ -147,6 +147,54 @@ int orc_richardson_apply(const orc_csr_t* A, const uint64_t* filt, uint64_t nfil
                          double omega_jac, double omega_rich, const orc_iter_cfg_t* it,
                          double* x, const double* b, double* def_hist, orc_result_t* res);
 
+/* ---- Stokes: SaddlePointMatrix<BCSR 2x2, 2x1, 1x2>, UnitFilterBlocked +
+ * MeanFilter, AmaVanka, MG (feat_oracle_stokes.c) ---- */
+typedef struct {
+  uint64_t rows, cols;      /* in blocks */
+  int bh, bw;
+  const double* val;        /* row-major blocks */
+  const uint64_t* col;
+  const uint64_t* rowptr;
+} orc_bcsr_t;
+typedef struct {
+  uint64_t nv, np;          /* velocity nodes (2 dofs each), pressure dofs */
+  orc_bcsr_t A, B, D;       /* system matrix blocks (unfiltered) */
+  const uint64_t* filt;     /* velocity unit filter: node indices ... */
+  const double* filt_val;   /* ... and 2 values per node */
+  uint64_t nfilt;
+  const double* prim;       /* pressure mean filter primal / dual vectors */
+  const double* dual;
+  orc_csr_t Pv, Rv, Pp, Rp; /* scalar Q2 / P1dc transfers to the next coarser level */
+} orc_stokes_level_t;
+typedef struct {
+  int steps;                /* Richardson smoothing steps (min = max) */
+  double omega_rich;        /* smoother damping */
+  double vanka_omega;       /* AmaVanka omega */
+  int coarse_skip_singular; /* AmaVanka::set_skip_singular on the coarsest level */
+  double omega_outer;       /* outer Richardson damping */
+} orc_stokes_cfg_t;
+enum { ORC_FILTER_RHS = 0, ORC_FILTER_SOL = 1, ORC_FILTER_DEF = 2, ORC_FILTER_COR = 3 };
+typedef struct orc_vanka_s orc_vanka_t;
+/* r = y + alpha*K*x, or r = K*x when y == NULL (saddle_point_matrix.hpp:446-491) */
+void orc_saddle_apply(const orc_stokes_level_t* L, double* r, const double* x, const double* y,
+                      double alpha);
+void orc_stokes_filter(const orc_stokes_level_t* L, double* v, int kind);
+/* AmaVanka init on the benchmark's type-1 (filtered) matrix; sizes:
+ * {macros, nnz uu, nnz up, nnz pu, nnz pp, stride} */
+orc_vanka_t* orc_vanka_create(const orc_stokes_level_t* L, double omega, int skip_singular);
+void orc_vanka_sizes(const orc_vanka_t* V, uint64_t* out);
+void orc_vanka_copy(const orc_vanka_t* V, uint64_t* ptr[4], uint64_t* col[4], double* val[4], int* mask);
+void orc_vanka_apply(const orc_vanka_t* V, const orc_stokes_level_t* L, double* c, const double* d);
+void orc_vanka_free(orc_vanka_t* V);
+void orc_stokes_transfer(const orc_stokes_level_t* fine, const orc_stokes_level_t* coarse, int prol,
+                         double* r, const double* x);
+void orc_stokes_smooth(const orc_stokes_level_t* L, int steps, double omega, double vanka_omega,
+                       double* sol, const double* rhs);
+/* Richardson(omega_outer, MG-V(Richardson(omega_rich, Schwarz(AmaVanka)), coarse
+ * Schwarz(AmaVanka))) -- benchmarks/stokes_multigrid-bench.cpp:714-810 */
+int orc_stokes_mg_solve(const orc_stokes_level_t* levels, int nl, const orc_stokes_cfg_t* cfg,
+                        const orc_iter_cfg_t* it, double* x, const double* b, double* hist,
+                        orc_result_t* res);
 #ifdef __cplusplus
 }
 #endif

@@ -73,6 +73,83 @@ class _ResultS(C.Structure):
                 ("def_final", C.c_double)]
 
 
+@dataclass
+class Bcsr:
+    """Block CSR with row-major bh x bw blocks (reference SparseMatrixBCSR)."""
+    rows: int
+    cols: int
+    bh: int
+    bw: int
+    val: np.ndarray
+    col: np.ndarray
+    rowptr: np.ndarray
+
+    def todense(self):
+        d = np.zeros((self.rows * self.bh, self.cols * self.bw))
+        v = self.val.reshape(-1, self.bh, self.bw)
+        for r in range(self.rows):
+            for i in range(int(self.rowptr[r]), int(self.rowptr[r + 1])):
+                c = int(self.col[i])
+                d[r * self.bh:(r + 1) * self.bh, c * self.bw:(c + 1) * self.bw] += v[i]
+        return d
+
+
+class _BcsrS(C.Structure):
+    _fields_ = [("rows", C.c_uint64), ("cols", C.c_uint64), ("bh", C.c_int), ("bw", C.c_int),
+                ("val", _vp), ("col", _vp), ("rowptr", _vp)]
+
+
+class _StokesLevelS(C.Structure):
+    _fields_ = [("nv", C.c_uint64), ("np", C.c_uint64), ("A", _BcsrS), ("B", _BcsrS), ("D", _BcsrS),
+                ("filt", _vp), ("filt_val", _vp), ("nfilt", C.c_uint64), ("prim", _vp), ("dual", _vp),
+                ("Pv", _CsrS), ("Rv", _CsrS), ("Pp", _CsrS), ("Rp", _CsrS)]
+
+
+class _StokesCfgS(C.Structure):
+    _fields_ = [("steps", C.c_int), ("omega_rich", C.c_double), ("vanka_omega", C.c_double),
+                ("coarse_skip_singular", C.c_int), ("omega_outer", C.c_double)]
+
+
+def _arr(a, dt, keep):
+    a = np.ascontiguousarray(a, dtype=dt)
+    keep.append(a)
+    return a.ctypes.data if a.size else None
+
+
+def _bcsr_s(m, keep):
+    return _BcsrS(m.rows, m.cols, m.bh, m.bw, _arr(m.val, np.float64, keep), _arr(m.col, np.uint64, keep),
+                  _arr(m.rowptr, np.uint64, keep))
+
+
+def _stokes_levels_s(levels):
+    """levels: list of dicts {nv, np, A, B, D, filt, filt_val, prim, dual, Pv, Rv, Pp, Rp}."""
+    keep = []
+    arr = (_StokesLevelS * len(levels))()
+    for i, lv in enumerate(levels):
+        arr[i] = _StokesLevelS(lv["nv"], lv["np"], _bcsr_s(lv["A"], keep), _bcsr_s(lv["B"], keep),
+                               _bcsr_s(lv["D"], keep), _arr(lv["filt"], np.uint64, keep),
+                               _arr(lv["filt_val"], np.float64, keep), len(lv["filt"]),
+                               _arr(lv["prim"], np.float64, keep), _arr(lv["dual"], np.float64, keep),
+                               _csr_s(lv.get("Pv"), keep), _csr_s(lv.get("Rv"), keep),
+                               _csr_s(lv.get("Pp"), keep), _csr_s(lv.get("Rp"), keep))
+    return arr, keep
+
+
+def _vanka_blocks(nv, np_, sizes):
+    shapes = [(nv, 2, 2), (nv, 2, 1), (np_, 1, 2), (np_, 1, 1)]
+    blocks = []
+    for (rows, bh, bw), nnz in zip(shapes, sizes):
+        blocks.append(dict(rows=rows, bh=bh, bw=bw, rowptr=np.zeros(rows + 1, np.uint64),
+                           col=np.zeros(max(int(nnz), 1), np.uint64),
+                           val=np.zeros(max(int(nnz), 1) * bh * bw)))
+    ptrs = [(C.POINTER(C.c_uint64) * 4)(), (C.POINTER(C.c_uint64) * 4)(), (C.POINTER(C.c_double) * 4)()]
+    for k, b in enumerate(blocks):
+        ptrs[0][k] = b["rowptr"].ctypes.data_as(C.POINTER(C.c_uint64))
+        ptrs[1][k] = b["col"].ctypes.data_as(C.POINTER(C.c_uint64))
+        ptrs[2][k] = b["val"].ctypes.data_as(C.POINTER(C.c_double))
+    return blocks, ptrs
+
+
 def _csr_s(m, keep):
     if m is None:
         return _CsrS(0, 0, 0, None, None, None)
@@ -149,6 +226,85 @@ class Oracle:
             f.restype = C.c_int
             f.argtypes = [C.POINTER(_CsrS), _vp, C.c_uint64, C.c_int, C.c_double, C.c_double,
                           C.POINTER(_IterCfgS), _f64p, _f64p, _vp, C.POINTER(_ResultS)]
+
+        L.orc_saddle_apply.argtypes = [C.POINTER(_StokesLevelS), _f64p, _f64p, _vp, C.c_double]
+        L.orc_stokes_filter.argtypes = [C.POINTER(_StokesLevelS), _f64p, C.c_int]
+        L.orc_stokes_transfer.argtypes = [C.POINTER(_StokesLevelS), C.POINTER(_StokesLevelS), C.c_int, _f64p, _f64p]
+        L.orc_stokes_smooth.argtypes = [C.POINTER(_StokesLevelS), C.c_int, C.c_double, C.c_double, _f64p, _f64p]
+        L.orc_vanka_create.restype = _vp
+        L.orc_vanka_create.argtypes = [C.POINTER(_StokesLevelS), C.c_double, C.c_int]
+        L.orc_vanka_sizes.argtypes = [_vp, _u64p]
+        L.orc_vanka_copy.argtypes = [_vp, C.POINTER(C.POINTER(C.c_uint64)), C.POINTER(C.POINTER(C.c_uint64)),
+                                     C.POINTER(C.POINTER(C.c_double)), _vp]
+        L.orc_vanka_apply.argtypes = [_vp, C.POINTER(_StokesLevelS), _f64p, _f64p]
+        L.orc_vanka_free.argtypes = [_vp]
+        L.orc_stokes_mg_solve.restype = C.c_int
+        L.orc_stokes_mg_solve.argtypes = [C.POINTER(_StokesLevelS), C.c_int, C.POINTER(_StokesCfgS),
+                                          C.POINTER(_IterCfgS), _f64p, _f64p, _vp, C.POINTER(_ResultS)]
+
+    # -- Stokes
+    def saddle_apply(self, lv, x, y=None, alpha=1.0):
+        arr, keep = _stokes_levels_s([lv])
+        r = np.zeros(2 * lv["nv"] + lv["np"])
+        yy = None if y is None else np.ascontiguousarray(y, dtype=np.float64)
+        self.L.orc_saddle_apply(arr, r, np.ascontiguousarray(x, dtype=np.float64),
+                                None if yy is None else yy.ctypes.data, alpha)
+        return r
+
+    def stokes_filter(self, lv, v, kind):
+        arr, keep = _stokes_levels_s([lv])
+        v = np.array(v, dtype=np.float64)
+        self.L.orc_stokes_filter(arr, v, {"rhs": 0, "sol": 1, "def": 2, "cor": 3}[kind])
+        return v
+
+    def stokes_transfer(self, fine, coarse, x, prol):
+        arr, keep = _stokes_levels_s([fine, coarse])
+        out = coarse if not prol else fine
+        r = np.zeros(2 * out["nv"] + out["np"])
+        self.L.orc_stokes_transfer(C.byref(arr[0]), C.byref(arr[1]), int(prol), r,
+                                     np.ascontiguousarray(x, dtype=np.float64))
+        return r
+
+    def stokes_smooth(self, lv, rhs, steps=5, omega=0.7, vanka_omega=1.0):
+        arr, keep = _stokes_levels_s([lv])
+        sol = np.zeros(2 * lv["nv"] + lv["np"])
+        self.L.orc_stokes_smooth(arr, steps, omega, vanka_omega, sol, np.ascontiguousarray(rhs, dtype=np.float64))
+        return sol
+
+    def vanka(self, lv, omega=1.0, skip_singular=False, d=None):
+        """AmaVanka on the type-1 matrix: the four Vanka blocks, the macro mask and,
+        if d is given, the smoother applied to d."""
+        arr, keep = _stokes_levels_s([lv])
+        h = self.L.orc_vanka_create(arr, omega, int(skip_singular))
+        try:
+            sz = np.zeros(6, np.uint64)
+            self.L.orc_vanka_sizes(h, sz)
+            blocks, ptrs = _vanka_blocks(lv["nv"], lv["np"], sz[1:5])
+            mask = np.zeros(max(int(sz[0]), 1), np.int32)
+            self.L.orc_vanka_copy(h, ptrs[0], ptrs[1], ptrs[2], mask.ctypes.data)
+            out = dict(blocks=blocks, macros=int(sz[0]), stride=int(sz[5]),
+                       mask=mask[:int(sz[0])] if skip_singular else None)
+            if d is not None:
+                c = np.zeros(2 * lv["nv"] + lv["np"])
+                self.L.orc_vanka_apply(h, arr, c, np.ascontiguousarray(d, dtype=np.float64))
+                out["c"] = c
+        finally:
+            self.L.orc_vanka_free(h)
+        return out
+
+    def stokes_mg_solve(self, levels, x0, b, steps=5, omega_rich=0.7, vanka_omega=1.0,
+                        coarse_skip_singular=True, omega_outer=1.0, max_iter=100, **kw):
+        arr, keep = _stokes_levels_s(levels)
+        cfg = _StokesCfgS(steps, omega_rich, vanka_omega, int(coarse_skip_singular), omega_outer)
+        it = self.iter_cfg(max_iter=max_iter, **kw)
+        x = np.array(x0, dtype=np.float64)
+        hist = np.full(max_iter + 1, np.nan)
+        res = _ResultS()
+        st = self.L.orc_stokes_mg_solve(arr, len(levels), C.byref(cfg), C.byref(it), x,
+                                        np.ascontiguousarray(b, dtype=np.float64), hist.ctypes.data,
+                                        C.byref(res))
+        return dict(status=STATUS[st], iters=int(res.num_iter), x=x, hist=hist[:res.num_iter + 1],
+                    def_init=res.def_init, def_final=res.def_final)
 
     # -- RNG
     def rng_fill(self, seed, n, a, b):
@@ -356,6 +512,80 @@ class Reference:
         L.ref_q1_sizes.argtypes = [_vp, _u64p]
         L.ref_q1_copy.argtypes = [_vp, _f64p, _u64p, _u64p, _u64p, _f64p, _f64p, _vp, _vp, _vp]
         L.ref_q1_free.argtypes = [_vp]
+        L.ref_vanka_create.restype = _vp
+        L.ref_vanka_create.argtypes = [C.POINTER(_StokesLevelS), C.c_double, C.c_int, _u64p]
+        L.ref_vanka_copy.argtypes = [_vp, C.POINTER(_StokesLevelS), C.POINTER(C.POINTER(C.c_uint64)),
+                                     C.POINTER(C.POINTER(C.c_uint64)), C.POINTER(C.POINTER(C.c_double)), _vp]
+        L.ref_vanka_apply.argtypes = [_vp, C.POINTER(_StokesLevelS), _f64p, _f64p]
+        L.ref_vanka_free.argtypes = [_vp]
+        L.ref_saddle_apply.argtypes = [C.POINTER(_StokesLevelS), _f64p, _f64p, _vp, C.c_double]
+        L.ref_stokes_filter.argtypes = [C.POINTER(_StokesLevelS), _f64p, C.c_int]
+        L.ref_stokes_transfer.argtypes = [C.POINTER(_StokesLevelS), C.POINTER(_StokesLevelS), C.c_int, _f64p, _f64p]
+        L.ref_stokes_smooth.argtypes = [C.POINTER(_StokesLevelS), C.c_int, C.c_double, C.c_double, _f64p, _f64p]
+        L.ref_stokes_mg_solve.restype = C.c_int
+        L.ref_stokes_mg_solve.argtypes = [C.POINTER(_StokesLevelS), C.c_int, C.c_int, C.c_double, C.c_double,
+                                          C.c_int, C.c_double, C.c_double, C.c_uint64, _f64p, _f64p, _vp,
+                                          C.c_uint64, C.POINTER(C.c_uint64), C.POINTER(C.c_double),
+                                          C.POINTER(C.c_double)]
+
+    # -- Stokes
+    def saddle_apply(self, lv, x, y=None, alpha=1.0):
+        arr, keep = _stokes_levels_s([lv])
+        r = np.zeros(2 * lv["nv"] + lv["np"])
+        yy = None if y is None else np.ascontiguousarray(y, dtype=np.float64)
+        self.L.ref_saddle_apply(arr, r, np.ascontiguousarray(x, dtype=np.float64),
+                                None if yy is None else yy.ctypes.data, alpha)
+        return r
+
+    def stokes_filter(self, lv, v, kind):
+        arr, keep = _stokes_levels_s([lv])
+        v = np.array(v, dtype=np.float64)
+        self.L.ref_stokes_filter(arr, v, {"rhs": 0, "sol": 1, "def": 2, "cor": 3}[kind])
+        return v
+
+    def stokes_transfer(self, fine, coarse, x, prol):
+        arr, keep = _stokes_levels_s([fine, coarse])
+        out = coarse if not prol else fine
+        r = np.zeros(2 * out["nv"] + out["np"])
+        self.L.ref_stokes_transfer(C.byref(arr[0]), C.byref(arr[1]), int(prol), r,
+                                     np.ascontiguousarray(x, dtype=np.float64))
+        return r
+
+    def stokes_smooth(self, lv, rhs, steps=5, omega=0.7, vanka_omega=1.0):
+        arr, keep = _stokes_levels_s([lv])
+        sol = np.zeros(2 * lv["nv"] + lv["np"])
+        self.L.ref_stokes_smooth(arr, steps, omega, vanka_omega, sol, np.ascontiguousarray(rhs, dtype=np.float64))
+        return sol
+
+    def vanka(self, lv, omega=1.0, skip_singular=False, d=None):
+        arr, keep = _stokes_levels_s([lv])
+        sz = np.zeros(5, np.uint64)
+        h = self.L.ref_vanka_create(arr, omega, int(skip_singular), sz)
+        try:
+            blocks, ptrs = _vanka_blocks(lv["nv"], lv["np"], sz[1:5])
+            mask = np.zeros(max(int(sz[0]), 1), np.int32)
+            self.L.ref_vanka_copy(h, arr, ptrs[0], ptrs[1], ptrs[2], mask.ctypes.data)
+            out = dict(blocks=blocks, mask=mask[:int(sz[0])] if skip_singular else None)
+            if d is not None:
+                c = np.zeros(2 * lv["nv"] + lv["np"])
+                self.L.ref_vanka_apply(h, arr, c, np.ascontiguousarray(d, dtype=np.float64))
+                out["c"] = c
+        finally:
+            self.L.ref_vanka_free(h)
+        return out
+
+    def stokes_mg_solve(self, levels, x0, b, steps=5, omega_rich=0.7, vanka_omega=1.0,
+                        coarse_skip_singular=True, omega_outer=1.0, tol_rel=1e-8, max_iter=100):
+        arr, keep = _stokes_levels_s(levels)
+        x = np.array(x0, dtype=np.float64)
+        hist = np.full(max_iter + 1, np.nan)
+        it, dfin, secs = C.c_uint64(), C.c_double(), C.c_double()
+        st = self.L.ref_stokes_mg_solve(arr, len(levels), steps, omega_rich, vanka_omega,
+                                        int(coarse_skip_singular), omega_outer, tol_rel, max_iter, x,
+                                        np.ascontiguousarray(b, dtype=np.float64), hist.ctypes.data,
+                                        hist.size, C.byref(it), C.byref(dfin), C.byref(secs))
+        return dict(status=STATUS[st], iters=int(it.value), x=x, hist=hist[:it.value + 1],
+                    def_final=dfin.value, seconds=secs.value)
 
     def csr_apply(self, A, x, y=None, a=1.0, b=0.0, transposed=False):
         nr = A.cols if transposed else A.rows

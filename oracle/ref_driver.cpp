@@ -44,8 +44,18 @@
 #include <kernel/assembly/common_operators.hpp>
 #include <kernel/assembly/common_functionals.hpp>
 #include <kernel/assembly/grid_transfer.hpp>
+#include <kernel/lafem/dense_vector_blocked.hpp>
+#include <kernel/lafem/sparse_matrix_bwrappedcsr.hpp>
+#include <kernel/lafem/saddle_point_matrix.hpp>
+#include <kernel/lafem/tuple_vector.hpp>
+#include <kernel/lafem/tuple_filter.hpp>
+#include <kernel/lafem/tuple_diag_matrix.hpp>
+#include <kernel/lafem/unit_filter_blocked.hpp>
+#include <kernel/lafem/mean_filter.hpp>
+#include <kernel/solver/amavanka.hpp>
 
 #include <chrono>
+#include <deque>
 #include <cstdint>
 #include <cstring>
 #include <memory>
@@ -249,13 +259,14 @@ public:
   std::vector<double> hist;
   template<typename... A_>
   explicit Recorder(A_&&... args) : Base_(std::forward<A_>(args)...) {}
-  virtual Solver::Status _set_initial_defect(const Vec& d, const Vec& s) override
+  typedef typename Base_::VectorType VecT;
+  virtual Solver::Status _set_initial_defect(const VecT& d, const VecT& s) override
   {
     Solver::Status st = Base_::_set_initial_defect(d, s);
     hist.assign(1, this->_def_init);
     return st;
   }
-  virtual Solver::Status _set_new_defect(const Vec& d, const Vec& s) override
+  virtual Solver::Status _set_new_defect(const VecT& d, const VecT& s) override
   {
     Solver::Status st = Base_::_set_new_defect(d, s);
     hist.push_back(this->_def_cur);
@@ -523,5 +534,354 @@ void ref_q1_copy(void* h, double* val, std::uint64_t* col, std::uint64_t* rowptr
 }
 
 void ref_q1_free(void* h) { delete (ref_q1_t*)h; }
+
+// ---------------- Stokes: SaddlePointMatrix, AmaVanka, multigrid ----------------
+// Mirrors orc_bcsr_t / orc_stokes_level_t of oracle/feat_oracle.h.
+struct ref_bcsr_t
+{
+  std::uint64_t rows, cols;
+  int bh, bw;
+  const double* val;
+  const std::uint64_t* col;
+  const std::uint64_t* rowptr;
+};
+
+struct ref_stokes_level_t
+{
+  std::uint64_t nv, np;
+  ref_bcsr_t A, B, D;
+  const std::uint64_t* filt;
+  const double* filt_val;
+  std::uint64_t nfilt;
+  const double* prim;
+  const double* dual;
+  ref_csr_t Pv, Rv, Pp, Rp;
+};
+
+} // extern "C"
+
+namespace
+{
+  typedef LAFEM::SparseMatrixBCSR<DT, IT, 2, 2> MatA;
+  typedef LAFEM::SparseMatrixBCSR<DT, IT, 2, 1> MatB;
+  typedef LAFEM::SparseMatrixBCSR<DT, IT, 1, 2> MatD;
+  typedef LAFEM::SaddlePointMatrix<MatA, MatB, MatD> SysMat;
+  typedef LAFEM::DenseVectorBlocked<DT, IT, 2> VecV;
+  typedef LAFEM::TupleVector<VecV, Vec> SysVec;
+  typedef LAFEM::UnitFilterBlocked<DT, IT, 2> FiltV;
+  typedef LAFEM::MeanFilter<DT, IT> FiltP;
+  typedef LAFEM::TupleFilter<FiltV, FiltP> SysFilt;
+  typedef LAFEM::SparseMatrixBWrappedCSR<DT, IT, 2> MatTV;
+  typedef LAFEM::TupleDiagMatrix<MatTV, Csr> SysTransMat;
+  typedef LAFEM::Transfer<SysTransMat> SysTrans;
+
+  template<typename M_>
+  M_ make_bcsr(const ref_bcsr_t& m)
+  {
+    const Index nnz = m.rowptr[m.rows];
+    IVec ci = make_ivec(m.col, nnz);
+    Vec va = make_vec(m.val, nnz * Index(m.bh * m.bw));
+    IVec rp = make_ivec(m.rowptr, m.rows + 1);
+    return M_(m.rows, m.cols, ci, va, rp);
+  }
+
+  SysMat make_saddle(const ref_stokes_level_t& L)
+  {
+    return SysMat(make_bcsr<MatA>(L.A), make_bcsr<MatB>(L.B), make_bcsr<MatD>(L.D));
+  }
+
+  SysFilt make_sys_filter(const ref_stokes_level_t& L)
+  {
+    FiltV fv(L.nv);
+    for(std::uint64_t i = 0; i < L.nfilt; ++i)
+    {
+      Tiny::Vector<DT, 2> v;
+      v[0] = L.filt_val[2 * i];
+      v[1] = L.filt_val[2 * i + 1];
+      fv.add(IT(L.filt[i]), v);
+    }
+    FiltP fp(make_vec(L.prim, L.np), make_vec(L.dual, L.np));
+    return SysFilt(std::move(fv), std::move(fp));
+  }
+
+  SysVec make_sysvec(const double* p, const ref_stokes_level_t& L)
+  {
+    VecV u(L.nv);
+    Vec q(L.np);
+    if(p)
+    {
+      std::memcpy(u.template elements<LAFEM::Perspective::pod>(), p, sizeof(double) * 2 * L.nv);
+      std::memcpy(q.elements(), p + 2 * L.nv, sizeof(double) * L.np);
+    }
+    else
+    {
+      u.format();
+      q.format();
+    }
+    return SysVec(std::move(u), std::move(q));
+  }
+
+  void copy_sysvec(double* p, const SysVec& v, const ref_stokes_level_t& L)
+  {
+    std::memcpy(p, v.first().template elements<LAFEM::Perspective::pod>(), sizeof(double) * 2 * L.nv);
+    std::memcpy(p + 2 * L.nv, v.rest().first().elements(), sizeof(double) * L.np);
+  }
+
+  // type-1 matrix of the benchmark (stokes_multigrid-bench.cpp:652-662)
+  SysMat make_type1(const ref_stokes_level_t& L, const SysFilt& F)
+  {
+    SysMat m = make_saddle(L);
+    F.first().filter_mat(m.block_a());
+    F.first().filter_offdiag_row_mat(m.block_b());
+    return m;
+  }
+
+  // exposes the assembled Vanka matrix of the reference AmaVanka
+  class VankaX : public Solver::AmaVanka<SysMat, SysFilt>
+  {
+  public:
+    using Solver::AmaVanka<SysMat, SysFilt>::AmaVanka;
+    const typename Solver::Intern::AmaVankaMatrixHelper<SysMat>::VankaMatrix& vmat() const { return this->_vanka; }
+    const std::vector<int>& mask() const { return this->_macro_mask; }
+  };
+
+  // SchwarzPrecond on one process (schwarz_precond.hpp:125-160): local solve, then filter_cor
+  class LocalSchwarz : public Solver::SolverBase<SysVec>
+  {
+    std::shared_ptr<Solver::SolverBase<SysVec>> _local;
+    const SysFilt& _filter;
+
+  public:
+    LocalSchwarz(std::shared_ptr<Solver::SolverBase<SysVec>> local, const SysFilt& f) : _local(local), _filter(f) {}
+    String name() const override { return "Schwarz"; }
+    void init_symbolic() override { _local->init_symbolic(); }
+    void init_numeric() override { _local->init_numeric(); }
+    void done_numeric() override { _local->done_numeric(); }
+    void done_symbolic() override { _local->done_symbolic(); }
+    Solver::Status apply(SysVec& c, const SysVec& d) override
+    {
+      Solver::Status st = _local->apply(c, d);
+      if(st == Solver::Status::success) _filter.filter_cor(c);
+      return st;
+    }
+  };
+
+  template<typename M_>
+  void copy_bcsr(const M_& m, std::uint64_t* ptr, std::uint64_t* col, double* val)
+  {
+    const Index nr = m.rows(), nnz = m.used_elements();
+    for(Index i = 0; i <= nr; ++i) ptr[i] = m.row_ptr()[i];
+    for(Index i = 0; i < nnz; ++i) col[i] = m.col_ind()[i];
+    std::memcpy(val, m.template val<LAFEM::Perspective::pod>(), sizeof(double) * nnz * Index(M_::BlockHeight * M_::BlockWidth));
+  }
+}
+
+extern "C" {
+
+// AmaVanka on the type-1 matrix of level L; sizes {macros, nnz uu, up, pu, pp}
+void* ref_vanka_create(const ref_stokes_level_t* L, double omega, int skip_singular, std::uint64_t* sizes)
+{
+  ensure_runtime();
+  struct H
+  {
+    SysFilt F;
+    SysMat M;
+    std::shared_ptr<VankaX> V;
+  };
+  H* h = new H{make_sys_filter(*L), SysMat(), nullptr};
+  h->M = make_type1(*L, h->F);
+  h->V = std::make_shared<VankaX>(h->M, h->F, omega);
+  h->V->set_skip_singular(skip_singular != 0);
+  h->V->init();
+  const auto& vm = h->V->vmat();
+  sizes[0] = skip_singular ? h->V->mask().size() : 0;
+  sizes[1] = vm.first().first().used_elements();
+  sizes[2] = vm.first().rest().first().used_elements();
+  sizes[3] = vm.rest().first().first().used_elements();
+  sizes[4] = vm.rest().first().rest().first().used_elements();
+  return h;
+}
+
+void ref_vanka_copy(void* hv, const ref_stokes_level_t* L, std::uint64_t** ptr, std::uint64_t** col, double** val,
+                    int* mask)
+{
+  struct H
+  {
+    SysFilt F;
+    SysMat M;
+    std::shared_ptr<VankaX> V;
+  };
+  H* h = (H*)hv;
+  const auto& vm = h->V->vmat();
+  copy_bcsr(vm.first().first(), ptr[0], col[0], val[0]);
+  copy_bcsr(vm.first().rest().first(), ptr[1], col[1], val[1]);
+  copy_bcsr(vm.rest().first().first(), ptr[2], col[2], val[2]);
+  copy_bcsr(vm.rest().first().rest().first(), ptr[3], col[3], val[3]);
+  if(mask)
+    for(std::size_t i = 0; i < h->V->mask().size(); ++i) mask[i] = h->V->mask()[i];
+  (void)L;
+}
+
+// c = AmaVanka(d) (Vanka matrix apply + filter_cor)
+void ref_vanka_apply(void* hv, const ref_stokes_level_t* L, double* c, const double* d)
+{
+  struct H
+  {
+    SysFilt F;
+    SysMat M;
+    std::shared_ptr<VankaX> V;
+  };
+  H* h = (H*)hv;
+  SysVec vc = make_sysvec(nullptr, *L), vd = make_sysvec(d, *L);
+  h->V->apply(vc, vd);
+  copy_sysvec(c, vc, *L);
+}
+
+void ref_vanka_free(void* hv)
+{
+  struct H
+  {
+    SysFilt F;
+    SysMat M;
+    std::shared_ptr<VankaX> V;
+  };
+  H* h = (H*)hv;
+  h->V->done();
+  delete h;
+}
+
+// r = y + alpha*K*x (y != NULL) or r = K*x
+void ref_saddle_apply(const ref_stokes_level_t* L, double* r, const double* x, const double* y, double alpha)
+{
+  ensure_runtime();
+  SysMat K = make_saddle(*L);
+  SysVec vr = make_sysvec(nullptr, *L), vx = make_sysvec(x, *L);
+  if(y)
+  {
+    SysVec vy = make_sysvec(y, *L);
+    K.apply(vr, vx, vy, alpha);
+  }
+  else
+    K.apply(vr, vx);
+  copy_sysvec(r, vr, *L);
+}
+
+// kind: 0 rhs, 1 sol, 2 def, 3 cor
+void ref_stokes_filter(const ref_stokes_level_t* L, double* v, int kind)
+{
+  ensure_runtime();
+  SysFilt F = make_sys_filter(*L);
+  SysVec vv = make_sysvec(v, *L);
+  if(kind == 0) F.filter_rhs(vv);
+  else if(kind == 1) F.filter_sol(vv);
+  else if(kind == 2) F.filter_def(vv);
+  else F.filter_cor(vv);
+  copy_sysvec(v, vv, *L);
+}
+
+void ref_stokes_transfer(const ref_stokes_level_t* fine, const ref_stokes_level_t* coarse, int prol, double* r,
+                         const double* x)
+{
+  ensure_runtime();
+  SysTransMat P(MatTV(make_csr(fine->Pv)), make_csr(fine->Pp));
+  SysTransMat R(MatTV(make_csr(fine->Rv)), make_csr(fine->Rp));
+  SysTrans T(std::move(P), std::move(R));
+  if(prol)
+  {
+    SysVec vf = make_sysvec(nullptr, *fine), vc = make_sysvec(x, *coarse);
+    T.prol(vf, vc);
+    copy_sysvec(r, vf, *fine);
+  }
+  else
+  {
+    SysVec vf = make_sysvec(x, *fine), vc = make_sysvec(nullptr, *coarse);
+    T.rest(vf, vc);
+    copy_sysvec(r, vc, *coarse);
+  }
+}
+
+void ref_stokes_smooth(const ref_stokes_level_t* L, int steps, double omega, double vanka_omega, double* sol,
+                       const double* rhs)
+{
+  ensure_runtime();
+  SysFilt F = make_sys_filter(*L);
+  SysMat K = make_saddle(*L);
+  SysMat K1 = make_type1(*L, F);
+  auto vanka = Solver::new_amavanka(K1, F, vanka_omega);
+  auto schwarz = std::make_shared<LocalSchwarz>(vanka, F);
+  auto smoother = Solver::new_richardson(K, F, omega, schwarz);
+  smoother->set_min_iter(steps);
+  smoother->set_max_iter(steps);
+  smoother->init();
+  SysVec vs = make_sysvec(nullptr, *L), vr = make_sysvec(rhs, *L);
+  smoother->apply(vs, vr);
+  copy_sysvec(sol, vs, *L);
+  smoother->done();
+}
+
+// Richardson(omega_outer, MG-V(Richardson(omega_rich, Schwarz(AmaVanka))^steps, coarse
+// Schwarz(AmaVanka, skip_singular))) -- stokes_multigrid-bench.cpp:714-810, one process
+int ref_stokes_mg_solve(const ref_stokes_level_t* L, int nl, int steps, double omega_rich, double vanka_omega,
+                        int coarse_skip_singular, double omega_outer, double tol_rel, std::uint64_t max_iter,
+                        double* x, const double* b, double* hist, std::uint64_t hist_cap, std::uint64_t* iters,
+                        double* def_final, double* solve_seconds)
+{
+  ensure_runtime();
+  std::deque<SysMat> K, K1;
+  std::deque<SysFilt> F;
+  std::deque<SysTrans> T;
+  for(int i = 0; i < nl; ++i)
+  {
+    F.push_back(make_sys_filter(L[i]));
+    K.push_back(make_saddle(L[i]));
+    K1.push_back(make_type1(L[i], F.back()));
+    if(i + 1 < nl)
+    {
+      SysTransMat P(MatTV(make_csr(L[i].Pv)), make_csr(L[i].Pp));
+      SysTransMat R(MatTV(make_csr(L[i].Rv)), make_csr(L[i].Rp));
+      T.emplace_back(std::move(P), std::move(R));
+    }
+    else
+      T.emplace_back();
+  }
+  auto hierarchy = std::make_shared<Solver::MultiGridHierarchy<SysMat, SysFilt, SysTrans>>(nl);
+  for(int i = 0; i < nl; ++i)
+  {
+    auto vanka = Solver::new_amavanka(K1[i], F[i], vanka_omega);
+    if(i + 1 < nl)
+    {
+      auto schwarz = std::make_shared<LocalSchwarz>(vanka, F[i]);
+      auto smoother = Solver::new_richardson(K[i], F[i], omega_rich, schwarz);
+      smoother->set_min_iter(steps);
+      smoother->set_max_iter(steps);
+      hierarchy->push_level(K[i], F[i], T[i], smoother, smoother, smoother);
+    }
+    else
+    {
+      vanka->set_skip_singular(coarse_skip_singular != 0);
+      auto schwarz = std::make_shared<LocalSchwarz>(vanka, F[i]);
+      hierarchy->push_level(K[i], F[i], schwarz);
+    }
+  }
+  auto mgv = Solver::new_multigrid(hierarchy, Solver::MultiGridCycle::V);
+  auto solver = std::make_shared<Recorder<Solver::Richardson<SysMat, SysFilt>>>(K[0], F[0], omega_outer, mgv);
+  solver->set_tol_rel(tol_rel);
+  solver->set_max_iter(max_iter);
+  hierarchy->init();
+  solver->init();
+  SysVec vx = make_sysvec(x, L[0]);
+  SysVec vb = make_sysvec(b, L[0]);
+  const auto t0 = std::chrono::steady_clock::now();
+  Solver::Status st = solver->correct(vx, vb);
+  const auto t1 = std::chrono::steady_clock::now();
+  if(solve_seconds) *solve_seconds = std::chrono::duration<double>(t1 - t0).count();
+  copy_sysvec(x, vx, L[0]);
+  copy_hist(solver->hist, hist, hist_cap);
+  *iters = solver->get_num_iter();
+  *def_final = solver->get_def_final();
+  solver->done();
+  hierarchy->done();
+  return int(st);
+}
 
 } // extern "C"

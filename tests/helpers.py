@@ -1,7 +1,7 @@
 """Shared builders for the tests (inputs only; the oracle is the checker)."""
 import numpy as np
 
-from oracle.pyoracle import Csr, csr_transpose
+from oracle.pyoracle import Bcsr, Csr, csr_transpose
 
 
 def random_csr(rng, rows, cols, min_len=0, max_len=30, long_rows=(), sort=True):
@@ -44,3 +44,30 @@ def q1_levels(F, dim, lf, lc):
             lv["R"] = csr_transpose(P)
         levels.append(lv)
     return levels
+
+
+def stokes_levels(F, lf, lc):
+    """Q2/P1dc lid-driven cavity hierarchy from the product's setup code (finest first)."""
+    levels = []
+    for lvl in range(lf, lc - 1, -1):
+        s = F.stokes_problem(lvl)
+        nv, np_ = s["nv"], s["np"]
+        lv = dict(level=lvl, nv=nv, np=np_, filt=s["filt"], filt_val=s["filt_val"], prim=s["prim"],
+                  dual=s["dual"],
+                  A=Bcsr(nv, nv, 2, 2, *s["A"]), B=Bcsr(nv, np_, 2, 1, *s["B"]), D=Bcsr(np_, nv, 1, 2, *s["D"]))
+        if lvl > lc:
+            pv, pp = s["Pv"], s["Pp"]
+            lv["Pv"] = Csr(nv, pv[3], pv[0], pv[1], pv[2])
+            lv["Pp"] = Csr(np_, pp[3], pp[0], pp[1], pp[2])
+            lv["Rv"] = csr_transpose(lv["Pv"])
+            lv["Rp"] = csr_transpose(lv["Pp"])
+        levels.append(lv)
+    return levels
+
+
+def stokes_rhs(lv, oracle):
+    """Lid-driven cavity: zero force, filter_rhs; x0 = filter_sol(0)."""
+    n = 2 * lv["nv"] + lv["np"]
+    b = oracle.stokes_filter(lv, np.zeros(n), "rhs")
+    x0 = oracle.stokes_filter(lv, np.zeros(n), "sol")
+    return x0, b
