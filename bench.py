@@ -77,6 +77,69 @@ def cpu_baseline(feat, b, iters_gpu):
                 iters=r["iters"])
 
 
+STOKES_LF, STOKES_LC = 6, 1
+
+
+def stokes_cpu_reference(feat, x0, b):
+    """The reference's AmaVanka multigrid (oracle/_ref) on one host core, same matrices."""
+    from oracle.pyoracle import Reference, LIB_REF, Bcsr, Csr, csr_transpose
+    if not os.path.exists(LIB_REF):
+        return None
+    levels = []
+    for lvl in range(STOKES_LF, STOKES_LC - 1, -1):
+        s = feat.stokes_problem(lvl)
+        nv, np_ = s["nv"], s["np"]
+        lv = dict(nv=nv, np=np_, filt=s["filt"], filt_val=s["filt_val"], prim=s["prim"], dual=s["dual"],
+                  A=Bcsr(nv, nv, 2, 2, *s["A"]), B=Bcsr(nv, np_, 2, 1, *s["B"]), D=Bcsr(np_, nv, 1, 2, *s["D"]))
+        if lvl > STOKES_LC:
+            pv, pp = s["Pv"], s["Pp"]
+            lv["Pv"], lv["Pp"] = Csr(nv, pv[3], pv[0], pv[1], pv[2]), Csr(np_, pp[3], pp[0], pp[1], pp[2])
+            lv["Rv"], lv["Rp"] = csr_transpose(lv["Pv"]), csr_transpose(lv["Pp"])
+        levels.append(lv)
+    r = Reference().stokes_mg_solve(levels, x0, b, tol_rel=1e-8)
+    return dict(value=r["seconds"], unit="s", cores=1, kind="reference", iters=r["iters"])
+
+
+def stokes_leg(feat, torch, dev, args):
+    t0 = time.perf_counter()
+    st = feat.Stokes(STOKES_LF, STOKES_LC, steps=5, omega_rich=0.7, tol_rel=1e-8, max_iter=100)
+    setup = time.perf_counter() - t0
+    z = st.sizes(0)
+    n = 2 * z["nv"] + z["np"]
+    b = torch.zeros(n, dtype=torch.float64, device=dev)
+    st.filter(0, b, "rhs")
+    x0 = torch.zeros(n, dtype=torch.float64, device=dev)
+    st.filter(0, x0, "sol")
+    x = torch.empty_like(x0)
+    res, times = None, []
+    for k in range(args.warmup + args.steps):
+        x.copy_(x0)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        res = st.solve(x, b)
+        torch.cuda.synchronize()
+        if k >= args.warmup:
+            times.append(time.perf_counter() - t)
+    assert res["status"] == "success", res
+    t_init = float(np.median([st.init_numeric() for _ in range(3)]))
+    out = {"workload": f"2D Stokes lid-driven cavity Q2/P1dc level {STOKES_LF} (64x64 quads, {2 * z['nv']} velocity "
+                       f"+ {z['np']} pressure dofs), Richardson(MG-V(Richardson 0.7 x5, Schwarz(AmaVanka))), "
+                       f"levels {STOKES_LF}..{STOKES_LC}, tol 1e-8",
+           "s_per_solve": float(np.mean(times)), "iters": res["iters"],
+           "final_rel_defect": res["def_final"] / res["def_init"], "vanka_macros": z["macros"],
+           "vanka_local_size": z["stride"], "vanka_init_numeric_s_all_levels": t_init,
+           "setup_s": setup}
+    log(f"stokes L{STOKES_LF}: {out['s_per_solve'] * 1e3:.2f} ms per solve, {res['iters']} iterations, "
+        f"rel. defect {out['final_rel_defect']:.3e}, AmaVanka init {t_init * 1e3:.2f} ms")
+    if not args.no_cpu_baseline:
+        cb = stokes_cpu_reference(feat, x0.cpu().numpy(), b.cpu().numpy())
+        if cb:
+            out["cpu_baseline"] = cb
+            log(f"stokes cpu reference: {cb['value']:.3f} s, {cb['iters']} iterations")
+    st.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -191,6 +254,9 @@ def main():
             f"speedup {t_mean / tmean:.2f}x")
     hh.close()
 
+    # ---- Stokes: Q2/P1dc lid-driven cavity level 6, Richardson(MG-V(Richardson(Schwarz(AmaVanka))))
+    stokes = stokes_leg(feat, torch, dev, args)
+
     # ---- SpMV r = A*x on the same matrix, HIP events on the launch stream:
     # the caller's CSR (u64) and the SELL-64 plan the solver runs on
     k, _ = feat.libs()
@@ -262,6 +328,7 @@ def main():
                         "bytes_per_launch": csr_bytes, "gbps": csr_gbs,
                         "gibps_reference_model": ref_model / (csr_ms * 1e-3) / 1024 ** 3},
         },
+        "stokes": stokes,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS, "traffic": None,
                      "kernel": "SELL-64 SpMV r = A*x (the solver's matrix format), bytes = its arrays + x + r"},

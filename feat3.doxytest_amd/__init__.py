@@ -149,6 +149,12 @@ KERNEL_API = [
     ("feat_dense_gemv_f32", _i, [_vp, _i, _vp, _vp, _vp, _vp]),
     ("feat_convert_dot_f32_f64", _i, [_vp, _vp, _vp, _vp, _u64, _vp, C.POINTER(ReduceWs)]),
     ("feat_dense_gemv_mixed", _i, [_vp, _i, _vp, _vp, _vp, _vp]),
+    ("feat_tuple_apply_f64", _i, [_vp, _vp, _vp, _vp, _vp, _d, _vp]),
+    ("feat_mean_filter_passes_f64", _i, [_vp, _vp, _vp, _vp, _d, _u64, _i, _i]),
+    ("feat_tuple_norm2_f64", _i, [_vp, _vp, _u64, _u64, _vp, _vp]),
+    ("feat_axpy_gated_f64", _i, [_vp, _vp, _d, _vp, _u64, _vp]),
+    ("feat_csrsb2_apply_f64_u64", _i, [_vp, _vp, _vp, _vp, _vp, _vp, _u64]),
+    ("feat_vanka_numeric_f64", _i, [_vp, _vp, _vp, _vp, _d, _i, _vp, _vp, _vp]),
 ]
 
 HOST_API = [
@@ -159,6 +165,17 @@ HOST_API = [
     ("feat_rhs_random", _i, [_u64, _u64, _d, _d, _vp]),
     ("feat_stokes_sizes", _i, [_i, _vp]),
     ("feat_stokes_assemble", _i, [_i, _vp]),
+    ("feat_stokes_create", _i, [C.POINTER(_vp), _i, _i, _vp]),
+    ("feat_stokes_solve", _i, [_vp, _vp, _vp, C.POINTER(SolveResult), _vp, _u64]),
+    ("feat_stokes_init_numeric", _i, [_vp, C.POINTER(_d)]),
+    ("feat_stokes_sizes_of", _i, [_vp, _i, _vp]),
+    ("feat_stokes_vanka_copy", _i, [_vp, _i, _vp, _vp, _vp, _vp]),
+    ("feat_stokes_apply", _i, [_vp, _i, _vp, _vp, _vp, _d]),
+    ("feat_stokes_vanka_apply", _i, [_vp, _i, _vp, _vp, _i]),
+    ("feat_stokes_smooth", _i, [_vp, _i, _vp, _vp]),
+    ("feat_stokes_filter", _i, [_vp, _i, _vp, _i]),
+    ("feat_stokes_transfer", _i, [_vp, _i, _i, _vp, _vp]),
+    ("feat_stokes_destroy", _i, [_vp]),
     ("feat_mgpcg_create", _i, [C.POINTER(_vp), C.POINTER(LevelHost), _i, C.POINTER(MgPcgCfg)]),
     ("feat_mgpcg_create_q1", _i, [C.POINTER(_vp), _i, _i, _i, C.POINTER(MgPcgCfg)]),
     ("feat_mgpcg_solve", _i, [_vp, _vp, _vp, C.POINTER(SolveResult), _vp, _u64]),
@@ -371,6 +388,91 @@ def runtime_init(device=0, stream=None):
         import torch
         stream = torch.cuda.current_stream(device).cuda_stream
     check(h.feat_runtime_init(device, stream), "feat_runtime_init")
+
+
+class StokesCfg(C.Structure):
+    _fields_ = [("steps", _i), ("omega_rich", _d), ("vanka_omega", _d), ("coarse_skip_singular", _i),
+                ("omega_outer", _d), ("tol_rel", _d), ("max_iter", _u64)]
+
+
+class Stokes:
+    """Richardson(MG-V(Richardson(Schwarz(AmaVanka)))) on the Q2/P1dc lid-driven cavity,
+    levels lf..lc, of the C++ host layer (AmaVanka factorised on the device)."""
+
+    KIND = {"rhs": 0, "sol": 1, "def": 2, "cor": 3}
+
+    def __init__(self, lf, lc, steps=5, omega_rich=0.7, vanka_omega=1.0, coarse_skip_singular=True,
+                 omega_outer=1.0, tol_rel=1e-8, max_iter=100):
+        _, self.hl = libs()
+        runtime_init()
+        self.s = C.c_void_p()
+        self.cfg = StokesCfg(steps, omega_rich, vanka_omega, int(coarse_skip_singular), omega_outer, tol_rel,
+                             max_iter)
+        check(self.hl.feat_stokes_create(C.byref(self.s), lf, lc, C.byref(self.cfg)), "feat_stokes_create")
+
+    def sizes(self, lvl=0):
+        o = np.zeros(8, np.uint64)
+        check(self.hl.feat_stokes_sizes_of(self.s, lvl, o.ctypes.data), "feat_stokes_sizes_of")
+        return dict(nv=int(o[0]), np=int(o[1]), macros=int(o[2]), stride=int(o[3]),
+                    nnz=[int(v) for v in o[4:8]])
+
+    def n(self, lvl=0):
+        z = self.sizes(lvl)
+        return 2 * z["nv"] + z["np"]
+
+    def vanka(self, lvl=0):
+        z = self.sizes(lvl)
+        nv, np_ = z["nv"], z["np"]
+        shapes = [(nv, 2, 2), (nv, 2, 1), (np_, 1, 2), (np_, 1, 1)]
+        blocks = [dict(rowptr=np.zeros(r + 1, np.uint64), col=np.zeros(max(k, 1), np.uint64),
+                       val=np.zeros(max(k, 1) * bh * bw)) for (r, bh, bw), k in zip(shapes, z["nnz"])]
+        P = (C.c_void_p * 4)(*[b["rowptr"].ctypes.data for b in blocks])
+        Q = (C.c_void_p * 4)(*[b["col"].ctypes.data for b in blocks])
+        V = (C.c_void_p * 4)(*[b["val"].ctypes.data for b in blocks])
+        mask = np.zeros(max(z["macros"], 1), np.int32)
+        check(self.hl.feat_stokes_vanka_copy(self.s, lvl, P, Q, V, mask.ctypes.data), "feat_stokes_vanka_copy")
+        return dict(blocks=blocks, mask=mask[:z["macros"]], **z)
+
+    def apply(self, lvl, r, x, y=None, alpha=1.0):
+        check(self.hl.feat_stokes_apply(self.s, lvl, ptr(r), ptr(x), ptr(y) if y is not None else None, alpha),
+              "feat_stokes_apply")
+
+    def vanka_apply(self, lvl, c, d, passes=1):
+        check(self.hl.feat_stokes_vanka_apply(self.s, lvl, ptr(c), ptr(d), passes), "feat_stokes_vanka_apply")
+
+    def smooth(self, lvl, sol, rhs):
+        check(self.hl.feat_stokes_smooth(self.s, lvl, ptr(sol), ptr(rhs)), "feat_stokes_smooth")
+
+    def filter(self, lvl, v, kind):
+        check(self.hl.feat_stokes_filter(self.s, lvl, ptr(v), self.KIND[kind]), "feat_stokes_filter")
+
+    def transfer(self, lvl, prol, r, x):
+        check(self.hl.feat_stokes_transfer(self.s, lvl, int(prol), ptr(r), ptr(x)), "feat_stokes_transfer")
+
+    def init_numeric(self):
+        t = C.c_double()
+        check(self.hl.feat_stokes_init_numeric(self.s, C.byref(t)), "feat_stokes_init_numeric")
+        return t.value
+
+    def solve(self, x, b, hist_cap=1000):
+        res = SolveResult()
+        hist = np.full(hist_cap, np.nan)
+        check(self.hl.feat_stokes_solve(self.s, ptr(x), ptr(b), C.byref(res), hist.ctypes.data, hist_cap),
+              "feat_stokes_solve")
+        it = int(res.iters)
+        return dict(status=STATUS[res.status], iters=it, def_init=res.def_init, def_final=res.def_final,
+                    hist=hist[:it + 1])
+
+    def close(self):
+        if self.s:
+            self.hl.feat_stokes_destroy(self.s)
+            self.s = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class MgPcg:

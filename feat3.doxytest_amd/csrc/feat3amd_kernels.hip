@@ -24,16 +24,9 @@
 #include <string>
 #include <type_traits>
 
-#include "feat3amd.h"
+#include "feat3amd_common.hpp"
 
-namespace {
-
-constexpr int kThreads = 256;
-constexpr int kXcd = 8;
-constexpr int kMaxGrid = FEAT_REDUCE_MAX_BLOCKS;
-constexpr int kRowsPerBlock = 64;   // rows per CSR row-block
-constexpr int kChunk = 2048;        // nonzeros staged in LDS per pass
-constexpr int kSellUnroll = 8;      // SELL entries per row in flight
+namespace feat3amd_detail {
 
 thread_local std::string g_err;
 
@@ -55,133 +48,9 @@ int check_launch(const char* where)
   return check_hip(hipGetLastError(), where);
 }
 
-inline int grid_for(uint64_t work, int per_thread = 1)
-{
-  uint64_t b = (work + uint64_t(kThreads) * per_thread - 1) / (uint64_t(kThreads) * per_thread);
-  if(b < 1) b = 1;
-  if(b > uint64_t(kMaxGrid)) b = kMaxGrid;
-  return int(b);
-}
+} // namespace feat3amd_detail
 
-// ---------------------------------------------------------------------------
-// device helpers
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ bool is_masked(const uint32_t* mask, uint64_t i)
-{
-  return mask != nullptr && ((mask[i >> 5] >> (i & 31u)) & 1u);
-}
-
-__device__ __forceinline__ double dfma(double a, double b, double c) { return __fma_rn(a, b, c); }
-__device__ __forceinline__ float dfma(float a, float b, float c) { return __fmaf_rn(a, b, c); }
-
-// Skip rule of IterativeSolver::_set_initial_defect (iterative.hpp:761-766)
-// for a smoother whose initial defect norm is sqrt(nrm2).
-template<typename DT>
-__device__ __forceinline__ bool smoother_skips(const double* nrm2)
-{
-  if(nrm2 == nullptr) return false;
-  const double d = sqrt(*nrm2);
-  const double eps = std::is_same<DT, float>::value ? double(FLT_EPSILON) : DBL_EPSILON;
-  return !isfinite(d) || d <= eps * eps;
-}
-
-__device__ __forceinline__ double wave_sum(double v)
-{
-  for(int off = 32; off > 0; off >>= 1)
-    v += __shfl_down(v, off, 64);
-  return v;
-}
-
-// Sum over the workgroup; the result is valid in thread 0.
-__device__ double block_sum(double v, double* red /* >= kThreads/64 */)
-{
-  v = wave_sum(v);
-  const int lane = threadIdx.x & 63;
-  const int w = threadIdx.x >> 6;
-  if(lane == 0) red[w] = v;
-  __syncthreads();
-  double t = 0.0;
-  if(threadIdx.x == 0)
-  {
-    t = red[0];
-    for(int i = 1; i < int(blockDim.x >> 6); ++i) t += red[i];
-  }
-  __syncthreads();
-  return t;
-}
-
-// Deterministic grid reduction of up to two values: partial per workgroup,
-// the last workgroup to arrive sums the partials in index order.
-struct GridReduce
-{
-  double* partials;   // 2 * kMaxGrid
-  uint32_t* counter;
-  double* out0;       // may be null
-  double* out1;       // may be null
-};
-
-__device__ void grid_reduce_finish(const GridReduce& g, double v0, double v1)
-{
-  __shared__ double red[kThreads / 64];
-  __shared__ int last;
-  const bool r0 = g.out0 != nullptr, r1 = g.out1 != nullptr;
-  if(!r0 && !r1) return;
-  double b0 = r0 ? block_sum(v0, red) : 0.0;
-  double b1 = r1 ? block_sum(v1, red) : 0.0;
-  if(threadIdx.x == 0)
-  {
-    // Publish the partials with agent-scope atomic exchanges: they are performed
-    // at the device's coherence point, so no cache writeback (an agent-scope
-    // release fence flushes the XCD's L2 and costs ~50 ns per workgroup).  The
-    // returned values are kept live so the exchanges are issued with return,
-    // and s_waitcnt(0) waits for them before this workgroup is counted in.
-    double o0 = 0.0, o1 = 0.0;
-    if(r0) o0 = __hip_atomic_exchange(&g.partials[blockIdx.x], b0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if(r1) o1 = __hip_atomic_exchange(&g.partials[kMaxGrid + blockIdx.x], b1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __builtin_amdgcn_s_waitcnt(0);
-    asm volatile("" ::"v"(o0), "v"(o1));
-    const uint32_t prev = __hip_atomic_fetch_add(g.counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    last = (prev == gridDim.x - 1) ? 1 : 0;
-  }
-  __syncthreads();
-  if(!last) return;
-  double s0 = 0.0, s1 = 0.0;
-  for(uint32_t i = threadIdx.x; i < gridDim.x; i += blockDim.x)
-  {
-    if(r0) s0 += __hip_atomic_load(&g.partials[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if(r1) s1 += __hip_atomic_load(&g.partials[kMaxGrid + i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
-  if(r0) s0 = block_sum(s0, red);
-  if(r1) s1 = block_sum(s1, red);
-  if(threadIdx.x == 0)
-  {
-    if(r0) *g.out0 = s0;
-    if(r1) *g.out1 = s1;
-    __hip_atomic_store(g.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
-}
-
-// Contiguous range of row-blocks for this workgroup's XCD; the workgroups of
-// one XCD stride through it.
-struct XcdRange
-{
-  uint64_t first, end, stride;
-};
-
-__device__ __forceinline__ XcdRange xcd_range(uint64_t nitems)
-{
-  const uint32_t g = gridDim.x, b = blockIdx.x;
-  const uint32_t x = b % kXcd, j = b / kXcd;
-  const uint32_t q = g / kXcd, rem = g % kXcd;
-  const uint32_t gx = q + (x < rem ? 1u : 0u);
-  const uint64_t pre = uint64_t(x) * q + (x < rem ? x : rem);
-  XcdRange r;
-  const uint64_t begin = nitems * pre / g;
-  r.end = nitems * (pre + gx) / g;
-  r.first = begin + j;
-  r.stride = gx;
-  return r;
-}
+namespace {
 
 // ---------------------------------------------------------------------------
 // CSR row-block kernel with a fused epilogue.
@@ -838,57 +707,9 @@ __global__ void k_copy_scalar(double* dst, const double* src)
 
 } // namespace
 
-// ===========================================================================
-// handle
-// ===========================================================================
-struct feat_graph_s
-{
-  hipGraph_t graph;
-  hipGraphExec_t exec;
-};
-
-struct feat_handle_s
-{
-  int device;
-  hipStream_t stream;
-  bool own_stream;
-  hipEvent_t event;   // for feat_handle_wait
-  double* partials;   // 2 * kMaxGrid
-  uint32_t* counter;
-  double* scal;       // scratch device scalars
-  double* host;       // pinned host scalars
-};
-
-namespace {
-GridReduce handle_reduce(feat_handle_t h, double* out0, double* out1)
-{
-  GridReduce g;
-  g.partials = h->partials;
-  g.counter = h->counter;
-  g.out0 = out0;
-  g.out1 = out1;
-  return g;
-}
-
-GridReduce ws_reduce(const feat_reduce_ws_t* ws, double* out0, double* out1)
-{
-  GridReduce g;
-  g.partials = ws ? ws->partials : nullptr;
-  g.counter = ws ? ws->counter : nullptr;
-  g.out0 = out0;
-  g.out1 = out1;
-  return g;
-}
-
-bool need_ws(const feat_reduce_ws_t* ws, const void* a, const void* b = nullptr)
-{
-  return (a != nullptr || b != nullptr) && (ws == nullptr || ws->partials == nullptr || ws->counter == nullptr);
-}
-} // namespace
-
 extern "C" {
 
-const char* feat_last_error(void) { return g_err.c_str(); }
+const char* feat_last_error(void) { return feat3amd_detail::g_err.c_str(); }
 
 int feat_handle_create(feat_handle_t* out, int device, feat_stream_t stream)
 {

@@ -5,9 +5,11 @@
 #include <feat3amd/lafem.hpp>
 #include <feat3amd/mixed.hpp>
 #include <feat3amd/solver.hpp>
+#include <feat3amd/stokes.hpp>
 #include <feat3amd/stokes_assembly.hpp>
 #include <feat3amd_host.h>
 
+#include <chrono>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -334,6 +336,181 @@ int feat_mgpcg_matrix(feat_mgpcg_t s, int lvl, std::uint64_t* rows, std::uint64_
 }
 
 int feat_mgpcg_destroy(feat_mgpcg_t s)
+{
+  return guarded([&]() { delete s; });
+}
+
+} // extern "C"
+
+struct feat_stokes_s
+{
+  std::unique_ptr<Solver::StokesMultiGrid> mg;
+};
+
+namespace
+{
+  Solver::StokesMultiGrid::Level& stokes_level(feat_stokes_t s, int lvl)
+  {
+    if(!s || lvl < 0 || std::size_t(lvl) >= s->mg->num_levels()) throw Error("feat_stokes: bad level");
+    return s->mg->level(std::size_t(lvl));
+  }
+  LAFEM::StokesVector svec(const Solver::StokesMultiGrid::Level& L, const double* p)
+  {
+    return LAFEM::StokesVector::wrap(const_cast<double*>(p), L.K.size());
+  }
+}
+
+extern "C" {
+
+int feat_stokes_create(feat_stokes_t* out, int level_fine, int level_coarse, const feat_stokes_cfg_t* cfg)
+{
+  return guarded([&]() {
+    if(!out || !cfg || level_coarse < 0 || level_fine < level_coarse || level_fine > 10)
+      throw Error("feat_stokes_create: bad argument");
+    Solver::StokesConfig c;
+    c.steps = cfg->steps;
+    c.omega_rich = cfg->omega_rich;
+    c.vanka_omega = cfg->vanka_omega;
+    c.coarse_skip_singular = cfg->coarse_skip_singular != 0;
+    c.omega_outer = cfg->omega_outer;
+    auto s = std::make_unique<feat_stokes_s>();
+    s->mg = std::make_unique<Solver::StokesMultiGrid>(c);
+    s->mg->set_tol_rel(cfg->tol_rel);
+    s->mg->set_max_iter(cfg->max_iter);
+    s->mg->build(level_fine, level_coarse);
+    s->mg->init();
+    Runtime::synchronize();
+    *out = s.release();
+  });
+}
+
+int feat_stokes_solve(feat_stokes_t s, double* x, const double* b, feat_solve_result_t* res, double* hist,
+                      std::uint64_t hist_cap)
+{
+  return guarded([&]() {
+    if(!s || !x || !b || !res) throw Error("feat_stokes_solve: bad argument");
+    auto& L = s->mg->level(0);
+    LAFEM::StokesVector vx = svec(L, x), vb = svec(L, b);
+    Solver::Status st = s->mg->correct(vx, vb);
+    Runtime::synchronize();
+    res->status = int(st);
+    res->iters = s->mg->get_num_iter();
+    res->def_init = s->mg->get_def_initial();
+    res->def_final = s->mg->get_def_final();
+    if(hist)
+    {
+      const auto& h = s->mg->history();
+      for(std::size_t i = 0; i < h.size() && i < hist_cap; ++i) hist[i] = h[i];
+    }
+  });
+}
+
+int feat_stokes_init_numeric(feat_stokes_t s, double* seconds)
+{
+  return guarded([&]() {
+    if(!s) throw Error("feat_stokes_init_numeric: null");
+    Runtime::synchronize();
+    const auto t0 = std::chrono::steady_clock::now();
+    s->mg->init_numeric();
+    Runtime::synchronize();
+    if(seconds) *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  });
+}
+
+int feat_stokes_sizes_of(feat_stokes_t s, int lvl, std::uint64_t* out)
+{
+  return guarded([&]() {
+    auto& L = stokes_level(s, lvl);
+    out[0] = L.K.nv();
+    out[1] = L.K.np();
+    out[2] = L.V->num_macros();
+    out[3] = L.V->stride();
+    for(int k = 0; k < 4; ++k) out[4 + k] = L.V->block_nnz(k);
+  });
+}
+
+int feat_stokes_vanka_copy(feat_stokes_t s, int lvl, std::uint64_t** ptr, std::uint64_t** col, double** val, int* mask)
+{
+  return guarded([&]() {
+    auto& L = stokes_level(s, lvl);
+    for(int k = 0; k < 4; ++k) L.V->copy_block(k, ptr ? ptr[k] : nullptr, col ? col[k] : nullptr, val ? val[k] : nullptr);
+    if(mask) L.V->copy_mask(mask);
+  });
+}
+
+int feat_stokes_apply(feat_stokes_t s, int lvl, double* r, const double* x, const double* y, double alpha)
+{
+  return guarded([&]() {
+    auto& L = stokes_level(s, lvl);
+    LAFEM::StokesVector vr = svec(L, r), vx = svec(L, x);
+    if(y)
+    {
+      LAFEM::StokesVector vy = svec(L, y);
+      L.K.apply(vr, vx, vy, alpha);
+    }
+    else
+      L.K.apply(vr, vx);
+    Runtime::synchronize();
+  });
+}
+
+int feat_stokes_vanka_apply(feat_stokes_t s, int lvl, double* c, const double* d, int passes)
+{
+  return guarded([&]() {
+    auto& L = stokes_level(s, lvl);
+    LAFEM::StokesVector vc = svec(L, c), vd = svec(L, d);
+    L.V->apply(vc, vd, passes);
+    Runtime::synchronize();
+  });
+}
+
+int feat_stokes_smooth(feat_stokes_t s, int lvl, double* sol, const double* rhs)
+{
+  return guarded([&]() {
+    auto& L = stokes_level(s, lvl);
+    LAFEM::StokesVector vs = svec(L, sol), vr = svec(L, rhs);
+    s->mg->smooth_level(std::size_t(lvl), vs, vr);
+    Runtime::synchronize();
+  });
+}
+
+int feat_stokes_filter(feat_stokes_t s, int lvl, double* v, int kind)
+{
+  return guarded([&]() {
+    auto& L = stokes_level(s, lvl);
+    LAFEM::StokesVector vv = svec(L, v);
+    switch(kind)
+    {
+    case 0: L.F.filter_rhs(vv); break;
+    case 1: L.F.filter_sol(vv); break;
+    case 2: L.F.filter_def(vv); break;
+    case 3: L.F.filter_cor(vv); break;
+    default: throw Error("feat_stokes_filter: bad kind");
+    }
+    Runtime::synchronize();
+  });
+}
+
+int feat_stokes_transfer(feat_stokes_t s, int lvl, int prol, double* r, const double* x)
+{
+  return guarded([&]() {
+    auto& F = stokes_level(s, lvl);
+    auto& C = stokes_level(s, lvl + 1);
+    if(prol)
+    {
+      LAFEM::StokesVector vr = svec(F, r), vx = svec(C, x);
+      F.T.prol(vr, vx);
+    }
+    else
+    {
+      LAFEM::StokesVector vr = svec(C, r), vx = svec(F, x);
+      F.T.rest(vx, vr);
+    }
+    Runtime::synchronize();
+  });
+}
+
+int feat_stokes_destroy(feat_stokes_t s)
 {
   return guarded([&]() { delete s; });
 }

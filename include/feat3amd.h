@@ -322,6 +322,79 @@ int feat_convert_dot_f32_f64(feat_handle_t h, double* z, const float* z32, const
 int feat_dense_gemv_mixed(feat_handle_t h, int n, const double* ainv, const float* b, float* x,
                           const uint32_t* mask);
 
+/* ---------------- Stokes: SaddlePointMatrix, system filter, AmaVanka ----------------
+ * A Stokes vector is [u | p]: 2 values per velocity node (node major), then one per
+ * pressure dof.  A feat_tuple_f64_t is a 2x2 tuple of BCSR blocks over that layout
+ * (u64 col / rowptr, row-major blocks):
+ *   block 0: nv x nv, 2x2 blocks   block 1: nv x np, 2x1 blocks
+ *   block 2: np x nv, 1x2 blocks   block 3: np x np, 1x1 blocks (val == NULL: absent)
+ * SaddlePointMatrix<A, B, D> (lafem/saddle_point_matrix.hpp:117-140) is {A, B, D, -};
+ * the AmaVanka matrix (solver/amavanka.hpp:102-119, a TupleMatrix) uses all four. */
+typedef struct {
+  uint64_t nv, np;
+  const double* val[4];
+  const uint64_t* col[4];
+  const uint64_t* ptr[4];
+} feat_tuple_f64_t;
+
+/* r = y + alpha*M*x (y != NULL) or r = M*x, with the reference's op order: row
+ * block 0 is A.apply(r, x_u, y_u, alpha) then B.apply(r, x_p, r, alpha), row block 1
+ * D.apply(r_p, x_u, y_p, alpha) then C (saddle_point_matrix.hpp:446-491,
+ * tuple_matrix.hpp:305-315; each block as Arch::Apply::bcsr, apply_generic.hpp:121-154).
+ * vmask (may be NULL): velocity nodes whose two rows are zeroed afterwards
+ * (UnitFilterBlocked::filter_def, unit_filter_blocked.hpp:478-499).  r must not alias x. */
+int feat_tuple_apply_f64(feat_handle_t h, const feat_tuple_f64_t* M, double* r, const double* x,
+                         const double* y, double alpha, const uint32_t* vmask);
+
+/* MeanFilter passes on one vector v of length n, in one workgroup
+ * (lafem/mean_filter.hpp:213-262): per pass alpha = -(v . w_dot)/volume
+ * (sol_form: 0 - (v . w_dot)/volume) and, unless |alpha| < eps (DenseVector::axpy,
+ * dense_vector.hpp:815-820), v = alpha*w_add + v.  passes = 1 or 2 (two filter_cor
+ * calls in a row, as AmaVanka followed by SchwarzPrecond). */
+int feat_mean_filter_passes_f64(feat_handle_t h, double* v, const double* w_dot,
+                                const double* w_add, double volume, uint64_t n, int passes,
+                                int sol_form);
+
+/* TupleVector::norm2 of [u | p] (tuple_vector.hpp:362-370):
+ * sqrt(|u|^2 + |p|^2) with |.| = DenseVector::norm2, written to *out (device). */
+int feat_tuple_norm2_f64(feat_handle_t h, const double* v, uint64_t nu, uint64_t np, double* out,
+                         const feat_reduce_ws_t* ws);
+
+/* x = omega*c + x (DenseVector::axpy, r == y branch), skipped entirely when the
+ * device value *gate_norm (may be NULL) is non-finite or <= eps^2: the skip rule
+ * of IterativeSolver::_set_initial_defect (iterative.hpp:761-766) for a smoother
+ * whose iterate stays formatted. */
+int feat_axpy_gated_f64(feat_handle_t h, double* x, double omega, const double* c, uint64_t n,
+                        const double* gate_norm);
+
+/* Velocity transfer: SparseMatrixBWrappedCSR<2> apply r = P*x on 2-blocked vectors
+ * (Arch::Apply::csrsb_generic, apply_generic.hpp:157-181; a = 1, b = 0). */
+int feat_csrsb2_apply_f64_u64(feat_handle_t h, double* r, const double* x, const double* val,
+                              const uint64_t* col, const uint64_t* rowptr, uint64_t rows);
+
+/* AmaVanka macro graphs (amavanka.hpp:224-350 deduct_macro_dofs and their
+ * transposes), device arrays. */
+typedef struct {
+  uint64_t nm, nv, np, stride;
+  const uint64_t* vptr; const uint64_t* vidx;   /* macro -> velocity nodes, ascending */
+  const uint64_t* pptr; const uint64_t* pidx;   /* macro -> pressure dofs, ascending  */
+  const uint64_t* dvptr; const uint64_t* dvidx; /* velocity node -> macros, ascending */
+  const uint64_t* dpptr; const uint64_t* dpidx; /* pressure dof -> macros, ascending  */
+} feat_vanka_graph_t;
+
+/* AmaVanka::init_numeric (amavanka.hpp:1478-1559) on device for K = {A, B, D}:
+ * per macro (one workgroup each) gather the local matrix of the benchmark's type-1
+ * matrix (vmask rows: unit rows in A, zero rows in B; unit_filter_blocked.hpp:313-372),
+ * invert it by Gauss-Jordan with diagonal pivoting (util/math.hpp:1169-1274), and with
+ * skip_singular mark macros with |I - A*A^-1|_F^2 >= eps (macro_mask, nm ints); then
+ * every Vanka matrix entry is the sum of the local inverses in ascending macro order
+ * (the reference's sequential scatter-add) times omega / #macros of its row
+ * (scale_rows, :1032-1066).  V holds the Vanka pattern; its val arrays are written.
+ * local_inv: nm * stride * stride doubles of scratch.  stride <= 32. */
+int feat_vanka_numeric_f64(feat_handle_t h, const feat_vanka_graph_t* g, const feat_tuple_f64_t* K,
+                           const uint32_t* vmask, double omega, int skip_singular,
+                           double* local_inv, int* macro_mask, const feat_tuple_f64_t* V);
+
 #ifdef __cplusplus
 }
 #endif

@@ -102,6 +102,41 @@ const char* feat_host_last_error(void);
  * host-layer function; later calls are ignored once initialised. */
 int feat_runtime_init(int device, void* stream);
 
+/* ---- Stokes multigrid: Richardson(MG-V(Richardson(Schwarz(AmaVanka)))) ----
+ * benchmarks/stokes_multigrid-bench.cpp:714-810 on levels level_fine..level_coarse
+ * of the lid-driven cavity (one process). */
+typedef struct {
+  int steps;               /* smoother steps, min = max (default 5)         */
+  double omega_rich;       /* smoother damping (0.7)                        */
+  double vanka_omega;      /* AmaVanka omega (1.0)                          */
+  int coarse_skip_singular;/* AmaVanka::set_skip_singular on the coarse level */
+  double omega_outer;      /* outer Richardson damping (1.0)                */
+  double tol_rel;
+  uint64_t max_iter;
+} feat_stokes_cfg_t;
+typedef struct feat_stokes_s* feat_stokes_t;
+/* builds the levels and runs init_symbolic + init_numeric (AmaVanka on device) */
+int feat_stokes_create(feat_stokes_t* s, int level_fine, int level_coarse, const feat_stokes_cfg_t* cfg);
+/* x (initial guess, overwritten), b: device arrays of 2*nv + np doubles */
+int feat_stokes_solve(feat_stokes_t s, double* x, const double* b, feat_solve_result_t* res,
+                      double* hist, uint64_t hist_cap);
+/* re-runs every level's AmaVanka::init_numeric; *seconds = wall time incl. sync */
+int feat_stokes_init_numeric(feat_stokes_t s, double* seconds);
+/* sizes of level lvl (0 = finest): {nv, np, macros, stride, nnz uu, up, pu, pp} */
+int feat_stokes_sizes_of(feat_stokes_t s, int lvl, uint64_t* out);
+/* host copies of the Vanka blocks (arrays of 4 pointers each; any may be NULL) and mask */
+int feat_stokes_vanka_copy(feat_stokes_t s, int lvl, uint64_t** ptr, uint64_t** col, double** val, int* mask);
+/* device-pointer operations on level lvl, for tests:
+ * r = K x (y == NULL) or y + alpha K x;  c = Vanka(d) with `passes` filter_cor calls;
+ * one smoother application;  filter kind 0 rhs, 1 sol, 2 def, 3 cor;
+ * transfer between lvl and lvl+1: prol (r on lvl) or rest (r on lvl+1) */
+int feat_stokes_apply(feat_stokes_t s, int lvl, double* r, const double* x, const double* y, double alpha);
+int feat_stokes_vanka_apply(feat_stokes_t s, int lvl, double* c, const double* d, int passes);
+int feat_stokes_smooth(feat_stokes_t s, int lvl, double* sol, const double* rhs);
+int feat_stokes_filter(feat_stokes_t s, int lvl, double* v, int kind);
+int feat_stokes_transfer(feat_stokes_t s, int lvl, int prol, double* r, const double* x);
+int feat_stokes_destroy(feat_stokes_t s);
+
 #ifdef __cplusplus
 }
 #endif
