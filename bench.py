@@ -140,6 +140,85 @@ def stokes_leg(feat, torch, dev, args):
     return out
 
 
+BATCH = dict(seed=28054777172512, ninst=64, lmin=3, lmax=8, sigma=1.0)
+
+
+def batch_leg(feat, torch, dev, args):
+    """64 independent 2D Q1 diffusion MG-PCG solves (levels 3..8, log-normal
+    coefficients): the batched solver vs the same solves one after another."""
+    t0 = time.perf_counter()
+    bs = feat.BatchSolver(BATCH["seed"], BATCH["ninst"], BATCH["lmin"], BATCH["lmax"], BATCH["sigma"], 1e-8, 100)
+    setup = time.perf_counter() - t0
+    b_host = feat.batch_rhs(bs, BATCH["seed"])
+    b = torch.from_numpy(b_host).to(dev)
+    x = torch.zeros(bs.n, dtype=torch.float64, device=dev)
+    times, res = [], None
+    for k in range(args.warmup + args.steps):
+        x.zero_()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        res = bs.solve(x, b)
+        torch.cuda.synchronize()
+        if k >= args.warmup:
+            times.append(time.perf_counter() - t)
+    assert all(s == "success" for s in res["status"]), res["status"]
+    t_batch = float(np.mean(times))
+    tunes, us = bs.tuning()
+    # the same 64 solves one after another with the single-instance solver
+    cfg = feat.mgpcg_cfg(coarse="jacobi", coarse_omega=1.0, tol_rel=1e-8, max_iter=100)
+    singles = []
+    for i, L in enumerate(bs.levels):
+        coef = feat.lognormal_cells(BATCH["seed"] + 1000 + i, 4 ** int(L), BATCH["sigma"])
+        lo, hi = int(bs.off[i]), int(bs.off[i + 1])
+        singles.append((feat.MgPcg(cfg, levels=feat.batch_instance_levels(int(L), coef)), lo, hi))
+    xs = torch.zeros(bs.n, dtype=torch.float64, device=dev)
+    seq = []
+    for k in range(args.warmup + args.steps):
+        xs.zero_()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for s, lo, hi in singles:
+            s.solve(xs[lo:hi], b[lo:hi])
+        torch.cuda.synchronize()
+        if k >= args.warmup:
+            seq.append(time.perf_counter() - t)
+    for s, _, _ in singles:
+        s.close()
+    t_seq = float(np.mean(seq))
+    err = float(torch.linalg.norm(x - xs) / torch.linalg.norm(xs))
+    out = {"workload": "64 independent 2D Q1 diffusion problems, levels 3..8 from the seed "
+                       f"({bs.n} dofs in total), log-normal(0,1) cell coefficients, PCG + MG V(4,4) "
+                       "Richardson(Jacobi 0.5), coarse level 1 Jacobi, tol 1e-8 each",
+           "s_per_batch": t_batch, "s_sequential_single_solves": t_seq, "speedup_vs_sequential": t_seq / t_batch,
+           "iters_min": int(res["iters"].min()), "iters_max": int(res["iters"].max()),
+           "rel_diff_vs_sequential": err, "levels": [int(v) for v in bs.levels], "setup_s": setup,
+           "autotuned_shapes": [int(t) for t in tunes], "autotuned_spmv_us": [float(u) for u in us]}
+    log(f"batch of 64: {t_batch * 1e3:.2f} ms (sequential single solves {t_seq * 1e3:.2f} ms, "
+        f"{t_seq / t_batch:.2f}x), iterations {out['iters_min']}..{out['iters_max']}, |x-x_seq|/|x_seq| {err:.2e}")
+    if not args.no_cpu_baseline:
+        from oracle.pyoracle import Reference, LIB_REF, Csr, csr_transpose, COARSE_JACOBI
+        if os.path.exists(LIB_REF):
+            ref = Reference()
+            tot = 0.0
+            for i, L in enumerate(bs.levels):
+                coef = feat.lognormal_cells(BATCH["seed"] + 1000 + i, 4 ** int(L), BATCH["sigma"])
+                lvls = []
+                for q in feat.batch_instance_levels(int(L), coef):
+                    lv = dict(A=Csr(q["n"], q["n"], q["val"], q["col"], q["rowptr"]), filt=q["filt"])
+                    if q.get("pcols"):
+                        P = Csr(q["n"], q["pcols"], q["pval"], q["pcol"], q["prowptr"])
+                        lv["P"], lv["R"] = P, csr_transpose(P)
+                    lvls.append(lv)
+                lo, hi = int(bs.off[i]), int(bs.off[i + 1])
+                r = ref.mg_pcg_solve(lvls, np.zeros(hi - lo), b_host[lo:hi], coarse=COARSE_JACOBI)
+                tot += r["seconds"]
+            out["cpu_baseline"] = {"value": tot, "unit": "s", "cores": 1, "kind": "reference",
+                                   "sample": "the 64 solves one after another, reference generic code"}
+            log(f"batch cpu reference: {tot:.3f} s")
+    bs.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -256,6 +335,7 @@ def main():
 
     # ---- Stokes: Q2/P1dc lid-driven cavity level 6, Richardson(MG-V(Richardson(Schwarz(AmaVanka))))
     stokes = stokes_leg(feat, torch, dev, args)
+    batch = batch_leg(feat, torch, dev, args)
 
     # ---- SpMV r = A*x on the same matrix, HIP events on the launch stream:
     # the caller's CSR (u64) and the SELL-64 plan the solver runs on
@@ -329,6 +409,7 @@ def main():
                         "gibps_reference_model": ref_model / (csr_ms * 1e-3) / 1024 ** 3},
         },
         "stokes": stokes,
+        "batch": batch,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS, "traffic": None,
                      "kernel": "SELL-64 SpMV r = A*x (the solver's matrix format), bytes = its arrays + x + r"},

@@ -50,7 +50,7 @@ class Matrix(C.Structure):
     """feat_matrix_f64_t / feat_matrix_f32_t (same layout)."""
     _fields_ = [("format", _i), ("index", _i), ("rows", _u64), ("cols", _u64), ("nnz", _u64),
                 ("val", _vp), ("col", _vp), ("ptr", _vp), ("rowlen", _vp), ("value_kind", _i),
-                ("scale", _f)]
+                ("scale", _f), ("tune", _i)]
 
 
 FMT_CSR, FMT_SELL = 0, 1
@@ -155,6 +155,14 @@ KERNEL_API = [
     ("feat_axpy_gated_f64", _i, [_vp, _vp, _d, _vp, _u64, _vp]),
     ("feat_csrsb2_apply_f64_u64", _i, [_vp, _vp, _vp, _vp, _vp, _vp, _u64]),
     ("feat_vanka_numeric_f64", _i, [_vp, _vp, _vp, _vp, _d, _i, _vp, _vp, _vp]),
+    ("feat_batch_dot_f64", _i, [_vp, _vp, _vp, _vp, _vp]),
+    ("feat_batch_pcg_update_f64", _i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    ("feat_batch_pcg_direction_f64", _i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i]),
+    ("feat_batch_copy_scalars_f64", _i, [_vp, C.c_uint32, _vp, _vp, _vp]),
+    ("feat_batch_status_f64", _i, [_vp, C.c_uint32, _vp, _vp, _vp, _i]),
+    ("feat_batch_gather_f64", _i, [_vp, C.c_uint32, _vp, _vp, _vp, _vp, _vp]),
+    ("feat_copy_int_to_host", _i, [_vp, _vp, _vp]),
+    ("feat_autotune_matrix_f64", _i, [_vp, _vp, _vp, _vp, _i, _vp]),
 ]
 
 HOST_API = [
@@ -176,6 +184,14 @@ HOST_API = [
     ("feat_stokes_filter", _i, [_vp, _i, _vp, _i]),
     ("feat_stokes_transfer", _i, [_vp, _i, _i, _vp, _vp]),
     ("feat_stokes_destroy", _i, [_vp]),
+    ("feat_batchsolver_create", _i, [C.POINTER(_vp), _vp]),
+    ("feat_batchsolver_layout", _i, [_vp, _vp, _vp]),
+    ("feat_batchsolver_solve", _i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    ("feat_batchsolver_tuning", _i, [_vp, _vp, _vp, _i, C.POINTER(_i)]),
+    ("feat_batchsolver_destroy", _i, [_vp]),
+    ("feat_batch_levels", _i, [_u64, _i, _i, _i, _vp]),
+    ("feat_lognormal_cells", _i, [_u64, _u64, _d, _vp]),
+    ("feat_q1_diffusion_assemble", _i, [_i, _i, _vp, _vp, _vp, _vp]),
     ("feat_mgpcg_create", _i, [C.POINTER(_vp), C.POINTER(LevelHost), _i, C.POINTER(MgPcgCfg)]),
     ("feat_mgpcg_create_q1", _i, [C.POINTER(_vp), _i, _i, _i, C.POINTER(MgPcgCfg)]),
     ("feat_mgpcg_solve", _i, [_vp, _vp, _vp, C.POINTER(SolveResult), _vp, _u64]),
@@ -466,6 +482,111 @@ class Stokes:
     def close(self):
         if self.s:
             self.hl.feat_stokes_destroy(self.s)
+            self.s = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class BatchCfg(C.Structure):
+    _fields_ = [("seed", _u64), ("ninst", _i), ("lmin", _i), ("lmax", _i), ("sigma", _d), ("tol_rel", _d),
+                ("max_iter", _u64), ("autotune", _i), ("graphs", _i)]
+
+
+def batch_levels(seed, ninst, lmin, lmax):
+    _, h = libs()
+    out = np.zeros(ninst, np.int32)
+    check(h.feat_batch_levels(seed, ninst, lmin, lmax, out.ctypes.data), "feat_batch_levels")
+    return out
+
+
+def lognormal_cells(seed, n, sigma):
+    _, h = libs()
+    out = np.zeros(n)
+    check(h.feat_lognormal_cells(seed, n, sigma, out.ctypes.data), "feat_lognormal_cells")
+    return out
+
+
+def q1_diffusion(level, level_c, coef):
+    """Q1 diffusion matrix values on `level` (pattern of q1_problem(2, level))."""
+    _, h = libs()
+    q = q1_problem(2, level)
+    val = np.zeros(q["val"].size)
+    coef = np.ascontiguousarray(coef, dtype=np.float64)
+    check(h.feat_q1_diffusion_assemble(level, level_c, coef.ctypes.data, val.ctypes.data, None, None),
+          "feat_q1_diffusion_assemble")
+    return val
+
+
+def batch_rhs(bs, seed=28054777172512):
+    """Right-hand sides of the standard batch in the PCG layout (host): instance i
+    uniform [-1, 1] from seed + 2000 + i, boundary rows zero."""
+    b = np.zeros(int(bs.off[-1]))
+    for i, L in enumerate(bs.levels):
+        lo, hi = int(bs.off[i]), int(bs.off[i + 1])
+        bi = rhs_random(seed + 2000 + i, hi - lo)
+        m = 2 ** int(L) + 1
+        idx = np.arange(hi - lo)
+        bi[(idx % m == 0) | (idx % m == m - 1) | (idx // m == 0) | (idx // m == m - 1)] = 0.0
+        b[lo:hi] = bi
+    return b
+
+
+def batch_instance_levels(L, coef):
+    """Host levels (finest first, as for MgPcg(levels=...)) of one batch instance."""
+    levels = []
+    for k in range(L, 0, -1):
+        q = dict(q1_problem(2, k))
+        q["val"] = q1_diffusion(k, L, coef)
+        levels.append(q)
+    return levels
+
+
+class BatchSolver:
+    """The batch of independent Q1 diffusion MG-PCG solves (C++ host layer)."""
+
+    def __init__(self, seed=28054777172512, ninst=64, lmin=3, lmax=8, sigma=1.0, tol_rel=1e-8, max_iter=100,
+                 autotune=True, graphs=True):
+        _, self.hl = libs()
+        runtime_init()
+        self.s = C.c_void_p()
+        self.cfg = BatchCfg(seed, ninst, lmin, lmax, sigma, tol_rel, max_iter, int(autotune), int(graphs))
+        check(self.hl.feat_batchsolver_create(C.byref(self.s), C.byref(self.cfg)), "feat_batchsolver_create")
+        self.levels = np.zeros(ninst, np.int32)
+        self.off = np.zeros(ninst + 1, np.uint64)
+        check(self.hl.feat_batchsolver_layout(self.s, self.levels.ctypes.data, self.off.ctypes.data),
+              "feat_batchsolver_layout")
+        self.ninst, self.max_iter = ninst, max_iter
+
+    @property
+    def n(self):
+        return int(self.off[-1])
+
+    def solve(self, x, b):
+        ni = self.ninst
+        st = np.zeros(ni, np.int32)
+        it = np.zeros(ni, np.uint64)
+        d0, d1 = np.zeros(ni), np.zeros(ni)
+        hist = np.zeros(ni * (self.max_iter + 1))
+        check(self.hl.feat_batchsolver_solve(self.s, ptr(x), ptr(b), st.ctypes.data, it.ctypes.data, d0.ctypes.data,
+                                             d1.ctypes.data, hist.ctypes.data), "feat_batchsolver_solve")
+        return dict(status=[STATUS[int(v)] for v in st], iters=it.astype(int), def_init=d0, def_final=d1,
+                    hist=hist.reshape(ni, self.max_iter + 1))
+
+    def tuning(self):
+        t = np.zeros(64, np.int32)
+        u = np.zeros(64)
+        cnt = C.c_int()
+        check(self.hl.feat_batchsolver_tuning(self.s, t.ctypes.data, u.ctypes.data, 64, C.byref(cnt)),
+              "feat_batchsolver_tuning")
+        return t[:cnt.value], u[:cnt.value]
+
+    def close(self):
+        if self.s:
+            self.hl.feat_batchsolver_destroy(self.s)
             self.s = C.c_void_p()
 
     def __del__(self):

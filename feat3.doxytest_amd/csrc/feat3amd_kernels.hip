@@ -255,7 +255,7 @@ __device__ __forceinline__ float ld_val<float, uint16_t>(const uint16_t* p)
   return float(__builtin_bit_cast(_Float16, __builtin_nontemporal_load(p)));
 }
 
-template<typename DT, typename VT, typename CT, bool DELTA, class Epi>
+template<typename DT, typename VT, typename CT, bool DELTA, class Epi, int UNR = kSellUnroll>
 __global__ __launch_bounds__(kThreads) void k_sell_rows(const VT* __restrict__ val,
                                                         const CT* __restrict__ col,
                                                         const uint64_t* __restrict__ soff,
@@ -280,21 +280,21 @@ __global__ __launch_bounds__(kThreads) void k_sell_rows(const VT* __restrict__ v
       DT sum = DT(0);
       int k = 0;
       // matrix arrays are streamed once: nontemporal loads keep x resident in L2
-      for(; k + kSellUnroll <= len; k += kSellUnroll)
+      for(; k + UNR <= len; k += UNR)
       {
-        DT vv[kSellUnroll];
-        uint64_t cc[kSellUnroll];
-        DT xv[kSellUnroll];
+        DT vv[UNR];
+        uint64_t cc[UNR];
+        DT xv[UNR];
 #pragma unroll
-        for(int u = 0; u < kSellUnroll; ++u)
+        for(int u = 0; u < UNR; ++u)
         {
           vv[u] = ld_val<DT, VT>(v + (k + u) * 64);
           cc[u] = sell_col<DELTA>(__builtin_nontemporal_load(c + (k + u) * 64), row);
         }
 #pragma unroll
-        for(int u = 0; u < kSellUnroll; ++u) xv[u] = x[cc[u]];
+        for(int u = 0; u < UNR; ++u) xv[u] = x[cc[u]];
 #pragma unroll
-        for(int u = 0; u < kSellUnroll; ++u) sum = dfma(vv[u], xv[u], sum);
+        for(int u = 0; u < UNR; ++u) sum = dfma(vv[u], xv[u], sum);
       }
       for(; k < len; ++k)
         sum = dfma(ld_val<DT, VT>(v + k * 64), x[sell_col<DELTA>(__builtin_nontemporal_load(c + k * 64), row)], sum);
@@ -320,9 +320,13 @@ int launch_mat(hipStream_t s, const M* A, const DT* x, const Epi& epi, const Gri
   else if(A->format == FEAT_FMT_SELL)
   {
     // one workgroup per 256-row chunk up to the reduction-partials limit, so the
-    // hardware dispatcher balances the load; larger matrices stride
+    // hardware dispatcher balances the load; larger matrices stride.  A tuned
+    // descriptor caps the grid lower (workgroups stride over the chunks).
     const uint64_t nch = (A->rows + kThreads - 1) / kThreads;
-    const int grid = int(nch < uint64_t(kMaxGrid) ? nch : uint64_t(kMaxGrid));
+    const int cap_code = (A->tune >> 2) & 7;
+    const uint64_t cap = cap_code ? (uint64_t(kMaxGrid) >> cap_code) : uint64_t(kMaxGrid);
+    const int grid = int(nch < cap ? nch : cap);
+    const int unr_code = A->tune & 3;
     const uint64_t* soff = (const uint64_t*)A->ptr;
     const DT sc = DT(A->scale);
     if(A->value_kind == FEAT_VAL_F16)
@@ -341,14 +345,28 @@ int launch_mat(hipStream_t s, const M* A, const DT* x, const Epi& epi, const Gri
     }
     if(A->index == FEAT_IDX_U32)
     {
-      hipLaunchKernelGGL((k_sell_rows<DT, DT, uint32_t, false, Epi>), dim3(grid), dim3(kThreads), 0, s, A->val,
-                         (const uint32_t*)A->col, soff, A->rowlen, x, A->rows, sc, epi, gr);
+      if(std::is_same<DT, double>::value && unr_code == 1)
+        hipLaunchKernelGGL((k_sell_rows<DT, DT, uint32_t, false, Epi, 4>), dim3(grid), dim3(kThreads), 0, s, A->val,
+                           (const uint32_t*)A->col, soff, A->rowlen, x, A->rows, sc, epi, gr);
+      else if(std::is_same<DT, double>::value && unr_code == 3)
+        hipLaunchKernelGGL((k_sell_rows<DT, DT, uint32_t, false, Epi, 16>), dim3(grid), dim3(kThreads), 0, s, A->val,
+                           (const uint32_t*)A->col, soff, A->rowlen, x, A->rows, sc, epi, gr);
+      else
+        hipLaunchKernelGGL((k_sell_rows<DT, DT, uint32_t, false, Epi>), dim3(grid), dim3(kThreads), 0, s, A->val,
+                           (const uint32_t*)A->col, soff, A->rowlen, x, A->rows, sc, epi, gr);
       return check_launch(name);
     }
     if(A->index == FEAT_IDX_D16)
     {
-      hipLaunchKernelGGL((k_sell_rows<DT, DT, int16_t, true, Epi>), dim3(grid), dim3(kThreads), 0, s, A->val,
-                         (const int16_t*)A->col, soff, A->rowlen, x, A->rows, sc, epi, gr);
+      if(std::is_same<DT, double>::value && unr_code == 1)
+        hipLaunchKernelGGL((k_sell_rows<DT, DT, int16_t, true, Epi, 4>), dim3(grid), dim3(kThreads), 0, s, A->val,
+                           (const int16_t*)A->col, soff, A->rowlen, x, A->rows, sc, epi, gr);
+      else if(std::is_same<DT, double>::value && unr_code == 3)
+        hipLaunchKernelGGL((k_sell_rows<DT, DT, int16_t, true, Epi, 16>), dim3(grid), dim3(kThreads), 0, s, A->val,
+                           (const int16_t*)A->col, soff, A->rowlen, x, A->rows, sc, epi, gr);
+      else
+        hipLaunchKernelGGL((k_sell_rows<DT, DT, int16_t, true, Epi>), dim3(grid), dim3(kThreads), 0, s, A->val,
+                           (const int16_t*)A->col, soff, A->rowlen, x, A->rows, sc, epi, gr);
       return check_launch(name);
     }
   }
@@ -1305,6 +1323,7 @@ int sell_create_impl(feat_handle_t h, const feat_csr_f64_t* A, M* out, const cha
   out->val = (decltype(out->val))(void*)sval;
   out->value_kind = std::is_same<DT, uint16_t>::value ? FEAT_VAL_F16 : FEAT_VAL_NATIVE;
   out->scale = scale;
+  out->tune = 0;
   out->col = scol;
   out->ptr = soff;
   out->rowlen = rlen;
@@ -1478,6 +1497,45 @@ int feat_matrix_apply_f64(feat_handle_t h, const feat_matrix_f64_t* A, double* r
   EpiApply<double> e{r, y, a, b, bzero};
   GridReduce g{nullptr, nullptr, nullptr, nullptr};
   return launch_mat<double>(h->stream, A, x, e, g, "feat_matrix_apply_f64");
+}
+
+int feat_autotune_matrix_f64(feat_handle_t h, feat_matrix_f64_t* A, const double* x, double* r, int reps,
+                             double* best_us)
+{
+  if(!h || !mat_ok(A) || reps < 1 || (A->rows && (!x || !r))) return fail(FEAT_ERR_INVALID, "feat_autotune_matrix_f64: bad argument");
+  if(r == x && A->rows) return fail(FEAT_ERR_ALIAS, "feat_autotune_matrix_f64: r and x must not alias");
+  A->tune = 0;
+  if(best_us) *best_us = -1.0;
+  if(A->format != FEAT_FMT_SELL || A->rows == 0) return FEAT_OK;  // only SELL launch shapes are tunable
+  const uint64_t nch = (A->rows + kThreads - 1) / kThreads;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = check_hip(hipEventCreate(&e0), "hipEventCreate");
+  if(!rc) rc = check_hip(hipEventCreate(&e1), "hipEventCreate");
+  int best = 0;
+  float best_ms = 0.0f;
+  for(int unr = 1; unr <= 3 && !rc; ++unr)
+    for(int cap = 0; cap <= 5 && !rc; ++cap)
+    {
+      if(cap && (uint64_t(kMaxGrid) >> cap) >= nch) continue;  // same launch as the uncapped grid
+      A->tune = unr | (cap << 2);
+      for(int w = 0; w < 2 && !rc; ++w) rc = feat_matrix_apply_f64(h, A, r, 1.0, x, 0.0, nullptr);
+      if(!rc) rc = check_hip(hipEventRecord(e0, h->stream), "hipEventRecord");
+      for(int k = 0; k < reps && !rc; ++k) rc = feat_matrix_apply_f64(h, A, r, 1.0, x, 0.0, nullptr);
+      if(!rc) rc = check_hip(hipEventRecord(e1, h->stream), "hipEventRecord");
+      if(!rc) rc = check_hip(hipEventSynchronize(e1), "hipEventSynchronize");
+      float ms = 0.0f;
+      if(!rc) rc = check_hip(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+      if(!rc && (best == 0 || ms < best_ms))
+      {
+        best = A->tune;
+        best_ms = ms;
+      }
+    }
+  if(e0) (void)hipEventDestroy(e0);
+  if(e1) (void)hipEventDestroy(e1);
+  A->tune = rc ? 0 : best;
+  if(!rc && best_us) *best_us = 1e3 * double(best_ms) / reps;
+  return rc;
 }
 
 int feat_defect_f64(feat_handle_t h, const feat_matrix_f64_t* A, double* d, const double* rhs, const double* x,

@@ -2,6 +2,7 @@
 // Built by g++ (no HIP headers) into libfeat3amd_host.so, linked against the
 // kernel library libfeat3amd.so; every device operation goes through feat3amd.h.
 #include <feat3amd/assembly.hpp>
+#include <feat3amd/batch.hpp>
 #include <feat3amd/lafem.hpp>
 #include <feat3amd/mixed.hpp>
 #include <feat3amd/solver.hpp>
@@ -513,6 +514,112 @@ int feat_stokes_transfer(feat_stokes_t s, int lvl, int prol, double* r, const do
 int feat_stokes_destroy(feat_stokes_t s)
 {
   return guarded([&]() { delete s; });
+}
+
+} // extern "C"
+
+struct feat_batchsolver_s
+{
+  std::unique_ptr<Solver::BatchedMgPcg> b;
+};
+
+extern "C" {
+
+int feat_batchsolver_create(feat_batchsolver_t* out, const feat_batch_cfg_t* cfg)
+{
+  return guarded([&]() {
+    if(!out || !cfg || cfg->ninst < 1 || cfg->lmin < 1 || cfg->lmax < cfg->lmin || cfg->lmax > 11)
+      throw Error("feat_batchsolver_create: bad argument");
+    Solver::BatchConfig c;
+    c.spec.seed = cfg->seed;
+    c.spec.ninst = cfg->ninst;
+    c.spec.lmin = cfg->lmin;
+    c.spec.lmax = cfg->lmax;
+    c.spec.sigma = cfg->sigma;
+    c.tol_rel = cfg->tol_rel;
+    c.max_iter = cfg->max_iter;
+    c.autotune = cfg->autotune != 0;
+    c.graphs = cfg->graphs != 0;
+    auto s = std::make_unique<feat_batchsolver_s>();
+    s->b = std::make_unique<Solver::BatchedMgPcg>(c);
+    s->b->init();
+    *out = s.release();
+  });
+}
+
+int feat_batchsolver_layout(feat_batchsolver_t s, int* levels, std::uint64_t* off)
+{
+  return guarded([&]() {
+    if(!s) throw Error("feat_batchsolver_layout: null");
+    const auto& l = s->b->levels();
+    const auto& o = s->b->offsets();
+    if(levels) std::copy(l.begin(), l.end(), levels);
+    if(off) std::copy(o.begin(), o.end(), off);
+  });
+}
+
+int feat_batchsolver_solve(feat_batchsolver_t s, double* x, const double* b, int* status, std::uint64_t* iters,
+                           double* def_init, double* def_final, double* hist)
+{
+  return guarded([&]() {
+    if(!s || !x || !b) throw Error("feat_batchsolver_solve: bad argument");
+    const auto R = s->b->solve(x, b);
+    Runtime::synchronize();
+    if(status) std::copy(R.status.begin(), R.status.end(), status);
+    if(iters) std::copy(R.iters.begin(), R.iters.end(), iters);
+    if(def_init) std::copy(R.def_init.begin(), R.def_init.end(), def_init);
+    if(def_final) std::copy(R.def_final.begin(), R.def_final.end(), def_final);
+    if(hist) std::copy(R.hist.begin(), R.hist.end(), hist);
+  });
+}
+
+int feat_batchsolver_tuning(feat_batchsolver_t s, int* tunes, double* us, int cap, int* count)
+{
+  return guarded([&]() {
+    if(!s) throw Error("feat_batchsolver_tuning: null");
+    const auto t = s->b->tunes();
+    const auto& u = s->b->tuned_us();
+    if(count) *count = int(t.size());
+    for(int i = 0; i < cap && i < int(t.size()); ++i)
+    {
+      if(tunes) tunes[i] = t[i];
+      if(us) us[i] = i < int(u.size()) ? u[i] : -1.0;
+    }
+  });
+}
+
+int feat_batchsolver_destroy(feat_batchsolver_t s)
+{
+  return guarded([&]() { delete s; });
+}
+
+int feat_batch_levels(std::uint64_t seed, int ninst, int lmin, int lmax, int* out)
+{
+  return guarded([&]() {
+    const auto l = Assembly::batch_levels(seed, ninst, lmin, lmax);
+    std::copy(l.begin(), l.end(), out);
+  });
+}
+
+int feat_lognormal_cells(std::uint64_t seed, std::uint64_t n, double sigma, double* out)
+{
+  return guarded([&]() {
+    const auto c = Assembly::lognormal_cells(seed, n, sigma);
+    std::copy(c.begin(), c.end(), out);
+  });
+}
+
+int feat_q1_diffusion_assemble(int level, int level_c, const double* coef, double* val, std::uint64_t* col,
+                               std::uint64_t* rowptr)
+{
+  return guarded([&]() {
+    if(level < 1 || level_c < level || !coef) throw Error("feat_q1_diffusion_assemble: bad argument");
+    std::vector<double> c(coef, coef + (std::size_t(1) << (2 * level_c)));
+    const auto A = Assembly::q1_diffusion_2d(level, c, level_c);
+    std::copy(A.val.begin(), A.val.end(), val);
+    if(col) std::copy(A.col.begin(), A.col.end(), col);
+    if(rowptr) std::copy(A.rowptr.begin(), A.rowptr.end(), rowptr);
+  });
 }
 
 } // extern "C"

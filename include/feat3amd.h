@@ -217,6 +217,7 @@ typedef struct {
   const uint16_t* rowlen;    /* SELL row lengths                             */
   int value_kind;            /* FEAT_VAL_NATIVE                               */
   float scale;               /* 1                                             */
+  int tune;                  /* SELL launch shape, 0 = default (autotuner)    */
 } feat_matrix_f64_t;
 
 typedef struct {             /* FP32 arithmetic (mixed-precision inner solver) */
@@ -228,6 +229,7 @@ typedef struct {             /* FP32 arithmetic (mixed-precision inner solver) *
   const uint16_t* rowlen;
   int value_kind;            /* FEAT_VAL_NATIVE or FEAT_VAL_F16 (SELL only)    */
   float scale;               /* row sums are multiplied by this power of two  */
+  int tune;                  /* SELL launch shape, 0 = default                */
 } feat_matrix_f32_t;
 
 /* A CSR descriptor (no copy) of the caller's arrays. */
@@ -394,6 +396,81 @@ typedef struct {
 int feat_vanka_numeric_f64(feat_handle_t h, const feat_vanka_graph_t* g, const feat_tuple_f64_t* K,
                            const uint32_t* vmask, double omega, int skip_singular,
                            double* local_inv, int* macro_mask, const feat_tuple_f64_t* V);
+
+/* ---------------- batches of independent solves ----------------
+ * Instances stored back to back: instance i owns elements [off[i], off[i+1]).
+ * Batch kernels run one workgroup per chunk of at most FEAT_BATCH_CHUNK elements
+ * inside one instance (chunk_inst[c], range [chunk_lo[c], chunk_hi[c])); the chunks
+ * of instance i are inst_chunk[i] .. inst_chunk[i+1]-1 in ascending order, so
+ * per-instance sums are deterministic.  partials: nchunks doubles of scratch.
+ * These entry points have no single reference function: they run, for every
+ * instance at once, the scalar operations of PCG::_apply_intern (pcg.hpp:188-288)
+ * with each instance's own scalars and stopping state. */
+#define FEAT_BATCH_CHUNK 4096
+typedef struct {
+  uint32_t ninst, nchunks;
+  const uint64_t* off;
+  const uint32_t* chunk_inst;
+  const uint64_t* chunk_lo;
+  const uint64_t* chunk_hi;
+  const uint32_t* inst_chunk;
+  double* partials;
+} feat_batch_t;
+
+/* out[i] = x_i . y_i for every instance (fma accumulation) */
+int feat_batch_dot_f64(feat_handle_t h, const feat_batch_t* b, const double* x, const double* y,
+                       double* out);
+/* per active instance (active[i] != 0): alpha = gamma[i] / tp[i];
+ * x += alpha p, r -= alpha t (DenseVector::axpy r == y branch, skipped when
+ * |alpha| < eps, dense_vector.hpp:815-820); rr[i] = r_i . r_i */
+int feat_batch_pcg_update_f64(feat_handle_t h, const feat_batch_t* b, double* x, double* r,
+                              const double* p, const double* t, const double* gamma,
+                              const double* tp, const int* active, double* rr);
+/* per active instance: p = z (first != 0) or p = (gamma_new[i]/gamma_old[i]) p + z
+ * (pcg.hpp:276-282: p.axpy(p, z, beta), the r == x branch rounds twice) */
+int feat_batch_pcg_direction_f64(feat_handle_t h, const feat_batch_t* b, double* p, const double* z,
+                                 const double* gamma_new, const double* gamma_old,
+                                 const int* active, int first);
+/* device copy of every instance's gamma_new into gamma_old for the active ones */
+int feat_batch_copy_scalars_f64(feat_handle_t h, uint32_t ninst, double* dst, const double* src,
+                                const int* active);
+
+/* Per-instance IterativeSolver state (iterative.hpp:748-885) on the device. */
+typedef struct {
+  double tol_rel, tol_abs, div_rel, div_abs;
+  uint64_t min_iter, max_iter;
+} feat_batch_stop_t;
+typedef struct {
+  double* def_init;   /* ninst */
+  double* def_cur;    /* ninst */
+  uint64_t* iters;    /* ninst */
+  int* status;        /* ninst, solver/base.hpp:48-64 codes */
+  int* active;        /* ninst: status == progress */
+  double* hist;       /* ninst * hist_cap, or NULL */
+  uint64_t hist_cap;
+  int* nactive;       /* one int: number of active instances */
+} feat_batch_state_t;
+/* initial != 0: _set_initial_defect(sqrt(rr[i])) for every instance;
+ * else _set_new_defect(sqrt(rr[i])) for the active ones */
+int feat_batch_status_f64(feat_handle_t h, uint32_t ninst, const double* rr,
+                          const feat_batch_stop_t* stop, const feat_batch_state_t* st, int initial);
+/* copies segments dst[dst_lo[c] + k] = src[src_lo[c] + k], k < len[c], c < nseg
+ * (between the PCG layout and a multigrid level's layout) */
+int feat_batch_gather_f64(feat_handle_t h, uint32_t nseg, const uint64_t* dst_lo,
+                          const uint64_t* src_lo, const uint64_t* len, double* dst,
+                          const double* src);
+/* copies a device int to pinned host memory (stream-ordered) */
+int feat_copy_int_to_host(feat_handle_t h, const int* dev, int* host_pinned);
+
+/* ---------------- autotuner ----------------
+ * Launch shapes of the SELL-64 row kernels that every fused entry point uses
+ * (entries per row in flight: 4, 8 or 16; workgroup cap: all row chunks, or
+ * 8192/4096/2048/1024/512 workgroups striding over them).  Every shape gives
+ * bit-identical results (each row is summed in CSR order).  The tuner times
+ * r = A*x for every shape with HIP events (reps launches each) and stores the
+ * fastest in A->tune; *best_us (may be NULL) receives its time per launch. */
+int feat_autotune_matrix_f64(feat_handle_t h, feat_matrix_f64_t* A, const double* x, double* r,
+                             int reps, double* best_us);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,493 @@
+// feat3amd/batch.hpp -- many independent PCG(MG-V(Richardson(Jacobi))) solves of
+// uneven size advanced together.
+//
+// Every instance runs the reference's method (solver/pcg.hpp:188-288 over
+// solver/multigrid.hpp:1270-2000, as the single-instance PCG of solver.hpp):
+// its own scalars (gamma, alpha, beta), its own stopping state
+// (iterative.hpp:748-885) and its own multigrid hierarchy from its finest
+// level down to level 1.  The batch turns this into a fixed sequence of
+// launches over all instances at once:
+//   * PCG vectors of all instances back to back ("PCG layout"); scalar work per
+//     instance through chunked kernels (feat_batch_*); an instance whose status
+//     leaves `progress` is frozen (x, r no longer touched);
+//   * multigrid level k holds every instance whose finest level is >= k, back to
+//     back; its matrix, restriction and prolongation are block-diagonal, so the
+//     fused smoother / transfer / defect kernels of the single solve run over all
+//     of them in one launch; an instance enters the V-cycle at its own finest
+//     level (its residual is copied in there) and its correction is copied out
+//     after that level's post-smoothing;
+//   * the SELL-64 launch shape of every batch matrix is picked by the autotuner;
+//   * one PCG iteration of the whole batch is recorded as a HIP graph and
+//     replayed until no instance is active.
+// The smoother skip rule (iterative.hpp:761-766: skip when the level's right-hand
+// side norm is <= eps^2) is not evaluated per instance inside the batch.
+#pragma once
+
+#include <feat3amd/batch_assembly.hpp>
+#include <feat3amd/solver.hpp>
+
+#include <algorithm>
+#include <memory>
+#include <vector>
+
+namespace feat3amd
+{
+  namespace Solver
+  {
+    struct BatchConfig
+    {
+      Assembly::BatchSpec spec;
+      int nu_pre = 4, nu_post = 4;
+      double omega_jac = 0.5, omega_rich = 1.0;
+      double tol_rel = 1e-8;
+      Index max_iter = 100;
+      bool autotune = true;
+      bool graphs = true;
+    };
+
+    class BatchedMgPcg
+    {
+      typedef LAFEM::SparseMatrixCSR<double, Index> Csr;
+      typedef LAFEM::DenseVector<double, Index> Vec;
+
+      // chunked segment structure of a back-to-back layout
+      struct Segs
+      {
+        std::uint32_t ninst = 0, nchunks = 0;
+        DeviceArray<std::uint64_t> off, lo, hi;
+        DeviceArray<std::uint32_t> cinst, ichunk;
+        DeviceArray<double> partials;
+        feat_batch_t desc{};
+        void build(const std::vector<std::uint64_t>& o)
+        {
+          ninst = std::uint32_t(o.size() - 1);
+          std::vector<std::uint64_t> l, h;
+          std::vector<std::uint32_t> ci, ic(ninst + 1, 0);
+          for(std::uint32_t i = 0; i < ninst; ++i)
+          {
+            ic[i] = std::uint32_t(l.size());
+            for(std::uint64_t a = o[i]; a < o[i + 1]; a += FEAT_BATCH_CHUNK)
+            {
+              l.push_back(a);
+              h.push_back(std::min<std::uint64_t>(a + FEAT_BATCH_CHUNK, o[i + 1]));
+              ci.push_back(i);
+            }
+          }
+          ic[ninst] = std::uint32_t(l.size());
+          nchunks = std::uint32_t(l.size());
+          off = DeviceArray<std::uint64_t>::from_host(o.data(), o.size());
+          lo = DeviceArray<std::uint64_t>::from_host(l.data(), l.size());
+          hi = DeviceArray<std::uint64_t>::from_host(h.data(), h.size());
+          cinst = DeviceArray<std::uint32_t>::from_host(ci.data(), ci.size());
+          ichunk = DeviceArray<std::uint32_t>::from_host(ic.data(), ic.size());
+          partials = DeviceArray<double>(nchunks);
+          desc = feat_batch_t{ninst, nchunks, off.get(), cinst.get(), lo.get(), hi.get(), ichunk.get(), partials.get()};
+        }
+      };
+
+      // segment copies between the PCG layout and one level's layout
+      struct Copies
+      {
+        std::uint32_t n = 0;
+        DeviceArray<std::uint64_t> lvl_lo, pcg_lo, len;
+        void build(const std::vector<std::uint64_t>& a, const std::vector<std::uint64_t>& b,
+                   const std::vector<std::uint64_t>& l)
+        {
+          n = std::uint32_t(l.size());
+          if(!n) return;
+          lvl_lo = DeviceArray<std::uint64_t>::from_host(a.data(), n);
+          pcg_lo = DeviceArray<std::uint64_t>::from_host(b.data(), n);
+          len = DeviceArray<std::uint64_t>::from_host(l.data(), n);
+        }
+      };
+
+      struct Level
+      {
+        int k = 0;
+        std::vector<int> inst;            // instances with finest level >= k
+        std::vector<std::uint64_t> loff;  // their offsets in this layout
+        Csr A, P, R;                      // P: from level k-1 to k; R = P^T
+        Vec inv, rhs, sol, def, cor, tmp;
+        DeviceArray<std::uint32_t> mask;
+        Copies top;                       // instances whose finest level is k
+      };
+
+      BatchConfig _cfg;
+      std::vector<int> _lev;
+      std::vector<std::uint64_t> _off;
+      int _lmax = 0;
+      std::vector<std::unique_ptr<Level>> _lv;  // index k (1.._lmax); 0 unused
+      Csr _A;                                    // PCG layout: each instance's finest matrix
+      DeviceArray<std::uint32_t> _mask;
+      Segs _segs;
+      Vec _r, _p, _t, _z;
+      DeviceArray<double> _gamma, _gold, _tp, _rr, _dinit, _dcur, _hist;
+      DeviceArray<std::uint64_t> _iters;
+      DeviceArray<int> _status, _active, _nactive;
+      FusedWorkspace _fw;
+      // graph replay state
+      feat_stream_t _stream = nullptr;
+      feat_handle_t _gh = nullptr;
+      feat_graph_t _graph = nullptr;
+      int* _host_nactive = nullptr;
+      const void* _key_x = nullptr;
+      std::vector<double> _tune_us;
+
+      static std::vector<std::uint32_t> unit_mask(Index n, const std::vector<std::uint64_t>& idx)
+      {
+        std::vector<std::uint32_t> m((n + 31) / 32, 0u);
+        for(auto i : idx) m[i >> 5] |= 1u << (i & 31);
+        return m;
+      }
+
+      static void append(Assembly::CsrHost& B, const Assembly::CsrHost& A, std::uint64_t roff, std::uint64_t coff)
+      {
+        if(B.rowptr.empty()) B.rowptr.push_back(0);
+        for(std::uint64_t r = 0; r < A.rows; ++r)
+        {
+          for(auto i = A.rowptr[r]; i < A.rowptr[r + 1]; ++i)
+          {
+            B.col.push_back(A.col[i] + coff);
+            B.val.push_back(A.val[i]);
+          }
+          B.rowptr.push_back(B.col.size());
+        }
+        (void)roff;
+      }
+
+    public:
+      explicit BatchedMgPcg(const BatchConfig& cfg) : _cfg(cfg) {}
+      ~BatchedMgPcg()
+      {
+        feat_graph_destroy(_graph);
+        if(_gh) feat_handle_destroy(_gh);
+        if(_stream) feat_stream_destroy(_stream);
+        if(_host_nactive) feat_host_free(_host_nactive);
+      }
+      BatchedMgPcg(const BatchedMgPcg&) = delete;
+      BatchedMgPcg& operator=(const BatchedMgPcg&) = delete;
+
+      const std::vector<int>& levels() const { return _lev; }
+      const std::vector<std::uint64_t>& offsets() const { return _off; }
+      Index size() const { return _off.back(); }
+      const std::vector<double>& tuned_us() const { return _tune_us; }
+      std::vector<int> tunes() const
+      {
+        std::vector<int> t;
+        for(int k = 1; k <= _lmax; ++k)
+        {
+          t.push_back(_lv[k]->A.tune());
+          if(k > 1)
+          {
+            t.push_back(_lv[k]->P.tune());
+            t.push_back(_lv[k]->R.tune());
+          }
+        }
+        t.push_back(_A.tune());
+        return t;
+      }
+
+      /// host assembly of every instance and level, device upload, Jacobi diagonals, autotuning
+      void init()
+      {
+        const auto& S = _cfg.spec;
+        _lev = Assembly::batch_levels(S.seed, S.ninst, S.lmin, S.lmax);
+        _lmax = *std::max_element(_lev.begin(), _lev.end());
+        const int ni = S.ninst;
+        std::vector<std::vector<double>> coef(ni);
+        for(int i = 0; i < ni; ++i)
+        {
+          const std::uint64_t nc = std::uint64_t(1) << (2 * _lev[i]);
+          coef[i] = Assembly::lognormal_cells(S.coef_seed(i), nc, S.sigma);
+        }
+        // PCG layout
+        _off.assign(1, 0);
+        Assembly::CsrHost top;
+        std::vector<std::uint64_t> topfilt;
+        for(int i = 0; i < ni; ++i)
+        {
+          const auto Ai = Assembly::q1_diffusion_2d(_lev[i], coef[i], _lev[i]);
+          append(top, Ai, _off.back(), _off.back());
+          for(auto b : Assembly::q1_boundary(2, _lev[i])) topfilt.push_back(b + _off.back());
+          _off.push_back(_off.back() + Ai.rows);
+        }
+        top.rows = top.cols = _off.back();
+        _A = Csr::from_host(top.rows, top.cols, top.val.data(), top.col.data(), top.rowptr.data());
+        _mask = DeviceArray<std::uint32_t>::from_host(unit_mask(top.rows, topfilt).data(), (top.rows + 31) / 32);
+        _segs.build(_off);
+        // multigrid levels 1.._lmax
+        _lv.clear();
+        _lv.resize(_lmax + 1);
+        for(int k = 1; k <= _lmax; ++k)
+        {
+          auto L = std::make_unique<Level>();
+          L->k = k;
+          Assembly::CsrHost Ak, Pk;
+          std::vector<std::uint64_t> filt, clo, plo, len;
+          const std::uint64_t nk = Assembly::ipow(Assembly::q1_nodes_1d(k), 2);
+          const auto bnd = Assembly::q1_boundary(2, k);
+          std::uint64_t o = 0;
+          for(int i = 0; i < ni; ++i)
+          {
+            if(_lev[i] < k) continue;
+            L->inst.push_back(i);
+            L->loff.push_back(o);
+            append(Ak, Assembly::q1_diffusion_2d(k, coef[i], _lev[i]), o, o);
+            for(auto b : bnd) filt.push_back(b + o);
+            if(_lev[i] == k)
+            {
+              clo.push_back(o);
+              plo.push_back(_off[i]);
+              len.push_back(nk);
+            }
+            o += nk;
+          }
+          Ak.rows = Ak.cols = o;
+          L->A = Csr::from_host(o, o, Ak.val.data(), Ak.col.data(), Ak.rowptr.data());
+          L->mask = DeviceArray<std::uint32_t>::from_host(unit_mask(o, filt).data(), (o + 31) / 32);
+          L->top.build(clo, plo, len);
+          for(Vec* v : {&L->inv, &L->rhs, &L->sol, &L->def, &L->cor, &L->tmp}) *v = Vec(o);
+          // Jacobi: inv = omega / diag (coarse level: one Jacobi step with omega 1)
+          L->A.extract_diag(L->inv);
+          L->inv.component_invert(L->inv, k == 1 ? 1.0 : _cfg.omega_jac);
+          if(k > 1)
+          {
+            // prolongation from level k-1: rows = this layout, columns = level k-1 layout
+            const Level& C = *_lv[k - 1];
+            const auto P1 = Assembly::q1_prolongation(2, k);
+            for(std::size_t j = 0; j < L->inst.size(); ++j)
+            {
+              const int i = L->inst[j];
+              const auto it = std::find(C.inst.begin(), C.inst.end(), i);
+              append(Pk, P1, L->loff[j], C.loff[std::size_t(it - C.inst.begin())]);
+            }
+            Pk.rows = o;
+            Pk.cols = C.A.rows();
+            L->P = Csr::from_host(Pk.rows, Pk.cols, Pk.val.data(), Pk.col.data(), Pk.rowptr.data());
+            const auto Rk = Assembly::transpose(Pk);
+            L->R = Csr::from_host(Rk.rows, Rk.cols, Rk.val.data(), Rk.col.data(), Rk.rowptr.data());
+          }
+          _lv[k] = std::move(L);
+        }
+        for(Vec* v : {&_r, &_p, &_t, &_z}) *v = Vec(size());
+        for(DeviceArray<double>* a : {&_gamma, &_gold, &_tp, &_rr, &_dinit, &_dcur}) *a = DeviceArray<double>(ni);
+        _hist = DeviceArray<double>(Index(ni) * (_cfg.max_iter + 1));
+        _iters = DeviceArray<std::uint64_t>(ni);
+        _status = DeviceArray<int>(ni);
+        _active = DeviceArray<int>(ni);
+        _nactive = DeviceArray<int>(1);
+        _fw.create(4);
+        // execution plans (and their launch shapes) before any recording
+        _tune_us.clear();
+        auto plan = [&](const Csr& m) {
+          if(_cfg.autotune)
+            _tune_us.push_back(m.autotune());
+          else
+            (void)m.exec();
+        };
+        for(int k = 1; k <= _lmax; ++k)
+        {
+          plan(_lv[k]->A);
+          if(k > 1)
+          {
+            plan(_lv[k]->P);
+            plan(_lv[k]->R);
+          }
+        }
+        plan(_A);
+        for(Vec* v : {&_r, &_p, &_t, &_z}) v->format();
+        {
+          void* hp = nullptr;
+          check(feat_host_malloc(&hp, 64), "batch pinned");
+          _host_nactive = static_cast<int*>(hp);
+        }
+        if(_cfg.graphs)
+        {
+          check(feat_stream_create(&_stream), "batch stream");
+          check(feat_handle_create(&_gh, 0, _stream), "batch handle");
+        }
+        Runtime::synchronize();
+      }
+
+      /// z = MG(r) for every instance (fused kernels of the single solve over block-diagonal levels)
+      void mg_apply(double* z, const double* r)
+      {
+        feat_handle_t h = Runtime::handle();
+        const feat_reduce_ws_t* ws = &_fw.ws;
+        for(int k = _lmax; k >= 2; --k)
+        {
+          Level& L = *_lv[k];
+          if(L.top.n)
+            check(feat_batch_gather_f64(h, L.top.n, L.top.lvl_lo.get(), L.top.pcg_lo.get(), L.top.len.get(),
+                                        L.rhs.elements(), r),
+                  "batch inject");
+          const feat_matrix_f64_t& A = L.A.exec();
+          check(feat_rj_smooth_f64(h, &A, L.inv.elements(), L.mask.get(), _cfg.nu_pre, _cfg.omega_rich, L.rhs.elements(),
+                                   nullptr, L.sol.elements(), L.tmp.elements(), L.def.elements(), nullptr, nullptr,
+                                   nullptr, ws),
+                "batch pre-smoothing");
+          Level& C = *_lv[k - 1];
+          const feat_matrix_f64_t& R = L.R.exec();
+          check(feat_apply_masked_f64(h, &R, C.rhs.elements(), L.def.elements(), C.mask.get(), nullptr, nullptr,
+                                      nullptr, ws),
+                "batch restriction");
+        }
+        {
+          Level& L = *_lv[1];
+          if(L.top.n)
+            check(feat_batch_gather_f64(h, L.top.n, L.top.lvl_lo.get(), L.top.pcg_lo.get(), L.top.len.get(),
+                                        L.rhs.elements(), r),
+                  "batch inject");
+          const feat_matrix_f64_t& A = L.A.exec();
+          check(feat_rj_smooth_f64(h, &A, L.inv.elements(), L.mask.get(), 1, 1.0, L.rhs.elements(), nullptr,
+                                   L.sol.elements(), L.tmp.elements(), nullptr, nullptr, nullptr, nullptr, ws),
+                "batch coarse Jacobi");
+          if(L.top.n)
+            check(feat_batch_gather_f64(h, L.top.n, L.top.pcg_lo.get(), L.top.lvl_lo.get(), L.top.len.get(), z,
+                                        L.sol.elements()),
+                  "batch extract");
+        }
+        for(int k = 2; k <= _lmax; ++k)
+        {
+          Level& L = *_lv[k];
+          const feat_matrix_f64_t& P = L.P.exec();
+          check(feat_prol_add_f64(h, &P, L.sol.elements(), _lv[k - 1]->sol.elements(), L.mask.get()), "batch prolongation");
+          const feat_matrix_f64_t& A = L.A.exec();
+          check(feat_defect_f64(h, &A, L.def.elements(), L.rhs.elements(), L.sol.elements(), L.mask.get(), nullptr, ws),
+                "batch defect");
+          check(feat_rj_smooth_f64(h, &A, L.inv.elements(), L.mask.get(), _cfg.nu_post, _cfg.omega_rich, L.def.elements(),
+                                   nullptr, L.cor.elements(), L.tmp.elements(), nullptr, L.sol.elements(), nullptr,
+                                   nullptr, ws),
+                "batch post-smoothing");
+          if(L.top.n)
+            check(feat_batch_gather_f64(h, L.top.n, L.top.pcg_lo.get(), L.top.lvl_lo.get(), L.top.len.get(), z,
+                                        L.sol.elements()),
+                  "batch extract");
+        }
+      }
+
+      struct Result
+      {
+        std::vector<int> status;
+        std::vector<std::uint64_t> iters;
+        std::vector<double> def_init, def_final, hist;
+      };
+
+    private:
+      feat_batch_state_t state() const
+      {
+        return feat_batch_state_t{_dinit.get(), _dcur.get(), _iters.get(), _status.get(), _active.get(), _hist.get(),
+                                  _cfg.max_iter + 1, _nactive.get()};
+      }
+      feat_batch_stop_t stop() const
+      {
+        const double e = 2.220446049250313e-16;
+        return feat_batch_stop_t{_cfg.tol_rel, 1.0 / (e * e), 1.0 / e, 1.0 / (e * e), 0, _cfg.max_iter};
+      }
+
+      // t = A p, tp, update, status, nactive -> host, z = MG(r), gamma_new, direction, gamma_old = gamma
+      void body(double* x, feat_handle_t h, int* host_nactive)
+      {
+        const feat_reduce_ws_t* ws = &_fw.ws;
+        const feat_batch_t* b = &_segs.desc;
+        const feat_batch_stop_t sp = stop();
+        const feat_batch_state_t st = state();
+        const feat_matrix_f64_t& A = _A.exec();
+        check(feat_apply_masked_f64(h, &A, _t.elements(), _p.elements(), _mask.get(), nullptr, nullptr, nullptr, ws), "batch A*p");
+        check(feat_batch_dot_f64(h, b, _t.elements(), _p.elements(), _tp.get()), "batch t.p");
+        check(feat_batch_pcg_update_f64(h, b, x, _r.elements(), _p.elements(), _t.elements(), _gamma.get(), _tp.get(),
+                                        _active.get(), _rr.get()),
+              "batch update");
+        check(feat_batch_status_f64(h, _segs.ninst, _rr.get(), &sp, &st, 0), "batch status");
+        check(feat_copy_int_to_host(h, _nactive.get(), host_nactive), "batch nactive");
+        mg_apply(_z.elements(), _r.elements());
+        check(feat_batch_dot_f64(h, b, _r.elements(), _z.elements(), _tp.get()), "batch gamma");
+        check(feat_batch_pcg_direction_f64(h, b, _p.elements(), _z.elements(), _tp.get(), _gamma.get(), _active.get(), 0),
+              "batch direction");
+        check(feat_batch_copy_scalars_f64(h, _segs.ninst, _gamma.get(), _tp.get(), _active.get()), "batch gamma copy");
+      }
+
+    public:
+      /// x (initial guesses, overwritten) and b in the PCG layout (device)
+      Result solve(double* x, const double* b)
+      {
+        feat_handle_t caller = Runtime::handle();
+        const bool use_graph = _cfg.graphs && _gh;
+        feat_handle_t h = use_graph ? _gh : caller;
+        Runtime::Use scope(h);
+        if(use_graph) check(feat_handle_wait(h, caller), "batch: order after caller");
+        const feat_reduce_ws_t* ws = &_fw.ws;
+        const feat_batch_t* bd = &_segs.desc;
+        const feat_batch_stop_t sp = stop();
+        const feat_batch_state_t st = state();
+        int* hn = _host_nactive;
+        // r = b - A x, initial defects (pcg.hpp:159-173, iterative.hpp:748-770)
+        const feat_matrix_f64_t& A = _A.exec();
+        check(feat_defect_f64(h, &A, _r.elements(), b, x, _mask.get(), nullptr, ws), "batch defect");
+        check(feat_batch_dot_f64(h, bd, _r.elements(), _r.elements(), _rr.get()), "batch r.r");
+        check(feat_batch_status_f64(h, _segs.ninst, _rr.get(), &sp, &st, 1), "batch status");
+        // z = MG(r), gamma = r.z, p = z
+        mg_apply(_z.elements(), _r.elements());
+        check(feat_batch_dot_f64(h, bd, _r.elements(), _z.elements(), _gamma.get()), "batch gamma");
+        check(feat_batch_pcg_direction_f64(h, bd, _p.elements(), _z.elements(), nullptr, nullptr, _active.get(), 1),
+              "batch direction");
+        if(use_graph)
+        {
+          check(feat_copy_int_to_host(h, _nactive.get(), hn), "batch nactive");
+          check(feat_sync(h), "batch sync");
+          if(_key_x != x)
+          {
+            feat_graph_destroy(_graph);
+            _graph = nullptr;
+            _key_x = x;
+          }
+          if(!_graph)
+          {
+            check(feat_graph_begin(h), "batch graph begin");
+            try
+            {
+              body(x, h, hn);
+            }
+            catch(...)
+            {
+              feat_graph_t g = nullptr;
+              feat_graph_end(h, &g);
+              feat_graph_destroy(g);
+              throw;
+            }
+            check(feat_graph_end(h, &_graph), "batch graph end");
+          }
+          while(*hn > 0)
+          {
+            check(feat_graph_launch(h, _graph), "batch graph launch");
+            check(feat_sync(h), "batch sync");
+          }
+          check(feat_handle_wait(caller, h), "batch: order before caller");
+        }
+        else
+        {
+          check(feat_copy_int_to_host(h, _nactive.get(), hn), "batch nactive");
+          check(feat_sync(h), "batch sync");
+          while(*hn > 0)
+          {
+            body(x, h, hn);
+            check(feat_sync(h), "batch sync");
+          }
+        }
+        Result R;
+        const Index ni = _segs.ninst;
+        R.status.resize(ni);
+        R.iters.resize(ni);
+        R.def_init.resize(ni);
+        R.def_final.resize(ni);
+        R.hist.resize(ni * (_cfg.max_iter + 1));
+        _status.to_host(R.status.data());
+        _iters.to_host(R.iters.data());
+        _dinit.to_host(R.def_init.data());
+        _dcur.to_host(R.def_final.data());
+        _hist.to_host(R.hist.data());
+        return R;
+      }
+    };
+  } // namespace Solver
+} // namespace feat3amd

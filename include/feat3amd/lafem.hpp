@@ -323,6 +323,21 @@ namespace feat3amd
         return *_plan;
       }
 
+      /// Picks the fastest launch shape of the execution plan's row kernels
+      /// (feat_autotune_matrix_f64; bit-identical results for every shape).
+      /// Returns the time per SpMV of the chosen shape in microseconds (-1: nothing to tune).
+      double autotune(int reps = 20) const
+      {
+        const feat_matrix_f64_t& e = exec();
+        DenseVector<DT_, IT_> x(_cols, DT_(1)), r(_rows);
+        double us = -1.0;
+        check(feat_autotune_matrix_f64(Runtime::handle(), const_cast<feat_matrix_f64_t*>(&e), (const double*)x.elements(),
+                                       (double*)r.elements(), reps, &us),
+              "autotune");
+        return us;
+      }
+      int tune() const { return exec().tune; }
+
       /// FP32 SELL-64 plan (values rounded once) for mixed-precision inner solvers.
       const feat_matrix_f32_t& exec32() const
       {

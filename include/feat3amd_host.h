@@ -137,6 +137,39 @@ int feat_stokes_filter(feat_stokes_t s, int lvl, double* v, int kind);
 int feat_stokes_transfer(feat_stokes_t s, int lvl, int prol, double* r, const double* x);
 int feat_stokes_destroy(feat_stokes_t s);
 
+/* ---- batch of independent 2D Q1 diffusion problems (uneven sizes) ----
+ * Instance i: level levels[i] (uniform in [lmin, lmax] from `seed`), log-normal
+ * cell coefficient exp(sigma z) (seed + 1000 + i), PCG(MG-V(4,4) Richardson(Jacobi
+ * 0.5), coarse level 1 Jacobi) to tol_rel, all instances advanced together. */
+typedef struct {
+  uint64_t seed;
+  int ninst, lmin, lmax;
+  double sigma;
+  double tol_rel;
+  uint64_t max_iter;
+  int autotune;   /* 1: autotune every batch matrix's launch shape at setup */
+  int graphs;     /* 1: replay one recorded PCG iteration per step */
+} feat_batch_cfg_t;
+typedef struct feat_batchsolver_s* feat_batchsolver_t;
+int feat_batchsolver_create(feat_batchsolver_t* s, const feat_batch_cfg_t* cfg);
+/* levels (ninst ints) and PCG-layout offsets (ninst + 1) */
+int feat_batchsolver_layout(feat_batchsolver_t s, int* levels, uint64_t* off);
+/* x, b: device arrays in the PCG layout; per-instance outputs are host arrays
+ * (ninst entries each, hist ninst * (max_iter + 1), any may be NULL) */
+int feat_batchsolver_solve(feat_batchsolver_t s, double* x, const double* b, int* status, uint64_t* iters,
+                           double* def_init, double* def_final, double* hist);
+/* chosen launch shapes and their SpMV times (us) of the batch matrices:
+ * per level k = 1..lmax: A_k (and P_k, R_k for k > 1), then the PCG-layout matrix */
+int feat_batchsolver_tuning(feat_batchsolver_t s, int* tunes, double* us, int cap, int* count);
+int feat_batchsolver_destroy(feat_batchsolver_t s);
+/* setup pieces of one instance, for tests */
+int feat_batch_levels(uint64_t seed, int ninst, int lmin, int lmax, int* out);
+int feat_lognormal_cells(uint64_t seed, uint64_t n, double sigma, double* out);
+/* Q1 diffusion matrix on `level` for a per-cell coefficient of `level_c` (>= level);
+ * the pattern (rowptr, col) is the Q1 pattern of feat_q1_assemble(2, level) */
+int feat_q1_diffusion_assemble(int level, int level_c, const double* coef, double* val, uint64_t* col,
+                               uint64_t* rowptr);
+
 #ifdef __cplusplus
 }
 #endif
