@@ -163,6 +163,12 @@ KERNEL_API = [
     ("feat_batch_gather_f64", _i, [_vp, C.c_uint32, _vp, _vp, _vp, _vp, _vp]),
     ("feat_copy_int_to_host", _i, [_vp, _vp, _vp]),
     ("feat_autotune_matrix_f64", _i, [_vp, _vp, _vp, _vp, _i, _vp]),
+    ("feat_mlp_to_bf16", _i, [_vp, _vp, _vp]),
+    ("feat_mlp_infer", _i, [_vp, _vp, _vp, _vp, _u64, _vp]),
+    ("feat_mlp_grad", _i, [_vp, _vp, _vp, _vp, _vp, _u64, _vp, _u64, _vp]),
+    ("feat_mlp_adam", _i, [_vp, _vp, _vp, _vp, _vp, _vp, _i, _f, _f, _f, _f]),
+    ("feat_mlp_features", _i, [_vp, _u64, _vp, _vp, _vp, _vp, _vp, _vp, _u64, _vp, _vp]),
+    ("feat_mlp_smooth_f64", _i, [_vp, _u64, _vp, _vp, _vp, _vp, _vp, _d, _vp]),
 ]
 
 HOST_API = [
@@ -192,6 +198,14 @@ HOST_API = [
     ("feat_batch_levels", _i, [_u64, _i, _i, _i, _vp]),
     ("feat_lognormal_cells", _i, [_u64, _u64, _d, _vp]),
     ("feat_q1_diffusion_assemble", _i, [_i, _i, _vp, _vp, _vp, _vp]),
+    ("feat_learned_create", _i, [C.POINTER(_vp), _vp]),
+    ("feat_learned_generate", _i, [_vp, C.POINTER(_d)]),
+    ("feat_learned_train", _i, [_vp, _i, _u64, _vp]),
+    ("feat_learned_solve", _i, [_vp, _i, _i, _vp, _vp, _vp, _u64]),
+    ("feat_learned_size", _i, [_vp, C.POINTER(_u64), C.POINTER(_u64)]),
+    ("feat_learned_params", _i, [_vp, _vp, _vp]),
+    ("feat_learned_param_ptrs", _i, [_vp, C.POINTER(_vp), C.POINTER(_vp)]),
+    ("feat_learned_destroy", _i, [_vp]),
     ("feat_mgpcg_create", _i, [C.POINTER(_vp), C.POINTER(LevelHost), _i, C.POINTER(MgPcgCfg)]),
     ("feat_mgpcg_create_q1", _i, [C.POINTER(_vp), _i, _i, _i, C.POINTER(MgPcgCfg)]),
     ("feat_mgpcg_solve", _i, [_vp, _vp, _vp, C.POINTER(SolveResult), _vp, _u64]),
@@ -587,6 +601,80 @@ class BatchSolver:
     def close(self):
         if self.s:
             self.hl.feat_batchsolver_destroy(self.s)
+            self.s = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class LearnedCfg(C.Structure):
+    _fields_ = [("level_fine", _i), ("level_coarse", _i), ("sigma", _d), ("kz", _d), ("seed", _u64), ("nu", _i),
+                ("omega_jac", _d), ("omega_learned", _d), ("tol_rel", _d), ("max_iter", _u64), ("n_init", _i),
+                ("n_snap", _i), ("batch", _u64), ("lr", _f)]
+
+
+class LearnedResult(C.Structure):
+    _fields_ = [("status", _i), ("iters", _u64), ("fell_back", _i), ("fallback_iter", _u64), ("def_init", _d),
+                ("def_final", _d), ("seconds", _d)]
+
+
+MLP_NPARAM, MLP_NPARAM_PAD = 8385, 8448
+
+
+class LearnedMG:
+    """Outer Richardson(MG) on the 3D Q1 anisotropic variable-coefficient problem with
+    a Jacobi or a learned (BF16 MLP) finest-level smoother (C++ host layer)."""
+
+    def __init__(self, level_fine=6, level_coarse=1, sigma=1.5, kz=10.0, seed=28054777172512, nu=4, omega_jac=0.5,
+                 omega_learned=1.0, tol_rel=1e-8, max_iter=200, n_init=4, n_snap=3, batch=8192, lr=1e-3):
+        _, self.hl = libs()
+        runtime_init()
+        self.s = C.c_void_p()
+        self.cfg = LearnedCfg(level_fine, level_coarse, sigma, kz, seed, nu, omega_jac, omega_learned, tol_rel, max_iter,
+                              n_init, n_snap, batch, lr)
+        check(self.hl.feat_learned_create(C.byref(self.s), C.byref(self.cfg)), "feat_learned_create")
+        n, m = _u64(), _u64()
+        check(self.hl.feat_learned_size(self.s, C.byref(n), C.byref(m)), "feat_learned_size")
+        self.n, self.m = n.value, m.value
+
+    def generate(self):
+        rho = C.c_double()
+        check(self.hl.feat_learned_generate(self.s, C.byref(rho)), "feat_learned_generate")
+        return rho.value
+
+    def train(self, steps, seed=1):
+        loss = np.zeros(max(steps, 1), np.float32)
+        check(self.hl.feat_learned_train(self.s, steps, seed, loss.ctypes.data), "feat_learned_train")
+        return loss[:steps]
+
+    def solve(self, learned, allow_fallback=True, x=None, hist_cap=1000):
+        res = LearnedResult()
+        hist = np.full(hist_cap, np.nan)
+        check(self.hl.feat_learned_solve(self.s, int(learned), int(allow_fallback), ptr(x) if x is not None else None,
+                                         C.byref(res), hist.ctypes.data, hist_cap), "feat_learned_solve")
+        it = int(res.iters)
+        return dict(status=STATUS[res.status], iters=it, fell_back=bool(res.fell_back),
+                    fallback_iter=int(res.fallback_iter), def_init=res.def_init, def_final=res.def_final,
+                    seconds=res.seconds, hist=hist[:it + 1])
+
+    def params(self, new=None):
+        cur = np.zeros(MLP_NPARAM, np.float32)
+        nw = None if new is None else np.ascontiguousarray(new, dtype=np.float32)
+        check(self.hl.feat_learned_params(self.s, cur.ctypes.data, nw.ctypes.data if nw is not None else None),
+              "feat_learned_params")
+        return cur
+
+    def param_ptrs(self):
+        p, w = _vp(), _vp()
+        check(self.hl.feat_learned_param_ptrs(self.s, C.byref(p), C.byref(w)), "feat_learned_param_ptrs")
+        return p, w
+
+    def close(self):
+        if self.s:
+            self.hl.feat_learned_destroy(self.s)
             self.s = C.c_void_p()
 
     def __del__(self):

@@ -3,6 +3,7 @@
 // kernel library libfeat3amd.so; every device operation goes through feat3amd.h.
 #include <feat3amd/assembly.hpp>
 #include <feat3amd/batch.hpp>
+#include <feat3amd/learned.hpp>
 #include <feat3amd/lafem.hpp>
 #include <feat3amd/mixed.hpp>
 #include <feat3amd/solver.hpp>
@@ -620,6 +621,127 @@ int feat_q1_diffusion_assemble(int level, int level_c, const double* coef, doubl
     if(col) std::copy(A.col.begin(), A.col.end(), col);
     if(rowptr) std::copy(A.rowptr.begin(), A.rowptr.end(), rowptr);
   });
+}
+
+} // extern "C"
+
+struct feat_learned_s
+{
+  std::unique_ptr<Solver::LearnedMG> mg;
+  LAFEM::DenseVector<double, Index> x0;
+};
+
+extern "C" {
+
+int feat_learned_create(feat_learned_t* out, const feat_learned_cfg_t* cfg)
+{
+  return guarded([&]() {
+    if(!out || !cfg || cfg->level_coarse < 1 || cfg->level_fine < cfg->level_coarse || cfg->level_fine > 8 || cfg->batch < 1)
+      throw Error("feat_learned_create: bad argument");
+    Solver::LearnedConfig c;
+    c.level_fine = cfg->level_fine;
+    c.level_coarse = cfg->level_coarse;
+    c.sigma = cfg->sigma;
+    c.kz = cfg->kz;
+    c.seed = cfg->seed;
+    c.nu = cfg->nu;
+    c.omega_jac = cfg->omega_jac;
+    c.omega_learned = cfg->omega_learned;
+    c.tol_rel = cfg->tol_rel;
+    c.max_iter = cfg->max_iter;
+    c.n_init = cfg->n_init;
+    c.n_snap = cfg->n_snap;
+    c.batch = cfg->batch;
+    c.lr = cfg->lr;
+    auto s = std::make_unique<feat_learned_s>();
+    s->mg = std::make_unique<Solver::LearnedMG>(c);
+    s->mg->init();
+    s->x0 = LAFEM::DenseVector<double, Index>(s->mg->size());
+    *out = s.release();
+  });
+}
+
+int feat_learned_generate(feat_learned_t s, double* rho)
+{
+  return guarded([&]() {
+    if(!s) throw Error("feat_learned_generate: null");
+    s->mg->generate_data();
+    if(rho) *rho = s->mg->rho_jacobi();
+  });
+}
+
+int feat_learned_train(feat_learned_t s, int steps, std::uint64_t seed, float* loss)
+{
+  return guarded([&]() {
+    if(!s || steps < 0) throw Error("feat_learned_train: bad argument");
+    const std::size_t before = s->mg->loss_history().size();
+    s->mg->train(steps, seed);
+    const auto& h = s->mg->loss_history();
+    if(loss)
+      for(int i = 0; i < steps; ++i) loss[i] = h[before + std::size_t(i)];
+  });
+}
+
+int feat_learned_solve(feat_learned_t s, int learned, int allow_fallback, double* x, feat_learned_result_t* res,
+                       double* hist, std::uint64_t hist_cap)
+{
+  return guarded([&]() {
+    if(!s || !res) throw Error("feat_learned_solve: bad argument");
+    LAFEM::DenseVector<double, Index> vx;
+    if(x)
+      vx = LAFEM::DenseVector<double, Index>::wrap(x, s->mg->size());
+    else
+    {
+      s->x0.format();
+      vx = s->x0;
+    }
+    const auto info = s->mg->solve(vx, s->mg->rhs(), learned != 0, allow_fallback != 0);
+    res->status = int(info.status);
+    res->iters = info.iters;
+    res->fell_back = info.fell_back ? 1 : 0;
+    res->fallback_iter = info.fallback_iter;
+    res->def_init = info.def_init;
+    res->def_final = info.def_final;
+    res->seconds = info.seconds;
+    if(hist)
+      for(std::size_t i = 0; i < info.hist.size() && i < hist_cap; ++i) hist[i] = info.hist[i];
+  });
+}
+
+int feat_learned_size(feat_learned_t s, std::uint64_t* n, std::uint64_t* m)
+{
+  return guarded([&]() {
+    if(!s) throw Error("feat_learned_size: null");
+    if(n) *n = s->mg->size();
+    if(m) *m = s->mg->nodes_1d();
+  });
+}
+
+int feat_learned_params(feat_learned_t s, float* get, const float* set)
+{
+  return guarded([&]() {
+    if(!s) throw Error("feat_learned_params: null");
+    if(get) s->mg->get_params(get);
+    if(set)
+    {
+      s->mg->set_params(set);
+      Runtime::synchronize();
+    }
+  });
+}
+
+int feat_learned_param_ptrs(feat_learned_t s, float** params, std::uint16_t** wbf)
+{
+  return guarded([&]() {
+    if(!s) throw Error("feat_learned_param_ptrs: null");
+    if(params) *params = s->mg->params();
+    if(wbf) *wbf = s->mg->params_bf16();
+  });
+}
+
+int feat_learned_destroy(feat_learned_t s)
+{
+  return guarded([&]() { delete s; });
 }
 
 } // extern "C"

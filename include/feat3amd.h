@@ -472,6 +472,43 @@ int feat_copy_int_to_host(feat_handle_t h, const int* dev, int* host_pinned);
 int feat_autotune_matrix_f64(feat_handle_t h, feat_matrix_f64_t* A, const double* x, double* r,
                              int reps, double* best_us);
 
+/* ---------------- learned smoother network ----------------
+ * No counterpart in the reference.  MLP 54 -> 64 -> 64 -> 1, GELU, on BF16 matrix
+ * cores (FP32 accumulation).  Parameters (FP32 master, FEAT_MLP_NPARAM floats):
+ *   W1[64][64] (row = hidden unit, inputs 54..63 are padding and stay 0) at 0,
+ *   b1[64] at 4096, W2[64][64] at 4160, b2[64] at 8256, w3[64] at 8320, b3 at 8384.
+ * wbf: BF16 copies of W1 then W2 (2 * 4096 values).  Network:
+ *   z1 = W1 x + b1, h1 = bf16(gelu(z1)), z2 = W2 h1 + b2, y = w3 . gelu(z2) + b3,
+ * x a row of 64 BF16 inputs.  Training loss: mean of (y - t)^2. */
+#define FEAT_MLP_IN 54
+#define FEAT_MLP_HID 64
+#define FEAT_MLP_NPARAM 8385
+#define FEAT_MLP_NPARAM_PAD 8448
+int feat_mlp_to_bf16(feat_handle_t h, const float* params, uint16_t* wbf);
+int feat_mlp_infer(feat_handle_t h, const float* params, const uint16_t* wbf, const uint16_t* X,
+                   uint64_t nsamp, float* y);
+/* grad (FEAT_MLP_NPARAM_PAD floats; grad[FEAT_MLP_NPARAM_PAD-1] = sum of squared
+ * errors) of the mean squared error over nsamp samples; partials: scratch of
+ * partials_cap * FEAT_MLP_NPARAM_PAD floats, partials_cap >= 4 * min(2048, ceil(nsamp/256)) */
+int feat_mlp_grad(feat_handle_t h, const float* params, const uint16_t* wbf, const uint16_t* X,
+                  const float* t, uint64_t nsamp, float* partials, uint64_t partials_cap, float* grad);
+/* one Adam step (bias-corrected, step >= 1); refreshes wbf */
+int feat_mlp_adam(feat_handle_t h, float* params, uint16_t* wbf, const float* grad, float* m, float* v,
+                  int step, float lr, float beta1, float beta2, float eps);
+/* Inputs of the learned smoother on a structured m^3 node grid (lexicographic, x
+ * fastest): for interior node i, rho_j = r_j / a_ii over its 3x3x3 patch (j = dz,dy,dx
+ * lexicographic), s = max |rho_j|; x = [rho / s (27), cfeat over the patch (27), 0 (10)].
+ * Training samples: node[k] of snapshot snap[k] of R and E (snapshots of n = m^3
+ * doubles); target t = e_i / s.  Rows of boundary nodes or s == 0 are zero (t = 0). */
+int feat_mlp_features(feat_handle_t h, uint64_t m, const double* R, const double* E,
+                      const double* dinv, const float* cfeat, const uint64_t* node,
+                      const uint32_t* snap, uint64_t nsamp, uint16_t* X, float* t);
+/* learned smoothing step: x_i += omega * s_i * y_i for every interior node, y the
+ * network output for the patch of the defect r (dinv = 1 / diag(A)) */
+int feat_mlp_smooth_f64(feat_handle_t h, uint64_t m, const float* params, const uint16_t* wbf,
+                        const double* r, const double* dinv, const float* cfeat, double omega,
+                        double* x);
+
 #ifdef __cplusplus
 }
 #endif

@@ -96,6 +96,110 @@ namespace feat3amd
       return A;
     }
 
+    /// 3D Q1 diffusion matrix -div(c K grad u), K = diag(1, 1, kz), on `level` of the unit
+    /// cube for a coefficient given per cell of `level_c` >= level (2x2x2 Gauss, every Gauss
+    /// point samples the finest cell containing it).  Pattern: the 27-point pattern of
+    /// q1_laplace(3, level).
+    inline CsrHost q1_diffusion_3d(int level, const std::vector<double>& coef, int level_c, double kz)
+    {
+      CsrHost A = q1_laplace(3, level);
+      for(auto& v : A.val) v = 0.0;
+      const std::uint64_t m = q1_nodes_1d(level), N = m - 1, Nc = std::uint64_t(1) << level_c;
+      const double h = 1.0 / double(N);
+      const double g = 1.0 / std::sqrt(3.0);
+      // gradient products of the trilinear basis at the 8 Gauss points (reference cube [-1,1]^3)
+      static double G[8][8][8];
+      static double lastkz = -1.0;
+      if(lastkz != kz)
+      {
+        for(int q = 0; q < 8; ++q)
+        {
+          const double p[3] = {(q & 1) ? g : -g, (q & 2) ? g : -g, (q & 4) ? g : -g};
+          double d[8][3];
+          for(int a = 0; a < 8; ++a)
+          {
+            double l[3], dl[3];
+            for(int k = 0; k < 3; ++k)
+            {
+              const int bit = (a >> k) & 1;
+              l[k] = bit ? 0.5 * (1.0 + p[k]) : 0.5 * (1.0 - p[k]);
+              dl[k] = bit ? 0.5 : -0.5;
+            }
+            d[a][0] = dl[0] * l[1] * l[2];
+            d[a][1] = l[0] * dl[1] * l[2];
+            d[a][2] = l[0] * l[1] * dl[2];
+          }
+          for(int a = 0; a < 8; ++a)
+            for(int b = 0; b < 8; ++b) G[q][a][b] = d[a][0] * d[b][0] + d[a][1] * d[b][1] + kz * (d[a][2] * d[b][2]);
+        }
+        lastkz = kz;
+      }
+      auto pos = [&](std::uint64_t r, std::uint64_t c) {
+        std::uint64_t k = A.rowptr[r];
+        while(A.col[k] != c) ++k;
+        return k;
+      };
+      // (2/h)^2 from the gradients times (h/2)^3 from the volume
+      const double scale = 0.5 * h;
+      for(std::uint64_t ez = 0; ez < N; ++ez)
+        for(std::uint64_t ey = 0; ey < N; ++ey)
+          for(std::uint64_t ex = 0; ex < N; ++ex)
+          {
+            double cq[8];
+            for(int q = 0; q < 8; ++q)
+            {
+              const double x = (double(ex) + 0.5 + 0.5 * ((q & 1) ? g : -g)) * h;
+              const double y = (double(ey) + 0.5 + 0.5 * ((q & 2) ? g : -g)) * h;
+              const double z = (double(ez) + 0.5 + 0.5 * ((q & 4) ? g : -g)) * h;
+              std::uint64_t ix = std::uint64_t(x * double(Nc)), iy = std::uint64_t(y * double(Nc)),
+                            iz = std::uint64_t(z * double(Nc));
+              ix = ix < Nc ? ix : Nc - 1;
+              iy = iy < Nc ? iy : Nc - 1;
+              iz = iz < Nc ? iz : Nc - 1;
+              cq[q] = coef[(iz * Nc + iy) * Nc + ix];
+            }
+            std::uint64_t node[8];
+            for(int a = 0; a < 8; ++a)
+              node[a] = ((ez + std::uint64_t((a >> 2) & 1)) * m + ey + std::uint64_t((a >> 1) & 1)) * m + ex +
+                        std::uint64_t(a & 1);
+            for(int a = 0; a < 8; ++a)
+              for(int b = 0; b < 8; ++b)
+              {
+                double k = 0.0;
+                for(int q = 0; q < 8; ++q) k = std::fma(cq[q], G[q][a][b], k);
+                A.val[pos(node[a], node[b])] += scale * k;
+              }
+          }
+      return A;
+    }
+
+    /// node feature of a cell coefficient field: log of the mean coefficient of the
+    /// (up to 8) cells around each node of level `level_c`
+    inline std::vector<float> node_log_coef_3d(const std::vector<double>& coef, int level_c)
+    {
+      const std::uint64_t Nc = std::uint64_t(1) << level_c, m = Nc + 1;
+      std::vector<float> f(m * m * m);
+      for(std::uint64_t z = 0; z < m; ++z)
+        for(std::uint64_t y = 0; y < m; ++y)
+          for(std::uint64_t x = 0; x < m; ++x)
+          {
+            double s = 0.0;
+            int n = 0;
+            for(int dz = -1; dz <= 0; ++dz)
+              for(int dy = -1; dy <= 0; ++dy)
+                for(int dx = -1; dx <= 0; ++dx)
+                {
+                  const std::int64_t cx = std::int64_t(x) + dx, cy = std::int64_t(y) + dy, cz = std::int64_t(z) + dz;
+                  if(cx < 0 || cy < 0 || cz < 0 || cx >= std::int64_t(Nc) || cy >= std::int64_t(Nc) || cz >= std::int64_t(Nc))
+                    continue;
+                  s += coef[(std::uint64_t(cz) * Nc + std::uint64_t(cy)) * Nc + std::uint64_t(cx)];
+                  ++n;
+                }
+            f[(z * m + y) * m + x] = float(std::log(s / double(n)));
+          }
+      return f;
+    }
+
     /// xorshift64* step (kernel/util/random.hpp:84-110)
     inline std::uint64_t rng_next(std::uint64_t& x)
     {

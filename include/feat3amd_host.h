@@ -170,6 +170,43 @@ int feat_lognormal_cells(uint64_t seed, uint64_t n, double sigma, double* out);
 int feat_q1_diffusion_assemble(int level, int level_c, const double* coef, double* val, uint64_t* col,
                                uint64_t* rowptr);
 
+/* ---- multigrid with a learned finest-level smoother (3D Q1 anisotropic diffusion) ----
+ * No counterpart in the reference (see include/feat3amd/learned.hpp). */
+typedef struct {
+  int level_fine, level_coarse;
+  double sigma, kz;          /* log-normal coefficient, anisotropy diag(1,1,kz) */
+  uint64_t seed;
+  int nu;                    /* pre/post smoothing steps */
+  double omega_jac, omega_learned, tol_rel;
+  uint64_t max_iter;
+  int n_init, n_snap;        /* training data: random starts x outer iterations */
+  uint64_t batch;            /* training batch */
+  float lr;
+} feat_learned_cfg_t;
+typedef struct {
+  int status;
+  uint64_t iters;
+  int fell_back;
+  uint64_t fallback_iter;
+  double def_init, def_final, seconds;
+} feat_learned_result_t;
+typedef struct feat_learned_s* feat_learned_t;
+int feat_learned_create(feat_learned_t* s, const feat_learned_cfg_t* cfg);
+/* the FP64 Jacobi multigrid's residual/error snapshots; *rho_jacobi its contraction */
+int feat_learned_generate(feat_learned_t s, double* rho_jacobi);
+/* `steps` Adam steps; loss (steps floats, may be NULL): mean squared error per step */
+int feat_learned_train(feat_learned_t s, int steps, uint64_t seed, float* loss);
+/* outer Richardson(MG) on the built-in right-hand side from x (device, n doubles;
+ * NULL: from zero); learned = 0: Jacobi smoother on every level */
+int feat_learned_solve(feat_learned_t s, int learned, int allow_fallback, double* x,
+                       feat_learned_result_t* res, double* hist, uint64_t hist_cap);
+int feat_learned_size(feat_learned_t s, uint64_t* n, uint64_t* m);
+/* parameters (FEAT_MLP_NPARAM floats): get (may be NULL), then set (may be NULL) */
+int feat_learned_params(feat_learned_t s, float* get, const float* set);
+/* device pointers of the FP32 parameters and their BF16 copies */
+int feat_learned_param_ptrs(feat_learned_t s, float** params, uint16_t** wbf);
+int feat_learned_destroy(feat_learned_t s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -195,6 +195,13 @@ void orc_stokes_smooth(const orc_stokes_level_t* L, int steps, double omega, dou
 int orc_stokes_mg_solve(const orc_stokes_level_t* levels, int nl, const orc_stokes_cfg_t* cfg,
                         const orc_iter_cfg_t* it, double* x, const double* b, double* hist,
                         orc_result_t* res);
+/* ---- learned smoother network and features (feat_oracle_mlp.c) ---- */
+void orc_mlp_forward(const float* p, const uint16_t* X, uint64_t n, int round_bf16, float* y);
+void orc_mlp_grad(const float* p, const uint16_t* X, const float* t, uint64_t n, int round_bf16, float* grad);
+void orc_mlp_features(uint64_t m, const double* r, const double* e, const double* dinv, const float* cfeat,
+                      const uint64_t* node, uint64_t nsamp, uint16_t* X, float* t);
+void orc_mlp_smooth(uint64_t m, const float* p, const double* r, const double* dinv, const float* cfeat,
+                    double omega, double* x);
 #ifdef __cplusplus
 }
 #endif

@@ -242,6 +242,52 @@ class Oracle:
         L.orc_stokes_mg_solve.argtypes = [C.POINTER(_StokesLevelS), C.c_int, C.POINTER(_StokesCfgS),
                                           C.POINTER(_IterCfgS), _f64p, _f64p, _vp, C.POINTER(_ResultS)]
 
+    # -- learned smoother network
+    def mlp_forward(self, p, X, round_bf16=True):
+        self._mlp_decl()
+        X = np.ascontiguousarray(X, dtype=np.uint16)
+        y = np.zeros(X.shape[0], np.float32)
+        self.L.orc_mlp_forward(np.ascontiguousarray(p, dtype=np.float32).ctypes.data, X.ctypes.data, X.shape[0],
+                               int(round_bf16), y.ctypes.data)
+        return y
+
+    def mlp_grad(self, p, X, t, round_bf16=True):
+        self._mlp_decl()
+        X = np.ascontiguousarray(X, dtype=np.uint16)
+        g = np.zeros(8448, np.float32)
+        self.L.orc_mlp_grad(np.ascontiguousarray(p, dtype=np.float32).ctypes.data, X.ctypes.data,
+                            np.ascontiguousarray(t, dtype=np.float32).ctypes.data, X.shape[0], int(round_bf16),
+                            g.ctypes.data)
+        return g
+
+    def mlp_features(self, m, r, e, dinv, cfeat, nodes):
+        self._mlp_decl()
+        nodes = np.ascontiguousarray(nodes, dtype=np.uint64)
+        X = np.zeros((nodes.size, 64), np.uint16)
+        t = np.zeros(nodes.size, np.float32)
+        self.L.orc_mlp_features(m, *(np.ascontiguousarray(a).ctypes.data for a in (r, e, dinv)),
+                                np.ascontiguousarray(cfeat, dtype=np.float32).ctypes.data, nodes.ctypes.data,
+                                nodes.size, X.ctypes.data, t.ctypes.data)
+        return X, t
+
+    def mlp_smooth(self, m, p, r, dinv, cfeat, omega, x):
+        self._mlp_decl()
+        x = np.array(x, dtype=np.float64)
+        self.L.orc_mlp_smooth(m, np.ascontiguousarray(p, dtype=np.float32).ctypes.data,
+                              np.ascontiguousarray(r).ctypes.data, np.ascontiguousarray(dinv).ctypes.data,
+                              np.ascontiguousarray(cfeat, dtype=np.float32).ctypes.data, omega, x.ctypes.data)
+        return x
+
+    def _mlp_decl(self):
+        L = self.L
+        if getattr(self, "_mlp_ok", False):
+            return
+        L.orc_mlp_forward.argtypes = [_vp, _vp, C.c_uint64, C.c_int, _vp]
+        L.orc_mlp_grad.argtypes = [_vp, _vp, _vp, C.c_uint64, C.c_int, _vp]
+        L.orc_mlp_features.argtypes = [C.c_uint64, _vp, _vp, _vp, _vp, _vp, C.c_uint64, _vp, _vp]
+        L.orc_mlp_smooth.argtypes = [C.c_uint64, _vp, _vp, _vp, _vp, C.c_double, _vp]
+        self._mlp_ok = True
+
     # -- Stokes
     def saddle_apply(self, lv, x, y=None, alpha=1.0):
         arr, keep = _stokes_levels_s([lv])
