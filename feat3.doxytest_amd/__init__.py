@@ -168,7 +168,9 @@ KERNEL_API = [
     ("feat_mlp_grad", _i, [_vp, _vp, _vp, _vp, _vp, _u64, _vp, _u64, _vp]),
     ("feat_mlp_adam", _i, [_vp, _vp, _vp, _vp, _vp, _vp, _i, _f, _f, _f, _f]),
     ("feat_mlp_features", _i, [_vp, _u64, _vp, _vp, _vp, _vp, _vp, _vp, _u64, _vp, _vp]),
-    ("feat_mlp_smooth_f64", _i, [_vp, _u64, _vp, _vp, _vp, _vp, _vp, _d, _vp]),
+    ("feat_mlp_smooth_f64", _i, [_vp, _u64, _vp, _vp, _vp, _vp, _vp, _d, _d, _vp]),
+    ("feat_mlp_features_full", _i, [_vp, _u64, _vp, _vp, _vp, _vp, _vp, _vp]),
+    ("feat_mlp_energy", _i, [_vp, _u64, _i, _d, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
 ]
 
 HOST_API = [
@@ -199,10 +201,11 @@ HOST_API = [
     ("feat_lognormal_cells", _i, [_u64, _u64, _d, _vp]),
     ("feat_q1_diffusion_assemble", _i, [_i, _i, _vp, _vp, _vp, _vp]),
     ("feat_learned_create", _i, [C.POINTER(_vp), _vp]),
-    ("feat_learned_generate", _i, [_vp, C.POINTER(_d)]),
+    ("feat_learned_generate", _i, [_vp, _i, C.POINTER(_d)]),
     ("feat_learned_train", _i, [_vp, _i, _u64, _vp]),
     ("feat_learned_solve", _i, [_vp, _i, _i, _vp, _vp, _vp, _u64]),
     ("feat_learned_size", _i, [_vp, C.POINTER(_u64), C.POINTER(_u64)]),
+    ("feat_learned_energy", _i, [_vp, C.POINTER(_d)]),
     ("feat_learned_params", _i, [_vp, _vp, _vp]),
     ("feat_learned_param_ptrs", _i, [_vp, C.POINTER(_vp), C.POINTER(_vp)]),
     ("feat_learned_destroy", _i, [_vp]),
@@ -625,7 +628,7 @@ MLP_NPARAM, MLP_NPARAM_PAD = 8385, 8448
 
 
 class LearnedMG:
-    """Outer Richardson(MG) on the 3D Q1 anisotropic variable-coefficient problem with
+    """Flexible PCG(MG-V) on the 3D Q1 anisotropic variable-coefficient problem with
     a Jacobi or a learned (BF16 MLP) finest-level smoother (C++ host layer)."""
 
     def __init__(self, level_fine=6, level_coarse=1, sigma=1.5, kz=10.0, seed=28054777172512, nu=4, omega_jac=0.5,
@@ -640,10 +643,15 @@ class LearnedMG:
         check(self.hl.feat_learned_size(self.s, C.byref(n), C.byref(m)), "feat_learned_size")
         self.n, self.m = n.value, m.value
 
-    def generate(self):
+    def generate(self, on_policy=False):
         rho = C.c_double()
-        check(self.hl.feat_learned_generate(self.s, C.byref(rho)), "feat_learned_generate")
+        check(self.hl.feat_learned_generate(self.s, int(on_policy), C.byref(rho)), "feat_learned_generate")
         return rho.value
+
+    def energy(self):
+        e = C.c_double()
+        check(self.hl.feat_learned_energy(self.s, C.byref(e)), "feat_learned_energy")
+        return e.value
 
     def train(self, steps, seed=1):
         loss = np.zeros(max(steps, 1), np.float32)

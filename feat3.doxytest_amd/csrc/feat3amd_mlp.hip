@@ -366,46 +366,163 @@ __global__ void k_mlp_features(Grid3 G, const double* __restrict__ R, const doub
   t[k] = ok ? float(e[i] / s) : 0.0f;
 }
 
-// The learned smoothing step: x_i += omega * s_i * f(patch_i) on the interior nodes.
+// The learned smoothing step on the interior nodes:
+//   x_i += omega * (jacobi * r_i / a_ii + s_i * f(patch_i)).
+// Inference layout: the weights stay in registers as MFMA B operands for the whole
+// kernel; each wavefront takes 16 nodes, lane l builds the 16 inputs of node l%16 that
+// it feeds to the A operand (quarter l/16 of the 64), the four lanes of a node combine
+// their maxima for s; only h1 passes through LDS (D layout -> A layout).
 __global__ __launch_bounds__(64 * kWaves) void k_mlp_smooth(Grid3 G, const float* __restrict__ p,
                                                             const uint16_t* __restrict__ wbf,
                                                             const double* __restrict__ r, const double* __restrict__ dinv,
-                                                            const float* __restrict__ cfeat, double omega,
+                                                            const float* __restrict__ cfeat, double omega, double jacobi,
                                                             double* __restrict__ x)
 {
-  __shared__ SharedWeights W;
-  __shared__ WaveBuf Bw[kWaves];
-  __shared__ double sc[kWaves][16];
-  const int wave = threadIdx.x >> 6, l = threadIdx.x & 63;
-  WaveBuf& B = Bw[wave];
-  load_weights(W, p, wbf);
-  __syncthreads();
+  __shared__ uint16_t h1s[kWaves][16 * kHid];
+  __shared__ double scs[kWaves][16];
+  const int wave = threadIdx.x >> 6, l = threadIdx.x & 63, c = l & 15, q = l >> 4;
+  // B operands: B1[nt][ks][i] = W1[nt*16 + c][ks*16 + 4q + i], same for W2
+  bf16x4 B1[4][4], B2[4][4];
+  float b1r[4], b2r[4], w3r[4];
+  for(int nt = 0; nt < 4; ++nt)
+  {
+    for(int ks = 0; ks < 4; ++ks)
+#pragma unroll
+      for(int i = 0; i < 4; ++i)
+      {
+        B1[nt][ks][i] = short(wbf[(nt * 16 + c) * kInP + ks * 16 + 4 * q + i]);
+        B2[nt][ks][i] = short(wbf[kHid * kInP + (nt * 16 + c) * kHid + ks * 16 + 4 * q + i]);
+      }
+    b1r[nt] = p[kB1 + nt * 16 + c];
+    b2r[nt] = p[kB2 + nt * 16 + c];
+    w3r[nt] = p[kW3 + nt * 16 + c];
+  }
+  const float b3 = p[kB3];
+  const uint64_t m = G.m, mm = m * m;
   const uint64_t ngroups = (G.n + 16 * kWaves - 1) / (16 * kWaves);
   for(uint64_t g = blockIdx.x; g < ngroups; g += gridDim.x)
   {
     const uint64_t s0 = (g * kWaves + uint64_t(wave)) * 16;
-    if(l < 16)
+    // ---- inputs of node s0 + c: this lane's 16 of them (k = ks*16 + 4q + i)
+    const uint64_t node = s0 + uint64_t(c);
+    bool interior = false;
+    if(node < G.n)
     {
-      const uint64_t i = s0 + uint64_t(l);
-      double s = 0.0;
-      if(i < G.n)
+      const uint64_t xx = node % m, yy = (node / m) % m, zz = node / mm;
+      interior = xx > 0 && yy > 0 && zz > 0 && xx + 1 < m && yy + 1 < m && zz + 1 < m;
+    }
+    const double di = interior ? dinv[node] : 0.0;
+    double rv[16];
+    float fv[16];
+    double mx = 0.0;
+#pragma unroll
+    for(int ks = 0; ks < 4; ++ks)
+#pragma unroll
+      for(int i = 0; i < 4; ++i)
       {
-        if(!node_features(G, i, r, dinv, cfeat, B.x + l * kInP, s)) s = 0.0;
+        const int k = ks * 16 + 4 * q + i, e = ks * 4 + i;
+        rv[e] = 0.0;
+        fv[e] = 0.0f;
+        if(!interior || k >= kIn) continue;
+        const int j = k < 27 ? k : k - 27;
+        const int64_t off = int64_t(j % 3 - 1) + int64_t((j / 3) % 3 - 1) * int64_t(m) + int64_t(j / 9 - 1) * int64_t(mm);
+        const uint64_t nb = uint64_t(int64_t(node) + off);
+        if(k < 27)
+        {
+          rv[e] = r[nb] * di;
+          mx = fmax(mx, fabs(rv[e]));
+        }
+        else
+          fv[e] = cfeat[nb];
       }
-      else
-        for(int k = 0; k < kInP; ++k) B.x[l * kInP + k] = 0;
-      sc[wave][l] = s;
+    // s = max over the node's 27 residuals: the four lanes c, c+16, c+32, c+48
+    mx = fmax(mx, __shfl_xor(mx, 16, 64));
+    mx = fmax(mx, __shfl_xor(mx, 32, 64));
+    const double s = mx;
+    bf16x4 A1[4];
+#pragma unroll
+    for(int ks = 0; ks < 4; ++ks)
+#pragma unroll
+      for(int i = 0; i < 4; ++i)
+      {
+        const int k = ks * 16 + 4 * q + i, e = ks * 4 + i;
+        float v = 0.0f;
+        if(s > 0.0) v = k < 27 ? float(rv[e] / s) : fv[e];
+        A1[ks][i] = short(f2bf(v));
+      }
+    if(q == 0) scs[wave][c] = interior ? s : 0.0;
+    // ---- layer 1 (MFMA), GELU, h1 to LDS in A layout
+#pragma unroll
+    for(int nt = 0; nt < 4; ++nt)
+    {
+      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for(int ks = 0; ks < 4; ++ks) acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A1[ks], B1[nt][ks], acc, 0, 0, 0);
+#pragma unroll
+      for(int i = 0; i < 4; ++i) h1s[wave][(4 * q + i) * kHid + nt * 16 + c] = f2bf(gelu(acc[i] + b1r[nt]));
     }
     __syncthreads();
-    forward12(W, B);
-    if(l < 16)
+    bf16x4 A2[4];
+#pragma unroll
+    for(int ks = 0; ks < 4; ++ks)
+#pragma unroll
+      for(int i = 0; i < 4; ++i) A2[ks][i] = short(h1s[wave][c * kHid + ks * 16 + 4 * q + i]);
+    // ---- layer 2 (MFMA) and layer 3: per node sum of w3 gelu(z2) over the 64 units
+    float part[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for(int nt = 0; nt < 4; ++nt)
     {
-      const uint64_t i = s0 + uint64_t(l);
-      const double s = sc[wave][l];
-      if(i < G.n && s > 0.0) x[i] = __fma_rn(omega, s * double(B.y[l]), x[i]);
+      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for(int ks = 0; ks < 4; ++ks) acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A2[ks], B2[nt][ks], acc, 0, 0, 0);
+#pragma unroll
+      for(int i = 0; i < 4; ++i) part[i] = __fmaf_rn(w3r[nt], gelu(acc[i] + b2r[nt]), part[i]);
+    }
+    // the 16 lanes with the same q hold the same 4 nodes (4q + i) and different units
+#pragma unroll
+    for(int i = 0; i < 4; ++i)
+      for(int o = 1; o < 16; o <<= 1) part[i] += __shfl_xor(part[i], o, 64);
+    if(c < 4)
+    {
+      const int nl = 4 * q + c;
+      const uint64_t nd = s0 + uint64_t(nl);
+      const double sn = scs[wave][nl];
+      if(nd < G.n && sn > 0.0)
+      {
+        const double cval = __fma_rn(jacobi, r[nd] * dinv[nd], sn * double(part[c] + b3));
+        x[nd] = __fma_rn(omega, cval, x[nd]);
+      }
     }
     __syncthreads();
   }
+}
+
+// inputs of every node of the grid (training on whole fields): X rows, s, rho_i = r_i / a_ii
+__global__ void k_mlp_features_full(Grid3 G, const double* __restrict__ r, const double* __restrict__ dinv,
+                                    const float* __restrict__ cfeat, uint16_t* __restrict__ X, double* __restrict__ sc,
+                                    double* __restrict__ rho)
+{
+  const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if(i >= G.n) return;
+  double s;
+  if(!node_features(G, i, r, dinv, cfeat, X + i * kInP, s)) s = 0.0;
+  sc[i] = s;
+  rho[i] = r[i] * dinv[i];
+}
+
+// stage 0: e' = e - (jacobi rho + s y) where s > 0 (else e' = e - jacobi rho masked to interior by s)
+// stage 1: t = y + s q / 2 (the target whose squared-error gradient is the energy gradient)
+__global__ void k_mlp_energy(uint64_t n, int stage, double coef, const float* __restrict__ y,
+                             const double* __restrict__ sc, const double* __restrict__ rho, const double* __restrict__ e,
+                             const double* __restrict__ q, double* __restrict__ ep, float* __restrict__ t)
+{
+  const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if(i >= n) return;
+  const double s = sc[i];
+  if(stage == 0)
+    ep[i] = s > 0.0 ? e[i] - __fma_rn(coef, rho[i], s * double(y[i])) : 0.0;
+  else
+    t[i] = s > 0.0 ? float(double(y[i]) + 0.5 * coef * s * q[i]) : y[i];
 }
 
 int mlp_grid(uint64_t nsamp)
@@ -473,13 +590,31 @@ int feat_mlp_features(feat_handle_t h, uint64_t m, const double* R, const double
 }
 
 int feat_mlp_smooth_f64(feat_handle_t h, uint64_t m, const float* params, const uint16_t* wbf, const double* r,
-                        const double* dinv, const float* cfeat, double omega, double* x)
+                        const double* dinv, const float* cfeat, double omega, double jacobi, double* x)
 {
   if(!h || m < 3 || !params || !wbf || !r || !dinv || !cfeat || !x) return fail(FEAT_ERR_INVALID, "feat_mlp_smooth_f64: bad argument");
   Grid3 G{m, m * m * m};
   hipLaunchKernelGGL(k_mlp_smooth, dim3(mlp_grid(G.n)), dim3(64 * kWaves), 0, h->stream, G, params, wbf, r, dinv, cfeat,
-                     omega, x);
+                     omega, jacobi, x);
   return check_launch("feat_mlp_smooth_f64");
+}
+
+int feat_mlp_features_full(feat_handle_t h, uint64_t m, const double* r, const double* dinv, const float* cfeat,
+                           uint16_t* X, double* s, double* rho)
+{
+  if(!h || m < 3 || !r || !dinv || !cfeat || !X || !s || !rho) return fail(FEAT_ERR_INVALID, "feat_mlp_features_full: bad argument");
+  Grid3 G{m, m * m * m};
+  hipLaunchKernelGGL(k_mlp_features_full, dim3((G.n + 255) / 256), dim3(256), 0, h->stream, G, r, dinv, cfeat, X, s, rho);
+  return check_launch("feat_mlp_features_full");
+}
+
+int feat_mlp_energy(feat_handle_t h, uint64_t n, int stage, double coef, const float* y, const double* s,
+                    const double* rho, const double* e, const double* q, double* ep, float* t)
+{
+  if(!h || !n || !y || !s || (stage == 0 && (!rho || !e || !ep)) || (stage == 1 && (!q || !t)))
+    return fail(FEAT_ERR_INVALID, "feat_mlp_energy: bad argument");
+  hipLaunchKernelGGL(k_mlp_energy, dim3((n + 255) / 256), dim3(256), 0, h->stream, n, stage, coef, y, s, rho, e, q, ep, t);
+  return check_launch("feat_mlp_energy");
 }
 
 } // extern "C"

@@ -661,11 +661,11 @@ int feat_learned_create(feat_learned_t* out, const feat_learned_cfg_t* cfg)
   });
 }
 
-int feat_learned_generate(feat_learned_t s, double* rho)
+int feat_learned_generate(feat_learned_t s, int on_policy, double* rho)
 {
   return guarded([&]() {
     if(!s) throw Error("feat_learned_generate: null");
-    s->mg->generate_data();
+    s->mg->generate_data(on_policy != 0);
     if(rho) *rho = s->mg->rho_jacobi();
   });
 }
@@ -714,6 +714,14 @@ int feat_learned_size(feat_learned_t s, std::uint64_t* n, std::uint64_t* m)
     if(!s) throw Error("feat_learned_size: null");
     if(n) *n = s->mg->size();
     if(m) *m = s->mg->nodes_1d();
+  });
+}
+
+int feat_learned_energy(feat_learned_t s, double* energy)
+{
+  return guarded([&]() {
+    if(!s || !energy) throw Error("feat_learned_energy: null");
+    *energy = s->mg->energy();
   });
 }
 

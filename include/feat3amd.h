@@ -503,11 +503,24 @@ int feat_mlp_adam(feat_handle_t h, float* params, uint16_t* wbf, const float* gr
 int feat_mlp_features(feat_handle_t h, uint64_t m, const double* R, const double* E,
                       const double* dinv, const float* cfeat, const uint64_t* node,
                       const uint32_t* snap, uint64_t nsamp, uint16_t* X, float* t);
-/* learned smoothing step: x_i += omega * s_i * y_i for every interior node, y the
- * network output for the patch of the defect r (dinv = 1 / diag(A)) */
+/* learned smoothing step on every interior node: x_i += omega * (jacobi * r_i / a_ii
+ * + s_i * y_i), y the network output for the patch of the defect r (dinv = 1/diag(A)):
+ * damped Jacobi plus the learned correction */
 int feat_mlp_smooth_f64(feat_handle_t h, uint64_t m, const float* params, const uint16_t* wbf,
                         const double* r, const double* dinv, const float* cfeat, double omega,
-                        double* x);
+                        double jacobi, double* x);
+/* inputs of every node (whole-field training): X (m^3 rows), s, rho_i = r_i / a_ii */
+int feat_mlp_features_full(feat_handle_t h, uint64_t m, const double* r, const double* dinv,
+                           const float* cfeat, uint16_t* X, double* s, double* rho);
+/* energy-norm training of the learned correction on a field pair (r = A e, e):
+ * stage 0 (coef = the Jacobi weight): ep = e - (coef rho + s y) (the error after one
+ * learned step, 0 where s = 0);
+ * stage 1 (coef = a loss weight w), given q = A ep: t = y + w s q / 2, so that the
+ * squared-error gradient of feat_mlp_grad over all n nodes is the gradient of
+ * w ep^T A ep / (2 n) */
+int feat_mlp_energy(feat_handle_t h, uint64_t n, int stage, double coef, const float* y,
+                    const double* s, const double* rho, const double* e, const double* q,
+                    double* ep, float* t);
 
 #ifdef __cplusplus
 }

@@ -7,14 +7,19 @@
 // preconditioner of a flexible PCG (pcg.hpp:188-288 with the Polak-Ribiere beta,
 // since the learned smoother makes the preconditioner nonlinear) with the plain
 // FP64 defect norm as the stopping criterion.  On the finest level the
-// smoother is either damped Jacobi (Richardson(JacobiPrecond)) or the learned map
-//   x_i += omega * s_i * f(rho / s_i, log-coefficient patch),  rho = r / a_ii over
-//   the node's 3x3x3 patch, s_i = max |rho|,
-// with f the BF16 MLP of feat3amd.h trained on residual/error pairs that the FP64
-// Jacobi multigrid itself produces on the error equation.  The
-// learned smoother falls back to Jacobi for the rest of a solve as soon as an
-// outer iteration contracts the FP64 defect less than the Jacobi multigrid does
-// (its contraction measured while the training data is generated).
+// smoother is either damped Jacobi (Richardson(JacobiPrecond)) or damped Jacobi plus a
+// learned correction,
+//   x_i += omega * (omega_jac rho_i + s_i f(rho / s_i, log-coefficient patch)),
+//   rho = r / a_ii over the node's 3x3x3 patch, s_i = max |rho|,
+// with f the BF16 MLP of feat3amd.h (output layer initialised to zero, so the untrained
+// smoother is Jacobi) trained on residual/error fields that the FP64 Jacobi multigrid
+// itself produces on the error equation -- the input of every finest-level smoothing step,
+// first with the Jacobi smoother, then (on-policy refresh) with the learned one -- to
+// minimise the energy norm of the error one learned step leaves, relative to the energy
+// it started from.  The learned smoother falls back to Jacobi for the rest of a solve as
+// soon as its FP64 defect falls a factor 3 behind the Jacobi multigrid's defect history
+// at the same iteration (recorded on the same right-hand side before training), or the
+// defect grows past its initial value.
 #pragma once
 
 #include <feat3amd/batch_assembly.hpp>
@@ -37,10 +42,10 @@ namespace feat3amd
       std::uint64_t seed = 28054777172512ull;
       int nu = 4;                 // pre- and post-smoothing steps
       double omega_jac = 0.5;     // Jacobi damping (inv_diag = omega / a_ii)
-      double omega_learned = 0.5; // learned smoother damping
+      double omega_learned = 1.0; // learned smoother damping (x += omega (omega_jac r/a_ii + s f))
       double tol_rel = 1e-8;
       Index max_iter = 200;
-      // training data and optimisation
+      // training data (n_init random starts x n_snap V-cycles x 2 nu smoothing steps) and optimisation
       int n_init = 4, n_snap = 3;
       Index batch = 8192;
       float lr = 1e-3f;
@@ -67,15 +72,24 @@ namespace feat3amd
       FusedWorkspace _fw;
       // network
       DeviceArray<float> _p, _gm, _gv, _grad, _partials, _t;
-      DeviceArray<std::uint16_t> _wbf, _X;
-      DeviceArray<std::uint64_t> _snode;
-      DeviceArray<std::uint32_t> _ssnap;
+      DeviceArray<std::uint16_t> _wbf, _X;   // _X: inputs of every node of every snapshot
+      DeviceArray<float> _p0;                 // zero network: the learned step is damped Jacobi
+      DeviceArray<std::uint16_t> _wbf0;
+      DeviceArray<float> _y;
+      DeviceArray<double> _S, _Rho, _ep, _q;
       Index _partials_cap = 0;
       int _adam_step = 0;
       // training data
       DeviceArray<double> _R, _E;
       Index _nsnap = 0;
+      int _refresh = 0;
+      std::vector<double> _enorm;     // e^T A e of every snapshot
       double _rho_jacobi = 1.0;
+      std::vector<double> _jac_rel;   // Jacobi multigrid defect history / initial defect
+      // capture of the finest smoothing steps' inputs (training data generation)
+      double *_cap_r = nullptr, *_cap_e = nullptr;
+      Index _cap_next = 0, _cap_max = 0;
+      const Vec* _cap_x = nullptr;
       std::vector<float> _loss;
       DeviceArray<std::uint64_t> _bnd;
       std::vector<std::uint64_t> _bnd_host;
@@ -89,8 +103,12 @@ namespace feat3amd
 
       Level& L0() { return *_lv[0]; }
 
-      // finest-level learned smoothing: x = 0, `steps` steps of x += omega s f(patch(rhs - A x))
-      void learned_smooth(Vec& x, const Vec& rhs, int steps)
+      // finest-level learned smoothing: x = 0, `steps` steps of
+      // x += omega (omega_jac rho + s f(patch(rhs - A x))); net = false: zero network (damped
+      // Jacobi).  While capturing, every step's input defect and the error of the finest
+      // iterate, -(cap_x + base + x), are stored (the homogeneous error equation with iterate
+      // cap_x, base: the finest solution before these steps, nullptr for zero).
+      void learned_smooth(Vec& x, const Vec& rhs, int steps, bool net, const Vec* base)
       {
         feat_handle_t h = Runtime::handle();
         Level& L = L0();
@@ -105,17 +123,25 @@ namespace feat3amd
                   "learned defect");
             d = L.tmp.elements();
           }
-          check(feat_mlp_smooth_f64(h, _m, _p.get(), _wbf.get(), d, _dinv.elements(), _cfeat.get(), _cfg.omega_learned,
-                                    x.elements()),
+          if(_cap_r && _cap_next < _cap_max)
+          {
+            check(feat_copy(h, _cap_r + _cap_next * _n, d, _n * sizeof(double)), "capture");
+            Vec e = Vec::wrap(_cap_e + _cap_next * _n, _n);
+            e.axpy(x, *_cap_x, 1.0);
+            if(base) e.axpy(*base, e, 1.0);
+            e.scale(e, -1.0);
+            ++_cap_next;
+          }
+          check(feat_mlp_smooth_f64(h, _m, net ? _p.get() : _p0.get(), net ? _wbf.get() : _wbf0.get(), d,
+                                    _dinv.elements(), _cfeat.get(), _cfg.omega_learned, _cfg.omega_jac, x.elements()),
                 "learned smoothing");
         }
       }
 
       // V-cycle: cor = MG(def) (multigrid.hpp:1270-2000 with the fused kernels of solver.hpp).
-      // cap_r/cap_e (optional): the finest level's post-smoothing input defect and, for the
-      // homogeneous error equation with current iterate cap_x, the error -(cap_x + sol) there.
-      void vcycle(Vec& cor, const Vec& def, bool learned, double* cap_r = nullptr, double* cap_e = nullptr,
-                  const Vec* cap_x = nullptr)
+      // Finest-level smoother: mode 0 damped Jacobi (fused kernels), 1 learned, 2 the learned
+      // step with the zero network (damped Jacobi through the learned path, for capture).
+      void vcycle(Vec& cor, const Vec& def, int mode)
       {
         feat_handle_t h = Runtime::handle();
         const feat_reduce_ws_t* ws = &_fw.ws;
@@ -126,10 +152,10 @@ namespace feat3amd
         {
           Level& L = *_lv[i];
           const feat_matrix_f64_t& A = L.A.exec();
-          if(i == 0 && learned)
+          if(i == 0 && mode != 0)
           {
             Vec x = Vec::wrap(sol_of(0), _n), r = Vec::wrap(const_cast<double*>(rhs_of(0)), _n);
-            learned_smooth(x, r, _cfg.nu);
+            learned_smooth(x, r, _cfg.nu, mode == 1, nullptr);
             check(feat_defect_f64(h, &A, L.def.elements(), rhs_of(0), sol_of(0), L.mask.get(), nullptr, ws), "defect");
           }
           else
@@ -156,17 +182,10 @@ namespace feat3amd
           check(feat_prol_add_f64(h, &P, sol_of(i), sol_of(i + 1), L.mask.get()), "prolongation");
           const feat_matrix_f64_t& A = L.A.exec();
           check(feat_defect_f64(h, &A, L.def.elements(), rhs_of(i), sol_of(i), L.mask.get(), nullptr, ws), "defect");
-          if(i == 0 && cap_r)
+          if(i == 0 && mode != 0)
           {
-            check(feat_copy(h, cap_r, L.def.elements(), _n * sizeof(double)), "capture");
-            Vec e = Vec::wrap(cap_e, _n), sol = Vec::wrap(sol_of(0), _n);
-            e.axpy(*cap_x, sol, 1.0);  // x + sol
-            e.scale(e, -1.0);
-          }
-          if(i == 0 && learned)
-          {
-            learned_smooth(L.cor, L.def, _cfg.nu);
             Vec s = Vec::wrap(sol_of(0), _n);
+            learned_smooth(L.cor, L.def, _cfg.nu, mode == 1, &s);
             s.axpy(L.cor, s, 1.0);
           }
           else
@@ -204,6 +223,14 @@ namespace feat3amd
       Index size() const { return _n; }
       Index nodes_1d() const { return _m; }
       double rho_jacobi() const { return _rho_jacobi; }
+      /// relative defect of the Jacobi multigrid after k iterations (recorded history,
+      /// extended by its mean contraction)
+      double jacobi_rel(Index k) const
+      {
+        if(_jac_rel.empty()) return 1.0;
+        if(k < _jac_rel.size()) return _jac_rel[k];
+        return _jac_rel.back() * std::pow(_rho_jacobi, double(k + 1 - _jac_rel.size()));
+      }
       const std::vector<float>& loss_history() const { return _loss; }
       float* params() { return _p.get(); }
       std::uint16_t* params_bf16() { return _wbf.get(); }
@@ -265,8 +292,15 @@ namespace feat3amd
         for(int o = 0; o < 64; ++o)
           for(int i = 0; i < FEAT_MLP_IN; ++i) p[o * 64 + i] = uni(std::sqrt(6.0f / FEAT_MLP_IN));
         for(int i = 0; i < 64 * 64; ++i) p[4160 + i] = uni(std::sqrt(6.0f / 64.0f));
-        for(int i = 0; i < 64; ++i) p[8320 + i] = uni(std::sqrt(3.0f / 64.0f));
+        // output layer zero: the untrained learned smoother is exactly damped Jacobi
         set_params(p.data());
+        _p0 = DeviceArray<float>(FEAT_MLP_NPARAM_PAD);
+        _wbf0 = DeviceArray<std::uint16_t>(2 * 64 * 64);
+        {
+          std::vector<float> z0(FEAT_MLP_NPARAM_PAD, 0.0f);
+          check(feat_copy(Runtime::handle(), _p0.get(), z0.data(), z0.size() * 4), "zero network");
+          check(feat_mlp_to_bf16(Runtime::handle(), _p0.get(), _wbf0.get()), "zero network bf16");
+        }
         _gm = DeviceArray<float>(FEAT_MLP_NPARAM_PAD);
         _gv = DeviceArray<float>(FEAT_MLP_NPARAM_PAD);
         std::vector<float> z(FEAT_MLP_NPARAM_PAD, 0.0f);
@@ -275,10 +309,8 @@ namespace feat3amd
         _grad = DeviceArray<float>(FEAT_MLP_NPARAM_PAD);
         _partials_cap = 4 * 2048;
         _partials = DeviceArray<float>(_partials_cap * FEAT_MLP_NPARAM_PAD);
-        _X = DeviceArray<std::uint16_t>(_cfg.batch * 64);
-        _t = DeviceArray<float>(_cfg.batch);
-        _snode = DeviceArray<std::uint64_t>(_cfg.batch);
-        _ssnap = DeviceArray<std::uint32_t>(_cfg.batch);
+        _t = DeviceArray<float>(_n);
+        _y = DeviceArray<float>(_n);
         Runtime::synchronize();
       }
 
@@ -299,7 +331,7 @@ namespace feat3amd
       /// preconditioner is admissible; with the linear Jacobi V-cycle it is PCG in exact
       /// arithmetic).  learned = false: the Jacobi multigrid.  The FP64 defect norm decides
       /// convergence; with allow_fallback the learned smoother is dropped for the rest of the
-      /// solve when an iteration contracts the defect less than rho_jacobi.
+      /// solve when its mean contraction falls behind rho_jacobi.
       SolveInfo solve(Vec& x, const Vec& b, bool learned, bool allow_fallback = true)
       {
         SolveInfo info;
@@ -314,7 +346,7 @@ namespace feat3amd
         info.status = std::isfinite(d) ? (d <= eps * eps ? Status::success : Status::progress) : Status::aborted;
         if(info.status == Status::progress)
         {
-          vcycle(z, _r, use_learned);
+          vcycle(z, _r, use_learned ? 1 : 0);
           p.copy(z);
           double gamma = _r.dot(z);
           while(info.status == Status::progress)
@@ -328,7 +360,11 @@ namespace feat3amd
             const double dn = _r.norm2();
             ++info.iters;
             info.hist.push_back(dn);
-            if(use_learned && allow_fallback && (!std::isfinite(dn) || dn > _rho_jacobi * d))
+            // contraction check against the Jacobi multigrid's relative defect at the same
+            // iteration (CG defect norms are not monotone, hence the factor 3); immediate
+            // fallback on growth past the initial defect or a non-finite defect
+            if(use_learned && allow_fallback &&
+               (!std::isfinite(dn) || dn > info.def_init || (info.iters >= 2 && dn > 3.0 * info.def_init * jacobi_rel(info.iters))))
             {
               use_learned = false;
               info.fell_back = true;
@@ -342,7 +378,7 @@ namespace feat3amd
             else if(info.iters >= _cfg.max_iter)
               info.status = Status::max_iter;
             if(info.status != Status::progress) break;
-            vcycle(z, _r, use_learned);
+            vcycle(z, _r, use_learned ? 1 : 0);
             const double gnew = _r.dot(z);
             const double beta = (gnew - rold.dot(z)) / gamma;
             gamma = gnew;
@@ -355,78 +391,129 @@ namespace feat3amd
         return info;
       }
 
-      /// Residual/error pairs produced by the FP64 Jacobi multigrid itself on the error
-      /// equation A e = 0 from random (white-noise) starts: the start (r = -A x, e = -x) and,
-      /// per V-cycle, the finest level's post-smoothing input after the coarse-grid
-      /// correction -- the rough error a fine-level smoother has to remove.  Also measures
-      /// the Jacobi multigrid's contraction (flexible PCG on the right-hand side).
-      void generate_data()
+      /// Residual/error pairs of the multigrid itself on the error equation A e = 0 from
+      /// random (white-noise) starts: the input of every finest-level smoothing step (pre and
+      /// post) of n_snap V-cycles per start, with the Jacobi smoother (on_policy = false) or
+      /// the current learned one (on-policy refresh, new starts).  The first call also
+      /// records the Jacobi multigrid's defect history on the right-hand side (flexible PCG).
+      void generate_data(bool on_policy = false)
       {
         const Index n = _n;
-        Vec xj(n, 0.0);
-        _rho_jacobi = 1.0;
-        const auto ref = solve(xj, _b, false, false);
-        _rho_jacobi = std::pow(ref.def_final / ref.def_init, 1.0 / double(std::max<Index>(ref.iters, 1)));
-        _nsnap = Index(_cfg.n_init) * Index(_cfg.n_snap);
-        _R = DeviceArray<double>(_nsnap * n);
-        _E = DeviceArray<double>(_nsnap * n);
+        if(_jac_rel.empty())
+        {
+          Vec xj(n, 0.0);
+          const auto ref = solve(xj, _b, false, false);
+          _rho_jacobi = std::pow(ref.def_final / ref.def_init, 1.0 / double(std::max<Index>(ref.iters, 1)));
+          for(double d : ref.hist) _jac_rel.push_back(d / ref.def_init);
+        }
+        const Index nsnap = Index(_cfg.n_init) * Index(_cfg.n_snap) * 2 * Index(_cfg.nu);
+        if(_nsnap != nsnap)
+        {
+          _nsnap = nsnap;
+          _R = DeviceArray<double>(_nsnap * n);
+          _E = DeviceArray<double>(_nsnap * n);
+          _X = DeviceArray<std::uint16_t>(_nsnap * n * 64);
+          _S = DeviceArray<double>(_nsnap * n);
+          _Rho = DeviceArray<double>(_nsnap * n);
+          _ep = DeviceArray<double>(n);
+          _q = DeviceArray<double>(n);
+        }
         Vec zero(n, 0.0);
-        Index k = 0;
+        _cap_r = _R.get();
+        _cap_e = _E.get();
+        _cap_next = 0;
+        _cap_max = _nsnap;
         for(int s = 0; s < _cfg.n_init; ++s)
         {
           std::vector<double> x0(n);
-          Assembly::random_fill(_cfg.seed + 4000u + std::uint64_t(s), x0.data(), n, -1.0, 1.0);
+          Assembly::random_fill(_cfg.seed + 4000u + std::uint64_t(s) + 100u * std::uint64_t(_refresh), x0.data(), n, -1.0,
+                                1.0);
           for(Index b = 0; b < _bnd.size(); ++b) x0[_bnd_host[b]] = 0.0;
           Vec x = Vec::from_host(x0);
-          {
-            Vec r = Vec::wrap(_R.get() + k * n, n), e = Vec::wrap(_E.get() + k * n, n);
-            (void)defect_norm(r, x, zero);
-            e.scale(x, -1.0);
-            ++k;
-          }
-          for(int it = 1; it < _cfg.n_snap; ++it, ++k)
+          _cap_x = &x;
+          for(int it = 0; it < _cfg.n_snap; ++it)
           {
             (void)defect_norm(_r, x, zero);
-            vcycle(_c, _r, false, _R.get() + k * n, _E.get() + k * n, &x);
+            vcycle(_c, _r, on_policy ? 1 : 2);
             x.axpy(_c, x, 1.0);
           }
+          Runtime::synchronize();
+        }
+        _cap_r = _cap_e = nullptr;
+        _cap_x = nullptr;
+        ++_refresh;
+        // network inputs of every node of every snapshot (fixed during training) and the
+        // energy each snapshot starts from
+        feat_handle_t h = Runtime::handle();
+        Level& L = L0();
+        _enorm.assign(_nsnap, 0.0);
+        for(Index j = 0; j < _nsnap; ++j)
+        {
+          check(feat_mlp_features_full(h, _m, _R.get() + j * n, _dinv.elements(), _cfeat.get(), _X.get() + j * n * 64,
+                                       _S.get() + j * n, _Rho.get() + j * n),
+                "features");
+          check(feat_apply_masked_f64(h, &L.A.exec(), _q.get(), _E.get() + j * n, L.mask.get(), nullptr, _E.get() + j * n,
+                                      _fw.scalar(1), &_fw.ws),
+                "A e");
+          check(feat_read_scalar_f64(h, _fw.scalar(1), &_enorm[j]), "energy");
         }
         Runtime::synchronize();
       }
 
-      /// Adam training on random (node, snapshot) samples; loss recorded every step
+      /// energy left by one learned step on snapshot k relative to the snapshot's energy,
+      /// ep^T A ep / e^T A e; with grad: the training target for its gradient
+      double step_energy(Index k, bool grad)
+      {
+        feat_handle_t h = Runtime::handle();
+        Level& L = L0();
+        const feat_matrix_f64_t& A = L.A.exec();
+        const Index n = _n;
+        const std::uint16_t* X = _X.get() + k * n * 64;
+        const double *S = _S.get() + k * n, *Rho = _Rho.get() + k * n, *E = _E.get() + k * n;
+        const double e0 = _enorm[k] > 0.0 ? _enorm[k] : 1.0;
+        check(feat_mlp_infer(h, _p.get(), _wbf.get(), X, n, _y.get()), "infer");
+        check(feat_mlp_energy(h, n, 0, _cfg.omega_jac, _y.get(), S, Rho, E, nullptr, _ep.get(), nullptr), "energy 0");
+        check(feat_apply_masked_f64(h, &A, _q.get(), _ep.get(), L.mask.get(), nullptr, _ep.get(), _fw.scalar(1), &_fw.ws),
+              "A ep");
+        if(grad)
+          check(feat_mlp_energy(h, n, 1, 2.0 * double(n) / e0, _y.get(), S, nullptr, nullptr, _q.get(), nullptr, _t.get()),
+                "energy 1");
+        double e = 0.0;
+        check(feat_read_scalar_f64(h, _fw.scalar(1), &e), "energy");
+        return e / e0;
+      }
+
+      /// mean over the snapshots of the relative energy one learned step leaves
+      double energy()
+      {
+        if(_nsnap == 0) generate_data();
+        double sum = 0.0;
+        for(Index k = 0; k < _nsnap; ++k) sum += step_energy(k, false);
+        return sum / double(_nsnap);
+      }
+
+      /// Adam training of the learned correction on whole snapshot fields: one random
+      /// snapshot per step, objective = the relative energy ep^T A ep / e^T A e of the error
+      /// ep left by one learned smoothing step (ep = e - (omega_jac r/a_ii + s f)); its value
+      /// is recorded
       void train(int steps, std::uint64_t seed)
       {
         if(_nsnap == 0) generate_data();
         feat_handle_t h = Runtime::handle();
+        const Index n = _n;
         std::mt19937_64 g(seed);
-        std::uniform_int_distribution<std::uint64_t> du(1, _m - 2);
-        std::uniform_int_distribution<std::uint32_t> ds(0, std::uint32_t(_nsnap - 1));
-        std::vector<std::uint64_t> nodes(_cfg.batch);
-        std::vector<std::uint32_t> snaps(_cfg.batch);
-        std::vector<float> lossv(FEAT_MLP_NPARAM_PAD);
+        std::uniform_int_distribution<Index> ds(0, _nsnap - 1);
         for(int st = 0; st < steps; ++st)
         {
-          for(Index k = 0; k < _cfg.batch; ++k)
-          {
-            nodes[k] = (du(g) * _m + du(g)) * _m + du(g);
-            snaps[k] = ds(g);
-          }
-          check(feat_copy(h, _snode.get(), nodes.data(), nodes.size() * 8), "samples");
-          check(feat_copy(h, _ssnap.get(), snaps.data(), snaps.size() * 4), "samples");
-          check(feat_mlp_features(h, _m, _R.get(), _E.get(), _dinv.elements(), _cfeat.get(), _snode.get(), _ssnap.get(),
-                                  _cfg.batch, _X.get(), _t.get()),
-                "features");
-          check(feat_mlp_grad(h, _p.get(), _wbf.get(), _X.get(), _t.get(), _cfg.batch, _partials.get(), _partials_cap,
+          const Index k = ds(g);
+          const double rel = step_energy(k, true);
+          check(feat_mlp_grad(h, _p.get(), _wbf.get(), _X.get() + k * n * 64, _t.get(), n, _partials.get(), _partials_cap,
                               _grad.get()),
                 "grad");
-          float loss = 0.0f;
-          check(feat_copy(h, &loss, _grad.get() + FEAT_MLP_NPARAM_PAD - 1, 4), "loss");
           check(feat_mlp_adam(h, _p.get(), _wbf.get(), _grad.get(), _gm.get(), _gv.get(), ++_adam_step, _cfg.lr, 0.9f,
                               0.999f, 1e-8f),
                 "adam");
-          Runtime::synchronize();
-          _loss.push_back(loss / float(_cfg.batch));
+          _loss.push_back(float(rel));
         }
       }
     };

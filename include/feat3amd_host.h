@@ -192,15 +192,20 @@ typedef struct {
 } feat_learned_result_t;
 typedef struct feat_learned_s* feat_learned_t;
 int feat_learned_create(feat_learned_t* s, const feat_learned_cfg_t* cfg);
-/* the FP64 Jacobi multigrid's residual/error snapshots; *rho_jacobi its contraction */
-int feat_learned_generate(feat_learned_t s, double* rho_jacobi);
-/* `steps` Adam steps; loss (steps floats, may be NULL): mean squared error per step */
+/* residual/error snapshots of every finest smoothing step of the FP64 multigrid on the
+ * error equation: Jacobi smoother (on_policy = 0) or the current learned one (on-policy
+ * refresh from new random starts); *rho_jacobi: the Jacobi multigrid's contraction */
+int feat_learned_generate(feat_learned_t s, int on_policy, double* rho_jacobi);
+/* `steps` Adam steps; loss (steps floats, may be NULL): relative energy ep^T A ep / e^T A e
+ * one learned step leaves on the step's snapshot */
 int feat_learned_train(feat_learned_t s, int steps, uint64_t seed, float* loss);
-/* outer Richardson(MG) on the built-in right-hand side from x (device, n doubles;
+/* flexible PCG(MG-V) on the built-in right-hand side from x (device, n doubles;
  * NULL: from zero); learned = 0: Jacobi smoother on every level */
 int feat_learned_solve(feat_learned_t s, int learned, int allow_fallback, double* x,
                        feat_learned_result_t* res, double* hist, uint64_t hist_cap);
 int feat_learned_size(feat_learned_t s, uint64_t* n, uint64_t* m);
+/* mean relative energy ep^T A ep / e^T A e one learned step leaves on the snapshots */
+int feat_learned_energy(feat_learned_t s, double* energy);
 /* parameters (FEAT_MLP_NPARAM floats): get (may be NULL), then set (may be NULL) */
 int feat_learned_params(feat_learned_t s, float* get, const float* set);
 /* device pointers of the FP32 parameters and their BF16 copies */

@@ -201,7 +201,7 @@ void orc_mlp_grad(const float* p, const uint16_t* X, const float* t, uint64_t n,
 void orc_mlp_features(uint64_t m, const double* r, const double* e, const double* dinv, const float* cfeat,
                       const uint64_t* node, uint64_t nsamp, uint16_t* X, float* t);
 void orc_mlp_smooth(uint64_t m, const float* p, const double* r, const double* dinv, const float* cfeat,
-                    double omega, double* x);
+                    double omega, double jacobi, double* x);
 #ifdef __cplusplus
 }
 #endif

@@ -177,9 +177,9 @@ void orc_mlp_features(uint64_t m, const double* r, const double* e, const double
   }
 }
 
-/* x_i += omega * s_i * y_i for every interior node (the device's learned smoothing step) */
+/* x_i += omega * (jacobi * r_i / a_ii + s_i * y_i) on every interior node (the device's learned smoothing step) */
 void orc_mlp_smooth(uint64_t m, const float* p, const double* r, const double* dinv, const float* cfeat, double omega,
-                    double* x)
+                    double jacobi, double* x)
 {
   const uint64_t n = m * m * m;
   float w1[HID * INP], w2[HID * HID], xin[INP];
@@ -194,7 +194,7 @@ void orc_mlp_smooth(uint64_t m, const float* p, const double* r, const double* d
     if(!features(m, i, r, dinv, cfeat, row, &s)) continue;
     for(int k = 0; k < INP; ++k) xin[k] = bf2f(row[k]);
     forward1(p, w1, w2, xin, 1, &a);
-    upd[i] = s * (double)a.y;
+    upd[i] = fma(jacobi, r[i] * dinv[i], s * (double)a.y);
     has[i] = 1;
   }
   for(uint64_t i = 0; i < n; ++i)

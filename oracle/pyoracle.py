@@ -270,12 +270,12 @@ class Oracle:
                                 nodes.size, X.ctypes.data, t.ctypes.data)
         return X, t
 
-    def mlp_smooth(self, m, p, r, dinv, cfeat, omega, x):
+    def mlp_smooth(self, m, p, r, dinv, cfeat, omega, jacobi, x):
         self._mlp_decl()
         x = np.array(x, dtype=np.float64)
         self.L.orc_mlp_smooth(m, np.ascontiguousarray(p, dtype=np.float32).ctypes.data,
                               np.ascontiguousarray(r).ctypes.data, np.ascontiguousarray(dinv).ctypes.data,
-                              np.ascontiguousarray(cfeat, dtype=np.float32).ctypes.data, omega, x.ctypes.data)
+                              np.ascontiguousarray(cfeat, dtype=np.float32).ctypes.data, omega, jacobi, x.ctypes.data)
         return x
 
     def _mlp_decl(self):
@@ -285,7 +285,7 @@ class Oracle:
         L.orc_mlp_forward.argtypes = [_vp, _vp, C.c_uint64, C.c_int, _vp]
         L.orc_mlp_grad.argtypes = [_vp, _vp, _vp, C.c_uint64, C.c_int, _vp]
         L.orc_mlp_features.argtypes = [C.c_uint64, _vp, _vp, _vp, _vp, _vp, C.c_uint64, _vp, _vp]
-        L.orc_mlp_smooth.argtypes = [C.c_uint64, _vp, _vp, _vp, _vp, C.c_double, _vp]
+        L.orc_mlp_smooth.argtypes = [C.c_uint64, _vp, _vp, _vp, _vp, C.c_double, C.c_double, _vp]
         self._mlp_ok = True
 
     # -- Stokes

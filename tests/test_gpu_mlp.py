@@ -114,8 +114,8 @@ def test_features_and_smoothing_step_vs_oracle(feat, handle, torch_gpu, oracle):
     x0 = rng.standard_normal(n)
     xd = dev(torch, x0)
     feat.check(k.feat_mlp_smooth_f64(handle.h, m, feat.ptr(pd), feat.ptr(wbf), feat.ptr(rd), feat.ptr(dd), feat.ptr(cd),
-                                     0.8, feat.ptr(xd)), "smooth")
-    xo = oracle.mlp_smooth(m, p, r, dinv, cfeat, 0.8, x0)
+                                     0.8, 0.5, feat.ptr(xd)), "smooth")
+    xo = oracle.mlp_smooth(m, p, r, dinv, cfeat, 0.8, 0.5, x0)
     xg = host(xd)
     idx = np.arange(n)
     bnd = (idx % m == 0) | (idx % m == m - 1) | ((idx // m) % m == 0) | ((idx // m) % m == m - 1) | \
@@ -136,25 +136,41 @@ def test_training_and_gain_over_jacobi(lmg):
     assert 0 < rho < 1
     jac = lmg.solve(learned=False)
     assert jac["status"] == "success"
-    loss = lmg.train(400, seed=1)
+    e0 = lmg.energy()
+    loss = lmg.train(300, seed=1)
     assert np.all(np.isfinite(loss))
-    assert np.mean(loss[-50:]) < 0.5 * np.mean(loss[:10])
+    # the relative energy one learned step leaves on the training fields falls below
+    # the untrained (damped Jacobi) value
+    e1 = lmg.energy()
+    print(f"relative one-step energy: Jacobi {e0:.4f}, learned {e1:.4f}")
+    assert e1 < 0.95 * e0, (e0, e1)
     lrn = lmg.solve(learned=True)
     raw = lmg.solve(learned=True, allow_fallback=False)
     print(f"jacobi-MG {jac['iters']} iterations, learned-MG {lrn['iters']} iterations "
           f"(fallback {lrn['fell_back']} at {lrn['fallback_iter']}), without fallback {raw['status']} "
           f"{raw['iters']}, rho_jacobi {rho:.3f}")
     # the plain FP64 defect decides: the learned method converges (falling back to
-    # Jacobi if it must) and its iteration count is reported next to Jacobi's
+    # Jacobi if it must) in no more iterations than Jacobi
     assert lrn["status"] == "success"
     assert lrn["def_final"] <= 1e-8 * lrn["def_init"]
+    assert lrn["iters"] <= jac["iters"] + 1
 
 
-def test_fallback_to_jacobi(lmg):
-    # an untrained (zero) network does nothing: the contraction degrades at once and
-    # the solver finishes with Jacobi
+def test_untrained_is_jacobi_and_fallback(lmg):
     saved = lmg.params()
-    lmg.params(np.zeros_like(saved))
+    # zero output layer: the learned smoother is damped Jacobi, same iteration count
+    zero = saved.copy()
+    zero[8320:8385] = 0.0
+    lmg.params(zero)
+    jac = lmg.solve(learned=False)
+    r0 = lmg.solve(learned=True)
+    assert r0["status"] == "success" and not r0["fell_back"]
+    assert abs(r0["iters"] - jac["iters"]) <= 1
+    # a network that outputs garbage degrades the contraction at once: the solver
+    # continues with Jacobi and still converges, judged by the FP64 defect
+    bad = saved.copy()
+    bad[8384] = 1e3
+    lmg.params(bad)
     r = lmg.solve(learned=True)
     lmg.params(saved)
     assert r["fell_back"] and r["fallback_iter"] == 1
