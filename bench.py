@@ -5,8 +5,10 @@ nonzeros), on one MI355X.
 One step = one complete PCG(MultiGrid-V(4,4), Richardson(Jacobi 0.5)) solve of
 A x = b from x = 0 to |r| <= 1e-8 |r0| (coarse level 1, exact coarse solve),
 inputs resident in HBM.  `value` is the time to solution in seconds (lower is
-better).  The roofline object is for the CSR SpMV kernel r = A*x on the same
-matrix, timed with HIP events on the stream it is launched on.
+better).  The roofline object is for the SpMV kernel r = A*x in the format the
+solver runs on (the pair-dictionary SELL-64 plan of the matrix), timed with HIP
+events on the stream it is launched on; the plain SELL-64 and CSR SpMVs and the
+solve on plain SELL-64 plans are reported beside it.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
 """
@@ -219,6 +221,48 @@ def batch_leg(feat, torch, dev, args):
     return out
 
 
+def spmv_only(feat, torch, dev, stream, args):
+    """SpMV r = A*x on the level-7 matrix in each format, args.spmv_reps launches each,
+    then args.spmv_reps device-to-device copies of 1 GiB (the counter calibration)."""
+    q = feat.q1_problem(DIM, LF)
+    n = q["n"]
+    val = torch.from_numpy(q["val"]).to(dev)
+    col = torch.from_numpy(q["col"].view(np.int64)).to(dev)
+    rp = torch.from_numpy(q["rowptr"].view(np.int64)).to(dev)
+    k, _ = feat.libs()
+    h = feat.Handle(0, stream=stream.cuda_stream)
+    xs = torch.rand(n, dtype=torch.float64, device=dev)
+    r = torch.empty(n, dtype=torch.float64, device=dev)
+    plans = {"sell64_dict": feat.SellPlan(h, val, col, rp, n, n), "sell64": feat.SellPlan(h, val, col, rp, n, n, dict=False)}
+    mats = {"csr_u64": feat.csr_matrix(val, col, rp, n, n)}
+    mats.update({kk: p.m for kk, p in plans.items()})
+    out = {}
+    for name, m in mats.items():
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(args.spmv_reps):
+            feat.check(k.feat_matrix_apply_f64(h.h, C.byref(m), feat.ptr(r), 1.0, feat.ptr(xs), 0.0, None), name)
+        e1.record(stream)
+        e1.synchronize()
+        out[name + "_us"] = e0.elapsed_time(e1) * 1e3 / args.spmv_reps
+        log(f"{name}: {out[name + '_us']:.1f} us per launch")
+    src = torch.empty(1 << 27, dtype=torch.float64, device=dev)
+    dst = torch.empty_like(src)
+    src.fill_(1.0)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(args.spmv_reps):
+        feat.check(k.feat_copy(h.h, feat.ptr(dst), feat.ptr(src), src.numel() * 8), "copy")
+    e1.record(stream)
+    e1.synchronize()
+    out["copy_1gib_us"] = e0.elapsed_time(e1) * 1e3 / args.spmv_reps
+    log(f"copy 1 GiB: {out['copy_1gib_us']:.1f} us per launch")
+    for p in plans.values():
+        p.close()
+    h.close()
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -226,6 +270,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--spmv-reps", type=int, default=50)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--spmv-only", action="store_true",
+                    help="only the SpMV loops (pair dictionary, plain SELL-64, CSR) and a 1 GiB device copy, "
+                         "for hardware-counter runs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
@@ -237,6 +284,9 @@ def main():
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
     stream = torch.cuda.current_stream()
+    if args.spmv_only:
+        spmv_only(feat, torch, dev, stream, args)
+        return
 
     # ---- setup (host assembly + upload; not timed)
     t0 = time.perf_counter()
@@ -293,6 +343,25 @@ def main():
     t_eager = float(np.mean(te))
     log(f"eager launches: {t_eager * 1e3:.2f} ms per solve (graphs {t_mean * 1e3:.2f} ms), identical result")
 
+    # ---- the same solver on plain SELL-64 plans (full FP64 values, i16 column offsets)
+    feat.runtime_sell_flags(feat.SELL_NO_DICT)
+    sp = feat.MgPcg(cfg, q1=(DIM, LF, LC))
+    feat.runtime_sell_flags(0)
+    xp = torch.zeros(n, dtype=torch.float64, device=dev)
+    tp = []
+    for k in range(args.warmup + args.steps):
+        xp.zero_()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        rp_ = sp.solve(xp, b)
+        torch.cuda.synchronize()
+        if k >= args.warmup:
+            tp.append(time.perf_counter() - t)
+    sp.close()
+    assert rp_["iters"] == res["iters"] and bool(torch.equal(xp, x)), "plain and dictionary plans differ"
+    t_plain = float(np.mean(tp))
+    log(f"plain SELL-64 plans: {t_plain * 1e3:.2f} ms per solve (pair dictionary {t_mean * 1e3:.2f} ms), identical result")
+
     # ---- the mixed-precision forms: FP32 V-cycle (FP32 or FP16 matrix values)
     # inside the FP64 PCG; outer residual, update, convergence test and coarse solve FP64
     kk, _ = feat.libs()
@@ -325,6 +394,7 @@ def main():
         rr[bnd_t] = 0.0
         tmean = float(np.mean(tm))
         mixed[prec] = {"s_per_solve": tmean, "iters": rm["iters"], "speedup_vs_fp64": t_mean / tmean,
+                       "speedup_vs_fp64_plain_plans": t_plain / tmean,
                        "true_rel_residual_fp64": float(torch.linalg.norm(rr) / torch.linalg.norm(b)),
                        "rel_error_vs_fp64_solution": float(torch.linalg.norm(xm - x) / torch.linalg.norm(x))}
         sm.close()
@@ -345,7 +415,10 @@ def main():
     r = torch.empty(n, dtype=torch.float64, device=dev)
     csr = feat.Matrix(feat.FMT_CSR, feat.IDX_U64, n, n, nnz, A.val, A.col, A.rowptr, None, 0, 1.0)
     plan = feat.Matrix()
-    feat.check(k.feat_sell_create_f64(h.h, C.byref(A), C.byref(plan)), "feat_sell_create_f64")
+    feat.check(k.feat_sell_create_ex_f64(h.h, C.byref(A), feat.SELL_NO_DICT, C.byref(plan)), "feat_sell_create_ex_f64")
+    dplan = feat.Matrix()
+    feat.check(k.feat_sell_create_f64(h.h, C.byref(A), C.byref(dplan)), "feat_sell_create_f64")
+    assert dplan.index == feat.IDX_DICT
 
     def time_apply(m):
         for _ in range(5):
@@ -360,20 +433,27 @@ def main():
 
     csr_ms = time_apply(csr)
     sell_ms = time_apply(plan)
+    dict_ms = time_apply(dplan)
     idx_bytes = {feat.IDX_D16: 2, feat.IDX_U32: 4}[plan.index]
     nslices = (n + 63) // 64
     # bytes each kernel must move: its own arrays once, x once, r once
     csr_bytes = spmv_bytes(n, nnz)
     sell_bytes = int(plan.nnz) * (8 + idx_bytes) + n * 2 + (nslices + 1) * 8 + 2 * n * 8
+    # pair dictionary: one byte per stored entry, row lengths, slice offsets, the 3 KB pair table, x, r
+    dict_bytes = int(dplan.nnz) + n * 2 + (nslices + 1) * 8 + 256 * 12 + 2 * n * 8
     csr_gbs = csr_bytes / (csr_ms * 1e-3) / 1e9
     sell_gbs = sell_bytes / (sell_ms * 1e-3) / 1e9
+    dict_gbs = dict_bytes / (dict_ms * 1e-3) / 1e9
     ref_model = nnz * 16 + n * 8             # product_matvec-bench.cpp:41-44 model (GiB = 1024^3)
     log(f"spmv csr-u64: {csr_ms * 1e3:.1f} us, {csr_gbs:.0f} GB/s | sell-64 ({'i16 delta' if idx_bytes == 2 else 'u32'}): "
-        f"{sell_ms * 1e3:.1f} us, {sell_gbs:.0f} GB/s own bytes, {csr_bytes / (sell_ms * 1e-3) / 1e9:.0f} GB/s CSR-equivalent")
+        f"{sell_ms * 1e3:.1f} us, {sell_gbs:.0f} GB/s own bytes, {csr_bytes / (sell_ms * 1e-3) / 1e9:.0f} GB/s CSR-equivalent"
+        f" | sell-64 pair dictionary: {dict_ms * 1e3:.1f} us, {dict_gbs:.0f} GB/s own bytes")
     padded = int(plan.nnz)
+    dict_padded = int(dplan.nnz)
     feat.check(k.feat_sell_destroy_f64(C.byref(plan)), "feat_sell_destroy_f64")
+    feat.check(k.feat_sell_destroy_f64(C.byref(dplan)), "feat_sell_destroy_f64")
     h.close()
-    achieved = sell_gbs
+    achieved = dict_gbs
 
     out = {
         "metric": "MG-PCG time-to-1e-8 residual (s) + SpMV HBM GB/s, 3D Q1 Poisson 2.1M dofs",
@@ -395,11 +475,16 @@ def main():
             "dofs": n, "nnz": nnz, "levels": LF - LC + 1, "iters": res["iters"],
             "final_rel_defect": res["def_final"] / res["def_init"],
             "eager_launch_s_per_solve": t_eager,
+            "plain_sell_plans_s_per_solve": t_plain,
+            "matrix_plan": "SELL-64 pair dictionary (1 byte per entry, 42 distinct (col-row, value) pairs)",
             "parallelism": "single GPU",
         },
         "mixed_precision": {"form": "FP32 V-cycle on FP32 / FP16-valued SELL-64 plans (FP64 coarse solve) "
                                     "inside FP64 PCG", "fp32": mixed["f32"], "fp16_values": mixed["f16"]},
         "spmv": {
+            "sell64_dict": {"kernel": "k_sell_dict_rows<double,EpiApply<double>,2>", "us_per_launch": dict_ms * 1e3,
+                            "bytes_per_launch": dict_bytes, "gbps": dict_gbs,
+                            "csr_equivalent_gbps": csr_bytes / (dict_ms * 1e-3) / 1e9, "stored_entries": dict_padded},
             "sell64": {"kernel": "k_sell_rows<double,short,true,EpiApply<double>>", "us_per_launch": sell_ms * 1e3,
                        "bytes_per_launch": sell_bytes, "gbps": sell_gbs,
                        "csr_equivalent_gbps": csr_bytes / (sell_ms * 1e-3) / 1e9,
@@ -412,7 +497,8 @@ def main():
         "batch": batch,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS, "traffic": None,
-                     "kernel": "SELL-64 SpMV r = A*x (the solver's matrix format), bytes = its arrays + x + r"},
+                     "kernel": "pair-dictionary SELL-64 SpMV r = A*x (the solver's matrix format), bytes = its "
+                               "arrays (1 B/entry codes, row lengths, slice offsets, pair table) + x + r"},
     }
     if not args.no_cpu_baseline:
         log("cpu baseline: reference MG-PCG on one host core ...")

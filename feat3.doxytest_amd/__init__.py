@@ -54,7 +54,8 @@ class Matrix(C.Structure):
 
 
 FMT_CSR, FMT_SELL = 0, 1
-IDX_U64, IDX_U32, IDX_D16 = 0, 1, 2
+IDX_U64, IDX_U32, IDX_D16, IDX_DICT = 0, 1, 2, 3
+SELL_NO_DICT = 1
 
 
 class ReduceWs(C.Structure):
@@ -128,6 +129,8 @@ KERNEL_API = [
     ("feat_matrix_from_csr_f64", _i, [C.POINTER(CsrF64), C.POINTER(Matrix)]),
     ("feat_sell_create_f64", _i, [_vp, C.POINTER(CsrF64), C.POINTER(Matrix)]),
     ("feat_sell_create_f32", _i, [_vp, C.POINTER(CsrF64), C.POINTER(Matrix)]),
+    ("feat_sell_create_ex_f64", _i, [_vp, C.POINTER(CsrF64), _i, C.POINTER(Matrix)]),
+    ("feat_sell_create_ex_f32", _i, [_vp, C.POINTER(CsrF64), _i, C.POINTER(Matrix)]),
     ("feat_sell_create_f16", _i, [_vp, C.POINTER(CsrF64), C.POINTER(Matrix)]),
     ("feat_sell_destroy_f64", _i, [C.POINTER(Matrix)]),
     ("feat_sell_destroy_f32", _i, [C.POINTER(Matrix)]),
@@ -216,6 +219,7 @@ HOST_API = [
                                C.POINTER(_vp)]),
     ("feat_mgpcg_destroy", _i, [_vp]),
     ("feat_runtime_init", _i, [_i, _vp]),
+    ("feat_runtime_sell_flags", _i, [_i]),
 ]
 
 
@@ -391,16 +395,20 @@ def csr_matrix(val, col, rowptr, rows, cols):
 
 
 class SellPlan:
-    """SELL-64 plan of a device CSR (f64 values with u64 indices); fp='f32' rounds values."""
+    """SELL-64 plan of a device CSR (f64 values with u64 indices); fp='f32' rounds values.
+    dict=False keeps full values and column offsets where the pair dictionary applies."""
 
-    def __init__(self, handle, val, col, rowptr, rows, cols, fp="f64"):
+    def __init__(self, handle, val, col, rowptr, rows, cols, fp="f64", dict=True):
         self.k, _ = libs()
         self.fp = fp
         c = csr_desc(val, col, rowptr, rows, cols)
         self.m = Matrix()
-        fn = {"f64": self.k.feat_sell_create_f64, "f32": self.k.feat_sell_create_f32,
-              "f16": self.k.feat_sell_create_f16}[fp]
-        check(fn(handle.h, C.byref(c), C.byref(self.m)), fn.__name__)
+        flags = 0 if dict else SELL_NO_DICT
+        if fp == "f16":
+            check(self.k.feat_sell_create_f16(handle.h, C.byref(c), C.byref(self.m)), "feat_sell_create_f16")
+        else:
+            fn = {"f64": self.k.feat_sell_create_ex_f64, "f32": self.k.feat_sell_create_ex_f32}[fp]
+            check(fn(handle.h, C.byref(c), flags, C.byref(self.m)), fn.__name__)
 
     def close(self):
         if self.m.format == FMT_SELL:
@@ -421,6 +429,12 @@ def runtime_init(device=0, stream=None):
         import torch
         stream = torch.cuda.current_stream(device).cuda_stream
     check(h.feat_runtime_init(device, stream), "feat_runtime_init")
+
+
+def runtime_sell_flags(flags):
+    """Flags of the SELL-64 plans the host layer builds from now on (SELL_NO_DICT: plain plans)."""
+    _, h = libs()
+    check(h.feat_runtime_sell_flags(int(flags)), "feat_runtime_sell_flags")
 
 
 class StokesCfg(C.Structure):

@@ -20,7 +20,10 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
+#include <utility>
+#include <vector>
 #include <string>
 #include <type_traits>
 
@@ -305,6 +308,81 @@ __global__ __launch_bounds__(kThreads) void k_sell_rows(const VT* __restrict__ v
   grid_reduce_finish(gr, acc0, acc1);
 }
 
+// ---------------------------------------------------------------------------
+// Pair-dictionary SELL-64: each stored entry is one byte, the code of its
+// (column - row, value) pair in a table of at most 256 pairs that the
+// workgroup stages in LDS.  Entry k of a row sits in byte k % 4 of word k / 4
+// of its lane (word w of lane l of slice s at codes[soff[s] + 64 w + l]), so a
+// lane fetches four entries per load and a wavefront reads 256 contiguous
+// bytes.  Row sums run in CSR order with one fma per entry, bit-identical to
+// the CSR loop.
+// ---------------------------------------------------------------------------
+template<typename DT, class Epi, int UNRW>
+__global__ __launch_bounds__(kThreads) void k_sell_dict_rows(const DT* __restrict__ tab,
+                                                             const uint32_t* __restrict__ codes,
+                                                             const uint64_t* __restrict__ soff,
+                                                             const uint16_t* __restrict__ rlen,
+                                                             const DT* __restrict__ x, uint64_t rows, Epi epi,
+                                                             GridReduce gr)
+{
+  static_assert(kThreads == 256, "one table entry per thread");
+  __shared__ DT s_val[256];
+  __shared__ int s_del[256];
+  s_val[threadIdx.x] = tab[threadIdx.x];
+  s_del[threadIdx.x] = reinterpret_cast<const int*>(tab + 256)[threadIdx.x];
+  __syncthreads();
+  const uint64_t nch = (rows + kThreads - 1) / kThreads;
+  const XcdRange xr = xcd_range(nch);
+  const int lane = threadIdx.x & 63;
+  double acc0 = 0.0, acc1 = 0.0;
+  epi.begin();
+  for(uint64_t ch = xr.first; ch < xr.end; ch += xr.stride)
+  {
+    const uint64_t row = ch * kThreads + threadIdx.x;
+    if(row < rows)
+    {
+      const uint32_t* cw = codes + soff[row >> 6] + lane;
+      const int len = rlen[row];
+      const int nw = (len + 3) >> 2;
+      DT sum = DT(0);
+      int w = 0;
+      for(; w + UNRW <= nw; w += UNRW)
+      {
+        uint32_t wd[UNRW];
+#pragma unroll
+        for(int u = 0; u < UNRW; ++u) wd[u] = __builtin_nontemporal_load(cw + (w + u) * 64);
+        DT vv[4 * UNRW], xv[4 * UNRW];
+#pragma unroll
+        for(int u = 0; u < UNRW; ++u)
+#pragma unroll
+          for(int j = 0; j < 4; ++j)
+          {
+            const uint32_t c = (wd[u] >> (8 * j)) & 255u;
+            const bool in = 4 * (w + u) + j < len;
+            vv[4 * u + j] = s_val[c];
+            xv[4 * u + j] = x[in ? uint64_t(std::int64_t(row) + s_del[c]) : uint64_t(0)];
+          }
+#pragma unroll
+        for(int k = 0; k < 4 * UNRW; ++k)
+          if(4 * w + k < len) sum = dfma(vv[k], xv[k], sum);
+      }
+      for(; w < nw; ++w)
+      {
+        const uint32_t wd = __builtin_nontemporal_load(cw + w * 64);
+#pragma unroll
+        for(int j = 0; j < 4; ++j)
+          if(4 * w + j < len)
+          {
+            const uint32_t c = (wd >> (8 * j)) & 255u;
+            sum = dfma(s_val[c], x[uint64_t(std::int64_t(row) + s_del[c])], sum);
+          }
+      }
+      epi.row(row, sum, acc0, acc1);
+    }
+  }
+  grid_reduce_finish(gr, acc0, acc1);
+}
+
 // Runs Epi over the rows of a descriptor (CSR or SELL-64).
 template<typename DT, class Epi, class M>
 int launch_mat(hipStream_t s, const M* A, const DT* x, const Epi& epi, const GridReduce& gr, const char* name)
@@ -341,6 +419,21 @@ int launch_mat(hipStream_t s, const M* A, const DT* x, const Epi& epi, const Gri
                            (const int16_t*)A->col, soff, A->rowlen, x, A->rows, sc, epi, gr);
       else
         return fail(FEAT_ERR_UNSUPPORTED, "SELL index kind");
+      return check_launch(name);
+    }
+    if(A->index == FEAT_IDX_DICT)
+    {
+      const DT* tab = reinterpret_cast<const DT*>(A->val);
+      const uint32_t* cw = static_cast<const uint32_t*>(A->col);
+      if(unr_code == 1)
+        hipLaunchKernelGGL((k_sell_dict_rows<DT, Epi, 1>), dim3(grid), dim3(kThreads), 0, s, tab, cw, soff, A->rowlen, x,
+                           A->rows, epi, gr);
+      else if(unr_code == 3)
+        hipLaunchKernelGGL((k_sell_dict_rows<DT, Epi, 4>), dim3(grid), dim3(kThreads), 0, s, tab, cw, soff, A->rowlen, x,
+                           A->rows, epi, gr);
+      else
+        hipLaunchKernelGGL((k_sell_dict_rows<DT, Epi, 2>), dim3(grid), dim3(kThreads), 0, s, tab, cw, soff, A->rowlen, x,
+                           A->rows, epi, gr);
       return check_launch(name);
     }
     if(A->index == FEAT_IDX_U32)
@@ -440,6 +533,120 @@ __global__ void k_sell_fill(const double* val, const uint64_t* col, const uint64
         sval[p] = DT(0);
         scol[p] = DELTA ? CT(0) : CT(r);
       }
+    }
+  }
+}
+
+// Pair-dictionary construction.  Distinct value bit patterns and distinct
+// column - row offsets are collected in two open-addressing tables of
+// kDictSlots keys (relaxed reads first, compare-and-swap only to insert); the
+// host numbers the keys in ascending order, marks the (value, offset) code
+// pairs present, numbers those, and the fill kernel writes every entry's pair
+// code.  Collection stops early once either table holds more than 256 keys.
+constexpr uint32_t kDictSlots = 1024;
+constexpr uint64_t kDictEmpty = ~uint64_t(0);
+
+__device__ __forceinline__ uint32_t dict_hash(uint64_t key)
+{
+  return uint32_t((key * 0x9E3779B97F4A7C15ull) >> 54) & (kDictSlots - 1);
+}
+__device__ __forceinline__ uint64_t dict_delta_key(uint64_t col, uint64_t row)
+{
+  return uint64_t(std::int64_t(col) - std::int64_t(row)) ^ (uint64_t(1) << 63);
+}
+
+// slot of `key`, inserting it if absent; -1 when the table is full
+__device__ int dict_insert(uint64_t* tab, uint32_t* count, uint64_t key)
+{
+  uint32_t h = dict_hash(key);
+  for(uint32_t p = 0; p < kDictSlots; ++p)
+  {
+    const uint64_t cur = __hip_atomic_load(tab + h, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if(cur == key) return int(h);
+    if(cur == kDictEmpty)
+    {
+      const uint64_t prev = atomicCAS(reinterpret_cast<unsigned long long*>(tab + h), (unsigned long long)kDictEmpty,
+                                      (unsigned long long)key);
+      if(prev == kDictEmpty)
+      {
+        atomicAdd(count, 1u);
+        return int(h);
+      }
+      if(prev == key) return int(h);
+    }
+    h = (h + 1) & (kDictSlots - 1);
+  }
+  return -1;
+}
+
+// slot of a key known to be present
+__device__ __forceinline__ uint32_t dict_find(const uint64_t* tab, uint64_t key)
+{
+  uint32_t h = dict_hash(key);
+  for(uint32_t p = 0; p < kDictSlots && tab[h] != key; ++p) h = (h + 1) & (kDictSlots - 1);
+  return h;
+}
+
+// state[0], state[1]: distinct values / offsets inserted; state[2]: failure flag
+__global__ void k_dict_collect(const double* val, const uint64_t* col, const uint64_t* rp, uint64_t rows,
+                               uint64_t* vtab, uint64_t* dtab, uint32_t* state)
+{
+  for(uint64_t r = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; r < rows; r += uint64_t(gridDim.x) * blockDim.x)
+  {
+    if(__hip_atomic_load(state, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) > 256u ||
+       __hip_atomic_load(state + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) > 256u ||
+       __hip_atomic_load(state + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u)
+      return;
+    for(uint64_t i = rp[r]; i < rp[r + 1]; ++i)
+    {
+      const uint64_t vb = __builtin_bit_cast(uint64_t, val[i]);
+      if(vb == kDictEmpty || dict_insert(vtab, state, vb) < 0 || dict_insert(dtab, state + 1, dict_delta_key(col[i], r)) < 0)
+      {
+        atomicOr(state + 2, 1u);
+        return;
+      }
+    }
+  }
+}
+
+// present[cv * 256 + cd] = 1 for every entry's (value code, offset code)
+__global__ void k_dict_mark(const double* val, const uint64_t* col, const uint64_t* rp, uint64_t rows,
+                            const uint64_t* vtab, const uint64_t* dtab, const uint8_t* vcode, const uint8_t* dcode,
+                            uint32_t* present)
+{
+  for(uint64_t r = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; r < rows; r += uint64_t(gridDim.x) * blockDim.x)
+    for(uint64_t i = rp[r]; i < rp[r + 1]; ++i)
+    {
+      const uint32_t cv = vcode[dict_find(vtab, __builtin_bit_cast(uint64_t, val[i]))];
+      const uint32_t cd = dcode[dict_find(dtab, dict_delta_key(col[i], r))];
+      present[cv * 256u + cd] = 1u;
+    }
+}
+
+// the 4-entries-per-word code layout of the plan (padding bytes 0, never read)
+__global__ void k_dict_fill(const double* val, const uint64_t* col, const uint64_t* rp, uint64_t rows,
+                            const uint64_t* soff, const uint64_t* vtab, const uint64_t* dtab, const uint8_t* vcode,
+                            const uint8_t* dcode, const uint8_t* pcode, uint32_t* codes)
+{
+  for(uint64_t r = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; r < rows; r += uint64_t(gridDim.x) * blockDim.x)
+  {
+    const uint64_t s = r >> 6, lane = r & 63;
+    const uint64_t off = soff[s], nw = (soff[s + 1] - soff[s]) / 64;
+    const uint64_t b = rp[r], len = rp[r + 1] - b;
+    for(uint64_t w = 0; w < nw; ++w)
+    {
+      uint32_t word = 0;
+      for(uint64_t j = 0; j < 4; ++j)
+      {
+        const uint64_t k = 4 * w + j;
+        if(k < len)
+        {
+          const uint32_t cv = vcode[dict_find(vtab, __builtin_bit_cast(uint64_t, val[b + k]))];
+          const uint32_t cd = dcode[dict_find(dtab, dict_delta_key(col[b + k], r))];
+          word |= uint32_t(pcode[cv * 256u + cd]) << (8 * j);
+        }
+      }
+      codes[off + w * 64 + lane] = word;
     }
   }
 }
@@ -1219,8 +1426,127 @@ int feat_matrix_from_csr_f64(const feat_csr_f64_t* A, feat_matrix_f64_t* out)
 } // extern "C"
 
 namespace {
+// Pair-dictionary plan of a scanned CSR (slice maxima hmax).  Returns FEAT_OK with *out
+// filled, or FEAT_ERR_UNSUPPORTED when the matrix has more than 256 distinct
+// (column - row, value) pairs (the caller then builds the plain plan).
 template<typename DT, typename M>
-int sell_create_impl(feat_handle_t h, const feat_csr_f64_t* A, M* out, const char* name)
+int dict_create(feat_handle_t h, const feat_csr_f64_t* A, const std::vector<uint32_t>& hmax, uint16_t* rlen, M* out,
+                const char* name)
+{
+  const uint64_t n = A->rows, ns = (n + 63) / 64;
+  uint64_t *vtab = nullptr, *dtab = nullptr, *soff = nullptr;
+  uint32_t *state = nullptr, *present = nullptr, *codes = nullptr;
+  uint8_t* maps = nullptr;  // vcode[kDictSlots], dcode[kDictSlots], pcode[65536]
+  DT* table = nullptr;
+  auto cleanup = [&]() {
+    for(void* p : {(void*)vtab, (void*)dtab, (void*)soff, (void*)state, (void*)present, (void*)codes, (void*)maps,
+                   (void*)table})
+      if(p) (void)hipFree(p);
+  };
+  const size_t tb = sizeof(uint64_t) * kDictSlots;
+  int rc = check_hip(hipMalloc(&vtab, tb), "dict table");
+  if(!rc) rc = check_hip(hipMalloc(&dtab, tb), "dict table");
+  if(!rc) rc = check_hip(hipMalloc(&state, 3 * sizeof(uint32_t)), "dict state");
+  if(!rc) rc = check_hip(hipMemsetAsync(vtab, 0xFF, tb, h->stream), "memset");
+  if(!rc) rc = check_hip(hipMemsetAsync(dtab, 0xFF, tb, h->stream), "memset");
+  if(!rc) rc = check_hip(hipMemsetAsync(state, 0, 3 * sizeof(uint32_t), h->stream), "memset");
+  if(rc) { cleanup(); return rc; }
+  hipLaunchKernelGGL(k_dict_collect, dim3(grid_for(n)), dim3(kThreads), 0, h->stream, A->val, A->col, A->rowptr, n,
+                     vtab, dtab, state);
+  std::vector<uint64_t> hv(kDictSlots), hd(kDictSlots);
+  uint32_t hs[3] = {0, 0, 0};
+  rc = check_launch(name);
+  if(!rc) rc = check_hip(hipMemcpyAsync(hv.data(), vtab, tb, hipMemcpyDeviceToHost, h->stream), "copy");
+  if(!rc) rc = check_hip(hipMemcpyAsync(hd.data(), dtab, tb, hipMemcpyDeviceToHost, h->stream), "copy");
+  if(!rc) rc = check_hip(hipMemcpyAsync(hs, state, sizeof(hs), hipMemcpyDeviceToHost, h->stream), "copy");
+  if(!rc) rc = check_hip(hipStreamSynchronize(h->stream), "sync");
+  if(rc) { cleanup(); return rc; }
+  if(hs[2] || hs[0] > 256 || hs[1] > 256)
+  {
+    cleanup();
+    return FEAT_ERR_UNSUPPORTED;
+  }
+  // codes of the distinct keys in ascending key order (deterministic)
+  std::vector<uint8_t> hmap(2 * kDictSlots + 65536, 0);
+  std::vector<std::pair<uint64_t, uint32_t>> kv, kd;
+  for(uint32_t i = 0; i < kDictSlots; ++i)
+  {
+    if(hv[i] != kDictEmpty) kv.push_back({hv[i], i});
+    if(hd[i] != kDictEmpty) kd.push_back({hd[i], i});
+  }
+  std::sort(kv.begin(), kv.end());
+  std::sort(kd.begin(), kd.end());
+  for(size_t c = 0; c < kv.size(); ++c) hmap[kv[c].second] = uint8_t(c);
+  for(size_t c = 0; c < kd.size(); ++c) hmap[kDictSlots + kd[c].second] = uint8_t(c);
+  rc = check_hip(hipMalloc(&maps, hmap.size()), "dict maps");
+  if(!rc) rc = check_hip(hipMalloc(&present, sizeof(uint32_t) * 65536), "dict pairs");
+  if(!rc) rc = check_hip(hipMemcpyAsync(maps, hmap.data(), hmap.size(), hipMemcpyHostToDevice, h->stream), "copy");
+  if(!rc) rc = check_hip(hipMemsetAsync(present, 0, sizeof(uint32_t) * 65536, h->stream), "memset");
+  if(rc) { cleanup(); return rc; }
+  hipLaunchKernelGGL(k_dict_mark, dim3(grid_for(n)), dim3(kThreads), 0, h->stream, A->val, A->col, A->rowptr, n, vtab,
+                     dtab, maps, maps + kDictSlots, present);
+  std::vector<uint32_t> hp(65536);
+  rc = check_launch(name);
+  if(!rc) rc = check_hip(hipMemcpyAsync(hp.data(), present, sizeof(uint32_t) * 65536, hipMemcpyDeviceToHost, h->stream), "copy");
+  if(!rc) rc = check_hip(hipStreamSynchronize(h->stream), "sync");
+  if(rc) { cleanup(); return rc; }
+  // pair table: values in the plan's arithmetic type, then the offsets as int32
+  std::vector<unsigned char> htab(256 * sizeof(DT) + 256 * sizeof(int), 0);
+  DT* tv = reinterpret_cast<DT*>(htab.data());
+  int* td = reinterpret_cast<int*>(htab.data() + 256 * sizeof(DT));
+  uint32_t np = 0;
+  for(uint32_t cv = 0; cv < kv.size(); ++cv)
+    for(uint32_t cd = 0; cd < kd.size(); ++cd)
+      if(hp[cv * 256u + cd])
+      {
+        if(np == 256)
+        {
+          cleanup();
+          return FEAT_ERR_UNSUPPORTED;
+        }
+        tv[np] = DT(__builtin_bit_cast(double, kv[cv].first));
+        td[np] = int(std::int64_t(kd[cd].first ^ (uint64_t(1) << 63)));
+        hmap[2 * kDictSlots + cv * 256u + cd] = uint8_t(np);
+        ++np;
+      }
+  // word offsets of the slices: ceil(slice max / 4) words per lane
+  std::vector<uint64_t> hoff(ns + 1, 0);
+  for(uint64_t i = 0; i < ns; ++i) hoff[i + 1] = hoff[i] + (uint64_t(hmax[i]) + 3) / 4 * 64;
+  const uint64_t words = hoff[ns];
+  rc = check_hip(hipMalloc(&table, htab.size()), "dict pair table");
+  if(!rc) rc = check_hip(hipMalloc(&soff, sizeof(uint64_t) * (ns + 1)), "sell soff");
+  if(!rc) rc = check_hip(hipMalloc(&codes, sizeof(uint32_t) * (words ? words : 1)), "dict codes");
+  if(!rc) rc = check_hip(hipMemcpyAsync(table, htab.data(), htab.size(), hipMemcpyHostToDevice, h->stream), "copy");
+  if(!rc) rc = check_hip(hipMemcpyAsync(soff, hoff.data(), sizeof(uint64_t) * (ns + 1), hipMemcpyHostToDevice, h->stream), "copy");
+  if(!rc) rc = check_hip(hipMemcpyAsync(maps, hmap.data(), hmap.size(), hipMemcpyHostToDevice, h->stream), "copy");
+  if(rc) { cleanup(); return rc; }
+  hipLaunchKernelGGL(k_dict_fill, dim3(grid_for(n)), dim3(kThreads), 0, h->stream, A->val, A->col, A->rowptr, n, soff,
+                     vtab, dtab, maps, maps + kDictSlots, maps + 2 * kDictSlots, codes);
+  rc = check_launch(name);
+  if(!rc) rc = check_hip(hipStreamSynchronize(h->stream), "sync");
+  for(void** p : {(void**)&vtab, (void**)&dtab, (void**)&state, (void**)&present, (void**)&maps})
+  {
+    (void)hipFree(*p);
+    *p = nullptr;
+  }
+  if(rc) { cleanup(); return rc; }
+  out->format = FEAT_FMT_SELL;
+  out->index = FEAT_IDX_DICT;
+  out->rows = n;
+  out->cols = A->cols;
+  out->nnz = 4 * words;
+  out->val = (decltype(out->val))(void*)table;
+  out->value_kind = FEAT_VAL_NATIVE;
+  out->scale = 1.0f;
+  out->tune = 0;
+  out->col = codes;
+  out->ptr = soff;
+  out->rowlen = rlen;
+  return FEAT_OK;
+}
+
+template<typename DT, typename M>
+int sell_create_impl(feat_handle_t h, const feat_csr_f64_t* A, M* out, const char* name, int flags = 0)
 {
   // DT is the stored value type: double, float, or uint16_t (FP16 bits)
   if(!h || !A || !out || !A->rowptr || (A->nnz && (!A->val || !A->col))) return fail(FEAT_ERR_INVALID, name);
@@ -1265,6 +1591,24 @@ int sell_create_impl(feat_handle_t h, const feat_csr_f64_t* A, M* out, const cha
   {
     cleanup();
     return fail(FEAT_ERR_UNSUPPORTED, "SELL-64 plan: row too long or padding too large; keep CSR");
+  }
+  if(!(flags & FEAT_SELL_NO_DICT) && !std::is_same<DT, uint16_t>::value && n && A->nnz && hst[0] <= 0x7FFFFFFFu)
+  {
+    (void)hipFree(slmax);
+    (void)hipFree(stats);
+    (void)hipFree(soff);
+    slmax = nullptr;
+    stats = nullptr;
+    soff = nullptr;
+    rc = dict_create<DT>(h, A, hmax, rlen, out, name);
+    if(rc == FEAT_OK) return FEAT_OK;
+    if(rc != FEAT_ERR_UNSUPPORTED)
+    {
+      cleanup();
+      return rc;
+    }
+    rc = check_hip(hipMalloc(&soff, sizeof(uint64_t) * (ns + 1)), "sell soff");
+    if(rc) { cleanup(); return rc; }
   }
   const bool d16 = hst[0] <= 32767u;
   if(!d16 && A->cols > 0xFFFFFFFFull)
@@ -1477,6 +1821,16 @@ int feat_sell_create_f64(feat_handle_t h, const feat_csr_f64_t* A, feat_matrix_f
 int feat_sell_create_f32(feat_handle_t h, const feat_csr_f64_t* A, feat_matrix_f32_t* out)
 {
   return sell_create_impl<float>(h, A, out, "feat_sell_create_f32");
+}
+
+int feat_sell_create_ex_f64(feat_handle_t h, const feat_csr_f64_t* A, int flags, feat_matrix_f64_t* out)
+{
+  return sell_create_impl<double>(h, A, out, "feat_sell_create_ex_f64", flags);
+}
+
+int feat_sell_create_ex_f32(feat_handle_t h, const feat_csr_f64_t* A, int flags, feat_matrix_f32_t* out)
+{
+  return sell_create_impl<float>(h, A, out, "feat_sell_create_ex_f32", flags);
 }
 
 int feat_sell_create_f16(feat_handle_t h, const feat_csr_f64_t* A, feat_matrix_f32_t* out)

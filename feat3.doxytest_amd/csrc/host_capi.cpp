@@ -73,6 +73,11 @@ int feat_runtime_init(int device, void* stream)
   return guarded([&]() { Runtime::initialize(device, stream); });
 }
 
+int feat_runtime_sell_flags(int flags)
+{
+  return guarded([&]() { Runtime::sell_flags() = flags; });
+}
+
 int feat_q1_sizes(int dim, int level, std::uint64_t* sizes)
 {
   return guarded([&]() {

@@ -205,6 +205,10 @@ typedef struct {            /* a CSR matrix on the device (u64 indices) */
 #define FEAT_IDX_U64 0   /* CSR: u64 col/rowptr                              */
 #define FEAT_IDX_U32 1   /* CSR: u32 col/rowptr; SELL: u32 absolute columns  */
 #define FEAT_IDX_D16 2   /* SELL: int16 column minus row                      */
+#define FEAT_IDX_DICT 3  /* SELL: one-byte codes of (column - row, value) pairs;
+                          * val -> pair table: 256 values of the arithmetic type
+                          * followed by 256 int32 offsets; col -> codes, four
+                          * entries per 32-bit lane word; ptr -> word offsets   */
 #define FEAT_VAL_NATIVE 0 /* values in the descriptor's arithmetic type     */
 #define FEAT_VAL_F16 1    /* FP32 arithmetic on FP16 values times `scale`   */
 typedef struct {
@@ -236,8 +240,17 @@ typedef struct {             /* FP32 arithmetic (mixed-precision inner solver) *
 int feat_matrix_from_csr_f64(const feat_csr_f64_t* A, feat_matrix_f64_t* out);
 /* Builds the SELL-64 plan of a CSR matrix on the device (setup time; allocates).
  * Fails with FEAT_ERR_UNSUPPORTED when a row has more than 65535 entries or the
- * slice padding would exceed the CSR size; the caller then keeps the CSR. */
+ * slice padding would exceed the CSR size; the caller then keeps the CSR.
+ * A matrix with at most 256 distinct (column - row, value) pairs -- FE matrices
+ * on uniformly refined meshes: 42 for 3D Q1 -- gets the lossless pair-dictionary
+ * plan (FEAT_IDX_DICT, one byte per entry); others get i16/u32 columns and full
+ * values. */
 int feat_sell_create_f64(feat_handle_t h, const feat_csr_f64_t* A, feat_matrix_f64_t* out);
+/* Same with flags: FEAT_SELL_NO_DICT builds the plain plan even where the
+ * pair dictionary applies. */
+#define FEAT_SELL_NO_DICT 1
+int feat_sell_create_ex_f64(feat_handle_t h, const feat_csr_f64_t* A, int flags, feat_matrix_f64_t* out);
+int feat_sell_create_ex_f32(feat_handle_t h, const feat_csr_f64_t* A, int flags, feat_matrix_f32_t* out);
 /* Same plan with values rounded to FP32 (ConvertPrecond, convert_precond.hpp:100-119). */
 int feat_sell_create_f32(feat_handle_t h, const feat_csr_f64_t* A, feat_matrix_f32_t* out);
 /* Same plan with FP16 values: v16 = half(float(a) * 2^-e) with 2^(e-1) <= max|a| < 2^e;

@@ -68,6 +68,12 @@ namespace feat3amd
       static thread_local feat_handle_t h = nullptr;
       return h;
     }
+    /// flags of the SELL-64 plans built from now on (FEAT_SELL_NO_DICT: no pair dictionary)
+    static int& sell_flags()
+    {
+      static int f = 0;
+      return f;
+    }
     /// Scope in which this thread's calls launch on `h` (graph recording on a solver stream).
     class Use
     {
@@ -316,7 +322,7 @@ namespace feat3amd
             delete m;
           });
           const feat_csr_f64_t c = desc();
-          if(!_use_sell || feat_sell_create_f64(Runtime::handle(), &c, p.get()) != FEAT_OK)
+          if(!_use_sell || feat_sell_create_ex_f64(Runtime::handle(), &c, Runtime::sell_flags(), p.get()) != FEAT_OK)
             check(feat_matrix_from_csr_f64(&c, p.get()), "exec");
           _plan = p;
         }
@@ -348,7 +354,7 @@ namespace feat3amd
             delete m;
           });
           const feat_csr_f64_t c = desc();
-          if(feat_sell_create_f32(Runtime::handle(), &c, p.get()) != FEAT_OK)
+          if(feat_sell_create_ex_f32(Runtime::handle(), &c, Runtime::sell_flags(), p.get()) != FEAT_OK)
           {
             // FP32 CSR copy with 32-bit indices (rows too long for a SELL-64 plan)
             if(_nnz >= (Index(1) << 32) || _cols >= (Index(1) << 32)) throw Error("exec32: matrix too large for u32 indices");
@@ -368,9 +374,12 @@ namespace feat3amd
       }
 
       /// FP16-value SELL-64 plan (FP32 arithmetic, exact power-of-two scale); falls
-      /// back to exec32() when no SELL-64 plan can be built.
+      /// back to exec32() when no SELL-64 plan can be built, and uses it when it is a
+      /// pair-dictionary plan (one byte per entry, smaller than FP16 values).
       const feat_matrix_f32_t& exec16() const
       {
+        if(!_plan16 && exec32().index == FEAT_IDX_DICT)
+          _plan16 = std::shared_ptr<feat_matrix_f32_t>(std::shared_ptr<feat_matrix_f32_t>(), const_cast<feat_matrix_f32_t*>(&exec32()));
         if(!_plan16)
         {
           auto p = std::shared_ptr<feat_matrix_f32_t>(new feat_matrix_f32_t(), [](feat_matrix_f32_t* m) {

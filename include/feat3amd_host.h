@@ -101,6 +101,10 @@ const char* feat_host_last_error(void);
  * to `device` and `stream` (NULL: the null stream).  Call before any other
  * host-layer function; later calls are ignored once initialised. */
 int feat_runtime_init(int device, void* stream);
+/* Flags of the SELL-64 plans the host layer builds from now on (feat3amd.h:
+ * FEAT_SELL_NO_DICT keeps full values and column offsets even where the pair
+ * dictionary applies).  No counterpart in the reference (format selection). */
+int feat_runtime_sell_flags(int flags);
 
 /* ---- Stokes multigrid: Richardson(MG-V(Richardson(Schwarz(AmaVanka)))) ----
  * benchmarks/stokes_multigrid-bench.cpp:714-810 on levels level_fine..level_coarse

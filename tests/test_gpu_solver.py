@@ -185,6 +185,73 @@ def test_sell_plan_apply_bitwise(feat, handle, torch_gpu, oracle, seed, rows, co
     p32.close()
 
 
+def stencil_csr(rng, rows, cols, offsets, values, p_keep):
+    """Rows holding a random subset of `offsets` (col - row, clipped to the matrix),
+    values drawn from `values`: at most len(offsets) * len(values) distinct pairs."""
+    from oracle.pyoracle import Csr
+    rp, col = [0], []
+    for r in range(rows):
+        c = sorted({r + o for o in offsets if 0 <= r + o < cols and rng.random() < p_keep})
+        col += c
+        rp.append(len(col))
+    val = rng.choice(np.asarray(values, np.float64), size=len(col))
+    return Csr(rows, cols, val, np.asarray(col, np.uint64), np.asarray(rp, np.uint64))
+
+
+@pytest.mark.parametrize("case", ["q1", "stencil", "partial", "many_pairs", "many_offsets", "no_dict"])
+def test_sell_dict_plan_bitwise(feat, handle, torch_gpu, oracle, case):
+    """Pair-dictionary SELL plans (one byte per entry) reproduce the CSR loop bit for
+    bit in FP64 and FP32; matrices with more than 256 distinct (col - row, value)
+    pairs, or dict=False, get the plain plan."""
+    torch = torch_gpu
+    k, _ = feat.libs()
+    P = feat.ptr
+    rng = np.random.default_rng(77)
+    offs = [-130, -129, -128, -2, -1, 0, 1, 2, 128, 129, 130, 4000]
+    vals = [0.25, -1.0, 3.5, -0.125, 2.0 / 3.0, 1e-300, -7.0]
+    want = feat.IDX_DICT
+    if case == "q1":
+        A = q1_level(feat, 3, 4)[1]
+    elif case == "stencil":
+        A = stencil_csr(rng, 5000, 5000, offs, vals, 0.8)
+    elif case == "partial":
+        A = stencil_csr(rng, 129, 140, offs[:9], vals, 0.5)      # last slice holds one row, empty rows
+    elif case == "many_pairs":
+        A = stencil_csr(rng, 4000, 4000, list(range(-20, 20)), [float(v) for v in range(1, 9)], 1.0)  # 320 pairs
+        want = feat.IDX_D16
+    elif case == "many_offsets":
+        from tests.helpers import random_csr
+        A = random_csr(rng, 3000, 70000, 10, 20)
+        A.val = rng.choice(np.asarray(vals), size=A.nnz)           # few values, 1000s of offsets
+        want = feat.IDX_U32
+    else:
+        A = q1_level(feat, 3, 4)[1]
+        want = feat.IDX_D16
+    v, c, r = upload_csr(torch, A)
+    plan = feat.SellPlan(handle, v, c, r, A.rows, A.cols, dict=(case != "no_dict"))
+    assert plan.m.format == feat.FMT_SELL and plan.m.index == want, (plan.m.index, want)
+    x = rng.standard_normal(A.cols)
+    y = rng.standard_normal(A.rows)
+    xd, yd = dev(torch, x), dev(torch, y)
+    rd = torch.empty(A.rows, dtype=torch.float64, device="cuda")
+    for tune in (0, 1, 3):
+        plan.m.tune = tune
+        for a, b in [(1.0, 0.0), (-1.0, 1.0)]:
+            feat.check(k.feat_matrix_apply_f64(handle.h, plan.m, P(rd), a, P(xd), b, P(yd)), "dict apply")
+            handle.sync()
+            assert np.array_equal(host(rd), oracle.csr_apply(A, x, y, a, b)), (tune, a, b)
+    plan.close()
+    p32 = feat.SellPlan(handle, v, c, r, A.rows, A.cols, fp="f32", dict=(case != "no_dict"))
+    assert p32.m.index == want
+    x32 = dev(torch, x.astype(np.float32))
+    r32 = torch.empty(A.rows, dtype=torch.float32, device="cuda")
+    keep_ws, ws = feat.reduce_ws(torch, "cuda")
+    feat.check(k.feat_apply_masked_f32(handle.h, p32.m, P(r32), P(x32), None, None, ws), "dict f32")
+    handle.sync()
+    assert np.array_equal(host(r32), oracle.csr_apply_f32(A, x.astype(np.float32)))
+    p32.close()
+
+
 def test_sell_plan_rejects_long_rows(feat, handle, torch_gpu):
     # a 70000-entry row exceeds the 16-bit row length; the plan refuses and the CSR stays usable
     torch = torch_gpu
