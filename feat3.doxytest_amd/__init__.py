@@ -174,6 +174,7 @@ KERNEL_API = [
     ("feat_mlp_smooth_f64", _i, [_vp, _u64, _vp, _vp, _vp, _vp, _vp, _d, _d, _vp]),
     ("feat_mlp_features_full", _i, [_vp, _u64, _vp, _vp, _vp, _vp, _vp, _vp]),
     ("feat_mlp_energy", _i, [_vp, _u64, _i, _d, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    ("feat_mlp_step_f64", _i, [_vp, _u64, _vp, _vp, _vp, _vp, _vp, _vp]),
 ]
 
 HOST_API = [

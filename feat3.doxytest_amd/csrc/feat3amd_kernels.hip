@@ -399,9 +399,12 @@ int launch_mat(hipStream_t s, const M* A, const DT* x, const Epi& epi, const Gri
   {
     // one workgroup per 256-row chunk up to the reduction-partials limit, so the
     // hardware dispatcher balances the load; larger matrices stride.  A tuned
-    // descriptor caps the grid lower (workgroups stride over the chunks).
+    // descriptor caps the grid lower (workgroups stride over the chunks) -- except
+    // for launches that reduce: their partial sums follow the grid, and the default
+    // grid keeps every dot and norm independent of the tuning.
     const uint64_t nch = (A->rows + kThreads - 1) / kThreads;
-    const int cap_code = (A->tune >> 2) & 7;
+    const bool reduces = gr.out0 != nullptr || gr.out1 != nullptr;
+    const int cap_code = reduces ? 0 : (A->tune >> 2) & 7;
     const uint64_t cap = cap_code ? (uint64_t(kMaxGrid) >> cap_code) : uint64_t(kMaxGrid);
     const int grid = int(nch < cap ? nch : cap);
     const int unr_code = A->tune & 3;
@@ -555,11 +558,13 @@ __device__ __forceinline__ uint64_t dict_delta_key(uint64_t col, uint64_t row)
   return uint64_t(std::int64_t(col) - std::int64_t(row)) ^ (uint64_t(1) << 63);
 }
 
-// slot of `key`, inserting it if absent; -1 when the table is full
+// slot of `key`, inserting it if absent; -1 after kDictProbes occupied slots (at most 256
+// keys in 1024 slots: longer probe runs mean the matrix has too many pairs anyway)
+constexpr uint32_t kDictProbes = 64;
 __device__ int dict_insert(uint64_t* tab, uint32_t* count, uint64_t key)
 {
   uint32_t h = dict_hash(key);
-  for(uint32_t p = 0; p < kDictSlots; ++p)
+  for(uint32_t p = 0; p < kDictProbes; ++p)
   {
     const uint64_t cur = __hip_atomic_load(tab + h, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if(cur == key) return int(h);
@@ -600,7 +605,9 @@ __global__ void k_dict_collect(const double* val, const uint64_t* col, const uin
     for(uint64_t i = rp[r]; i < rp[r + 1]; ++i)
     {
       const uint64_t vb = __builtin_bit_cast(uint64_t, val[i]);
-      if(vb == kDictEmpty || dict_insert(vtab, state, vb) < 0 || dict_insert(dtab, state + 1, dict_delta_key(col[i], r)) < 0)
+      if(vb == kDictEmpty || dict_insert(vtab, state, vb) < 0 || dict_insert(dtab, state + 1, dict_delta_key(col[i], r)) < 0 ||
+         __hip_atomic_load(state, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) > 256u ||
+         __hip_atomic_load(state + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) > 256u)
       {
         atomicOr(state + 2, 1u);
         return;

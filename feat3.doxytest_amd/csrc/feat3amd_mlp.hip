@@ -240,13 +240,25 @@ __global__ __launch_bounds__(64 * kWaves) void k_mlp_train(const float* __restri
   for(int i = kB3 + 1 + l; i < kLoss; i += 64) out[i] = 0.0f;
 }
 
-// grad[i] = sum over the partial vectors in index order; padded inputs get no gradient
-__global__ void k_mlp_reduce(const float* __restrict__ partials, uint64_t nparts, float* __restrict__ grad)
+// Gradient reduction in a fixed order: segment g sums partial vectors 32g .. 32g+31 in
+// index order (stage 0), then grad[i] sums the segments in index order (stage 1);
+// padded inputs get no gradient.
+constexpr int kRedSeg = 32;
+__global__ void k_mlp_reduce_seg(const float* __restrict__ partials, uint64_t nparts, float* __restrict__ seg)
+{
+  const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if(i >= uint64_t(FEAT_MLP_NPARAM_PAD)) return;
+  const uint64_t w0 = uint64_t(blockIdx.y) * kRedSeg, w1 = w0 + kRedSeg < nparts ? w0 + kRedSeg : nparts;
+  float s = 0.0f;
+  for(uint64_t w = w0; w < w1; ++w) s += partials[w * FEAT_MLP_NPARAM_PAD + i];
+  seg[uint64_t(blockIdx.y) * FEAT_MLP_NPARAM_PAD + i] = s;
+}
+__global__ void k_mlp_reduce(const float* __restrict__ seg, uint64_t nseg, float* __restrict__ grad)
 {
   const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if(i >= uint64_t(FEAT_MLP_NPARAM_PAD)) return;
   float s = 0.0f;
-  for(uint64_t w = 0; w < nparts; ++w) s += partials[w * FEAT_MLP_NPARAM_PAD + i];
+  for(uint64_t w = 0; w < nseg; ++w) s += seg[w * FEAT_MLP_NPARAM_PAD + i];
   if(i < uint64_t(kB1) && (i % kInP) >= uint64_t(kIn)) s = 0.0f;
   grad[i] = s;
 }
@@ -278,30 +290,113 @@ __global__ void k_mlp_bf16(const float* __restrict__ p, uint16_t* __restrict__ w
   else if(i < 2 * kHid * kInP) wbf[i] = f2bf(p[kW2 + (i - kHid * kInP)]);
 }
 
-// Plain inference on given inputs (tests, evaluation).
+// Orders this wavefront's LDS accesses (the buffers are private to the wavefront, and
+// a wavefront's LDS operations execute in issue order): no workgroup barrier needed.
+__device__ __forceinline__ void wave_sync_lds()
+{
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Register-resident weights of the inference kernels: the MFMA B operands of layers 1
+// and 2 (B1[nt][ks][i] = W1[nt*16 + c][ks*16 + 4q + i], same for W2) and this lane's
+// biases and output weights (unit nt*16 + c).
+struct RegWeights
+{
+  bf16x4 B1[4][4], B2[4][4];
+  float b1r[4], b2r[4], w3r[4];
+  float b3;
+};
+
+__device__ __forceinline__ void load_reg_weights(RegWeights& R, const float* __restrict__ p, const uint16_t* __restrict__ wbf)
+{
+  const int l = threadIdx.x & 63, c = l & 15, q = l >> 4;
+  for(int nt = 0; nt < 4; ++nt)
+  {
+    for(int ks = 0; ks < 4; ++ks)
+#pragma unroll
+      for(int i = 0; i < 4; ++i)
+      {
+        R.B1[nt][ks][i] = short(wbf[(nt * 16 + c) * kInP + ks * 16 + 4 * q + i]);
+        R.B2[nt][ks][i] = short(wbf[kHid * kInP + (nt * 16 + c) * kHid + ks * 16 + 4 * q + i]);
+      }
+    R.b1r[nt] = p[kB1 + nt * 16 + c];
+    R.b2r[nt] = p[kB2 + nt * 16 + c];
+    R.w3r[nt] = p[kW3 + nt * 16 + c];
+  }
+  R.b3 = p[kB3];
+}
+
+// Forward pass of the wavefront's 16 samples from their layer-1 A operands (lane l holds
+// inputs ks*16 + 4(l/16) + i of sample l%16); h1: the wavefront's 16 x 64 BF16 LDS
+// buffer.  Returns, in lane l with l%16 < 4, the output y of sample 4(l/16) + l%16
+// (layer 3 summed over the units in a fixed order: 4 per lane, then a butterfly over
+// the 16 lanes).
+__device__ __forceinline__ float forward_reg(const RegWeights& R, const bf16x4 (&A1)[4], uint16_t* h1)
+{
+  const int l = threadIdx.x & 63, c = l & 15, q = l >> 4;
+#pragma unroll
+  for(int nt = 0; nt < 4; ++nt)
+  {
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for(int ks = 0; ks < 4; ++ks) acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A1[ks], R.B1[nt][ks], acc, 0, 0, 0);
+#pragma unroll
+    for(int i = 0; i < 4; ++i) h1[(4 * q + i) * kHid + nt * 16 + c] = f2bf(gelu(acc[i] + R.b1r[nt]));
+  }
+  wave_sync_lds();
+  bf16x4 A2[4];
+#pragma unroll
+  for(int ks = 0; ks < 4; ++ks)
+#pragma unroll
+    for(int i = 0; i < 4; ++i) A2[ks][i] = short(h1[c * kHid + ks * 16 + 4 * q + i]);
+  float part[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for(int nt = 0; nt < 4; ++nt)
+  {
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for(int ks = 0; ks < 4; ++ks) acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A2[ks], R.B2[nt][ks], acc, 0, 0, 0);
+#pragma unroll
+    for(int i = 0; i < 4; ++i) part[i] = __fmaf_rn(R.w3r[nt], gelu(acc[i] + R.b2r[nt]), part[i]);
+  }
+  // the 16 lanes with the same q hold the same 4 samples (4q + i) and different units
+#pragma unroll
+  for(int i = 0; i < 4; ++i)
+    for(int o = 1; o < 16; o <<= 1) part[i] += __shfl_xor(part[i], o, 64);
+  const float yc = (c & 3) == 0 ? part[0] : (c & 3) == 1 ? part[1] : (c & 3) == 2 ? part[2] : part[3];
+  wave_sync_lds();
+  return yc + R.b3;
+}
+
+
+// Plain inference on given inputs (training, tests, evaluation): register-resident
+// weights, each lane loads its 4 x 4 inputs of sample l%16 straight into the A operand.
 __global__ __launch_bounds__(64 * kWaves) void k_mlp_infer(const float* __restrict__ p, const uint16_t* __restrict__ wbf,
                                                            const uint16_t* __restrict__ X, uint64_t nsamp,
                                                            float* __restrict__ y)
 {
-  __shared__ SharedWeights W;
-  __shared__ WaveBuf Bw[kWaves];
-  const int wave = threadIdx.x >> 6, l = threadIdx.x & 63;
-  WaveBuf& B = Bw[wave];
-  load_weights(W, p, wbf);
-  __syncthreads();
+  __shared__ uint16_t h1s[kWaves][16 * kHid];
+  const int wave = threadIdx.x >> 6, l = threadIdx.x & 63, c = l & 15, q = l >> 4;
+  RegWeights RW;
+  load_reg_weights(RW, p, wbf);
   const uint64_t ngroups = (nsamp + 16 * kWaves - 1) / (16 * kWaves);
   for(uint64_t g = blockIdx.x; g < ngroups; g += gridDim.x)
   {
     const uint64_t s0 = (g * kWaves + uint64_t(wave)) * 16;
-    for(int e = l; e < 16 * kInP; e += 64)
+    const uint64_t smp = s0 + uint64_t(c);
+    bf16x4 A1[4];
+#pragma unroll
+    for(int ks = 0; ks < 4; ++ks)
     {
-      const uint64_t s = s0 + uint64_t(e / kInP);
-      B.x[e] = s < nsamp ? X[s * kInP + uint64_t(e % kInP)] : uint16_t(0);
+      // X rows are 128-byte aligned (64 BF16): 4 consecutive inputs are one 8-byte load
+      const bf16x4 v = *reinterpret_cast<const bf16x4*>(X + (smp < nsamp ? smp : 0) * kInP + ks * 16 + 4 * q);
+      A1[ks] = smp < nsamp ? v : bf16x4{0, 0, 0, 0};
     }
-    __syncthreads();
-    forward12(W, B);
-    if(l < 16 && s0 + uint64_t(l) < nsamp) y[s0 + uint64_t(l)] = B.y[l];
-    __syncthreads();
+    const float yv = forward_reg(RW, A1, h1s[wave]);
+    const uint64_t so = s0 + uint64_t(4 * q + (c & 3));
+    if(c < 4 && so < nsamp) y[so] = yv;
   }
 }
 
@@ -366,6 +461,7 @@ __global__ void k_mlp_features(Grid3 G, const double* __restrict__ R, const doub
   t[k] = ok ? float(e[i] / s) : 0.0f;
 }
 
+
 // The learned smoothing step on the interior nodes:
 //   x_i += omega * (jacobi * r_i / a_ii + s_i * f(patch_i)).
 // Inference layout: the weights stay in registers as MFMA B operands for the whole
@@ -381,23 +477,8 @@ __global__ __launch_bounds__(64 * kWaves) void k_mlp_smooth(Grid3 G, const float
   __shared__ uint16_t h1s[kWaves][16 * kHid];
   __shared__ double scs[kWaves][16];
   const int wave = threadIdx.x >> 6, l = threadIdx.x & 63, c = l & 15, q = l >> 4;
-  // B operands: B1[nt][ks][i] = W1[nt*16 + c][ks*16 + 4q + i], same for W2
-  bf16x4 B1[4][4], B2[4][4];
-  float b1r[4], b2r[4], w3r[4];
-  for(int nt = 0; nt < 4; ++nt)
-  {
-    for(int ks = 0; ks < 4; ++ks)
-#pragma unroll
-      for(int i = 0; i < 4; ++i)
-      {
-        B1[nt][ks][i] = short(wbf[(nt * 16 + c) * kInP + ks * 16 + 4 * q + i]);
-        B2[nt][ks][i] = short(wbf[kHid * kInP + (nt * 16 + c) * kHid + ks * 16 + 4 * q + i]);
-      }
-    b1r[nt] = p[kB1 + nt * 16 + c];
-    b2r[nt] = p[kB2 + nt * 16 + c];
-    w3r[nt] = p[kW3 + nt * 16 + c];
-  }
-  const float b3 = p[kB3];
+  RegWeights RW;
+  load_reg_weights(RW, p, wbf);
   const uint64_t m = G.m, mm = m * m;
   const uint64_t ngroups = (G.n + 16 * kWaves - 1) / (16 * kWaves);
   for(uint64_t g = blockIdx.x; g < ngroups; g += gridDim.x)
@@ -411,9 +492,28 @@ __global__ __launch_bounds__(64 * kWaves) void k_mlp_smooth(Grid3 G, const float
       const uint64_t xx = node % m, yy = (node / m) % m, zz = node / mm;
       interior = xx > 0 && yy > 0 && zz > 0 && xx + 1 < m && yy + 1 < m && zz + 1 < m;
     }
-    const double di = interior ? dinv[node] : 0.0;
+    // every load is issued before any is used (no branches around them): patch
+    // entries this lane does not need read element 0, which always exists
+    const double di = dinv[interior ? node : 0];
     double rv[16];
     float fv[16];
+#pragma unroll
+    for(int ks = 0; ks < 4; ++ks)
+#pragma unroll
+      for(int i = 0; i < 4; ++i)
+      {
+        const int k = ks * 16 + 4 * q + i, e = ks * 4 + i;
+        const int j = k < 27 ? k : (k < kIn ? k - 27 : 13);
+        const int64_t off = int64_t(j % 3 - 1) + int64_t((j / 3) % 3 - 1) * int64_t(m) + int64_t(j / 9 - 1) * int64_t(mm);
+        const uint64_t nb = interior ? uint64_t(int64_t(node) + off) : 0;
+        rv[e] = r[k < 27 ? nb : 0];
+        fv[e] = cfeat[(k >= 27 && k < kIn) ? nb : 0];
+      }
+    // this lane's share of the final update (lanes c < 4 write node s0 + 4q + c)
+    const uint64_t nd_out = s0 + uint64_t(4 * q + (c & 3));
+    const bool writer = c < 4 && nd_out < G.n;
+    const uint64_t ndl = writer ? nd_out : 0;
+    const double r_out = r[ndl], d_out = dinv[ndl], x_out = x[ndl];
     double mx = 0.0;
 #pragma unroll
     for(int ks = 0; ks < 4; ++ks)
@@ -421,19 +521,10 @@ __global__ __launch_bounds__(64 * kWaves) void k_mlp_smooth(Grid3 G, const float
       for(int i = 0; i < 4; ++i)
       {
         const int k = ks * 16 + 4 * q + i, e = ks * 4 + i;
-        rv[e] = 0.0;
-        fv[e] = 0.0f;
-        if(!interior || k >= kIn) continue;
-        const int j = k < 27 ? k : k - 27;
-        const int64_t off = int64_t(j % 3 - 1) + int64_t((j / 3) % 3 - 1) * int64_t(m) + int64_t(j / 9 - 1) * int64_t(mm);
-        const uint64_t nb = uint64_t(int64_t(node) + off);
-        if(k < 27)
-        {
-          rv[e] = r[nb] * di;
-          mx = fmax(mx, fabs(rv[e]));
-        }
-        else
-          fv[e] = cfeat[nb];
+        const bool use_r = interior && k < 27;
+        rv[e] = use_r ? rv[e] * di : 0.0;
+        fv[e] = (interior && k >= 27 && k < kIn) ? fv[e] : 0.0f;
+        mx = fmax(mx, fabs(rv[e]));
       }
     // s = max over the node's 27 residuals: the four lanes c, c+16, c+32, c+48
     mx = fmax(mx, __shfl_xor(mx, 16, 64));
@@ -451,49 +542,17 @@ __global__ __launch_bounds__(64 * kWaves) void k_mlp_smooth(Grid3 G, const float
         A1[ks][i] = short(f2bf(v));
       }
     if(q == 0) scs[wave][c] = interior ? s : 0.0;
-    // ---- layer 1 (MFMA), GELU, h1 to LDS in A layout
-#pragma unroll
-    for(int nt = 0; nt < 4; ++nt)
+    const float y = forward_reg(RW, A1, h1s[wave]);
+    if(writer)
     {
-      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-      for(int ks = 0; ks < 4; ++ks) acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A1[ks], B1[nt][ks], acc, 0, 0, 0);
-#pragma unroll
-      for(int i = 0; i < 4; ++i) h1s[wave][(4 * q + i) * kHid + nt * 16 + c] = f2bf(gelu(acc[i] + b1r[nt]));
-    }
-    __syncthreads();
-    bf16x4 A2[4];
-#pragma unroll
-    for(int ks = 0; ks < 4; ++ks)
-#pragma unroll
-      for(int i = 0; i < 4; ++i) A2[ks][i] = short(h1s[wave][c * kHid + ks * 16 + 4 * q + i]);
-    // ---- layer 2 (MFMA) and layer 3: per node sum of w3 gelu(z2) over the 64 units
-    float part[4] = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for(int nt = 0; nt < 4; ++nt)
-    {
-      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-      for(int ks = 0; ks < 4; ++ks) acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A2[ks], B2[nt][ks], acc, 0, 0, 0);
-#pragma unroll
-      for(int i = 0; i < 4; ++i) part[i] = __fmaf_rn(w3r[nt], gelu(acc[i] + b2r[nt]), part[i]);
-    }
-    // the 16 lanes with the same q hold the same 4 nodes (4q + i) and different units
-#pragma unroll
-    for(int i = 0; i < 4; ++i)
-      for(int o = 1; o < 16; o <<= 1) part[i] += __shfl_xor(part[i], o, 64);
-    if(c < 4)
-    {
-      const int nl = 4 * q + c;
-      const uint64_t nd = s0 + uint64_t(nl);
-      const double sn = scs[wave][nl];
-      if(nd < G.n && sn > 0.0)
+      const double sn = scs[wave][4 * q + c];
+      if(sn > 0.0)
       {
-        const double cval = __fma_rn(jacobi, r[nd] * dinv[nd], sn * double(part[c] + b3));
-        x[nd] = __fma_rn(omega, cval, x[nd]);
+        const double cval = __fma_rn(jacobi, r_out * d_out, sn * double(y));
+        x[nd_out] = __fma_rn(omega, cval, x_out);
       }
     }
-    __syncthreads();
+    wave_sync_lds();
   }
 }
 
@@ -510,8 +569,8 @@ __global__ void k_mlp_features_full(Grid3 G, const double* __restrict__ r, const
   rho[i] = r[i] * dinv[i];
 }
 
-// stage 0: e' = e - (jacobi rho + s y) where s > 0 (else e' = e - jacobi rho masked to interior by s)
-// stage 1: t = y + s q / 2 (the target whose squared-error gradient is the energy gradient)
+// stage 0: e' = e - (coef rho + s y) where s > 0, else 0 (e == nullptr: e = 0)
+// stage 1: t = y + coef s q / 2 (the target whose squared-error gradient is the energy gradient)
 __global__ void k_mlp_energy(uint64_t n, int stage, double coef, const float* __restrict__ y,
                              const double* __restrict__ sc, const double* __restrict__ rho, const double* __restrict__ e,
                              const double* __restrict__ q, double* __restrict__ ep, float* __restrict__ t)
@@ -520,9 +579,23 @@ __global__ void k_mlp_energy(uint64_t n, int stage, double coef, const float* __
   if(i >= n) return;
   const double s = sc[i];
   if(stage == 0)
-    ep[i] = s > 0.0 ? e[i] - __fma_rn(coef, rho[i], s * double(y[i])) : 0.0;
+    ep[i] = s > 0.0 ? (e ? e[i] : 0.0) - __fma_rn(coef, rho[i], s * double(y[i])) : 0.0;
   else
     t[i] = s > 0.0 ? float(double(y[i]) + 0.5 * coef * s * q[i]) : y[i];
+}
+
+// energy-optimal step along c: alpha = a / b (0 unless b > 0 and the ratio is finite),
+// x += alpha c, d -= alpha q
+__global__ void k_mlp_step(uint64_t n, double* __restrict__ x, double* __restrict__ d, const double* __restrict__ c,
+                           const double* __restrict__ q, const double* __restrict__ a, const double* __restrict__ b)
+{
+  const double bb = *b, al = bb > 0.0 ? (*a) / bb : 0.0;
+  const double alpha = std::isfinite(al) ? al : 0.0;
+  for(uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
+  {
+    x[i] = __fma_rn(alpha, c[i], x[i]);
+    d[i] = __fma_rn(-alpha, q[i], d[i]);
+  }
 }
 
 int mlp_grid(uint64_t nsamp)
@@ -556,13 +629,17 @@ int feat_mlp_grad(feat_handle_t h, const float* params, const uint16_t* wbf, con
 {
   if(!h || !params || !wbf || !X || !t || !partials || !grad || !nsamp)
     return fail(FEAT_ERR_INVALID, "feat_mlp_grad: bad argument");
-  const int grid = mlp_grid(nsamp);
-  const uint64_t nparts = uint64_t(grid) * kWaves;
-  if(nparts > partials_cap) return fail(FEAT_ERR_INVALID, "feat_mlp_grad: partials buffer too small");
+  // at most 256 workgroups (each strides over its sample groups): few partial vectors
+  const uint64_t ng = (nsamp + 16 * kWaves - 1) / (16 * kWaves);
+  const int grid = int(ng < 256 ? ng : 256);
+  const uint64_t nparts = uint64_t(grid) * kWaves, nseg = (nparts + kRedSeg - 1) / kRedSeg;
+  if(nparts + nseg > partials_cap) return fail(FEAT_ERR_INVALID, "feat_mlp_grad: partials buffer too small");
+  float* seg = partials + nparts * FEAT_MLP_NPARAM_PAD;
   hipLaunchKernelGGL(k_mlp_train, dim3(grid), dim3(64 * kWaves), 0, h->stream, params, wbf, X, t, nsamp,
                      1.0f / float(nsamp), partials);
-  hipLaunchKernelGGL(k_mlp_reduce, dim3((FEAT_MLP_NPARAM_PAD + 255) / 256), dim3(256), 0, h->stream, partials, nparts,
-                     grad);
+  hipLaunchKernelGGL(k_mlp_reduce_seg, dim3((FEAT_MLP_NPARAM_PAD + 255) / 256, unsigned(nseg)), dim3(256), 0, h->stream,
+                     partials, nparts, seg);
+  hipLaunchKernelGGL(k_mlp_reduce, dim3((FEAT_MLP_NPARAM_PAD + 255) / 256), dim3(256), 0, h->stream, seg, nseg, grad);
   return check_launch("feat_mlp_grad");
 }
 
@@ -611,10 +688,21 @@ int feat_mlp_features_full(feat_handle_t h, uint64_t m, const double* r, const d
 int feat_mlp_energy(feat_handle_t h, uint64_t n, int stage, double coef, const float* y, const double* s,
                     const double* rho, const double* e, const double* q, double* ep, float* t)
 {
-  if(!h || !n || !y || !s || (stage == 0 && (!rho || !e || !ep)) || (stage == 1 && (!q || !t)))
+  if(!h || !n || !y || !s || (stage == 0 && (!rho || !ep)) || (stage == 1 && (!q || !t)))
     return fail(FEAT_ERR_INVALID, "feat_mlp_energy: bad argument");
   hipLaunchKernelGGL(k_mlp_energy, dim3((n + 255) / 256), dim3(256), 0, h->stream, n, stage, coef, y, s, rho, e, q, ep, t);
   return check_launch("feat_mlp_energy");
+}
+
+int feat_mlp_step_f64(feat_handle_t h, uint64_t n, double* x, double* d, const double* c, const double* q,
+                      const double* a, const double* b)
+{
+  if(!h || !a || !b || (n && (!x || !d || !c || !q))) return fail(FEAT_ERR_INVALID, "feat_mlp_step_f64: null argument");
+  if(x == c || d == q || x == d) return fail(FEAT_ERR_ALIAS, "feat_mlp_step_f64: aliasing arguments");
+  if(!n) return FEAT_OK;
+  const uint64_t g = (n + 4 * 256 - 1) / (4 * 256);
+  hipLaunchKernelGGL(k_mlp_step, dim3(g < 8192 ? g : 8192), dim3(256), 0, h->stream, n, x, d, c, q, a, b);
+  return check_launch("feat_mlp_step_f64");
 }
 
 } // extern "C"

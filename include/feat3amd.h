@@ -477,9 +477,11 @@ int feat_copy_int_to_host(feat_handle_t h, const int* dev, int* host_pinned);
 
 /* ---------------- autotuner ----------------
  * Launch shapes of the SELL-64 row kernels that every fused entry point uses
- * (entries per row in flight: 4, 8 or 16; workgroup cap: all row chunks, or
- * 8192/4096/2048/1024/512 workgroups striding over them).  Every shape gives
- * bit-identical results (each row is summed in CSR order).  The tuner times
+ * (entries per row in flight: 4, 8 or 16 -- pair-dictionary plans: 4, 8 or 16
+ * codes; workgroup cap: all row chunks, or 8192/4096/2048/1024/512 workgroups
+ * striding over them; launches that also compute a dot or norm keep the full
+ * grid).  Every shape gives bit-identical results (each row is summed in CSR
+ * order, reductions do not depend on the shape).  The tuner times
  * r = A*x for every shape with HIP events (reps launches each) and stores the
  * fastest in A->tune; *best_us (may be NULL) receives its time per launch. */
 int feat_autotune_matrix_f64(feat_handle_t h, feat_matrix_f64_t* A, const double* x, double* r,
@@ -502,7 +504,8 @@ int feat_mlp_infer(feat_handle_t h, const float* params, const uint16_t* wbf, co
                    uint64_t nsamp, float* y);
 /* grad (FEAT_MLP_NPARAM_PAD floats; grad[FEAT_MLP_NPARAM_PAD-1] = sum of squared
  * errors) of the mean squared error over nsamp samples; partials: scratch of
- * partials_cap * FEAT_MLP_NPARAM_PAD floats, partials_cap >= 4 * min(2048, ceil(nsamp/256)) */
+ * partials_cap * FEAT_MLP_NPARAM_PAD floats, partials_cap >= 4 w + ceil(4 w / 32) with
+ * w = min(256, ceil(nsamp / 64)) (1056 always suffices) */
 int feat_mlp_grad(feat_handle_t h, const float* params, const uint16_t* wbf, const uint16_t* X,
                   const float* t, uint64_t nsamp, float* partials, uint64_t partials_cap, float* grad);
 /* one Adam step (bias-corrected, step >= 1); refreshes wbf */
@@ -525,9 +528,14 @@ int feat_mlp_smooth_f64(feat_handle_t h, uint64_t m, const float* params, const 
 /* inputs of every node (whole-field training): X (m^3 rows), s, rho_i = r_i / a_ii */
 int feat_mlp_features_full(feat_handle_t h, uint64_t m, const double* r, const double* dinv,
                            const float* cfeat, uint16_t* X, double* s, double* rho);
+/* energy-optimal step of the learned smoother along c (q = A c, *a = c . d, *b = c . q on
+ * the device): alpha = *a / *b (0 unless *b > 0 and the ratio is finite), x += alpha c,
+ * d -= alpha q -- the error energy never grows */
+int feat_mlp_step_f64(feat_handle_t h, uint64_t n, double* x, double* d, const double* c,
+                      const double* q, const double* a, const double* b);
 /* energy-norm training of the learned correction on a field pair (r = A e, e):
  * stage 0 (coef = the Jacobi weight): ep = e - (coef rho + s y) (the error after one
- * learned step, 0 where s = 0);
+ * learned step, 0 where s = 0; e may be NULL for e = 0);
  * stage 1 (coef = a loss weight w), given q = A ep: t = y + w s q / 2, so that the
  * squared-error gradient of feat_mlp_grad over all n nodes is the gradient of
  * w ep^T A ep / (2 n) */

@@ -12,7 +12,10 @@
 //   x_i += omega * (omega_jac rho_i + s_i f(rho / s_i, log-coefficient patch)),
 //   rho = r / a_ii over the node's 3x3x3 patch, s_i = max |rho|,
 // with f the BF16 MLP of feat3amd.h (output layer initialised to zero, so the untrained
-// smoother is Jacobi) trained on residual/error fields that the FP64 Jacobi multigrid
+// smoother is Jacobi), each step taken with the energy-optimal length along its direction
+// (so no smoothing step can increase the error's energy norm: the V-cycle stays
+// energy-stable whatever the network outputs), trained on residual/error fields that the
+// FP64 Jacobi multigrid
 // itself produces on the error equation -- the input of every finest-level smoothing step,
 // first with the Jacobi smoother, then (on-policy refresh) with the learned one -- to
 // minimise the energy norm of the error one learned step leaves, relative to the energy
@@ -77,6 +80,7 @@ namespace feat3amd
       DeviceArray<std::uint16_t> _wbf0;
       DeviceArray<float> _y;
       DeviceArray<double> _S, _Rho, _ep, _q;
+      Vec _cv, _cq;   // learned step direction c and A c
       Index _partials_cap = 0;
       int _adam_step = 0;
       // training data
@@ -103,38 +107,42 @@ namespace feat3amd
 
       Level& L0() { return *_lv[0]; }
 
-      // finest-level learned smoothing: x = 0, `steps` steps of
-      // x += omega (omega_jac rho + s f(patch(rhs - A x))); net = false: zero network (damped
-      // Jacobi).  While capturing, every step's input defect and the error of the finest
-      // iterate, -(cap_x + base + x), are stored (the homogeneous error equation with iterate
-      // cap_x, base: the finest solution before these steps, nullptr for zero).
+      // finest-level learned smoothing: x = 0, d = rhs, `steps` steps of
+      //   c = omega (omega_jac rho + s f(patch(d))),  x += alpha c,  d -= alpha A c,
+      // alpha = c.d / c.Ac (energy-optimal); net = false: zero network (Jacobi direction).
+      // While capturing, every step's input defect and the error of the finest iterate,
+      // -(cap_x + base + x), are stored (the homogeneous error equation with iterate cap_x,
+      // base: the finest solution before these steps, nullptr for zero).
       void learned_smooth(Vec& x, const Vec& rhs, int steps, bool net, const Vec* base)
       {
         feat_handle_t h = Runtime::handle();
         Level& L = L0();
         x.format();
+        Vec d = Vec::wrap(L.tmp.elements(), _n);
+        d.copy(rhs);
         const feat_matrix_f64_t& A = L.A.exec();
+        double *ca = _fw.scalar(4), *cb = _fw.scalar(5);
         for(int s = 0; s < steps; ++s)
         {
-          const double* d = rhs.elements();
-          if(s > 0)
-          {
-            check(feat_defect_f64(h, &A, L.tmp.elements(), rhs.elements(), x.elements(), L.mask.get(), nullptr, &_fw.ws),
-                  "learned defect");
-            d = L.tmp.elements();
-          }
           if(_cap_r && _cap_next < _cap_max)
           {
-            check(feat_copy(h, _cap_r + _cap_next * _n, d, _n * sizeof(double)), "capture");
+            check(feat_copy(h, _cap_r + _cap_next * _n, d.elements(), _n * sizeof(double)), "capture");
             Vec e = Vec::wrap(_cap_e + _cap_next * _n, _n);
             e.axpy(x, *_cap_x, 1.0);
             if(base) e.axpy(*base, e, 1.0);
             e.scale(e, -1.0);
             ++_cap_next;
           }
-          check(feat_mlp_smooth_f64(h, _m, net ? _p.get() : _p0.get(), net ? _wbf.get() : _wbf0.get(), d,
-                                    _dinv.elements(), _cfeat.get(), _cfg.omega_learned, _cfg.omega_jac, x.elements()),
-                "learned smoothing");
+          _cv.format();
+          check(feat_mlp_smooth_f64(h, _m, net ? _p.get() : _p0.get(), net ? _wbf.get() : _wbf0.get(), d.elements(),
+                                    _dinv.elements(), _cfeat.get(), _cfg.omega_learned, _cfg.omega_jac, _cv.elements()),
+                "learned direction");
+          check(feat_apply_masked_f64(h, &A, _cq.elements(), _cv.elements(), L.mask.get(), nullptr, _cv.elements(), cb,
+                                      &_fw.ws),
+                "learned A c");
+          check(feat_dot_f64(h, ca, _cv.elements(), d.elements(), _n), "learned c.d");
+          check(feat_mlp_step_f64(h, _n, x.elements(), d.elements(), _cv.elements(), _cq.elements(), ca, cb),
+                "learned step");
         }
       }
 
@@ -271,6 +279,8 @@ namespace feat3amd
         _m = Assembly::q1_nodes_1d(Lf);
         _n = _m * _m * _m;
         _dinv = Vec(_n);
+        _cv = Vec(_n);
+        _cq = Vec(_n);
         L0().A.extract_diag(_dinv);
         _dinv.component_invert(_dinv, 1.0);
         _bnd_host = Assembly::q1_boundary(3, Lf);
@@ -460,8 +470,11 @@ namespace feat3amd
         Runtime::synchronize();
       }
 
-      /// energy left by one learned step on snapshot k relative to the snapshot's energy,
-      /// ep^T A ep / e^T A e; with grad: the training target for its gradient
+      /// energy left by one learned step (energy-optimal length) on snapshot k relative to the
+      /// snapshot's energy: with c the step direction and r = A e,
+      ///   ep = e - alpha c, alpha = c.r / c.Ac,  ep^T A ep / e^T A e = 1 - (c.r)^2 / (c.Ac e^T A e);
+      /// with grad: the training target of its gradient (alpha is optimal, so the gradient at
+      /// fixed alpha is the full one: -2 alpha s (r - alpha A c) / e^T A e per output)
       double step_energy(Index k, bool grad)
       {
         feat_handle_t h = Runtime::handle();
@@ -469,18 +482,29 @@ namespace feat3amd
         const feat_matrix_f64_t& A = L.A.exec();
         const Index n = _n;
         const std::uint16_t* X = _X.get() + k * n * 64;
-        const double *S = _S.get() + k * n, *Rho = _Rho.get() + k * n, *E = _E.get() + k * n;
+        const double *S = _S.get() + k * n, *Rho = _Rho.get() + k * n, *R = _R.get() + k * n;
         const double e0 = _enorm[k] > 0.0 ? _enorm[k] : 1.0;
         check(feat_mlp_infer(h, _p.get(), _wbf.get(), X, n, _y.get()), "infer");
-        check(feat_mlp_energy(h, n, 0, _cfg.omega_jac, _y.get(), S, Rho, E, nullptr, _ep.get(), nullptr), "energy 0");
+        // ep = -c (e = 0), q = A ep = -A c, b = c.Ac, a' = ep.r = -c.r
+        check(feat_mlp_energy(h, n, 0, _cfg.omega_jac, _y.get(), S, Rho, nullptr, nullptr, _ep.get(), nullptr), "energy 0");
         check(feat_apply_masked_f64(h, &A, _q.get(), _ep.get(), L.mask.get(), nullptr, _ep.get(), _fw.scalar(1), &_fw.ws),
-              "A ep");
+              "A c");
+        check(feat_dot_f64(h, _fw.scalar(2), _ep.get(), R, n), "c.r");
+        double b = 0.0, am = 0.0;
+        check(feat_read_scalar_f64(h, _fw.scalar(1), &b), "c.Ac");
+        check(feat_read_scalar_f64(h, _fw.scalar(2), &am), "c.r");
+        const double a = -am;
+        const double alpha = (b > 0.0 && std::isfinite(a / b)) ? a / b : 0.0;
         if(grad)
-          check(feat_mlp_energy(h, n, 1, 2.0 * double(n) / e0, _y.get(), S, nullptr, nullptr, _q.get(), nullptr, _t.get()),
+        {
+          // r_new = r - alpha A c = r + alpha q, into ep
+          Vec rn = Vec::wrap(_ep.get(), n), q = Vec::wrap(_q.get(), n), r = Vec::wrap(const_cast<double*>(R), n);
+          rn.axpy(q, r, alpha);
+          check(feat_mlp_energy(h, n, 1, 2.0 * double(n) * alpha / e0, _y.get(), S, nullptr, nullptr, _ep.get(), nullptr,
+                                _t.get()),
                 "energy 1");
-        double e = 0.0;
-        check(feat_read_scalar_f64(h, _fw.scalar(1), &e), "energy");
-        return e / e0;
+        }
+        return b > 0.0 ? 1.0 - a * alpha / e0 : 1.0;
       }
 
       /// mean over the snapshots of the relative energy one learned step leaves

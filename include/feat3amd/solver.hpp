@@ -488,12 +488,16 @@ namespace feat3amd
       std::shared_ptr<HierarchyType> _h;
       FusedWorkspace _fw;
       bool _fusable = false;
+      bool _autotune = true;
       enum class Coarse { none, direct, jacobi, richardson, other } _coarse_kind = Coarse::other;
 
     public:
       explicit MultiGrid(std::shared_ptr<HierarchyType> h, MultiGridCycle = MultiGridCycle::V) : _h(h) {}
       std::string name() const override { return "MultiGrid-V"; }
       HierarchyType& hierarchy() { return *_h; }
+      /// pick the SELL-64 launch shape of every level's matrices at init_symbolic (default on;
+      /// results are bit-identical for every shape)
+      void set_autotune(bool on) { _autotune = on; }
 
       void init_symbolic() override
       {
@@ -522,10 +526,16 @@ namespace feat3amd
         for(Index i = 0; i < L; ++i)
         {
           (void)_h->levels[i].matrix->exec();
+          if(_autotune) (void)_h->levels[i].matrix->autotune();
           if(i + 1 < L)
           {
             (void)_h->levels[i].transfer->get_mat_prol().exec();
             (void)_h->levels[i].transfer->get_mat_rest().exec();
+            if(_autotune)
+            {
+              (void)_h->levels[i].transfer->get_mat_prol().autotune();
+              (void)_h->levels[i].transfer->get_mat_rest().autotune();
+            }
           }
         }
       }

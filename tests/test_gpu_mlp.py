@@ -149,29 +149,52 @@ def test_training_and_gain_over_jacobi(lmg):
     print(f"jacobi-MG {jac['iters']} iterations, learned-MG {lrn['iters']} iterations "
           f"(fallback {lrn['fell_back']} at {lrn['fallback_iter']}), without fallback {raw['status']} "
           f"{raw['iters']}, rho_jacobi {rho:.3f}")
-    # the plain FP64 defect decides: the learned method converges (falling back to
-    # Jacobi if it must) in no more iterations than Jacobi
+    # the plain FP64 defect decides: the trained smoother reaches 1e-8 in fewer
+    # iterations than the Jacobi multigrid (training is deterministic: seeded data,
+    # index-ordered gradient reductions)
     assert lrn["status"] == "success"
     assert lrn["def_final"] <= 1e-8 * lrn["def_init"]
-    assert lrn["iters"] <= jac["iters"] + 1
+    assert lrn["iters"] < jac["iters"], (lrn["iters"], jac["iters"])
 
 
 def test_untrained_is_jacobi_and_fallback(lmg):
     saved = lmg.params()
-    # zero output layer: the learned smoother is damped Jacobi, same iteration count
+    # zero output layer: the learned smoother steps along the Jacobi direction (with the
+    # energy-optimal length), no fallback, about Jacobi's iteration count
     zero = saved.copy()
     zero[8320:8385] = 0.0
     lmg.params(zero)
     jac = lmg.solve(learned=False)
     r0 = lmg.solve(learned=True)
     assert r0["status"] == "success" and not r0["fell_back"]
-    assert abs(r0["iters"] - jac["iters"]) <= 1
-    # a network that outputs garbage degrades the contraction at once: the solver
-    # continues with Jacobi and still converges, judged by the FP64 defect
+    assert abs(r0["iters"] - jac["iters"]) <= 3, (r0["iters"], jac["iters"])
+    # a network that outputs garbage (a huge constant) makes the smoothing useless: the
+    # solver falls back to Jacobi and still converges, judged by the FP64 defect
     bad = saved.copy()
     bad[8384] = 1e3
     lmg.params(bad)
     r = lmg.solve(learned=True)
     lmg.params(saved)
-    assert r["fell_back"] and r["fallback_iter"] == 1
+    assert r["fell_back"] and r["fallback_iter"] >= 1
     assert r["status"] == "success"
+
+
+def test_energy_optimal_step(feat, handle, torch_gpu):
+    # x += alpha c, d -= alpha q with alpha = a / b; alpha = 0 when b <= 0
+    torch = torch_gpu
+    k, _ = feat.libs()
+    P = feat.ptr
+    rng = np.random.default_rng(5)
+    n = 1000
+    x, d, c, q = (rng.standard_normal(n) for _ in range(4))
+    for a, b in [(0.7, 2.0), (1.0, 0.0), (1.0, -1.0)]:
+        xd, dd, cd, qd = (torch.from_numpy(v.copy()).cuda() for v in (x, d, c, q))
+        ab = torch.tensor([a, b], dtype=torch.float64, device="cuda")
+        feat.check(k.feat_mlp_step_f64(handle.h, n, P(xd), P(dd), P(cd), P(qd), P(ab), P(ab[1:])), "step")
+        torch.cuda.synchronize()
+        alpha = a / b if b > 0 else 0.0
+        # one fma per element: within one rounding of the separately rounded numpy form
+        assert np.allclose(xd.cpu().numpy(), x + alpha * c, rtol=4e-16, atol=1e-15)
+        assert np.allclose(dd.cpu().numpy(), d - alpha * q, rtol=4e-16, atol=1e-15)
+        if alpha == 0.0:
+            assert np.array_equal(xd.cpu().numpy(), x) and np.array_equal(dd.cpu().numpy(), d)

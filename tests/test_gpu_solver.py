@@ -418,3 +418,30 @@ def test_mgpcg_graph_replay_matches_eager(feat, torch_gpu, precision, coarse):
     finally:
         for s in solvers:
             s.close()
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_mgpcg_dictionary_vs_plain_plans(feat, torch_gpu, precision):
+    """Pair-dictionary and plain SELL-64 plans (and whatever launch shapes the autotuner
+    picks for each) give bit-identical iterates and iteration counts."""
+    torch = torch_gpu
+    levels = q1_levels(feat, 3, 5, 1)
+    n = levels[0]["n"]
+    filt = levels[0]["filt"].astype(np.int64)
+    bh = feat.rhs_random(4242, n)
+    bh[filt] = 0.0
+    b = torch.from_numpy(bh).cuda()
+    out = []
+    for flags in (0, feat.SELL_NO_DICT):
+        feat.runtime_sell_flags(flags)
+        try:
+            s = feat.MgPcg(feat.mgpcg_cfg(precision=precision), levels=levels)
+        finally:
+            feat.runtime_sell_flags(0)
+        x = torch.zeros(n, dtype=torch.float64, device="cuda")
+        r = s.solve(x, b)
+        s.close()
+        assert r["status"] == "success"
+        out.append((r["iters"], host(x)))
+    assert out[0][0] == out[1][0]
+    assert np.array_equal(out[0][1], out[1][1])
