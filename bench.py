@@ -221,6 +221,49 @@ def batch_leg(feat, torch, dev, args):
     return out
 
 
+def learned_leg(feat, torch, dev, args):
+    """Learned smoother on 3D Q1 anisotropic log-normal diffusion, level 6 (274,625 dofs):
+    flexible PCG(MG-V(4,4)) with the Jacobi smoother vs the trained BF16-MLP smoother,
+    both to a 1e-8 FP64 relative defect; training on the GPU from the multigrid's own
+    residual/error fields (fixed seeds, deterministic)."""
+    t0 = time.perf_counter()
+    s = feat.LearnedMG(level_fine=6, level_coarse=1, lr=1e-3)
+    rho = s.generate()
+    e_jac = s.energy()
+    t1 = time.perf_counter()
+    loss = s.train(300, seed=1)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t1
+    e_lrn = s.energy()
+    setup = time.perf_counter() - t0 - t_train
+
+    def timed(learned):
+        for _ in range(args.warmup):
+            r = s.solve(learned)
+        ts = []
+        for _ in range(args.steps):
+            r = s.solve(learned)
+            ts.append(r["seconds"])
+        return r, float(np.mean(ts))
+
+    rj, tj = timed(False)
+    rl, tl = timed(True)
+    s.close()
+    assert rj["status"] == "success" and rl["status"] == "success", (rj, rl)
+    out = {"workload": "3D Q1 -div(c K grad u) = f, K = diag(1,1,10), log-normal c (sigma 1.5), level 6, "
+                       "274,625 dofs; flexible PCG + MG V(4,4), levels 6..1, tol 1e-8 (FP64 defect)",
+           "jacobi": {"iters": rj["iters"], "s_per_solve": tj, "final_rel_defect": rj["def_final"] / rj["def_init"]},
+           "learned": {"iters": rl["iters"], "s_per_solve": tl, "final_rel_defect": rl["def_final"] / rl["def_init"],
+                       "fell_back": rl["fell_back"], "fallback_iter": rl["fallback_iter"]},
+           "speedup_vs_jacobi": tj / tl, "iteration_ratio": rj["iters"] / max(rl["iters"], 1),
+           "train": {"adam_steps": len(loss), "s": t_train, "setup_and_data_s": setup,
+                     "one_step_rel_energy_jacobi": e_jac, "one_step_rel_energy_learned": e_lrn},
+           "rho_jacobi": rho}
+    log(f"learned smoother L6: Jacobi {rj['iters']} it {tj * 1e3:.2f} ms, learned {rl['iters']} it {tl * 1e3:.2f} ms "
+        f"(fallback {rl['fell_back']}), training {t_train:.2f} s, one-step energy {e_jac:.3f} -> {e_lrn:.3f}")
+    return out
+
+
 def spmv_only(feat, torch, dev, stream, args):
     """SpMV r = A*x on the level-7 matrix in each format, args.spmv_reps launches each,
     then args.spmv_reps device-to-device copies of 1 GiB (the counter calibration)."""
@@ -270,6 +313,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--spmv-reps", type=int, default=50)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="only the FP64 MG-PCG solves (for kernel-trace profiles of the solve)")
     ap.add_argument("--spmv-only", action="store_true",
                     help="only the SpMV loops (pair dictionary, plain SELL-64, CSR) and a 1 GiB device copy, "
                          "for hardware-counter runs")
@@ -325,6 +370,12 @@ def main():
     assert res["status"] == "success", res
     t_mean = float(np.mean(times))
 
+    if args.headline_only:
+        solver.close()
+        print(json.dumps({"metric": "MG-PCG time-to-1e-8 residual (s), 3D Q1 Poisson 2.1M dofs (headline solves only)",
+                          "value": t_mean, "unit": "s", "iters": res["iters"], "steps": args.steps}))
+        return
+
     # ---- the same solver launching every kernel eagerly (no recorded graphs)
     se = feat.MgPcg(feat.mgpcg_cfg(nu_pre=4, nu_post=4, omega_jac=0.5, omega_rich=1.0, coarse="direct",
                                    tol_rel=1e-8, max_iter=100, graphs=False), q1=(DIM, LF, LC))
@@ -361,6 +412,23 @@ def main():
     assert rp_["iters"] == res["iters"] and bool(torch.equal(xp, x)), "plain and dictionary plans differ"
     t_plain = float(np.mean(tp))
     log(f"plain SELL-64 plans: {t_plain * 1e3:.2f} ms per solve (pair dictionary {t_mean * 1e3:.2f} ms), identical result")
+
+    # ---- the same solver with the default launch shapes (no autotuning)
+    su = feat.MgPcg(feat.mgpcg_cfg(nu_pre=4, nu_post=4, omega_jac=0.5, omega_rich=1.0, coarse="direct",
+                                   tol_rel=1e-8, max_iter=100, autotune=False), q1=(DIM, LF, LC))
+    tu = []
+    for k in range(args.warmup + args.steps):
+        xp.zero_()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        ru_ = su.solve(xp, b)
+        torch.cuda.synchronize()
+        if k >= args.warmup:
+            tu.append(time.perf_counter() - t)
+    su.close()
+    assert ru_["iters"] == res["iters"] and bool(torch.equal(xp, x)), "tuned and untuned solves differ"
+    t_untuned = float(np.mean(tu))
+    log(f"default launch shapes: {t_untuned * 1e3:.2f} ms per solve (autotuned {t_mean * 1e3:.2f} ms), identical result")
 
     # ---- the mixed-precision forms: FP32 V-cycle (FP32 or FP16 matrix values)
     # inside the FP64 PCG; outer residual, update, convergence test and coarse solve FP64
@@ -406,6 +474,7 @@ def main():
     # ---- Stokes: Q2/P1dc lid-driven cavity level 6, Richardson(MG-V(Richardson(Schwarz(AmaVanka))))
     stokes = stokes_leg(feat, torch, dev, args)
     batch = batch_leg(feat, torch, dev, args)
+    learned = learned_leg(feat, torch, dev, args)
 
     # ---- SpMV r = A*x on the same matrix, HIP events on the launch stream:
     # the caller's CSR (u64) and the SELL-64 plan the solver runs on
@@ -476,13 +545,14 @@ def main():
             "final_rel_defect": res["def_final"] / res["def_init"],
             "eager_launch_s_per_solve": t_eager,
             "plain_sell_plans_s_per_solve": t_plain,
+            "default_launch_shapes_s_per_solve": t_untuned,
             "matrix_plan": "SELL-64 pair dictionary (1 byte per entry, 42 distinct (col-row, value) pairs)",
             "parallelism": "single GPU",
         },
         "mixed_precision": {"form": "FP32 V-cycle on FP32 / FP16-valued SELL-64 plans (FP64 coarse solve) "
                                     "inside FP64 PCG", "fp32": mixed["f32"], "fp16_values": mixed["f16"]},
         "spmv": {
-            "sell64_dict": {"kernel": "k_sell_dict_rows<double,EpiApply<double>,2>", "us_per_launch": dict_ms * 1e3,
+            "sell64_dict": {"kernel": "k_sell_dict_rows<double,EpiApply<double>,8>", "us_per_launch": dict_ms * 1e3,
                             "bytes_per_launch": dict_bytes, "gbps": dict_gbs,
                             "csr_equivalent_gbps": csr_bytes / (dict_ms * 1e-3) / 1e9, "stored_entries": dict_padded},
             "sell64": {"kernel": "k_sell_rows<double,short,true,EpiApply<double>>", "us_per_launch": sell_ms * 1e3,
@@ -495,6 +565,7 @@ def main():
         },
         "stokes": stokes,
         "batch": batch,
+        "learned_smoother": learned,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS, "traffic": None,
                      "kernel": "pair-dictionary SELL-64 SpMV r = A*x (the solver's matrix format), bytes = its "

@@ -88,15 +88,63 @@ __device__ double block_sum(double v, double* red /* >= kThreads/64 */)
   return t;
 }
 
-// Deterministic grid reduction of up to two values: partial per workgroup,
-// the last workgroup to arrive sums the partials in index order.
+// Deterministic grid reduction of up to two values: partial per workgroup, summed
+// in index order -- by the last workgroup to arrive (small grids), or, with `defer`,
+// by a one-workgroup kernel launched right after (large grids: the workgroups then
+// store their partials and end without any atomic or wait).  Both sum the same
+// way: thread t adds partials t, t + kThreads, ... in order, then a block sum.
 struct GridReduce
 {
   double* partials;   // 2 * kMaxGrid
   uint32_t* counter;
   double* out0;       // may be null
   double* out1;       // may be null
+  int defer;          // partials only; k_grid_finish completes the sum
 };
+
+// the in-order sum of nblocks partials (kThreads threads; result to out0/out1)
+template<bool ATOMIC>
+__device__ void grid_reduce_tail(const GridReduce& g, uint32_t nblocks, double* red)
+{
+  const bool r0 = g.out0 != nullptr, r1 = g.out1 != nullptr;
+  // eight loads are issued before any is added, so a large grid costs few round trips
+  double s0 = 0.0, s1 = 0.0;
+  for(uint32_t i0 = threadIdx.x; i0 < nblocks; i0 += 8 * blockDim.x)
+  {
+    double v0[8], v1[8];
+#pragma unroll
+    for(int u = 0; u < 8; ++u)
+    {
+      const uint32_t i = i0 + uint32_t(u) * blockDim.x;
+      const bool in = i < nblocks;
+      if(ATOMIC)
+      {
+        v0[u] = (r0 && in) ? __hip_atomic_load(&g.partials[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0.0;
+        v1[u] = (r1 && in) ? __hip_atomic_load(&g.partials[kMaxGrid + i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                           : 0.0;
+      }
+      else
+      {
+        v0[u] = (r0 && in) ? g.partials[i] : 0.0;
+        v1[u] = (r1 && in) ? g.partials[kMaxGrid + i] : 0.0;
+      }
+    }
+#pragma unroll
+    for(int u = 0; u < 8; ++u)
+      if(i0 + uint32_t(u) * blockDim.x < nblocks)
+      {
+        s0 += v0[u];
+        s1 += v1[u];
+      }
+  }
+  if(r0) s0 = block_sum(s0, red);
+  if(r1) s1 = block_sum(s1, red);
+  if(threadIdx.x == 0)
+  {
+    if(r0) *g.out0 = s0;
+    if(r1) *g.out1 = s1;
+  }
+}
 
 __device__ void grid_reduce_finish(const GridReduce& g, double v0, double v1)
 {
@@ -106,6 +154,15 @@ __device__ void grid_reduce_finish(const GridReduce& g, double v0, double v1)
   if(!r0 && !r1) return;
   double b0 = r0 ? block_sum(v0, red) : 0.0;
   double b1 = r1 ? block_sum(v1, red) : 0.0;
+  if(g.defer)
+  {
+    if(threadIdx.x == 0)
+    {
+      if(r0) g.partials[blockIdx.x] = b0;
+      if(r1) g.partials[kMaxGrid + blockIdx.x] = b1;
+    }
+    return;
+  }
   if(threadIdx.x == 0)
   {
     // Publish the partials with agent-scope atomic exchanges: they are performed
@@ -123,20 +180,28 @@ __device__ void grid_reduce_finish(const GridReduce& g, double v0, double v1)
   }
   __syncthreads();
   if(!last) return;
-  double s0 = 0.0, s1 = 0.0;
-  for(uint32_t i = threadIdx.x; i < gridDim.x; i += blockDim.x)
-  {
-    if(r0) s0 += __hip_atomic_load(&g.partials[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if(r1) s1 += __hip_atomic_load(&g.partials[kMaxGrid + i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
-  if(r0) s0 = block_sum(s0, red);
-  if(r1) s1 = block_sum(s1, red);
-  if(threadIdx.x == 0)
-  {
-    if(r0) *g.out0 = s0;
-    if(r1) *g.out1 = s1;
-    __hip_atomic_store(g.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
+  grid_reduce_tail<true>(g, gridDim.x, red);
+  if(threadIdx.x == 0) __hip_atomic_store(g.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// completes a deferred grid reduction of nblocks partials (one workgroup of kThreads)
+__global__ __launch_bounds__(kThreads) void k_grid_finish(GridReduce g, uint32_t nblocks)
+{
+  __shared__ double red[kThreads / 64];
+  grid_reduce_tail<false>(g, nblocks, red);
+}
+
+// grids above this many workgroups defer their reduction to k_grid_finish
+constexpr uint64_t kDeferGrid = 128;
+
+__host__ inline GridReduce reduce_for_grid(GridReduce g, uint64_t grid)
+{
+  g.defer = (grid > kDeferGrid && (g.out0 != nullptr || g.out1 != nullptr)) ? 1 : 0;
+  return g;
+}
+__host__ inline void finish_reduce(hipStream_t s, const GridReduce& g, uint64_t grid)
+{
+  if(g.defer) hipLaunchKernelGGL(k_grid_finish, dim3(1), dim3(kThreads), 0, s, g, uint32_t(grid));
 }
 
 // Contiguous range of row-blocks for this workgroup's XCD; the workgroups of
@@ -192,6 +257,7 @@ inline GridReduce handle_reduce(feat_handle_t h, double* out0, double* out1)
   g.counter = h->counter;
   g.out0 = out0;
   g.out1 = out1;
+  g.defer = 0;
   return g;
 }
 
@@ -202,6 +268,7 @@ inline GridReduce ws_reduce(const feat_reduce_ws_t* ws, double* out0, double* ou
   g.counter = ws ? ws->counter : nullptr;
   g.out0 = out0;
   g.out1 = out1;
+  g.defer = 0;
   return g;
 }
 

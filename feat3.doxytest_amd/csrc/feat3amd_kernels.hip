@@ -229,8 +229,10 @@ int launch_rows(hipStream_t s, const DT* val, const IT* col, const IT* rowptr, c
   if(rows == 0) return FEAT_OK;
   const uint64_t nrb = (rows + kRowsPerBlock - 1) / kRowsPerBlock;
   const int grid = int(nrb < uint64_t(kMaxGrid) ? nrb : uint64_t(kMaxGrid));
+  const GridReduce gd = reduce_for_grid(gr, uint64_t(grid));
   hipLaunchKernelGGL((k_csr_rows<DT, IT, Epi>), dim3(grid), dim3(kThreads), 0, s, val, col, rowptr,
-                     x, rows, epi, gr);
+                     x, rows, epi, gd);
+  finish_reduce(s, gd, uint64_t(grid));
   return check_launch(name);
 }
 
@@ -317,7 +319,7 @@ __global__ __launch_bounds__(kThreads) void k_sell_rows(const VT* __restrict__ v
 // bytes.  Row sums run in CSR order with one fma per entry, bit-identical to
 // the CSR loop.
 // ---------------------------------------------------------------------------
-template<typename DT, class Epi, int UNRW>
+template<typename DT, class Epi, int G>
 __global__ __launch_bounds__(kThreads) void k_sell_dict_rows(const DT* __restrict__ tab,
                                                              const uint32_t* __restrict__ codes,
                                                              const uint64_t* __restrict__ soff,
@@ -326,6 +328,7 @@ __global__ __launch_bounds__(kThreads) void k_sell_dict_rows(const DT* __restric
                                                              GridReduce gr)
 {
   static_assert(kThreads == 256, "one table entry per thread");
+  static_assert(G == 4 || G == 8 || G == 16, "gather group");
   __shared__ DT s_val[256];
   __shared__ int s_del[256];
   s_val[threadIdx.x] = tab[threadIdx.x];
@@ -345,37 +348,31 @@ __global__ __launch_bounds__(kThreads) void k_sell_dict_rows(const DT* __restric
       const int len = rlen[row];
       const int nw = (len + 3) >> 2;
       DT sum = DT(0);
-      int w = 0;
-      for(; w + UNRW <= nw; w += UNRW)
+      // up to 8 code words (32 entries) are fetched before any column is gathered;
+      // the gathers then run G entries at a time
+      for(int w0 = 0; w0 < nw; w0 += 8)
       {
-        uint32_t wd[UNRW];
+        uint32_t wd[8];
 #pragma unroll
-        for(int u = 0; u < UNRW; ++u) wd[u] = __builtin_nontemporal_load(cw + (w + u) * 64);
-        DT vv[4 * UNRW], xv[4 * UNRW];
+        for(int u = 0; u < 8; ++u) wd[u] = w0 + u < nw ? __builtin_nontemporal_load(cw + (w0 + u) * 64) : 0u;
 #pragma unroll
-        for(int u = 0; u < UNRW; ++u)
+        for(int g0 = 0; g0 < 32; g0 += G)
+        {
+          const int kb = 4 * w0 + g0;
+          if(kb >= len) break;
+          DT vv[G], xv[G];
 #pragma unroll
-          for(int j = 0; j < 4; ++j)
+          for(int j = 0; j < G; ++j)
           {
-            const uint32_t c = (wd[u] >> (8 * j)) & 255u;
-            const bool in = 4 * (w + u) + j < len;
-            vv[4 * u + j] = s_val[c];
-            xv[4 * u + j] = x[in ? uint64_t(std::int64_t(row) + s_del[c]) : uint64_t(0)];
+            const int e = g0 + j;
+            const uint32_t c = (wd[e >> 2] >> (8 * (e & 3))) & 255u;
+            vv[j] = s_val[c];
+            xv[j] = x[kb + j < len ? uint64_t(std::int64_t(row) + s_del[c]) : uint64_t(0)];
           }
 #pragma unroll
-        for(int k = 0; k < 4 * UNRW; ++k)
-          if(4 * w + k < len) sum = dfma(vv[k], xv[k], sum);
-      }
-      for(; w < nw; ++w)
-      {
-        const uint32_t wd = __builtin_nontemporal_load(cw + w * 64);
-#pragma unroll
-        for(int j = 0; j < 4; ++j)
-          if(4 * w + j < len)
-          {
-            const uint32_t c = (wd >> (8 * j)) & 255u;
-            sum = dfma(s_val[c], x[uint64_t(std::int64_t(row) + s_del[c])], sum);
-          }
+          for(int j = 0; j < G; ++j)
+            if(kb + j < len) sum = dfma(vv[j], xv[j], sum);
+        }
       }
       epi.row(row, sum, acc0, acc1);
     }
@@ -407,6 +404,7 @@ int launch_mat(hipStream_t s, const M* A, const DT* x, const Epi& epi, const Gri
     const int cap_code = reduces ? 0 : (A->tune >> 2) & 7;
     const uint64_t cap = cap_code ? (uint64_t(kMaxGrid) >> cap_code) : uint64_t(kMaxGrid);
     const int grid = int(nch < cap ? nch : cap);
+    const GridReduce gd = reduce_for_grid(gr, uint64_t(grid));
     const int unr_code = A->tune & 3;
     const uint64_t* soff = (const uint64_t*)A->ptr;
     const DT sc = DT(A->scale);
@@ -416,12 +414,13 @@ int launch_mat(hipStream_t s, const M* A, const DT* x, const Epi& epi, const Gri
       const uint16_t* vh = (const uint16_t*)(const void*)A->val;
       if(A->index == FEAT_IDX_U32)
         hipLaunchKernelGGL((k_sell_rows<DT, uint16_t, uint32_t, false, Epi>), dim3(grid), dim3(kThreads), 0, s, vh,
-                           (const uint32_t*)A->col, soff, A->rowlen, x, A->rows, sc, epi, gr);
+                           (const uint32_t*)A->col, soff, A->rowlen, x, A->rows, sc, epi, gd);
       else if(A->index == FEAT_IDX_D16)
         hipLaunchKernelGGL((k_sell_rows<DT, uint16_t, int16_t, true, Epi>), dim3(grid), dim3(kThreads), 0, s, vh,
-                           (const int16_t*)A->col, soff, A->rowlen, x, A->rows, sc, epi, gr);
+                           (const int16_t*)A->col, soff, A->rowlen, x, A->rows, sc, epi, gd);
       else
         return fail(FEAT_ERR_UNSUPPORTED, "SELL index kind");
+      finish_reduce(s, gd, uint64_t(grid));
       return check_launch(name);
     }
     if(A->index == FEAT_IDX_DICT)
@@ -429,40 +428,43 @@ int launch_mat(hipStream_t s, const M* A, const DT* x, const Epi& epi, const Gri
       const DT* tab = reinterpret_cast<const DT*>(A->val);
       const uint32_t* cw = static_cast<const uint32_t*>(A->col);
       if(unr_code == 1)
-        hipLaunchKernelGGL((k_sell_dict_rows<DT, Epi, 1>), dim3(grid), dim3(kThreads), 0, s, tab, cw, soff, A->rowlen, x,
-                           A->rows, epi, gr);
-      else if(unr_code == 3)
         hipLaunchKernelGGL((k_sell_dict_rows<DT, Epi, 4>), dim3(grid), dim3(kThreads), 0, s, tab, cw, soff, A->rowlen, x,
-                           A->rows, epi, gr);
+                           A->rows, epi, gd);
+      else if(unr_code == 3)
+        hipLaunchKernelGGL((k_sell_dict_rows<DT, Epi, 16>), dim3(grid), dim3(kThreads), 0, s, tab, cw, soff, A->rowlen, x,
+                           A->rows, epi, gd);
       else
-        hipLaunchKernelGGL((k_sell_dict_rows<DT, Epi, 2>), dim3(grid), dim3(kThreads), 0, s, tab, cw, soff, A->rowlen, x,
-                           A->rows, epi, gr);
+        hipLaunchKernelGGL((k_sell_dict_rows<DT, Epi, 8>), dim3(grid), dim3(kThreads), 0, s, tab, cw, soff, A->rowlen, x,
+                           A->rows, epi, gd);
+      finish_reduce(s, gd, uint64_t(grid));
       return check_launch(name);
     }
     if(A->index == FEAT_IDX_U32)
     {
       if(std::is_same<DT, double>::value && unr_code == 1)
         hipLaunchKernelGGL((k_sell_rows<DT, DT, uint32_t, false, Epi, 4>), dim3(grid), dim3(kThreads), 0, s, A->val,
-                           (const uint32_t*)A->col, soff, A->rowlen, x, A->rows, sc, epi, gr);
+                           (const uint32_t*)A->col, soff, A->rowlen, x, A->rows, sc, epi, gd);
       else if(std::is_same<DT, double>::value && unr_code == 3)
         hipLaunchKernelGGL((k_sell_rows<DT, DT, uint32_t, false, Epi, 16>), dim3(grid), dim3(kThreads), 0, s, A->val,
-                           (const uint32_t*)A->col, soff, A->rowlen, x, A->rows, sc, epi, gr);
+                           (const uint32_t*)A->col, soff, A->rowlen, x, A->rows, sc, epi, gd);
       else
         hipLaunchKernelGGL((k_sell_rows<DT, DT, uint32_t, false, Epi>), dim3(grid), dim3(kThreads), 0, s, A->val,
-                           (const uint32_t*)A->col, soff, A->rowlen, x, A->rows, sc, epi, gr);
+                           (const uint32_t*)A->col, soff, A->rowlen, x, A->rows, sc, epi, gd);
+      finish_reduce(s, gd, uint64_t(grid));
       return check_launch(name);
     }
     if(A->index == FEAT_IDX_D16)
     {
       if(std::is_same<DT, double>::value && unr_code == 1)
         hipLaunchKernelGGL((k_sell_rows<DT, DT, int16_t, true, Epi, 4>), dim3(grid), dim3(kThreads), 0, s, A->val,
-                           (const int16_t*)A->col, soff, A->rowlen, x, A->rows, sc, epi, gr);
+                           (const int16_t*)A->col, soff, A->rowlen, x, A->rows, sc, epi, gd);
       else if(std::is_same<DT, double>::value && unr_code == 3)
         hipLaunchKernelGGL((k_sell_rows<DT, DT, int16_t, true, Epi, 16>), dim3(grid), dim3(kThreads), 0, s, A->val,
-                           (const int16_t*)A->col, soff, A->rowlen, x, A->rows, sc, epi, gr);
+                           (const int16_t*)A->col, soff, A->rowlen, x, A->rows, sc, epi, gd);
       else
         hipLaunchKernelGGL((k_sell_rows<DT, DT, int16_t, true, Epi>), dim3(grid), dim3(kThreads), 0, s, A->val,
-                           (const int16_t*)A->col, soff, A->rowlen, x, A->rows, sc, epi, gr);
+                           (const int16_t*)A->col, soff, A->rowlen, x, A->rows, sc, epi, gd);
+      finish_reduce(s, gd, uint64_t(grid));
       return check_launch(name);
     }
   }
@@ -1277,8 +1279,10 @@ int reduce_impl(feat_handle_t h, double* res, const double* x, const double* y, 
   if(!h || !res || (n && (!x || !y || (kind == 2 && !z)))) return fail(FEAT_ERR_INVALID, name);
   if(!n)
     return feat_set_memory_f64(h, res, 0.0, 1);
-  hipLaunchKernelGGL(k_reduce, dim3(grid_for(n, 8)), dim3(kThreads), 0, h->stream, x, y, z, n, kind,
-                     handle_reduce(h, res, nullptr));
+  const int grid = grid_for(n, 8);
+  const GridReduce gd = reduce_for_grid(handle_reduce(h, res, nullptr), uint64_t(grid));
+  hipLaunchKernelGGL(k_reduce, dim3(grid), dim3(kThreads), 0, h->stream, x, y, z, n, kind, gd);
+  finish_reduce(h->stream, gd, uint64_t(grid));
   if(take_sqrt)
     hipLaunchKernelGGL(k_sqrt_inplace, dim3(1), dim3(1), 0, h->stream, res);
   return check_launch(name);
@@ -1753,9 +1757,11 @@ int smooth_impl(feat_handle_t h, const M* A, const DT* inv, const uint32_t* mask
   GridReduce none{nullptr, nullptr, nullptr, nullptr};
   const bool acc_first = (acc != nullptr && steps == 1);
   {
-    GridReduce g = acc_first ? ws_reduce(ws, nullptr, dot) : none;
-    hipLaunchKernelGGL((k_smooth_first<DT>), dim3(grid_for(n, 4)), dim3(kThreads), 0, h->stream, B(1), inv, rhs, mask,
+    const int grid = grid_for(n, 4);
+    GridReduce g = reduce_for_grid(acc_first ? ws_reduce(ws, nullptr, dot) : none, uint64_t(grid));
+    hipLaunchKernelGGL((k_smooth_first<DT>), dim3(grid), dim3(kThreads), 0, h->stream, B(1), inv, rhs, mask,
                        omega, rhs_nrm2, acc_first ? acc : (DT*)nullptr, acc_first ? dotw : (const DT*)nullptr, n, g);
+    finish_reduce(h->stream, g, uint64_t(grid));
     int rc = check_launch(name);
     if(rc) return rc;
   }
@@ -1940,8 +1946,10 @@ int feat_pcg_update_f64(feat_handle_t h, double* x, double* r, const double* p, 
 {
   if(!h || !gamma || !qp || !rnrm2 || (n && (!x || !r || !p || !q))) return fail(FEAT_ERR_INVALID, "feat_pcg_update_f64: null argument");
   if(need_ws(ws, rnrm2)) return fail(FEAT_ERR_INVALID, "reduction workspace missing");
-  hipLaunchKernelGGL(k_pcg_update, dim3(grid_for(n, 4)), dim3(kThreads), 0, h->stream, x, r, p, q, gamma, qp, n,
-                     ws_reduce(ws, rnrm2, nullptr));
+  const int grid = grid_for(n, 4);
+  const GridReduce gd = reduce_for_grid(ws_reduce(ws, rnrm2, nullptr), uint64_t(grid));
+  hipLaunchKernelGGL(k_pcg_update, dim3(grid), dim3(kThreads), 0, h->stream, x, r, p, q, gamma, qp, n, gd);
+  finish_reduce(h->stream, gd, uint64_t(grid));
   return check_launch("feat_pcg_update_f64");
 }
 
@@ -2000,8 +2008,10 @@ int feat_convert_dot_f32_f64(feat_handle_t h, double* z, const float* z32, const
 {
   if(!h || (n && (!z || !z32)) || (dot && !w)) return fail(FEAT_ERR_INVALID, "feat_convert_dot_f32_f64: null argument");
   if(need_ws(ws, dot)) return fail(FEAT_ERR_INVALID, "reduction workspace missing");
-  hipLaunchKernelGGL(k_convert_dot, dim3(grid_for(n, 4)), dim3(kThreads), 0, h->stream, z, z32, dot ? w : nullptr, n,
-                     ws_reduce(ws, dot, nullptr));
+  const int grid = grid_for(n, 4);
+  const GridReduce gd = reduce_for_grid(ws_reduce(ws, dot, nullptr), uint64_t(grid));
+  hipLaunchKernelGGL(k_convert_dot, dim3(grid), dim3(kThreads), 0, h->stream, z, z32, dot ? w : nullptr, n, gd);
+  finish_reduce(h->stream, gd, uint64_t(grid));
   return check_launch("feat_convert_dot_f32_f64");
 }
 

@@ -443,8 +443,10 @@ int feat_tuple_norm2_f64(feat_handle_t h, const double* v, uint64_t nu, uint64_t
   if(!h || !out || (nu + np && !v)) return fail(FEAT_ERR_INVALID, "feat_tuple_norm2_f64: null argument");
   if(need_ws(ws, out)) return fail(FEAT_ERR_INVALID, "feat_tuple_norm2_f64: workspace required");
   double* sums = h->scal + 8;   // two scratch scalars of the handle
-  hipLaunchKernelGGL(k_sumsq2, dim3(grid_for(nu + np, 8)), dim3(kThreads), 0, h->stream, v, nu, np,
-                     ws_reduce(ws, sums, sums + 1));
+  const int grid = grid_for(nu + np, 8);
+  const GridReduce gd = reduce_for_grid(ws_reduce(ws, sums, sums + 1), uint64_t(grid));
+  hipLaunchKernelGGL(k_sumsq2, dim3(grid), dim3(kThreads), 0, h->stream, v, nu, np, gd);
+  finish_reduce(h->stream, gd, uint64_t(grid));
   hipLaunchKernelGGL(k_tuple_norm_finish, dim3(1), dim3(1), 0, h->stream, sums, out);
   return check_launch("feat_tuple_norm2_f64");
 }

@@ -64,6 +64,7 @@ typedef struct {
                              (ConvertPrecond, solver/convert_precond.hpp:100-119);
                              2: the FP32 cycle on FP16-valued matrices */
   int no_graphs;          /* 1: launch every kernel eagerly instead of replaying HIP graphs */
+  int no_autotune;        /* 1: default SELL-64 launch shapes (no autotuning at init) */
 } feat_mgpcg_cfg_t;
 
 typedef struct {          /* one level, host arrays; P maps the next coarser level onto this one */

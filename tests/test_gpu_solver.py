@@ -198,7 +198,7 @@ def stencil_csr(rng, rows, cols, offsets, values, p_keep):
     return Csr(rows, cols, val, np.asarray(col, np.uint64), np.asarray(rp, np.uint64))
 
 
-@pytest.mark.parametrize("case", ["q1", "stencil", "partial", "many_pairs", "many_offsets", "no_dict"])
+@pytest.mark.parametrize("case", ["q1", "stencil", "partial", "wide", "many_pairs", "many_offsets", "no_dict"])
 def test_sell_dict_plan_bitwise(feat, handle, torch_gpu, oracle, case):
     """Pair-dictionary SELL plans (one byte per entry) reproduce the CSR loop bit for
     bit in FP64 and FP32; matrices with more than 256 distinct (col - row, value)
@@ -216,6 +216,9 @@ def test_sell_dict_plan_bitwise(feat, handle, torch_gpu, oracle, case):
         A = stencil_csr(rng, 5000, 5000, offs, vals, 0.8)
     elif case == "partial":
         A = stencil_csr(rng, 129, 140, offs[:9], vals, 0.5)      # last slice holds one row, empty rows
+    elif case == "wide":
+        # offsets spread over +-1000
+        A = stencil_csr(rng, 3000, 3000, [-1000 + 100 * i for i in range(21)], vals[:3], 0.7)
     elif case == "many_pairs":
         A = stencil_csr(rng, 4000, 4000, list(range(-20, 20)), [float(v) for v in range(1, 9)], 1.0)  # 320 pairs
         want = feat.IDX_D16
