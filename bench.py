@@ -29,6 +29,13 @@ sys.path.insert(0, ROOT)
 DIM, LF, LC = 3, 7, 1
 SEED = 28054777172512
 PEAK_HBM_GBS = 8000.0
+# HBM bytes per launch of the level-7 SpMV r = A*x kernels, from two hardware-counter
+# runs of `bench.py --spmv-only` (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch); FETCH_SIZE
+# is scaled by 2.0 and WRITE_SIZE by 1.0, the factors that make the same counters read
+# 1 GiB for a 1 GiB device-to-device copy (profiles/spmv_traffic_3d_q1_l7.md)
+SPMV_TRAFFIC_BYTES = {"sell64_dict": 40574.2 * 2048 + 16771.0 * 1024,
+                      "sell64": 295808.6 * 2048 + 16771.0 * 1024,
+                      "csr_u64": 481390.4 * 2048 + 16771.0 * 1024}
 
 
 def load_feat():
@@ -567,9 +574,13 @@ def main():
         "batch": batch,
         "learned_smoother": learned,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": achieved / PEAK_HBM_GBS, "traffic": None,
+                     "frac": achieved / PEAK_HBM_GBS, "traffic": SPMV_TRAFFIC_BYTES["sell64_dict"],
+                     "algorithmic_bytes": dict_bytes,
                      "kernel": "pair-dictionary SELL-64 SpMV r = A*x (the solver's matrix format), bytes = its "
-                               "arrays (1 B/entry codes, row lengths, slice offsets, pair table) + x + r"},
+                               "arrays (1 B/entry codes, row lengths, slice offsets, pair table) + x + r",
+                     "csr_equivalent_gbps": csr_bytes / (dict_ms * 1e-3) / 1e9,
+                     "plain_sell64": {"achieved": sell_gbs, "frac": sell_gbs / PEAK_HBM_GBS,
+                                      "traffic": SPMV_TRAFFIC_BYTES["sell64"], "algorithmic_bytes": sell_bytes}},
     }
     if not args.no_cpu_baseline:
         log("cpu baseline: reference MG-PCG on one host core ...")

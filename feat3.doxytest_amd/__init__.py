@@ -50,7 +50,7 @@ class Matrix(C.Structure):
     """feat_matrix_f64_t / feat_matrix_f32_t (same layout)."""
     _fields_ = [("format", _i), ("index", _i), ("rows", _u64), ("cols", _u64), ("nnz", _u64),
                 ("val", _vp), ("col", _vp), ("ptr", _vp), ("rowlen", _vp), ("value_kind", _i),
-                ("scale", _f), ("tune", _i)]
+                ("scale", _f), ("tune", _i), ("slice_words", C.c_uint32)]
 
 
 FMT_CSR, FMT_SELL = 0, 1

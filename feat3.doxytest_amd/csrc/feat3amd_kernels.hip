@@ -322,7 +322,7 @@ __global__ __launch_bounds__(kThreads) void k_sell_rows(const VT* __restrict__ v
 template<typename DT, class Epi, int G>
 __global__ __launch_bounds__(kThreads) void k_sell_dict_rows(const DT* __restrict__ tab,
                                                              const uint32_t* __restrict__ codes,
-                                                             const uint64_t* __restrict__ soff,
+                                                             const uint64_t* __restrict__ soff, uint32_t sw,
                                                              const uint16_t* __restrict__ rlen,
                                                              const DT* __restrict__ x, uint64_t rows, Epi epi,
                                                              GridReduce gr)
@@ -344,7 +344,8 @@ __global__ __launch_bounds__(kThreads) void k_sell_dict_rows(const DT* __restric
     const uint64_t row = ch * kThreads + threadIdx.x;
     if(row < rows)
     {
-      const uint32_t* cw = codes + soff[row >> 6] + lane;
+      // uniform slices: the code address needs no offset load
+      const uint32_t* cw = codes + (sw ? (row >> 6) * 64 * uint64_t(sw) : soff[row >> 6]) + lane;
       const int len = rlen[row];
       const int nw = (len + 3) >> 2;
       DT sum = DT(0);
@@ -428,14 +429,14 @@ int launch_mat(hipStream_t s, const M* A, const DT* x, const Epi& epi, const Gri
       const DT* tab = reinterpret_cast<const DT*>(A->val);
       const uint32_t* cw = static_cast<const uint32_t*>(A->col);
       if(unr_code == 1)
-        hipLaunchKernelGGL((k_sell_dict_rows<DT, Epi, 4>), dim3(grid), dim3(kThreads), 0, s, tab, cw, soff, A->rowlen, x,
-                           A->rows, epi, gd);
+        hipLaunchKernelGGL((k_sell_dict_rows<DT, Epi, 4>), dim3(grid), dim3(kThreads), 0, s, tab, cw, soff, A->slice_words,
+                           A->rowlen, x, A->rows, epi, gd);
       else if(unr_code == 3)
-        hipLaunchKernelGGL((k_sell_dict_rows<DT, Epi, 16>), dim3(grid), dim3(kThreads), 0, s, tab, cw, soff, A->rowlen, x,
-                           A->rows, epi, gd);
+        hipLaunchKernelGGL((k_sell_dict_rows<DT, Epi, 16>), dim3(grid), dim3(kThreads), 0, s, tab, cw, soff, A->slice_words,
+                           A->rowlen, x, A->rows, epi, gd);
       else
-        hipLaunchKernelGGL((k_sell_dict_rows<DT, Epi, 8>), dim3(grid), dim3(kThreads), 0, s, tab, cw, soff, A->rowlen, x,
-                           A->rows, epi, gd);
+        hipLaunchKernelGGL((k_sell_dict_rows<DT, Epi, 8>), dim3(grid), dim3(kThreads), 0, s, tab, cw, soff, A->slice_words,
+                           A->rowlen, x, A->rows, epi, gd);
       finish_reduce(s, gd, uint64_t(grid));
       return check_launch(name);
     }
@@ -1520,9 +1521,19 @@ int dict_create(feat_handle_t h, const feat_csr_f64_t* A, const std::vector<uint
         hmap[2 * kDictSlots + cv * 256u + cd] = uint8_t(np);
         ++np;
       }
-  // word offsets of the slices: ceil(slice max / 4) words per lane
+  // word offsets of the slices: ceil(slice max / 4) words per lane; when padding every
+  // slice to the longest costs at most a quarter more, all slices get that stride
   std::vector<uint64_t> hoff(ns + 1, 0);
-  for(uint64_t i = 0; i < ns; ++i) hoff[i + 1] = hoff[i] + (uint64_t(hmax[i]) + 3) / 4 * 64;
+  uint64_t maxw = 0;
+  for(uint64_t i = 0; i < ns; ++i)
+  {
+    const uint64_t w = (uint64_t(hmax[i]) + 3) / 4;
+    hoff[i + 1] = hoff[i] + w * 64;
+    maxw = w > maxw ? w : maxw;
+  }
+  const bool uniform = ns * 64 * maxw * 4 <= hoff[ns] * 5 && maxw < (uint64_t(1) << 16);
+  if(uniform)
+    for(uint64_t i = 0; i <= ns; ++i) hoff[i] = i * 64 * maxw;
   const uint64_t words = hoff[ns];
   rc = check_hip(hipMalloc(&table, htab.size()), "dict pair table");
   if(!rc) rc = check_hip(hipMalloc(&soff, sizeof(uint64_t) * (ns + 1)), "sell soff");
@@ -1543,6 +1554,7 @@ int dict_create(feat_handle_t h, const feat_csr_f64_t* A, const std::vector<uint
   if(rc) { cleanup(); return rc; }
   out->format = FEAT_FMT_SELL;
   out->index = FEAT_IDX_DICT;
+  out->slice_words = uniform ? uint32_t(maxw) : 0u;
   out->rows = n;
   out->cols = A->cols;
   out->nnz = 4 * words;

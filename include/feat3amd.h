@@ -222,6 +222,9 @@ typedef struct {
   int value_kind;            /* FEAT_VAL_NATIVE                               */
   float scale;               /* 1                                             */
   int tune;                  /* SELL launch shape, 0 = default (autotuner)    */
+  uint32_t slice_words;      /* pair-dictionary plans: code words per lane in
+                                every slice (slice s at s * 64 * slice_words),
+                                0: per-slice offsets in ptr                   */
 } feat_matrix_f64_t;
 
 typedef struct {             /* FP32 arithmetic (mixed-precision inner solver) */
@@ -234,6 +237,7 @@ typedef struct {             /* FP32 arithmetic (mixed-precision inner solver) *
   int value_kind;            /* FEAT_VAL_NATIVE or FEAT_VAL_F16 (SELL only)    */
   float scale;               /* row sums are multiplied by this power of two  */
   int tune;                  /* SELL launch shape, 0 = default                */
+  uint32_t slice_words;      /* as feat_matrix_f64_t::slice_words             */
 } feat_matrix_f32_t;
 
 /* A CSR descriptor (no copy) of the caller's arrays. */

@@ -198,7 +198,8 @@ def stencil_csr(rng, rows, cols, offsets, values, p_keep):
     return Csr(rows, cols, val, np.asarray(col, np.uint64), np.asarray(rp, np.uint64))
 
 
-@pytest.mark.parametrize("case", ["q1", "stencil", "partial", "wide", "many_pairs", "many_offsets", "no_dict"])
+@pytest.mark.parametrize("case", ["q1", "stencil", "partial", "wide", "skewed", "many_pairs", "many_offsets",
+                                  "no_dict"])
 def test_sell_dict_plan_bitwise(feat, handle, torch_gpu, oracle, case):
     """Pair-dictionary SELL plans (one byte per entry) reproduce the CSR loop bit for
     bit in FP64 and FP32; matrices with more than 256 distinct (col - row, value)
@@ -219,6 +220,17 @@ def test_sell_dict_plan_bitwise(feat, handle, torch_gpu, oracle, case):
     elif case == "wide":
         # offsets spread over +-1000
         A = stencil_csr(rng, 3000, 3000, [-1000 + 100 * i for i in range(21)], vals[:3], 0.7)
+    elif case == "skewed":
+        # one slice of 40-entry rows, the rest 4: slices keep their own offsets
+        from oracle.pyoracle import Csr
+        n = 2000
+        rp, col = [0], []
+        for r in range(n):
+            offs_r = range(-20, 20) if r < 64 else (-1, 0, 1, 2)
+            col += [r + o for o in offs_r if 0 <= r + o < n]
+            rp.append(len(col))
+        A = Csr(n, n, rng.choice(np.asarray(vals[:2]), size=len(col)), np.asarray(col, np.uint64),
+                np.asarray(rp, np.uint64))
     elif case == "many_pairs":
         A = stencil_csr(rng, 4000, 4000, list(range(-20, 20)), [float(v) for v in range(1, 9)], 1.0)  # 320 pairs
         want = feat.IDX_D16
@@ -233,6 +245,9 @@ def test_sell_dict_plan_bitwise(feat, handle, torch_gpu, oracle, case):
     v, c, r = upload_csr(torch, A)
     plan = feat.SellPlan(handle, v, c, r, A.rows, A.cols, dict=(case != "no_dict"))
     assert plan.m.format == feat.FMT_SELL and plan.m.index == want, (plan.m.index, want)
+    if want == feat.IDX_DICT:
+        # one code-word stride for all slices unless that would pad by more than a quarter
+        assert (plan.m.slice_words == 0) == (case == "skewed"), plan.m.slice_words
     x = rng.standard_normal(A.cols)
     y = rng.standard_normal(A.rows)
     xd, yd = dev(torch, x), dev(torch, y)
