@@ -176,6 +176,7 @@ KERNEL_API = [
     ("feat_mlp_features_full", _i, [_vp, _u64, _vp, _vp, _vp, _vp, _vp, _vp]),
     ("feat_mlp_energy", _i, [_vp, _u64, _i, _d, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     ("feat_mlp_step_f64", _i, [_vp, _u64, _vp, _vp, _vp, _vp, _vp, _vp]),
+    ("feat_mlp_direction_f64", _i, [_vp, _u64, _vp, _vp, _vp, _vp, _vp, _d, _d, _vp, _vp, C.POINTER(ReduceWs)]),
 ]
 
 HOST_API = [
@@ -211,6 +212,7 @@ HOST_API = [
     ("feat_learned_solve", _i, [_vp, _i, _i, _vp, _vp, _vp, _u64]),
     ("feat_learned_size", _i, [_vp, C.POINTER(_u64), C.POINTER(_u64)]),
     ("feat_learned_energy", _i, [_vp, C.POINTER(_d)]),
+    ("feat_learned_fit", _i, [_vp, _i, _i, _i, _u64, C.POINTER(_i), C.POINTER(_u64), C.POINTER(_u64), _vp]),
     ("feat_learned_params", _i, [_vp, _vp, _vp]),
     ("feat_learned_param_ptrs", _i, [_vp, C.POINTER(_vp), C.POINTER(_vp)]),
     ("feat_learned_destroy", _i, [_vp]),
@@ -632,7 +634,7 @@ class BatchSolver:
 class LearnedCfg(C.Structure):
     _fields_ = [("level_fine", _i), ("level_coarse", _i), ("sigma", _d), ("kz", _d), ("seed", _u64), ("nu", _i),
                 ("omega_jac", _d), ("omega_learned", _d), ("tol_rel", _d), ("max_iter", _u64), ("n_init", _i),
-                ("n_snap", _i), ("batch", _u64), ("lr", _f)]
+                ("n_snap", _i), ("batch", _u64), ("lr", _f), ("fcg_m", _i)]
 
 
 class LearnedResult(C.Structure):
@@ -648,12 +650,12 @@ class LearnedMG:
     a Jacobi or a learned (BF16 MLP) finest-level smoother (C++ host layer)."""
 
     def __init__(self, level_fine=6, level_coarse=1, sigma=1.5, kz=10.0, seed=28054777172512, nu=4, omega_jac=0.5,
-                 omega_learned=1.0, tol_rel=1e-8, max_iter=200, n_init=4, n_snap=3, batch=8192, lr=1e-3):
+                 omega_learned=1.0, tol_rel=1e-8, max_iter=200, n_init=4, n_snap=2, batch=8192, lr=1e-3, fcg_m=4):
         _, self.hl = libs()
         runtime_init()
         self.s = C.c_void_p()
         self.cfg = LearnedCfg(level_fine, level_coarse, sigma, kz, seed, nu, omega_jac, omega_learned, tol_rel, max_iter,
-                              n_init, n_snap, batch, lr)
+                              n_init, n_snap, batch, lr, fcg_m)
         check(self.hl.feat_learned_create(C.byref(self.s), C.byref(self.cfg)), "feat_learned_create")
         n, m = _u64(), _u64()
         check(self.hl.feat_learned_size(self.s, C.byref(n), C.byref(m)), "feat_learned_size")
@@ -668,6 +670,14 @@ class LearnedMG:
         e = C.c_double()
         check(self.hl.feat_learned_energy(self.s, C.byref(e)), "feat_learned_energy")
         return e.value
+
+    def fit(self, rounds=4, steps=300, on_policy=True, seed=1):
+        """Training with model selection on a second right-hand side (see feat_learned_fit)."""
+        br, ji, bi = C.c_int(), _u64(), _u64()
+        it = np.zeros(rounds, np.uint64)
+        check(self.hl.feat_learned_fit(self.s, rounds, steps, int(on_policy), seed, C.byref(br), C.byref(ji), C.byref(bi),
+                                       it.ctypes.data), "feat_learned_fit")
+        return dict(best_round=br.value, jacobi_iters=ji.value, best_iters=bi.value, round_iters=[int(v) for v in it])
 
     def train(self, steps, seed=1):
         loss = np.zeros(max(steps, 1), np.float32)

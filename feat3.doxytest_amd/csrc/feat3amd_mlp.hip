@@ -31,6 +31,20 @@ __device__ __forceinline__ uint16_t f2bf(float f)
 }
 __device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float(uint32_t(b) << 16); }
 __device__ __forceinline__ float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
+// GELU of the inference kernels: erf by Abramowitz & Stegun 7.1.26 (|error| <= 1.5e-7,
+// far below the BF16 rounding its result goes through), one reciprocal and one exp
+__device__ __forceinline__ float gelu_fast(float z)
+{
+  const float x = fabsf(z) * 0.70710678118654752f;
+  const float t = __builtin_amdgcn_rcpf(__fmaf_rn(0.3275911f, x, 1.0f));
+  float p = 1.061405429f;
+  p = __fmaf_rn(p, t, -1.453152027f);
+  p = __fmaf_rn(p, t, 1.421413741f);
+  p = __fmaf_rn(p, t, -0.284496736f);
+  p = __fmaf_rn(p, t, 0.254829592f);
+  const float e = 1.0f - p * t * __expf(-x * x);
+  return 0.5f * z * (1.0f + copysignf(e, z));
+}
 __device__ __forceinline__ float gelu_d(float z)
 {
   return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) + z * 0.39894228040143268f * expf(-0.5f * z * z);
@@ -290,113 +304,97 @@ __global__ void k_mlp_bf16(const float* __restrict__ p, uint16_t* __restrict__ w
   else if(i < 2 * kHid * kInP) wbf[i] = f2bf(p[kW2 + (i - kHid * kInP)]);
 }
 
-// Orders this wavefront's LDS accesses (the buffers are private to the wavefront, and
-// a wavefront's LDS operations execute in issue order): no workgroup barrier needed.
-__device__ __forceinline__ void wave_sync_lds()
+// Weights of the inference kernels, staged once per workgroup in LDS: the MFMA A-operand
+// fragments of W1 and W2 (fragment (nt, ks) of lane l = W[nt*16 + l%16][ks*16 + 4(l/16) + i],
+// i = 0..3, one 8-byte read per MFMA) and the biases / output weights.
+struct LdsWeights
 {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-// Register-resident weights of the inference kernels: the MFMA B operands of layers 1
-// and 2 (B1[nt][ks][i] = W1[nt*16 + c][ks*16 + 4q + i], same for W2) and this lane's
-// biases and output weights (unit nt*16 + c).
-struct RegWeights
-{
-  bf16x4 B1[4][4], B2[4][4];
-  float b1r[4], b2r[4], w3r[4];
+  bf16x4 w1[16 * 64], w2[16 * 64];
+  float b1[kHid], b2[kHid], w3[kHid];
   float b3;
 };
 
-__device__ __forceinline__ void load_reg_weights(RegWeights& R, const float* __restrict__ p, const uint16_t* __restrict__ wbf)
+__device__ void load_lds_weights(LdsWeights& W, const float* __restrict__ p, const uint16_t* __restrict__ wbf)
 {
-  const int l = threadIdx.x & 63, c = l & 15, q = l >> 4;
+  for(int f = threadIdx.x; f < 16 * 64; f += blockDim.x)
+  {
+    const int ntks = f >> 6, l = f & 63, nt = ntks >> 2, ks = ntks & 3, c = l & 15, q = l >> 4;
+    W.w1[f] = *reinterpret_cast<const bf16x4*>(wbf + (nt * 16 + c) * kInP + ks * 16 + 4 * q);
+    W.w2[f] = *reinterpret_cast<const bf16x4*>(wbf + kHid * kInP + (nt * 16 + c) * kHid + ks * 16 + 4 * q);
+  }
+  for(int i = threadIdx.x; i < kHid; i += blockDim.x)
+  {
+    W.b1[i] = p[kB1 + i];
+    W.b2[i] = p[kB2 + i];
+    W.w3[i] = p[kW3 + i];
+  }
+  if(threadIdx.x == 0) W.b3 = p[kB3];
+  __syncthreads();
+}
+
+// Forward pass of the wavefront's 16 samples.  X: the transposed-input operand, lane l
+// holding inputs ks*16 + 4(l/16) + i of sample l%16.  Each layer is computed transposed,
+// Z^T = W * X^T (weights as the A operand), so its output arrives in exactly the operand
+// layout the next layer needs: lane l holds units nt*16 + 4(l/16) + i of sample l%16, and
+// no transpose through memory is needed.  Returns y of sample l%16 in all four lanes of
+// that sample (layer 3: the lane's 16 units in order, then over the four lanes).
+__device__ __forceinline__ float forward_t(const LdsWeights& W, const bf16x4 (&X)[4])
+{
+  const int l = threadIdx.x & 63, q = l >> 4;
+  bf16x4 H[4];
+#pragma unroll
   for(int nt = 0; nt < 4; ++nt)
   {
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
     for(int ks = 0; ks < 4; ++ks)
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(W.w1[(nt * 4 + ks) * 64 + l], X[ks], acc, 0, 0, 0);
 #pragma unroll
-      for(int i = 0; i < 4; ++i)
-      {
-        R.B1[nt][ks][i] = short(wbf[(nt * 16 + c) * kInP + ks * 16 + 4 * q + i]);
-        R.B2[nt][ks][i] = short(wbf[kHid * kInP + (nt * 16 + c) * kHid + ks * 16 + 4 * q + i]);
-      }
-    R.b1r[nt] = p[kB1 + nt * 16 + c];
-    R.b2r[nt] = p[kB2 + nt * 16 + c];
-    R.w3r[nt] = p[kW3 + nt * 16 + c];
+    for(int i = 0; i < 4; ++i) H[nt][i] = short(f2bf(gelu_fast(acc[i] + W.b1[nt * 16 + 4 * q + i])));
   }
-  R.b3 = p[kB3];
-}
-
-// Forward pass of the wavefront's 16 samples from their layer-1 A operands (lane l holds
-// inputs ks*16 + 4(l/16) + i of sample l%16); h1: the wavefront's 16 x 64 BF16 LDS
-// buffer.  Returns, in lane l with l%16 < 4, the output y of sample 4(l/16) + l%16
-// (layer 3 summed over the units in a fixed order: 4 per lane, then a butterfly over
-// the 16 lanes).
-__device__ __forceinline__ float forward_reg(const RegWeights& R, const bf16x4 (&A1)[4], uint16_t* h1)
-{
-  const int l = threadIdx.x & 63, c = l & 15, q = l >> 4;
+  float part = 0.0f;
 #pragma unroll
   for(int nt = 0; nt < 4; ++nt)
   {
     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for(int ks = 0; ks < 4; ++ks) acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A1[ks], R.B1[nt][ks], acc, 0, 0, 0);
+    for(int ks = 0; ks < 4; ++ks)
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(W.w2[(nt * 4 + ks) * 64 + l], H[ks], acc, 0, 0, 0);
 #pragma unroll
-    for(int i = 0; i < 4; ++i) h1[(4 * q + i) * kHid + nt * 16 + c] = f2bf(gelu(acc[i] + R.b1r[nt]));
+    for(int i = 0; i < 4; ++i)
+    {
+      const int u = nt * 16 + 4 * q + i;
+      part = __fmaf_rn(W.w3[u], gelu_fast(acc[i] + W.b2[u]), part);
+    }
   }
-  wave_sync_lds();
-  bf16x4 A2[4];
-#pragma unroll
-  for(int ks = 0; ks < 4; ++ks)
-#pragma unroll
-    for(int i = 0; i < 4; ++i) A2[ks][i] = short(h1[c * kHid + ks * 16 + 4 * q + i]);
-  float part[4] = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-  for(int nt = 0; nt < 4; ++nt)
-  {
-    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for(int ks = 0; ks < 4; ++ks) acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A2[ks], R.B2[nt][ks], acc, 0, 0, 0);
-#pragma unroll
-    for(int i = 0; i < 4; ++i) part[i] = __fmaf_rn(R.w3r[nt], gelu(acc[i] + R.b2r[nt]), part[i]);
-  }
-  // the 16 lanes with the same q hold the same 4 samples (4q + i) and different units
-#pragma unroll
-  for(int i = 0; i < 4; ++i)
-    for(int o = 1; o < 16; o <<= 1) part[i] += __shfl_xor(part[i], o, 64);
-  const float yc = (c & 3) == 0 ? part[0] : (c & 3) == 1 ? part[1] : (c & 3) == 2 ? part[2] : part[3];
-  wave_sync_lds();
-  return yc + R.b3;
+  part += __shfl_xor(part, 16, 64);
+  part += __shfl_xor(part, 32, 64);
+  return part + W.b3;
 }
 
-
-// Plain inference on given inputs (training, tests, evaluation): register-resident
-// weights, each lane loads its 4 x 4 inputs of sample l%16 straight into the A operand.
+// Plain inference on given inputs (training, tests, evaluation): each lane loads its
+// 4 x 4 inputs of sample l%16 straight into the operand (8-byte loads of the 128-byte rows).
 __global__ __launch_bounds__(64 * kWaves) void k_mlp_infer(const float* __restrict__ p, const uint16_t* __restrict__ wbf,
                                                            const uint16_t* __restrict__ X, uint64_t nsamp,
                                                            float* __restrict__ y)
 {
-  __shared__ uint16_t h1s[kWaves][16 * kHid];
+  __shared__ LdsWeights W;
+  load_lds_weights(W, p, wbf);
   const int wave = threadIdx.x >> 6, l = threadIdx.x & 63, c = l & 15, q = l >> 4;
-  RegWeights RW;
-  load_reg_weights(RW, p, wbf);
   const uint64_t ngroups = (nsamp + 16 * kWaves - 1) / (16 * kWaves);
   for(uint64_t g = blockIdx.x; g < ngroups; g += gridDim.x)
   {
-    const uint64_t s0 = (g * kWaves + uint64_t(wave)) * 16;
-    const uint64_t smp = s0 + uint64_t(c);
-    bf16x4 A1[4];
+    asm volatile("" ::: "memory");
+    const uint64_t smp = (g * kWaves + uint64_t(wave)) * 16 + uint64_t(c);
+    bf16x4 Xo[4];
 #pragma unroll
     for(int ks = 0; ks < 4; ++ks)
     {
-      // X rows are 128-byte aligned (64 BF16): 4 consecutive inputs are one 8-byte load
       const bf16x4 v = *reinterpret_cast<const bf16x4*>(X + (smp < nsamp ? smp : 0) * kInP + ks * 16 + 4 * q);
-      A1[ks] = smp < nsamp ? v : bf16x4{0, 0, 0, 0};
+      Xo[ks] = smp < nsamp ? v : bf16x4{0, 0, 0, 0};
     }
-    const float yv = forward_reg(RW, A1, h1s[wave]);
-    const uint64_t so = s0 + uint64_t(4 * q + (c & 3));
-    if(c < 4 && so < nsamp) y[so] = yv;
+    const float yv = forward_t(W, Xo);
+    if(q == 0 && smp < nsamp) y[smp] = yv;
   }
 }
 
@@ -462,39 +460,45 @@ __global__ void k_mlp_features(Grid3 G, const double* __restrict__ R, const doub
 }
 
 
-// The learned smoothing step on the interior nodes:
-//   x_i += omega * (jacobi * r_i / a_ii + s_i * f(patch_i)).
-// Inference layout: the weights stay in registers as MFMA B operands for the whole
-// kernel; each wavefront takes 16 nodes, lane l builds the 16 inputs of node l%16 that
-// it feeds to the A operand (quarter l/16 of the 64), the four lanes of a node combine
-// their maxima for s; only h1 passes through LDS (D layout -> A layout).
+// The learned smoothing direction on the interior nodes,
+//   c_i = omega * (jacobi * r_i / a_ii + s_i * f(patch_i)),
+// written to out (DIR: c, 0 on boundary nodes and where s_i = 0, and the dot c . r is
+// reduced) or added to it (x += c).  Each wavefront takes 16 nodes; lane l builds 16 of
+// the 64 inputs of node l%16 (quarter l/16) in the operand layout, the four lanes of a
+// node combine their maxima for s, and the lanes of quarter 0 write the result.
+template<bool DIR>
 __global__ __launch_bounds__(64 * kWaves) void k_mlp_smooth(Grid3 G, const float* __restrict__ p,
                                                             const uint16_t* __restrict__ wbf,
                                                             const double* __restrict__ r, const double* __restrict__ dinv,
                                                             const float* __restrict__ cfeat, double omega, double jacobi,
-                                                            double* __restrict__ x)
+                                                            double* __restrict__ out, GridReduce gr)
 {
-  __shared__ uint16_t h1s[kWaves][16 * kHid];
-  __shared__ double scs[kWaves][16];
+  __shared__ LdsWeights W;
+  load_lds_weights(W, p, wbf);
   const int wave = threadIdx.x >> 6, l = threadIdx.x & 63, c = l & 15, q = l >> 4;
-  RegWeights RW;
-  load_reg_weights(RW, p, wbf);
   const uint64_t m = G.m, mm = m * m;
   const uint64_t ngroups = (G.n + 16 * kWaves - 1) / (16 * kWaves);
+  double dot = 0.0;
   for(uint64_t g = blockIdx.x; g < ngroups; g += gridDim.x)
   {
-    const uint64_t s0 = (g * kWaves + uint64_t(wave)) * 16;
-    // ---- inputs of node s0 + c: this lane's 16 of them (k = ks*16 + 4q + i)
-    const uint64_t node = s0 + uint64_t(c);
+    // keeps the weight fragments in LDS (read per use) instead of 64+ registers
+    asm volatile("" ::: "memory");
+    const uint64_t node = (g * kWaves + uint64_t(wave)) * 16 + uint64_t(c);
+    const bool valid = node < G.n;
+    // node indices fit 32 bits (m^3 < 2^32 is checked at launch)
+    const uint32_t n32 = uint32_t(node), m32 = uint32_t(m), mm32 = uint32_t(mm);
     bool interior = false;
-    if(node < G.n)
+    if(valid)
     {
-      const uint64_t xx = node % m, yy = (node / m) % m, zz = node / mm;
-      interior = xx > 0 && yy > 0 && zz > 0 && xx + 1 < m && yy + 1 < m && zz + 1 < m;
+      const uint32_t xx = n32 % m32, yy = (n32 / m32) % m32, zz = n32 / mm32;
+      interior = xx > 0 && yy > 0 && zz > 0 && xx + 1 < m32 && yy + 1 < m32 && zz + 1 < m32;
     }
     // every load is issued before any is used (no branches around them): patch
     // entries this lane does not need read element 0, which always exists
-    const double di = dinv[interior ? node : 0];
+    const uint32_t nd = valid ? n32 : 0u;
+    const double di = dinv[nd];
+    const double r_c = r[nd];
+    const double o_c = DIR ? 0.0 : out[nd];
     double rv[16];
     float fv[16];
 #pragma unroll
@@ -504,16 +508,11 @@ __global__ __launch_bounds__(64 * kWaves) void k_mlp_smooth(Grid3 G, const float
       {
         const int k = ks * 16 + 4 * q + i, e = ks * 4 + i;
         const int j = k < 27 ? k : (k < kIn ? k - 27 : 13);
-        const int64_t off = int64_t(j % 3 - 1) + int64_t((j / 3) % 3 - 1) * int64_t(m) + int64_t(j / 9 - 1) * int64_t(mm);
-        const uint64_t nb = interior ? uint64_t(int64_t(node) + off) : 0;
-        rv[e] = r[k < 27 ? nb : 0];
-        fv[e] = cfeat[(k >= 27 && k < kIn) ? nb : 0];
+        const int32_t off = int32_t(j % 3 - 1) + int32_t((j / 3) % 3 - 1) * int32_t(m32) + int32_t(j / 9 - 1) * int32_t(mm32);
+        const uint32_t nb = interior ? uint32_t(int32_t(n32) + off) : 0u;
+        rv[e] = r[k < 27 ? nb : 0u];
+        fv[e] = cfeat[(k >= 27 && k < kIn) ? nb : 0u];
       }
-    // this lane's share of the final update (lanes c < 4 write node s0 + 4q + c)
-    const uint64_t nd_out = s0 + uint64_t(4 * q + (c & 3));
-    const bool writer = c < 4 && nd_out < G.n;
-    const uint64_t ndl = writer ? nd_out : 0;
-    const double r_out = r[ndl], d_out = dinv[ndl], x_out = x[ndl];
     double mx = 0.0;
 #pragma unroll
     for(int ks = 0; ks < 4; ++ks)
@@ -521,8 +520,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_mlp_smooth(Grid3 G, const float
       for(int i = 0; i < 4; ++i)
       {
         const int k = ks * 16 + 4 * q + i, e = ks * 4 + i;
-        const bool use_r = interior && k < 27;
-        rv[e] = use_r ? rv[e] * di : 0.0;
+        rv[e] = (interior && k < 27) ? rv[e] * di : 0.0;
         fv[e] = (interior && k >= 27 && k < kIn) ? fv[e] : 0.0f;
         mx = fmax(mx, fabs(rv[e]));
       }
@@ -530,7 +528,8 @@ __global__ __launch_bounds__(64 * kWaves) void k_mlp_smooth(Grid3 G, const float
     mx = fmax(mx, __shfl_xor(mx, 16, 64));
     mx = fmax(mx, __shfl_xor(mx, 32, 64));
     const double s = mx;
-    bf16x4 A1[4];
+    const double inv_s = s > 0.0 ? 1.0 / s : 0.0;
+    bf16x4 Xo[4];
 #pragma unroll
     for(int ks = 0; ks < 4; ++ks)
 #pragma unroll
@@ -538,22 +537,24 @@ __global__ __launch_bounds__(64 * kWaves) void k_mlp_smooth(Grid3 G, const float
       {
         const int k = ks * 16 + 4 * q + i, e = ks * 4 + i;
         float v = 0.0f;
-        if(s > 0.0) v = k < 27 ? float(rv[e] / s) : fv[e];
-        A1[ks][i] = short(f2bf(v));
+        if(s > 0.0) v = k < 27 ? float(rv[e] * inv_s) : fv[e];
+        Xo[ks][i] = short(f2bf(v));
       }
-    if(q == 0) scs[wave][c] = interior ? s : 0.0;
-    const float y = forward_reg(RW, A1, h1s[wave]);
-    if(writer)
+    const float y = forward_t(W, Xo);
+    if(q == 0 && valid)
     {
-      const double sn = scs[wave][4 * q + c];
-      if(sn > 0.0)
+      const bool act = interior && s > 0.0;
+      const double cval = act ? __fma_rn(omega, __fma_rn(jacobi, r_c * di, s * double(y)), 0.0) : 0.0;
+      if(DIR)
       {
-        const double cval = __fma_rn(jacobi, r_out * d_out, sn * double(y));
-        x[nd_out] = __fma_rn(omega, cval, x_out);
+        out[node] = cval;
+        dot = __fma_rn(cval, r_c, dot);
       }
+      else if(act)
+        out[node] = __fma_rn(omega, __fma_rn(jacobi, r_c * di, s * double(y)), o_c);
     }
-    wave_sync_lds();
   }
+  if(DIR) grid_reduce_finish(gr, dot, 0.0);
 }
 
 // inputs of every node of the grid (training on whole fields): X rows, s, rho_i = r_i / a_ii
@@ -669,11 +670,31 @@ int feat_mlp_features(feat_handle_t h, uint64_t m, const double* R, const double
 int feat_mlp_smooth_f64(feat_handle_t h, uint64_t m, const float* params, const uint16_t* wbf, const double* r,
                         const double* dinv, const float* cfeat, double omega, double jacobi, double* x)
 {
-  if(!h || m < 3 || !params || !wbf || !r || !dinv || !cfeat || !x) return fail(FEAT_ERR_INVALID, "feat_mlp_smooth_f64: bad argument");
+  if(!h || m < 3 || m > 1600 || !params || !wbf || !r || !dinv || !cfeat || !x)
+    return fail(FEAT_ERR_INVALID, "feat_mlp_smooth_f64: bad argument");
+  if(x == r) return fail(FEAT_ERR_ALIAS, "feat_mlp_smooth_f64: x and r must differ");
   Grid3 G{m, m * m * m};
-  hipLaunchKernelGGL(k_mlp_smooth, dim3(mlp_grid(G.n)), dim3(64 * kWaves), 0, h->stream, G, params, wbf, r, dinv, cfeat,
-                     omega, jacobi, x);
+  GridReduce none{nullptr, nullptr, nullptr, nullptr, 0};
+  hipLaunchKernelGGL(k_mlp_smooth<false>, dim3(mlp_grid(G.n)), dim3(64 * kWaves), 0, h->stream, G, params, wbf, r, dinv,
+                     cfeat, omega, jacobi, x, none);
   return check_launch("feat_mlp_smooth_f64");
+}
+
+int feat_mlp_direction_f64(feat_handle_t h, uint64_t m, const float* params, const uint16_t* wbf, const double* r,
+                           const double* dinv, const float* cfeat, double omega, double jacobi, double* c, double* dot,
+                           const feat_reduce_ws_t* ws)
+{
+  if(!h || m < 3 || m > 1600 || !params || !wbf || !r || !dinv || !cfeat || !c || !dot)
+    return fail(FEAT_ERR_INVALID, "feat_mlp_direction_f64: bad argument");
+  if(c == r) return fail(FEAT_ERR_ALIAS, "feat_mlp_direction_f64: c and r must differ");
+  if(need_ws(ws, dot)) return fail(FEAT_ERR_INVALID, "feat_mlp_direction_f64: reduction workspace missing");
+  Grid3 G{m, m * m * m};
+  const int grid = mlp_grid(G.n);
+  const GridReduce gd = reduce_for_grid(ws_reduce(ws, dot, nullptr), uint64_t(grid));
+  hipLaunchKernelGGL(k_mlp_smooth<true>, dim3(grid), dim3(64 * kWaves), 0, h->stream, G, params, wbf, r, dinv, cfeat,
+                     omega, jacobi, c, gd);
+  finish_reduce(h->stream, gd, uint64_t(grid));
+  return check_launch("feat_mlp_direction_f64");
 }
 
 int feat_mlp_features_full(feat_handle_t h, uint64_t m, const double* r, const double* dinv, const float* cfeat,

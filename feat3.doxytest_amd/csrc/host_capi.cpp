@@ -659,6 +659,7 @@ int feat_learned_create(feat_learned_t* out, const feat_learned_cfg_t* cfg)
     c.n_snap = cfg->n_snap;
     c.batch = cfg->batch;
     c.lr = cfg->lr;
+    c.fcg_m = cfg->fcg_m > 0 ? cfg->fcg_m : 1;
     auto s = std::make_unique<feat_learned_s>();
     s->mg = std::make_unique<Solver::LearnedMG>(c);
     s->mg->init();
@@ -720,6 +721,20 @@ int feat_learned_size(feat_learned_t s, std::uint64_t* n, std::uint64_t* m)
     if(!s) throw Error("feat_learned_size: null");
     if(n) *n = s->mg->size();
     if(m) *m = s->mg->nodes_1d();
+  });
+}
+
+int feat_learned_fit(feat_learned_t s, int rounds, int steps, int on_policy, uint64_t seed, int* best_round,
+                     uint64_t* jacobi_iters, uint64_t* best_iters, uint64_t* round_iters)
+{
+  return guarded([&]() {
+    if(!s || rounds < 1 || steps < 0) throw Error("feat_learned_fit: bad argument");
+    const auto fi = s->mg->fit(rounds, steps, on_policy != 0, seed);
+    if(best_round) *best_round = fi.best_round;
+    if(jacobi_iters) *jacobi_iters = fi.jacobi_iters;
+    if(best_iters) *best_iters = fi.best_iters;
+    if(round_iters)
+      for(size_t r = 0; r < fi.round_iters.size(); ++r) round_iters[r] = fi.round_iters[r];
   });
 }
 

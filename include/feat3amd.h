@@ -529,6 +529,12 @@ int feat_mlp_features(feat_handle_t h, uint64_t m, const double* R, const double
 int feat_mlp_smooth_f64(feat_handle_t h, uint64_t m, const float* params, const uint16_t* wbf,
                         const double* r, const double* dinv, const float* cfeat, double omega,
                         double jacobi, double* x);
+/* the same step's direction: c_i = omega * (jacobi * r_i / a_ii + s_i * y_i) on every
+ * interior node with s_i > 0, c_i = 0 elsewhere, and *dot = c . r (device scalar,
+ * deterministic reduction through ws) */
+int feat_mlp_direction_f64(feat_handle_t h, uint64_t m, const float* params, const uint16_t* wbf,
+                           const double* r, const double* dinv, const float* cfeat, double omega,
+                           double jacobi, double* c, double* dot, const feat_reduce_ws_t* ws);
 /* inputs of every node (whole-field training): X (m^3 rows), s, rho_i = r_i / a_ii */
 int feat_mlp_features_full(feat_handle_t h, uint64_t m, const double* r, const double* dinv,
                            const float* cfeat, uint16_t* X, double* s, double* rho);

@@ -4,9 +4,9 @@
 // No counterpart in the reference: the method around the network is the
 // reference's geometric multigrid (solver/multigrid.hpp V-cycle, Richardson
 // smoothing, restriction/prolongation, Jacobi coarse solve) used as the
-// preconditioner of a flexible PCG (pcg.hpp:188-288 with the Polak-Ribiere beta,
-// since the learned smoother makes the preconditioner nonlinear) with the plain
-// FP64 defect norm as the stopping criterion.  On the finest level the
+// preconditioner of a truncated flexible CG (the PCG of pcg.hpp:188-288 with each new
+// direction A-orthogonalised against the last fcg_m ones, since the learned smoother makes
+// the preconditioner nonlinear) with the plain FP64 defect norm as the stopping criterion.  On the finest level the
 // smoother is either damped Jacobi (Richardson(JacobiPrecond)) or damped Jacobi plus a
 // learned correction,
 //   x_i += omega * (omega_jac rho_i + s_i f(rho / s_i, log-coefficient patch)),
@@ -48,8 +48,9 @@ namespace feat3amd
       double omega_learned = 1.0; // learned smoother damping (x += omega (omega_jac r/a_ii + s f))
       double tol_rel = 1e-8;
       Index max_iter = 200;
+      int fcg_m = 4;              // flexible CG: previous directions each new one is A-orthogonalised to
       // training data (n_init random starts x n_snap V-cycles x 2 nu smoothing steps) and optimisation
-      int n_init = 4, n_snap = 3;
+      int n_init = 4, n_snap = 2;
       Index batch = 8192;
       float lr = 1e-3f;
     };
@@ -70,7 +71,7 @@ namespace feat3amd
       std::vector<std::unique_ptr<Level>> _lv;  // 0 = finest
       std::vector<double> _coef;
       Index _m = 0, _n = 0;
-      Vec _dinv, _b, _r, _c, _xs;
+      Vec _dinv, _b, _bval, _r, _c, _xs;
       DeviceArray<float> _cfeat;
       FusedWorkspace _fw;
       // network
@@ -133,14 +134,13 @@ namespace feat3amd
             e.scale(e, -1.0);
             ++_cap_next;
           }
-          _cv.format();
-          check(feat_mlp_smooth_f64(h, _m, net ? _p.get() : _p0.get(), net ? _wbf.get() : _wbf0.get(), d.elements(),
-                                    _dinv.elements(), _cfeat.get(), _cfg.omega_learned, _cfg.omega_jac, _cv.elements()),
+          check(feat_mlp_direction_f64(h, _m, net ? _p.get() : _p0.get(), net ? _wbf.get() : _wbf0.get(), d.elements(),
+                                       _dinv.elements(), _cfeat.get(), _cfg.omega_learned, _cfg.omega_jac, _cv.elements(),
+                                       ca, &_fw.ws),
                 "learned direction");
           check(feat_apply_masked_f64(h, &A, _cq.elements(), _cv.elements(), L.mask.get(), nullptr, _cv.elements(), cb,
                                       &_fw.ws),
                 "learned A c");
-          check(feat_dot_f64(h, ca, _cv.elements(), d.elements(), _n), "learned c.d");
           check(feat_mlp_step_f64(h, _n, x.elements(), d.elements(), _cv.elements(), _cq.elements(), ca, cb),
                 "learned step");
         }
@@ -293,6 +293,10 @@ namespace feat3amd
         Assembly::random_fill(_cfg.seed + 2000u, b.data(), _n, -1.0, 1.0);
         for(auto i : Assembly::q1_boundary(3, Lf)) b[i] = 0.0;
         _b = Vec::from_host(b);
+        // a second right-hand side (seed + 2500) for model selection in fit()
+        Assembly::random_fill(_cfg.seed + 2500u, b.data(), _n, -1.0, 1.0);
+        for(auto i : Assembly::q1_boundary(3, Lf)) b[i] = 0.0;
+        _bval = Vec::from_host(b);
         _r = Vec(_n);
         _c = Vec(_n);
         // network: He-style uniform initialisation from a fixed seed
@@ -337,63 +341,73 @@ namespace feat3amd
       }
       void get_params(float* host) const { DeviceArray<float>::wrap(_p.get(), FEAT_MLP_NPARAM).to_host(host); }
 
-      /// Flexible PCG preconditioned by the V-cycle (Polak-Ribiere beta, so a nonlinear
-      /// preconditioner is admissible; with the linear Jacobi V-cycle it is PCG in exact
-      /// arithmetic).  learned = false: the Jacobi multigrid.  The FP64 defect norm decides
+      /// Flexible CG preconditioned by the V-cycle: each new direction is made A-orthogonal
+      /// to the last fcg_m ones (truncated FCG, so a nonlinear preconditioner is admissible;
+      /// with the linear Jacobi V-cycle it is PCG in exact arithmetic).  learned = false: the Jacobi multigrid.  The FP64 defect norm decides
       /// convergence; with allow_fallback the learned smoother is dropped for the rest of the
       /// solve when its mean contraction falls behind rho_jacobi.
-      SolveInfo solve(Vec& x, const Vec& b, bool learned, bool allow_fallback = true)
+      SolveInfo solve(Vec& x, const Vec& b, bool learned, bool allow_fallback = true, Index max_iter = 0)
       {
+        const Index iter_cap = max_iter ? max_iter : _cfg.max_iter;
         SolveInfo info;
         Runtime::synchronize();
         const auto t0 = std::chrono::steady_clock::now();
-        Vec z(_n), p(_n), q(_n), rold(_n);
+        // ring of the last m directions plus the new one
+        const int mfcg = std::max(1, _cfg.fcg_m), slots = mfcg + 1;
+        std::vector<Vec> P, Q;
+        std::vector<double> dpq;
+        for(int j = 0; j < slots; ++j)
+        {
+          P.emplace_back(_n);
+          Q.emplace_back(_n);
+        }
+        dpq.assign(slots, 0.0);
+        Vec z(_n);
         double d = defect_norm(_r, x, b);
         info.def_init = d;
         info.hist.push_back(d);
         bool use_learned = learned;
         const double eps = 2.220446049250313e-16;
         info.status = std::isfinite(d) ? (d <= eps * eps ? Status::success : Status::progress) : Status::aborted;
-        if(info.status == Status::progress)
+        Index k = 0;
+        while(info.status == Status::progress)
         {
           vcycle(z, _r, use_learned ? 1 : 0);
-          p.copy(z);
-          double gamma = _r.dot(z);
-          while(info.status == Status::progress)
+          // p_k = z - sum over the last m directions of (z . A p_j / p_j . A p_j) p_j
+          Vec& pk = P[k % slots];
+          Vec& qk = Q[k % slots];
+          const Index jlo = k >= Index(mfcg) ? k - Index(mfcg) : 0;
+          std::vector<double> beta;
+          for(Index j = jlo; j < k; ++j) beta.push_back(dpq[j % slots] > 0.0 ? z.dot(Q[j % slots]) / dpq[j % slots] : 0.0);
+          pk.copy(z);
+          for(Index j = jlo; j < k; ++j) pk.axpy(P[j % slots], pk, -beta[j - jlo]);
+          L0().A.apply(qk, pk);
+          check(feat_unit_filter_def_f64_u64(Runtime::handle(), qk.elements(), _bnd.get(), _bnd.size()), "filter");
+          dpq[k % slots] = pk.dot(qk);
+          const double alpha = dpq[k % slots] > 0.0 ? pk.dot(_r) / dpq[k % slots] : 0.0;
+          x.axpy(pk, x, alpha);
+          _r.axpy(qk, _r, -alpha);
+          ++k;
+          const double dn = _r.norm2();
+          ++info.iters;
+          info.hist.push_back(dn);
+          // contraction check against the Jacobi multigrid's relative defect at the same
+          // iteration (CG defect norms are not monotone, hence the factor 3); immediate
+          // fallback on growth past the initial defect or a non-finite defect
+          if(use_learned && allow_fallback &&
+             (!std::isfinite(dn) || dn > info.def_init || (info.iters >= 2 && dn > 3.0 * info.def_init * jacobi_rel(info.iters))))
           {
-            L0().A.apply(q, p);
-            check(feat_unit_filter_def_f64_u64(Runtime::handle(), q.elements(), _bnd.get(), _bnd.size()), "filter");
-            const double alpha = gamma / p.dot(q);
-            x.axpy(p, x, alpha);
-            rold.copy(_r);
-            _r.axpy(q, _r, -alpha);
-            const double dn = _r.norm2();
-            ++info.iters;
-            info.hist.push_back(dn);
-            // contraction check against the Jacobi multigrid's relative defect at the same
-            // iteration (CG defect norms are not monotone, hence the factor 3); immediate
-            // fallback on growth past the initial defect or a non-finite defect
-            if(use_learned && allow_fallback &&
-               (!std::isfinite(dn) || dn > info.def_init || (info.iters >= 2 && dn > 3.0 * info.def_init * jacobi_rel(info.iters))))
-            {
-              use_learned = false;
-              info.fell_back = true;
-              info.fallback_iter = info.iters;
-            }
-            d = dn;
-            if(!std::isfinite(d))
-              info.status = Status::aborted;
-            else if(d <= _cfg.tol_rel * info.def_init)
-              info.status = Status::success;
-            else if(info.iters >= _cfg.max_iter)
-              info.status = Status::max_iter;
-            if(info.status != Status::progress) break;
-            vcycle(z, _r, use_learned ? 1 : 0);
-            const double gnew = _r.dot(z);
-            const double beta = (gnew - rold.dot(z)) / gamma;
-            gamma = gnew;
-            p.axpy(p, z, beta);
+            use_learned = false;
+            info.fell_back = true;
+            info.fallback_iter = info.iters;
           }
+          d = dn;
+          if(!std::isfinite(d))
+            info.status = Status::aborted;
+          else if(d <= _cfg.tol_rel * info.def_init)
+            info.status = Status::success;
+          else if(info.iters >= iter_cap)
+            info.status = Status::max_iter;
         }
         Runtime::synchronize();
         info.def_final = d;
@@ -405,7 +419,7 @@ namespace feat3amd
       /// random (white-noise) starts: the input of every finest-level smoothing step (pre and
       /// post) of n_snap V-cycles per start, with the Jacobi smoother (on_policy = false) or
       /// the current learned one (on-policy refresh, new starts).  The first call also
-      /// records the Jacobi multigrid's defect history on the right-hand side (flexible PCG).
+      /// records the Jacobi multigrid's defect history on the right-hand side (flexible CG).
       void generate_data(bool on_policy = false)
       {
         const Index n = _n;
@@ -514,6 +528,46 @@ namespace feat3amd
         double sum = 0.0;
         for(Index k = 0; k < _nsnap; ++k) sum += step_energy(k, false);
         return sum / double(_nsnap);
+      }
+
+      struct FitInfo
+      {
+        int best_round = -1;          // -1: no round beat the Jacobi multigrid
+        Index jacobi_iters = 0, best_iters = 0;
+        std::vector<Index> round_iters;   // validation iterations per round (0: no convergence)
+      };
+
+      /// Training with model selection: `rounds` rounds of `steps` Adam steps (from the
+      /// second round on, after an on-policy data refresh when on_policy), each followed by a
+      /// validation solve (flexible CG, learned smoother, no fallback) on a second
+      /// right-hand side; the parameters of the round with the fewest validation iterations
+      /// are kept when it beats the Jacobi multigrid there, else the initial parameters.
+      FitInfo fit(int rounds, int steps, bool on_policy, std::uint64_t seed)
+      {
+        if(_nsnap == 0) generate_data();
+        FitInfo fi;
+        Vec x(_n, 0.0);
+        const auto jac = solve(x, _bval, false, false);
+        fi.jacobi_iters = fi.best_iters = jac.iters;
+        std::vector<float> best(FEAT_MLP_NPARAM);
+        get_params(best.data());
+        for(int r = 0; r < rounds; ++r)
+        {
+          if(r > 0 && on_policy) generate_data(true);
+          train(steps, seed + std::uint64_t(r));
+          x.format();
+          const auto li = solve(x, _bval, true, false, 2 * jac.iters);
+          const Index it = li.status == Status::success ? li.iters : 0;
+          fi.round_iters.push_back(it);
+          if(it > 0 && it < fi.best_iters)
+          {
+            fi.best_iters = it;
+            fi.best_round = r;
+            get_params(best.data());
+          }
+        }
+        set_params(best.data());
+        return fi;
       }
 
       /// Adam training of the learned correction on whole snapshot fields: one random

@@ -187,6 +187,7 @@ typedef struct {
   int n_init, n_snap;        /* training data: random starts x outer iterations */
   uint64_t batch;            /* training batch */
   float lr;
+  int fcg_m;                 /* flexible CG: directions kept for A-orthogonalisation (>= 1) */
 } feat_learned_cfg_t;
 typedef struct {
   int status;
@@ -204,6 +205,12 @@ int feat_learned_generate(feat_learned_t s, int on_policy, double* rho_jacobi);
 /* `steps` Adam steps; loss (steps floats, may be NULL): relative energy ep^T A ep / e^T A e
  * one learned step leaves on the step's snapshot */
 int feat_learned_train(feat_learned_t s, int steps, uint64_t seed, float* loss);
+/* training with model selection: `rounds` x `steps` Adam steps (on-policy data refresh
+ * before rounds 2.. when on_policy), a validation solve on a second right-hand side after
+ * each round; keeps the best round's parameters if they beat the Jacobi multigrid there.
+ * round_iters (rounds entries, may be NULL): validation iterations, 0 = no convergence */
+int feat_learned_fit(feat_learned_t s, int rounds, int steps, int on_policy, uint64_t seed,
+                     int* best_round, uint64_t* jacobi_iters, uint64_t* best_iters, uint64_t* round_iters);
 /* flexible PCG(MG-V) on the built-in right-hand side from x (device, n doubles;
  * NULL: from zero); learned = 0: Jacobi smoother on every level */
 int feat_learned_solve(feat_learned_t s, int learned, int allow_fallback, double* x,
