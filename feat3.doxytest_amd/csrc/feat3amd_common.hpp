@@ -191,8 +191,9 @@ __global__ __launch_bounds__(kThreads) void k_grid_finish(GridReduce g, uint32_t
   grid_reduce_tail<false>(g, nblocks, red);
 }
 
-// grids above this many workgroups defer their reduction to k_grid_finish
-constexpr uint64_t kDeferGrid = 128;
+// grids above this many workgroups defer their reduction to k_grid_finish (below it the
+// in-kernel finish costs less than the extra launch)
+constexpr uint64_t kDeferGrid = 512;
 
 __host__ inline GridReduce reduce_for_grid(GridReduce g, uint64_t grid)
 {
