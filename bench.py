@@ -326,6 +326,8 @@ def main():
     ap.add_argument("--headline-only", action="store_true",
                     help="only the FP64 MG-PCG solves (for kernel-trace profiles of the solve)")
     ap.add_argument("--learned-only", action="store_true", help="only the learned-smoother leg")
+    ap.add_argument("--stokes-only", action="store_true", help="only the Stokes leg")
+    ap.add_argument("--batch-only", action="store_true", help="only the 64-instance batch leg")
     ap.add_argument("--spmv-only", action="store_true",
                     help="only the SpMV loops (pair dictionary, plain SELL-64, CSR) and a 1 GiB device copy, "
                          "for hardware-counter runs")
@@ -345,6 +347,12 @@ def main():
         return
     if args.learned_only:
         print(json.dumps({"learned_smoother": learned_leg(feat, torch, dev, args)}))
+        return
+    if args.stokes_only:
+        print(json.dumps({"stokes": stokes_leg(feat, torch, dev, args)}))
+        return
+    if args.batch_only:
+        print(json.dumps({"batch": batch_leg(feat, torch, dev, args)}))
         return
 
     # ---- setup (host assembly + upload; not timed)

@@ -443,7 +443,7 @@ def runtime_sell_flags(flags):
 
 class StokesCfg(C.Structure):
     _fields_ = [("steps", _i), ("omega_rich", _d), ("vanka_omega", _d), ("coarse_skip_singular", _i),
-                ("omega_outer", _d), ("tol_rel", _d), ("max_iter", _u64)]
+                ("omega_outer", _d), ("tol_rel", _d), ("max_iter", _u64), ("no_graphs", _i)]
 
 
 class Stokes:
@@ -453,12 +453,12 @@ class Stokes:
     KIND = {"rhs": 0, "sol": 1, "def": 2, "cor": 3}
 
     def __init__(self, lf, lc, steps=5, omega_rich=0.7, vanka_omega=1.0, coarse_skip_singular=True,
-                 omega_outer=1.0, tol_rel=1e-8, max_iter=100):
+                 omega_outer=1.0, tol_rel=1e-8, max_iter=100, graphs=True):
         _, self.hl = libs()
         runtime_init()
         self.s = C.c_void_p()
         self.cfg = StokesCfg(steps, omega_rich, vanka_omega, int(coarse_skip_singular), omega_outer, tol_rel,
-                             max_iter)
+                             max_iter, int(not graphs))
         check(self.hl.feat_stokes_create(C.byref(self.s), lf, lc, C.byref(self.cfg)), "feat_stokes_create")
 
     def sizes(self, lvl=0):

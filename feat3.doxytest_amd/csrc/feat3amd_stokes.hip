@@ -29,6 +29,7 @@ struct TupleDev
   const uint64_t* ptr[4];
 };
 
+constexpr int kUnr = 8;   // blocks per row fetched ahead in the tuple SpMV
 __global__ __launch_bounds__(kThreads) void k_tuple_apply(TupleDev M, double* __restrict__ r,
                                                           const double* __restrict__ x,
                                                           const double* __restrict__ y, double alpha,
@@ -42,7 +43,32 @@ __global__ __launch_bounds__(kThreads) void k_tuple_apply(TupleDev M, double* __
     if(row < M.nv)
     {
       double s0 = 0.0, s1 = 0.0;
-      for(uint64_t k = M.ptr[0][row]; k < M.ptr[0][row + 1]; ++k)
+      uint64_t k = M.ptr[0][row];
+      const uint64_t ke = M.ptr[0][row + 1];
+      // blocks are fetched kUnr at a time before any x is gathered (same fma order)
+      for(; k + kUnr <= ke; k += kUnr)
+      {
+        double4 b[kUnr];
+        uint64_t c[kUnr];
+        double2 xv[kUnr];
+#pragma unroll
+        for(int u = 0; u < kUnr; ++u)
+        {
+          b[u] = reinterpret_cast<const double4*>(M.val[0])[k + u];
+          c[u] = M.col[0][k + u];
+        }
+#pragma unroll
+        for(int u = 0; u < kUnr; ++u) xv[u] = reinterpret_cast<const double2*>(x)[c[u]];
+#pragma unroll
+        for(int u = 0; u < kUnr; ++u)
+        {
+          s0 = dfma(b[u].x, xv[u].x, s0);
+          s0 = dfma(b[u].y, xv[u].y, s0);
+          s1 = dfma(b[u].z, xv[u].x, s1);
+          s1 = dfma(b[u].w, xv[u].y, s1);
+        }
+      }
+      for(; k < ke; ++k)
       {
         const double* b = M.val[0] + 4 * k;
         const uint64_t c = M.col[0][k];
@@ -59,10 +85,32 @@ __global__ __launch_bounds__(kThreads) void k_tuple_apply(TupleDev M, double* __
       if(M.val[1])
       {
         double t0 = 0.0, t1 = 0.0;
-        for(uint64_t k = M.ptr[1][row]; k < M.ptr[1][row + 1]; ++k)
+        uint64_t kb = M.ptr[1][row];
+        const uint64_t kbe = M.ptr[1][row + 1];
+        for(; kb + kUnr <= kbe; kb += kUnr)
         {
-          const double* b = M.val[1] + 2 * k;
-          const double xv = xp[M.col[1][k]];
+          double2 b[kUnr];
+          uint64_t c[kUnr];
+          double xv[kUnr];
+#pragma unroll
+          for(int u = 0; u < kUnr; ++u)
+          {
+            b[u] = reinterpret_cast<const double2*>(M.val[1])[kb + u];
+            c[u] = M.col[1][kb + u];
+          }
+#pragma unroll
+          for(int u = 0; u < kUnr; ++u) xv[u] = xp[c[u]];
+#pragma unroll
+          for(int u = 0; u < kUnr; ++u)
+          {
+            t0 = dfma(b[u].x, xv[u], t0);
+            t1 = dfma(b[u].y, xv[u], t1);
+          }
+        }
+        for(; kb < kbe; ++kb)
+        {
+          const double* b = M.val[1] + 2 * kb;
+          const double xv = xp[M.col[1][kb]];
           t0 = dfma(b[0], xv, t0);
           t1 = dfma(b[1], xv, t1);
         }
@@ -78,7 +126,28 @@ __global__ __launch_bounds__(kThreads) void k_tuple_apply(TupleDev M, double* __
     {
       const uint64_t pr = row - M.nv;
       double s = 0.0;
-      for(uint64_t k = M.ptr[2][pr]; k < M.ptr[2][pr + 1]; ++k)
+      uint64_t k = M.ptr[2][pr];
+      const uint64_t ke = M.ptr[2][pr + 1];
+      for(; k + kUnr <= ke; k += kUnr)
+      {
+        double2 b[kUnr], xv[kUnr];
+        uint64_t c[kUnr];
+#pragma unroll
+        for(int u = 0; u < kUnr; ++u)
+        {
+          b[u] = reinterpret_cast<const double2*>(M.val[2])[k + u];
+          c[u] = M.col[2][k + u];
+        }
+#pragma unroll
+        for(int u = 0; u < kUnr; ++u) xv[u] = reinterpret_cast<const double2*>(x)[c[u]];
+#pragma unroll
+        for(int u = 0; u < kUnr; ++u)
+        {
+          s = dfma(b[u].x, xv[u].x, s);
+          s = dfma(b[u].y, xv[u].y, s);
+        }
+      }
+      for(; k < ke; ++k)
       {
         const double* b = M.val[2] + 2 * k;
         const uint64_t c = M.col[2][k];
@@ -101,16 +170,31 @@ __global__ __launch_bounds__(kThreads) void k_tuple_apply(TupleDev M, double* __
 // ---------------------------------------------------------------------------
 // mean filter passes (one workgroup) and small vector kernels
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kThreads) void k_mean_passes(double* __restrict__ v, const double* __restrict__ w_dot,
-                                                          const double* __restrict__ w_add, double volume,
-                                                          uint64_t n, int passes, int sol_form)
+constexpr int kMeanThreads = 1024;
+__global__ __launch_bounds__(kMeanThreads) void k_mean_passes(double* __restrict__ v, const double* __restrict__ w_dot,
+                                                              const double* __restrict__ w_add, double volume,
+                                                              uint64_t n, int passes, int sol_form)
 {
-  __shared__ double red[kThreads / 64];
+  __shared__ double red[kMeanThreads / 64];
   __shared__ double s_alpha;
   for(int pass = 0; pass < passes; ++pass)
   {
+    // thread t adds v_i w_i for i = t, t + 1024, ... in order, four products fetched ahead
     double part = 0.0;
-    for(uint64_t i = threadIdx.x; i < n; i += blockDim.x) part += v[i] * w_dot[i];
+    uint64_t i = threadIdx.x;
+    for(; i + 3 * kMeanThreads < n; i += 4 * kMeanThreads)
+    {
+      double a[4], b[4];
+#pragma unroll
+      for(int u = 0; u < 4; ++u)
+      {
+        a[u] = v[i + u * kMeanThreads];
+        b[u] = w_dot[i + u * kMeanThreads];
+      }
+#pragma unroll
+      for(int u = 0; u < 4; ++u) part += a[u] * b[u];
+    }
+    for(; i < n; i += kMeanThreads) part += v[i] * w_dot[i];
     const double dot = block_sum(part, red);
     if(threadIdx.x == 0) s_alpha = sol_form ? 0.0 - dot / volume : -dot / volume;
     __syncthreads();
@@ -432,7 +516,7 @@ int feat_mean_filter_passes_f64(feat_handle_t h, double* v, const double* w_dot,
   if(!h || (n && (!v || !w_dot || !w_add)) || passes < 0 || passes > 2)
     return fail(FEAT_ERR_INVALID, "feat_mean_filter_passes_f64: bad argument");
   if(!n || !passes) return FEAT_OK;
-  hipLaunchKernelGGL(k_mean_passes, dim3(1), dim3(kThreads), 0, h->stream, v, w_dot, w_add, volume, n, passes,
+  hipLaunchKernelGGL(k_mean_passes, dim3(1), dim3(kMeanThreads), 0, h->stream, v, w_dot, w_add, volume, n, passes,
                      sol_form);
   return check_launch("feat_mean_filter_passes_f64");
 }

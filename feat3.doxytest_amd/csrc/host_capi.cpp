@@ -381,6 +381,7 @@ int feat_stokes_create(feat_stokes_t* out, int level_fine, int level_coarse, con
     c.vanka_omega = cfg->vanka_omega;
     c.coarse_skip_singular = cfg->coarse_skip_singular != 0;
     c.omega_outer = cfg->omega_outer;
+    c.graphs = cfg->no_graphs == 0;
     auto s = std::make_unique<feat_stokes_s>();
     s->mg = std::make_unique<Solver::StokesMultiGrid>(c);
     s->mg->set_tol_rel(cfg->tol_rel);

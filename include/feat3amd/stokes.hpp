@@ -419,6 +419,7 @@ namespace feat3amd
       double vanka_omega = 1.0;
       bool coarse_skip_singular = true;
       double omega_outer = 1.0;        // outer Richardson (:761)
+      bool graphs = true;              // replay each outer iteration as a recorded HIP graph
     };
 
     /// Richardson(MG-V(Richardson(SchwarzPrecond(AmaVanka)))) on a Q2/P1dc hierarchy.
@@ -442,6 +443,30 @@ namespace feat3amd
       LAFEM::StokesVector _d, _c;
       DeviceArray<double> _scal;
       FusedWorkspace _fw;
+      // one outer iteration recorded as a HIP graph on a solver-owned stream
+      struct Graph
+      {
+        feat_stream_t stream = nullptr;
+        feat_handle_t h = nullptr;
+        double* host = nullptr;
+        feat_graph_t g = nullptr;
+        const void* key_x = nullptr;
+        const void* key_b = nullptr;
+        bool failed = false;
+        void drop()
+        {
+          feat_graph_destroy(g);
+          g = nullptr;
+        }
+        ~Graph()
+        {
+          drop();
+          if(h) feat_handle_destroy(h);
+          if(stream) feat_stream_destroy(stream);
+          if(host) feat_host_free(host);
+        }
+      };
+      std::unique_ptr<Graph> _graph;
 
       Index n(const Level& L) const { return L.K.size(); }
 
@@ -545,10 +570,21 @@ namespace feat3amd
         _c = _lv[0]->K.create_vector();
         _scal = DeviceArray<double>(4);
         _fw.create(4);
+        if(_cfg.graphs && !_graph)
+        {
+          auto g = std::make_unique<Graph>();
+          void* hp = nullptr;
+          check(feat_stream_create(&g->stream), "Stokes graph: stream");
+          check(feat_handle_create(&g->h, 0, g->stream), "Stokes graph: handle");
+          check(feat_host_malloc(&hp, 8 * sizeof(double)), "Stokes graph: pinned scalar");
+          g->host = static_cast<double*>(hp);
+          _graph = std::move(g);
+        }
       }
       void init_numeric() override
       {
         for(auto& L : _lv) L->V->init_numeric();
+        if(_graph) _graph->drop();   // the recording holds the numeric data's addresses
       }
 
       /// one smoother application on level i (for tests)
@@ -572,6 +608,7 @@ namespace feat3amd
     private:
       Status _run(LAFEM::StokesVector& x, const LAFEM::StokesVector& b)
       {
+        if(_graph && !_graph->failed) return _run_graph(x, b);
         Level& L = *_lv[0];
         Status st = this->_set_initial_defect(read_norm(_d));
         while(st == Status::progress)
@@ -581,6 +618,73 @@ namespace feat3amd
           defect(L, _d, x, b);
           st = this->_set_new_defect([&]() { return read_norm(_d); });
         }
+        this->_status = st;
+        return st;
+      }
+
+      // The same loop with each outer iteration (V-cycle, update, defect, norm) recorded
+      // once as a HIP graph on the solver's stream and replayed: identical kernels in the
+      // identical order, so identical results; one host read per iteration as before.
+      Status _run_graph(LAFEM::StokesVector& x, const LAFEM::StokesVector& b)
+      {
+        Graph& g = *_graph;
+        Level& L = *_lv[0];
+        feat_handle_t caller = Runtime::handle();
+        if(g.key_x != x.elements() || g.key_b != b.elements())
+        {
+          g.drop();
+          g.key_x = x.elements();
+          g.key_b = b.elements();
+        }
+        Status st = this->_set_initial_defect(read_norm(_d));
+        if(st == Status::progress && !g.g)
+        {
+          Runtime::Use use(g.h);
+          bool ok = feat_graph_begin(g.h) == FEAT_OK;
+          if(ok)
+          {
+            try
+            {
+              mg_apply(_c, _d);
+              x.axpy(_c, x, _cfg.omega_outer);
+              defect(L, _d, x, b);
+              check(feat_tuple_norm2_f64(g.h, _d.elements(), 2 * L.K.nv(), L.K.np(), _scal.get(), &_fw.ws), "norm");
+              check(feat_copy_scalar_to_host(g.h, _scal.get(), g.host), "read-back");
+            }
+            catch(const Error&)
+            {
+              ok = false;
+            }
+            feat_graph_t gr = nullptr;
+            if(feat_graph_end(g.h, &gr) != FEAT_OK) ok = false;
+            if(ok)
+              g.g = gr;
+            else
+              feat_graph_destroy(gr);
+          }
+          if(!ok)
+          {
+            g.failed = true;   // recording is not available: this and later solves run eagerly
+            g.drop();
+            while(st == Status::progress)
+            {
+              mg_apply(_c, _d);
+              x.axpy(_c, x, _cfg.omega_outer);
+              defect(L, _d, x, b);
+              st = this->_set_new_defect([&]() { return read_norm(_d); });
+            }
+            this->_status = st;
+            return st;
+          }
+        }
+        if(st == Status::progress) check(feat_handle_wait(g.h, caller), "Stokes: order after caller");
+        while(st == Status::progress)
+        {
+          check(feat_graph_launch(g.h, g.g), "Stokes graph");
+          check(feat_sync(g.h), "Stokes sync");
+          st = this->_set_new_defect([&]() { return g.host[0]; });
+        }
+        check(feat_handle_wait(caller, g.h), "Stokes: order before caller");
         this->_status = st;
         return st;
       }

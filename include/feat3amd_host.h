@@ -118,6 +118,7 @@ typedef struct {
   double omega_outer;      /* outer Richardson damping (1.0)                */
   double tol_rel;
   uint64_t max_iter;
+  int no_graphs;           /* 1: launch every kernel eagerly (no recorded iteration) */
 } feat_stokes_cfg_t;
 typedef struct feat_stokes_s* feat_stokes_t;
 /* builds the levels and runs init_symbolic + init_numeric (AmaVanka on device) */

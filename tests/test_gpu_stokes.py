@@ -144,3 +144,27 @@ def test_stokes_solve_vs_oracle(feat, oracle, torch_gpu, lf, lc):
     # FP64 residual of the GPU solution, with the oracle's operator and filter
     res = oracle.stokes_filter(lv[0], oracle.saddle_apply(lv[0], xg, b, -1.0), "def")
     assert np.linalg.norm(res) <= 1e-8 * o["def_init"] * 1.01
+
+
+def test_stokes_graph_replay_matches_eager(feat, oracle, torch_gpu):
+    """Each outer iteration replayed as a recorded HIP graph gives bit-identical iterates
+    and defect histories to eager launches, over repeated solves (the second one replays
+    the recording) and a new solution buffer (re-recorded)."""
+    torch = torch_gpu
+    lv = stokes_levels(feat, 4, 1)
+    x0, b = stokes_rhs(lv[0], oracle)
+    bd = dev(torch, b)
+    runs = []
+    for graphs in (True, False):
+        s = feat.Stokes(4, 1, graphs=graphs)
+        out = []
+        for case in range(3):
+            x = dev(torch, x0)
+            r = s.solve(x, bd)
+            out.append((r["iters"], np.asarray(r["hist"]), host(x)))
+        s.close()
+        runs.append(out)
+    for (ig, hg, xg), (ie, he, xe) in zip(*runs):
+        assert ig == ie
+        assert np.array_equal(hg, he)
+        assert np.array_equal(xg, xe)
