@@ -24,10 +24,8 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ uint16_t f2bf(float f)
 {
-  // round to nearest even (the values here are finite)
-  uint32_t u = __float_as_uint(f);
-  u += 0x7FFFu + ((u >> 16) & 1u);
-  return uint16_t(u >> 16);
+  // round to nearest even: gfx950's v_cvt_pk_bf16_f32
+  return __builtin_bit_cast(uint16_t, static_cast<__bf16>(f));
 }
 __device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float(uint32_t(b) << 16); }
 __device__ __forceinline__ float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
@@ -479,6 +477,16 @@ __global__ __launch_bounds__(64 * kWaves) void k_mlp_smooth(Grid3 G, const float
   const uint64_t m = G.m, mm = m * m;
   const uint64_t ngroups = (G.n + 16 * kWaves - 1) / (16 * kWaves);
   double dot = 0.0;
+  // patch offsets of this lane's 16 inputs (k = ks*16 + 4q + i; residual k < 27, coefficient
+  // 27 <= k < 54 at patch position k - 27; padding reads the node itself)
+  int32_t poff[16];
+#pragma unroll
+  for(int e = 0; e < 16; ++e)
+  {
+    const int k = (e >> 2) * 16 + 4 * q + (e & 3);
+    const int j = k < 27 ? k : (k < kIn ? k - 27 : 13);
+    poff[e] = int32_t(j % 3 - 1) + int32_t((j / 3) % 3 - 1) * int32_t(m) + int32_t(j / 9 - 1) * int32_t(mm);
+  }
   for(uint64_t g = blockIdx.x; g < ngroups; g += gridDim.x)
   {
     // keeps the weight fragments in LDS (read per use) instead of 64+ registers
@@ -507,9 +515,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_mlp_smooth(Grid3 G, const float
       for(int i = 0; i < 4; ++i)
       {
         const int k = ks * 16 + 4 * q + i, e = ks * 4 + i;
-        const int j = k < 27 ? k : (k < kIn ? k - 27 : 13);
-        const int32_t off = int32_t(j % 3 - 1) + int32_t((j / 3) % 3 - 1) * int32_t(m32) + int32_t(j / 9 - 1) * int32_t(mm32);
-        const uint32_t nb = interior ? uint32_t(int32_t(n32) + off) : 0u;
+        const uint32_t nb = interior ? uint32_t(int32_t(n32) + poff[e]) : 0u;
         rv[e] = r[k < 27 ? nb : 0u];
         fv[e] = cfeat[(k >= 27 && k < kIn) ? nb : 0u];
       }
