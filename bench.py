@@ -228,6 +228,64 @@ def batch_leg(feat, torch, dev, args):
     return out
 
 
+def assembly_leg(feat, torch, dev, args):
+    """Device assembly of the structured Q1 level (stiffness, boundary list,
+    prolongation, restriction) against the host assembly of the same arrays, and
+    the MG-PCG setup with device- vs host-assembled levels."""
+    k, _ = feat.libs()
+    stream = torch.cuda.current_stream()
+    h = feat.Handle(0, stream=stream.cuda_stream)
+    d = feat.q1_problem_device(torch, h, DIM, LF)
+    names = ("val", "col", "rowptr", "filt", "pval", "pcol", "prowptr", "rval", "rcol", "rrowptr")
+    ptrs = [d[nm].data_ptr() for nm in names]
+
+    def launch():
+        feat.check(k.feat_q1_assemble_device(h.h, DIM, LF, *ptrs), "feat_q1_assemble_device")
+
+    for _ in range(3):
+        launch()
+    reps = 20
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        launch()
+    e1.record(stream)
+    e1.synchronize()
+    dev_ms = e0.elapsed_time(e1) / reps
+    # write-only: 16 B per entry of A, P and R, the three row-pointer arrays, the boundary list
+    wbytes = sum(d[nm].numel() * d[nm].element_size() for nm in names)
+    t = time.perf_counter()
+    q = feat.q1_problem(DIM, LF)
+    host_s = time.perf_counter() - t
+    same = (np.array_equal(d["val"].cpu().numpy().view(np.uint64), q["val"].view(np.uint64))
+            and np.array_equal(d["col"].cpu().numpy().view(np.uint64), q["col"])
+            and np.array_equal(d["pval"].cpu().numpy().view(np.uint64), q["pval"].view(np.uint64))
+            and np.array_equal(d["filt"].cpu().numpy().view(np.uint64), q["filt"]))
+    del d, q
+    h.close()
+    torch.cuda.synchronize()
+    cfg = feat.mgpcg_cfg(nu_pre=4, nu_post=4, omega_jac=0.5, omega_rich=1.0, coarse="direct",
+                         tol_rel=1e-8, max_iter=100)
+    setup = {}
+    for host in (False, True):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        s = feat.MgPcg(cfg, q1=(DIM, LF, LC), host_assembly=host)
+        torch.cuda.synchronize()
+        setup["host" if host else "device"] = time.perf_counter() - t
+        s.close()
+    log(f"assembly level {LF}: device {dev_ms:.2f} ms ({wbytes / (dev_ms * 1e-3) / 1e9:.0f} GB/s written), "
+        f"host {host_s:.2f} s, identical {same}; MG-PCG setup device {setup['device']:.2f} s, host {setup['host']:.2f} s")
+    return {"what": f"Q1 level {LF}: stiffness (CSR f64/u64), boundary list, prolongation from level {LF - 1} "
+                    "and restriction (its transpose)",
+            "device_ms": dev_ms, "bytes_written": wbytes, "gbps_written": wbytes / (dev_ms * 1e-3) / 1e9,
+            "host_s": host_s, "host_what": "host assembly of stiffness, boundary and prolongation (no transpose), "
+                                          "one core",
+            "bitwise_identical": bool(same),
+            "mgpcg_setup_s": {"device_assembly": setup["device"], "host_assembly_and_upload": setup["host"],
+                              "includes": "levels 7..1, SELL-64 dictionary plans, autotuning, coarse factorisation"}}
+
+
 def learned_leg(feat, torch, dev, args):
     """Learned smoother on 3D Q1 anisotropic log-normal diffusion, level 6 (274,625 dofs):
     flexible PCG(MG-V(4,4)) with the Jacobi smoother vs the trained BF16-MLP smoother,
@@ -328,6 +386,7 @@ def main():
     ap.add_argument("--learned-only", action="store_true", help="only the learned-smoother leg")
     ap.add_argument("--stokes-only", action="store_true", help="only the Stokes leg")
     ap.add_argument("--batch-only", action="store_true", help="only the 64-instance batch leg")
+    ap.add_argument("--assembly-only", action="store_true", help="only the device-assembly leg")
     ap.add_argument("--spmv-only", action="store_true",
                     help="only the SpMV loops (pair dictionary, plain SELL-64, CSR) and a 1 GiB device copy, "
                          "for hardware-counter runs")
@@ -354,8 +413,11 @@ def main():
     if args.batch_only:
         print(json.dumps({"batch": batch_leg(feat, torch, dev, args)}))
         return
+    if args.assembly_only:
+        print(json.dumps({"assembly": assembly_leg(feat, torch, dev, args)}))
+        return
 
-    # ---- setup (host assembly + upload; not timed)
+    # ---- setup (device assembly of levels 7..2, host assembly of level 1; not timed)
     t0 = time.perf_counter()
     cfg = feat.mgpcg_cfg(nu_pre=4, nu_post=4, omega_jac=0.5, omega_rich=1.0, coarse="direct",
                          tol_rel=1e-8, max_iter=100)
@@ -497,6 +559,7 @@ def main():
     stokes = stokes_leg(feat, torch, dev, args)
     batch = batch_leg(feat, torch, dev, args)
     learned = learned_leg(feat, torch, dev, args)
+    assembly = assembly_leg(feat, torch, dev, args)
 
     # ---- SpMV r = A*x on the same matrix, HIP events on the launch stream:
     # the caller's CSR (u64) and the SELL-64 plan the solver runs on
@@ -588,6 +651,7 @@ def main():
         "stokes": stokes,
         "batch": batch,
         "learned_smoother": learned,
+        "assembly": assembly,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS, "traffic": SPMV_TRAFFIC_BYTES["sell64_dict"],
                      "algorithmic_bytes": dict_bytes,

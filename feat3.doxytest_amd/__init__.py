@@ -56,6 +56,7 @@ class Matrix(C.Structure):
 FMT_CSR, FMT_SELL = 0, 1
 IDX_U64, IDX_U32, IDX_D16, IDX_DICT = 0, 1, 2, 3
 SELL_NO_DICT = 1
+Q1_HOST_ASSEMBLY = 1
 
 
 class ReduceWs(C.Structure):
@@ -136,6 +137,8 @@ KERNEL_API = [
     ("feat_sell_destroy_f64", _i, [C.POINTER(Matrix)]),
     ("feat_sell_destroy_f32", _i, [C.POINTER(Matrix)]),
     ("feat_matrix_apply_f64", _i, [_vp, C.POINTER(Matrix), _vp, _d, _vp, _d, _vp]),
+    ("feat_q1_device_sizes", _i, [_i, _i, _vp]),
+    ("feat_q1_assemble_device", _i, [_vp, _i, _i] + [_vp] * 10),
     ("feat_defect_f64", _i, [_vp, C.POINTER(Matrix), _vp, _vp, _vp, _vp, _vp, C.POINTER(ReduceWs)]),
     ("feat_apply_masked_f64", _i, [_vp, C.POINTER(Matrix), _vp, _vp, _vp, _vp, _vp, _vp, C.POINTER(ReduceWs)]),
     ("feat_prol_add_f64", _i, [_vp, C.POINTER(Matrix), _vp, _vp, _vp]),
@@ -218,6 +221,7 @@ HOST_API = [
     ("feat_learned_destroy", _i, [_vp]),
     ("feat_mgpcg_create", _i, [C.POINTER(_vp), C.POINTER(LevelHost), _i, C.POINTER(MgPcgCfg)]),
     ("feat_mgpcg_create_q1", _i, [C.POINTER(_vp), _i, _i, _i, C.POINTER(MgPcgCfg)]),
+    ("feat_mgpcg_create_q1_ex", _i, [C.POINTER(_vp), _i, _i, _i, C.POINTER(MgPcgCfg), _i]),
     ("feat_mgpcg_solve", _i, [_vp, _vp, _vp, C.POINTER(SolveResult), _vp, _u64]),
     ("feat_mgpcg_matrix", _i, [_vp, _i, C.POINTER(_u64), C.POINTER(_u64), C.POINTER(_vp), C.POINTER(_vp),
                                C.POINTER(_vp)]),
@@ -359,6 +363,29 @@ def q1_problem(dim, level):
     out = dict(n=n, val=val, col=col, rowptr=rp, filt=bnd)
     if level > 0:
         out.update(pval=pval[:pnnz], pcol=pcol[:pnnz], prowptr=prp, pcols=pcols)
+    return out
+
+
+def q1_problem_device(torch_mod, handle, dim, level, transfer=True):
+    """The level of q1_problem() assembled on the device (torch tensors on the
+    handle's device, bit-identical to the host arrays); with transfer and level > 0
+    also the restriction R = P^T (rval, rcol, rrowptr)."""
+    k, _ = libs()
+    s = np.zeros(5, np.uint64)
+    check(k.feat_q1_device_sizes(dim, level, s.ctypes.data), "feat_q1_device_sizes")
+    n, nnz, nb, pnnz, nc = (int(v) for v in s)
+    dev = torch_mod.device("cuda", torch_mod.cuda.current_device())
+    u64 = torch_mod.int64   # the uint64 indices, as int64 tensors (all below 2^63)
+    e = lambda m, dt: torch_mod.empty(m, dtype=dt, device=dev)
+    out = dict(n=n, val=e(nnz, torch_mod.float64), col=e(nnz, u64), rowptr=e(n + 1, u64), filt=e(nb, u64))
+    tr = [None] * 6
+    if transfer and level > 0:
+        out.update(pval=e(pnnz, torch_mod.float64), pcol=e(pnnz, u64), prowptr=e(n + 1, u64), pcols=nc,
+                   rval=e(pnnz, torch_mod.float64), rcol=e(pnnz, u64), rrowptr=e(nc + 1, u64))
+        tr = [out[nm].data_ptr() for nm in ("pval", "pcol", "prowptr", "rval", "rcol", "rrowptr")]
+    check(k.feat_q1_assemble_device(handle.h, dim, level, out["val"].data_ptr(), out["col"].data_ptr(),
+                                    out["rowptr"].data_ptr(), out["filt"].data_ptr(), *tr),
+          "feat_q1_assemble_device")
     return out
 
 
@@ -721,15 +748,16 @@ class LearnedMG:
 class MgPcg:
     """PCG(MultiGrid-V(Richardson(Jacobi))) of the C++ host layer."""
 
-    def __init__(self, cfg, levels=None, q1=None):
+    def __init__(self, cfg, levels=None, q1=None, host_assembly=False):
         _, self.hl = libs()
         runtime_init()
         self.s = C.c_void_p()
         self.cfg = cfg
         if q1 is not None:
             dim, lf, lc = q1
-            check(self.hl.feat_mgpcg_create_q1(C.byref(self.s), dim, lf, lc, C.byref(cfg)),
-                  "feat_mgpcg_create_q1")
+            check(self.hl.feat_mgpcg_create_q1_ex(C.byref(self.s), dim, lf, lc, C.byref(cfg),
+                                                  Q1_HOST_ASSEMBLY if host_assembly else 0),
+                  "feat_mgpcg_create_q1_ex")
         else:
             arr = (LevelHost * len(levels))()
             keep = []

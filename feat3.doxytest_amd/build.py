@@ -21,7 +21,8 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 CXX = os.environ.get("CXX", "g++")
 
 # every HIP translation unit (one object each, linked into one .so)
-HIP_SOURCES = ["feat3amd_kernels.hip", "feat3amd_stokes.hip", "feat3amd_batch.hip", "feat3amd_mlp.hip"]
+HIP_SOURCES = ["feat3amd_kernels.hip", "feat3amd_stokes.hip", "feat3amd_batch.hip", "feat3amd_mlp.hip",
+               "feat3amd_assembly.hip"]
 HOST_SOURCES = ["host_capi.cpp"]
 
 HIP_FLAGS = [

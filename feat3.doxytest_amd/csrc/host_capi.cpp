@@ -285,13 +285,43 @@ int feat_mgpcg_create(feat_mgpcg_t* out, const feat_level_host_t* lv, int nl, co
   });
 }
 
-int feat_mgpcg_create_q1(feat_mgpcg_t* out, int dim, int lf, int lc, const feat_mgpcg_cfg_t* cfg)
+// A structured Q1 level assembled on the device (stiffness, boundary, and the
+// prolongation from level-1 with its transpose when `transfer`).
+static std::unique_ptr<LevelData> make_level_device(int dim, int l, bool transfer)
+{
+  std::uint64_t sz[5];
+  check(feat_q1_device_sizes(dim, l, sz), "feat_q1_device_sizes");
+  auto L = std::make_unique<LevelData>();
+  L->A = Csr::allocate(sz[0], sz[0], sz[1]);
+  DeviceArray<std::uint64_t> bnd(sz[2]);
+  Csr P, R;
+  if(transfer)
+  {
+    P = Csr::allocate(sz[0], sz[4], sz[3]);
+    R = Csr::allocate(sz[4], sz[0], sz[3]);
+  }
+  check(feat_q1_assemble_device(Runtime::handle(), dim, l, L->A.val_mut(), L->A.col_mut(), L->A.row_ptr_mut(),
+                                bnd.get(), transfer ? P.val_mut() : nullptr, transfer ? P.col_mut() : nullptr,
+                                transfer ? P.row_ptr_mut() : nullptr, transfer ? R.val_mut() : nullptr,
+                                transfer ? R.col_mut() : nullptr, transfer ? R.row_ptr_mut() : nullptr),
+        "feat_q1_assemble_device");
+  L->F = Filt(sz[0], bnd);
+  if(transfer) L->T = Trans(std::move(P), std::move(R));
+  return L;
+}
+
+int feat_mgpcg_create_q1_ex(feat_mgpcg_t* out, int dim, int lf, int lc, const feat_mgpcg_cfg_t* cfg, int flags)
 {
   return guarded([&]() {
     if(!out || !cfg || lf < lc || lc < 0 || (dim != 2 && dim != 3)) throw Error("feat_mgpcg_create_q1: bad argument");
     auto s = std::make_unique<feat_mgpcg_s>();
     for(int l = lf; l >= lc; --l)
     {
+      if(l > lc && !(flags & FEAT_Q1_HOST_ASSEMBLY))
+      {
+        s->levels.push_back(make_level_device(dim, l, true));
+        continue;
+      }
       const auto A = Assembly::q1_laplace(dim, l);
       const auto f = Assembly::q1_boundary(dim, l);
       if(l > lc)
@@ -299,12 +329,17 @@ int feat_mgpcg_create_q1(feat_mgpcg_t* out, int dim, int lf, int lc, const feat_
         const auto P = Assembly::q1_prolongation(dim, l);
         s->levels.push_back(make_level(A, f, &P, false));
       }
-      else
+      else  // the coarse solver factorises the host copy
         s->levels.push_back(make_level(A, f, nullptr, true));
     }
     build_solver(s.get(), cfg);
     *out = s.release();
   });
+}
+
+int feat_mgpcg_create_q1(feat_mgpcg_t* out, int dim, int lf, int lc, const feat_mgpcg_cfg_t* cfg)
+{
+  return feat_mgpcg_create_q1_ex(out, dim, lf, lc, cfg, 0);
 }
 
 int feat_mgpcg_solve(feat_mgpcg_t s, double* x, const double* b, feat_solve_result_t* res, double* hist,

@@ -479,6 +479,21 @@ int feat_batch_gather_f64(feat_handle_t h, uint32_t nseg, const uint64_t* dst_lo
 /* copies a device int to pinned host memory (stream-ordered) */
 int feat_copy_int_to_host(feat_handle_t h, const int* dev, int* host_pinned);
 
+/* ---------------- structured Q1 hierarchy on the device ----------------
+ * The Q1 Laplace problem on level `level` of the unit square (dim 2) / cube (dim 3),
+ * lexicographic node numbering: stiffness matrix (SymbolicAssembler + LaplaceOperator),
+ * Dirichlet boundary list (UnitFilterAssembler), prolongation from level-1 and its
+ * transpose, the restriction (GridTransfer::assemble_prolongation and transpose,
+ * control/scalar_basic.hpp:282-331).  Bit-identical to the host assembly of
+ * include/feat3amd/assembly.hpp.  sizes: [rows, stiffness nnz, boundary nodes,
+ * prolongation nnz, coarse rows]; the restriction has the prolongation's nnz.
+ * No counterpart in the reference (its assembly runs on the host). */
+int feat_q1_device_sizes(int dim, int level, uint64_t* sizes);
+/* Any of bnd / the prolongation arrays / the restriction arrays may be NULL (skipped). */
+int feat_q1_assemble_device(feat_handle_t h, int dim, int level, double* val, uint64_t* col,
+                            uint64_t* rowptr, uint64_t* bnd, double* pval, uint64_t* pcol,
+                            uint64_t* prowptr, double* rval, uint64_t* rcol, uint64_t* rrowptr);
+
 /* ---------------- autotuner ----------------
  * Launch shapes of the SELL-64 row kernels that every fused entry point uses
  * (entries per row in flight: 4, 8 or 16 -- pair-dictionary plans: 4, 8 or 16

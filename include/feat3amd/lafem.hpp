@@ -282,6 +282,21 @@ namespace feat3amd
         m._rowptr = DeviceArray<IT_>::from_host(rowptr, rows + 1);
         return m;
       }
+      /// uninitialised device arrays of the given shape, to be filled by a device assembly
+      static SparseMatrixCSR allocate(Index rows, Index cols, Index nnz)
+      {
+        SparseMatrixCSR m;
+        m._rows = rows;
+        m._cols = cols;
+        m._nnz = nnz;
+        m._val = DeviceArray<DT_>(nnz);
+        m._col = DeviceArray<IT_>(nnz);
+        m._rowptr = DeviceArray<IT_>(rows + 1);
+        return m;
+      }
+      DT_* val_mut() { return _val.get(); }
+      IT_* col_mut() { return _col.get(); }
+      IT_* row_ptr_mut() { return _rowptr.get(); }
       static SparseMatrixCSR wrap(Index rows, Index cols, Index nnz, DT_* val, IT_* col, IT_* rowptr)
       {
         SparseMatrixCSR m;
@@ -469,6 +484,14 @@ namespace feat3amd
         _val = DeviceArray<double>::from_host(values, ue);
         _mask = DeviceArray<std::uint32_t>((size + 31) / 32);
         check(feat_unit_filter_mask_u64(Runtime::handle(), _mask.get(), size, _idx.get(), ue), "UnitFilter mask");
+      }
+      /// homogeneous filter on device-resident indices
+      UnitFilter(Index size, const DeviceArray<std::uint64_t>& idx) : _size(size), _ue(idx.size()), _idx(idx)
+      {
+        _val = DeviceArray<double>(_ue);
+        if(_ue) check(feat_set_memory_f64(Runtime::handle(), _val.get(), 0.0, _ue), "UnitFilter values");
+        _mask = DeviceArray<std::uint32_t>((size + 31) / 32);
+        check(feat_unit_filter_mask_u64(Runtime::handle(), _mask.get(), size, _idx.get(), _ue), "UnitFilter mask");
       }
       Index size() const { return _size; }
       Index used_elements() const { return _ue; }

@@ -88,8 +88,14 @@ typedef struct {
 
 int feat_mgpcg_create(feat_mgpcg_t* s, const feat_level_host_t* levels, int nlevels,
                       const feat_mgpcg_cfg_t* cfg);
-/* builds the structured Q1 hierarchy levels lf (finest) .. lc (coarsest) itself */
+/* builds the structured Q1 hierarchy levels lf (finest) .. lc (coarsest) itself: levels
+ * above lc on the device (feat_q1_assemble_device), the coarsest on the host (its direct
+ * solver factorises the host copy) */
 int feat_mgpcg_create_q1(feat_mgpcg_t* s, int dim, int lf, int lc, const feat_mgpcg_cfg_t* cfg);
+/* flags: FEAT_Q1_HOST_ASSEMBLY assembles every level on the host and uploads it (the
+ * arrays are bit-identical either way) */
+#define FEAT_Q1_HOST_ASSEMBLY 1
+int feat_mgpcg_create_q1_ex(feat_mgpcg_t* s, int dim, int lf, int lc, const feat_mgpcg_cfg_t* cfg, int flags);
 /* Solver::solve(pcg, x, b, A, F) -> PCG::correct: x is the initial guess (device), b device */
 int feat_mgpcg_solve(feat_mgpcg_t s, double* x, const double* b, feat_solve_result_t* res,
                      double* hist, uint64_t hist_cap);
