@@ -274,6 +274,36 @@ def assembly_leg(feat, torch, dev, args):
         torch.cuda.synchronize()
         setup["host" if host else "device"] = time.perf_counter() - t
         s.close()
+    # ---- Q2/P1dc Stokes level (A, B, D, filters, both prolongations and restrictions)
+    hs = feat.Handle(0, stream=stream.cuda_stream)
+    stokes = {}
+    for lvl in (STOKES_LF, 9):
+        sd = feat.stokes_problem_device(torch, hs, lvl)
+        sa = feat.StokesLevelDev()
+        for key, pre in (("A", "a"), ("B", "b"), ("D", "d"), ("Pv", "pv"), ("Pp", "pp"), ("Rv", "rv"), ("Rp", "rp")):
+            for nm, tt in zip(("val", "col", "ptr"), sd[key][:3]):
+                setattr(sa, f"{pre}_{nm}", tt.data_ptr())
+        for nm in ("filt", "filt_val", "prim", "dual"):
+            setattr(sa, nm, sd[nm].data_ptr())
+        for _ in range(3):
+            feat.check(k.feat_stokes_assemble_device(hs.h, lvl, C.byref(sa)), "feat_stokes_assemble_device")
+        e0.record(stream)
+        for _ in range(reps):
+            feat.check(k.feat_stokes_assemble_device(hs.h, lvl, C.byref(sa)), "feat_stokes_assemble_device")
+        e1.record(stream)
+        e1.synchronize()
+        sms = e0.elapsed_time(e1) / reps
+        sbytes = sum(tt.numel() * tt.element_size() for key in ("A", "B", "D", "Pv", "Pp", "Rv", "Rp")
+                     for tt in sd[key][:3]) + sum(sd[nm].numel() * 8 for nm in ("filt", "filt_val", "prim", "dual"))
+        t = time.perf_counter()
+        feat.stokes_problem(lvl)
+        shost = time.perf_counter() - t
+        stokes[f"level_{lvl}"] = {"device_ms": sms, "bytes_written": sbytes, "gbps_written": sbytes / (sms * 1e-3) / 1e9,
+                                  "host_s": shost}
+        log(f"stokes assembly level {lvl}: device {sms * 1e3:.1f} us ({sbytes / (sms * 1e-3) / 1e9:.0f} GB/s written), "
+            f"host {shost * 1e3:.1f} ms")
+        del sd
+    hs.close()
     log(f"assembly level {LF}: device {dev_ms:.2f} ms ({wbytes / (dev_ms * 1e-3) / 1e9:.0f} GB/s written), "
         f"host {host_s:.2f} s, identical {same}; MG-PCG setup device {setup['device']:.2f} s, host {setup['host']:.2f} s")
     return {"what": f"Q1 level {LF}: stiffness (CSR f64/u64), boundary list, prolongation from level {LF - 1} "
@@ -283,7 +313,10 @@ def assembly_leg(feat, torch, dev, args):
                                           "one core",
             "bitwise_identical": bool(same),
             "mgpcg_setup_s": {"device_assembly": setup["device"], "host_assembly_and_upload": setup["host"],
-                              "includes": "levels 7..1, SELL-64 dictionary plans, autotuning, coarse factorisation"}}
+                              "includes": "levels 7..1, SELL-64 dictionary plans, autotuning, coarse factorisation"},
+            "stokes_q2p1dc": dict(stokes, what="A (2x2 blocks), B, D, velocity filter, mean-filter vectors, Q2 and "
+                                               "P1dc prolongations and restrictions; host: the same without the "
+                                               "restrictions, one core")}
 
 
 def learned_leg(feat, torch, dev, args):

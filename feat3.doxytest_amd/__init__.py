@@ -139,6 +139,8 @@ KERNEL_API = [
     ("feat_matrix_apply_f64", _i, [_vp, C.POINTER(Matrix), _vp, _d, _vp, _d, _vp]),
     ("feat_q1_device_sizes", _i, [_i, _i, _vp]),
     ("feat_q1_assemble_device", _i, [_vp, _i, _i] + [_vp] * 10),
+    ("feat_stokes_device_sizes", _i, [_i, _vp]),
+    ("feat_stokes_assemble_device", _i, [_vp, _i, _vp]),
     ("feat_defect_f64", _i, [_vp, C.POINTER(Matrix), _vp, _vp, _vp, _vp, _vp, C.POINTER(ReduceWs)]),
     ("feat_apply_masked_f64", _i, [_vp, C.POINTER(Matrix), _vp, _vp, _vp, _vp, _vp, _vp, C.POINTER(ReduceWs)]),
     ("feat_prol_add_f64", _i, [_vp, C.POINTER(Matrix), _vp, _vp, _vp]),
@@ -341,6 +343,43 @@ def stokes_problem(level):
     for nm in ("filt", "filt_val", "prim", "dual"):
         setattr(a, nm, out[nm].ctypes.data)
     check(h.feat_stokes_assemble(level, C.byref(a)), "feat_stokes_assemble")
+    return out
+
+
+class StokesLevelDev(C.Structure):
+    """feat_stokes_level_dev_t: device output arrays of feat_stokes_assemble_device."""
+    _fields_ = [(n, _vp) for n in ("a_val", "a_col", "a_ptr", "b_val", "b_col", "b_ptr", "d_val", "d_col",
+                                   "d_ptr", "filt", "filt_val", "prim", "dual", "pv_val", "pv_col", "pv_ptr",
+                                   "pp_val", "pp_col", "pp_ptr", "rv_val", "rv_col", "rv_ptr", "rp_val",
+                                   "rp_col", "rp_ptr")]
+
+
+def stokes_problem_device(torch_mod, handle, level):
+    """The level of stokes_problem() assembled on the device (torch tensors, index arrays
+    as int64), plus the restrictions Rv = Pv^T and Rp = Pp^T on levels > 0."""
+    k, _ = libs()
+    s = np.zeros(10, np.uint64)
+    check(k.feat_stokes_device_sizes(level, s.ctypes.data), "feat_stokes_device_sizes")
+    nv, np_, na, nb, nd, nf, npv, npp, nvc, npc = (int(v) for v in s)
+    dev = torch_mod.device("cuda", torch_mod.cuda.current_device())
+    f64 = lambda m: torch_mod.empty(m, dtype=torch_mod.float64, device=dev)
+    i64 = lambda m: torch_mod.empty(m, dtype=torch_mod.int64, device=dev)
+    out = dict(nv=nv, np=np_, A=(f64(4 * na), i64(na), i64(nv + 1)), B=(f64(2 * nb), i64(nb), i64(nv + 1)),
+               D=(f64(2 * nd), i64(nd), i64(np_ + 1)), filt=i64(nf), filt_val=f64(2 * nf), prim=f64(np_),
+               dual=f64(np_))
+    if level > 0:
+        out["Pv"] = (f64(npv), i64(npv), i64(nv + 1), nvc)
+        out["Pp"] = (f64(npp), i64(npp), i64(np_ + 1), npc)
+        out["Rv"] = (f64(npv), i64(npv), i64(nvc + 1), nv)
+        out["Rp"] = (f64(npp), i64(npp), i64(npc + 1), np_)
+    a = StokesLevelDev()
+    for key, pre in (("A", "a"), ("B", "b"), ("D", "d"), ("Pv", "pv"), ("Pp", "pp"), ("Rv", "rv"), ("Rp", "rp")):
+        if key in out:
+            for nm, t in zip(("val", "col", "ptr"), out[key][:3]):
+                setattr(a, f"{pre}_{nm}", t.data_ptr())
+    for nm in ("filt", "filt_val", "prim", "dual"):
+        setattr(a, nm, out[nm].data_ptr())
+    check(k.feat_stokes_assemble_device(handle.h, level, C.byref(a)), "feat_stokes_assemble_device")
     return out
 
 

@@ -494,6 +494,33 @@ int feat_q1_assemble_device(feat_handle_t h, int dim, int level, double* val, ui
                             uint64_t* rowptr, uint64_t* bnd, double* pval, uint64_t* pcol,
                             uint64_t* prowptr, double* rval, uint64_t* rcol, uint64_t* rrowptr);
 
+/* ---------------- Q2/P1dc Stokes level on the device ----------------
+ * The lid-driven-cavity level of benchmarks/stokes_multigrid-bench.cpp:576-697 on the unit
+ * square (2^level cells per axis, lexicographic velocity nodes, P1dc dofs 3e..3e+2): velocity
+ * block A = DuDvOperatorBlocked<2> (2x2 blocks, assembly/common_operators.hpp:875-961),
+ * B = -(div) (2x1, control/stokes_blocked.hpp:784-798), D = B^T (1x2), UnitFilterBlocked
+ * velocity boundary (lid u = (1, 0)), MeanFilter vectors (mean_filter_assembler.hpp:67-85),
+ * Q2 and P1dc prolongations from level-1 and their transposes (GridTransfer,
+ * control/stokes_blocked.hpp).  Bit-identical to the host assembly of
+ * include/feat3amd/stokes_assembly.hpp (and the host transpose for the restrictions).
+ * sizes: {nv, np, nnz A, nnz B, nnz D, nfilt, nnz Pv (= nnz Rv), nnz Pp (= nnz Rp),
+ * nv coarse, np coarse}.  Every group of three pointers (filt/filt_val and prim/dual as
+ * pairs) is either complete or all NULL (skipped); level 0 has no transfer.
+ * No counterpart in the reference (its assembly runs on the host). */
+typedef struct {
+  double* a_val; uint64_t* a_col; uint64_t* a_ptr;
+  double* b_val; uint64_t* b_col; uint64_t* b_ptr;
+  double* d_val; uint64_t* d_col; uint64_t* d_ptr;
+  uint64_t* filt; double* filt_val;
+  double* prim; double* dual;
+  double* pv_val; uint64_t* pv_col; uint64_t* pv_ptr;
+  double* pp_val; uint64_t* pp_col; uint64_t* pp_ptr;
+  double* rv_val; uint64_t* rv_col; uint64_t* rv_ptr;
+  double* rp_val; uint64_t* rp_col; uint64_t* rp_ptr;
+} feat_stokes_level_dev_t;
+int feat_stokes_device_sizes(int level, uint64_t* sizes);
+int feat_stokes_assemble_device(feat_handle_t h, int level, const feat_stokes_level_dev_t* out);
+
 /* ---------------- autotuner ----------------
  * Launch shapes of the SELL-64 row kernels that every fused entry point uses
  * (entries per row in flight: 4, 8 or 16 -- pair-dictionary plans: 4, 8 or 16

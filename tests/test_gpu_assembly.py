@@ -74,3 +74,37 @@ def test_mgpcg_device_vs_host_assembly(feat, torch_gpu, dim, lf, lc):
     assert rd["iters"] == rh["iters"]
     assert np.array_equal(rd["hist"].view(np.uint64), rh["hist"].view(np.uint64))
     assert np.array_equal(xd.view(np.uint64), xh.view(np.uint64))
+
+
+def _csr_host_transpose(val, col, ptr, rows, cols):
+    return csr_transpose(Csr(rows, cols, val, col, ptr))
+
+
+@pytest.mark.parametrize("level", [0, 1, 2, 4, 6, 7])
+def test_stokes_device_assembly_bitwise(feat, handle, torch_gpu, level):
+    """Q2/P1dc level: A, B, D, filters, prolongations bit-identical to the host assembly,
+    restrictions bit-identical to the host transpose of the prolongations."""
+    print(f"stokes device assembly level={level}", flush=True)
+    S = feat.stokes_problem(level)
+    d = feat.stokes_problem_device(torch_gpu, handle, level)
+    torch_gpu.cuda.synchronize()
+    assert (d["nv"], d["np"]) == (S["nv"], S["np"])
+    u = lambda t: host(t).view(np.uint64)
+    for key in ("A", "B", "D"):
+        for hv, dv in zip(S[key], d[key]):
+            assert np.array_equal(u(dv), hv.view(np.uint64)), key
+    for key in ("filt", "filt_val", "prim", "dual"):
+        assert np.array_equal(u(d[key]), S[key].view(np.uint64)), key
+    if level == 0:
+        assert "Pv" not in d
+        return
+    for key, rows in (("Pv", S["nv"]), ("Pp", S["np"])):
+        hv, hc, hp, cols = S[key]
+        dv, dc, dp, dcols = d[key]
+        assert dcols == cols
+        assert np.array_equal(u(dv), hv.view(np.uint64)) and np.array_equal(u(dc), hc) and np.array_equal(u(dp), hp), key
+        R = _csr_host_transpose(hv, hc, hp, rows, cols)
+        rv, rc, rp, rcols = d["R" + key[1]]
+        assert rcols == rows
+        assert np.array_equal(u(rv), R.val.view(np.uint64)), key
+        assert np.array_equal(u(rc), R.col.astype(np.uint64)) and np.array_equal(u(rp), R.rowptr.astype(np.uint64)), key
