@@ -187,6 +187,15 @@ KERNEL_API = [
 HOST_API = [
     ("feat_host_last_error", C.c_char_p, []),
     ("feat_q1_sizes", _i, [_i, _i, _vp]),
+    ("feat_csr_write_file", _i, [_i, C.c_char_p, _u64, _u64, _vp, _vp, _vp, _i]),
+    ("feat_csr_read_file", _i, [_i, C.c_char_p, C.POINTER(_vp), C.POINTER(_u64), C.POINTER(_u64),
+                                C.POINTER(_u64)]),
+    ("feat_io_fetch_csr", _i, [_vp, _vp, _vp, _vp]),
+    ("feat_dv_write_file", _i, [_i, C.c_char_p, _vp, _u64]),
+    ("feat_dv_read_file", _i, [_i, C.c_char_p, C.POINTER(_vp), C.POINTER(_u64)]),
+    ("feat_io_fetch_dv", _i, [_vp, _vp]),
+    ("feat_io_free", _i, [_vp]),
+    ("feat_mgpcg_write_matrix", _i, [_vp, _i, _i, C.c_char_p, _i]),
     ("feat_q1_assemble", _i, [_i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     ("feat_q1_rhs_sine", _i, [_i, _i, _vp]),
     ("feat_rhs_random", _i, [_u64, _u64, _d, _d, _vp]),
@@ -426,6 +435,60 @@ def q1_problem_device(torch_mod, handle, dim, level, transfer=True):
                                     out["rowptr"].data_ptr(), out["filt"].data_ptr(), *tr),
           "feat_q1_assemble_device")
     return out
+
+
+FILE_MODE = {"exp": 0, "dv": 1, "mtx": 2, "csr": 4, "binary": 13}   # LAFEM FileMode values
+
+
+def _fmode(mode):
+    return FILE_MODE[mode] if isinstance(mode, str) else int(mode)
+
+
+def csr_write_file(mode, path, rows, cols, val, col, rowptr, symmetric=False):
+    """SparseMatrixCSR::write_out of host arrays (mode "csr", "binary" or "mtx")."""
+    _, h = libs()
+    val = np.ascontiguousarray(val, np.float64)
+    col = np.ascontiguousarray(col, np.uint64)
+    rowptr = np.ascontiguousarray(rowptr, np.uint64)
+    check(h.feat_csr_write_file(_fmode(mode), str(path).encode(), rows, cols, val.ctypes.data, col.ctypes.data,
+                                rowptr.ctypes.data, int(symmetric)), "feat_csr_write_file")
+
+
+def csr_read_file(mode, path):
+    """SparseMatrixCSR::read_from: dict(rows, cols, val, col, rowptr) of host arrays."""
+    _, h = libs()
+    f = C.c_void_p()
+    rows, cols, nnz = _u64(), _u64(), _u64()
+    check(h.feat_csr_read_file(_fmode(mode), str(path).encode(), C.byref(f), C.byref(rows), C.byref(cols),
+                               C.byref(nnz)), "feat_csr_read_file")
+    try:
+        val, col = np.empty(nnz.value), np.empty(nnz.value, np.uint64)
+        rp = np.empty(rows.value + 1, np.uint64)
+        check(h.feat_io_fetch_csr(f, val.ctypes.data, col.ctypes.data, rp.ctypes.data), "feat_io_fetch_csr")
+    finally:
+        h.feat_io_free(f)
+    return dict(rows=rows.value, cols=cols.value, val=val, col=col, rowptr=rp)
+
+
+def dv_write_file(mode, path, v):
+    """DenseVector::write_out of a host array (mode "dv", "binary", "mtx" or "exp")."""
+    _, h = libs()
+    v = np.ascontiguousarray(v, np.float64)
+    check(h.feat_dv_write_file(_fmode(mode), str(path).encode(), v.ctypes.data, v.size), "feat_dv_write_file")
+
+
+def dv_read_file(mode, path):
+    """DenseVector::read_from: host array."""
+    _, h = libs()
+    f = C.c_void_p()
+    n = _u64()
+    check(h.feat_dv_read_file(_fmode(mode), str(path).encode(), C.byref(f), C.byref(n)), "feat_dv_read_file")
+    try:
+        v = np.empty(n.value)
+        check(h.feat_io_fetch_dv(f, v.ctypes.data), "feat_io_fetch_dv")
+    finally:
+        h.feat_io_free(f)
+    return v
 
 
 def q1_rhs_sine(dim, level, n):
@@ -823,6 +886,11 @@ class MgPcg:
         it = int(res.iters)
         return dict(status=STATUS[res.status], iters=it, def_init=res.def_init,
                     def_final=res.def_final, hist=hist[:it + 1])
+
+    def write_matrix(self, lvl, mode, path, symmetric=False):
+        """SparseMatrixCSR::write_out of level lvl's device-resident matrix."""
+        check(self.hl.feat_mgpcg_write_matrix(self.s, lvl, _fmode(mode), str(path).encode(), int(symmetric)),
+              "feat_mgpcg_write_matrix")
 
     def matrix(self, lvl=0):
         rows, nnz = _u64(), _u64()

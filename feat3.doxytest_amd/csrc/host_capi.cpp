@@ -9,6 +9,7 @@
 #include <feat3amd/solver.hpp>
 #include <feat3amd/stokes.hpp>
 #include <feat3amd/stokes_assembly.hpp>
+#include <feat3amd/io.hpp>
 #include <feat3amd_host.h>
 
 #include <chrono>
@@ -162,6 +163,87 @@ int feat_stokes_assemble(int level, const feat_stokes_arrays_t* o)
       put(o->pp_val, S.Pp.val); put(o->pp_col, S.Pp.col); put(o->pp_ptr, S.Pp.rowptr);
     }
   });
+}
+
+struct feat_io_s
+{
+  Assembly::CsrHost A;
+  std::vector<double> v;
+  bool is_matrix = false;
+};
+
+int feat_csr_write_file(int mode, const char* path, std::uint64_t rows, std::uint64_t cols, const double* val,
+                        const std::uint64_t* col, const std::uint64_t* rowptr, int symmetric)
+{
+  return guarded([&]() {
+    if(!path || !rowptr) throw Error("feat_csr_write_file: null argument");
+    Assembly::CsrHost A;
+    A.rows = rows;
+    A.cols = cols;
+    A.rowptr.assign(rowptr, rowptr + rows + 1);
+    const std::uint64_t nnz = A.rowptr[rows];
+    if(nnz && (!val || !col)) throw Error("feat_csr_write_file: null arrays");
+    A.val.assign(val, val + nnz);
+    A.col.assign(col, col + nnz);
+    LAFEM::write_csr(LAFEM::FileMode(mode), std::string(path), A, symmetric != 0);
+  });
+}
+
+int feat_csr_read_file(int mode, const char* path, feat_io_t* out, std::uint64_t* rows, std::uint64_t* cols,
+                       std::uint64_t* nnz)
+{
+  return guarded([&]() {
+    if(!path || !out) throw Error("feat_csr_read_file: null argument");
+    auto f = std::make_unique<feat_io_s>();
+    f->A = LAFEM::read_csr(LAFEM::FileMode(mode), std::string(path));
+    f->is_matrix = true;
+    if(rows) *rows = f->A.rows;
+    if(cols) *cols = f->A.cols;
+    if(nnz) *nnz = f->A.val.size();
+    *out = f.release();
+  });
+}
+
+int feat_io_fetch_csr(feat_io_t f, double* val, std::uint64_t* col, std::uint64_t* rowptr)
+{
+  return guarded([&]() {
+    if(!f || !f->is_matrix) throw Error("feat_io_fetch_csr: not a matrix");
+    if(val) std::copy(f->A.val.begin(), f->A.val.end(), val);
+    if(col) std::copy(f->A.col.begin(), f->A.col.end(), col);
+    if(rowptr) std::copy(f->A.rowptr.begin(), f->A.rowptr.end(), rowptr);
+  });
+}
+
+int feat_dv_write_file(int mode, const char* path, const double* v, std::uint64_t n)
+{
+  return guarded([&]() {
+    if(!path || (n && !v)) throw Error("feat_dv_write_file: null argument");
+    LAFEM::write_dv(LAFEM::FileMode(mode), std::string(path), v, n);
+  });
+}
+
+int feat_dv_read_file(int mode, const char* path, feat_io_t* out, std::uint64_t* n)
+{
+  return guarded([&]() {
+    if(!path || !out) throw Error("feat_dv_read_file: null argument");
+    auto f = std::make_unique<feat_io_s>();
+    f->v = LAFEM::read_dv(LAFEM::FileMode(mode), std::string(path));
+    if(n) *n = f->v.size();
+    *out = f.release();
+  });
+}
+
+int feat_io_fetch_dv(feat_io_t f, double* v)
+{
+  return guarded([&]() {
+    if(!f || f->is_matrix) throw Error("feat_io_fetch_dv: not a vector");
+    if(v) std::copy(f->v.begin(), f->v.end(), v);
+  });
+}
+
+int feat_io_free(feat_io_t f)
+{
+  return guarded([&]() { delete f; });
 }
 
 int feat_rhs_random(std::uint64_t seed, std::uint64_t n, double a, double b, double* out)
@@ -361,6 +443,14 @@ int feat_mgpcg_solve(feat_mgpcg_t s, double* x, const double* b, feat_solve_resu
       const auto& h = s->pcg->history();
       for(std::size_t i = 0; i < h.size() && i < hist_cap; ++i) hist[i] = h[i];
     }
+  });
+}
+
+int feat_mgpcg_write_matrix(feat_mgpcg_t s, int lvl, int mode, const char* path, int symmetric)
+{
+  return guarded([&]() {
+    if(!s || !path || lvl < 0 || lvl >= int(s->levels.size())) throw Error("feat_mgpcg_write_matrix: bad argument");
+    LAFEM::write_out(s->levels[lvl]->A, LAFEM::FileMode(mode), path, symmetric != 0);
   });
 }
 

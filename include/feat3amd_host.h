@@ -25,6 +25,23 @@ int feat_q1_assemble(int dim, int level, double* val, uint64_t* col, uint64_t* r
 int feat_q1_rhs_sine(int dim, int level, double* b);
 int feat_rhs_random(uint64_t seed, uint64_t n, double a, double b, double* out);
 
+/* ---- LAFEM file I/O (SparseMatrixCSR / DenseVector read_from, write_out) ----
+ * mode = FileMode (lafem/base.hpp:31-47): 0 fm_exp, 1 fm_dv, 2 fm_mtx, 4 fm_csr, 13 fm_binary.
+ * Matrices: fm_csr / fm_binary (container image, exact), fm_mtx (MatrixMarket coordinate,
+ * symmetric != 0 writes the lower triangle as "symmetric"); sparse_matrix_csr.hpp:877-1087.
+ * Vectors: fm_dv / fm_binary, fm_mtx (array n x 1), fm_exp; dense_vector.hpp:549-723.
+ * Host arrays.  A read returns an object holding the arrays: query the sizes, fetch, free. */
+typedef struct feat_io_s* feat_io_t;
+int feat_csr_write_file(int mode, const char* path, uint64_t rows, uint64_t cols, const double* val,
+                        const uint64_t* col, const uint64_t* rowptr, int symmetric);
+int feat_csr_read_file(int mode, const char* path, feat_io_t* out, uint64_t* rows, uint64_t* cols,
+                       uint64_t* nnz);
+int feat_io_fetch_csr(feat_io_t f, double* val, uint64_t* col, uint64_t* rowptr);
+int feat_dv_write_file(int mode, const char* path, const double* v, uint64_t n);
+int feat_dv_read_file(int mode, const char* path, feat_io_t* out, uint64_t* n);
+int feat_io_fetch_dv(feat_io_t f, double* v);
+int feat_io_free(feat_io_t f);
+
 /* ---- Q2/P1dc Stokes levels on the unit square (lid-driven cavity) ----
  * Problem setup of benchmarks/stokes_multigrid-bench.cpp:576-697 (velocity
  * blocks A = DuDv, B = -grad, D = B^T, UnitFilterBlocked velocity boundary,
@@ -100,6 +117,8 @@ int feat_mgpcg_create_q1_ex(feat_mgpcg_t* s, int dim, int lf, int lc, const feat
 int feat_mgpcg_solve(feat_mgpcg_t s, double* x, const double* b, feat_solve_result_t* res,
                      double* hist, uint64_t hist_cap);
 /* device descriptor of level `lvl`'s system matrix (0 = finest) */
+/* SparseMatrixCSR::write_out of level lvl's (device-resident) system matrix */
+int feat_mgpcg_write_matrix(feat_mgpcg_t s, int lvl, int mode, const char* path, int symmetric);
 int feat_mgpcg_matrix(feat_mgpcg_t s, int lvl, uint64_t* rows, uint64_t* nnz, const double** val,
                       const uint64_t** col, const uint64_t** rowptr);
 int feat_mgpcg_destroy(feat_mgpcg_t s);

@@ -553,6 +553,11 @@ class Reference:
                                        C.c_double, C.c_double, C.c_double, C.c_uint64, C.c_int,
                                        _f64p, _f64p, _vp, C.c_uint64, C.POINTER(C.c_uint64),
                                        C.POINTER(C.c_double)]
+        L.ref_csr_write.argtypes = [C.c_int, C.c_char_p, C.POINTER(_CsrS), C.c_int]
+        L.ref_csr_read.argtypes = [C.c_int, C.c_char_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
+                                   C.POINTER(C.c_uint64), _vp, _vp, _vp, C.c_uint64]
+        L.ref_dv_write.argtypes = [C.c_int, C.c_char_p, _vp, C.c_uint64]
+        L.ref_dv_read.argtypes = [C.c_int, C.c_char_p, C.POINTER(C.c_uint64), _vp, C.c_uint64]
         L.ref_q1_create.restype = _vp
         L.ref_q1_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_char_p]
         L.ref_q1_sizes.argtypes = [_vp, _u64p]
@@ -632,6 +637,37 @@ class Reference:
                                         hist.size, C.byref(it), C.byref(dfin), C.byref(secs))
         return dict(status=STATUS[st], iters=int(it.value), x=x, hist=hist[:it.value + 1],
                     def_final=dfin.value, seconds=secs.value)
+
+    def csr_write(self, mode, path, A, symmetric=False):
+        """SparseMatrixCSR::write_out (mode: the reference's FileMode value)."""
+        val = np.ascontiguousarray(A.val, np.float64)
+        col = np.ascontiguousarray(A.col, np.uint64)
+        rp = np.ascontiguousarray(A.rowptr, np.uint64)
+        m = _CsrS(A.rows, A.cols, val.size, val.ctypes.data, col.ctypes.data, rp.ctypes.data)
+        self.L.ref_csr_write(mode, str(path).encode(), C.byref(m), int(symmetric))
+
+    def csr_read(self, mode, path):
+        """SparseMatrixCSR::read_from: Csr of host arrays."""
+        rows, cols, nnz = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        self.L.ref_csr_read(mode, str(path).encode(), C.byref(rows), C.byref(cols), C.byref(nnz),
+                            None, None, None, 0)
+        cap = max(nnz.value, rows.value + 1)
+        val, col, rp = np.empty(cap), np.empty(cap, np.uint64), np.empty(cap, np.uint64)
+        self.L.ref_csr_read(mode, str(path).encode(), C.byref(rows), C.byref(cols), C.byref(nnz),
+                            val.ctypes.data, col.ctypes.data, rp.ctypes.data, cap)
+        return Csr(rows.value, cols.value, val[:nnz.value].copy(), col[:nnz.value].copy(),
+                   rp[:rows.value + 1].copy())
+
+    def dv_write(self, mode, path, v):
+        v = np.ascontiguousarray(v, np.float64)
+        self.L.ref_dv_write(mode, str(path).encode(), v.ctypes.data, v.size)
+
+    def dv_read(self, mode, path):
+        n = C.c_uint64()
+        self.L.ref_dv_read(mode, str(path).encode(), C.byref(n), None, 0)
+        v = np.empty(max(n.value, 1))
+        self.L.ref_dv_read(mode, str(path).encode(), C.byref(n), v.ctypes.data, v.size)
+        return v[:n.value].copy()
 
     def csr_apply(self, A, x, y=None, a=1.0, b=0.0, transposed=False):
         nr = A.cols if transposed else A.rows

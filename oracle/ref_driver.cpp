@@ -885,3 +885,59 @@ int ref_stokes_mg_solve(const ref_stokes_level_t* L, int nl, int steps, double o
 }
 
 } // extern "C"
+
+// ---------------- LAFEM file I/O (SparseMatrixCSR / DenseVector read_from, write_out) ----------------
+// mode is the reference's FileMode value (lafem/base.hpp:31-47).  Valid inputs only: the
+// reference aborts the process on a malformed file.
+extern "C" {
+
+int ref_csr_write(int mode, const char* path, const ref_csr_t* m, int symmetric)
+{
+  ensure_runtime();
+  Csr a = make_csr(*m);
+  a.write_out(LAFEM::FileMode(mode), String(path), symmetric != 0);
+  return 0;
+}
+
+// First call with cap 0 for the sizes; a second with cap >= nnz (and rows + 1) copies the arrays.
+int ref_csr_read(int mode, const char* path, std::uint64_t* rows, std::uint64_t* cols, std::uint64_t* nnz,
+                 double* val, std::uint64_t* col, std::uint64_t* rowptr, std::uint64_t cap)
+{
+  ensure_runtime();
+  Csr a;
+  a.read_from(LAFEM::FileMode(mode), String(path));
+  *rows = a.rows();
+  *cols = a.columns();
+  *nnz = a.used_elements();
+  if(cap >= a.used_elements() && cap >= a.rows() + 1)
+  {
+    for(Index i = 0; i < a.used_elements(); ++i)
+    {
+      val[i] = a.val()[i];
+      col[i] = a.col_ind()[i];
+    }
+    for(Index i = 0; i <= a.rows(); ++i) rowptr[i] = a.row_ptr()[i];
+  }
+  return 0;
+}
+
+int ref_dv_write(int mode, const char* path, const double* v, std::uint64_t n)
+{
+  ensure_runtime();
+  Vec x = make_vec(v, n);
+  x.write_out(LAFEM::FileMode(mode), String(path));
+  return 0;
+}
+
+int ref_dv_read(int mode, const char* path, std::uint64_t* n, double* out, std::uint64_t cap)
+{
+  ensure_runtime();
+  Vec x;
+  x.read_from(LAFEM::FileMode(mode), String(path));
+  *n = x.size();
+  if(cap >= x.size())
+    for(Index i = 0; i < x.size(); ++i) out[i] = x.elements()[i];
+  return 0;
+}
+
+} // extern "C"
