@@ -138,6 +138,13 @@ KERNEL_API = [
     ("feat_sell_destroy_f32", _i, [C.POINTER(Matrix)]),
     ("feat_matrix_apply_f64", _i, [_vp, C.POINTER(Matrix), _vp, _d, _vp, _d, _vp]),
     ("feat_q1_device_sizes", _i, [_i, _i, _vp]),
+    ("feat_sor_init_symbolic", _i, [_vp, C.POINTER(CsrF64), _i, C.POINTER(_vp)]),
+    ("feat_sor_init_numeric", _i, [_vp, _vp, C.POINTER(CsrF64)]),
+    ("feat_sor_colors", _i, [_vp, C.POINTER(_i), _vp, _vp]),
+    ("feat_sor_apply", _i, [_vp, _vp, _vp, _vp, _d]),
+    ("feat_ssor_forward_apply", _i, [_vp, _vp, _vp, _vp, _d]),
+    ("feat_ssor_backward_apply", _i, [_vp, _vp, _vp, _d]),
+    ("feat_sor_done_symbolic", _i, [_vp]),
     ("feat_q1_assemble_device", _i, [_vp, _i, _i] + [_vp] * 10),
     ("feat_stokes_device_sizes", _i, [_i, _vp]),
     ("feat_stokes_assemble_device", _i, [_vp, _i, _vp]),
@@ -489,6 +496,53 @@ def dv_read_file(mode, path):
     finally:
         h.feat_io_free(f)
     return v
+
+
+SOR_ROW_COLORS = 1
+
+
+class SorPlan:
+    """Multicolour SOR / SSOR plan of a device CSR matrix (feat_sor_*)."""
+
+    def __init__(self, handle, desc, row_colors=False):
+        self.k, _ = libs()
+        self.h = handle
+        self.desc = desc
+        self.s = C.c_void_p()
+        check(self.k.feat_sor_init_symbolic(handle.h, C.byref(desc), SOR_ROW_COLORS if row_colors else 0,
+                                            C.byref(self.s)), "feat_sor_init_symbolic")
+        nc = C.c_int()
+        check(self.k.feat_sor_colors(self.s, C.byref(nc), None, None), "feat_sor_colors")
+        self.ncolors = nc.value
+
+    def colors(self):
+        """(rank of each row's colour, rows per colour in processing order)."""
+        rank = np.empty(self.desc.rows, np.uint32)
+        rows = np.empty(max(self.ncolors, 1), np.uint64)
+        check(self.k.feat_sor_colors(self.s, None, rank.ctypes.data, rows.ctypes.data), "feat_sor_colors")
+        return rank, rows[:self.ncolors]
+
+    def init_numeric(self):
+        check(self.k.feat_sor_init_numeric(self.h.h, self.s, C.byref(self.desc)), "feat_sor_init_numeric")
+
+    def sor(self, y, x, omega):
+        check(self.k.feat_sor_apply(self.h.h, self.s, ptr(y), ptr(x), omega), "feat_sor_apply")
+
+    def ssor(self, y, x, omega):
+        """forward and backward sweeps (without the omega (2 - omega) scaling)"""
+        check(self.k.feat_ssor_forward_apply(self.h.h, self.s, ptr(y), ptr(x), omega), "feat_ssor_forward_apply")
+        check(self.k.feat_ssor_backward_apply(self.h.h, self.s, ptr(y), omega), "feat_ssor_backward_apply")
+
+    def close(self):
+        if self.s:
+            self.k.feat_sor_done_symbolic(self.s)
+            self.s = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def q1_rhs_sine(dim, level, n):

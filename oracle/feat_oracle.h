@@ -202,6 +202,11 @@ void orc_mlp_features(uint64_t m, const double* r, const double* e, const double
                       const uint64_t* node, uint64_t nsamp, uint16_t* X, float* t);
 void orc_mlp_smooth(uint64_t m, const float* p, const double* r, const double* dinv, const float* cfeat,
                     double omega, double jacobi, double* x);
+/* ---- multicolour SOR / SSOR (solver/sor_precond.cu, ssor_precond.cu semantics) ---- */
+uint32_t orc_color_ranks(uint64_t n, const uint64_t* rowptr, const uint64_t* col, int row_colors, uint32_t* rank);
+void orc_mcsor_apply(uint64_t n, const uint64_t* rowptr, const uint64_t* col, const double* val,
+                     const uint32_t* rank, uint32_t ncolors, double omega, int ssor, const double* x, double* y);
+
 #ifdef __cplusplus
 }
 #endif

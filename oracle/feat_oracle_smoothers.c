@@ -1,0 +1,114 @@
+/*
+ * feat_oracle_smoothers.c -- CPU oracle for the multicolour SOR / SSOR sweeps and
+ * ILU(0).  TEST INFRASTRUCTURE ONLY (see feat_oracle.h).
+ *
+ * Multicolour SOR / SSOR: the semantics of the reference's CUDA backend
+ * (solver/sor_precond.cu:20-33 and 280-300, solver/ssor_precond.cu:25-57 and
+ * 182-235), restated for a given colouring: colours processed in ascending order of
+ * their row count (ties by colour number, the std::sort of (count, colour) pairs at
+ * sor_precond.cu:207-218), y cleared first, every lower entry read (a row of a colour
+ * not yet processed holds y = 0), the SSOR backward sweep over the colours in the
+ * same order.  The colouring restates the product's rule (greedy first fit in row
+ * order over the symmetrised pattern) so that tests can check it; the reference
+ * takes cusparseDcsrcolor's, which is closed.
+ */
+#include "feat_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* rank[i] = processing rank of row i's colour; returns the number of colours */
+uint32_t orc_color_ranks(uint64_t n, const uint64_t* rowptr, const uint64_t* col, int row_colors, uint32_t* rank)
+{
+  uint32_t* color = (uint32_t*)malloc(sizeof(uint32_t) * (n ? n : 1));
+  uint32_t ncol = 0;
+  if(row_colors)
+  {
+    for(uint64_t i = 0; i < n; ++i) color[i] = (uint32_t)i;
+    ncol = (uint32_t)n;
+  }
+  else
+  {
+    /* neighbour j < i of row i: a_ij != 0 or a_ji != 0 (scan of all rows j < i, O(nnz * n)
+       is too slow; use the explicit transposed lists) */
+    uint64_t* tcnt = (uint64_t*)calloc(n + 1, sizeof(uint64_t));
+    for(uint64_t k = 0; k < n; ++k)
+      for(uint64_t e = rowptr[k]; e < rowptr[k + 1]; ++e)
+        if(col[e] > k) ++tcnt[col[e] + 1];
+    for(uint64_t i = 0; i < n; ++i) tcnt[i + 1] += tcnt[i];
+    uint64_t* tk = (uint64_t*)malloc(sizeof(uint64_t) * (tcnt[n] ? tcnt[n] : 1));
+    uint64_t* fill = (uint64_t*)malloc(sizeof(uint64_t) * (n ? n : 1));
+    memcpy(fill, tcnt, sizeof(uint64_t) * n);
+    for(uint64_t k = 0; k < n; ++k)
+      for(uint64_t e = rowptr[k]; e < rowptr[k + 1]; ++e)
+        if(col[e] > k) tk[fill[col[e]]++] = k;
+    uint8_t* used = (uint8_t*)calloc(n + 1, 1);
+    for(uint64_t i = 0; i < n; ++i)
+    {
+      for(uint64_t e = rowptr[i]; e < rowptr[i + 1]; ++e)
+        if(col[e] < i) used[color[col[e]]] = 1;
+      for(uint64_t e = tcnt[i]; e < tcnt[i + 1]; ++e) used[color[tk[e]]] = 1;
+      uint32_t c = 0;
+      while(used[c]) ++c;
+      color[i] = c;
+      if(c + 1 > ncol) ncol = c + 1;
+      for(uint64_t e = rowptr[i]; e < rowptr[i + 1]; ++e)
+        if(col[e] < i) used[color[col[e]]] = 0;
+      for(uint64_t e = tcnt[i]; e < tcnt[i + 1]; ++e) used[color[tk[e]]] = 0;
+    }
+    free(used);
+    free(fill);
+    free(tk);
+    free(tcnt);
+  }
+  /* order: ascending (count, colour) -- selection by repeated minimum */
+  uint64_t* cnt = (uint64_t*)calloc(ncol ? ncol : 1, sizeof(uint64_t));
+  for(uint64_t i = 0; i < n; ++i) ++cnt[color[i]];
+  uint32_t* crank = (uint32_t*)malloc(sizeof(uint32_t) * (ncol ? ncol : 1));
+  uint8_t* done = (uint8_t*)calloc(ncol ? ncol : 1, 1);
+  uint32_t nr = 0;
+  for(uint32_t r = 0; r < ncol; ++r)
+  {
+    int64_t best = -1;
+    for(uint32_t c = 0; c < ncol; ++c)
+      if(!done[c] && cnt[c] && (best < 0 || cnt[c] < cnt[best])) best = c;
+    if(best < 0) break;
+    done[best] = 1;
+    crank[best] = nr++;
+  }
+  for(uint64_t i = 0; i < n; ++i) rank[i] = crank[color[i]];
+  free(done);
+  free(crank);
+  free(cnt);
+  free(color);
+  return nr;
+}
+
+/* y = M^{-1} x: ssor == 0: SOR (omega (x - L y) / a), ssor != 0: forward, backward, scale */
+void orc_mcsor_apply(uint64_t n, const uint64_t* rowptr, const uint64_t* col, const double* val,
+                     const uint32_t* rank, uint32_t ncolors, double omega, int ssor, const double* x, double* y)
+{
+  for(uint64_t i = 0; i < n; ++i) y[i] = 0.0;
+  for(uint32_t c = 0; c < ncolors; ++c)
+    for(uint64_t i = 0; i < n; ++i)
+    {
+      if(rank[i] != c) continue;
+      double d = 0.0;
+      uint64_t k = rowptr[i];
+      for(; col[k] < i; ++k) d = fma(val[k], y[col[k]], d);
+      y[i] = ssor ? (x[i] - omega * d) / val[k] : omega * (x[i] - d) / val[k];
+    }
+  if(!ssor) return;
+  for(uint32_t c = 0; c < ncolors; ++c)
+    for(uint64_t i = 0; i < n; ++i)
+    {
+      if(rank[i] != c) continue;
+      double d = 0.0;
+      uint64_t k = rowptr[i + 1] - 1;
+      for(; col[k] > i; --k) d = fma(val[k], y[col[k]], d);
+      y[i] -= omega * d / val[k];
+    }
+  const double s = omega * (2.0 - omega);
+  for(uint64_t i = 0; i < n; ++i) y[i] = y[i] * s;
+}

@@ -364,6 +364,31 @@ class Oracle:
         return np.array([self.L.orc_rng_next(C.byref(s)) for _ in range(k)], dtype=np.uint64)
 
     # -- kernels (all return new arrays; inputs untouched)
+    def color_ranks(self, A, row_colors=False):
+        """Processing rank of each row's colour (greedy first fit, ascending row count)."""
+        L = self.L
+        L.orc_color_ranks.restype = C.c_uint32
+        L.orc_color_ranks.argtypes = [C.c_uint64, _vp, _vp, C.c_int, _vp]
+        rank = np.empty(A.rows, np.uint32)
+        rp = np.ascontiguousarray(A.rowptr, np.uint64)
+        col = np.ascontiguousarray(A.col, np.uint64)
+        nc = L.orc_color_ranks(A.rows, rp.ctypes.data, col.ctypes.data, int(row_colors), rank.ctypes.data)
+        return rank, int(nc)
+
+    def mcsor_apply(self, A, rank, ncolors, omega, x, ssor=False):
+        """y = M^{-1} x of the multicolour SOR (ssor=False) or SSOR sweep."""
+        L = self.L
+        L.orc_mcsor_apply.argtypes = [C.c_uint64, _vp, _vp, _vp, _vp, C.c_uint32, C.c_double, C.c_int, _vp, _vp]
+        rp = np.ascontiguousarray(A.rowptr, np.uint64)
+        col = np.ascontiguousarray(A.col, np.uint64)
+        val = np.ascontiguousarray(A.val, np.float64)
+        rank = np.ascontiguousarray(rank, np.uint32)
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.empty(A.rows)
+        L.orc_mcsor_apply(A.rows, rp.ctypes.data, col.ctypes.data, val.ctypes.data, rank.ctypes.data, ncolors,
+                          omega, int(ssor), x.ctypes.data, y.ctypes.data)
+        return y
+
     def csr_apply(self, A, x, y=None, a=1.0, b=0.0, transposed=False, r=None):
         nr = A.cols if transposed else A.rows
         if y is None:
@@ -637,6 +662,20 @@ class Reference:
                                         hist.size, C.byref(it), C.byref(dfin), C.byref(secs))
         return dict(status=STATUS[st], iters=int(it.value), x=x, hist=hist[:it.value + 1],
                     def_final=dfin.value, seconds=secs.value)
+
+    def precond_apply(self, kind, A, omega, x):
+        """y = M^{-1} x of the reference's generic SORPrecond ("sor"), SSORPrecond ("ssor")
+        or ILUPrecond ("ilu", level p = omega)."""
+        L = self.L
+        L.ref_precond_apply.argtypes = [C.c_int, C.POINTER(_CsrS), C.c_double, _vp, _vp]
+        val = np.ascontiguousarray(A.val, np.float64)
+        col = np.ascontiguousarray(A.col, np.uint64)
+        rp = np.ascontiguousarray(A.rowptr, np.uint64)
+        m = _CsrS(A.rows, A.cols, val.size, val.ctypes.data, col.ctypes.data, rp.ctypes.data)
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.empty(A.rows)
+        L.ref_precond_apply({"sor": 0, "ssor": 1, "ilu": 2}[kind], C.byref(m), omega, x.ctypes.data, y.ctypes.data)
+        return y
 
     def csr_write(self, mode, path, A, symmetric=False):
         """SparseMatrixCSR::write_out (mode: the reference's FileMode value)."""

@@ -53,6 +53,9 @@
 #include <kernel/lafem/unit_filter_blocked.hpp>
 #include <kernel/lafem/mean_filter.hpp>
 #include <kernel/solver/amavanka.hpp>
+#include <kernel/solver/sor_precond.hpp>
+#include <kernel/solver/ssor_precond.hpp>
+#include <kernel/solver/ilu_precond.hpp>
 
 #include <chrono>
 #include <deque>
@@ -937,6 +940,34 @@ int ref_dv_read(int mode, const char* path, std::uint64_t* n, double* out, std::
   *n = x.size();
   if(cap >= x.size())
     for(Index i = 0; i < x.size(); ++i) out[i] = x.elements()[i];
+  return 0;
+}
+
+} // extern "C"
+
+// ---------------- generic SOR / SSOR / ILU(p) preconditioners (CPU backend) ----------------
+// kind 0: SORPrecond, 1: SSORPrecond (omega), 2: ILUPrecond (level p = int(omega));
+// y = M^{-1} x with the empty unit filter.
+extern "C" {
+
+int ref_precond_apply(int kind, const ref_csr_t* m, double omega, const double* x, double* y)
+{
+  ensure_runtime();
+  Csr A = make_csr(*m);
+  UFilter F(m->rows);
+  std::shared_ptr<Solver::SolverBase<Vec>> pre;
+  if(kind == 0)
+    pre = Solver::new_sor_precond(PreferredBackend::generic, A, F, omega);
+  else if(kind == 1)
+    pre = Solver::new_ssor_precond(PreferredBackend::generic, A, F, omega);
+  else
+    pre = Solver::new_ilu_precond(PreferredBackend::generic, A, F, int(omega));
+  pre->init();
+  Vec vx = make_vec(x, m->rows);
+  Vec vy(m->rows, 0.0);
+  pre->apply(vy, vx);
+  std::memcpy(y, vy.elements(), sizeof(double) * m->rows);
+  pre->done();
   return 0;
 }
 

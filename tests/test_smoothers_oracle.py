@@ -1,0 +1,113 @@
+"""The multicolour SOR / SSOR oracle, pinned.
+
+* With one colour per row (processed in row order) the multicolour SOR sweep is the
+  reference's generic sequential SOR: compared with the reference's own
+  SORPrecond<generic> (oracle/_ref), exactly.
+* For a real colouring the oracle's SOR / SSOR equal the closed forms
+      SOR:  y = (D/omega + L_e)^{-1} x
+      SSOR: y = omega (2 - omega) (D + omega U_e)^{-1} (D - omega U_l) (D + omega L_e)^{-1} x
+  with L_e / U_e the lower / upper entries whose colour is processed earlier and U_l the
+  upper entries of colours processed later (dense solves, relative 1e-12).
+* The colouring is proper (no coupling inside a colour) and ordered by ascending row count.
+"""
+import numpy as np
+import pytest
+
+from oracle.pyoracle import Csr
+from tests.helpers import random_csr
+
+
+def dominant(rng, n, max_len=9):
+    """Random sorted CSR with a dominant positive diagonal."""
+    A = random_csr(rng, n, n, min_len=1, max_len=max_len)
+    rp, col, val = [0], [], []
+    for r in range(n):
+        c = set(int(x) for x in A.col[int(A.rowptr[r]):int(A.rowptr[r + 1])]) | {r}
+        c = sorted(c)
+        v = rng.standard_normal(len(c))
+        v[c.index(r)] = 2.0 + np.abs(v).sum()
+        col.extend(c)
+        val.extend(v.tolist())
+        rp.append(len(col))
+    return Csr(n, n, np.array(val), np.array(col, np.uint64), np.array(rp, np.uint64))
+
+
+def dense(A):
+    M = np.zeros((A.rows, A.cols))
+    for r in range(A.rows):
+        for k in range(int(A.rowptr[r]), int(A.rowptr[r + 1])):
+            M[r, int(A.col[k])] = A.val[k]
+    return M
+
+
+def q1_csr(feat, dim, level):
+    q = feat.q1_problem(dim, level)
+    return Csr(q["n"], q["n"], q["val"], q["col"], q["rowptr"])
+
+
+@pytest.mark.parametrize("seed,n", [(1, 40), (2, 200)])
+def test_row_colors_sor_is_reference_generic_sor(oracle, reference, seed, n):
+    rng = np.random.default_rng(seed)
+    A = dominant(rng, n)
+    x = rng.standard_normal(n)
+    rank, nc = oracle.color_ranks(A, row_colors=True)
+    assert nc == n and np.array_equal(rank, np.arange(n))
+    for omega in (1.0, 0.7):
+        y = oracle.mcsor_apply(A, rank, nc, omega, x)
+        assert np.array_equal(y, reference.precond_apply("sor", A, omega, x))
+
+
+def test_row_colors_sor_q1(feat, oracle, reference):
+    A = q1_csr(feat, 2, 3)
+    x = np.random.default_rng(3).standard_normal(A.rows)
+    rank, nc = oracle.color_ranks(A, row_colors=True)
+    assert np.array_equal(oracle.mcsor_apply(A, rank, nc, 1.2, x), reference.precond_apply("sor", A, 1.2, x))
+
+
+def check_coloring(A, rank, nc):
+    counts = np.bincount(rank, minlength=nc)
+    assert np.all(counts > 0) and np.all(np.diff(counts) >= 0)      # ascending row counts
+    for r in range(A.rows):
+        for k in range(int(A.rowptr[r]), int(A.rowptr[r + 1])):
+            c = int(A.col[k])
+            assert c == r or rank[c] != rank[r]
+
+
+@pytest.mark.parametrize("case", ["random", "q1_2d", "q1_3d"])
+def test_multicolor_closed_forms(feat, oracle, case):
+    rng = np.random.default_rng(17)
+    A = dominant(rng, 120) if case == "random" else q1_csr(feat, 2 if case == "q1_2d" else 3, 3 if case == "q1_2d" else 2)
+    rank, nc = oracle.color_ranks(A)
+    check_coloring(A, rank, nc)
+    if case == "q1_3d":
+        assert nc == 8
+    if case == "q1_2d":
+        assert nc == 4
+    M = dense(A)
+    n = A.rows
+    i, j = np.indices((n, n))
+    D = np.diag(np.diag(M))
+    earlier = rank[j] < rank[i]
+    Le = np.where((j < i) & earlier, M, 0.0)
+    Ue = np.where((j > i) & earlier, M, 0.0)
+    Ul = np.where((j > i) & ~earlier & (rank[j] != rank[i]), M, 0.0)
+    x = rng.standard_normal(n)
+    for omega in (1.0, 1.3):
+        y = oracle.mcsor_apply(A, rank, nc, omega, x)
+        want = np.linalg.solve(D / omega + Le, x)
+        assert np.linalg.norm(y - want) <= 1e-12 * np.linalg.norm(want)
+        y = oracle.mcsor_apply(A, rank, nc, omega, x, ssor=True)
+        yf = np.linalg.solve(D + omega * Le, x)
+        want = omega * (2 - omega) * np.linalg.solve(D + omega * Ue, (D - omega * Ul) @ yf)
+        assert np.linalg.norm(y - want) <= 1e-12 * np.linalg.norm(want)
+
+
+def test_planted_error_is_caught(oracle):
+    """A colour order reversed (a plausible slip) changes the SOR result."""
+    rng = np.random.default_rng(5)
+    A = dominant(rng, 60)
+    rank, nc = oracle.color_ranks(A)
+    x = rng.standard_normal(A.rows)
+    y = oracle.mcsor_apply(A, rank, nc, 1.0, x)
+    y_rev = oracle.mcsor_apply(A, (nc - 1 - rank).astype(np.uint32), nc, 1.0, x)
+    assert not np.allclose(y, y_rev)
