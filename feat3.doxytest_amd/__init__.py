@@ -140,7 +140,7 @@ KERNEL_API = [
     ("feat_q1_device_sizes", _i, [_i, _i, _vp]),
     ("feat_sor_init_symbolic", _i, [_vp, C.POINTER(CsrF64), _i, C.POINTER(_vp)]),
     ("feat_sor_init_numeric", _i, [_vp, _vp, C.POINTER(CsrF64)]),
-    ("feat_sor_colors", _i, [_vp, C.POINTER(_i), _vp, _vp]),
+    ("feat_sor_colors", _i, [_vp, _i, C.POINTER(_i), _vp, _vp]),
     ("feat_sor_apply", _i, [_vp, _vp, _vp, _vp, _d]),
     ("feat_ssor_forward_apply", _i, [_vp, _vp, _vp, _vp, _d]),
     ("feat_ssor_backward_apply", _i, [_vp, _vp, _vp, _d]),
@@ -203,6 +203,10 @@ HOST_API = [
     ("feat_io_fetch_dv", _i, [_vp, _vp]),
     ("feat_io_free", _i, [_vp]),
     ("feat_mgpcg_write_matrix", _i, [_vp, _i, _i, C.c_char_p, _i]),
+    ("feat_krylov_create", _i, [C.POINTER(_vp), _i, _u64, _vp, _vp, _vp, _vp, _u64, _i, _d, _d, _d, _u64]),
+    ("feat_krylov_solve", _i, [_vp, _vp, _vp, C.POINTER(SolveResult), _vp, _u64]),
+    ("feat_krylov_precond_apply", _i, [_vp, _vp, _vp]),
+    ("feat_krylov_destroy", _i, [_vp]),
     ("feat_q1_assemble", _i, [_i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     ("feat_q1_rhs_sine", _i, [_i, _i, _vp]),
     ("feat_rhs_random", _i, [_u64, _u64, _d, _d, _vp]),
@@ -498,29 +502,33 @@ def dv_read_file(mode, path):
     return v
 
 
-SOR_ROW_COLORS = 1
+SOR_MULTICOLOR = 1
 
 
 class SorPlan:
-    """Multicolour SOR / SSOR plan of a device CSR matrix (feat_sor_*)."""
+    """SOR / SSOR plan of a device CSR matrix (feat_sor_*): level-set order (the generic
+    sequential sweeps exactly) or, with multicolor=True, the CUDA backend's colour order."""
 
-    def __init__(self, handle, desc, row_colors=False):
+    def __init__(self, handle, desc, multicolor=False):
         self.k, _ = libs()
         self.h = handle
         self.desc = desc
         self.s = C.c_void_p()
-        check(self.k.feat_sor_init_symbolic(handle.h, C.byref(desc), SOR_ROW_COLORS if row_colors else 0,
+        check(self.k.feat_sor_init_symbolic(handle.h, C.byref(desc), SOR_MULTICOLOR if multicolor else 0,
                                             C.byref(self.s)), "feat_sor_init_symbolic")
         nc = C.c_int()
-        check(self.k.feat_sor_colors(self.s, C.byref(nc), None, None), "feat_sor_colors")
+        check(self.k.feat_sor_colors(self.s, 0, C.byref(nc), None, None), "feat_sor_colors")
         self.ncolors = nc.value
 
-    def colors(self):
+    def colors(self, backward=False):
         """(rank of each row's colour, rows per colour in processing order)."""
+        nc = C.c_int()
+        check(self.k.feat_sor_colors(self.s, int(backward), C.byref(nc), None, None), "feat_sor_colors")
         rank = np.empty(self.desc.rows, np.uint32)
-        rows = np.empty(max(self.ncolors, 1), np.uint64)
-        check(self.k.feat_sor_colors(self.s, None, rank.ctypes.data, rows.ctypes.data), "feat_sor_colors")
-        return rank, rows[:self.ncolors]
+        rows = np.empty(max(nc.value, 1), np.uint64)
+        check(self.k.feat_sor_colors(self.s, int(backward), None, rank.ctypes.data, rows.ctypes.data),
+              "feat_sor_colors")
+        return rank, rows[:nc.value]
 
     def init_numeric(self):
         check(self.k.feat_sor_init_numeric(self.h.h, self.s, C.byref(self.desc)), "feat_sor_init_numeric")
@@ -892,6 +900,51 @@ class LearnedMG:
     def close(self):
         if self.s:
             self.hl.feat_learned_destroy(self.s)
+            self.s = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+PRECOND = {"none": 0, "jacobi": 1, "sor": 2, "ssor": 3, "ilu": 4, "sor_mc": 5, "ssor_mc": 6}
+
+
+class KrylovSolver:
+    """PCG ("pcg") or Richardson ("richardson") around a device preconditioner
+    ("none", "jacobi", "sor", "ssor", "ilu") of the C++ host layer."""
+
+    def __init__(self, solver, A, filt, precond, omega_pre=1.0, omega_rich=1.0, tol_rel=1e-8, max_iter=1000):
+        _, self.hl = libs()
+        runtime_init()
+        self.s = C.c_void_p()
+        self.n = int(A.rows)
+        val = np.ascontiguousarray(A.val, np.float64)
+        col = np.ascontiguousarray(A.col, np.uint64)
+        rp = np.ascontiguousarray(A.rowptr, np.uint64)
+        f = np.ascontiguousarray(filt, np.uint64)
+        check(self.hl.feat_krylov_create(C.byref(self.s), {"pcg": 0, "richardson": 1}[solver], self.n,
+                                         val.ctypes.data, col.ctypes.data, rp.ctypes.data,
+                                         f.ctypes.data if f.size else None, f.size, PRECOND[precond],
+                                         omega_pre, omega_rich, tol_rel, max_iter), "feat_krylov_create")
+
+    def solve(self, x, b, hist_cap=2000):
+        res = SolveResult()
+        hist = np.full(hist_cap, np.nan)
+        check(self.hl.feat_krylov_solve(self.s, ptr(x), ptr(b), C.byref(res), hist.ctypes.data, hist_cap),
+              "feat_krylov_solve")
+        it = int(res.iters)
+        return dict(status=STATUS[res.status], iters=it, def_init=res.def_init, def_final=res.def_final,
+                    hist=hist[:it + 1])
+
+    def precond(self, cor, d):
+        check(self.hl.feat_krylov_precond_apply(self.s, ptr(cor), ptr(d)), "feat_krylov_precond_apply")
+
+    def close(self):
+        if self.s:
+            self.hl.feat_krylov_destroy(self.s)
             self.s = C.c_void_p()
 
     def __del__(self):

@@ -1,26 +1,31 @@
-// feat3amd_smoothers.hip -- multicolour SOR / SSOR sweeps for CSR matrices.
+// feat3amd_smoothers.hip -- SOR / SSOR sweeps for CSR matrices, rows of a "colour" in
+// parallel, one launch per colour.
 //
-// What the reference's CUDA backend computes for SORPrecond / SSORPrecond
-// (solver/sor_precond.cu:20-33, 280-300; solver/ssor_precond.cu:25-57, 182-235):
-// the rows are coloured so that no two rows of one colour are coupled, the colours
-// are processed in ascending order of their row count (ties: colour number), and
-// within a colour every row is independent:
-//   SOR          y = 0; per colour: y_i = omega (x_i - sum_{j<i} a_ij y_j) / a_ii
-//   SSOR forward y = 0; per colour: y_i = (x_i - omega sum_{j<i} a_ij y_j) / a_ii
-//   SSOR backward, the same colour order: y_i -= omega (sum_{j>i} a_ij y_j) / a_ii
-// (the sums run in the CSR's column order, ascending for j<i, descending for j>i; a
-// lower entry of a colour processed later reads y_j = 0 and is left out here).
-// The colouring is this library's own (greedy first fit in row order on the
-// symmetrised pattern; the reference takes cusparseDcsrcolor's), so the colour
-// classes, and with them the preconditioner, can differ from a CUDA run; the
-// update formulas and the colour order rule are the reference's.
+// Two orderings share the kernels:
+//  * level sets (default): the colours are the level sets of the triangular dependency
+//    graph (forward: lev(i) = 1 + max lev(j) over the lower entries j of row i; backward:
+//    the same over the upper entries), processed in level order.  Every row sees the
+//    final values of all rows it depends on, so the sweeps are exactly the reference's
+//    generic sequential SORPrecond / SSORPrecond (solver/sor_precond.hpp:201-222,
+//    ssor_precond.hpp:193-231), bit for bit (same sums in the same order).
+//  * multicolour (FEAT_SOR_MULTICOLOR): what the reference's CUDA backend computes
+//    (solver/sor_precond.cu:20-33, 280-300; ssor_precond.cu:25-57, 182-235): a proper
+//    colouring (here greedy first fit in row order on the symmetrised pattern; the
+//    reference takes cusparseDcsrcolor's), colours in ascending order of their row count
+//    (ties by colour number), y cleared first so that lower entries of colours not yet
+//    processed read 0 (left out here), the SSOR backward sweep over the colours in the
+//    same order as the forward sweep.  Few launches, but the SSOR so defined is not
+//    symmetric, and PCG around it stalls, as it would around the reference's.
+// Update formulas (both orderings):
+//   SOR          y_i = omega (x_i - sum_{j<i} a_ij y_j) / a_ii
+//   SSOR forward y_i = (x_i - omega sum_{j<i} a_ij y_j) / a_ii
+//   SSOR backward y_i -= omega (sum_{j>i} a_ij y_j) / a_ii
+// (sums in the CSR's column order: ascending for j<i, descending for j>i).
 //
-// Layout: the rows are permuted colour by colour (ascending row number inside a
-// colour); each colour is cut into slices of 64 consecutive positions stored
-// column-major (SELL-64), one slice set for the forward entries (lower entries of
-// earlier colours) and one for the backward entries (all upper entries, descending
-// column order).  One launch per colour and sweep, one lane per row, coalesced
-// value/column loads, gathers of y.
+// Layout: per sweep direction the rows are permuted colour by colour (ascending row
+// number inside a colour) and each colour is cut into slices of 64 consecutive positions
+// stored column-major (SELL-64): values, u32 columns, u16 row lengths.  One lane per row,
+// coalesced value/column loads, gathers of y.
 #include "feat3amd_common.hpp"
 
 #include <algorithm>
@@ -61,7 +66,7 @@ __global__ __launch_bounds__(256) void k_sor_color(uint64_t off, uint64_t cnt, u
   if(MODE == 0)
     y[row] = omega * (x[row] - d) / diag[row];
   else if(MODE == 1)
-    y[row] = (x[row] - omega * d) / diag[row];
+    y[row] = dfma(-omega, d, x[row]) / diag[row];   // x - omega d contracted, as the reference's build does
   else
     y[row] = y[row] - omega * d / diag[row];
 }
@@ -85,16 +90,24 @@ int upload(T*& dst, const std::vector<T>& src)
 
 } // namespace
 
-struct feat_sor_s
+// the rows of one sweep direction, colour by colour
+struct Part
 {
-  uint64_t n = 0, nnz = 0;
   std::vector<uint64_t> color_off;     // positions of each colour, in processing order (ncolors + 1)
   std::vector<uint64_t> slice_base;    // first slice of each colour (ncolors + 1)
   std::vector<uint32_t> rank_of_row;   // processing rank of each row's colour
-  uint32_t* perm = nullptr;            // position -> row
+  std::vector<uint32_t> perm_h;        // position -> row
+  uint32_t* perm = nullptr;
+  SliceSet ss;
+};
+
+struct feat_sor_s
+{
+  uint64_t n = 0, nnz = 0;
+  int flags = 0;
   uint64_t* diag_idx = nullptr;
   double* diag = nullptr;
-  SliceSet fw, bw;
+  Part fw, bw;
 };
 
 namespace {
@@ -112,21 +125,36 @@ void free_slices(SliceSet& s)
 void free_sor(feat_sor_s* s)
 {
   if(!s) return;
-  (void)hipFree(s->perm);
+  (void)hipFree(s->fw.perm);
+  (void)hipFree(s->bw.perm);
   (void)hipFree(s->diag_idx);
   (void)hipFree(s->diag);
-  free_slices(s->fw);
-  free_slices(s->bw);
+  free_slices(s->fw.ss);
+  free_slices(s->bw.ss);
   delete s;
 }
 
+// positions, colour offsets and slice bases of a partition given each row's rank
+void make_part(Part& P, uint64_t n, uint32_t nranks)
+{
+  P.color_off.assign(nranks + 1, 0);
+  for(uint64_t i = 0; i < n; ++i) ++P.color_off[P.rank_of_row[i] + 1];
+  for(uint32_t r = 0; r < nranks; ++r) P.color_off[r + 1] += P.color_off[r];
+  P.perm_h.resize(n);
+  std::vector<uint64_t> pos(P.color_off.begin(), P.color_off.end() - 1);
+  for(uint64_t i = 0; i < n; ++i) P.perm_h[pos[P.rank_of_row[i]]++] = uint32_t(i);
+  P.slice_base.assign(nranks + 1, 0);
+  for(uint32_t r = 0; r < nranks; ++r)
+    P.slice_base[r + 1] = P.slice_base[r] + (P.color_off[r + 1] - P.color_off[r] + 63) / 64;
+}
+
 // entries of each position (CSR entry numbers, in sweep order) -> SELL-64 slices per colour
-int build_slices(const feat_sor_s& s, const std::vector<std::vector<uint64_t>>& ent, const std::vector<uint64_t>& colv,
+int build_slices(const Part& s, const std::vector<std::vector<uint64_t>>& ent, const std::vector<uint64_t>& colv,
                  SliceSet& out)
 {
   const size_t nc = s.color_off.size() - 1;
   std::vector<uint64_t> soff{0};
-  std::vector<uint16_t> len(s.n);
+  std::vector<uint16_t> len(s.perm_h.size());
   std::vector<uint64_t> map;
   for(size_t c = 0; c < nc; ++c)
     for(uint64_t p0 = s.color_off[c]; p0 < s.color_off[c + 1]; p0 += 64)
@@ -158,15 +186,15 @@ int build_slices(const feat_sor_s& s, const std::vector<std::vector<uint64_t>>& 
 }
 
 template<int MODE>
-int sweep(feat_handle_t h, const feat_sor_s* s, const SliceSet& ss, double* y, const double* x, double omega)
+int sweep(feat_handle_t h, const feat_sor_s* s, const Part& P, double* y, const double* x, double omega)
 {
-  const size_t nc = s->color_off.size() - 1;
+  const size_t nc = P.color_off.size() - 1;
   for(size_t c = 0; c < nc; ++c)
   {
-    const uint64_t off = s->color_off[c], cnt = s->color_off[c + 1] - off;
+    const uint64_t off = P.color_off[c], cnt = P.color_off[c + 1] - off;
     if(!cnt) continue;
     hipLaunchKernelGGL(k_sor_color<MODE>, dim3(unsigned((cnt + 255) / 256)), dim3(256), 0, h->stream, off, cnt,
-                       s->slice_base[c], s->perm, ss.soff, ss.len, ss.val, ss.col, s->diag, y, x, omega);
+                       P.slice_base[c], P.perm, P.ss.soff, P.ss.len, P.ss.val, P.ss.col, s->diag, y, x, omega);
   }
   return check_launch("feat_sor sweep");
 }
@@ -202,27 +230,28 @@ int feat_sor_init_symbolic(feat_handle_t h, const feat_csr_f64_t* A, int flags, 
   for(uint64_t i = 0; i < n; ++i)
     if(didx[i] == kNoEntry) return fail(FEAT_ERR_INVALID, "feat_sor_init_symbolic: missing main diagonal entry");
 
-  // colouring: greedy first fit in row order over the symmetrised pattern
-  std::vector<uint32_t> color(n);
-  uint32_t ncol = 0;
-  if(flags & FEAT_SOR_ROW_COLORS)
+  auto s = new feat_sor_s();
+  s->n = n;
+  s->nnz = nnz;
+  s->flags = flags;
+  // forward entries: lower (ascending column); backward entries: upper (descending column)
+  if(flags & FEAT_SOR_MULTICOLOR)
   {
-    std::iota(color.begin(), color.end(), 0u);
-    ncol = uint32_t(n);
-  }
-  else
-  {
-    // rows k < i that hold column i (transposed strictly upper pattern)
-    std::vector<uint64_t> tp(n + 1, 0), tk;
+    // greedy first fit in row order over the symmetrised pattern
+    std::vector<uint32_t> color(n);
+    uint32_t ncol = 0;
+    std::vector<uint64_t> tp(n + 1, 0), tk;   // rows k < i that hold column i
     for(uint64_t k = 0; k < n; ++k)
       for(uint64_t e = rp[k]; e < rp[k + 1]; ++e)
         if(colv[e] > k) ++tp[colv[e] + 1];
     for(uint64_t i = 0; i < n; ++i) tp[i + 1] += tp[i];
     tk.resize(tp[n]);
-    std::vector<uint64_t> fill(tp.begin(), tp.end() - 1);
-    for(uint64_t k = 0; k < n; ++k)
-      for(uint64_t e = rp[k]; e < rp[k + 1]; ++e)
-        if(colv[e] > k) tk[fill[colv[e]]++] = k;
+    {
+      std::vector<uint64_t> fill(tp.begin(), tp.end() - 1);
+      for(uint64_t k = 0; k < n; ++k)
+        for(uint64_t e = rp[k]; e < rp[k + 1]; ++e)
+          if(colv[e] > k) tk[fill[colv[e]]++] = k;
+    }
     std::vector<uint64_t> seen;   // seen[c] == i + 1: colour c used by a neighbour of row i
     for(uint64_t i = 0; i < n; ++i)
     {
@@ -240,51 +269,68 @@ int feat_sor_init_symbolic(feat_handle_t h, const feat_csr_f64_t* A, int flags, 
       color[i] = c;
       ncol = std::max(ncol, c + 1);
     }
+    // colours in ascending order of their row count, ties by colour number
+    std::vector<uint64_t> cnt(ncol, 0);
+    for(uint64_t i = 0; i < n; ++i) ++cnt[color[i]];
+    std::vector<std::pair<uint64_t, uint32_t>> ord;
+    for(uint32_t c = 0; c < ncol; ++c)
+      if(cnt[c]) ord.push_back({cnt[c], c});
+    std::sort(ord.begin(), ord.end());
+    std::vector<uint32_t> rank(ncol, 0);
+    for(uint32_t r = 0; r < ord.size(); ++r) rank[ord[r].second] = r;
+    s->fw.rank_of_row.resize(n);
+    for(uint64_t i = 0; i < n; ++i) s->fw.rank_of_row[i] = rank[color[i]];
+    make_part(s->fw, n, uint32_t(ord.size()));
+    s->bw.rank_of_row = s->fw.rank_of_row;   // the backward sweep keeps the forward colour order
+    make_part(s->bw, n, uint32_t(ord.size()));
   }
-  // colours in ascending order of their row count, ties by colour number
-  std::vector<uint64_t> cnt(ncol, 0);
-  for(uint64_t i = 0; i < n; ++i) ++cnt[color[i]];
-  std::vector<std::pair<uint64_t, uint32_t>> ord;
-  for(uint32_t c = 0; c < ncol; ++c)
-    if(cnt[c]) ord.push_back({cnt[c], c});
-  std::sort(ord.begin(), ord.end());
-  std::vector<uint32_t> rank(ncol, 0);
-  for(uint32_t r = 0; r < ord.size(); ++r) rank[ord[r].second] = r;
-
-  auto s = new feat_sor_s();
-  s->n = n;
-  s->nnz = nnz;
-  s->rank_of_row.resize(n);
-  for(uint64_t i = 0; i < n; ++i) s->rank_of_row[i] = rank[color[i]];
-  s->color_off.assign(ord.size() + 1, 0);
-  for(size_t r = 0; r < ord.size(); ++r) s->color_off[r + 1] = s->color_off[r] + ord[r].first;
-  std::vector<uint32_t> perm(n);
+  else
   {
-    std::vector<uint64_t> pos(s->color_off.begin(), s->color_off.end() - 1);
-    for(uint64_t i = 0; i < n; ++i) perm[pos[s->rank_of_row[i]]++] = uint32_t(i);
+    // level sets of the lower and of the upper triangle
+    s->fw.rank_of_row.assign(n, 0);
+    uint32_t nf = n ? 1 : 0;
+    for(uint64_t i = 0; i < n; ++i)
+    {
+      uint32_t l = 0;
+      for(uint64_t e = rp[i]; e < rp[i + 1] && colv[e] < i; ++e) l = std::max(l, s->fw.rank_of_row[colv[e]] + 1);
+      s->fw.rank_of_row[i] = l;
+      nf = std::max(nf, l + 1);
+    }
+    s->bw.rank_of_row.assign(n, 0);
+    uint32_t nb = n ? 1 : 0;
+    for(uint64_t i = n; i-- > 0;)
+    {
+      uint32_t l = 0;
+      for(uint64_t e = rp[i + 1]; e > rp[i] && colv[e - 1] > i; --e) l = std::max(l, s->bw.rank_of_row[colv[e - 1]] + 1);
+      s->bw.rank_of_row[i] = l;
+      nb = std::max(nb, l + 1);
+    }
+    make_part(s->fw, n, nf);
+    make_part(s->bw, n, nb);
   }
-  s->slice_base.assign(ord.size() + 1, 0);
-  for(size_t r = 0; r < ord.size(); ++r)
-    s->slice_base[r + 1] = s->slice_base[r] + (s->color_off[r + 1] - s->color_off[r] + 63) / 64;
-  // forward: lower entries of earlier colours (ascending column); backward: upper entries (descending)
   std::vector<std::vector<uint64_t>> fe(n), be(n);
   for(uint64_t p = 0; p < n; ++p)
   {
-    const uint64_t i = perm[p];
+    const uint64_t i = s->fw.perm_h[p];
     for(uint64_t e = rp[i]; e < rp[i + 1] && colv[e] < i; ++e)
-      if(s->rank_of_row[colv[e]] < s->rank_of_row[i]) fe[p].push_back(e);
+      if(s->fw.rank_of_row[colv[e]] < s->fw.rank_of_row[i]) fe[p].push_back(e);
+  }
+  for(uint64_t p = 0; p < n; ++p)
+  {
+    const uint64_t i = s->bw.perm_h[p];
     for(uint64_t e = rp[i + 1]; e > rp[i] && colv[e - 1] > i; --e) be[p].push_back(e - 1);
   }
-  rc = upload(s->perm, perm);
-  if(!rc) rc = upload(s->diag_idx, didx);
-  if(!rc) rc = check_hip(hipMalloc(&s->diag, sizeof(double) * (n ? n : 1)), "feat_sor_init_symbolic: hipMalloc");
-  if(!rc) rc = build_slices(*s, fe, colv, s->fw);
-  if(!rc) rc = build_slices(*s, be, colv, s->bw);
-  if(!rc) rc = feat_sor_init_numeric(h, s, A);
-  if(rc)
+  int rc2 = upload(s->fw.perm, s->fw.perm_h);
+  if(!rc2) rc2 = upload(s->bw.perm, s->bw.perm_h);
+  if(!rc2) rc2 = upload(s->diag_idx, didx);
+  if(!rc2) rc2 = check_hip(hipMalloc(&s->diag, sizeof(double) * (n ? n : 1)), "feat_sor_init_symbolic: hipMalloc");
+  if(!rc2) rc2 = build_slices(s->fw, fe, colv, s->fw.ss);
+  if(!rc2) rc2 = build_slices(s->bw, be, colv, s->bw.ss);
+  if(!rc2) rc2 = feat_sor_init_numeric(h, s, A);
+  if(rc2)
   {
     free_sor(s);
-    return rc;
+    return rc2;
   }
   *out = s;
   return FEAT_OK;
@@ -295,23 +341,22 @@ int feat_sor_init_numeric(feat_handle_t h, feat_sor_t s, const feat_csr_f64_t* A
   if(!h || !s || !A) return fail(FEAT_ERR_INVALID, "feat_sor_init_numeric: null argument");
   if(A->rows != s->n || A->nnz != s->nnz) return fail(FEAT_ERR_INVALID, "feat_sor_init_numeric: matrix does not match the plan");
   hipLaunchKernelGGL(k_gather_values, dim3(grid_for(s->n)), dim3(kThreads), 0, h->stream, s->n, s->diag_idx, A->val, s->diag);
-  if(s->fw.slots)
-    hipLaunchKernelGGL(k_gather_values, dim3(grid_for(s->fw.slots)), dim3(kThreads), 0, h->stream, s->fw.slots, s->fw.map,
-                       A->val, s->fw.val);
-  if(s->bw.slots)
-    hipLaunchKernelGGL(k_gather_values, dim3(grid_for(s->bw.slots)), dim3(kThreads), 0, h->stream, s->bw.slots, s->bw.map,
-                       A->val, s->bw.val);
+  for(const SliceSet* ss : {&s->fw.ss, &s->bw.ss})
+    if(ss->slots)
+      hipLaunchKernelGGL(k_gather_values, dim3(grid_for(ss->slots)), dim3(kThreads), 0, h->stream, ss->slots, ss->map,
+                         A->val, ss->val);
   return check_launch("feat_sor_init_numeric");
 }
 
-int feat_sor_colors(feat_sor_t s, int* ncolors, uint32_t* rank_of_row, uint64_t* rows_per_color)
+int feat_sor_colors(feat_sor_t s, int backward, int* ncolors, uint32_t* rank_of_row, uint64_t* rows_per_color)
 {
   if(!s) return fail(FEAT_ERR_INVALID, "feat_sor_colors: null plan");
-  const size_t nc = s->color_off.size() - 1;
+  const Part& P = backward ? s->bw : s->fw;
+  const size_t nc = P.color_off.size() - 1;
   if(ncolors) *ncolors = int(nc);
-  if(rank_of_row) std::copy(s->rank_of_row.begin(), s->rank_of_row.end(), rank_of_row);
+  if(rank_of_row) std::copy(P.rank_of_row.begin(), P.rank_of_row.end(), rank_of_row);
   if(rows_per_color)
-    for(size_t c = 0; c < nc; ++c) rows_per_color[c] = s->color_off[c + 1] - s->color_off[c];
+    for(size_t c = 0; c < nc; ++c) rows_per_color[c] = P.color_off[c + 1] - P.color_off[c];
   return FEAT_OK;
 }
 

@@ -10,6 +10,7 @@
 #include <feat3amd/stokes.hpp>
 #include <feat3amd/stokes_assembly.hpp>
 #include <feat3amd/io.hpp>
+#include <feat3amd/sor.hpp>
 #include <feat3amd_host.h>
 
 #include <chrono>
@@ -465,6 +466,91 @@ int feat_mgpcg_matrix(feat_mgpcg_t s, int lvl, std::uint64_t* rows, std::uint64_
     *val = A.val();
     *col = A.col_ind();
     *rowptr = A.row_ptr();
+  });
+}
+
+struct feat_krylov_s
+{
+  Csr A;
+  Filt F;
+  std::shared_ptr<Solver::SolverBase<Vec>> pre;
+  std::shared_ptr<Solver::IterativeSolver<Vec>> solver;
+};
+
+int feat_krylov_create(feat_krylov_t* out, int solver, std::uint64_t n, const double* val, const std::uint64_t* col,
+                       const std::uint64_t* rowptr, const std::uint64_t* filt, std::uint64_t nfilt, int precond,
+                       double omega_pre, double omega_rich, double tol_rel, std::uint64_t max_iter)
+{
+  return guarded([&]() {
+    if(!out || !val || !col || !rowptr || (nfilt && !filt) || (solver != 0 && solver != 1))
+      throw Error("feat_krylov_create: bad argument");
+    auto s = std::make_unique<feat_krylov_s>();
+    s->A = Csr::from_host(n, n, val, col, rowptr);
+    s->F = Filt(n, filt, nullptr, nfilt);
+    switch(precond)
+    {
+    case FEAT_PRECOND_NONE: break;
+    case FEAT_PRECOND_JACOBI: s->pre = Solver::new_jacobi_precond(s->A, s->F, omega_pre); break;
+    case FEAT_PRECOND_SOR: s->pre = Solver::new_sor_precond(s->A, s->F, omega_pre); break;
+    case FEAT_PRECOND_SSOR: s->pre = Solver::new_ssor_precond(s->A, s->F, omega_pre); break;
+    case FEAT_PRECOND_SOR_MULTICOLOR:
+      s->pre = std::make_shared<Solver::SORPrecond<Csr, Filt>>(s->A, s->F, omega_pre, FEAT_SOR_MULTICOLOR);
+      break;
+    case FEAT_PRECOND_SSOR_MULTICOLOR:
+      s->pre = std::make_shared<Solver::SSORPrecond<Csr, Filt>>(s->A, s->F, omega_pre, FEAT_SOR_MULTICOLOR);
+      break;
+    default: throw Error("feat_krylov_create: unknown preconditioner");
+    }
+    if(solver == 0)
+      s->solver = Solver::new_pcg(s->A, s->F, s->pre);
+    else
+      s->solver = Solver::new_richardson(s->A, s->F, omega_rich, s->pre);
+    s->solver->set_tol_rel(tol_rel);
+    s->solver->set_max_iter(max_iter);
+    s->solver->init();
+    *out = s.release();
+  });
+}
+
+int feat_krylov_solve(feat_krylov_t s, double* x, const double* b, feat_solve_result_t* res, double* hist,
+                      std::uint64_t hist_cap)
+{
+  return guarded([&]() {
+    if(!s || !x || !b || !res) throw Error("feat_krylov_solve: bad argument");
+    const Index n = s->A.rows();
+    Vec vx = Vec::wrap(x, n);
+    Vec vb = Vec::wrap(const_cast<double*>(b), n);
+    Solver::Status st = s->solver->correct(vx, vb);
+    Runtime::synchronize();
+    res->status = int(st);
+    res->iters = s->solver->get_num_iter();
+    res->def_init = s->solver->get_def_initial();
+    res->def_final = s->solver->get_def_final();
+    if(hist)
+    {
+      const auto& h = s->solver->history();
+      for(std::size_t i = 0; i < h.size() && i < hist_cap; ++i) hist[i] = h[i];
+    }
+  });
+}
+
+int feat_krylov_precond_apply(feat_krylov_t s, double* cor, const double* def)
+{
+  return guarded([&]() {
+    if(!s || !cor || !def || !s->pre) throw Error("feat_krylov_precond_apply: bad argument");
+    const Index n = s->A.rows();
+    Vec vc = Vec::wrap(cor, n);
+    Vec vd = Vec::wrap(const_cast<double*>(def), n);
+    s->pre->apply(vc, vd);
+    Runtime::synchronize();
+  });
+}
+
+int feat_krylov_destroy(feat_krylov_t s)
+{
+  return guarded([&]() {
+    if(s && s->solver) s->solver->done();
+    delete s;
   });
 }
 

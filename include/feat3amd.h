@@ -521,28 +521,32 @@ typedef struct {
 int feat_stokes_device_sizes(int level, uint64_t* sizes);
 int feat_stokes_assemble_device(feat_handle_t h, int level, const feat_stokes_level_dev_t* out);
 
-/* ---------------- multicolour SOR / SSOR (CSR, f64) ----------------
- * The CUDA backend of SORPrecond / SSORPrecond (solver/sor_precond.hpp:21-28 Intern::
- * cuda_sor_init_symbolic / cuda_sor_apply / cuda_sor_done_symbolic; solver/ssor_precond.hpp:
- * 20-24 cuda_ssor_forward_apply / cuda_ssor_backward_apply): rows coloured so that rows of
- * one colour are uncoupled, colours processed in ascending order of their row count (ties by
- * colour number), rows of a colour in parallel.
- *   feat_sor_apply:           y = 0; y_i = omega (x_i - sum_{j<i} a_ij y_j) / a_ii
- *   feat_ssor_forward_apply:  y = 0; y_i = (x_i - omega sum_{j<i} a_ij y_j) / a_ii
- *   feat_ssor_backward_apply: same colour order, y_i -= omega (sum_{j>i} a_ij y_j) / a_ii
- * (SSORPrecond::apply then scales by omega (2 - omega) and filters, ssor_precond.hpp:515-525).
- * The colouring is greedy first fit in row order on the symmetrised pattern (the reference
- * takes cusparseDcsrcolor's); FEAT_SOR_ROW_COLORS gives every row its own colour, which
- * turns feat_sor_apply into the sequential generic SOR (sor_precond.hpp:201-222).
+/* ---------------- SOR / SSOR sweeps (CSR, f64) ----------------
+ * SORPrecond / SSORPrecond (solver/sor_precond.hpp, solver/ssor_precond.hpp) with the rows of
+ * one "colour" updated in parallel, one launch per colour:
+ *   feat_sor_apply:           y_i = omega (x_i - sum_{j<i} a_ij y_j) / a_ii
+ *   feat_ssor_forward_apply:  y_i = (x_i - omega sum_{j<i} a_ij y_j) / a_ii
+ *   feat_ssor_backward_apply: y_i -= omega (sum_{j>i} a_ij y_j) / a_ii
+ * (SSORPrecond::apply then scales by omega (2 - omega) and filters, ssor_precond.hpp:177-183).
+ * Default ordering: the colours are the level sets of the lower (forward) and upper
+ * (backward) triangle, so the result is exactly the generic sequential sweep
+ * (sor_precond.hpp:201-222, ssor_precond.hpp:193-231); a 27-point 3D stencil in lexicographic
+ * order has 7N+1 levels per sweep.  FEAT_SOR_MULTICOLOR: the CUDA backend's scheme
+ * (sor_precond.hpp:21-28 Intern::cuda_sor_init_symbolic / cuda_sor_apply, ssor_precond.hpp:
+ * 20-24 cuda_ssor_forward_apply / cuda_ssor_backward_apply): a proper colouring (greedy first
+ * fit in row order on the symmetrised pattern; the reference takes cusparseDcsrcolor's),
+ * colours in ascending order of their row count (ties by colour number), y cleared first,
+ * backward sweep in the forward colour order -- 8 colours for a 27-point stencil, but the
+ * resulting SSOR is not symmetric (as the reference's CUDA SSOR).
  * A: device CSR with sorted rows and a main diagonal entry in every row; the plan keeps a
  * copy of the values: call feat_sor_init_numeric after they change.  y must not alias x. */
 typedef struct feat_sor_s* feat_sor_t;
-#define FEAT_SOR_ROW_COLORS 1
+#define FEAT_SOR_MULTICOLOR 1
 int feat_sor_init_symbolic(feat_handle_t h, const feat_csr_f64_t* A, int flags, feat_sor_t* out);
 int feat_sor_init_numeric(feat_handle_t h, feat_sor_t s, const feat_csr_f64_t* A);
-/* number of colours, the processing rank of each row's colour (n entries), rows per colour
- * in processing order (ncolors entries); the arrays are host memory and may be NULL */
-int feat_sor_colors(feat_sor_t s, int* ncolors, uint32_t* rank_of_row, uint64_t* rows_per_color);
+/* the forward (backward = 0) or backward partition: number of colours, the processing rank of
+ * each row's colour (n entries), rows per colour in processing order; host arrays, may be NULL */
+int feat_sor_colors(feat_sor_t s, int backward, int* ncolors, uint32_t* rank_of_row, uint64_t* rows_per_color);
 int feat_sor_apply(feat_handle_t h, feat_sor_t s, double* y, const double* x, double omega);
 int feat_ssor_forward_apply(feat_handle_t h, feat_sor_t s, double* y, const double* x, double omega);
 int feat_ssor_backward_apply(feat_handle_t h, feat_sor_t s, double* y, double omega);

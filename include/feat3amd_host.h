@@ -122,6 +122,29 @@ int feat_mgpcg_write_matrix(feat_mgpcg_t s, int lvl, int mode, const char* path,
 int feat_mgpcg_matrix(feat_mgpcg_t s, int lvl, uint64_t* rows, uint64_t* nnz, const double** val,
                       const uint64_t** col, const uint64_t** rowptr);
 int feat_mgpcg_destroy(feat_mgpcg_t s);
+
+/* ---- single Krylov solver with a device preconditioner ----
+ * PCG (solver = 0, pcg.hpp) or Richardson (solver = 1, richardson.hpp, damping omega_rich)
+ * around precond 0 none, 1 JacobiPrecond, 2 SORPrecond, 3 SSORPrecond (omega_pre =
+ * damping; level-set order, the generic sweeps exactly), 4 ILUPrecond (ILU(0)), 5 / 6 SOR /
+ * SSOR in the multicolour order of the CUDA backend; homogeneous unit filter on filt[0..nfilt).  The matrix
+ * is host CSR (copied to the device); x (initial guess) and b are device vectors. */
+#define FEAT_PRECOND_NONE 0
+#define FEAT_PRECOND_JACOBI 1
+#define FEAT_PRECOND_SOR 2
+#define FEAT_PRECOND_SSOR 3
+#define FEAT_PRECOND_ILU 4
+#define FEAT_PRECOND_SOR_MULTICOLOR 5
+#define FEAT_PRECOND_SSOR_MULTICOLOR 6
+typedef struct feat_krylov_s* feat_krylov_t;
+int feat_krylov_create(feat_krylov_t* s, int solver, uint64_t n, const double* val, const uint64_t* col,
+                       const uint64_t* rowptr, const uint64_t* filt, uint64_t nfilt, int precond,
+                       double omega_pre, double omega_rich, double tol_rel, uint64_t max_iter);
+int feat_krylov_solve(feat_krylov_t s, double* x, const double* b, feat_solve_result_t* res, double* hist,
+                      uint64_t hist_cap);
+/* cor = M^{-1} def of the preconditioner alone (device vectors) */
+int feat_krylov_precond_apply(feat_krylov_t s, double* cor, const double* def);
+int feat_krylov_destroy(feat_krylov_t s);
 const char* feat_host_last_error(void);
 /* Binds the host layer's runtime (Runtime::initialize, kernel/runtime.cpp:37-80)
  * to `device` and `stream` (NULL: the null stream).  Call before any other

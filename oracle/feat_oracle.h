@@ -206,6 +206,9 @@ void orc_mlp_smooth(uint64_t m, const float* p, const double* r, const double* d
 uint32_t orc_color_ranks(uint64_t n, const uint64_t* rowptr, const uint64_t* col, int row_colors, uint32_t* rank);
 void orc_mcsor_apply(uint64_t n, const uint64_t* rowptr, const uint64_t* col, const double* val,
                      const uint32_t* rank, uint32_t ncolors, double omega, int ssor, const double* x, double* y);
+/* the generic sequential SOR / SSOR (solver/sor_precond.hpp:201-222, ssor_precond.hpp:177-231) */
+void orc_sor_generic_apply(uint64_t n, const uint64_t* rowptr, const uint64_t* col, const double* val, double omega,
+                           int ssor, const double* x, double* y);
 
 #ifdef __cplusplus
 }

@@ -97,7 +97,8 @@ void orc_mcsor_apply(uint64_t n, const uint64_t* rowptr, const uint64_t* col, co
       double d = 0.0;
       uint64_t k = rowptr[i];
       for(; col[k] < i; ++k) d = fma(val[k], y[col[k]], d);
-      y[i] = ssor ? (x[i] - omega * d) / val[k] : omega * (x[i] - d) / val[k];
+      /* x - omega d: one fused multiply-add in the reference's build */
+      y[i] = ssor ? fma(-omega, d, x[i]) / val[k] : omega * (x[i] - d) / val[k];
     }
   if(!ssor) return;
   for(uint32_t c = 0; c < ncolors; ++c)
@@ -109,6 +110,34 @@ void orc_mcsor_apply(uint64_t n, const uint64_t* rowptr, const uint64_t* col, co
       for(; col[k] > i; --k) d = fma(val[k], y[col[k]], d);
       y[i] -= omega * d / val[k];
     }
+  const double s = omega * (2.0 - omega);
+  for(uint64_t i = 0; i < n; ++i) y[i] = y[i] * s;
+}
+
+/* The reference's generic sequential sweeps: SORPrecondWithBackend<generic>::_apply_intern
+ * (solver/sor_precond.hpp:201-222) and SSORPrecondWithBackend<generic>::_apply_intern plus
+ * the omega (2 - omega) scaling (solver/ssor_precond.hpp:177-231). */
+void orc_sor_generic_apply(uint64_t n, const uint64_t* rowptr, const uint64_t* col, const double* val, double omega,
+                           int ssor, const double* x, double* y)
+{
+  for(uint64_t i = 0; i < n; ++i) y[i] = x[i];
+  for(uint64_t i = 0; i < n; ++i)
+  {
+    double d = 0.0;
+    uint64_t k = rowptr[i];
+    for(; col[k] < i; ++k) d = fma(val[k], y[col[k]], d);
+    /* x - omega d: one fused multiply-add in the reference's build */
+      y[i] = ssor ? fma(-omega, d, x[i]) / val[k] : omega * (x[i] - d) / val[k];
+  }
+  if(!ssor) return;
+  for(uint64_t i = n; i > 0;)
+  {
+    --i;
+    double d = 0.0;
+    uint64_t k = rowptr[i + 1] - 1;
+    for(; col[k] > i; --k) d = fma(val[k], y[col[k]], d);
+    y[i] -= omega * d / val[k];
+  }
   const double s = omega * (2.0 - omega);
   for(uint64_t i = 0; i < n; ++i) y[i] = y[i] * s;
 }

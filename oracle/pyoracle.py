@@ -389,6 +389,19 @@ class Oracle:
                           omega, int(ssor), x.ctypes.data, y.ctypes.data)
         return y
 
+    def sor_generic_apply(self, A, omega, x, ssor=False):
+        """y = M^{-1} x of the generic sequential SOR / SSOR."""
+        L = self.L
+        L.orc_sor_generic_apply.argtypes = [C.c_uint64, _vp, _vp, _vp, C.c_double, C.c_int, _vp, _vp]
+        rp = np.ascontiguousarray(A.rowptr, np.uint64)
+        col = np.ascontiguousarray(A.col, np.uint64)
+        val = np.ascontiguousarray(A.val, np.float64)
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.empty(A.rows)
+        L.orc_sor_generic_apply(A.rows, rp.ctypes.data, col.ctypes.data, val.ctypes.data, omega, int(ssor),
+                                x.ctypes.data, y.ctypes.data)
+        return y
+
     def csr_apply(self, A, x, y=None, a=1.0, b=0.0, transposed=False, r=None):
         nr = A.cols if transposed else A.rows
         if y is None:

@@ -1,4 +1,7 @@
-"""The multicolour SOR / SSOR oracle, pinned.
+"""The SOR / SSOR oracles, pinned.
+
+* The generic sequential SOR / SSOR restatement equals the reference's own
+  SORPrecond / SSORPrecond<generic> (oracle/_ref) exactly.
 
 * With one colour per row (processed in row order) the multicolour SOR sweep is the
   reference's generic sequential SOR: compared with the reference's own
@@ -111,3 +114,14 @@ def test_planted_error_is_caught(oracle):
     y = oracle.mcsor_apply(A, rank, nc, 1.0, x)
     y_rev = oracle.mcsor_apply(A, (nc - 1 - rank).astype(np.uint32), nc, 1.0, x)
     assert not np.allclose(y, y_rev)
+
+
+@pytest.mark.parametrize("case", ["random", "q1_2d", "q1_3d"])
+def test_generic_sor_ssor_is_reference(feat, oracle, reference, case):
+    rng = np.random.default_rng(31)
+    A = dominant(rng, 150) if case == "random" else q1_csr(feat, 2 if case == "q1_2d" else 3, 4 if case == "q1_2d" else 2)
+    x = rng.standard_normal(A.rows)
+    for omega in (1.0, 0.75, 1.4):
+        assert np.array_equal(oracle.sor_generic_apply(A, omega, x), reference.precond_apply("sor", A, omega, x))
+        assert np.array_equal(oracle.sor_generic_apply(A, omega, x, ssor=True),
+                              reference.precond_apply("ssor", A, omega, x))
