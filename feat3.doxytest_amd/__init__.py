@@ -145,6 +145,12 @@ KERNEL_API = [
     ("feat_ssor_forward_apply", _i, [_vp, _vp, _vp, _vp, _d]),
     ("feat_ssor_backward_apply", _i, [_vp, _vp, _vp, _d]),
     ("feat_sor_done_symbolic", _i, [_vp]),
+    ("feat_ilu_init_symbolic", _i, [_vp, C.POINTER(CsrF64), C.POINTER(_vp)]),
+    ("feat_ilu_init_numeric", _i, [_vp, _vp, C.POINTER(CsrF64)]),
+    ("feat_ilu_apply", _i, [_vp, _vp, _vp, _vp]),
+    ("feat_ilu_info", _i, [_vp, C.POINTER(_u64), C.POINTER(_u64), C.POINTER(_i), C.POINTER(_i)]),
+    ("feat_ilu_factors", _i, [_vp, _vp, _vp, _vp, _vp]),
+    ("feat_ilu_done_symbolic", _i, [_vp]),
     ("feat_q1_assemble_device", _i, [_vp, _i, _i] + [_vp] * 10),
     ("feat_stokes_device_sizes", _i, [_i, _vp]),
     ("feat_stokes_assemble_device", _i, [_vp, _i, _vp]),
@@ -544,6 +550,43 @@ class SorPlan:
     def close(self):
         if self.s:
             self.k.feat_sor_done_symbolic(self.s)
+            self.s = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class IluPlan:
+    """ILU(0) of a device CSR matrix (feat_ilu_*), factorised at construction."""
+
+    def __init__(self, handle, desc):
+        self.k, _ = libs()
+        self.h = handle
+        self.desc = desc
+        self.s = C.c_void_p()
+        check(self.k.feat_ilu_init_symbolic(handle.h, C.byref(desc), C.byref(self.s)), "feat_ilu_init_symbolic")
+        self.init_numeric()
+        nl, nu, ll, lu = _u64(), _u64(), _i(), _i()
+        check(self.k.feat_ilu_info(self.s, C.byref(nl), C.byref(nu), C.byref(ll), C.byref(lu)), "feat_ilu_info")
+        self.nnz_l, self.nnz_u, self.levels_l, self.levels_u = nl.value, nu.value, ll.value, lu.value
+
+    def init_numeric(self):
+        check(self.k.feat_ilu_init_numeric(self.h.h, self.s, C.byref(self.desc)), "feat_ilu_init_numeric")
+
+    def factors(self):
+        l, u, d = np.empty(max(self.nnz_l, 1)), np.empty(max(self.nnz_u, 1)), np.empty(self.desc.rows)
+        check(self.k.feat_ilu_factors(self.h.h, self.s, l.ctypes.data, u.ctypes.data, d.ctypes.data), "feat_ilu_factors")
+        return l[:self.nnz_l], u[:self.nnz_u], d
+
+    def apply(self, y, x):
+        check(self.k.feat_ilu_apply(self.h.h, self.s, ptr(y), ptr(x)), "feat_ilu_apply")
+
+    def close(self):
+        if self.s:
+            self.k.feat_ilu_done_symbolic(self.s)
             self.s = C.c_void_p()
 
     def __del__(self):

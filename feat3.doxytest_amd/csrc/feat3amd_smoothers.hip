@@ -389,3 +389,276 @@ int feat_sor_done_symbolic(feat_sor_t s)
 }
 
 } // extern "C"
+
+// ============================== ILU(0) ==============================
+// ILUPrecond's generic backend (solver/ilu_precond.hpp:104-147 structure, 404-432 data,
+// 453-527 numeric factorisation, 540-582 solves) run level by level: row i of the
+// factorisation reads only the finished rows of its L pattern (their inverted diagonal
+// and U rows) and writes only its own L, D and U entries, so the rows of one level set of
+// the lower triangle are factorised in parallel with the sequential operation order of
+// each row kept -- the factors, and the solves, are the generic ones bit for bit (each
+// `a -= b * c` as one fused multiply-add, as the reference's build contracts it).
+namespace {
+
+__global__ __launch_bounds__(256) void k_ilu_factor_level(uint64_t off, uint64_t cnt, const uint32_t* __restrict__ perm,
+                                                          const uint64_t* __restrict__ rptr_l, const uint32_t* __restrict__ cl,
+                                                          const uint64_t* __restrict__ rptr_u, const uint32_t* __restrict__ cu,
+                                                          double* __restrict__ l, double* __restrict__ u,
+                                                          double* __restrict__ d)
+{
+  const uint64_t q = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+  if(q >= cnt) return;
+  const uint32_t i = perm[off + q];
+  const uint64_t ql = rptr_l[i + 1], qu = rptr_u[i + 1];
+  double di = d[i];
+  for(uint64_t j = rptr_l[i]; j < ql; ++j)
+  {
+    const uint32_t cj = cl[j];
+    uint64_t pl = j, pu = rptr_u[i];
+    const double lj = l[j] * d[cj];
+    l[j] = lj;
+    const uint64_t ke = rptr_u[cj + 1];
+    uint64_t k = rptr_u[cj];
+    for(; k < ke; ++k)
+    {
+      const uint32_t ck = cu[k];
+      if(ck >= i) break;
+      for(; pl < ql && cl[pl] <= ck; ++pl)
+        if(cl[pl] == ck) l[pl] = dfma(-lj, u[k], l[pl]);
+    }
+    if(k < ke && cu[k] == i)
+    {
+      di = dfma(-lj, u[k], di);
+      ++k;
+    }
+    for(; k < ke; ++k)
+    {
+      const uint32_t ck = cu[k];
+      for(; pu < qu && cu[pu] <= ck; ++pu)
+        if(cu[pu] == ck) u[pu] = dfma(-lj, u[k], u[pu]);
+    }
+  }
+  d[i] = 1.0 / di;
+}
+
+// (I+L) y = x over one level set of the lower triangle
+__global__ __launch_bounds__(256) void k_ilu_solve_l(uint64_t off, uint64_t cnt, const uint32_t* __restrict__ perm,
+                                                     const uint64_t* __restrict__ rptr_l, const uint32_t* __restrict__ cl,
+                                                     const double* __restrict__ l, double* y, const double* x)
+{
+  const uint64_t q = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+  if(q >= cnt) return;
+  const uint32_t i = perm[off + q];
+  double r = x[i];
+  for(uint64_t j = rptr_l[i]; j < rptr_l[i + 1]; ++j) r = dfma(-l[j], y[cl[j]], r);
+  y[i] = r;
+}
+
+// (D+U) y = y over one level set of the upper triangle (D holds the inverted diagonal)
+__global__ __launch_bounds__(256) void k_ilu_solve_u(uint64_t off, uint64_t cnt, const uint32_t* __restrict__ perm,
+                                                     const uint64_t* __restrict__ rptr_u, const uint32_t* __restrict__ cu,
+                                                     const double* __restrict__ u, const double* __restrict__ d, double* y)
+{
+  const uint64_t q = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+  if(q >= cnt) return;
+  const uint32_t i = perm[off + q];
+  double r = y[i];
+  for(uint64_t j = rptr_u[i]; j < rptr_u[i + 1]; ++j) r = dfma(-u[j], y[cu[j]], r);
+  y[i] = d[i] * r;
+}
+
+} // namespace
+
+struct feat_ilu_s
+{
+  uint64_t n = 0, nnz = 0, nl = 0, nu = 0;
+  std::vector<uint64_t> fw_off, bw_off;   // level offsets into the permutations
+  uint32_t *fw_perm = nullptr, *bw_perm = nullptr;
+  uint64_t *rptr_l = nullptr, *rptr_u = nullptr;
+  uint32_t *cl = nullptr, *cu = nullptr;
+  uint64_t *map_l = nullptr, *map_u = nullptr, *map_d = nullptr;
+  double *l = nullptr, *u = nullptr, *d = nullptr;
+};
+
+namespace {
+
+void free_ilu(feat_ilu_s* s)
+{
+  if(!s) return;
+  for(void* p : {(void*)s->fw_perm, (void*)s->bw_perm, (void*)s->rptr_l, (void*)s->rptr_u, (void*)s->cl, (void*)s->cu,
+                 (void*)s->map_l, (void*)s->map_u, (void*)s->map_d, (void*)s->l, (void*)s->u, (void*)s->d})
+    (void)hipFree(p);
+  delete s;
+}
+
+// level sets: rank per row -> permutation (ascending rows inside a level) and offsets
+void level_perm(const std::vector<uint32_t>& lev, uint32_t nlev, std::vector<uint32_t>& perm, std::vector<uint64_t>& off)
+{
+  const uint64_t n = lev.size();
+  off.assign(nlev + 1, 0);
+  for(uint64_t i = 0; i < n; ++i) ++off[lev[i] + 1];
+  for(uint32_t r = 0; r < nlev; ++r) off[r + 1] += off[r];
+  perm.resize(n);
+  std::vector<uint64_t> pos(off.begin(), off.end() - 1);
+  for(uint64_t i = 0; i < n; ++i) perm[pos[lev[i]]++] = uint32_t(i);
+}
+
+} // namespace
+
+extern "C" {
+
+int feat_ilu_init_symbolic(feat_handle_t h, const feat_csr_f64_t* A, feat_ilu_t* out)
+{
+  if(!h || !A || !out) return fail(FEAT_ERR_INVALID, "feat_ilu_init_symbolic: null argument");
+  if(A->rows != A->cols) return fail(FEAT_ERR_INVALID, "feat_ilu_init_symbolic: matrix is not square");
+  if(A->rows >= (uint64_t(1) << 32)) return fail(FEAT_ERR_INVALID, "feat_ilu_init_symbolic: more than 2^32-1 rows");
+  const uint64_t n = A->rows, nnz = A->nnz;
+  std::vector<uint64_t> rp(n + 1), colv(nnz);
+  int rc = check_hip(hipMemcpyAsync(rp.data(), A->rowptr, sizeof(uint64_t) * (n + 1), hipMemcpyDeviceToHost, h->stream),
+                     "feat_ilu_init_symbolic: download");
+  if(!rc && nnz)
+    rc = check_hip(hipMemcpyAsync(colv.data(), A->col, sizeof(uint64_t) * nnz, hipMemcpyDeviceToHost, h->stream),
+                   "feat_ilu_init_symbolic: download");
+  if(!rc) rc = check_hip(hipStreamSynchronize(h->stream), "feat_ilu_init_symbolic: sync");
+  if(rc) return rc;
+  if(rp[0] != 0 || rp[n] != nnz) return fail(FEAT_ERR_INVALID, "feat_ilu_init_symbolic: bad row pointers");
+  // L / D / U split (set_struct_csr: every row needs its diagonal entry)
+  std::vector<uint64_t> rl{0}, ru{0}, ml, mu, md(n);
+  std::vector<uint32_t> cl, cu;
+  for(uint64_t i = 0; i < n; ++i)
+  {
+    bool diag = false;
+    for(uint64_t k = rp[i]; k < rp[i + 1]; ++k)
+    {
+      if(colv[k] >= n) return fail(FEAT_ERR_INVALID, "feat_ilu_init_symbolic: column index out of range");
+      if(k > rp[i] && colv[k] <= colv[k - 1]) return fail(FEAT_ERR_INVALID, "feat_ilu_init_symbolic: unsorted row");
+      if(colv[k] < i)
+      {
+        cl.push_back(uint32_t(colv[k]));
+        ml.push_back(k);
+      }
+      else if(colv[k] == i)
+      {
+        md[i] = k;
+        diag = true;
+      }
+      else
+      {
+        cu.push_back(uint32_t(colv[k]));
+        mu.push_back(k);
+      }
+    }
+    if(!diag) return fail(FEAT_ERR_INVALID, "feat_ilu_init_symbolic: missing main diagonal entry");
+    rl.push_back(cl.size());
+    ru.push_back(cu.size());
+  }
+  // level sets of L (factorisation and forward solve) and of U (backward solve)
+  std::vector<uint32_t> lf(n, 0), lb(n, 0);
+  uint32_t nf = n ? 1 : 0, nb = n ? 1 : 0;
+  for(uint64_t i = 0; i < n; ++i)
+  {
+    uint32_t v = 0;
+    for(uint64_t j = rl[i]; j < rl[i + 1]; ++j) v = std::max(v, lf[cl[j]] + 1);
+    lf[i] = v;
+    nf = std::max(nf, v + 1);
+  }
+  for(uint64_t i = n; i-- > 0;)
+  {
+    uint32_t v = 0;
+    for(uint64_t j = ru[i]; j < ru[i + 1]; ++j) v = std::max(v, lb[cu[j]] + 1);
+    lb[i] = v;
+    nb = std::max(nb, v + 1);
+  }
+  auto s = new feat_ilu_s();
+  s->n = n;
+  s->nnz = nnz;
+  s->nl = cl.size();
+  s->nu = cu.size();
+  std::vector<uint32_t> pf, pb;
+  level_perm(lf, nf, pf, s->fw_off);
+  level_perm(lb, nb, pb, s->bw_off);
+  rc = upload(s->fw_perm, pf);
+  if(!rc) rc = upload(s->bw_perm, pb);
+  if(!rc) rc = upload(s->rptr_l, rl);
+  if(!rc) rc = upload(s->rptr_u, ru);
+  if(!rc) rc = upload(s->cl, cl);
+  if(!rc) rc = upload(s->cu, cu);
+  if(!rc) rc = upload(s->map_l, ml);
+  if(!rc) rc = upload(s->map_u, mu);
+  if(!rc) rc = upload(s->map_d, md);
+  if(!rc) rc = check_hip(hipMalloc(&s->l, sizeof(double) * (s->nl ? s->nl : 1)), "feat_ilu_init_symbolic: hipMalloc");
+  if(!rc) rc = check_hip(hipMalloc(&s->u, sizeof(double) * (s->nu ? s->nu : 1)), "feat_ilu_init_symbolic: hipMalloc");
+  if(!rc) rc = check_hip(hipMalloc(&s->d, sizeof(double) * (n ? n : 1)), "feat_ilu_init_symbolic: hipMalloc");
+  if(rc)
+  {
+    free_ilu(s);
+    return rc;
+  }
+  *out = s;
+  return FEAT_OK;
+}
+
+int feat_ilu_init_numeric(feat_handle_t h, feat_ilu_t s, const feat_csr_f64_t* A)
+{
+  if(!h || !s || !A) return fail(FEAT_ERR_INVALID, "feat_ilu_init_numeric: null argument");
+  if(A->rows != s->n || A->nnz != s->nnz) return fail(FEAT_ERR_INVALID, "feat_ilu_init_numeric: matrix does not match the plan");
+  if(s->nl)
+    hipLaunchKernelGGL(k_gather_values, dim3(grid_for(s->nl)), dim3(kThreads), 0, h->stream, s->nl, s->map_l, A->val, s->l);
+  if(s->nu)
+    hipLaunchKernelGGL(k_gather_values, dim3(grid_for(s->nu)), dim3(kThreads), 0, h->stream, s->nu, s->map_u, A->val, s->u);
+  hipLaunchKernelGGL(k_gather_values, dim3(grid_for(s->n)), dim3(kThreads), 0, h->stream, s->n, s->map_d, A->val, s->d);
+  for(size_t v = 0; v + 1 < s->fw_off.size(); ++v)
+  {
+    const uint64_t off = s->fw_off[v], cnt = s->fw_off[v + 1] - off;
+    hipLaunchKernelGGL(k_ilu_factor_level, dim3(unsigned((cnt + 255) / 256)), dim3(256), 0, h->stream, off, cnt,
+                       s->fw_perm, s->rptr_l, s->cl, s->rptr_u, s->cu, s->l, s->u, s->d);
+  }
+  return check_launch("feat_ilu_init_numeric");
+}
+
+int feat_ilu_apply(feat_handle_t h, feat_ilu_t s, double* y, const double* x)
+{
+  if(!h || !s || !y || !x) return fail(FEAT_ERR_INVALID, "feat_ilu_apply: null argument");
+  for(size_t v = 0; v + 1 < s->fw_off.size(); ++v)
+  {
+    const uint64_t off = s->fw_off[v], cnt = s->fw_off[v + 1] - off;
+    hipLaunchKernelGGL(k_ilu_solve_l, dim3(unsigned((cnt + 255) / 256)), dim3(256), 0, h->stream, off, cnt, s->fw_perm,
+                       s->rptr_l, s->cl, s->l, y, x);
+  }
+  for(size_t v = 0; v + 1 < s->bw_off.size(); ++v)
+  {
+    const uint64_t off = s->bw_off[v], cnt = s->bw_off[v + 1] - off;
+    hipLaunchKernelGGL(k_ilu_solve_u, dim3(unsigned((cnt + 255) / 256)), dim3(256), 0, h->stream, off, cnt, s->bw_perm,
+                       s->rptr_u, s->cu, s->u, s->d, y);
+  }
+  return check_launch("feat_ilu_apply");
+}
+
+int feat_ilu_info(feat_ilu_t s, uint64_t* nnz_l, uint64_t* nnz_u, int* levels_l, int* levels_u)
+{
+  if(!s) return fail(FEAT_ERR_INVALID, "feat_ilu_info: null plan");
+  if(nnz_l) *nnz_l = s->nl;
+  if(nnz_u) *nnz_u = s->nu;
+  if(levels_l) *levels_l = int(s->fw_off.size() - 1);
+  if(levels_u) *levels_u = int(s->bw_off.size() - 1);
+  return FEAT_OK;
+}
+
+int feat_ilu_factors(feat_handle_t h, feat_ilu_t s, double* l, double* u, double* d)
+{
+  if(!h || !s) return fail(FEAT_ERR_INVALID, "feat_ilu_factors: null argument");
+  int rc = FEAT_OK;
+  if(l && s->nl) rc = check_hip(hipMemcpyAsync(l, s->l, sizeof(double) * s->nl, hipMemcpyDeviceToHost, h->stream), "feat_ilu_factors");
+  if(!rc && u && s->nu) rc = check_hip(hipMemcpyAsync(u, s->u, sizeof(double) * s->nu, hipMemcpyDeviceToHost, h->stream), "feat_ilu_factors");
+  if(!rc && d) rc = check_hip(hipMemcpyAsync(d, s->d, sizeof(double) * s->n, hipMemcpyDeviceToHost, h->stream), "feat_ilu_factors");
+  if(!rc) rc = check_hip(hipStreamSynchronize(h->stream), "feat_ilu_factors");
+  return rc;
+}
+
+int feat_ilu_done_symbolic(feat_ilu_t s)
+{
+  free_ilu(s);
+  return FEAT_OK;
+}
+
+} // extern "C"

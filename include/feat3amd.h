@@ -552,6 +552,27 @@ int feat_ssor_forward_apply(feat_handle_t h, feat_sor_t s, double* y, const doub
 int feat_ssor_backward_apply(feat_handle_t h, feat_sor_t s, double* y, double omega);
 int feat_sor_done_symbolic(feat_sor_t s);
 
+/* ---------------- ILU(0) (CSR, f64) ----------------
+ * ILUPrecond, generic backend with p = 0 (solver/ilu_precond.hpp: ILUCoreSymbolic::
+ * set_struct_csr :104-147, copy_data_csr :404-432, factorize_numeric_il_du :453-527,
+ * solve_il / solve_du :540-582; the CUDA backend's cusparse csrilu02 computes the same
+ * factorisation): (I+L)(D+U) ~ A on A's pattern, D stored inverted.  Factorisation and
+ * forward solve run over the level sets of L, the backward solve over those of U, one launch
+ * per level, so every row keeps the sequential operation order: bit-identical to the generic
+ * backend.  init_numeric copies the values of A and factorises; apply computes
+ * y = (D+U)^{-1} (I+L)^{-1} x (y may alias x).  A: device CSR, sorted rows, every row with
+ * its diagonal entry.  Fill-in levels p > 0 are not provided. */
+typedef struct feat_ilu_s* feat_ilu_t;
+int feat_ilu_init_symbolic(feat_handle_t h, const feat_csr_f64_t* A, feat_ilu_t* out);
+int feat_ilu_init_numeric(feat_handle_t h, feat_ilu_t s, const feat_csr_f64_t* A);
+int feat_ilu_apply(feat_handle_t h, feat_ilu_t s, double* y, const double* x);
+/* entries of L and U, number of level sets of L and of U */
+int feat_ilu_info(feat_ilu_t s, uint64_t* nnz_l, uint64_t* nnz_u, int* levels_l, int* levels_u);
+/* copies the factors to host arrays (L and U in the CSR order of A's lower / upper entries,
+ * the inverted diagonal); any pointer may be NULL */
+int feat_ilu_factors(feat_handle_t h, feat_ilu_t s, double* l, double* u, double* d);
+int feat_ilu_done_symbolic(feat_ilu_t s);
+
 /* ---------------- autotuner ----------------
  * Launch shapes of the SELL-64 row kernels that every fused entry point uses
  * (entries per row in flight: 4, 8 or 16 -- pair-dictionary plans: 4, 8 or 16

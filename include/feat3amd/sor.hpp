@@ -1,11 +1,13 @@
 // feat3amd/sor.hpp -- SORPrecond / SSORPrecond / ILUPrecond on the device.
 //
-// SolverBase preconditioners over the multicolour sweeps and the level-scheduled
-// ILU(0) of libfeat3amd.so (include/feat3amd.h): the interface and lifecycle of the
+// SolverBase preconditioners over the SOR / SSOR sweeps and the level-scheduled ILU(0)
+// of libfeat3amd.so (include/feat3amd.h): the interface and lifecycle of the
 // reference's SORPrecond (solver/sor_precond.hpp:741-925), SSORPrecond
 // (solver/ssor_precond.hpp:720-899) and ILUPrecond (solver/ilu_precond.hpp:1670-1856):
-// init_symbolic builds the colouring / level sets, init_numeric takes the values,
-// apply(cor, def) computes cor = M^{-1} def and filters it.
+// init_symbolic builds the level sets (or, with FEAT_SOR_MULTICOLOR, the colouring of
+// the CUDA backend), init_numeric takes the values, apply(cor, def) computes
+// cor = M^{-1} def and filters it.  In level-set order the results are those of the
+// reference's generic backend bit for bit.
 #pragma once
 
 #include <feat3amd/lafem.hpp>
@@ -15,7 +17,7 @@ namespace feat3amd
 {
   namespace Solver
   {
-    /// SOR (sor_precond.cu:280-300 semantics): M = D / omega + L_colour-ordered
+    /// SOR (sor_precond.hpp:159-222): cor_i = omega (def_i - sum_{j<i} a_ij cor_j) / a_ii
     template<typename Matrix_, typename Filter_>
     class SORPrecond : public SolverBase<typename Matrix_::VectorTypeL>
     {
@@ -68,7 +70,7 @@ namespace feat3amd
       }
     };
 
-    /// SSOR (ssor_precond.hpp:511-525 with the cuda sweeps): forward, backward, omega (2 - omega)
+    /// SSOR (ssor_precond.hpp:166-231): forward sweep, backward sweep, scaling by omega (2 - omega)
     template<typename Matrix_, typename Filter_>
     class SSORPrecond : public SORPrecond<Matrix_, Filter_>
     {
@@ -89,6 +91,57 @@ namespace feat3amd
         return Status::success;
       }
     };
+
+    /// ILU(0) (ilu_precond.hpp:1040-1104, generic backend with p = 0): cor = (D+U)^{-1}(I+L)^{-1} def
+    template<typename Matrix_, typename Filter_>
+    class ILUPrecond : public SolverBase<typename Matrix_::VectorTypeL>
+    {
+    public:
+      typedef typename Matrix_::VectorTypeL VectorType;
+      typedef typename Matrix_::DataType DataType;
+
+    private:
+      const Matrix_& _matrix;
+      const Filter_& _filter;
+      feat_ilu_t _plan = nullptr;
+
+    public:
+      ILUPrecond(const Matrix_& m, const Filter_& f, int p = 0) : _matrix(m), _filter(f)
+      {
+        if(m.rows() != m.columns()) throw Error("ILUPrecond: matrix is not square");
+        if(p != 0) throw Error("ILUPrecond: only ILU(0) is provided on the device");
+      }
+      ~ILUPrecond() override { done_symbolic(); }
+      std::string name() const override { return "ILU"; }
+      void init_symbolic() override
+      {
+        done_symbolic();
+        const feat_csr_f64_t d = _matrix.desc();
+        check(feat_ilu_init_symbolic(Runtime::handle(), &d, &_plan), "ILUPrecond::init_symbolic");
+      }
+      void init_numeric() override
+      {
+        const feat_csr_f64_t d = _matrix.desc();
+        check(feat_ilu_init_numeric(Runtime::handle(), _plan, &d), "ILUPrecond::init_numeric");
+      }
+      void done_symbolic() override
+      {
+        if(_plan) feat_ilu_done_symbolic(_plan);
+        _plan = nullptr;
+      }
+      Status apply(VectorType& cor, const VectorType& def) override
+      {
+        check(feat_ilu_apply(Runtime::handle(), _plan, cor.elements(), def.elements()), "ILUPrecond::apply");
+        _filter.filter_cor(cor);
+        return Status::success;
+      }
+    };
+
+    template<typename Matrix_, typename Filter_>
+    std::shared_ptr<ILUPrecond<Matrix_, Filter_>> new_ilu_precond(const Matrix_& m, const Filter_& f, int p = 0)
+    {
+      return std::make_shared<ILUPrecond<Matrix_, Filter_>>(m, f, p);
+    }
 
     template<typename Matrix_, typename Filter_>
     std::shared_ptr<SORPrecond<Matrix_, Filter_>> new_sor_precond(const Matrix_& m, const Filter_& f,

@@ -206,6 +206,11 @@ void orc_mlp_smooth(uint64_t m, const float* p, const double* r, const double* d
 uint32_t orc_color_ranks(uint64_t n, const uint64_t* rowptr, const uint64_t* col, int row_colors, uint32_t* rank);
 void orc_mcsor_apply(uint64_t n, const uint64_t* rowptr, const uint64_t* col, const double* val,
                      const uint32_t* rank, uint32_t ncolors, double omega, int ssor, const double* x, double* y);
+/* ILU(0) (solver/ilu_precond.hpp generic backend): factors, then y = (D+U)^{-1}(I+L)^{-1} x */
+void orc_ilu0_factor(uint64_t n, const uint64_t* rowptr, const uint64_t* col, const double* val, double* l,
+                     double* u, double* d, uint64_t* rptr_l, uint64_t* rptr_u);
+void orc_ilu0_apply(uint64_t n, const uint64_t* rowptr, const uint64_t* col, const double* l, const double* u,
+                    const double* d, const uint64_t* rptr_l, const uint64_t* rptr_u, const double* x, double* y);
 /* the generic sequential SOR / SSOR (solver/sor_precond.hpp:201-222, ssor_precond.hpp:177-231) */
 void orc_sor_generic_apply(uint64_t n, const uint64_t* rowptr, const uint64_t* col, const double* val, double omega,
                            int ssor, const double* x, double* y);

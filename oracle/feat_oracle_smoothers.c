@@ -141,3 +141,96 @@ void orc_sor_generic_apply(uint64_t n, const uint64_t* rowptr, const uint64_t* c
   const double s = omega * (2.0 - omega);
   for(uint64_t i = 0; i < n; ++i) y[i] = y[i] * s;
 }
+
+/* ILU(0) of the reference's generic backend: ILUCoreSymbolic::set_struct_csr
+ * (solver/ilu_precond.hpp:104-147: L = entries left of the diagonal, U = right of it),
+ * copy_data_csr (:404-432), factorize_numeric_il_du (:453-527, the "(I+L)(D+U)" form
+ * with the inverted diagonal stored), solve_il / solve_du (:540-582); the reference's
+ * build contracts each `a -= b * c` into one fused multiply-add.
+ * l, u: n_l / n_u values in the CSR order of A's lower / upper entries; d: n values. */
+void orc_ilu0_factor(uint64_t n, const uint64_t* rowptr, const uint64_t* col, const double* val, double* l,
+                     double* u, double* d, uint64_t* rptr_l, uint64_t* rptr_u)
+{
+  /* split structure */
+  uint64_t nl = 0, nu = 0;
+  rptr_l[0] = rptr_u[0] = 0;
+  for(uint64_t i = 0; i < n; ++i)
+  {
+    for(uint64_t k = rowptr[i]; k < rowptr[i + 1]; ++k)
+    {
+      if(col[k] < i)
+        l[nl++] = val[k];
+      else if(col[k] == i)
+        d[i] = val[k];
+      else
+        u[nu++] = val[k];
+    }
+    rptr_l[i + 1] = nl;
+    rptr_u[i + 1] = nu;
+  }
+  /* column indices of L and U in the same order */
+  uint64_t* cl = (uint64_t*)malloc(sizeof(uint64_t) * (nl ? nl : 1));
+  uint64_t* cu = (uint64_t*)malloc(sizeof(uint64_t) * (nu ? nu : 1));
+  nl = nu = 0;
+  for(uint64_t i = 0; i < n; ++i)
+    for(uint64_t k = rowptr[i]; k < rowptr[i + 1]; ++k)
+    {
+      if(col[k] < i) cl[nl++] = col[k];
+      if(col[k] > i) cu[nu++] = col[k];
+    }
+  for(uint64_t i = 0; i < n; ++i)
+  {
+    const uint64_t ql = rptr_l[i + 1], qu = rptr_u[i + 1];
+    for(uint64_t j = rptr_l[i]; j < ql; ++j)
+    {
+      const uint64_t cj = cl[j];
+      uint64_t pl = j, pu = rptr_u[i];
+      l[j] *= d[cj];
+      uint64_t k = rptr_u[cj];
+      for(; k < rptr_u[cj + 1]; ++k)
+      {
+        const uint64_t ck = cu[k];
+        if(ck >= i) break;
+        for(; (pl < ql) && (cl[pl] <= ck); ++pl)
+          if(cl[pl] == ck) l[pl] = fma(-l[j], u[k], l[pl]);
+      }
+      if((k < rptr_u[cj + 1]) && (cu[k] == i))
+      {
+        d[i] = fma(-l[j], u[k], d[i]);
+        ++k;
+      }
+      for(; k < rptr_u[cj + 1]; ++k)
+      {
+        const uint64_t ck = cu[k];
+        for(; (pu < qu) && (cu[pu] <= ck); ++pu)
+          if(cu[pu] == ck) u[pu] = fma(-l[j], u[k], u[pu]);
+      }
+    }
+    d[i] = 1.0 / d[i];
+  }
+  free(cl);
+  free(cu);
+}
+
+/* y = (D+U)^{-1} (I+L)^{-1} x with the factors of orc_ilu0_factor */
+void orc_ilu0_apply(uint64_t n, const uint64_t* rowptr, const uint64_t* col, const double* l, const double* u,
+                    const double* d, const uint64_t* rptr_l, const uint64_t* rptr_u, const double* x, double* y)
+{
+  for(uint64_t i = 0; i < n; ++i)
+  {
+    double r = x[i];
+    uint64_t j = rptr_l[i];
+    for(uint64_t k = rowptr[i]; k < rowptr[i + 1]; ++k)
+      if(col[k] < i) r = fma(-l[j++], y[col[k]], r);
+    y[i] = r;
+  }
+  for(uint64_t i = n; i > 0;)
+  {
+    --i;
+    double r = y[i];
+    uint64_t j = rptr_u[i];
+    for(uint64_t k = rowptr[i]; k < rowptr[i + 1]; ++k)
+      if(col[k] > i) r = fma(-u[j++], y[col[k]], r);
+    y[i] = d[i] * r;
+  }
+}

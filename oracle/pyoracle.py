@@ -402,6 +402,31 @@ class Oracle:
                                 x.ctypes.data, y.ctypes.data)
         return y
 
+    def ilu0_factor(self, A):
+        """ILU(0) factors (l, u, d_inverted, rptr_l, rptr_u) in the CSR order of A."""
+        L = self.L
+        L.orc_ilu0_factor.argtypes = [C.c_uint64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
+        rp = np.ascontiguousarray(A.rowptr, np.uint64)
+        col = np.ascontiguousarray(A.col, np.uint64)
+        val = np.ascontiguousarray(A.val, np.float64)
+        l, u, d = np.empty(max(val.size, 1)), np.empty(max(val.size, 1)), np.empty(A.rows)
+        rl, ru = np.empty(A.rows + 1, np.uint64), np.empty(A.rows + 1, np.uint64)
+        L.orc_ilu0_factor(A.rows, rp.ctypes.data, col.ctypes.data, val.ctypes.data, l.ctypes.data, u.ctypes.data,
+                          d.ctypes.data, rl.ctypes.data, ru.ctypes.data)
+        return l[:int(rl[-1])].copy(), u[:int(ru[-1])].copy(), d, rl, ru
+
+    def ilu0_apply(self, A, f, x):
+        L = self.L
+        L.orc_ilu0_apply.argtypes = [C.c_uint64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
+        l, u, d, rl, ru = (np.ascontiguousarray(a) for a in f)
+        rp = np.ascontiguousarray(A.rowptr, np.uint64)
+        col = np.ascontiguousarray(A.col, np.uint64)
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.empty(A.rows)
+        L.orc_ilu0_apply(A.rows, rp.ctypes.data, col.ctypes.data, l.ctypes.data, u.ctypes.data, d.ctypes.data,
+                         rl.ctypes.data, ru.ctypes.data, x.ctypes.data, y.ctypes.data)
+        return y
+
     def csr_apply(self, A, x, y=None, a=1.0, b=0.0, transposed=False, r=None):
         nr = A.cols if transposed else A.rows
         if y is None:

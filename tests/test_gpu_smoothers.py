@@ -155,6 +155,7 @@ def np_krylov(oracle, A, filt, prec, b, tol, maxit, pcg=True, omega_rich=1.0):
 @pytest.mark.parametrize("solver,precond,omega,dim,level", [("pcg", "ssor", 1.0, 2, 5), ("pcg", "ssor", 1.2, 3, 3),
                                                             ("richardson", "sor", 1.0, 2, 4),
                                                             ("richardson", "ssor_mc", 1.0, 2, 4),
+                                                            ("pcg", "ilu", 1.0, 2, 5), ("pcg", "ilu", 1.0, 3, 3),
                                                             ("pcg", "jacobi", 0.8, 2, 5)])
 def test_krylov_with_device_precond(feat, oracle, torch_gpu, solver, precond, omega, dim, level):
     """The solver tree around the device preconditioners follows the numpy restatement
@@ -166,11 +167,13 @@ def test_krylov_with_device_precond(feat, oracle, torch_gpu, solver, precond, om
     b = np.random.default_rng(9).uniform(-1, 1, A.rows)
     b[q["filt"].astype(np.int64)] = 0.0
     rank, nc = oracle.color_ranks(A)
+    ilu = oracle.ilu0_factor(A)
     inv_diag = omega / np.array([q["val"][int(q["rowptr"][i]) + int(np.searchsorted(
         q["col"][int(q["rowptr"][i]):int(q["rowptr"][i + 1])], i))] for i in range(A.rows)])
     prec = {"ssor": lambda r: oracle.sor_generic_apply(A, omega, r, ssor=True),
             "sor": lambda r: oracle.sor_generic_apply(A, omega, r),
             "ssor_mc": lambda r: oracle.mcsor_apply(A, rank, nc, omega, r, ssor=True),
+            "ilu": lambda r: oracle.ilu0_apply(A, ilu, r),
             "jacobi": lambda r: inv_diag * r}[precond]
     maxit = 2000
     x_ref, h_ref = np_krylov(oracle, A, q["filt"], prec, b, 1e-8, maxit, pcg=(solver == "pcg"))
@@ -183,3 +186,30 @@ def test_krylov_with_device_precond(feat, oracle, torch_gpu, solver, precond, om
     assert np.allclose(res["hist"], h_ref, rtol=1e-10, atol=0)
     xs = x.cpu().numpy()
     assert np.linalg.norm(xs - x_ref) <= 1e-10 * np.linalg.norm(x_ref)
+
+
+@pytest.mark.parametrize("name", ["random", "q1_2d_l6", "q1_3d_l4"])
+def test_ilu0_vs_oracle_and_reference(feat, oracle, reference, handle, torch_gpu, name):
+    print(f"ilu0 {name}", flush=True)
+    torch = torch_gpu
+    A = cases(feat)[name]
+    keep, desc = upload(feat, torch, A)
+    plan = feat.IluPlan(handle, desc)
+    l, u, d, rl, ru = oracle.ilu0_factor(A)
+    assert (plan.nnz_l, plan.nnz_u) == (l.size, u.size)
+    assert plan.levels_l == int(levels(A).max()) + 1 and plan.levels_u == int(levels(A, upper=True).max()) + 1
+    gl, gu, gd = plan.factors()
+    assert np.array_equal(gl, l) and np.array_equal(gu, u) and np.array_equal(gd, d)
+    x = np.random.default_rng(6).standard_normal(A.rows)
+    xd = torch.from_numpy(x).cuda()
+    y = torch.empty_like(xd)
+    plan.apply(y, xd)
+    torch.cuda.synchronize()
+    want = oracle.ilu0_apply(A, (l, u, d, rl, ru), x)
+    assert np.array_equal(y.cpu().numpy(), want)
+    if name != "random":
+        assert np.array_equal(want, reference.precond_apply("ilu", A, 0, x))
+    plan.apply(xd, xd)        # in place, as solve_il / solve_du allow
+    torch.cuda.synchronize()
+    assert np.array_equal(xd.cpu().numpy(), want)
+    plan.close()

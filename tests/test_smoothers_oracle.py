@@ -1,7 +1,9 @@
 """The SOR / SSOR oracles, pinned.
 
 * The generic sequential SOR / SSOR restatement equals the reference's own
-  SORPrecond / SSORPrecond<generic> (oracle/_ref) exactly.
+  SORPrecond / SSORPrecond<generic> (oracle/_ref) exactly; so does the ILU(0)
+  restatement (ILUPrecond<generic>, p = 0), whose factors also satisfy
+  (L U)_ij = a_ij on the pattern of A (to rounding).
 
 * With one colour per row (processed in row order) the multicolour SOR sweep is the
   reference's generic sequential SOR: compared with the reference's own
@@ -125,3 +127,30 @@ def test_generic_sor_ssor_is_reference(feat, oracle, reference, case):
         assert np.array_equal(oracle.sor_generic_apply(A, omega, x), reference.precond_apply("sor", A, omega, x))
         assert np.array_equal(oracle.sor_generic_apply(A, omega, x, ssor=True),
                               reference.precond_apply("ssor", A, omega, x))
+
+
+@pytest.mark.parametrize("case", ["random", "q1_2d", "q1_3d"])
+def test_ilu0_is_reference(feat, oracle, reference, case):
+    rng = np.random.default_rng(41)
+    A = dominant(rng, 150) if case == "random" else q1_csr(feat, 2 if case == "q1_2d" else 3, 4 if case == "q1_2d" else 2)
+    f = oracle.ilu0_factor(A)
+    for seed in range(3):
+        x = np.random.default_rng(seed).standard_normal(A.rows)
+        assert np.array_equal(oracle.ilu0_apply(A, f, x), reference.precond_apply("ilu", A, 0, x))
+    # ILU(0) property: (I+L)(D+U) reproduces A on A's pattern
+    l, u, dinv, rl, ru = f
+    n = A.rows
+    Lm, Um = np.eye(n), np.diag(1.0 / dinv)
+    jl = ju = 0
+    for i in range(n):
+        for k in range(int(A.rowptr[i]), int(A.rowptr[i + 1])):
+            c = int(A.col[k])
+            if c < i:
+                Lm[i, c] = l[jl]
+                jl += 1
+            elif c > i:
+                Um[i, c] = u[ju]
+                ju += 1
+    LU, M = Lm @ Um, dense(A)
+    mask = M != 0
+    assert np.allclose(LU[mask], M[mask], rtol=1e-12, atol=1e-12 * np.abs(M).max())
