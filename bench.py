@@ -319,6 +319,88 @@ def assembly_leg(feat, torch, dev, args):
                                                "restrictions, one core")}
 
 
+def smoothers_leg(feat, torch, dev, args):
+    """SOR / SSOR (level-set and multicolour order) and ILU(0) on the level-7 Q1
+    matrix: one application each (HIP events), the ILU(0) factorisation, and PCG
+    around SSOR / ILU(0) to 1e-8 on level 5 (3D)."""
+    k, _ = feat.libs()
+    stream = torch.cuda.current_stream()
+    h = feat.Handle(0, stream=stream.cuda_stream)
+    d = feat.q1_problem_device(torch, h, DIM, LF, transfer=False)
+    n, nnz = d["n"], int(d["val"].numel())
+    desc = feat.CsrF64(n, n, nnz, d["val"].data_ptr(), d["col"].data_ptr(), d["rowptr"].data_ptr())
+    x = torch.rand(n, dtype=torch.float64, device=dev)
+    y = torch.empty_like(x)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+    def timed(fn, reps):
+        fn()
+        torch.cuda.synchronize()
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        e1.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    out = {}
+    t = time.perf_counter()
+    lv = feat.SorPlan(h, desc)
+    setup_lv = time.perf_counter() - t
+    t = time.perf_counter()
+    mc = feat.SorPlan(h, desc, multicolor=True)
+    setup_mc = time.perf_counter() - t
+    # bytes of one sweep: the entries it reads (8 B value + 4 B column), diagonal, x and y
+    lower = (nnz - n) // 2
+    sweep_bytes = lower * 12 + n * 8 * 3
+    for name, plan in (("level_sets", lv), ("multicolour", mc)):
+        ms_sor = timed(lambda: plan.sor(y, x, 1.0), 5)
+        ms_ssor = timed(lambda: plan.ssor(y, x, 1.0), 5)
+        out[f"sor_{name}"] = {"colours": plan.ncolors, "ms_per_apply": ms_sor,
+                              "gbps_sweep_bytes": sweep_bytes / (ms_sor * 1e-3) / 1e9}
+        out[f"ssor_{name}"] = {"ms_per_apply": ms_ssor, "gbps_sweep_bytes": 2 * sweep_bytes / (ms_ssor * 1e-3) / 1e9}
+        log(f"{name}: {plan.ncolors} colours, SOR {ms_sor:.3f} ms, SSOR {ms_ssor:.3f} ms per application")
+    out["sor_level_sets"]["setup_s"] = setup_lv
+    out["sor_multicolour"]["setup_s"] = setup_mc
+    lv.close()
+    mc.close()
+    t = time.perf_counter()
+    ilu = feat.IluPlan(h, desc)
+    torch.cuda.synchronize()
+    setup_ilu = time.perf_counter() - t
+    ms_fact = timed(ilu.init_numeric, 3)
+    ms_ilu = timed(lambda: ilu.apply(y, x), 5)
+    out["ilu0"] = {"levels_l": ilu.levels_l, "levels_u": ilu.levels_u, "ms_factorise": ms_fact, "ms_per_apply": ms_ilu,
+                   "setup_s": setup_ilu}
+    log(f"ilu(0): {ilu.levels_l}+{ilu.levels_u} levels, factorise {ms_fact:.2f} ms, apply {ms_ilu:.2f} ms")
+    ilu.close()
+    del d
+    h.close()
+    # PCG to 1e-8 on 3D Q1 level 5 around the device preconditioners
+    q = feat.q1_problem(3, 5)
+    from types import SimpleNamespace
+    A = SimpleNamespace(rows=q["n"], cols=q["n"], val=q["val"], col=q["col"], rowptr=q["rowptr"])
+    b = feat.rhs_random(SEED, q["n"], -1.0, 1.0)
+    b[q["filt"].astype(np.int64)] = 0.0
+    bd = torch.from_numpy(b).to(dev)
+    for pre in ("jacobi", "ssor", "ilu"):
+        sv = feat.KrylovSolver("pcg", A, q["filt"], pre, omega_pre=1.0, tol_rel=1e-8, max_iter=1000)
+        xs = torch.zeros(q["n"], dtype=torch.float64, device=dev)
+        sv.solve(xs, bd)
+        xs.zero_()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        r = sv.solve(xs, bd)
+        torch.cuda.synchronize()
+        out[f"pcg_{pre}_3d_l5"] = {"s_per_solve": time.perf_counter() - t, "iters": r["iters"],
+                                   "final_rel_defect": r["def_final"] / r["def_init"]}
+        log(f"pcg-{pre} 3D L5: {r['iters']} iterations, {out[f'pcg_{pre}_3d_l5']['s_per_solve'] * 1e3:.1f} ms")
+        sv.close()
+    out["what"] = (f"3D Q1 level {LF} matrix ({n} rows): one SOR / SSOR / ILU(0) application, level-set order = the "
+                   "generic sequential sweeps bit for bit, multicolour = the CUDA backend's colour order; PCG on level 5")
+    return out
+
+
 def learned_leg(feat, torch, dev, args):
     """Learned smoother on 3D Q1 anisotropic log-normal diffusion, level 6 (274,625 dofs):
     flexible PCG(MG-V(4,4)) with the Jacobi smoother vs the trained BF16-MLP smoother,
@@ -420,6 +502,7 @@ def main():
     ap.add_argument("--stokes-only", action="store_true", help="only the Stokes leg")
     ap.add_argument("--batch-only", action="store_true", help="only the 64-instance batch leg")
     ap.add_argument("--assembly-only", action="store_true", help="only the device-assembly leg")
+    ap.add_argument("--smoothers-only", action="store_true", help="only the SOR / SSOR / ILU(0) leg")
     ap.add_argument("--spmv-only", action="store_true",
                     help="only the SpMV loops (pair dictionary, plain SELL-64, CSR) and a 1 GiB device copy, "
                          "for hardware-counter runs")
@@ -445,6 +528,9 @@ def main():
         return
     if args.batch_only:
         print(json.dumps({"batch": batch_leg(feat, torch, dev, args)}))
+        return
+    if args.smoothers_only:
+        print(json.dumps({"smoothers": smoothers_leg(feat, torch, dev, args)}))
         return
     if args.assembly_only:
         print(json.dumps({"assembly": assembly_leg(feat, torch, dev, args)}))
@@ -593,6 +679,7 @@ def main():
     batch = batch_leg(feat, torch, dev, args)
     learned = learned_leg(feat, torch, dev, args)
     assembly = assembly_leg(feat, torch, dev, args)
+    smoothers = smoothers_leg(feat, torch, dev, args)
 
     # ---- SpMV r = A*x on the same matrix, HIP events on the launch stream:
     # the caller's CSR (u64) and the SELL-64 plan the solver runs on
@@ -685,6 +772,7 @@ def main():
         "batch": batch,
         "learned_smoother": learned,
         "assembly": assembly,
+        "smoothers": smoothers,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS, "traffic": SPMV_TRAFFIC_BYTES["sell64_dict"],
                      "algorithmic_bytes": dict_bytes,

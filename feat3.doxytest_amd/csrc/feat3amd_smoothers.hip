@@ -62,7 +62,22 @@ __global__ __launch_bounds__(256) void k_sor_color(uint64_t off, uint64_t cnt, u
   const uint64_t base = soff[sb + q / 64] + (q % 64);
   const uint32_t L = len[p];
   double d = 0.0;
-  for(uint32_t k = 0; k < L; ++k) d = dfma(val[base + 64 * uint64_t(k)], y[col[base + 64 * uint64_t(k)]], d);
+  // 8 entries in flight: loads first, then the fma chain in entry order
+  for(uint32_t k = 0; k < L; k += 8)
+  {
+    const uint32_t rem = L - k < 8 ? L - k : 8;
+    double v[8], yv[8];
+#pragma unroll
+    for(uint32_t t = 0; t < 8; ++t)
+    {
+      const uint64_t e = base + 64 * uint64_t(k + t);
+      v[t] = t < rem ? val[e] : 0.0;
+      yv[t] = t < rem ? y[col[e]] : 0.0;
+    }
+#pragma unroll
+    for(uint32_t t = 0; t < 8; ++t)
+      if(t < rem) d = dfma(v[t], yv[t], d);
+  }
   if(MODE == 0)
     y[row] = omega * (x[row] - d) / diag[row];
   else if(MODE == 1)
@@ -450,7 +465,21 @@ __global__ __launch_bounds__(256) void k_ilu_solve_l(uint64_t off, uint64_t cnt,
   if(q >= cnt) return;
   const uint32_t i = perm[off + q];
   double r = x[i];
-  for(uint64_t j = rptr_l[i]; j < rptr_l[i + 1]; ++j) r = dfma(-l[j], y[cl[j]], r);
+  const uint64_t j1 = rptr_l[i + 1];
+  for(uint64_t j = rptr_l[i]; j < j1; j += 8)
+  {
+    const uint64_t rem = j1 - j < 8 ? j1 - j : 8;
+    double v[8], yv[8];
+#pragma unroll
+    for(uint32_t t = 0; t < 8; ++t)
+    {
+      v[t] = t < rem ? l[j + t] : 0.0;
+      yv[t] = t < rem ? y[cl[j + t]] : 0.0;
+    }
+#pragma unroll
+    for(uint32_t t = 0; t < 8; ++t)
+      if(t < rem) r = dfma(-v[t], yv[t], r);
+  }
   y[i] = r;
 }
 
@@ -463,7 +492,21 @@ __global__ __launch_bounds__(256) void k_ilu_solve_u(uint64_t off, uint64_t cnt,
   if(q >= cnt) return;
   const uint32_t i = perm[off + q];
   double r = y[i];
-  for(uint64_t j = rptr_u[i]; j < rptr_u[i + 1]; ++j) r = dfma(-u[j], y[cu[j]], r);
+  const uint64_t j1 = rptr_u[i + 1];
+  for(uint64_t j = rptr_u[i]; j < j1; j += 8)
+  {
+    const uint64_t rem = j1 - j < 8 ? j1 - j : 8;
+    double v[8], yv[8];
+#pragma unroll
+    for(uint32_t t = 0; t < 8; ++t)
+    {
+      v[t] = t < rem ? u[j + t] : 0.0;
+      yv[t] = t < rem ? y[cu[j + t]] : 0.0;
+    }
+#pragma unroll
+    for(uint32_t t = 0; t < 8; ++t)
+      if(t < rem) r = dfma(-v[t], yv[t], r);
+  }
   y[i] = d[i] * r;
 }
 
