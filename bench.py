@@ -459,7 +459,9 @@ def spmv_only(feat, torch, dev, stream, args):
     h = feat.Handle(0, stream=stream.cuda_stream)
     xs = torch.rand(n, dtype=torch.float64, device=dev)
     r = torch.empty(n, dtype=torch.float64, device=dev)
-    plans = {"sell64_dict": feat.SellPlan(h, val, col, rp, n, n), "sell64": feat.SellPlan(h, val, col, rp, n, n, dict=False)}
+    plans = {"sell64_diag": feat.SellPlan(h, val, col, rp, n, n),
+             "sell64_dict": feat.SellPlan(h, val, col, rp, n, n, diag=False),
+             "sell64": feat.SellPlan(h, val, col, rp, n, n, dict=False)}
     mats = {"csr_u64": feat.csr_matrix(val, col, rp, n, n)}
     mats.update({kk: p.m for kk, p in plans.items()})
     out = {}
@@ -690,9 +692,12 @@ def main():
     csr = feat.Matrix(feat.FMT_CSR, feat.IDX_U64, n, n, nnz, A.val, A.col, A.rowptr, None, 0, 1.0)
     plan = feat.Matrix()
     feat.check(k.feat_sell_create_ex_f64(h.h, C.byref(A), feat.SELL_NO_DICT, C.byref(plan)), "feat_sell_create_ex_f64")
+    eplan = feat.Matrix()
+    feat.check(k.feat_sell_create_ex_f64(h.h, C.byref(A), feat.SELL_NO_DIAG, C.byref(eplan)), "feat_sell_create_ex_f64")
+    assert eplan.index == feat.IDX_DICT
     dplan = feat.Matrix()
     feat.check(k.feat_sell_create_f64(h.h, C.byref(A), C.byref(dplan)), "feat_sell_create_f64")
-    assert dplan.index == feat.IDX_DICT
+    assert dplan.index == feat.IDX_DIAG
 
     def time_apply(m):
         for _ in range(5):
@@ -707,25 +712,30 @@ def main():
 
     csr_ms = time_apply(csr)
     sell_ms = time_apply(plan)
+    edict_ms = time_apply(eplan)
     dict_ms = time_apply(dplan)
     idx_bytes = {feat.IDX_D16: 2, feat.IDX_U32: 4}[plan.index]
     nslices = (n + 63) // 64
     # bytes each kernel must move: its own arrays once, x once, r once
     csr_bytes = spmv_bytes(n, nnz)
     sell_bytes = int(plan.nnz) * (8 + idx_bytes) + n * 2 + (nslices + 1) * 8 + 2 * n * 8
-    # pair dictionary: one byte per stored entry, row lengths, slice offsets, the 3 KB pair table, x, r
-    dict_bytes = int(dplan.nnz) + n * 2 + (nslices + 1) * 8 + 256 * 12 + 2 * n * 8
+    # pair dictionary in entry order: one byte per stored entry, row lengths, slice offsets,
+    # the 3 KB pair table, x, r; by diagonal: one byte per slot (27 per row), the table, x, r
+    edict_bytes = int(eplan.nnz) + n * 2 + (nslices + 1) * 8 + 256 * 12 + 2 * n * 8
+    dict_bytes = int(dplan.nnz) + 256 * 12 + 2 * n * 8
+    edict_gbs = edict_bytes / (edict_ms * 1e-3) / 1e9
     csr_gbs = csr_bytes / (csr_ms * 1e-3) / 1e9
     sell_gbs = sell_bytes / (sell_ms * 1e-3) / 1e9
     dict_gbs = dict_bytes / (dict_ms * 1e-3) / 1e9
     ref_model = nnz * 16 + n * 8             # product_matvec-bench.cpp:41-44 model (GiB = 1024^3)
     log(f"spmv csr-u64: {csr_ms * 1e3:.1f} us, {csr_gbs:.0f} GB/s | sell-64 ({'i16 delta' if idx_bytes == 2 else 'u32'}): "
         f"{sell_ms * 1e3:.1f} us, {sell_gbs:.0f} GB/s own bytes, {csr_bytes / (sell_ms * 1e-3) / 1e9:.0f} GB/s CSR-equivalent"
-        f" | sell-64 pair dictionary: {dict_ms * 1e3:.1f} us, {dict_gbs:.0f} GB/s own bytes")
+        f" | pair dictionary (entry order): {edict_ms * 1e3:.1f} us, {edict_gbs:.0f} GB/s"
+        f" | pair dictionary by diagonal: {dict_ms * 1e3:.1f} us, {dict_gbs:.0f} GB/s own bytes")
     padded = int(plan.nnz)
     dict_padded = int(dplan.nnz)
-    feat.check(k.feat_sell_destroy_f64(C.byref(plan)), "feat_sell_destroy_f64")
-    feat.check(k.feat_sell_destroy_f64(C.byref(dplan)), "feat_sell_destroy_f64")
+    for m_ in (plan, eplan, dplan):
+        feat.check(k.feat_sell_destroy_f64(C.byref(m_)), "feat_sell_destroy_f64")
     h.close()
     achieved = dict_gbs
 
@@ -757,9 +767,12 @@ def main():
         "mixed_precision": {"form": "FP32 V-cycle on FP32 / FP16-valued SELL-64 plans (FP64 coarse solve) "
                                     "inside FP64 PCG", "fp32": mixed["f32"], "fp16_values": mixed["f16"]},
         "spmv": {
-            "sell64_dict": {"kernel": "k_sell_dict_rows<double,EpiApply<double>,8>", "us_per_launch": dict_ms * 1e3,
+            "sell64_diag": {"kernel": "k_sell_diag_rows<double,EpiApply<double>>", "us_per_launch": dict_ms * 1e3,
                             "bytes_per_launch": dict_bytes, "gbps": dict_gbs,
-                            "csr_equivalent_gbps": csr_bytes / (dict_ms * 1e-3) / 1e9, "stored_entries": dict_padded},
+                            "csr_equivalent_gbps": csr_bytes / (dict_ms * 1e-3) / 1e9, "stored_slots": dict_padded},
+            "sell64_dict": {"kernel": "k_sell_dict_rows<double,EpiApply<double>,8>", "us_per_launch": edict_ms * 1e3,
+                            "bytes_per_launch": edict_bytes, "gbps": edict_gbs,
+                            "csr_equivalent_gbps": csr_bytes / (edict_ms * 1e-3) / 1e9, "stored_entries": int(eplan.nnz)},
             "sell64": {"kernel": "k_sell_rows<double,short,true,EpiApply<double>>", "us_per_launch": sell_ms * 1e3,
                        "bytes_per_launch": sell_bytes, "gbps": sell_gbs,
                        "csr_equivalent_gbps": csr_bytes / (sell_ms * 1e-3) / 1e9,

@@ -54,8 +54,8 @@ class Matrix(C.Structure):
 
 
 FMT_CSR, FMT_SELL = 0, 1
-IDX_U64, IDX_U32, IDX_D16, IDX_DICT = 0, 1, 2, 3
-SELL_NO_DICT = 1
+IDX_U64, IDX_U32, IDX_D16, IDX_DICT, IDX_DIAG = 0, 1, 2, 3, 4
+SELL_NO_DICT, SELL_NO_DIAG = 1, 2
 Q1_HOST_ASSEMBLY = 1
 
 
@@ -634,14 +634,15 @@ def csr_matrix(val, col, rowptr, rows, cols):
 
 class SellPlan:
     """SELL-64 plan of a device CSR (f64 values with u64 indices); fp='f32' rounds values.
-    dict=False keeps full values and column offsets where the pair dictionary applies."""
+    dict=False keeps full values and column offsets where the pair dictionary applies;
+    diag=False keeps the dictionary in entry order where the diagonal layout applies."""
 
-    def __init__(self, handle, val, col, rowptr, rows, cols, fp="f64", dict=True):
+    def __init__(self, handle, val, col, rowptr, rows, cols, fp="f64", dict=True, diag=True):
         self.k, _ = libs()
         self.fp = fp
         c = csr_desc(val, col, rowptr, rows, cols)
         self.m = Matrix()
-        flags = 0 if dict else SELL_NO_DICT
+        flags = (0 if dict else SELL_NO_DICT) | (0 if diag else SELL_NO_DIAG)
         if fp == "f16":
             check(self.k.feat_sell_create_f16(handle.h, C.byref(c), C.byref(self.m)), "feat_sell_create_f16")
         else:

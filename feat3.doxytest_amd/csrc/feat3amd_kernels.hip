@@ -381,6 +381,97 @@ __global__ __launch_bounds__(kThreads) void k_sell_dict_rows(const DT* __restric
   grid_reduce_finish(gr, acc0, acc1);
 }
 
+// DPP wave shifts (wave64): lane l receives lane l-1's value (shr) or lane l+1's (shl); the
+// edge lane gets 0 and is overwritten by the caller
+constexpr int kDppWaveShl1 = 0x130, kDppWaveShr1 = 0x138;
+template<int CTRL>
+__device__ __forceinline__ double wave_shift(double v)
+{
+  const uint64_t b = __builtin_bit_cast(uint64_t, v);
+  const int lo = __builtin_amdgcn_update_dpp(0, int(uint32_t(b)), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, int(uint32_t(b >> 32)), CTRL, 0xF, 0xF, false);
+  return __builtin_bit_cast(double, uint64_t(uint32_t(lo)) | (uint64_t(uint32_t(hi)) << 32));
+}
+template<int CTRL>
+__device__ __forceinline__ float wave_shift(float v)
+{
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xF, 0xF, false));
+}
+
+// Diagonal plan (FEAT_IDX_DIAG): the distinct column offsets are covered by triples
+// (c-1, c, c+1) around kMaxTrip or fewer centres c_g; byte 3g+j of a row's code words is the
+// pair code of its entry at offset c_g + j - 1 (kDiagNone where the row has none).  The rows
+// of a wave are consecutive, so x[row + c - 1] is the left lane's x[row' + c] and
+// x[row + c + 1] the right lane's: one load per triple, two wave shifts, and a load by lane
+// 0 / lane 63 for the values beyond the wave.
+constexpr int kMaxTrip = 10;
+constexpr uint32_t kDiagNone = 255u;   // slot without an entry
+
+// SpMV-type row loop over the diagonal plan: the slots run in ascending offset order (=
+// ascending columns, the CSR order), each lane adds its entry of a slot if it has one.
+// Metadata after the pair table: the number of triples, their centres.
+template<typename DT, class Epi>
+__global__ __launch_bounds__(kThreads) void k_sell_diag_rows(const DT* __restrict__ tab,
+                                                             const uint32_t* __restrict__ codes, uint32_t nw,
+                                                             const DT* __restrict__ x, uint64_t rows, uint64_t cols,
+                                                             Epi epi, GridReduce gr)
+{
+  static_assert(kThreads == 256, "one table entry per thread");
+  __shared__ DT s_val[256];
+  s_val[threadIdx.x] = tab[threadIdx.x];
+  const int* meta = reinterpret_cast<const int*>(tab + 256) + 256;
+  const int nt = meta[0];
+  __syncthreads();
+  const uint64_t nch = (rows + kThreads - 1) / kThreads;
+  const XcdRange xr = xcd_range(nch);
+  const int lane = threadIdx.x & 63;
+  const std::int64_t last = std::int64_t(cols) - 1;   // x holds the columns (a wide matrix has more than rows)
+  double acc0 = 0.0, acc1 = 0.0;
+  epi.begin();
+  for(uint64_t ch = xr.first; ch < xr.end; ch += xr.stride)
+  {
+    // every lane of a live wave runs the slot loop (the shifts read all 64 lanes); rows
+    // past the end load clamped addresses and skip the epilogue
+    const uint64_t row = ch * kThreads + threadIdx.x;
+    if(((row >> 6) << 6) >= rows) continue;   // the whole wave is past the last slice
+    const uint32_t* cw = codes + (row >> 6) * 64 * uint64_t(nw) + lane;
+    uint32_t wd[8];
+#pragma unroll
+    for(int w = 0; w < 8; ++w) wd[w] = w < int(nw) ? __builtin_nontemporal_load(cw + w * 64) : 0xFFFFFFFFu;
+    auto clampi = [&](std::int64_t i) { return i < 0 ? 0 : (i > last ? last : i); };
+    // all loads first
+    DT cen[kMaxTrip], edge[kMaxTrip];
+    const std::int64_t eo = lane == 0 ? -1 : 1;
+#pragma unroll
+    for(int g = 0; g < kMaxTrip; ++g)
+      if(g < nt)
+      {
+        const std::int64_t c = std::int64_t(row) + meta[1 + g];
+        cen[g] = x[clampi(c)];
+        edge[g] = (lane == 0 || lane == 63) ? x[clampi(c + eo)] : DT(0);
+      }
+    DT sum = DT(0);
+#pragma unroll
+    for(int g = 0; g < kMaxTrip; ++g)
+      if(g < nt)
+      {
+        DT left = wave_shift<kDppWaveShr1>(cen[g]), right = wave_shift<kDppWaveShl1>(cen[g]);
+        if(lane == 0) left = edge[g];
+        if(lane == 63) right = edge[g];
+        const DT v3[3] = {left, cen[g], right};
+#pragma unroll
+        for(int j = 0; j < 3; ++j)
+        {
+          const int t = 3 * g + j;
+          const uint32_t c = (wd[t >> 2] >> (8 * (t & 3))) & 255u;
+          if(c != kDiagNone) sum = dfma(s_val[c], v3[j], sum);
+        }
+      }
+    if(row < rows) epi.row(row, sum, acc0, acc1);
+  }
+  grid_reduce_finish(gr, acc0, acc1);
+}
+
 // Runs Epi over the rows of a descriptor (CSR or SELL-64).
 template<typename DT, class Epi, class M>
 int launch_mat(hipStream_t s, const M* A, const DT* x, const Epi& epi, const GridReduce& gr, const char* name)
@@ -421,6 +512,13 @@ int launch_mat(hipStream_t s, const M* A, const DT* x, const Epi& epi, const Gri
                            (const int16_t*)A->col, soff, A->rowlen, x, A->rows, sc, epi, gd);
       else
         return fail(FEAT_ERR_UNSUPPORTED, "SELL index kind");
+      finish_reduce(s, gd, uint64_t(grid));
+      return check_launch(name);
+    }
+    if(A->index == FEAT_IDX_DIAG)
+    {
+      hipLaunchKernelGGL((k_sell_diag_rows<DT, Epi>), dim3(grid), dim3(kThreads), 0, s, reinterpret_cast<const DT*>(A->val),
+                         static_cast<const uint32_t*>(A->col), A->slice_words, x, A->rows, A->cols, epi, gd);
       finish_reduce(s, gd, uint64_t(grid));
       return check_launch(name);
     }
@@ -658,6 +756,38 @@ __global__ void k_dict_fill(const double* val, const uint64_t* col, const uint64
       }
       codes[off + w * 64 + lane] = word;
     }
+  }
+}
+
+// the slot layout of the diagonal plan: byte slot_of[d] of a row's code words holds the pair
+// code of its entry at the d-th smallest column offset, 255 where the row has none; rows past
+// the end of the matrix (the last slice) hold 255 only.  state: set when a row is not
+// strictly ascending (two entries would share a slot or leave CSR order)
+__global__ void k_dict_fill_diag(const double* val, const uint64_t* col, const uint64_t* rp, uint64_t rows,
+                                 uint64_t padded, uint32_t nw, const uint64_t* vtab, const uint64_t* dtab,
+                                 const uint8_t* vcode, const uint8_t* dcode, const uint8_t* pcode, const uint8_t* slot_of,
+                                 uint32_t* codes, uint32_t* state)
+{
+  for(uint64_t r = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; r < padded; r += uint64_t(gridDim.x) * blockDim.x)
+  {
+    uint32_t word[8];
+#pragma unroll
+    for(int w = 0; w < 8; ++w) word[w] = 0xFFFFFFFFu;
+    if(r < rows)
+      for(uint64_t i = rp[r]; i < rp[r + 1]; ++i)
+      {
+        if(i > rp[r] && col[i] <= col[i - 1]) atomicOr(state, 1u);
+        const uint32_t cv = vcode[dict_find(vtab, __builtin_bit_cast(uint64_t, val[i]))];
+        const uint32_t cd = dcode[dict_find(dtab, dict_delta_key(col[i], r))];
+        const uint32_t sl = slot_of[cd], code = pcode[cv * 256u + cd], sh = 8u * (sl & 3u);
+#pragma unroll
+        for(uint32_t w = 0; w < 8; ++w)
+          if(w == (sl >> 2)) word[w] = (word[w] & ~(0xFFu << sh)) | (code << sh);
+      }
+    const uint64_t s = r >> 6, lane = r & 63;
+#pragma unroll
+    for(uint32_t w = 0; w < 8; ++w)
+      if(w < nw) codes[(s * nw + w) * 64 + lane] = word[w];
   }
 }
 
@@ -1443,7 +1573,7 @@ namespace {
 // (column - row, value) pairs (the caller then builds the plain plan).
 template<typename DT, typename M>
 int dict_create(feat_handle_t h, const feat_csr_f64_t* A, const std::vector<uint32_t>& hmax, uint16_t* rlen, M* out,
-                const char* name)
+                const char* name, int flags)
 {
   const uint64_t n = A->rows, ns = (n + 63) / 64;
   uint64_t *vtab = nullptr, *dtab = nullptr, *soff = nullptr;
@@ -1479,7 +1609,7 @@ int dict_create(feat_handle_t h, const feat_csr_f64_t* A, const std::vector<uint
     return FEAT_ERR_UNSUPPORTED;
   }
   // codes of the distinct keys in ascending key order (deterministic)
-  std::vector<uint8_t> hmap(2 * kDictSlots + 65536, 0);
+  std::vector<uint8_t> hmap(2 * kDictSlots + 65536 + 256, 0);
   std::vector<std::pair<uint64_t, uint32_t>> kv, kd;
   for(uint32_t i = 0; i < kDictSlots; ++i)
   {
@@ -1502,8 +1632,9 @@ int dict_create(feat_handle_t h, const feat_csr_f64_t* A, const std::vector<uint
   if(!rc) rc = check_hip(hipMemcpyAsync(hp.data(), present, sizeof(uint32_t) * 65536, hipMemcpyDeviceToHost, h->stream), "copy");
   if(!rc) rc = check_hip(hipStreamSynchronize(h->stream), "sync");
   if(rc) { cleanup(); return rc; }
-  // pair table: values in the plan's arithmetic type, then the offsets as int32
-  std::vector<unsigned char> htab(256 * sizeof(DT) + 256 * sizeof(int), 0);
+  // pair table: values in the plan's arithmetic type, then the offsets as int32, then
+  // the diagonal plan's metadata (count, offsets, kinds)
+  std::vector<unsigned char> htab(256 * sizeof(DT) + (256 + 1 + kMaxTrip) * sizeof(int), 0);
   DT* tv = reinterpret_cast<DT*>(htab.data());
   int* td = reinterpret_cast<int*>(htab.data() + 256 * sizeof(DT));
   uint32_t np = 0;
@@ -1521,6 +1652,31 @@ int dict_create(feat_handle_t h, const feat_csr_f64_t* A, const std::vector<uint
         hmap[2 * kDictSlots + cv * 256u + cd] = uint8_t(np);
         ++np;
       }
+  // diagonal plan: the distinct offsets covered by at most kMaxTrip triples (c-1, c, c+1),
+  // greedily from the smallest; a free code for "no entry"; slot padding within a quarter
+  const uint64_t nd = kd.size();
+  std::vector<int> centres;
+  std::vector<uint8_t> slot_of(256, 0);
+  {
+    std::vector<std::int64_t> o(nd);
+    for(uint64_t t = 0; t < nd; ++t) o[t] = std::int64_t(kd[t].first ^ (uint64_t(1) << 63));
+    for(uint64_t t = 0; t < nd && centres.size() <= uint64_t(kMaxTrip);)
+    {
+      const std::int64_t c = o[t] + 1;
+      const uint32_t g = uint32_t(centres.size());
+      centres.push_back(int(c));
+      for(; t < nd && o[t] <= c + 1; ++t) slot_of[t] = uint8_t(3 * g + uint32_t(o[t] - (c - 1)));
+    }
+  }
+  const uint64_t nslot = 3 * centres.size();
+  const bool diag = !(flags & FEAT_SELL_NO_DIAG) && nd && centres.size() <= uint64_t(kMaxTrip) && np < 256 &&
+                    ns * 64 * nslot <= A->nnz + A->nnz / 4 + 64 * nslot;
+  if(diag)
+  {
+    int* meta = td + 256;
+    meta[0] = int(centres.size());
+    for(size_t g = 0; g < centres.size(); ++g) meta[1 + g] = centres[g];
+  }
   // word offsets of the slices: ceil(slice max / 4) words per lane; when padding every
   // slice to the longest costs at most a quarter more, all slices get that stride
   std::vector<uint64_t> hoff(ns + 1, 0);
@@ -1534,7 +1690,9 @@ int dict_create(feat_handle_t h, const feat_csr_f64_t* A, const std::vector<uint
   const bool uniform = ns * 64 * maxw * 4 <= hoff[ns] * 5 && maxw < (uint64_t(1) << 16);
   if(uniform)
     for(uint64_t i = 0; i <= ns; ++i) hoff[i] = i * 64 * maxw;
-  const uint64_t words = hoff[ns];
+  const uint32_t dw = uint32_t((nslot + 3) / 4);
+  const uint64_t words = diag ? ns * 64 * dw : hoff[ns];
+  std::copy(slot_of.begin(), slot_of.end(), hmap.begin() + 2 * kDictSlots + 65536);
   rc = check_hip(hipMalloc(&table, htab.size()), "dict pair table");
   if(!rc) rc = check_hip(hipMalloc(&soff, sizeof(uint64_t) * (ns + 1)), "sell soff");
   if(!rc) rc = check_hip(hipMalloc(&codes, sizeof(uint32_t) * (words ? words : 1)), "dict codes");
@@ -1542,10 +1700,30 @@ int dict_create(feat_handle_t h, const feat_csr_f64_t* A, const std::vector<uint
   if(!rc) rc = check_hip(hipMemcpyAsync(soff, hoff.data(), sizeof(uint64_t) * (ns + 1), hipMemcpyHostToDevice, h->stream), "copy");
   if(!rc) rc = check_hip(hipMemcpyAsync(maps, hmap.data(), hmap.size(), hipMemcpyHostToDevice, h->stream), "copy");
   if(rc) { cleanup(); return rc; }
-  hipLaunchKernelGGL(k_dict_fill, dim3(grid_for(n)), dim3(kThreads), 0, h->stream, A->val, A->col, A->rowptr, n, soff,
-                     vtab, dtab, maps, maps + kDictSlots, maps + 2 * kDictSlots, codes);
-  rc = check_launch(name);
+  uint32_t unsorted = 0;
+  if(diag)
+  {
+    rc = check_hip(hipMemsetAsync(state, 0, sizeof(uint32_t), h->stream), "memset");
+    if(rc) { cleanup(); return rc; }
+    hipLaunchKernelGGL(k_dict_fill_diag, dim3(grid_for(ns * 64)), dim3(kThreads), 0, h->stream, A->val, A->col,
+                       A->rowptr, n, ns * 64, dw, vtab, dtab, maps, maps + kDictSlots, maps + 2 * kDictSlots,
+                       maps + 2 * kDictSlots + 65536, codes, state);
+    rc = check_launch(name);
+    if(!rc) rc = check_hip(hipMemcpyAsync(&unsorted, state, sizeof(uint32_t), hipMemcpyDeviceToHost, h->stream), "copy");
+  }
+  else
+  {
+    hipLaunchKernelGGL(k_dict_fill, dim3(grid_for(n)), dim3(kThreads), 0, h->stream, A->val, A->col, A->rowptr, n, soff,
+                       vtab, dtab, maps, maps + kDictSlots, maps + 2 * kDictSlots, codes);
+    rc = check_launch(name);
+  }
   if(!rc) rc = check_hip(hipStreamSynchronize(h->stream), "sync");
+  if(!rc && unsorted)
+  {
+    // rows out of column order keep the entry-order plan
+    cleanup();
+    return dict_create<DT>(h, A, hmax, rlen, out, name, flags | FEAT_SELL_NO_DIAG);
+  }
   for(void** p : {(void**)&vtab, (void**)&dtab, (void**)&state, (void**)&present, (void**)&maps})
   {
     (void)hipFree(*p);
@@ -1553,8 +1731,8 @@ int dict_create(feat_handle_t h, const feat_csr_f64_t* A, const std::vector<uint
   }
   if(rc) { cleanup(); return rc; }
   out->format = FEAT_FMT_SELL;
-  out->index = FEAT_IDX_DICT;
-  out->slice_words = uniform ? uint32_t(maxw) : 0u;
+  out->index = diag ? FEAT_IDX_DIAG : FEAT_IDX_DICT;
+  out->slice_words = diag ? dw : (uniform ? uint32_t(maxw) : 0u);
   out->rows = n;
   out->cols = A->cols;
   out->nnz = 4 * words;
@@ -1623,7 +1801,7 @@ int sell_create_impl(feat_handle_t h, const feat_csr_f64_t* A, M* out, const cha
     slmax = nullptr;
     stats = nullptr;
     soff = nullptr;
-    rc = dict_create<DT>(h, A, hmax, rlen, out, name);
+    rc = dict_create<DT>(h, A, hmax, rlen, out, name, flags);
     if(rc == FEAT_OK) return FEAT_OK;
     if(rc != FEAT_ERR_UNSUPPORTED)
     {

@@ -209,6 +209,12 @@ typedef struct {            /* a CSR matrix on the device (u64 indices) */
                           * val -> pair table: 256 values of the arithmetic type
                           * followed by 256 int32 offsets; col -> codes, four
                           * entries per 32-bit lane word; ptr -> word offsets   */
+#define FEAT_IDX_DIAG 4  /* SELL: the pair dictionary by diagonal: byte t of a
+                          * row's code words is the pair code of its entry at the
+                          * t-th smallest column offset (255: none); val -> the
+                          * pair table followed by int32 count, offsets, and how
+                          * each x value is obtained; col -> codes, slice_words
+                          * words per row, 64-row slices lane-interleaved        */
 #define FEAT_VAL_NATIVE 0 /* values in the descriptor's arithmetic type     */
 #define FEAT_VAL_F16 1    /* FP32 arithmetic on FP16 values times `scale`   */
 typedef struct {
@@ -253,6 +259,7 @@ int feat_sell_create_f64(feat_handle_t h, const feat_csr_f64_t* A, feat_matrix_f
 /* Same with flags: FEAT_SELL_NO_DICT builds the plain plan even where the
  * pair dictionary applies. */
 #define FEAT_SELL_NO_DICT 1
+#define FEAT_SELL_NO_DIAG 2   /* keep the entry-order dictionary (FEAT_IDX_DICT) */
 int feat_sell_create_ex_f64(feat_handle_t h, const feat_csr_f64_t* A, int flags, feat_matrix_f64_t* out);
 int feat_sell_create_ex_f32(feat_handle_t h, const feat_csr_f64_t* A, int flags, feat_matrix_f32_t* out);
 /* Same plan with values rounded to FP32 (ConvertPrecond, convert_precond.hpp:100-119). */

@@ -393,7 +393,7 @@ namespace feat3amd
       /// pair-dictionary plan (one byte per entry, smaller than FP16 values).
       const feat_matrix_f32_t& exec16() const
       {
-        if(!_plan16 && exec32().index == FEAT_IDX_DICT)
+        if(!_plan16 && (exec32().index == FEAT_IDX_DICT || exec32().index == FEAT_IDX_DIAG))
           _plan16 = std::shared_ptr<feat_matrix_f32_t>(std::shared_ptr<feat_matrix_f32_t>(), const_cast<feat_matrix_f32_t*>(&exec32()));
         if(!_plan16)
         {

@@ -198,12 +198,15 @@ def stencil_csr(rng, rows, cols, offsets, values, p_keep):
     return Csr(rows, cols, val, np.asarray(col, np.uint64), np.asarray(rp, np.uint64))
 
 
+@pytest.mark.parametrize("diag", [True, False])
 @pytest.mark.parametrize("case", ["q1", "stencil", "partial", "wide", "skewed", "many_pairs", "many_offsets",
-                                  "no_dict"])
-def test_sell_dict_plan_bitwise(feat, handle, torch_gpu, oracle, case):
-    """Pair-dictionary SELL plans (one byte per entry) reproduce the CSR loop bit for
-    bit in FP64 and FP32; matrices with more than 256 distinct (col - row, value)
-    pairs, or dict=False, get the plain plan."""
+                                  "no_dict", "q1_2d_odd", "singles", "wide_rect", "tall_rect"])
+def test_sell_dict_plan_bitwise(feat, handle, torch_gpu, oracle, case, diag):
+    """Pair-dictionary SELL plans (one byte per entry, in entry order or by diagonal)
+    reproduce the CSR loop bit for bit in FP64 and FP32; matrices with more than 256
+    distinct (col - row, value) pairs, or dict=False, get the plain plan; the diagonal
+    layout applies to at most 32 distinct offsets with little slot padding."""
+    print(f"dict plan {case} diag={diag}", flush=True)
     torch = torch_gpu
     k, _ = feat.libs()
     P = feat.ptr
@@ -239,15 +242,37 @@ def test_sell_dict_plan_bitwise(feat, handle, torch_gpu, oracle, case):
         A = random_csr(rng, 3000, 70000, 10, 20)
         A.val = rng.choice(np.asarray(vals), size=A.nnz)           # few values, 1000s of offsets
         want = feat.IDX_U32
+    elif case == "q1_2d_odd":
+        A = q1_level(feat, 2, 5)[1]                                # 1089 rows: a partial last chunk
+    elif case == "wide_rect":
+        # a 9-point pattern whose columns run beyond the rows
+        A = stencil_csr(rng, 700, 3000, [-1, 0, 1, 899, 900, 901, 2199, 2200, 2201], vals, 0.95)
+    elif case == "tall_rect":
+        A = stencil_csr(rng, 3000, 700, [-2300, -1, 0, 1], vals, 0.9)
+    elif case == "singles":
+        A = stencil_csr(rng, 3000, 3000, [-300, -7, 0, 5, 300], vals, 0.9)   # no consecutive offsets
     else:
         A = q1_level(feat, 3, 4)[1]
         want = feat.IDX_D16
+    # the diagonal layout: offsets covered greedily by at most 10 triples (c-1, c, c+1)
+    offs_all = sorted({int(c) - r for r in range(A.rows) for c in A.col[int(A.rowptr[r]):int(A.rowptr[r + 1])]})
+    ntrip, i = 0, 0
+    while i < len(offs_all):
+        centre = offs_all[i] + 1
+        ntrip += 1
+        while i < len(offs_all) and offs_all[i] <= centre + 1:
+            i += 1
+    nslot = 3 * ntrip
+    if want == feat.IDX_DICT and diag and ntrip <= 10 and 64 * ((A.rows + 63) // 64) * nslot <= A.nnz + A.nnz // 4 + 64 * nslot:
+        want = feat.IDX_DIAG
+    if case in ("q1", "q1_2d_odd", "wide_rect") and diag:
+        assert want == feat.IDX_DIAG
     v, c, r = upload_csr(torch, A)
-    plan = feat.SellPlan(handle, v, c, r, A.rows, A.cols, dict=(case != "no_dict"))
+    plan = feat.SellPlan(handle, v, c, r, A.rows, A.cols, dict=(case != "no_dict"), diag=diag)
     assert plan.m.format == feat.FMT_SELL and plan.m.index == want, (plan.m.index, want)
     if want == feat.IDX_DICT:
         # one code-word stride for all slices unless that would pad by more than a quarter
-        assert (plan.m.slice_words == 0) == (case == "skewed"), plan.m.slice_words
+        assert (plan.m.slice_words == 0) == (case in ("skewed", "tall_rect")), plan.m.slice_words
     x = rng.standard_normal(A.cols)
     y = rng.standard_normal(A.rows)
     xd, yd = dev(torch, x), dev(torch, y)
@@ -259,7 +284,7 @@ def test_sell_dict_plan_bitwise(feat, handle, torch_gpu, oracle, case):
             handle.sync()
             assert np.array_equal(host(rd), oracle.csr_apply(A, x, y, a, b)), (tune, a, b)
     plan.close()
-    p32 = feat.SellPlan(handle, v, c, r, A.rows, A.cols, fp="f32", dict=(case != "no_dict"))
+    p32 = feat.SellPlan(handle, v, c, r, A.rows, A.cols, fp="f32", dict=(case != "no_dict"), diag=diag)
     assert p32.m.index == want
     x32 = dev(torch, x.astype(np.float32))
     r32 = torch.empty(A.rows, dtype=torch.float32, device="cuda")
