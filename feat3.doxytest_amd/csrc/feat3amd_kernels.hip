@@ -418,38 +418,46 @@ __global__ __launch_bounds__(kThreads) void k_sell_diag_rows(const DT* __restric
 {
   static_assert(kThreads == 256, "one table entry per thread");
   __shared__ DT s_val[256];
-  s_val[threadIdx.x] = tab[threadIdx.x];
+  const DT tv = tab[threadIdx.x];   // staged into LDS once the first chunk's loads are in flight
   const int* meta = reinterpret_cast<const int*>(tab + 256) + 256;
   const int nt = meta[0];
-  __syncthreads();
   const uint64_t nch = (rows + kThreads - 1) / kThreads;
   const XcdRange xr = xcd_range(nch);
   const int lane = threadIdx.x & 63;
   const std::int64_t last = std::int64_t(cols) - 1;   // x holds the columns (a wide matrix has more than rows)
+  const std::int64_t eo = lane == 0 ? -1 : 1;
   double acc0 = 0.0, acc1 = 0.0;
   epi.begin();
   for(uint64_t ch = xr.first; ch < xr.end; ch += xr.stride)
   {
     // every lane of a live wave runs the slot loop (the shifts read all 64 lanes); rows
-    // past the end load clamped addresses and skip the epilogue
+    // past the end load clamped addresses and skip the epilogue; a wave wholly past the
+    // last slice does nothing but the block's staging barrier
     const uint64_t row = ch * kThreads + threadIdx.x;
-    if(((row >> 6) << 6) >= rows) continue;   // the whole wave is past the last slice
-    const uint32_t* cw = codes + (row >> 6) * 64 * uint64_t(nw) + lane;
+    const bool live = ((row >> 6) << 6) < rows;
     uint32_t wd[8];
-#pragma unroll
-    for(int w = 0; w < 8; ++w) wd[w] = w < int(nw) ? __builtin_nontemporal_load(cw + w * 64) : 0xFFFFFFFFu;
-    auto clampi = [&](std::int64_t i) { return i < 0 ? 0 : (i > last ? last : i); };
-    // all loads first
     DT cen[kMaxTrip], edge[kMaxTrip];
-    const std::int64_t eo = lane == 0 ? -1 : 1;
+    auto clampi = [&](std::int64_t i) { return i < 0 ? 0 : (i > last ? last : i); };
+    if(live)
+    {
+      const uint32_t* cw = codes + (row >> 6) * 64 * uint64_t(nw) + lane;
 #pragma unroll
-    for(int g = 0; g < kMaxTrip; ++g)
-      if(g < nt)
-      {
-        const std::int64_t c = std::int64_t(row) + meta[1 + g];
-        cen[g] = x[clampi(c)];
-        edge[g] = (lane == 0 || lane == 63) ? x[clampi(c + eo)] : DT(0);
-      }
+      for(int w = 0; w < 8; ++w) wd[w] = w < int(nw) ? __builtin_nontemporal_load(cw + w * 64) : 0xFFFFFFFFu;
+#pragma unroll
+      for(int g = 0; g < kMaxTrip; ++g)
+        if(g < nt)
+        {
+          const std::int64_t c = std::int64_t(row) + meta[1 + g];
+          cen[g] = x[clampi(c)];
+          edge[g] = (lane == 0 || lane == 63) ? x[clampi(c + eo)] : DT(0);
+        }
+    }
+    if(ch == xr.first)   // block-uniform: every wave of the block passes here once
+    {
+      s_val[threadIdx.x] = tv;
+      __syncthreads();
+    }
+    if(!live) continue;
     DT sum = DT(0);
 #pragma unroll
     for(int g = 0; g < kMaxTrip; ++g)
