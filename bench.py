@@ -33,7 +33,8 @@ PEAK_HBM_GBS = 8000.0
 # runs of `bench.py --spmv-only` (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch); FETCH_SIZE
 # is scaled by 2.0 and WRITE_SIZE by 1.0, the factors that make the same counters read
 # 1 GiB for a 1 GiB device-to-device copy (profiles/spmv_traffic_3d_q1_l7.md)
-SPMV_TRAFFIC_BYTES = {"sell64_dict": 40574.2 * 2048 + 16771.0 * 1024,
+SPMV_TRAFFIC_BYTES = {"sell64_diag": 38730.1 * 2048 + 16771.0 * 1024,
+                      "sell64_dict": 40574.2 * 2048 + 16771.0 * 1024,
                       "sell64": 295808.6 * 2048 + 16771.0 * 1024,
                       "csr_u64": 481390.4 * 2048 + 16771.0 * 1024}
 
@@ -761,7 +762,8 @@ def main():
             "eager_launch_s_per_solve": t_eager,
             "plain_sell_plans_s_per_solve": t_plain,
             "default_launch_shapes_s_per_solve": t_untuned,
-            "matrix_plan": "SELL-64 pair dictionary (1 byte per entry, 42 distinct (col-row, value) pairs)",
+            "matrix_plan": "SELL-64 pair dictionary by diagonal (1 byte per offset slot, 9 triples of offsets, "
+                           "42 distinct (col-row, value) pairs)",
             "parallelism": "single GPU",
         },
         "mixed_precision": {"form": "FP32 V-cycle on FP32 / FP16-valued SELL-64 plans (FP64 coarse solve) "
@@ -787,10 +789,14 @@ def main():
         "assembly": assembly,
         "smoothers": smoothers,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": achieved / PEAK_HBM_GBS, "traffic": SPMV_TRAFFIC_BYTES["sell64_dict"],
+                     "frac": achieved / PEAK_HBM_GBS, "traffic": SPMV_TRAFFIC_BYTES["sell64_diag"],
                      "algorithmic_bytes": dict_bytes,
-                     "kernel": "pair-dictionary SELL-64 SpMV r = A*x (the solver's matrix format), bytes = its "
-                               "arrays (1 B/entry codes, row lengths, slice offsets, pair table) + x + r",
+                     "kernel": "k_sell_diag_rows: diagonal pair-dictionary SpMV r = A*x (the solver's matrix "
+                               "format), bytes = its arrays (1 B per offset slot, 27 slots per row in 7 words, "
+                               "pair table) + x + r",
+                     "entry_order_dictionary": {"achieved": edict_gbs, "frac": edict_gbs / PEAK_HBM_GBS,
+                                                "traffic": SPMV_TRAFFIC_BYTES["sell64_dict"],
+                                                "algorithmic_bytes": edict_bytes},
                      "csr_equivalent_gbps": csr_bytes / (dict_ms * 1e-3) / 1e9,
                      "plain_sell64": {"achieved": sell_gbs, "frac": sell_gbs / PEAK_HBM_GBS,
                                       "traffic": SPMV_TRAFFIC_BYTES["sell64"], "algorithmic_bytes": sell_bytes}},
