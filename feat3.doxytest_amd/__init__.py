@@ -67,7 +67,7 @@ class MgPcgCfg(C.Structure):
     _fields_ = [("nu_pre", _i), ("nu_post", _i), ("omega_jac", _d), ("omega_rich", _d),
                 ("coarse", _i), ("coarse_omega", _d), ("tol_rel", _d), ("max_iter", _u64),
                 ("force_generic", _i), ("csr_only", _i), ("precision", _i), ("no_graphs", _i),
-                ("no_autotune", _i)]
+                ("no_autotune", _i), ("no_coarse_fusion", _i)]
 
 
 class LevelHost(C.Structure):
@@ -138,6 +138,7 @@ KERNEL_API = [
     ("feat_sell_destroy_f32", _i, [C.POINTER(Matrix)]),
     ("feat_matrix_apply_f64", _i, [_vp, C.POINTER(Matrix), _vp, _d, _vp, _d, _vp]),
     ("feat_q1_device_sizes", _i, [_i, _i, _vp]),
+    ("feat_mg_tail_f64", _i, [_vp, _vp, _vp, _vp, _vp]),
     ("feat_sor_init_symbolic", _i, [_vp, C.POINTER(CsrF64), _i, C.POINTER(_vp)]),
     ("feat_sor_init_numeric", _i, [_vp, _vp, C.POINTER(CsrF64)]),
     ("feat_sor_colors", _i, [_vp, _i, C.POINTER(_i), _vp, _vp]),
@@ -612,10 +613,10 @@ def rhs_random(seed, n, a=-1.0, b=1.0):
 
 def mgpcg_cfg(nu_pre=4, nu_post=4, omega_jac=0.5, omega_rich=1.0, coarse="direct",
               coarse_omega=1.0, tol_rel=1e-8, max_iter=100, force_generic=False, csr_only=False,
-              precision="f64", graphs=True, autotune=True):
+              precision="f64", graphs=True, autotune=True, coarse_fusion=True):
     return MgPcgCfg(nu_pre, nu_post, omega_jac, omega_rich, COARSE[coarse], coarse_omega, tol_rel,
                     max_iter, int(force_generic), int(csr_only), {"f64": 0, "f32": 1, "f16": 2}[precision],
-                    int(not graphs), int(not autotune))
+                    int(not graphs), int(not autotune), int(not coarse_fusion))
 
 
 def csr_matrix(val, col, rowptr, rows, cols):

@@ -2222,3 +2222,281 @@ int feat_dense_gemv_mixed(feat_handle_t h, int n, const double* ainv, const floa
 }
 
 } // extern "C"
+
+// ---------------------------------------------------------------------------
+// fused coarse levels: one 1024-thread workgroup, vectors in LDS
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr int kTailThreads = 1024;
+
+struct TailBufs
+{
+  double *rhs, *sol, *tmp, *def, *cor;
+};
+
+// One thread per row on every level of the tail (at most kTailThreads rows), so each
+// thread keeps its row's code words, inverted diagonal entry and filter bit in registers
+// for the whole V-cycle: the smoothing and defect sweeps then touch only LDS.
+constexpr int kTailSmoothed = FEAT_MG_TAIL_MAX_LEVELS - 1;
+
+struct TailRow
+{
+  uint32_t wd[8];
+  double inv;
+  bool masked;
+};
+
+// sum of the thread's DIAG-plan row (slot order = CSR order) with x in LDS
+__device__ __forceinline__ double tail_row_a(const TailRow R, const int* meta, const double* tab, uint64_t i,
+                                             const double* x)
+{
+  const int nt = meta[0];
+  double sum = 0.0;
+#pragma unroll
+  for(int g = 0; g < kMaxTrip; ++g)
+    if(g < nt)
+    {
+      const std::int64_t c = std::int64_t(i) + meta[1 + g];
+#pragma unroll
+      for(int j = 0; j < 3; ++j)
+      {
+        const int t = 3 * g + j;
+        const uint32_t code = (R.wd[t >> 2] >> (8 * (t & 3))) & 255u;
+        if(code != kDiagNone) sum = dfma(tab[code], x[c + j - 1], sum);
+      }
+    }
+  return sum;
+}
+
+__device__ __forceinline__ double tail_row_csr(const double* val, const uint64_t* col, const uint64_t* rp, uint64_t r,
+                                               const double* x)
+{
+  // 8 entries in flight, then the fmas in entry order
+  const uint64_t k1 = rp[r + 1];
+  double sum = 0.0;
+  for(uint64_t k = rp[r]; k < k1; k += 8)
+  {
+    double v[8];
+    uint64_t c[8];
+#pragma unroll
+    for(int u = 0; u < 8; ++u)
+    {
+      v[u] = k + u < k1 ? val[k + u] : 0.0;
+      c[u] = k + u < k1 ? col[k + u] : 0;
+    }
+#pragma unroll
+    for(int u = 0; u < 8; ++u)
+      if(k + u < k1) sum = dfma(v[u], x[c[u]], sum);
+  }
+  return sum;
+}
+
+// block-wide sum of one value per thread (all threads call it)
+__device__ double tail_sum(double v, double* red)
+{
+  v = wave_sum(v);
+  if((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double t = 0.0;
+  for(int w = 0; w < kTailThreads / 64; ++w) t += red[w];
+  __syncthreads();
+  return t;
+}
+
+__device__ __forceinline__ bool tail_skips(double nrm2)
+{
+  const double d = sqrt(nrm2);
+  return !isfinite(d) || d <= DBL_EPSILON * DBL_EPSILON;
+}
+
+// Richardson(Jacobi) with `steps` steps from x = 0 on rhs (smooth_impl): with def the last
+// step writes the defect of x_steps into def and x_steps ends in b0; else the last update is
+// accumulated into acc.  b0/b1 are the ping-pong pair.  Thread = row `live ? threadIdx.x : -`.
+__device__ __forceinline__ void tail_smooth(const TailRow R, bool live, const int* meta, const double* tab, int steps,
+                                            double omega, bool skip, const double* rhs, double* b0, double* b1,
+                                            double* def, double* acc)
+{
+  const uint64_t i = threadIdx.x;
+  auto B = [&](int k) { return ((steps - k) & 1) ? b1 : b0; };
+  const bool acc_first = acc != nullptr && steps == 1;
+  if(live)
+  {
+    double c = R.inv * rhs[i];
+    if(R.masked) c = 0.0;
+    double xn = dfma(omega, c, 0.0);
+    if(skip) xn = 0.0;
+    if(acc_first)
+      acc[i] = dfma(1.0, xn, acc[i]);
+    else
+      B(1)[i] = xn;
+  }
+  __syncthreads();
+  const int nsp = def ? steps : steps - 1;
+  for(int k = 1; k <= nsp; ++k)
+  {
+    const double* xk = B(k);
+    const int mode = (def && k == steps) ? 1 : ((acc && k == steps - 1) ? 2 : 0);
+    double* out = mode == 1 ? def : (mode == 2 ? acc : B(k + 1));
+    if(live)
+    {
+      double d = dfma(tail_row_a(R, meta, tab, i, xk), -1.0, rhs[i]);
+      if(R.masked) d = 0.0;
+      if(mode == 1)
+        out[i] = d;
+      else
+      {
+        double c = R.inv * d;
+        if(R.masked) c = 0.0;
+        double xn = dfma(omega, c, xk[i]);
+        if(skip) xn = 0.0;
+        out[i] = mode == 0 ? xn : dfma(1.0, xn, out[i]);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(kTailThreads) void k_mg_tail(feat_mg_tail_t t, const double* __restrict__ rhs_top,
+                                                          const double* __restrict__ nrm_top, double* __restrict__ sol_top)
+{
+  extern __shared__ double lds[];
+  __shared__ double red[kTailThreads / 64];
+  const int nl = t.nlevels, c = nl - 1;
+  const uint64_t tid = threadIdx.x;
+  TailBufs b[FEAT_MG_TAIL_MAX_LEVELS];
+  TailRow row[kTailSmoothed];
+  const int* meta[kTailSmoothed];
+  double* tabs = lds;   // the smoothed levels' pair-table values, 256 each
+  {
+    double* p = lds + 256 * c;
+#pragma unroll
+    for(int l = 0; l < FEAT_MG_TAIL_MAX_LEVELS; ++l)
+      if(l < nl)
+      {
+        const uint64_t n = t.lv[l].n;
+        b[l].rhs = p;
+        b[l].sol = p + n;
+        p += 2 * n;
+        if(l < c)
+        {
+          b[l].tmp = p;
+          b[l].def = p + n;
+          b[l].cor = p + 2 * n;
+          p += 3 * n;
+        }
+      }
+  }
+#pragma unroll
+  for(int l = 0; l < kTailSmoothed; ++l)
+    if(l < c)
+    {
+      const feat_mg_tail_level_t& L = t.lv[l];
+      meta[l] = reinterpret_cast<const int*>(L.table + 256) + 256;
+      for(int e = threadIdx.x; e < 256; e += kTailThreads) tabs[256 * l + e] = L.table[e];
+      if(tid < L.n)
+      {
+        const uint32_t* cw = L.codes + (tid >> 6) * 64 * uint64_t(L.words) + (tid & 63);
+#pragma unroll
+        for(int w = 0; w < 8; ++w) row[l].wd[w] = w < int(L.words) ? cw[w * 64] : 0xFFFFFFFFu;
+        row[l].inv = L.inv_diag[tid];
+        row[l].masked = is_masked(L.mask, tid);
+      }
+    }
+  if(tid < t.lv[0].n) b[0].rhs[tid] = rhs_top[tid];
+  __syncthreads();
+  double nrm[FEAT_MG_TAIL_MAX_LEVELS];
+  nrm[0] = *nrm_top;
+  // restriction leg
+#pragma unroll
+  for(int l = 0; l < kTailSmoothed; ++l)
+    if(l < c)
+    {
+      const feat_mg_tail_level_t& L = t.lv[l];
+      const bool live = tid < L.n;
+      tail_smooth(row[l], live, meta[l], tabs + 256 * l, L.pre_steps, L.pre_omega, tail_skips(nrm[l]), b[l].rhs,
+                  b[l].sol, b[l].tmp, b[l].def, nullptr);
+      const feat_mg_tail_level_t& C = t.lv[l + 1];
+      double part = 0.0;
+      if(tid < C.n)
+      {
+        double v = dfma(tail_row_csr(L.r_val, L.r_col, L.r_ptr, tid, b[l].def), 1.0, 0.0);
+        if(is_masked(C.mask, tid)) v = 0.0;
+        b[l + 1].rhs[tid] = v;
+        part = v * v;
+      }
+      nrm[l + 1] = tail_sum(part, red);   // also orders the rhs stores before their use
+    }
+  // coarsest level: dense inverse
+  {
+    const int n = int(t.lv[c].n);
+    if(int(tid) < n)
+    {
+      const double* ar = t.coarse_inverse + size_t(tid) * n;
+      double sv = 0.0;
+      for(int k = 0; k < n; k += 8)
+      {
+        double a[8];
+#pragma unroll
+        for(int u = 0; u < 8; ++u) a[u] = k + u < n ? ar[k + u] : 0.0;
+#pragma unroll
+        for(int u = 0; u < 8; ++u)
+          if(k + u < n) sv = dfma(a[u], b[c].rhs[k + u], sv);
+      }
+      if(is_masked(t.lv[c].mask, tid)) sv = 0.0;
+      b[c].sol[tid] = sv;
+    }
+    __syncthreads();
+  }
+  // prolongation leg
+#pragma unroll
+  for(int l = kTailSmoothed - 1; l >= 0; --l)
+    if(l < c)
+    {
+      const feat_mg_tail_level_t& L = t.lv[l];
+      const bool live = tid < L.n;
+      if(live)
+      {
+        double cc = dfma(tail_row_csr(L.p_val, L.p_col, L.p_ptr, tid, b[l + 1].sol), 1.0, 0.0);
+        if(row[l].masked) cc = 0.0;
+        b[l].sol[tid] = dfma(1.0, cc, b[l].sol[tid]);
+      }
+      __syncthreads();
+      double part = 0.0;
+      if(live)
+      {
+        double v = dfma(tail_row_a(row[l], meta[l], tabs + 256 * l, tid, b[l].sol), -1.0, b[l].rhs[tid]);
+        if(row[l].masked) v = 0.0;
+        b[l].def[tid] = v;
+        part = v * v;
+      }
+      const double dn = tail_sum(part, red);
+      tail_smooth(row[l], live, meta[l], tabs + 256 * l, L.post_steps, L.post_omega, tail_skips(dn), b[l].def,
+                  b[l].cor, b[l].tmp, nullptr, b[l].sol);
+    }
+  if(tid < t.lv[0].n) sol_top[tid] = b[0].sol[tid];
+}
+
+} // namespace
+
+extern "C" int feat_mg_tail_f64(feat_handle_t h, const feat_mg_tail_t* t, const double* rhs, const double* rhs_nrm2,
+                                double* sol)
+{
+  if(!h || !t || !rhs || !rhs_nrm2 || !sol || t->nlevels < 2 || t->nlevels > FEAT_MG_TAIL_MAX_LEVELS ||
+     !t->coarse_inverse)
+    return fail(FEAT_ERR_INVALID, "feat_mg_tail_f64: bad argument");
+  uint64_t doubles = 0;
+  for(int l = 0; l < t->nlevels; ++l)
+  {
+    const feat_mg_tail_level_t& L = t->lv[l];
+    doubles += (l + 1 < t->nlevels ? 5 * L.n + 256 : 2 * L.n);
+    if(L.n > uint64_t(kTailThreads)) return fail(FEAT_ERR_INVALID, "feat_mg_tail_f64: a level has more than 1024 rows");
+    if(l + 1 < t->nlevels &&
+       (!L.table || !L.codes || L.words > 8 || !L.inv_diag || !L.r_val || !L.r_col || !L.r_ptr || !L.p_val ||
+        !L.p_col || !L.p_ptr || L.pre_steps < 1 || L.post_steps < 1))
+      return fail(FEAT_ERR_INVALID, "feat_mg_tail_f64: incomplete level");
+  }
+  if(doubles * sizeof(double) > FEAT_MG_TAIL_MAX_LDS) return fail(FEAT_ERR_INVALID, "feat_mg_tail_f64: levels exceed the LDS budget");
+  hipLaunchKernelGGL(k_mg_tail, dim3(1), dim3(kTailThreads), doubles * sizeof(double), h->stream, *t, rhs, rhs_nrm2, sol);
+  return check_launch("feat_mg_tail_f64");
+}

@@ -306,6 +306,7 @@ static void build_solver(feat_mgpcg_s* s, const feat_mgpcg_cfg_t* cfg)
   {
     s->mg = std::make_shared<MG>(s->hier);
     s->mg->set_autotune(cfg->no_autotune == 0);
+    s->mg->set_coarse_fusion(cfg->no_coarse_fusion == 0);
     s->pcg = Solver::new_pcg(s->levels[0]->A, s->levels[0]->F, s->mg);
   }
   s->pcg->set_tol_rel(cfg->tol_rel);

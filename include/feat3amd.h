@@ -580,6 +580,40 @@ int feat_ilu_info(feat_ilu_t s, uint64_t* nnz_l, uint64_t* nnz_u, int* levels_l,
 int feat_ilu_factors(feat_handle_t h, feat_ilu_t s, double* l, double* u, double* d);
 int feat_ilu_done_symbolic(feat_ilu_t s);
 
+/* ---------------- fused coarse levels of the V-cycle ----------------
+ * One workgroup (1024 threads) runs the coarse part of MultiGrid::apply (multigrid.hpp:
+ * 1713-2000) from level t down to the coarsest and back, with every vector of those levels
+ * in LDS: per smoothed level the fused Richardson(Jacobi) pre-smoothing with its final
+ * defect (feat_rj_smooth_f64 semantics), the masked restriction with |rhs|^2, then on the
+ * coarsest level the dense inverse (feat_dense_gemv_f64), and back up the prolongation-add,
+ * the defect with |d|^2 and the post-smoothing accumulated into the solution.  Every row
+ * value is computed with the same operations in the same order as the separate launches
+ * (matrix sums in CSR / slot order), so the solution is bit-identical; the |.|^2 sums
+ * feeding the smoothers' skip rule (exact zero or non-finite) are summed in another order.
+ * lv[0 .. nlevels-2]: smoothed levels (A as a FEAT_IDX_DIAG plan, restriction to and
+ * prolongation from the next level as CSR); lv[nlevels-1]: the coarsest level (n, mask).
+ * rhs / rhs_nrm2: level t's right-hand side and |rhs|^2 (device); sol: level t's solution
+ * (written).  The caller keeps 8 bytes x (5 n + 256 over the smoothed levels + 2 n coarse)
+ * within FEAT_MG_TAIL_MAX_LDS bytes.  No counterpart in the reference (a launch-count fusion). */
+#define FEAT_MG_TAIL_MAX_LEVELS 6
+#define FEAT_MG_TAIL_MAX_ROWS 1024   /* rows per level (one thread per row) */
+#define FEAT_MG_TAIL_MAX_LDS 61440
+typedef struct {
+  uint64_t n;
+  const double* table; const uint32_t* codes; uint32_t words;   /* FEAT_IDX_DIAG plan of A */
+  const double* inv_diag; const uint32_t* mask;
+  const double* r_val; const uint64_t* r_col; const uint64_t* r_ptr;   /* restriction to the next level */
+  const double* p_val; const uint64_t* p_col; const uint64_t* p_ptr;   /* prolongation from the next level */
+  int pre_steps, post_steps;
+  double pre_omega, post_omega;
+} feat_mg_tail_level_t;
+typedef struct {
+  int nlevels;
+  feat_mg_tail_level_t lv[FEAT_MG_TAIL_MAX_LEVELS];
+  const double* coarse_inverse;   /* n x n row-major inverse of the filtered coarsest matrix */
+} feat_mg_tail_t;
+int feat_mg_tail_f64(feat_handle_t h, const feat_mg_tail_t* t, const double* rhs, const double* rhs_nrm2, double* sol);
+
 /* ---------------- autotuner ----------------
  * Launch shapes of the SELL-64 row kernels that every fused entry point uses
  * (entries per row in flight: 4, 8 or 16 -- pair-dictionary plans: 4, 8 or 16

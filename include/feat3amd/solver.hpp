@@ -489,7 +489,69 @@ namespace feat3amd
       FusedWorkspace _fw;
       bool _fusable = false;
       bool _autotune = true;
+      bool _tail_on = true, _tail_built = false;
+      Index _tail_t = 0;   // first level of the fused coarse part; 0: none (level 0 is never in it)
+      feat_mg_tail_t _tail{};
       enum class Coarse { none, direct, jacobi, richardson, other } _coarse_kind = Coarse::other;
+
+      /// The coarse levels that run as one LDS-resident workgroup (feat_mg_tail_f64): the
+      /// longest run ending at the coarsest level, starting at level 1 or below, whose
+      /// vectors fit the LDS budget, with diagonal-dictionary plans and a direct coarse solve.
+      void _build_tail()
+      {
+        _tail_built = true;
+        _tail_t = 0;
+        const Index L = _h->size(), c = L - 1;
+        if(!_tail_on || !_fusable || _coarse_kind != Coarse::direct || L < 3 ||
+           _h->levels[c].matrix->rows() > FEAT_MG_TAIL_MAX_ROWS)
+          return;
+        uint64_t doubles = 2 * _h->levels[c].matrix->rows();
+        Index t = c;
+        for(Index s = c; s-- > 1;)
+        {
+          const auto& l = _h->levels[s];
+          const feat_matrix_f64_t& A = l.matrix->exec();
+          const uint64_t need = doubles + 5 * l.matrix->rows() + 256;   // vectors and the pair table
+          if(A.index != FEAT_IDX_DIAG || A.slice_words > 8 || need * sizeof(double) > FEAT_MG_TAIL_MAX_LDS ||
+             c - s + 1 > FEAT_MG_TAIL_MAX_LEVELS || l.matrix->rows() > FEAT_MG_TAIL_MAX_ROWS)
+            break;
+          doubles = need;
+          t = s;
+        }
+        if(t == c) return;
+        _tail = feat_mg_tail_t{};
+        _tail.nlevels = int(c - t + 1);
+        for(Index s = t; s <= c; ++s)
+        {
+          auto& l = _h->levels[s];
+          feat_mg_tail_level_t& d = _tail.lv[s - t];
+          d.n = l.matrix->rows();
+          d.mask = l.filter->mask();
+          if(s == c) break;
+          const feat_matrix_f64_t& A = l.matrix->exec();
+          d.table = static_cast<const double*>(A.val);
+          d.codes = static_cast<const uint32_t*>(A.col);
+          d.words = A.slice_words;
+          auto* pre = static_cast<RichType*>(l.pre.get());
+          auto* post = static_cast<RichType*>(l.post.get());
+          d.inv_diag = (const double*)pre->fusable_jacobi()->inv_diag().elements();
+          if(post->fusable_jacobi()->inv_diag().elements() != pre->fusable_jacobi()->inv_diag().elements()) return;
+          const Matrix_& R = l.transfer->get_mat_rest();
+          const Matrix_& P = l.transfer->get_mat_prol();
+          d.r_val = (const double*)R.val();
+          d.r_col = (const uint64_t*)R.col_ind();
+          d.r_ptr = (const uint64_t*)R.row_ptr();
+          d.p_val = (const double*)P.val();
+          d.p_col = (const uint64_t*)P.col_ind();
+          d.p_ptr = (const uint64_t*)P.row_ptr();
+          d.pre_steps = int(pre->get_max_iter());
+          d.post_steps = int(post->get_max_iter());
+          d.pre_omega = pre->omega();
+          d.post_omega = post->omega();
+        }
+        _tail.coarse_inverse = static_cast<DirectCoarseSolver<Matrix_, Filter_>*>(_h->levels[c].coarse.get())->inverse();
+        _tail_t = t;
+      }
 
     public:
       explicit MultiGrid(std::shared_ptr<HierarchyType> h, MultiGridCycle = MultiGridCycle::V) : _h(h) {}
@@ -498,6 +560,14 @@ namespace feat3amd
       /// pick the SELL-64 launch shape of every level's matrices at init_symbolic (default on;
       /// results are bit-identical for every shape)
       void set_autotune(bool on) { _autotune = on; }
+      /// run the coarse levels as one LDS-resident workgroup where they qualify (default on;
+      /// the solution is bit-identical either way)
+      void set_coarse_fusion(bool on)
+      {
+        _tail_on = on;
+        _tail_built = false;
+      }
+      Index coarse_fusion_level() { if(!_tail_built) _build_tail(); return _tail_t; }
 
       void init_symbolic() override
       {
@@ -569,6 +639,8 @@ namespace feat3amd
         const Index c = L - 1;
         feat_handle_t hd = Runtime::handle();
         const feat_reduce_ws_t* ws = &_fw.ws;
+        if(!_tail_built) _build_tail();
+        const Index t = _tail_t ? _tail_t : c;   // levels below t run in the fused tail (when _tail_t)
         auto rhs_of = [&](Index i) -> const double* {
           return i == 0 ? (const double*)def.elements() : (const double*)_h->levels[i].rhs.elements();
         };
@@ -577,7 +649,7 @@ namespace feat3amd
         };
         auto nrm_of = [&](Index i) -> const double* { return i == 0 ? def_nrm2 : _fw.scalar(2 + i); };
         // restriction leg (multigrid.hpp:1713-1840)
-        for(Index i = 0; i < c; ++i)
+        for(Index i = 0; i < t; ++i)
         {
           auto& l = _h->levels[i];
           auto* pre = static_cast<RichType*>(l.pre.get());
@@ -593,7 +665,10 @@ namespace feat3amd
                                       lc.filter->mask(), _fw.scalar(2 + i + 1), nullptr, nullptr, ws),
                 "MultiGrid restriction");
         }
+        if(_tail_t)
+          check(feat_mg_tail_f64(hd, &_tail, rhs_of(t), nrm_of(t), sol_of(t)), "MultiGrid fused coarse levels");
         // coarse solve (multigrid.hpp:1526-1589)
+        else
         {
           auto& l = _h->levels[c];
           double* xs = sol_of(c);
@@ -636,7 +711,7 @@ namespace feat3amd
           }
         }
         // prolongation leg (multigrid.hpp:1845-2000)
-        for(Index k = c; k-- > 0;)
+        for(Index k = t; k-- > 0;)
         {
           auto& l = _h->levels[k];
           auto* post = static_cast<RichType*>(l.post.get());

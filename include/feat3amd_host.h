@@ -82,6 +82,7 @@ typedef struct {
                              2: the FP32 cycle on FP16-valued matrices */
   int no_graphs;          /* 1: launch every kernel eagerly instead of replaying HIP graphs */
   int no_autotune;        /* 1: default SELL-64 launch shapes (no autotuning at init) */
+  int no_coarse_fusion;   /* 1: launch every coarse-level operation separately (feat_mg_tail_f64 off) */
 } feat_mgpcg_cfg_t;
 
 typedef struct {          /* one level, host arrays; P maps the next coarser level onto this one */

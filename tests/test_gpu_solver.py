@@ -488,3 +488,30 @@ def test_mgpcg_dictionary_vs_plain_plans(feat, torch_gpu, precision):
         out.append((r["iters"], host(x)))
     assert out[0][0] == out[1][0]
     assert np.array_equal(out[0][1], out[1][1])
+
+
+@pytest.mark.parametrize("dim,lf,lc", [(3, 5, 1), (2, 7, 1), (3, 7, 1)])
+def test_coarse_fusion_bitwise(feat, torch_gpu, dim, lf, lc):
+    """The coarse levels run as one LDS-resident workgroup (feat_mg_tail_f64) give the
+    solution and defect history of the separate launches bit for bit."""
+    print(f"coarse fusion dim={dim} levels {lf}..{lc}", flush=True)
+    torch = torch_gpu
+    q = feat.q1_problem(dim, lf)
+    b = feat.rhs_random(11, q["n"], -1.0, 1.0)
+    b[q["filt"].astype(np.int64)] = 0.0
+    bd = torch.from_numpy(b).cuda()
+    out = []
+    for fusion in (True, False):
+        cfg = feat.mgpcg_cfg(nu_pre=4, nu_post=4, omega_jac=0.5, omega_rich=1.0, coarse="direct",
+                             tol_rel=1e-8, max_iter=100, graphs=False, coarse_fusion=fusion)
+        s = feat.MgPcg(cfg, q1=(dim, lf, lc))
+        x = torch.zeros(q["n"], dtype=torch.float64, device="cuda")
+        r = s.solve(x, bd)
+        torch.cuda.synchronize()
+        s.close()
+        assert r["status"] == "success"
+        out.append((r, x.cpu().numpy()))
+    (r1, x1), (r0, x0) = out
+    assert r1["iters"] == r0["iters"]
+    assert np.array_equal(r1["hist"].view(np.uint64), r0["hist"].view(np.uint64))
+    assert np.array_equal(x1.view(np.uint64), x0.view(np.uint64))
