@@ -139,6 +139,7 @@ KERNEL_API = [
     ("feat_matrix_apply_f64", _i, [_vp, C.POINTER(Matrix), _vp, _d, _vp, _d, _vp]),
     ("feat_q1_device_sizes", _i, [_i, _i, _vp]),
     ("feat_mg_tail_f64", _i, [_vp, _vp, _vp, _vp, _vp]),
+    ("feat_mg_tail_f32", _i, [_vp, _vp, _vp, _vp, _vp]),
     ("feat_sor_init_symbolic", _i, [_vp, C.POINTER(CsrF64), _i, C.POINTER(_vp)]),
     ("feat_sor_init_numeric", _i, [_vp, _vp, C.POINTER(CsrF64)]),
     ("feat_sor_colors", _i, [_vp, _i, C.POINTER(_i), _vp, _vp]),
@@ -254,6 +255,7 @@ HOST_API = [
     ("feat_mgpcg_solve", _i, [_vp, _vp, _vp, C.POINTER(SolveResult), _vp, _u64]),
     ("feat_mgpcg_matrix", _i, [_vp, _i, C.POINTER(_u64), C.POINTER(_u64), C.POINTER(_vp), C.POINTER(_vp),
                                C.POINTER(_vp)]),
+    ("feat_mgpcg_coarse_fusion_level", _i, [_vp, C.POINTER(_i)]),
     ("feat_mgpcg_destroy", _i, [_vp]),
     ("feat_runtime_init", _i, [_i, _vp]),
     ("feat_runtime_sell_flags", _i, [_i]),
@@ -1050,6 +1052,12 @@ class MgPcg:
         check(self.hl.feat_mgpcg_matrix(self.s, lvl, C.byref(rows), C.byref(nnz), C.byref(v),
                                         C.byref(c), C.byref(r)), "feat_mgpcg_matrix")
         return CsrF64(rows.value, rows.value, nnz.value, v.value, c.value, r.value)
+
+    def coarse_fusion_level(self):
+        """First level of the coarse levels fused into one workgroup (0: none)."""
+        lv = C.c_int(0)
+        check(self.hl.feat_mgpcg_coarse_fusion_level(self.s, C.byref(lv)), "feat_mgpcg_coarse_fusion_level")
+        return lv.value
 
     def close(self):
         if self.s:

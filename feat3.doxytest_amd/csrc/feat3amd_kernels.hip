@@ -2230,9 +2230,10 @@ namespace {
 
 constexpr int kTailThreads = 1024;
 
+template<typename DT>
 struct TailBufs
 {
-  double *rhs, *sol, *tmp, *def, *cor;
+  DT *rhs, *sol, *tmp, *def, *cor;
 };
 
 // One thread per row on every level of the tail (at most kTailThreads rows), so each
@@ -2240,19 +2241,20 @@ struct TailBufs
 // for the whole V-cycle: the smoothing and defect sweeps then touch only LDS.
 constexpr int kTailSmoothed = FEAT_MG_TAIL_MAX_LEVELS - 1;
 
+template<typename DT>
 struct TailRow
 {
   uint32_t wd[8];
-  double inv;
+  DT inv;
   bool masked;
 };
 
 // sum of the thread's DIAG-plan row (slot order = CSR order) with x in LDS
-__device__ __forceinline__ double tail_row_a(const TailRow R, const int* meta, const double* tab, uint64_t i,
-                                             const double* x)
+template<typename DT>
+__device__ __forceinline__ DT tail_row_a(const TailRow<DT> R, const int* meta, const DT* tab, uint64_t i, const DT* x)
 {
   const int nt = meta[0];
-  double sum = 0.0;
+  DT sum = DT(0);
 #pragma unroll
   for(int g = 0; g < kMaxTrip; ++g)
     if(g < nt)
@@ -2269,20 +2271,23 @@ __device__ __forceinline__ double tail_row_a(const TailRow R, const int* meta, c
   return sum;
 }
 
-__device__ __forceinline__ double tail_row_csr(const double* val, const uint64_t* col, const uint64_t* rp, uint64_t r,
-                                               const double* x)
+// sum of a transfer row from its FP64 CSR arrays; an FP32 cycle rounds each value to FP32
+// as its FP32 plan does
+template<typename DT>
+__device__ __forceinline__ DT tail_row_csr(const double* val, const uint64_t* col, const uint64_t* rp, uint64_t r,
+                                           const DT* x)
 {
   // 8 entries in flight, then the fmas in entry order
   const uint64_t k1 = rp[r + 1];
-  double sum = 0.0;
+  DT sum = DT(0);
   for(uint64_t k = rp[r]; k < k1; k += 8)
   {
-    double v[8];
+    DT v[8];
     uint64_t c[8];
 #pragma unroll
     for(int u = 0; u < 8; ++u)
     {
-      v[u] = k + u < k1 ? val[k + u] : 0.0;
+      v[u] = k + u < k1 ? DT(val[k + u]) : DT(0);
       c[u] = k + u < k1 ? col[k + u] : 0;
     }
 #pragma unroll
@@ -2304,30 +2309,33 @@ __device__ double tail_sum(double v, double* red)
   return t;
 }
 
+// the skip rule of smoother_skips with the norm already summed
+template<typename DT>
 __device__ __forceinline__ bool tail_skips(double nrm2)
 {
   const double d = sqrt(nrm2);
-  return !isfinite(d) || d <= DBL_EPSILON * DBL_EPSILON;
+  const double eps = std::is_same<DT, float>::value ? double(FLT_EPSILON) : DBL_EPSILON;
+  return !isfinite(d) || d <= eps * eps;
 }
 
 // Richardson(Jacobi) with `steps` steps from x = 0 on rhs (smooth_impl): with def the last
 // step writes the defect of x_steps into def and x_steps ends in b0; else the last update is
 // accumulated into acc.  b0/b1 are the ping-pong pair.  Thread = row `live ? threadIdx.x : -`.
-__device__ __forceinline__ void tail_smooth(const TailRow R, bool live, const int* meta, const double* tab, int steps,
-                                            double omega, bool skip, const double* rhs, double* b0, double* b1,
-                                            double* def, double* acc)
+template<typename DT>
+__device__ __forceinline__ void tail_smooth(const TailRow<DT> R, bool live, const int* meta, const DT* tab, int steps,
+                                            DT omega, bool skip, const DT* rhs, DT* b0, DT* b1, DT* def, DT* acc)
 {
   const uint64_t i = threadIdx.x;
   auto B = [&](int k) { return ((steps - k) & 1) ? b1 : b0; };
   const bool acc_first = acc != nullptr && steps == 1;
   if(live)
   {
-    double c = R.inv * rhs[i];
-    if(R.masked) c = 0.0;
-    double xn = dfma(omega, c, 0.0);
-    if(skip) xn = 0.0;
+    DT c = R.inv * rhs[i];
+    if(R.masked) c = DT(0);
+    DT xn = dfma(omega, c, DT(0));
+    if(skip) xn = DT(0);
     if(acc_first)
-      acc[i] = dfma(1.0, xn, acc[i]);
+      acc[i] = dfma(DT(1), xn, acc[i]);
     else
       B(1)[i] = xn;
   }
@@ -2335,41 +2343,44 @@ __device__ __forceinline__ void tail_smooth(const TailRow R, bool live, const in
   const int nsp = def ? steps : steps - 1;
   for(int k = 1; k <= nsp; ++k)
   {
-    const double* xk = B(k);
+    const DT* xk = B(k);
     const int mode = (def && k == steps) ? 1 : ((acc && k == steps - 1) ? 2 : 0);
-    double* out = mode == 1 ? def : (mode == 2 ? acc : B(k + 1));
+    DT* out = mode == 1 ? def : (mode == 2 ? acc : B(k + 1));
     if(live)
     {
-      double d = dfma(tail_row_a(R, meta, tab, i, xk), -1.0, rhs[i]);
-      if(R.masked) d = 0.0;
+      DT d = dfma(tail_row_a(R, meta, tab, i, xk), DT(-1), rhs[i]);
+      if(R.masked) d = DT(0);
       if(mode == 1)
         out[i] = d;
       else
       {
-        double c = R.inv * d;
-        if(R.masked) c = 0.0;
-        double xn = dfma(omega, c, xk[i]);
-        if(skip) xn = 0.0;
-        out[i] = mode == 0 ? xn : dfma(1.0, xn, out[i]);
+        DT c = R.inv * d;
+        if(R.masked) c = DT(0);
+        DT xn = dfma(omega, c, xk[i]);
+        if(skip) xn = DT(0);
+        out[i] = mode == 0 ? xn : dfma(DT(1), xn, out[i]);
       }
     }
     __syncthreads();
   }
 }
 
-__global__ __launch_bounds__(kTailThreads) void k_mg_tail(feat_mg_tail_t t, const double* __restrict__ rhs_top,
-                                                          const double* __restrict__ nrm_top, double* __restrict__ sol_top)
+// TT: feat_mg_tail_t (FP64 cycle) or feat_mg_tail_f32_t (FP32 cycle of the mixed multigrid)
+template<typename DT, typename TT>
+__global__ __launch_bounds__(kTailThreads) void k_mg_tail(TT t, const DT* __restrict__ rhs_top,
+                                                          const double* __restrict__ nrm_top, DT* __restrict__ sol_top)
 {
-  extern __shared__ double lds[];
+  extern __shared__ double lds_words[];
   __shared__ double red[kTailThreads / 64];
+  DT* lds = reinterpret_cast<DT*>(lds_words);
   const int nl = t.nlevels, c = nl - 1;
   const uint64_t tid = threadIdx.x;
-  TailBufs b[FEAT_MG_TAIL_MAX_LEVELS];
-  TailRow row[kTailSmoothed];
+  TailBufs<DT> b[FEAT_MG_TAIL_MAX_LEVELS];
+  TailRow<DT> row[kTailSmoothed];
   const int* meta[kTailSmoothed];
-  double* tabs = lds;   // the smoothed levels' pair-table values, 256 each
+  DT* tabs = lds;   // the smoothed levels' pair-table values, 256 each
   {
-    double* p = lds + 256 * c;
+    DT* p = lds + 256 * c;
 #pragma unroll
     for(int l = 0; l < FEAT_MG_TAIL_MAX_LEVELS; ++l)
       if(l < nl)
@@ -2391,7 +2402,7 @@ __global__ __launch_bounds__(kTailThreads) void k_mg_tail(feat_mg_tail_t t, cons
   for(int l = 0; l < kTailSmoothed; ++l)
     if(l < c)
     {
-      const feat_mg_tail_level_t& L = t.lv[l];
+      const auto& L = t.lv[l];
       meta[l] = reinterpret_cast<const int*>(L.table + 256) + 256;
       for(int e = threadIdx.x; e < 256; e += kTailThreads) tabs[256 * l + e] = L.table[e];
       if(tid < L.n)
@@ -2412,22 +2423,22 @@ __global__ __launch_bounds__(kTailThreads) void k_mg_tail(feat_mg_tail_t t, cons
   for(int l = 0; l < kTailSmoothed; ++l)
     if(l < c)
     {
-      const feat_mg_tail_level_t& L = t.lv[l];
+      const auto& L = t.lv[l];
       const bool live = tid < L.n;
-      tail_smooth(row[l], live, meta[l], tabs + 256 * l, L.pre_steps, L.pre_omega, tail_skips(nrm[l]), b[l].rhs,
-                  b[l].sol, b[l].tmp, b[l].def, nullptr);
-      const feat_mg_tail_level_t& C = t.lv[l + 1];
+      tail_smooth(row[l], live, meta[l], tabs + 256 * l, L.pre_steps, L.pre_omega, tail_skips<DT>(nrm[l]), b[l].rhs,
+                  b[l].sol, b[l].tmp, b[l].def, (DT*)nullptr);
+      const auto& C = t.lv[l + 1];
       double part = 0.0;
       if(tid < C.n)
       {
-        double v = dfma(tail_row_csr(L.r_val, L.r_col, L.r_ptr, tid, b[l].def), 1.0, 0.0);
-        if(is_masked(C.mask, tid)) v = 0.0;
+        DT v = dfma(tail_row_csr(L.r_val, L.r_col, L.r_ptr, tid, b[l].def), DT(1), DT(0));
+        if(is_masked(C.mask, tid)) v = DT(0);
         b[l + 1].rhs[tid] = v;
-        part = v * v;
+        part = double(v) * double(v);
       }
       nrm[l + 1] = tail_sum(part, red);   // also orders the rhs stores before their use
     }
-  // coarsest level: dense inverse
+  // coarsest level: dense inverse, summed in FP64 (feat_dense_gemv_f64 / _mixed)
   {
     const int n = int(t.lv[c].n);
     if(int(tid) < n)
@@ -2441,10 +2452,10 @@ __global__ __launch_bounds__(kTailThreads) void k_mg_tail(feat_mg_tail_t t, cons
         for(int u = 0; u < 8; ++u) a[u] = k + u < n ? ar[k + u] : 0.0;
 #pragma unroll
         for(int u = 0; u < 8; ++u)
-          if(k + u < n) sv = dfma(a[u], b[c].rhs[k + u], sv);
+          if(k + u < n) sv = dfma(a[u], double(b[c].rhs[k + u]), sv);
       }
       if(is_masked(t.lv[c].mask, tid)) sv = 0.0;
-      b[c].sol[tid] = sv;
+      b[c].sol[tid] = DT(sv);
     }
     __syncthreads();
   }
@@ -2453,28 +2464,50 @@ __global__ __launch_bounds__(kTailThreads) void k_mg_tail(feat_mg_tail_t t, cons
   for(int l = kTailSmoothed - 1; l >= 0; --l)
     if(l < c)
     {
-      const feat_mg_tail_level_t& L = t.lv[l];
+      const auto& L = t.lv[l];
       const bool live = tid < L.n;
       if(live)
       {
-        double cc = dfma(tail_row_csr(L.p_val, L.p_col, L.p_ptr, tid, b[l + 1].sol), 1.0, 0.0);
-        if(row[l].masked) cc = 0.0;
-        b[l].sol[tid] = dfma(1.0, cc, b[l].sol[tid]);
+        DT cc = dfma(tail_row_csr(L.p_val, L.p_col, L.p_ptr, tid, b[l + 1].sol), DT(1), DT(0));
+        if(row[l].masked) cc = DT(0);
+        b[l].sol[tid] = dfma(DT(1), cc, b[l].sol[tid]);
       }
       __syncthreads();
       double part = 0.0;
       if(live)
       {
-        double v = dfma(tail_row_a(row[l], meta[l], tabs + 256 * l, tid, b[l].sol), -1.0, b[l].rhs[tid]);
-        if(row[l].masked) v = 0.0;
+        DT v = dfma(tail_row_a(row[l], meta[l], tabs + 256 * l, tid, b[l].sol), DT(-1), b[l].rhs[tid]);
+        if(row[l].masked) v = DT(0);
         b[l].def[tid] = v;
-        part = v * v;
+        part = double(v) * double(v);
       }
       const double dn = tail_sum(part, red);
-      tail_smooth(row[l], live, meta[l], tabs + 256 * l, L.post_steps, L.post_omega, tail_skips(dn), b[l].def,
-                  b[l].cor, b[l].tmp, nullptr, b[l].sol);
+      tail_smooth(row[l], live, meta[l], tabs + 256 * l, L.post_steps, L.post_omega, tail_skips<DT>(dn), b[l].def,
+                  b[l].cor, b[l].tmp, (DT*)nullptr, b[l].sol);
     }
   if(tid < t.lv[0].n) sol_top[tid] = b[0].sol[tid];
+}
+
+template<typename DT, typename TT>
+int mg_tail_launch(feat_handle_t h, const TT* t, const DT* rhs, const double* rhs_nrm2, DT* sol, const char* name)
+{
+  if(!h || !t || !rhs || !rhs_nrm2 || !sol || t->nlevels < 2 || t->nlevels > FEAT_MG_TAIL_MAX_LEVELS ||
+     !t->coarse_inverse)
+    return fail(FEAT_ERR_INVALID, name);
+  uint64_t words = 0;
+  for(int l = 0; l < t->nlevels; ++l)
+  {
+    const auto& L = t->lv[l];
+    words += (l + 1 < t->nlevels ? 5 * L.n + 256 : 2 * L.n);
+    if(L.n > uint64_t(kTailThreads)) return fail(FEAT_ERR_INVALID, "fused coarse levels: a level has more than 1024 rows");
+    if(l + 1 < t->nlevels &&
+       (!L.table || !L.codes || L.words > 8 || !L.inv_diag || !L.r_val || !L.r_col || !L.r_ptr || !L.p_val ||
+        !L.p_col || !L.p_ptr || L.pre_steps < 1 || L.post_steps < 1))
+      return fail(FEAT_ERR_INVALID, "fused coarse levels: incomplete level");
+  }
+  if(words * sizeof(DT) > FEAT_MG_TAIL_MAX_LDS) return fail(FEAT_ERR_INVALID, "fused coarse levels: levels exceed the LDS budget");
+  hipLaunchKernelGGL((k_mg_tail<DT, TT>), dim3(1), dim3(kTailThreads), words * sizeof(DT), h->stream, *t, rhs, rhs_nrm2, sol);
+  return check_launch(name);
 }
 
 } // namespace
@@ -2482,21 +2515,11 @@ __global__ __launch_bounds__(kTailThreads) void k_mg_tail(feat_mg_tail_t t, cons
 extern "C" int feat_mg_tail_f64(feat_handle_t h, const feat_mg_tail_t* t, const double* rhs, const double* rhs_nrm2,
                                 double* sol)
 {
-  if(!h || !t || !rhs || !rhs_nrm2 || !sol || t->nlevels < 2 || t->nlevels > FEAT_MG_TAIL_MAX_LEVELS ||
-     !t->coarse_inverse)
-    return fail(FEAT_ERR_INVALID, "feat_mg_tail_f64: bad argument");
-  uint64_t doubles = 0;
-  for(int l = 0; l < t->nlevels; ++l)
-  {
-    const feat_mg_tail_level_t& L = t->lv[l];
-    doubles += (l + 1 < t->nlevels ? 5 * L.n + 256 : 2 * L.n);
-    if(L.n > uint64_t(kTailThreads)) return fail(FEAT_ERR_INVALID, "feat_mg_tail_f64: a level has more than 1024 rows");
-    if(l + 1 < t->nlevels &&
-       (!L.table || !L.codes || L.words > 8 || !L.inv_diag || !L.r_val || !L.r_col || !L.r_ptr || !L.p_val ||
-        !L.p_col || !L.p_ptr || L.pre_steps < 1 || L.post_steps < 1))
-      return fail(FEAT_ERR_INVALID, "feat_mg_tail_f64: incomplete level");
-  }
-  if(doubles * sizeof(double) > FEAT_MG_TAIL_MAX_LDS) return fail(FEAT_ERR_INVALID, "feat_mg_tail_f64: levels exceed the LDS budget");
-  hipLaunchKernelGGL(k_mg_tail, dim3(1), dim3(kTailThreads), doubles * sizeof(double), h->stream, *t, rhs, rhs_nrm2, sol);
-  return check_launch("feat_mg_tail_f64");
+  return mg_tail_launch(h, t, rhs, rhs_nrm2, sol, "feat_mg_tail_f64");
+}
+
+extern "C" int feat_mg_tail_f32(feat_handle_t h, const feat_mg_tail_f32_t* t, const float* rhs, const double* rhs_nrm2,
+                                float* sol)
+{
+  return mg_tail_launch(h, t, rhs, rhs_nrm2, sol, "feat_mg_tail_f32");
 }

@@ -300,6 +300,7 @@ static void build_solver(feat_mgpcg_s* s, const feat_mgpcg_cfg_t* cfg)
   if(cfg->precision == 1 || cfg->precision == 2)
   {
     s->mg32 = std::make_shared<MG32>(s->hier, cfg->precision == 2);
+    s->mg32->set_coarse_fusion(cfg->no_coarse_fusion == 0);
     s->pcg = Solver::new_pcg(s->levels[0]->A, s->levels[0]->F, s->mg32);
   }
   else
@@ -553,6 +554,14 @@ int feat_krylov_destroy(feat_krylov_t s)
   return guarded([&]() {
     if(s && s->solver) s->solver->done();
     delete s;
+  });
+}
+
+int feat_mgpcg_coarse_fusion_level(feat_mgpcg_t s, int* level)
+{
+  return guarded([&]() {
+    if(!s || !level) throw Error("feat_mgpcg_coarse_fusion_level: bad argument");
+    *level = int(s->mg32 ? s->mg32->coarse_fusion_level() : (s->mg ? s->mg->coarse_fusion_level() : 0));
   });
 }
 

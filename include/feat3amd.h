@@ -613,6 +613,28 @@ typedef struct {
   const double* coarse_inverse;   /* n x n row-major inverse of the filtered coarsest matrix */
 } feat_mg_tail_t;
 int feat_mg_tail_f64(feat_handle_t h, const feat_mg_tail_t* t, const double* rhs, const double* rhs_nrm2, double* sol);
+/* The same for the FP32 V-cycle of the mixed-precision multigrid (feat_rj_smooth_f32,
+ * feat_apply_masked_f32, feat_dense_gemv_mixed, feat_prol_add_f32, feat_defect_f32
+ * semantics): FP32 pair tables (the FP32 FEAT_IDX_DIAG plans), FP32 inverted diagonals,
+ * omegas and vectors; the transfers' FP64 CSR values are rounded to FP32 as the FP32
+ * plans round them; the coarse inverse is applied in FP64 and rounded once.  The caller
+ * keeps 4 bytes x (5 n + 256 over the smoothed levels + 2 n coarse) within
+ * FEAT_MG_TAIL_MAX_LDS bytes. */
+typedef struct {
+  uint64_t n;
+  const float* table; const uint32_t* codes; uint32_t words;   /* FP32 FEAT_IDX_DIAG plan of A */
+  const float* inv_diag; const uint32_t* mask;
+  const double* r_val; const uint64_t* r_col; const uint64_t* r_ptr;
+  const double* p_val; const uint64_t* p_col; const uint64_t* p_ptr;
+  int pre_steps, post_steps;
+  float pre_omega, post_omega;
+} feat_mg_tail_level_f32_t;
+typedef struct {
+  int nlevels;
+  feat_mg_tail_level_f32_t lv[FEAT_MG_TAIL_MAX_LEVELS];
+  const double* coarse_inverse;
+} feat_mg_tail_f32_t;
+int feat_mg_tail_f32(feat_handle_t h, const feat_mg_tail_f32_t* t, const float* rhs, const double* rhs_nrm2, float* sol);
 
 /* ---------------- autotuner ----------------
  * Launch shapes of the SELL-64 row kernels that every fused entry point uses

@@ -41,6 +41,9 @@ namespace feat3amd
       enum class Coarse { direct, jacobi, richardson, none } _coarse = Coarse::none;
       bool _ok = false;
       bool _half = false;
+      bool _tail_on = true, _tail_built = false;
+      Index _tail_t = 0;   // first level of the fused coarse part; 0: none
+      feat_mg_tail_f32_t _tail{};
 
       // execution plan of a level's matrix: FP32 values or FP16 values (FP32 arithmetic)
       const feat_matrix_f32_t& plan(const Matrix_& m) const { return _half ? m.exec16() : m.exec32(); }
@@ -52,11 +55,86 @@ namespace feat3amd
         return r;
       }
 
+      /// The coarse levels that run as one LDS-resident workgroup (feat_mg_tail_f32): the
+      /// longest run ending at the coarsest level, starting at level 1 or below, whose
+      /// vectors fit the LDS budget, with FP32 diagonal-dictionary plans, FP32-valued transfer
+      /// plans (an FP16-valued cycle uses FP32 plans where the matrix has a dictionary plan)
+      /// and a direct coarse solve.
+      void _build_tail()
+      {
+        _tail_built = true;
+        _tail_t = 0;
+        const Index L = _h->size(), c = L - 1;
+        if(!_tail_on || _coarse != Coarse::direct || L < 3 || _h->levels[c].matrix->rows() > FEAT_MG_TAIL_MAX_ROWS)
+          return;
+        uint64_t words = 2 * _h->levels[c].matrix->rows();
+        Index t = c;
+        for(Index s = c; s-- > 1;)
+        {
+          const auto& l = _h->levels[s];
+          const feat_matrix_f32_t& A = plan(*l.matrix);
+          const bool f16 = plan(l.transfer->get_mat_rest()).value_kind == FEAT_VAL_F16 ||
+                           plan(l.transfer->get_mat_prol()).value_kind == FEAT_VAL_F16;
+          const uint64_t need = words + 5 * l.matrix->rows() + 256;   // vectors and the pair table
+          if(f16 || A.index != FEAT_IDX_DIAG || A.slice_words > 8 || need * sizeof(float) > FEAT_MG_TAIL_MAX_LDS ||
+             c - s + 1 > FEAT_MG_TAIL_MAX_LEVELS || l.matrix->rows() > FEAT_MG_TAIL_MAX_ROWS)
+            break;
+          words = need;
+          t = s;
+        }
+        if(t == c) return;
+        _tail = feat_mg_tail_f32_t{};
+        _tail.nlevels = int(c - t + 1);
+        for(Index s = t; s <= c; ++s)
+        {
+          auto& lv = _h->levels[s];
+          feat_mg_tail_level_f32_t& d = _tail.lv[s - t];
+          d.n = lv.matrix->rows();
+          d.mask = lv.filter->mask();
+          if(s == c) break;
+          const feat_matrix_f32_t& A = plan(*lv.matrix);
+          d.table = static_cast<const float*>(A.val);
+          d.codes = static_cast<const uint32_t*>(A.col);
+          d.words = A.slice_words;
+          // one inverted diagonal serves both smoothers: their FP64 sources must be one vector
+          auto* pre = static_cast<RichType*>(lv.pre.get());
+          auto* post = static_cast<RichType*>(lv.post.get());
+          if(post->fusable_jacobi()->inv_diag().elements() != pre->fusable_jacobi()->inv_diag().elements()) return;
+          d.inv_diag = _l[s].inv_pre.elements();
+          const Matrix_& R = lv.transfer->get_mat_rest();
+          const Matrix_& P = lv.transfer->get_mat_prol();
+          d.r_val = (const double*)R.val();
+          d.r_col = (const uint64_t*)R.col_ind();
+          d.r_ptr = (const uint64_t*)R.row_ptr();
+          d.p_val = (const double*)P.val();
+          d.p_col = (const uint64_t*)P.col_ind();
+          d.p_ptr = (const uint64_t*)P.row_ptr();
+          d.pre_steps = _l[s].pre_steps;
+          d.post_steps = _l[s].post_steps;
+          d.pre_omega = _l[s].omega_pre;
+          d.post_omega = _l[s].omega_post;
+        }
+        _tail.coarse_inverse = static_cast<DirectType*>(_h->levels[c].coarse.get())->inverse();
+        _tail_t = t;
+      }
+
     public:
       /// half = true stores the matrices of the cycle as FP16 (FP32 arithmetic)
       explicit MixedMultiGrid(std::shared_ptr<HierarchyType> h, bool half = false) : _h(h), _half(half) {}
       std::string name() const override { return _half ? "MultiGrid-V(FP16/FP32)" : "MultiGrid-V(FP32)"; }
       bool fusable() const override { return _ok; }
+      /// run the coarse levels as one LDS-resident workgroup where they qualify (default on)
+      void set_coarse_fusion(bool on)
+      {
+        _tail_on = on;
+        _tail_built = false;
+      }
+      /// first level of the fused coarse part (0: none)
+      Index coarse_fusion_level()
+      {
+        if(!_tail_built) _build_tail();
+        return _tail_t;
+      }
 
       void init_symbolic() override
       {
@@ -130,6 +208,7 @@ namespace feat3amd
           _l[L - 1].pre_steps = int(r->get_max_iter());
           _l[L - 1].omega_pre = float(r->omega());
         }
+        _tail_built = false;   // the tail holds pointers to the vectors made here
       }
 
       Status apply(VectorType& cor, const VectorType& def) override
@@ -148,8 +227,10 @@ namespace feat3amd
         feat_handle_t hd = Runtime::handle();
         const feat_reduce_ws_t* ws = &_fw.ws;
         auto nrm_of = [&](Index i) -> const double* { return i == 0 ? def_nrm2 : _fw.scalar(2 + i); };
+        if(!_tail_built) _build_tail();
+        const Index t = _tail_t ? _tail_t : c;   // levels from t down run in the fused tail (when _tail_t)
         check(feat_convert_f64_f32(hd, _l[0].rhs.elements(), (const double*)def.elements(), def.size()), "to FP32");
-        for(Index i = 0; i < c; ++i)
+        for(Index i = 0; i < t; ++i)
         {
           auto& lv = _h->levels[i];
           Level32& l = _l[i];
@@ -163,6 +244,10 @@ namespace feat3amd
                                       _fw.scalar(2 + i + 1), ws),
                 "FP32 restriction");
         }
+        if(_tail_t)
+          check(feat_mg_tail_f32(hd, &_tail, _l[t].rhs.elements(), nrm_of(t), _l[t].sol.elements()),
+                "FP32 fused coarse levels");
+        else
         {
           auto& lv = _h->levels[c];
           Level32& l = _l[c];
@@ -194,7 +279,7 @@ namespace feat3amd
             throw Error("MixedMultiGrid: coarse solver not supported");
           }
         }
-        for(Index k = c; k-- > 0;)
+        for(Index k = t; k-- > 0;)
         {
           auto& lv = _h->levels[k];
           Level32& l = _l[k];

@@ -122,6 +122,9 @@ int feat_mgpcg_solve(feat_mgpcg_t s, double* x, const double* b, feat_solve_resu
 int feat_mgpcg_write_matrix(feat_mgpcg_t s, int lvl, int mode, const char* path, int symmetric);
 int feat_mgpcg_matrix(feat_mgpcg_t s, int lvl, uint64_t* rows, uint64_t* nnz, const double** val,
                       const uint64_t** col, const uint64_t** rowptr);
+/* first level of the coarse levels that run fused as one workgroup (feat_mg_tail_f64 /
+ * feat_mg_tail_f32) in s's V-cycle; 0 = none.  No counterpart in the reference. */
+int feat_mgpcg_coarse_fusion_level(feat_mgpcg_t s, int* level);
 int feat_mgpcg_destroy(feat_mgpcg_t s);
 
 /* ---- single Krylov solver with a device preconditioner ----

@@ -490,11 +490,14 @@ def test_mgpcg_dictionary_vs_plain_plans(feat, torch_gpu, precision):
     assert np.array_equal(out[0][1], out[1][1])
 
 
-@pytest.mark.parametrize("dim,lf,lc", [(3, 5, 1), (2, 7, 1), (3, 7, 1)])
-def test_coarse_fusion_bitwise(feat, torch_gpu, dim, lf, lc):
-    """The coarse levels run as one LDS-resident workgroup (feat_mg_tail_f64) give the
-    solution and defect history of the separate launches bit for bit."""
-    print(f"coarse fusion dim={dim} levels {lf}..{lc}", flush=True)
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+@pytest.mark.parametrize("dim,lf,lc,first", [(3, 5, 1, 2), (2, 7, 1, 3), (3, 7, 1, 4)])
+def test_coarse_fusion_bitwise(feat, torch_gpu, dim, lf, lc, first, precision):
+    """The coarse levels run as one LDS-resident workgroup (feat_mg_tail_f64, or
+    feat_mg_tail_f32 in the FP32 cycle of the mixed multigrid) give the solution and
+    defect history of the separate launches bit for bit.  `first` is the first fused
+    level: the finest level with at most 1,024 rows."""
+    print(f"coarse fusion {precision} dim={dim} levels {lf}..{lc}", flush=True)
     torch = torch_gpu
     q = feat.q1_problem(dim, lf)
     b = feat.rhs_random(11, q["n"], -1.0, 1.0)
@@ -503,8 +506,10 @@ def test_coarse_fusion_bitwise(feat, torch_gpu, dim, lf, lc):
     out = []
     for fusion in (True, False):
         cfg = feat.mgpcg_cfg(nu_pre=4, nu_post=4, omega_jac=0.5, omega_rich=1.0, coarse="direct",
-                             tol_rel=1e-8, max_iter=100, graphs=False, coarse_fusion=fusion)
+                             tol_rel=1e-8, max_iter=100, graphs=False, coarse_fusion=fusion,
+                             precision=precision)
         s = feat.MgPcg(cfg, q1=(dim, lf, lc))
+        assert s.coarse_fusion_level() == (first if fusion else 0)
         x = torch.zeros(q["n"], dtype=torch.float64, device="cuda")
         r = s.solve(x, bd)
         torch.cuda.synchronize()
