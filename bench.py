@@ -33,7 +33,7 @@ PEAK_HBM_GBS = 8000.0
 # runs of `bench.py --spmv-only` (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch); FETCH_SIZE
 # is scaled by 2.0 and WRITE_SIZE by 1.0, the factors that make the same counters read
 # 1 GiB for a 1 GiB device-to-device copy (profiles/spmv_traffic_3d_q1_l7.md)
-SPMV_TRAFFIC_BYTES = {"sell64_diag": 38730.1 * 2048 + 16771.0 * 1024,
+SPMV_TRAFFIC_BYTES = {"sell64_diag": 38762.8 * 2048 + 16771.0 * 1024,
                       "sell64_dict": 40574.2 * 2048 + 16771.0 * 1024,
                       "sell64": 295808.6 * 2048 + 16771.0 * 1024,
                       "csr_u64": 481390.4 * 2048 + 16771.0 * 1024}
@@ -604,6 +604,8 @@ def main():
 
     # ---- the same solver on plain SELL-64 plans (full FP64 values, i16 column offsets) and
     # on entry-order dictionary plans
+    plan_rel_diff = {}
+
     def solve_with_plans(flags, what):
         feat.runtime_sell_flags(flags)
         sp = feat.MgPcg(cfg, q1=(DIM, LF, LC))
@@ -619,9 +621,14 @@ def main():
             if k >= args.warmup:
                 tp.append(time.perf_counter() - t)
         sp.close()
-        assert rp_["iters"] == res["iters"] and bool(torch.equal(xp, x)), f"{what} and diagonal plans differ"
+        # every SpMV row is bit-identical across plans; the dots and norms are summed per
+        # workgroup chunk, whose rows differ (248 on diagonal plans, 256 on the others)
+        rd = float(torch.linalg.norm(xp - x) / torch.linalg.norm(x))
+        assert rp_["iters"] == res["iters"] and rd <= 1e-12, f"{what} and diagonal plans differ: {rd}"
+        plan_rel_diff[what] = rd
         tt = float(np.mean(tp))
-        log(f"{what}: {tt * 1e3:.2f} ms per solve (diagonal dictionary {t_mean * 1e3:.2f} ms), identical result")
+        log(f"{what}: {tt * 1e3:.2f} ms per solve (diagonal dictionary {t_mean * 1e3:.2f} ms), "
+            f"same iterations, |x - x_diag| / |x_diag| = {rd:.1e}")
         return tt
 
     t_plain = solve_with_plans(feat.SELL_NO_DICT, "plain SELL-64 plans")
@@ -772,6 +779,7 @@ def main():
             "launches": "eager (the graph replay of the same kernels is graph_replay_s_per_solve)",
             "plain_sell_plans_s_per_solve": t_plain,
             "entry_order_dictionary_plans_s_per_solve": t_entry,
+            "other_plans_rel_diff_x": plan_rel_diff,
             "default_launch_shapes_s_per_solve": t_untuned,
             "matrix_plan": "SELL-64 pair dictionary by diagonal (1 byte per offset slot, 9 triples of offsets, "
                            "42 distinct (col-row, value) pairs)",

@@ -406,10 +406,9 @@ __device__ __forceinline__ float wave_shift(float v)
 constexpr int kMaxTrip = 10;
 constexpr uint32_t kDiagNone = 255u;   // slot without an entry
 
-// SpMV-type row loop over the diagonal plan for launches without a dot or norm: the slots
-// run in ascending offset order (= ascending columns, the CSR order), each lane adds its
-// entry of a slot if it has one.  Metadata after the pair table: the number of triples,
-// their centres.  A wave covers
+// SpMV-type row loop over the diagonal plan: the slots run in ascending offset order (=
+// ascending columns, the CSR order), each lane adds its entry of a slot if it has one.
+// Metadata after the pair table: the number of triples, their centres.  A wave covers
 // kHaloRows = 62 rows with its lanes 1..62; lanes 0 and 63 only load x for the rows just
 // outside (base - 1 and base + 62), so the left / right value of every triple comes from
 // a neighbouring lane by a wave shift and no wave needs loads for its edges (3 % idle
@@ -477,80 +476,6 @@ __global__ __launch_bounds__(kThreads) void k_sell_diag_rows(const DT* __restric
   grid_reduce_finish(gr, acc0, acc1);
 }
 
-// The diagonal-plan row loop on the 256-row chunks of the other SELL-64 kernels, for
-// launches that reduce: one row per lane, lanes 0 and 63 also load the x values beyond
-// the wave.  A dot or norm is summed per chunk exactly as the plain and entry-order
-// dictionary kernels sum it, so it does not depend on the plan.
-template<typename DT, class Epi>
-__global__ __launch_bounds__(kThreads) void k_sell_diag_slices(const DT* __restrict__ tab,
-                                                             const uint32_t* __restrict__ codes, uint32_t nw,
-                                                             const DT* __restrict__ x, uint64_t rows, uint64_t cols,
-                                                             Epi epi, GridReduce gr)
-{
-  static_assert(kThreads == 256, "one table entry per thread");
-  __shared__ DT s_val[256];
-  const DT tv = tab[threadIdx.x];   // staged into LDS once the first chunk's loads are in flight
-  const int* meta = reinterpret_cast<const int*>(tab + 256) + 256;
-  const int nt = meta[0];
-  const uint64_t nch = (rows + kThreads - 1) / kThreads;
-  const XcdRange xr = xcd_range(nch);
-  const int lane = threadIdx.x & 63;
-  const std::int64_t last = std::int64_t(cols) - 1;   // x holds the columns (a wide matrix has more than rows)
-  const std::int64_t eo = lane == 0 ? -1 : 1;
-  double acc0 = 0.0, acc1 = 0.0;
-  epi.begin();
-  for(uint64_t ch = xr.first; ch < xr.end; ch += xr.stride)
-  {
-    // every lane of a live wave runs the slot loop (the shifts read all 64 lanes); rows
-    // past the end load clamped addresses and skip the epilogue; a wave wholly past the
-    // last slice does nothing but the block's staging barrier
-    const uint64_t row = ch * kThreads + threadIdx.x;
-    const bool live = ((row >> 6) << 6) < rows;
-    uint32_t wd[8];
-    DT cen[kMaxTrip], edge[kMaxTrip];
-    auto clampi = [&](std::int64_t i) { return i < 0 ? 0 : (i > last ? last : i); };
-    if(live)
-    {
-      const uint32_t* cw = codes + (row >> 6) * 64 * uint64_t(nw) + lane;
-#pragma unroll
-      for(int w = 0; w < 8; ++w) wd[w] = w < int(nw) ? __builtin_nontemporal_load(cw + w * 64) : 0xFFFFFFFFu;
-#pragma unroll
-      for(int g = 0; g < kMaxTrip; ++g)
-        if(g < nt)
-        {
-          const std::int64_t c = std::int64_t(row) + meta[1 + g];
-          cen[g] = x[clampi(c)];
-          edge[g] = (lane == 0 || lane == 63) ? x[clampi(c + eo)] : DT(0);
-        }
-    }
-    if(ch == xr.first)   // block-uniform: every wave of the block passes here once
-    {
-      s_val[threadIdx.x] = tv;
-      __syncthreads();
-    }
-    if(!live) continue;
-    DT sum = DT(0);
-#pragma unroll
-    for(int g = 0; g < kMaxTrip; ++g)
-      if(g < nt)
-      {
-        DT left = wave_shift<kDppWaveShr1>(cen[g]), right = wave_shift<kDppWaveShl1>(cen[g]);
-        if(lane == 0) left = edge[g];
-        if(lane == 63) right = edge[g];
-        const DT v3[3] = {left, cen[g], right};
-#pragma unroll
-        for(int j = 0; j < 3; ++j)
-        {
-          const int t = 3 * g + j;
-          const uint32_t c = (wd[t >> 2] >> (8 * (t & 3))) & 255u;
-          if(c != kDiagNone) sum = dfma(s_val[c], v3[j], sum);
-        }
-      }
-    if(row < rows) epi.row(row, sum, acc0, acc1);
-  }
-  grid_reduce_finish(gr, acc0, acc1);
-}
-
 // Runs Epi over the rows of a descriptor (CSR or SELL-64).
 template<typename DT, class Epi, class M>
 int launch_mat(hipStream_t s, const M* A, const DT* x, const Epi& epi, const GridReduce& gr, const char* name)
@@ -596,15 +521,8 @@ int launch_mat(hipStream_t s, const M* A, const DT* x, const Epi& epi, const Gri
     }
     if(A->index == FEAT_IDX_DIAG)
     {
-      if(reduces)
-      {
-        hipLaunchKernelGGL((k_sell_diag_slices<DT, Epi>), dim3(grid), dim3(kThreads), 0, s,
-                           reinterpret_cast<const DT*>(A->val), static_cast<const uint32_t*>(A->col), A->slice_words, x,
-                           A->rows, A->cols, epi, gd);
-        finish_reduce(s, gd, uint64_t(grid));
-        return check_launch(name);
-      }
-      // chunks of kHaloChunk rows (62 per wave)
+      // chunks of kHaloChunk rows (62 per wave); a dot or norm is summed per chunk, so it
+      // rounds differently from the same sum over the 256-row chunks of the other plans
       const uint64_t nh = (A->rows + kHaloChunk - 1) / kHaloChunk;
       const int gh = int(nh < cap ? nh : cap);
       const GridReduce gdh = reduce_for_grid(gr, uint64_t(gh));

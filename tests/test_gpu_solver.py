@@ -465,8 +465,11 @@ def test_mgpcg_graph_replay_matches_eager(feat, torch_gpu, precision, coarse):
 
 @pytest.mark.parametrize("precision", ["f64", "f32"])
 def test_mgpcg_dictionary_vs_plain_plans(feat, torch_gpu, precision):
-    """Pair-dictionary and plain SELL-64 plans (and whatever launch shapes the autotuner
-    picks for each) give bit-identical iterates and iteration counts."""
+    """Entry-order pair-dictionary and plain SELL-64 plans (and whatever launch shapes the
+    autotuner picks for each) give bit-identical iterates and iteration counts; the
+    diagonal dictionary plan gives the same iteration count and iterates equal to 1e-13:
+    its SpMV rows are bit-identical (test_sell_dict_plan_bitwise), but it sums dots
+    and norms over chunks of 248 rows instead of 256, which rounds differently."""
     torch = torch_gpu
     levels = q1_levels(feat, 3, 5, 1)
     n = levels[0]["n"]
@@ -475,7 +478,7 @@ def test_mgpcg_dictionary_vs_plain_plans(feat, torch_gpu, precision):
     bh[filt] = 0.0
     b = torch.from_numpy(bh).cuda()
     out = []
-    for flags in (0, feat.SELL_NO_DICT):
+    for flags in (feat.SELL_NO_DIAG, feat.SELL_NO_DICT, 0):
         feat.runtime_sell_flags(flags)
         try:
             s = feat.MgPcg(feat.mgpcg_cfg(precision=precision), levels=levels)
@@ -486,8 +489,9 @@ def test_mgpcg_dictionary_vs_plain_plans(feat, torch_gpu, precision):
         s.close()
         assert r["status"] == "success"
         out.append((r["iters"], host(x)))
-    assert out[0][0] == out[1][0]
+    assert out[0][0] == out[1][0] == out[2][0]
     assert np.array_equal(out[0][1], out[1][1])
+    assert np.linalg.norm(out[2][1] - out[0][1]) <= 1e-13 * np.linalg.norm(out[0][1])
 
 
 @pytest.mark.parametrize("precision", ["f64", "f32"])
