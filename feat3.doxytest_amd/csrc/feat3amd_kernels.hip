@@ -154,13 +154,21 @@ struct EpiDefect
   const uint32_t* mask;
   bool nrm;
   __device__ void begin() {}
-  __device__ __forceinline__ void row(uint64_t i, DT sum, double& a0, double&) const
+  // the row's operands, loadable before its sum is formed
+  struct Pre
   {
-    DT v = dfma(sum, DT(-1), rhs[i]);
-    if(is_masked(mask, i)) v = DT(0);
+    DT rhs;
+    bool m;
+  };
+  __device__ __forceinline__ Pre pre(uint64_t i) const { return Pre{rhs[i], is_masked(mask, i)}; }
+  __device__ __forceinline__ void row(uint64_t i, DT sum, const Pre& p, double& a0, double&) const
+  {
+    DT v = dfma(sum, DT(-1), p.rhs);
+    if(p.m) v = DT(0);
     d[i] = v;
     if(nrm) a0 = fma(double(v), double(v), a0);
   }
+  __device__ __forceinline__ void row(uint64_t i, DT sum, double& a0, double& a1) const { row(i, sum, pre(i), a0, a1); }
 };
 
 // sol += filter(P*xc)                                  [multigrid.hpp:1885-1930]
@@ -197,29 +205,66 @@ struct EpiSmooth
   int mode;
   bool skip;
   __device__ void begin() { skip = smoother_skips<DT>(rhs_nrm2); }
-  __device__ __forceinline__ void row(uint64_t i, DT sum, double&, double& a1) const
+  // the row's operands, loadable before its sum is formed (out[i] is read only by its row)
+  struct Pre
   {
-    const bool m = is_masked(mask, i);
-    DT d = dfma(sum, DT(-1), rhs[i]);
-    if(m) d = DT(0);
+    DT rhs, inv, xk, acc, w;
+    bool m;
+  };
+  __device__ __forceinline__ Pre pre(uint64_t i) const
+  {
+    Pre p{rhs[i], DT(0), DT(0), DT(0), DT(0), is_masked(mask, i)};
+    if(mode != 1)
+    {
+      p.inv = inv[i];
+      p.xk = xk[i];
+    }
+    if(mode == 2)
+    {
+      p.acc = out[i];
+      if(dotw) p.w = dotw[i];
+    }
+    return p;
+  }
+  __device__ __forceinline__ void row(uint64_t i, DT sum, const Pre& p, double&, double& a1) const
+  {
+    DT d = dfma(sum, DT(-1), p.rhs);
+    if(p.m) d = DT(0);
     if(mode == 1)
     {
       out[i] = d;
       return;
     }
-    DT c = inv[i] * d;
-    if(m) c = DT(0);
-    DT xn = dfma(omega, c, xk[i]);
+    DT c = p.inv * d;
+    if(p.m) c = DT(0);
+    DT xn = dfma(omega, c, p.xk);
     if(skip) xn = DT(0);
     if(mode == 0)
       out[i] = xn;
     else
     {
-      const DT a = dfma(DT(1), xn, out[i]);
+      const DT a = dfma(DT(1), xn, p.acc);
       out[i] = a;
-      if(dotw) a1 = a1 + double(a) * double(dotw[i]);
+      if(dotw) a1 = a1 + double(a) * double(p.w);
     }
   }
+  __device__ __forceinline__ void row(uint64_t i, DT sum, double& a0, double& a1) const { row(i, sum, pre(i), a0, a1); }
+};
+
+// EpiPre<E>::on: E loads its row operands through E::pre, ahead of the row sum
+template<class E, class = void>
+struct EpiPre
+{
+  struct type
+  {
+  };
+  static constexpr bool on = false;
+};
+template<class E>
+struct EpiPre<E, std::void_t<typename E::Pre>>
+{
+  typedef typename E::Pre type;
+  static constexpr bool on = true;
 };
 
 template<typename DT, typename IT, class Epi>
@@ -441,6 +486,7 @@ __global__ __launch_bounds__(kThreads) void k_sell_diag_rows(const DT* __restric
     const bool mine = lane >= 1 && lane <= kHaloRows && row < std::int64_t(rows);
     uint32_t wd[8];
     DT cen[kMaxTrip];
+    typename EpiPre<Epi>::type pf{};
     auto clampi = [&](std::int64_t i) { return i < 0 ? 0 : (i > last ? last : i); };
     if(live)
     {
@@ -450,6 +496,8 @@ __global__ __launch_bounds__(kThreads) void k_sell_diag_rows(const DT* __restric
 #pragma unroll
       for(int g = 0; g < kMaxTrip; ++g)
         if(g < nt) cen[g] = x[clampi(row + meta[1 + g])];
+      if constexpr(EpiPre<Epi>::on)
+        if(mine) pf = epi.pre(uint64_t(row));
     }
     if(ch == xr.first)   // block-uniform: every wave of the block passes here once
     {
@@ -471,7 +519,13 @@ __global__ __launch_bounds__(kThreads) void k_sell_diag_rows(const DT* __restric
           if(c != kDiagNone) sum = dfma(s_val[c], v3[j], sum);
         }
       }
-    if(mine) epi.row(uint64_t(row), sum, acc0, acc1);
+    if(mine)
+    {
+      if constexpr(EpiPre<Epi>::on)
+        epi.row(uint64_t(row), sum, pf, acc0, acc1);
+      else
+        epi.row(uint64_t(row), sum, acc0, acc1);
+    }
   }
   grid_reduce_finish(gr, acc0, acc1);
 }
