@@ -119,11 +119,17 @@ struct EpiApply
   DT a, b;
   bool bzero;
   __device__ void begin() {}
-  __device__ __forceinline__ void row(uint64_t i, DT sum, double&, double&) const
+  // the row's operands, loadable before its sum is formed
+  struct Pre
   {
-    const DT yv = bzero ? DT(0) : y[i];
-    r[i] = dfma(sum, a, b * yv);
+    DT yv;
+  };
+  __device__ __forceinline__ Pre pre(uint64_t i) const { return Pre{bzero ? DT(0) : y[i]}; }
+  __device__ __forceinline__ void row(uint64_t i, DT sum, const Pre& p, double&, double&) const
+  {
+    r[i] = dfma(sum, a, b * p.yv);
   }
+  __device__ __forceinline__ void row(uint64_t i, DT sum, double& a0, double& a1) const { row(i, sum, pre(i), a0, a1); }
 };
 
 // r = filter(A*x) with optional |r|^2 and r.w       [transfer.hpp:247-251 + unit_filter]
@@ -135,14 +141,22 @@ struct EpiMasked
   const DT* dotw;
   bool nrm, dot;
   __device__ void begin() {}
-  __device__ __forceinline__ void row(uint64_t i, DT sum, double& a0, double& a1) const
+  // the row's operands, loadable before its sum is formed
+  struct Pre
+  {
+    DT w;
+    bool m;
+  };
+  __device__ __forceinline__ Pre pre(uint64_t i) const { return Pre{dot ? dotw[i] : DT(0), is_masked(mask, i)}; }
+  __device__ __forceinline__ void row(uint64_t i, DT sum, const Pre& p, double& a0, double& a1) const
   {
     DT v = dfma(sum, DT(1), DT(0));
-    if(is_masked(mask, i)) v = DT(0);
+    if(p.m) v = DT(0);
     r[i] = v;
     if(nrm) a0 = fma(double(v), double(v), a0);
-    if(dot) a1 = fma(double(v), double(dotw[i]), a1);
+    if(dot) a1 = fma(double(v), double(p.w), a1);
   }
+  __device__ __forceinline__ void row(uint64_t i, DT sum, double& a0, double& a1) const { row(i, sum, pre(i), a0, a1); }
 };
 
 // d = filter(rhs - A*x) with optional |d|^2          [sparse_matrix_csr.hpp:1807-1843]
@@ -178,12 +192,20 @@ struct EpiProlAdd
   DT* sol;
   const uint32_t* mask;
   __device__ void begin() {}
-  __device__ __forceinline__ void row(uint64_t i, DT sum, double&, double&) const
+  // the row's operands, loadable before its sum is formed (sol[i] is read only by its row)
+  struct Pre
+  {
+    DT s;
+    bool m;
+  };
+  __device__ __forceinline__ Pre pre(uint64_t i) const { return Pre{sol[i], is_masked(mask, i)}; }
+  __device__ __forceinline__ void row(uint64_t i, DT sum, const Pre& p, double&, double&) const
   {
     DT c = dfma(sum, DT(1), DT(0));
-    if(is_masked(mask, i)) c = DT(0);
-    sol[i] = dfma(DT(1), c, sol[i]);
+    if(p.m) c = DT(0);
+    sol[i] = dfma(DT(1), c, p.s);
   }
+  __device__ __forceinline__ void row(uint64_t i, DT sum, double& a0, double& a1) const { row(i, sum, pre(i), a0, a1); }
 };
 
 // One Richardson(Jacobi) step, fused (richardson.hpp:205-221, jacobi_precond.hpp:159-164):
@@ -327,6 +349,8 @@ __global__ __launch_bounds__(kThreads) void k_sell_rows(const VT* __restrict__ v
       const int len = rlen[row];
       const VT* v = val + off + lane;
       const CT* c = col + off + lane;
+      typename EpiPre<Epi>::type pf{};
+      if constexpr(EpiPre<Epi>::on) pf = epi.pre(row);
       DT sum = DT(0);
       int k = 0;
       // matrix arrays are streamed once: nontemporal loads keep x resident in L2
@@ -349,7 +373,10 @@ __global__ __launch_bounds__(kThreads) void k_sell_rows(const VT* __restrict__ v
       for(; k < len; ++k)
         sum = dfma(ld_val<DT, VT>(v + k * 64), x[sell_col<DELTA>(__builtin_nontemporal_load(c + k * 64), row)], sum);
       if(!std::is_same<VT, DT>::value) sum = sum * scale;  // exact: a power of two
-      epi.row(row, sum, acc0, acc1);
+      if constexpr(EpiPre<Epi>::on)
+        epi.row(row, sum, pf, acc0, acc1);
+      else
+        epi.row(row, sum, acc0, acc1);
     }
   }
   grid_reduce_finish(gr, acc0, acc1);
