@@ -2154,15 +2154,17 @@ int feat_autotune_matrix_f64(feat_handle_t h, feat_matrix_f64_t* A, const double
   A->tune = 0;
   if(best_us) *best_us = -1.0;
   if(A->format != FEAT_FMT_SELL || A->rows == 0) return FEAT_OK;  // only SELL launch shapes are tunable
-  const uint64_t chunk = A->index == FEAT_IDX_DIAG ? uint64_t(kHaloChunk) : uint64_t(kThreads);
-  const uint64_t nch = (A->rows + chunk - 1) / chunk;
+  // The diagonal kernel keeps its full grid: it has no unroll choice, a capped grid times
+  // the same for r = A*x (within noise), and the smoothing epilogue, whose row operands
+  // lengthen each chunk's chain, loses up to half its speed on a capped grid.
+  if(A->index == FEAT_IDX_DIAG) return FEAT_OK;
+  const uint64_t nch = (A->rows + kThreads - 1) / kThreads;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   int rc = check_hip(hipEventCreate(&e0), "hipEventCreate");
   if(!rc) rc = check_hip(hipEventCreate(&e1), "hipEventCreate");
   int best = 0;
   float best_ms = 0.0f;
-  const int max_unr = A->index == FEAT_IDX_DIAG ? 1 : 3;   // the diagonal kernel has no unroll choice
-  for(int unr = 1; unr <= max_unr && !rc; ++unr)
+  for(int unr = 1; unr <= 3 && !rc; ++unr)
     for(int cap = 0; cap <= 5 && !rc; ++cap)
     {
       if(cap && (uint64_t(kMaxGrid) >> cap) >= nch) continue;  // same launch as the uncapped grid

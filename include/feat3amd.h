@@ -644,7 +644,8 @@ int feat_mg_tail_f32(feat_handle_t h, const feat_mg_tail_f32_t* t, const float* 
  * grid).  Every shape gives bit-identical results (each row is summed in CSR
  * order, reductions do not depend on the shape).  The tuner times
  * r = A*x for every shape with HIP events (reps launches each) and stores the
- * fastest in A->tune; *best_us (may be NULL) receives its time per launch. */
+ * fastest in A->tune; *best_us (may be NULL) receives its time per launch.
+ * Diagonal dictionary plans (FEAT_IDX_DIAG) have one shape: tune stays 0. */
 int feat_autotune_matrix_f64(feat_handle_t h, feat_matrix_f64_t* A, const double* x, double* r,
                              int reps, double* best_us);
 
