@@ -56,6 +56,7 @@ class Matrix(C.Structure):
 FMT_CSR, FMT_SELL = 0, 1
 IDX_U64, IDX_U32, IDX_D16, IDX_DICT, IDX_DIAG = 0, 1, 2, 3, 4
 SELL_NO_DICT, SELL_NO_DIAG = 1, 2
+VAL_NATIVE, VAL_F16, VAL_CODE8 = 0, 1, 2
 Q1_HOST_ASSEMBLY = 1
 
 

@@ -327,14 +327,29 @@ __device__ __forceinline__ float ld_val<float, uint16_t>(const uint16_t* p)
   return float(__builtin_bit_cast(_Float16, __builtin_nontemporal_load(p)));
 }
 
-template<typename DT, typename VT, typename CT, bool DELTA, class Epi, int UNR = kSellUnroll>
+// VCODE: val holds 1-byte codes into vtab (256 values, staged in LDS), else the values.
+template<typename DT, typename VT, typename CT, bool DELTA, class Epi, int UNR = kSellUnroll, bool VCODE = false>
 __global__ __launch_bounds__(kThreads) void k_sell_rows(const VT* __restrict__ val,
                                                         const CT* __restrict__ col,
                                                         const uint64_t* __restrict__ soff,
                                                         const uint16_t* __restrict__ rlen,
                                                         const DT* __restrict__ x, uint64_t rows,
-                                                        DT scale, Epi epi, GridReduce gr)
+                                                        DT scale, Epi epi, GridReduce gr,
+                                                        const DT* __restrict__ vtab = nullptr)
 {
+  __shared__ DT s_tab[VCODE ? kThreads : 1];
+  if constexpr(VCODE)
+  {
+    static_assert(kThreads == 256, "one table entry per thread");
+    s_tab[threadIdx.x] = vtab[threadIdx.x];
+    __syncthreads();
+  }
+  auto value = [&](const VT* p) -> DT {
+    if constexpr(VCODE)
+      return s_tab[__builtin_nontemporal_load(p)];
+    else
+      return ld_val<DT, VT>(p);
+  };
   const uint64_t nch = (rows + kThreads - 1) / kThreads;
   const XcdRange xr = xcd_range(nch);
   const int lane = threadIdx.x & 63;
@@ -362,7 +377,7 @@ __global__ __launch_bounds__(kThreads) void k_sell_rows(const VT* __restrict__ v
 #pragma unroll
         for(int u = 0; u < UNR; ++u)
         {
-          vv[u] = ld_val<DT, VT>(v + (k + u) * 64);
+          vv[u] = value(v + (k + u) * 64);
           cc[u] = sell_col<DELTA>(__builtin_nontemporal_load(c + (k + u) * 64), row);
         }
 #pragma unroll
@@ -371,8 +386,8 @@ __global__ __launch_bounds__(kThreads) void k_sell_rows(const VT* __restrict__ v
         for(int u = 0; u < UNR; ++u) sum = dfma(vv[u], xv[u], sum);
       }
       for(; k < len; ++k)
-        sum = dfma(ld_val<DT, VT>(v + k * 64), x[sell_col<DELTA>(__builtin_nontemporal_load(c + k * 64), row)], sum);
-      if(!std::is_same<VT, DT>::value) sum = sum * scale;  // exact: a power of two
+        sum = dfma(value(v + k * 64), x[sell_col<DELTA>(__builtin_nontemporal_load(c + k * 64), row)], sum);
+      if(!VCODE && !std::is_same<VT, DT>::value) sum = sum * scale;  // exact: a power of two
       if constexpr(EpiPre<Epi>::on)
         epi.row(row, sum, pf, acc0, acc1);
       else
@@ -628,6 +643,41 @@ int launch_mat(hipStream_t s, const M* A, const DT* x, const Epi& epi, const Gri
       finish_reduce(s, gd, uint64_t(grid));
       return check_launch(name);
     }
+    if(A->value_kind == FEAT_VAL_CODE8)
+    {
+      const DT* vt = reinterpret_cast<const DT*>(A->val);
+      const uint8_t* vc = reinterpret_cast<const uint8_t*>(vt + 256);
+      if(A->index == FEAT_IDX_U32)
+      {
+        if(std::is_same<DT, double>::value && unr_code == 1)
+          hipLaunchKernelGGL((k_sell_rows<DT, uint8_t, uint32_t, false, Epi, 4, true>), dim3(grid), dim3(kThreads), 0, s,
+                             vc, (const uint32_t*)A->col, soff, A->rowlen, x, A->rows, DT(1), epi, gd, vt);
+        else if(std::is_same<DT, double>::value && unr_code == 3)
+          hipLaunchKernelGGL((k_sell_rows<DT, uint8_t, uint32_t, false, Epi, 16, true>), dim3(grid), dim3(kThreads), 0, s,
+                             vc, (const uint32_t*)A->col, soff, A->rowlen, x, A->rows, DT(1), epi, gd, vt);
+        else
+          hipLaunchKernelGGL((k_sell_rows<DT, uint8_t, uint32_t, false, Epi, kSellUnroll, true>), dim3(grid),
+                             dim3(kThreads), 0, s, vc, (const uint32_t*)A->col, soff, A->rowlen, x, A->rows, DT(1), epi,
+                             gd, vt);
+      }
+      else if(A->index == FEAT_IDX_D16)
+      {
+        if(std::is_same<DT, double>::value && unr_code == 1)
+          hipLaunchKernelGGL((k_sell_rows<DT, uint8_t, int16_t, true, Epi, 4, true>), dim3(grid), dim3(kThreads), 0, s,
+                             vc, (const int16_t*)A->col, soff, A->rowlen, x, A->rows, DT(1), epi, gd, vt);
+        else if(std::is_same<DT, double>::value && unr_code == 3)
+          hipLaunchKernelGGL((k_sell_rows<DT, uint8_t, int16_t, true, Epi, 16, true>), dim3(grid), dim3(kThreads), 0, s,
+                             vc, (const int16_t*)A->col, soff, A->rowlen, x, A->rows, DT(1), epi, gd, vt);
+        else
+          hipLaunchKernelGGL((k_sell_rows<DT, uint8_t, int16_t, true, Epi, kSellUnroll, true>), dim3(grid),
+                             dim3(kThreads), 0, s, vc, (const int16_t*)A->col, soff, A->rowlen, x, A->rows, DT(1), epi,
+                             gd, vt);
+      }
+      else
+        return fail(FEAT_ERR_UNSUPPORTED, "SELL index kind");
+      finish_reduce(s, gd, uint64_t(grid));
+      return check_launch(name);
+    }
     if(A->index == FEAT_IDX_U32)
     {
       if(std::is_same<DT, double>::value && unr_code == 1)
@@ -725,6 +775,66 @@ __global__ void k_sell_fill(const double* val, const uint64_t* col, const uint64
       else
       {
         sval[p] = DT(0);
+        scol[p] = DELTA ? CT(0) : CT(r);
+      }
+    }
+  }
+}
+
+// bit pattern of a stored value (the key of the value-code table) and back
+template<typename DT>
+__host__ __device__ inline uint64_t value_key(DT v)
+{
+  if constexpr(sizeof(DT) == 8)
+    return __builtin_bit_cast(uint64_t, v);
+  else if constexpr(sizeof(DT) == 4)
+    return __builtin_bit_cast(uint32_t, v);
+  else
+    return __builtin_bit_cast(uint16_t, v);
+}
+template<typename DT>
+__host__ __device__ inline DT key_value(uint64_t k)
+{
+  if constexpr(sizeof(DT) == 8)
+    return __builtin_bit_cast(DT, k);
+  else if constexpr(sizeof(DT) == 4)
+    return __builtin_bit_cast(DT, uint32_t(k));
+  else
+    return __builtin_bit_cast(DT, uint16_t(k));
+}
+
+// Value-code fill: the SELL layout of k_sell_fill with each value replaced by its index
+// in keys[0..nk) (the ascending bit patterns of the distinct values in the arithmetic type).
+template<typename DT, typename CT, bool DELTA>
+__global__ void k_sell_fill_code(const double* val, const uint64_t* col, const uint64_t* rp, uint64_t rows,
+                                 const uint64_t* soff, const uint64_t* keys, int nk, uint8_t* codes, CT* scol)
+{
+  for(uint64_t r = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; r < rows; r += uint64_t(gridDim.x) * blockDim.x)
+  {
+    const uint64_t s = r >> 6, lane = r & 63;
+    const uint64_t off = soff[s], smax = (soff[s + 1] - soff[s]) / 64;
+    const uint64_t b = rp[r], len = rp[r + 1] - b;
+    for(uint64_t k = 0; k < smax; ++k)
+    {
+      const uint64_t p = off + k * 64 + lane;
+      if(k < len)
+      {
+        const uint64_t key = value_key<DT>(DT(val[b + k]));
+        int lo = 0, hi = nk - 1;   // the key is present: binary search for it
+        while(lo < hi)
+        {
+          const int mid = (lo + hi) >> 1;
+          if(keys[mid] < key)
+            lo = mid + 1;
+          else
+            hi = mid;
+        }
+        codes[p] = uint8_t(lo);
+        scol[p] = DELTA ? CT(std::int64_t(col[b + k]) - std::int64_t(r)) : CT(col[b + k]);
+      }
+      else
+      {
+        codes[p] = 0;
         scol[p] = DELTA ? CT(0) : CT(r);
       }
     }
@@ -1930,11 +2040,68 @@ int sell_create_impl(feat_handle_t h, const feat_csr_f64_t* A, M* out, const cha
       inv_scale = std::ldexp(1.0f, -e);
     }
   }
-  rc = check_hip(hipMalloc(&sval, sizeof(DT) * (total ? total : 1)), "sell val");
+  // value codes: at most 256 distinct values (bit patterns in the arithmetic type)
+  std::vector<uint64_t> vkeys;
+  if(!(flags & FEAT_SELL_NO_DICT) && !std::is_same<DT, uint16_t>::value && n && A->nnz)
+  {
+    std::vector<double> hv(A->nnz);
+    rc = check_hip(hipMemcpyAsync(hv.data(), A->val, sizeof(double) * A->nnz, hipMemcpyDeviceToHost, h->stream), "copy");
+    if(!rc) rc = check_hip(hipStreamSynchronize(h->stream), "sync");
+    if(rc) { cleanup(); return rc; }
+    uint64_t last = ~uint64_t(0);
+    for(const double d : hv)
+    {
+      if(std::isnan(d))   // NaN payloads may convert differently on host and device
+      {
+        vkeys.clear();
+        break;
+      }
+      const uint64_t key = value_key<DT>(DT(d));
+      if(key == last) continue;
+      last = key;
+      auto it = std::lower_bound(vkeys.begin(), vkeys.end(), key);
+      if(it != vkeys.end() && *it == key) continue;
+      if(vkeys.size() == 256)
+      {
+        vkeys.clear();
+        break;
+      }
+      vkeys.insert(it, key);
+    }
+  }
+  const bool vcode = !vkeys.empty();
+  uint64_t* dkeys = nullptr;
+  const size_t vbytes = vcode ? sizeof(DT) * 256 + (total ? total : 1) : sizeof(DT) * (total ? total : 1);
+  rc = check_hip(hipMalloc(&sval, vbytes), "sell val");
   if(!rc) rc = check_hip(hipMalloc(&scol, cb * (total ? total : 1)), "sell col");
   if(!rc) rc = check_hip(hipMemcpyAsync(soff, hoff.data(), sizeof(uint64_t) * (ns + 1), hipMemcpyHostToDevice, h->stream), "copy");
+  if(!rc && vcode)
+  {
+    std::vector<DT> tab(256, DT(0));
+    for(size_t i = 0; i < vkeys.size(); ++i) tab[i] = key_value<DT>(vkeys[i]);
+    rc = check_hip(hipMalloc(&dkeys, sizeof(uint64_t) * vkeys.size()), "value keys");
+    if(!rc) rc = check_hip(hipMemcpyAsync(dkeys, vkeys.data(), sizeof(uint64_t) * vkeys.size(), hipMemcpyHostToDevice, h->stream), "copy");
+    if(!rc) rc = check_hip(hipMemcpyAsync(sval, tab.data(), sizeof(DT) * 256, hipMemcpyHostToDevice, h->stream), "copy");
+    if constexpr(!std::is_same<DT, uint16_t>::value)   // FP16 plans keep their values
+    {
+      if(!rc)
+      {
+        uint8_t* codes = reinterpret_cast<uint8_t*>(sval + 256);
+        if(d16)
+          hipLaunchKernelGGL((k_sell_fill_code<DT, int16_t, true>), dim3(grid_for(n)), dim3(kThreads), 0, h->stream,
+                             A->val, A->col, A->rowptr, n, soff, dkeys, int(vkeys.size()), codes, (int16_t*)scol);
+        else
+          hipLaunchKernelGGL((k_sell_fill_code<DT, uint32_t, false>), dim3(grid_for(n)), dim3(kThreads), 0, h->stream,
+                             A->val, A->col, A->rowptr, n, soff, dkeys, int(vkeys.size()), codes, (uint32_t*)scol);
+        rc = check_launch(name);
+        if(!rc) rc = check_hip(hipStreamSynchronize(h->stream), "sync");
+      }
+    }
+    if(dkeys) (void)hipFree(dkeys);
+    if(rc) { cleanup(); return rc; }
+  }
   if(rc) { cleanup(); return rc; }
-  if(n)
+  if(n && !vcode)
   {
     if(d16)
       hipLaunchKernelGGL((k_sell_fill<DT, int16_t, true>), dim3(grid_for(n)), dim3(kThreads), 0, h->stream, A->val, A->col,
@@ -1956,7 +2123,7 @@ int sell_create_impl(feat_handle_t h, const feat_csr_f64_t* A, M* out, const cha
   out->cols = A->cols;
   out->nnz = total;
   out->val = (decltype(out->val))(void*)sval;
-  out->value_kind = std::is_same<DT, uint16_t>::value ? FEAT_VAL_F16 : FEAT_VAL_NATIVE;
+  out->value_kind = std::is_same<DT, uint16_t>::value ? FEAT_VAL_F16 : (vcode ? FEAT_VAL_CODE8 : FEAT_VAL_NATIVE);
   out->scale = scale;
   out->tune = 0;
   out->col = scol;

@@ -217,6 +217,10 @@ typedef struct {            /* a CSR matrix on the device (u64 indices) */
                           * words per row, 64-row slices lane-interleaved        */
 #define FEAT_VAL_NATIVE 0 /* values in the descriptor's arithmetic type     */
 #define FEAT_VAL_F16 1    /* FP32 arithmetic on FP16 values times `scale`   */
+#define FEAT_VAL_CODE8 2  /* SELL (i16/u32 index): each stored value is a
+                           * 1-byte code into a table of 256 values of the
+                           * arithmetic type; val -> the table, followed by
+                           * the codes in the layout of the columns          */
 typedef struct {
   int format, index;
   uint64_t rows, cols;
@@ -225,7 +229,7 @@ typedef struct {
   const void* col;
   const void* ptr;           /* CSR row pointer / SELL slice offsets (u64)    */
   const uint16_t* rowlen;    /* SELL row lengths                             */
-  int value_kind;            /* FEAT_VAL_NATIVE                               */
+  int value_kind;            /* FEAT_VAL_NATIVE or FEAT_VAL_CODE8 (SELL)      */
   float scale;               /* 1                                             */
   int tune;                  /* SELL launch shape, 0 = default (autotuner)    */
   uint32_t slice_words;      /* pair-dictionary plans: code words per lane in
@@ -240,7 +244,7 @@ typedef struct {             /* FP32 arithmetic (mixed-precision inner solver) *
   const void* col;
   const void* ptr;
   const uint16_t* rowlen;
-  int value_kind;            /* FEAT_VAL_NATIVE or FEAT_VAL_F16 (SELL only)    */
+  int value_kind;            /* FEAT_VAL_NATIVE, _F16 or _CODE8 (SELL only)    */
   float scale;               /* row sums are multiplied by this power of two  */
   int tune;                  /* SELL launch shape, 0 = default                */
   uint32_t slice_words;      /* as feat_matrix_f64_t::slice_words             */
@@ -253,11 +257,12 @@ int feat_matrix_from_csr_f64(const feat_csr_f64_t* A, feat_matrix_f64_t* out);
  * slice padding would exceed the CSR size; the caller then keeps the CSR.
  * A matrix with at most 256 distinct (column - row, value) pairs -- FE matrices
  * on uniformly refined meshes: 42 for 3D Q1 -- gets the lossless pair-dictionary
- * plan (FEAT_IDX_DICT, one byte per entry); others get i16/u32 columns and full
- * values. */
+ * plan (FEAT_IDX_DICT, one byte per entry); others get i16/u32 columns and, with
+ * at most 256 distinct values (transfer operators: 4), 1-byte value codes
+ * (FEAT_VAL_CODE8), else full values. */
 int feat_sell_create_f64(feat_handle_t h, const feat_csr_f64_t* A, feat_matrix_f64_t* out);
-/* Same with flags: FEAT_SELL_NO_DICT builds the plain plan even where the
- * pair dictionary applies. */
+/* Same with flags: FEAT_SELL_NO_DICT builds the plain plan (full values) even
+ * where the pair dictionary or the value codes apply. */
 #define FEAT_SELL_NO_DICT 1
 #define FEAT_SELL_NO_DIAG 2   /* keep the entry-order dictionary (FEAT_IDX_DICT) */
 int feat_sell_create_ex_f64(feat_handle_t h, const feat_csr_f64_t* A, int flags, feat_matrix_f64_t* out);

@@ -204,8 +204,9 @@ def stencil_csr(rng, rows, cols, offsets, values, p_keep):
 def test_sell_dict_plan_bitwise(feat, handle, torch_gpu, oracle, case, diag):
     """Pair-dictionary SELL plans (one byte per entry, in entry order or by diagonal)
     reproduce the CSR loop bit for bit in FP64 and FP32; matrices with more than 256
-    distinct (col - row, value) pairs, or dict=False, get the plain plan; the diagonal
-    layout applies to at most 32 distinct offsets with little slot padding."""
+    distinct (col - row, value) pairs, or dict=False, get the plain plan, with 1-byte
+    value codes where at most 256 distinct values occur (not with dict=False); the
+    diagonal layout applies to at most 32 distinct offsets with little slot padding."""
     print(f"dict plan {case} diag={diag}", flush=True)
     torch = torch_gpu
     k, _ = feat.libs()
@@ -270,6 +271,8 @@ def test_sell_dict_plan_bitwise(feat, handle, torch_gpu, oracle, case, diag):
     v, c, r = upload_csr(torch, A)
     plan = feat.SellPlan(handle, v, c, r, A.rows, A.cols, dict=(case != "no_dict"), diag=diag)
     assert plan.m.format == feat.FMT_SELL and plan.m.index == want, (plan.m.index, want)
+    want_vk = feat.VAL_CODE8 if case in ("many_pairs", "many_offsets") else feat.VAL_NATIVE
+    assert plan.m.value_kind == want_vk, (plan.m.value_kind, want_vk)
     if want == feat.IDX_DICT:
         # one code-word stride for all slices unless that would pad by more than a quarter
         assert (plan.m.slice_words == 0) == (case in ("skewed", "tall_rect")), plan.m.slice_words
@@ -285,7 +288,7 @@ def test_sell_dict_plan_bitwise(feat, handle, torch_gpu, oracle, case, diag):
             assert np.array_equal(host(rd), oracle.csr_apply(A, x, y, a, b)), (tune, a, b)
     plan.close()
     p32 = feat.SellPlan(handle, v, c, r, A.rows, A.cols, fp="f32", dict=(case != "no_dict"), diag=diag)
-    assert p32.m.index == want
+    assert p32.m.index == want and p32.m.value_kind == want_vk
     x32 = dev(torch, x.astype(np.float32))
     r32 = torch.empty(A.rows, dtype=torch.float32, device="cuda")
     keep_ws, ws = feat.reduce_ws(torch, "cuda")
