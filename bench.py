@@ -731,6 +731,29 @@ def main():
     sell_ms = time_apply(plan)
     edict_ms = time_apply(eplan)
     dict_ms = time_apply(dplan)
+
+    # one fused Richardson(Jacobi) step on the diagonal plan (the solve's most frequent
+    # launch): smoother calls of 9 and of 1 steps, the difference over 8 steps
+    inv_t = torch.rand(n, dtype=torch.float64, device=dev)
+    rhs_t = torch.rand(n, dtype=torch.float64, device=dev)
+    xo, tmp_t = torch.empty_like(rhs_t), torch.empty_like(rhs_t)
+    keep_ws, ws = feat.reduce_ws(torch, dev)
+
+    def time_smooth(steps):
+        def call():
+            return k.feat_rj_smooth_f64(h.h, C.byref(dplan), feat.ptr(inv_t), None, steps, 0.5, feat.ptr(rhs_t),
+                                        None, feat.ptr(xo), feat.ptr(tmp_t), None, None, None, None, ws)
+        for _ in range(3):
+            feat.check(call(), "feat_rj_smooth_f64")
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(args.spmv_reps):
+            call()
+        e1.record(stream)
+        e1.synchronize()
+        return e0.elapsed_time(e1) / args.spmv_reps
+
+    step_ms = (time_smooth(9) - time_smooth(1)) / 8
     idx_bytes = {feat.IDX_D16: 2, feat.IDX_U32: 4}[plan.index]
     nslices = (n + 63) // 64
     # bytes each kernel must move: its own arrays once, x once, r once
@@ -744,6 +767,11 @@ def main():
     csr_gbs = csr_bytes / (csr_ms * 1e-3) / 1e9
     sell_gbs = sell_bytes / (sell_ms * 1e-3) / 1e9
     dict_gbs = dict_bytes / (dict_ms * 1e-3) / 1e9
+    # Jacobi step: the plan's bytes, x_k (gathered and read by the epilogue: once), rhs,
+    # inverse diagonal, x_{k+1}
+    step_bytes = int(dplan.nnz) + 256 * 12 + 4 * n * 8
+    step_gbs = step_bytes / (step_ms * 1e-3) / 1e9
+    log(f"jacobi step on the diagonal plan: {step_ms * 1e3:.1f} us, {step_gbs:.0f} GB/s")
     ref_model = nnz * 16 + n * 8             # product_matvec-bench.cpp:41-44 model (GiB = 1024^3)
     log(f"spmv csr-u64: {csr_ms * 1e3:.1f} us, {csr_gbs:.0f} GB/s | sell-64 ({'i16 delta' if idx_bytes == 2 else 'u32'}): "
         f"{sell_ms * 1e3:.1f} us, {sell_gbs:.0f} GB/s own bytes, {csr_bytes / (sell_ms * 1e-3) / 1e9:.0f} GB/s CSR-equivalent"
@@ -790,7 +818,11 @@ def main():
         "spmv": {
             "sell64_diag": {"kernel": "k_sell_diag_rows<double,EpiApply<double>>", "us_per_launch": dict_ms * 1e3,
                             "bytes_per_launch": dict_bytes, "gbps": dict_gbs,
-                            "csr_equivalent_gbps": csr_bytes / (dict_ms * 1e-3) / 1e9, "stored_slots": dict_padded},
+                            "csr_equivalent_gbps": csr_bytes / (dict_ms * 1e-3) / 1e9,
+                     "jacobi_step": {"kernel": "k_sell_diag_rows<double,EpiSmooth<double>> (one fused "
+                                               "Richardson(Jacobi) step, the solve's most frequent launch)",
+                                     "us_per_launch": step_ms * 1e3, "achieved": step_gbs,
+                                     "frac": step_gbs / PEAK_HBM_GBS, "algorithmic_bytes": step_bytes}, "stored_slots": dict_padded},
             "sell64_dict": {"kernel": "k_sell_dict_rows<double,EpiApply<double>,8>", "us_per_launch": edict_ms * 1e3,
                             "bytes_per_launch": edict_bytes, "gbps": edict_gbs,
                             "csr_equivalent_gbps": csr_bytes / (edict_ms * 1e-3) / 1e9, "stored_entries": int(eplan.nnz)},
