@@ -512,7 +512,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
-              "domain decomposition is design-only in this build.", file=sys.stderr)
+              "domain decomposition is not implemented here.", file=sys.stderr)
         sys.exit(2)
 
     import torch

@@ -17,7 +17,7 @@
 // is exact for every integrand here); the reference integrates the same
 // polynomials with gauss-legendre:3, so values agree to rounding.
 //
-// Numbering (this build's own, lexicographic, x fastest):
+// Numbering (lexicographic, x fastest):
 //   velocity node (a,b) of the (2N+1)^2 Q2 node grid -> b*(2N+1)+a,
 //   P1dc dofs of element e = ey*N+ex -> 3e (constant), 3e+1 (x^), 3e+2 (y^),
 //   where (x^, y^) are the element's reference coordinates in [-1,1]^2

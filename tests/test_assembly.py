@@ -85,7 +85,7 @@ def test_q1_closed_forms(feat, dim, level):
     xc = (np.arange(q["pcols"]) % mc) / (mc - 1)
     xf = (np.arange(n) % m) / (m - 1)
     assert np.allclose(P @ xc, xf, rtol=0, atol=1e-15)
-    # sizes of SURVEY-style formulas: n = (2^L+1)^d, nnz = (3(2^L+1)-2)^d
+    # closed-form sizes: n = (2^L+1)^d, nnz = (3(2^L+1)-2)^d
     assert n == m ** dim and A.nnz == (3 * m - 2) ** dim
 
 
