@@ -298,6 +298,43 @@ def test_sell_dict_plan_bitwise(feat, handle, torch_gpu, oracle, case, diag):
     p32.close()
 
 
+@pytest.mark.parametrize("with_nan", [False, True])
+def test_sell_value_codes(feat, handle, torch_gpu, oracle, with_nan):
+    """Plain SELL-64 plans of a matrix with few distinct values (a Q1 prolongation: 1, 1/2,
+    1/4, 1/8) store 1-byte value codes and reproduce the CSR loop bit for bit in FP64 and
+    FP32; a NaN entry turns the codes off (its payload may convert differently on host and
+    device) and the plan keeps full values."""
+    print(f"value codes nan={with_nan}", flush=True)
+    torch = torch_gpu
+    k, _ = feat.libs()
+    P = feat.ptr
+    q = feat.q1_problem(3, 4)
+    A = Csr(q["n"], q["pcols"], q["pval"].copy(), q["pcol"], q["prowptr"])
+    if with_nan:
+        A.val[A.nnz // 2] = np.nan
+    v, c, r = upload_csr(torch, A)
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal(A.cols)
+    xd = dev(torch, x)
+    rd = torch.empty(A.rows, dtype=torch.float64, device="cuda")
+    plan = feat.SellPlan(handle, v, c, r, A.rows, A.cols)
+    assert plan.m.index in (feat.IDX_U32, feat.IDX_D16)
+    assert plan.m.value_kind == (feat.VAL_NATIVE if with_nan else feat.VAL_CODE8)
+    feat.check(k.feat_matrix_apply_f64(handle.h, plan.m, P(rd), 1.0, P(xd), 0.0, None), "apply")
+    handle.sync()
+    assert np.array_equal(host(rd), oracle.csr_apply(A, x), equal_nan=True)
+    plan.close()
+    p32 = feat.SellPlan(handle, v, c, r, A.rows, A.cols, fp="f32")
+    assert p32.m.value_kind == (feat.VAL_NATIVE if with_nan else feat.VAL_CODE8)
+    x32 = dev(torch, x.astype(np.float32))
+    r32 = torch.empty(A.rows, dtype=torch.float32, device="cuda")
+    keep_ws, ws = feat.reduce_ws(torch, "cuda")
+    feat.check(k.feat_apply_masked_f32(handle.h, p32.m, P(r32), P(x32), None, None, ws), "apply f32")
+    handle.sync()
+    assert np.array_equal(host(r32), oracle.csr_apply_f32(A, x.astype(np.float32)), equal_nan=True)
+    p32.close()
+
+
 def test_sell_plan_rejects_long_rows(feat, handle, torch_gpu):
     # a 70000-entry row exceeds the 16-bit row length; the plan refuses and the CSR stays usable
     torch = torch_gpu
