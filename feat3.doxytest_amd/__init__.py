@@ -122,6 +122,7 @@ KERNEL_API = [
     ("feat_scale_f64", _i, [_vp, _vp, _vp, _d, _u64]),
     ("feat_component_product_f64", _i, [_vp, _vp, _vp, _vp, _u64]),
     ("feat_component_invert_f64", _i, [_vp, _vp, _vp, _d, _u64]),
+    ("feat_component_invert_f32", _i, [_vp, _vp, _vp, _f, _u64]),
     ("feat_csr_diag_index_u64", _i, [_vp, _vp, _vp, _vp, _u64]),
     ("feat_csr_extract_diag_f64_u64", _i, [_vp, _vp, _vp, _vp, _vp, _u64]),
     ("feat_unit_filter_rhs_f64_u64", _i, [_vp, _vp, _vp, _vp, _u64]),
@@ -257,6 +258,7 @@ HOST_API = [
     ("feat_mgpcg_matrix", _i, [_vp, _i, C.POINTER(_u64), C.POINTER(_u64), C.POINTER(_vp), C.POINTER(_vp),
                                C.POINTER(_vp)]),
     ("feat_mgpcg_coarse_fusion_level", _i, [_vp, C.POINTER(_i)]),
+    ("feat_mgpcg_set_values", _i, [_vp, _i, _vp]),
     ("feat_mgpcg_destroy", _i, [_vp]),
     ("feat_runtime_init", _i, [_i, _vp]),
     ("feat_runtime_sell_flags", _i, [_i]),
@@ -1053,6 +1055,12 @@ class MgPcg:
         check(self.hl.feat_mgpcg_matrix(self.s, lvl, C.byref(rows), C.byref(nnz), C.byref(v),
                                         C.byref(c), C.byref(r)), "feat_mgpcg_matrix")
         return CsrF64(rows.value, rows.value, nnz.value, v.value, c.value, r.value)
+
+    def set_values(self, lvl, val):
+        """New values (host array, same pattern) for level lvl's matrix, then the
+        numeric re-initialisation of the hierarchy and the solver tree."""
+        v = np.ascontiguousarray(val, np.float64)
+        check(self.hl.feat_mgpcg_set_values(self.s, lvl, v.ctypes.data), "feat_mgpcg_set_values")
 
     def coarse_fusion_level(self):
         """First level of the coarse levels fused into one workgroup (0: none)."""

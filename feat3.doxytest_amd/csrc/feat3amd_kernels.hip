@@ -804,10 +804,13 @@ __host__ __device__ inline DT key_value(uint64_t k)
 }
 
 // Value-code fill: the SELL layout of k_sell_fill with each value replaced by its index
-// in keys[0..nk) (the ascending bit patterns of the distinct values in the arithmetic type).
+// in keys[0..nk) (the ascending bit patterns of the distinct values in the arithmetic type,
+// found on the host).  A value whose key is not in the table (a device conversion that
+// rounds otherwise than the host's) sets *miss; the plan then keeps full values.
 template<typename DT, typename CT, bool DELTA>
 __global__ void k_sell_fill_code(const double* val, const uint64_t* col, const uint64_t* rp, uint64_t rows,
-                                 const uint64_t* soff, const uint64_t* keys, int nk, uint8_t* codes, CT* scol)
+                                 const uint64_t* soff, const uint64_t* keys, int nk, uint8_t* codes, CT* scol,
+                                 int* miss)
 {
   for(uint64_t r = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; r < rows; r += uint64_t(gridDim.x) * blockDim.x)
   {
@@ -820,7 +823,7 @@ __global__ void k_sell_fill_code(const double* val, const uint64_t* col, const u
       if(k < len)
       {
         const uint64_t key = value_key<DT>(DT(val[b + k]));
-        int lo = 0, hi = nk - 1;   // the key is present: binary search for it
+        int lo = 0, hi = nk - 1;   // binary search for the key
         while(lo < hi)
         {
           const int mid = (lo + hi) >> 1;
@@ -829,6 +832,7 @@ __global__ void k_sell_fill_code(const double* val, const uint64_t* col, const u
           else
             hi = mid;
         }
+        if(keys[lo] != key) *miss = 1;
         codes[p] = uint8_t(lo);
         scol[p] = DELTA ? CT(std::int64_t(col[b + k]) - std::int64_t(r)) : CT(col[b + k]);
       }
@@ -1051,7 +1055,8 @@ __global__ void k_cprod(double* r, const double* x, const double* y, uint64_t n)
     r[i] = x[i] * y[i];
 }
 
-__global__ void k_cinv(double* r, const double* x, double s, uint64_t n)
+template<typename DT>
+__global__ void k_cinv(DT* r, const DT* x, DT s, uint64_t n)
 {
   for(uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
     r[i] = s / x[i];
@@ -1684,8 +1689,16 @@ int feat_component_invert_f64(feat_handle_t h, double* r, const double* x, doubl
 {
   if(!h || (n && (!r || !x))) return fail(FEAT_ERR_INVALID, "feat_component_invert_f64: null argument");
   if(!n) return FEAT_OK;
-  hipLaunchKernelGGL(k_cinv, dim3(grid_for(n, 4)), dim3(kThreads), 0, h->stream, r, x, s, n);
+  hipLaunchKernelGGL(k_cinv<double>, dim3(grid_for(n, 4)), dim3(kThreads), 0, h->stream, r, x, s, n);
   return check_launch("feat_component_invert_f64");
+}
+
+int feat_component_invert_f32(feat_handle_t h, float* r, const float* x, float s, uint64_t n)
+{
+  if(!h || (n && (!r || !x))) return fail(FEAT_ERR_INVALID, "feat_component_invert_f32: null argument");
+  if(!n) return FEAT_OK;
+  hipLaunchKernelGGL(k_cinv<float>, dim3(grid_for(n, 4)), dim3(kThreads), 0, h->stream, r, x, s, n);
+  return check_launch("feat_component_invert_f32");
 }
 
 int feat_csr_diag_index_u64(feat_handle_t h, uint64_t* diag, const uint64_t* col, const uint64_t* rowptr,
@@ -2040,36 +2053,45 @@ int sell_create_impl(feat_handle_t h, const feat_csr_f64_t* A, M* out, const cha
       inv_scale = std::ldexp(1.0f, -e);
     }
   }
-  // value codes: at most 256 distinct values (bit patterns in the arithmetic type)
+  // value codes: at most 256 distinct values (bit patterns in the arithmetic type).  The
+  // values are scanned in chunks, and the scan stops at the 257th distinct value.
   std::vector<uint64_t> vkeys;
   if(!(flags & FEAT_SELL_NO_DICT) && !std::is_same<DT, uint16_t>::value && n && A->nnz)
   {
-    std::vector<double> hv(A->nnz);
-    rc = check_hip(hipMemcpyAsync(hv.data(), A->val, sizeof(double) * A->nnz, hipMemcpyDeviceToHost, h->stream), "copy");
-    if(!rc) rc = check_hip(hipStreamSynchronize(h->stream), "sync");
-    if(rc) { cleanup(); return rc; }
+    const uint64_t chunk = std::min<uint64_t>(A->nnz, uint64_t(1) << 20);
+    std::vector<double> hv(chunk);
     uint64_t last = ~uint64_t(0);
-    for(const double d : hv)
+    bool stop = false;
+    for(uint64_t c0 = 0; c0 < A->nnz && !stop; c0 += chunk)
     {
-      if(std::isnan(d))   // NaN payloads may convert differently on host and device
+      const uint64_t cn = std::min<uint64_t>(chunk, A->nnz - c0);
+      rc = check_hip(hipMemcpyAsync(hv.data(), A->val + c0, sizeof(double) * cn, hipMemcpyDeviceToHost, h->stream), "copy");
+      if(!rc) rc = check_hip(hipStreamSynchronize(h->stream), "sync");
+      if(rc) { cleanup(); return rc; }
+      for(uint64_t i = 0; i < cn; ++i)
       {
-        vkeys.clear();
-        break;
+        const double d = hv[i];
+        if(std::isnan(d))   // NaN payloads may convert differently on host and device
+        {
+          stop = true;
+          break;
+        }
+        const uint64_t key = value_key<DT>(DT(d));
+        if(key == last) continue;
+        last = key;
+        auto it = std::lower_bound(vkeys.begin(), vkeys.end(), key);
+        if(it != vkeys.end() && *it == key) continue;
+        if(vkeys.size() == 256)
+        {
+          stop = true;
+          break;
+        }
+        vkeys.insert(it, key);
       }
-      const uint64_t key = value_key<DT>(DT(d));
-      if(key == last) continue;
-      last = key;
-      auto it = std::lower_bound(vkeys.begin(), vkeys.end(), key);
-      if(it != vkeys.end() && *it == key) continue;
-      if(vkeys.size() == 256)
-      {
-        vkeys.clear();
-        break;
-      }
-      vkeys.insert(it, key);
     }
+    if(stop) vkeys.clear();
   }
-  const bool vcode = !vkeys.empty();
+  bool vcode = !vkeys.empty();
   uint64_t* dkeys = nullptr;
   const size_t vbytes = vcode ? sizeof(DT) * 256 + (total ? total : 1) : sizeof(DT) * (total ? total : 1);
   rc = check_hip(hipMalloc(&sval, vbytes), "sell val");
@@ -2082,23 +2104,37 @@ int sell_create_impl(feat_handle_t h, const feat_csr_f64_t* A, M* out, const cha
     rc = check_hip(hipMalloc(&dkeys, sizeof(uint64_t) * vkeys.size()), "value keys");
     if(!rc) rc = check_hip(hipMemcpyAsync(dkeys, vkeys.data(), sizeof(uint64_t) * vkeys.size(), hipMemcpyHostToDevice, h->stream), "copy");
     if(!rc) rc = check_hip(hipMemcpyAsync(sval, tab.data(), sizeof(DT) * 256, hipMemcpyHostToDevice, h->stream), "copy");
+    int* dmiss = nullptr;
+    int hmiss = 0;
     if constexpr(!std::is_same<DT, uint16_t>::value)   // FP16 plans keep their values
     {
+      if(!rc) rc = check_hip(hipMalloc(&dmiss, sizeof(int)), "value-code miss flag");
+      if(!rc) rc = check_hip(hipMemsetAsync(dmiss, 0, sizeof(int), h->stream), "memset");
       if(!rc)
       {
         uint8_t* codes = reinterpret_cast<uint8_t*>(sval + 256);
         if(d16)
           hipLaunchKernelGGL((k_sell_fill_code<DT, int16_t, true>), dim3(grid_for(n)), dim3(kThreads), 0, h->stream,
-                             A->val, A->col, A->rowptr, n, soff, dkeys, int(vkeys.size()), codes, (int16_t*)scol);
+                             A->val, A->col, A->rowptr, n, soff, dkeys, int(vkeys.size()), codes, (int16_t*)scol, dmiss);
         else
           hipLaunchKernelGGL((k_sell_fill_code<DT, uint32_t, false>), dim3(grid_for(n)), dim3(kThreads), 0, h->stream,
-                             A->val, A->col, A->rowptr, n, soff, dkeys, int(vkeys.size()), codes, (uint32_t*)scol);
+                             A->val, A->col, A->rowptr, n, soff, dkeys, int(vkeys.size()), codes, (uint32_t*)scol, dmiss);
         rc = check_launch(name);
+        if(!rc) rc = check_hip(hipMemcpyAsync(&hmiss, dmiss, sizeof(int), hipMemcpyDeviceToHost, h->stream), "copy");
         if(!rc) rc = check_hip(hipStreamSynchronize(h->stream), "sync");
       }
     }
+    if(dmiss) (void)hipFree(dmiss);
     if(dkeys) (void)hipFree(dkeys);
     if(rc) { cleanup(); return rc; }
+    if(hmiss)   // a value missed the host's table: full values instead of codes
+    {
+      vcode = false;
+      (void)hipFree(sval);
+      sval = nullptr;
+      rc = check_hip(hipMalloc(&sval, sizeof(DT) * (total ? total : 1)), "sell val");
+      if(rc) { cleanup(); return rc; }
+    }
   }
   if(rc) { cleanup(); return rc; }
   if(n && !vcode)

@@ -471,6 +471,20 @@ int feat_mgpcg_matrix(feat_mgpcg_t s, int lvl, std::uint64_t* rows, std::uint64_
   });
 }
 
+int feat_mgpcg_set_values(feat_mgpcg_t s, int lvl, const double* val_host)
+{
+  return guarded([&]() {
+    if(!s || !val_host || lvl < 0 || lvl >= int(s->levels.size())) throw Error("feat_mgpcg_set_values: bad argument");
+    Csr& A = s->levels[lvl]->A;
+    check(feat_copy(Runtime::handle(), A.val_mut(), val_host, sizeof(double) * A.used_elements()),
+          "feat_mgpcg_set_values: upload");
+    A.values_changed();
+    s->hier->init_numeric();
+    s->pcg->init_numeric();
+    check(feat_sync(Runtime::handle()), "feat_mgpcg_set_values: sync");
+  });
+}
+
 struct feat_krylov_s
 {
   Csr A;

@@ -142,6 +142,9 @@ int feat_component_product_f64(feat_handle_t h, double* r, const double* x, cons
 /* ComponentInvert::value (arch/component_invert_generic.hpp:21-36): r = s/x */
 int feat_component_invert_f64(feat_handle_t h, double* r, const double* x, double s,
                               uint64_t n);
+/* the same for float (JacobiPrecond<float>::init_numeric, jacobi_precond.hpp:136-143,
+ * inverts the FP32 diagonal with float(omega)) */
+int feat_component_invert_f32(feat_handle_t h, float* r, const float* x, float s, uint64_t n);
 /* Diagonal::csr (arch/diagonal_generic.hpp:21-38): index of the diagonal
  * entry of each row, nnz where absent */
 int feat_csr_diag_index_u64(feat_handle_t h, uint64_t* diag, const uint64_t* col,

@@ -344,6 +344,8 @@ namespace feat3amd
         return *_plan;
       }
 
+      bool has_plan() const { return bool(_plan); }
+
       /// Picks the fastest launch shape of the execution plan's row kernels
       /// (feat_autotune_matrix_f64; bit-identical results for every shape).
       /// Returns the time per SpMV of the chosen shape in microseconds (-1: nothing to tune).
@@ -410,6 +412,15 @@ namespace feat3amd
           _plan16 = p;
         }
         return *_plan16;
+      }
+
+      /// The values (not the pattern) were overwritten: the execution plans, which
+      /// hold value codes or converted values, are built again on next use.
+      void values_changed()
+      {
+        _plan.reset();
+        _plan32.reset();
+        _plan16.reset();
       }
 
       /// Selects CSR (false) or the SELL-64 plan (true, default) for exec().

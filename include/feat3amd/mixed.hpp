@@ -3,8 +3,9 @@
 // The ConvertPrecond pattern (solver/convert_precond.hpp:100-119): the FP64
 // defect is converted to FP32, an FP32 V-cycle is applied and the correction
 // is converted back.  Here the V-cycle runs on FP32 SELL-64 plans of the same
-// matrices (values rounded once), FP32 Jacobi diagonals (omega/a_ii computed
-// in FP64, rounded once) and FP32 vectors, while the coarse-grid solve stays
+// matrices (values rounded once), FP32 Jacobi diagonals (float(omega) divided by
+// the FP32 diagonal in FP32, as JacobiPrecond<float> forms them,
+// jacobi_precond.hpp:136-143) and FP32 vectors, while the coarse-grid solve stays
 // FP64 (the dense inverse of DirectCoarseSolver applied in FP64).  Used as the
 // preconditioner of an FP64 PCG, the outer residual, solution update and
 // convergence test stay FP64: iterative refinement with Krylov acceleration.
@@ -52,6 +53,19 @@ namespace feat3amd
       {
         Vector32 r(v.size());
         check(feat_convert_f64_f32(Runtime::handle(), r.elements(), (const double*)v.elements(), v.size()), "to32");
+        return r;
+      }
+
+      /// JacobiPrecond<float>::init_numeric on the FP32 copy of the matrix: its
+      /// diagonal (each a_ii rounded once), then float(omega) / d_ii in FP32
+      template<typename Jac_>
+      static Vector32 inv_diag32(const Jac_& jac)
+      {
+        VectorType d(jac.matrix().rows());
+        jac.matrix().extract_diag(d);
+        Vector32 r = to32(d);
+        check(feat_component_invert_f32(Runtime::handle(), r.elements(), r.elements(), float(jac.omega()), r.size()),
+              "inv_diag32");
         return r;
       }
 
@@ -190,8 +204,8 @@ namespace feat3amd
             (void)plan(lv.transfer->get_mat_rest());
             auto* pre = static_cast<RichType*>(lv.pre.get());
             auto* post = static_cast<RichType*>(lv.post.get());
-            _l[i].inv_pre = to32(pre->fusable_jacobi()->inv_diag());
-            _l[i].inv_post = to32(post->fusable_jacobi()->inv_diag());
+            _l[i].inv_pre = inv_diag32(*pre->fusable_jacobi());
+            _l[i].inv_post = inv_diag32(*post->fusable_jacobi());
             _l[i].pre_steps = int(pre->get_max_iter());
             _l[i].post_steps = int(post->get_max_iter());
             _l[i].omega_pre = float(pre->omega());
@@ -200,11 +214,11 @@ namespace feat3amd
         }
         auto& lc = _h->levels[L - 1];
         if(_coarse == Coarse::jacobi)
-          _l[L - 1].inv_pre = to32(static_cast<JacType*>(lc.coarse.get())->inv_diag());
+          _l[L - 1].inv_pre = inv_diag32(*static_cast<JacType*>(lc.coarse.get()));
         else if(_coarse == Coarse::richardson)
         {
           auto* r = static_cast<RichType*>(lc.coarse.get());
-          _l[L - 1].inv_pre = to32(r->fusable_jacobi()->inv_diag());
+          _l[L - 1].inv_pre = inv_diag32(*r->fusable_jacobi());
           _l[L - 1].pre_steps = int(r->get_max_iter());
           _l[L - 1].omega_pre = float(r->omega());
         }

@@ -573,6 +573,7 @@ namespace feat3amd
       {
         const Index L = _h->size();
         _fw.create(2 * L + 8);
+        _tail_built = false;
         _fusable = true;
         for(Index i = 0; i + 1 < L; ++i)
         {
@@ -592,23 +593,40 @@ namespace feat3amd
         else
           _coarse_kind = Coarse::other;
         if(_coarse_kind == Coarse::other) _fusable = false;
-        // execution plans are built here (they allocate), never inside a recorded graph
+        _build_plans(true);
+      }
+
+      /// Execution plans (and launch shapes) of every level's matrices that have none,
+      /// at init_symbolic and again at init_numeric for matrices whose values changed:
+      /// plans allocate, so they are never built inside a recorded graph.
+      void _build_plans(bool all = false)
+      {
+        auto prep = [&](const Matrix_& m) {
+          const bool fresh = all || !m.has_plan();
+          (void)m.exec();
+          if(_autotune && fresh) (void)m.autotune();
+        };
+        const Index L = _h->size();
         for(Index i = 0; i < L; ++i)
         {
-          (void)_h->levels[i].matrix->exec();
-          if(_autotune) (void)_h->levels[i].matrix->autotune();
+          prep(*_h->levels[i].matrix);
           if(i + 1 < L)
           {
-            (void)_h->levels[i].transfer->get_mat_prol().exec();
-            (void)_h->levels[i].transfer->get_mat_rest().exec();
-            if(_autotune)
-            {
-              (void)_h->levels[i].transfer->get_mat_prol().autotune();
-              (void)_h->levels[i].transfer->get_mat_rest().autotune();
-            }
+            prep(_h->levels[i].transfer->get_mat_prol());
+            prep(_h->levels[i].transfer->get_mat_rest());
           }
         }
       }
+      /// multigrid.hpp:1235-1241: the hierarchy's init_numeric runs first (it may
+      /// replace the coarse inverse and the Jacobi diagonals), so the fused coarse
+      /// levels' descriptor, which holds their addresses, is built again on next use
+      void init_numeric() override
+      {
+        _tail_built = false;
+        _build_plans();
+      }
+      void done_numeric() override { _tail_built = false; }
+      void done_symbolic() override { _tail_built = false; }
       bool fusable() const override { return _fusable; }
       const FusedWorkspace& workspace() const { return _fw; }
 

@@ -117,11 +117,15 @@ int feat_mgpcg_create_q1_ex(feat_mgpcg_t* s, int dim, int lf, int lc, const feat
 /* Solver::solve(pcg, x, b, A, F) -> PCG::correct: x is the initial guess (device), b device */
 int feat_mgpcg_solve(feat_mgpcg_t s, double* x, const double* b, feat_solve_result_t* res,
                      double* hist, uint64_t hist_cap);
-/* device descriptor of level `lvl`'s system matrix (0 = finest) */
 /* SparseMatrixCSR::write_out of level lvl's (device-resident) system matrix */
 int feat_mgpcg_write_matrix(feat_mgpcg_t s, int lvl, int mode, const char* path, int symmetric);
+/* device arrays of level `lvl`'s system matrix (0 = finest) */
 int feat_mgpcg_matrix(feat_mgpcg_t s, int lvl, uint64_t* rows, uint64_t* nnz, const double** val,
                       const uint64_t** col, const uint64_t** rowptr);
+/* New values (host array, same pattern) for level lvl's system matrix, then the
+ * reference's numeric re-initialisation order: hierarchy init_numeric, then the
+ * solver tree's init_numeric (multigrid.hpp:1225-1240) */
+int feat_mgpcg_set_values(feat_mgpcg_t s, int lvl, const double* val_host);
 /* first level of the coarse levels that run fused as one workgroup (feat_mg_tail_f64 /
  * feat_mg_tail_f32) in s's V-cycle; 0 = none.  No counterpart in the reference. */
 int feat_mgpcg_coarse_fusion_level(feat_mgpcg_t s, int* level);

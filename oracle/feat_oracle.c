@@ -765,9 +765,33 @@ typedef struct {
   double* cinv;   /* coarse: FP64 dense inverse */
 } lvl32_t;
 
-/* half = 0: values rounded to FP32.  half = 1: FP16 values half(float(a)*2^-e)
- * with 2^(e-1) <= max|float(a)| < 2^e, kept here as the FP32 value of the FP16
- * number; row sums are multiplied by 2^e (F16C conversions, round to nearest even). */
+/* FP16 values of a matrix: half(float(a)*2^-e) with 2^(e-1) <= max|float(a)| < 2^e
+ * (no scaling when the maximum is 0 or not finite), returned as the FP32 value of
+ * each FP16 number; *scale = 2^e multiplies every row sum.  F16C conversions,
+ * round to nearest even. */
+void orc_f16_values(const double* val, uint64_t nnz, float* out, float* scale)
+{
+  float inv = 1.0f;
+  *scale = 1.0f;
+  float mx = 0.0f;
+  for(uint64_t i = 0; i < nnz; ++i)
+  {
+    const float a = fabsf((float)val[i]);
+    if(a > mx) mx = a;
+  }
+  if(mx > 0.0f && isfinite(mx))
+  {
+    int e = 0;
+    (void)frexpf(mx, &e);
+    *scale = ldexpf(1.0f, e);
+    inv = ldexpf(1.0f, -e);
+  }
+  for(uint64_t i = 0; i < nnz; ++i)
+    out[i] = _cvtsh_ss(_cvtss_sh((float)val[i] * inv, 0));
+}
+
+/* half = 0: values rounded to FP32 (LAFEM convert, one static_cast per value).
+ * half = 1: FP16 values (orc_f16_values). */
 static csr32_t csr_to32(const orc_csr_t* m, int half)
 {
   csr32_t r;
@@ -777,28 +801,10 @@ static csr32_t csr_to32(const orc_csr_t* m, int half)
   r.rowptr = m->rowptr;
   r.scale = 1.0f;
   r.val = (float*)malloc(sizeof(float) * (m->nnz ? m->nnz : 1));
-  float inv = 1.0f;
   if(half)
-  {
-    float mx = 0.0f;
-    for(uint64_t i = 0; i < m->nnz; ++i)
-    {
-      const float a = fabsf((float)m->val[i]);
-      if(a > mx) mx = a;
-    }
-    if(mx > 0.0f && isfinite(mx))
-    {
-      int e = 0;
-      (void)frexpf(mx, &e);
-      r.scale = ldexpf(1.0f, e);
-      inv = ldexpf(1.0f, -e);
-    }
-  }
-  for(uint64_t i = 0; i < m->nnz; ++i)
-  {
-    const float f = (float)m->val[i];
-    r.val[i] = half ? _cvtsh_ss(_cvtss_sh(f * inv, 0)) : f;
-  }
+    orc_f16_values(m->val, m->nnz, r.val, &r.scale);
+  else
+    for(uint64_t i = 0; i < m->nnz; ++i) r.val[i] = (float)m->val[i];
   return r;
 }
 
@@ -869,8 +875,9 @@ typedef struct {
   const orc_mg_cfg_t* cfg;
 } mg32_t;
 
-static void mg32_init(mg32_t* m, const orc_mg_level_t* lv, int nl, const orc_mg_cfg_t* cfg, int half)
+static void mg32_init(mg32_t* m, const orc_mg_level_t* lv, int nl, const orc_mg_cfg_t* cfg, int flags)
 {
+  const int half = flags & ORC_MIXED_F16;
   m->nl = nl;
   m->cfg = cfg;
   m->L = (lvl32_t*)calloc((size_t)nl, sizeof(lvl32_t));
@@ -889,10 +896,17 @@ static void mg32_init(mg32_t* m, const orc_mg_level_t* lv, int nl, const orc_mg_
     L->mask = (char*)calloc(n ? n : 1, 1);
     for(uint64_t k = 0; k < lv[i].nfilt; ++k) L->mask[lv[i].filt[k]] = 1;
     const double om = (i + 1 == nl && cfg->coarse == ORC_COARSE_JACOBI) ? cfg->coarse_omega : cfg->omega_jac;
-    double* inv64 = jacobi_init(&lv[i].A, om);
+    /* JacobiPrecond<float>::init_numeric (jacobi_precond.hpp:136-143): the diagonal
+     * of the FP32 matrix (each a_ii rounded once), then float(omega) / d_ii in FP32
+     * (arch/component_invert_generic.hpp:21-37).  The diagonal is a vector and keeps
+     * FP32 precision in the FP16-valued form as well.  ORC_MIXED_SLIP_INV64 is a
+     * planted slip for the oracle's own pin test: omega / a_ii in FP64, rounded. */
+    double* d64 = (double*)malloc(sizeof(double) * (n ? n : 1));
+    orc_extract_diag_f64(d64, lv[i].A.val, lv[i].A.col, lv[i].A.rowptr, n);
     L->inv = (float*)malloc(sizeof(float) * (n ? n : 1));
-    for(uint64_t k = 0; k < n; ++k) L->inv[k] = (float)inv64[k];
-    free(inv64);
+    for(uint64_t k = 0; k < n; ++k)
+      L->inv[k] = (flags & ORC_MIXED_SLIP_INV64) ? (float)(om / d64[k]) : (float)om / (float)d64[k];
+    free(d64);
     float** vecs[6] = {&L->rhs, &L->sol, &L->def, &L->cor, &L->xa, &L->xb};
     for(int k = 0; k < 6; ++k) *vecs[k] = (float*)calloc(n ? n : 1, sizeof(float));
     if(i + 1 == nl && cfg->coarse == ORC_COARSE_DIRECT)
@@ -982,13 +996,13 @@ static void mg32_precond(void* ctx, double* cor, const double* def)
 }
 
 int orc_mg_pcg_solve_mixed(const orc_mg_level_t* levels, int nlevels, const orc_mg_cfg_t* cfg,
-                           const orc_iter_cfg_t* it, int half, double* x, const double* b,
+                           const orc_iter_cfg_t* it, int flags, double* x, const double* b,
                            double* hist, orc_result_t* res)
 {
   const orc_csr_t* A = &levels[0].A;
   const uint64_t n = A->rows;
   mg32ctx_t ctx;
-  mg32_init(&ctx.mg, levels, nlevels, cfg, half);
+  mg32_init(&ctx.mg, levels, nlevels, cfg, flags);
   double* r = (double*)malloc(sizeof(double) * (n ? n : 1));
   csr_apply(A, r, -1.0, x, 1.0, b);
   orc_unit_filter_def_f64(r, levels[0].filt, levels[0].nfilt);

@@ -122,12 +122,18 @@ int orc_mg_pcg_solve(const orc_mg_level_t* levels, int nlevels, const orc_mg_cfg
 
 /* The same PCG with an FP32 V-cycle as preconditioner (the ConvertPrecond
  * pattern, solver/convert_precond.hpp:100-119): FP64 outer residual, update and
- * convergence test; FP32 matrices (half != 0: FP16 values with a power-of-two
- * scale, FP32 arithmetic), Jacobi diagonals, smoothing and transfers; FP64
- * coarse solve. */
+ * convergence test; FP32 matrices (flags & ORC_MIXED_F16: FP16 values with a
+ * power-of-two scale, FP32 arithmetic), FP32 Jacobi diagonals (float(omega) /
+ * float(a_ii), as JacobiPrecond<float>), smoothing and transfers; the coarse
+ * solve is FP64 for ORC_COARSE_DIRECT and FP32 otherwise.
+ * ORC_MIXED_SLIP_INV64 plants a rounding slip (omega / a_ii in FP64, then
+ * rounded): it exists only so that a test can show the pin catches it. */
+enum { ORC_MIXED_F16 = 1, ORC_MIXED_SLIP_INV64 = 2 };
 int orc_mg_pcg_solve_mixed(const orc_mg_level_t* levels, int nlevels, const orc_mg_cfg_t* mg,
-                           const orc_iter_cfg_t* it, int half, double* x, const double* b,
+                           const orc_iter_cfg_t* it, int flags, double* x, const double* b,
                            double* def_hist, orc_result_t* res);
+/* FP16 values of a matrix as the FP16-valued form stores them (see feat_oracle.c) */
+void orc_f16_values(const double* val, uint64_t nnz, float* out, float* scale);
 
 /* Stand-alone iterative solvers on one CSR matrix with a unit filter.
  * precond: 0 = none, 1 = Jacobi(omega).  Variant of Solver::solve

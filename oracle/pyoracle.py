@@ -24,6 +24,8 @@ _vp = C.c_void_p
 STATUS = {0: "undefined", 1: "progress", 2: "success", 3: "aborted", 4: "diverged",
           5: "max_iter", 6: "stagnated"}
 COARSE_NONE, COARSE_SMOOTHER, COARSE_JACOBI, COARSE_DIRECT = 0, 1, 2, 3
+# flags of orc_mg_pcg_solve_mixed (oracle/feat_oracle.h)
+MIXED_F16, MIXED_SLIP_INV64 = 1, 2
 
 
 @dataclass
@@ -216,6 +218,7 @@ class Oracle:
         L.orc_mg_pcg_solve_mixed.argtypes = [C.POINTER(_LevelS), C.c_int, C.POINTER(_MgCfgS),
                                              C.POINTER(_IterCfgS), C.c_int, _f64p, _f64p, _vp,
                                              C.POINTER(_ResultS)]
+        L.orc_f16_values.argtypes = [_f64p, C.c_uint64, _f32p, C.POINTER(C.c_float)]
         for fn in ("orc_pcg_solve", "orc_pcg_apply"):
             f = getattr(L, fn)
             f.restype = C.c_int
@@ -447,6 +450,15 @@ class Oracle:
                                  A.rows, A.cols, int(transposed))
         return r
 
+    def f16_values(self, val):
+        """(FP32 values of the FP16 numbers, power-of-two scale) as the FP16-valued
+        form stores a matrix."""
+        v = np.ascontiguousarray(val, np.float64)
+        out = np.empty(max(v.size, 1), np.float32)
+        sc = C.c_float()
+        self.L.orc_f16_values(v, v.size, out, C.byref(sc))
+        return out[:v.size], sc.value
+
     def bcsr_apply(self, bh, bw, rows, val, col, rowptr, x, y=None, a=1.0, b=0.0):
         y = np.zeros(rows * bh) if y is None else y
         r = np.empty(rows * bh)
@@ -536,13 +548,16 @@ class Oracle:
 
     def mg_pcg_solve(self, levels, x0, b, nu_pre=4, nu_post=4, omega_jac=0.5, omega_rich=1.0,
                      coarse=COARSE_DIRECT, coarse_omega=1.0, tol_rel=1e-8, max_iter=100,
-                     precision="f64", **kw):
+                     precision="f64", planted_slip=False, **kw):
+        """precision: "f64", "f32" or "f16" (FP16 values).  planted_slip (mixed forms
+        only) turns on the oracle's planted rounding slip, for the test that shows
+        the pin against the reference catches it."""
         arr, keep = _levels_s(levels)
         if precision == "f64":
             fn = self.L.orc_mg_pcg_solve
         else:
-            half = 1 if precision == "f16" else 0
-            fn = lambda *a: self.L.orc_mg_pcg_solve_mixed(*a[:4], half, *a[4:])
+            flags = (MIXED_F16 if precision == "f16" else 0) | (MIXED_SLIP_INV64 if planted_slip else 0)
+            fn = lambda *a: self.L.orc_mg_pcg_solve_mixed(*a[:4], flags, *a[4:])
         mg = _MgCfgS(nu_pre, nu_post, omega_jac, omega_rich, coarse, coarse_omega)
         it = self.iter_cfg(tol_rel=tol_rel, max_iter=max_iter, **kw)
         x = np.ascontiguousarray(x0, np.float64).copy()
@@ -611,6 +626,14 @@ class Reference:
                                        C.c_double, C.c_int, C.c_double, C.c_double, C.c_uint64,
                                        _f64p, _f64p, _vp, C.c_uint64, C.POINTER(C.c_uint64),
                                        C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.ref_mg_pcg_solve_mixed.restype = C.c_int
+        L.ref_mg_pcg_solve_mixed.argtypes = [C.POINTER(_LevelS), C.c_int, C.c_int, C.c_int, C.c_double,
+                                             C.c_double, C.c_int, C.c_double, C.c_double, C.c_uint64,
+                                             _f64p, _f64p, _vp, C.c_uint64, C.POINTER(C.c_uint64),
+                                             C.POINTER(C.c_double)]
+        L.ref_csr_apply_f32.argtypes = [_f32p, C.c_float, _f32p, C.c_float, _f32p, _f32p,
+                                        _u64p, _u64p, C.c_uint64, C.c_uint64, C.c_uint64, C.c_int]
+        L.ref_component_invert_f32.argtypes = [_f32p, _f32p, C.c_float, C.c_uint64]
         L.ref_single_solve.restype = C.c_int
         L.ref_single_solve.argtypes = [C.c_int, C.POINTER(_CsrS), _vp, C.c_uint64, C.c_int,
                                        C.c_double, C.c_double, C.c_double, C.c_uint64, C.c_int,
@@ -753,6 +776,40 @@ class Reference:
         self.L.ref_csr_apply_f64(r, a, np.ascontiguousarray(x, np.float64), b, y, A.val, A.col,
                                  A.rowptr, A.rows, A.cols, A.nnz, int(transposed))
         return r
+
+    def csr_apply_f32(self, A, x, y=None, a=1.0, b=0.0, transposed=False):
+        """SparseMatrixCSR<float, Index> apply (arch/apply_generic.hpp:25-69 for float)."""
+        nr = A.cols if transposed else A.rows
+        y = np.zeros(nr, np.float32) if y is None else np.ascontiguousarray(y, np.float32)
+        r = np.empty(nr, np.float32)
+        self.L.ref_csr_apply_f32(r, a, np.ascontiguousarray(x, np.float32), b, y,
+                                 np.ascontiguousarray(A.val, np.float32), A.col, A.rowptr,
+                                 A.rows, A.cols, A.nnz, int(transposed))
+        return r
+
+    def component_invert_f32(self, x, s=1.0):
+        x = np.ascontiguousarray(x, np.float32)
+        r = np.empty_like(x)
+        self.L.ref_component_invert_f32(r, x, s, x.size)
+        return r
+
+    def mg_pcg_solve_mixed(self, levels, x0, b, nu_pre=4, nu_post=4, omega_jac=0.5, omega_rich=1.0,
+                           coarse=COARSE_JACOBI, coarse_omega=1.0, tol_rel=1e-8, max_iter=100):
+        """PCG<double> preconditioned by ConvertPrecond around a float V-cycle of the
+        reference (oracle/ref_driver.cpp ref_mg_pcg_solve_mixed); COARSE_DIRECT is the
+        driver's FP64 dense-inverse coarse solver."""
+        ref_coarse = {COARSE_SMOOTHER: 1, COARSE_JACOBI: 2, COARSE_DIRECT: 3}[coarse]
+        arr, keep = _levels_s(levels)
+        x = np.ascontiguousarray(x0, np.float64).copy()
+        hist = np.full(max_iter + 2, np.nan)
+        iters = C.c_uint64()
+        dfin = C.c_double()
+        st = self.L.ref_mg_pcg_solve_mixed(arr, len(levels), nu_pre, nu_post, omega_jac, omega_rich,
+                                           ref_coarse, coarse_omega, tol_rel, max_iter, x,
+                                           np.ascontiguousarray(b, np.float64), hist.ctypes.data,
+                                           hist.size, C.byref(iters), C.byref(dfin))
+        return dict(x=x, status=STATUS[st], iters=int(iters.value), def_final=dfin.value,
+                    hist=hist[:int(iters.value) + 1])
 
     def bcsr_apply(self, bh, bw, rows, cols, val, col, rowptr, x, y=None, a=1.0, b=0.0):
         y = np.zeros(rows * bh) if y is None else np.ascontiguousarray(y, np.float64)

@@ -29,6 +29,7 @@
 #include <kernel/solver/richardson.hpp>
 #include <kernel/solver/jacobi_precond.hpp>
 #include <kernel/solver/multigrid.hpp>
+#include <kernel/solver/convert_precond.hpp>
 #include <kernel/geometry/boundary_factory.hpp>
 #include <kernel/geometry/conformal_mesh.hpp>
 #include <kernel/geometry/common_factories.hpp>
@@ -398,6 +399,165 @@ int ref_single_solve(int kind, const ref_csr_t* m, const std::uint64_t* filt, st
   *iters = s->get_num_iter();
   *def_final = s->get_def_final();
   s->done();
+  return int(st);
+}
+
+} // extern "C"
+
+// ---------------- FP32 kernels and the mixed-precision V-cycle ----------------
+typedef LAFEM::DenseVector<float, IT> Vec32;
+typedef LAFEM::SparseMatrixCSR<float, IT> Csr32;
+typedef LAFEM::UnitFilter<float, IT> UFilter32;
+typedef LAFEM::Transfer<Csr32> Trans32;
+
+namespace
+{
+  // A coarse solver of the float hierarchy that applies the FP64 inverse of the
+  // filtered coarse matrix to the float right-hand side (one FP64 dot per row in
+  // column order, rounded once to float; filtered rows get 0).  The reference
+  // plugs coarse solvers in through SolverBase (multigrid.hpp:1549-1557); this
+  // one is the driver's own: the product keeps its coarse solve in FP64.
+  class DenseInverseCoarse : public Solver::SolverBase<Vec32>
+  {
+    Index _n;
+    std::vector<double> _inv;
+    std::vector<char> _mask;
+  public:
+    DenseInverseCoarse(const Csr& A, const std::uint64_t* filt, std::uint64_t nf) :
+      _n(A.rows()), _inv(std::size_t(A.rows() * A.rows()), 0.0), _mask(std::size_t(A.rows()), 0)
+    {
+      for(std::uint64_t k = 0; k < nf; ++k) _mask[filt[k]] = 1;
+      const IT* rp = A.row_ptr();
+      const IT* ci = A.col_ind();
+      const double* va = A.val();
+      for(Index r = 0; r < _n; ++r)
+      {
+        if(_mask[r]) { _inv[r * _n + r] = 1.0; continue; }
+        for(IT i = rp[r]; i < rp[r + 1]; ++i)
+          if(!_mask[ci[i]]) _inv[r * _n + ci[i]] = va[i];
+      }
+      std::vector<int> piv(_n, 0);
+      Math::invert_matrix(int(_n), int(_n), _inv.data(), piv.data());
+    }
+    virtual String name() const override { return "DenseInverseCoarse"; }
+    virtual Solver::Status apply(Vec32& cor, const Vec32& def) override
+    {
+      const float* d = def.elements();
+      float* c = cor.elements();
+      for(Index r = 0; r < _n; ++r)
+      {
+        double s = 0.0;
+        for(Index k = 0; k < _n; ++k) s = std::fma(_inv[r * _n + k], double(d[k]), s);
+        c[r] = _mask[r] ? 0.0f : float(s);
+      }
+      return Solver::Status::success;
+    }
+  };
+
+  Csr32 make_csr32(const ref_csr_t& m)
+  {
+    Csr32 a;
+    a.convert(make_csr(m));
+    return a;
+  }
+
+  UFilter32 make_filter32(Index n, const std::uint64_t* idx, std::uint64_t nf)
+  {
+    UFilter32 f(n);
+    for(std::uint64_t i = 0; i < nf; ++i)
+      f.add(IT(idx[i]), 0.0f);
+    return f;
+  }
+}
+
+extern "C" {
+
+void ref_csr_apply_f32(float* r, float a, const float* x, float b, const float* y,
+                       const float* val, const std::uint64_t* col, const std::uint64_t* rowptr,
+                       std::uint64_t rows, std::uint64_t cols, std::uint64_t nnz, int transposed)
+{
+  ensure_runtime();
+  LAFEM::Arch::Apply::csr_generic(r, a, x, b, y, val, (const IT*)col, (const IT*)rowptr,
+                                  Index(rows), Index(cols), Index(nnz), transposed != 0);
+}
+
+void ref_component_invert_f32(float* r, const float* x, float s, std::uint64_t n)
+{
+  LAFEM::Arch::ComponentInvert::value_generic(r, x, s, Index(n));
+}
+
+// PCG<double> preconditioned by ConvertPrecond<double -> float> around a float
+// MultiGrid V-cycle (convert_precond.hpp:100-119) whose levels hold
+// SparseMatrixCSR<float> copies of the given matrices (LAFEM convert: each value
+// rounded once), float JacobiPrecond / Richardson smoothers and float Transfer
+// operators.  coarse: 1 float Richardson-Jacobi smoother, 2 float Jacobi
+// (coarse_omega), 3 FP64 dense inverse (DenseInverseCoarse above).
+int ref_mg_pcg_solve_mixed(const ref_level_t* L, int nl, int nu_pre, int nu_post, double omega_jac,
+                           double omega_rich, int coarse, double coarse_omega, double tol_rel,
+                           std::uint64_t max_iter, double* x, const double* b, double* hist,
+                           std::uint64_t hist_cap, std::uint64_t* iters, double* def_final)
+{
+  ensure_runtime();
+  Csr A0 = make_csr(L[0].A);
+  UFilter F0 = make_filter(L[0].A.rows, L[0].filt, L[0].nfilt);
+  std::vector<Csr32> A(nl);
+  std::vector<UFilter32> F;
+  std::vector<Trans32> T(nl);
+  F.reserve(nl);
+  for(int i = 0; i < nl; ++i)
+  {
+    A[i] = make_csr32(L[i].A);
+    F.push_back(make_filter32(L[i].A.rows, L[i].filt, L[i].nfilt));
+    if(i + 1 < nl)
+      T[i] = Trans32(make_csr32(L[i].P), make_csr32(L[i].R));
+  }
+  auto hierarchy = std::make_shared<Solver::MultiGridHierarchy<Csr32, UFilter32, Trans32>>(nl);
+  for(int i = 0; i < nl; ++i)
+  {
+    auto jac = Solver::new_jacobi_precond(A[i], F[i], float(omega_jac));
+    if(i + 1 < nl)
+    {
+      auto pre = Solver::new_richardson(A[i], F[i], float(omega_rich), jac);
+      pre->set_min_iter(nu_pre);
+      pre->set_max_iter(nu_pre);
+      auto post = Solver::new_richardson(A[i], F[i], float(omega_rich), jac);
+      post->set_min_iter(nu_post);
+      post->set_max_iter(nu_post);
+      hierarchy->push_level(A[i], F[i], T[i], pre, post, pre);
+    }
+    else
+    {
+      std::shared_ptr<Solver::SolverBase<Vec32>> cs;
+      if(coarse == 1)
+      {
+        auto s = Solver::new_richardson(A[i], F[i], float(omega_rich), jac);
+        s->set_min_iter(nu_pre);
+        s->set_max_iter(nu_pre);
+        cs = s;
+      }
+      else if(coarse == 2)
+        cs = Solver::new_jacobi_precond(A[i], F[i], float(coarse_omega));
+      else if(coarse == 3)
+        cs = std::make_shared<DenseInverseCoarse>(make_csr(L[i].A), L[i].filt, L[i].nfilt);
+      hierarchy->push_level(A[i], F[i], cs);
+    }
+  }
+  auto mgv = Solver::new_multigrid(hierarchy, Solver::MultiGridCycle::V);
+  auto conv = Solver::new_convert_precond<Vec, Vec32>(mgv);
+  auto pcg = std::make_shared<Recorder<Solver::PCG<Csr, UFilter>>>(A0, F0, conv);
+  pcg->set_tol_rel(tol_rel);
+  pcg->set_max_iter(max_iter);
+  hierarchy->init();
+  pcg->init();
+  Vec vx = make_vec(x, L[0].A.rows);
+  Vec vb = make_vec(b, L[0].A.rows);
+  Solver::Status st = Solver::solve(*pcg, vx, vb, A0, F0);
+  std::memcpy(x, vx.elements(), sizeof(double) * L[0].A.rows);
+  copy_hist(pcg->hist, hist, hist_cap);
+  *iters = pcg->get_num_iter();
+  *def_final = pcg->get_def_final();
+  pcg->done();
+  hierarchy->done();
   return int(st);
 }
 

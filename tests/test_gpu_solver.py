@@ -335,6 +335,58 @@ def test_sell_value_codes(feat, handle, torch_gpu, oracle, with_nan):
     p32.close()
 
 
+@pytest.mark.parametrize("nvals,specials", [(256, False), (257, False), (256, True), (257, True)])
+def test_sell_value_codes_boundary(feat, handle, torch_gpu, oracle, nvals, specials):
+    """Exactly 256 distinct values get codes 0..255, a 257th keeps full values; +0.0 and
+    -0.0 are distinct keys, and +-Inf are values like any other.  Both forms reproduce
+    the CSR loop bit for bit in FP64 and FP32 (values exact in FP32, so both precisions
+    see the same distinct count).  Random columns keep the pair dictionary out."""
+    print(f"value-code boundary nvals={nvals} specials={specials}", flush=True)
+    torch = torch_gpu
+    k, _ = feat.libs()
+    P = feat.ptr
+    rng = np.random.default_rng(nvals + 7 * specials)
+    table = (np.arange(nvals) - 100) / 8.0
+    if specials:
+        table[:4] = [0.0, -0.0, np.inf, -np.inf]
+    rows, cols, per = 700, 900, 5
+    rp = (np.arange(rows + 1) * per).astype(np.uint64)
+    col = np.concatenate([np.sort(rng.choice(cols, per, replace=False)) for _ in range(rows)]).astype(np.uint64)
+    idx = np.concatenate([np.arange(nvals), rng.integers(0, nvals, rows * per - nvals)])
+    val = table[rng.permutation(idx)]
+    A = Csr(rows, cols, val, col, rp)
+    v, c, r = upload_csr(torch, A)
+    x = rng.standard_normal(cols).astype(np.float32).astype(np.float64)
+    xd = dev(torch, x)
+    rd = torch.empty(rows, dtype=torch.float64, device="cuda")
+    want = feat.VAL_CODE8 if nvals <= 256 else feat.VAL_NATIVE
+    plan = feat.SellPlan(handle, v, c, r, rows, cols)
+    assert plan.m.index in (feat.IDX_U32, feat.IDX_D16)
+    assert plan.m.value_kind == want
+    feat.check(k.feat_matrix_apply_f64(handle.h, plan.m, P(rd), 1.0, P(xd), 0.0, None), "apply")
+    handle.sync()
+    _same_bits(host(rd), oracle.csr_apply(A, x))
+    plan.close()
+    p32 = feat.SellPlan(handle, v, c, r, rows, cols, fp="f32")
+    assert p32.m.value_kind == want
+    x32 = dev(torch, x.astype(np.float32))
+    r32 = torch.empty(rows, dtype=torch.float32, device="cuda")
+    keep_ws, ws = feat.reduce_ws(torch, "cuda")
+    feat.check(k.feat_apply_masked_f32(handle.h, p32.m, P(r32), P(x32), None, None, ws), "apply f32")
+    handle.sync()
+    _same_bits(host(r32), oracle.csr_apply_f32(A, x.astype(np.float32)))
+    p32.close()
+
+
+def _same_bits(a, b):
+    """Bit equality, with NaN only required in the same places (the payload and sign
+    of a NaN made by Inf - Inf differ between the host CPU and the GPU)."""
+    na, nb = np.isnan(a), np.isnan(b)
+    assert np.array_equal(na, nb)
+    u = np.uint64 if a.dtype == np.float64 else np.uint32
+    assert np.array_equal(a[~na].view(u), b[~nb].view(u))
+
+
 def test_sell_plan_rejects_long_rows(feat, handle, torch_gpu):
     # a 70000-entry row exceeds the 16-bit row length; the plan refuses and the CSR stays usable
     torch = torch_gpu
@@ -564,3 +616,79 @@ def test_coarse_fusion_bitwise(feat, torch_gpu, dim, lf, lc, first, precision):
     assert r1["iters"] == r0["iters"]
     assert np.array_equal(r1["hist"].view(np.uint64), r0["hist"].view(np.uint64))
     assert np.array_equal(x1.view(np.uint64), x0.view(np.uint64))
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+@pytest.mark.parametrize("nu_pre,nu_post", [(1, 1), (3, 2), (2, 5)])
+def test_coarse_fusion_bitwise_step_counts(feat, torch_gpu, nu_pre, nu_post, precision):
+    """The fused coarse levels with one smoothing step (the first-iterate-only
+    branch), odd step counts (the ping-pong parity of the iterates) and unequal
+    pre/post counts equal the separate launches bit for bit (3D level 5)."""
+    print(f"coarse fusion {precision} V({nu_pre},{nu_post})", flush=True)
+    torch = torch_gpu
+    q = feat.q1_problem(3, 5)
+    b = feat.rhs_random(23, q["n"], -1.0, 1.0)
+    b[q["filt"].astype(np.int64)] = 0.0
+    bd = torch.from_numpy(b).cuda()
+    out = []
+    for fusion in (True, False):
+        cfg = feat.mgpcg_cfg(nu_pre=nu_pre, nu_post=nu_post, omega_jac=0.5, omega_rich=1.0, coarse="direct",
+                             tol_rel=1e-8, max_iter=200, graphs=False, coarse_fusion=fusion,
+                             precision=precision)
+        s = feat.MgPcg(cfg, q1=(3, 5, 1))
+        assert s.coarse_fusion_level() == (2 if fusion else 0)
+        x = torch.zeros(q["n"], dtype=torch.float64, device="cuda")
+        r = s.solve(x, bd)
+        torch.cuda.synchronize()
+        s.close()
+        assert r["status"] == "success"
+        out.append((r, x.cpu().numpy()))
+    (r1, x1), (r0, x0) = out
+    assert r1["iters"] == r0["iters"]
+    assert np.array_equal(r1["hist"].view(np.uint64), r0["hist"].view(np.uint64))
+    assert np.array_equal(x1.view(np.uint64), x0.view(np.uint64))
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_numeric_reinit_rebuilds_fused_levels(feat, torch_gpu, precision):
+    """New matrix values followed by the reference's re-initialisation order
+    (hierarchy init_numeric, then the solver tree's) replace the coarse inverse,
+    the Jacobi diagonals and the value codes: a solve with the fused coarse levels
+    afterwards equals, bit for bit, fresh solvers built on the new values with and
+    without the fusion (no stale addresses from before the re-init)."""
+    print(f"numeric re-init {precision}", flush=True)
+    torch = torch_gpu
+    levels = q1_levels(feat, 3, 5, 1)
+    new = [lv["val"] * s for lv, s in zip(levels, (1.0, 1.0, 1.25, 1.0, 1.5))]
+    n = levels[0]["n"]
+    b = feat.rhs_random(29, n, -1.0, 1.0)
+    b[levels[0]["filt"].astype(np.int64)] = 0.0
+    bd = torch.from_numpy(b).cuda()
+
+    def cfg(fusion):
+        return feat.mgpcg_cfg(nu_pre=4, nu_post=4, omega_jac=0.5, omega_rich=1.0, coarse="direct",
+                              tol_rel=1e-8, max_iter=100, graphs=True, coarse_fusion=fusion,
+                              precision=precision)
+
+    def run(s):
+        x = torch.zeros(n, dtype=torch.float64, device="cuda")
+        r = s.solve(x, bd)
+        torch.cuda.synchronize()
+        assert r["status"] == "success"
+        return r, x.cpu().numpy()
+
+    s = feat.MgPcg(cfg(True), levels=levels)
+    assert s.coarse_fusion_level() == 2
+    r_old, x_old = run(s)
+    for i in (2, 4):
+        s.set_values(i, new[i])
+    r_re, x_re = run(s)
+    s.close()
+    assert not np.array_equal(x_re, x_old)
+    changed = [dict(lv, val=v) for lv, v in zip(levels, new)]
+    for fusion in (True, False):
+        f = feat.MgPcg(cfg(fusion), levels=changed)
+        r_f, x_f = run(f)
+        f.close()
+        assert r_f["iters"] == r_re["iters"]
+        assert np.array_equal(x_f.view(np.uint64), x_re.view(np.uint64))
