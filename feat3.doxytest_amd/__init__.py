@@ -128,7 +128,8 @@ KERNEL_API = [
     ("feat_unit_filter_rhs_f64_u64", _i, [_vp, _vp, _vp, _vp, _u64]),
     ("feat_unit_filter_def_f64_u64", _i, [_vp, _vp, _vp, _u64]),
     ("feat_unit_filter_mask_u64", _i, [_vp, _vp, _u64, _vp, _u64]),
-    ("feat_unit_filter_blocked_def_f64_u64", _i, [_vp, _i, _vp, _vp, _u64]),
+    ("feat_unit_filter_blocked_rhs_f64_u64", _i, [_vp, _i, _vp, _vp, _vp, _u64, _i]),
+    ("feat_unit_filter_blocked_def_f64_u64", _i, [_vp, _i, _vp, _vp, _vp, _u64, _i]),
     ("feat_mean_filter_f64", _i, [_vp, _vp, _vp, _vp, _d, _u64]),
     ("feat_matrix_from_csr_f64", _i, [C.POINTER(CsrF64), C.POINTER(Matrix)]),
     ("feat_sell_create_f64", _i, [_vp, C.POINTER(CsrF64), C.POINTER(Matrix)]),

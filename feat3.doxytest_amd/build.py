@@ -22,7 +22,7 @@ CXX = os.environ.get("CXX", "g++")
 
 # every HIP translation unit (one object each, linked into one .so)
 HIP_SOURCES = ["feat3amd_kernels.hip", "feat3amd_stokes.hip", "feat3amd_batch.hip", "feat3amd_mlp.hip",
-               "feat3amd_assembly.hip", "feat3amd_smoothers.hip"]
+               "feat3amd_assembly.hip", "feat3amd_smoothers.hip", "feat3amd_transpose.hip"]
 HOST_SOURCES = ["host_capi.cpp"]
 
 HIP_FLAGS = [

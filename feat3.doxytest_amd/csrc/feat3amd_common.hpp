@@ -18,6 +18,10 @@ extern thread_local std::string g_err;
 int fail(int code, const char* what);
 int check_hip(hipError_t e, const char* where);
 int check_launch(const char* where);
+// the transposed CSR apply in the reference's sequential order (feat3amd_transpose.hip)
+template<typename DT, typename IT>
+int csr_apply_transposed(feat_handle_t h, DT* r, DT a, const DT* x, DT b, bool bzero, const DT* y, const DT* val,
+                         const IT* col, const IT* rowptr, uint64_t rows, uint64_t cols, uint64_t nnz, const char* name);
 } // namespace feat3amd_detail
 using feat3amd_detail::check_hip;
 using feat3amd_detail::check_launch;

@@ -1128,10 +1128,19 @@ __global__ void k_filter_def(double* v, const uint64_t* idx, uint64_t ue)
     v[idx[i]] = 0.0;
 }
 
-__global__ void k_filter_blocked_def(double* v, int bs, const uint64_t* idx, uint64_t ue)
+// UnitFilterBlocked: one thread per filtered component (entry i / bs, component i % bs),
+// so the value reads are coalesced; RHS writes the value, DEF a zero; with ign_nans a
+// NaN value leaves its component alone (unit_filter_blocked_generic.hpp:24-60)
+template<bool RHS>
+__global__ void k_filter_blocked(double* v, int bs, const double* sv, const uint64_t* idx, uint64_t ue, int ign_nans)
 {
-  for(uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < ue; i += uint64_t(gridDim.x) * blockDim.x)
-    for(int k = 0; k < bs; ++k) v[idx[i] * bs + k] = 0.0;
+  const uint64_t total = ue * uint64_t(bs);
+  for(uint64_t t = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; t < total; t += uint64_t(gridDim.x) * blockDim.x)
+  {
+    const uint64_t e = t / uint64_t(bs), k = t - e * uint64_t(bs);
+    if(ign_nans && sv[t] != sv[t]) continue;
+    v[idx[e] * uint64_t(bs) + k] = RHS ? sv[t] : 0.0;
+  }
 }
 
 __global__ void k_mask_zero(uint32_t* m, uint64_t nw)
@@ -1152,28 +1161,6 @@ __global__ void k_mean_apply(double* v, const double* w_add, const double* dotv,
   const double alpha = -(*dotv) / vol;
   for(uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
     v[i] = fma(alpha, w_add[i], v[i]);
-}
-
-// transposed SpMV helpers: r = (b/a)*y (or 0); r[col] += val*x[row]; r *= a
-__global__ void k_tr_init(double* r, const double* y, double ba, int bzero, uint64_t n)
-{
-  for(uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
-    r[i] = bzero ? 0.0 : ba * y[i];
-}
-
-template<typename DT, typename IT>
-__global__ void k_tr_scatter(DT* r, const DT* x, const DT* val, const IT* col, const IT* rowptr, uint64_t rows)
-{
-  for(uint64_t row = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; row < rows; row += uint64_t(gridDim.x) * blockDim.x)
-    for(IT i = rowptr[row]; i < rowptr[row + 1]; ++i)
-      atomicAdd(&r[col[i]], val[i] * x[row]);
-}
-
-template<typename DT>
-__global__ void k_tr_scale(DT* r, DT a, uint64_t n)
-{
-  for(uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
-    r[i] = a * r[i];
 }
 
 // BCSR: one lane per block row, sequential over blocks, h, w (arch/apply_generic.hpp:121-154)
@@ -1506,19 +1493,8 @@ int csr_apply_impl(feat_handle_t h, DT* r, DT a, const DT* x, DT b, const DT* y,
     GridReduce g{nullptr, nullptr, nullptr, nullptr};
     return launch_rows<DT, IT>(h->stream, val, col, rowptr, x, rows, e, g, name);
   }
-  if(!std::is_same<DT, double>::value)
-    return fail(FEAT_ERR_UNSUPPORTED, "csr_apply: transposed apply is FP64 only");
   // r = (b/a)*y ; r[col] += val*x[row] ; r = a*r    (arch/apply_generic.hpp:37-53)
-  double* rd = reinterpret_cast<double*>(r);
-  if(nr)
-    hipLaunchKernelGGL(k_tr_init, dim3(grid_for(nr, 4)), dim3(kThreads), 0, h->stream, rd,
-                       reinterpret_cast<const double*>(y), double(b) / double(a), int(bzero), nr);
-  if(rows)
-    hipLaunchKernelGGL((k_tr_scatter<DT, IT>), dim3(grid_for(rows)), dim3(kThreads), 0, h->stream,
-                       r, x, val, col, rowptr, rows);
-  if(nr)
-    hipLaunchKernelGGL((k_tr_scale<DT>), dim3(grid_for(nr, 4)), dim3(kThreads), 0, h->stream, r, a, nr);
-  return check_launch(name);
+  return feat3amd_detail::csr_apply_transposed<DT, IT>(h, r, a, x, b, bzero, y, val, col, rowptr, rows, cols, nnz, name);
 }
 } // namespace
 
@@ -1749,11 +1725,25 @@ int feat_unit_filter_mask_u64(feat_handle_t h, uint32_t* mask, uint64_t n, const
   return check_launch("feat_unit_filter_mask_u64");
 }
 
-int feat_unit_filter_blocked_def_f64_u64(feat_handle_t h, int bs, double* v, const uint64_t* sv_idx, uint64_t ue)
+int feat_unit_filter_blocked_rhs_f64_u64(feat_handle_t h, int bs, double* v, const double* sv_elements,
+                                         const uint64_t* sv_indices, uint64_t ue, int ign_nans)
 {
-  if(!h || bs <= 0 || (ue && (!v || !sv_idx))) return fail(FEAT_ERR_INVALID, "feat_unit_filter_blocked_def: bad argument");
+  if(!h || bs <= 0 || (ue && (!v || !sv_elements || !sv_indices)))
+    return fail(FEAT_ERR_INVALID, "feat_unit_filter_blocked_rhs: bad argument");
   if(!ue) return FEAT_OK;
-  hipLaunchKernelGGL(k_filter_blocked_def, dim3(grid_for(ue)), dim3(kThreads), 0, h->stream, v, bs, sv_idx, ue);
+  hipLaunchKernelGGL(k_filter_blocked<true>, dim3(grid_for(ue * uint64_t(bs))), dim3(kThreads), 0, h->stream, v, bs,
+                     sv_elements, sv_indices, ue, ign_nans);
+  return check_launch("feat_unit_filter_blocked_rhs_f64_u64");
+}
+
+int feat_unit_filter_blocked_def_f64_u64(feat_handle_t h, int bs, double* v, const double* sv_elements,
+                                         const uint64_t* sv_indices, uint64_t ue, int ign_nans)
+{
+  if(!h || bs <= 0 || (ue && (!v || !sv_indices || (ign_nans && !sv_elements))))
+    return fail(FEAT_ERR_INVALID, "feat_unit_filter_blocked_def: bad argument");
+  if(!ue) return FEAT_OK;
+  hipLaunchKernelGGL(k_filter_blocked<false>, dim3(grid_for(ue * uint64_t(bs))), dim3(kThreads), 0, h->stream, v, bs,
+                     sv_elements, sv_indices, ue, ign_nans);
   return check_launch("feat_unit_filter_blocked_def_f64_u64");
 }
 

@@ -89,8 +89,11 @@ int feat_convert_u64_u32(feat_handle_t h, uint32_t* dst, const uint64_t* src, ui
  * r = a*A*x + b*y, or r = a*A^T*x + b*y when transposed != 0.  |b| < eps
  * means y is not read (r is treated as 0), as in the generic code.  The
  * non-transposed form sums each row in storage order with one fma per entry
- * (bit-identical to the sequential CPU loop); the transposed form scatters
- * with atomics (rounding order not fixed). */
+ * (bit-identical to the sequential CPU loop); the transposed form computes
+ * (b/a)*y, adds each column's entries in ascending entry order (the order of the
+ * generic row-by-row scatter) and scales by a: bit-identical too.  The
+ * transposed form allocates stream-ordered sort buffers (about 20 bytes per
+ * entry) in each call and needs nnz < 2^31; it is not for recorded graphs. */
 int feat_csr_apply_f64_u64(feat_handle_t h, double* r, double a, const double* x, double b,
                            const double* y, const double* val, const uint64_t* col,
                            const uint64_t* rowptr, uint64_t rows, uint64_t cols, uint64_t nnz,
@@ -167,10 +170,16 @@ int feat_unit_filter_def_f64_u64(feat_handle_t h, double* v, const uint64_t* sv_
  * apply filter_def/filter_cor in their epilogues. */
 int feat_unit_filter_mask_u64(feat_handle_t h, uint32_t* mask, uint64_t n,
                               const uint64_t* sv_idx, uint64_t ue);
-/* UnitFilterBlocked::filter_def (lafem/unit_filter_blocked.hpp:449-495): zero
- * whole blocks of size bs at the given block indices */
-int feat_unit_filter_blocked_def_f64_u64(feat_handle_t h, int bs, double* v,
-                                         const uint64_t* sv_idx, uint64_t ue);
+/* UnitFilterBlocked::filter_rhs / filter_def (lafem/unit_filter_blocked.hpp:449-495) ->
+ * Arch::UnitFilterBlocked::filter_rhs<bs> / filter_def<bs>(v, sv_elements, sv_indices, ue,
+ * ign_nans) (arch/unit_filter_blocked_generic.hpp:24-60): for entry i and component j,
+ * v[bs*sv_indices[i] + j] = sv_elements[bs*i + j] (rhs) or 0 (def); with ign_nans a NaN
+ * in sv_elements[bs*i + j] leaves that component unchanged.  filter_def reads
+ * sv_elements only when ign_nans is set (may be NULL otherwise). */
+int feat_unit_filter_blocked_rhs_f64_u64(feat_handle_t h, int bs, double* v, const double* sv_elements,
+                                         const uint64_t* sv_indices, uint64_t ue, int ign_nans);
+int feat_unit_filter_blocked_def_f64_u64(feat_handle_t h, int bs, double* v, const double* sv_elements,
+                                         const uint64_t* sv_indices, uint64_t ue, int ign_nans);
 /* MeanFilter::filter_def / filter_cor (lafem/mean_filter.hpp:214-264):
  * v += (-(v . w_dot) / volume) * w_add, all on device */
 int feat_mean_filter_f64(feat_handle_t h, double* v, const double* w_dot, const double* w_add,

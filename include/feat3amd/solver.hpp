@@ -332,6 +332,14 @@ namespace feat3amd
       {
         const Index n = _matrix.rows();
         if(n > 4096) throw Error("DirectCoarseSolver: coarse matrix too large for a dense inverse");
+        // the values as they are now (the pattern is fixed since init_symbolic)
+        _host_val.resize(_matrix.used_elements());
+        if(!_host_val.empty())
+        {
+          check(feat_copy(Runtime::handle(), _host_val.data(), _matrix.val(), sizeof(double) * _host_val.size()),
+                "DirectCoarseSolver: values");
+          check(feat_sync(Runtime::handle()), "DirectCoarseSolver: sync");
+        }
         std::vector<char> f(n, 0);
         for(auto i : _host_filt) f[i] = 1;
         std::vector<double> a(n * n, 0.0);

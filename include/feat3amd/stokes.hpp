@@ -106,8 +106,8 @@ namespace feat3amd
     class StokesFilter
     {
       Index _nv = 0, _np = 0, _nf = 0;
-      DeviceArray<std::uint64_t> _nodes, _sidx; // blocked node indices; scalar indices 2i, 2i+1
-      DeviceArray<double> _sval;                // values for the scalar indices
+      DeviceArray<std::uint64_t> _nodes;        // blocked node indices
+      DeviceArray<double> _sval;                // two values per filtered node
       DeviceArray<std::uint32_t> _vmask;        // bit per velocity node
       DeviceArray<double> _prim, _dual;
       double _vol = 0.0;
@@ -120,14 +120,7 @@ namespace feat3amd
         f._nv = S.nv;
         f._np = S.np;
         f._nf = S.filt.size();
-        std::vector<std::uint64_t> sidx(2 * f._nf);
-        for(Index i = 0; i < f._nf; ++i)
-        {
-          sidx[2 * i] = 2 * S.filt[i];
-          sidx[2 * i + 1] = 2 * S.filt[i] + 1;
-        }
         f._nodes = DeviceArray<std::uint64_t>::from_host(S.filt.data(), f._nf);
-        f._sidx = DeviceArray<std::uint64_t>::from_host(sidx.data(), sidx.size());
         f._sval = DeviceArray<double>::from_host(S.filt_val.data(), S.filt_val.size());
         f._vmask = DeviceArray<std::uint32_t>((S.nv + 31) / 32);
         check(feat_unit_filter_mask_u64(Runtime::handle(), f._vmask.get(), S.nv, f._nodes.get(), f._nf), "StokesFilter mask");
@@ -152,11 +145,15 @@ namespace feat3amd
 
       void unit_rhs(StokesVector& v) const
       {
-        if(_nf) check(feat_unit_filter_rhs_f64_u64(Runtime::handle(), v.elements(), _sval.get(), _sidx.get(), 2 * _nf), "filter_rhs");
+        if(_nf)
+          check(feat_unit_filter_blocked_rhs_f64_u64(Runtime::handle(), 2, v.elements(), _sval.get(), _nodes.get(), _nf, 0),
+                "filter_rhs");
       }
       void unit_def(StokesVector& v) const
       {
-        if(_nf) check(feat_unit_filter_blocked_def_f64_u64(Runtime::handle(), 2, v.elements(), _nodes.get(), _nf), "filter_def");
+        if(_nf)
+          check(feat_unit_filter_blocked_def_f64_u64(Runtime::handle(), 2, v.elements(), nullptr, _nodes.get(), _nf, 0),
+                "filter_def");
       }
       /// mean part: def/rhs subtract dual-weighted, cor/sol prim-weighted (passes: repeated calls)
       void mean_def(StokesVector& v) const

@@ -236,6 +236,20 @@ void orc_unit_filter_def_f64(double* v, const uint64_t* sv_idx, uint64_t ue)
     v[sv_idx[i]] = 0.0;
 }
 
+/* arch/unit_filter_blocked_generic.hpp:24-60: blocks of bs components; with ign_nans a
+ * NaN filter value leaves its component unchanged (rhs and def alike) */
+void orc_unit_filter_blocked_f64(int rhs, int bs, double* v, const double* sv_el, const uint64_t* sv_idx,
+                                 uint64_t ue, int ign_nans)
+{
+  for(uint64_t i = 0; i < ue; ++i)
+    for(int j = 0; j < bs; ++j)
+    {
+      const double e = sv_el ? sv_el[(uint64_t)bs * i + (uint64_t)j] : 0.0;
+      if(ign_nans && isnan(e)) continue;
+      v[(uint64_t)bs * sv_idx[i] + (uint64_t)j] = rhs ? e : 0.0;
+    }
+}
+
 /* lafem/mean_filter.hpp:214-264.  filter_def/rhs: v += (-(v.prim)/vol) * dual;
  * filter_cor: v += (-(v.dual)/vol) * prim.  The caller passes the vector that
  * is dotted first and the one that is added second. */

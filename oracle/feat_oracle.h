@@ -56,6 +56,9 @@ void   orc_extract_diag_f64(double* diag, const double* val, const uint64_t* col
                             const uint64_t* rowptr, uint64_t rows);
 void   orc_unit_filter_rhs_f64(double* v, const double* sv_val, const uint64_t* sv_idx, uint64_t ue);
 void   orc_unit_filter_def_f64(double* v, const uint64_t* sv_idx, uint64_t ue);
+/* UnitFilterBlocked filter_rhs (rhs = 1) / filter_def (rhs = 0), block size bs */
+void   orc_unit_filter_blocked_f64(int rhs, int bs, double* v, const double* sv_el, const uint64_t* sv_idx,
+                                   uint64_t ue, int ign_nans);
 void   orc_mean_filter_f64(double* v, const double* vec_prim, const double* vec_dual,
                            double volume, uint64_t n);
 

@@ -207,6 +207,7 @@ class Oracle:
         L.orc_extract_diag_f64.argtypes = [_f64p, _f64p, _u64p, _u64p, C.c_uint64]
         L.orc_unit_filter_rhs_f64.argtypes = [_f64p, _f64p, _u64p, C.c_uint64]
         L.orc_unit_filter_def_f64.argtypes = [_f64p, _u64p, C.c_uint64]
+        L.orc_unit_filter_blocked_f64.argtypes = [C.c_int, C.c_int, _f64p, _vp, _u64p, C.c_uint64, C.c_int]
         L.orc_mean_filter_f64.argtypes = [_f64p, _f64p, _f64p, C.c_double, C.c_uint64]
         L.orc_invert_matrix_f64.restype = C.c_double
         L.orc_invert_matrix_f64.argtypes = [C.c_int, C.c_int, _f64p, _i32p]
@@ -513,6 +514,15 @@ class Oracle:
         self.L.orc_extract_diag_f64(d, A.val, A.col, A.rowptr, A.rows)
         return d
 
+    def unit_filter_blocked(self, rhs, bs, v, vals, idx, ign_nans=False):
+        """UnitFilterBlocked filter_rhs (rhs=True) / filter_def of block size bs."""
+        v = np.array(v, np.float64)
+        idx = np.ascontiguousarray(idx, np.uint64)
+        vals = None if vals is None else np.ascontiguousarray(vals, np.float64)
+        self.L.orc_unit_filter_blocked_f64(int(rhs), bs, v, None if vals is None else vals.ctypes.data, idx,
+                                           idx.size, int(ign_nans))
+        return v
+
     def unit_filter_def(self, v, idx):
         v = v.copy()
         idx = np.ascontiguousarray(idx, np.uint64)
@@ -634,6 +644,7 @@ class Reference:
         L.ref_csr_apply_f32.argtypes = [_f32p, C.c_float, _f32p, C.c_float, _f32p, _f32p,
                                         _u64p, _u64p, C.c_uint64, C.c_uint64, C.c_uint64, C.c_int]
         L.ref_component_invert_f32.argtypes = [_f32p, _f32p, C.c_float, C.c_uint64]
+        L.ref_unit_filter_blocked_f64.argtypes = [C.c_int, C.c_int, _f64p, _f64p, _u64p, C.c_uint64, C.c_int]
         L.ref_single_solve.restype = C.c_int
         L.ref_single_solve.argtypes = [C.c_int, C.POINTER(_CsrS), _vp, C.c_uint64, C.c_int,
                                        C.c_double, C.c_double, C.c_double, C.c_uint64, C.c_int,
@@ -786,6 +797,13 @@ class Reference:
                                  np.ascontiguousarray(A.val, np.float32), A.col, A.rowptr,
                                  A.rows, A.cols, A.nnz, int(transposed))
         return r
+
+    def unit_filter_blocked(self, rhs, bs, v, vals, idx, ign_nans=False):
+        v = np.array(v, np.float64)
+        idx = np.ascontiguousarray(idx, np.uint64)
+        self.L.ref_unit_filter_blocked_f64(int(rhs), bs, v, np.ascontiguousarray(vals, np.float64), idx, idx.size,
+                                           int(ign_nans))
+        return v
 
     def component_invert_f32(self, x, s=1.0):
         x = np.ascontiguousarray(x, np.float32)

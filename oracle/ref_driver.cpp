@@ -25,6 +25,7 @@
 #include <kernel/lafem/arch/component_product.hpp>
 #include <kernel/lafem/arch/component_invert.hpp>
 #include <kernel/lafem/arch/diagonal.hpp>
+#include <kernel/lafem/arch/unit_filter_blocked.hpp>
 #include <kernel/solver/pcg.hpp>
 #include <kernel/solver/richardson.hpp>
 #include <kernel/solver/jacobi_precond.hpp>
@@ -208,6 +209,31 @@ void ref_component_product_f64(double* r, const double* x, const double* y, std:
 void ref_component_invert_f64(double* r, const double* x, double s, std::uint64_t n)
 {
   LAFEM::Arch::ComponentInvert::value_generic(r, x, s, Index(n));
+}
+
+// Arch::UnitFilterBlocked::filter_rhs / filter_def (arch/unit_filter_blocked_generic.hpp:24-60)
+void ref_unit_filter_blocked_f64(int rhs, int bs, double* v, const double* sv_el, const std::uint64_t* sv_idx,
+                                 std::uint64_t ue, int ign_nans)
+{
+  const IT* ix = (const IT*)sv_idx;
+  const bool ig = ign_nans != 0;
+  switch(bs)
+  {
+  case 1:
+    if(rhs) LAFEM::Arch::UnitFilterBlocked::filter_rhs_generic<1>(v, sv_el, ix, Index(ue), ig);
+    else LAFEM::Arch::UnitFilterBlocked::filter_def_generic<1>(v, sv_el, ix, Index(ue), ig);
+    break;
+  case 2:
+    if(rhs) LAFEM::Arch::UnitFilterBlocked::filter_rhs_generic<2>(v, sv_el, ix, Index(ue), ig);
+    else LAFEM::Arch::UnitFilterBlocked::filter_def_generic<2>(v, sv_el, ix, Index(ue), ig);
+    break;
+  case 3:
+    if(rhs) LAFEM::Arch::UnitFilterBlocked::filter_rhs_generic<3>(v, sv_el, ix, Index(ue), ig);
+    else LAFEM::Arch::UnitFilterBlocked::filter_def_generic<3>(v, sv_el, ix, Index(ue), ig);
+    break;
+  default:
+    break;
+  }
 }
 
 void ref_diag_index(std::uint64_t* diag, const std::uint64_t* col, const std::uint64_t* rowptr,

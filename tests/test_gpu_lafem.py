@@ -1,8 +1,8 @@
 """GPU parity of the LAFEM kernels against the oracle, through the C-ABI.
 
 Bar: bit-exact where the per-element arithmetic is the oracle's (SpMV rows,
-axpy, scale, component ops, filters, diag); dot/norm within 1e-13 relative
-(tree order vs sequential); transposed SpMV within 1e-12 (atomic scatter).
+transposed SpMV, axpy, scale, component ops, filters, diag); dot/norm within
+1e-13 relative (tree order vs sequential).
 """
 import ctypes as C
 
@@ -83,14 +83,47 @@ def test_csr_apply_f32_bitwise(feat, handle, torch_gpu, oracle):
         assert np.array_equal(got, ref)
 
 
-def test_csr_apply_transposed(feat, handle, torch_gpu, oracle):
-    rng = np.random.default_rng(12)
-    A = random_csr(rng, 800, 600, 0, 20)
-    x = rng.standard_normal(800)
-    y = rng.standard_normal(600)
-    ref = oracle.csr_apply(A, x, y, 4711.1, 1.0, transposed=True)
-    got = csr_apply(feat, handle, torch_gpu, A, x, y, 4711.1, 1.0, tr=True)
-    assert np.allclose(got, ref, rtol=1e-12, atol=1e-9 * np.abs(ref).max())
+@pytest.mark.parametrize("seed,rows,cols,maxlen,longrows", [
+    (12, 800, 600, 20, ()),
+    (13, 3000, 2000, 40, ((7, 1900),)),      # one long row: its entries spread over most columns
+    (14, 500, 5000, 6, ()),                   # many empty columns
+    (15, 100, 1, 1, ()),                      # one column
+])
+def test_csr_apply_transposed(feat, handle, torch_gpu, oracle, seed, rows, cols, maxlen, longrows):
+    """The transposed apply adds each column's entries in the generic scatter's order:
+    bit-identical to the oracle (pinned to the reference's generic code), for every
+    (a, b), both index widths, FP32, r == y and |b| < eps (y not read)."""
+    rng = np.random.default_rng(seed)
+    A = random_csr(rng, rows, cols, 0, maxlen, longrows)
+    x = rng.standard_normal(rows)
+    y = rng.standard_normal(cols)
+    for a, b in [(4711.1, 1.0), (1.0, 0.0), (-1.0, 1.0), (0.3, -2.5)]:
+        ref = oracle.csr_apply(A, x, y, a, b, transposed=True)
+        for idx in ("u64", "u32"):
+            got = csr_apply(feat, handle, torch_gpu, A, x, y, a, b, tr=True, idx=idx)
+            assert np.array_equal(got, ref), (a, b, idx)
+    got = csr_apply(feat, handle, torch_gpu, A, x, y, -1.0, 1.0, tr=True, alias_ry=True)
+    assert np.array_equal(got, oracle.csr_apply(A, x, y, -1.0, 1.0, transposed=True))
+    got = csr_apply(feat, handle, torch_gpu, A, x, np.full(cols, np.nan), 2.0, 0.0, tr=True)
+    assert np.array_equal(got, oracle.csr_apply(A, x, None, 2.0, 0.0, transposed=True))
+    x32, y32 = x.astype(np.float32), y.astype(np.float32)
+    ref32 = oracle.csr_apply_f32(A, x32, y32, -1.5, 1.0, transposed=True)
+    got32 = csr_apply(feat, handle, torch_gpu, A, x32, y32, -1.5, 1.0, tr=True, idx="u32", fp="f32")
+    assert np.array_equal(got32.view(np.uint32), ref32.view(np.uint32))
+
+
+def test_csr_apply_transposed_equals_explicit_restriction(feat, handle, torch_gpu):
+    # P^T x through the transposed apply equals R x with the explicit R = P^T (whose
+    # rows list P's column entries in ascending row order): the same sums, bit for bit
+    from oracle.pyoracle import Csr, csr_transpose
+    q = feat.q1_problem(3, 4)
+    P = Csr(q["n"], q["pcols"], q["pval"], q["pcol"], q["prowptr"])
+    R = csr_transpose(P)
+    x = np.random.default_rng(16).standard_normal(P.rows)
+    z = np.zeros(P.cols)
+    got = csr_apply(feat, handle, torch_gpu, P, x, z, 1.0, 0.0, tr=True)
+    ref = csr_apply(feat, handle, torch_gpu, R, x, z, 1.0, 0.0)
+    assert np.array_equal(got, ref)
 
 
 def test_csr_apply_kat_golden(feat, handle, torch_gpu, golden):
@@ -186,6 +219,32 @@ def test_blas1(feat, handle, torch_gpu, oracle):
         feat.check(k.feat_dot_f64_host(handle.h, C.byref(d), P(xd), P(yd), n), "dot")
         vals.add(d.value)
     assert len(vals) == 1
+
+
+@pytest.mark.parametrize("bs", [1, 2, 3])
+def test_unit_filter_blocked(feat, handle, torch_gpu, oracle, bs):
+    """UnitFilterBlocked filter_rhs / filter_def through the C-ABI, with and without
+    ign_nans (NaNs planted in the filter values), bit for bit against the oracle."""
+    k, _ = feat.libs()
+    P = feat.ptr
+    rng = np.random.default_rng(60 + bs)
+    nb = 5000
+    idx = np.unique(rng.integers(0, nb, 900)).astype(np.uint64)
+    vals = rng.standard_normal(idx.size * bs)
+    vals[::7] = np.nan
+    x = rng.standard_normal(nb * bs)
+    idd = dev(torch_gpu, idx.astype(np.int64))
+    svd = dev(torch_gpu, vals)
+    for ign in (0, 1):
+        xd = dev(torch_gpu, x)
+        feat.check(k.feat_unit_filter_blocked_rhs_f64_u64(handle.h, bs, P(xd), P(svd), P(idd), idx.size, ign), "rhs")
+        handle.sync()
+        assert np.array_equal(host(xd), oracle.unit_filter_blocked(True, bs, x, vals, idx, ign), equal_nan=True)
+        xd = dev(torch_gpu, x)
+        feat.check(k.feat_unit_filter_blocked_def_f64_u64(handle.h, bs, P(xd), P(svd) if ign else None, P(idd),
+                                                          idx.size, ign), "def")
+        handle.sync()
+        assert np.array_equal(host(xd), oracle.unit_filter_blocked(False, bs, x, vals, idx, ign))
 
 
 def test_diag_filters_convert(feat, handle, torch_gpu, oracle):

@@ -346,7 +346,7 @@ def test_sell_value_codes_boundary(feat, handle, torch_gpu, oracle, nvals, speci
     k, _ = feat.libs()
     P = feat.ptr
     rng = np.random.default_rng(nvals + 7 * specials)
-    table = (np.arange(nvals) - 100) / 8.0
+    table = (np.arange(nvals) - 1000) / 8.0     # no zero among them
     if specials:
         table[:4] = [0.0, -0.0, np.inf, -np.inf]
     rows, cols, per = 700, 900, 5

@@ -113,3 +113,141 @@ def test_oracle_bf16_rounding_is_small(oracle):
     g1 = oracle.mlp_grad(p, Xb, t, round_bf16=True)
     rel = np.linalg.norm(g1[:NP] - g0[:NP]) / np.linalg.norm(g0[:NP])
     assert 0 < rel < 2e-2
+
+
+# ---------------------------------------------------------------- features and smoothing step
+# Pinned by invariants and closed forms rather than a restatement of the same code:
+# a single-entry residual patch fixes the slot order, scaling r by a power of two
+# leaves the inputs unchanged (and scales the target inversely), reflecting the
+# grid permutes the slots, a zero output layer reduces the step to damped Jacobi,
+# and the step's network term is the pinned forward pass on the features.
+
+def _grid_fields(m, seed):
+    rng = np.random.default_rng(seed)
+    n = m ** 3
+    r = rng.standard_normal(n)
+    e = rng.standard_normal(n)
+    dinv = rng.uniform(0.5, 2.0, n)
+    cfeat = rng.uniform(-1.0, 1.0, n).astype(np.float32)
+    return r, e, dinv, cfeat
+
+
+def _slot(dx, dy, dz):
+    return (dz + 1) * 9 + (dy + 1) * 3 + (dx + 1)
+
+
+def _node(m, x, y, z):
+    return x + m * (y + m * z)
+
+
+def test_features_single_entry_patch(oracle):
+    m = 6
+    n = m ** 3
+    i = _node(m, 2, 3, 2)
+    dinv = np.full(n, 0.25)
+    cfeat = np.arange(n, dtype=np.float32) / 256.0   # exact in bfloat16 below 2
+    e = np.zeros(n)
+    e[i] = 3.0
+    for dx, dy, dz in [(-1, 0, 0), (1, 1, -1), (0, -1, 1), (0, 0, 0)]:
+        r = np.zeros(n)
+        r[_node(m, 2 + dx, 3 + dy, 2 + dz)] = -8.0
+        X, t = oracle.mlp_features(m, r, e, dinv, cfeat, [i])
+        x = from_bf16(X[0])
+        want = np.zeros(64, np.float32)
+        want[_slot(dx, dy, dz)] = -1.0          # rho / max|rho| with one entry
+        for ddz in (-1, 0, 1):
+            for ddy in (-1, 0, 1):
+                for ddx in (-1, 0, 1):
+                    want[27 + _slot(ddx, ddy, ddz)] = cfeat[_node(m, 2 + ddx, 3 + ddy, 2 + ddz)]
+        assert np.array_equal(x, want)
+        assert t[0] == np.float32(3.0 / (8.0 * 0.25))   # e_i / s
+
+
+def test_features_boundary_and_zero_patch(oracle):
+    m = 5
+    r, e, dinv, cfeat = _grid_fields(m, 4)
+    nodes = [_node(m, 0, 2, 2), _node(m, 2, 4, 2), _node(m, 2, 2, 0)]
+    X, t = oracle.mlp_features(m, r, e, dinv, cfeat, nodes)
+    assert not X.any() and not t.any()
+    X, t = oracle.mlp_features(m, np.zeros(m ** 3), e, dinv, cfeat, [_node(m, 2, 2, 2)])
+    assert not X.any() and not t.any()
+
+
+def test_features_scale_invariance(oracle):
+    m = 7
+    r, e, dinv, cfeat = _grid_fields(m, 5)
+    nodes = np.arange(m ** 3, dtype=np.uint64)
+    X0, t0 = oracle.mlp_features(m, r, e, dinv, cfeat, nodes)
+    for c in (4.0, 0.125):
+        X1, t1 = oracle.mlp_features(m, c * r, e, dinv, cfeat, nodes)
+        assert np.array_equal(X1, X0)
+        assert np.array_equal(t1, (t0 / np.float32(c)).astype(np.float32))
+    # a sign flip negates the residual inputs only (the scale max|rho| and the
+    # target e / s keep their values)
+    X2, t2 = oracle.mlp_features(m, -r, e, dinv, cfeat, nodes)
+    assert np.array_equal(X2[:, :27], X0[:, :27] ^ np.where(X0[:, :27] != 0, 0x8000, 0).astype(np.uint16))
+    assert np.array_equal(X2[:, 27:], X0[:, 27:])
+    assert np.array_equal(t2, t0)
+
+
+def test_features_reflection_permutes_slots(oracle):
+    m = 6
+    r, e, dinv, cfeat = _grid_fields(m, 6)
+    g = lambda a: a.reshape(m, m, m)   # [z, y, x]
+    fx = lambda a: g(a)[:, :, ::-1].ravel().copy()
+    fz = lambda a: g(a)[::-1, :, :].ravel().copy()
+    nodes = np.arange(m ** 3, dtype=np.uint64)
+    X0, t0 = oracle.mlp_features(m, r, e, dinv, cfeat, nodes)
+    for flip, perm in [(fx, lambda dx, dy, dz: (-dx, dy, dz)), (fz, lambda dx, dy, dz: (dx, dy, -dz))]:
+        X1, t1 = oracle.mlp_features(m, flip(r), flip(e), flip(dinv), flip(cfeat), nodes)
+        # node i of the flipped grid is node flip(i) of the original
+        src = flip(np.arange(m ** 3))
+        order = np.zeros(54, np.int64)
+        for dz in (-1, 0, 1):
+            for dy in (-1, 0, 1):
+                for dx in (-1, 0, 1):
+                    q = _slot(*perm(dx, dy, dz))
+                    order[_slot(dx, dy, dz)] = q
+                    order[27 + _slot(dx, dy, dz)] = 27 + q
+        assert np.array_equal(X1[:, :54], X0[src][:, order])
+        assert np.array_equal(t1, t0[src])
+
+
+def test_smooth_zero_output_layer_is_damped_jacobi(oracle):
+    m = 6
+    r, _, dinv, cfeat = _grid_fields(m, 7)
+    p, _, _ = random_problem(1, 8)
+    p[OW3:OB3] = 0.0
+    p[OB3] = 0.0
+    x0 = np.random.default_rng(9).standard_normal(m ** 3)
+    omega, jac = 0.8, 0.6
+    x1 = oracle.mlp_smooth(m, p, r, dinv, cfeat, omega, jac, x0)
+    zi = np.arange(m ** 3)
+    cx, cy, cz = zi % m, (zi // m) % m, zi // (m * m)
+    inner = (cx > 0) & (cy > 0) & (cz > 0) & (cx < m - 1) & (cy < m - 1) & (cz < m - 1)
+    patch_nonzero = inner.copy()
+    want = np.where(patch_nonzero, x0 + omega * (jac * (r * dinv)), x0)
+    assert np.array_equal(x1[~inner], x0[~inner])
+    assert np.allclose(x1[inner], want[inner], rtol=4e-16, atol=4e-16)
+
+
+def test_smooth_network_term_is_forward_on_features(oracle):
+    m = 6
+    r, _, dinv, cfeat = _grid_fields(m, 10)
+    p, _, _ = random_problem(1, 11)
+    x0 = np.zeros(m ** 3)
+    omega, jac = 1.0, 0.0
+    x1 = oracle.mlp_smooth(m, p, r, dinv, cfeat, omega, jac, x0)
+    nodes = np.arange(m ** 3, dtype=np.uint64)
+    X, _ = oracle.mlp_features(m, r, np.zeros(m ** 3), dinv, cfeat, nodes)
+    y = oracle.mlp_forward(p, X, round_bf16=True).astype(np.float64)
+    zi = np.arange(m ** 3)
+    cx, cy, cz = zi % m, (zi // m) % m, zi // (m * m)
+    inner = (cx > 0) & (cy > 0) & (cz > 0) & (cx < m - 1) & (cy < m - 1) & (cz < m - 1)
+    s = np.zeros(m ** 3)
+    for i in np.nonzero(inner)[0]:
+        x, y_, z = cx[i], cy[i], cz[i]
+        patch = [(_node(m, x + dx, y_ + dy, z + dz)) for dz in (-1, 0, 1) for dy in (-1, 0, 1) for dx in (-1, 0, 1)]
+        s[i] = np.max(np.abs(r[patch] * dinv[i]))
+    assert np.array_equal(x1[~inner], x0[~inner])
+    assert np.allclose(x1[inner], s[inner] * y[inner], rtol=1e-15, atol=0)

@@ -268,3 +268,29 @@ def test_mg_pcg_bitwise_vs_reference(oracle, reference, feat):
         # differently depending on where the compiler inlines it, so the
         # defect norms agree to the last bit or two
         assert np.allclose(o["hist"], r["hist"], rtol=4 * EPS, atol=0)
+
+
+@pytest.mark.parametrize("bs", [1, 2, 3])
+def test_unit_filter_blocked_vs_reference(oracle, reference, bs):
+    # unit_filter_blocked_generic.hpp:24-60 with and without ign_nans; NaNs planted
+    # in the filter values, so both branches differ
+    rng = np.random.default_rng(40 + bs)
+    nb, ue = 50, 17
+    idx = np.sort(rng.choice(nb, ue, replace=False)).astype(np.uint64)
+    vals = rng.standard_normal(ue * bs)
+    vals[::4] = np.nan
+    v = rng.standard_normal(nb * bs)
+    for rhs in (True, False):
+        for ign in (False, True):
+            o = oracle.unit_filter_blocked(rhs, bs, v, vals, idx, ign)
+            r = reference.unit_filter_blocked(rhs, bs, v, vals, idx, ign)
+            assert np.array_equal(o, r, equal_nan=True)
+            # definition: only the filtered components change
+            touched = np.zeros(nb * bs, bool)
+            for i, b in enumerate(idx):
+                for j in range(bs):
+                    if not (ign and np.isnan(vals[i * bs + j])):
+                        touched[int(b) * bs + j] = True
+                        exp = vals[i * bs + j] if rhs else 0.0
+                        assert o[int(b) * bs + j] == exp or (np.isnan(exp) and np.isnan(o[int(b) * bs + j]))
+            assert np.array_equal(o[~touched], v[~touched])
