@@ -242,6 +242,7 @@ HOST_API = [
     ("feat_batch_levels", _i, [_u64, _i, _i, _i, _vp]),
     ("feat_lognormal_cells", _i, [_u64, _u64, _d, _vp]),
     ("feat_q1_diffusion_assemble", _i, [_i, _i, _vp, _vp, _vp, _vp]),
+    ("feat_q1_diffusion3d_assemble", _i, [_i, _i, _vp, _d, _vp, _vp, _vp]),
     ("feat_learned_create", _i, [C.POINTER(_vp), _vp]),
     ("feat_learned_generate", _i, [_vp, _i, C.POINTER(_d)]),
     ("feat_learned_train", _i, [_vp, _i, _u64, _vp]),
@@ -796,6 +797,29 @@ def q1_diffusion(level, level_c, coef):
     check(h.feat_q1_diffusion_assemble(level, level_c, coef.ctypes.data, val.ctypes.data, None, None),
           "feat_q1_diffusion_assemble")
     return val
+
+
+def q1_diffusion3d(level, level_c, coef, kz):
+    """3D Q1 diffusion matrix values of diag(1, 1, kz) on `level` (pattern of q1_problem(3, level))."""
+    _, h = libs()
+    q = q1_problem(3, level)
+    val = np.zeros(q["val"].size)
+    coef = np.ascontiguousarray(coef, dtype=np.float64)
+    check(h.feat_q1_diffusion3d_assemble(level, level_c, coef.ctypes.data, kz, val.ctypes.data, None, None),
+          "feat_q1_diffusion3d_assemble")
+    return val
+
+
+def diffusion3d_levels(lf, lc, seed, sigma, kz):
+    """Host levels (finest first, as for MgPcg(levels=...)) of the 3D Q1 anisotropic diffusion
+    hierarchy: a log-normal(sigma) coefficient per finest cell (seed), tensor diag(1, 1, kz)."""
+    coef = lognormal_cells(seed, 1 << (3 * lf), sigma)
+    levels = []
+    for k in range(lf, lc - 1, -1):
+        q = dict(q1_problem(3, k))
+        q["val"] = q1_diffusion3d(k, lf, coef, kz)
+        levels.append(q)
+    return levels
 
 
 def batch_rhs(bs, seed=28054777172512):

@@ -866,6 +866,20 @@ int feat_q1_diffusion_assemble(int level, int level_c, const double* coef, doubl
   });
 }
 
+int feat_q1_diffusion3d_assemble(int level, int level_c, const double* coef, double kz, double* val,
+                                 std::uint64_t* col, std::uint64_t* rowptr)
+{
+  return guarded([&]() {
+    if(level < 1 || level_c < level || level_c > 8 || !coef || !val || !(kz > 0.0))
+      throw Error("feat_q1_diffusion3d_assemble: bad argument");
+    std::vector<double> c(coef, coef + (std::size_t(1) << (3 * level_c)));
+    const auto A = Assembly::q1_diffusion_3d(level, c, level_c, kz);
+    std::copy(A.val.begin(), A.val.end(), val);
+    if(col) std::copy(A.col.begin(), A.col.end(), col);
+    if(rowptr) std::copy(A.rowptr.begin(), A.rowptr.end(), rowptr);
+  });
+}
+
 } // extern "C"
 
 struct feat_learned_s

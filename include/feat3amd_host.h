@@ -231,6 +231,11 @@ int feat_lognormal_cells(uint64_t seed, uint64_t n, double sigma, double* out);
  * the pattern (rowptr, col) is the Q1 pattern of feat_q1_assemble(2, level) */
 int feat_q1_diffusion_assemble(int level, int level_c, const double* coef, double* val, uint64_t* col,
                                uint64_t* rowptr);
+/* 3D Q1 diffusion matrix of the tensor diag(1, 1, kz) on `level` for a per-cell coefficient of
+ * `level_c` (>= level; cell (i, j, k) of level_c at i + 2^level_c (j + 2^level_c k)); the pattern
+ * is the Q1 pattern of feat_q1_assemble(3, level).  2x2x2 Gauss points per cell. */
+int feat_q1_diffusion3d_assemble(int level, int level_c, const double* coef, double kz, double* val,
+                                 uint64_t* col, uint64_t* rowptr);
 
 /* ---- multigrid with a learned finest-level smoother (3D Q1 anisotropic diffusion) ----
  * No counterpart in the reference (see include/feat3amd/learned.hpp). */
