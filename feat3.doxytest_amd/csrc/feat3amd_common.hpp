@@ -32,8 +32,9 @@ namespace {
 constexpr int kThreads = 256;
 constexpr int kXcd = 8;
 constexpr int kMaxGrid = FEAT_REDUCE_MAX_BLOCKS;
-constexpr int kRowsPerBlock = 64;   // rows per CSR row-block
-constexpr int kChunk = 2048;        // nonzeros staged in LDS per pass
+constexpr int kRowsPerBlock = 256;  // rows per CSR row-block (one lane per row)
+constexpr int kCsrUnroll = 4;       // CSR entries each thread loads per pass
+constexpr int kChunk = kCsrUnroll * kThreads;   // nonzeros staged in LDS per pass
 constexpr int kSellUnroll = 8;      // SELL entries per row in flight
 
 

@@ -1,11 +1,12 @@
 // feat3amd_kernels.hip -- gfx950 kernels and the C-ABI of include/feat3amd.h.
 //
 // Design (MI355X / CDNA4):
-//  * CSR SpMV: rows are processed in row-blocks of kRowsPerBlock.  A 256-thread
-//    workgroup streams the contiguous nonzero range of its row-block through
-//    LDS (val and the gathered x[col] side by side, fully coalesced 64-wide
-//    loads), then one lane per row sums its entries from LDS in storage order
-//    with one fma per entry.  The per-row arithmetic is therefore exactly the
+//  * CSR SpMV: rows are processed in row-blocks of kRowsPerBlock (256, one lane
+//    per row).  A 256-thread workgroup streams the contiguous nonzero range of its
+//    row-block through LDS in passes of kChunk entries (val and the gathered
+//    x[col] side by side, fully coalesced 64-wide loads, all of a pass in flight
+//    before the first store), then each lane sums its row's entries from LDS in
+//    storage order with one fma per entry.  The per-row arithmetic is therefore exactly the
 //    reference's sequential loop (arch/apply_generic.hpp:57-66), bit for bit,
 //    while HBM sees only coalesced streams.  Fused epilogues apply the unit
 //    filter (bit mask), the defect/Richardson/Jacobi updates and reductions.
@@ -55,6 +56,22 @@ int check_launch(const char* where)
 
 namespace {
 
+// EpiPre<E>::on: E loads its row operands through E::pre, ahead of the row sum
+template<class E, class = void>
+struct EpiPre
+{
+  struct type
+  {
+  };
+  static constexpr bool on = false;
+};
+template<class E>
+struct EpiPre<E, std::void_t<typename E::Pre>>
+{
+  typedef typename E::Pre type;
+  static constexpr bool on = true;
+};
+
 // ---------------------------------------------------------------------------
 // CSR row-block kernel with a fused epilogue.
 //   Epi::row(row, sum, acc0, acc1) is called once per row by its lane.
@@ -66,6 +83,7 @@ __global__ __launch_bounds__(kThreads) void k_csr_rows(const DT* __restrict__ va
                                                        const DT* __restrict__ x, uint64_t rows,
                                                        Epi epi, GridReduce gr)
 {
+  static_assert(kChunk == kCsrUnroll * kThreads, "one pass = kCsrUnroll entries per thread");
   __shared__ DT s_val[kChunk];
   __shared__ DT s_x[kChunk];
   const uint64_t nrb = (rows + kRowsPerBlock - 1) / kRowsPerBlock;
@@ -79,23 +97,42 @@ __global__ __launch_bounds__(kThreads) void k_csr_rows(const DT* __restrict__ va
     const uint64_t nr = (rows - r0) < uint64_t(kRowsPerBlock) ? (rows - r0) : uint64_t(kRowsPerBlock);
     const uint64_t p0 = uint64_t(rowptr[r0]);
     const uint64_t p1 = uint64_t(rowptr[r0 + nr]);
+    const bool mine = uint64_t(t) < nr;
     uint64_t rs = 0, re = 0;
-    if(uint64_t(t) < nr)
+    typename EpiPre<Epi>::type pf{};
+    if(mine)
     {
       rs = uint64_t(rowptr[r0 + t]);
       re = uint64_t(rowptr[r0 + t + 1]);
+      if constexpr(EpiPre<Epi>::on) pf = epi.pre(r0 + t);
     }
     DT sum = DT(0);
     for(uint64_t c0 = p0; c0 < p1; c0 += kChunk)
     {
       const uint64_t cn = (p1 - c0) < uint64_t(kChunk) ? (p1 - c0) : uint64_t(kChunk);
-      for(uint64_t j = t; j < cn; j += kThreads)
+      // every value and column of the pass is requested before the first gather of x,
+      // and every gather before the first LDS store (the matrix is streamed once:
+      // nontemporal loads keep x resident in L2)
+      DT v[kCsrUnroll], xv[kCsrUnroll];
+      uint64_t cc[kCsrUnroll];
+#pragma unroll
+      for(int u = 0; u < kCsrUnroll; ++u)
       {
-        s_val[j] = val[c0 + j];
-        s_x[j] = x[uint64_t(col[c0 + j])];
+        const uint64_t j = uint64_t(t) + uint64_t(u) * kThreads;
+        v[u] = j < cn ? __builtin_nontemporal_load(val + c0 + j) : DT(0);
+        cc[u] = j < cn ? uint64_t(__builtin_nontemporal_load(col + c0 + j)) : 0;
+      }
+#pragma unroll
+      for(int u = 0; u < kCsrUnroll; ++u)
+        xv[u] = uint64_t(t) + uint64_t(u) * kThreads < cn ? x[cc[u]] : DT(0);
+#pragma unroll
+      for(int u = 0; u < kCsrUnroll; ++u)
+      {
+        s_val[t + u * kThreads] = v[u];
+        s_x[t + u * kThreads] = xv[u];
       }
       __syncthreads();
-      if(uint64_t(t) < nr)
+      if(mine)
       {
         const uint64_t kb = rs > c0 ? rs : c0;
         const uint64_t ke = re < c0 + cn ? re : c0 + cn;
@@ -104,8 +141,13 @@ __global__ __launch_bounds__(kThreads) void k_csr_rows(const DT* __restrict__ va
       }
       __syncthreads();
     }
-    if(uint64_t(t) < nr)
-      epi.row(r0 + t, sum, acc0, acc1);
+    if(mine)
+    {
+      if constexpr(EpiPre<Epi>::on)
+        epi.row(r0 + t, sum, pf, acc0, acc1);
+      else
+        epi.row(r0 + t, sum, acc0, acc1);
+    }
   }
   grid_reduce_finish(gr, acc0, acc1);
 }
@@ -271,22 +313,6 @@ struct EpiSmooth
     }
   }
   __device__ __forceinline__ void row(uint64_t i, DT sum, double& a0, double& a1) const { row(i, sum, pre(i), a0, a1); }
-};
-
-// EpiPre<E>::on: E loads its row operands through E::pre, ahead of the row sum
-template<class E, class = void>
-struct EpiPre
-{
-  struct type
-  {
-  };
-  static constexpr bool on = false;
-};
-template<class E>
-struct EpiPre<E, std::void_t<typename E::Pre>>
-{
-  typedef typename E::Pre type;
-  static constexpr bool on = true;
 };
 
 template<typename DT, typename IT, class Epi>

@@ -1,0 +1,135 @@
+"""The 3D Q1 variable-coefficient Poisson problem at level 6 (274,625 dofs): a
+log-normal(sigma = 1.5) coefficient per cell with 10:1 anisotropy along z,
+right-hand side uniform [-1, 1].
+
+* PCG(MG-V(4,4) Richardson(Jacobi omega = 0.5)) on the GPU against the oracle's
+  solve on the same matrices: iterations equal within +-1, final relative defect
+  <= 1e-8, and the solution as accurate as the oracle's (both measured against the
+  oracle's solve to 1e-13).
+* The learned smoother's multigrid on the same problem class reaches 1e-8 by the
+  plain FP64 defect of the returned solution.
+"""
+import numpy as np
+import pytest
+
+from oracle.pyoracle import Csr, csr_transpose, COARSE_DIRECT
+from tests.test_gpu_lafem import dev, host
+
+pytestmark = pytest.mark.gpu
+
+LF, LC, SIGMA, KZ = 6, 1, 1.5, 10.0
+SEED = 28054777172512
+
+
+def c5_levels(feat, lf=LF, lc=LC):
+    levels = feat.diffusion3d_levels(lf, lc, SEED + 1000, SIGMA, KZ)
+    for i, lv in enumerate(levels):
+        lv["A"] = Csr(lv["n"], lv["n"], lv["val"], lv["col"], lv["rowptr"])
+        if i + 1 < len(levels):
+            P = Csr(lv["n"], lv["pcols"], lv["pval"], lv["pcol"], lv["prowptr"])
+            lv["P"], lv["R"] = P, csr_transpose(P)
+    return levels
+
+
+@pytest.fixture(scope="module")
+def c5(feat):
+    levels = c5_levels(feat)
+    n = levels[0]["n"]
+    b = feat.rhs_random(SEED + 2000, n)
+    b[levels[0]["filt"].astype(np.int64)] = 0.0
+    return levels, b
+
+
+def test_c5_matrix_invariants(c5):
+    # symmetric, zero row sums on interior rows (constants lie in the kernel of the
+    # diffusion operator), positive diagonal, anisotropy visible in the z couplings
+    levels, _ = c5
+    A = levels[0]["A"]
+    import scipy.sparse as sp
+    S = sp.csr_matrix((A.val, A.col.astype(np.int64), A.rowptr.astype(np.int64)), shape=(A.rows, A.cols))
+    assert abs(S - S.T).max() <= 1e-14 * abs(S).max()
+    m = 2 ** LF + 1
+    idx = np.arange(A.rows)
+    i0, i1, i2 = idx % m, (idx // m) % m, idx // (m * m)
+    interior = (i0 > 0) & (i0 < m - 1) & (i1 > 0) & (i1 < m - 1) & (i2 > 0) & (i2 < m - 1)
+    rs = np.asarray(S.sum(axis=1)).ravel()
+    assert np.max(np.abs(rs[interior])) <= 1e-12 * np.max(S.diagonal())
+    assert np.all(S.diagonal() > 0)
+
+
+def test_c5_jacobi_mg_vs_oracle(feat, torch_gpu, oracle, c5):
+    """Iterations within +-1 of the oracle's, the FP64 defect of the returned x below
+    1e-8, and equal accuracy: measured against x* (the oracle's solve to 1e-13), the
+    GPU solution is no further off than 1.5 times the oracle's own solution.
+
+    This problem's PCG is rounding-sensitive: one-ulp changes of 1,000 right-hand-side
+    entries move the oracle's own defect history by about 5 % (tests/test_c5_oracle.py),
+    so a solve whose dots sum in another order may stop one iteration later (the GPU
+    solve then equals the oracle's one-iteration-longer iterate).  At the oracle's
+    iteration count (max_iter fixed) the two solutions differ by less than the oracle's
+    own distance from x*."""
+    torch = torch_gpu
+    levels, b = c5
+    n = levels[0]["n"]
+    o = oracle.mg_pcg_solve(levels, np.zeros(n), b, coarse=COARSE_DIRECT, max_iter=200)
+    assert o["status"] == "success" and o["iters"] > 5
+    xs = oracle.mg_pcg_solve(levels, np.zeros(n), b, coarse=COARSE_DIRECT, max_iter=400, tol_rel=1e-13)["x"]
+    nxs = np.linalg.norm(xs)
+    e_o = np.linalg.norm(o["x"] - xs) / nxs
+    res = {}
+    for tag, mi in (("free", 200), ("fixed", o["iters"])):
+        s = feat.MgPcg(feat.mgpcg_cfg(coarse="direct", max_iter=mi), levels=levels)
+        x = torch.zeros(n, dtype=torch.float64, device="cuda")
+        r = s.solve(x, dev(torch, b))
+        s.close()
+        res[tag] = (r, host(x))
+    r, xg = res["free"]
+    e_g = np.linalg.norm(xg - xs) / nxs
+    print(f"C5 L6 Jacobi MG-PCG: gpu {r['iters']} iterations, oracle {o['iters']}; error vs x*: "
+          f"gpu {e_g:.2e}, oracle {e_o:.2e}")
+    assert r["status"] == "success"
+    assert abs(r["iters"] - o["iters"]) <= 1, (r["iters"], o["iters"])
+    assert r["def_final"] <= 1e-8 * r["def_init"]
+    # the FP64 defect of the returned solution, formed on the host
+    A = levels[0]["A"]
+    import scipy.sparse as sp
+    S = sp.csr_matrix((A.val, A.col.astype(np.int64), A.rowptr.astype(np.int64)), shape=(n, n))
+    d = b - S @ xg
+    d[levels[0]["filt"].astype(np.int64)] = 0.0
+    print(f"true rel. defect {np.linalg.norm(d) / np.linalg.norm(b):.2e}")
+    assert np.linalg.norm(d) <= 1.01e-8 * np.linalg.norm(b)
+    assert e_g <= 1.5 * e_o
+    rf, xf = res["fixed"]
+    err = np.linalg.norm(xf - o["x"]) / np.linalg.norm(o["x"])
+    print(f"at {rf['iters']} iterations: |x - x_oracle| / |x_oracle| = {err:.2e}")
+    assert rf["iters"] == o["iters"]
+    assert err <= e_o
+
+
+def test_c5_learned_solve_reaches_tolerance(feat, torch_gpu, c5):
+    """The learned smoother's multigrid, trained on the GPU on this problem, returns a
+    solution whose plain FP64 defect (formed on the host) is below 1e-8."""
+    torch = torch_gpu
+    levels, b = c5
+    n = levels[0]["n"]
+    s = feat.LearnedMG(level_fine=LF, level_coarse=LC, sigma=SIGMA, kz=KZ, seed=SEED)
+    try:
+        assert s.n == n
+        rho = s.generate()
+        fit = s.fit(rounds=4, steps=300)
+        jac = s.solve(learned=False)
+        x = torch.zeros(n, dtype=torch.float64, device="cuda")
+        r = s.solve(learned=True, x=x)
+    finally:
+        s.close()
+    print(f"C5 L6: Jacobi-MG {jac['iters']} iterations, learned {r['iters']} (fallback {r['fell_back']}), "
+          f"rho_jacobi {rho:.3f}, fit {fit}")
+    assert jac["status"] == "success" and r["status"] == "success"
+    A = levels[0]["A"]
+    import scipy.sparse as sp
+    S = sp.csr_matrix((A.val, A.col.astype(np.int64), A.rowptr.astype(np.int64)), shape=(n, n))
+    xg = host(x)
+    d = b - S @ xg
+    d[levels[0]["filt"].astype(np.int64)] = 0.0
+    print(f"learned solve: true rel. defect {np.linalg.norm(d) / np.linalg.norm(b):.2e}")
+    assert np.linalg.norm(d) <= 1.01e-8 * np.linalg.norm(b)
