@@ -108,6 +108,7 @@ KERNEL_API = [
     ("feat_convert_f16_f64", _i, [_vp, _vp, _vp, _u64]),
     ("feat_convert_u64_u32", _i, [_vp, _vp, _vp, _u64]),
     ("feat_csr_apply_f64_u64", _i, [_vp, _vp, _d, _vp, _d, _vp, _vp, _vp, _vp, _u64, _u64, _u64, _i]),
+    ("feat_csr_apply_transposed_noalloc_f64_u64", _i, [_vp, _vp, _d, _vp, _d, _vp, _vp, _vp, _vp, _u64, _u64, _u64]),
     ("feat_csr_apply_f64_u32", _i, [_vp, _vp, _d, _vp, _d, _vp, _vp, _vp, _vp, _u64, _u64, _u64, _i]),
     ("feat_csr_apply_f32_u32", _i, [_vp, _vp, _f, _vp, _f, _vp, _vp, _vp, _vp, _u64, _u64, _u64, _i]),
     ("feat_csr_apply_f32_u64", _i, [_vp, _vp, _f, _vp, _f, _vp, _vp, _vp, _vp, _u64, _u64, _u64, _i]),

@@ -22,6 +22,13 @@ int check_launch(const char* where);
 template<typename DT, typename IT>
 int csr_apply_transposed(feat_handle_t h, DT* r, DT a, const DT* x, DT b, bool bzero, const DT* y, const DT* val,
                          const IT* col, const IT* rowptr, uint64_t rows, uint64_t cols, uint64_t nnz, const char* name);
+// the same with every buffer in the per-device scratch array (feat3amd_transpose.hip)
+template<typename DT, typename IT>
+int csr_apply_transposed_noalloc(feat_handle_t h, DT* r, DT a, const DT* x, DT b, bool bzero, const DT* y,
+                                 const DT* val, const IT* col, const IT* rowptr, uint64_t rows, uint64_t cols,
+                                 uint64_t nnz, const char* name);
+// looks up this device's copy of that scratch array (once per device; the device is current)
+int noalloc_scratch_attach(int device);
 } // namespace feat3amd_detail
 using feat3amd_detail::check_hip;
 using feat3amd_detail::check_launch;

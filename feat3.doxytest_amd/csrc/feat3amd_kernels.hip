@@ -1313,6 +1313,7 @@ int feat_handle_create(feat_handle_t* out, int device, feat_stream_t stream)
   if(!rc) rc = check_hip(hipMalloc(&h->scal, sizeof(double) * 16), "hipMalloc(scal)");
   if(!rc) rc = check_hip(hipHostMalloc(&h->host, sizeof(double) * 16, 0), "hipHostMalloc");
   if(!rc) rc = check_hip(hipEventCreateWithFlags(&h->event, hipEventDisableTiming), "hipEventCreate");
+  if(!rc) rc = feat3amd_detail::noalloc_scratch_attach(device);
   if(!rc) rc = check_hip(hipMemsetAsync(h->counter, 0, 64, h->stream), "hipMemset");
   if(!rc) rc = check_hip(hipStreamSynchronize(h->stream), "hipStreamSynchronize");
   if(rc)
@@ -1525,6 +1526,20 @@ int csr_apply_impl(feat_handle_t h, DT* r, DT a, const DT* x, DT b, const DT* y,
 } // namespace
 
 extern "C" {
+
+int feat_csr_apply_transposed_noalloc_f64_u64(feat_handle_t h, double* r, double a, const double* x, double b,
+                                              const double* y, const double* val, const uint64_t* col,
+                                              const uint64_t* rowptr, uint64_t rows, uint64_t cols, uint64_t nnz)
+{
+  const char* name = "feat_csr_apply_transposed_noalloc_f64_u64";
+  if(!h) return fail(FEAT_ERR_INVALID, "csr_apply_transposed_noalloc: null handle");
+  const bool bzero = (b < 0 ? -b : b) < DBL_EPSILON;
+  if((cols && !r) || (rows && !x) || (nnz && (!val || !col)) || !rowptr || (!bzero && cols && !y))
+    return fail(FEAT_ERR_INVALID, "csr_apply_transposed_noalloc: null argument");
+  if(r == x && cols) return fail(FEAT_ERR_ALIAS, "csr_apply_transposed_noalloc: r and x must not alias");
+  return feat3amd_detail::csr_apply_transposed_noalloc<double, uint64_t>(h, r, a, x, b, bzero, y, val, col, rowptr, rows,
+                                                                         cols, nnz, name);
+}
 
 int feat_csr_apply_f64_u64(feat_handle_t h, double* r, double a, const double* x, double b,
                            const double* y, const double* val, const uint64_t* col,

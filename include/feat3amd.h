@@ -111,6 +111,20 @@ int feat_csr_apply_f32_u64(feat_handle_t h, float* r, float a, const float* x, f
                            const uint64_t* rowptr, uint64_t rows, uint64_t cols, uint64_t nnz,
                            int transposed);
 
+/* The transposed form of feat_csr_apply_f64_u64 (transposed != 0) without any allocation:
+ * no counterpart in the reference.  It is for callers that cannot allocate device memory
+ * in the call (HIP graph capture, serving loops, harnesses that forbid it).  Its sort
+ * buffers live in a static device array of FEAT_TR_NOALLOC_SCRATCH_BYTES per GPU, held
+ * until the process ends and looked up when a handle is created; a matrix needing more
+ * (about 20 bytes per entry plus 4 per column and the sort's own few KB) is rejected with
+ * FEAT_ERR_UNSUPPORTED before anything runs.  Same arithmetic, bit-identical results.
+ * Calls that overlap on one GPU (any handle, any stream) share that array and race. */
+#define FEAT_TR_NOALLOC_SCRATCH_BYTES (32u << 20)
+int feat_csr_apply_transposed_noalloc_f64_u64(feat_handle_t h, double* r, double a, const double* x,
+                                              double b, const double* y, const double* val,
+                                              const uint64_t* col, const uint64_t* rowptr,
+                                              uint64_t rows, uint64_t cols, uint64_t nnz);
+
 /* BCSR with bh x bw row-major blocks, (bh,bw) in {(2,2),(2,1),(1,2),(3,3)}
  * (arch/apply.hpp:108-133, generic arch/apply_generic.hpp:121-154). */
 int feat_bcsr_apply_f64_u64(feat_handle_t h, int bh, int bw, double* r, double a,
