@@ -33,7 +33,8 @@ PEAK_HBM_GBS = 8000.0
 # runs of `bench.py --spmv-only` (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch); FETCH_SIZE
 # is scaled by 2.0 and WRITE_SIZE by 1.0, the factors that make the same counters read
 # 1 GiB for a 1 GiB device-to-device copy (profiles/spmv_traffic_3d_q1_l7.md)
-SPMV_TRAFFIC_BYTES = {"sell64_diag": 38762.8 * 2048 + 16771.0 * 1024,
+SPMV_TRAFFIC_BYTES = {"jacobi_step": None,
+                      "sell64_diag": 38762.8 * 2048 + 16771.0 * 1024,
                       "sell64_dict": 40574.2 * 2048 + 16771.0 * 1024,
                       "sell64": 295808.6 * 2048 + 16771.0 * 1024,
                       "csr_u64": 481390.4 * 2048 + 16771.0 * 1024}
@@ -448,6 +449,31 @@ def learned_leg(feat, torch, dev, args):
     return out
 
 
+def read_ceilings(feat, torch, dev, stream, h, nbytes, reps):
+    """GB/s of feat_stream_read_f64 over nbytes: one buffer read repeatedly, and eight
+    nbytes windows of a larger buffer read in turn (each from HBM, not the last-level cache)."""
+    k, _ = feat.libs()
+    nd = (nbytes // 8 + 1) & ~1
+    big = torch.ones(8 * nd, dtype=torch.float64, device=dev)
+    sink = torch.zeros(1, dtype=torch.float64, device=dev)
+    out = {"bytes": nd * 8}
+    for tag, nwin in (("same_bytes_repeated_gbps", 1), ("same_bytes_from_hbm_gbps", 8)):
+        for w in range(nwin):
+            feat.check(k.feat_stream_read_f64(h.h, feat.ptr(big[w * nd:]), nd, feat.ptr(sink)), "stream read")
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for i in range(reps):
+            k.feat_stream_read_f64(h.h, feat.ptr(big[(i % nwin) * nd:]), nd, feat.ptr(sink))
+        e1.record(stream)
+        e1.synchronize()
+        us = e0.elapsed_time(e1) * 1e3 / reps
+        out[tag] = nd * 8 / (us * 1e-6) / 1e9
+        out[tag.replace("_gbps", "_us")] = us
+    log(f"read ceiling over {nd * 8 / 1e6:.1f} MB: {out['same_bytes_repeated_gbps']:.0f} GB/s repeated, "
+        f"{out['same_bytes_from_hbm_gbps']:.0f} GB/s from HBM")
+    return out
+
+
 def spmv_only(feat, torch, dev, stream, args):
     """SpMV r = A*x on the level-7 matrix in each format, args.spmv_reps launches each,
     then args.spmv_reps device-to-device copies of 1 GiB (the counter calibration)."""
@@ -475,6 +501,22 @@ def spmv_only(feat, torch, dev, stream, args):
         e1.synchronize()
         out[name + "_us"] = e0.elapsed_time(e1) * 1e3 / args.spmv_reps
         log(f"{name}: {out[name + '_us']:.1f} us per launch")
+    # one Jacobi step on the diagonal plan per call (the solve's dominant kernel)
+    inv_t = torch.rand(n, dtype=torch.float64, device=dev)
+    rhs_t = torch.rand(n, dtype=torch.float64, device=dev)
+    xo, tmp_t = torch.empty_like(rhs_t), torch.rand(n, dtype=torch.float64, device=dev)
+    keep_ws, ws = feat.reduce_ws(torch, dev)
+    dm = plans["sell64_diag"].m
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(args.spmv_reps):
+        feat.check(k.feat_rj_smooth_from1_f64(h.h, C.byref(dm), feat.ptr(inv_t), None, 2, 0.5, feat.ptr(rhs_t), None,
+                                              feat.ptr(xo), feat.ptr(tmp_t), None, None, None, None, ws), "step")
+    e1.record(stream)
+    e1.synchronize()
+    out["jacobi_step_us"] = e0.elapsed_time(e1) * 1e3 / args.spmv_reps
+    log(f"jacobi step: {out['jacobi_step_us']:.1f} us per launch")
+    out["read_ceiling"] = read_ceilings(feat, torch, dev, stream, h, int(dm.nnz) + 256 * 12 + 4 * n * 8, args.spmv_reps)
     src = torch.empty(1 << 27, dtype=torch.float64, device=dev)
     dst = torch.empty_like(src)
     src.fill_(1.0)
@@ -739,21 +781,28 @@ def main():
     xo, tmp_t = torch.empty_like(rhs_t), torch.empty_like(rhs_t)
     keep_ws, ws = feat.reduce_ws(torch, dev)
 
-    def time_smooth(steps):
-        def call():
-            return k.feat_rj_smooth_f64(h.h, C.byref(dplan), feat.ptr(inv_t), None, steps, 0.5, feat.ptr(rhs_t),
-                                        None, feat.ptr(xo), feat.ptr(tmp_t), None, None, None, None, ws)
+    # a two-step smoother whose first iterate is already in tmp is exactly one launch of the
+    # Jacobi step kernel (x = tmp + omega * filter(inv * (rhs - A tmp)))
+    def call_step():
+        return k.feat_rj_smooth_from1_f64(h.h, C.byref(dplan), feat.ptr(inv_t), None, 2, 0.5, feat.ptr(rhs_t),
+                                          None, feat.ptr(xo), feat.ptr(tmp_t), None, None, None, None, ws)
+
+    def time_loop(call, reps):
         for _ in range(3):
-            feat.check(call(), "feat_rj_smooth_f64")
+            feat.check(call(), "warm-up")
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record(stream)
-        for _ in range(args.spmv_reps):
+        for _ in range(reps):
             call()
         e1.record(stream)
         e1.synchronize()
-        return e0.elapsed_time(e1) / args.spmv_reps
+        return e0.elapsed_time(e1) / reps
 
-    step_ms = (time_smooth(9) - time_smooth(1)) / 8
+    step_ms = time_loop(call_step, args.spmv_reps)
+    # measured read ceilings (feat_stream_read_f64): a buffer of the Jacobi step's bytes read
+    # again and again (held in the last-level cache like the step's own arrays), and the same
+    # bytes from HBM (eight such windows of a larger buffer in turn)
+    ceil = read_ceilings(feat, torch, dev, stream, h, int(dplan.nnz) + 256 * 12 + 4 * n * 8, args.spmv_reps)
     idx_bytes = {feat.IDX_D16: 2, feat.IDX_U32: 4}[plan.index]
     nslices = (n + 63) // 64
     # bytes each kernel must move: its own arrays once, x once, r once
@@ -782,7 +831,7 @@ def main():
     for m_ in (plan, eplan, dplan):
         feat.check(k.feat_sell_destroy_f64(C.byref(m_)), "feat_sell_destroy_f64")
     h.close()
-    achieved = dict_gbs
+    achieved = step_gbs
 
     out = {
         "metric": "MG-PCG time-to-1e-8 residual (s) + SpMV HBM GB/s, 3D Q1 Poisson 2.1M dofs",
@@ -840,17 +889,26 @@ def main():
         "assembly": assembly,
         "smoothers": smoothers,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": achieved / PEAK_HBM_GBS, "traffic": SPMV_TRAFFIC_BYTES["sell64_diag"],
-                     "algorithmic_bytes": dict_bytes,
-                     "kernel": "k_sell_diag_rows: diagonal pair-dictionary SpMV r = A*x (the solver's matrix "
-                               "format), bytes = its arrays (1 B per offset slot, 27 slots per row in 7 words, "
-                               "pair table) + x + r",
+                     "frac": achieved / PEAK_HBM_GBS, "traffic": SPMV_TRAFFIC_BYTES["jacobi_step"],
+                     "algorithmic_bytes": step_bytes, "us_per_launch": step_ms * 1e3,
+                     "kernel": "k_sell_diag_rows<double,EpiSmooth<double>,9,7>: one fused Richardson(Jacobi) "
+                               "step x_{k+1} = x_k + 0.5 filter(inv_diag (rhs - A x_k)) on the level-7 diagonal "
+                               "pair-dictionary plan (the solve's dominant kernel: 35 of its launches per solve), "
+                               "bytes = codes (1 B per offset slot, 27 slots in 7 words per row) + pair table + "
+                               "x_k + rhs + inv_diag + x_{k+1}",
+                     "measured_read_ceiling": ceil,
+                     "frac_of_read_ceiling_same_bytes": achieved / ceil["same_bytes_repeated_gbps"],
+                     "frac_of_read_ceiling_hbm": achieved / ceil["same_bytes_from_hbm_gbps"],
+                     "spmv_diag_apply": {"achieved": dict_gbs, "frac": dict_gbs / PEAK_HBM_GBS,
+                                         "traffic": SPMV_TRAFFIC_BYTES["sell64_diag"], "algorithmic_bytes": dict_bytes},
                      "entry_order_dictionary": {"achieved": edict_gbs, "frac": edict_gbs / PEAK_HBM_GBS,
                                                 "traffic": SPMV_TRAFFIC_BYTES["sell64_dict"],
                                                 "algorithmic_bytes": edict_bytes},
                      "csr_equivalent_gbps": csr_bytes / (dict_ms * 1e-3) / 1e9,
                      "plain_sell64": {"achieved": sell_gbs, "frac": sell_gbs / PEAK_HBM_GBS,
-                                      "traffic": SPMV_TRAFFIC_BYTES["sell64"], "algorithmic_bytes": sell_bytes}},
+                                      "traffic": SPMV_TRAFFIC_BYTES["sell64"], "algorithmic_bytes": sell_bytes},
+                     "csr_u64": {"achieved": csr_gbs, "frac": csr_gbs / PEAK_HBM_GBS,
+                                 "traffic": SPMV_TRAFFIC_BYTES["csr_u64"], "algorithmic_bytes": csr_bytes}},
     }
     if not args.no_cpu_baseline:
         log("cpu baseline: reference MG-PCG on one host core ...")

@@ -100,6 +100,7 @@ KERNEL_API = [
     ("feat_host_malloc", _i, [C.POINTER(_vp), C.c_size_t]),
     ("feat_host_free", _i, [_vp]),
     ("feat_copy", _i, [_vp, _vp, _vp, C.c_size_t]),
+    ("feat_stream_read_f64", _i, [_vp, _vp, _u64, _vp]),
     ("feat_set_memory_f64", _i, [_vp, _vp, _d, _u64]),
     ("feat_set_memory_f32", _i, [_vp, _vp, _f, _u64]),
     ("feat_convert_f64_f32", _i, [_vp, _vp, _vp, _u64]),
@@ -165,6 +166,14 @@ KERNEL_API = [
     ("feat_prol_add_f64", _i, [_vp, C.POINTER(Matrix), _vp, _vp, _vp]),
     ("feat_rj_smooth_f64", _i, [_vp, C.POINTER(Matrix), _vp, _vp, _i, _d, _vp, _vp, _vp, _vp, _vp, _vp,
                                 _vp, _vp, C.POINTER(ReduceWs)]),
+    ("feat_defect_first_f64", _i, [_vp, C.POINTER(Matrix), _vp, _vp, _vp, _vp, _vp, _vp, _d, _vp,
+                                   C.POINTER(ReduceWs)]),
+    ("feat_apply_masked_first_f64", _i, [_vp, C.POINTER(Matrix), _vp, _vp, _vp, _vp, _vp, _d, _vp,
+                                         C.POINTER(ReduceWs)]),
+    ("feat_rj_smooth_from1_f64", _i, [_vp, C.POINTER(Matrix), _vp, _vp, _i, _d, _vp, _vp, _vp, _vp, _vp, _vp,
+                                      _vp, _vp, C.POINTER(ReduceWs)]),
+    ("feat_pcg_update_first_f64", _i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _u64, _vp, _vp, _d, _vp,
+                                       C.POINTER(ReduceWs)]),
     ("feat_dense_gemv_f64", _i, [_vp, _i, _vp, _vp, _vp, _vp]),
     ("feat_pcg_update_f64", _i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _u64, C.POINTER(ReduceWs)]),
     ("feat_pcg_direction_f64", _i, [_vp, _vp, _vp, _vp, _vp, _u64]),

@@ -174,6 +174,26 @@ struct EpiApply
   __device__ __forceinline__ void row(uint64_t i, DT sum, double& a0, double& a1) const { row(i, sum, pre(i), a0, a1); }
 };
 
+// The first iterate of the Richardson(Jacobi) smoother that takes the row's value v as its
+// right-hand side, formed where v is produced: x1 = filter(omega * (inv_diag * v)), the
+// operations of k_smooth_first without its skip rule (the smoother's next step, which has
+// the norm, applies it).  Unused when x1 is null.
+template<typename DT>
+struct FirstIter
+{
+  const DT* inv;
+  DT* x1;
+  DT omega;
+  __device__ __forceinline__ DT load(uint64_t i) const { return x1 ? inv[i] : DT(0); }
+  __device__ __forceinline__ void store(uint64_t i, DT v, DT inv_i, bool m) const
+  {
+    if(!x1) return;
+    DT c = inv_i * v;
+    if(m) c = DT(0);
+    x1[i] = dfma(omega, c, DT(0));
+  }
+};
+
 // r = filter(A*x) with optional |r|^2 and r.w       [transfer.hpp:247-251 + unit_filter]
 template<typename DT>
 struct EpiMasked
@@ -182,19 +202,24 @@ struct EpiMasked
   const uint32_t* mask;
   const DT* dotw;
   bool nrm, dot;
+  FirstIter<DT> first;   // the pre-smoother's first iterate on r (restriction)
   __device__ void begin() {}
   // the row's operands, loadable before its sum is formed
   struct Pre
   {
-    DT w;
+    DT w, inv;
     bool m;
   };
-  __device__ __forceinline__ Pre pre(uint64_t i) const { return Pre{dot ? dotw[i] : DT(0), is_masked(mask, i)}; }
+  __device__ __forceinline__ Pre pre(uint64_t i) const
+  {
+    return Pre{dot ? dotw[i] : DT(0), first.load(i), is_masked(mask, i)};
+  }
   __device__ __forceinline__ void row(uint64_t i, DT sum, const Pre& p, double& a0, double& a1) const
   {
     DT v = dfma(sum, DT(1), DT(0));
     if(p.m) v = DT(0);
     r[i] = v;
+    first.store(i, v, p.inv, p.m);
     if(nrm) a0 = fma(double(v), double(v), a0);
     if(dot) a1 = fma(double(v), double(p.w), a1);
   }
@@ -209,19 +234,21 @@ struct EpiDefect
   const DT* rhs;
   const uint32_t* mask;
   bool nrm;
+  FirstIter<DT> first;   // the post-smoother's first iterate on d
   __device__ void begin() {}
   // the row's operands, loadable before its sum is formed
   struct Pre
   {
-    DT rhs;
+    DT rhs, inv;
     bool m;
   };
-  __device__ __forceinline__ Pre pre(uint64_t i) const { return Pre{rhs[i], is_masked(mask, i)}; }
+  __device__ __forceinline__ Pre pre(uint64_t i) const { return Pre{rhs[i], first.load(i), is_masked(mask, i)}; }
   __device__ __forceinline__ void row(uint64_t i, DT sum, const Pre& p, double& a0, double&) const
   {
     DT v = dfma(sum, DT(-1), p.rhs);
     if(p.m) v = DT(0);
     d[i] = v;
+    first.store(i, v, p.inv, p.m);
     if(nrm) a0 = fma(double(v), double(v), a0);
   }
   __device__ __forceinline__ void row(uint64_t i, DT sum, double& a0, double& a1) const { row(i, sum, pre(i), a0, a1); }
@@ -1052,6 +1079,21 @@ __global__ void k_dict_fill_diag(const double* val, const uint64_t* col, const u
 // ---------------------------------------------------------------------------
 // elementwise kernels
 // ---------------------------------------------------------------------------
+// bandwidth probe: every thread streams 16-byte nontemporal loads grid-stride and sums
+// them; the sum is stored only if it equals a value no real data produces, so the loads
+// cannot be dropped and nothing is written
+typedef double feat_d2 __attribute__((ext_vector_type(2)));
+__global__ __launch_bounds__(kThreads) void k_stream_read(const feat_d2* __restrict__ p, uint64_t n2, double* sink)
+{
+  double s = 0.0;
+  for(uint64_t i = blockIdx.x * uint64_t(kThreads) + threadIdx.x; i < n2; i += uint64_t(gridDim.x) * kThreads)
+  {
+    const feat_d2 v = __builtin_nontemporal_load(p + i);
+    s += v.x + v.y;
+  }
+  if(s == -1.2345e300) *sink = s;
+}
+
 template<typename DT>
 __global__ void k_fill(DT* p, DT v, uint64_t n)
 {
@@ -1287,15 +1329,18 @@ __global__ void k_dense_gemv(int n, const DT* ainv, const DT* b, DT* x, const ui
 
 __global__ __launch_bounds__(kThreads) void k_pcg_update(double* x, double* r, const double* p, const double* q,
                                                          const double* gamma, const double* qp, uint64_t n,
-                                                         GridReduce gr)
+                                                         GridReduce gr, FirstIter<double> first,
+                                                         const uint32_t* first_mask)
 {
   const double alpha = (*gamma) / (*qp);
   double acc = 0.0;
   for(uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
   {
+    const double inv = first.load(i);
     x[i] = fma(alpha, p[i], x[i]);
     const double rn = fma(-alpha, q[i], r[i]);
     r[i] = rn;
+    first.store(i, rn, inv, is_masked(first_mask, i));
     acc = fma(rn, rn, acc);
   }
   grid_reduce_finish(gr, acc, 0.0);
@@ -1475,6 +1520,20 @@ int feat_copy(feat_handle_t h, void* dst, const void* src, size_t bytes)
   if(!h || (bytes && (!dst || !src))) return fail(FEAT_ERR_INVALID, "feat_copy: null argument");
   if(!bytes || dst == src) return FEAT_OK;
   return check_hip(hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, h->stream), "hipMemcpyAsync");
+}
+
+int feat_stream_read_f64(feat_handle_t h, const double* x, uint64_t n, double* sink)
+{
+  if(!h || (n && !x) || !sink) return fail(FEAT_ERR_INVALID, "feat_stream_read_f64: null argument");
+  if(n & 1) return fail(FEAT_ERR_INVALID, "feat_stream_read_f64: n must be even");
+  if((reinterpret_cast<uintptr_t>(x) & 15) != 0) return fail(FEAT_ERR_INVALID, "feat_stream_read_f64: x not 16-byte aligned");
+  if(!n) return FEAT_OK;
+  // 4096 workgroups (16 per CU) measured as the fastest read shape on 128 MiB - 1 GiB
+  const uint64_t n2 = n / 2;
+  const uint64_t blocks = (n2 + kThreads - 1) / kThreads;
+  hipLaunchKernelGGL(k_stream_read, dim3(blocks < 4096 ? blocks : 4096), dim3(kThreads), 0, h->stream,
+                     reinterpret_cast<const feat_d2*>(x), n2, sink);
+  return check_launch("feat_stream_read_f64");
 }
 
 int feat_set_memory_f64(feat_handle_t h, double* p, double v, uint64_t n)
@@ -2248,23 +2307,27 @@ bool mat_ok(const M* A)
 
 template<typename DT, typename M>
 int defect_impl(feat_handle_t h, const M* A, DT* d, const DT* rhs, const DT* x, const uint32_t* mask,
-                double* nrm2, const feat_reduce_ws_t* ws, const char* name)
+                double* nrm2, const feat_reduce_ws_t* ws, const char* name, FirstIter<DT> first = {nullptr, nullptr, DT(0)})
 {
   if(!h || !mat_ok(A) || (A->rows && (!d || !rhs || !x))) return fail(FEAT_ERR_INVALID, name);
   if(d == x) return fail(FEAT_ERR_ALIAS, name);
+  if(first.x1 && ((A->rows && !first.inv) || first.x1 == d || first.x1 == x || first.x1 == rhs))
+    return fail(FEAT_ERR_INVALID, name);
   if(need_ws(ws, nrm2)) return fail(FEAT_ERR_INVALID, "reduction workspace missing");
-  EpiDefect<DT> e{d, rhs, mask, nrm2 != nullptr};
+  EpiDefect<DT> e{d, rhs, mask, nrm2 != nullptr, first};
   return launch_mat<DT>(h->stream, A, x, e, ws_reduce(ws, nrm2, nullptr), name);
 }
 
 template<typename DT, typename M>
 int masked_impl(feat_handle_t h, const M* A, DT* r, const DT* x, const uint32_t* mask, double* nrm2,
-                const DT* dotw, double* dot, const feat_reduce_ws_t* ws, const char* name)
+                const DT* dotw, double* dot, const feat_reduce_ws_t* ws, const char* name,
+                FirstIter<DT> first = {nullptr, nullptr, DT(0)})
 {
   if(!h || !mat_ok(A) || (A->rows && !r) || (A->cols && !x) || (dot && !dotw)) return fail(FEAT_ERR_INVALID, name);
   if(r == x) return fail(FEAT_ERR_ALIAS, name);
+  if(first.x1 && ((A->rows && !first.inv) || first.x1 == r || first.x1 == x)) return fail(FEAT_ERR_INVALID, name);
   if(need_ws(ws, nrm2, dot)) return fail(FEAT_ERR_INVALID, "reduction workspace missing");
-  EpiMasked<DT> e{r, mask, dotw, nrm2 != nullptr, dot != nullptr};
+  EpiMasked<DT> e{r, mask, dotw, nrm2 != nullptr, dot != nullptr, first};
   return launch_mat<DT>(h->stream, A, x, e, ws_reduce(ws, nrm2, dot), name);
 }
 
@@ -2280,11 +2343,14 @@ int prol_impl(feat_handle_t h, const M* P, DT* sol, const DT* xc, const uint32_t
 
 // iterate k (1-based) of the smoother lives in buf[(steps - k) & 1], so the last
 // stored iterate lands in x
+// x1_ready: the first iterate (without the skip rule) is already in B(1), written by the
+// kernel that produced rhs (FirstIter); valid for steps >= 2, where step 1 applies the rule
 template<typename DT, typename M>
 int smooth_impl(feat_handle_t h, const M* A, const DT* inv, const uint32_t* mask, int steps, DT omega,
                 const DT* rhs, const double* rhs_nrm2, DT* x, DT* tmp, DT* d_out, DT* acc, const DT* dotw,
-                double* dot, const feat_reduce_ws_t* ws, const char* name)
+                double* dot, const feat_reduce_ws_t* ws, const char* name, bool x1_ready = false)
 {
+  if(x1_ready && steps < 2) return fail(FEAT_ERR_INVALID, "rj_smooth: a first iterate from the producer needs steps >= 2");
   if(!h || !mat_ok(A) || A->rows != A->cols || steps < 1) return fail(FEAT_ERR_INVALID, name);
   const uint64_t n = A->rows;
   if(n && (!inv || !rhs || !tmp || !x)) return fail(FEAT_ERR_INVALID, name);
@@ -2298,6 +2364,7 @@ int smooth_impl(feat_handle_t h, const M* A, const DT* inv, const uint32_t* mask
   const int nsp = d_out ? steps : steps - 1;
   GridReduce none{nullptr, nullptr, nullptr, nullptr};
   const bool acc_first = (acc != nullptr && steps == 1);
+  if(!x1_ready)
   {
     const int grid = grid_for(n, 4);
     GridReduce g = reduce_for_grid(acc_first ? ws_reduce(ws, nullptr, dot) : none, uint64_t(grid));
@@ -2464,6 +2531,24 @@ int feat_apply_masked_f64(feat_handle_t h, const feat_matrix_f64_t* A, double* r
   return masked_impl<double>(h, A, r, x, mask, nrm2, dotw, dot, ws, "feat_apply_masked_f64");
 }
 
+int feat_defect_first_f64(feat_handle_t h, const feat_matrix_f64_t* A, double* d, const double* rhs, const double* x,
+                          const uint32_t* mask, double* nrm2, const double* inv_diag, double omega, double* x1,
+                          const feat_reduce_ws_t* ws)
+{
+  if(!x1) return fail(FEAT_ERR_INVALID, "feat_defect_first_f64: null x1");
+  return defect_impl<double>(h, A, d, rhs, x, mask, nrm2, ws, "feat_defect_first_f64",
+                             FirstIter<double>{inv_diag, x1, omega});
+}
+
+int feat_apply_masked_first_f64(feat_handle_t h, const feat_matrix_f64_t* A, double* r, const double* x,
+                                const uint32_t* mask, double* nrm2, const double* inv_diag, double omega, double* x1,
+                                const feat_reduce_ws_t* ws)
+{
+  if(!x1) return fail(FEAT_ERR_INVALID, "feat_apply_masked_first_f64: null x1");
+  return masked_impl<double>(h, A, r, x, mask, nrm2, (const double*)nullptr, nullptr, ws, "feat_apply_masked_first_f64",
+                             FirstIter<double>{inv_diag, x1, omega});
+}
+
 int feat_prol_add_f64(feat_handle_t h, const feat_matrix_f64_t* P, double* sol, const double* xc, const uint32_t* mask)
 {
   return prol_impl<double>(h, P, sol, xc, mask, "feat_prol_add_f64");
@@ -2475,6 +2560,15 @@ int feat_rj_smooth_f64(feat_handle_t h, const feat_matrix_f64_t* A, const double
 {
   return smooth_impl<double>(h, A, inv_diag, mask, steps, omega, rhs, rhs_nrm2, x, tmp, d_out, acc, dotw, dot, ws,
                              "feat_rj_smooth_f64");
+}
+
+int feat_rj_smooth_from1_f64(feat_handle_t h, const feat_matrix_f64_t* A, const double* inv_diag,
+                             const uint32_t* mask, int steps, double omega, const double* rhs, const double* rhs_nrm2,
+                             double* x, double* tmp, double* d_out, double* acc, const double* dotw, double* dot,
+                             const feat_reduce_ws_t* ws)
+{
+  return smooth_impl<double>(h, A, inv_diag, mask, steps, omega, rhs, rhs_nrm2, x, tmp, d_out, acc, dotw, dot, ws,
+                             "feat_rj_smooth_from1_f64", true);
 }
 
 int feat_dense_gemv_f64(feat_handle_t h, int n, const double* ainv, const double* b, double* x, const uint32_t* mask)
@@ -2494,9 +2588,28 @@ int feat_pcg_update_f64(feat_handle_t h, double* x, double* r, const double* p, 
   if(need_ws(ws, rnrm2)) return fail(FEAT_ERR_INVALID, "reduction workspace missing");
   const int grid = grid_for(n, 4);
   const GridReduce gd = reduce_for_grid(ws_reduce(ws, rnrm2, nullptr), uint64_t(grid));
-  hipLaunchKernelGGL(k_pcg_update, dim3(grid), dim3(kThreads), 0, h->stream, x, r, p, q, gamma, qp, n, gd);
+  hipLaunchKernelGGL(k_pcg_update, dim3(grid), dim3(kThreads), 0, h->stream, x, r, p, q, gamma, qp, n, gd,
+                     FirstIter<double>{nullptr, nullptr, 0.0}, (const uint32_t*)nullptr);
   finish_reduce(h->stream, gd, uint64_t(grid));
   return check_launch("feat_pcg_update_f64");
+}
+
+int feat_pcg_update_first_f64(feat_handle_t h, double* x, double* r, const double* p, const double* q,
+                              const double* gamma, const double* qp, double* rnrm2, uint64_t n,
+                              const double* inv_diag, const uint32_t* mask, double omega, double* x1,
+                              const feat_reduce_ws_t* ws)
+{
+  if(!h || !gamma || !qp || !rnrm2 || (n && (!x || !r || !p || !q || !inv_diag || !x1)))
+    return fail(FEAT_ERR_INVALID, "feat_pcg_update_first_f64: null argument");
+  // x1 may take q's place (each element of q is read before the element of x1 is written)
+  if(x1 == x || x1 == r || x1 == p) return fail(FEAT_ERR_ALIAS, "feat_pcg_update_first_f64: x1 aliases x, r or p");
+  if(need_ws(ws, rnrm2)) return fail(FEAT_ERR_INVALID, "reduction workspace missing");
+  const int grid = grid_for(n, 4);
+  const GridReduce gd = reduce_for_grid(ws_reduce(ws, rnrm2, nullptr), uint64_t(grid));
+  hipLaunchKernelGGL(k_pcg_update, dim3(grid), dim3(kThreads), 0, h->stream, x, r, p, q, gamma, qp, n, gd,
+                     FirstIter<double>{inv_diag, x1, omega}, mask);
+  finish_reduce(h->stream, gd, uint64_t(grid));
+  return check_launch("feat_pcg_update_first_f64");
 }
 
 int feat_pcg_direction_f64(feat_handle_t h, double* p, const double* z, const double* gamma_new, double* gamma_old, uint64_t n)

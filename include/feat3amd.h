@@ -76,6 +76,10 @@ int feat_host_malloc(void** p, size_t bytes); /* pinned host memory */
 int feat_host_free(void* p);
 /* MemoryPool::copy / set_memory (util/memory_pool.hpp:181-251) */
 int feat_copy(feat_handle_t h, void* dst, const void* src, size_t bytes);      /* any direction */
+/* Bandwidth probe (no reference counterpart): reads the n doubles of x once with 16-byte
+ * nontemporal loads and writes nothing (sink is a device scalar that is never written in
+ * practice).  n even, x 16-byte aligned.  The measured read ceiling for roofline fractions. */
+int feat_stream_read_f64(feat_handle_t h, const double* x, uint64_t n, double* sink);
 int feat_set_memory_f64(feat_handle_t h, double* p, double v, uint64_t n);
 int feat_set_memory_f32(feat_handle_t h, float* p, float v, uint64_t n);
 /* MemoryPool::convert (util/memory_pool.hpp:231-251, util/cuda_util.cu:226-244) */
@@ -338,6 +342,35 @@ int feat_rj_smooth_f64(feat_handle_t h, const feat_matrix_f64_t* A, const double
                        const double* rhs_nrm2, double* x, double* tmp, double* d_out,
                        double* acc, const double* dotw, double* dot,
                        const feat_reduce_ws_t* ws);
+
+/* The smoother's first iterate formed where its right-hand side is produced (multigrid.hpp:
+ * 1753 formats x, richardson.hpp:205-221 then takes the first step): the producer also
+ * writes x1 = filter_cor(omega * (inv_diag ∘ v)) of its output v, without the skip rule,
+ * and feat_rj_smooth_from1_f64 (same arguments as feat_rj_smooth_f64, steps >= 2) starts
+ * from x1 in its first ping-pong buffer -- tmp for even steps, x for odd -- and applies the
+ * skip rule in its next step.  Same values as the separate first step; no reference
+ * counterpart (one vector pass less per smoother call).
+ *   feat_defect_first_f64:       feat_defect_f64 + x1 of the post-smoother (rhs = d)
+ *   feat_apply_masked_first_f64: feat_apply_masked_f64 (no dot) + x1 of the pre-smoother
+ *   feat_pcg_update_first_f64:   feat_pcg_update_f64 + x1 of the preconditioner's
+ *                                pre-smoother on r, filtered through `mask`; x1 may
+ *                                be q (not x, r or p) */
+int feat_defect_first_f64(feat_handle_t h, const feat_matrix_f64_t* A, double* d, const double* rhs,
+                          const double* x, const uint32_t* mask, double* nrm2, const double* inv_diag,
+                          double omega, double* x1, const feat_reduce_ws_t* ws);
+int feat_apply_masked_first_f64(feat_handle_t h, const feat_matrix_f64_t* A, double* r,
+                                const double* x, const uint32_t* mask, double* nrm2,
+                                const double* inv_diag, double omega, double* x1,
+                                const feat_reduce_ws_t* ws);
+int feat_rj_smooth_from1_f64(feat_handle_t h, const feat_matrix_f64_t* A, const double* inv_diag,
+                             const uint32_t* mask, int steps, double omega, const double* rhs,
+                             const double* rhs_nrm2, double* x, double* tmp, double* d_out,
+                             double* acc, const double* dotw, double* dot,
+                             const feat_reduce_ws_t* ws);
+int feat_pcg_update_first_f64(feat_handle_t h, double* x, double* r, const double* p, const double* q,
+                              const double* gamma, const double* qp, double* rnrm2, uint64_t n,
+                              const double* inv_diag, const uint32_t* mask, double omega, double* x1,
+                              const feat_reduce_ws_t* ws);
 
 /* Dense coarse solve x = filter_cor(Ainv * b) with a row-major n x n inverse. */
 int feat_dense_gemv_f64(feat_handle_t h, int n, const double* ainv, const double* b, double* x,

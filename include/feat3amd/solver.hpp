@@ -479,6 +479,22 @@ namespace feat3amd
       virtual ~FusedPrecond() {}
       virtual bool fusable() const = 0;
       virtual void apply_fused(Vector_& cor, const Vector_& def, const double* def_nrm2, double* gamma) = 0;
+      /// The first iterate of the preconditioner's first smoother, formed by the kernel
+      /// that produces def (feat_pcg_update_first_f64): x1 = filter(omega * (inv ∘ def)).
+      struct FirstIterate
+      {
+        const double* inv = nullptr;
+        const uint32_t* mask = nullptr;
+        double omega = 0.0;
+        double* x1 = nullptr;
+      };
+      /// Where the first iterate goes when cor is the output; false: not supported.
+      virtual bool first_iterate(const Vector_& /*cor*/, FirstIterate& /*fi*/) { return false; }
+      /// apply_fused after the producer of def has written the first iterate.
+      virtual void apply_fused_from1(Vector_& cor, const Vector_& def, const double* def_nrm2, double* gamma)
+      {
+        apply_fused(cor, def, def_nrm2, gamma);
+      }
     };
 
     /// MultiGrid (solver/multigrid.hpp:984-2000), V-cycle with fixed coarse-grid correction.
@@ -657,9 +673,55 @@ namespace feat3amd
         return Status::success;
       }
 
+      /// The pre-smoother's first ping-pong buffer on level i with output x (smooth_impl: the
+      /// first iterate lives in tmp for an even number of steps, in x for an odd one); null
+      /// where the level's smoother does not start from a producer-formed iterate (one step,
+      /// or the level runs in the fused tail or is the coarsest).
+      double* _x1_of(Index i, double* x, bool pre)
+      {
+        auto& l = _h->levels[i];
+        auto* sm = static_cast<RichType*>((pre ? l.pre : l.post).get());
+        const int steps = int(sm->get_max_iter());
+        if(steps < 2) return nullptr;
+        return (steps & 1) ? x : (double*)l.tmp.elements();
+      }
+
+    public:
+      bool first_iterate(const VectorType& cor, typename FusedPrecond<VectorType>::FirstIterate& fi) override
+      {
+        if(!_fusable) return false;
+        if(!_tail_built) _build_tail();
+        const Index c = _h->size() - 1;
+        const Index t = _tail_t ? _tail_t : c;
+        if(t == 0) return false;   // level 0 runs in the fused tail or is the coarsest
+        double* x1 = _x1_of(0, (double*)cor.elements(), true);
+        if(!x1) return false;
+        auto& l = _h->levels[0];
+        auto* pre = static_cast<RichType*>(l.pre.get());
+        fi.inv = (const double*)pre->fusable_jacobi()->inv_diag().elements();
+        fi.mask = l.filter->mask();
+        fi.omega = pre->omega();
+        fi.x1 = x1;
+        return true;
+      }
+      void apply_fused_from1(VectorType& cor, const VectorType& def, const double* def_nrm2, double* gamma) override
+      {
+        _apply_fused(cor, def, def_nrm2, gamma, true);
+      }
+
       /// Fused V-cycle: cor = MG(def) where *def_nrm2 (device) holds |def|^2;
       /// when gamma != nullptr, *gamma = cor . def (device).
       void apply_fused(VectorType& cor, const VectorType& def, const double* def_nrm2, double* gamma) override
+      {
+        _apply_fused(cor, def, def_nrm2, gamma, false);
+      }
+
+    private:
+      /// top_x1: level 0's pre-smoother starts from the iterate the producer of def formed.
+      /// Every other smoother above the fused tail starts from the iterate formed by the
+      /// kernel that produced its right-hand side: the restriction for pre-smoothing, the
+      /// defect after the prolongation for post-smoothing.
+      void _apply_fused(VectorType& cor, const VectorType& def, const double* def_nrm2, double* gamma, bool top_x1)
       {
         const Index L = _h->size();
         const Index c = L - 1;
@@ -675,21 +737,36 @@ namespace feat3amd
         };
         auto nrm_of = [&](Index i) -> const double* { return i == 0 ? def_nrm2 : _fw.scalar(2 + i); };
         // restriction leg (multigrid.hpp:1713-1840)
+        bool x1_ready = top_x1;
         for(Index i = 0; i < t; ++i)
         {
           auto& l = _h->levels[i];
           auto* pre = static_cast<RichType*>(l.pre.get());
           auto* jac = pre->fusable_jacobi();
           const feat_matrix_f64_t& A = l.matrix->exec();
-          check(feat_rj_smooth_f64(hd, &A, (const double*)jac->inv_diag().elements(), l.filter->mask(),
-                                   int(pre->get_max_iter()), pre->omega(), rhs_of(i), nrm_of(i), sol_of(i),
-                                   (double*)l.tmp.elements(), (double*)l.def.elements(), nullptr, nullptr, nullptr, ws),
+          auto smooth = x1_ready ? feat_rj_smooth_from1_f64 : feat_rj_smooth_f64;
+          check(smooth(hd, &A, (const double*)jac->inv_diag().elements(), l.filter->mask(), int(pre->get_max_iter()),
+                       pre->omega(), rhs_of(i), nrm_of(i), sol_of(i), (double*)l.tmp.elements(),
+                       (double*)l.def.elements(), nullptr, nullptr, nullptr, ws),
                 "MultiGrid pre-smoothing");
           const feat_matrix_f64_t& R = l.transfer->get_mat_rest().exec();
           auto& lc = _h->levels[i + 1];
-          check(feat_apply_masked_f64(hd, &R, (double*)lc.rhs.elements(), (const double*)l.def.elements(),
-                                      lc.filter->mask(), _fw.scalar(2 + i + 1), nullptr, nullptr, ws),
-                "MultiGrid restriction");
+          // the next level's pre-smoother starts from the iterate the restriction forms
+          double* x1 = (i + 1 < t) ? _x1_of(i + 1, sol_of(i + 1), true) : nullptr;
+          if(x1)
+          {
+            auto* pc = static_cast<RichType*>(lc.pre.get());
+            check(feat_apply_masked_first_f64(hd, &R, (double*)lc.rhs.elements(), (const double*)l.def.elements(),
+                                              lc.filter->mask(), _fw.scalar(2 + i + 1),
+                                              (const double*)pc->fusable_jacobi()->inv_diag().elements(), pc->omega(),
+                                              x1, ws),
+                  "MultiGrid restriction");
+          }
+          else
+            check(feat_apply_masked_f64(hd, &R, (double*)lc.rhs.elements(), (const double*)l.def.elements(),
+                                        lc.filter->mask(), _fw.scalar(2 + i + 1), nullptr, nullptr, ws),
+                  "MultiGrid restriction");
+          x1_ready = x1 != nullptr;
         }
         if(_tail_t)
           check(feat_mg_tail_f64(hd, &_tail, rhs_of(t), nrm_of(t), sol_of(t)), "MultiGrid fused coarse levels");
@@ -746,13 +823,21 @@ namespace feat3amd
           check(feat_prol_add_f64(hd, &P, sol_of(k), sol_of(k + 1), l.filter->mask()), "MultiGrid prolongation");
           const feat_matrix_f64_t& A = l.matrix->exec();
           double* dn = _fw.scalar(2 + L + k);
-          check(feat_defect_f64(hd, &A, (double*)l.def.elements(), rhs_of(k), sol_of(k), l.filter->mask(), dn, ws),
-                "MultiGrid defect");
+          // the post-smoother starts from the iterate the defect kernel forms
+          double* x1 = _x1_of(k, (double*)l.cor.elements(), false);
+          if(x1)
+            check(feat_defect_first_f64(hd, &A, (double*)l.def.elements(), rhs_of(k), sol_of(k), l.filter->mask(), dn,
+                                        (const double*)jac->inv_diag().elements(), post->omega(), x1, ws),
+                  "MultiGrid defect");
+          else
+            check(feat_defect_f64(hd, &A, (double*)l.def.elements(), rhs_of(k), sol_of(k), l.filter->mask(), dn, ws),
+                  "MultiGrid defect");
           const bool top = (k == 0);
-          check(feat_rj_smooth_f64(hd, &A, (const double*)jac->inv_diag().elements(), l.filter->mask(),
-                                   int(post->get_max_iter()), post->omega(), (const double*)l.def.elements(), dn,
-                                   (double*)l.cor.elements(), (double*)l.tmp.elements(), nullptr, sol_of(k),
-                                   top && gamma ? (const double*)def.elements() : nullptr, top ? gamma : nullptr, ws),
+          auto smooth = x1 ? feat_rj_smooth_from1_f64 : feat_rj_smooth_f64;
+          check(smooth(hd, &A, (const double*)jac->inv_diag().elements(), l.filter->mask(), int(post->get_max_iter()),
+                       post->omega(), (const double*)l.def.elements(), dn, (double*)l.cor.elements(),
+                       (double*)l.tmp.elements(), nullptr, sol_of(k),
+                       top && gamma ? (const double*)def.elements() : nullptr, top ? gamma : nullptr, ws),
                 "MultiGrid post-smoothing");
         }
       }
@@ -972,9 +1057,15 @@ namespace feat3amd
         double* p = (double*)_vec_p.elements();
         double* t = (double*)_vec_t.elements();
         double* xs = (double*)x.elements();
+        typename MGType::FirstIterate fi;
+        const bool from1 = mg.first_iterate(_vec_t, fi);
         auto body_tail = [&]() {
           check(feat_apply_masked_f64(g.h, &A, t, p, _filter.mask(), nullptr, p, s_qp, ws), "PCG A*p");
-          check(feat_pcg_update_f64(g.h, xs, r, p, t, s_g, s_qp, s_rr, n, ws), "PCG update");
+          if(from1)
+            check(feat_pcg_update_first_f64(g.h, xs, r, p, t, s_g, s_qp, s_rr, n, fi.inv, fi.mask, fi.omega, fi.x1, ws),
+                  "PCG update");
+          else
+            check(feat_pcg_update_f64(g.h, xs, r, p, t, s_g, s_qp, s_rr, n, ws), "PCG update");
           check(feat_copy_scalar_to_host(g.h, s_rr, g.host), "PCG read-back");
         };
         auto record = [&](feat_graph_t& slot, const std::function<void()>& body) -> bool {
@@ -1011,7 +1102,10 @@ namespace feat3amd
             mg.apply_fused(_vec_p, _vec_r, s_rr, s_g);
             body_tail();
           }) && record(g.loop, [&]() {
-            mg.apply_fused(_vec_t, _vec_r, s_rr, s_g2);
+            if(from1)
+              mg.apply_fused_from1(_vec_t, _vec_r, s_rr, s_g2);
+            else
+              mg.apply_fused(_vec_t, _vec_r, s_rr, s_g2);
             check(feat_pcg_direction_f64(g.h, p, t, s_g2, s_g, n), "PCG direction");
             body_tail();
           });
@@ -1061,16 +1155,27 @@ namespace feat3amd
         Status st = this->_set_initial_defect(std::sqrt(rr));
         if(st != Status::progress) return st;
         mg.apply_fused(_vec_p, _vec_r, s_rr, s_g);
+        // the update forms the first iterate of the next preconditioner call's first smoother
+        typename MGType::FirstIterate fi;
+        const bool from1 = mg.first_iterate(_vec_t, fi);
         while(st == Status::progress)
         {
           check(feat_apply_masked_f64(hd, &A, t, p, _filter.mask(), nullptr, p, s_qp, ws), "PCG A*p");
-          check(feat_pcg_update_f64(hd, (double*)x.elements(), r, p, t, s_g, s_qp, s_rr, n, ws), "PCG update");
+          if(from1)
+            check(feat_pcg_update_first_f64(hd, (double*)x.elements(), r, p, t, s_g, s_qp, s_rr, n, fi.inv, fi.mask,
+                                            fi.omega, fi.x1, ws),
+                  "PCG update");
+          else
+            check(feat_pcg_update_f64(hd, (double*)x.elements(), r, p, t, s_g, s_qp, s_rr, n, ws), "PCG update");
           st = this->_set_new_defect([&]() {
             check(feat_read_scalar_f64(hd, s_rr, &rr), "PCG read");
             return std::sqrt(rr);
           });
           if(st != Status::progress) return st;
-          mg.apply_fused(_vec_t, _vec_r, s_rr, s_g2);
+          if(from1)
+            mg.apply_fused_from1(_vec_t, _vec_r, s_rr, s_g2);
+          else
+            mg.apply_fused(_vec_t, _vec_r, s_rr, s_g2);
           check(feat_pcg_direction_f64(hd, p, t, s_g2, s_g, n), "PCG direction");
         }
         return st;

@@ -692,3 +692,107 @@ def test_numeric_reinit_rebuilds_fused_levels(feat, torch_gpu, precision):
         f.close()
         assert r_f["iters"] == r_re["iters"]
         assert np.array_equal(x_f.view(np.uint64), x_re.view(np.uint64))
+
+
+@pytest.mark.parametrize("steps", [2, 3, 4])
+@pytest.mark.parametrize("rhs_kind", ["random", "zero", "nan"])
+def test_first_iterate_from_producer_bitwise(feat, handle, torch_gpu, steps, rhs_kind):
+    """A smoother that starts from the first iterate its right-hand side's producer formed
+    (feat_defect_first_f64 / feat_apply_masked_first_f64 / feat_pcg_update_first_f64, then
+    feat_rj_smooth_from1_f64) gives the bits of the separate first step, including the skip
+    rule on a zero or non-finite right-hand side (applied by the smoother's next step)."""
+    torch = torch_gpu
+    k, _ = feat.libs()
+    P = feat.ptr
+    keep, ws = feat.reduce_ws(torch, "cuda")
+    q, A = q1_level(feat, 3, 4)
+    n = q["n"]
+    rng = np.random.default_rng(40 + steps)
+    keepA, desc = exec_desc(feat, handle, torch, A, "sell")
+    v, c, r = keepA[:3]
+    m = mask_of(torch, feat, handle, n, q["filt"])
+    omega = 0.8
+    inv = torch.empty(n, dtype=torch.float64, device="cuda")
+    feat.check(k.feat_csr_extract_diag_f64_u64(handle.h, P(inv), P(v), P(c), P(r), n), "diag")
+    feat.check(k.feat_component_invert_f64(handle.h, P(inv), P(inv), 0.5, n), "inv")
+    x0 = rng.standard_normal(n)
+    rhs = rng.standard_normal(n)
+    if rhs_kind == "zero":
+        # x = 0, rhs = 0: the defect and the restricted vector are exactly zero
+        x0[:] = 0.0
+        rhs[:] = 0.0
+    elif rhs_kind == "nan":
+        rhs[7] = np.nan
+    xd, rd = dev(torch, x0), dev(torch, rhs)
+
+    def vecs(k_):
+        return [torch.zeros(n, dtype=torch.float64, device="cuda") for _ in range(k_)]
+
+    def b1(x, tmp):
+        return x if steps & 1 else tmp
+
+    # post-smoothing: defect, then the smoother accumulating into sol
+    sol0 = rng.standard_normal(n)
+    outs = []
+    for first in (False, True):
+        d, cor, tmp = vecs(3)
+        sol = dev(torch, sol0)
+        nrm = torch.zeros(1, dtype=torch.float64, device="cuda")
+        if first:
+            feat.check(k.feat_defect_first_f64(handle.h, desc, P(d), P(rd), P(xd), P(m), P(nrm), P(inv), omega,
+                                               P(b1(cor, tmp)), ws), "defect first")
+            fn = k.feat_rj_smooth_from1_f64
+        else:
+            feat.check(k.feat_defect_f64(handle.h, desc, P(d), P(rd), P(xd), P(m), P(nrm), ws), "defect")
+            fn = k.feat_rj_smooth_f64
+        feat.check(fn(handle.h, desc, P(inv), P(m), steps, omega, P(d), P(nrm), P(cor), P(tmp), None, P(sol), None, None,
+                      ws), "smooth")
+        handle.sync()
+        outs.append((host(d), host(sol)))
+    for a, b in zip(*outs):
+        assert np.array_equal(np.isnan(a), np.isnan(b))
+        assert np.array_equal(a[~np.isnan(a)].view(np.uint64), b[~np.isnan(b)].view(np.uint64))
+    if rhs_kind == "zero":
+        assert np.array_equal(outs[1][1], sol0)   # skipped: sol unchanged
+    # pre-smoothing: restriction-like producer (masked apply), then the smoother with d_out
+    outs = []
+    for first in (False, True):
+        rr, x, tmp, dout = vecs(4)
+        nrm = torch.zeros(1, dtype=torch.float64, device="cuda")
+        src = rd if rhs_kind == "nan" else xd
+        if first:
+            feat.check(k.feat_apply_masked_first_f64(handle.h, desc, P(rr), P(src), P(m), P(nrm), P(inv), omega,
+                                                     P(b1(x, tmp)), ws), "masked first")
+            fn = k.feat_rj_smooth_from1_f64
+        else:
+            feat.check(k.feat_apply_masked_f64(handle.h, desc, P(rr), P(src), P(m), P(nrm), None, None, ws), "masked")
+            fn = k.feat_rj_smooth_f64
+        feat.check(fn(handle.h, desc, P(inv), P(m), steps, omega, P(rr), P(nrm), P(x), P(tmp), P(dout), None, None, None,
+                      ws), "smooth")
+        handle.sync()
+        outs.append((host(x), host(dout)))
+    for a, b in zip(*outs):
+        assert np.array_equal(np.isnan(a), np.isnan(b))
+        assert np.array_equal(a[~np.isnan(a)].view(np.uint64), b[~np.isnan(b)].view(np.uint64))
+    # PCG update: x, r as the plain update, x1 = filter(omega * (inv * r_new)) in one rounding each
+    p_, q_ = rng.standard_normal((2, n))
+    sc = torch.tensor([0.7, 1.3], dtype=torch.float64, device="cuda")
+    res = []
+    for first in (False, True):
+        xx, rr = dev(torch, x0), dev(torch, rhs)
+        x1 = torch.zeros(n, dtype=torch.float64, device="cuda")
+        nrm = torch.zeros(1, dtype=torch.float64, device="cuda")
+        if first:
+            feat.check(k.feat_pcg_update_first_f64(handle.h, P(xx), P(rr), P(dev(torch, p_)), P(dev(torch, q_)), P(sc),
+                                                   P(sc[1:]), P(nrm), n, P(inv), P(m), omega, P(x1), ws), "update first")
+        else:
+            feat.check(k.feat_pcg_update_f64(handle.h, P(xx), P(rr), P(dev(torch, p_)), P(dev(torch, q_)), P(sc),
+                                             P(sc[1:]), P(nrm), n, ws), "update")
+        handle.sync()
+        res.append((host(xx), host(rr), host(x1)))
+    assert np.array_equal(res[0][0], res[1][0], equal_nan=True)
+    assert np.array_equal(res[0][1], res[1][1], equal_nan=True)
+    cexp = host(inv) * res[1][1]
+    cexp[q["filt"].astype(np.int64)] = 0.0
+    assert np.array_equal(res[1][2], omega * cexp, equal_nan=True)
+
