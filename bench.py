@@ -29,15 +29,15 @@ sys.path.insert(0, ROOT)
 DIM, LF, LC = 3, 7, 1
 SEED = 28054777172512
 PEAK_HBM_GBS = 8000.0
-# HBM bytes per launch of the level-7 SpMV r = A*x kernels, from two hardware-counter
-# runs of `bench.py --spmv-only` (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch); FETCH_SIZE
-# is scaled by 2.0 and WRITE_SIZE by 1.0, the factors that make the same counters read
-# 1 GiB for a 1 GiB device-to-device copy (profiles/spmv_traffic_3d_q1_l7.md)
-SPMV_TRAFFIC_BYTES = {"jacobi_step": None,
-                      "sell64_diag": 38762.8 * 2048 + 16771.0 * 1024,
-                      "sell64_dict": 40574.2 * 2048 + 16771.0 * 1024,
-                      "sell64": 295808.6 * 2048 + 16771.0 * 1024,
-                      "csr_u64": 481390.4 * 2048 + 16771.0 * 1024}
+# HBM bytes per launch of the level-7 kernels, from two hardware-counter runs of
+# `bench.py --spmv-only` (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch); FETCH_SIZE is scaled
+# by 2.0 and WRITE_SIZE by 1.0, the factors that make the same counters read 1 GiB for a
+# 1 GiB device-to-device copy (profiles/spmv_traffic_3d_q1_l7.md)
+SPMV_TRAFFIC_BYTES = {"jacobi_step": 55483.9 * 2048 + 16771.0 * 1024,
+                      "sell64_diag": 38712.2 * 2048 + 16771.0 * 1024,
+                      "sell64_dict": 40575.7 * 2048 + 16771.0 * 1024,
+                      "sell64": 295808.9 * 2048 + 16771.0 * 1024,
+                      "csr_u64": 474623.3 * 2048 + 16771.0 * 1024}
 
 
 def load_feat():
@@ -385,8 +385,10 @@ def smoothers_leg(feat, torch, dev, args):
     b = feat.rhs_random(SEED, q["n"], -1.0, 1.0)
     b[q["filt"].astype(np.int64)] = 0.0
     bd = torch.from_numpy(b).to(dev)
-    for pre in ("jacobi", "ssor", "ilu"):
-        sv = feat.KrylovSolver("pcg", A, q["filt"], pre, omega_pre=1.0, tol_rel=1e-8, max_iter=1000)
+    # (ILU: omega_pre is the fill-in level p)
+    for pre, om, tag in (("jacobi", 1.0, "jacobi"), ("ssor", 1.0, "ssor"), ("ilu", 0.0, "ilu"), ("ilu", 1.0, "ilu1"),
+                         ("ilu", 2.0, "ilu2")):
+        sv = feat.KrylovSolver("pcg", A, q["filt"], pre, omega_pre=om, tol_rel=1e-8, max_iter=1000)
         xs = torch.zeros(q["n"], dtype=torch.float64, device=dev)
         sv.solve(xs, bd)
         xs.zero_()
@@ -394,9 +396,9 @@ def smoothers_leg(feat, torch, dev, args):
         t = time.perf_counter()
         r = sv.solve(xs, bd)
         torch.cuda.synchronize()
-        out[f"pcg_{pre}_3d_l5"] = {"s_per_solve": time.perf_counter() - t, "iters": r["iters"],
+        out[f"pcg_{tag}_3d_l5"] = {"s_per_solve": time.perf_counter() - t, "iters": r["iters"],
                                    "final_rel_defect": r["def_final"] / r["def_init"]}
-        log(f"pcg-{pre} 3D L5: {r['iters']} iterations, {out[f'pcg_{pre}_3d_l5']['s_per_solve'] * 1e3:.1f} ms")
+        log(f"pcg-{tag} 3D L5: {r['iters']} iterations, {out[f'pcg_{tag}_3d_l5']['s_per_solve'] * 1e3:.1f} ms")
         sv.close()
     out["what"] = (f"3D Q1 level {LF} matrix ({n} rows): one SOR / SSOR / ILU(0) application, level-set order = the "
                    "generic sequential sweeps bit for bit, multicolour = the CUDA backend's colour order; PCG on level 5")

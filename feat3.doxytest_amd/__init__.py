@@ -154,6 +154,7 @@ KERNEL_API = [
     ("feat_sor_done_symbolic", _i, [_vp]),
     ("feat_ilu_init_symbolic", _i, [_vp, C.POINTER(CsrF64), C.POINTER(_vp)]),
     ("feat_ilu_init_numeric", _i, [_vp, _vp, C.POINTER(CsrF64)]),
+    ("feat_ilu_init_symbolic_p", _i, [_vp, C.POINTER(CsrF64), _i, C.POINTER(_vp)]),
     ("feat_ilu_apply", _i, [_vp, _vp, _vp, _vp]),
     ("feat_ilu_info", _i, [_vp, C.POINTER(_u64), C.POINTER(_u64), C.POINTER(_i), C.POINTER(_i)]),
     ("feat_ilu_factors", _i, [_vp, _vp, _vp, _vp, _vp]),
@@ -578,14 +579,15 @@ class SorPlan:
 
 
 class IluPlan:
-    """ILU(0) of a device CSR matrix (feat_ilu_*), factorised at construction."""
+    """ILU(p) of a device CSR matrix (feat_ilu_*), factorised at construction."""
 
-    def __init__(self, handle, desc):
+    def __init__(self, handle, desc, p=0):
         self.k, _ = libs()
         self.h = handle
         self.desc = desc
         self.s = C.c_void_p()
-        check(self.k.feat_ilu_init_symbolic(handle.h, C.byref(desc), C.byref(self.s)), "feat_ilu_init_symbolic")
+        check(self.k.feat_ilu_init_symbolic_p(handle.h, C.byref(desc), int(p), C.byref(self.s)),
+              "feat_ilu_init_symbolic_p")
         self.init_numeric()
         nl, nu, ll, lu = _u64(), _u64(), _i(), _i()
         check(self.k.feat_ilu_info(self.s, C.byref(nl), C.byref(nu), C.byref(ll), C.byref(lu)), "feat_ilu_info")

@@ -546,13 +546,102 @@ void level_perm(const std::vector<uint32_t>& lev, uint32_t nlev, std::vector<uin
   for(uint64_t i = 0; i < n; ++i) perm[pos[lev[i]]++] = uint32_t(i);
 }
 
+// ILUCoreSymbolic::factorize_symbolic (ilu_precond.hpp:191-262): the level-p pattern from
+// the level-0 split (rl, cl, ru, cu).  Row by row, every entry (i, cj) of the growing row of
+// L combines with the entries (cj, ck) of the final row cj of U into (i, ck) of level
+// lev(i,cj) + lev(cj,ck) + 1; entries of level <= p are inserted at their sorted place (or
+// lower an existing entry's level), as the reference's linear insertion does.  The value
+// maps ml / mu are rebuilt: A's entry index where the pattern has it, kNoEntry at fill-in.
+void ilu_level_p(uint64_t n, int p, const std::vector<uint64_t>& rp, const std::vector<uint64_t>& colv,
+                 std::vector<uint64_t>& rl, std::vector<uint32_t>& cl, std::vector<uint64_t>& ml,
+                 std::vector<uint64_t>& ru, std::vector<uint32_t>& cu, std::vector<uint64_t>& mu)
+{
+  std::vector<uint64_t> nrl{0}, nru{0};
+  std::vector<uint32_t> ncl, ncu;
+  std::vector<int> lvl_l, lvl_u;
+  auto insert = [](std::vector<uint32_t>& idx, std::vector<int>& lvl, uint64_t i, uint32_t j, int l) -> uint64_t {
+    const uint64_t m = idx.size();
+    for(; i < m; ++i)
+    {
+      if(idx[i] == j)
+      {
+        if(l < lvl[i]) lvl[i] = l;
+        return i + 1;
+      }
+      if(j < idx[i]) break;
+    }
+    idx.insert(idx.begin() + std::ptrdiff_t(i), j);
+    lvl.insert(lvl.begin() + std::ptrdiff_t(i), l);
+    return i + 1;
+  };
+  for(uint64_t i = 0; i < n; ++i)
+  {
+    for(uint64_t j = rl[i]; j < rl[i + 1]; ++j)
+    {
+      ncl.push_back(cl[j]);
+      lvl_l.push_back(0);
+    }
+    for(uint64_t j = ru[i]; j < ru[i + 1]; ++j)
+    {
+      ncu.push_back(cu[j]);
+      lvl_u.push_back(0);
+    }
+    for(uint64_t j = nrl[i]; j < ncl.size(); ++j)
+    {
+      const uint32_t cj = ncl[j];
+      const int lj = lvl_l[j];
+      uint64_t olj = j, ouj = nru[i];
+      for(uint64_t k = nru[cj]; k < nru[cj + 1]; ++k)
+      {
+        const uint32_t ck = ncu[k];
+        const int lev = lj + lvl_u[k] + 1;
+        if(lev > p) continue;
+        if(ck < i)
+          olj = insert(ncl, lvl_l, olj, ck, lev);
+        else if(ck > i)
+          ouj = insert(ncu, lvl_u, ouj, ck, lev);
+      }
+    }
+    nrl.push_back(ncl.size());
+    nru.push_back(ncu.size());
+  }
+  // value maps: merge each pattern row with A's (sorted) row
+  std::vector<uint64_t> nml(ncl.size(), kNoEntry), nmu(ncu.size(), kNoEntry);
+  for(uint64_t i = 0; i < n; ++i)
+  {
+    uint64_t ka = rp[i];
+    for(uint64_t j = nrl[i]; j < nrl[i + 1]; ++j)
+    {
+      while(ka < rp[i + 1] && colv[ka] < ncl[j]) ++ka;
+      if(ka < rp[i + 1] && colv[ka] == ncl[j]) nml[j] = ka;
+    }
+    ka = rp[i];
+    for(uint64_t j = nru[i]; j < nru[i + 1]; ++j)
+    {
+      while(ka < rp[i + 1] && colv[ka] < ncu[j]) ++ka;
+      if(ka < rp[i + 1] && colv[ka] == ncu[j]) nmu[j] = ka;
+    }
+  }
+  rl.swap(nrl);
+  cl.swap(ncl);
+  ml.swap(nml);
+  ru.swap(nru);
+  cu.swap(ncu);
+  mu.swap(nmu);
+}
+
 } // namespace
 
 extern "C" {
 
 int feat_ilu_init_symbolic(feat_handle_t h, const feat_csr_f64_t* A, feat_ilu_t* out)
 {
-  if(!h || !A || !out) return fail(FEAT_ERR_INVALID, "feat_ilu_init_symbolic: null argument");
+  return feat_ilu_init_symbolic_p(h, A, 0, out);
+}
+
+int feat_ilu_init_symbolic_p(feat_handle_t h, const feat_csr_f64_t* A, int p, feat_ilu_t* out)
+{
+  if(!h || !A || !out || p < 0) return fail(FEAT_ERR_INVALID, "feat_ilu_init_symbolic: null argument or p < 0");
   if(A->rows != A->cols) return fail(FEAT_ERR_INVALID, "feat_ilu_init_symbolic: matrix is not square");
   if(A->rows >= (uint64_t(1) << 32)) return fail(FEAT_ERR_INVALID, "feat_ilu_init_symbolic: more than 2^32-1 rows");
   const uint64_t n = A->rows, nnz = A->nnz;
@@ -595,6 +684,8 @@ int feat_ilu_init_symbolic(feat_handle_t h, const feat_csr_f64_t* A, feat_ilu_t*
     rl.push_back(cl.size());
     ru.push_back(cu.size());
   }
+  if(p > 0)
+    ilu_level_p(n, p, rp, colv, rl, cl, ml, ru, cu, mu);
   // level sets of L (factorisation and forward solve) and of U (backward solve)
   std::vector<uint32_t> lf(n, 0), lb(n, 0);
   uint32_t nf = n ? 1 : 0, nb = n ? 1 : 0;

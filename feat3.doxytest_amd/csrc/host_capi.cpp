@@ -509,7 +509,11 @@ int feat_krylov_create(feat_krylov_t* out, int solver, std::uint64_t n, const do
     case FEAT_PRECOND_JACOBI: s->pre = Solver::new_jacobi_precond(s->A, s->F, omega_pre); break;
     case FEAT_PRECOND_SOR: s->pre = Solver::new_sor_precond(s->A, s->F, omega_pre); break;
     case FEAT_PRECOND_SSOR: s->pre = Solver::new_ssor_precond(s->A, s->F, omega_pre); break;
-    case FEAT_PRECOND_ILU: s->pre = Solver::new_ilu_precond(s->A, s->F, 0); break;
+    case FEAT_PRECOND_ILU:
+      // ILUPrecond's fill-in level p travels in omega_pre (0 = ILU(0))
+      if(!(omega_pre >= 0.0) || omega_pre != double(int(omega_pre))) throw Error("feat_krylov_create: ILU level p must be a whole number >= 0");
+      s->pre = Solver::new_ilu_precond(s->A, s->F, int(omega_pre));
+      break;
     case FEAT_PRECOND_SOR_MULTICOLOR:
       s->pre = std::make_shared<Solver::SORPrecond<Csr, Filt>>(s->A, s->F, omega_pre, FEAT_SOR_MULTICOLOR);
       break;

@@ -632,15 +632,21 @@ int feat_sor_done_symbolic(feat_sor_t s);
  * per level, so every row keeps the sequential operation order: bit-identical to the generic
  * backend.  init_numeric copies the values of A and factorises; apply computes
  * y = (D+U)^{-1} (I+L)^{-1} x (y may alias x).  A: device CSR, sorted rows, every row with
- * its diagonal entry.  Fill-in levels p > 0 are not provided. */
+ * its diagonal entry.
+ * feat_ilu_init_symbolic_p: ILU(p), the level-p pattern of ILUCoreSymbolic::factorize_symbolic
+ * (ilu_precond.hpp:191-262, the same linear insertion), values of A copied onto it with zeros
+ * at fill-in (copy_data_csr), the same level-scheduled factorisation and solves on it;
+ * p = 0 is feat_ilu_init_symbolic. */
 typedef struct feat_ilu_s* feat_ilu_t;
 int feat_ilu_init_symbolic(feat_handle_t h, const feat_csr_f64_t* A, feat_ilu_t* out);
+int feat_ilu_init_symbolic_p(feat_handle_t h, const feat_csr_f64_t* A, int p, feat_ilu_t* out);
 int feat_ilu_init_numeric(feat_handle_t h, feat_ilu_t s, const feat_csr_f64_t* A);
 int feat_ilu_apply(feat_handle_t h, feat_ilu_t s, double* y, const double* x);
 /* entries of L and U, number of level sets of L and of U */
 int feat_ilu_info(feat_ilu_t s, uint64_t* nnz_l, uint64_t* nnz_u, int* levels_l, int* levels_u);
-/* copies the factors to host arrays (L and U in the CSR order of A's lower / upper entries,
- * the inverted diagonal); any pointer may be NULL */
+/* copies the factors to host arrays (L and U in the order of the pattern's rows: A's lower /
+ * upper entries for ILU(0), the level-p pattern's for ILU(p); the inverted diagonal); any
+ * pointer may be NULL */
 int feat_ilu_factors(feat_handle_t h, feat_ilu_t s, double* l, double* u, double* d);
 int feat_ilu_done_symbolic(feat_ilu_t s);
 

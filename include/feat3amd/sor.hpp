@@ -92,7 +92,8 @@ namespace feat3amd
       }
     };
 
-    /// ILU(0) (ilu_precond.hpp:1040-1104, generic backend with p = 0): cor = (D+U)^{-1}(I+L)^{-1} def
+    /// ILU(p) (ilu_precond.hpp:1040-1104, generic backend; the level-p pattern of
+    /// ILUCoreSymbolic::factorize_symbolic): cor = (D+U)^{-1}(I+L)^{-1} def
     template<typename Matrix_, typename Filter_>
     class ILUPrecond : public SolverBase<typename Matrix_::VectorTypeL>
     {
@@ -104,12 +105,13 @@ namespace feat3amd
       const Matrix_& _matrix;
       const Filter_& _filter;
       feat_ilu_t _plan = nullptr;
+      int _p = 0;
 
     public:
-      ILUPrecond(const Matrix_& m, const Filter_& f, int p = 0) : _matrix(m), _filter(f)
+      ILUPrecond(const Matrix_& m, const Filter_& f, int p = 0) : _matrix(m), _filter(f), _p(p)
       {
         if(m.rows() != m.columns()) throw Error("ILUPrecond: matrix is not square");
-        if(p != 0) throw Error("ILUPrecond: only ILU(0) is provided on the device");
+        if(p < 0) throw Error("ILUPrecond: negative fill-in level");
       }
       ~ILUPrecond() override { done_symbolic(); }
       std::string name() const override { return "ILU"; }
@@ -117,7 +119,7 @@ namespace feat3amd
       {
         done_symbolic();
         const feat_csr_f64_t d = _matrix.desc();
-        check(feat_ilu_init_symbolic(Runtime::handle(), &d, &_plan), "ILUPrecond::init_symbolic");
+        check(feat_ilu_init_symbolic_p(Runtime::handle(), &d, _p, &_plan), "ILUPrecond::init_symbolic");
       }
       void init_numeric() override
       {

@@ -134,7 +134,8 @@ int feat_mgpcg_destroy(feat_mgpcg_t s);
 /* ---- single Krylov solver with a device preconditioner ----
  * PCG (solver = 0, pcg.hpp) or Richardson (solver = 1, richardson.hpp, damping omega_rich)
  * around precond 0 none, 1 JacobiPrecond, 2 SORPrecond, 3 SSORPrecond (omega_pre =
- * damping; level-set order, the generic sweeps exactly), 4 ILUPrecond (ILU(0)), 5 / 6 SOR /
+ * damping; level-set order, the generic sweeps exactly), 4 ILUPrecond (ILU(p), the fill-in
+ * level p = omega_pre, a whole number >= 0), 5 / 6 SOR /
  * SSOR in the multicolour order of the CUDA backend; homogeneous unit filter on filt[0..nfilt).  The matrix
  * is host CSR (copied to the device); x (initial guess) and b are device vectors. */
 #define FEAT_PRECOND_NONE 0

@@ -220,6 +220,15 @@ void orc_ilu0_factor(uint64_t n, const uint64_t* rowptr, const uint64_t* col, co
                      double* u, double* d, uint64_t* rptr_l, uint64_t* rptr_u);
 void orc_ilu0_apply(uint64_t n, const uint64_t* rowptr, const uint64_t* col, const double* l, const double* u,
                     const double* d, const uint64_t* rptr_l, const uint64_t* rptr_u, const double* x, double* y);
+/* ILU(p) (solver/ilu_precond.hpp generic backend): the level-p split pattern (0, or -1 when
+ * cap_l / cap_u are too small), the factors on a pattern, and y = (D+U)^{-1}(I+L)^{-1} x */
+int orc_ilup_pattern(uint64_t n, const uint64_t* rowptr, const uint64_t* col, int p, uint64_t cap_l, uint64_t cap_u,
+                     uint64_t* rl, uint64_t* cl, uint64_t* ru, uint64_t* cu, uint64_t* nl, uint64_t* nu);
+void orc_ilu_factor_pattern(uint64_t n, const uint64_t* rowptr, const uint64_t* col, const double* val,
+                            const uint64_t* rl, const uint64_t* cl, const uint64_t* ru, const uint64_t* cu, double* l,
+                            double* u, double* d);
+void orc_ilu_apply_pattern(uint64_t n, const uint64_t* rl, const uint64_t* cl, const uint64_t* ru, const uint64_t* cu,
+                           const double* l, const double* u, const double* d, const double* x, double* y);
 /* the generic sequential SOR / SSOR (solver/sor_precond.hpp:201-222, ssor_precond.hpp:177-231) */
 void orc_sor_generic_apply(uint64_t n, const uint64_t* rowptr, const uint64_t* col, const double* val, double omega,
                            int ssor, const double* x, double* y);

@@ -234,3 +234,171 @@ void orc_ilu0_apply(uint64_t n, const uint64_t* rowptr, const uint64_t* col, con
     y[i] = d[i] * r;
   }
 }
+
+/* ---- ILU(p) (solver/ilu_precond.hpp, generic backend) ----
+ * The level-p pattern of ILUCoreSymbolic::factorize_symbolic (ilu_precond.hpp:191-262) with
+ * its linear insertion _insert (:287-333): for every row i the level-0 entries of A split into
+ * L (col < i) and U (col > i); then, walking the (growing) row of L, each entry (i, cj) of
+ * level lj combines with every entry (cj, ck) of the already final row cj of U, level lk,
+ * into (i, ck) of level lj + lk + 1 <= p, inserted into L (ck < i) or U (ck > i) at its
+ * sorted place, or lowering the level of an existing entry.  Returns 0, or -1 when the
+ * capacities cap_l / cap_u (entries) are too small. */
+static uint64_t ilu_insert(uint64_t* idx, int* lvl, uint64_t* n, uint64_t i, uint64_t j, int l)
+{
+  for(; i < *n; ++i)
+  {
+    if(idx[i] == j)
+    {
+      if(l < lvl[i]) lvl[i] = l;
+      return i + 1;
+    }
+    if(j < idx[i]) break;
+  }
+  for(uint64_t k = *n; k > i; --k)
+  {
+    idx[k] = idx[k - 1];
+    lvl[k] = lvl[k - 1];
+  }
+  idx[i] = j;
+  lvl[i] = l;
+  ++*n;
+  return i + 1;
+}
+
+int orc_ilup_pattern(uint64_t n, const uint64_t* rowptr, const uint64_t* col, int p, uint64_t cap_l, uint64_t cap_u,
+                     uint64_t* rl, uint64_t* cl, uint64_t* ru, uint64_t* cu, uint64_t* nl_out, uint64_t* nu_out)
+{
+  int* ll = (int*)malloc(sizeof(int) * (cap_l ? cap_l : 1));
+  int* lu = (int*)malloc(sizeof(int) * (cap_u ? cap_u : 1));
+  uint64_t nl = 0, nu = 0;
+  int rc = 0;
+  rl[0] = ru[0] = 0;
+  for(uint64_t i = 0; i < n && !rc; ++i)
+  {
+    for(uint64_t k = rowptr[i]; k < rowptr[i + 1]; ++k)
+    {
+      if(col[k] < i)
+      {
+        if(nl >= cap_l) { rc = -1; break; }
+        cl[nl] = col[k];
+        ll[nl++] = 0;
+      }
+      else if(col[k] > i)
+      {
+        if(nu >= cap_u) { rc = -1; break; }
+        cu[nu] = col[k];
+        lu[nu++] = 0;
+      }
+    }
+    if(rc) break;
+    if(p >= 1)
+      for(uint64_t j = rl[i]; j < nl; ++j)
+      {
+        const uint64_t cj = cl[j];
+        const int lj = ll[j];
+        uint64_t olj = j, ouj = ru[i];
+        for(uint64_t k = ru[cj]; k < ru[cj + 1]; ++k)
+        {
+          const uint64_t ck = cu[k];
+          const int lev = lj + lu[k] + 1;
+          if(lev > p) continue;
+          if(ck < i)
+          {
+            if(nl + 1 > cap_l) { rc = -1; break; }
+            olj = ilu_insert(cl, ll, &nl, olj, ck, lev);
+          }
+          else if(ck > i)
+          {
+            if(nu + 1 > cap_u) { rc = -1; break; }
+            ouj = ilu_insert(cu, lu, &nu, ouj, ck, lev);
+          }
+        }
+        if(rc) break;
+      }
+    rl[i + 1] = nl;
+    ru[i + 1] = nu;
+  }
+  free(ll);
+  free(lu);
+  *nl_out = nl;
+  *nu_out = nu;
+  return rc;
+}
+
+/* ILUCoreScalar::copy_data_csr (ilu_precond.hpp:404-432) and factorize_numeric_il_du
+ * (:453-527) on a split pattern (rl, cl, ru, cu): values of A where the pattern has A's
+ * entries, zero at fill-in positions, then the IKJ elimination; d is stored inverted. */
+void orc_ilu_factor_pattern(uint64_t n, const uint64_t* rowptr, const uint64_t* col, const double* val,
+                            const uint64_t* rl, const uint64_t* cl, const uint64_t* ru, const uint64_t* cu, double* l,
+                            double* u, double* d)
+{
+  for(uint64_t i = 0; i < n; ++i)
+  {
+    uint64_t ra = rowptr[i];
+    const uint64_t xa = rowptr[i + 1];
+    for(uint64_t j = rl[i]; j < rl[i + 1]; ++j)
+    {
+      if(ra < xa && cl[j] == col[ra])
+        l[j] = val[ra++];
+      else
+        l[j] = 0.0;
+    }
+    d[i] = val[ra++];
+    for(uint64_t j = ru[i]; j < ru[i + 1]; ++j)
+    {
+      if(ra < xa && cu[j] == col[ra])
+        u[j] = val[ra++];
+      else
+        u[j] = 0.0;
+    }
+  }
+  for(uint64_t i = 0; i < n; ++i)
+  {
+    const uint64_t ql = rl[i + 1], qu = ru[i + 1];
+    for(uint64_t j = rl[i]; j < ql; ++j)
+    {
+      const uint64_t cj = cl[j];
+      uint64_t pl = j, pu = ru[i];
+      l[j] *= d[cj];
+      uint64_t k = ru[cj];
+      for(; k < ru[cj + 1]; ++k)
+      {
+        const uint64_t ck = cu[k];
+        if(ck >= i) break;
+        for(; (pl < ql) && (cl[pl] <= ck); ++pl)
+          if(cl[pl] == ck) l[pl] = fma(-l[j], u[k], l[pl]);
+      }
+      if((k < ru[cj + 1]) && (cu[k] == i))
+      {
+        d[i] = fma(-l[j], u[k], d[i]);
+        ++k;
+      }
+      for(; k < ru[cj + 1]; ++k)
+      {
+        const uint64_t ck = cu[k];
+        for(; (pu < qu) && (cu[pu] <= ck); ++pu)
+          if(cu[pu] == ck) u[pu] = fma(-l[j], u[k], u[pu]);
+      }
+    }
+    d[i] = 1.0 / d[i];
+  }
+}
+
+/* solve_il then solve_du (ilu_precond.hpp:540-582) on a split pattern */
+void orc_ilu_apply_pattern(uint64_t n, const uint64_t* rl, const uint64_t* cl, const uint64_t* ru, const uint64_t* cu,
+                           const double* l, const double* u, const double* d, const double* x, double* y)
+{
+  for(uint64_t i = 0; i < n; ++i)
+  {
+    double r = x[i];
+    for(uint64_t j = rl[i]; j < rl[i + 1]; ++j) r = fma(-l[j], y[cl[j]], r);
+    y[i] = r;
+  }
+  for(uint64_t i = n; i > 0;)
+  {
+    --i;
+    double r = y[i];
+    for(uint64_t j = ru[i]; j < ru[i + 1]; ++j) r = fma(-u[j], y[cu[j]], r);
+    y[i] = d[i] * r;
+  }
+}

@@ -431,6 +431,50 @@ class Oracle:
                          rl.ctypes.data, ru.ctypes.data, x.ctypes.data, y.ctypes.data)
         return y
 
+    def ilup_pattern(self, A, p):
+        """The level-p split pattern (rl, cl, ru, cu) of ILUCoreSymbolic::factorize_symbolic."""
+        L = self.L
+        L.orc_ilup_pattern.restype = C.c_int
+        L.orc_ilup_pattern.argtypes = [C.c_uint64, _vp, _vp, C.c_int, C.c_uint64, C.c_uint64, _vp, _vp, _vp, _vp,
+                                       C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
+        rp = np.ascontiguousarray(A.rowptr, np.uint64)
+        col = np.ascontiguousarray(A.col, np.uint64)
+        cap = max(col.size, 16)
+        while True:
+            rl, ru = np.empty(A.rows + 1, np.uint64), np.empty(A.rows + 1, np.uint64)
+            cl, cu = np.empty(cap, np.uint64), np.empty(cap, np.uint64)
+            nl, nu = C.c_uint64(), C.c_uint64()
+            rc = L.orc_ilup_pattern(A.rows, rp.ctypes.data, col.ctypes.data, int(p), cap, cap, rl.ctypes.data,
+                                    cl.ctypes.data, ru.ctypes.data, cu.ctypes.data, C.byref(nl), C.byref(nu))
+            if rc == 0:
+                return rl, cl[:nl.value].copy(), ru, cu[:nu.value].copy()
+            cap *= 2
+
+    def ilup_factor(self, A, p):
+        """ILU(p) factors (l, u, d_inverted) on the level-p pattern, with the pattern."""
+        L = self.L
+        L.orc_ilu_factor_pattern.argtypes = [C.c_uint64] + [_vp] * 10
+        rl, cl, ru, cu = self.ilup_pattern(A, p)
+        rp = np.ascontiguousarray(A.rowptr, np.uint64)
+        col = np.ascontiguousarray(A.col, np.uint64)
+        val = np.ascontiguousarray(A.val, np.float64)
+        l, u, d = np.empty(max(cl.size, 1)), np.empty(max(cu.size, 1)), np.empty(A.rows)
+        L.orc_ilu_factor_pattern(A.rows, rp.ctypes.data, col.ctypes.data, val.ctypes.data, rl.ctypes.data,
+                                 cl.ctypes.data, ru.ctypes.data, cu.ctypes.data, l.ctypes.data, u.ctypes.data,
+                                 d.ctypes.data)
+        return dict(l=l[:cl.size].copy(), u=u[:cu.size].copy(), d=d, rl=rl, cl=cl, ru=ru, cu=cu)
+
+    def ilup_apply(self, f, x):
+        L = self.L
+        L.orc_ilu_apply_pattern.argtypes = [C.c_uint64] + [_vp] * 9
+        n = f["d"].size
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.empty(n)
+        L.orc_ilu_apply_pattern(n, f["rl"].ctypes.data, f["cl"].ctypes.data, f["ru"].ctypes.data, f["cu"].ctypes.data,
+                                np.ascontiguousarray(f["l"]).ctypes.data, np.ascontiguousarray(f["u"]).ctypes.data,
+                                f["d"].ctypes.data, x.ctypes.data, y.ctypes.data)
+        return y
+
     def csr_apply(self, A, x, y=None, a=1.0, b=0.0, transposed=False, r=None):
         nr = A.cols if transposed else A.rows
         if y is None:

@@ -155,11 +155,12 @@ def np_krylov(oracle, A, filt, prec, b, tol, maxit, pcg=True, omega_rich=1.0):
 @pytest.mark.parametrize("solver,precond,omega,dim,level", [("pcg", "ssor", 1.0, 2, 5), ("pcg", "ssor", 1.2, 3, 3),
                                                             ("richardson", "sor", 1.0, 2, 4),
                                                             ("richardson", "ssor_mc", 1.0, 2, 4),
-                                                            ("pcg", "ilu", 1.0, 2, 5), ("pcg", "ilu", 1.0, 3, 3),
+                                                            ("pcg", "ilu", 0.0, 2, 5), ("pcg", "ilu", 0.0, 3, 3),
                                                             ("pcg", "jacobi", 0.8, 2, 5)])
 def test_krylov_with_device_precond(feat, oracle, torch_gpu, solver, precond, omega, dim, level):
     """The solver tree around the device preconditioners follows the numpy restatement
-    (iteration count equal, defect history and solution to 1e-10)."""
+    (iteration count equal, defect history and solution to 1e-10); for ILU, omega is the
+    fill-in level p (0 here)."""
     print(f"krylov {solver}-{precond} omega={omega} dim={dim} level={level}", flush=True)
     torch = torch_gpu
     q = feat.q1_problem(dim, level)
@@ -213,3 +214,50 @@ def test_ilu0_vs_oracle_and_reference(feat, oracle, reference, handle, torch_gpu
     torch.cuda.synchronize()
     assert np.array_equal(xd.cpu().numpy(), want)
     plan.close()
+
+
+@pytest.mark.parametrize("p", [1, 2])
+@pytest.mark.parametrize("name", ["random", "q1_2d_l6", "q1_3d_l4"])
+def test_ilup_vs_oracle(feat, oracle, handle, torch_gpu, name, p):
+    """ILU(p): the level-p pattern, the factors on it (zeros at fill-in) and both solves equal
+    the oracle bit for bit (the oracle's ILU(p) is pinned to the reference's ILUPrecond)."""
+    print(f"ilu({p}) {name}", flush=True)
+    torch = torch_gpu
+    A = cases(feat)[name]
+    keep, desc = upload(feat, torch, A)
+    plan = feat.IluPlan(handle, desc, p=p)
+    f = oracle.ilup_factor(A, p)
+    assert (plan.nnz_l, plan.nnz_u) == (f["l"].size, f["u"].size)
+    assert f["l"].size > int(np.sum(A.col < np.repeat(np.arange(A.rows), np.diff(A.rowptr.astype(np.int64)))))
+    gl, gu, gd = plan.factors()
+    assert np.array_equal(gl, f["l"]) and np.array_equal(gu, f["u"]) and np.array_equal(gd, f["d"])
+    x = np.random.default_rng(7).standard_normal(A.rows)
+    xd = torch.from_numpy(x).cuda()
+    y = torch.empty_like(xd)
+    plan.apply(y, xd)
+    torch.cuda.synchronize()
+    assert np.array_equal(y.cpu().numpy(), oracle.ilup_apply(f, x))
+    plan.close()
+
+
+def test_pcg_ilup_iterations(feat, oracle, torch_gpu):
+    """PCG with ILU(0), ILU(1) and ILU(2) follows the numpy restatement around the oracle's
+    ILU(p) (same iterations, history to 1e-10).  (On this 27-point stencil, whose face
+    couplings are stored zeros, more fill does not lower the iteration count.)"""
+    torch = torch_gpu
+    q = feat.q1_problem(3, 4)
+    A = Csr(q["n"], q["n"], q["val"], q["col"], q["rowptr"])
+    b = np.random.default_rng(10).uniform(-1, 1, A.rows)
+    b[q["filt"].astype(np.int64)] = 0.0
+    its = []
+    for p in (0, 1, 2):
+        f = oracle.ilup_factor(A, p)
+        x_ref, h_ref = np_krylov(oracle, A, q["filt"], lambda r: oracle.ilup_apply(f, r), b, 1e-8, 500, pcg=True)
+        s = feat.KrylovSolver("pcg", A, q["filt"], "ilu", omega_pre=float(p), tol_rel=1e-8, max_iter=500)
+        x = torch.zeros(A.rows, dtype=torch.float64, device="cuda")
+        res = s.solve(x, torch.from_numpy(b).cuda(), hist_cap=501)
+        s.close()
+        assert res["status"] == "success" and res["iters"] == len(h_ref) - 1, (p, res["iters"], len(h_ref) - 1)
+        assert np.allclose(res["hist"], h_ref, rtol=1e-10, atol=0)
+        its.append(res["iters"])
+    print(f"PCG-ILU(p) iterations, 3D level 4: {its}")

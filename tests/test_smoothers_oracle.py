@@ -154,3 +154,65 @@ def test_ilu0_is_reference(feat, oracle, reference, case):
     LU, M = Lm @ Um, dense(A)
     mask = M != 0
     assert np.allclose(LU[mask], M[mask], rtol=1e-12, atol=1e-12 * np.abs(M).max())
+
+
+def brute_level_pattern(A, p):
+    """Level-of-fill pattern by the textbook rule on the dense index grid (lev(i,j) = 0 on
+    A's pattern, then lev(i,j) = min(lev(i,j), lev(i,k) + lev(k,j) + 1) for k < min(i, j),
+    rows in order): the entries of level <= p, as a set of (i, j) with i != j."""
+    n = A.rows
+    INF = 1 << 30
+    lev = np.full((n, n), INF, dtype=np.int64)
+    for r in range(n):
+        for k in range(int(A.rowptr[r]), int(A.rowptr[r + 1])):
+            lev[r, int(A.col[k])] = 0
+    for i in range(n):
+        for k in range(i):
+            if lev[i, k] > p:
+                continue
+            for j in range(k + 1, n):
+                if lev[k, j] <= p:
+                    lev[i, j] = min(lev[i, j], lev[i, k] + lev[k, j] + 1)
+    return {(i, j) for i in range(n) for j in range(n) if i != j and lev[i, j] <= p}
+
+
+@pytest.mark.parametrize("p", [1, 2, 3])
+@pytest.mark.parametrize("case", ["random", "q1_2d", "q1_3d"])
+def test_ilup_is_reference(feat, oracle, reference, case, p):
+    """ILU(p) restated (level-p pattern, factorisation with zero fill values, solves): the
+    reference's own ILUPrecond<generic> with p fill levels gives the same bits."""
+    rng = np.random.default_rng(43)
+    A = dominant(rng, 120) if case == "random" else q1_csr(feat, 2 if case == "q1_2d" else 3, 3 if case == "q1_2d" else 2)
+    f = oracle.ilup_factor(A, p)
+    for seed in range(3):
+        x = np.random.default_rng(seed).standard_normal(A.rows)
+        assert np.array_equal(oracle.ilup_apply(f, x), reference.precond_apply("ilu", A, p, x))
+
+
+@pytest.mark.parametrize("case", ["random", "q1_2d"])
+def test_ilup_pattern_and_closed_forms(feat, oracle, case):
+    """Without the reference build: the level-p pattern equals the textbook level-of-fill rule
+    on the dense grid; p = 0 is ILU(0); a large p fills the whole LU pattern, so
+    (I+L)(D+U) = A exactly up to rounding (complete LU of a diagonally dominant matrix)."""
+    rng = np.random.default_rng(47)
+    A = dominant(rng, 60, max_len=5) if case == "random" else q1_csr(feat, 2, 3)
+    n = A.rows
+    for p in (1, 2):
+        rl, cl, ru, cu = oracle.ilup_pattern(A, p)
+        got = {(i, int(c)) for i in range(n) for c in cl[int(rl[i]):int(rl[i + 1])]}
+        got |= {(i, int(c)) for i in range(n) for c in cu[int(ru[i]):int(ru[i + 1])]}
+        assert got == brute_level_pattern(A, p)
+        for i in range(n):   # sorted rows
+            assert np.all(np.diff(cl[int(rl[i]):int(rl[i + 1])].astype(np.int64)) > 0)
+            assert np.all(np.diff(cu[int(ru[i]):int(ru[i + 1])].astype(np.int64)) > 0)
+    f0, g = oracle.ilu0_factor(A), oracle.ilup_factor(A, 0)
+    assert np.array_equal(f0[0], g["l"]) and np.array_equal(f0[1], g["u"]) and np.array_equal(f0[2], g["d"])
+    f = oracle.ilup_factor(A, n)
+    Lm, Um = np.eye(n), np.diag(1.0 / f["d"])
+    for i in range(n):
+        for j in range(int(f["rl"][i]), int(f["rl"][i + 1])):
+            Lm[i, int(f["cl"][j])] = f["l"][j]
+        for j in range(int(f["ru"][i]), int(f["ru"][i + 1])):
+            Um[i, int(f["cu"][j])] = f["u"][j]
+    M = dense(A)
+    assert np.allclose(Lm @ Um, M, rtol=0, atol=1e-12 * np.abs(M).max())
