@@ -779,6 +779,31 @@ class Reference:
         return dict(status=STATUS[st], iters=int(it.value), x=x, hist=hist[:it.value + 1],
                     def_final=dfin.value, seconds=secs.value)
 
+    def stokes_asm(self, level):
+        """The reference's own Q2/P1dc assembly of the unit square refined `level` times
+        (DuDvOperatorBlocked<2>, GradPresDivVeloAssembler, cubature auto-degree:5): A, B, D as
+        (val, col, rowptr) with row-major blocks, the mesh's vertex coordinates, edge->vertex
+        and cell->vertex lists."""
+        L = self.L
+        L.ref_stokes_asm_create.restype = _vp
+        L.ref_stokes_asm_create.argtypes = [C.c_int]
+        L.ref_stokes_asm_sizes.argtypes = [_vp, _vp]
+        L.ref_stokes_asm_copy.argtypes = [_vp] * 13
+        L.ref_stokes_asm_free.argtypes = [_vp]
+        h = L.ref_stokes_asm_create(int(level))
+        z = np.zeros(8, np.uint64)
+        L.ref_stokes_asm_sizes(h, z.ctypes.data)
+        nv, np_, na, nb, nd, nvert, nedge, ncell = (int(v) for v in z)
+        A = (np.empty(4 * na), np.empty(na, np.uint64), np.empty(nv + 1, np.uint64))
+        B = (np.empty(2 * nb), np.empty(nb, np.uint64), np.empty(nv + 1, np.uint64))
+        D = (np.empty(2 * nd), np.empty(nd, np.uint64), np.empty(np_ + 1, np.uint64))
+        coords, edges, cells = np.empty(2 * nvert), np.empty(2 * nedge, np.uint64), np.empty(4 * ncell, np.uint64)
+        L.ref_stokes_asm_copy(h, *(a.ctypes.data for a in A + B + D), coords.ctypes.data, edges.ctypes.data,
+                              cells.ctypes.data)
+        L.ref_stokes_asm_free(h)
+        return dict(nv=nv, np=np_, A=A, B=B, D=D, coords=coords.reshape(-1, 2), edges=edges.reshape(-1, 2),
+                    cells=cells.reshape(-1, 4))
+
     def precond_apply(self, kind, A, omega, x):
         """y = M^{-1} x of the reference's generic SORPrecond ("sor"), SSORPrecond ("ssor")
         or ILUPrecond ("ilu", level p = omega)."""

@@ -1158,3 +1158,111 @@ int ref_precond_apply(int kind, const ref_csr_t* m, double omega, const double* 
 }
 
 } // extern "C"
+
+// ---------------- Q2/P1dc Stokes assembly by the reference (amavanka-test.cpp:86-117) ----------------
+#include <kernel/space/lagrange2/element.hpp>
+#include <kernel/space/discontinuous/element.hpp>
+#include <kernel/assembly/bilinear_operator_assembler.hpp>
+#include <kernel/assembly/gpdv_assembler.hpp>
+
+struct ref_stokes_asm_t
+{
+  Index nv = 0, np = 0, nvert = 0, nedge = 0, ncell = 0;
+  std::vector<double> a_val, b_val, d_val, coords;
+  std::vector<std::uint64_t> a_col, a_ptr, b_col, b_ptr, d_col, d_ptr, edges, cells;
+};
+
+extern "C" {
+
+// A = DuDvOperatorBlocked<2> (BCSR 2x2), B / D = GradPresDivVeloAssembler (BCSR 2x1 / 1x2) on the
+// unit square refined `level` times, Lagrange2 velocity and StdPolyP<1> discontinuous pressure,
+// cubature auto-degree:5; the mesh's vertex coordinates, edge->vertex and cell->vertex lists
+// let the caller place every dof.
+void* ref_stokes_asm_create(int level)
+{
+  ensure_runtime();
+  typedef Shape::Hypercube<2> ShapeType;
+  typedef Geometry::ConformalMesh<ShapeType> MeshType;
+  typedef Trafo::Standard::Mapping<MeshType> TrafoType;
+  typedef Space::Lagrange2::Element<TrafoType> SpaceV;
+  typedef Space::Discontinuous::Element<TrafoType, Space::Discontinuous::Variant::StdPolyP<1>> SpaceP;
+  const Index lvl = Index(level);
+  Geometry::RefinedUnitCubeFactory<MeshType> fac(lvl);
+  MeshType mesh(fac);
+  TrafoType trafo(mesh);
+  SpaceV space_v(trafo);
+  SpaceP space_p(trafo);
+  Cubature::DynamicFactory cubature("auto-degree:5");
+  MatA mat_a;
+  MatB mat_b;
+  MatD mat_d;
+  Assembly::SymbolicAssembler::assemble_matrix_std1(mat_a, space_v);
+  mat_a.format();
+  Assembly::Common::DuDvOperatorBlocked<2> dudv_op;
+  Assembly::BilinearOperatorAssembler::assemble_matrix1(mat_a, dudv_op, space_v, cubature);
+  Assembly::GradPresDivVeloAssembler::assemble(mat_b, mat_d, space_v, space_p, cubature);
+  auto* s = new ref_stokes_asm_t();
+  s->nv = mat_a.rows();
+  s->np = mat_d.rows();
+  auto grab = [](auto& m, int bs, std::vector<double>& val, std::vector<std::uint64_t>& col,
+                 std::vector<std::uint64_t>& ptr) {
+    const Index nnz = m.used_elements();
+    val.assign(m.template val<LAFEM::Perspective::pod>(), m.template val<LAFEM::Perspective::pod>() + nnz * Index(bs));
+    col.assign(m.col_ind(), m.col_ind() + nnz);
+    ptr.assign(m.row_ptr(), m.row_ptr() + m.rows() + 1);
+  };
+  grab(mat_a, 4, s->a_val, s->a_col, s->a_ptr);
+  grab(mat_b, 2, s->b_val, s->b_col, s->b_ptr);
+  grab(mat_d, 2, s->d_val, s->d_col, s->d_ptr);
+  const auto& vtx = mesh.get_vertex_set();
+  s->nvert = vtx.get_num_vertices();
+  s->coords.resize(2 * s->nvert);
+  for(Index i = 0; i < s->nvert; ++i)
+    for(int k = 0; k < 2; ++k) s->coords[2 * i + k] = double(vtx[i][k]);
+  const auto& iev = mesh.template get_index_set<1, 0>();
+  s->nedge = iev.get_num_entities();
+  for(Index e = 0; e < s->nedge; ++e)
+    for(int k = 0; k < 2; ++k) s->edges.push_back(iev[e][k]);
+  const auto& icv = mesh.template get_index_set<2, 0>();
+  s->ncell = icv.get_num_entities();
+  for(Index c = 0; c < s->ncell; ++c)
+    for(int k = 0; k < 4; ++k) s->cells.push_back(icv[c][k]);
+  return s;
+}
+
+// out: nv, np, nnz(A), nnz(B), nnz(D) (blocks), vertices, edges, cells
+void ref_stokes_asm_sizes(void* h, std::uint64_t* out)
+{
+  auto* s = (ref_stokes_asm_t*)h;
+  out[0] = s->nv;
+  out[1] = s->np;
+  out[2] = s->a_col.size();
+  out[3] = s->b_col.size();
+  out[4] = s->d_col.size();
+  out[5] = s->nvert;
+  out[6] = s->nedge;
+  out[7] = s->ncell;
+}
+
+void ref_stokes_asm_copy(void* h, double* a_val, std::uint64_t* a_col, std::uint64_t* a_ptr, double* b_val,
+                         std::uint64_t* b_col, std::uint64_t* b_ptr, double* d_val, std::uint64_t* d_col,
+                         std::uint64_t* d_ptr, double* coords, std::uint64_t* edges, std::uint64_t* cells)
+{
+  auto* s = (ref_stokes_asm_t*)h;
+  std::copy(s->a_val.begin(), s->a_val.end(), a_val);
+  std::copy(s->a_col.begin(), s->a_col.end(), a_col);
+  std::copy(s->a_ptr.begin(), s->a_ptr.end(), a_ptr);
+  std::copy(s->b_val.begin(), s->b_val.end(), b_val);
+  std::copy(s->b_col.begin(), s->b_col.end(), b_col);
+  std::copy(s->b_ptr.begin(), s->b_ptr.end(), b_ptr);
+  std::copy(s->d_val.begin(), s->d_val.end(), d_val);
+  std::copy(s->d_col.begin(), s->d_col.end(), d_col);
+  std::copy(s->d_ptr.begin(), s->d_ptr.end(), d_ptr);
+  std::copy(s->coords.begin(), s->coords.end(), coords);
+  std::copy(s->edges.begin(), s->edges.end(), edges);
+  std::copy(s->cells.begin(), s->cells.end(), cells);
+}
+
+void ref_stokes_asm_free(void* h) { delete (ref_stokes_asm_t*)h; }
+
+} // extern "C"

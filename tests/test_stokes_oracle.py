@@ -165,3 +165,79 @@ def test_stokes_mg_solve_vs_reference(feat, oracle, reference, lf, lc):
     # defect norms: the reference's TupleVector norm contracts differently where it is inlined
     assert np.allclose(o["hist"], r["hist"], rtol=4 * np.finfo(float).eps, atol=0)
     assert np.all(np.diff(np.log(o["hist"])) < 0)
+
+
+def _bcsr_dense(m, rows, cols, bh, bw):
+    val, col, rp = m
+    M = np.zeros((rows * bh, cols * bw))
+    for r in range(rows):
+        for k in range(int(rp[r]), int(rp[r + 1])):
+            c = int(col[k])
+            M[r * bh:(r + 1) * bh, c * bw:(c + 1) * bw] = val[k * bh * bw:(k + 1) * bh * bw].reshape(bh, bw)
+    return M
+
+
+@pytest.mark.parametrize("level", [1, 2, 3])
+def test_stokes_matrices_equal_reference_assembly(feat, reference, level):
+    """A, B and D of the product's Q2/P1dc assembly equal the reference's own assembly
+    (DuDvOperatorBlocked<2> and GradPresDivVeloAssembler on a RefinedUnitCubeFactory mesh,
+    as kernel/solver/amavanka-test.cpp:110-117 assembles them) after renumbering.  Velocity
+    dofs: the reference numbers vertices, then edges, then cells (Lagrange2's uniform dof
+    assignment); each is placed at its vertex, edge midpoint or cell centre and mapped to the
+    lexicographic node there.  Pressure: the reference's P1dc basis of a cell is (1, x^, y^)
+    with (x^, y^) the cell's reference coordinates, whose axes are the directions of its
+    local edges v0->v1 and v0->v2; the product's is (1, x^, y^) along +x and +y.  Each cell's
+    three columns are related by the signed permutation T read off its vertices; on the
+    refined unit square T is the identity for every cell (asserted), so the pressure dofs
+    map by cell centre alone."""
+    r = reference.stokes_asm(level)
+    o = feat.stokes_problem(level)
+    N = 2 ** level
+    m = 2 * N + 1
+    h = 1.0 / N
+    assert (r["nv"], r["np"]) == (o["nv"], o["np"]) == (m * m, 3 * N * N)
+    xy = r["coords"]
+    cells = r["cells"].astype(np.int64)
+    vpos = np.concatenate([xy, xy[r["edges"].astype(np.int64)].mean(axis=1), xy[cells].mean(axis=1)])
+    assert vpos.shape[0] == r["nv"]
+    g = np.rint(vpos * 2 * N).astype(np.int64)
+    assert np.allclose(g, vpos * 2 * N, atol=1e-9)
+    vmap = g[:, 1] * m + g[:, 0]                      # reference velocity dof -> product node
+    assert np.array_equal(np.sort(vmap), np.arange(m * m))
+    cen = xy[cells].mean(axis=1)
+    cmap = np.rint((cen[:, 1] - 0.5 * h) / h).astype(np.int64) * N + np.rint((cen[:, 0] - 0.5 * h) / h).astype(np.int64)
+    assert np.array_equal(np.sort(cmap), np.arange(N * N))
+    # per reference cell: T (3x3) with psi_ref = T psi_product
+    T = np.zeros((N * N, 3, 3))
+    T[:, 0, 0] = 1.0
+    d1 = (xy[cells[:, 1]] - xy[cells[:, 0]]) / h
+    d2 = (xy[cells[:, 2]] - xy[cells[:, 0]]) / h
+    T[:, 1, 1:], T[:, 2, 1:] = d1, d2
+    assert np.allclose(np.abs(T[:, 1:, 1:]).sum(axis=2), 1.0)   # axis-aligned cells: signed permutations
+    assert np.array_equal(T, np.broadcast_to(np.eye(3), T.shape))  # every cell oriented +x, +y
+    # dense matrices, reference renumbered to the product's velocity ordering
+    nv, np_ = o["nv"], o["np"]
+    Ar = _bcsr_dense(r["A"], nv, nv, 2, 2)
+    Br = _bcsr_dense(r["B"], nv, np_, 2, 1)
+    Dr = _bcsr_dense(r["D"], np_, nv, 1, 2)
+    Ao = _bcsr_dense(o["A"], nv, nv, 2, 2)
+    Bo = _bcsr_dense(o["B"], nv, np_, 2, 1)
+    Do = _bcsr_dense(o["D"], np_, nv, 1, 2)
+    vp = np.empty(2 * nv, np.int64)                   # product velocity row of each reference row
+    vp[0::2], vp[1::2] = 2 * vmap, 2 * vmap + 1
+    Ar_p = np.zeros_like(Ar)
+    Ar_p[np.ix_(vp, vp)] = Ar
+    scale = np.abs(Ao).max()
+    assert np.abs(Ar_p - Ao).max() <= 1e-14 * scale
+    # B: reference pressure columns = product columns combined by T per cell
+    Bo_t = np.zeros_like(Bo)
+    for cr in range(N * N):
+        co = cmap[cr]
+        Bo_t[:, 3 * cr:3 * cr + 3] = Bo[:, 3 * co:3 * co + 3] @ T[cr].T
+    Br_p = np.zeros_like(Br)
+    Br_p[vp, :] = Br
+    assert np.abs(Br_p - Bo_t).max() <= 1e-14 * np.abs(Bo).max()
+    Dr_p = np.zeros_like(Dr)
+    Dr_p[:, vp] = Dr
+    assert np.abs(Dr_p - Bo_t.T).max() <= 1e-14 * np.abs(Do).max()
+    assert np.array_equal(Do, Bo.T)
