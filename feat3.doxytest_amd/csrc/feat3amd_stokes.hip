@@ -29,180 +29,389 @@ struct TupleDev
   const uint64_t* ptr[4];
 };
 
-constexpr int kUnr = 8;   // blocks per row fetched ahead in the tuple SpMV
+// Tuple apply staged through LDS.  A workgroup covers kTupRows block rows of one kind:
+// velocity rows (blocks 0 = A, 2x2, and 1 = B, 2x1) or pressure rows (2 = D, 1x2, and
+// 3 = C, 1x1).  All 256 threads load the workgroup's blocks and column indices and gather
+// the x they multiply into LDS -- every load of a pass issued before the first gather,
+// every gather before the first LDS store -- then one lane per row adds its blocks from LDS
+// in storage order, with the fmas of the row-serial loop (bit-identical to it).  A
+// workgroup whose rows hold more blocks than one pass stages takes several passes (48 Q2
+// rows hold at most 24 x 25 + 24 x 15 = 960 A blocks: one pass).
+constexpr int kTupRows = 48;
+constexpr int kTupE0 = 1024;   // blocks of the first matrix (A or D) per pass
+constexpr int kTupE1 = 512;    // blocks of the second (B or C) per pass
+constexpr int kTupU0 = kTupE0 / kThreads, kTupU1 = kTupE1 / kThreads;
+
+struct TupleRowSums
+{
+  double s0, s1, t0, t1;   // first-matrix sums (rows 0/1 of a block), second-matrix sums
+};
+
+__device__ __forceinline__ TupleRowSums tuple_row_sums(const TupleDev& M, const double* __restrict__ x, bool vel,
+                                                      uint64_t rb, int nr)
+{
+  __shared__ double4 s_v0[kTupE0];   // 32 KB: A blocks (D: .x .y)
+  __shared__ double2 s_x0[kTupE0];   // 16 KB
+  __shared__ double2 s_v1[kTupE1];   //  8 KB: B blocks (C: .x)
+  __shared__ double s_x1[kTupE1];    //  4 KB
+  __shared__ uint64_t s_p0[kTupRows + 1], s_p1[kTupRows + 1];
+  const int tid = threadIdx.x;
+  const int m0 = vel ? 0 : 2, m1 = vel ? 1 : 3;
+  const bool has1 = M.val[m1] != nullptr;
+  const double* xp = x + 2 * M.nv;
+  if(tid <= nr)
+  {
+    s_p0[tid] = M.ptr[m0][rb + tid];
+    s_p1[tid] = has1 ? M.ptr[m1][rb + tid] : 0;
+  }
+  __syncthreads();
+  const uint64_t b0 = s_p0[0], e0 = s_p0[nr], b1 = s_p1[0], e1 = s_p1[nr];
+  const uint64_t n0 = (e0 - b0 + kTupE0 - 1) / kTupE0, n1 = (e1 - b1 + kTupE1 - 1) / kTupE1;
+  const uint64_t npass = n0 > n1 ? n0 : n1;
+  const bool rl = tid < nr;
+  uint64_t ra0 = 0, re0 = 0, ra1 = 0, re1 = 0;
+  if(rl)
+  {
+    ra0 = s_p0[tid];
+    re0 = s_p0[tid + 1];
+    ra1 = s_p1[tid];
+    re1 = s_p1[tid + 1];
+  }
+  TupleRowSums o{0.0, 0.0, 0.0, 0.0};
+  const double* v0 = M.val[m0];
+  const uint64_t* c0 = M.col[m0];
+  const double* v1 = M.val[m1];
+  const uint64_t* c1 = M.col[m1];
+  for(uint64_t q = 0; q < npass; ++q)
+  {
+    const uint64_t w0 = b0 + q * kTupE0, w1 = b1 + q * kTupE1;
+    double4 a[kTupU0];
+    double2 xa[kTupU0], b[kTupU1];
+    uint64_t ca[kTupU0], cb[kTupU1];
+    double xb[kTupU1];
+#pragma unroll
+    for(int u = 0; u < kTupU0; ++u)
+    {
+      const uint64_t e = w0 + uint64_t(tid + u * kThreads);
+      ca[u] = 0;
+      a[u] = double4{0.0, 0.0, 0.0, 0.0};
+      if(e < e0)
+      {
+        if(vel)
+          a[u] = reinterpret_cast<const double4*>(v0)[e];
+        else
+        {
+          const double2 d = reinterpret_cast<const double2*>(v0)[e];
+          a[u].x = d.x;
+          a[u].y = d.y;
+        }
+        ca[u] = c0[e];
+      }
+    }
+#pragma unroll
+    for(int u = 0; u < kTupU1; ++u)
+    {
+      const uint64_t e = w1 + uint64_t(tid + u * kThreads);
+      cb[u] = 0;
+      b[u] = double2{0.0, 0.0};
+      if(e < e1)
+      {
+        if(vel)
+          b[u] = reinterpret_cast<const double2*>(v1)[e];
+        else
+          b[u].x = v1[e];
+        cb[u] = c1[e];
+      }
+    }
+#pragma unroll
+    for(int u = 0; u < kTupU0; ++u)
+    {
+      xa[u] = double2{0.0, 0.0};
+      if(w0 + uint64_t(tid + u * kThreads) < e0) xa[u] = reinterpret_cast<const double2*>(x)[ca[u]];
+    }
+#pragma unroll
+    for(int u = 0; u < kTupU1; ++u)
+    {
+      xb[u] = 0.0;
+      if(w1 + uint64_t(tid + u * kThreads) < e1) xb[u] = xp[cb[u]];
+    }
+#pragma unroll
+    for(int u = 0; u < kTupU0; ++u)
+      if(w0 + uint64_t(tid + u * kThreads) < e0)
+      {
+        s_v0[tid + u * kThreads] = a[u];
+        s_x0[tid + u * kThreads] = xa[u];
+      }
+#pragma unroll
+    for(int u = 0; u < kTupU1; ++u)
+      if(w1 + uint64_t(tid + u * kThreads) < e1)
+      {
+        s_v1[tid + u * kThreads] = b[u];
+        s_x1[tid + u * kThreads] = xb[u];
+      }
+    __syncthreads();
+    if(rl)
+    {
+      const uint64_t lo0 = ra0 > w0 ? ra0 : w0, h0 = w0 + kTupE0, hi0 = re0 < h0 ? re0 : h0;
+      for(uint64_t k = lo0; k < hi0; ++k)
+      {
+        const double4 v = s_v0[k - w0];
+        const double2 xv = s_x0[k - w0];
+        o.s0 = dfma(v.x, xv.x, o.s0);
+        o.s0 = dfma(v.y, xv.y, o.s0);
+        if(vel)
+        {
+          o.s1 = dfma(v.z, xv.x, o.s1);
+          o.s1 = dfma(v.w, xv.y, o.s1);
+        }
+      }
+      const uint64_t lo1 = ra1 > w1 ? ra1 : w1, h1 = w1 + kTupE1, hi1 = re1 < h1 ? re1 : h1;
+      for(uint64_t k = lo1; k < hi1; ++k)
+      {
+        const double2 v = s_v1[k - w1];
+        const double xv = s_x1[k - w1];
+        o.t0 = dfma(v.x, xv, o.t0);
+        if(vel) o.t1 = dfma(v.y, xv, o.t1);
+      }
+    }
+    __syncthreads();
+  }
+  return o;
+}
+
+// The pressure mean filter of MeanFilter::filter_def / filter_cor / filter_sol
+// (mean_filter.hpp:200-262) on v[0, n), `passes` times: dot = v . w_dot, alpha = -dot/volume
+// (sol_form: 0 - dot/volume), v += alpha w_add unless |alpha| < eps (DenseVector::axpy's copy,
+// dense_vector.hpp:815-820).  The dot is summed as by 1,024 threads: virtual thread t adds
+// v_i w_i for i = t, t + 1,024, ... in order, each wave of 64 virtual threads is summed by
+// wave_sum, the 16 wave sums in order -- so a workgroup of NT = 256 or 1,024 threads gives
+// the same bits.  Every thread of the workgroup calls it.
+constexpr int kMeanThreads = 1024;
+constexpr int kMeanBatch = 4;    // elements per virtual thread whose loads are issued together
+constexpr int kUpdBatch = 16;    // elements per thread per batch of an update loop
+
+// v[i] = a * w[i] + v[i] over [0, n) by the NT threads of the workgroup, kUpdBatch loads of
+// each thread in flight
+template<int NT>
+__device__ __forceinline__ void axpy_block(double* __restrict__ v, double a, const double* __restrict__ w, uint64_t n)
+{
+  for(uint64_t i0 = threadIdx.x; i0 < n; i0 += uint64_t(NT) * kUpdBatch)
+  {
+    double vv[kUpdBatch], ww[kUpdBatch];
+#pragma unroll
+    for(int u = 0; u < kUpdBatch; ++u)
+    {
+      const uint64_t i = i0 + uint64_t(u) * NT;
+      vv[u] = i < n ? v[i] : 0.0;
+      ww[u] = i < n ? w[i] : 0.0;
+    }
+#pragma unroll
+    for(int u = 0; u < kUpdBatch; ++u)
+    {
+      const uint64_t i = i0 + uint64_t(u) * NT;
+      if(i < n) v[i] = dfma(a, ww[u], vv[u]);
+    }
+  }
+}
+
+template<int NT>
+__device__ void mean_passes_block(double* __restrict__ v, const double* __restrict__ w_dot,
+                                  const double* __restrict__ w_add, double volume, uint64_t n, int passes,
+                                  int sol_form, double* red /* kMeanThreads / 64 */, double* s_alpha)
+{
+  static_assert(NT == 256 || NT == 1024, "workgroup size");
+  constexpr int kV = kMeanThreads / NT;   // virtual threads per thread: T + k NT, k < kV
+  const int T = threadIdx.x, w = T >> 6;
+  for(int pass = 0; pass < passes; ++pass)
+  {
+    double part[kV];
+#pragma unroll
+    for(int k = 0; k < kV; ++k) part[k] = 0.0;
+    // element j of virtual thread t is i = t + j * 1024; kMeanBatch of them per round
+    for(uint64_t j0 = 0; uint64_t(T) + j0 * kMeanThreads < n; j0 += kMeanBatch)
+    {
+      double a[kV][kMeanBatch], b[kV][kMeanBatch];
+#pragma unroll
+      for(int k = 0; k < kV; ++k)
+#pragma unroll
+        for(int u = 0; u < kMeanBatch; ++u)
+        {
+          const uint64_t i = uint64_t(T + k * NT) + (j0 + u) * kMeanThreads;
+          a[k][u] = i < n ? v[i] : 0.0;
+          b[k][u] = i < n ? w_dot[i] : 0.0;
+        }
+#pragma unroll
+      for(int k = 0; k < kV; ++k)
+#pragma unroll
+        for(int u = 0; u < kMeanBatch; ++u)
+          if(uint64_t(T + k * NT) + (j0 + u) * kMeanThreads < n) part[k] += a[k][u] * b[k][u];
+    }
+#pragma unroll
+    for(int k = 0; k < kV; ++k)
+    {
+      const double ws = wave_sum(part[k]);
+      if((T & 63) == 0) red[w + k * (NT / 64)] = ws;
+    }
+    __syncthreads();
+    if(T == 0)
+    {
+      double dot = red[0];
+      for(int i = 1; i < kMeanThreads / 64; ++i) dot += red[i];
+      *s_alpha = sol_form ? 0.0 - dot / volume : -dot / volume;
+    }
+    __syncthreads();
+    const double alpha = *s_alpha;
+    if(fabs(alpha) >= DBL_EPSILON) axpy_block<NT>(v, alpha, w_add, n);
+    __syncthreads();
+  }
+}
+
+// gate of the smoother update (k_axpy_gated): skipped when the smoother's initial defect norm
+// is not finite or below eps^2
+__device__ __forceinline__ bool gate_open(const double* gate)
+{
+  if(!gate) return true;
+  const double g = *gate;
+  return isfinite(g) && g > DBL_EPSILON * DBL_EPSILON;
+}
+
+// What follows a tuple apply in the same launch: the pressure mean filter on r (passes 0 =
+// none) and the smoother update x_upd += omega r (gated; x_upd null = none).
+struct TuplePost
+{
+  const double* w_dot;
+  const double* w_add;
+  double volume;
+  int passes, sol_form;
+  double* x_upd;
+  double omega;
+  const double* gate;
+  uint32_t* counter;   // zero between launches; counts the pressure workgroups in
+  int tail;            // 1: filter + pressure update in the last workgroup; 0: left to k_mean_passes
+};
+
+// Pressure parts above this many rows are filtered by k_mean_passes (1,024 threads) in a launch
+// of its own: one 256-thread workgroup needs ~15 dependent memory round trips for 12,288 rows
+// (Q2/P1dc level 6: 36 us fused against 21 us for the separate launches; 14 against 19 us at
+// level 5 with 3,072).
+constexpr uint64_t kTupTailMaxP = 4096;
+
+// r = K x (y null) or r = y + alpha K x, velocity rows zeroed where vmask is set
+// (SaddlePointMatrix::apply, saddle_point_matrix.hpp:446-491: A.apply then B.apply into the
+// velocity part, D.apply then C.apply into the pressure part).  Workgroups 0 .. nvb-1 take
+// the velocity rows, the rest the pressure rows.  POST: then the TuplePost operations --
+// velocity workgroups update their own rows of x_upd (the filter leaves velocity rows alone);
+// pressure workgroups publish their rows and count in, and the last one filters the whole
+// pressure part and updates x_upd's.
+template<bool POST>
 __global__ __launch_bounds__(kThreads) void k_tuple_apply(TupleDev M, double* __restrict__ r,
                                                           const double* __restrict__ x,
                                                           const double* __restrict__ y, double alpha,
-                                                          const uint32_t* __restrict__ vmask)
+                                                          const uint32_t* __restrict__ vmask, uint32_t nvb,
+                                                          TuplePost post)
 {
-  const uint64_t nrows = M.nv + M.np;
-  const double* xp = x + 2 * M.nv;
-  for(uint64_t row = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; row < nrows;
-      row += uint64_t(gridDim.x) * blockDim.x)
+  const bool vel = blockIdx.x < nvb;
+  const uint64_t nrows = vel ? M.nv : M.np;
+  const uint64_t rb = uint64_t(vel ? blockIdx.x : blockIdx.x - nvb) * kTupRows;
+  const uint64_t left = nrows - rb;
+  const int nr = int(left < uint64_t(kTupRows) ? left : uint64_t(kTupRows));
+  const bool rl = int(threadIdx.x) < nr;
+  const uint64_t row = rb + threadIdx.x;
+  // the row's own operands, loaded before the staging
+  double y0 = 0.0, y1 = 0.0, u0 = 0.0, u1 = 0.0;
+  bool masked = false;
+  const bool upd = POST && post.x_upd != nullptr && vel && fabs(post.omega) >= DBL_EPSILON && gate_open(post.gate);
+  if(rl)
   {
-    if(row < M.nv)
+    if(vel)
     {
-      double s0 = 0.0, s1 = 0.0;
-      uint64_t k = M.ptr[0][row];
-      const uint64_t ke = M.ptr[0][row + 1];
-      // blocks are fetched kUnr at a time before any x is gathered (same fma order)
-      for(; k + kUnr <= ke; k += kUnr)
+      if(y)
       {
-        double4 b[kUnr];
-        uint64_t c[kUnr];
-        double2 xv[kUnr];
-#pragma unroll
-        for(int u = 0; u < kUnr; ++u)
-        {
-          b[u] = reinterpret_cast<const double4*>(M.val[0])[k + u];
-          c[u] = M.col[0][k + u];
-        }
-#pragma unroll
-        for(int u = 0; u < kUnr; ++u) xv[u] = reinterpret_cast<const double2*>(x)[c[u]];
-#pragma unroll
-        for(int u = 0; u < kUnr; ++u)
-        {
-          s0 = dfma(b[u].x, xv[u].x, s0);
-          s0 = dfma(b[u].y, xv[u].y, s0);
-          s1 = dfma(b[u].z, xv[u].x, s1);
-          s1 = dfma(b[u].w, xv[u].y, s1);
-        }
+        y0 = y[2 * row];
+        y1 = y[2 * row + 1];
       }
-      for(; k < ke; ++k)
+      masked = is_masked(vmask, row);
+      if(upd)
       {
-        const double* b = M.val[0] + 4 * k;
-        const uint64_t c = M.col[0][k];
-        const double x0 = x[2 * c], x1 = x[2 * c + 1];
-        s0 = dfma(b[0], x0, s0);
-        s0 = dfma(b[1], x1, s0);
-        s1 = dfma(b[2], x0, s1);
-        s1 = dfma(b[3], x1, s1);
+        u0 = post.x_upd[2 * row];
+        u1 = post.x_upd[2 * row + 1];
       }
+    }
+    else if(y)
+      y0 = y[2 * M.nv + row];
+  }
+  const TupleRowSums o = tuple_row_sums(M, x, vel, rb, nr);
+  const double a1 = y ? alpha : 1.0;
+  if(rl)
+  {
+    if(vel)
+    {
       // A.apply(r, x_u, y_u, alpha): r = sum*alpha + beta*y (beta = 0 without y)
-      double r0 = y ? dfma(s0, alpha, y[2 * row]) : dfma(s0, 1.0, 0.0);
-      double r1 = y ? dfma(s1, alpha, y[2 * row + 1]) : dfma(s1, 1.0, 0.0);
-      const double a1 = y ? alpha : 1.0;
+      double r0 = y ? dfma(o.s0, alpha, y0) : dfma(o.s0, 1.0, 0.0);
+      double r1 = y ? dfma(o.s1, alpha, y1) : dfma(o.s1, 1.0, 0.0);
       if(M.val[1])
       {
-        double t0 = 0.0, t1 = 0.0;
-        uint64_t kb = M.ptr[1][row];
-        const uint64_t kbe = M.ptr[1][row + 1];
-        for(; kb + kUnr <= kbe; kb += kUnr)
-        {
-          double2 b[kUnr];
-          uint64_t c[kUnr];
-          double xv[kUnr];
-#pragma unroll
-          for(int u = 0; u < kUnr; ++u)
-          {
-            b[u] = reinterpret_cast<const double2*>(M.val[1])[kb + u];
-            c[u] = M.col[1][kb + u];
-          }
-#pragma unroll
-          for(int u = 0; u < kUnr; ++u) xv[u] = xp[c[u]];
-#pragma unroll
-          for(int u = 0; u < kUnr; ++u)
-          {
-            t0 = dfma(b[u].x, xv[u], t0);
-            t1 = dfma(b[u].y, xv[u], t1);
-          }
-        }
-        for(; kb < kbe; ++kb)
-        {
-          const double* b = M.val[1] + 2 * kb;
-          const double xv = xp[M.col[1][kb]];
-          t0 = dfma(b[0], xv, t0);
-          t1 = dfma(b[1], xv, t1);
-        }
         // B.apply(r, x_p, r, alpha)
-        r0 = dfma(t0, a1, r0);
-        r1 = dfma(t1, a1, r1);
+        r0 = dfma(o.t0, a1, r0);
+        r1 = dfma(o.t1, a1, r1);
       }
-      if(is_masked(vmask, row)) r0 = r1 = 0.0;
+      if(masked) r0 = r1 = 0.0;
       r[2 * row] = r0;
       r[2 * row + 1] = r1;
+      if(upd)
+      {
+        post.x_upd[2 * row] = dfma(post.omega, r0, u0);
+        post.x_upd[2 * row + 1] = dfma(post.omega, r1, u1);
+      }
     }
     else
     {
-      const uint64_t pr = row - M.nv;
-      double s = 0.0;
-      uint64_t k = M.ptr[2][pr];
-      const uint64_t ke = M.ptr[2][pr + 1];
-      for(; k + kUnr <= ke; k += kUnr)
-      {
-        double2 b[kUnr], xv[kUnr];
-        uint64_t c[kUnr];
-#pragma unroll
-        for(int u = 0; u < kUnr; ++u)
-        {
-          b[u] = reinterpret_cast<const double2*>(M.val[2])[k + u];
-          c[u] = M.col[2][k + u];
-        }
-#pragma unroll
-        for(int u = 0; u < kUnr; ++u) xv[u] = reinterpret_cast<const double2*>(x)[c[u]];
-#pragma unroll
-        for(int u = 0; u < kUnr; ++u)
-        {
-          s = dfma(b[u].x, xv[u].x, s);
-          s = dfma(b[u].y, xv[u].y, s);
-        }
-      }
-      for(; k < ke; ++k)
-      {
-        const double* b = M.val[2] + 2 * k;
-        const uint64_t c = M.col[2][k];
-        s = dfma(b[0], x[2 * c], s);
-        s = dfma(b[1], x[2 * c + 1], s);
-      }
-      double rp = y ? dfma(s, alpha, y[2 * M.nv + pr]) : dfma(s, 1.0, 0.0);
-      if(M.val[3])
-      {
-        double t = 0.0;
-        for(uint64_t k = M.ptr[3][pr]; k < M.ptr[3][pr + 1]; ++k)
-          t = dfma(M.val[3][k], xp[M.col[3][k]], t);
-        rp = dfma(t, y ? alpha : 1.0, rp);
-      }
-      r[2 * M.nv + pr] = rp;
+      double rp = y ? dfma(o.s0, alpha, y0) : dfma(o.s0, 1.0, 0.0);
+      if(M.val[3]) rp = dfma(o.t0, a1, rp);
+      r[2 * M.nv + row] = rp;
     }
+  }
+  if constexpr(POST)
+  {
+    if(vel || !post.tail) return;
+    __shared__ double red[kMeanThreads / 64];
+    __shared__ double s_alpha;
+    __shared__ int last;
+    // publish this workgroup's pressure rows (agent-scope release by every writer), then count in
+    __threadfence();
+    __syncthreads();
+    if(threadIdx.x == 0)
+    {
+      const uint32_t npb = gridDim.x - nvb;
+      const uint32_t prev = __hip_atomic_fetch_add(post.counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      last = (prev == npb - 1) ? 1 : 0;
+    }
+    __syncthreads();
+    if(!last) return;
+    __threadfence();   // acquire: the other workgroups' rows
+    double* rp = r + 2 * M.nv;
+    mean_passes_block<kThreads>(rp, post.w_dot, post.w_add, post.volume, M.np, post.passes, post.sol_form, red,
+                                &s_alpha);
+    if(post.x_upd && fabs(post.omega) >= DBL_EPSILON && gate_open(post.gate))
+      axpy_block<kThreads>(post.x_upd + 2 * M.nv, post.omega, rp, M.np);
+    if(threadIdx.x == 0) __hip_atomic_store(post.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 
 // ---------------------------------------------------------------------------
 // mean filter passes (one workgroup) and small vector kernels
 // ---------------------------------------------------------------------------
-constexpr int kMeanThreads = 1024;
+// the passes, then (x_upd != null) x_upd += omega v gated as k_axpy_gated
 __global__ __launch_bounds__(kMeanThreads) void k_mean_passes(double* __restrict__ v, const double* __restrict__ w_dot,
                                                               const double* __restrict__ w_add, double volume,
-                                                              uint64_t n, int passes, int sol_form)
+                                                              uint64_t n, int passes, int sol_form,
+                                                              double* __restrict__ x_upd, double omega,
+                                                              const double* __restrict__ gate)
 {
   __shared__ double red[kMeanThreads / 64];
   __shared__ double s_alpha;
-  for(int pass = 0; pass < passes; ++pass)
-  {
-    // thread t adds v_i w_i for i = t, t + 1024, ... in order, four products fetched ahead
-    double part = 0.0;
-    uint64_t i = threadIdx.x;
-    for(; i + 3 * kMeanThreads < n; i += 4 * kMeanThreads)
-    {
-      double a[4], b[4];
-#pragma unroll
-      for(int u = 0; u < 4; ++u)
-      {
-        a[u] = v[i + u * kMeanThreads];
-        b[u] = w_dot[i + u * kMeanThreads];
-      }
-#pragma unroll
-      for(int u = 0; u < 4; ++u) part += a[u] * b[u];
-    }
-    for(; i < n; i += kMeanThreads) part += v[i] * w_dot[i];
-    const double dot = block_sum(part, red);
-    if(threadIdx.x == 0) s_alpha = sol_form ? 0.0 - dot / volume : -dot / volume;
-    __syncthreads();
-    const double alpha = s_alpha;
-    if(fabs(alpha) >= DBL_EPSILON)
-      for(uint64_t i = threadIdx.x; i < n; i += blockDim.x) v[i] = dfma(alpha, w_add[i], v[i]);
-    __syncthreads();
-  }
+  mean_passes_block<kMeanThreads>(v, w_dot, w_add, volume, n, passes, sol_form, red, &s_alpha);
+  if(x_upd && fabs(omega) >= DBL_EPSILON && gate_open(gate)) axpy_block<kMeanThreads>(x_upd, omega, v, n);
 }
 
 __global__ __launch_bounds__(kThreads) void k_sumsq2(const double* __restrict__ v, uint64_t nu, uint64_t np,
@@ -505,9 +714,45 @@ int feat_tuple_apply_f64(feat_handle_t h, const feat_tuple_f64_t* M, double* r, 
   if(r == x) return fail(FEAT_ERR_ALIAS, "feat_tuple_apply_f64: r must not alias x");
   const uint64_t rows = M->nv + M->np;
   if(!rows) return FEAT_OK;
-  hipLaunchKernelGGL(k_tuple_apply, dim3(rows_grid(rows)), dim3(kThreads), 0, h->stream, to_dev(M), r, x, y, alpha,
-                     vmask);
+  const uint64_t nvb = (M->nv + kTupRows - 1) / kTupRows, npb = (M->np + kTupRows - 1) / kTupRows;
+  if(nvb + npb > uint64_t(INT32_MAX)) return fail(FEAT_ERR_UNSUPPORTED, "feat_tuple_apply_f64: too many rows");
+  hipLaunchKernelGGL(k_tuple_apply<false>, dim3(unsigned(nvb + npb)), dim3(kThreads), 0, h->stream, to_dev(M), r, x, y,
+                     alpha, vmask, uint32_t(nvb), TuplePost{});
   return check_launch("feat_tuple_apply_f64");
+}
+
+int feat_tuple_apply_post_f64(feat_handle_t h, const feat_tuple_f64_t* M, double* r, const double* x, const double* y,
+                              double alpha, const uint32_t* vmask, const double* w_dot, const double* w_add,
+                              double volume, int passes, int sol_form, double* x_upd, double omega,
+                              const double* gate_norm, const feat_reduce_ws_t* ws)
+{
+  if(!h || !tuple_ok(M, 0, 0) || !tuple_ok(M, 2, 2) || !r || !x)
+    return fail(FEAT_ERR_INVALID, "feat_tuple_apply_post_f64: null argument");
+  if(M->val[1] && (!M->col[1] || !M->ptr[1])) return fail(FEAT_ERR_INVALID, "feat_tuple_apply_post_f64: block 1");
+  if(M->val[3] && (!M->col[3] || !M->ptr[3])) return fail(FEAT_ERR_INVALID, "feat_tuple_apply_post_f64: block 3");
+  if(passes < 0 || passes > 2 || (passes && M->np && (!w_dot || !w_add)))
+    return fail(FEAT_ERR_INVALID, "feat_tuple_apply_post_f64: mean filter arguments");
+  if(!ws || !ws->counter) return fail(FEAT_ERR_INVALID, "feat_tuple_apply_post_f64: workspace required");
+  if(r == x) return fail(FEAT_ERR_ALIAS, "feat_tuple_apply_post_f64: r must not alias x");
+  if(x_upd && (x_upd == x || x_upd == r || x_upd == y))
+    return fail(FEAT_ERR_ALIAS, "feat_tuple_apply_post_f64: x_upd must not alias x, r or y");
+  const uint64_t rows = M->nv + M->np;
+  if(!rows) return FEAT_OK;
+  const uint64_t nvb = (M->nv + kTupRows - 1) / kTupRows, npb = (M->np + kTupRows - 1) / kTupRows;
+  if(nvb + npb > uint64_t(INT32_MAX)) return fail(FEAT_ERR_UNSUPPORTED, "feat_tuple_apply_post_f64: too many rows");
+  const int tail = M->np <= kTupTailMaxP ? 1 : 0;
+  const TuplePost post{w_dot, w_add, volume, passes, sol_form, x_upd, omega, gate_norm, ws->counter, tail};
+  if(npb && tail)
+    hipLaunchKernelGGL(k_tuple_apply<true>, dim3(unsigned(nvb + npb)), dim3(kThreads), 0, h->stream, to_dev(M), r, x,
+                       y, alpha, vmask, uint32_t(nvb), post);
+  else
+    hipLaunchKernelGGL(k_tuple_apply<true>, dim3(unsigned(nvb + npb)), dim3(kThreads), 0, h->stream, to_dev(M), r, x,
+                       y, alpha, vmask, uint32_t(nvb), TuplePost{w_dot, w_add, volume, passes, sol_form, x_upd, omega,
+                                                                  gate_norm, ws->counter, 0});
+  if(npb && !tail && (passes || x_upd))
+    hipLaunchKernelGGL(k_mean_passes, dim3(1), dim3(kMeanThreads), 0, h->stream, r + 2 * M->nv, w_dot, w_add, volume,
+                       M->np, passes, sol_form, x_upd ? x_upd + 2 * M->nv : nullptr, omega, gate_norm);
+  return check_launch("feat_tuple_apply_post_f64");
 }
 
 int feat_mean_filter_passes_f64(feat_handle_t h, double* v, const double* w_dot, const double* w_add,
@@ -517,7 +762,7 @@ int feat_mean_filter_passes_f64(feat_handle_t h, double* v, const double* w_dot,
     return fail(FEAT_ERR_INVALID, "feat_mean_filter_passes_f64: bad argument");
   if(!n || !passes) return FEAT_OK;
   hipLaunchKernelGGL(k_mean_passes, dim3(1), dim3(kMeanThreads), 0, h->stream, v, w_dot, w_add, volume, n, passes,
-                     sol_form);
+                     sol_form, nullptr, 0.0, nullptr);
   return check_launch("feat_mean_filter_passes_f64");
 }
 

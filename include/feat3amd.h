@@ -435,6 +435,19 @@ typedef struct {
  * (UnitFilterBlocked::filter_def, unit_filter_blocked.hpp:478-499).  r must not alias x. */
 int feat_tuple_apply_f64(feat_handle_t h, const feat_tuple_f64_t* M, double* r, const double* x,
                          const double* y, double alpha, const uint32_t* vmask);
+/* The same apply followed by what the Stokes smoother does next
+ * (richardson.hpp:160-226 with SchwarzPrecond(AmaVanka), amavanka.hpp:1562-1567,
+ * schwarz_precond.hpp:155-158): `passes` (0..2) MeanFilter passes on r's pressure part, as
+ * feat_mean_filter_passes_f64(r + 2 nv, w_dot, w_add, volume, np, passes, sol_form), then,
+ * when x_upd != NULL, x_upd += omega * r gated by *gate_norm as feat_axpy_gated_f64.  Results
+ * are bit-identical to the separate calls.  With np <= 4096 all of it is one launch, the
+ * filter running in the workgroup that finishes last (ws->counter: zero before the call and
+ * after it); larger pressure parts take a second, one-workgroup launch for the filter and the
+ * pressure rows of the update.  x_upd must not alias x, r or y. */
+int feat_tuple_apply_post_f64(feat_handle_t h, const feat_tuple_f64_t* M, double* r, const double* x,
+                              const double* y, double alpha, const uint32_t* vmask, const double* w_dot,
+                              const double* w_add, double volume, int passes, int sol_form, double* x_upd,
+                              double omega, const double* gate_norm, const feat_reduce_ws_t* ws);
 
 /* MeanFilter passes on one vector v of length n, in one workgroup
  * (lafem/mean_filter.hpp:213-262): per pass alpha = -(v . w_dot)/volume

@@ -101,6 +101,21 @@ namespace feat3amd
       }
     };
 
+    /// One launch (two above 4,096 pressure rows): r = y + alpha M x (r = M x without y), velocity
+    /// rows zeroed by vmask, then
+    /// `passes` MeanFilter passes on r's pressure part (weights w_dot, w_add, volume), then
+    /// x_upd += omega r gated by *gate (x_upd null: no update) -- bit-identical to
+    /// feat_tuple_apply_f64, feat_mean_filter_passes_f64 and feat_axpy_gated_f64 in a row.
+    inline void tuple_apply_post(const feat_tuple_f64_t& M, StokesVector& r, const StokesVector& x,
+                                 const StokesVector* y, double alpha, const std::uint32_t* vmask, const double* w_dot,
+                                 const double* w_add, double volume, int passes, double* x_upd, double omega,
+                                 const double* gate, const feat_reduce_ws_t& ws)
+    {
+      check(feat_tuple_apply_post_f64(Runtime::handle(), &M, r.elements(), x.elements(), y ? y->elements() : nullptr,
+                                      alpha, vmask, w_dot, w_add, volume, passes, 0, x_upd, omega, gate, &ws),
+            "tuple apply + filter + update");
+    }
+
     /// TupleFilter<UnitFilterBlocked<DT,IT,2>, MeanFilter<DT,IT>> on [u | p]
     /// (lafem/unit_filter_blocked.hpp:449-499, lafem/mean_filter.hpp:200-262)
     class StokesFilter
@@ -398,6 +413,15 @@ namespace feat3amd
         _filter.mean_cor(c, cor_passes);
       }
 
+      /// apply(c, d, cor_passes) followed by x_upd += omega c gated by *gate (the Richardson
+      /// step, richardson.hpp:203-211), in one or two launches (x_upd null: no update)
+      void apply_update(LAFEM::StokesVector& c, const LAFEM::StokesVector& d, int cor_passes, double* x_upd,
+                        double omega, const double* gate, const feat_reduce_ws_t& ws) const
+      {
+        LAFEM::tuple_apply_post(vanka_desc(), c, d, nullptr, 1.0, _filter.vmask(), _filter.dual(), _filter.prim(),
+                                _filter.volume(), cor_passes, x_upd, omega, gate, ws);
+      }
+
       /// host copy of Vanka block k (0: uu, 1: up, 2: pu, 3: pp) and the macro mask
       void copy_block(int k, std::uint64_t* ptr, std::uint64_t* col, double* val) const
       {
@@ -474,23 +498,23 @@ namespace feat3amd
         feat_handle_t h = Runtime::handle();
         sol.format();
         check(feat_tuple_norm2_f64(h, rhs.elements(), 2 * L.K.nv(), L.K.np(), L.gate.get(), &_fw.ws), "smoother norm");
+        // each step: c = filter_cor(V d) with sol += omega c, then d = filter_def(rhs - K sol),
+        // one launch each on small levels
+        const feat_tuple_f64_t K = L.K.desc();
         for(int s = 0; s < _cfg.steps; ++s)
         {
           const LAFEM::StokesVector& d = (s == 0) ? rhs : L.rdef;
-          L.V->apply(L.rcor, d, 2);
-          check(feat_axpy_gated_f64(h, sol.elements(), _cfg.omega_rich, L.rcor.elements(), n(L), L.gate.get()), "smoother update");
+          L.V->apply_update(L.rcor, d, 2, sol.elements(), _cfg.omega_rich, L.gate.get(), _fw.ws);
           if(s + 1 < _cfg.steps)
-          {
-            L.K.apply(L.rdef, sol, rhs, -1.0, L.F.vmask());
-            L.F.mean_def(L.rdef);
-          }
+            LAFEM::tuple_apply_post(K, L.rdef, sol, &rhs, -1.0, L.F.vmask(), L.F.prim(), L.F.dual(), L.F.volume(), 1,
+                                    nullptr, 0.0, nullptr, _fw.ws);
         }
       }
 
       void defect(Level& L, LAFEM::StokesVector& d, const LAFEM::StokesVector& x, const LAFEM::StokesVector& b)
       {
-        L.K.apply(d, x, b, -1.0, L.F.vmask());
-        L.F.mean_def(d);
+        LAFEM::tuple_apply_post(L.K.desc(), d, x, &b, -1.0, L.F.vmask(), L.F.prim(), L.F.dual(), L.F.volume(), 1, nullptr,
+                                0.0, nullptr, _fw.ws);
       }
 
       // MultiGrid::apply with the V-cycle (multigrid.hpp:1270-1350, 1526-1570, 1713-1980)
@@ -508,7 +532,7 @@ namespace feat3amd
           C.F.filter_def(C.rhs);
         }
         Level& Lc = *_lv[nl - 1];
-        Lc.V->apply(Lc.sol, Lc.rhs, 2);
+        Lc.V->apply_update(Lc.sol, Lc.rhs, 2, nullptr, 0.0, nullptr, _fw.ws);
         for(std::size_t i = nl - 1; i-- > 0;)
         {
           Level& L = *_lv[i];
