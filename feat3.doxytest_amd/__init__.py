@@ -188,6 +188,7 @@ KERNEL_API = [
     ("feat_convert_dot_f32_f64", _i, [_vp, _vp, _vp, _vp, _u64, _vp, C.POINTER(ReduceWs)]),
     ("feat_dense_gemv_mixed", _i, [_vp, _i, _vp, _vp, _vp, _vp]),
     ("feat_tuple_apply_f64", _i, [_vp, _vp, _vp, _vp, _vp, _d, _vp]),
+    ("feat_stokes_tail_f64", _i, [_vp, _vp, _vp, _vp]),
     ("feat_tuple_apply_post_f64", _i, [_vp, _vp, _vp, _vp, _vp, _d, _vp, _vp, _vp, _d, _i, _i, _vp, _d, _vp, _vp]),
     ("feat_mean_filter_passes_f64", _i, [_vp, _vp, _vp, _vp, _d, _u64, _i, _i]),
     ("feat_tuple_norm2_f64", _i, [_vp, _vp, _u64, _u64, _vp, _vp]),
@@ -699,7 +700,8 @@ def runtime_sell_flags(flags):
 
 class StokesCfg(C.Structure):
     _fields_ = [("steps", _i), ("omega_rich", _d), ("vanka_omega", _d), ("coarse_skip_singular", _i),
-                ("omega_outer", _d), ("tol_rel", _d), ("max_iter", _u64), ("no_graphs", _i)]
+                ("omega_outer", _d), ("tol_rel", _d), ("max_iter", _u64), ("no_graphs", _i),
+                ("no_coarse_fusion", _i), ("tail_max_level_dofs", _u64)]
 
 
 class Stokes:
@@ -709,12 +711,13 @@ class Stokes:
     KIND = {"rhs": 0, "sol": 1, "def": 2, "cor": 3}
 
     def __init__(self, lf, lc, steps=5, omega_rich=0.7, vanka_omega=1.0, coarse_skip_singular=True,
-                 omega_outer=1.0, tol_rel=1e-8, max_iter=100, graphs=True):
+                 omega_outer=1.0, tol_rel=1e-8, max_iter=100, graphs=True, coarse_fusion=True,
+                 tail_max_level_dofs=0):
         _, self.hl = libs()
         runtime_init()
         self.s = C.c_void_p()
         self.cfg = StokesCfg(steps, omega_rich, vanka_omega, int(coarse_skip_singular), omega_outer, tol_rel,
-                             max_iter, int(not graphs))
+                             max_iter, int(not graphs), int(not coarse_fusion), tail_max_level_dofs)
         check(self.hl.feat_stokes_create(C.byref(self.s), lf, lc, C.byref(self.cfg)), "feat_stokes_create")
 
     def sizes(self, lvl=0):

@@ -152,8 +152,32 @@ __device__ __forceinline__ TupleRowSums tuple_row_sums(const TupleDev& M, const 
     __syncthreads();
     if(rl)
     {
+      // four entries' LDS loads before their fmas (same fma order)
       const uint64_t lo0 = ra0 > w0 ? ra0 : w0, h0 = w0 + kTupE0, hi0 = re0 < h0 ? re0 : h0;
-      for(uint64_t k = lo0; k < hi0; ++k)
+      uint64_t k = lo0;
+      for(; k + 4 <= hi0; k += 4)
+      {
+        double4 v[4];
+        double2 xv[4];
+#pragma unroll
+        for(int u = 0; u < 4; ++u)
+        {
+          v[u] = s_v0[k + u - w0];
+          xv[u] = s_x0[k + u - w0];
+        }
+#pragma unroll
+        for(int u = 0; u < 4; ++u)
+        {
+          o.s0 = dfma(v[u].x, xv[u].x, o.s0);
+          o.s0 = dfma(v[u].y, xv[u].y, o.s0);
+          if(vel)
+          {
+            o.s1 = dfma(v[u].z, xv[u].x, o.s1);
+            o.s1 = dfma(v[u].w, xv[u].y, o.s1);
+          }
+        }
+      }
+      for(; k < hi0; ++k)
       {
         const double4 v = s_v0[k - w0];
         const double2 xv = s_x0[k - w0];
@@ -166,7 +190,25 @@ __device__ __forceinline__ TupleRowSums tuple_row_sums(const TupleDev& M, const 
         }
       }
       const uint64_t lo1 = ra1 > w1 ? ra1 : w1, h1 = w1 + kTupE1, hi1 = re1 < h1 ? re1 : h1;
-      for(uint64_t k = lo1; k < hi1; ++k)
+      k = lo1;
+      for(; k + 4 <= hi1; k += 4)
+      {
+        double2 v[4];
+        double xv[4];
+#pragma unroll
+        for(int u = 0; u < 4; ++u)
+        {
+          v[u] = s_v1[k + u - w1];
+          xv[u] = s_x1[k + u - w1];
+        }
+#pragma unroll
+        for(int u = 0; u < 4; ++u)
+        {
+          o.t0 = dfma(v[u].x, xv[u], o.t0);
+          if(vel) o.t1 = dfma(v[u].y, xv[u], o.t1);
+        }
+      }
+      for(; k < hi1; ++k)
       {
         const double2 v = s_v1[k - w1];
         const double xv = s_x1[k - w1];
@@ -188,13 +230,13 @@ __device__ __forceinline__ TupleRowSums tuple_row_sums(const TupleDev& M, const 
 // the same bits.  Every thread of the workgroup calls it.
 constexpr int kMeanThreads = 1024;
 constexpr int kMeanBatch = 4;    // elements per virtual thread whose loads are issued together
-constexpr int kUpdBatch = 16;    // elements per thread per batch of an update loop
 
 // v[i] = a * w[i] + v[i] over [0, n) by the NT threads of the workgroup, kUpdBatch loads of
-// each thread in flight
+// each thread in flight (16 for 256 threads, 4 for 512 or more)
 template<int NT>
 __device__ __forceinline__ void axpy_block(double* __restrict__ v, double a, const double* __restrict__ w, uint64_t n)
 {
+  constexpr int kUpdBatch = NT >= 512 ? 4 : 16;
   for(uint64_t i0 = threadIdx.x; i0 < n; i0 += uint64_t(NT) * kUpdBatch)
   {
     double vv[kUpdBatch], ww[kUpdBatch];
@@ -215,11 +257,11 @@ __device__ __forceinline__ void axpy_block(double* __restrict__ v, double a, con
 }
 
 template<int NT>
-__device__ void mean_passes_block(double* __restrict__ v, const double* __restrict__ w_dot,
+__device__ __forceinline__ void mean_passes_block(double* __restrict__ v, const double* __restrict__ w_dot,
                                   const double* __restrict__ w_add, double volume, uint64_t n, int passes,
                                   int sol_form, double* red /* kMeanThreads / 64 */, double* s_alpha)
 {
-  static_assert(NT == 256 || NT == 1024, "workgroup size");
+  static_assert(NT == 256 || NT == 512 || NT == 1024, "workgroup size");
   constexpr int kV = kMeanThreads / NT;   // virtual threads per thread: T + k NT, k < kV
   const int T = threadIdx.x, w = T >> 6;
   for(int pass = 0; pass < passes; ++pass)
@@ -255,8 +297,12 @@ __device__ void mean_passes_block(double* __restrict__ v, const double* __restri
     __syncthreads();
     if(T == 0)
     {
-      double dot = red[0];
-      for(int i = 1; i < kMeanThreads / 64; ++i) dot += red[i];
+      double rr[kMeanThreads / 64];
+#pragma unroll
+      for(int i = 0; i < kMeanThreads / 64; ++i) rr[i] = red[i];
+      double dot = rr[0];
+#pragma unroll
+      for(int i = 1; i < kMeanThreads / 64; ++i) dot += rr[i];
       *s_alpha = sol_form ? 0.0 - dot / volume : -dot / volume;
     }
     __syncthreads();
@@ -470,6 +516,766 @@ __global__ __launch_bounds__(kThreads) void k_csrsb2(double* __restrict__ r, con
     r[2 * row] = dfma(s0, 1.0, 0.0);
     r[2 * row + 1] = dfma(s1, 1.0, 0.0);
   }
+}
+
+// ---------------------------------------------------------------------------
+// the coarse levels of the Stokes V-cycle in one workgroup
+// ---------------------------------------------------------------------------
+// Every operation of StokesMultiGrid::mg_apply on the tail's levels -- smoother
+// (format, norm gate, AmaVanka apply + filter_cor + gated update, defect + filter_def),
+// defect, restriction + filter_def, coarse AmaVanka apply, prolongation + filter_cor,
+// the two axpys -- with the arithmetic of the kernels the separate launches run, so
+// the tail is bit-identical to them.  The level vectors stay in LDS; matrix blocks and
+// column indices are staged through LDS in passes (every load of a pass issued before
+// the first LDS store), each row then summed by its thread in storage order with x read
+// from the LDS vectors.
+constexpr int kSt = 512;                     // threads of the tail workgroup (256 registers each)
+constexpr int kStE0 = 2048, kStE1 = 1024;    // entries per pass: A / D / transfer, B / C
+constexpr int kStMaxLevels = FEAT_STOKES_TAIL_MAX_LEVELS;
+
+struct StStage
+{
+  double4 v0[kStE0];    // 64 KB (transfers: 8,192 doubles)
+  uint32_t c0[kStE0];   //  8 KB
+  double2 v1[kStE1];    // 16 KB
+  uint32_t c1[kStE1];   //  4 KB
+};
+
+struct StCsr
+{
+  const double* val;
+  const uint64_t* col;
+  const uint64_t* ptr;
+  uint64_t rows;
+};
+
+struct StLevel
+{
+  TupleDev K, V;
+  const uint32_t* vmask;
+  const double* prim;
+  const double* dual;
+  double vol;
+  StCsr rv, rp, pv, pp;   // restriction to / prolongation from the next coarser level
+};
+
+struct StTail
+{
+  int nl, steps;
+  double omega;
+  StLevel lv[kStMaxLevels];
+};
+
+// Row sums over staged entries [lo, hi) of a pass starting at w, in storage order: the
+// entries' LDS loads and the x they address are fetched four at a time before their fmas.
+// First matrix (A: 2x2 blocks, vel; D: 1x2 blocks), x with two components per column:
+__device__ __forceinline__ void st_sum0(const StStage& S, uint64_t w, uint64_t lo, uint64_t hi, const double* x,
+                                        bool vel, double& s0, double& s1)
+{
+  uint64_t k = lo;
+  for(; k + 4 <= hi; k += 4)
+  {
+    double4 v[4];
+    uint32_t c[4];
+    double x0[4], x1[4];
+#pragma unroll
+    for(int u = 0; u < 4; ++u)
+    {
+      v[u] = S.v0[k + u - w];
+      c[u] = S.c0[k + u - w];
+    }
+#pragma unroll
+    for(int u = 0; u < 4; ++u)
+    {
+      x0[u] = x[2 * c[u]];
+      x1[u] = x[2 * c[u] + 1];
+    }
+#pragma unroll
+    for(int u = 0; u < 4; ++u)
+    {
+      s0 = dfma(v[u].x, x0[u], s0);
+      s0 = dfma(v[u].y, x1[u], s0);
+      if(vel)
+      {
+        s1 = dfma(v[u].z, x0[u], s1);
+        s1 = dfma(v[u].w, x1[u], s1);
+      }
+    }
+  }
+  for(; k < hi; ++k)
+  {
+    const double4 v = S.v0[k - w];
+    const uint32_t c = S.c0[k - w];
+    const double x0 = x[2 * c], x1 = x[2 * c + 1];
+    s0 = dfma(v.x, x0, s0);
+    s0 = dfma(v.y, x1, s0);
+    if(vel)
+    {
+      s1 = dfma(v.z, x0, s1);
+      s1 = dfma(v.w, x1, s1);
+    }
+  }
+}
+// second matrix (B: 2x1 blocks, vel; C: 1x1), x with one component per column
+__device__ __forceinline__ void st_sum1(const StStage& S, uint64_t w, uint64_t lo, uint64_t hi, const double* xp,
+                                        bool vel, double& t0, double& t1)
+{
+  uint64_t k = lo;
+  for(; k + 4 <= hi; k += 4)
+  {
+    double2 v[4];
+    double xv[4];
+#pragma unroll
+    for(int u = 0; u < 4; ++u) v[u] = S.v1[k + u - w];
+#pragma unroll
+    for(int u = 0; u < 4; ++u) xv[u] = xp[S.c1[k + u - w]];
+#pragma unroll
+    for(int u = 0; u < 4; ++u)
+    {
+      t0 = dfma(v[u].x, xv[u], t0);
+      if(vel) t1 = dfma(v[u].y, xv[u], t1);
+    }
+  }
+  for(; k < hi; ++k)
+  {
+    const double2 v = S.v1[k - w];
+    const double xv = xp[S.c1[k - w]];
+    t0 = dfma(v.x, xv, t0);
+    if(vel) t1 = dfma(v.y, xv, t1);
+  }
+}
+
+// r = y + alpha M x (y null: r = M x) over rows [rb, rb + nr) of one kind, vectors in LDS
+// (k_tuple_apply's arithmetic)
+__device__ __forceinline__ void st_tuple_group(const TupleDev& M, bool vel, uint64_t rb, int nr, double* r, const double* x,
+                               const double* y, double alpha, const uint32_t* vmask, StStage& S)
+{
+  const int t = threadIdx.x;
+  const int m0 = vel ? 0 : 2, m1 = vel ? 1 : 3;
+  const bool has1 = M.val[m1] != nullptr;
+  const uint64_t* p0 = M.ptr[m0];
+  const uint64_t* p1 = M.ptr[m1];
+  const uint64_t b0 = p0[rb], e0 = p0[rb + nr];
+  const uint64_t b1 = has1 ? p1[rb] : 0, e1 = has1 ? p1[rb + nr] : 0;
+  const bool rl = t < nr;
+  uint64_t ra0 = 0, re0 = 0, ra1 = 0, re1 = 0;
+  if(rl)
+  {
+    ra0 = p0[rb + t];
+    re0 = p0[rb + t + 1];
+    if(has1)
+    {
+      ra1 = p1[rb + t];
+      re1 = p1[rb + t + 1];
+    }
+  }
+  const uint64_t n0 = (e0 - b0 + kStE0 - 1) / kStE0, n1 = (e1 - b1 + kStE1 - 1) / kStE1;
+  const uint64_t npass = n0 > n1 ? n0 : n1;
+  const double* v0 = M.val[m0];
+  const uint64_t* c0 = M.col[m0];
+  const double* v1 = M.val[m1];
+  const uint64_t* c1 = M.col[m1];
+  const double* xp = x + 2 * M.nv;
+  double s0 = 0.0, s1 = 0.0, t0 = 0.0, t1 = 0.0;
+  for(uint64_t q = 0; q < npass; ++q)
+  {
+    const uint64_t w0 = b0 + q * kStE0, w1 = b1 + q * kStE1;
+    double4 a[kStE0 / kSt];
+    uint32_t ca[kStE0 / kSt];
+    double2 b[kStE1 / kSt];
+    uint32_t cb[kStE1 / kSt];
+#pragma unroll
+    for(int u = 0; u < kStE0 / kSt; ++u)
+    {
+      const uint64_t e = w0 + uint64_t(t + u * kSt);
+      a[u] = double4{0.0, 0.0, 0.0, 0.0};
+      ca[u] = 0;
+      if(e < e0)
+      {
+        if(vel)
+          a[u] = reinterpret_cast<const double4*>(v0)[e];
+        else
+        {
+          const double2 d = reinterpret_cast<const double2*>(v0)[e];
+          a[u].x = d.x;
+          a[u].y = d.y;
+        }
+        ca[u] = uint32_t(c0[e]);
+      }
+    }
+#pragma unroll
+    for(int u = 0; u < kStE1 / kSt; ++u)
+    {
+      const uint64_t e = w1 + uint64_t(t + u * kSt);
+      b[u] = double2{0.0, 0.0};
+      cb[u] = 0;
+      if(e < e1)
+      {
+        if(vel)
+          b[u] = reinterpret_cast<const double2*>(v1)[e];
+        else
+          b[u].x = v1[e];
+        cb[u] = uint32_t(c1[e]);
+      }
+    }
+#pragma unroll
+    for(int u = 0; u < kStE0 / kSt; ++u)
+    {
+      S.v0[t + u * kSt] = a[u];
+      S.c0[t + u * kSt] = ca[u];
+    }
+#pragma unroll
+    for(int u = 0; u < kStE1 / kSt; ++u)
+    {
+      S.v1[t + u * kSt] = b[u];
+      S.c1[t + u * kSt] = cb[u];
+    }
+    __syncthreads();
+    if(rl)
+    {
+      const uint64_t lo0 = ra0 > w0 ? ra0 : w0, h0 = w0 + kStE0, hi0 = re0 < h0 ? re0 : h0;
+      st_sum0(S, w0, lo0, hi0, x, vel, s0, s1);
+      const uint64_t lo1 = ra1 > w1 ? ra1 : w1, h1 = w1 + kStE1, hi1 = re1 < h1 ? re1 : h1;
+      st_sum1(S, w1, lo1, hi1, xp, vel, t0, t1);
+    }
+    __syncthreads();
+  }
+  if(rl)
+  {
+    const uint64_t row = rb + t;
+    const double a1 = y ? alpha : 1.0;
+    if(vel)
+    {
+      double r0 = y ? dfma(s0, alpha, y[2 * row]) : dfma(s0, 1.0, 0.0);
+      double r1 = y ? dfma(s1, alpha, y[2 * row + 1]) : dfma(s1, 1.0, 0.0);
+      if(has1)
+      {
+        r0 = dfma(t0, a1, r0);
+        r1 = dfma(t1, a1, r1);
+      }
+      if(is_masked(vmask, row)) r0 = r1 = 0.0;
+      r[2 * row] = r0;
+      r[2 * row + 1] = r1;
+    }
+    else
+    {
+      double rp = y ? dfma(s0, alpha, y[2 * M.nv + row]) : dfma(s0, 1.0, 0.0);
+      if(has1) rp = dfma(t0, a1, rp);
+      r[2 * M.nv + row] = rp;
+    }
+  }
+  __syncthreads();
+}
+
+// The same for all rows at once when nv + np <= kSt: the velocity rows' A blocks and the
+// pressure rows' D blocks form one staged sequence (B and C blocks another), so a pass serves
+// both kinds and the row pointers of both are fetched in one round trip.
+__device__ __forceinline__ void st_tuple_all(const TupleDev& M, double* r, const double* x, const double* y,
+                                             double alpha, const uint32_t* vmask, StStage& S)
+{
+  const int t = threadIdx.x;
+  const uint64_t nv = M.nv, np = M.np;
+  const bool hasB = M.val[1] != nullptr, hasC = M.val[3] != nullptr;
+  const uint64_t bA = M.ptr[0][0], eA = M.ptr[0][nv], bD = M.ptr[2][0], eD = M.ptr[2][np];
+  const uint64_t bB = hasB ? M.ptr[1][0] : 0, eB = hasB ? M.ptr[1][nv] : 0;
+  const uint64_t bC = hasC ? M.ptr[3][0] : 0, eC = hasC ? M.ptr[3][np] : 0;
+  const uint64_t nA = eA - bA, nD = eD - bD, nB = eB - bB, nC = eC - bC;
+  const bool vel = uint64_t(t) < nv, pre = !vel && uint64_t(t) < nv + np;
+  const uint64_t row = vel ? uint64_t(t) : uint64_t(t) - nv;
+  // this row's range in the two staged sequences
+  uint64_t ra0 = 0, re0 = 0, ra1 = 0, re1 = 0;
+  double y0 = 0.0, y1 = 0.0;
+  if(vel)
+  {
+    ra0 = M.ptr[0][row] - bA;
+    re0 = M.ptr[0][row + 1] - bA;
+    if(hasB)
+    {
+      ra1 = M.ptr[1][row] - bB;
+      re1 = M.ptr[1][row + 1] - bB;
+    }
+    if(y)
+    {
+      y0 = y[2 * row];
+      y1 = y[2 * row + 1];
+    }
+  }
+  else if(pre)
+  {
+    ra0 = nA + (M.ptr[2][row] - bD);
+    re0 = nA + (M.ptr[2][row + 1] - bD);
+    if(hasC)
+    {
+      ra1 = nB + (M.ptr[3][row] - bC);
+      re1 = nB + (M.ptr[3][row + 1] - bC);
+    }
+    if(y) y0 = y[2 * nv + row];
+  }
+  const uint64_t n0 = nA + nD, n1 = nB + nC;
+  const uint64_t p0 = (n0 + kStE0 - 1) / kStE0, p1 = (n1 + kStE1 - 1) / kStE1;
+  const uint64_t npass = p0 > p1 ? p0 : p1;
+  const double* xp = x + 2 * nv;
+  double s0 = 0.0, s1 = 0.0, t0 = 0.0, t1 = 0.0;
+  for(uint64_t q = 0; q < npass; ++q)
+  {
+    const uint64_t w0 = q * kStE0, w1 = q * kStE1;
+    double4 a[kStE0 / kSt];
+    uint32_t ca[kStE0 / kSt];
+    double2 b[kStE1 / kSt];
+    uint32_t cb[kStE1 / kSt];
+#pragma unroll
+    for(int u = 0; u < kStE0 / kSt; ++u)
+    {
+      const uint64_t j = w0 + uint64_t(t + u * kSt);
+      a[u] = double4{0.0, 0.0, 0.0, 0.0};
+      ca[u] = 0;
+      if(j < nA)
+      {
+        a[u] = reinterpret_cast<const double4*>(M.val[0])[bA + j];
+        ca[u] = uint32_t(M.col[0][bA + j]);
+      }
+      else if(j < n0)
+      {
+        const double2 d = reinterpret_cast<const double2*>(M.val[2])[bD + (j - nA)];
+        a[u].x = d.x;
+        a[u].y = d.y;
+        ca[u] = uint32_t(M.col[2][bD + (j - nA)]);
+      }
+    }
+#pragma unroll
+    for(int u = 0; u < kStE1 / kSt; ++u)
+    {
+      const uint64_t j = w1 + uint64_t(t + u * kSt);
+      b[u] = double2{0.0, 0.0};
+      cb[u] = 0;
+      if(j < nB)
+      {
+        b[u] = reinterpret_cast<const double2*>(M.val[1])[bB + j];
+        cb[u] = uint32_t(M.col[1][bB + j]);
+      }
+      else if(j < n1)
+      {
+        b[u].x = M.val[3][bC + (j - nB)];
+        cb[u] = uint32_t(M.col[3][bC + (j - nB)]);
+      }
+    }
+#pragma unroll
+    for(int u = 0; u < kStE0 / kSt; ++u)
+    {
+      S.v0[t + u * kSt] = a[u];
+      S.c0[t + u * kSt] = ca[u];
+    }
+#pragma unroll
+    for(int u = 0; u < kStE1 / kSt; ++u)
+    {
+      S.v1[t + u * kSt] = b[u];
+      S.c1[t + u * kSt] = cb[u];
+    }
+    __syncthreads();
+    if(vel || pre)
+    {
+      const uint64_t lo0 = ra0 > w0 ? ra0 : w0, h0 = w0 + kStE0, hi0 = re0 < h0 ? re0 : h0;
+      st_sum0(S, w0, lo0, hi0, x, vel, s0, s1);
+      const uint64_t lo1 = ra1 > w1 ? ra1 : w1, h1 = w1 + kStE1, hi1 = re1 < h1 ? re1 : h1;
+      st_sum1(S, w1, lo1, hi1, xp, vel, t0, t1);
+    }
+    __syncthreads();
+  }
+  const double a1 = y ? alpha : 1.0;
+  if(vel)
+  {
+    double r0 = y ? dfma(s0, alpha, y0) : dfma(s0, 1.0, 0.0);
+    double r1 = y ? dfma(s1, alpha, y1) : dfma(s1, 1.0, 0.0);
+    if(hasB)
+    {
+      r0 = dfma(t0, a1, r0);
+      r1 = dfma(t1, a1, r1);
+    }
+    if(is_masked(vmask, row)) r0 = r1 = 0.0;
+    r[2 * row] = r0;
+    r[2 * row + 1] = r1;
+  }
+  else if(pre)
+  {
+    double rp = y ? dfma(s0, alpha, y0) : dfma(s0, 1.0, 0.0);
+    if(hasC) rp = dfma(t0, a1, rp);
+    r[2 * nv + row] = rp;
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ void st_tuple(const TupleDev& M, double* r, const double* x, const double* y, double alpha,
+                         const uint32_t* vmask, StStage& S)
+{
+  if(M.nv + M.np <= uint64_t(kSt))
+  {
+    st_tuple_all(M, r, x, y, alpha, vmask, S);
+    return;
+  }
+  for(uint64_t rb = 0; rb < M.nv; rb += kSt)
+    st_tuple_group(M, true, rb, int(M.nv - rb < uint64_t(kSt) ? M.nv - rb : uint64_t(kSt)), r, x, y, alpha, vmask, S);
+  for(uint64_t rb = 0; rb < M.np; rb += kSt)
+    st_tuple_group(M, false, rb, int(M.np - rb < uint64_t(kSt) ? M.np - rb : uint64_t(kSt)), r, x, y, alpha, vmask, S);
+}
+
+// r = R x with NC components per row (NC = 2: k_csrsb2, BWrappedCSR; NC = 1: k_csr_rows with
+// a = 1, b = 0): sums from zero in storage order, then fma(sum, 1, 0)
+template<int NC>
+__device__ __forceinline__ void st_csr(const StCsr& R, const double* x, double* r, StStage& S)
+{
+  double* sv = reinterpret_cast<double*>(S.v0);
+  constexpr int kE = kStE0;
+  const int t = threadIdx.x;
+  for(uint64_t rb = 0; rb < R.rows; rb += kSt)
+  {
+    const int nr = int(R.rows - rb < uint64_t(kSt) ? R.rows - rb : uint64_t(kSt));
+    const uint64_t b = R.ptr[rb], e = R.ptr[rb + nr];
+    const bool rl = t < nr;
+    const uint64_t ra = rl ? R.ptr[rb + t] : 0, re = rl ? R.ptr[rb + t + 1] : 0;
+    double s0 = 0.0, s1 = 0.0;
+    for(uint64_t w = b; w < e; w += kE)
+    {
+      double v[kE / kSt];
+      uint32_t c[kE / kSt];
+#pragma unroll
+      for(int u = 0; u < kE / kSt; ++u)
+      {
+        const uint64_t k = w + uint64_t(t + u * kSt);
+        v[u] = k < e ? R.val[k] : 0.0;
+        c[u] = k < e ? uint32_t(R.col[k]) : 0u;
+      }
+#pragma unroll
+      for(int u = 0; u < kE / kSt; ++u)
+      {
+        sv[t + u * kSt] = v[u];
+        S.c0[t + u * kSt] = c[u];
+      }
+      __syncthreads();
+      if(rl)
+      {
+        const uint64_t lo = ra > w ? ra : w, hw = w + kE, hi = re < hw ? re : hw;
+        for(uint64_t k = lo; k < hi; ++k)
+        {
+          const double a = sv[k - w];
+          const uint32_t cc = S.c0[k - w];
+          s0 = dfma(a, x[NC * cc], s0);
+          if(NC == 2) s1 = dfma(a, x[NC * cc + 1], s1);
+        }
+      }
+      __syncthreads();
+    }
+    if(rl)
+    {
+      r[NC * (rb + t)] = dfma(s0, 1.0, 0.0);
+      if(NC == 2) r[NC * (rb + t) + 1] = dfma(s1, 1.0, 0.0);
+    }
+  }
+  __syncthreads();
+}
+
+// r = T x for a Stokes transfer (velocity rows by V with two components, pressure rows by P),
+// both staged in one sequence when rows(V) + rows(P) <= kSt (else st_csr twice);
+// x: [u (2 per node, nvx nodes) | p], r: [u (2 per node, V.rows nodes) | p]
+__device__ __forceinline__ void st_transfer(const StCsr& V, const StCsr& P, const double* x, uint64_t nvx, double* r,
+                                            StStage& S)
+{
+  if(V.rows + P.rows > uint64_t(kSt))
+  {
+    st_csr<2>(V, x, r, S);
+    st_csr<1>(P, x + 2 * nvx, r + 2 * V.rows, S);
+    return;
+  }
+  double* sv = reinterpret_cast<double*>(S.v0);
+  constexpr int kE = kStE0;
+  const int t = threadIdx.x;
+  const uint64_t bV = V.ptr[0], eV = V.ptr[V.rows], bP = P.ptr[0], eP = P.ptr[P.rows];
+  const uint64_t nV = eV - bV, n = nV + (eP - bP);
+  const bool vel = uint64_t(t) < V.rows, pre = !vel && uint64_t(t) < V.rows + P.rows;
+  const uint64_t row = vel ? uint64_t(t) : uint64_t(t) - V.rows;
+  uint64_t ra = 0, re = 0;
+  if(vel)
+  {
+    ra = V.ptr[row] - bV;
+    re = V.ptr[row + 1] - bV;
+  }
+  else if(pre)
+  {
+    ra = nV + (P.ptr[row] - bP);
+    re = nV + (P.ptr[row + 1] - bP);
+  }
+  const double* xp = x + 2 * nvx;
+  double s0 = 0.0, s1 = 0.0;
+  for(uint64_t w = 0; w < n; w += kE)
+  {
+    double v[kE / kSt];
+    uint32_t c[kE / kSt];
+#pragma unroll
+    for(int u = 0; u < kE / kSt; ++u)
+    {
+      const uint64_t j = w + uint64_t(t + u * kSt);
+      v[u] = 0.0;
+      c[u] = 0u;
+      if(j < nV)
+      {
+        v[u] = V.val[bV + j];
+        c[u] = uint32_t(V.col[bV + j]);
+      }
+      else if(j < n)
+      {
+        v[u] = P.val[bP + (j - nV)];
+        c[u] = uint32_t(P.col[bP + (j - nV)]);
+      }
+    }
+#pragma unroll
+    for(int u = 0; u < kE / kSt; ++u)
+    {
+      sv[t + u * kSt] = v[u];
+      S.c0[t + u * kSt] = c[u];
+    }
+    __syncthreads();
+    const uint64_t lo = ra > w ? ra : w, hw = w + kE, hi = re < hw ? re : hw;
+    if(vel || pre)
+    {
+      const double* xx = vel ? x : xp;
+      const int nc = vel ? 2 : 1;
+      uint64_t k = lo;
+      for(; k + 4 <= hi; k += 4)
+      {
+        double a[4], x0[4], x1[4];
+        uint32_t cc[4];
+#pragma unroll
+        for(int u = 0; u < 4; ++u)
+        {
+          a[u] = sv[k + u - w];
+          cc[u] = S.c0[k + u - w];
+        }
+#pragma unroll
+        for(int u = 0; u < 4; ++u)
+        {
+          x0[u] = xx[nc * cc[u]];
+          x1[u] = vel ? xx[nc * cc[u] + 1] : 0.0;
+        }
+#pragma unroll
+        for(int u = 0; u < 4; ++u)
+        {
+          s0 = dfma(a[u], x0[u], s0);
+          if(vel) s1 = dfma(a[u], x1[u], s1);
+        }
+      }
+      for(; k < hi; ++k)
+      {
+        const double a = sv[k - w];
+        const uint32_t cc = S.c0[k - w];
+        s0 = dfma(a, xx[nc * cc], s0);
+        if(vel) s1 = dfma(a, xx[nc * cc + 1], s1);
+      }
+    }
+    __syncthreads();
+  }
+  if(vel)
+  {
+    r[2 * row] = dfma(s0, 1.0, 0.0);
+    r[2 * row + 1] = dfma(s1, 1.0, 0.0);
+  }
+  else if(pre)
+    r[2 * V.rows + row] = dfma(s0, 1.0, 0.0);
+  __syncthreads();
+}
+
+// the norm of feat_tuple_norm2_f64 (k_sumsq2 on grid_for(n, 8) workgroups, the in-kernel grid
+// finish, k_tuple_norm_finish) for up to kSt / 256 of those workgroups (n <= 1,024 kSt / 256);
+// result in *out
+__device__ __forceinline__ void st_norm(const double* v, uint64_t nu, uint64_t np, double* redu, double* redp, double* out)
+{
+  const uint64_t n = nu + np;
+  const uint64_t G = (n + 2047) / 2048;
+  const int T = threadIdx.x, b = T >> 8, t = T & 255, w = T >> 6;
+  double su = 0.0, sp = 0.0;
+  if(uint64_t(b) < G)
+    for(uint64_t i = uint64_t(b) * 256 + t; i < n; i += G * 256)
+    {
+      const double q = v[i];
+      if(i < nu)
+        su = dfma(q, q, su);
+      else
+        sp = dfma(q, q, sp);
+    }
+  su = wave_sum(su);
+  sp = wave_sum(sp);
+  if((T & 63) == 0)
+  {
+    redu[w] = su;
+    redp[w] = sp;
+  }
+  __syncthreads();
+  if(w == 0)
+  {
+    // partial of virtual workgroup `lane` (its block sum), then the grid finish's block sum
+    double pu = 0.0, pp = 0.0;
+    if(uint64_t(T) < G)
+    {
+      pu = redu[4 * T];
+      pp = redp[4 * T];
+      for(int i = 1; i < 4; ++i)
+      {
+        pu += redu[4 * T + i];
+        pp += redp[4 * T + i];
+      }
+      pu = 0.0 + pu;
+      pp = 0.0 + pp;
+    }
+    pu = wave_sum(pu);
+    pp = wave_sum(pp);
+    if(T == 0)
+    {
+      for(int i = 1; i < 4; ++i)
+      {
+        pu += 0.0;
+        pp += 0.0;
+      }
+      const double a = sqrt(pu), c = sqrt(pp);
+      *out = sqrt(dfma(a, a, c * c));
+    }
+  }
+  __syncthreads();
+}
+
+// one AmaVanka multigrid smoother application (StokesMultiGrid::smooth)
+__device__ __forceinline__ void st_smooth(const StLevel& L, int steps, double omega, double* sol, const double* rhs, double* rdef,
+                          double* rcor, StStage& S, double* red, double* red2, double* scal)
+{
+  const uint64_t n = 2 * L.K.nv + L.K.np;
+  for(uint64_t i = threadIdx.x; i < n; i += kSt) sol[i] = 0.0;
+  __syncthreads();
+  st_norm(rhs, 2 * L.K.nv, L.K.np, red, red2, scal);
+  const bool open = fabs(omega) >= DBL_EPSILON && gate_open(scal);
+  for(int s = 0; s < steps; ++s)
+  {
+    const double* d = (s == 0) ? rhs : rdef;
+    st_tuple(L.V, rcor, d, nullptr, 1.0, L.vmask, S);
+    mean_passes_block<kSt>(rcor + 2 * L.K.nv, L.dual, L.prim, L.vol, L.K.np, 2, 0, red, scal + 1);
+    if(open)
+    {
+      for(uint64_t i = threadIdx.x; i < n; i += kSt) sol[i] = dfma(omega, rcor[i], sol[i]);
+      __syncthreads();
+    }
+    if(s + 1 < steps)
+    {
+      st_tuple(L.K, rdef, sol, rhs, -1.0, L.vmask, S);
+      mean_passes_block<kSt>(rdef + 2 * L.K.nv, L.prim, L.dual, L.vol, L.K.np, 1, 0, red, scal + 1);
+    }
+  }
+}
+
+__device__ __forceinline__ void st_defect(const StLevel& L, double* d, const double* x, const double* b, StStage& S, double* red,
+                          double* scal)
+{
+  st_tuple(L.K, d, x, b, -1.0, L.vmask, S);
+  mean_passes_block<kSt>(d + 2 * L.K.nv, L.prim, L.dual, L.vol, L.K.np, 1, 0, red, scal + 1);
+}
+
+// UnitFilterBlocked::filter_def on the velocity part
+__device__ __forceinline__ void st_unit_def(const StLevel& L, double* v)
+{
+  if(L.vmask)
+    for(uint64_t i = threadIdx.x; i < L.K.nv; i += kSt)
+      if(is_masked(L.vmask, i))
+      {
+        v[2 * i] = 0.0;
+        v[2 * i + 1] = 0.0;
+      }
+  __syncthreads();
+}
+
+__device__ __forceinline__ void st_axpy1(double* sol, const double* cor, uint64_t n)
+{
+  for(uint64_t i = threadIdx.x; i < n; i += kSt) sol[i] = dfma(1.0, cor[i], sol[i]);
+  __syncthreads();
+}
+
+// The V-cycle over the tail's levels (0 = its finest): rhs of level 0 in, its solution out.
+__global__ __launch_bounds__(kSt) void k_stokes_tail(StTail Targ, const double* __restrict__ rhs_in,
+                                                     double* __restrict__ sol_out)
+{
+  // the level descriptors are indexed by level at run time: one copy in LDS
+  __shared__ StTail T;
+  if(threadIdx.x == 0) T = Targ;
+  __syncthreads();
+  __shared__ StStage S;
+  __shared__ double s_vec[6 * FEAT_STOKES_TAIL_MAX_DOFS];
+  __shared__ double red[kMeanThreads / 64], red2[kMeanThreads / 64], scal[4];
+  // every level's MeanFilter vectors and velocity mask, read by each filter pass
+  __shared__ double s_pd[2 * FEAT_STOKES_TAIL_MAX_PRESSURE];
+  __shared__ uint32_t s_mask[FEAT_STOKES_TAIL_MAX_MASK_WORDS];
+  {
+    uint64_t op = 0, om = 0;
+    for(int l = 0; l < T.nl; ++l)
+    {
+      StLevel& L = T.lv[l];
+      const uint64_t np = L.K.np, nw = L.vmask ? (L.K.nv + 31) / 32 : 0;
+      for(uint64_t i = threadIdx.x; i < np; i += kSt)
+      {
+        s_pd[op + i] = L.prim[i];
+        s_pd[op + np + i] = L.dual[i];
+      }
+      for(uint64_t i = threadIdx.x; i < nw; i += kSt) s_mask[om + i] = L.vmask[i];
+      __syncthreads();
+      if(threadIdx.x == 0)
+      {
+        L.prim = s_pd + op;
+        L.dual = s_pd + op + np;
+        if(L.vmask) L.vmask = s_mask + om;
+      }
+      op += 2 * np;
+      om += nw;
+    }
+  }
+  // per level six vectors: rhs, sol, def, cor, rdef, rcor
+  __syncthreads();
+  enum { RHS = 0, SOL, DEF, COR, RDEF, RCOR };
+  auto vec = [&](int l, int k) -> double* {
+    uint64_t o = 0;
+#pragma unroll
+    for(int i = 0; i < kStMaxLevels; ++i)
+      if(i < l) o += 6 * (2 * T.lv[i].K.nv + T.lv[i].K.np);
+    return s_vec + o + uint64_t(k) * (2 * T.lv[l].K.nv + T.lv[l].K.np);
+  };
+  const int nl = T.nl;
+  {
+    const uint64_t n0 = 2 * T.lv[0].K.nv + T.lv[0].K.np;
+    for(uint64_t i = threadIdx.x; i < n0; i += kSt) vec(0, RHS)[i] = rhs_in[i];
+    __syncthreads();
+  }
+  for(int l = 0; l + 1 < nl; ++l)
+  {
+    const StLevel& L = T.lv[l];
+    const StLevel& C = T.lv[l + 1];
+    st_smooth(L, T.steps, T.omega, vec(l, SOL), vec(l, RHS), vec(l, RDEF), vec(l, RCOR), S, red, red2, scal);
+    st_defect(L, vec(l, DEF), vec(l, SOL), vec(l, RHS), S, red, scal);
+    st_transfer(L.rv, L.rp, vec(l, DEF), L.K.nv, vec(l + 1, RHS), S);
+    st_unit_def(C, vec(l + 1, RHS));
+    mean_passes_block<kSt>(vec(l + 1, RHS) + 2 * C.K.nv, C.prim, C.dual, C.vol, C.K.np, 1, 0, red, scal + 1);
+  }
+  {
+    const StLevel& Lc = T.lv[nl - 1];
+    st_tuple(Lc.V, vec(nl - 1, SOL), vec(nl - 1, RHS), nullptr, 1.0, Lc.vmask, S);
+    mean_passes_block<kSt>(vec(nl - 1, SOL) + 2 * Lc.K.nv, Lc.dual, Lc.prim, Lc.vol, Lc.K.np, 2, 0, red, scal + 1);
+  }
+  for(int l = nl - 2; l >= 0; --l)
+  {
+    const StLevel& L = T.lv[l];
+    const StLevel& C = T.lv[l + 1];
+    const uint64_t n = 2 * L.K.nv + L.K.np;
+    st_transfer(L.pv, L.pp, vec(l + 1, SOL), C.K.nv, vec(l, COR), S);
+    st_unit_def(L, vec(l, COR));
+    mean_passes_block<kSt>(vec(l, COR) + 2 * L.K.nv, L.dual, L.prim, L.vol, L.K.np, 1, 0, red, scal + 1);
+    st_axpy1(vec(l, SOL), vec(l, COR), n);
+    st_defect(L, vec(l, DEF), vec(l, SOL), vec(l, RHS), S, red, scal);
+    st_smooth(L, T.steps, T.omega, vec(l, COR), vec(l, DEF), vec(l, RDEF), vec(l, RCOR), S, red, red2, scal);
+    st_axpy1(vec(l, SOL), vec(l, COR), n);
+  }
+  const uint64_t n0 = 2 * T.lv[0].K.nv + T.lv[0].K.np;
+  for(uint64_t i = threadIdx.x; i < n0; i += kSt) sol_out[i] = vec(0, SOL)[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -798,6 +1604,61 @@ int feat_csrsb2_apply_f64_u64(feat_handle_t h, double* r, const double* x, const
   if(!rows) return FEAT_OK;
   hipLaunchKernelGGL(k_csrsb2, dim3(rows_grid(rows)), dim3(kThreads), 0, h->stream, r, x, val, col, rowptr, rows);
   return check_launch("feat_csrsb2_apply_f64_u64");
+}
+
+int feat_stokes_tail_f64(feat_handle_t h, const feat_stokes_tail_t* t, const double* rhs, double* sol)
+{
+  if(!h || !t || !rhs || !sol) return fail(FEAT_ERR_INVALID, "feat_stokes_tail_f64: null argument");
+  if(t->nlevels < 1 || t->nlevels > FEAT_STOKES_TAIL_MAX_LEVELS || t->steps < 1)
+    return fail(FEAT_ERR_INVALID, "feat_stokes_tail_f64: 1..FEAT_STOKES_TAIL_MAX_LEVELS levels, steps >= 1");
+  StTail T{};
+  T.nl = t->nlevels;
+  T.steps = t->steps;
+  T.omega = t->omega;
+  uint64_t dofs = 0;
+  for(int l = 0; l < t->nlevels; ++l)
+  {
+    const feat_stokes_tail_level_t& L = t->lv[l];
+    if(!tuple_ok(&L.K, 0, 2) || !tuple_ok(&L.V, 0, 3) || (L.K.val[3] && (!L.K.col[3] || !L.K.ptr[3])))
+      return fail(FEAT_ERR_INVALID, "feat_stokes_tail_f64: matrix blocks");
+    if(L.V.nv != L.K.nv || L.V.np != L.K.np || (L.K.np && (!L.prim || !L.dual)))
+      return fail(FEAT_ERR_INVALID, "feat_stokes_tail_f64: level sizes");
+    dofs += 2 * L.K.nv + L.K.np;
+    if(l + 1 < t->nlevels)
+    {
+      const feat_stokes_tail_level_t& C = t->lv[l + 1];
+      const feat_csr_f64_t* m[4] = {&L.rv, &L.rp, &L.pv, &L.pp};
+      for(const feat_csr_f64_t* c : m)
+        if(!c->val || !c->col || !c->rowptr) return fail(FEAT_ERR_INVALID, "feat_stokes_tail_f64: transfer");
+      if(L.rv.rows != C.K.nv || L.rv.cols != L.K.nv || L.rp.rows != C.K.np || L.rp.cols != L.K.np ||
+         L.pv.rows != L.K.nv || L.pv.cols != C.K.nv || L.pp.rows != L.K.np || L.pp.cols != C.K.np)
+        return fail(FEAT_ERR_INVALID, "feat_stokes_tail_f64: transfer shapes");
+    }
+    StLevel& D = T.lv[l];
+    D.K = to_dev(&L.K);
+    D.V = to_dev(&L.V);
+    D.vmask = L.vmask;
+    D.prim = L.prim;
+    D.dual = L.dual;
+    D.vol = L.volume;
+    D.rv = StCsr{L.rv.val, L.rv.col, L.rv.rowptr, L.rv.rows};
+    D.rp = StCsr{L.rp.val, L.rp.col, L.rp.rowptr, L.rp.rows};
+    D.pv = StCsr{L.pv.val, L.pv.col, L.pv.rowptr, L.pv.rows};
+    D.pp = StCsr{L.pp.val, L.pp.col, L.pp.rowptr, L.pp.rows};
+  }
+  static_assert(FEAT_STOKES_TAIL_MAX_DOFS <= 2048 * (kSt / 256), "the norm's workgroups");
+  if(dofs > FEAT_STOKES_TAIL_MAX_DOFS)
+    return fail(FEAT_ERR_UNSUPPORTED, "feat_stokes_tail_f64: more than FEAT_STOKES_TAIL_MAX_DOFS unknowns");
+  uint64_t pres = 0, words = 0;
+  for(int l = 0; l < t->nlevels; ++l)
+  {
+    pres += t->lv[l].K.np;
+    words += t->lv[l].vmask ? (t->lv[l].K.nv + 31) / 32 : 0;
+  }
+  if(pres > FEAT_STOKES_TAIL_MAX_PRESSURE || words > FEAT_STOKES_TAIL_MAX_MASK_WORDS)
+    return fail(FEAT_ERR_UNSUPPORTED, "feat_stokes_tail_f64: pressure / filter sizes above the LDS budget");
+  hipLaunchKernelGGL(k_stokes_tail, dim3(1), dim3(kSt), 0, h->stream, T, rhs, sol);
+  return check_launch("feat_stokes_tail_f64");
 }
 
 int feat_vanka_numeric_f64(feat_handle_t h, const feat_vanka_graph_t* g, const feat_tuple_f64_t* K,

@@ -622,6 +622,8 @@ int feat_stokes_create(feat_stokes_t* out, int level_fine, int level_coarse, con
     c.coarse_skip_singular = cfg->coarse_skip_singular != 0;
     c.omega_outer = cfg->omega_outer;
     c.graphs = cfg->no_graphs == 0;
+    c.coarse_fusion = cfg->no_coarse_fusion == 0;
+    if(cfg->tail_max_level_dofs) c.tail_max_level_dofs = cfg->tail_max_level_dofs;
     auto s = std::make_unique<feat_stokes_s>();
     s->mg = std::make_unique<Solver::StokesMultiGrid>(c);
     s->mg->set_tol_rel(cfg->tol_rel);

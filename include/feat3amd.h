@@ -449,6 +449,42 @@ int feat_tuple_apply_post_f64(feat_handle_t h, const feat_tuple_f64_t* M, double
                               const double* w_add, double volume, int passes, int sol_form, double* x_upd,
                               double omega, const double* gate_norm, const feat_reduce_ws_t* ws);
 
+/* The coarse levels of the Stokes multigrid V-cycle in one workgroup (no counterpart in the
+ * reference: a launch-count fusion of StokesMultiGrid's mg_apply over its coarsest levels,
+ * multigrid.hpp:1270-1350, 1713-1980 with the smoother of richardson.hpp:160-226 and
+ * SchwarzPrecond(AmaVanka)).  lv[0] is the finest level of the tail, lv[nlevels-1] the
+ * coarsest (solved by one AmaVanka application + filter_cor twice); on every other level:
+ * smoother (steps Richardson steps with damping omega, gated by the right-hand side's
+ * norm), defect + filter_def, restriction (rv, rp) + filter_def, and on the way up
+ * prolongation (pv, pp) + filter_cor, sol += cor, defect, smoother on the defect, sol +=
+ * cor.  Every operation has the arithmetic of the separate entry points
+ * (feat_tuple_apply_f64, feat_mean_filter_passes_f64, feat_axpy_gated_f64,
+ * feat_tuple_norm2_f64, feat_csrsb2_apply_f64_u64, feat_csr_apply_f64_u64,
+ * feat_unit_filter_blocked_def_f64_u64, feat_axpy_f64), so the result is bit-identical to
+ * them.  rhs: level 0's right-hand side (read), sol: its solution (written); the levels'
+ * other vectors live in the workgroup's LDS: sum over the levels of 2 nv + np at most
+ * FEAT_STOKES_TAIL_MAX_DOFS (and of np, of the mask words at most the two limits below). */
+#define FEAT_STOKES_TAIL_MAX_LEVELS 4
+#define FEAT_STOKES_TAIL_MAX_DOFS 1100
+#define FEAT_STOKES_TAIL_MAX_PRESSURE 384     /* sum over the levels of np */
+#define FEAT_STOKES_TAIL_MAX_MASK_WORDS 32    /* sum over the levels of ceil(nv / 32) */
+typedef struct {
+  feat_tuple_f64_t K;             /* SaddlePointMatrix {A, B, D, -} */
+  feat_tuple_f64_t V;             /* AmaVanka matrix (four blocks) */
+  const uint32_t* vmask;          /* velocity nodes of the UnitFilterBlocked (may be NULL) */
+  const double* prim;             /* MeanFilter vectors and volume (mean_filter.hpp:200-262) */
+  const double* dual;
+  double volume;
+  feat_csr_f64_t rv, rp;          /* restriction to the next coarser level (velocity nodes, pressure) */
+  feat_csr_f64_t pv, pp;          /* prolongation from it */
+} feat_stokes_tail_level_t;
+typedef struct {
+  int nlevels, steps;
+  double omega;
+  feat_stokes_tail_level_t lv[FEAT_STOKES_TAIL_MAX_LEVELS];
+} feat_stokes_tail_t;
+int feat_stokes_tail_f64(feat_handle_t h, const feat_stokes_tail_t* t, const double* rhs, double* sol);
+
 /* MeanFilter passes on one vector v of length n, in one workgroup
  * (lafem/mean_filter.hpp:213-262): per pass alpha = -(v . w_dot)/volume
  * (sol_form: 0 - (v . w_dot)/volume) and, unless |alpha| < eps (DenseVector::axpy,

@@ -257,6 +257,13 @@ namespace feat3amd
       {
         apply(_rv, _rp, coarse.elements(), _nv_c, fine.elements(), _nv_f);
       }
+      /// device descriptors: 0 = Rv, 1 = Rp, 2 = Pv, 3 = Pp
+      feat_csr_f64_t desc(int k) const
+      {
+        const Csr* m[4] = {&_rv, &_rp, &_pv, &_pp};
+        const Csr& c = *m[k];
+        return feat_csr_f64_t{c.rows, c.cols, c.nnz, c.val.get(), c.col.get(), c.ptr.get()};
+      }
     };
   } // namespace LAFEM
 
@@ -441,6 +448,10 @@ namespace feat3amd
       bool coarse_skip_singular = true;
       double omega_outer = 1.0;        // outer Richardson (:761)
       bool graphs = true;              // replay each outer iteration as a recorded HIP graph
+      bool coarse_fusion = true;       // the coarsest levels in one workgroup (feat_stokes_tail_f64)
+      // largest level in that workgroup: it applies a level-2 tuple (210 unknowns) in 4.7 us
+      // against ~9 us for a launch of its own, but a level-3 tuple (770) in 12.5 us against 9.5 us
+      std::uint64_t tail_max_level_dofs = 512;
     };
 
     /// Richardson(MG-V(Richardson(SchwarzPrecond(AmaVanka)))) on a Q2/P1dc hierarchy.
@@ -464,6 +475,9 @@ namespace feat3amd
       LAFEM::StokesVector _d, _c;
       DeviceArray<double> _scal;
       FusedWorkspace _fw;
+      // levels _tail_start.. run in one workgroup (== number of levels: none do)
+      std::size_t _tail_start = 0;
+      feat_stokes_tail_t _tail{};
       // one outer iteration recorded as a HIP graph on a solver-owned stream
       struct Graph
       {
@@ -522,7 +536,7 @@ namespace feat3amd
       {
         const std::size_t nl = _lv.size();
         _lv[0]->rhs.copy(def);
-        for(std::size_t i = 0; i + 1 < nl; ++i)
+        for(std::size_t i = 0; i + 1 < nl && i < _tail_start; ++i)
         {
           Level& L = *_lv[i];
           Level& C = *_lv[i + 1];
@@ -531,9 +545,19 @@ namespace feat3amd
           L.T.rest(L.def, C.rhs);
           C.F.filter_def(C.rhs);
         }
-        Level& Lc = *_lv[nl - 1];
-        Lc.V->apply_update(Lc.sol, Lc.rhs, 2, nullptr, 0.0, nullptr, _fw.ws);
-        for(std::size_t i = nl - 1; i-- > 0;)
+        const std::size_t t = _tail_start;
+        if(t < nl)
+        {
+          // levels t.. (down, coarse solve, up) in one workgroup: rhs of level t in, its sol out
+          check(feat_stokes_tail_f64(Runtime::handle(), &_tail, _lv[t]->rhs.elements(), _lv[t]->sol.elements()),
+                "Stokes coarse levels");
+        }
+        else
+        {
+          Level& Lc = *_lv[nl - 1];
+          Lc.V->apply_update(Lc.sol, Lc.rhs, 2, nullptr, 0.0, nullptr, _fw.ws);
+        }
+        for(std::size_t i = std::min(t, nl - 1); i-- > 0;)
         {
           Level& L = *_lv[i];
           L.T.prol(L.cor, _lv[i + 1]->sol);
@@ -544,6 +568,53 @@ namespace feat3amd
           L.sol.axpy(L.cor, L.sol, 1.0);
         }
         cor.copy(_lv[0]->sol);
+      }
+
+      // The coarsest levels whose vectors fit the fused workgroup's LDS, each of at most
+      // StokesConfig::tail_max_level_dofs unknowns.
+      void _tail_plan()
+      {
+        const std::size_t nl = _lv.size();
+        _tail_start = nl;
+        if(!_cfg.coarse_fusion) return;
+        std::uint64_t dofs = 0, pres = 0, words = 0;
+        std::size_t t = nl;
+        while(t > 0 && nl - t < std::size_t(FEAT_STOKES_TAIL_MAX_LEVELS))
+        {
+          const Level& L = *_lv[t - 1];
+          const std::uint64_t w = (L.K.nv() + 31) / 32;
+          if(L.K.size() > _cfg.tail_max_level_dofs || dofs + L.K.size() > FEAT_STOKES_TAIL_MAX_DOFS ||
+             pres + L.K.np() > FEAT_STOKES_TAIL_MAX_PRESSURE || words + w > FEAT_STOKES_TAIL_MAX_MASK_WORDS)
+            break;
+          dofs += L.K.size();
+          pres += L.K.np();
+          words += w;
+          --t;
+        }
+        if(t == nl) return;
+        _tail = feat_stokes_tail_t{};
+        _tail.nlevels = int(nl - t);
+        _tail.steps = _cfg.steps;
+        _tail.omega = _cfg.omega_rich;
+        for(std::size_t i = t; i < nl; ++i)
+        {
+          const Level& L = *_lv[i];
+          feat_stokes_tail_level_t& d = _tail.lv[i - t];
+          d.K = L.K.desc();
+          d.V = L.V->vanka_desc();
+          d.vmask = L.F.vmask();
+          d.prim = L.F.prim();
+          d.dual = L.F.dual();
+          d.volume = L.F.volume();
+          if(i + 1 < nl)
+          {
+            d.rv = L.T.desc(0);
+            d.rp = L.T.desc(1);
+            d.pv = L.T.desc(2);
+            d.pp = L.T.desc(3);
+          }
+        }
+        _tail_start = t;
       }
 
       double read_norm(const LAFEM::StokesVector& v)
@@ -587,6 +658,7 @@ namespace feat3amd
           for(LAFEM::StokesVector* v : {&L.rhs, &L.sol, &L.def, &L.cor, &L.rdef, &L.rcor}) *v = L.K.create_vector();
           L.gate = DeviceArray<double>(1);
         }
+        _tail_plan();
         _d = _lv[0]->K.create_vector();
         _c = _lv[0]->K.create_vector();
         _scal = DeviceArray<double>(4);

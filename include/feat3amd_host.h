@@ -176,6 +176,8 @@ typedef struct {
   double tol_rel;
   uint64_t max_iter;
   int no_graphs;           /* 1: launch every kernel eagerly (no recorded iteration) */
+  int no_coarse_fusion;    /* 1: the coarsest levels as separate launches (feat_stokes_tail_f64 off) */
+  uint64_t tail_max_level_dofs;  /* largest level run in that workgroup (0: 512) */
 } feat_stokes_cfg_t;
 typedef struct feat_stokes_s* feat_stokes_t;
 /* builds the levels and runs init_symbolic + init_numeric (AmaVanka on device) */

@@ -168,3 +168,34 @@ def test_stokes_graph_replay_matches_eager(feat, oracle, torch_gpu):
         assert ig == ie
         assert np.array_equal(hg, he)
         assert np.array_equal(xg, xe)
+
+
+@pytest.mark.parametrize("lf,tail_dofs", [(2, 0), (3, 0), (3, 1000), (4, 1000), (6, 0)])
+def test_stokes_coarse_fusion_bitwise(feat, oracle, torch_gpu, lf, tail_dofs):
+    """The coarsest levels of the V-cycle in one workgroup (feat_stokes_tail_f64: levels 2..1 by
+    default, 3..1 with levels of up to 1,000 unknowns admitted; at lf = 2 and lf = 3 with 1,000
+    the whole cycle) give bit-identical iterates and defect histories to the same levels as
+    separate launches, eagerly and replayed from the recorded iteration, over two solves in a
+    row."""
+    torch = torch_gpu
+    lv = stokes_levels(feat, lf, lf)
+    x0, b = stokes_rhs(lv[0], oracle)
+    bd = dev(torch, b)
+    runs = {}
+    for fusion in (True, False):
+        for graphs in (True, False):
+            s = feat.Stokes(lf, 1, coarse_fusion=fusion, graphs=graphs, tail_max_level_dofs=tail_dofs)
+            out = []
+            for _ in range(2):
+                x = dev(torch, x0)
+                r = s.solve(x, bd)
+                out.append((r["status"], r["iters"], np.asarray(r["hist"]), host(x)))
+            s.close()
+            runs[(fusion, graphs)] = out
+    ref = runs[(False, False)]
+    assert ref[0][0] == "success"
+    for key, out in runs.items():
+        for (sa, ia, ha, xa), (sb, ib, hb, xb) in zip(out, ref):
+            assert sa == sb and ia == ib, key
+            assert np.array_equal(ha, hb), key
+            assert np.array_equal(xa, xb), key
