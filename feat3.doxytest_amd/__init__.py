@@ -189,7 +189,9 @@ KERNEL_API = [
     ("feat_dense_gemv_mixed", _i, [_vp, _i, _vp, _vp, _vp, _vp]),
     ("feat_tuple_apply_f64", _i, [_vp, _vp, _vp, _vp, _vp, _d, _vp]),
     ("feat_stokes_tail_f64", _i, [_vp, _vp, _vp, _vp]),
-    ("feat_tuple_apply_post_f64", _i, [_vp, _vp, _vp, _vp, _vp, _d, _vp, _vp, _vp, _d, _i, _i, _vp, _d, _vp, _vp]),
+    ("feat_tuple_apply_post_f64", _i, [_vp, _vp, _vp, _vp, _vp, _d, _vp, _vp, _vp, _d, _i, _i, _vp, _d, _vp, _i,
+                                       _vp]),
+    ("feat_stokes_transfer_post_f64", _i, [_vp, _vp, _vp, _vp, _u64, _vp, _vp, _vp, _vp, _d, _i, _vp, _vp]),
     ("feat_mean_filter_passes_f64", _i, [_vp, _vp, _vp, _vp, _d, _u64, _i, _i]),
     ("feat_tuple_norm2_f64", _i, [_vp, _vp, _u64, _u64, _vp, _vp]),
     ("feat_axpy_gated_f64", _i, [_vp, _vp, _d, _vp, _u64, _vp]),

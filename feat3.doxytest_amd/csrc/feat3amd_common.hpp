@@ -158,12 +158,14 @@ __device__ void grid_reduce_tail(const GridReduce& g, uint32_t nblocks, double* 
   }
 }
 
-__device__ void grid_reduce_finish(const GridReduce& g, double v0, double v1)
+// Returns true in the workgroup that completed the sum (its thread 0 wrote out0/out1); false
+// everywhere else and when the sum is deferred or nothing is reduced.
+__device__ bool grid_reduce_finish(const GridReduce& g, double v0, double v1)
 {
   __shared__ double red[kThreads / 64];
   __shared__ int last;
   const bool r0 = g.out0 != nullptr, r1 = g.out1 != nullptr;
-  if(!r0 && !r1) return;
+  if(!r0 && !r1) return false;
   double b0 = r0 ? block_sum(v0, red) : 0.0;
   double b1 = r1 ? block_sum(v1, red) : 0.0;
   if(g.defer)
@@ -173,7 +175,7 @@ __device__ void grid_reduce_finish(const GridReduce& g, double v0, double v1)
       if(r0) g.partials[blockIdx.x] = b0;
       if(r1) g.partials[kMaxGrid + blockIdx.x] = b1;
     }
-    return;
+    return false;
   }
   if(threadIdx.x == 0)
   {
@@ -191,9 +193,10 @@ __device__ void grid_reduce_finish(const GridReduce& g, double v0, double v1)
     last = (prev == gridDim.x - 1) ? 1 : 0;
   }
   __syncthreads();
-  if(!last) return;
+  if(!last) return false;
   grid_reduce_tail<true>(g, gridDim.x, red);
   if(threadIdx.x == 0) __hip_atomic_store(g.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return true;
 }
 
 // completes a deferred grid reduction of nblocks partials (one workgroup of kThreads)

@@ -29,6 +29,15 @@ struct TupleDev
   const uint64_t* ptr[4];
 };
 
+// a CSR matrix with u64 indices (transfers)
+struct StCsr
+{
+  const double* val;
+  const uint64_t* col;
+  const uint64_t* ptr;
+  uint64_t rows;
+};
+
 // Tuple apply staged through LDS.  A workgroup covers kTupRows block rows of one kind:
 // velocity rows (blocks 0 = A, 2x2, and 1 = B, 2x1) or pressure rows (2 = D, 1x2, and
 // 3 = C, 1x1).  All 256 threads load the workgroup's blocks and column indices and gather
@@ -334,7 +343,48 @@ struct TuplePost
   const double* gate;
   uint32_t* counter;   // zero between launches; counts the pressure workgroups in
   int tail;            // 1: filter + pressure update in the last workgroup; 0: left to k_mean_passes
+  int x_zero;          // x_upd holds zeros and is not read: the update writes omega r, or zeros
+                       // where it is skipped (the smoother's first step after sol.format())
 };
+
+// the update's value for one entry: x + omega r (x_zero: 0 + omega r), or x unchanged (x_zero:
+// zero) when the update is skipped
+__device__ __forceinline__ double post_update(bool open, bool x_zero, double omega, double r, double x)
+{
+  return open ? dfma(omega, r, x_zero ? 0.0 : x) : (x_zero ? 0.0 : x);
+}
+
+// The pressure part of a launch's TuplePost, run by every pressure workgroup after it stored
+// its rows of rp: each publishes its rows and counts in; the last one applies the mean filter
+// passes to rp[0, np) and the update to x_upd_p[0, np).
+__device__ __forceinline__ void post_pressure_tail(const TuplePost& post, double* rp, double* x_upd_p, uint64_t np,
+                                                   uint32_t npb)
+{
+  __shared__ double red[kMeanThreads / 64];
+  __shared__ double s_alpha;
+  __shared__ int last;
+  // agent-scope release by every writer, then count in
+  __threadfence();
+  __syncthreads();
+  if(threadIdx.x == 0)
+  {
+    const uint32_t prev = __hip_atomic_fetch_add(post.counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    last = (prev == npb - 1) ? 1 : 0;
+  }
+  __syncthreads();
+  if(!last) return;
+  __threadfence();   // acquire: the other workgroups' rows
+  mean_passes_block<kThreads>(rp, post.w_dot, post.w_add, post.volume, np, post.passes, post.sol_form, red, &s_alpha);
+  if(x_upd_p)
+  {
+    const bool open = fabs(post.omega) >= DBL_EPSILON && gate_open(post.gate);
+    if(open && !post.x_zero)
+      axpy_block<kThreads>(x_upd_p, post.omega, rp, np);
+    else if(open || post.x_zero)
+      for(uint64_t i = threadIdx.x; i < np; i += kThreads) x_upd_p[i] = post_update(open, true, post.omega, rp[i], 0.0);
+  }
+  if(threadIdx.x == 0) __hip_atomic_store(post.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 
 // Pressure parts above this many rows are filtered by k_mean_passes (1,024 threads) in a launch
 // of its own: one 256-thread workgroup needs ~15 dependent memory round trips for 12,288 rows
@@ -366,7 +416,9 @@ __global__ __launch_bounds__(kThreads) void k_tuple_apply(TupleDev M, double* __
   // the row's own operands, loaded before the staging
   double y0 = 0.0, y1 = 0.0, u0 = 0.0, u1 = 0.0;
   bool masked = false;
-  const bool upd = POST && post.x_upd != nullptr && vel && fabs(post.omega) >= DBL_EPSILON && gate_open(post.gate);
+  const bool open = POST && fabs(post.omega) >= DBL_EPSILON && gate_open(post.gate);
+  const bool upd = POST && post.x_upd != nullptr && vel && (open || post.x_zero);
+  const bool upd_read = upd && open && !post.x_zero;
   if(rl)
   {
     if(vel)
@@ -377,7 +429,7 @@ __global__ __launch_bounds__(kThreads) void k_tuple_apply(TupleDev M, double* __
         y1 = y[2 * row + 1];
       }
       masked = is_masked(vmask, row);
-      if(upd)
+      if(upd_read)
       {
         u0 = post.x_upd[2 * row];
         u1 = post.x_upd[2 * row + 1];
@@ -406,8 +458,8 @@ __global__ __launch_bounds__(kThreads) void k_tuple_apply(TupleDev M, double* __
       r[2 * row + 1] = r1;
       if(upd)
       {
-        post.x_upd[2 * row] = dfma(post.omega, r0, u0);
-        post.x_upd[2 * row + 1] = dfma(post.omega, r1, u1);
+        post.x_upd[2 * row] = post_update(open, post.x_zero, post.omega, r0, u0);
+        post.x_upd[2 * row + 1] = post_update(open, post.x_zero, post.omega, r1, u1);
       }
     }
     else
@@ -420,27 +472,117 @@ __global__ __launch_bounds__(kThreads) void k_tuple_apply(TupleDev M, double* __
   if constexpr(POST)
   {
     if(vel || !post.tail) return;
-    __shared__ double red[kMeanThreads / 64];
-    __shared__ double s_alpha;
-    __shared__ int last;
-    // publish this workgroup's pressure rows (agent-scope release by every writer), then count in
-    __threadfence();
-    __syncthreads();
-    if(threadIdx.x == 0)
+    post_pressure_tail(post, r + 2 * M.nv, post.x_upd ? post.x_upd + 2 * M.nv : nullptr, M.np, gridDim.x - nvb);
+  }
+}
+
+// Stokes transfer with what follows it (TupleDiagMatrix<BWrappedCSR<2>, CSR>::apply, then
+// filter_def or filter_cor and an optional axpy): r = [V x_u ; P x_p] (velocity rows: the scalar
+// CSR V on both components, k_csrsb2; pressure rows: P, k_csr_rows with a = 1, b = 0), velocity
+// rows zeroed by vmask, `post.passes` MeanFilter passes on the pressure part, then x_upd +=
+// post.omega r.  Rows are staged through LDS like the tuple apply; workgroups 0 .. nvb-1 take
+// the velocity rows.  x: [u (2 per node, nvx nodes) | p].
+constexpr int kTrRows = 32;     // rows per workgroup (restriction rows hold up to 25 entries)
+constexpr int kTrE = 1024;      // entries staged per pass
+template<bool POST>
+__global__ __launch_bounds__(kThreads) void k_stokes_transfer(StCsr V, StCsr P, const double* __restrict__ x, uint64_t nvx,
+                                                              double* __restrict__ r, const uint32_t* __restrict__ vmask,
+                                                              uint32_t nvb, TuplePost post)
+{
+  __shared__ double s_a[kTrE];
+  __shared__ double2 s_x[kTrE];
+  const int t = threadIdx.x;
+  const bool vel = blockIdx.x < nvb;
+  const StCsr& M = vel ? V : P;
+  const uint64_t rb = uint64_t(vel ? blockIdx.x : blockIdx.x - nvb) * kTrRows;
+  const uint64_t left = M.rows - rb;
+  const int nr = int(left < uint64_t(kTrRows) ? left : uint64_t(kTrRows));
+  const bool rl = t < nr;
+  const uint64_t row = rb + t;
+  const uint64_t b = M.ptr[rb], e = M.ptr[rb + nr];
+  uint64_t ra = 0, re = 0;
+  double u0 = 0.0, u1 = 0.0;
+  const bool open = POST && fabs(post.omega) >= DBL_EPSILON && gate_open(post.gate);
+  const bool upd = POST && post.x_upd != nullptr && vel && (open || post.x_zero);
+  const bool upd_read = upd && open && !post.x_zero;
+  if(rl)
+  {
+    ra = M.ptr[row];
+    re = M.ptr[row + 1];
+    if(upd_read)
     {
-      const uint32_t npb = gridDim.x - nvb;
-      const uint32_t prev = __hip_atomic_fetch_add(post.counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      last = (prev == npb - 1) ? 1 : 0;
+      u0 = post.x_upd[2 * row];
+      u1 = post.x_upd[2 * row + 1];
+    }
+  }
+  const double* xp = x + 2 * nvx;
+  double s0 = 0.0, s1 = 0.0;
+  for(uint64_t w = b; w < e; w += kTrE)
+  {
+    double a[kTrE / kThreads];
+    uint64_t c[kTrE / kThreads];
+    double2 xv[kTrE / kThreads];
+#pragma unroll
+    for(int u = 0; u < kTrE / kThreads; ++u)
+    {
+      const uint64_t k = w + uint64_t(t + u * kThreads);
+      a[u] = k < e ? M.val[k] : 0.0;
+      c[u] = k < e ? M.col[k] : 0;
+    }
+#pragma unroll
+    for(int u = 0; u < kTrE / kThreads; ++u)
+    {
+      xv[u] = double2{0.0, 0.0};
+      if(w + uint64_t(t + u * kThreads) < e)
+      {
+        if(vel)
+          xv[u] = double2{x[2 * c[u]], x[2 * c[u] + 1]};
+        else
+          xv[u].x = xp[c[u]];
+      }
+    }
+#pragma unroll
+    for(int u = 0; u < kTrE / kThreads; ++u)
+    {
+      s_a[t + u * kThreads] = a[u];
+      s_x[t + u * kThreads] = xv[u];
     }
     __syncthreads();
-    if(!last) return;
-    __threadfence();   // acquire: the other workgroups' rows
-    double* rp = r + 2 * M.nv;
-    mean_passes_block<kThreads>(rp, post.w_dot, post.w_add, post.volume, M.np, post.passes, post.sol_form, red,
-                                &s_alpha);
-    if(post.x_upd && fabs(post.omega) >= DBL_EPSILON && gate_open(post.gate))
-      axpy_block<kThreads>(post.x_upd + 2 * M.nv, post.omega, rp, M.np);
-    if(threadIdx.x == 0) __hip_atomic_store(post.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if(rl)
+    {
+      const uint64_t lo = ra > w ? ra : w, hw = w + kTrE, hi = re < hw ? re : hw;
+      for(uint64_t k = lo; k < hi; ++k)
+      {
+        const double aa = s_a[k - w];
+        const double2 xx = s_x[k - w];
+        s0 = dfma(aa, xx.x, s0);
+        if(vel) s1 = dfma(aa, xx.y, s1);
+      }
+    }
+    __syncthreads();
+  }
+  if(rl)
+  {
+    if(vel)
+    {
+      double r0 = dfma(s0, 1.0, 0.0), r1 = dfma(s1, 1.0, 0.0);
+      if(POST && is_masked(vmask, row)) r0 = r1 = 0.0;
+      r[2 * row] = r0;
+      r[2 * row + 1] = r1;
+      if(upd)
+      {
+        post.x_upd[2 * row] = post_update(open, post.x_zero, post.omega, r0, u0);
+        post.x_upd[2 * row + 1] = post_update(open, post.x_zero, post.omega, r1, u1);
+      }
+    }
+    else
+      r[2 * V.rows + row] = dfma(s0, 1.0, 0.0);
+  }
+  if constexpr(POST)
+  {
+    if(vel || !post.tail) return;
+    post_pressure_tail(post, r + 2 * V.rows, post.x_upd ? post.x_upd + 2 * V.rows : nullptr, P.rows,
+                       gridDim.x - nvb);
   }
 }
 
@@ -452,16 +594,24 @@ __global__ __launch_bounds__(kMeanThreads) void k_mean_passes(double* __restrict
                                                               const double* __restrict__ w_add, double volume,
                                                               uint64_t n, int passes, int sol_form,
                                                               double* __restrict__ x_upd, double omega,
-                                                              const double* __restrict__ gate)
+                                                              const double* __restrict__ gate, int x_zero)
 {
   __shared__ double red[kMeanThreads / 64];
   __shared__ double s_alpha;
   mean_passes_block<kMeanThreads>(v, w_dot, w_add, volume, n, passes, sol_form, red, &s_alpha);
-  if(x_upd && fabs(omega) >= DBL_EPSILON && gate_open(gate)) axpy_block<kMeanThreads>(x_upd, omega, v, n);
+  if(!x_upd) return;
+  const bool open = fabs(omega) >= DBL_EPSILON && gate_open(gate);
+  if(open && !x_zero)
+    axpy_block<kMeanThreads>(x_upd, omega, v, n);
+  else if(open || x_zero)
+    for(uint64_t i = threadIdx.x; i < n; i += kMeanThreads) x_upd[i] = post_update(open, true, omega, v[i], 0.0);
 }
 
+// sums of squares of v[0, nu) and v[nu, nu + np); with norm != null (and the sum finished in
+// this launch) the workgroup that completes it also writes sqrt(|u|^2 + |p|^2) as
+// k_tuple_norm_finish does
 __global__ __launch_bounds__(kThreads) void k_sumsq2(const double* __restrict__ v, uint64_t nu, uint64_t np,
-                                                     GridReduce gr)
+                                                     GridReduce gr, double* __restrict__ norm)
 {
   double su = 0.0, sp = 0.0;
   const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
@@ -473,7 +623,11 @@ __global__ __launch_bounds__(kThreads) void k_sumsq2(const double* __restrict__ 
     else
       sp = dfma(t, t, sp);
   }
-  grid_reduce_finish(gr, su, sp);
+  if(grid_reduce_finish(gr, su, sp) && norm && threadIdx.x == 0)
+  {
+    const double a = sqrt(*gr.out0), b = sqrt(*gr.out1);
+    *norm = sqrt(dfma(a, a, b * b));
+  }
 }
 
 __global__ void k_tuple_norm_finish(const double* __restrict__ sums, double* __restrict__ out)
@@ -539,14 +693,6 @@ struct StStage
   uint32_t c0[kStE0];   //  8 KB
   double2 v1[kStE1];    // 16 KB
   uint32_t c1[kStE1];   //  4 KB
-};
-
-struct StCsr
-{
-  const double* val;
-  const uint64_t* col;
-  const uint64_t* ptr;
-  uint64_t rows;
 };
 
 struct StLevel
@@ -1530,7 +1676,7 @@ int feat_tuple_apply_f64(feat_handle_t h, const feat_tuple_f64_t* M, double* r, 
 int feat_tuple_apply_post_f64(feat_handle_t h, const feat_tuple_f64_t* M, double* r, const double* x, const double* y,
                               double alpha, const uint32_t* vmask, const double* w_dot, const double* w_add,
                               double volume, int passes, int sol_form, double* x_upd, double omega,
-                              const double* gate_norm, const feat_reduce_ws_t* ws)
+                              const double* gate_norm, int x_upd_zero, const feat_reduce_ws_t* ws)
 {
   if(!h || !tuple_ok(M, 0, 0) || !tuple_ok(M, 2, 2) || !r || !x)
     return fail(FEAT_ERR_INVALID, "feat_tuple_apply_post_f64: null argument");
@@ -1546,19 +1692,43 @@ int feat_tuple_apply_post_f64(feat_handle_t h, const feat_tuple_f64_t* M, double
   if(!rows) return FEAT_OK;
   const uint64_t nvb = (M->nv + kTupRows - 1) / kTupRows, npb = (M->np + kTupRows - 1) / kTupRows;
   if(nvb + npb > uint64_t(INT32_MAX)) return fail(FEAT_ERR_UNSUPPORTED, "feat_tuple_apply_post_f64: too many rows");
-  const int tail = M->np <= kTupTailMaxP ? 1 : 0;
-  const TuplePost post{w_dot, w_add, volume, passes, sol_form, x_upd, omega, gate_norm, ws->counter, tail};
-  if(npb && tail)
-    hipLaunchKernelGGL(k_tuple_apply<true>, dim3(unsigned(nvb + npb)), dim3(kThreads), 0, h->stream, to_dev(M), r, x,
-                       y, alpha, vmask, uint32_t(nvb), post);
-  else
-    hipLaunchKernelGGL(k_tuple_apply<true>, dim3(unsigned(nvb + npb)), dim3(kThreads), 0, h->stream, to_dev(M), r, x,
-                       y, alpha, vmask, uint32_t(nvb), TuplePost{w_dot, w_add, volume, passes, sol_form, x_upd, omega,
-                                                                  gate_norm, ws->counter, 0});
+  const int tail = (npb && M->np <= kTupTailMaxP) ? 1 : 0;
+  const int xz = (x_upd && x_upd_zero) ? 1 : 0;
+  const TuplePost post{w_dot, w_add, volume, passes, sol_form, x_upd, omega, gate_norm, ws->counter, tail, xz};
+  hipLaunchKernelGGL(k_tuple_apply<true>, dim3(unsigned(nvb + npb)), dim3(kThreads), 0, h->stream, to_dev(M), r, x, y,
+                     alpha, vmask, uint32_t(nvb), post);
   if(npb && !tail && (passes || x_upd))
     hipLaunchKernelGGL(k_mean_passes, dim3(1), dim3(kMeanThreads), 0, h->stream, r + 2 * M->nv, w_dot, w_add, volume,
-                       M->np, passes, sol_form, x_upd ? x_upd + 2 * M->nv : nullptr, omega, gate_norm);
+                       M->np, passes, sol_form, x_upd ? x_upd + 2 * M->nv : nullptr, omega, gate_norm, xz);
   return check_launch("feat_tuple_apply_post_f64");
+}
+
+int feat_stokes_transfer_post_f64(feat_handle_t h, const feat_csr_f64_t* V, const feat_csr_f64_t* P, const double* x,
+                                  uint64_t nvx, double* r, const uint32_t* vmask, const double* w_dot,
+                                  const double* w_add, double volume, int passes, double* x_upd,
+                                  const feat_reduce_ws_t* ws)
+{
+  if(!h || !V || !P || !x || !r || (V->rows && (!V->rowptr || (V->nnz && (!V->val || !V->col)))) ||
+     (P->rows && (!P->rowptr || (P->nnz && (!P->val || !P->col)))))
+    return fail(FEAT_ERR_INVALID, "feat_stokes_transfer_post_f64: null argument");
+  if(V->cols != nvx) return fail(FEAT_ERR_INVALID, "feat_stokes_transfer_post_f64: V columns != nvx");
+  if(passes < 0 || passes > 2 || (passes && P->rows && (!w_dot || !w_add)))
+    return fail(FEAT_ERR_INVALID, "feat_stokes_transfer_post_f64: mean filter arguments");
+  if(!ws || !ws->counter) return fail(FEAT_ERR_INVALID, "feat_stokes_transfer_post_f64: workspace required");
+  if(r == x || (x_upd && (x_upd == x || x_upd == r)))
+    return fail(FEAT_ERR_ALIAS, "feat_stokes_transfer_post_f64: r, x and x_upd must be distinct");
+  const uint64_t nvb = (V->rows + kTrRows - 1) / kTrRows, npb = (P->rows + kTrRows - 1) / kTrRows;
+  if(!nvb && !npb) return FEAT_OK;
+  if(nvb + npb > uint64_t(INT32_MAX)) return fail(FEAT_ERR_UNSUPPORTED, "feat_stokes_transfer_post_f64: too many rows");
+  const int tail = (npb && P->rows <= kTupTailMaxP) ? 1 : 0;
+  const TuplePost post{w_dot, w_add, volume, passes, 0, x_upd, 1.0, nullptr, ws->counter, tail, 0};
+  const StCsr v{V->val, V->col, V->rowptr, V->rows}, p{P->val, P->col, P->rowptr, P->rows};
+  hipLaunchKernelGGL(k_stokes_transfer<true>, dim3(unsigned(nvb + npb)), dim3(kThreads), 0, h->stream, v, p, x, nvx, r,
+                     vmask, uint32_t(nvb), post);
+  if(npb && !tail && (passes || x_upd))
+    hipLaunchKernelGGL(k_mean_passes, dim3(1), dim3(kMeanThreads), 0, h->stream, r + 2 * V->rows, w_dot, w_add, volume,
+                       P->rows, passes, 0, x_upd ? x_upd + 2 * V->rows : nullptr, 1.0, nullptr, 0);
+  return check_launch("feat_stokes_transfer_post_f64");
 }
 
 int feat_mean_filter_passes_f64(feat_handle_t h, double* v, const double* w_dot, const double* w_add,
@@ -1568,7 +1738,7 @@ int feat_mean_filter_passes_f64(feat_handle_t h, double* v, const double* w_dot,
     return fail(FEAT_ERR_INVALID, "feat_mean_filter_passes_f64: bad argument");
   if(!n || !passes) return FEAT_OK;
   hipLaunchKernelGGL(k_mean_passes, dim3(1), dim3(kMeanThreads), 0, h->stream, v, w_dot, w_add, volume, n, passes,
-                     sol_form, nullptr, 0.0, nullptr);
+                     sol_form, nullptr, 0.0, nullptr, 0);
   return check_launch("feat_mean_filter_passes_f64");
 }
 
@@ -1580,9 +1750,13 @@ int feat_tuple_norm2_f64(feat_handle_t h, const double* v, uint64_t nu, uint64_t
   double* sums = h->scal + 8;   // two scratch scalars of the handle
   const int grid = grid_for(nu + np, 8);
   const GridReduce gd = reduce_for_grid(ws_reduce(ws, sums, sums + 1), uint64_t(grid));
-  hipLaunchKernelGGL(k_sumsq2, dim3(grid), dim3(kThreads), 0, h->stream, v, nu, np, gd);
-  finish_reduce(h->stream, gd, uint64_t(grid));
-  hipLaunchKernelGGL(k_tuple_norm_finish, dim3(1), dim3(1), 0, h->stream, sums, out);
+  // the norm is formed by the workgroup that completes the sums, unless they are deferred
+  hipLaunchKernelGGL(k_sumsq2, dim3(grid), dim3(kThreads), 0, h->stream, v, nu, np, gd, gd.defer ? nullptr : out);
+  if(gd.defer)
+  {
+    finish_reduce(h->stream, gd, uint64_t(grid));
+    hipLaunchKernelGGL(k_tuple_norm_finish, dim3(1), dim3(1), 0, h->stream, sums, out);
+  }
   return check_launch("feat_tuple_norm2_f64");
 }
 

@@ -447,7 +447,24 @@ int feat_tuple_apply_f64(feat_handle_t h, const feat_tuple_f64_t* M, double* r, 
 int feat_tuple_apply_post_f64(feat_handle_t h, const feat_tuple_f64_t* M, double* r, const double* x,
                               const double* y, double alpha, const uint32_t* vmask, const double* w_dot,
                               const double* w_add, double volume, int passes, int sol_form, double* x_upd,
-                              double omega, const double* gate_norm, const feat_reduce_ws_t* ws);
+                              double omega, const double* gate_norm, int x_upd_zero, const feat_reduce_ws_t* ws);
+/* x_upd_zero != 0: x_upd is taken to hold zeros and is not read -- the update writes
+ * omega * r, or zeros where it is skipped -- which is the smoother's first step after
+ * vec_cor.format() (Richardson::apply, richardson.hpp:166) without the format. */
+
+/* Stokes transfer and what follows it, in one launch (two above 4096 pressure rows):
+ * r = [V x_u ; P x_p] (TupleDiagMatrix<BWrappedCSR<2>, CSR>::apply, control/stokes_blocked.hpp:
+ * 193-202, transfer.hpp:241-259: velocity rows by the scalar CSR V on both components as
+ * feat_csrsb2_apply_f64_u64, pressure rows as feat_csr_apply_f64_u64 with a = 1, b = 0),
+ * velocity rows zeroed by vmask (UnitFilterBlocked::filter_def), `passes` MeanFilter passes on
+ * the pressure part (w_dot, w_add, volume: filter_def or filter_cor), then, with x_upd != NULL,
+ * x_upd += r (DenseVector::axpy with alpha 1, multigrid.hpp:1930).  Bit-identical to those
+ * calls in a row.  x: [u (2 per node, nvx nodes) | p]; r and x_upd: [u (2 per node, V->rows
+ * nodes) | p]; x, r and x_upd distinct; ws->counter as for feat_tuple_apply_post_f64. */
+int feat_stokes_transfer_post_f64(feat_handle_t h, const feat_csr_f64_t* V, const feat_csr_f64_t* P,
+                                  const double* x, uint64_t nvx, double* r, const uint32_t* vmask,
+                                  const double* w_dot, const double* w_add, double volume, int passes,
+                                  double* x_upd, const feat_reduce_ws_t* ws);
 
 /* The coarse levels of the Stokes multigrid V-cycle in one workgroup (no counterpart in the
  * reference: a launch-count fusion of StokesMultiGrid's mg_apply over its coarsest levels,

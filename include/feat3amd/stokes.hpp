@@ -106,13 +106,15 @@ namespace feat3amd
     /// `passes` MeanFilter passes on r's pressure part (weights w_dot, w_add, volume), then
     /// x_upd += omega r gated by *gate (x_upd null: no update) -- bit-identical to
     /// feat_tuple_apply_f64, feat_mean_filter_passes_f64 and feat_axpy_gated_f64 in a row.
+    /// (x_zero: x_upd holds zeros and is not read)
     inline void tuple_apply_post(const feat_tuple_f64_t& M, StokesVector& r, const StokesVector& x,
                                  const StokesVector* y, double alpha, const std::uint32_t* vmask, const double* w_dot,
                                  const double* w_add, double volume, int passes, double* x_upd, double omega,
-                                 const double* gate, const feat_reduce_ws_t& ws)
+                                 const double* gate, const feat_reduce_ws_t& ws, bool x_zero = false)
     {
       check(feat_tuple_apply_post_f64(Runtime::handle(), &M, r.elements(), x.elements(), y ? y->elements() : nullptr,
-                                      alpha, vmask, w_dot, w_add, volume, passes, 0, x_upd, omega, gate, &ws),
+                                      alpha, vmask, w_dot, w_add, volume, passes, 0, x_upd, omega, gate,
+                                      x_zero ? 1 : 0, &ws),
             "tuple apply + filter + update");
     }
 
@@ -256,6 +258,24 @@ namespace feat3amd
       void rest(const StokesVector& fine, StokesVector& coarse) const
       {
         apply(_rv, _rp, coarse.elements(), _nv_c, fine.elements(), _nv_f);
+      }
+      /// coarse = filter_def(R fine) in one launch (filter: the coarse level's)
+      void rest_filtered(const StokesVector& fine, StokesVector& coarse, const StokesFilter& cf,
+                         const feat_reduce_ws_t& ws) const
+      {
+        const feat_csr_f64_t v = desc(0), p = desc(1);
+        check(feat_stokes_transfer_post_f64(Runtime::handle(), &v, &p, fine.elements(), _nv_f, coarse.elements(),
+                                            cf.vmask(), cf.prim(), cf.dual(), cf.volume(), 1, nullptr, &ws),
+              "restriction + filter_def");
+      }
+      /// cor = filter_cor(P coarse), then sol += cor, in one launch (filter: the fine level's)
+      void prol_filtered_add(StokesVector& cor, const StokesVector& coarse, const StokesFilter& ff, StokesVector& sol,
+                             const feat_reduce_ws_t& ws) const
+      {
+        const feat_csr_f64_t v = desc(2), p = desc(3);
+        check(feat_stokes_transfer_post_f64(Runtime::handle(), &v, &p, coarse.elements(), _nv_c, cor.elements(),
+                                            ff.vmask(), ff.dual(), ff.prim(), ff.volume(), 1, sol.elements(), &ws),
+              "prolongation + filter_cor + axpy");
       }
       /// device descriptors: 0 = Rv, 1 = Rp, 2 = Pv, 3 = Pp
       feat_csr_f64_t desc(int k) const
@@ -423,10 +443,10 @@ namespace feat3amd
       /// apply(c, d, cor_passes) followed by x_upd += omega c gated by *gate (the Richardson
       /// step, richardson.hpp:203-211), in one or two launches (x_upd null: no update)
       void apply_update(LAFEM::StokesVector& c, const LAFEM::StokesVector& d, int cor_passes, double* x_upd,
-                        double omega, const double* gate, const feat_reduce_ws_t& ws) const
+                        double omega, const double* gate, const feat_reduce_ws_t& ws, bool x_zero = false) const
       {
         LAFEM::tuple_apply_post(vanka_desc(), c, d, nullptr, 1.0, _filter.vmask(), _filter.dual(), _filter.prim(),
-                                _filter.volume(), cor_passes, x_upd, omega, gate, ws);
+                                _filter.volume(), cor_passes, x_upd, omega, gate, ws, x_zero);
       }
 
       /// host copy of Vanka block k (0: uu, 1: up, 2: pu, 3: pp) and the macro mask
@@ -510,7 +530,7 @@ namespace feat3amd
       void smooth(Level& L, LAFEM::StokesVector& sol, const LAFEM::StokesVector& rhs)
       {
         feat_handle_t h = Runtime::handle();
-        sol.format();
+        // sol.format() is left to the first update, which writes omega c (or zeros) into sol
         check(feat_tuple_norm2_f64(h, rhs.elements(), 2 * L.K.nv(), L.K.np(), L.gate.get(), &_fw.ws), "smoother norm");
         // each step: c = filter_cor(V d) with sol += omega c, then d = filter_def(rhs - K sol),
         // one launch each on small levels
@@ -518,7 +538,7 @@ namespace feat3amd
         for(int s = 0; s < _cfg.steps; ++s)
         {
           const LAFEM::StokesVector& d = (s == 0) ? rhs : L.rdef;
-          L.V->apply_update(L.rcor, d, 2, sol.elements(), _cfg.omega_rich, L.gate.get(), _fw.ws);
+          L.V->apply_update(L.rcor, d, 2, sol.elements(), _cfg.omega_rich, L.gate.get(), _fw.ws, s == 0);
           if(s + 1 < _cfg.steps)
             LAFEM::tuple_apply_post(K, L.rdef, sol, &rhs, -1.0, L.F.vmask(), L.F.prim(), L.F.dual(), L.F.volume(), 1,
                                     nullptr, 0.0, nullptr, _fw.ws);
@@ -542,8 +562,7 @@ namespace feat3amd
           Level& C = *_lv[i + 1];
           smooth(L, L.sol, L.rhs);
           defect(L, L.def, L.sol, L.rhs);
-          L.T.rest(L.def, C.rhs);
-          C.F.filter_def(C.rhs);
+          L.T.rest_filtered(L.def, C.rhs, C.F, _fw.ws);
         }
         const std::size_t t = _tail_start;
         if(t < nl)
@@ -560,9 +579,7 @@ namespace feat3amd
         for(std::size_t i = std::min(t, nl - 1); i-- > 0;)
         {
           Level& L = *_lv[i];
-          L.T.prol(L.cor, _lv[i + 1]->sol);
-          L.F.filter_cor(L.cor);
-          L.sol.axpy(L.cor, L.sol, 1.0);
+          L.T.prol_filtered_add(L.cor, _lv[i + 1]->sol, L.F, L.sol, _fw.ws);
           defect(L, L.def, L.sol, L.rhs);
           smooth(L, L.cor, L.def);
           L.sol.axpy(L.cor, L.sol, 1.0);
