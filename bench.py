@@ -462,12 +462,18 @@ def learned_leg(feat, torch, dev, args):
     both to a 1e-8 FP64 relative defect; training on the GPU from the multigrid's own
     residual/error fields (fixed seeds, deterministic)."""
     t0 = time.perf_counter()
-    s = feat.LearnedMG(level_fine=6, level_coarse=1, lr=1e-3)
+    # the network on the first of the four steps of each finest smoother call, damped Jacobi on
+    # the other three (measured: 1 learned step 21.2 ms / 26 it, 2: 22.4 ms / 24 it, all four:
+    # 27.2 ms / 23 it, Jacobi 24.9 ms / 37 it)
+    learned_steps = 1
+    s = feat.LearnedMG(level_fine=6, level_coarse=1, lr=1e-3, learned_steps=learned_steps)
     rho = s.generate()
     e_jac = s.energy()
     t1 = time.perf_counter()
-    rounds, steps = 4, 300
-    fit = s.fit(rounds=rounds, steps=steps, on_policy=True, seed=1)
+    # trained on the fixed Jacobi-multigrid snapshots: networks trained after an on-policy
+    # refresh (snapshots of the learned smoother's own cycles) failed their validation solves
+    rounds, steps, on_policy = 4, 300, False
+    fit = s.fit(rounds=rounds, steps=steps, on_policy=on_policy, seed=1)
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t1
     e_lrn = s.energy()
@@ -492,7 +498,8 @@ def learned_leg(feat, torch, dev, args):
            "learned": {"iters": rl["iters"], "s_per_solve": tl, "final_rel_defect": rl["def_final"] / rl["def_init"],
                        "fell_back": rl["fell_back"], "fallback_iter": rl["fallback_iter"]},
            "speedup_vs_jacobi": tj / tl, "iteration_ratio": rj["iters"] / max(rl["iters"], 1),
-           "train": {"adam_steps": rounds * steps, "rounds": rounds, "on_policy_refresh": True, "s": t_train,
+           "learned_steps_per_smoother_call": learned_steps,
+           "train": {"adam_steps": rounds * steps, "rounds": rounds, "on_policy_refresh": on_policy, "s": t_train,
                      "setup_and_data_s": setup, "validation": fit,
                      "one_step_rel_energy_jacobi": e_jac, "one_step_rel_energy_learned": e_lrn},
            "rho_jacobi": rho}

@@ -918,7 +918,7 @@ class BatchSolver:
 class LearnedCfg(C.Structure):
     _fields_ = [("level_fine", _i), ("level_coarse", _i), ("sigma", _d), ("kz", _d), ("seed", _u64), ("nu", _i),
                 ("omega_jac", _d), ("omega_learned", _d), ("tol_rel", _d), ("max_iter", _u64), ("n_init", _i),
-                ("n_snap", _i), ("batch", _u64), ("lr", _f), ("fcg_m", _i)]
+                ("n_snap", _i), ("batch", _u64), ("lr", _f), ("fcg_m", _i), ("learned_steps", _i)]
 
 
 class LearnedResult(C.Structure):
@@ -934,12 +934,13 @@ class LearnedMG:
     a Jacobi or a learned (BF16 MLP) finest-level smoother (C++ host layer)."""
 
     def __init__(self, level_fine=6, level_coarse=1, sigma=1.5, kz=10.0, seed=28054777172512, nu=4, omega_jac=0.5,
-                 omega_learned=1.0, tol_rel=1e-8, max_iter=200, n_init=4, n_snap=2, batch=8192, lr=1e-3, fcg_m=4):
+                 omega_learned=1.0, tol_rel=1e-8, max_iter=200, n_init=4, n_snap=2, batch=8192, lr=1e-3, fcg_m=4,
+                 learned_steps=0):
         _, self.hl = libs()
         runtime_init()
         self.s = C.c_void_p()
         self.cfg = LearnedCfg(level_fine, level_coarse, sigma, kz, seed, nu, omega_jac, omega_learned, tol_rel, max_iter,
-                              n_init, n_snap, batch, lr, fcg_m)
+                              n_init, n_snap, batch, lr, fcg_m, learned_steps)
         check(self.hl.feat_learned_create(C.byref(self.s), C.byref(self.cfg)), "feat_learned_create")
         n, m = _u64(), _u64()
         check(self.hl.feat_learned_size(self.s, C.byref(n), C.byref(m)), "feat_learned_size")
@@ -955,8 +956,10 @@ class LearnedMG:
         check(self.hl.feat_learned_energy(self.s, C.byref(e)), "feat_learned_energy")
         return e.value
 
-    def fit(self, rounds=4, steps=300, on_policy=True, seed=1):
-        """Training with model selection on a second right-hand side (see feat_learned_fit)."""
+    def fit(self, rounds=4, steps=300, on_policy=False, seed=1):
+        """Training with model selection on a second right-hand side (see feat_learned_fit).
+        on_policy refreshes the snapshots with the learned smoother's own cycles before rounds
+        2.. (off by default: networks trained on them failed their validation solves)."""
         br, ji, bi = C.c_int(), _u64(), _u64()
         it = np.zeros(rounds, np.uint64)
         check(self.hl.feat_learned_fit(self.s, rounds, steps, int(on_policy), seed, C.byref(br), C.byref(ji), C.byref(bi),

@@ -917,6 +917,7 @@ int feat_learned_create(feat_learned_t* out, const feat_learned_cfg_t* cfg)
     c.batch = cfg->batch;
     c.lr = cfg->lr;
     c.fcg_m = cfg->fcg_m > 0 ? cfg->fcg_m : 1;
+    c.learned_steps = (cfg->learned_steps > 0 && cfg->learned_steps < cfg->nu) ? cfg->learned_steps : cfg->nu;
     auto s = std::make_unique<feat_learned_s>();
     s->mg = std::make_unique<Solver::LearnedMG>(c);
     s->mg->init();

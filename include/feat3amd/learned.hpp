@@ -53,6 +53,9 @@ namespace feat3amd
       int n_init = 4, n_snap = 2;
       Index batch = 8192;
       float lr = 1e-3f;
+      // learned steps per finest-level smoother call (1..nu); the remaining nu - learned_steps
+      // are damped Jacobi steps (the fused smoother on the current defect)
+      int learned_steps = 4;
     };
 
     class LearnedMG
@@ -82,6 +85,7 @@ namespace feat3amd
       DeviceArray<float> _y;
       DeviceArray<double> _S, _Rho, _ep, _q;
       Vec _cv, _cq;   // learned step direction c and A c
+      Vec _jy, _jt;   // Jacobi steps after the learned ones: their correction and scratch
       Index _partials_cap = 0;
       int _adam_step = 0;
       // training data
@@ -143,6 +147,17 @@ namespace feat3amd
                 "learned A c");
           check(feat_mlp_step_f64(h, _n, x.elements(), d.elements(), _cv.elements(), _cq.elements(), ca, cb),
                 "learned step");
+          // the trained network on its first learned_steps steps only: the rest are damped
+          // Jacobi steps from the current defect d (Richardson with the Jacobi preconditioner
+          // on A y = d, then x += y -- continuing the smoother from x)
+          if(net && !_cap_r && s + 1 == _cfg.learned_steps && s + 1 < steps)
+          {
+            check(feat_rj_smooth_f64(h, &A, L.inv.elements(), L.mask.get(), steps - s - 1, 1.0, d.elements(), nullptr,
+                                     _jy.elements(), _jt.elements(), nullptr, nullptr, nullptr, nullptr, &_fw.ws),
+                  "Jacobi steps after the learned ones");
+            x.axpy(_jy, x, 1.0);
+            break;
+          }
         }
       }
 
@@ -280,6 +295,8 @@ namespace feat3amd
         _n = _m * _m * _m;
         _dinv = Vec(_n);
         _cv = Vec(_n);
+        _jy = Vec(_n);
+        _jt = Vec(_n);
         _cq = Vec(_n);
         L0().A.extract_diag(_dinv);
         _dinv.component_invert(_dinv, 1.0);

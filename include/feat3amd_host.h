@@ -253,6 +253,8 @@ typedef struct {
   uint64_t batch;            /* training batch */
   float lr;
   int fcg_m;                 /* flexible CG: directions kept for A-orthogonalisation (>= 1) */
+  int learned_steps;         /* learned steps per finest smoother call, the rest damped Jacobi
+                                (0 or >= nu: all nu steps learned) */
 } feat_learned_cfg_t;
 typedef struct {
   int status;

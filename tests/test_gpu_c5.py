@@ -107,24 +107,33 @@ def test_c5_jacobi_mg_vs_oracle(feat, torch_gpu, oracle, c5):
 
 
 def test_c5_learned_solve_reaches_tolerance(feat, torch_gpu, c5):
-    """The learned smoother's multigrid, trained on the GPU on this problem, returns a
-    solution whose plain FP64 defect (formed on the host) is below 1e-8."""
+    """The learned smoother's multigrid, trained on the GPU on this problem (the network on the
+    first step of each finest smoother call, damped Jacobi on the other three): every
+    validation round of the training converges, the relative energy one learned step leaves
+    falls below the damped Jacobi step's, the solve takes fewer iterations than the Jacobi
+    multigrid without falling back, and its plain FP64 defect (formed on the host) is below
+    1e-8."""
     torch = torch_gpu
     levels, b = c5
     n = levels[0]["n"]
-    s = feat.LearnedMG(level_fine=LF, level_coarse=LC, sigma=SIGMA, kz=KZ, seed=SEED)
+    s = feat.LearnedMG(level_fine=LF, level_coarse=LC, sigma=SIGMA, kz=KZ, seed=SEED, learned_steps=1)
     try:
         assert s.n == n
         rho = s.generate()
+        e_jac = s.energy()
         fit = s.fit(rounds=4, steps=300)
+        e_lrn = s.energy()
         jac = s.solve(learned=False)
         x = torch.zeros(n, dtype=torch.float64, device="cuda")
         r = s.solve(learned=True, x=x)
     finally:
         s.close()
     print(f"C5 L6: Jacobi-MG {jac['iters']} iterations, learned {r['iters']} (fallback {r['fell_back']}), "
-          f"rho_jacobi {rho:.3f}, fit {fit}")
+          f"rho_jacobi {rho:.3f}, fit {fit}, one-step energy {e_jac:.3f} -> {e_lrn:.3f}")
+    assert all(it > 0 for it in fit["round_iters"]), fit
+    assert e_lrn < e_jac
     assert jac["status"] == "success" and r["status"] == "success"
+    assert not r["fell_back"] and r["iters"] < jac["iters"]
     A = levels[0]["A"]
     import scipy.sparse as sp
     S = sp.csr_matrix((A.val, A.col.astype(np.int64), A.rowptr.astype(np.int64)), shape=(n, n))
