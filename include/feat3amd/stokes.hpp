@@ -46,6 +46,19 @@ namespace feat3amd
         d.ptr = DeviceArray<std::uint64_t>::from_host(m.rowptr.data(), m.rowptr.size());
         return d;
       }
+      static BcsrDev adopt(Index rows, Index cols, int bh, int bw, DeviceArray<double> val,
+                           DeviceArray<std::uint64_t> col, DeviceArray<std::uint64_t> ptr)
+      {
+        BcsrDev d;
+        d.rows = rows;
+        d.cols = cols;
+        d.bh = bh;
+        d.bw = bw;
+        d.val = val;
+        d.col = col;
+        d.ptr = ptr;
+        return d;
+      }
     };
 
     /// SaddlePointMatrix<BCSR<2,2>, BCSR<2,1>, BCSR<1,2>> (lafem/saddle_point_matrix.hpp)
@@ -64,6 +77,17 @@ namespace feat3amd
         m._a = BcsrDev::from_host(S.A);
         m._b = BcsrDev::from_host(S.B);
         m._d = BcsrDev::from_host(S.D);
+        return m;
+      }
+      /// from blocks already on the device (feat_stokes_assemble_device)
+      static SaddlePointMatrix from_device(Index nv, Index np, BcsrDev a, BcsrDev b, BcsrDev d)
+      {
+        SaddlePointMatrix m;
+        m._nv = nv;
+        m._np = np;
+        m._a = a;
+        m._b = b;
+        m._d = d;
         return m;
       }
       Index nv() const { return _nv; }
@@ -148,6 +172,33 @@ namespace feat3amd
         for(Index i = 0; i < S.np; ++i)
         {
           const double t = S.prim[i] * S.dual[i];
+          v = v + t;
+        }
+        f._vol = v;
+        return f;
+      }
+      /// from arrays already on the device (feat_stokes_assemble_device); prim and dual are
+      /// read back once for the volume, summed as from_host sums it
+      static StokesFilter from_device(Index nv, Index np, DeviceArray<std::uint64_t> nodes, DeviceArray<double> sval,
+                                      DeviceArray<double> prim, DeviceArray<double> dual)
+      {
+        StokesFilter f;
+        f._nv = nv;
+        f._np = np;
+        f._nf = nodes.size();
+        f._nodes = nodes;
+        f._sval = sval;
+        f._vmask = DeviceArray<std::uint32_t>((nv + 31) / 32);
+        check(feat_unit_filter_mask_u64(Runtime::handle(), f._vmask.get(), nv, f._nodes.get(), f._nf), "StokesFilter mask");
+        f._prim = prim;
+        f._dual = dual;
+        std::vector<double> hp(np), hd(np);
+        prim.to_host(hp.data());
+        dual.to_host(hd.data());
+        double v = 0.0;
+        for(Index i = 0; i < np; ++i)
+        {
+          const double t = hp[i] * hd[i];
           v = v + t;
         }
         f._vol = v;
@@ -247,6 +298,36 @@ namespace feat3amd
         t._rp = Csr::from_host(Assembly::transpose(fine.Pp));
         t._nv_f = fine.Pv.rows;
         t._nv_c = fine.Pv.cols;
+        return t;
+      }
+      /// from the prolongations and restrictions already on the device (feat_stokes_assemble_device)
+      static StokesTransfer from_device(Index nv_f, Index np_f, Index nv_c, Index np_c, std::uint64_t nnz_v,
+                                        std::uint64_t nnz_p, DeviceArray<double> pv_val,
+                                        DeviceArray<std::uint64_t> pv_col, DeviceArray<std::uint64_t> pv_ptr,
+                                        DeviceArray<double> pp_val, DeviceArray<std::uint64_t> pp_col,
+                                        DeviceArray<std::uint64_t> pp_ptr, DeviceArray<double> rv_val,
+                                        DeviceArray<std::uint64_t> rv_col, DeviceArray<std::uint64_t> rv_ptr,
+                                        DeviceArray<double> rp_val, DeviceArray<std::uint64_t> rp_col,
+                                        DeviceArray<std::uint64_t> rp_ptr)
+      {
+        auto mk = [](Index rows, Index cols, std::uint64_t nnz, DeviceArray<double> v, DeviceArray<std::uint64_t> c,
+                     DeviceArray<std::uint64_t> p) {
+          Csr m;
+          m.rows = rows;
+          m.cols = cols;
+          m.nnz = nnz;
+          m.val = v;
+          m.col = c;
+          m.ptr = p;
+          return m;
+        };
+        StokesTransfer t;
+        t._pv = mk(nv_f, nv_c, nnz_v, pv_val, pv_col, pv_ptr);
+        t._pp = mk(np_f, np_c, nnz_p, pp_val, pp_col, pp_ptr);
+        t._rv = mk(nv_c, nv_f, nnz_v, rv_val, rv_col, rv_ptr);
+        t._rp = mk(np_c, np_f, nnz_p, rp_val, rp_col, rp_ptr);
+        t._nv_f = nv_f;
+        t._nv_c = nv_c;
         return t;
       }
       /// Transfer::prol (lafem/transfer.hpp:255-259): fine = P coarse
@@ -472,6 +553,7 @@ namespace feat3amd
       // largest level in that workgroup: it applies a level-2 tuple (210 unknowns) in 4.7 us
       // against ~9 us for a launch of its own, but a level-3 tuple (770) in 12.5 us against 9.5 us
       std::uint64_t tail_max_level_dofs = 512;
+      bool device_assembly = true;     // levels from feat_stokes_assemble_device (else the host assembly)
     };
 
     /// Richardson(MG-V(Richardson(SchwarzPrecond(AmaVanka)))) on a Q2/P1dc hierarchy.
@@ -587,6 +669,90 @@ namespace feat3amd
         cor.copy(_lv[0]->sol);
       }
 
+      // One level from feat_stokes_assemble_device: matrices, filter and transfers stay on
+      // the device; the host keeps the B and D patterns that AmaVanka's symbolic phase walks.
+      void _assemble_on_device(Level& L, int l, bool transfer)
+      {
+        std::uint64_t z[10];
+        check(feat_stokes_device_sizes(l, z), "feat_stokes_device_sizes");
+        const Index nv = z[0], np = z[1], na = z[2], nb = z[3], nd = z[4], nf = z[5], npv = z[6], npp = z[7];
+        const Index nvc = z[8], npc = z[9];
+        DeviceArray<double> av(4 * na), bv(2 * nb), dv(2 * nd), fv(2 * nf), pr(np), du(np);
+        DeviceArray<std::uint64_t> ac(na), ap(nv + 1), bc(nb), bp(nv + 1), dc(nd), dp(np + 1), fn(nf);
+        DeviceArray<double> pvv, ppv, rvv, rpv;
+        DeviceArray<std::uint64_t> pvc, pvp, ppc, ppp, rvc, rvp, rpc, rpp;
+        feat_stokes_level_dev_t o{};
+        o.a_val = av.get();
+        o.a_col = ac.get();
+        o.a_ptr = ap.get();
+        o.b_val = bv.get();
+        o.b_col = bc.get();
+        o.b_ptr = bp.get();
+        o.d_val = dv.get();
+        o.d_col = dc.get();
+        o.d_ptr = dp.get();
+        o.filt = fn.get();
+        o.filt_val = fv.get();
+        o.prim = pr.get();
+        o.dual = du.get();
+        if(transfer)
+        {
+          pvv = DeviceArray<double>(npv);
+          pvc = DeviceArray<std::uint64_t>(npv);
+          pvp = DeviceArray<std::uint64_t>(nv + 1);
+          ppv = DeviceArray<double>(npp);
+          ppc = DeviceArray<std::uint64_t>(npp);
+          ppp = DeviceArray<std::uint64_t>(np + 1);
+          rvv = DeviceArray<double>(npv);
+          rvc = DeviceArray<std::uint64_t>(npv);
+          rvp = DeviceArray<std::uint64_t>(nvc + 1);
+          rpv = DeviceArray<double>(npp);
+          rpc = DeviceArray<std::uint64_t>(npp);
+          rpp = DeviceArray<std::uint64_t>(npc + 1);
+          o.pv_val = pvv.get();
+          o.pv_col = pvc.get();
+          o.pv_ptr = pvp.get();
+          o.pp_val = ppv.get();
+          o.pp_col = ppc.get();
+          o.pp_ptr = ppp.get();
+          o.rv_val = rvv.get();
+          o.rv_col = rvc.get();
+          o.rv_ptr = rvp.get();
+          o.rp_val = rpv.get();
+          o.rp_col = rpc.get();
+          o.rp_ptr = rpp.get();
+        }
+        check(feat_stokes_assemble_device(Runtime::handle(), l, &o), "feat_stokes_assemble_device");
+        L.K = LAFEM::SaddlePointMatrix::from_device(nv, np, LAFEM::BcsrDev::adopt(nv, nv, 2, 2, av, ac, ap),
+                                                    LAFEM::BcsrDev::adopt(nv, np, 2, 1, bv, bc, bp),
+                                                    LAFEM::BcsrDev::adopt(np, nv, 1, 2, dv, dc, dp));
+        L.F = LAFEM::StokesFilter::from_device(nv, np, fn, fv, pr, du);
+        if(transfer)
+          L.T = LAFEM::StokesTransfer::from_device(nv, np, nvc, npc, npv, npp, pvv, pvc, pvp, ppv, ppc, ppp, rvv, rvc,
+                                                   rvp, rpv, rpc, rpp);
+        // the patterns of B and D for AmaVanka::init_symbolic
+        L.host = Assembly::StokesLevelHost{};
+        L.host.level = l;
+        L.host.nv = nv;
+        L.host.np = np;
+        L.host.B.rows = nv;
+        L.host.B.cols = np;
+        L.host.B.bh = 2;
+        L.host.B.bw = 1;
+        L.host.B.rowptr.resize(nv + 1);
+        L.host.B.col.resize(nb);
+        bp.to_host(L.host.B.rowptr.data());
+        bc.to_host(L.host.B.col.data());
+        L.host.D.rows = np;
+        L.host.D.cols = nv;
+        L.host.D.bh = 1;
+        L.host.D.bw = 2;
+        L.host.D.rowptr.resize(np + 1);
+        L.host.D.col.resize(nd);
+        dp.to_host(L.host.D.rowptr.data());
+        dc.to_host(L.host.D.col.data());
+      }
+
       // The coarsest levels whose vectors fit the fused workgroup's LDS, each of at most
       // StokesConfig::tail_max_level_dofs unknowns.
       void _tail_plan()
@@ -654,10 +820,15 @@ namespace feat3amd
         for(int l = level_fine; l >= level_coarse; --l)
         {
           auto L = std::make_unique<Level>();
-          L->host = Assembly::stokes_q2p1dc(l);
-          L->K = LAFEM::SaddlePointMatrix::from_host(L->host);
-          L->F = LAFEM::StokesFilter::from_host(L->host);
-          if(l > level_coarse) L->T = LAFEM::StokesTransfer::from_host(L->host);
+          if(_cfg.device_assembly)
+            _assemble_on_device(*L, l, l > level_coarse);
+          else
+          {
+            L->host = Assembly::stokes_q2p1dc(l);
+            L->K = LAFEM::SaddlePointMatrix::from_host(L->host);
+            L->F = LAFEM::StokesFilter::from_host(L->host);
+            if(l > level_coarse) L->T = LAFEM::StokesTransfer::from_host(L->host);
+          }
           _lv.push_back(std::move(L));
         }
       }
