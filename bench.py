@@ -203,6 +203,7 @@ def batch_leg(feat, torch, dev, args):
            "s_per_batch": t_batch, "s_sequential_single_solves": t_seq, "speedup_vs_sequential": t_seq / t_batch,
            "iters_min": int(res["iters"].min()), "iters_max": int(res["iters"].max()),
            "rel_diff_vs_sequential": err, "levels": [int(v) for v in bs.levels], "setup_s": setup,
+           "coarse_tail_levels": bs.tail_levels,
            "autotuned_shapes": [int(t) for t in tunes], "autotuned_spmv_us": [float(u) for u in us]}
     log(f"batch of 64: {t_batch * 1e3:.2f} ms (sequential single solves {t_seq * 1e3:.2f} ms, "
         f"{t_seq / t_batch:.2f}x), iterations {out['iters_min']}..{out['iters_max']}, |x-x_seq|/|x_seq| {err:.2e}")

@@ -203,6 +203,7 @@ KERNEL_API = [
     ("feat_batch_copy_scalars_f64", _i, [_vp, C.c_uint32, _vp, _vp, _vp]),
     ("feat_batch_status_f64", _i, [_vp, C.c_uint32, _vp, _vp, _vp, _i]),
     ("feat_batch_gather_f64", _i, [_vp, C.c_uint32, _vp, _vp, _vp, _vp, _vp]),
+    ("feat_batch_tail_f64", _i, [_vp, _vp, _vp, _vp]),
     ("feat_copy_int_to_host", _i, [_vp, _vp, _vp]),
     ("feat_autotune_matrix_f64", _i, [_vp, _vp, _vp, _vp, _i, _vp]),
     ("feat_mlp_to_bf16", _i, [_vp, _vp, _vp]),
@@ -253,6 +254,7 @@ HOST_API = [
     ("feat_batchsolver_layout", _i, [_vp, _vp, _vp]),
     ("feat_batchsolver_solve", _i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     ("feat_batchsolver_tuning", _i, [_vp, _vp, _vp, _i, C.POINTER(_i)]),
+    ("feat_batchsolver_tail_levels", _i, [_vp, C.POINTER(_i)]),
     ("feat_batchsolver_destroy", _i, [_vp]),
     ("feat_batch_levels", _i, [_u64, _i, _i, _i, _vp]),
     ("feat_lognormal_cells", _i, [_u64, _u64, _d, _vp]),
@@ -789,7 +791,7 @@ class Stokes:
 
 class BatchCfg(C.Structure):
     _fields_ = [("seed", _u64), ("ninst", _i), ("lmin", _i), ("lmax", _i), ("sigma", _d), ("tol_rel", _d),
-                ("max_iter", _u64), ("autotune", _i), ("graphs", _i)]
+                ("max_iter", _u64), ("autotune", _i), ("graphs", _i), ("tail_levels", _i)]
 
 
 def batch_levels(seed, ninst, lmin, lmax):
@@ -868,11 +870,12 @@ class BatchSolver:
     """The batch of independent Q1 diffusion MG-PCG solves (C++ host layer)."""
 
     def __init__(self, seed=28054777172512, ninst=64, lmin=3, lmax=8, sigma=1.0, tol_rel=1e-8, max_iter=100,
-                 autotune=True, graphs=True):
+                 autotune=True, graphs=True, tail_levels=5):
         _, self.hl = libs()
         runtime_init()
         self.s = C.c_void_p()
-        self.cfg = BatchCfg(seed, ninst, lmin, lmax, sigma, tol_rel, max_iter, int(autotune), int(graphs))
+        self.cfg = BatchCfg(seed, ninst, lmin, lmax, sigma, tol_rel, max_iter, int(autotune), int(graphs),
+                            int(tail_levels))
         check(self.hl.feat_batchsolver_create(C.byref(self.s), C.byref(self.cfg)), "feat_batchsolver_create")
         self.levels = np.zeros(ninst, np.int32)
         self.off = np.zeros(ninst + 1, np.uint64)
@@ -894,6 +897,12 @@ class BatchSolver:
                                              d1.ctypes.data, hist.ctypes.data), "feat_batchsolver_solve")
         return dict(status=[STATUS[int(v)] for v in st], iters=it.astype(int), def_init=d0, def_final=d1,
                     hist=hist.reshape(ni, self.max_iter + 1))
+
+    @property
+    def tail_levels(self):
+        d = C.c_int()
+        check(self.hl.feat_batchsolver_tail_levels(self.s, C.byref(d)), "feat_batchsolver_tail_levels")
+        return d.value
 
     def tuning(self):
         t = np.zeros(64, np.int32)

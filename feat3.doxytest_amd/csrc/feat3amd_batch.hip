@@ -191,6 +191,312 @@ __global__ __launch_bounds__(kThreads) void k_batch_gather(uint32_t nseg, const 
 
 __global__ void k_copy_int(const int* __restrict__ src, int* __restrict__ dst) { *dst = *src; }
 
+// ---------------------------------------------------------------------------
+// Coarse tail of the batched V-cycle: one workgroup per instance runs its levels
+// min(finest, T) .. 1 with every vector in LDS.  A thread owns rows tid and tid + kBt of
+// a level (a level has at most FEAT_BATCH_TAIL_MAX_ROWS < 2 kBt rows) and holds their
+// entries of A in registers for a whole smoother call, so the sweeps between barriers
+// touch LDS only; P and R rows are read once per transfer.  Each row is formed with
+// the operations of the batched launches it stands for: k_smooth_first (first
+// iterate), EpiSmooth modes 0 / 1 / 2, EpiMasked (restriction), EpiProlAdd and
+// EpiDefect, with the row sum as the sequential fma over the row's entries in CSR
+// order (the order of every SELL-64 plan of those launches): bit-identical results.
+constexpr int kBt = 1024;
+constexpr int kBtRows = 2;  // rows per thread
+constexpr int kBtLen = FEAT_BATCH_TAIL_MAX_ROW_LEN;
+static_assert(FEAT_BATCH_TAIL_MAX_ROWS <= kBt * kBtRows, "rows per thread");
+
+struct BtRow
+{
+  double v[kBtLen];
+  uint32_t c[kBtLen];  // column, local to the instance's block of the source vector
+  int len;
+};
+
+// the row's entries (len = 0 past the level's rows)
+__device__ __forceinline__ void bt_load(BtRow& R, const feat_csr_f64_t& M, uint64_t g, uint64_t coff, bool on)
+{
+  R.len = 0;
+  if(!on) return;
+  const uint64_t b = M.rowptr[g];
+  R.len = int(M.rowptr[g + 1] - b);
+#pragma unroll
+  for(int u = 0; u < kBtLen; ++u)
+    if(u < R.len)
+    {
+      R.v[u] = M.val[b + u];
+      R.c[u] = uint32_t(M.col[b + u] - coff);
+    }
+}
+
+__device__ __forceinline__ double bt_dot(const BtRow& R, const double* x)
+{
+  double s = 0.0;
+#pragma unroll
+  for(int u = 0; u < kBtLen; ++u)
+    if(u < R.len) s = dfma(R.v[u], x[R.c[u]], s);
+  return s;
+}
+
+__global__ __launch_bounds__(kBt) void k_batch_tail(feat_batch_tail_t Targ, const double* __restrict__ r,
+                                                    double* __restrict__ z)
+{
+  // the level descriptors are indexed by level at run time: one copy in LDS
+  __shared__ feat_batch_tail_t T;
+  __shared__ uint64_t s_loff[FEAT_BATCH_TAIL_MAX_LEVELS + 1];
+  __shared__ uint32_t s_base[FEAT_BATCH_TAIL_MAX_LEVELS + 1];
+  // per level (at s_base[k]): right-hand side, solution, Jacobi inverse diagonal, filter flag
+  __shared__ double s_rhs[FEAT_BATCH_TAIL_MAX_TOTAL], s_sol[FEAT_BATCH_TAIL_MAX_TOTAL];
+  __shared__ double s_inv[FEAT_BATCH_TAIL_MAX_TOTAL];
+  __shared__ uint8_t s_m[FEAT_BATCH_TAIL_MAX_TOTAL];
+  // the two smoother buffers and the defect of the level being worked on
+  __shared__ double s_b0[FEAT_BATCH_TAIL_MAX_ROWS], s_b1[FEAT_BATCH_TAIL_MAX_ROWS], s_d[FEAT_BATCH_TAIL_MAX_ROWS];
+  const uint32_t inst = blockIdx.x;
+  const int fin = Targ.finest[inst];
+  const int top = fin < Targ.nlevels ? fin : Targ.nlevels;
+  const bool deep = fin > Targ.nlevels;
+  if(threadIdx.x == 0)
+  {
+    T = Targ;
+    uint32_t o = 0;
+    for(int k = 1; k <= top; ++k)
+    {
+      s_loff[k] = Targ.loff[uint64_t(k - 1) * Targ.ninst + inst];
+      s_base[k] = o;
+      o += uint32_t(Targ.lv[k - 1].n);
+    }
+  }
+  __syncthreads();
+  const int tid = threadIdx.x;
+  const double omega = T.omega;
+  for(int k = 1; k <= top; ++k)
+  {
+    const feat_batch_tail_level_t& L = T.lv[k - 1];
+    const uint64_t g0 = s_loff[k];
+    const uint32_t b = s_base[k];
+    for(uint32_t j = tid; j < L.n; j += kBt)
+    {
+      s_inv[b + j] = L.inv[g0 + j];
+      s_m[b + j] = is_masked(L.mask, g0 + j) ? 1 : 0;
+    }
+  }
+  {
+    const uint64_t n = T.lv[top - 1].n;
+    const uint32_t b = s_base[top];
+    for(uint32_t j = tid; j < n; j += kBt)
+      s_rhs[b + j] = deep ? T.rhs_deep[s_loff[top] + j] : r[T.pcg_off[inst] + j];
+  }
+  __syncthreads();
+
+  // A's rows of this thread on level k, in registers
+  BtRow A[kBtRows];
+  auto load_a = [&](int k) {
+    const feat_batch_tail_level_t& L = T.lv[k - 1];
+#pragma unroll
+    for(int q = 0; q < kBtRows; ++q)
+    {
+      const uint32_t j = tid + q * kBt;
+      bt_load(A[q], L.A, s_loff[k] + j, s_loff[k], j < L.n);
+    }
+  };
+
+  // Richardson(Jacobi) pre-smoothing from x = 0 (feat_rj_smooth_f64 with x = sol, tmp and
+  // d_out): iterates alternate between sol and s_b1 so that the last one lands in sol;
+  // the defect of the last iterate goes to s_d.
+  auto pre = [&](int k) {
+    const uint32_t n = uint32_t(T.lv[k - 1].n), b = s_base[k];
+    const double* rhs = s_rhs + b;
+    const double* inv = s_inv + b;
+    const uint8_t* m = s_m + b;
+    double* x = s_sol + b;
+    const int nu = T.nu_pre;
+    auto buf = [&](int s) { return ((nu - s) & 1) ? s_b1 : x; };
+    load_a(k);
+    {
+      double* x1 = buf(1);
+#pragma unroll
+      for(int q = 0; q < kBtRows; ++q)
+      {
+        const uint32_t j = tid + q * kBt;
+        if(j < n)
+        {
+          double c = inv[j] * rhs[j];
+          if(m[j]) c = 0.0;
+          x1[j] = dfma(omega, c, 0.0);
+        }
+      }
+    }
+    for(int s = 1; s <= nu; ++s)
+    {
+      __syncthreads();
+      const double* xk = buf(s);
+      double* xo = buf(s + 1);
+#pragma unroll
+      for(int q = 0; q < kBtRows; ++q)
+      {
+        const uint32_t j = tid + q * kBt;
+        if(j < n)
+        {
+          const double sum = bt_dot(A[q], xk);
+          double d = dfma(sum, -1.0, rhs[j]);
+          if(m[j]) d = 0.0;
+          if(s == nu)
+            s_d[j] = d;
+          else
+          {
+            double c = inv[j] * d;
+            if(m[j]) c = 0.0;
+            xo[j] = dfma(omega, c, xk[j]);
+          }
+        }
+      }
+    }
+    __syncthreads();
+  };
+
+  // rhs of level k-1 = filter(R s_d)
+  auto restrict_to = [&](int k) {
+    const feat_batch_tail_level_t& L = T.lv[k - 1];
+    const uint32_t nc = uint32_t(T.lv[k - 2].n), bc = s_base[k - 1];
+#pragma unroll
+    for(int q = 0; q < kBtRows; ++q)
+    {
+      const uint32_t j = tid + q * kBt;
+      BtRow R;
+      bt_load(R, L.R, s_loff[k - 1] + j, s_loff[k], j < nc);
+      if(j < nc)
+      {
+        double v = dfma(bt_dot(R, s_d), 1.0, 0.0);
+        if(s_m[bc + j]) v = 0.0;
+        s_rhs[bc + j] = v;
+      }
+    }
+    __syncthreads();
+  };
+
+  // sol of level k += filter(P sol of level k-1), then s_d = filter(rhs - A sol)
+  auto prol_defect = [&](int k) {
+    const feat_batch_tail_level_t& L = T.lv[k - 1];
+    const uint32_t n = uint32_t(L.n), b = s_base[k];
+    const double* xc = s_sol + s_base[k - 1];
+    double* x = s_sol + b;
+    const uint8_t* m = s_m + b;
+#pragma unroll
+    for(int q = 0; q < kBtRows; ++q)
+    {
+      const uint32_t j = tid + q * kBt;
+      BtRow P;
+      bt_load(P, L.P, s_loff[k] + j, s_loff[k - 1], j < n);
+      if(j < n)
+      {
+        double c = dfma(bt_dot(P, xc), 1.0, 0.0);
+        if(m[j]) c = 0.0;
+        x[j] = dfma(1.0, c, x[j]);
+      }
+    }
+    load_a(k);
+    __syncthreads();
+#pragma unroll
+    for(int q = 0; q < kBtRows; ++q)
+    {
+      const uint32_t j = tid + q * kBt;
+      if(j < n)
+      {
+        double d = dfma(bt_dot(A[q], x), -1.0, s_rhs[b + j]);
+        if(m[j]) d = 0.0;
+        s_d[j] = d;
+      }
+    }
+    __syncthreads();
+  };
+
+  // Richardson(Jacobi) post-smoothing on s_d from 0, its last iterate added to sol
+  // (feat_rj_smooth_f64 with x = cor, tmp and acc = sol); A's rows are still loaded
+  auto post = [&](int k) {
+    const uint32_t n = uint32_t(T.lv[k - 1].n), b = s_base[k];
+    const double* inv = s_inv + b;
+    const uint8_t* m = s_m + b;
+    double* x = s_sol + b;
+    const int nu = T.nu_post;
+    auto buf = [&](int s) { return ((nu - s) & 1) ? s_b1 : s_b0; };
+    {
+      double* x1 = buf(1);
+#pragma unroll
+      for(int q = 0; q < kBtRows; ++q)
+      {
+        const uint32_t j = tid + q * kBt;
+        if(j < n)
+        {
+          double c = inv[j] * s_d[j];
+          if(m[j]) c = 0.0;
+          const double xn = dfma(omega, c, 0.0);
+          if(nu == 1)
+            x[j] = dfma(1.0, xn, x[j]);
+          else
+            x1[j] = xn;
+        }
+      }
+    }
+    for(int s = 1; s <= nu - 1; ++s)
+    {
+      __syncthreads();
+      const double* xk = buf(s);
+      double* xo = buf(s + 1);
+#pragma unroll
+      for(int q = 0; q < kBtRows; ++q)
+      {
+        const uint32_t j = tid + q * kBt;
+        if(j < n)
+        {
+          double d = dfma(bt_dot(A[q], xk), -1.0, s_d[j]);
+          if(m[j]) d = 0.0;
+          double c = inv[j] * d;
+          if(m[j]) c = 0.0;
+          const double xn = dfma(omega, c, xk[j]);
+          if(s == nu - 1)
+            x[j] = dfma(1.0, xn, x[j]);
+          else
+            xo[j] = xn;
+        }
+      }
+    }
+    __syncthreads();
+  };
+
+  for(int k = top; k >= 2; --k)
+  {
+    pre(k);
+    restrict_to(k);
+  }
+  {
+    // level 1: one Jacobi step with omega 1 from x = 0
+    const uint32_t n = uint32_t(T.lv[0].n), b = s_base[1];
+    for(uint32_t j = tid; j < n; j += kBt)
+    {
+      double c = s_inv[b + j] * s_rhs[b + j];
+      if(s_m[b + j]) c = 0.0;
+      s_sol[b + j] = dfma(1.0, c, 0.0);
+    }
+    __syncthreads();
+  }
+  for(int k = 2; k <= top; ++k)
+  {
+    prol_defect(k);
+    post(k);
+  }
+  {
+    const uint64_t n = T.lv[top - 1].n;
+    const uint32_t b = s_base[top];
+    for(uint32_t j = tid; j < n; j += kBt)
+    {
+      if(deep)
+        T.sol_deep[s_loff[top] + j] = s_sol[b + j];
+      else
+        z[T.pcg_off[inst] + j] = s_sol[b + j];
+    }
+  }
+}
+
 bool batch_ok(const feat_batch_t* b)
 {
   return b && b->off && b->chunk_inst && b->chunk_lo && b->chunk_hi && b->inst_chunk && b->partials;
@@ -266,6 +572,34 @@ int feat_batch_gather_f64(feat_handle_t h, uint32_t nseg, const uint64_t* dst_lo
   hipLaunchKernelGGL(k_batch_gather, dim3(64, nseg < 4096 ? nseg : 4096), dim3(kThreads), 0, h->stream, nseg, dst_lo,
                      src_lo, len, dst, src);
   return check_launch("feat_batch_gather_f64");
+}
+
+int feat_batch_tail_f64(feat_handle_t h, const feat_batch_tail_t* t, const double* r, double* z)
+{
+  const char* name = "feat_batch_tail_f64";
+  if(!h || !t || !t->finest || !t->loff || !t->pcg_off) return fail(FEAT_ERR_INVALID, name);
+  const int nl = t->nlevels;
+  if(nl < 1 || nl > FEAT_BATCH_TAIL_MAX_LEVELS || t->nu_pre < 1 || t->nu_post < 1)
+    return fail(FEAT_ERR_INVALID, "feat_batch_tail_f64: nlevels or smoothing steps out of range");
+  uint64_t total = 0;
+  for(int k = 1; k <= nl; ++k)
+  {
+    const feat_batch_tail_level_t& L = t->lv[k - 1];
+    const bool transfer = k > 1;
+    if(L.n < 1 || !L.inv || !L.mask || !L.A.val || !L.A.col || !L.A.rowptr ||
+       (transfer && (!L.P.val || !L.P.col || !L.P.rowptr || !L.R.val || !L.R.col || !L.R.rowptr)))
+      return fail(FEAT_ERR_INVALID, "feat_batch_tail_f64: level descriptor incomplete");
+    total += L.n;
+    if(L.n > FEAT_BATCH_TAIL_MAX_ROWS) total = FEAT_BATCH_TAIL_MAX_TOTAL + 1;
+  }
+  if(total > FEAT_BATCH_TAIL_MAX_TOTAL)
+    return fail(FEAT_ERR_INVALID, "feat_batch_tail_f64: levels exceed the workgroup's LDS vectors");
+  if(!t->ninst) return FEAT_OK;
+  if(!r || !z || !t->rhs_deep || !t->sol_deep) return fail(FEAT_ERR_INVALID, "feat_batch_tail_f64: null vector");
+  if(static_cast<const void*>(r) == z || t->rhs_deep == t->sol_deep)
+    return fail(FEAT_ERR_ALIAS, "feat_batch_tail_f64: input and output alias");
+  hipLaunchKernelGGL(k_batch_tail, dim3(t->ninst), dim3(kBt), 0, h->stream, *t, r, z);
+  return check_launch(name);
 }
 
 int feat_copy_int_to_host(feat_handle_t h, const int* dev, int* host_pinned)

@@ -790,6 +790,7 @@ int feat_batchsolver_create(feat_batchsolver_t* out, const feat_batch_cfg_t* cfg
     c.max_iter = cfg->max_iter;
     c.autotune = cfg->autotune != 0;
     c.graphs = cfg->graphs != 0;
+    c.tail_levels = cfg->tail_levels > 0 ? cfg->tail_levels : 0;
     auto s = std::make_unique<feat_batchsolver_s>();
     s->b = std::make_unique<Solver::BatchedMgPcg>(c);
     s->b->init();
@@ -820,6 +821,14 @@ int feat_batchsolver_solve(feat_batchsolver_t s, double* x, const double* b, int
     if(def_init) std::copy(R.def_init.begin(), R.def_init.end(), def_init);
     if(def_final) std::copy(R.def_final.begin(), R.def_final.end(), def_final);
     if(hist) std::copy(R.hist.begin(), R.hist.end(), hist);
+  });
+}
+
+int feat_batchsolver_tail_levels(feat_batchsolver_t s, int* levels)
+{
+  return guarded([&]() {
+    if(!s || !levels) throw Error("feat_batchsolver_tail_levels: null argument");
+    *levels = s->b->tail_depth();
   });
 }
 

@@ -616,6 +616,47 @@ int feat_batch_gather_f64(feat_handle_t h, uint32_t nseg, const uint64_t* dst_lo
 /* copies a device int to pinned host memory (stream-ordered) */
 int feat_copy_int_to_host(feat_handle_t h, const int* dev, int* host_pinned);
 
+/* The coarse levels 1..T of the batched V-cycle (solver/multigrid.hpp:1270-2000 per
+ * instance, as the batched launches run it: Richardson(Jacobi) pre-smoothing with the
+ * defect of the last iterate, restriction, one damped Jacobi step (omega 1) on level 1,
+ * prolongation + defect + post-smoothing accumulated into the level's solution), one
+ * workgroup per instance with the instance's vectors of every tail level in LDS.
+ * Level k (index k-1 of lv) is block-diagonal over the instances whose finest level is
+ * >= k; instance i owns rows [loff[(k-1)*ninst + i], + n) of A, inv, mask and of P's rows
+ * (P's columns: its rows of level k-1), and R = P^T has the transposed blocks.  An
+ * instance whose finest level f is <= T enters at level f with r[pcg_off[i] + j] and
+ * leaves z[pcg_off[i] + j]; one with f > T takes level T's right-hand side from
+ * rhs_deep and leaves level T's solution in sol_deep (both in level T's layout).
+ * Every row is formed with the operations of feat_rj_smooth_f64, feat_apply_masked_f64,
+ * feat_prol_add_f64 and feat_defect_f64 (row sums in CSR order), so the results are
+ * bit-identical to those calls.  Limits: n of every level <= FEAT_BATCH_TAIL_MAX_ROWS, the
+ * sum of n over the levels <= FEAT_BATCH_TAIL_MAX_TOTAL, at most FEAT_BATCH_TAIL_MAX_ROW_LEN
+ * entries in any row of A, P and R.  No single reference function: the coarse end of
+ * MultiGrid::apply for every instance at once. */
+#define FEAT_BATCH_TAIL_MAX_LEVELS 8
+#define FEAT_BATCH_TAIL_MAX_ROWS 1100
+#define FEAT_BATCH_TAIL_MAX_TOTAL 1536
+#define FEAT_BATCH_TAIL_MAX_ROW_LEN 9
+typedef struct {
+  uint64_t n;                  /* rows of one instance on this level */
+  feat_csr_f64_t A, P, R;      /* P, R: to and from level k-1 (unused on level 1) */
+  const double* inv;           /* Jacobi: omega_jac / diag (level 1: 1 / diag) */
+  const uint32_t* mask;        /* Dirichlet rows (UnitFilter), bit per row */
+} feat_batch_tail_level_t;
+typedef struct {
+  int nlevels;                 /* T */
+  uint32_t ninst;
+  int nu_pre, nu_post;
+  double omega;                /* Richardson damping of the smoothers */
+  const int* finest;           /* per instance */
+  const uint64_t* loff;        /* T * ninst */
+  const uint64_t* pcg_off;     /* per instance */
+  const double* rhs_deep;
+  double* sol_deep;
+  feat_batch_tail_level_t lv[FEAT_BATCH_TAIL_MAX_LEVELS];
+} feat_batch_tail_t;
+int feat_batch_tail_f64(feat_handle_t h, const feat_batch_tail_t* t, const double* r, double* z);
+
 /* ---------------- structured Q1 hierarchy on the device ----------------
  * The Q1 Laplace problem on level `level` of the unit square (dim 2) / cube (dim 3),
  * lexicographic node numbering: stiffness matrix (SymbolicAssembler + LaplaceOperator),

@@ -43,6 +43,9 @@ namespace feat3amd
       Index max_iter = 100;
       bool autotune = true;
       bool graphs = true;
+      /// levels 1..tail_levels of every instance run in one launch (feat_batch_tail_f64,
+      /// one workgroup per instance, bit-identical); lowered to what the workgroup holds; 0: off
+      int tail_levels = 5;
     };
 
     class BatchedMgPcg
@@ -132,6 +135,76 @@ namespace feat3amd
       int* _host_nactive = nullptr;
       const void* _key_x = nullptr;
       std::vector<double> _tune_us;
+      // coarse tail: levels 1.._tail.nlevels in one launch (0 levels: off)
+      feat_batch_tail_t _tail{};
+      DeviceArray<int> _tfin;
+      DeviceArray<std::uint64_t> _tloff, _tpcg;
+
+      static std::uint64_t max_row_len(const Assembly::CsrHost& M)
+      {
+        std::uint64_t m = 0;
+        for(std::size_t r = 0; r + 1 < M.rowptr.size(); ++r) m = std::max<std::uint64_t>(m, M.rowptr[r + 1] - M.rowptr[r]);
+        return m;
+      }
+      static feat_csr_f64_t csr_desc(const Csr& M)
+      {
+        return feat_csr_f64_t{M.rows(), M.columns(), M.used_elements(), M.val(), M.col_ind(), M.row_ptr()};
+      }
+
+      // the deepest T <= tail_levels whose levels fit one workgroup (rows, LDS vectors, row lengths)
+      void plan_tail(const std::vector<std::uint64_t>& rowlen)
+      {
+        _tail = feat_batch_tail_t{};
+        int T = std::min(_cfg.tail_levels, std::min(_lmax, FEAT_BATCH_TAIL_MAX_LEVELS));
+        auto rows_of = [&](int k) { return _lv[k]->A.rows() / _lv[k]->inst.size(); };
+        for(; T >= 1; --T)
+        {
+          std::uint64_t total = 0;
+          bool ok = rows_of(T) <= FEAT_BATCH_TAIL_MAX_ROWS;
+          for(int k = 1; k <= T; ++k)
+          {
+            total += rows_of(k);
+            ok = ok && rowlen[k] <= FEAT_BATCH_TAIL_MAX_ROW_LEN;
+          }
+          if(ok && total <= FEAT_BATCH_TAIL_MAX_TOTAL) break;
+        }
+        if(T < 1) return;
+        const int ni = int(_lev.size());
+        std::vector<std::uint64_t> loff(std::size_t(T) * ni, 0), pcg(ni);
+        for(int k = 1; k <= T; ++k)
+        {
+          const Level& L = *_lv[k];
+          for(std::size_t j = 0; j < L.inst.size(); ++j) loff[std::size_t(k - 1) * ni + L.inst[j]] = L.loff[j];
+        }
+        for(int i = 0; i < ni; ++i) pcg[i] = _off[i];
+        _tfin = DeviceArray<int>::from_host(_lev.data(), _lev.size());
+        _tloff = DeviceArray<std::uint64_t>::from_host(loff.data(), loff.size());
+        _tpcg = DeviceArray<std::uint64_t>::from_host(pcg.data(), pcg.size());
+        _tail.nlevels = T;
+        _tail.ninst = std::uint32_t(ni);
+        _tail.nu_pre = _cfg.nu_pre;
+        _tail.nu_post = _cfg.nu_post;
+        _tail.omega = _cfg.omega_rich;
+        _tail.finest = _tfin.get();
+        _tail.loff = _tloff.get();
+        _tail.pcg_off = _tpcg.get();
+        _tail.rhs_deep = _lv[T]->rhs.elements();
+        _tail.sol_deep = _lv[T]->sol.elements();
+        for(int k = 1; k <= T; ++k)
+        {
+          const Level& L = *_lv[k];
+          feat_batch_tail_level_t& d = _tail.lv[k - 1];
+          d.n = rows_of(k);
+          d.A = csr_desc(L.A);
+          if(k > 1)
+          {
+            d.P = csr_desc(L.P);
+            d.R = csr_desc(L.R);
+          }
+          d.inv = L.inv.elements();
+          d.mask = L.mask.get();
+        }
+      }
 
       static std::vector<std::uint32_t> unit_mask(Index n, const std::vector<std::uint64_t>& idx)
       {
@@ -171,6 +244,8 @@ namespace feat3amd
       const std::vector<std::uint64_t>& offsets() const { return _off; }
       Index size() const { return _off.back(); }
       const std::vector<double>& tuned_us() const { return _tune_us; }
+      /// levels 1..tail_depth() run in one launch per V-cycle (0: none)
+      int tail_depth() const { return _tail.nlevels; }
       std::vector<int> tunes() const
       {
         std::vector<int> t;
@@ -218,6 +293,7 @@ namespace feat3amd
         // multigrid levels 1.._lmax
         _lv.clear();
         _lv.resize(_lmax + 1);
+        std::vector<std::uint64_t> rowlen(_lmax + 1, 0);  // longest row of A_k, P_k, R_k
         for(int k = 1; k <= _lmax; ++k)
         {
           auto L = std::make_unique<Level>();
@@ -243,6 +319,7 @@ namespace feat3amd
             o += nk;
           }
           Ak.rows = Ak.cols = o;
+          rowlen[k] = max_row_len(Ak);
           L->A = Csr::from_host(o, o, Ak.val.data(), Ak.col.data(), Ak.rowptr.data());
           L->mask = DeviceArray<std::uint32_t>::from_host(unit_mask(o, filt).data(), (o + 31) / 32);
           L->top.build(clo, plo, len);
@@ -265,10 +342,12 @@ namespace feat3amd
             Pk.cols = C.A.rows();
             L->P = Csr::from_host(Pk.rows, Pk.cols, Pk.val.data(), Pk.col.data(), Pk.rowptr.data());
             const auto Rk = Assembly::transpose(Pk);
+            rowlen[k] = std::max(rowlen[k], std::max(max_row_len(Pk), max_row_len(Rk)));
             L->R = Csr::from_host(Rk.rows, Rk.cols, Rk.val.data(), Rk.col.data(), Rk.rowptr.data());
           }
           _lv[k] = std::move(L);
         }
+        plan_tail(rowlen);
         for(Vec* v : {&_r, &_p, &_t, &_z}) *v = Vec(size());
         for(DeviceArray<double>* a : {&_gamma, &_gold, &_tp, &_rr, &_dinit, &_dcur}) *a = DeviceArray<double>(ni);
         _hist = DeviceArray<double>(Index(ni) * (_cfg.max_iter + 1));
@@ -314,7 +393,8 @@ namespace feat3amd
       {
         feat_handle_t h = Runtime::handle();
         const feat_reduce_ws_t* ws = &_fw.ws;
-        for(int k = _lmax; k >= 2; --k)
+        const int T = _tail.nlevels;
+        for(int k = _lmax; k >= std::max(2, T + 1); --k)
         {
           Level& L = *_lv[k];
           if(L.top.n)
@@ -332,6 +412,9 @@ namespace feat3amd
                                       nullptr, ws),
                 "batch restriction");
         }
+        if(T > 0)
+          check(feat_batch_tail_f64(h, &_tail, r, z), "batch coarse tail");
+        else
         {
           Level& L = *_lv[1];
           if(L.top.n)
@@ -347,7 +430,7 @@ namespace feat3amd
                                         L.sol.elements()),
                   "batch extract");
         }
-        for(int k = 2; k <= _lmax; ++k)
+        for(int k = std::max(2, T + 1); k <= _lmax; ++k)
         {
           Level& L = *_lv[k];
           const feat_matrix_f64_t& P = L.P.exec();

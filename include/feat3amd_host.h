@@ -214,6 +214,9 @@ typedef struct {
   uint64_t max_iter;
   int autotune;   /* 1: autotune every batch matrix's launch shape at setup */
   int graphs;     /* 1: replay one recorded PCG iteration per step */
+  int tail_levels;  /* levels 1..tail_levels of every instance in one launch per V-cycle
+                       (feat_batch_tail_f64, bit-identical; lowered to what one workgroup
+                       holds); <= 0: every level as block-diagonal launches */
 } feat_batch_cfg_t;
 typedef struct feat_batchsolver_s* feat_batchsolver_t;
 int feat_batchsolver_create(feat_batchsolver_t* s, const feat_batch_cfg_t* cfg);
@@ -226,6 +229,8 @@ int feat_batchsolver_solve(feat_batchsolver_t s, double* x, const double* b, int
 /* chosen launch shapes and their SpMV times (us) of the batch matrices:
  * per level k = 1..lmax: A_k (and P_k, R_k for k > 1), then the PCG-layout matrix */
 int feat_batchsolver_tuning(feat_batchsolver_t s, int* tunes, double* us, int cap, int* count);
+/* levels run by the fused coarse tail (0: none) */
+int feat_batchsolver_tail_levels(feat_batchsolver_t s, int* levels);
 int feat_batchsolver_destroy(feat_batchsolver_t s);
 /* setup pieces of one instance, for tests */
 int feat_batch_levels(uint64_t seed, int ninst, int lmin, int lmax, int* out);

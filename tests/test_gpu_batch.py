@@ -1,7 +1,8 @@
 """The batch of uneven Q1 diffusion MG-PCG solves on the GPU against the oracle,
 instance by instance (same method, same stopping rule): equal iteration counts,
 solutions within 1e-10, and bit-identical results with and without the
-recorded-graph replay and with and without autotuned launch shapes."""
+recorded-graph replay, with and without autotuned launch shapes and with and
+without the fused coarse tail (levels 1..T of every instance in one launch)."""
 import numpy as np
 import pytest
 
@@ -73,14 +74,53 @@ def test_batch_graph_and_tuning_bitwise(feat, torch_gpu, small):
     assert len(tunes) > 0 and all(0 <= t < 32 for t in tunes)
 
 
+def _solve(torch, bs, b):
+    x = torch.zeros(bs.n, dtype=torch.float64, device="cuda")
+    r = bs.solve(x, b)
+    return host(x), r
+
+
+@pytest.mark.parametrize("ninst,lmin,lmax,tail,depth", [
+    (12, 3, 6, 1, 1),   # only level 1 in the tail: every instance enters it from level 2
+    (12, 3, 6, 3, 3),   # instances whose finest level is 3 run entirely inside the tail
+    (12, 3, 6, 8, 5),   # lowered to the deepest tail one workgroup holds (level 5: 1089 rows)
+    (6, 2, 4, 5, 4),    # the whole hierarchy of every instance in the tail
+])
+def test_batch_coarse_tail_bitwise(feat, torch_gpu, ninst, lmin, lmax, tail, depth):
+    torch = torch_gpu
+    ref = feat.BatchSolver(SEED, ninst=ninst, lmin=lmin, lmax=lmax, tail_levels=0)
+    bs = feat.BatchSolver(SEED, ninst=ninst, lmin=lmin, lmax=lmax, tail_levels=tail)
+    try:
+        assert ref.tail_levels == 0 and bs.tail_levels == depth
+        b = dev(torch, batch_rhs(feat, ref))
+        x0, r0 = _solve(torch, ref, b)
+        for rep in range(2):
+            x1, r1 = _solve(torch, bs, b)
+            assert np.array_equal(x1, x0), rep
+            assert np.array_equal(r1["iters"], r0["iters"])
+            for i, it in enumerate(r0["iters"]):  # entries past an instance's last iteration are not written
+                assert np.array_equal(r1["hist"][i, :it + 1], r0["hist"][i, :it + 1]), i
+        assert all(s == "success" for s in r0["status"])
+    finally:
+        bs.close()
+        ref.close()
+
+
 def test_batch_full_64(feat, torch_gpu):
-    # the full batch: 64 instances, levels 3..8, all converge to 1e-8
+    # the full batch: 64 instances, levels 3..8, all converge to 1e-8; the fused coarse
+    # tail (levels 1..5) gives the same bits as the block-diagonal launches
     torch = torch_gpu
     bs = feat.BatchSolver(SEED, ninst=64, lmin=3, lmax=8)
-    b = batch_rhs(feat, bs)
-    x = torch.zeros(bs.n, dtype=torch.float64, device="cuda")
-    r = bs.solve(x, dev(torch, b))
-    bs.close()
+    ref = feat.BatchSolver(SEED, ninst=64, lmin=3, lmax=8, tail_levels=0)
+    try:
+        assert bs.tail_levels == 5
+        b = dev(torch, batch_rhs(feat, bs))
+        xg, r = _solve(torch, bs, b)
+        x0, r0 = _solve(torch, ref, b)
+    finally:
+        bs.close()
+        ref.close()
     assert all(s == "success" for s in r["status"])
     assert np.all(r["def_final"] <= 1e-8 * r["def_init"])
     assert set(int(v) for v in bs.levels) >= {3, 8}
+    assert np.array_equal(xg, x0) and np.array_equal(r["iters"], r0["iters"])
