@@ -204,6 +204,8 @@ KERNEL_API = [
     ("feat_batch_status_f64", _i, [_vp, C.c_uint32, _vp, _vp, _vp, _i]),
     ("feat_batch_gather_f64", _i, [_vp, C.c_uint32, _vp, _vp, _vp, _vp, _vp]),
     ("feat_batch_tail_f64", _i, [_vp, _vp, _vp, _vp]),
+    ("feat_batch_pre_tiled_f64", _i, [_vp, _vp]),
+    ("feat_batch_post_tiled_f64", _i, [_vp, _vp]),
     ("feat_copy_int_to_host", _i, [_vp, _vp, _vp]),
     ("feat_autotune_matrix_f64", _i, [_vp, _vp, _vp, _vp, _i, _vp]),
     ("feat_mlp_to_bf16", _i, [_vp, _vp, _vp]),
@@ -255,6 +257,7 @@ HOST_API = [
     ("feat_batchsolver_solve", _i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     ("feat_batchsolver_tuning", _i, [_vp, _vp, _vp, _i, C.POINTER(_i)]),
     ("feat_batchsolver_tail_levels", _i, [_vp, C.POINTER(_i)]),
+    ("feat_batchsolver_tiled_levels", _i, [_vp, _vp, _i, C.POINTER(_i)]),
     ("feat_batchsolver_destroy", _i, [_vp]),
     ("feat_batch_levels", _i, [_u64, _i, _i, _i, _vp]),
     ("feat_lognormal_cells", _i, [_u64, _u64, _d, _vp]),
@@ -791,7 +794,8 @@ class Stokes:
 
 class BatchCfg(C.Structure):
     _fields_ = [("seed", _u64), ("ninst", _i), ("lmin", _i), ("lmax", _i), ("sigma", _d), ("tol_rel", _d),
-                ("max_iter", _u64), ("autotune", _i), ("graphs", _i), ("tail_levels", _i)]
+                ("max_iter", _u64), ("autotune", _i), ("graphs", _i), ("tail_levels", _i),
+                ("no_tiled", _i)]
 
 
 def batch_levels(seed, ninst, lmin, lmax):
@@ -870,12 +874,12 @@ class BatchSolver:
     """The batch of independent Q1 diffusion MG-PCG solves (C++ host layer)."""
 
     def __init__(self, seed=28054777172512, ninst=64, lmin=3, lmax=8, sigma=1.0, tol_rel=1e-8, max_iter=100,
-                 autotune=True, graphs=True, tail_levels=5):
+                 autotune=True, graphs=True, tail_levels=5, tiled=True):
         _, self.hl = libs()
         runtime_init()
         self.s = C.c_void_p()
         self.cfg = BatchCfg(seed, ninst, lmin, lmax, sigma, tol_rel, max_iter, int(autotune), int(graphs),
-                            int(tail_levels))
+                            int(tail_levels), int(not tiled))
         check(self.hl.feat_batchsolver_create(C.byref(self.s), C.byref(self.cfg)), "feat_batchsolver_create")
         self.levels = np.zeros(ninst, np.int32)
         self.off = np.zeros(ninst + 1, np.uint64)
@@ -903,6 +907,14 @@ class BatchSolver:
         d = C.c_int()
         check(self.hl.feat_batchsolver_tail_levels(self.s, C.byref(d)), "feat_batchsolver_tail_levels")
         return d.value
+
+    @property
+    def tiled_levels(self):
+        v = np.zeros(16, np.int32)
+        cnt = C.c_int()
+        check(self.hl.feat_batchsolver_tiled_levels(self.s, v.ctypes.data, 16, C.byref(cnt)),
+              "feat_batchsolver_tiled_levels")
+        return [int(x) for x in v[:cnt.value]]
 
     def tuning(self):
         t = np.zeros(64, np.int32)

@@ -196,44 +196,51 @@ __global__ void k_copy_int(const int* __restrict__ src, int* __restrict__ dst) {
 // min(finest, T) .. 1 with every vector in LDS.  A thread owns rows tid and tid + kBt of
 // a level (a level has at most FEAT_BATCH_TAIL_MAX_ROWS < 2 kBt rows) and holds their
 // entries of A in registers for a whole smoother call, so the sweeps between barriers
-// touch LDS only; P and R rows are read once per transfer.  Each row is formed with
-// the operations of the batched launches it stands for: k_smooth_first (first
-// iterate), EpiSmooth modes 0 / 1 / 2, EpiMasked (restriction), EpiProlAdd and
-// EpiDefect, with the row sum as the sequential fma over the row's entries in CSR
-// order (the order of every SELL-64 plan of those launches): bit-identical results.
+// touch LDS only.  The matrices are in ELL form (entry u of row j at [u*n + j]): a row's
+// entries load in one coalesced round trip with no row-pointer read in front of them, and
+// the loads of R (next to A's at the start of the pre-smoothing) and of P and A (on the
+// way up) are issued together.  Each row is formed with the operations of the batched
+// launches it stands for: k_smooth_first (first iterate), EpiSmooth modes 0 / 1 / 2,
+// EpiMasked (restriction), EpiProlAdd and EpiDefect, with the row sum as the sequential
+// fma over the row's entries in CSR order (the order of every SELL-64 plan of those
+// launches): bit-identical results.
 constexpr int kBt = 1024;
 constexpr int kBtRows = 2;  // rows per thread
-constexpr int kBtLen = FEAT_BATCH_TAIL_MAX_ROW_LEN;
 static_assert(FEAT_BATCH_TAIL_MAX_ROWS <= kBt * kBtRows, "rows per thread");
 
-struct BtRow
+template<int W>
+struct EllRow
 {
-  double v[kBtLen];
-  uint32_t c[kBtLen];  // column, local to the instance's block of the source vector
+  double v[W];
+  uint32_t c[W];  // column, local to the instance's block of the source vector
   int len;
 };
 
-// the row's entries (len = 0 past the level's rows)
-__device__ __forceinline__ void bt_load(BtRow& R, const feat_csr_f64_t& M, uint64_t g, uint64_t coff, bool on)
+// row j's entries (len = 0 when off); every entry below the level's width is loaded,
+// the row length only selects the terms of the sum
+template<int W>
+__device__ __forceinline__ void ell_load(EllRow<W>& R, const double* __restrict__ val,
+                                         const uint16_t* __restrict__ col, const uint8_t* __restrict__ len,
+                                         uint32_t n, uint32_t w, uint32_t j, bool on)
 {
   R.len = 0;
   if(!on) return;
-  const uint64_t b = M.rowptr[g];
-  R.len = int(M.rowptr[g + 1] - b);
+  R.len = len[j];
 #pragma unroll
-  for(int u = 0; u < kBtLen; ++u)
-    if(u < R.len)
+  for(int u = 0; u < W; ++u)
+    if(uint32_t(u) < w)
     {
-      R.v[u] = M.val[b + u];
-      R.c[u] = uint32_t(M.col[b + u] - coff);
+      R.v[u] = val[uint64_t(u) * n + j];
+      R.c[u] = col[uint64_t(u) * n + j];
     }
 }
 
-__device__ __forceinline__ double bt_dot(const BtRow& R, const double* x)
+template<int W>
+__device__ __forceinline__ double ell_dot(const EllRow<W>& R, const double* x)
 {
   double s = 0.0;
 #pragma unroll
-  for(int u = 0; u < kBtLen; ++u)
+  for(int u = 0; u < W; ++u)
     if(u < R.len) s = dfma(R.v[u], x[R.c[u]], s);
   return s;
 }
@@ -241,6 +248,7 @@ __device__ __forceinline__ double bt_dot(const BtRow& R, const double* x)
 __global__ __launch_bounds__(kBt) void k_batch_tail(feat_batch_tail_t Targ, const double* __restrict__ r,
                                                     double* __restrict__ z)
 {
+  constexpr int WA = FEAT_BATCH_TAIL_MAX_ROW_LEN, WP = FEAT_BATCH_TAIL_MAX_PROL_LEN, WR = FEAT_BATCH_TAIL_MAX_ROW_LEN;
   // the level descriptors are indexed by level at run time: one copy in LDS
   __shared__ feat_batch_tail_t T;
   __shared__ uint64_t s_loff[FEAT_BATCH_TAIL_MAX_LEVELS + 1];
@@ -263,7 +271,7 @@ __global__ __launch_bounds__(kBt) void k_batch_tail(feat_batch_tail_t Targ, cons
     {
       s_loff[k] = Targ.loff[uint64_t(k - 1) * Targ.ninst + inst];
       s_base[k] = o;
-      o += uint32_t(Targ.lv[k - 1].n);
+      o += Targ.lv[k - 1].n;
     }
   }
   __syncthreads();
@@ -281,30 +289,32 @@ __global__ __launch_bounds__(kBt) void k_batch_tail(feat_batch_tail_t Targ, cons
     }
   }
   {
-    const uint64_t n = T.lv[top - 1].n;
-    const uint32_t b = s_base[top];
+    const uint32_t n = T.lv[top - 1].n, b = s_base[top];
     for(uint32_t j = tid; j < n; j += kBt)
       s_rhs[b + j] = deep ? T.rhs_deep[s_loff[top] + j] : r[T.pcg_off[inst] + j];
   }
   __syncthreads();
 
   // A's rows of this thread on level k, in registers
-  BtRow A[kBtRows];
+  EllRow<WA> A[kBtRows];
   auto load_a = [&](int k) {
     const feat_batch_tail_level_t& L = T.lv[k - 1];
+    const double* val = L.a_val + s_loff[k] * L.wa;
 #pragma unroll
     for(int q = 0; q < kBtRows; ++q)
     {
       const uint32_t j = tid + q * kBt;
-      bt_load(A[q], L.A, s_loff[k] + j, s_loff[k], j < L.n);
+      ell_load(A[q], val, L.a_col, L.a_len, L.n, L.wa, j, j < L.n);
     }
   };
 
   // Richardson(Jacobi) pre-smoothing from x = 0 (feat_rj_smooth_f64 with x = sol, tmp and
   // d_out): iterates alternate between sol and s_b1 so that the last one lands in sol;
-  // the defect of the last iterate goes to s_d.
-  auto pre = [&](int k) {
-    const uint32_t n = uint32_t(T.lv[k - 1].n), b = s_base[k];
+  // the defect of the last iterate goes to s_d; then the rhs of level k-1 = filter(R s_d)
+  auto pre_restrict = [&](int k) {
+    const feat_batch_tail_level_t& L = T.lv[k - 1];
+    const uint32_t n = L.n, b = s_base[k];
+    const uint32_t nc = T.lv[k - 2].n, bc = s_base[k - 1];
     const double* rhs = s_rhs + b;
     const double* inv = s_inv + b;
     const uint8_t* m = s_m + b;
@@ -312,6 +322,8 @@ __global__ __launch_bounds__(kBt) void k_batch_tail(feat_batch_tail_t Targ, cons
     const int nu = T.nu_pre;
     auto buf = [&](int s) { return ((nu - s) & 1) ? s_b1 : x; };
     load_a(k);
+    EllRow<WR> R;  // level k-1 has at most kBt rows
+    ell_load(R, L.r_val, L.r_col, L.r_len, nc, L.wr, tid, uint32_t(tid) < nc);
     {
       double* x1 = buf(1);
 #pragma unroll
@@ -337,7 +349,7 @@ __global__ __launch_bounds__(kBt) void k_batch_tail(feat_batch_tail_t Targ, cons
         const uint32_t j = tid + q * kBt;
         if(j < n)
         {
-          const double sum = bt_dot(A[q], xk);
+          const double sum = ell_dot(A[q], xk);
           double d = dfma(sum, -1.0, rhs[j]);
           if(m[j]) d = 0.0;
           if(s == nu)
@@ -352,24 +364,11 @@ __global__ __launch_bounds__(kBt) void k_batch_tail(feat_batch_tail_t Targ, cons
       }
     }
     __syncthreads();
-  };
-
-  // rhs of level k-1 = filter(R s_d)
-  auto restrict_to = [&](int k) {
-    const feat_batch_tail_level_t& L = T.lv[k - 1];
-    const uint32_t nc = uint32_t(T.lv[k - 2].n), bc = s_base[k - 1];
-#pragma unroll
-    for(int q = 0; q < kBtRows; ++q)
+    if(uint32_t(tid) < nc)
     {
-      const uint32_t j = tid + q * kBt;
-      BtRow R;
-      bt_load(R, L.R, s_loff[k - 1] + j, s_loff[k], j < nc);
-      if(j < nc)
-      {
-        double v = dfma(bt_dot(R, s_d), 1.0, 0.0);
-        if(s_m[bc + j]) v = 0.0;
-        s_rhs[bc + j] = v;
-      }
+      double v = dfma(ell_dot(R, s_d), 1.0, 0.0);
+      if(s_m[bc + tid]) v = 0.0;
+      s_rhs[bc + tid] = v;
     }
     __syncthreads();
   };
@@ -377,24 +376,29 @@ __global__ __launch_bounds__(kBt) void k_batch_tail(feat_batch_tail_t Targ, cons
   // sol of level k += filter(P sol of level k-1), then s_d = filter(rhs - A sol)
   auto prol_defect = [&](int k) {
     const feat_batch_tail_level_t& L = T.lv[k - 1];
-    const uint32_t n = uint32_t(L.n), b = s_base[k];
+    const uint32_t n = L.n, b = s_base[k];
     const double* xc = s_sol + s_base[k - 1];
     double* x = s_sol + b;
     const uint8_t* m = s_m + b;
+    EllRow<WP> P[kBtRows];
 #pragma unroll
     for(int q = 0; q < kBtRows; ++q)
     {
       const uint32_t j = tid + q * kBt;
-      BtRow P;
-      bt_load(P, L.P, s_loff[k] + j, s_loff[k - 1], j < n);
+      ell_load(P[q], L.p_val, L.p_col, L.p_len, n, L.wp, j, j < n);
+    }
+    load_a(k);
+#pragma unroll
+    for(int q = 0; q < kBtRows; ++q)
+    {
+      const uint32_t j = tid + q * kBt;
       if(j < n)
       {
-        double c = dfma(bt_dot(P, xc), 1.0, 0.0);
+        double c = dfma(ell_dot(P[q], xc), 1.0, 0.0);
         if(m[j]) c = 0.0;
         x[j] = dfma(1.0, c, x[j]);
       }
     }
-    load_a(k);
     __syncthreads();
 #pragma unroll
     for(int q = 0; q < kBtRows; ++q)
@@ -402,7 +406,7 @@ __global__ __launch_bounds__(kBt) void k_batch_tail(feat_batch_tail_t Targ, cons
       const uint32_t j = tid + q * kBt;
       if(j < n)
       {
-        double d = dfma(bt_dot(A[q], x), -1.0, s_rhs[b + j]);
+        double d = dfma(ell_dot(A[q], x), -1.0, s_rhs[b + j]);
         if(m[j]) d = 0.0;
         s_d[j] = d;
       }
@@ -413,7 +417,7 @@ __global__ __launch_bounds__(kBt) void k_batch_tail(feat_batch_tail_t Targ, cons
   // Richardson(Jacobi) post-smoothing on s_d from 0, its last iterate added to sol
   // (feat_rj_smooth_f64 with x = cor, tmp and acc = sol); A's rows are still loaded
   auto post = [&](int k) {
-    const uint32_t n = uint32_t(T.lv[k - 1].n), b = s_base[k];
+    const uint32_t n = T.lv[k - 1].n, b = s_base[k];
     const double* inv = s_inv + b;
     const uint8_t* m = s_m + b;
     double* x = s_sol + b;
@@ -448,7 +452,7 @@ __global__ __launch_bounds__(kBt) void k_batch_tail(feat_batch_tail_t Targ, cons
         const uint32_t j = tid + q * kBt;
         if(j < n)
         {
-          double d = dfma(bt_dot(A[q], xk), -1.0, s_d[j]);
+          double d = dfma(ell_dot(A[q], xk), -1.0, s_d[j]);
           if(m[j]) d = 0.0;
           double c = inv[j] * d;
           if(m[j]) c = 0.0;
@@ -463,14 +467,10 @@ __global__ __launch_bounds__(kBt) void k_batch_tail(feat_batch_tail_t Targ, cons
     __syncthreads();
   };
 
-  for(int k = top; k >= 2; --k)
-  {
-    pre(k);
-    restrict_to(k);
-  }
+  for(int k = top; k >= 2; --k) pre_restrict(k);
   {
     // level 1: one Jacobi step with omega 1 from x = 0
-    const uint32_t n = uint32_t(T.lv[0].n), b = s_base[1];
+    const uint32_t n = T.lv[0].n, b = s_base[1];
     for(uint32_t j = tid; j < n; j += kBt)
     {
       double c = s_inv[b + j] * s_rhs[b + j];
@@ -485,8 +485,7 @@ __global__ __launch_bounds__(kBt) void k_batch_tail(feat_batch_tail_t Targ, cons
     post(k);
   }
   {
-    const uint64_t n = T.lv[top - 1].n;
-    const uint32_t b = s_base[top];
+    const uint32_t n = T.lv[top - 1].n, b = s_base[top];
     for(uint32_t j = tid; j < n; j += kBt)
     {
       if(deep)
@@ -495,6 +494,224 @@ __global__ __launch_bounds__(kBt) void k_batch_tail(feat_batch_tail_t Targ, cons
         z[T.pcg_off[inst] + j] = s_sol[b + j];
     }
   }
+}
+
+// ---------------------------------------------------------------------------
+// Tiled smoother calls of a batched level on structured grids.  A workgroup owns a
+// kTlTile x kTlTile tile of one instance's n1 x n1 grid and works on the tile plus a halo of
+// kTlHalo nodes (clipped to the grid), kTlPts region nodes per thread with their nine
+// stencil values, right-hand side, inverse diagonal and filter flag in registers and the
+// iterates in LDS.  A sweep over the region is exact at the nodes at least s nodes away
+// from a cut (non-grid) edge of the region after s stencil applications, so with at most
+// kTlHalo applications the tile's nodes are exact; nodes nearer a cut edge are computed
+// with a zero row sum and never reach the tile.  Filtered rows never use their row sum
+// (every kernel they stand for sets their defect / correction to zero), and a free row's
+// sum is the sequential fma over its nine neighbours in CSR (lexicographic) order, so the
+// results are bit-identical to the SELL-64 launches of the same calls.
+constexpr int kTl = 512;
+constexpr int kTlTile = 32;
+constexpr int kTlHalo = FEAT_BATCH_TILED_MAX_STEPS;
+constexpr int kTlReg = kTlTile + 2 * kTlHalo;
+constexpr int kTlPts = (kTlReg * kTlReg + kTl - 1) / kTl;
+
+template<bool POST>
+__global__ __launch_bounds__(kTl) void k_batch_tiled(feat_batch_tiled_t t)
+{
+  __shared__ double s_b[POST ? 3 : 2][kTlReg * kTlReg];
+  const int n1 = int(t.n1);
+  const int tiles = (n1 + kTlTile - 1) / kTlTile;
+  const uint32_t ntl = uint32_t(tiles) * uint32_t(tiles);
+  const uint32_t p = blockIdx.x / ntl, tl = blockIdx.x % ntl;
+  const int tx0 = int(tl % uint32_t(tiles)) * kTlTile, ty0 = int(tl / uint32_t(tiles)) * kTlTile;
+  const int rx0 = max(tx0 - kTlHalo, 0), ry0 = max(ty0 - kTlHalo, 0);
+  const int rw = min(tx0 + kTlTile + kTlHalo, n1) - rx0, rh = min(ty0 + kTlTile + kTlHalo, n1) - ry0;
+  const uint64_t nk = uint64_t(n1) * uint64_t(n1), base = uint64_t(p) * nk;
+  const int tid = threadIdx.x;
+  const double omega = t.omega;
+  const int nu = t.nu;
+
+  double a[kTlPts][9], rv[kTlPts], iv[kTlPts], dq[kTlPts], xs[kTlPts];
+  int li[kTlPts];
+  uint32_t row[kTlPts];
+  bool on[kTlPts], m[kTlPts], in[kTlPts], out[kTlPts];
+#pragma unroll
+  for(int q = 0; q < kTlPts; ++q)
+  {
+    const int idx = tid + q * kTl;
+    on[q] = idx < rw * rh;
+    const int lx = on[q] ? idx % rw : 0, ly = on[q] ? idx / rw : 0;
+    const int gx = rx0 + lx, gy = ry0 + ly;
+    li[q] = idx;
+    row[q] = uint32_t(gy) * uint32_t(n1) + uint32_t(gx);
+    in[q] = lx > 0 && lx < rw - 1 && ly > 0 && ly < rh - 1;
+    out[q] = on[q] && gx >= tx0 && gx < tx0 + kTlTile && gy >= ty0 && gy < ty0 + kTlTile;
+    m[q] = true;
+    rv[q] = iv[q] = dq[q] = xs[q] = 0.0;
+    if(on[q])
+    {
+      const uint64_t g = base + row[q];
+      m[q] = is_masked(t.mask, g);
+      rv[q] = t.rhs[g];
+      iv[q] = t.inv[g];
+      if(POST) xs[q] = t.sol[g];
+    }
+#pragma unroll
+    for(int u = 0; u < 9; ++u) a[q][u] = (on[q] && !m[q] && in[q]) ? t.sval[(uint64_t(p) * 9 + u) * nk + row[q]] : 0.0;
+  }
+  auto stencil = [&](int q, const double* x) -> double {
+    double s = 0.0;
+    if(in[q] && !m[q])
+    {
+      const int c = li[q];
+      s = dfma(a[q][0], x[c - rw - 1], s);
+      s = dfma(a[q][1], x[c - rw], s);
+      s = dfma(a[q][2], x[c - rw + 1], s);
+      s = dfma(a[q][3], x[c - 1], s);
+      s = dfma(a[q][4], x[c], s);
+      s = dfma(a[q][5], x[c + 1], s);
+      s = dfma(a[q][6], x[c + rw - 1], s);
+      s = dfma(a[q][7], x[c + rw], s);
+      s = dfma(a[q][8], x[c + rw + 1], s);
+    }
+    return s;
+  };
+  auto buf = [&](int s) { return ((nu - s) & 1) ? s_b[1] : s_b[0]; };
+
+  if constexpr(!POST)
+  {
+    // first iterate from x = 0 (k_smooth_first)
+    double* x1 = buf(1);
+#pragma unroll
+    for(int q = 0; q < kTlPts; ++q)
+      if(on[q])
+      {
+        double c = iv[q] * rv[q];
+        if(m[q]) c = 0.0;
+        x1[li[q]] = dfma(omega, c, 0.0);
+      }
+    for(int s = 1; s <= nu; ++s)
+    {
+      __syncthreads();
+      const double* xk = buf(s);
+      double* xo = buf(s + 1);
+#pragma unroll
+      for(int q = 0; q < kTlPts; ++q)
+        if(on[q])
+        {
+          double d = dfma(stencil(q, xk), -1.0, rv[q]);
+          if(m[q]) d = 0.0;
+          if(s == nu)
+            dq[q] = d;
+          else
+          {
+            double c = iv[q] * d;
+            if(m[q]) c = 0.0;
+            xo[li[q]] = dfma(omega, c, xk[li[q]]);
+          }
+        }
+    }
+    const double* xl = buf(nu);
+#pragma unroll
+    for(int q = 0; q < kTlPts; ++q)
+      if(out[q])
+      {
+        t.sol[base + row[q]] = xl[li[q]];
+        t.def[base + row[q]] = dq[q];
+      }
+  }
+  else
+  {
+    // sol += filter(P xc)
+    const double* xc = t.xc + t.coff[p];
+#pragma unroll
+    for(int q = 0; q < kTlPts; ++q)
+      if(on[q])
+      {
+        const uint32_t j = row[q];
+        const int len = t.p_len[j];
+        double pv[FEAT_BATCH_TAIL_MAX_PROL_LEN];
+        uint32_t pc[FEAT_BATCH_TAIL_MAX_PROL_LEN];
+#pragma unroll
+        for(int u = 0; u < FEAT_BATCH_TAIL_MAX_PROL_LEN; ++u)
+          if(uint32_t(u) < t.wp)
+          {
+            pv[u] = t.p_val[uint64_t(u) * nk + j];
+            pc[u] = t.p_col[uint64_t(u) * nk + j];
+          }
+        double sum = 0.0;
+#pragma unroll
+        for(int u = 0; u < FEAT_BATCH_TAIL_MAX_PROL_LEN; ++u)
+          if(u < len) sum = dfma(pv[u], xc[pc[u]], sum);
+        double c = dfma(sum, 1.0, 0.0);
+        if(m[q]) c = 0.0;
+        xs[q] = dfma(1.0, c, xs[q]);
+        s_b[2][li[q]] = xs[q];
+      }
+    __syncthreads();
+    // defect, then the first iterate on it
+    double* y1 = buf(1);
+#pragma unroll
+    for(int q = 0; q < kTlPts; ++q)
+      if(on[q])
+      {
+        double d = dfma(stencil(q, s_b[2]), -1.0, rv[q]);
+        if(m[q]) d = 0.0;
+        dq[q] = d;
+        double c = iv[q] * d;
+        if(m[q]) c = 0.0;
+        const double xn = dfma(omega, c, 0.0);
+        if(nu == 1)
+          xs[q] = dfma(1.0, xn, xs[q]);
+        else
+          y1[li[q]] = xn;
+      }
+    for(int s = 1; s <= nu - 1; ++s)
+    {
+      __syncthreads();
+      const double* yk = buf(s);
+      double* yo = buf(s + 1);
+#pragma unroll
+      for(int q = 0; q < kTlPts; ++q)
+        if(on[q])
+        {
+          double d = dfma(stencil(q, yk), -1.0, dq[q]);
+          if(m[q]) d = 0.0;
+          double c = iv[q] * d;
+          if(m[q]) c = 0.0;
+          const double xn = dfma(omega, c, yk[li[q]]);
+          if(s == nu - 1)
+            xs[q] = dfma(1.0, xn, xs[q]);
+          else
+            yo[li[q]] = xn;
+        }
+    }
+#pragma unroll
+    for(int q = 0; q < kTlPts; ++q)
+      if(out[q]) t.sol[base + row[q]] = xs[q];
+  }
+}
+
+int tiled_launch(feat_handle_t h, const feat_batch_tiled_t* t, bool post, const char* name)
+{
+  if(!h || !t) return fail(FEAT_ERR_INVALID, name);
+  if(t->n1 < 3 || t->n1 > 65535 || t->nu < 1 || t->nu > FEAT_BATCH_TILED_MAX_STEPS)
+    return fail(FEAT_ERR_INVALID, "batch tiled smoother: grid size or steps out of range");
+  if(!t->ninst) return FEAT_OK;
+  if(!t->sval || !t->inv || !t->mask || !t->rhs || !t->sol || (!post && !t->def))
+    return fail(FEAT_ERR_INVALID, "batch tiled smoother: null array");
+  if(post && (!t->xc || !t->coff || !t->p_val || !t->p_col || !t->p_len || t->wp > FEAT_BATCH_TAIL_MAX_PROL_LEN))
+    return fail(FEAT_ERR_INVALID, "batch tiled smoother: prolongation incomplete");
+  if(static_cast<const void*>(t->rhs) == t->sol || (!post && (t->def == t->sol || static_cast<const void*>(t->rhs) == t->def)) ||
+     (post && static_cast<const void*>(t->xc) == t->sol))
+    return fail(FEAT_ERR_ALIAS, "batch tiled smoother: vectors alias");
+  const uint64_t tiles = (t->n1 + kTlTile - 1) / kTlTile;
+  const uint64_t grid = tiles * tiles * t->ninst;
+  if(grid > 0x7fffffffull) return fail(FEAT_ERR_INVALID, "batch tiled smoother: grid too large");
+  if(post)
+    hipLaunchKernelGGL(k_batch_tiled<true>, dim3(uint32_t(grid)), dim3(kTl), 0, h->stream, *t);
+  else
+    hipLaunchKernelGGL(k_batch_tiled<false>, dim3(uint32_t(grid)), dim3(kTl), 0, h->stream, *t);
+  return check_launch(name);
 }
 
 bool batch_ok(const feat_batch_t* b)
@@ -586,11 +803,15 @@ int feat_batch_tail_f64(feat_handle_t h, const feat_batch_tail_t* t, const doubl
   {
     const feat_batch_tail_level_t& L = t->lv[k - 1];
     const bool transfer = k > 1;
-    if(L.n < 1 || !L.inv || !L.mask || !L.A.val || !L.A.col || !L.A.rowptr ||
-       (transfer && (!L.P.val || !L.P.col || !L.P.rowptr || !L.R.val || !L.R.col || !L.R.rowptr)))
+    if(L.n < 1 || !L.inv || !L.mask || !L.a_val || !L.a_col || !L.a_len ||
+       (transfer && (!L.p_val || !L.p_col || !L.p_len || !L.r_val || !L.r_col || !L.r_len)))
       return fail(FEAT_ERR_INVALID, "feat_batch_tail_f64: level descriptor incomplete");
+    if(L.wa > FEAT_BATCH_TAIL_MAX_ROW_LEN ||
+       (transfer && (L.wr > FEAT_BATCH_TAIL_MAX_ROW_LEN || L.wp > FEAT_BATCH_TAIL_MAX_PROL_LEN)))
+      return fail(FEAT_ERR_INVALID, "feat_batch_tail_f64: ELL width above the limit");
     total += L.n;
-    if(L.n > FEAT_BATCH_TAIL_MAX_ROWS) total = FEAT_BATCH_TAIL_MAX_TOTAL + 1;
+    // the restriction to a level gives one row to each thread
+    if(L.n > FEAT_BATCH_TAIL_MAX_ROWS || (k < nl && L.n > uint64_t(kBt))) total = FEAT_BATCH_TAIL_MAX_TOTAL + 1;
   }
   if(total > FEAT_BATCH_TAIL_MAX_TOTAL)
     return fail(FEAT_ERR_INVALID, "feat_batch_tail_f64: levels exceed the workgroup's LDS vectors");
@@ -600,6 +821,16 @@ int feat_batch_tail_f64(feat_handle_t h, const feat_batch_tail_t* t, const doubl
     return fail(FEAT_ERR_ALIAS, "feat_batch_tail_f64: input and output alias");
   hipLaunchKernelGGL(k_batch_tail, dim3(t->ninst), dim3(kBt), 0, h->stream, *t, r, z);
   return check_launch(name);
+}
+
+int feat_batch_pre_tiled_f64(feat_handle_t h, const feat_batch_tiled_t* t)
+{
+  return tiled_launch(h, t, false, "feat_batch_pre_tiled_f64");
+}
+
+int feat_batch_post_tiled_f64(feat_handle_t h, const feat_batch_tiled_t* t)
+{
+  return tiled_launch(h, t, true, "feat_batch_post_tiled_f64");
 }
 
 int feat_copy_int_to_host(feat_handle_t h, const int* dev, int* host_pinned)

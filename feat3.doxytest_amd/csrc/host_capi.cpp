@@ -791,6 +791,7 @@ int feat_batchsolver_create(feat_batchsolver_t* out, const feat_batch_cfg_t* cfg
     c.autotune = cfg->autotune != 0;
     c.graphs = cfg->graphs != 0;
     c.tail_levels = cfg->tail_levels > 0 ? cfg->tail_levels : 0;
+    c.tiled = cfg->no_tiled == 0;
     auto s = std::make_unique<feat_batchsolver_s>();
     s->b = std::make_unique<Solver::BatchedMgPcg>(c);
     s->b->init();
@@ -829,6 +830,16 @@ int feat_batchsolver_tail_levels(feat_batchsolver_t s, int* levels)
   return guarded([&]() {
     if(!s || !levels) throw Error("feat_batchsolver_tail_levels: null argument");
     *levels = s->b->tail_depth();
+  });
+}
+
+int feat_batchsolver_tiled_levels(feat_batchsolver_t s, int* levels, int cap, int* count)
+{
+  return guarded([&]() {
+    if(!s || !count || (cap > 0 && !levels)) throw Error("feat_batchsolver_tiled_levels: null argument");
+    const auto v = s->b->tiled_levels();
+    *count = int(v.size());
+    for(int i = 0; i < cap && i < int(v.size()); ++i) levels[i] = v[i];
   });
 }
 

@@ -621,27 +621,43 @@ int feat_copy_int_to_host(feat_handle_t h, const int* dev, int* host_pinned);
  * defect of the last iterate, restriction, one damped Jacobi step (omega 1) on level 1,
  * prolongation + defect + post-smoothing accumulated into the level's solution), one
  * workgroup per instance with the instance's vectors of every tail level in LDS.
- * Level k (index k-1 of lv) is block-diagonal over the instances whose finest level is
- * >= k; instance i owns rows [loff[(k-1)*ninst + i], + n) of A, inv, mask and of P's rows
- * (P's columns: its rows of level k-1), and R = P^T has the transposed blocks.  An
- * instance whose finest level f is <= T enters at level f with r[pcg_off[i] + j] and
- * leaves z[pcg_off[i] + j]; one with f > T takes level T's right-hand side from
- * rhs_deep and leaves level T's solution in sol_deep (both in level T's layout).
- * Every row is formed with the operations of feat_rj_smooth_f64, feat_apply_masked_f64,
- * feat_prol_add_f64 and feat_defect_f64 (row sums in CSR order), so the results are
- * bit-identical to those calls.  Limits: n of every level <= FEAT_BATCH_TAIL_MAX_ROWS, the
- * sum of n over the levels <= FEAT_BATCH_TAIL_MAX_TOTAL, at most FEAT_BATCH_TAIL_MAX_ROW_LEN
- * entries in any row of A, P and R.  No single reference function: the coarse end of
+ * Level k (index k-1 of lv) holds, back to back, the instances whose finest level is
+ * >= k, n rows each; instance i's rows start at loff[(k-1)*ninst + i] (a multiple of n).
+ * Its matrices in ELL form, entry u of local row j at [u*n + j] (R: [u*n_{k-1} + j]):
+ *   A  values per instance at a_val + loff*wa, the column pattern shared;
+ *   P  (rows of level k, columns of level k-1) and R = P^T: values and pattern shared
+ *      by every instance (the geometric transfer of one mesh);
+ * columns local to the instance's block, row lengths in *_len, entries past a row's
+ * length unused (any column, any value).  An instance whose finest level f is <= T
+ * enters at level f with r[pcg_off[i] + j] and leaves z[pcg_off[i] + j]; one with f > T
+ * takes level T's right-hand side from rhs_deep and leaves level T's solution in
+ * sol_deep (both in level T's layout).  Every row is formed with the operations of
+ * feat_rj_smooth_f64, feat_apply_masked_f64, feat_prol_add_f64 and feat_defect_f64 (row
+ * sums as the sequential fma over the row's entries in CSR order), so the results are
+ * bit-identical to those calls on the same matrices in CSR form.  Limits: n of every
+ * level <= FEAT_BATCH_TAIL_MAX_ROWS (below level T: <= 1024), the sum of n over the levels <=
+ * FEAT_BATCH_TAIL_MAX_TOTAL, widths wa, wr <= FEAT_BATCH_TAIL_MAX_ROW_LEN, wp <=
+ * FEAT_BATCH_TAIL_MAX_PROL_LEN.  No single reference function: the coarse end of
  * MultiGrid::apply for every instance at once. */
 #define FEAT_BATCH_TAIL_MAX_LEVELS 8
 #define FEAT_BATCH_TAIL_MAX_ROWS 1100
 #define FEAT_BATCH_TAIL_MAX_TOTAL 1536
 #define FEAT_BATCH_TAIL_MAX_ROW_LEN 9
+#define FEAT_BATCH_TAIL_MAX_PROL_LEN 4
 typedef struct {
-  uint64_t n;                  /* rows of one instance on this level */
-  feat_csr_f64_t A, P, R;      /* P, R: to and from level k-1 (unused on level 1) */
-  const double* inv;           /* Jacobi: omega_jac / diag (level 1: 1 / diag) */
-  const uint32_t* mask;        /* Dirichlet rows (UnitFilter), bit per row */
+  uint32_t n;                  /* rows of one instance on this level */
+  uint32_t wa, wp, wr;         /* ELL widths of A, P, R (P, R unused on level 1) */
+  const double* a_val;
+  const uint16_t* a_col;
+  const uint8_t* a_len;
+  const double* p_val;
+  const uint16_t* p_col;
+  const uint8_t* p_len;
+  const double* r_val;
+  const uint16_t* r_col;
+  const uint8_t* r_len;
+  const double* inv;           /* Jacobi: omega_jac / diag (level 1: 1 / diag), level layout */
+  const uint32_t* mask;        /* Dirichlet rows (UnitFilter), bit per row of the level layout */
 } feat_batch_tail_level_t;
 typedef struct {
   int nlevels;                 /* T */
@@ -656,6 +672,43 @@ typedef struct {
   feat_batch_tail_level_t lv[FEAT_BATCH_TAIL_MAX_LEVELS];
 } feat_batch_tail_t;
 int feat_batch_tail_f64(feat_handle_t h, const feat_batch_tail_t* t, const double* r, double* z);
+
+/* The smoother calls of one batched level on structured grids, tiled: instance p's rows
+ * are p*n1*n1 .. of the level layout, lexicographic on an n1 x n1 grid; every row that the
+ * unit filter (mask) leaves free is an interior node whose matrix row holds exactly the
+ * nine neighbours (dy, dx) in {-1,0,1}^2 in CSR order, with values at
+ * sval[(p*9 + u)*n1*n1 + row] (u = 3*(dy+1) + (dx+1); filtered rows: any values, unused).
+ * One workgroup per 32x32 tile of one instance runs all sweeps of a smoother call in LDS
+ * on the tile and a halo of FEAT_BATCH_TILED_MAX_STEPS nodes.
+ *   pre:  feat_rj_smooth_f64(nu, omega, rhs, x = sol, d_out = def) from x = 0;
+ *   post: feat_prol_add_f64(P, sol, xc), feat_defect_f64(rhs, sol) and
+ *         feat_rj_smooth_f64(nu, omega, defect, acc = sol) from 0, the prolongation from
+ *         level k-1 in ELL form shared by the instances (entry u of row j at [u*n1*n1 + j],
+ *         columns local to the coarse block, which starts at coff[p] of xc's layout).
+ * Each row is formed with the operations of those calls (row sums in CSR order), so the
+ * results are bit-identical to them.  1 <= nu <= FEAT_BATCH_TILED_MAX_STEPS, wp <=
+ * FEAT_BATCH_TAIL_MAX_PROL_LEN.  No single reference function: the smoother calls of
+ * MultiGrid::apply (multigrid.hpp:1790-1930) for every instance of a level at once. */
+#define FEAT_BATCH_TILED_MAX_STEPS 4
+typedef struct {
+  uint32_t n1, ninst;
+  int nu;
+  double omega;
+  const double* sval;
+  const double* inv;           /* omega_jac / diag */
+  const uint32_t* mask;
+  const double* rhs;
+  double* sol;
+  double* def;                 /* pre: the defect of the last iterate (out) */
+  const double* xc;            /* post: level k-1 solution */
+  const uint64_t* coff;        /* post: per instance */
+  uint32_t wp;
+  const double* p_val;
+  const uint16_t* p_col;
+  const uint8_t* p_len;
+} feat_batch_tiled_t;
+int feat_batch_pre_tiled_f64(feat_handle_t h, const feat_batch_tiled_t* t);
+int feat_batch_post_tiled_f64(feat_handle_t h, const feat_batch_tiled_t* t);
 
 /* ---------------- structured Q1 hierarchy on the device ----------------
  * The Q1 Laplace problem on level `level` of the unit square (dim 2) / cube (dim 3),

@@ -27,6 +27,7 @@
 #include <feat3amd/solver.hpp>
 
 #include <algorithm>
+#include <cstring>
 #include <memory>
 #include <vector>
 
@@ -46,6 +47,9 @@ namespace feat3amd
       /// levels 1..tail_levels of every instance run in one launch (feat_batch_tail_f64,
       /// one workgroup per instance, bit-identical); lowered to what the workgroup holds; 0: off
       int tail_levels = 5;
+      /// the smoother calls of the levels above the tail as tiled launches on the structured
+      /// grids (feat_batch_pre_tiled_f64 / feat_batch_post_tiled_f64, bit-identical)
+      bool tiled = true;
     };
 
     class BatchedMgPcg
@@ -136,39 +140,148 @@ namespace feat3amd
       const void* _key_x = nullptr;
       std::vector<double> _tune_us;
       // coarse tail: levels 1.._tail.nlevels in one launch (0 levels: off)
+      struct Ell
+      {
+        std::uint32_t n = 0, w = 0;
+        std::vector<double> val;
+        std::vector<std::uint16_t> col;
+        std::vector<std::uint8_t> len;
+        bool ok = true;
+        DeviceArray<double> dval;
+        DeviceArray<std::uint16_t> dcol;
+        DeviceArray<std::uint8_t> dlen;
+        void upload()
+        {
+          dval = DeviceArray<double>::from_host(val.data(), val.size());
+          dcol = DeviceArray<std::uint16_t>::from_host(col.data(), col.size());
+          dlen = DeviceArray<std::uint8_t>::from_host(len.data(), len.size());
+        }
+      };
+      struct TailLevel
+      {
+        Ell A, P, R;
+      };
       feat_batch_tail_t _tail{};
+      std::vector<TailLevel> _tl;  // index k (1..T); 0 unused
       DeviceArray<int> _tfin;
       DeviceArray<std::uint64_t> _tloff, _tpcg;
-
-      static std::uint64_t max_row_len(const Assembly::CsrHost& M)
+      // tiled smoother calls of the levels above the tail (index k; empty sval: not tiled)
+      struct TiledLevel
       {
-        std::uint64_t m = 0;
-        for(std::size_t r = 0; r + 1 < M.rowptr.size(); ++r) m = std::max<std::uint64_t>(m, M.rowptr[r + 1] - M.rowptr[r]);
-        return m;
-      }
-      static feat_csr_f64_t csr_desc(const Csr& M)
+        std::uint32_t n1 = 0;
+        DeviceArray<double> sval;
+        Ell P;
+        DeviceArray<std::uint64_t> coff;
+      };
+      std::vector<TiledLevel> _tiled;
+
+      // the nine stencil values of every free row of the diagonal blocks of A (block b: rows
+      // b*n1*n1 ..), checked to be the full 9-point neighbourhood in CSR order; empty when not
+      static std::vector<double> stencil_values(const Assembly::CsrHost& A, std::size_t nb, std::uint64_t n1,
+                                                const std::vector<std::uint64_t>& bnd)
       {
-        return feat_csr_f64_t{M.rows(), M.columns(), M.used_elements(), M.val(), M.col_ind(), M.row_ptr()};
+        const std::uint64_t nk = n1 * n1;
+        std::vector<char> filt(nk, 0);
+        for(auto b : bnd) filt[b] = 1;
+        std::vector<double> v(nb * 9 * nk, 0.0);
+        const std::int64_t w = std::int64_t(n1);
+        const std::int64_t offs[9] = {-w - 1, -w, -w + 1, -1, 0, 1, w - 1, w, w + 1};
+        for(std::size_t b = 0; b < nb; ++b)
+          for(std::uint64_t r = 0; r < nk; ++r)
+          {
+            if(filt[r]) continue;
+            const std::uint64_t g = b * nk + r, e0 = A.rowptr[g];
+            const std::int64_t x = std::int64_t(r % n1), y = std::int64_t(r / n1);
+            if(A.rowptr[g + 1] - e0 != 9 || x == 0 || y == 0 || x == w - 1 || y == w - 1) return {};
+            for(int u = 0; u < 9; ++u)
+            {
+              if(std::int64_t(A.col[e0 + u]) != std::int64_t(b * nk + r) + offs[u]) return {};
+              v[(b * 9 + u) * nk + r] = A.val[e0 + u];
+            }
+          }
+        return v;
       }
 
-      // the deepest T <= tail_levels whose levels fit one workgroup (rows, LDS vectors, row lengths)
-      void plan_tail(const std::vector<std::uint64_t>& rowlen)
+      // ELL form of the diagonal blocks of M: block b has rows roff[b] .. roff[b] + n - 1 and
+      // columns from coff[b], the same pattern in every block; values per block (in block
+      // order) or, when shared, one set equal (bitwise) in every block.  ok = false when the
+      // blocks differ where they must agree or a block does not fit the limits.
+      static Ell to_ell(const Assembly::CsrHost& M, const std::vector<std::uint64_t>& roff,
+                        const std::vector<std::uint64_t>& coff, std::uint64_t n, std::uint64_t ncols, bool shared)
+      {
+        Ell E;
+        E.n = std::uint32_t(n);
+        const std::size_t nb = roff.size();
+        for(std::size_t b = 0; b < nb; ++b)
+          for(std::uint64_t j = 0; j < n; ++j)
+            E.w = std::max<std::uint32_t>(E.w, std::uint32_t(M.rowptr[roff[b] + j + 1] - M.rowptr[roff[b] + j]));
+        if(ncols > 65536 || E.w > 255)  // 16-bit columns, 8-bit row lengths
+        {
+          E.ok = false;
+          return E;
+        }
+        E.col.assign(std::size_t(E.w) * n, 0);
+        E.len.assign(n, 0);
+        E.val.assign((shared ? 1 : nb) * std::size_t(E.w) * n, 0.0);
+        for(std::size_t b = 0; b < nb && E.ok; ++b)
+        {
+          for(std::uint64_t j = 0; j < n; ++j)
+          {
+            const std::uint64_t g = roff[b] + j, e0 = M.rowptr[g];
+            const std::uint64_t l = M.rowptr[g + 1] - e0;
+            if(b == 0)
+              E.len[j] = std::uint8_t(l);
+            else if(E.len[j] != l)
+              E.ok = false;
+            for(std::uint64_t u = 0; u < l && E.ok; ++u)
+            {
+              const std::uint64_t c = M.col[e0 + u] - coff[b];
+              if(M.col[e0 + u] < coff[b] || c >= ncols) E.ok = false;
+              const std::size_t at = std::size_t(u) * n + j;
+              if(b == 0)
+                E.col[at] = std::uint16_t(c);
+              else if(E.col[at] != c)
+                E.ok = false;
+              const double v = M.val[e0 + u];
+              if(!shared)
+                E.val[(b * E.w) * n + at] = v;
+              else if(b == 0)
+                E.val[at] = v;
+              else if(std::memcmp(&E.val[at], &v, sizeof v) != 0)
+                E.ok = false;
+            }
+          }
+        }
+        return E;
+      }
+
+      // the deepest T <= tail_levels whose levels fit one workgroup (rows, LDS vectors, ELL widths)
+      void plan_tail()
       {
         _tail = feat_batch_tail_t{};
-        int T = std::min(_cfg.tail_levels, std::min(_lmax, FEAT_BATCH_TAIL_MAX_LEVELS));
-        auto rows_of = [&](int k) { return _lv[k]->A.rows() / _lv[k]->inst.size(); };
+        int T = std::min<int>(int(_tl.size()) - 1, std::min(_lmax, FEAT_BATCH_TAIL_MAX_LEVELS));
         for(; T >= 1; --T)
         {
           std::uint64_t total = 0;
-          bool ok = rows_of(T) <= FEAT_BATCH_TAIL_MAX_ROWS;
+          bool ok = true;
           for(int k = 1; k <= T; ++k)
           {
-            total += rows_of(k);
-            ok = ok && rowlen[k] <= FEAT_BATCH_TAIL_MAX_ROW_LEN;
+            const TailLevel& t = _tl[k];
+            total += t.A.n;
+            ok = ok && t.A.ok && t.A.n <= (k < T ? 1024u : FEAT_BATCH_TAIL_MAX_ROWS) &&
+                 t.A.w <= FEAT_BATCH_TAIL_MAX_ROW_LEN;
+            if(k > 1)
+              ok = ok && t.P.ok && t.R.ok && t.P.w <= FEAT_BATCH_TAIL_MAX_PROL_LEN &&
+                   t.R.w <= FEAT_BATCH_TAIL_MAX_ROW_LEN;
           }
           if(ok && total <= FEAT_BATCH_TAIL_MAX_TOTAL) break;
         }
-        if(T < 1) return;
+        if(T < 1)
+        {
+          _tl.clear();
+          return;
+        }
+        _tl.resize(T + 1);
         const int ni = int(_lev.size());
         std::vector<std::uint64_t> loff(std::size_t(T) * ni, 0), pcg(ni);
         for(int k = 1; k <= T; ++k)
@@ -192,17 +305,33 @@ namespace feat3amd
         _tail.sol_deep = _lv[T]->sol.elements();
         for(int k = 1; k <= T; ++k)
         {
-          const Level& L = *_lv[k];
+          TailLevel& t = _tl[k];
           feat_batch_tail_level_t& d = _tail.lv[k - 1];
-          d.n = rows_of(k);
-          d.A = csr_desc(L.A);
+          t.A.upload();
+          d.n = t.A.n;
+          d.wa = t.A.w;
+          d.a_val = t.A.dval.get();
+          d.a_col = t.A.dcol.get();
+          d.a_len = t.A.dlen.get();
           if(k > 1)
           {
-            d.P = csr_desc(L.P);
-            d.R = csr_desc(L.R);
+            t.P.upload();
+            t.R.upload();
+            d.wp = t.P.w;
+            d.p_val = t.P.dval.get();
+            d.p_col = t.P.dcol.get();
+            d.p_len = t.P.dlen.get();
+            d.wr = t.R.w;
+            d.r_val = t.R.dval.get();
+            d.r_col = t.R.dcol.get();
+            d.r_len = t.R.dlen.get();
           }
-          d.inv = L.inv.elements();
-          d.mask = L.mask.get();
+          d.inv = _lv[k]->inv.elements();
+          d.mask = _lv[k]->mask.get();
+          // the host copies are no longer needed
+          t.A.val.clear();
+          t.P.val.clear();
+          t.R.val.clear();
         }
       }
 
@@ -293,7 +422,11 @@ namespace feat3amd
         // multigrid levels 1.._lmax
         _lv.clear();
         _lv.resize(_lmax + 1);
-        std::vector<std::uint64_t> rowlen(_lmax + 1, 0);  // longest row of A_k, P_k, R_k
+        const int tcap = std::min(std::max(_cfg.tail_levels, 0), std::min(_lmax, FEAT_BATCH_TAIL_MAX_LEVELS));
+        _tl.clear();
+        _tl.resize(tcap + 1);
+        _tiled.clear();
+        _tiled.resize(_lmax + 1);
         for(int k = 1; k <= _lmax; ++k)
         {
           auto L = std::make_unique<Level>();
@@ -319,7 +452,19 @@ namespace feat3amd
             o += nk;
           }
           Ak.rows = Ak.cols = o;
-          rowlen[k] = max_row_len(Ak);
+          if(k <= tcap)
+            _tl[k].A = to_ell(Ak, L->loff, L->loff, nk, nk, false);
+          const bool tiled_k = _cfg.tiled && k > tcap && k > 1 && _cfg.nu_pre <= FEAT_BATCH_TILED_MAX_STEPS &&
+                               _cfg.nu_post <= FEAT_BATCH_TILED_MAX_STEPS;
+          if(tiled_k)
+          {
+            const auto sv = stencil_values(Ak, L->inst.size(), Assembly::q1_nodes_1d(k), bnd);
+            if(!sv.empty())
+            {
+              _tiled[k].n1 = std::uint32_t(Assembly::q1_nodes_1d(k));
+              _tiled[k].sval = DeviceArray<double>::from_host(sv.data(), sv.size());
+            }
+          }
           L->A = Csr::from_host(o, o, Ak.val.data(), Ak.col.data(), Ak.rowptr.data());
           L->mask = DeviceArray<std::uint32_t>::from_host(unit_mask(o, filt).data(), (o + 31) / 32);
           L->top.build(clo, plo, len);
@@ -342,12 +487,34 @@ namespace feat3amd
             Pk.cols = C.A.rows();
             L->P = Csr::from_host(Pk.rows, Pk.cols, Pk.val.data(), Pk.col.data(), Pk.rowptr.data());
             const auto Rk = Assembly::transpose(Pk);
-            rowlen[k] = std::max(rowlen[k], std::max(max_row_len(Pk), max_row_len(Rk)));
+            if(k <= tcap)
+            {
+              std::vector<std::uint64_t> cl;  // each instance's offset on level k-1
+              for(int i : L->inst) cl.push_back(C.loff[std::size_t(std::find(C.inst.begin(), C.inst.end(), i) - C.inst.begin())]);
+              const std::uint64_t nc = C.A.rows() / C.inst.size();
+              _tl[k].P = to_ell(Pk, L->loff, cl, nk, nc, true);
+              _tl[k].R = to_ell(Rk, cl, L->loff, nc, nk, true);
+            }
+            if(_tiled[k].n1)
+            {
+              std::vector<std::uint64_t> cl;
+              for(int i : L->inst) cl.push_back(C.loff[std::size_t(std::find(C.inst.begin(), C.inst.end(), i) - C.inst.begin())]);
+              TiledLevel& tk = _tiled[k];
+              tk.P = to_ell(Pk, L->loff, cl, nk, C.A.rows() / C.inst.size(), true);
+              if(tk.P.ok && tk.P.w <= FEAT_BATCH_TAIL_MAX_PROL_LEN)
+              {
+                tk.P.upload();
+                tk.P.val.clear();
+                tk.coff = DeviceArray<std::uint64_t>::from_host(cl.data(), cl.size());
+              }
+              else
+                tk = TiledLevel{};
+            }
             L->R = Csr::from_host(Rk.rows, Rk.cols, Rk.val.data(), Rk.col.data(), Rk.rowptr.data());
           }
           _lv[k] = std::move(L);
         }
-        plan_tail(rowlen);
+        plan_tail();
         for(Vec* v : {&_r, &_p, &_t, &_z}) *v = Vec(size());
         for(DeviceArray<double>* a : {&_gamma, &_gold, &_tp, &_rr, &_dinit, &_dcur}) *a = DeviceArray<double>(ni);
         _hist = DeviceArray<double>(Index(ni) * (_cfg.max_iter + 1));
@@ -388,6 +555,40 @@ namespace feat3amd
         Runtime::synchronize();
       }
 
+      feat_batch_tiled_t tiled_desc(int k, int nu)
+      {
+        Level& L = *_lv[k];
+        const TiledLevel& t = _tiled[k];
+        feat_batch_tiled_t d{};
+        d.n1 = t.n1;
+        d.ninst = std::uint32_t(L.inst.size());
+        d.nu = nu;
+        d.omega = _cfg.omega_rich;
+        d.sval = t.sval.get();
+        d.inv = L.inv.elements();
+        d.mask = L.mask.get();
+        d.rhs = L.rhs.elements();
+        d.sol = L.sol.elements();
+        d.def = L.def.elements();
+        d.xc = _lv[k - 1]->sol.elements();
+        d.coff = t.coff.get();
+        d.wp = t.P.w;
+        d.p_val = t.P.dval.get();
+        d.p_col = t.P.dcol.get();
+        d.p_len = t.P.dlen.get();
+        return d;
+      }
+
+    public:
+      /// levels whose smoother calls run tiled
+      std::vector<int> tiled_levels() const
+      {
+        std::vector<int> v;
+        for(int k = 1; k < int(_tiled.size()); ++k)
+          if(_tiled[k].n1) v.push_back(k);
+        return v;
+      }
+
       /// z = MG(r) for every instance (fused kernels of the single solve over block-diagonal levels)
       void mg_apply(double* z, const double* r)
       {
@@ -401,11 +602,19 @@ namespace feat3amd
             check(feat_batch_gather_f64(h, L.top.n, L.top.lvl_lo.get(), L.top.pcg_lo.get(), L.top.len.get(),
                                         L.rhs.elements(), r),
                   "batch inject");
-          const feat_matrix_f64_t& A = L.A.exec();
-          check(feat_rj_smooth_f64(h, &A, L.inv.elements(), L.mask.get(), _cfg.nu_pre, _cfg.omega_rich, L.rhs.elements(),
-                                   nullptr, L.sol.elements(), L.tmp.elements(), L.def.elements(), nullptr, nullptr,
-                                   nullptr, ws),
-                "batch pre-smoothing");
+          if(_tiled[k].n1)
+          {
+            const feat_batch_tiled_t d = tiled_desc(k, _cfg.nu_pre);
+            check(feat_batch_pre_tiled_f64(h, &d), "batch pre-smoothing (tiled)");
+          }
+          else
+          {
+            const feat_matrix_f64_t& A = L.A.exec();
+            check(feat_rj_smooth_f64(h, &A, L.inv.elements(), L.mask.get(), _cfg.nu_pre, _cfg.omega_rich,
+                                     L.rhs.elements(), nullptr, L.sol.elements(), L.tmp.elements(), L.def.elements(),
+                                     nullptr, nullptr, nullptr, ws),
+                  "batch pre-smoothing");
+          }
           Level& C = *_lv[k - 1];
           const feat_matrix_f64_t& R = L.R.exec();
           check(feat_apply_masked_f64(h, &R, C.rhs.elements(), L.def.elements(), C.mask.get(), nullptr, nullptr,
@@ -433,15 +642,24 @@ namespace feat3amd
         for(int k = std::max(2, T + 1); k <= _lmax; ++k)
         {
           Level& L = *_lv[k];
-          const feat_matrix_f64_t& P = L.P.exec();
-          check(feat_prol_add_f64(h, &P, L.sol.elements(), _lv[k - 1]->sol.elements(), L.mask.get()), "batch prolongation");
-          const feat_matrix_f64_t& A = L.A.exec();
-          check(feat_defect_f64(h, &A, L.def.elements(), L.rhs.elements(), L.sol.elements(), L.mask.get(), nullptr, ws),
-                "batch defect");
-          check(feat_rj_smooth_f64(h, &A, L.inv.elements(), L.mask.get(), _cfg.nu_post, _cfg.omega_rich, L.def.elements(),
-                                   nullptr, L.cor.elements(), L.tmp.elements(), nullptr, L.sol.elements(), nullptr,
-                                   nullptr, ws),
-                "batch post-smoothing");
+          if(_tiled[k].n1)
+          {
+            const feat_batch_tiled_t d = tiled_desc(k, _cfg.nu_post);
+            check(feat_batch_post_tiled_f64(h, &d), "batch prolongation + defect + post-smoothing (tiled)");
+          }
+          else
+          {
+            const feat_matrix_f64_t& P = L.P.exec();
+            check(feat_prol_add_f64(h, &P, L.sol.elements(), _lv[k - 1]->sol.elements(), L.mask.get()),
+                  "batch prolongation");
+            const feat_matrix_f64_t& A = L.A.exec();
+            check(feat_defect_f64(h, &A, L.def.elements(), L.rhs.elements(), L.sol.elements(), L.mask.get(), nullptr, ws),
+                  "batch defect");
+            check(feat_rj_smooth_f64(h, &A, L.inv.elements(), L.mask.get(), _cfg.nu_post, _cfg.omega_rich,
+                                     L.def.elements(), nullptr, L.cor.elements(), L.tmp.elements(), nullptr,
+                                     L.sol.elements(), nullptr, nullptr, ws),
+                  "batch post-smoothing");
+          }
           if(L.top.n)
             check(feat_batch_gather_f64(h, L.top.n, L.top.pcg_lo.get(), L.top.lvl_lo.get(), L.top.len.get(), z,
                                         L.sol.elements()),

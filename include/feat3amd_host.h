@@ -217,6 +217,8 @@ typedef struct {
   int tail_levels;  /* levels 1..tail_levels of every instance in one launch per V-cycle
                        (feat_batch_tail_f64, bit-identical; lowered to what one workgroup
                        holds); <= 0: every level as block-diagonal launches */
+  int no_tiled;     /* 1: the smoother calls above the tail as SELL-64 launches instead of
+                       the tiled structured-grid kernels (feat_batch_pre/post_tiled_f64) */
 } feat_batch_cfg_t;
 typedef struct feat_batchsolver_s* feat_batchsolver_t;
 int feat_batchsolver_create(feat_batchsolver_t* s, const feat_batch_cfg_t* cfg);
@@ -231,6 +233,8 @@ int feat_batchsolver_solve(feat_batchsolver_t s, double* x, const double* b, int
 int feat_batchsolver_tuning(feat_batchsolver_t s, int* tunes, double* us, int cap, int* count);
 /* levels run by the fused coarse tail (0: none) */
 int feat_batchsolver_tail_levels(feat_batchsolver_t s, int* levels);
+/* the levels whose smoother calls run tiled: up to cap of them into levels, their number in *count */
+int feat_batchsolver_tiled_levels(feat_batchsolver_t s, int* levels, int cap, int* count);
 int feat_batchsolver_destroy(feat_batchsolver_t s);
 /* setup pieces of one instance, for tests */
 int feat_batch_levels(uint64_t seed, int ninst, int lmin, int lmax, int* out);

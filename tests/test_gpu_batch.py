@@ -1,8 +1,9 @@
 """The batch of uneven Q1 diffusion MG-PCG solves on the GPU against the oracle,
 instance by instance (same method, same stopping rule): equal iteration counts,
 solutions within 1e-10, and bit-identical results with and without the
-recorded-graph replay, with and without autotuned launch shapes and with and
-without the fused coarse tail (levels 1..T of every instance in one launch)."""
+recorded-graph replay, with and without autotuned launch shapes, with and
+without the fused coarse tail (levels 1..T of every instance in one launch) and
+with and without the tiled smoother calls on the structured grids."""
 import numpy as np
 import pytest
 
@@ -88,8 +89,8 @@ def _solve(torch, bs, b):
 ])
 def test_batch_coarse_tail_bitwise(feat, torch_gpu, ninst, lmin, lmax, tail, depth):
     torch = torch_gpu
-    ref = feat.BatchSolver(SEED, ninst=ninst, lmin=lmin, lmax=lmax, tail_levels=0)
-    bs = feat.BatchSolver(SEED, ninst=ninst, lmin=lmin, lmax=lmax, tail_levels=tail)
+    ref = feat.BatchSolver(SEED, ninst=ninst, lmin=lmin, lmax=lmax, tail_levels=0, tiled=False)
+    bs = feat.BatchSolver(SEED, ninst=ninst, lmin=lmin, lmax=lmax, tail_levels=tail, tiled=False)
     try:
         assert ref.tail_levels == 0 and bs.tail_levels == depth
         b = dev(torch, batch_rhs(feat, ref))
@@ -106,14 +107,38 @@ def test_batch_coarse_tail_bitwise(feat, torch_gpu, ninst, lmin, lmax, tail, dep
         ref.close()
 
 
+@pytest.mark.parametrize("tail", [5, 0])
+def test_batch_tiled_smoother_bitwise(feat, torch_gpu, tail):
+    # tiled smoother calls on the levels above the tail (tail 0: levels 2..6, down to the
+    # 5 x 5 grid of level 2, one clipped tile) against the SELL-64 launches
+    torch = torch_gpu
+    ref = feat.BatchSolver(SEED, ninst=12, lmin=3, lmax=6, tail_levels=tail, tiled=False)
+    bs = feat.BatchSolver(SEED, ninst=12, lmin=3, lmax=6, tail_levels=tail)
+    try:
+        assert ref.tiled_levels == []
+        assert bs.tiled_levels == ([6] if tail else [2, 3, 4, 5, 6])
+        b = dev(torch, batch_rhs(feat, ref))
+        x0, r0 = _solve(torch, ref, b)
+        for rep in range(2):
+            x1, r1 = _solve(torch, bs, b)
+            assert np.array_equal(x1, x0), rep
+            assert np.array_equal(r1["iters"], r0["iters"])
+            for i, it in enumerate(r0["iters"]):
+                assert np.array_equal(r1["hist"][i, :it + 1], r0["hist"][i, :it + 1]), i
+    finally:
+        bs.close()
+        ref.close()
+
+
 def test_batch_full_64(feat, torch_gpu):
     # the full batch: 64 instances, levels 3..8, all converge to 1e-8; the fused coarse
-    # tail (levels 1..5) gives the same bits as the block-diagonal launches
+    # tail (levels 1..5) and the tiled smoother calls (levels 6..8) give the same bits as
+    # the block-diagonal SELL-64 launches
     torch = torch_gpu
     bs = feat.BatchSolver(SEED, ninst=64, lmin=3, lmax=8)
-    ref = feat.BatchSolver(SEED, ninst=64, lmin=3, lmax=8, tail_levels=0)
+    ref = feat.BatchSolver(SEED, ninst=64, lmin=3, lmax=8, tail_levels=0, tiled=False)
     try:
-        assert bs.tail_levels == 5
+        assert bs.tail_levels == 5 and bs.tiled_levels == [6, 7, 8]
         b = dev(torch, batch_rhs(feat, bs))
         xg, r = _solve(torch, bs, b)
         x0, r0 = _solve(torch, ref, b)
