@@ -551,13 +551,19 @@ __global__ __launch_bounds__(kTl) void k_batch_tiled(feat_batch_tiled_t t)
     {
       const uint64_t g = base + row[q];
       m[q] = is_masked(t.mask, g);
-      rv[q] = t.rhs[g];
+      if(!POST) rv[q] = t.rhs[g];
       iv[q] = t.inv[g];
       if(POST) xs[q] = t.sol[g];
     }
-#pragma unroll
-    for(int u = 0; u < 9; ++u) a[q][u] = (on[q] && !m[q] && in[q]) ? t.sval[(uint64_t(p) * 9 + u) * nk + row[q]] : 0.0;
   }
+  // the stencil values (after the prolongation's loads when there is one: fewer live registers)
+  auto load_stencil = [&]() {
+#pragma unroll
+    for(int q = 0; q < kTlPts; ++q)
+#pragma unroll
+      for(int u = 0; u < 9; ++u)
+        a[q][u] = (on[q] && !m[q] && in[q]) ? t.sval[(uint64_t(p) * 9 + u) * nk + row[q]] : 0.0;
+  };
   auto stencil = [&](int q, const double* x) -> double {
     double s = 0.0;
     if(in[q] && !m[q])
@@ -579,6 +585,7 @@ __global__ __launch_bounds__(kTl) void k_batch_tiled(feat_batch_tiled_t t)
 
   if constexpr(!POST)
   {
+    load_stencil();
     // first iterate from x = 0 (k_smooth_first)
     double* x1 = buf(1);
 #pragma unroll
@@ -647,6 +654,7 @@ __global__ __launch_bounds__(kTl) void k_batch_tiled(feat_batch_tiled_t t)
         xs[q] = dfma(1.0, c, xs[q]);
         s_b[2][li[q]] = xs[q];
       }
+    load_stencil();
     __syncthreads();
     // defect, then the first iterate on it
     double* y1 = buf(1);
@@ -654,14 +662,14 @@ __global__ __launch_bounds__(kTl) void k_batch_tiled(feat_batch_tiled_t t)
     for(int q = 0; q < kTlPts; ++q)
       if(on[q])
       {
-        double d = dfma(stencil(q, s_b[2]), -1.0, rv[q]);
+        double d = dfma(stencil(q, s_b[2]), -1.0, t.rhs[base + row[q]]);
         if(m[q]) d = 0.0;
         dq[q] = d;
         double c = iv[q] * d;
         if(m[q]) c = 0.0;
         const double xn = dfma(omega, c, 0.0);
         if(nu == 1)
-          xs[q] = dfma(1.0, xn, xs[q]);
+          xs[q] = dfma(1.0, xn, s_b[2][li[q]]);
         else
           y1[li[q]] = xn;
       }
@@ -680,7 +688,7 @@ __global__ __launch_bounds__(kTl) void k_batch_tiled(feat_batch_tiled_t t)
           if(m[q]) c = 0.0;
           const double xn = dfma(omega, c, yk[li[q]]);
           if(s == nu - 1)
-            xs[q] = dfma(1.0, xn, xs[q]);
+            xs[q] = dfma(1.0, xn, s_b[2][li[q]]);
           else
             yo[li[q]] = xn;
         }
