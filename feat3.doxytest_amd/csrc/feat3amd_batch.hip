@@ -530,6 +530,10 @@ __global__ __launch_bounds__(kTl) void k_batch_tiled(feat_batch_tiled_t t)
   const double omega = t.omega;
   const int nu = t.nu;
 
+  // an instance entering the V-cycle here reads its right-hand side from r (and post
+  // leaves its solution in z too)
+  const uint64_t top = t.top ? t.top[p] : ~uint64_t(0);
+  auto rhs_at = [&](uint32_t rw_) { return top != ~uint64_t(0) ? t.r[top + rw_] : t.rhs[base + rw_]; };
   double a[kTlPts][9], rv[kTlPts], iv[kTlPts], dq[kTlPts], xs[kTlPts];
   int li[kTlPts];
   uint32_t row[kTlPts];
@@ -551,7 +555,7 @@ __global__ __launch_bounds__(kTl) void k_batch_tiled(feat_batch_tiled_t t)
     {
       const uint64_t g = base + row[q];
       m[q] = is_masked(t.mask, g);
-      if(!POST) rv[q] = t.rhs[g];
+      if(!POST) rv[q] = rhs_at(row[q]);
       iv[q] = t.inv[g];
       if(POST) xs[q] = t.sol[g];
     }
@@ -662,7 +666,7 @@ __global__ __launch_bounds__(kTl) void k_batch_tiled(feat_batch_tiled_t t)
     for(int q = 0; q < kTlPts; ++q)
       if(on[q])
       {
-        double d = dfma(stencil(q, s_b[2]), -1.0, t.rhs[base + row[q]]);
+        double d = dfma(stencil(q, s_b[2]), -1.0, rhs_at(row[q]));
         if(m[q]) d = 0.0;
         dq[q] = d;
         double c = iv[q] * d;
@@ -695,7 +699,11 @@ __global__ __launch_bounds__(kTl) void k_batch_tiled(feat_batch_tiled_t t)
     }
 #pragma unroll
     for(int q = 0; q < kTlPts; ++q)
-      if(out[q]) t.sol[base + row[q]] = xs[q];
+      if(out[q])
+      {
+        t.sol[base + row[q]] = xs[q];
+        if(top != ~uint64_t(0)) t.z[top + row[q]] = xs[q];
+      }
   }
 }
 
@@ -709,6 +717,10 @@ int tiled_launch(feat_handle_t h, const feat_batch_tiled_t* t, bool post, const 
     return fail(FEAT_ERR_INVALID, "batch tiled smoother: null array");
   if(post && (!t->xc || !t->coff || !t->p_val || !t->p_col || !t->p_len || t->wp > FEAT_BATCH_TAIL_MAX_PROL_LEN))
     return fail(FEAT_ERR_INVALID, "batch tiled smoother: prolongation incomplete");
+  if(t->top && (!t->r || (post && !t->z)))
+    return fail(FEAT_ERR_INVALID, "batch tiled smoother: r / z missing for the entering instances");
+  if(t->top && (static_cast<const void*>(t->r) == t->sol || (post && static_cast<const void*>(t->r) == t->z)))
+    return fail(FEAT_ERR_ALIAS, "batch tiled smoother: r aliases an output");
   if(static_cast<const void*>(t->rhs) == t->sol || (!post && (t->def == t->sol || static_cast<const void*>(t->rhs) == t->def)) ||
      (post && static_cast<const void*>(t->xc) == t->sol))
     return fail(FEAT_ERR_ALIAS, "batch tiled smoother: vectors alias");

@@ -706,6 +706,12 @@ typedef struct {
   const double* p_val;
   const uint16_t* p_col;
   const uint8_t* p_len;
+  /* may be NULL: per instance, the offset in r / z (the PCG layout) of an instance whose
+   * finest level is this one (its right-hand side is read from r instead of rhs, and post
+   * also writes its solution to z), UINT64_MAX for the others */
+  const uint64_t* top;
+  const double* r;
+  double* z;
 } feat_batch_tiled_t;
 int feat_batch_pre_tiled_f64(feat_handle_t h, const feat_batch_tiled_t* t);
 int feat_batch_post_tiled_f64(feat_handle_t h, const feat_batch_tiled_t* t);

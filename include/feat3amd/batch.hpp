@@ -172,6 +172,7 @@ namespace feat3amd
         DeviceArray<double> sval;
         Ell P;
         DeviceArray<std::uint64_t> coff;
+        DeviceArray<std::uint64_t> top;  // per position: PCG offset of an instance entering here, else ~0
       };
       std::vector<TiledLevel> _tiled;
 
@@ -506,6 +507,9 @@ namespace feat3amd
                 tk.P.upload();
                 tk.P.val.clear();
                 tk.coff = DeviceArray<std::uint64_t>::from_host(cl.data(), cl.size());
+                std::vector<std::uint64_t> tp;
+                for(int i : L->inst) tp.push_back(_lev[i] == k ? _off[i] : ~std::uint64_t(0));
+                tk.top = DeviceArray<std::uint64_t>::from_host(tp.data(), tp.size());
               }
               else
                 tk = TiledLevel{};
@@ -555,7 +559,7 @@ namespace feat3amd
         Runtime::synchronize();
       }
 
-      feat_batch_tiled_t tiled_desc(int k, int nu)
+      feat_batch_tiled_t tiled_desc(int k, int nu, const double* r, double* z)
       {
         Level& L = *_lv[k];
         const TiledLevel& t = _tiled[k];
@@ -576,6 +580,9 @@ namespace feat3amd
         d.p_val = t.P.dval.get();
         d.p_col = t.P.dcol.get();
         d.p_len = t.P.dlen.get();
+        d.top = t.top.get();
+        d.r = r;
+        d.z = z;
         return d;
       }
 
@@ -598,13 +605,14 @@ namespace feat3amd
         for(int k = _lmax; k >= std::max(2, T + 1); --k)
         {
           Level& L = *_lv[k];
-          if(L.top.n)
+          // a tiled level reads the right-hand sides of its entering instances from r itself
+          if(L.top.n && !_tiled[k].n1)
             check(feat_batch_gather_f64(h, L.top.n, L.top.lvl_lo.get(), L.top.pcg_lo.get(), L.top.len.get(),
                                         L.rhs.elements(), r),
                   "batch inject");
           if(_tiled[k].n1)
           {
-            const feat_batch_tiled_t d = tiled_desc(k, _cfg.nu_pre);
+            const feat_batch_tiled_t d = tiled_desc(k, _cfg.nu_pre, r, z);
             check(feat_batch_pre_tiled_f64(h, &d), "batch pre-smoothing (tiled)");
           }
           else
@@ -644,7 +652,7 @@ namespace feat3amd
           Level& L = *_lv[k];
           if(_tiled[k].n1)
           {
-            const feat_batch_tiled_t d = tiled_desc(k, _cfg.nu_post);
+            const feat_batch_tiled_t d = tiled_desc(k, _cfg.nu_post, r, z);
             check(feat_batch_post_tiled_f64(h, &d), "batch prolongation + defect + post-smoothing (tiled)");
           }
           else
@@ -660,7 +668,7 @@ namespace feat3amd
                                      L.sol.elements(), nullptr, nullptr, ws),
                   "batch post-smoothing");
           }
-          if(L.top.n)
+          if(L.top.n && !_tiled[k].n1)
             check(feat_batch_gather_f64(h, L.top.n, L.top.pcg_lo.get(), L.top.lvl_lo.get(), L.top.len.get(), z,
                                         L.sol.elements()),
                   "batch extract");
