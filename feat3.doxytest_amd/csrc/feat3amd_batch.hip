@@ -634,6 +634,35 @@ __global__ __launch_bounds__(kTl) void k_batch_tiled(feat_batch_tiled_t t)
   {
     // sol += filter(P xc)
     const double* xc = t.xc + t.coff[p];
+    if(t.p_struct)
+    {
+      // bilinear injection: the row's entries in CSR order, from the node position
+      const int nc1 = (n1 + 1) / 2;
+      load_stencil();
+#pragma unroll
+      for(int q = 0; q < kTlPts; ++q)
+        if(on[q])
+        {
+          const int gx = int(row[q] % uint32_t(n1)), gy = int(row[q] / uint32_t(n1));
+          const double* xr = xc + (gy >> 1) * nc1 + (gx >> 1);
+          const bool ox = gx & 1, oy = gy & 1;
+          double sum;
+          if(!ox && !oy)
+            sum = dfma(1.0, xr[0], 0.0);
+          else if(!oy)
+            sum = dfma(0.5, xr[1], dfma(0.5, xr[0], 0.0));
+          else if(!ox)
+            sum = dfma(0.5, xr[nc1], dfma(0.5, xr[0], 0.0));
+          else
+            sum = dfma(0.25, xr[nc1 + 1], dfma(0.25, xr[nc1], dfma(0.25, xr[1], dfma(0.25, xr[0], 0.0))));
+          double c = dfma(sum, 1.0, 0.0);
+          if(m[q]) c = 0.0;
+          xs[q] = dfma(1.0, c, xs[q]);
+          s_b[2][li[q]] = xs[q];
+        }
+    }
+    else
+    {
 #pragma unroll
     for(int q = 0; q < kTlPts; ++q)
       if(on[q])
@@ -659,6 +688,7 @@ __global__ __launch_bounds__(kTl) void k_batch_tiled(feat_batch_tiled_t t)
         s_b[2][li[q]] = xs[q];
       }
     load_stencil();
+    }
     __syncthreads();
     // defect, then the first iterate on it
     double* y1 = buf(1);
@@ -715,7 +745,8 @@ int tiled_launch(feat_handle_t h, const feat_batch_tiled_t* t, bool post, const 
   if(!t->ninst) return FEAT_OK;
   if(!t->sval || !t->inv || !t->mask || !t->rhs || !t->sol || (!post && !t->def))
     return fail(FEAT_ERR_INVALID, "batch tiled smoother: null array");
-  if(post && (!t->xc || !t->coff || !t->p_val || !t->p_col || !t->p_len || t->wp > FEAT_BATCH_TAIL_MAX_PROL_LEN))
+  if(post && (!t->xc || !t->coff ||
+              (!t->p_struct && (!t->p_val || !t->p_col || !t->p_len || t->wp > FEAT_BATCH_TAIL_MAX_PROL_LEN))))
     return fail(FEAT_ERR_INVALID, "batch tiled smoother: prolongation incomplete");
   if(t->top && (!t->r || (post && !t->z)))
     return fail(FEAT_ERR_INVALID, "batch tiled smoother: r / z missing for the entering instances");

@@ -712,6 +712,11 @@ typedef struct {
   const uint64_t* top;
   const double* r;
   double* z;
+  /* 1: the prolongation is bilinear injection between the grids, row (x, y) of level k
+   * taking the level k-1 nodes (x/2 .. (x+1)/2, y/2 .. (y+1)/2) in lexicographic order with
+   * the weights 1, 1/2, 1/4 (checked by the caller): formed from the node position, the
+   * p_* arrays are not read */
+  int p_struct;
 } feat_batch_tiled_t;
 int feat_batch_pre_tiled_f64(feat_handle_t h, const feat_batch_tiled_t* t);
 int feat_batch_post_tiled_f64(feat_handle_t h, const feat_batch_tiled_t* t);

@@ -50,6 +50,8 @@ namespace feat3amd
       /// the smoother calls of the levels above the tail as tiled launches on the structured
       /// grids (feat_batch_pre_tiled_f64 / feat_batch_post_tiled_f64, bit-identical)
       bool tiled = true;
+      /// the tiled post-smoothing reads the prolongation's arrays even where it is bilinear injection
+      bool tiled_prol_arrays = false;
     };
 
     class BatchedMgPcg
@@ -173,8 +175,30 @@ namespace feat3amd
         Ell P;
         DeviceArray<std::uint64_t> coff;
         DeviceArray<std::uint64_t> top;  // per position: PCG offset of an instance entering here, else ~0
+        bool p_struct = false;           // the prolongation is bilinear injection (no P arrays read)
       };
       std::vector<TiledLevel> _tiled;
+
+      // true when every row of the one-instance prolongation P1 (level with n1 nodes per side
+      // from the level below) is bilinear injection: the coarse nodes (x/2 .. (x+1)/2,
+      // y/2 .. (y+1)/2) in lexicographic order with the weights 1, 1/2, 1/4
+      static bool bilinear_prolongation(const Assembly::CsrHost& P1, std::uint64_t n1)
+      {
+        const std::uint64_t nc1 = (n1 + 1) / 2;
+        if(P1.rows != n1 * n1 || P1.rowptr.size() != P1.rows + 1) return false;
+        for(std::uint64_t r = 0; r < P1.rows; ++r)
+        {
+          const std::uint64_t x = r % n1, y = r / n1, ox = x & 1, oy = y & 1;
+          const std::uint64_t e0 = P1.rowptr[r], len = P1.rowptr[r + 1] - e0;
+          if(len != (1 + ox) * (1 + oy)) return false;
+          const double w = (ox && oy) ? 0.25 : ((ox || oy) ? 0.5 : 1.0);
+          std::uint64_t u = 0;
+          for(std::uint64_t iy = 0; iy <= oy; ++iy)
+            for(std::uint64_t ix = 0; ix <= ox; ++ix, ++u)
+              if(P1.col[e0 + u] != (y / 2 + iy) * nc1 + (x / 2 + ix) || P1.val[e0 + u] != w) return false;
+        }
+        return true;
+      }
 
       // the nine stencil values of every free row of the diagonal blocks of A (block b: rows
       // b*n1*n1 ..), checked to be the full 9-point neighbourhood in CSR order; empty when not
@@ -501,6 +525,7 @@ namespace feat3amd
               std::vector<std::uint64_t> cl;
               for(int i : L->inst) cl.push_back(C.loff[std::size_t(std::find(C.inst.begin(), C.inst.end(), i) - C.inst.begin())]);
               TiledLevel& tk = _tiled[k];
+              tk.p_struct = !_cfg.tiled_prol_arrays && bilinear_prolongation(P1, tk.n1);
               tk.P = to_ell(Pk, L->loff, cl, nk, C.A.rows() / C.inst.size(), true);
               if(tk.P.ok && tk.P.w <= FEAT_BATCH_TAIL_MAX_PROL_LEN)
               {
@@ -581,6 +606,7 @@ namespace feat3amd
         d.p_col = t.P.dcol.get();
         d.p_len = t.P.dlen.get();
         d.top = t.top.get();
+        d.p_struct = t.p_struct ? 1 : 0;
         d.r = r;
         d.z = z;
         return d;

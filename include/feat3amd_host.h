@@ -219,6 +219,8 @@ typedef struct {
                        holds); <= 0: every level as block-diagonal launches */
   int no_tiled;     /* 1: the smoother calls above the tail as SELL-64 launches instead of
                        the tiled structured-grid kernels (feat_batch_pre/post_tiled_f64) */
+  int tiled_prol_arrays;  /* 1: the tiled post-smoothing reads the prolongation's ELL arrays
+                             even where it is bilinear injection (formed from the node position) */
 } feat_batch_cfg_t;
 typedef struct feat_batchsolver_s* feat_batchsolver_t;
 int feat_batchsolver_create(feat_batchsolver_t* s, const feat_batch_cfg_t* cfg);
