@@ -204,6 +204,10 @@ KERNEL_API = [
     ("feat_batch_status_f64", _i, [_vp, C.c_uint32, _vp, _vp, _vp, _i]),
     ("feat_batch_gather_f64", _i, [_vp, C.c_uint32, _vp, _vp, _vp, _vp, _vp]),
     ("feat_batch_tail_f64", _i, [_vp, _vp, _vp, _vp]),
+    ("feat_batch_dot_partials_f64", _i, [_vp, _vp, _vp, _vp, _vp]),
+    ("feat_batch_pcg_update_partials_f64", _i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    ("feat_batch_status_partials_f64", _i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    ("feat_batch_pcg_direction_partials_f64", _i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     ("feat_batch_pre_tiled_f64", _i, [_vp, _vp]),
     ("feat_batch_post_tiled_f64", _i, [_vp, _vp]),
     ("feat_copy_int_to_host", _i, [_vp, _vp, _vp]),

@@ -26,20 +26,29 @@ BatchDev to_dev(const feat_batch_t* b)
   return BatchDev{b->ninst, b->nchunks, b->off, b->chunk_inst, b->chunk_lo, b->chunk_hi, b->inst_chunk, b->partials};
 }
 
-__device__ __forceinline__ void chunk_partial(const BatchDev& b, double v)
+// the chunk's partial sum into part (b.partials when null)
+__device__ __forceinline__ void chunk_partial(const BatchDev& b, double v, double* part = nullptr)
 {
   __shared__ double red[kThreads / 64];
   const double t = block_sum(v, red);
-  if(threadIdx.x == 0) b.partials[blockIdx.x] = t;
+  if(threadIdx.x == 0) (part ? part : b.partials)[blockIdx.x] = t;
+}
+
+// instance i's sum of its chunk partials, in ascending chunk order (as k_batch_finish)
+__device__ __forceinline__ double inst_sum(const BatchDev& b, const double* __restrict__ part, uint32_t i)
+{
+  double s = 0.0;
+  for(uint32_t c = b.inst_chunk[i]; c < b.inst_chunk[i + 1]; ++c) s += part[c];
+  return s;
 }
 
 __global__ __launch_bounds__(kThreads) void k_batch_dot(BatchDev b, const double* __restrict__ x,
-                                                        const double* __restrict__ y)
+                                                        const double* __restrict__ y, double* part)
 {
   const uint64_t lo = b.chunk_lo[blockIdx.x], hi = b.chunk_hi[blockIdx.x];
   double s = 0.0;
   for(uint64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) s = dfma(x[i], y[i], s);
-  chunk_partial(b, s);
+  chunk_partial(b, s, part);
 }
 
 // out[i] = sum of instance i's chunk partials (ascending); only_active: skip inactive
@@ -53,17 +62,22 @@ __global__ void k_batch_finish(BatchDev b, double* __restrict__ out, const int* 
   out[i] = s;
 }
 
+// g_part / tp_part (may be null): gamma / tp as the instance sums of those chunk partials
 __global__ __launch_bounds__(kThreads) void k_batch_update(BatchDev b, double* __restrict__ x, double* __restrict__ r,
                                                            const double* __restrict__ p, const double* __restrict__ t,
                                                            const double* __restrict__ gamma,
-                                                           const double* __restrict__ tp, const int* __restrict__ active)
+                                                           const double* __restrict__ tp, const int* __restrict__ active,
+                                                           const double* __restrict__ g_part,
+                                                           const double* __restrict__ tp_part, double* rr_part)
 {
   const uint32_t inst = b.chunk_inst[blockIdx.x];
   const uint64_t lo = b.chunk_lo[blockIdx.x], hi = b.chunk_hi[blockIdx.x];
   double s = 0.0;
   if(active[inst])
   {
-    const double alpha = gamma[inst] / tp[inst];
+    const double g = g_part ? inst_sum(b, g_part, inst) : gamma[inst];
+    const double q = tp_part ? inst_sum(b, tp_part, inst) : tp[inst];
+    const double alpha = g / q;
     // DenseVector::axpy: |alpha| < eps leaves x and r unchanged
     const bool upd = fabs(alpha) >= DBL_EPSILON;
     for(uint64_t i = lo + threadIdx.x; i < hi; i += blockDim.x)
@@ -78,19 +92,21 @@ __global__ __launch_bounds__(kThreads) void k_batch_update(BatchDev b, double* _
       s = dfma(ri, ri, s);
     }
   }
-  chunk_partial(b, s);
+  chunk_partial(b, s, rr_part);
 }
 
+// g_part (may be null): gamma_new as the instance sum of those chunk partials
 __global__ __launch_bounds__(kThreads) void k_batch_direction(BatchDev b, double* __restrict__ p,
                                                               const double* __restrict__ z,
                                                               const double* __restrict__ gnew,
                                                               const double* __restrict__ gold,
-                                                              const int* __restrict__ active, int first)
+                                                              const int* __restrict__ active, int first,
+                                                              const double* __restrict__ g_part)
 {
   const uint32_t inst = b.chunk_inst[blockIdx.x];
   if(!active[inst]) return;
   const uint64_t lo = b.chunk_lo[blockIdx.x], hi = b.chunk_hi[blockIdx.x];
-  const double beta = first ? 0.0 : gnew[inst] / gold[inst];
+  const double beta = first ? 0.0 : (g_part ? inst_sum(b, g_part, inst) : gnew[inst]) / gold[inst];
   // DenseVector::axpy(p, z, beta): |beta| < eps copies z; else r == x branch: p *= beta; p += z
   const bool copy = first || fabs(beta) < DBL_EPSILON;
   for(uint64_t i = lo + threadIdx.x; i < hi; i += blockDim.x)
@@ -121,13 +137,18 @@ struct StopDev
 // Status codes of solver/base.hpp:48-64
 constexpr int kProgress = 1, kSuccess = 2, kAborted = 3, kDiverged = 4, kMaxIter = 5;
 
-__global__ void k_batch_status(uint32_t n, const double* __restrict__ rr, StopDev c, feat_batch_state_t st, int initial)
+// rr_part (may be null): rr as the instance sums of those chunk partials; gamma (may be
+// null): first, for the instances active on entry, gamma[i] = instance sum of g_part
+__global__ void k_batch_status(uint32_t n, const double* __restrict__ rr, StopDev c, feat_batch_state_t st, int initial,
+                               BatchDev b, const double* __restrict__ rr_part, double* __restrict__ gamma,
+                               const double* __restrict__ g_part)
 {
   __shared__ int cnt[kThreads / 64];
   int mine = 0;
   for(uint32_t i = threadIdx.x; i < n; i += blockDim.x)
   {
-    const double def = sqrt(rr[i]);
+    if(gamma && st.active[i]) gamma[i] = inst_sum(b, g_part, i);
+    const double def = sqrt(rr_part ? inst_sum(b, rr_part, i) : rr[i]);
     if(initial)
     {
       // iterative.hpp:748-770
@@ -784,7 +805,7 @@ int feat_batch_dot_f64(feat_handle_t h, const feat_batch_t* b, const double* x, 
 {
   if(!h || !batch_ok(b) || !x || !y || !out) return fail(FEAT_ERR_INVALID, "feat_batch_dot_f64: null argument");
   if(!b->nchunks) return FEAT_OK;
-  hipLaunchKernelGGL(k_batch_dot, dim3(b->nchunks), dim3(kThreads), 0, h->stream, to_dev(b), x, y);
+  hipLaunchKernelGGL(k_batch_dot, dim3(b->nchunks), dim3(kThreads), 0, h->stream, to_dev(b), x, y, (double*)nullptr);
   int rc = check_launch("feat_batch_dot_f64");
   return rc ? rc : finish(h, b, out, nullptr, "feat_batch_dot_f64");
 }
@@ -796,7 +817,7 @@ int feat_batch_pcg_update_f64(feat_handle_t h, const feat_batch_t* b, double* x,
     return fail(FEAT_ERR_INVALID, "feat_batch_pcg_update_f64: null argument");
   if(!b->nchunks) return FEAT_OK;
   hipLaunchKernelGGL(k_batch_update, dim3(b->nchunks), dim3(kThreads), 0, h->stream, to_dev(b), x, r, p, t, gamma, tp,
-                     active);
+                     active, (const double*)nullptr, (const double*)nullptr, (double*)nullptr);
   int rc = check_launch("feat_batch_pcg_update_f64");
   return rc ? rc : finish(h, b, rr, active, "feat_batch_pcg_update_f64");
 }
@@ -808,7 +829,7 @@ int feat_batch_pcg_direction_f64(feat_handle_t h, const feat_batch_t* b, double*
     return fail(FEAT_ERR_INVALID, "feat_batch_pcg_direction_f64: null argument");
   if(!b->nchunks) return FEAT_OK;
   hipLaunchKernelGGL(k_batch_direction, dim3(b->nchunks), dim3(kThreads), 0, h->stream, to_dev(b), p, z, gamma_new,
-                     gamma_old, active, first);
+                     gamma_old, active, first, (const double*)nullptr);
   return check_launch("feat_batch_pcg_direction_f64");
 }
 
@@ -827,8 +848,55 @@ int feat_batch_status_f64(feat_handle_t h, uint32_t ninst, const double* rr, con
      !st->nactive)
     return fail(FEAT_ERR_INVALID, "feat_batch_status_f64: null argument");
   StopDev c{stop->tol_rel, stop->tol_abs, stop->div_rel, stop->div_abs, stop->min_iter, stop->max_iter};
-  hipLaunchKernelGGL(k_batch_status, dim3(1), dim3(kThreads), 0, h->stream, ninst, rr, c, *st, initial);
+  hipLaunchKernelGGL(k_batch_status, dim3(1), dim3(kThreads), 0, h->stream, ninst, rr, c, *st, initial, BatchDev{},
+                     (const double*)nullptr, (double*)nullptr, (const double*)nullptr);
   return check_launch("feat_batch_status_f64");
+}
+
+int feat_batch_dot_partials_f64(feat_handle_t h, const feat_batch_t* b, const double* x, const double* y, double* part)
+{
+  if(!h || !batch_ok(b) || !x || !y || !part) return fail(FEAT_ERR_INVALID, "feat_batch_dot_partials_f64: null argument");
+  if(!b->nchunks) return FEAT_OK;
+  hipLaunchKernelGGL(k_batch_dot, dim3(b->nchunks), dim3(kThreads), 0, h->stream, to_dev(b), x, y, part);
+  return check_launch("feat_batch_dot_partials_f64");
+}
+
+int feat_batch_pcg_update_partials_f64(feat_handle_t h, const feat_batch_t* b, double* x, double* r, const double* p,
+                                       const double* t, const double* g_part, const double* tp_part,
+                                       const int* active, double* rr_part)
+{
+  if(!h || !batch_ok(b) || !x || !r || !p || !t || !g_part || !tp_part || !active || !rr_part)
+    return fail(FEAT_ERR_INVALID, "feat_batch_pcg_update_partials_f64: null argument");
+  if(rr_part == g_part || rr_part == tp_part)
+    return fail(FEAT_ERR_ALIAS, "feat_batch_pcg_update_partials_f64: rr_part aliases an input");
+  if(!b->nchunks) return FEAT_OK;
+  hipLaunchKernelGGL(k_batch_update, dim3(b->nchunks), dim3(kThreads), 0, h->stream, to_dev(b), x, r, p, t,
+                     (const double*)nullptr, (const double*)nullptr, active, g_part, tp_part, rr_part);
+  return check_launch("feat_batch_pcg_update_partials_f64");
+}
+
+int feat_batch_status_partials_f64(feat_handle_t h, const feat_batch_t* b, const double* rr_part,
+                                   const feat_batch_stop_t* stop, const feat_batch_state_t* st, double* gamma,
+                                   const double* g_part)
+{
+  if(!h || !batch_ok(b) || !rr_part || !stop || !st || !st->def_init || !st->def_cur || !st->iters || !st->status ||
+     !st->active || !st->nactive || (gamma && !g_part))
+    return fail(FEAT_ERR_INVALID, "feat_batch_status_partials_f64: null argument");
+  StopDev c{stop->tol_rel, stop->tol_abs, stop->div_rel, stop->div_abs, stop->min_iter, stop->max_iter};
+  hipLaunchKernelGGL(k_batch_status, dim3(1), dim3(kThreads), 0, h->stream, b->ninst, (const double*)nullptr, c, *st,
+                     0, to_dev(b), rr_part, gamma, g_part);
+  return check_launch("feat_batch_status_partials_f64");
+}
+
+int feat_batch_pcg_direction_partials_f64(feat_handle_t h, const feat_batch_t* b, double* p, const double* z,
+                                          const double* g_part, const double* gamma_old, const int* active)
+{
+  if(!h || !batch_ok(b) || !p || !z || !g_part || !gamma_old || !active)
+    return fail(FEAT_ERR_INVALID, "feat_batch_pcg_direction_partials_f64: null argument");
+  if(!b->nchunks) return FEAT_OK;
+  hipLaunchKernelGGL(k_batch_direction, dim3(b->nchunks), dim3(kThreads), 0, h->stream, to_dev(b), p, z,
+                     (const double*)nullptr, gamma_old, active, 0, g_part);
+  return check_launch("feat_batch_pcg_direction_partials_f64");
 }
 
 int feat_batch_gather_f64(feat_handle_t h, uint32_t nseg, const uint64_t* dst_lo, const uint64_t* src_lo,

@@ -608,6 +608,25 @@ typedef struct {
  * else _set_new_defect(sqrt(rr[i])) for the active ones */
 int feat_batch_status_f64(feat_handle_t h, uint32_t ninst, const double* rr,
                           const feat_batch_stop_t* stop, const feat_batch_state_t* st, int initial);
+/* The same steps with the per-instance sums left as chunk partials (nchunks doubles
+ * each) and summed by the kernel that uses them, in ascending chunk order, so every
+ * value equals the one of the entry points above (one launch fewer per sum):
+ * dot: part[c] = chunk c's partial of x . y;
+ * update: gamma = instance sum of g_part, tp = of tp_part, the r.r partials into rr_part;
+ * status (not initial): rr = instance sum of rr_part; first, when gamma is not NULL, for
+ * the instances active on entry gamma[i] = instance sum of g_part (the copy of
+ * feat_batch_copy_scalars_f64 after the previous direction);
+ * direction (not first): gamma_new = instance sum of g_part. */
+int feat_batch_dot_partials_f64(feat_handle_t h, const feat_batch_t* b, const double* x, const double* y,
+                                double* part);
+int feat_batch_pcg_update_partials_f64(feat_handle_t h, const feat_batch_t* b, double* x, double* r,
+                                       const double* p, const double* t, const double* g_part,
+                                       const double* tp_part, const int* active, double* rr_part);
+int feat_batch_status_partials_f64(feat_handle_t h, const feat_batch_t* b, const double* rr_part,
+                                   const feat_batch_stop_t* stop, const feat_batch_state_t* st,
+                                   double* gamma, const double* g_part);
+int feat_batch_pcg_direction_partials_f64(feat_handle_t h, const feat_batch_t* b, double* p, const double* z,
+                                          const double* g_part, const double* gamma_old, const int* active);
 /* copies segments dst[dst_lo[c] + k] = src[src_lo[c] + k], k < len[c], c < nseg
  * (between the PCG layout and a multigrid level's layout) */
 int feat_batch_gather_f64(feat_handle_t h, uint32_t nseg, const uint64_t* dst_lo,

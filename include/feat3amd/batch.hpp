@@ -131,6 +131,7 @@ namespace feat3amd
       Segs _segs;
       Vec _r, _p, _t, _z;
       DeviceArray<double> _gamma, _gold, _tp, _rr, _dinit, _dcur, _hist;
+      DeviceArray<double> _ptp, _prr;  // chunk partials of t.p and r.r
       DeviceArray<std::uint64_t> _iters;
       DeviceArray<int> _status, _active, _nactive;
       FusedWorkspace _fw;
@@ -444,6 +445,8 @@ namespace feat3amd
         _A = Csr::from_host(top.rows, top.cols, top.val.data(), top.col.data(), top.rowptr.data());
         _mask = DeviceArray<std::uint32_t>::from_host(unit_mask(top.rows, topfilt).data(), (top.rows + 31) / 32);
         _segs.build(_off);
+        _ptp = DeviceArray<double>(_segs.nchunks);
+        _prr = DeviceArray<double>(_segs.nchunks);
         // multigrid levels 1.._lmax
         _lv.clear();
         _lv.resize(_lmax + 1);
@@ -720,7 +723,8 @@ namespace feat3amd
         return feat_batch_stop_t{_cfg.tol_rel, 1.0 / (e * e), 1.0 / e, 1.0 / (e * e), 0, _cfg.max_iter};
       }
 
-      // t = A p, tp, update, status, nactive -> host, z = MG(r), gamma_new, direction, gamma_old = gamma
+      // t = A p, t.p partials, update (gamma and t.p summed from their partials), status (and
+      // gamma_old = gamma), nactive -> host, z = MG(r), r.z partials, direction
       void body(double* x, feat_handle_t h, int* host_nactive)
       {
         const feat_reduce_ws_t* ws = &_fw.ws;
@@ -728,18 +732,20 @@ namespace feat3amd
         const feat_batch_stop_t sp = stop();
         const feat_batch_state_t st = state();
         const feat_matrix_f64_t& A = _A.exec();
+        // the r.z partials (gamma of this iteration) are in the batch's own partials
+        // buffer: written by the previous iteration's (or the start's) r.z dot
+        double* gz = _segs.partials.get();
         check(feat_apply_masked_f64(h, &A, _t.elements(), _p.elements(), _mask.get(), nullptr, nullptr, nullptr, ws), "batch A*p");
-        check(feat_batch_dot_f64(h, b, _t.elements(), _p.elements(), _tp.get()), "batch t.p");
-        check(feat_batch_pcg_update_f64(h, b, x, _r.elements(), _p.elements(), _t.elements(), _gamma.get(), _tp.get(),
-                                        _active.get(), _rr.get()),
+        check(feat_batch_dot_partials_f64(h, b, _t.elements(), _p.elements(), _ptp.get()), "batch t.p");
+        check(feat_batch_pcg_update_partials_f64(h, b, x, _r.elements(), _p.elements(), _t.elements(), gz, _ptp.get(),
+                                                 _active.get(), _prr.get()),
               "batch update");
-        check(feat_batch_status_f64(h, _segs.ninst, _rr.get(), &sp, &st, 0), "batch status");
+        check(feat_batch_status_partials_f64(h, b, _prr.get(), &sp, &st, _gamma.get(), gz), "batch status");
         check(feat_copy_int_to_host(h, _nactive.get(), host_nactive), "batch nactive");
         mg_apply(_z.elements(), _r.elements());
-        check(feat_batch_dot_f64(h, b, _r.elements(), _z.elements(), _tp.get()), "batch gamma");
-        check(feat_batch_pcg_direction_f64(h, b, _p.elements(), _z.elements(), _tp.get(), _gamma.get(), _active.get(), 0),
+        check(feat_batch_dot_partials_f64(h, b, _r.elements(), _z.elements(), gz), "batch gamma");
+        check(feat_batch_pcg_direction_partials_f64(h, b, _p.elements(), _z.elements(), gz, _gamma.get(), _active.get()),
               "batch direction");
-        check(feat_batch_copy_scalars_f64(h, _segs.ninst, _gamma.get(), _tp.get(), _active.get()), "batch gamma copy");
       }
 
     public:
