@@ -260,7 +260,7 @@ HOST_API = [
     ("feat_batchsolver_layout", _i, [_vp, _vp, _vp]),
     ("feat_batchsolver_solve", _i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     ("feat_batchsolver_tuning", _i, [_vp, _vp, _vp, _i, C.POINTER(_i)]),
-    ("feat_batchsolver_tail_levels", _i, [_vp, C.POINTER(_i)]),
+    ("feat_batchsolver_tail_levels", _i, [_vp, C.POINTER(_i), C.POINTER(_i)]),
     ("feat_batchsolver_tiled_levels", _i, [_vp, _vp, _i, C.POINTER(_i)]),
     ("feat_batchsolver_destroy", _i, [_vp]),
     ("feat_batch_levels", _i, [_u64, _i, _i, _i, _vp]),
@@ -799,7 +799,7 @@ class Stokes:
 class BatchCfg(C.Structure):
     _fields_ = [("seed", _u64), ("ninst", _i), ("lmin", _i), ("lmax", _i), ("sigma", _d), ("tol_rel", _d),
                 ("max_iter", _u64), ("autotune", _i), ("graphs", _i), ("tail_levels", _i),
-                ("no_tiled", _i), ("tiled_prol_arrays", _i)]
+                ("no_tiled", _i), ("tiled_prol_arrays", _i), ("tail_ell", _i)]
 
 
 def batch_levels(seed, ninst, lmin, lmax):
@@ -878,12 +878,12 @@ class BatchSolver:
     """The batch of independent Q1 diffusion MG-PCG solves (C++ host layer)."""
 
     def __init__(self, seed=28054777172512, ninst=64, lmin=3, lmax=8, sigma=1.0, tol_rel=1e-8, max_iter=100,
-                 autotune=True, graphs=True, tail_levels=5, tiled=True, tiled_prol_arrays=False):
+                 autotune=True, graphs=True, tail_levels=5, tiled=True, tiled_prol_arrays=False, tail_ell=False):
         _, self.hl = libs()
         runtime_init()
         self.s = C.c_void_p()
         self.cfg = BatchCfg(seed, ninst, lmin, lmax, sigma, tol_rel, max_iter, int(autotune), int(graphs),
-                            int(tail_levels), int(not tiled), int(tiled_prol_arrays))
+                            int(tail_levels), int(not tiled), int(tiled_prol_arrays), int(tail_ell))
         check(self.hl.feat_batchsolver_create(C.byref(self.s), C.byref(self.cfg)), "feat_batchsolver_create")
         self.levels = np.zeros(ninst, np.int32)
         self.off = np.zeros(ninst + 1, np.uint64)
@@ -908,9 +908,15 @@ class BatchSolver:
 
     @property
     def tail_levels(self):
-        d = C.c_int()
-        check(self.hl.feat_batchsolver_tail_levels(self.s, C.byref(d)), "feat_batchsolver_tail_levels")
+        d, g = C.c_int(), C.c_int()
+        check(self.hl.feat_batchsolver_tail_levels(self.s, C.byref(d), C.byref(g)), "feat_batchsolver_tail_levels")
         return d.value
+
+    @property
+    def tail_grid(self):
+        d, g = C.c_int(), C.c_int()
+        check(self.hl.feat_batchsolver_tail_levels(self.s, C.byref(d), C.byref(g)), "feat_batchsolver_tail_levels")
+        return bool(g.value)
 
     @property
     def tiled_levels(self):

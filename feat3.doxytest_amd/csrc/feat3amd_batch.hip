@@ -518,6 +518,304 @@ __global__ __launch_bounds__(kBt) void k_batch_tail(feat_batch_tail_t Targ, cons
 }
 
 // ---------------------------------------------------------------------------
+// Grid form of the coarse tail: every level a structured n1 x n1 grid per instance, A as
+// the nine stencil values of the free rows, P bilinear injection and R its transpose
+// (formed from the node positions).  Every matrix value is read once at kernel start:
+// the top level's rows into registers, the levels below into LDS, so the V-cycle itself
+// touches LDS only.  A filtered row never uses its row sum (its defect and correction
+// are zeroed by every kernel it stands for), so only free (interior) rows are summed;
+// each sum is the sequential fma over the row's entries in CSR order, as everywhere.
+__device__ __forceinline__ double grid9(const double* v, size_t stride, const double* x, int j, int n1)
+{
+  double s = 0.0;
+  s = dfma(v[0 * stride], x[j - n1 - 1], s);
+  s = dfma(v[1 * stride], x[j - n1], s);
+  s = dfma(v[2 * stride], x[j - n1 + 1], s);
+  s = dfma(v[3 * stride], x[j - 1], s);
+  s = dfma(v[4 * stride], x[j], s);
+  s = dfma(v[5 * stride], x[j + 1], s);
+  s = dfma(v[6 * stride], x[j + n1 - 1], s);
+  s = dfma(v[7 * stride], x[j + n1], s);
+  s = dfma(v[8 * stride], x[j + n1 + 1], s);
+  return s;
+}
+
+// P row (x, y) of the fine grid (n1 per side) over the coarse grid xc: bilinear injection
+__device__ __forceinline__ double grid_prol(const double* xc, int j, int n1)
+{
+  const int nc1 = (n1 + 1) / 2, gx = j % n1, gy = j / n1;
+  const double* xr = xc + (gy >> 1) * nc1 + (gx >> 1);
+  const bool ox = gx & 1, oy = gy & 1;
+  if(!ox && !oy) return dfma(1.0, xr[0], 0.0);
+  if(!oy) return dfma(0.5, xr[1], dfma(0.5, xr[0], 0.0));
+  if(!ox) return dfma(0.5, xr[nc1], dfma(0.5, xr[0], 0.0));
+  return dfma(0.25, xr[nc1 + 1], dfma(0.25, xr[nc1], dfma(0.25, xr[1], dfma(0.25, xr[0], 0.0))));
+}
+
+// R row (X, Y) of the coarse grid (nc1 per side) over the fine grid d (n1 = 2 nc1 - 1):
+// the fine nodes (2X + dx, 2Y + dy) inside the grid, ascending, weights w(dx) w(dy)
+__device__ __forceinline__ double grid_rest(const double* d, int jc, int n1)
+{
+  const int nc1 = (n1 + 1) / 2, X = jc % nc1, Y = jc / nc1;
+  double s = 0.0;
+#pragma unroll
+  for(int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+    for(int dx = -1; dx <= 1; ++dx)
+    {
+      const int fx = 2 * X + dx, fy = 2 * Y + dy;
+      if(fx >= 0 && fx < n1 && fy >= 0 && fy < n1)
+        s = dfma((dx ? 0.5 : 1.0) * (dy ? 0.5 : 1.0), d[fy * n1 + fx], s);
+    }
+  return s;
+}
+
+__global__ __launch_bounds__(kBt) void k_batch_tail_grid(feat_batch_tail_t Targ, const double* __restrict__ r,
+                                                         double* __restrict__ z)
+{
+  __shared__ feat_batch_tail_t T;
+  __shared__ uint64_t s_loff[FEAT_BATCH_TAIL_MAX_LEVELS + 1];
+  __shared__ uint32_t s_base[FEAT_BATCH_TAIL_MAX_LEVELS + 1];
+  __shared__ double s_rhs[FEAT_BATCH_TAIL_MAX_TOTAL], s_sol[FEAT_BATCH_TAIL_MAX_TOTAL];
+  __shared__ double s_inv[FEAT_BATCH_TAIL_MAX_TOTAL];
+  __shared__ uint8_t s_m[FEAT_BATCH_TAIL_MAX_TOTAL];
+  __shared__ double s_b0[FEAT_BATCH_TAIL_MAX_ROWS], s_b1[FEAT_BATCH_TAIL_MAX_ROWS], s_d[FEAT_BATCH_TAIL_MAX_ROWS];
+  // stencil values of the levels below the top: level k's value u of row j at 9 s_base[k] + u n + j
+  __shared__ double s_a[9 * FEAT_BATCH_TAIL_MAX_BELOW];
+  const uint32_t inst = blockIdx.x;
+  const int fin = Targ.finest[inst];
+  const int top = fin < Targ.nlevels ? fin : Targ.nlevels;
+  const bool deep = fin > Targ.nlevels;
+  if(threadIdx.x == 0)
+  {
+    T = Targ;
+    uint32_t o = 0;
+    for(int k = 1; k <= top; ++k)
+    {
+      s_loff[k] = Targ.loff[uint64_t(k - 1) * Targ.ninst + inst];
+      s_base[k] = o;
+      o += Targ.lv[k - 1].n;
+    }
+  }
+  __syncthreads();
+  const int tid = threadIdx.x;
+  const double omega = T.omega;
+  // every read of the matrices, the inverse diagonals, the filters and the rhs, issued together
+  double at[kBtRows][9];
+  {
+    const feat_batch_tail_level_t& L = T.lv[top - 1];
+    const double* sv = L.sval + s_loff[top] * 9;
+#pragma unroll
+    for(int q = 0; q < kBtRows; ++q)
+    {
+      const uint32_t j = tid + q * kBt;
+#pragma unroll
+      for(int u = 0; u < 9; ++u) at[q][u] = j < L.n ? sv[uint64_t(u) * L.n + j] : 0.0;
+    }
+  }
+  for(int k = 1; k <= top; ++k)
+  {
+    const feat_batch_tail_level_t& L = T.lv[k - 1];
+    const uint64_t g0 = s_loff[k];
+    const uint32_t b = s_base[k];
+    for(uint32_t j = tid; j < L.n; j += kBt)
+    {
+      s_inv[b + j] = L.inv[g0 + j];
+      s_m[b + j] = is_masked(L.mask, g0 + j) ? 1 : 0;
+    }
+    if(k < top)
+    {
+      const double* sv = L.sval + g0 * 9;
+      for(uint32_t e = tid; e < 9 * L.n; e += kBt) s_a[9 * b + e] = sv[e];
+    }
+  }
+  {
+    const uint32_t n = T.lv[top - 1].n, b = s_base[top];
+    for(uint32_t j = tid; j < n; j += kBt)
+      s_rhs[b + j] = deep ? T.rhs_deep[s_loff[top] + j] : r[T.pcg_off[inst] + j];
+  }
+  __syncthreads();
+
+  // row sum of A on level k (row j = tid + q kBt), free rows only
+  auto asum = [&](int k, int q, const double* x) -> double {
+    const feat_batch_tail_level_t& L = T.lv[k - 1];
+    const int j = tid + q * kBt, n1 = int(L.n1);
+    if(k == top) return grid9(at[q], 1, x, j, n1);
+    return grid9(s_a + 9 * s_base[k] + j, L.n, x, j, n1);
+  };
+
+  // pre-smoothing from 0 with the defect of the last iterate into s_d, then the restriction
+  auto pre_restrict = [&](int k) {
+    const feat_batch_tail_level_t& L = T.lv[k - 1];
+    const uint32_t n = L.n, b = s_base[k];
+    const uint32_t nc = T.lv[k - 2].n, bc = s_base[k - 1];
+    const double* rhs = s_rhs + b;
+    const double* inv = s_inv + b;
+    const uint8_t* m = s_m + b;
+    double* x = s_sol + b;
+    const int nu = T.nu_pre;
+    auto buf = [&](int s) { return ((nu - s) & 1) ? s_b1 : x; };
+    {
+      double* x1 = buf(1);
+#pragma unroll
+      for(int q = 0; q < kBtRows; ++q)
+      {
+        const uint32_t j = tid + q * kBt;
+        if(j < n)
+        {
+          double c = inv[j] * rhs[j];
+          if(m[j]) c = 0.0;
+          x1[j] = dfma(omega, c, 0.0);
+        }
+      }
+    }
+    for(int s = 1; s <= nu; ++s)
+    {
+      __syncthreads();
+      const double* xk = buf(s);
+      double* xo = buf(s + 1);
+#pragma unroll
+      for(int q = 0; q < kBtRows; ++q)
+      {
+        const uint32_t j = tid + q * kBt;
+        if(j < n)
+        {
+          double d = 0.0;
+          if(!m[j])
+            d = dfma(asum(k, q, xk), -1.0, rhs[j]);
+          if(s == nu)
+            s_d[j] = d;
+          else
+          {
+            double c = inv[j] * d;
+            if(m[j]) c = 0.0;
+            xo[j] = dfma(omega, c, xk[j]);
+          }
+        }
+      }
+    }
+    __syncthreads();
+    if(uint32_t(tid) < nc)
+    {
+      double v = 0.0;
+      if(!s_m[bc + tid]) v = dfma(grid_rest(s_d, tid, int(L.n1)), 1.0, 0.0);
+      s_rhs[bc + tid] = v;
+    }
+    __syncthreads();
+  };
+
+  // sol += filter(P sol of level k-1), then s_d = filter(rhs - A sol)
+  auto prol_defect = [&](int k) {
+    const feat_batch_tail_level_t& L = T.lv[k - 1];
+    const uint32_t n = L.n, b = s_base[k];
+    const double* xc = s_sol + s_base[k - 1];
+    double* x = s_sol + b;
+    const uint8_t* m = s_m + b;
+#pragma unroll
+    for(int q = 0; q < kBtRows; ++q)
+    {
+      const uint32_t j = tid + q * kBt;
+      if(j < n)
+      {
+        double c = 0.0;
+        if(!m[j]) c = dfma(grid_prol(xc, int(j), int(L.n1)), 1.0, 0.0);
+        x[j] = dfma(1.0, c, x[j]);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for(int q = 0; q < kBtRows; ++q)
+    {
+      const uint32_t j = tid + q * kBt;
+      if(j < n)
+      {
+        double d = 0.0;
+        if(!m[j]) d = dfma(asum(k, q, x), -1.0, s_rhs[b + j]);
+        s_d[j] = d;
+      }
+    }
+    __syncthreads();
+  };
+
+  // post-smoothing on s_d from 0, its last iterate added to sol
+  auto post = [&](int k) {
+    const uint32_t n = T.lv[k - 1].n, b = s_base[k];
+    const double* inv = s_inv + b;
+    const uint8_t* m = s_m + b;
+    double* x = s_sol + b;
+    const int nu = T.nu_post;
+    auto buf = [&](int s) { return ((nu - s) & 1) ? s_b1 : s_b0; };
+    {
+      double* x1 = buf(1);
+#pragma unroll
+      for(int q = 0; q < kBtRows; ++q)
+      {
+        const uint32_t j = tid + q * kBt;
+        if(j < n)
+        {
+          double c = inv[j] * s_d[j];
+          if(m[j]) c = 0.0;
+          const double xn = dfma(omega, c, 0.0);
+          if(nu == 1)
+            x[j] = dfma(1.0, xn, x[j]);
+          else
+            x1[j] = xn;
+        }
+      }
+    }
+    for(int s = 1; s <= nu - 1; ++s)
+    {
+      __syncthreads();
+      const double* xk = buf(s);
+      double* xo = buf(s + 1);
+#pragma unroll
+      for(int q = 0; q < kBtRows; ++q)
+      {
+        const uint32_t j = tid + q * kBt;
+        if(j < n)
+        {
+          double c = 0.0;
+          if(!m[j]) c = inv[j] * dfma(asum(k, q, xk), -1.0, s_d[j]);
+          const double xn = dfma(omega, c, xk[j]);
+          if(s == nu - 1)
+            x[j] = dfma(1.0, xn, x[j]);
+          else
+            xo[j] = xn;
+        }
+      }
+    }
+    __syncthreads();
+  };
+
+  for(int k = top; k >= 2; --k) pre_restrict(k);
+  {
+    const uint32_t n = T.lv[0].n, b = s_base[1];
+    for(uint32_t j = tid; j < n; j += kBt)
+    {
+      double c = s_inv[b + j] * s_rhs[b + j];
+      if(s_m[b + j]) c = 0.0;
+      s_sol[b + j] = dfma(1.0, c, 0.0);
+    }
+    __syncthreads();
+  }
+  for(int k = 2; k <= top; ++k)
+  {
+    prol_defect(k);
+    post(k);
+  }
+  {
+    const uint32_t n = T.lv[top - 1].n, b = s_base[top];
+    for(uint32_t j = tid; j < n; j += kBt)
+    {
+      if(deep)
+        T.sol_deep[s_loff[top] + j] = s_sol[b + j];
+      else
+        z[T.pcg_off[inst] + j] = s_sol[b + j];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Tiled smoother calls of a batched level on structured grids.  A workgroup owns a
 // kTlTile x kTlTile tile of one instance's n1 x n1 grid and works on the tile plus a halo of
 // kTlHalo nodes (clipped to the grid), kTlPts region nodes per thread with their nine
@@ -922,12 +1220,23 @@ int feat_batch_tail_f64(feat_handle_t h, const feat_batch_tail_t* t, const doubl
   {
     const feat_batch_tail_level_t& L = t->lv[k - 1];
     const bool transfer = k > 1;
-    if(L.n < 1 || !L.inv || !L.mask || !L.a_val || !L.a_col || !L.a_len ||
-       (transfer && (!L.p_val || !L.p_col || !L.p_len || !L.r_val || !L.r_col || !L.r_len)))
-      return fail(FEAT_ERR_INVALID, "feat_batch_tail_f64: level descriptor incomplete");
-    if(L.wa > FEAT_BATCH_TAIL_MAX_ROW_LEN ||
-       (transfer && (L.wr > FEAT_BATCH_TAIL_MAX_ROW_LEN || L.wp > FEAT_BATCH_TAIL_MAX_PROL_LEN)))
-      return fail(FEAT_ERR_INVALID, "feat_batch_tail_f64: ELL width above the limit");
+    if(t->grid)
+    {
+      if(L.n < 1 || !L.inv || !L.mask || !L.sval || L.n1 < 3 || !(L.n1 & 1) || uint64_t(L.n1) * L.n1 != L.n ||
+         (transfer && t->lv[k - 2].n1 != (L.n1 + 1) / 2))
+        return fail(FEAT_ERR_INVALID, "feat_batch_tail_f64: grid level descriptor inconsistent");
+      if(k == nl && total > FEAT_BATCH_TAIL_MAX_BELOW)
+        return fail(FEAT_ERR_INVALID, "feat_batch_tail_f64: levels below the top exceed the LDS stencil store");
+    }
+    else
+    {
+      if(L.n < 1 || !L.inv || !L.mask || !L.a_val || !L.a_col || !L.a_len ||
+         (transfer && (!L.p_val || !L.p_col || !L.p_len || !L.r_val || !L.r_col || !L.r_len)))
+        return fail(FEAT_ERR_INVALID, "feat_batch_tail_f64: level descriptor incomplete");
+      if(L.wa > FEAT_BATCH_TAIL_MAX_ROW_LEN ||
+         (transfer && (L.wr > FEAT_BATCH_TAIL_MAX_ROW_LEN || L.wp > FEAT_BATCH_TAIL_MAX_PROL_LEN)))
+        return fail(FEAT_ERR_INVALID, "feat_batch_tail_f64: ELL width above the limit");
+    }
     total += L.n;
     // the restriction to a level gives one row to each thread
     if(L.n > FEAT_BATCH_TAIL_MAX_ROWS || (k < nl && L.n > uint64_t(kBt))) total = FEAT_BATCH_TAIL_MAX_TOTAL + 1;
@@ -938,7 +1247,10 @@ int feat_batch_tail_f64(feat_handle_t h, const feat_batch_tail_t* t, const doubl
   if(!r || !z || !t->rhs_deep || !t->sol_deep) return fail(FEAT_ERR_INVALID, "feat_batch_tail_f64: null vector");
   if(static_cast<const void*>(r) == z || t->rhs_deep == t->sol_deep)
     return fail(FEAT_ERR_ALIAS, "feat_batch_tail_f64: input and output alias");
-  hipLaunchKernelGGL(k_batch_tail, dim3(t->ninst), dim3(kBt), 0, h->stream, *t, r, z);
+  if(t->grid)
+    hipLaunchKernelGGL(k_batch_tail_grid, dim3(t->ninst), dim3(kBt), 0, h->stream, *t, r, z);
+  else
+    hipLaunchKernelGGL(k_batch_tail, dim3(t->ninst), dim3(kBt), 0, h->stream, *t, r, z);
   return check_launch(name);
 }
 

@@ -793,6 +793,7 @@ int feat_batchsolver_create(feat_batchsolver_t* out, const feat_batch_cfg_t* cfg
     c.tail_levels = cfg->tail_levels > 0 ? cfg->tail_levels : 0;
     c.tiled = cfg->no_tiled == 0;
     c.tiled_prol_arrays = cfg->tiled_prol_arrays != 0;
+    c.tail_ell = cfg->tail_ell != 0;
     auto s = std::make_unique<feat_batchsolver_s>();
     s->b = std::make_unique<Solver::BatchedMgPcg>(c);
     s->b->init();
@@ -826,11 +827,12 @@ int feat_batchsolver_solve(feat_batchsolver_t s, double* x, const double* b, int
   });
 }
 
-int feat_batchsolver_tail_levels(feat_batchsolver_t s, int* levels)
+int feat_batchsolver_tail_levels(feat_batchsolver_t s, int* levels, int* grid)
 {
   return guarded([&]() {
     if(!s || !levels) throw Error("feat_batchsolver_tail_levels: null argument");
     *levels = s->b->tail_depth();
+    if(grid) *grid = s->b->tail_grid() ? 1 : 0;
   });
 }
 

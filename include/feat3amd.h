@@ -663,8 +663,11 @@ int feat_copy_int_to_host(feat_handle_t h, const int* dev, int* host_pinned);
 #define FEAT_BATCH_TAIL_MAX_TOTAL 1536
 #define FEAT_BATCH_TAIL_MAX_ROW_LEN 9
 #define FEAT_BATCH_TAIL_MAX_PROL_LEN 4
+#define FEAT_BATCH_TAIL_MAX_BELOW 448   /* grid form: rows of one instance below level T, summed */
 typedef struct {
   uint32_t n;                  /* rows of one instance on this level */
+  uint32_t n1;                 /* grid form: nodes per side (n = n1 * n1) */
+  const double* sval;          /* grid form: 9-point stencil values, as feat_batch_tiled_t */
   uint32_t wa, wp, wr;         /* ELL widths of A, P, R (P, R unused on level 1) */
   const double* a_val;
   const uint16_t* a_col;
@@ -688,6 +691,12 @@ typedef struct {
   const uint64_t* pcg_off;     /* per instance */
   const double* rhs_deep;
   double* sol_deep;
+  /* 1: grid form.  Every level is a structured n1 x n1 grid per instance whose free rows
+   * hold the 9-point stencil (values in sval), P is bilinear injection from level k-1
+   * and R its transpose, with the weights 1, 1/2, 1/4 in CSR order (checked by the
+   * caller).  The ELL arrays are not read, every matrix is read once at kernel start,
+   * and the levels below T hold at most FEAT_BATCH_TAIL_MAX_BELOW rows in all. */
+  int grid;
   feat_batch_tail_level_t lv[FEAT_BATCH_TAIL_MAX_LEVELS];
 } feat_batch_tail_t;
 int feat_batch_tail_f64(feat_handle_t h, const feat_batch_tail_t* t, const double* r, double* z);

@@ -52,6 +52,8 @@ namespace feat3amd
       bool tiled = true;
       /// the tiled post-smoothing reads the prolongation's arrays even where it is bilinear injection
       bool tiled_prol_arrays = false;
+      /// the coarse tail in ELL form even where its levels are structured grids
+      bool tail_ell = false;
     };
 
     class BatchedMgPcg
@@ -163,6 +165,12 @@ namespace feat3amd
       struct TailLevel
       {
         Ell A, P, R;
+        // grid form: nodes per side, 9-point stencil values (empty: not a 9-point grid),
+        // transfers bilinear injection and its transpose
+        std::uint32_t n1 = 0;
+        std::vector<double> sv;
+        bool bilinear = false;
+        DeviceArray<double> dsv;
       };
       feat_batch_tail_t _tail{};
       std::vector<TailLevel> _tl;  // index k (1..T); 0 unused
@@ -197,6 +205,33 @@ namespace feat3amd
           for(std::uint64_t iy = 0; iy <= oy; ++iy)
             for(std::uint64_t ix = 0; ix <= ox; ++ix, ++u)
               if(P1.col[e0 + u] != (y / 2 + iy) * nc1 + (x / 2 + ix) || P1.val[e0 + u] != w) return false;
+        }
+        return true;
+      }
+
+      // true when R1 (rows: the coarse grid, nc1 per side; columns: the fine grid, n1 per side)
+      // is the transpose of bilinear injection: row (X, Y) holds the fine nodes
+      // (2X + dx, 2Y + dy), dx, dy in {-1, 0, 1}, inside the grid, ascending, weights w(dx) w(dy)
+      // with w(0) = 1, w(+-1) = 1/2
+      static bool bilinear_restriction(const Assembly::CsrHost& R1, std::uint64_t n1)
+      {
+        const std::int64_t nc1 = std::int64_t(n1 + 1) / 2, w1 = std::int64_t(n1);
+        if(R1.rows != std::uint64_t(nc1 * nc1) || R1.rowptr.size() != R1.rows + 1) return false;
+        for(std::int64_t r = 0; r < nc1 * nc1; ++r)
+        {
+          const std::int64_t X = r % nc1, Y = r / nc1;
+          std::uint64_t e = R1.rowptr[r];
+          for(std::int64_t dy = -1; dy <= 1; ++dy)
+            for(std::int64_t dx = -1; dx <= 1; ++dx)
+            {
+              const std::int64_t fx = 2 * X + dx, fy = 2 * Y + dy;
+              if(fx < 0 || fx >= w1 || fy < 0 || fy >= w1) continue;
+              if(e >= R1.rowptr[r + 1] || std::int64_t(R1.col[e]) != fy * w1 + fx ||
+                 R1.val[e] != (dx ? 0.5 : 1.0) * (dy ? 0.5 : 1.0))
+                return false;
+              ++e;
+            }
+          if(e != R1.rowptr[r + 1]) return false;
         }
         return true;
       }
@@ -329,10 +364,27 @@ namespace feat3amd
         _tail.pcg_off = _tpcg.get();
         _tail.rhs_deep = _lv[T]->rhs.elements();
         _tail.sol_deep = _lv[T]->sol.elements();
+        {
+          bool grid = !_cfg.tail_ell;
+          std::uint64_t below = 0;
+          for(int k = 1; k <= T; ++k)
+          {
+            grid = grid && !_tl[k].sv.empty() && _tl[k].bilinear;
+            if(k < T) below += _tl[k].A.n;
+          }
+          _tail.grid = (grid && below <= FEAT_BATCH_TAIL_MAX_BELOW) ? 1 : 0;
+        }
         for(int k = 1; k <= T; ++k)
         {
           TailLevel& t = _tl[k];
           feat_batch_tail_level_t& d = _tail.lv[k - 1];
+          if(_tail.grid)
+          {
+            t.dsv = DeviceArray<double>::from_host(t.sv.data(), t.sv.size());
+            d.n1 = t.n1;
+            d.sval = t.dsv.get();
+          }
+          t.sv.clear();
           t.A.upload();
           d.n = t.A.n;
           d.wa = t.A.w;
@@ -401,6 +453,8 @@ namespace feat3amd
       const std::vector<double>& tuned_us() const { return _tune_us; }
       /// levels 1..tail_depth() run in one launch per V-cycle (0: none)
       int tail_depth() const { return _tail.nlevels; }
+      /// the coarse tail runs in its grid form (structured levels, matrices prefetched)
+      bool tail_grid() const { return _tail.grid != 0; }
       std::vector<int> tunes() const
       {
         std::vector<int> t;
@@ -481,7 +535,12 @@ namespace feat3amd
           }
           Ak.rows = Ak.cols = o;
           if(k <= tcap)
+          {
             _tl[k].A = to_ell(Ak, L->loff, L->loff, nk, nk, false);
+            _tl[k].n1 = std::uint32_t(Assembly::q1_nodes_1d(k));
+            _tl[k].sv = stencil_values(Ak, L->inst.size(), _tl[k].n1, bnd);
+            _tl[k].bilinear = (k == 1);
+          }
           const bool tiled_k = _cfg.tiled && k > tcap && k > 1 && _cfg.nu_pre <= FEAT_BATCH_TILED_MAX_STEPS &&
                                _cfg.nu_post <= FEAT_BATCH_TILED_MAX_STEPS;
           if(tiled_k)
@@ -522,6 +581,8 @@ namespace feat3amd
               const std::uint64_t nc = C.A.rows() / C.inst.size();
               _tl[k].P = to_ell(Pk, L->loff, cl, nk, nc, true);
               _tl[k].R = to_ell(Rk, cl, L->loff, nc, nk, true);
+              _tl[k].bilinear = bilinear_prolongation(P1, _tl[k].n1) &&
+                                bilinear_restriction(Assembly::transpose(P1), _tl[k].n1);
             }
             if(_tiled[k].n1)
             {

@@ -221,6 +221,7 @@ typedef struct {
                        the tiled structured-grid kernels (feat_batch_pre/post_tiled_f64) */
   int tiled_prol_arrays;  /* 1: the tiled post-smoothing reads the prolongation's ELL arrays
                              even where it is bilinear injection (formed from the node position) */
+  int tail_ell;           /* 1: the coarse tail in ELL form even where its levels are structured grids */
 } feat_batch_cfg_t;
 typedef struct feat_batchsolver_s* feat_batchsolver_t;
 int feat_batchsolver_create(feat_batchsolver_t* s, const feat_batch_cfg_t* cfg);
@@ -233,8 +234,9 @@ int feat_batchsolver_solve(feat_batchsolver_t s, double* x, const double* b, int
 /* chosen launch shapes and their SpMV times (us) of the batch matrices:
  * per level k = 1..lmax: A_k (and P_k, R_k for k > 1), then the PCG-layout matrix */
 int feat_batchsolver_tuning(feat_batchsolver_t s, int* tunes, double* us, int cap, int* count);
-/* levels run by the fused coarse tail (0: none) */
-int feat_batchsolver_tail_levels(feat_batchsolver_t s, int* levels);
+/* levels run by the fused coarse tail (0: none); *grid (may be NULL): 1 when it runs in its
+ * grid form (feat_batch_tail_t.grid) */
+int feat_batchsolver_tail_levels(feat_batchsolver_t s, int* levels, int* grid);
 /* the levels whose smoother calls run tiled: up to cap of them into levels, their number in *count */
 int feat_batchsolver_tiled_levels(feat_batchsolver_t s, int* levels, int cap, int* count);
 int feat_batchsolver_destroy(feat_batchsolver_t s);

@@ -81,18 +81,21 @@ def _solve(torch, bs, b):
     return host(x), r
 
 
+@pytest.mark.parametrize("ell", [False, True])
 @pytest.mark.parametrize("ninst,lmin,lmax,tail,depth", [
     (12, 3, 6, 1, 1),   # only level 1 in the tail: every instance enters it from level 2
     (12, 3, 6, 3, 3),   # instances whose finest level is 3 run entirely inside the tail
     (12, 3, 6, 8, 5),   # lowered to the deepest tail one workgroup holds (level 5: 1089 rows)
     (6, 2, 4, 5, 4),    # the whole hierarchy of every instance in the tail
 ])
-def test_batch_coarse_tail_bitwise(feat, torch_gpu, ninst, lmin, lmax, tail, depth):
+def test_batch_coarse_tail_bitwise(feat, torch_gpu, ninst, lmin, lmax, tail, depth, ell):
+    # the tail in its grid form (stencils, bilinear transfers, matrices prefetched) and in
+    # its ELL form, against the block-diagonal SELL-64 launches
     torch = torch_gpu
     ref = feat.BatchSolver(SEED, ninst=ninst, lmin=lmin, lmax=lmax, tail_levels=0, tiled=False)
-    bs = feat.BatchSolver(SEED, ninst=ninst, lmin=lmin, lmax=lmax, tail_levels=tail, tiled=False)
+    bs = feat.BatchSolver(SEED, ninst=ninst, lmin=lmin, lmax=lmax, tail_levels=tail, tiled=False, tail_ell=ell)
     try:
-        assert ref.tail_levels == 0 and bs.tail_levels == depth
+        assert ref.tail_levels == 0 and bs.tail_levels == depth and bs.tail_grid == (not ell)
         b = dev(torch, batch_rhs(feat, ref))
         x0, r0 = _solve(torch, ref, b)
         for rep in range(2):
@@ -139,7 +142,7 @@ def test_batch_full_64(feat, torch_gpu):
     bs = feat.BatchSolver(SEED, ninst=64, lmin=3, lmax=8)
     ref = feat.BatchSolver(SEED, ninst=64, lmin=3, lmax=8, tail_levels=0, tiled=False)
     try:
-        assert bs.tail_levels == 5 and bs.tiled_levels == [6, 7, 8]
+        assert bs.tail_levels == 5 and bs.tail_grid and bs.tiled_levels == [6, 7, 8]
         b = dev(torch, batch_rhs(feat, bs))
         xg, r = _solve(torch, bs, b)
         x0, r0 = _solve(torch, ref, b)
