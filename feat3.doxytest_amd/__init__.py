@@ -284,10 +284,12 @@ HOST_API = [
     ("feat_mgpcg_matrix", _i, [_vp, _i, C.POINTER(_u64), C.POINTER(_u64), C.POINTER(_vp), C.POINTER(_vp),
                                C.POINTER(_vp)]),
     ("feat_mgpcg_coarse_fusion_level", _i, [_vp, C.POINTER(_i)]),
+    ("feat_mgpcg_coarse_fusion_grid", _i, [_vp, C.POINTER(_i)]),
     ("feat_mgpcg_set_values", _i, [_vp, _i, _vp]),
     ("feat_mgpcg_destroy", _i, [_vp]),
     ("feat_runtime_init", _i, [_i, _vp]),
     ("feat_runtime_sell_flags", _i, [_i]),
+    ("feat_runtime_tail_grid", _i, [_i]),
 ]
 
 
@@ -701,6 +703,12 @@ def runtime_init(device=0, stream=None):
         import torch
         stream = torch.cuda.current_stream(device).cuda_stream
     check(h.feat_runtime_init(device, stream), "feat_runtime_init")
+
+
+def runtime_tail_grid(on):
+    """Fused coarse levels built from now on in their grid form where they qualify (default on)."""
+    _, h = libs()
+    check(h.feat_runtime_tail_grid(int(bool(on))), "feat_runtime_tail_grid")
 
 
 def runtime_sell_flags(flags):
@@ -1138,6 +1146,11 @@ class MgPcg:
         numeric re-initialisation of the hierarchy and the solver tree."""
         v = np.ascontiguousarray(val, np.float64)
         check(self.hl.feat_mgpcg_set_values(self.s, lvl, v.ctypes.data), "feat_mgpcg_set_values")
+
+    def coarse_fusion_grid(self):
+        lv = C.c_int()
+        check(self.hl.feat_mgpcg_coarse_fusion_grid(self.s, C.byref(lv)), "feat_mgpcg_coarse_fusion_grid")
+        return lv.value
 
     def coarse_fusion_level(self):
         """First level of the coarse levels fused into one workgroup (0: none)."""

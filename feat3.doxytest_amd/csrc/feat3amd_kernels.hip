@@ -2732,6 +2732,74 @@ __device__ __forceinline__ DT tail_row_a(const TailRow<DT> R, const int* meta, c
   return sum;
 }
 
+// Grid form of a fused level (dim 0: the plan and the CSR transfers)
+struct TailGeom
+{
+  int dim, n1;
+};
+
+// sum of a free row of an n1^dim grid level whose rows all hold the stencil sten (3^dim values
+// at the offsets (dz, dy, dx) in lexicographic order = CSR order), x in LDS
+template<typename DT>
+__device__ __forceinline__ DT tail_row_grid(const DT* sten, TailGeom g, int i, const DT* x)
+{
+  DT s = DT(0);
+  const int zl = g.dim == 3 ? -1 : 0, zh = g.dim == 3 ? 1 : 0, nn = g.n1 * g.n1;
+  int u = 0;
+  for(int dz = zl; dz <= zh; ++dz)
+#pragma unroll
+    for(int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+      for(int dx = -1; dx <= 1; ++dx, ++u) s = dfma(sten[u], x[i + dz * nn + dy * g.n1 + dx], s);
+  return s;
+}
+
+// the level's row: grid form or plan
+template<typename DT>
+__device__ __forceinline__ DT tail_row(const TailRow<DT>& R, const int* meta, const DT* tab, TailGeom g, uint64_t i,
+                                       const DT* x)
+{
+  return g.dim ? tail_row_grid(tab, g, int(i), x) : tail_row_a(R, meta, tab, i, x);
+}
+
+// row i of the d-linear injection from the next grid ((n1 + 1) / 2 per side) to this one: the
+// coarse nodes around the fine node in ascending order, each weighted with 1/2 per odd axis
+template<typename DT>
+__device__ __forceinline__ DT tail_prol_grid(TailGeom g, int i, const DT* x)
+{
+  const int n1 = g.n1, nc = (n1 + 1) / 2;
+  const int fx = i % n1, fy = (i / n1) % n1, fz = g.dim == 3 ? i / (n1 * n1) : 0;
+  const int ox = fx & 1, oy = fy & 1, oz = fz & 1;
+  const DT w = DT((ox ? 0.5 : 1.0) * (oy ? 0.5 : 1.0) * (oz ? 0.5 : 1.0));
+  DT s = DT(0);
+  for(int iz = 0; iz <= oz; ++iz)
+    for(int iy = 0; iy <= oy; ++iy)
+      for(int ix = 0; ix <= ox; ++ix) s = dfma(w, x[((fz >> 1) + iz) * nc * nc + ((fy >> 1) + iy) * nc + (fx >> 1) + ix], s);
+  return s;
+}
+
+// row jc of its transpose: the fine nodes (2X + dx, 2Y + dy, 2Z + dz) inside the grid in
+// ascending order, weights 1/2 per nonzero offset
+template<typename DT>
+__device__ __forceinline__ DT tail_rest_grid(TailGeom g, int jc, const DT* x)
+{
+  const int n1 = g.n1, nc = (n1 + 1) / 2;
+  const int X = jc % nc, Y = (jc / nc) % nc, Z = g.dim == 3 ? jc / (nc * nc) : 0;
+  const int zl = g.dim == 3 ? -1 : 0, zh = g.dim == 3 ? 1 : 0;
+  DT s = DT(0);
+  for(int dz = zl; dz <= zh; ++dz)
+#pragma unroll
+    for(int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+      for(int dx = -1; dx <= 1; ++dx)
+      {
+        const int fx = 2 * X + dx, fy = 2 * Y + dy, fz = 2 * Z + dz;
+        if(fx >= 0 && fx < n1 && fy >= 0 && fy < n1 && fz >= 0 && (g.dim == 2 || fz < n1))
+          s = dfma(DT((dx ? 0.5 : 1.0) * (dy ? 0.5 : 1.0) * (dz ? 0.5 : 1.0)), x[(fz * n1 + fy) * n1 + fx], s);
+      }
+  return s;
+}
+
 // sum of a transfer row from its FP64 CSR arrays; an FP32 cycle rounds each value to FP32
 // as its FP32 plan does
 template<typename DT>
@@ -2783,8 +2851,8 @@ __device__ __forceinline__ bool tail_skips(double nrm2)
 // step writes the defect of x_steps into def and x_steps ends in b0; else the last update is
 // accumulated into acc.  b0/b1 are the ping-pong pair.  Thread = row `live ? threadIdx.x : -`.
 template<typename DT>
-__device__ __forceinline__ void tail_smooth(const TailRow<DT> R, bool live, const int* meta, const DT* tab, int steps,
-                                            DT omega, bool skip, const DT* rhs, DT* b0, DT* b1, DT* def, DT* acc)
+__device__ __forceinline__ void tail_smooth(const TailRow<DT> R, bool live, const int* meta, const DT* tab, TailGeom geo,
+                                            int steps, DT omega, bool skip, const DT* rhs, DT* b0, DT* b1, DT* def, DT* acc)
 {
   const uint64_t i = threadIdx.x;
   auto B = [&](int k) { return ((steps - k) & 1) ? b1 : b0; };
@@ -2809,8 +2877,9 @@ __device__ __forceinline__ void tail_smooth(const TailRow<DT> R, bool live, cons
     DT* out = mode == 1 ? def : (mode == 2 ? acc : B(k + 1));
     if(live)
     {
-      DT d = dfma(tail_row_a(R, meta, tab, i, xk), DT(-1), rhs[i]);
-      if(R.masked) d = DT(0);
+      // a filtered row's defect is zero whatever its sum: the sum is skipped
+      DT d = DT(0);
+      if(!R.masked) d = dfma(tail_row(R, meta, tab, geo, i, xk), DT(-1), rhs[i]);
       if(mode == 1)
         out[i] = d;
       else
@@ -2826,6 +2895,16 @@ __device__ __forceinline__ void tail_smooth(const TailRow<DT> R, bool live, cons
   }
 }
 
+// bytes of the tail's LDS vectors and pair tables, rounded up to a double
+template<typename DT, typename TT>
+__host__ __device__ inline uint64_t tail_words_bytes(const TT& t)
+{
+  uint64_t words = 0;
+  for(int l = 0; l < t.nlevels && l < FEAT_MG_TAIL_MAX_LEVELS; ++l)
+    words += (l + 1 < t.nlevels ? 5 * t.lv[l].n + 256 : 2 * t.lv[l].n);
+  return (words * sizeof(DT) + 7) / 8 * 8;
+}
+
 // TT: feat_mg_tail_t (FP64 cycle) or feat_mg_tail_f32_t (FP32 cycle of the mixed multigrid)
 template<typename DT, typename TT>
 __global__ __launch_bounds__(kTailThreads) void k_mg_tail(TT t, const DT* __restrict__ rhs_top,
@@ -2839,7 +2918,11 @@ __global__ __launch_bounds__(kTailThreads) void k_mg_tail(TT t, const DT* __rest
   TailBufs<DT> b[FEAT_MG_TAIL_MAX_LEVELS];
   TailRow<DT> row[kTailSmoothed];
   const int* meta[kTailSmoothed];
+  TailGeom geo[kTailSmoothed];
   DT* tabs = lds;   // the smoothed levels' pair-table values, 256 each
+  // the coarsest level's vectors by name: indexing b with the run-time c would put b in scratch
+  DT* crhs = nullptr;
+  DT* csol = nullptr;
   {
     DT* p = lds + 256 * c;
 #pragma unroll
@@ -2849,6 +2932,11 @@ __global__ __launch_bounds__(kTailThreads) void k_mg_tail(TT t, const DT* __rest
         const uint64_t n = t.lv[l].n;
         b[l].rhs = p;
         b[l].sol = p + n;
+        if(l == c)
+        {
+          crhs = p;
+          csol = p + n;
+        }
         p += 2 * n;
         if(l < c)
         {
@@ -2864,18 +2952,38 @@ __global__ __launch_bounds__(kTailThreads) void k_mg_tail(TT t, const DT* __rest
     if(l < c)
     {
       const auto& L = t.lv[l];
-      meta[l] = reinterpret_cast<const int*>(L.table + 256) + 256;
-      for(int e = threadIdx.x; e < 256; e += kTailThreads) tabs[256 * l + e] = L.table[e];
+      // grid form: the stencil in the pair table's place, no code words
+      geo[l] = TailGeom{L.sten ? L.dim : 0, int(L.n1)};
+      meta[l] = L.sten ? nullptr : reinterpret_cast<const int*>(L.table + 256) + 256;
+      if(L.sten)
+        for(int e = threadIdx.x; e < (L.dim == 3 ? 27 : 9); e += kTailThreads) tabs[256 * l + e] = L.sten[e];
+      else
+        for(int e = threadIdx.x; e < 256; e += kTailThreads) tabs[256 * l + e] = L.table[e];
       if(tid < L.n)
       {
-        const uint32_t* cw = L.codes + (tid >> 6) * 64 * uint64_t(L.words) + (tid & 63);
+        if(!L.sten)
+        {
+          const uint32_t* cw = L.codes + (tid >> 6) * 64 * uint64_t(L.words) + (tid & 63);
 #pragma unroll
-        for(int w = 0; w < 8; ++w) row[l].wd[w] = w < int(L.words) ? cw[w * 64] : 0xFFFFFFFFu;
+          for(int w = 0; w < 8; ++w) row[l].wd[w] = w < int(L.words) ? cw[w * 64] : 0xFFFFFFFFu;
+        }
         row[l].inv = L.inv_diag[tid];
         row[l].masked = is_masked(L.mask, tid);
       }
     }
   if(tid < t.lv[0].n) b[0].rhs[tid] = rhs_top[tid];
+  // the coarse inverse in LDS behind the vectors when it fits (as the launch sized it)
+  const double* cinv = t.coarse_inverse;
+  {
+    const uint64_t nc = t.lv[c].n;
+    const uint64_t off = tail_words_bytes<DT>(t);
+    if(off + nc * nc * sizeof(double) <= FEAT_MG_TAIL_MAX_LDS)
+    {
+      double* p = reinterpret_cast<double*>(reinterpret_cast<char*>(lds_words) + off);
+      for(uint64_t e = tid; e < nc * nc; e += kTailThreads) p[e] = t.coarse_inverse[e];
+      cinv = p;
+    }
+  }
   __syncthreads();
   double nrm[FEAT_MG_TAIL_MAX_LEVELS];
   nrm[0] = *nrm_top;
@@ -2886,14 +2994,18 @@ __global__ __launch_bounds__(kTailThreads) void k_mg_tail(TT t, const DT* __rest
     {
       const auto& L = t.lv[l];
       const bool live = tid < L.n;
-      tail_smooth(row[l], live, meta[l], tabs + 256 * l, L.pre_steps, L.pre_omega, tail_skips<DT>(nrm[l]), b[l].rhs,
-                  b[l].sol, b[l].tmp, b[l].def, (DT*)nullptr);
+      tail_smooth(row[l], live, meta[l], tabs + 256 * l, geo[l], L.pre_steps, L.pre_omega, tail_skips<DT>(nrm[l]),
+                  b[l].rhs, b[l].sol, b[l].tmp, b[l].def, (DT*)nullptr);
       const auto& C = t.lv[l + 1];
       double part = 0.0;
       if(tid < C.n)
       {
-        DT v = dfma(tail_row_csr(L.r_val, L.r_col, L.r_ptr, tid, b[l].def), DT(1), DT(0));
-        if(is_masked(C.mask, tid)) v = DT(0);
+        // a filtered row's value is zero whatever its sum: the sum is skipped
+        DT v = DT(0);
+        if(!is_masked(C.mask, tid))
+          v = dfma(L.dlinear ? tail_rest_grid(TailGeom{L.dim, int(L.n1)}, int(tid), b[l].def)
+                             : tail_row_csr(L.r_val, L.r_col, L.r_ptr, tid, b[l].def),
+                   DT(1), DT(0));
         b[l + 1].rhs[tid] = v;
         part = double(v) * double(v);
       }
@@ -2904,7 +3016,7 @@ __global__ __launch_bounds__(kTailThreads) void k_mg_tail(TT t, const DT* __rest
     const int n = int(t.lv[c].n);
     if(int(tid) < n)
     {
-      const double* ar = t.coarse_inverse + size_t(tid) * n;
+      const double* ar = cinv + size_t(tid) * n;
       double sv = 0.0;
       for(int k = 0; k < n; k += 8)
       {
@@ -2913,10 +3025,10 @@ __global__ __launch_bounds__(kTailThreads) void k_mg_tail(TT t, const DT* __rest
         for(int u = 0; u < 8; ++u) a[u] = k + u < n ? ar[k + u] : 0.0;
 #pragma unroll
         for(int u = 0; u < 8; ++u)
-          if(k + u < n) sv = dfma(a[u], double(b[c].rhs[k + u]), sv);
+          if(k + u < n) sv = dfma(a[u], double(crhs[k + u]), sv);
       }
       if(is_masked(t.lv[c].mask, tid)) sv = 0.0;
-      b[c].sol[tid] = DT(sv);
+      csol[tid] = DT(sv);
     }
     __syncthreads();
   }
@@ -2929,22 +3041,25 @@ __global__ __launch_bounds__(kTailThreads) void k_mg_tail(TT t, const DT* __rest
       const bool live = tid < L.n;
       if(live)
       {
-        DT cc = dfma(tail_row_csr(L.p_val, L.p_col, L.p_ptr, tid, b[l + 1].sol), DT(1), DT(0));
-        if(row[l].masked) cc = DT(0);
+        DT cc = DT(0);
+        if(!row[l].masked)
+          cc = dfma(L.dlinear ? tail_prol_grid(TailGeom{L.dim, int(L.n1)}, int(tid), b[l + 1].sol)
+                              : tail_row_csr(L.p_val, L.p_col, L.p_ptr, tid, b[l + 1].sol),
+                    DT(1), DT(0));
         b[l].sol[tid] = dfma(DT(1), cc, b[l].sol[tid]);
       }
       __syncthreads();
       double part = 0.0;
       if(live)
       {
-        DT v = dfma(tail_row_a(row[l], meta[l], tabs + 256 * l, tid, b[l].sol), DT(-1), b[l].rhs[tid]);
-        if(row[l].masked) v = DT(0);
+        DT v = DT(0);
+        if(!row[l].masked) v = dfma(tail_row(row[l], meta[l], tabs + 256 * l, geo[l], tid, b[l].sol), DT(-1), b[l].rhs[tid]);
         b[l].def[tid] = v;
         part = double(v) * double(v);
       }
       const double dn = tail_sum(part, red);
-      tail_smooth(row[l], live, meta[l], tabs + 256 * l, L.post_steps, L.post_omega, tail_skips<DT>(dn), b[l].def,
-                  b[l].cor, b[l].tmp, (DT*)nullptr, b[l].sol);
+      tail_smooth(row[l], live, meta[l], tabs + 256 * l, geo[l], L.post_steps, L.post_omega, tail_skips<DT>(dn),
+                  b[l].def, b[l].cor, b[l].tmp, (DT*)nullptr, b[l].sol);
     }
   if(tid < t.lv[0].n) sol_top[tid] = b[0].sol[tid];
 }
@@ -2952,6 +3067,15 @@ __global__ __launch_bounds__(kTailThreads) void k_mg_tail(TT t, const DT* __rest
 template<typename DT, typename TT>
 int mg_tail_launch(feat_handle_t h, const TT* t, const DT* rhs, const double* rhs_nrm2, DT* sol, const char* name)
 {
+  if(t)
+    for(int l = 0; l + 1 < t->nlevels && l < FEAT_MG_TAIL_MAX_LEVELS; ++l)
+    {
+      const auto& L = t->lv[l];
+      const uint64_t n1 = L.n1, nn = L.dim == 3 ? n1 * n1 * n1 : n1 * n1, nc = (n1 + 1) / 2;
+      const uint64_t ncn = L.dim == 3 ? nc * nc * nc : nc * nc;
+      if((L.sten || L.dlinear) && (L.dim < 2 || L.dim > 3 || n1 < 3 || !(n1 & 1) || nn != L.n || ncn != t->lv[l + 1].n))
+        return fail(FEAT_ERR_INVALID, "fused coarse levels: grid form inconsistent with the level sizes");
+    }
   if(!h || !t || !rhs || !rhs_nrm2 || !sol || t->nlevels < 2 || t->nlevels > FEAT_MG_TAIL_MAX_LEVELS ||
      !t->coarse_inverse)
     return fail(FEAT_ERR_INVALID, name);
@@ -2967,7 +3091,11 @@ int mg_tail_launch(feat_handle_t h, const TT* t, const DT* rhs, const double* rh
       return fail(FEAT_ERR_INVALID, "fused coarse levels: incomplete level");
   }
   if(words * sizeof(DT) > FEAT_MG_TAIL_MAX_LDS) return fail(FEAT_ERR_INVALID, "fused coarse levels: levels exceed the LDS budget");
-  hipLaunchKernelGGL((k_mg_tail<DT, TT>), dim3(1), dim3(kTailThreads), words * sizeof(DT), h->stream, *t, rhs, rhs_nrm2, sol);
+  // the coarse inverse is staged behind the vectors when it fits
+  const uint64_t off = tail_words_bytes<DT>(*t), nc = t->lv[t->nlevels - 1].n;
+  const uint64_t lds = off + nc * nc * sizeof(double) <= FEAT_MG_TAIL_MAX_LDS ? off + nc * nc * sizeof(double)
+                                                                               : words * sizeof(DT);
+  hipLaunchKernelGGL((k_mg_tail<DT, TT>), dim3(1), dim3(kTailThreads), lds, h->stream, *t, rhs, rhs_nrm2, sol);
   return check_launch(name);
 }
 

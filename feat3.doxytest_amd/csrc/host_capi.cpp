@@ -75,6 +75,11 @@ int feat_runtime_init(int device, void* stream)
   return guarded([&]() { Runtime::initialize(device, stream); });
 }
 
+int feat_runtime_tail_grid(int on)
+{
+  return guarded([&]() { Runtime::tail_grid() = on != 0; });
+}
+
 int feat_runtime_sell_flags(int flags)
 {
   return guarded([&]() { Runtime::sell_flags() = flags; });
@@ -572,6 +577,14 @@ int feat_krylov_destroy(feat_krylov_t s)
   return guarded([&]() {
     if(s && s->solver) s->solver->done();
     delete s;
+  });
+}
+
+int feat_mgpcg_coarse_fusion_grid(feat_mgpcg_t s, int* levels)
+{
+  return guarded([&]() {
+    if(!s || !levels) throw Error("feat_mgpcg_coarse_fusion_grid: bad argument");
+    *levels = s->mg32 ? s->mg32->coarse_fusion_grid_levels() : (s->mg ? s->mg->coarse_fusion_grid_levels() : 0);
   });
 }
 

@@ -863,7 +863,8 @@ int feat_ilu_done_symbolic(feat_ilu_t s);
  * prolongation from the next level as CSR); lv[nlevels-1]: the coarsest level (n, mask).
  * rhs / rhs_nrm2: level t's right-hand side and |rhs|^2 (device); sol: level t's solution
  * (written).  The caller keeps 8 bytes x (5 n + 256 over the smoothed levels + 2 n coarse)
- * within FEAT_MG_TAIL_MAX_LDS bytes.  No counterpart in the reference (a launch-count fusion). */
+ * within FEAT_MG_TAIL_MAX_LDS bytes (when the coarse inverse, 8 n_c^2 bytes, also fits, it
+ * is staged in LDS).  No counterpart in the reference (a launch-count fusion). */
 #define FEAT_MG_TAIL_MAX_LEVELS 6
 #define FEAT_MG_TAIL_MAX_ROWS 1024   /* rows per level (one thread per row) */
 #define FEAT_MG_TAIL_MAX_LDS 61440
@@ -875,6 +876,17 @@ typedef struct {
   const double* p_val; const uint64_t* p_col; const uint64_t* p_ptr;   /* prolongation from the next level */
   int pre_steps, post_steps;
   double pre_omega, post_omega;
+  /* grid form (0 / NULL: unused).  dim (2 or 3) and n1: the level is an n1^dim grid in
+   * lexicographic order.  sten: the 3^dim values that every free row holds at the offsets
+   * (dz, dy, dx) in {-1,0,1}^dim in lexicographic order, read instead of the plan; rows the
+   * mask filters skip their sum, which they never use.  dlinear: the prolongation from the
+   * next level ((n1 + 1) / 2 per side) is d-linear injection (weights 1, 1/2, ... in
+   * ascending columns) and the restriction its transpose, formed from the node positions
+   * instead of read.  All of it checked by the caller against the matrices. */
+  int dim;
+  uint32_t n1;
+  const double* sten;
+  int dlinear;
 } feat_mg_tail_level_t;
 typedef struct {
   int nlevels;
@@ -897,6 +909,10 @@ typedef struct {
   const double* p_val; const uint64_t* p_col; const uint64_t* p_ptr;
   int pre_steps, post_steps;
   float pre_omega, post_omega;
+  int dim;                     /* grid form, as feat_mg_tail_level_t (sten: FP32 values) */
+  uint32_t n1;
+  const float* sten;
+  int dlinear;
 } feat_mg_tail_level_f32_t;
 typedef struct {
   int nlevels;

@@ -74,6 +74,13 @@ namespace feat3amd
       static int f = 0;
       return f;
     }
+    /// fused coarse levels in their grid form where the levels qualify (stencils and
+    /// d-linear transfers formed in the kernel; off: the plans and CSR transfers are read)
+    static bool& tail_grid()
+    {
+      static bool on = true;
+      return on;
+    }
     /// Scope in which this thread's calls launch on `h` (graph recording on a solver stream).
     class Use
     {

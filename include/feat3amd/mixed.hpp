@@ -45,6 +45,29 @@ namespace feat3amd
       bool _tail_on = true, _tail_built = false;
       Index _tail_t = 0;   // first level of the fused coarse part; 0: none
       feat_mg_tail_f32_t _tail{};
+      std::vector<TailGridForm> _grid;              // per level: grid form if it is fused
+      std::vector<DeviceArray<float>> _grid_sten;   // its FP32 stencil (rounded as the FP32 plan rounds)
+
+      /// The grid form of every level that the fused coarse part could hold (host copies of
+      /// its matrices): at init_numeric, never inside a recorded graph.
+      void _scan_grid()
+      {
+        const Index L = _h->size();
+        _grid.assign(L, TailGridForm{});
+        _grid_sten.assign(L, DeviceArray<float>());
+        for(Index s = 1; s + 1 < L; ++s)
+        {
+          auto& lv = _h->levels[s];
+          if(lv.matrix->rows() > FEAT_MG_TAIL_MAX_ROWS || !lv.filter) continue;
+          _grid[s] = tail_grid_form(*lv.matrix, lv.filter->mask(), lv.transfer->get_mat_prol(),
+                                    lv.transfer->get_mat_rest(), _h->levels[s + 1].matrix->rows());
+          if(!_grid[s].sten.empty())
+          {
+            std::vector<float> s32(_grid[s].sten.begin(), _grid[s].sten.end());
+            _grid_sten[s] = DeviceArray<float>::from_host(s32.data(), s32.size());
+          }
+        }
+      }
 
       // execution plan of a level's matrix: FP32 values or FP16 values (FP32 arithmetic)
       const feat_matrix_f32_t& plan(const Matrix_& m) const { return _half ? m.exec16() : m.exec32(); }
@@ -127,6 +150,14 @@ namespace feat3amd
           d.post_steps = _l[s].post_steps;
           d.pre_omega = _l[s].omega_pre;
           d.post_omega = _l[s].omega_post;
+          // grid form found at init_numeric (no copies here: this may run inside a recorded graph)
+          if(s < _grid.size())
+          {
+            d.dim = _grid[s].dim;
+            d.n1 = _grid[s].n1;
+            d.dlinear = _grid[s].dlinear ? 1 : 0;
+            d.sten = _grid_sten[s].size() ? _grid_sten[s].get() : nullptr;
+          }
         }
         _tail.coarse_inverse = static_cast<DirectType*>(_h->levels[c].coarse.get())->inverse();
         _tail_t = t;
@@ -144,6 +175,14 @@ namespace feat3amd
         _tail_built = false;
       }
       /// first level of the fused coarse part (0: none)
+      /// fused levels in the grid form (stencil from the kernel, d-linear transfers)
+      int coarse_fusion_grid_levels()
+      {
+        if(!_tail_built) _build_tail();
+        int k = 0;
+        for(int l = 0; _tail_t && l + 1 < _tail.nlevels; ++l) k += (_tail.lv[l].sten && _tail.lv[l].dlinear) ? 1 : 0;
+        return k;
+      }
       Index coarse_fusion_level()
       {
         if(!_tail_built) _build_tail();
@@ -223,6 +262,7 @@ namespace feat3amd
           _l[L - 1].omega_pre = float(r->omega());
         }
         _tail_built = false;   // the tail holds pointers to the vectors made here
+        _scan_grid();
       }
 
       Status apply(VectorType& cor, const VectorType& def) override

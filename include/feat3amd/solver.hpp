@@ -21,6 +21,7 @@
 #include <limits>
 #include <memory>
 #include <string>
+#include <cstring>
 #include <vector>
 
 namespace feat3amd
@@ -404,6 +405,154 @@ namespace feat3amd
     enum class MultiGridCycle { V };
 
     /// MultiGridHierarchy (solver/multigrid.hpp:419-957); level 0 is the finest.
+    /// Grid form of a fused coarse level (feat_mg_tail_level_t dim / n1 / sten / dlinear):
+    /// the level is an n1^dim grid (dim 2 or 3, lexicographic) and the next one has
+    /// (n1 + 1) / 2 nodes per side.  sten is set when every free row (not in the mask)
+    /// holds the same 3^dim values at the neighbour offsets in CSR order.  dlinear is set
+    /// when the prolongation from the next level is d-linear injection (the coarse nodes
+    /// around each fine node, ascending, weight 1/2 per odd axis) and the restriction is its
+    /// transpose.  Both are checked entry by entry on host copies of the matrices.
+    struct TailGridForm
+    {
+      int dim = 0;
+      std::uint32_t n1 = 0;
+      std::vector<double> sten;   // empty: the rows come from the plan
+      bool dlinear = false;
+    };
+
+    template<typename Matrix_>
+    TailGridForm tail_grid_form(const Matrix_& A, const std::uint32_t* mask_dev, const Matrix_& P, const Matrix_& R,
+                                Index n_next)
+    {
+      TailGridForm f;
+      const Index n = A.rows();
+      int dim = 0;
+      std::int64_t n1 = 0;
+      for(int d = 3; d >= 2 && !dim; --d)
+      {
+        const std::int64_t r = std::llround(std::pow(double(n), 1.0 / d));
+        for(std::int64_t c = std::max<std::int64_t>(r - 1, 3); c <= r + 1; ++c)
+        {
+          const std::int64_t cn = d == 3 ? c * c * c : c * c, nc = (c + 1) / 2;
+          const std::int64_t ccn = d == 3 ? nc * nc * nc : nc * nc;
+          if((c & 1) && cn == std::int64_t(n) && ccn == std::int64_t(n_next))
+          {
+            dim = d;
+            n1 = c;
+            break;
+          }
+        }
+      }
+      if(!dim || !Runtime::tail_grid()) return f;
+      f.dim = dim;
+      f.n1 = std::uint32_t(n1);
+      auto down = [](auto* dst, const auto* src, std::size_t count) {
+        if(count) check(feat_copy(Runtime::handle(), dst, src, sizeof(*dst) * count), "tail grid form: copy");
+      };
+      auto fetch = [&](const Matrix_& M, std::vector<double>& v, std::vector<std::uint64_t>& c,
+                       std::vector<std::uint64_t>& p) {
+        p.resize(M.rows() + 1);
+        down(p.data(), (const std::uint64_t*)M.row_ptr(), p.size());
+        v.resize(M.used_elements());
+        c.resize(M.used_elements());
+        down(v.data(), (const double*)M.val(), v.size());
+        down(c.data(), (const std::uint64_t*)M.col_ind(), c.size());
+      };
+      std::vector<double> av, pv, rv;
+      std::vector<std::uint64_t> ac, ap, pc, pp, rc, rp;
+      fetch(A, av, ac, ap);
+      fetch(P, pv, pc, pp);
+      fetch(R, rv, rc, rp);
+      std::vector<std::uint32_t> mask(mask_dev ? (n + 31) / 32 : 0, 0u);
+      down(mask.data(), mask_dev, mask.size());
+      Runtime::synchronize();
+      const std::int64_t nn = n1 * n1, nc = (n1 + 1) / 2;
+      const int zl = dim == 3 ? -1 : 0, zh = dim == 3 ? 1 : 0;
+      auto coord = [&](std::int64_t i, std::int64_t side, std::int64_t& x, std::int64_t& y, std::int64_t& z) {
+        x = i % side;
+        y = (i / side) % side;
+        z = dim == 3 ? i / (side * side) : 0;
+      };
+      // the stencil of the free rows
+      bool sten_ok = true;
+      for(std::int64_t i = 0; i < std::int64_t(n) && sten_ok; ++i)
+      {
+        if(!mask.empty() && ((mask[i >> 5] >> (i & 31)) & 1u)) continue;
+        std::int64_t x, y, z;
+        coord(i, n1, x, y, z);
+        if(x < 1 || x > n1 - 2 || y < 1 || y > n1 - 2 || (dim == 3 && (z < 1 || z > n1 - 2)))
+        {
+          sten_ok = false;
+          break;
+        }
+        std::uint64_t e = ap[i];
+        if(ap[i + 1] - e != (dim == 3 ? 27u : 9u))
+        {
+          sten_ok = false;
+          break;
+        }
+        std::size_t u = 0;
+        for(int dz = zl; dz <= zh && sten_ok; ++dz)
+          for(int dy = -1; dy <= 1 && sten_ok; ++dy)
+            for(int dx = -1; dx <= 1; ++dx, ++u, ++e)
+            {
+              if(std::int64_t(ac[e]) != i + dz * nn + dy * n1 + dx)
+              {
+                sten_ok = false;
+                break;
+              }
+              if(f.sten.size() <= u)
+                f.sten.push_back(av[e]);
+              else if(std::memcmp(&f.sten[u], &av[e], sizeof(double)) != 0)
+              {
+                sten_ok = false;
+                break;
+              }
+            }
+      }
+      if(!sten_ok || f.sten.empty()) f.sten.clear();
+      // the transfers
+      bool p_ok = P.rows() == n && rp.size() == std::size_t(n_next) + 1;
+      for(std::int64_t i = 0; i < std::int64_t(n) && p_ok; ++i)
+      {
+        std::int64_t x, y, z;
+        coord(i, n1, x, y, z);
+        const int ox = int(x & 1), oy = int(y & 1), oz = int(z & 1);
+        const double w = (ox ? 0.5 : 1.0) * (oy ? 0.5 : 1.0) * (oz ? 0.5 : 1.0);
+        std::uint64_t e = pp[i];
+        if(pp[i + 1] - e != std::uint64_t((1 + ox) * (1 + oy) * (1 + oz)))
+        {
+          p_ok = false;
+          break;
+        }
+        for(int iz = 0; iz <= oz; ++iz)
+          for(int iy = 0; iy <= oy; ++iy)
+            for(int ix = 0; ix <= ox; ++ix, ++e)
+              if(std::int64_t(pc[e]) != ((z >> 1) + iz) * nc * nc + ((y >> 1) + iy) * nc + (x >> 1) + ix || pv[e] != w)
+                p_ok = false;
+      }
+      for(std::int64_t j = 0; j < std::int64_t(n_next) && p_ok; ++j)
+      {
+        std::int64_t X, Y, Z;
+        coord(j, nc, X, Y, Z);
+        std::uint64_t e = rp[j];
+        for(int dz = zl; dz <= zh; ++dz)
+          for(int dy = -1; dy <= 1; ++dy)
+            for(int dx = -1; dx <= 1; ++dx)
+            {
+              const std::int64_t fx = 2 * X + dx, fy = 2 * Y + dy, fz = 2 * Z + dz;
+              if(fx < 0 || fx >= n1 || fy < 0 || fy >= n1 || (dim == 3 && (fz < 0 || fz >= n1))) continue;
+              if(e >= rp[j + 1] || std::int64_t(rc[e]) != (fz * n1 + fy) * n1 + fx ||
+                 rv[e] != (dx ? 0.5 : 1.0) * (dy ? 0.5 : 1.0) * (dz ? 0.5 : 1.0))
+                p_ok = false;
+              ++e;
+            }
+        if(e != rp[j + 1]) p_ok = false;
+      }
+      f.dlinear = p_ok;
+      return f;
+    }
+
     template<typename Matrix_, typename Filter_, typename Transfer_>
     class MultiGridHierarchy
     {
@@ -516,6 +665,26 @@ namespace feat3amd
       bool _tail_on = true, _tail_built = false;
       Index _tail_t = 0;   // first level of the fused coarse part; 0: none (level 0 is never in it)
       feat_mg_tail_t _tail{};
+      std::vector<TailGridForm> _grid;               // per level: grid form if it is fused
+      std::vector<DeviceArray<double>> _grid_sten;   // its stencil on the device (empty: none)
+
+      /// The grid form of every level that the fused coarse part could hold, with host copies
+      /// of its matrices: at init_numeric, never inside a recorded graph.
+      void _scan_grid()
+      {
+        const Index L = _h->size();
+        _grid.assign(L, TailGridForm{});
+        _grid_sten.assign(L, DeviceArray<double>());
+        for(Index s = 1; s + 1 < L; ++s)
+        {
+          auto& l = _h->levels[s];
+          if(l.matrix->rows() > FEAT_MG_TAIL_MAX_ROWS || !l.filter) continue;
+          _grid[s] = tail_grid_form(*l.matrix, l.filter->mask(), l.transfer->get_mat_prol(), l.transfer->get_mat_rest(),
+                                    _h->levels[s + 1].matrix->rows());
+          if(!_grid[s].sten.empty())
+            _grid_sten[s] = DeviceArray<double>::from_host(_grid[s].sten.data(), _grid[s].sten.size());
+        }
+      }
       enum class Coarse { none, direct, jacobi, richardson, other } _coarse_kind = Coarse::other;
 
       /// The coarse levels that run as one LDS-resident workgroup (feat_mg_tail_f64): the
@@ -572,6 +741,14 @@ namespace feat3amd
           d.post_steps = int(post->get_max_iter());
           d.pre_omega = pre->omega();
           d.post_omega = post->omega();
+          // grid form found at init_numeric (no copies here: this may run inside a recorded graph)
+          if(s < _grid.size())
+          {
+            d.dim = _grid[s].dim;
+            d.n1 = _grid[s].n1;
+            d.dlinear = _grid[s].dlinear ? 1 : 0;
+            d.sten = _grid_sten[s].size() ? _grid_sten[s].get() : nullptr;
+          }
         }
         _tail.coarse_inverse = static_cast<DirectCoarseSolver<Matrix_, Filter_>*>(_h->levels[c].coarse.get())->inverse();
         _tail_t = t;
@@ -592,6 +769,14 @@ namespace feat3amd
         _tail_built = false;
       }
       Index coarse_fusion_level() { if(!_tail_built) _build_tail(); return _tail_t; }
+      /// fused levels in the grid form (stencil from the kernel, d-linear transfers)
+      int coarse_fusion_grid_levels()
+      {
+        if(!_tail_built) _build_tail();
+        int k = 0;
+        for(int l = 0; _tail_t && l + 1 < _tail.nlevels; ++l) k += (_tail.lv[l].sten && _tail.lv[l].dlinear) ? 1 : 0;
+        return k;
+      }
 
       void init_symbolic() override
       {
@@ -648,6 +833,7 @@ namespace feat3amd
       {
         _tail_built = false;
         _build_plans();
+        _scan_grid();
       }
       void done_numeric() override { _tail_built = false; }
       void done_symbolic() override { _tail_built = false; }

@@ -129,6 +129,8 @@ int feat_mgpcg_set_values(feat_mgpcg_t s, int lvl, const double* val_host);
 /* first level of the coarse levels that run fused as one workgroup (feat_mg_tail_f64 /
  * feat_mg_tail_f32) in s's V-cycle; 0 = none.  No counterpart in the reference. */
 int feat_mgpcg_coarse_fusion_level(feat_mgpcg_t s, int* level);
+/* the number of fused levels that run in the grid form (feat_mg_tail_level_t sten + dlinear) */
+int feat_mgpcg_coarse_fusion_grid(feat_mgpcg_t s, int* levels);
 int feat_mgpcg_destroy(feat_mgpcg_t s);
 
 /* ---- single Krylov solver with a device preconditioner ----
@@ -163,6 +165,9 @@ int feat_runtime_init(int device, void* stream);
  * FEAT_SELL_NO_DICT keeps full values and column offsets even where the pair
  * dictionary applies).  No counterpart in the reference (format selection). */
 int feat_runtime_sell_flags(int flags);
+/* fused coarse levels of the multigrid solvers built from now on in their grid form where
+ * the levels qualify (1, default) or always from the plans and CSR transfers (0) */
+int feat_runtime_tail_grid(int on);
 
 /* ---- Stokes multigrid: Richardson(MG-V(Richardson(Schwarz(AmaVanka)))) ----
  * benchmarks/stokes_multigrid-bench.cpp:714-810 on levels level_fine..level_coarse

@@ -591,8 +591,10 @@ def test_mgpcg_dictionary_vs_plain_plans(feat, torch_gpu, precision):
 def test_coarse_fusion_bitwise(feat, torch_gpu, dim, lf, lc, first, precision):
     """The coarse levels run as one LDS-resident workgroup (feat_mg_tail_f64, or
     feat_mg_tail_f32 in the FP32 cycle of the mixed multigrid) give the solution and
-    defect history of the separate launches bit for bit.  `first` is the first fused
-    level: the finest level with at most 1,024 rows."""
+    defect history of the separate launches bit for bit, in the grid form (the common
+    stencil of the free rows and the d-linear transfers formed in the kernel) and from
+    the plans and CSR transfers.  `first` is the first fused level: the finest level
+    with at most 1,024 rows."""
     print(f"coarse fusion {precision} dim={dim} levels {lf}..{lc}", flush=True)
     torch = torch_gpu
     q = feat.q1_problem(dim, lf)
@@ -600,22 +602,30 @@ def test_coarse_fusion_bitwise(feat, torch_gpu, dim, lf, lc, first, precision):
     b[q["filt"].astype(np.int64)] = 0.0
     bd = torch.from_numpy(b).cuda()
     out = []
-    for fusion in (True, False):
+    for fusion, grid in ((True, True), (True, False), (False, True)):
         cfg = feat.mgpcg_cfg(nu_pre=4, nu_post=4, omega_jac=0.5, omega_rich=1.0, coarse="direct",
                              tol_rel=1e-8, max_iter=100, graphs=False, coarse_fusion=fusion,
                              precision=precision)
-        s = feat.MgPcg(cfg, q1=(dim, lf, lc))
-        assert s.coarse_fusion_level() == (first if fusion else 0)
-        x = torch.zeros(q["n"], dtype=torch.float64, device="cuda")
-        r = s.solve(x, bd)
-        torch.cuda.synchronize()
+        feat.runtime_tail_grid(grid)
+        try:
+            s = feat.MgPcg(cfg, q1=(dim, lf, lc))
+            assert s.coarse_fusion_level() == (first if fusion else 0)
+            # every smoothed fused level (solver indices first .. lf - lc - 1) has the grid form
+            assert s.coarse_fusion_grid() == ((lf - lc - first) if (fusion and grid) else 0)
+            x = torch.zeros(q["n"], dtype=torch.float64, device="cuda")
+            r = s.solve(x, bd)
+            torch.cuda.synchronize()
+            assert s.coarse_fusion_grid() == ((lf - lc - first) if (fusion and grid) else 0)
+        finally:
+            feat.runtime_tail_grid(True)
         s.close()
         assert r["status"] == "success"
         out.append((r, x.cpu().numpy()))
-    (r1, x1), (r0, x0) = out
-    assert r1["iters"] == r0["iters"]
-    assert np.array_equal(r1["hist"].view(np.uint64), r0["hist"].view(np.uint64))
-    assert np.array_equal(x1.view(np.uint64), x0.view(np.uint64))
+    r0, x0 = out[-1]
+    for r1, x1 in out[:-1]:
+        assert r1["iters"] == r0["iters"]
+        assert np.array_equal(r1["hist"].view(np.uint64), r0["hist"].view(np.uint64))
+        assert np.array_equal(x1.view(np.uint64), x0.view(np.uint64))
 
 
 @pytest.mark.parametrize("precision", ["f64", "f32"])
