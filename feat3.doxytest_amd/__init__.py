@@ -53,7 +53,7 @@ class Matrix(C.Structure):
                 ("scale", _f), ("tune", _i), ("slice_words", C.c_uint32)]
 
 
-FMT_CSR, FMT_SELL = 0, 1
+FMT_CSR, FMT_SELL, FMT_DLINEAR = 0, 1, 2
 IDX_U64, IDX_U32, IDX_D16, IDX_DICT, IDX_DIAG = 0, 1, 2, 3, 4
 SELL_NO_DICT, SELL_NO_DIAG = 1, 2
 VAL_NATIVE, VAL_F16, VAL_CODE8 = 0, 1, 2
@@ -134,6 +134,8 @@ KERNEL_API = [
     ("feat_unit_filter_blocked_def_f64_u64", _i, [_vp, _i, _vp, _vp, _vp, _u64, _i]),
     ("feat_mean_filter_f64", _i, [_vp, _vp, _vp, _vp, _d, _u64]),
     ("feat_matrix_from_csr_f64", _i, [C.POINTER(CsrF64), C.POINTER(Matrix)]),
+    ("feat_matrix_dlinear_f64", _i, [_i, C.c_uint32, _i, C.POINTER(Matrix)]),
+    ("feat_matrix_dlinear_f32", _i, [_i, C.c_uint32, _i, C.POINTER(Matrix)]),
     ("feat_sell_create_f64", _i, [_vp, C.POINTER(CsrF64), C.POINTER(Matrix)]),
     ("feat_sell_create_f32", _i, [_vp, C.POINTER(CsrF64), C.POINTER(Matrix)]),
     ("feat_sell_create_ex_f64", _i, [_vp, C.POINTER(CsrF64), _i, C.POINTER(Matrix)]),
@@ -289,7 +291,7 @@ HOST_API = [
     ("feat_mgpcg_destroy", _i, [_vp]),
     ("feat_runtime_init", _i, [_i, _vp]),
     ("feat_runtime_sell_flags", _i, [_i]),
-    ("feat_runtime_tail_grid", _i, [_i]),
+    ("feat_runtime_grid_forms", _i, [_i]),
 ]
 
 
@@ -705,10 +707,11 @@ def runtime_init(device=0, stream=None):
     check(h.feat_runtime_init(device, stream), "feat_runtime_init")
 
 
-def runtime_tail_grid(on):
-    """Fused coarse levels built from now on in their grid form where they qualify (default on)."""
+def runtime_grid_forms(on):
+    """Multigrid solvers initialised from now on use the grid forms where the levels qualify
+    (d-linear transfer descriptors, fused coarse levels' stencils; default on)."""
     _, h = libs()
-    check(h.feat_runtime_tail_grid(int(bool(on))), "feat_runtime_tail_grid")
+    check(h.feat_runtime_grid_forms(int(bool(on))), "feat_runtime_grid_forms")
 
 
 def runtime_sell_flags(flags):

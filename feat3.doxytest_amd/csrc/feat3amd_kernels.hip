@@ -647,11 +647,90 @@ __global__ __launch_bounds__(kThreads) void k_sell_diag_rows(const DT* __restric
   grid_reduce_finish(gr, acc0, acc1);
 }
 
+// d-linear injection between structured grids (FEAT_FMT_DLINEAR): row sums formed from the
+// node positions, in the ascending column order of the assembled transfer
+struct TailGeom
+{
+  int dim, n1;   // dim 0: not a grid
+};
+
+// prolongation row i of the fine grid (n1 per side): the coarse nodes around the fine node in
+// ascending order, each weighted with 1/2 per odd axis
+template<typename DT>
+__device__ __forceinline__ DT dlinear_prol_row(TailGeom g, int64_t i, const DT* x)
+{
+  const int64_t n1 = g.n1, nc = (n1 + 1) / 2;
+  const int64_t fx = i % n1, fy = (i / n1) % n1, fz = g.dim == 3 ? i / (n1 * n1) : 0;
+  const int ox = int(fx & 1), oy = int(fy & 1), oz = int(fz & 1);
+  const DT w = DT((ox ? 0.5 : 1.0) * (oy ? 0.5 : 1.0) * (oz ? 0.5 : 1.0));
+  DT s = DT(0);
+  for(int iz = 0; iz <= oz; ++iz)
+    for(int iy = 0; iy <= oy; ++iy)
+      for(int ix = 0; ix <= ox; ++ix) s = dfma(w, x[((fz >> 1) + iz) * nc * nc + ((fy >> 1) + iy) * nc + (fx >> 1) + ix], s);
+  return s;
+}
+
+// restriction row jc of the coarse grid: the fine nodes (2X + dx, 2Y + dy, 2Z + dz) inside
+// the grid in ascending order, weights 1/2 per nonzero offset
+template<typename DT>
+__device__ __forceinline__ DT dlinear_rest_row(TailGeom g, int64_t jc, const DT* x)
+{
+  const int64_t n1 = g.n1, nc = (n1 + 1) / 2;
+  const int64_t X = jc % nc, Y = (jc / nc) % nc, Z = g.dim == 3 ? jc / (nc * nc) : 0;
+  const int zl = g.dim == 3 ? -1 : 0, zh = g.dim == 3 ? 1 : 0;
+  DT s = DT(0);
+  for(int dz = zl; dz <= zh; ++dz)
+#pragma unroll
+    for(int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+      for(int dx = -1; dx <= 1; ++dx)
+      {
+        const int64_t fx = 2 * X + dx, fy = 2 * Y + dy, fz = 2 * Z + dz;
+        if(fx >= 0 && fx < n1 && fy >= 0 && fy < n1 && fz >= 0 && (g.dim == 2 || fz < n1))
+          s = dfma(DT((dx ? 0.5 : 1.0) * (dy ? 0.5 : 1.0) * (dz ? 0.5 : 1.0)), x[(fz * n1 + fy) * n1 + fx], s);
+      }
+  return s;
+}
+
+template<typename DT, class Epi, bool REST>
+__global__ __launch_bounds__(kThreads) void k_dlinear_rows(int dim, int n1, const DT* __restrict__ x, uint64_t rows,
+                                                           Epi epi, GridReduce gr)
+{
+  double acc0 = 0.0, acc1 = 0.0;
+  epi.begin();
+  const TailGeom g{dim, n1};
+  for(uint64_t i = blockIdx.x * uint64_t(kThreads) + threadIdx.x; i < rows; i += uint64_t(gridDim.x) * kThreads)
+  {
+    typename EpiPre<Epi>::type pf{};
+    if constexpr(EpiPre<Epi>::on) pf = epi.pre(i);
+    const DT sum = REST ? dlinear_rest_row(g, int64_t(i), x) : dlinear_prol_row(g, int64_t(i), x);
+    if constexpr(EpiPre<Epi>::on)
+      epi.row(i, sum, pf, acc0, acc1);
+    else
+      epi.row(i, sum, acc0, acc1);
+  }
+  grid_reduce_finish(gr, acc0, acc1);
+}
+
 // Runs Epi over the rows of a descriptor (CSR or SELL-64).
 template<typename DT, class Epi, class M>
 int launch_mat(hipStream_t s, const M* A, const DT* x, const Epi& epi, const GridReduce& gr, const char* name)
 {
   if(A->rows == 0) return FEAT_OK;
+  if(A->format == FEAT_FMT_DLINEAR)
+  {
+    const uint64_t nb = (A->rows + kThreads - 1) / kThreads;
+    const int grid = int(nb < uint64_t(kMaxGrid) ? nb : uint64_t(kMaxGrid));
+    const GridReduce gd = reduce_for_grid(gr, uint64_t(grid));
+    if(A->value_kind)
+      hipLaunchKernelGGL((k_dlinear_rows<DT, Epi, true>), dim3(grid), dim3(kThreads), 0, s, A->index,
+                         int(A->slice_words), x, A->rows, epi, gd);
+    else
+      hipLaunchKernelGGL((k_dlinear_rows<DT, Epi, false>), dim3(grid), dim3(kThreads), 0, s, A->index,
+                         int(A->slice_words), x, A->rows, epi, gd);
+    finish_reduce(s, gd, uint64_t(grid));
+    return check_launch(name);
+  }
   if(A->format == FEAT_FMT_CSR)
   {
     if(A->index == FEAT_IDX_U64)
@@ -1899,6 +1978,40 @@ int feat_matrix_from_csr_f64(const feat_csr_f64_t* A, feat_matrix_f64_t* out)
 } // extern "C"
 
 namespace {
+template<typename M>
+int dlinear_desc(int dim, uint32_t n1, int transpose, M* out, const char* name)
+{
+  if(!out || (dim != 2 && dim != 3) || n1 < 3 || !(n1 & 1) || (transpose != 0 && transpose != 1))
+    return fail(FEAT_ERR_INVALID, name);
+  const uint64_t nc = (uint64_t(n1) + 1) / 2;
+  const uint64_t nf = dim == 3 ? uint64_t(n1) * n1 * n1 : uint64_t(n1) * n1, ncn = dim == 3 ? nc * nc * nc : nc * nc;
+  *out = M{};
+  out->format = FEAT_FMT_DLINEAR;
+  out->index = dim;
+  out->rows = transpose ? ncn : nf;
+  out->cols = transpose ? nf : ncn;
+  out->value_kind = transpose;
+  out->scale = 1.0f;
+  out->slice_words = n1;
+  return FEAT_OK;
+}
+} // namespace
+
+extern "C" {
+
+int feat_matrix_dlinear_f64(int dim, uint32_t n1, int transpose, feat_matrix_f64_t* out)
+{
+  return dlinear_desc(dim, n1, transpose, out, "feat_matrix_dlinear_f64");
+}
+
+int feat_matrix_dlinear_f32(int dim, uint32_t n1, int transpose, feat_matrix_f32_t* out)
+{
+  return dlinear_desc(dim, n1, transpose, out, "feat_matrix_dlinear_f32");
+}
+
+} // extern "C"
+
+namespace {
 // Pair-dictionary plan of a scanned CSR (slice maxima hmax).  Returns FEAT_OK with *out
 // filled, or FEAT_ERR_UNSUPPORTED when the matrix has more than 256 distinct
 // (column - row, value) pairs (the caller then builds the plain plan).
@@ -2302,6 +2415,13 @@ int sell_destroy_impl(M* m)
 template<typename M>
 bool mat_ok(const M* A)
 {
+  if(A && A->format == FEAT_FMT_DLINEAR)
+  {
+    const uint64_t n1 = A->slice_words, nc = (n1 + 1) / 2;
+    const uint64_t nf = A->index == 3 ? n1 * n1 * n1 : n1 * n1, ncn = A->index == 3 ? nc * nc * nc : nc * nc;
+    return (A->index == 2 || A->index == 3) && n1 >= 3 && (n1 & 1) && (A->value_kind == 0 || A->value_kind == 1) &&
+           A->rows == (A->value_kind ? ncn : nf) && A->cols == (A->value_kind ? nf : ncn) && nf <= (uint64_t(1) << 62);
+  }
   return A && A->ptr && (A->nnz == 0 || (A->val && A->col)) && (A->format != FEAT_FMT_SELL || A->rowlen);
 }
 
@@ -2732,11 +2852,6 @@ __device__ __forceinline__ DT tail_row_a(const TailRow<DT> R, const int* meta, c
   return sum;
 }
 
-// Grid form of a fused level (dim 0: the plan and the CSR transfers)
-struct TailGeom
-{
-  int dim, n1;
-};
 
 // sum of a free row of an n1^dim grid level whose rows all hold the stencil sten (3^dim values
 // at the offsets (dz, dy, dx) in lexicographic order = CSR order), x in LDS
@@ -2762,42 +2877,16 @@ __device__ __forceinline__ DT tail_row(const TailRow<DT>& R, const int* meta, co
   return g.dim ? tail_row_grid(tab, g, int(i), x) : tail_row_a(R, meta, tab, i, x);
 }
 
-// row i of the d-linear injection from the next grid ((n1 + 1) / 2 per side) to this one: the
-// coarse nodes around the fine node in ascending order, each weighted with 1/2 per odd axis
 template<typename DT>
 __device__ __forceinline__ DT tail_prol_grid(TailGeom g, int i, const DT* x)
 {
-  const int n1 = g.n1, nc = (n1 + 1) / 2;
-  const int fx = i % n1, fy = (i / n1) % n1, fz = g.dim == 3 ? i / (n1 * n1) : 0;
-  const int ox = fx & 1, oy = fy & 1, oz = fz & 1;
-  const DT w = DT((ox ? 0.5 : 1.0) * (oy ? 0.5 : 1.0) * (oz ? 0.5 : 1.0));
-  DT s = DT(0);
-  for(int iz = 0; iz <= oz; ++iz)
-    for(int iy = 0; iy <= oy; ++iy)
-      for(int ix = 0; ix <= ox; ++ix) s = dfma(w, x[((fz >> 1) + iz) * nc * nc + ((fy >> 1) + iy) * nc + (fx >> 1) + ix], s);
-  return s;
+  return dlinear_prol_row(g, int64_t(i), x);
 }
 
-// row jc of its transpose: the fine nodes (2X + dx, 2Y + dy, 2Z + dz) inside the grid in
-// ascending order, weights 1/2 per nonzero offset
 template<typename DT>
 __device__ __forceinline__ DT tail_rest_grid(TailGeom g, int jc, const DT* x)
 {
-  const int n1 = g.n1, nc = (n1 + 1) / 2;
-  const int X = jc % nc, Y = (jc / nc) % nc, Z = g.dim == 3 ? jc / (nc * nc) : 0;
-  const int zl = g.dim == 3 ? -1 : 0, zh = g.dim == 3 ? 1 : 0;
-  DT s = DT(0);
-  for(int dz = zl; dz <= zh; ++dz)
-#pragma unroll
-    for(int dy = -1; dy <= 1; ++dy)
-#pragma unroll
-      for(int dx = -1; dx <= 1; ++dx)
-      {
-        const int fx = 2 * X + dx, fy = 2 * Y + dy, fz = 2 * Z + dz;
-        if(fx >= 0 && fx < n1 && fy >= 0 && fy < n1 && fz >= 0 && (g.dim == 2 || fz < n1))
-          s = dfma(DT((dx ? 0.5 : 1.0) * (dy ? 0.5 : 1.0) * (dz ? 0.5 : 1.0)), x[(fz * n1 + fy) * n1 + fx], s);
-      }
-  return s;
+  return dlinear_rest_row(g, int64_t(jc), x);
 }
 
 // sum of a transfer row from its FP64 CSR arrays; an FP32 cycle rounds each value to FP32

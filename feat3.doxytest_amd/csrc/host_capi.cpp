@@ -75,9 +75,9 @@ int feat_runtime_init(int device, void* stream)
   return guarded([&]() { Runtime::initialize(device, stream); });
 }
 
-int feat_runtime_tail_grid(int on)
+int feat_runtime_grid_forms(int on)
 {
-  return guarded([&]() { Runtime::tail_grid() = on != 0; });
+  return guarded([&]() { Runtime::grid_forms() = on != 0; });
 }
 
 int feat_runtime_sell_flags(int flags)

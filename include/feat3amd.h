@@ -232,6 +232,9 @@ typedef struct {            /* a CSR matrix on the device (u64 indices) */
  * indices are u32 (absolute) or i16 (col - row) when the bandwidth allows. */
 #define FEAT_FMT_CSR 0
 #define FEAT_FMT_SELL 1
+#define FEAT_FMT_DLINEAR 2   /* no arrays: d-linear injection between structured grids,
+                              * see feat_matrix_dlinear_f64 (index = dim, slice_words =
+                              * n1, value_kind = transpose)                          */
 #define FEAT_IDX_U64 0   /* CSR: u64 col/rowptr                              */
 #define FEAT_IDX_U32 1   /* CSR: u32 col/rowptr; SELL: u32 absolute columns  */
 #define FEAT_IDX_D16 2   /* SELL: int16 column minus row                      */
@@ -282,6 +285,18 @@ typedef struct {             /* FP32 arithmetic (mixed-precision inner solver) *
 
 /* A CSR descriptor (no copy) of the caller's arrays. */
 int feat_matrix_from_csr_f64(const feat_csr_f64_t* A, feat_matrix_f64_t* out);
+/* A matrix-free descriptor of the d-linear injection from the grid with (n1 + 1) / 2 nodes
+ * per side to the grid with n1 (odd, >= 3) per side, dim 2 or 3, nodes in lexicographic
+ * order.  transpose = 0: rows = the fine nodes, each taking the coarse nodes around it in
+ * ascending order with the weight 1/2 per odd axis (the prolongation); transpose = 1: its
+ * transpose (the restriction), each coarse node taking the fine nodes (2X + dx, ...),
+ * dx in {-1, 0, 1}, inside the grid in ascending order.  Row sums run in that order, which
+ * is the CSR order of the assembled Q1 grid transfer of such grids, so where every entry of
+ * an assembled transfer is this injection (the caller checks) the results are bit-identical
+ * to its CSR or SELL-64 plan.  Every entry point that takes a matrix descriptor accepts it.
+ * No counterpart in the reference (its transfers are assembled matrices). */
+int feat_matrix_dlinear_f64(int dim, uint32_t n1, int transpose, feat_matrix_f64_t* out);
+int feat_matrix_dlinear_f32(int dim, uint32_t n1, int transpose, feat_matrix_f32_t* out);
 /* Builds the SELL-64 plan of a CSR matrix on the device (setup time; allocates).
  * Fails with FEAT_ERR_UNSUPPORTED when a row has more than 65535 entries or the
  * slice padding would exceed the CSR size; the caller then keeps the CSR.

@@ -74,9 +74,10 @@ namespace feat3amd
       static int f = 0;
       return f;
     }
-    /// fused coarse levels in their grid form where the levels qualify (stencils and
-    /// d-linear transfers formed in the kernel; off: the plans and CSR transfers are read)
-    static bool& tail_grid()
+    /// multigrid grid forms where the levels qualify: d-linear transfers as FEAT_FMT_DLINEAR
+    /// descriptors, the fused coarse levels' stencils and transfers formed in the kernel
+    /// (off: the plans and CSR transfers are read)
+    static bool& grid_forms()
     {
       static bool on = true;
       return on;

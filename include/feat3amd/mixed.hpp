@@ -47,6 +47,7 @@ namespace feat3amd
       feat_mg_tail_f32_t _tail{};
       std::vector<TailGridForm> _grid;              // per level: grid form if it is fused
       std::vector<DeviceArray<float>> _grid_sten;   // its FP32 stencil (rounded as the FP32 plan rounds)
+      std::vector<feat_matrix_f32_t> _dl_P, _dl_R;  // d-linear transfer descriptors (where dlinear)
 
       /// The grid form of every level that the fused coarse part could hold (host copies of
       /// its matrices): at init_numeric, never inside a recorded graph.
@@ -55,18 +56,36 @@ namespace feat3amd
         const Index L = _h->size();
         _grid.assign(L, TailGridForm{});
         _grid_sten.assign(L, DeviceArray<float>());
-        for(Index s = 1; s + 1 < L; ++s)
+        _dl_P.assign(L, feat_matrix_f32_t{});
+        _dl_R.assign(L, feat_matrix_f32_t{});
+        for(Index s = 0; s + 1 < L; ++s)
         {
           auto& lv = _h->levels[s];
-          if(lv.matrix->rows() > FEAT_MG_TAIL_MAX_ROWS || !lv.filter) continue;
+          if(!lv.transfer || !lv.filter) continue;
+          const bool cand = s >= 1 && lv.matrix->rows() <= FEAT_MG_TAIL_MAX_ROWS;
           _grid[s] = tail_grid_form(*lv.matrix, lv.filter->mask(), lv.transfer->get_mat_prol(),
-                                    lv.transfer->get_mat_rest(), _h->levels[s + 1].matrix->rows());
+                                    lv.transfer->get_mat_rest(), _h->levels[s + 1].matrix->rows(), cand);
           if(!_grid[s].sten.empty())
           {
             std::vector<float> s32(_grid[s].sten.begin(), _grid[s].sten.end());
             _grid_sten[s] = DeviceArray<float>::from_host(s32.data(), s32.size());
           }
+          // an FP16-valued cycle keeps its FP16 transfer plans
+          if(_grid[s].dlinear && !_half)
+          {
+            check(feat_matrix_dlinear_f32(_grid[s].dim, _grid[s].n1, 0, &_dl_P[s]), "MixedMultiGrid d-linear prolongation");
+            check(feat_matrix_dlinear_f32(_grid[s].dim, _grid[s].n1, 1, &_dl_R[s]), "MixedMultiGrid d-linear restriction");
+          }
         }
+      }
+      /// the FP32 transfer descriptors of level i: d-linear where checked, else the plans
+      const feat_matrix_f32_t& _rest(Index i) const
+      {
+        return (i < _grid.size() && _grid[i].dlinear && !_half) ? _dl_R[i] : plan(_h->levels[i].transfer->get_mat_rest());
+      }
+      const feat_matrix_f32_t& _prol(Index i) const
+      {
+        return (i < _grid.size() && _grid[i].dlinear && !_half) ? _dl_P[i] : plan(_h->levels[i].transfer->get_mat_prol());
       }
 
       // execution plan of a level's matrix: FP32 values or FP16 values (FP32 arithmetic)
@@ -151,7 +170,7 @@ namespace feat3amd
           d.pre_omega = _l[s].omega_pre;
           d.post_omega = _l[s].omega_post;
           // grid form found at init_numeric (no copies here: this may run inside a recorded graph)
-          if(s < _grid.size())
+          if(s < _grid.size() && (_grid[s].dlinear || _grid_sten[s].size()))
           {
             d.dim = _grid[s].dim;
             d.n1 = _grid[s].n1;
@@ -293,7 +312,7 @@ namespace feat3amd
                                    l.rhs.elements(), nrm_of(i), l.sol.elements(), l.tmp.elements(), l.def.elements(),
                                    nullptr, ws),
                 "FP32 pre-smoothing");
-          const feat_matrix_f32_t& R = plan(lv.transfer->get_mat_rest());
+          const feat_matrix_f32_t& R = _rest(i);
           check(feat_apply_masked_f32(hd, &R, _l[i + 1].rhs.elements(), l.def.elements(), _h->levels[i + 1].filter->mask(),
                                       _fw.scalar(2 + i + 1), ws),
                 "FP32 restriction");
@@ -337,7 +356,7 @@ namespace feat3amd
         {
           auto& lv = _h->levels[k];
           Level32& l = _l[k];
-          const feat_matrix_f32_t& P = plan(lv.transfer->get_mat_prol());
+          const feat_matrix_f32_t& P = _prol(k);
           check(feat_prol_add_f32(hd, &P, l.sol.elements(), _l[k + 1].sol.elements(), lv.filter->mask()), "FP32 prolongation");
           const feat_matrix_f32_t& A = plan(*lv.matrix);
           double* dn = _fw.scalar(2 + L + k);

@@ -405,13 +405,15 @@ namespace feat3amd
     enum class MultiGridCycle { V };
 
     /// MultiGridHierarchy (solver/multigrid.hpp:419-957); level 0 is the finest.
-    /// Grid form of a fused coarse level (feat_mg_tail_level_t dim / n1 / sten / dlinear):
-    /// the level is an n1^dim grid (dim 2 or 3, lexicographic) and the next one has
-    /// (n1 + 1) / 2 nodes per side.  sten is set when every free row (not in the mask)
-    /// holds the same 3^dim values at the neighbour offsets in CSR order.  dlinear is set
-    /// when the prolongation from the next level is d-linear injection (the coarse nodes
-    /// around each fine node, ascending, weight 1/2 per odd axis) and the restriction is its
-    /// transpose.  Both are checked entry by entry on host copies of the matrices.
+    /// Grid forms of a multigrid level and its transfers.  The level is an n1^dim grid
+    /// (dim 2 or 3, lexicographic) and the next one has (n1 + 1) / 2 nodes per side.
+    ///  * dlinear: the prolongation from the next level is d-linear injection (the coarse
+    ///    nodes around each fine node, ascending, weight 1/2 per odd axis) and the
+    ///    restriction its transpose.  The transfers then run as FEAT_FMT_DLINEAR
+    ///    descriptors, and in the fused coarse levels as feat_mg_tail_level_t dlinear.
+    ///  * sten (fused coarse levels only): every free row (not in the mask) holds the same
+    ///    3^dim values at the neighbour offsets in CSR order (feat_mg_tail_level_t sten).
+    /// Both are checked entry by entry on host copies of the matrices.
     struct TailGridForm
     {
       int dim = 0;
@@ -422,7 +424,7 @@ namespace feat3amd
 
     template<typename Matrix_>
     TailGridForm tail_grid_form(const Matrix_& A, const std::uint32_t* mask_dev, const Matrix_& P, const Matrix_& R,
-                                Index n_next)
+                                Index n_next, bool want_stencil)
     {
       TailGridForm f;
       const Index n = A.rows();
@@ -443,11 +445,11 @@ namespace feat3amd
           }
         }
       }
-      if(!dim || !Runtime::tail_grid()) return f;
+      if(!dim || !Runtime::grid_forms()) return f;
       f.dim = dim;
       f.n1 = std::uint32_t(n1);
       auto down = [](auto* dst, const auto* src, std::size_t count) {
-        if(count) check(feat_copy(Runtime::handle(), dst, src, sizeof(*dst) * count), "tail grid form: copy");
+        if(count) check(feat_copy(Runtime::handle(), dst, src, sizeof(*dst) * count), "grid form: copy");
       };
       auto fetch = [&](const Matrix_& M, std::vector<double>& v, std::vector<std::uint64_t>& c,
                        std::vector<std::uint64_t>& p) {
@@ -460,10 +462,10 @@ namespace feat3amd
       };
       std::vector<double> av, pv, rv;
       std::vector<std::uint64_t> ac, ap, pc, pp, rc, rp;
-      fetch(A, av, ac, ap);
+      if(want_stencil) fetch(A, av, ac, ap);
       fetch(P, pv, pc, pp);
       fetch(R, rv, rc, rp);
-      std::vector<std::uint32_t> mask(mask_dev ? (n + 31) / 32 : 0, 0u);
+      std::vector<std::uint32_t> mask(want_stencil && mask_dev ? (n + 31) / 32 : 0, 0u);
       down(mask.data(), mask_dev, mask.size());
       Runtime::synchronize();
       const std::int64_t nn = n1 * n1, nc = (n1 + 1) / 2;
@@ -474,7 +476,7 @@ namespace feat3amd
         z = dim == 3 ? i / (side * side) : 0;
       };
       // the stencil of the free rows
-      bool sten_ok = true;
+      bool sten_ok = want_stencil;
       for(std::int64_t i = 0; i < std::int64_t(n) && sten_ok; ++i)
       {
         if(!mask.empty() && ((mask[i >> 5] >> (i & 31)) & 1u)) continue;
@@ -667,6 +669,7 @@ namespace feat3amd
       feat_mg_tail_t _tail{};
       std::vector<TailGridForm> _grid;               // per level: grid form if it is fused
       std::vector<DeviceArray<double>> _grid_sten;   // its stencil on the device (empty: none)
+      std::vector<feat_matrix_f64_t> _dl_P, _dl_R;   // d-linear transfer descriptors (where dlinear)
 
       /// The grid form of every level that the fused coarse part could hold, with host copies
       /// of its matrices: at init_numeric, never inside a recorded graph.
@@ -675,15 +678,32 @@ namespace feat3amd
         const Index L = _h->size();
         _grid.assign(L, TailGridForm{});
         _grid_sten.assign(L, DeviceArray<double>());
-        for(Index s = 1; s + 1 < L; ++s)
+        _dl_P.assign(L, feat_matrix_f64_t{});
+        _dl_R.assign(L, feat_matrix_f64_t{});
+        for(Index s = 0; s + 1 < L; ++s)
         {
           auto& l = _h->levels[s];
-          if(l.matrix->rows() > FEAT_MG_TAIL_MAX_ROWS || !l.filter) continue;
+          if(!l.transfer || !l.filter) continue;
+          const bool cand = s >= 1 && l.matrix->rows() <= FEAT_MG_TAIL_MAX_ROWS;
           _grid[s] = tail_grid_form(*l.matrix, l.filter->mask(), l.transfer->get_mat_prol(), l.transfer->get_mat_rest(),
-                                    _h->levels[s + 1].matrix->rows());
+                                    _h->levels[s + 1].matrix->rows(), cand);
           if(!_grid[s].sten.empty())
             _grid_sten[s] = DeviceArray<double>::from_host(_grid[s].sten.data(), _grid[s].sten.size());
+          if(_grid[s].dlinear)
+          {
+            check(feat_matrix_dlinear_f64(_grid[s].dim, _grid[s].n1, 0, &_dl_P[s]), "MultiGrid d-linear prolongation");
+            check(feat_matrix_dlinear_f64(_grid[s].dim, _grid[s].n1, 1, &_dl_R[s]), "MultiGrid d-linear restriction");
+          }
         }
+      }
+      /// the transfer descriptors of level i: d-linear where checked, else the plans
+      const feat_matrix_f64_t& _rest(Index i)
+      {
+        return (i < _grid.size() && _grid[i].dlinear) ? _dl_R[i] : _h->levels[i].transfer->get_mat_rest().exec();
+      }
+      const feat_matrix_f64_t& _prol(Index i)
+      {
+        return (i < _grid.size() && _grid[i].dlinear) ? _dl_P[i] : _h->levels[i].transfer->get_mat_prol().exec();
       }
       enum class Coarse { none, direct, jacobi, richardson, other } _coarse_kind = Coarse::other;
 
@@ -742,7 +762,7 @@ namespace feat3amd
           d.pre_omega = pre->omega();
           d.post_omega = post->omega();
           // grid form found at init_numeric (no copies here: this may run inside a recorded graph)
-          if(s < _grid.size())
+          if(s < _grid.size() && (_grid[s].dlinear || _grid_sten[s].size()))
           {
             d.dim = _grid[s].dim;
             d.n1 = _grid[s].n1;
@@ -935,7 +955,7 @@ namespace feat3amd
                        pre->omega(), rhs_of(i), nrm_of(i), sol_of(i), (double*)l.tmp.elements(),
                        (double*)l.def.elements(), nullptr, nullptr, nullptr, ws),
                 "MultiGrid pre-smoothing");
-          const feat_matrix_f64_t& R = l.transfer->get_mat_rest().exec();
+          const feat_matrix_f64_t& R = _rest(i);
           auto& lc = _h->levels[i + 1];
           // the next level's pre-smoother starts from the iterate the restriction forms
           double* x1 = (i + 1 < t) ? _x1_of(i + 1, sol_of(i + 1), true) : nullptr;
@@ -1005,7 +1025,7 @@ namespace feat3amd
           auto& l = _h->levels[k];
           auto* post = static_cast<RichType*>(l.post.get());
           auto* jac = post->fusable_jacobi();
-          const feat_matrix_f64_t& P = l.transfer->get_mat_prol().exec();
+          const feat_matrix_f64_t& P = _prol(k);
           check(feat_prol_add_f64(hd, &P, sol_of(k), sol_of(k + 1), l.filter->mask()), "MultiGrid prolongation");
           const feat_matrix_f64_t& A = l.matrix->exec();
           double* dn = _fw.scalar(2 + L + k);

@@ -165,9 +165,10 @@ int feat_runtime_init(int device, void* stream);
  * FEAT_SELL_NO_DICT keeps full values and column offsets even where the pair
  * dictionary applies).  No counterpart in the reference (format selection). */
 int feat_runtime_sell_flags(int flags);
-/* fused coarse levels of the multigrid solvers built from now on in their grid form where
- * the levels qualify (1, default) or always from the plans and CSR transfers (0) */
-int feat_runtime_tail_grid(int on);
+/* multigrid solvers initialised from now on use the grid forms where the levels qualify
+ * (1, default): d-linear transfers as FEAT_FMT_DLINEAR descriptors, the fused coarse levels'
+ * stencils and transfers formed in the kernel; 0: the plans and CSR transfers throughout */
+int feat_runtime_grid_forms(int on);
 
 /* ---- Stokes multigrid: Richardson(MG-V(Richardson(Schwarz(AmaVanka)))) ----
  * benchmarks/stokes_multigrid-bench.cpp:714-810 on levels level_fine..level_coarse

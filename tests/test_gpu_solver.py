@@ -5,6 +5,8 @@ Bars: fused kernels bit-exact except their dot/norm outputs (1e-13 relative);
 solver iteration counts equal to the oracle's within +-1; solution error
 |x - x_oracle| / |x_oracle| <= 1e-11; final relative defect <= tol.
 """
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -15,6 +17,7 @@ from tests.test_gpu_lafem import dev, host, upload_csr
 pytestmark = pytest.mark.gpu
 
 SOL_TOL = 1e-11
+C_byref = ctypes.byref
 
 
 def mask_of(torch, feat, handle, n, idx):
@@ -196,6 +199,52 @@ def stencil_csr(rng, rows, cols, offsets, values, p_keep):
         rp.append(len(col))
     val = rng.choice(np.asarray(values, np.float64), size=len(col))
     return Csr(rows, cols, val, np.asarray(col, np.uint64), np.asarray(rp, np.uint64))
+
+
+@pytest.mark.parametrize("dim,level", [(2, 1), (2, 5), (3, 1), (3, 4)])
+def test_dlinear_transfer_descriptor_bitwise(feat, handle, torch_gpu, oracle, dim, level):
+    """The matrix-free d-linear transfer descriptors (FEAT_FMT_DLINEAR) equal the assembled
+    Q1 prolongation from level-1 and its transpose row for row, bit for bit, in FP64 and
+    FP32, through the plain apply and the masked apply with its norm."""
+    print(f"dlinear transfer dim={dim} level={level}", flush=True)
+    torch = torch_gpu
+    k, _ = feat.libs()
+    P_ = feat.ptr
+    q = feat.q1_problem(dim, level)
+    n, nc = q["n"], q["pcols"]
+    Pm = Csr(n, nc, q["pval"], q["pcol"], q["prowptr"])
+    from oracle.pyoracle import csr_transpose
+    Rm = csr_transpose(Pm)
+    n1 = 2 ** level + 1
+    rng = np.random.default_rng(5 + level)
+    keep_ws, ws = feat.reduce_ws(torch, "cuda")
+    for transpose, M in ((0, Pm), (1, Rm)):
+        d = feat.Matrix()
+        feat.check(k.feat_matrix_dlinear_f64(dim, n1, transpose, C_byref(d)), "dlinear f64")
+        assert d.format == feat.FMT_DLINEAR and d.rows == M.rows and d.cols == M.cols
+        x = rng.standard_normal(M.cols)
+        xd = dev(torch, x)
+        rd = torch.empty(M.rows, dtype=torch.float64, device="cuda")
+        feat.check(k.feat_matrix_apply_f64(handle.h, C_byref(d), P_(rd), 1.0, P_(xd), 0.0, None), "apply")
+        handle.sync()
+        assert np.array_equal(host(rd), oracle.csr_apply(M, x)), transpose
+        nrm = torch.zeros(1, dtype=torch.float64, device="cuda")
+        feat.check(k.feat_apply_masked_f64(handle.h, C_byref(d), P_(rd), P_(xd), None, P_(nrm), None, None, ws), "masked")
+        handle.sync()
+        want = oracle.csr_apply(M, x)
+        assert np.array_equal(host(rd), want)
+        assert abs(host(nrm)[0] - float(want @ want)) <= 1e-13 * float(want @ want)
+        d32 = feat.Matrix()
+        feat.check(k.feat_matrix_dlinear_f32(dim, n1, transpose, C_byref(d32)), "dlinear f32")
+        x32 = dev(torch, x.astype(np.float32))
+        r32 = torch.empty(M.rows, dtype=torch.float32, device="cuda")
+        feat.check(k.feat_apply_masked_f32(handle.h, C_byref(d32), P_(r32), P_(x32), None, None, ws), "masked f32")
+        handle.sync()
+        assert np.array_equal(host(r32), oracle.csr_apply_f32(M, x.astype(np.float32))), transpose
+    # sizes that are no such grid pair are refused
+    d = feat.Matrix()
+    assert k.feat_matrix_dlinear_f64(dim, n1 + 1, 0, C_byref(d)) != 0
+    assert k.feat_matrix_dlinear_f64(4, n1, 0, C_byref(d)) != 0
 
 
 @pytest.mark.parametrize("diag", [True, False])
@@ -606,7 +655,7 @@ def test_coarse_fusion_bitwise(feat, torch_gpu, dim, lf, lc, first, precision):
         cfg = feat.mgpcg_cfg(nu_pre=4, nu_post=4, omega_jac=0.5, omega_rich=1.0, coarse="direct",
                              tol_rel=1e-8, max_iter=100, graphs=False, coarse_fusion=fusion,
                              precision=precision)
-        feat.runtime_tail_grid(grid)
+        feat.runtime_grid_forms(grid)
         try:
             s = feat.MgPcg(cfg, q1=(dim, lf, lc))
             assert s.coarse_fusion_level() == (first if fusion else 0)
@@ -617,7 +666,7 @@ def test_coarse_fusion_bitwise(feat, torch_gpu, dim, lf, lc, first, precision):
             torch.cuda.synchronize()
             assert s.coarse_fusion_grid() == ((lf - lc - first) if (fusion and grid) else 0)
         finally:
-            feat.runtime_tail_grid(True)
+            feat.runtime_grid_forms(True)
         s.close()
         assert r["status"] == "success"
         out.append((r, x.cpu().numpy()))
