@@ -655,7 +655,8 @@ struct TailGeom
 };
 
 // prolongation row i of the fine grid (n1 per side): the coarse nodes around the fine node in
-// ascending order, each weighted with 1/2 per odd axis
+// ascending order, each weighted with 1/2 per odd axis.  The (up to 8) loads are issued
+// together, then summed in that order.
 template<typename DT>
 __device__ __forceinline__ DT dlinear_prol_row(TailGeom g, int64_t i, const DT* x)
 {
@@ -663,32 +664,51 @@ __device__ __forceinline__ DT dlinear_prol_row(TailGeom g, int64_t i, const DT* 
   const int64_t fx = i % n1, fy = (i / n1) % n1, fz = g.dim == 3 ? i / (n1 * n1) : 0;
   const int ox = int(fx & 1), oy = int(fy & 1), oz = int(fz & 1);
   const DT w = DT((ox ? 0.5 : 1.0) * (oy ? 0.5 : 1.0) * (oz ? 0.5 : 1.0));
+  const int64_t c0 = (fz >> 1) * nc * nc + (fy >> 1) * nc + (fx >> 1);
+  DT v[8];
+#pragma unroll
+  for(int t = 0; t < 8; ++t)
+  {
+    const int ix = t & 1, iy = (t >> 1) & 1, iz = t >> 2;
+    const bool on = ix <= ox && iy <= oy && iz <= oz;
+    v[t] = on ? x[c0 + iz * nc * nc + iy * nc + ix] : DT(0);
+  }
   DT s = DT(0);
-  for(int iz = 0; iz <= oz; ++iz)
-    for(int iy = 0; iy <= oy; ++iy)
-      for(int ix = 0; ix <= ox; ++ix) s = dfma(w, x[((fz >> 1) + iz) * nc * nc + ((fy >> 1) + iy) * nc + (fx >> 1) + ix], s);
+#pragma unroll
+  for(int t = 0; t < 8; ++t)
+  {
+    const int ix = t & 1, iy = (t >> 1) & 1, iz = t >> 2;
+    if(ix <= ox && iy <= oy && iz <= oz) s = dfma(w, v[t], s);
+  }
   return s;
 }
 
 // restriction row jc of the coarse grid: the fine nodes (2X + dx, 2Y + dy, 2Z + dz) inside
-// the grid in ascending order, weights 1/2 per nonzero offset
+// the grid in ascending order, weights 1/2 per nonzero offset; the (up to 27) loads are
+// issued together, then summed in that order
 template<typename DT>
 __device__ __forceinline__ DT dlinear_rest_row(TailGeom g, int64_t jc, const DT* x)
 {
   const int64_t n1 = g.n1, nc = (n1 + 1) / 2;
   const int64_t X = jc % nc, Y = (jc / nc) % nc, Z = g.dim == 3 ? jc / (nc * nc) : 0;
-  const int zl = g.dim == 3 ? -1 : 0, zh = g.dim == 3 ? 1 : 0;
+  DT v[27];
+#pragma unroll
+  for(int t = 0; t < 27; ++t)
+  {
+    const int dx = t % 3 - 1, dy = (t / 3) % 3 - 1, dz = t / 9 - 1;
+    const int64_t fx = 2 * X + dx, fy = 2 * Y + dy, fz = 2 * Z + dz;
+    const bool on = fx >= 0 && fx < n1 && fy >= 0 && fy < n1 && (g.dim == 3 ? (fz >= 0 && fz < n1) : dz == 0);
+    v[t] = on ? x[(fz * n1 + fy) * n1 + fx] : DT(0);
+  }
   DT s = DT(0);
-  for(int dz = zl; dz <= zh; ++dz)
 #pragma unroll
-    for(int dy = -1; dy <= 1; ++dy)
-#pragma unroll
-      for(int dx = -1; dx <= 1; ++dx)
-      {
-        const int64_t fx = 2 * X + dx, fy = 2 * Y + dy, fz = 2 * Z + dz;
-        if(fx >= 0 && fx < n1 && fy >= 0 && fy < n1 && fz >= 0 && (g.dim == 2 || fz < n1))
-          s = dfma(DT((dx ? 0.5 : 1.0) * (dy ? 0.5 : 1.0) * (dz ? 0.5 : 1.0)), x[(fz * n1 + fy) * n1 + fx], s);
-      }
+  for(int t = 0; t < 27; ++t)
+  {
+    const int dx = t % 3 - 1, dy = (t / 3) % 3 - 1, dz = t / 9 - 1;
+    const int64_t fx = 2 * X + dx, fy = 2 * Y + dy, fz = 2 * Z + dz;
+    const bool on = fx >= 0 && fx < n1 && fy >= 0 && fy < n1 && (g.dim == 3 ? (fz >= 0 && fz < n1) : dz == 0);
+    if(on) s = dfma(DT((dx ? 0.5 : 1.0) * (dy ? 0.5 : 1.0) * (dz ? 0.5 : 1.0)), v[t], s);
+  }
   return s;
 }
 
