@@ -2890,11 +2890,15 @@ __device__ __forceinline__ DT tail_row_grid(const DT* sten, TailGeom g, int i, c
 }
 
 // the level's row: grid form or plan
-template<typename DT>
+// GRID: every fused level in the grid form (the plan code is left out of the kernel)
+template<bool GRID, typename DT>
 __device__ __forceinline__ DT tail_row(const TailRow<DT>& R, const int* meta, const DT* tab, TailGeom g, uint64_t i,
                                        const DT* x)
 {
-  return g.dim ? tail_row_grid(tab, g, int(i), x) : tail_row_a(R, meta, tab, i, x);
+  if constexpr(GRID)
+    return tail_row_grid(tab, g, int(i), x);
+  else
+    return g.dim ? tail_row_grid(tab, g, int(i), x) : tail_row_a(R, meta, tab, i, x);
 }
 
 template<typename DT>
@@ -2959,7 +2963,7 @@ __device__ __forceinline__ bool tail_skips(double nrm2)
 // Richardson(Jacobi) with `steps` steps from x = 0 on rhs (smooth_impl): with def the last
 // step writes the defect of x_steps into def and x_steps ends in b0; else the last update is
 // accumulated into acc.  b0/b1 are the ping-pong pair.  Thread = row `live ? threadIdx.x : -`.
-template<typename DT>
+template<bool GRID, typename DT>
 __device__ __forceinline__ void tail_smooth(const TailRow<DT> R, bool live, const int* meta, const DT* tab, TailGeom geo,
                                             int steps, DT omega, bool skip, const DT* rhs, DT* b0, DT* b1, DT* def, DT* acc)
 {
@@ -2988,7 +2992,7 @@ __device__ __forceinline__ void tail_smooth(const TailRow<DT> R, bool live, cons
     {
       // a filtered row's defect is zero whatever its sum: the sum is skipped
       DT d = DT(0);
-      if(!R.masked) d = dfma(tail_row(R, meta, tab, geo, i, xk), DT(-1), rhs[i]);
+      if(!R.masked) d = dfma(tail_row<GRID>(R, meta, tab, geo, i, xk), DT(-1), rhs[i]);
       if(mode == 1)
         out[i] = d;
       else
@@ -3015,7 +3019,7 @@ __host__ __device__ inline uint64_t tail_words_bytes(const TT& t)
 }
 
 // TT: feat_mg_tail_t (FP64 cycle) or feat_mg_tail_f32_t (FP32 cycle of the mixed multigrid)
-template<typename DT, typename TT>
+template<typename DT, typename TT, bool GRID>
 __global__ __launch_bounds__(kTailThreads) void k_mg_tail(TT t, const DT* __restrict__ rhs_top,
                                                           const double* __restrict__ nrm_top, DT* __restrict__ sol_top)
 {
@@ -3064,13 +3068,13 @@ __global__ __launch_bounds__(kTailThreads) void k_mg_tail(TT t, const DT* __rest
       // grid form: the stencil in the pair table's place, no code words
       geo[l] = TailGeom{L.sten ? L.dim : 0, int(L.n1)};
       meta[l] = L.sten ? nullptr : reinterpret_cast<const int*>(L.table + 256) + 256;
-      if(L.sten)
+      if(GRID || L.sten)
         for(int e = threadIdx.x; e < (L.dim == 3 ? 27 : 9); e += kTailThreads) tabs[256 * l + e] = L.sten[e];
       else
         for(int e = threadIdx.x; e < 256; e += kTailThreads) tabs[256 * l + e] = L.table[e];
       if(tid < L.n)
       {
-        if(!L.sten)
+        if(!GRID && !L.sten)
         {
           const uint32_t* cw = L.codes + (tid >> 6) * 64 * uint64_t(L.words) + (tid & 63);
 #pragma unroll
@@ -3103,7 +3107,7 @@ __global__ __launch_bounds__(kTailThreads) void k_mg_tail(TT t, const DT* __rest
     {
       const auto& L = t.lv[l];
       const bool live = tid < L.n;
-      tail_smooth(row[l], live, meta[l], tabs + 256 * l, geo[l], L.pre_steps, L.pre_omega, tail_skips<DT>(nrm[l]),
+      tail_smooth<GRID>(row[l], live, meta[l], tabs + 256 * l, geo[l], L.pre_steps, L.pre_omega, tail_skips<DT>(nrm[l]),
                   b[l].rhs, b[l].sol, b[l].tmp, b[l].def, (DT*)nullptr);
       const auto& C = t.lv[l + 1];
       double part = 0.0;
@@ -3112,9 +3116,14 @@ __global__ __launch_bounds__(kTailThreads) void k_mg_tail(TT t, const DT* __rest
         // a filtered row's value is zero whatever its sum: the sum is skipped
         DT v = DT(0);
         if(!is_masked(C.mask, tid))
-          v = dfma(L.dlinear ? tail_rest_grid(TailGeom{L.dim, int(L.n1)}, int(tid), b[l].def)
-                             : tail_row_csr(L.r_val, L.r_col, L.r_ptr, tid, b[l].def),
-                   DT(1), DT(0));
+        {
+          if constexpr(GRID)
+            v = dfma(tail_rest_grid(TailGeom{L.dim, int(L.n1)}, int(tid), b[l].def), DT(1), DT(0));
+          else
+            v = dfma(L.dlinear ? tail_rest_grid(TailGeom{L.dim, int(L.n1)}, int(tid), b[l].def)
+                               : tail_row_csr(L.r_val, L.r_col, L.r_ptr, tid, b[l].def),
+                     DT(1), DT(0));
+        }
         b[l + 1].rhs[tid] = v;
         part = double(v) * double(v);
       }
@@ -3152,9 +3161,14 @@ __global__ __launch_bounds__(kTailThreads) void k_mg_tail(TT t, const DT* __rest
       {
         DT cc = DT(0);
         if(!row[l].masked)
-          cc = dfma(L.dlinear ? tail_prol_grid(TailGeom{L.dim, int(L.n1)}, int(tid), b[l + 1].sol)
-                              : tail_row_csr(L.p_val, L.p_col, L.p_ptr, tid, b[l + 1].sol),
-                    DT(1), DT(0));
+        {
+          if constexpr(GRID)
+            cc = dfma(tail_prol_grid(TailGeom{L.dim, int(L.n1)}, int(tid), b[l + 1].sol), DT(1), DT(0));
+          else
+            cc = dfma(L.dlinear ? tail_prol_grid(TailGeom{L.dim, int(L.n1)}, int(tid), b[l + 1].sol)
+                                : tail_row_csr(L.p_val, L.p_col, L.p_ptr, tid, b[l + 1].sol),
+                      DT(1), DT(0));
+        }
         b[l].sol[tid] = dfma(DT(1), cc, b[l].sol[tid]);
       }
       __syncthreads();
@@ -3162,12 +3176,12 @@ __global__ __launch_bounds__(kTailThreads) void k_mg_tail(TT t, const DT* __rest
       if(live)
       {
         DT v = DT(0);
-        if(!row[l].masked) v = dfma(tail_row(row[l], meta[l], tabs + 256 * l, geo[l], tid, b[l].sol), DT(-1), b[l].rhs[tid]);
+        if(!row[l].masked) v = dfma(tail_row<GRID>(row[l], meta[l], tabs + 256 * l, geo[l], tid, b[l].sol), DT(-1), b[l].rhs[tid]);
         b[l].def[tid] = v;
         part = double(v) * double(v);
       }
       const double dn = tail_sum(part, red);
-      tail_smooth(row[l], live, meta[l], tabs + 256 * l, geo[l], L.post_steps, L.post_omega, tail_skips<DT>(dn),
+      tail_smooth<GRID>(row[l], live, meta[l], tabs + 256 * l, geo[l], L.post_steps, L.post_omega, tail_skips<DT>(dn),
                   b[l].def, b[l].cor, b[l].tmp, (DT*)nullptr, b[l].sol);
     }
   if(tid < t.lv[0].n) sol_top[tid] = b[0].sol[tid];
@@ -3204,7 +3218,13 @@ int mg_tail_launch(feat_handle_t h, const TT* t, const DT* rhs, const double* rh
   const uint64_t off = tail_words_bytes<DT>(*t), nc = t->lv[t->nlevels - 1].n;
   const uint64_t lds = off + nc * nc * sizeof(double) <= FEAT_MG_TAIL_MAX_LDS ? off + nc * nc * sizeof(double)
                                                                                : words * sizeof(DT);
-  hipLaunchKernelGGL((k_mg_tail<DT, TT>), dim3(1), dim3(kTailThreads), lds, h->stream, *t, rhs, rhs_nrm2, sol);
+  // every smoothed level in the grid form: the kernel without the plan code
+  bool grid = true;
+  for(int l = 0; l + 1 < t->nlevels; ++l) grid = grid && t->lv[l].sten && t->lv[l].dlinear;
+  if(grid)
+    hipLaunchKernelGGL((k_mg_tail<DT, TT, true>), dim3(1), dim3(kTailThreads), lds, h->stream, *t, rhs, rhs_nrm2, sol);
+  else
+    hipLaunchKernelGGL((k_mg_tail<DT, TT, false>), dim3(1), dim3(kTailThreads), lds, h->stream, *t, rhs, rhs_nrm2, sol);
   return check_launch(name);
 }
 
