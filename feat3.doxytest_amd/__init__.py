@@ -810,7 +810,7 @@ class Stokes:
 class BatchCfg(C.Structure):
     _fields_ = [("seed", _u64), ("ninst", _i), ("lmin", _i), ("lmax", _i), ("sigma", _d), ("tol_rel", _d),
                 ("max_iter", _u64), ("autotune", _i), ("graphs", _i), ("tail_levels", _i),
-                ("no_tiled", _i), ("tiled_prol_arrays", _i), ("tail_ell", _i)]
+                ("no_tiled", _i), ("tiled_prol_arrays", _i), ("tail_ell", _i), ("no_tiled_restriction", _i)]
 
 
 def batch_levels(seed, ninst, lmin, lmax):
@@ -889,12 +889,14 @@ class BatchSolver:
     """The batch of independent Q1 diffusion MG-PCG solves (C++ host layer)."""
 
     def __init__(self, seed=28054777172512, ninst=64, lmin=3, lmax=8, sigma=1.0, tol_rel=1e-8, max_iter=100,
-                 autotune=True, graphs=True, tail_levels=5, tiled=True, tiled_prol_arrays=False, tail_ell=False):
+                 autotune=True, graphs=True, tail_levels=5, tiled=True, tiled_prol_arrays=False, tail_ell=False,
+                 tiled_restriction=True):
         _, self.hl = libs()
         runtime_init()
         self.s = C.c_void_p()
         self.cfg = BatchCfg(seed, ninst, lmin, lmax, sigma, tol_rel, max_iter, int(autotune), int(graphs),
-                            int(tail_levels), int(not tiled), int(tiled_prol_arrays), int(tail_ell))
+                            int(tail_levels), int(not tiled), int(tiled_prol_arrays), int(tail_ell),
+                            int(not tiled_restriction))
         check(self.hl.feat_batchsolver_create(C.byref(self.s), C.byref(self.cfg)), "feat_batchsolver_create")
         self.levels = np.zeros(ninst, np.int32)
         self.off = np.zeros(ninst + 1, np.uint64)

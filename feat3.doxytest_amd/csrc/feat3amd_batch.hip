@@ -829,13 +829,17 @@ __global__ __launch_bounds__(kBt) void k_batch_tail_grid(feat_batch_tail_t Targ,
 // results are bit-identical to the SELL-64 launches of the same calls.
 constexpr int kTl = 512;
 constexpr int kTlTile = 32;
-constexpr int kTlHalo = FEAT_BATCH_TILED_MAX_STEPS;
-constexpr int kTlReg = kTlTile + 2 * kTlHalo;
-constexpr int kTlPts = (kTlReg * kTlReg + kTl - 1) / kTl;
+constexpr int kTlPts = 4;   // region nodes per thread (42 x 42 <= 4 * 512)
 
-template<bool POST>
+// REST (pre only): the restriction of the defect to the next coarser level as well, for the
+// coarse nodes whose fine centre lies in the tile; the defect of the tile's neighbours is then
+// needed exact, so the halo is one node wider
+template<bool POST, bool REST = false>
 __global__ __launch_bounds__(kTl) void k_batch_tiled(feat_batch_tiled_t t)
 {
+  constexpr int kTlHalo = FEAT_BATCH_TILED_MAX_STEPS + (REST ? 1 : 0);
+  constexpr int kTlReg = kTlTile + 2 * kTlHalo;
+  static_assert(kTlReg * kTlReg <= kTlPts * kTl, "region nodes per thread");
   __shared__ double s_b[POST ? 3 : 2][kTlReg * kTlReg];
   const int n1 = int(t.n1);
   const int tiles = (n1 + kTlTile - 1) / kTlTile;
@@ -946,8 +950,40 @@ __global__ __launch_bounds__(kTl) void k_batch_tiled(feat_batch_tiled_t t)
       if(out[q])
       {
         t.sol[base + row[q]] = xl[li[q]];
-        t.def[base + row[q]] = dq[q];
+        if(!REST) t.def[base + row[q]] = dq[q];
       }
+    if constexpr(REST)
+    {
+      // the defect in the iterate buffer no longer read, then rhs_c = filter(R def) for the
+      // coarse nodes (X, Y) with (2X, 2Y) in the tile: the fine nodes around (2X, 2Y) inside
+      // the grid in ascending order, weights 1/2 per nonzero offset (EpiMasked)
+      double* dl = buf(nu + 1);
+#pragma unroll
+      for(int q = 0; q < kTlPts; ++q)
+        if(on[q]) dl[li[q]] = dq[q];
+      __syncthreads();
+      const int nc1 = (n1 + 1) / 2;
+      const int X0 = tx0 / 2, Y0 = ty0 / 2;
+      const int cw = (min(tx0 + kTlTile, n1) - 1) / 2 - X0 + 1, chh = (min(ty0 + kTlTile, n1) - 1) / 2 - Y0 + 1;
+      for(int e = tid; e < cw * chh; e += kTl)
+      {
+        const int X = X0 + e % cw, Y = Y0 + e / cw;
+        double sum = 0.0;
+#pragma unroll
+        for(int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+          for(int dx = -1; dx <= 1; ++dx)
+          {
+            const int gx = 2 * X + dx, gy = 2 * Y + dy;
+            if(gx >= 0 && gx < n1 && gy >= 0 && gy < n1)
+              sum = dfma((dx ? 0.5 : 1.0) * (dy ? 0.5 : 1.0), dl[(gy - ry0) * rw + (gx - rx0)], sum);
+          }
+        const uint64_t gc = t.coff[p] + uint64_t(Y) * nc1 + X;
+        double v = dfma(sum, 1.0, 0.0);
+        if(is_masked(t.cmask, gc)) v = 0.0;
+        t.crhs[gc] = v;
+      }
+    }
   }
   else
   {
@@ -1062,8 +1098,12 @@ int tiled_launch(feat_handle_t h, const feat_batch_tiled_t* t, bool post, const 
   if(t->n1 < 3 || t->n1 > 65535 || t->nu < 1 || t->nu > FEAT_BATCH_TILED_MAX_STEPS)
     return fail(FEAT_ERR_INVALID, "batch tiled smoother: grid size or steps out of range");
   if(!t->ninst) return FEAT_OK;
-  if(!t->sval || !t->inv || !t->mask || !t->rhs || !t->sol || (!post && !t->def))
+  if(!t->sval || !t->inv || !t->mask || !t->rhs || !t->sol || (!post && !t->rest && !t->def))
     return fail(FEAT_ERR_INVALID, "batch tiled smoother: null array");
+  if(!post && t->rest && (!t->coff || !t->crhs || !t->cmask))
+    return fail(FEAT_ERR_INVALID, "batch tiled smoother: restriction target missing");
+  if(!post && t->rest && (static_cast<const void*>(t->crhs) == t->sol || t->crhs == t->rhs))
+    return fail(FEAT_ERR_ALIAS, "batch tiled smoother: restriction target aliases a level vector");
   if(post && (!t->xc || !t->coff ||
               (!t->p_struct && (!t->p_val || !t->p_col || !t->p_len || t->wp > FEAT_BATCH_TAIL_MAX_PROL_LEN))))
     return fail(FEAT_ERR_INVALID, "batch tiled smoother: prolongation incomplete");
@@ -1078,9 +1118,11 @@ int tiled_launch(feat_handle_t h, const feat_batch_tiled_t* t, bool post, const 
   const uint64_t grid = tiles * tiles * t->ninst;
   if(grid > 0x7fffffffull) return fail(FEAT_ERR_INVALID, "batch tiled smoother: grid too large");
   if(post)
-    hipLaunchKernelGGL(k_batch_tiled<true>, dim3(uint32_t(grid)), dim3(kTl), 0, h->stream, *t);
+    hipLaunchKernelGGL((k_batch_tiled<true>), dim3(uint32_t(grid)), dim3(kTl), 0, h->stream, *t);
+  else if(t->rest)
+    hipLaunchKernelGGL((k_batch_tiled<false, true>), dim3(uint32_t(grid)), dim3(kTl), 0, h->stream, *t);
   else
-    hipLaunchKernelGGL(k_batch_tiled<false>, dim3(uint32_t(grid)), dim3(kTl), 0, h->stream, *t);
+    hipLaunchKernelGGL((k_batch_tiled<false>), dim3(uint32_t(grid)), dim3(kTl), 0, h->stream, *t);
   return check_launch(name);
 }
 

@@ -807,6 +807,7 @@ int feat_batchsolver_create(feat_batchsolver_t* out, const feat_batch_cfg_t* cfg
     c.tiled = cfg->no_tiled == 0;
     c.tiled_prol_arrays = cfg->tiled_prol_arrays != 0;
     c.tail_ell = cfg->tail_ell != 0;
+    c.tiled_restriction = cfg->no_tiled_restriction == 0;
     auto s = std::make_unique<feat_batchsolver_s>();
     s->b = std::make_unique<Solver::BatchedMgPcg>(c);
     s->b->init();

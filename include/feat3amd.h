@@ -760,6 +760,14 @@ typedef struct {
    * the weights 1, 1/2, 1/4 (checked by the caller): formed from the node position, the
    * p_* arrays are not read */
   int p_struct;
+  /* pre only, 1: the restriction of the defect to level k-1 in the same launch instead of
+   * def (not written): crhs[coff[p] + Y (n1+1)/2 + X] = filter(R def) with R the transpose
+   * of bilinear injection (the fine nodes around (2X, 2Y) inside the grid in ascending
+   * order, weights 1/2 per nonzero offset; checked by the caller), filtered by cmask, as
+   * feat_apply_masked_f64 forms it */
+  int rest;
+  double* crhs;
+  const uint32_t* cmask;
 } feat_batch_tiled_t;
 int feat_batch_pre_tiled_f64(feat_handle_t h, const feat_batch_tiled_t* t);
 int feat_batch_post_tiled_f64(feat_handle_t h, const feat_batch_tiled_t* t);

@@ -52,6 +52,9 @@ namespace feat3amd
       bool tiled = true;
       /// the tiled post-smoothing reads the prolongation's arrays even where it is bilinear injection
       bool tiled_prol_arrays = false;
+      /// the restriction of a tiled level in its pre-smoothing launch (where it is the
+      /// transpose of bilinear injection; else a SELL-64 launch of its own)
+      bool tiled_restriction = true;
       /// the coarse tail in ELL form even where its levels are structured grids
       bool tail_ell = false;
     };
@@ -185,6 +188,7 @@ namespace feat3amd
         DeviceArray<std::uint64_t> coff;
         DeviceArray<std::uint64_t> top;  // per position: PCG offset of an instance entering here, else ~0
         bool p_struct = false;           // the prolongation is bilinear injection (no P arrays read)
+        bool r_struct = false;           // the restriction is its transpose: fused into the pre-smoothing
       };
       std::vector<TiledLevel> _tiled;
 
@@ -590,6 +594,8 @@ namespace feat3amd
               for(int i : L->inst) cl.push_back(C.loff[std::size_t(std::find(C.inst.begin(), C.inst.end(), i) - C.inst.begin())]);
               TiledLevel& tk = _tiled[k];
               tk.p_struct = !_cfg.tiled_prol_arrays && bilinear_prolongation(P1, tk.n1);
+              tk.r_struct = _cfg.tiled_restriction && bilinear_prolongation(P1, tk.n1) &&
+                            bilinear_restriction(Assembly::transpose(P1), tk.n1);
               tk.P = to_ell(Pk, L->loff, cl, nk, C.A.rows() / C.inst.size(), true);
               if(tk.P.ok && tk.P.w <= FEAT_BATCH_TAIL_MAX_PROL_LEN)
               {
@@ -671,6 +677,9 @@ namespace feat3amd
         d.p_len = t.P.dlen.get();
         d.top = t.top.get();
         d.p_struct = t.p_struct ? 1 : 0;
+        d.rest = t.r_struct ? 1 : 0;
+        d.crhs = _lv[k - 1]->rhs.elements();
+        d.cmask = _lv[k - 1]->mask.get();
         d.r = r;
         d.z = z;
         return d;
@@ -714,10 +723,13 @@ namespace feat3amd
                   "batch pre-smoothing");
           }
           Level& C = *_lv[k - 1];
-          const feat_matrix_f64_t& R = L.R.exec();
-          check(feat_apply_masked_f64(h, &R, C.rhs.elements(), L.def.elements(), C.mask.get(), nullptr, nullptr,
-                                      nullptr, ws),
-                "batch restriction");
+          if(!(_tiled[k].n1 && _tiled[k].r_struct))   // else done by the tiled pre-smoothing
+          {
+            const feat_matrix_f64_t& R = L.R.exec();
+            check(feat_apply_masked_f64(h, &R, C.rhs.elements(), L.def.elements(), C.mask.get(), nullptr, nullptr,
+                                        nullptr, ws),
+                  "batch restriction");
+          }
         }
         if(T > 0)
           check(feat_batch_tail_f64(h, &_tail, r, z), "batch coarse tail");

@@ -228,6 +228,8 @@ typedef struct {
   int tiled_prol_arrays;  /* 1: the tiled post-smoothing reads the prolongation's ELL arrays
                              even where it is bilinear injection (formed from the node position) */
   int tail_ell;           /* 1: the coarse tail in ELL form even where its levels are structured grids */
+  int no_tiled_restriction;  /* 1: the restriction of a tiled level as a launch of its own instead of
+                                inside its tiled pre-smoothing */
 } feat_batch_cfg_t;
 typedef struct feat_batchsolver_s* feat_batchsolver_t;
 int feat_batchsolver_create(feat_batchsolver_t* s, const feat_batch_cfg_t* cfg);

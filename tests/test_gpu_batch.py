@@ -110,14 +110,17 @@ def test_batch_coarse_tail_bitwise(feat, torch_gpu, ninst, lmin, lmax, tail, dep
         ref.close()
 
 
-@pytest.mark.parametrize("tail,prol_arrays", [(5, False), (0, False), (5, True)])
-def test_batch_tiled_smoother_bitwise(feat, torch_gpu, tail, prol_arrays):
+@pytest.mark.parametrize("tail,prol_arrays,rest", [(5, False, True), (0, False, True), (5, True, True),
+                                                  (5, False, False)])
+def test_batch_tiled_smoother_bitwise(feat, torch_gpu, tail, prol_arrays, rest):
     # tiled smoother calls on the levels above the tail (tail 0: levels 2..6, down to the
     # 5 x 5 grid of level 2, one clipped tile) against the SELL-64 launches; the
-    # prolongation formed from the node position, or read from its ELL arrays
+    # prolongation formed from the node position, or read from its ELL arrays; the
+    # restriction inside the tiled pre-smoothing, or a launch of its own
     torch = torch_gpu
     ref = feat.BatchSolver(SEED, ninst=12, lmin=3, lmax=6, tail_levels=tail, tiled=False)
-    bs = feat.BatchSolver(SEED, ninst=12, lmin=3, lmax=6, tail_levels=tail, tiled_prol_arrays=prol_arrays)
+    bs = feat.BatchSolver(SEED, ninst=12, lmin=3, lmax=6, tail_levels=tail, tiled_prol_arrays=prol_arrays,
+                          tiled_restriction=rest)
     try:
         assert ref.tiled_levels == []
         assert bs.tiled_levels == ([6] if tail else [2, 3, 4, 5, 6])
