@@ -657,14 +657,15 @@ struct TailGeom
 // prolongation row i of the fine grid (n1 per side): the coarse nodes around the fine node in
 // ascending order, each weighted with 1/2 per odd axis.  The (up to 8) loads are issued
 // together, then summed in that order.
-template<typename DT>
-__device__ __forceinline__ DT dlinear_prol_row(TailGeom g, int64_t i, const DT* x)
+// I: the index type (int32_t where the grid has fewer than 2^31 nodes: cheaper divisions)
+template<typename DT, typename I = int64_t>
+__device__ __forceinline__ DT dlinear_prol_row(TailGeom g, I i, const DT* x)
 {
-  const int64_t n1 = g.n1, nc = (n1 + 1) / 2;
-  const int64_t fx = i % n1, fy = (i / n1) % n1, fz = g.dim == 3 ? i / (n1 * n1) : 0;
+  const I n1 = g.n1, nc = (n1 + 1) / 2;
+  const I fx = i % n1, fy = (i / n1) % n1, fz = g.dim == 3 ? i / (n1 * n1) : 0;
   const int ox = int(fx & 1), oy = int(fy & 1), oz = int(fz & 1);
   const DT w = DT((ox ? 0.5 : 1.0) * (oy ? 0.5 : 1.0) * (oz ? 0.5 : 1.0));
-  const int64_t c0 = (fz >> 1) * nc * nc + (fy >> 1) * nc + (fx >> 1);
+  const I c0 = (fz >> 1) * nc * nc + (fy >> 1) * nc + (fx >> 1);
   DT v[8];
 #pragma unroll
   for(int t = 0; t < 8; ++t)
@@ -686,17 +687,17 @@ __device__ __forceinline__ DT dlinear_prol_row(TailGeom g, int64_t i, const DT* 
 // restriction row jc of the coarse grid: the fine nodes (2X + dx, 2Y + dy, 2Z + dz) inside
 // the grid in ascending order, weights 1/2 per nonzero offset; the (up to 27) loads are
 // issued together, then summed in that order
-template<typename DT>
-__device__ __forceinline__ DT dlinear_rest_row(TailGeom g, int64_t jc, const DT* x)
+template<typename DT, typename I = int64_t>
+__device__ __forceinline__ DT dlinear_rest_row(TailGeom g, I jc, const DT* x)
 {
-  const int64_t n1 = g.n1, nc = (n1 + 1) / 2;
-  const int64_t X = jc % nc, Y = (jc / nc) % nc, Z = g.dim == 3 ? jc / (nc * nc) : 0;
+  const I n1 = g.n1, nc = (n1 + 1) / 2;
+  const I X = jc % nc, Y = (jc / nc) % nc, Z = g.dim == 3 ? jc / (nc * nc) : 0;
   DT v[27];
 #pragma unroll
   for(int t = 0; t < 27; ++t)
   {
     const int dx = t % 3 - 1, dy = (t / 3) % 3 - 1, dz = t / 9 - 1;
-    const int64_t fx = 2 * X + dx, fy = 2 * Y + dy, fz = 2 * Z + dz;
+    const I fx = 2 * X + dx, fy = 2 * Y + dy, fz = 2 * Z + dz;
     const bool on = fx >= 0 && fx < n1 && fy >= 0 && fy < n1 && (g.dim == 3 ? (fz >= 0 && fz < n1) : dz == 0);
     v[t] = on ? x[(fz * n1 + fy) * n1 + fx] : DT(0);
   }
@@ -705,14 +706,14 @@ __device__ __forceinline__ DT dlinear_rest_row(TailGeom g, int64_t jc, const DT*
   for(int t = 0; t < 27; ++t)
   {
     const int dx = t % 3 - 1, dy = (t / 3) % 3 - 1, dz = t / 9 - 1;
-    const int64_t fx = 2 * X + dx, fy = 2 * Y + dy, fz = 2 * Z + dz;
+    const I fx = 2 * X + dx, fy = 2 * Y + dy, fz = 2 * Z + dz;
     const bool on = fx >= 0 && fx < n1 && fy >= 0 && fy < n1 && (g.dim == 3 ? (fz >= 0 && fz < n1) : dz == 0);
     if(on) s = dfma(DT((dx ? 0.5 : 1.0) * (dy ? 0.5 : 1.0) * (dz ? 0.5 : 1.0)), v[t], s);
   }
   return s;
 }
 
-template<typename DT, class Epi, bool REST>
+template<typename DT, class Epi, bool REST, typename I>
 __global__ __launch_bounds__(kThreads) void k_dlinear_rows(int dim, int n1, const DT* __restrict__ x, uint64_t rows,
                                                            Epi epi, GridReduce gr)
 {
@@ -723,7 +724,7 @@ __global__ __launch_bounds__(kThreads) void k_dlinear_rows(int dim, int n1, cons
   {
     typename EpiPre<Epi>::type pf{};
     if constexpr(EpiPre<Epi>::on) pf = epi.pre(i);
-    const DT sum = REST ? dlinear_rest_row(g, int64_t(i), x) : dlinear_prol_row(g, int64_t(i), x);
+    const DT sum = REST ? dlinear_rest_row<DT, I>(g, I(i), x) : dlinear_prol_row<DT, I>(g, I(i), x);
     if constexpr(EpiPre<Epi>::on)
       epi.row(i, sum, pf, acc0, acc1);
     else
@@ -742,11 +743,19 @@ int launch_mat(hipStream_t s, const M* A, const DT* x, const Epi& epi, const Gri
     const uint64_t nb = (A->rows + kThreads - 1) / kThreads;
     const int grid = int(nb < uint64_t(kMaxGrid) ? nb : uint64_t(kMaxGrid));
     const GridReduce gd = reduce_for_grid(gr, uint64_t(grid));
-    if(A->value_kind)
-      hipLaunchKernelGGL((k_dlinear_rows<DT, Epi, true>), dim3(grid), dim3(kThreads), 0, s, A->index,
+    // 32-bit index arithmetic while the fine grid has fewer than 2^31 nodes
+    const bool small = A->rows < (uint64_t(1) << 31) && A->cols < (uint64_t(1) << 31);
+    if(A->value_kind && small)
+      hipLaunchKernelGGL((k_dlinear_rows<DT, Epi, true, int32_t>), dim3(grid), dim3(kThreads), 0, s, A->index,
+                         int(A->slice_words), x, A->rows, epi, gd);
+    else if(A->value_kind)
+      hipLaunchKernelGGL((k_dlinear_rows<DT, Epi, true, int64_t>), dim3(grid), dim3(kThreads), 0, s, A->index,
+                         int(A->slice_words), x, A->rows, epi, gd);
+    else if(small)
+      hipLaunchKernelGGL((k_dlinear_rows<DT, Epi, false, int32_t>), dim3(grid), dim3(kThreads), 0, s, A->index,
                          int(A->slice_words), x, A->rows, epi, gd);
     else
-      hipLaunchKernelGGL((k_dlinear_rows<DT, Epi, false>), dim3(grid), dim3(kThreads), 0, s, A->index,
+      hipLaunchKernelGGL((k_dlinear_rows<DT, Epi, false, int64_t>), dim3(grid), dim3(kThreads), 0, s, A->index,
                          int(A->slice_words), x, A->rows, epi, gd);
     finish_reduce(s, gd, uint64_t(grid));
     return check_launch(name);
@@ -2904,13 +2913,13 @@ __device__ __forceinline__ DT tail_row(const TailRow<DT>& R, const int* meta, co
 template<typename DT>
 __device__ __forceinline__ DT tail_prol_grid(TailGeom g, int i, const DT* x)
 {
-  return dlinear_prol_row(g, int64_t(i), x);
+  return dlinear_prol_row<DT, int>(g, i, x);
 }
 
 template<typename DT>
 __device__ __forceinline__ DT tail_rest_grid(TailGeom g, int jc, const DT* x)
 {
-  return dlinear_rest_row(g, int64_t(jc), x);
+  return dlinear_rest_row<DT, int>(g, jc, x);
 }
 
 // sum of a transfer row from its FP64 CSR arrays; an FP32 cycle rounds each value to FP32
