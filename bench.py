@@ -410,7 +410,8 @@ def mixed_varcoef_leg(feat, torch, dev, args):
     """Mixed precision where the matrices are the traffic: the 3D Q1 anisotropic diffusion
     problem of the learned-smoother leg (log-normal sigma 1.5, K = diag(1, 1, 10)) at level 7
     (2,146,689 dofs).  Variable coefficients leave no pair dictionary: every level runs on
-    plain SELL-64 plans (FP64 8 B values, FP32 4 B, FP16 2 B, plus 2 B column offsets).
+    banded plans (the grid-stencil rows: values only, columns from the node position; FP64
+    8 B, FP32 4 B, FP16 2 B per entry; the plain SELL-64 plan adds 2 B column offsets).
     PCG + MG V(4,4) Richardson(Jacobi 0.5), coarse level 1, tol 1e-8 on the FP64 defect."""
     import scipy.sparse as sp
     t0 = time.perf_counter()

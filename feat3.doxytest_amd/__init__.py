@@ -53,9 +53,9 @@ class Matrix(C.Structure):
                 ("scale", _f), ("tune", _i), ("slice_words", C.c_uint32)]
 
 
-FMT_CSR, FMT_SELL, FMT_DLINEAR = 0, 1, 2
+FMT_CSR, FMT_SELL, FMT_DLINEAR, FMT_BANDED = 0, 1, 2, 3
 IDX_U64, IDX_U32, IDX_D16, IDX_DICT, IDX_DIAG = 0, 1, 2, 3, 4
-SELL_NO_DICT, SELL_NO_DIAG = 1, 2
+SELL_NO_DICT, SELL_NO_DIAG, SELL_NO_BANDED = 1, 2, 4
 VAL_NATIVE, VAL_F16, VAL_CODE8 = 0, 1, 2
 Q1_HOST_ASSEMBLY = 1
 
@@ -141,6 +141,7 @@ KERNEL_API = [
     ("feat_sell_create_ex_f64", _i, [_vp, C.POINTER(CsrF64), _i, C.POINTER(Matrix)]),
     ("feat_sell_create_ex_f32", _i, [_vp, C.POINTER(CsrF64), _i, C.POINTER(Matrix)]),
     ("feat_sell_create_f16", _i, [_vp, C.POINTER(CsrF64), C.POINTER(Matrix)]),
+    ("feat_sell_create_ex_f16", _i, [_vp, C.POINTER(CsrF64), _i, C.POINTER(Matrix)]),
     ("feat_sell_destroy_f64", _i, [C.POINTER(Matrix)]),
     ("feat_sell_destroy_f32", _i, [C.POINTER(Matrix)]),
     ("feat_matrix_apply_f64", _i, [_vp, C.POINTER(Matrix), _vp, _d, _vp, _d, _vp]),
@@ -672,22 +673,21 @@ def csr_matrix(val, col, rowptr, rows, cols):
 class SellPlan:
     """SELL-64 plan of a device CSR (f64 values with u64 indices); fp='f32' rounds values.
     dict=False keeps full values and column offsets where the pair dictionary applies;
-    diag=False keeps the dictionary in entry order where the diagonal layout applies."""
+    diag=False keeps the dictionary in entry order where the diagonal layout applies;
+    banded=False keeps the plain plan where the banded plan (grid stencil rows) applies."""
 
-    def __init__(self, handle, val, col, rowptr, rows, cols, fp="f64", dict=True, diag=True):
+    def __init__(self, handle, val, col, rowptr, rows, cols, fp="f64", dict=True, diag=True, banded=True):
         self.k, _ = libs()
         self.fp = fp
         c = csr_desc(val, col, rowptr, rows, cols)
         self.m = Matrix()
-        flags = (0 if dict else SELL_NO_DICT) | (0 if diag else SELL_NO_DIAG)
-        if fp == "f16":
-            check(self.k.feat_sell_create_f16(handle.h, C.byref(c), C.byref(self.m)), "feat_sell_create_f16")
-        else:
-            fn = {"f64": self.k.feat_sell_create_ex_f64, "f32": self.k.feat_sell_create_ex_f32}[fp]
-            check(fn(handle.h, C.byref(c), flags, C.byref(self.m)), fn.__name__)
+        flags = (0 if dict else SELL_NO_DICT) | (0 if diag else SELL_NO_DIAG) | (0 if banded else SELL_NO_BANDED)
+        fn = {"f64": self.k.feat_sell_create_ex_f64, "f32": self.k.feat_sell_create_ex_f32,
+              "f16": self.k.feat_sell_create_ex_f16}[fp]
+        check(fn(handle.h, C.byref(c), flags, C.byref(self.m)), fn.__name__)
 
     def close(self):
-        if self.m.format == FMT_SELL:
+        if self.m.format in (FMT_SELL, FMT_BANDED):
             fn = self.k.feat_sell_destroy_f64 if self.fp == "f64" else self.k.feat_sell_destroy_f32
             fn(C.byref(self.m))
 

@@ -380,6 +380,16 @@ __device__ __forceinline__ float ld_val<float, uint16_t>(const uint16_t* p)
   return float(__builtin_bit_cast(_Float16, __builtin_nontemporal_load(p)));
 }
 
+// a stored value already loaded -> arithmetic type
+template<typename DT, typename VT>
+__device__ __forceinline__ DT cvt_val(VT v)
+{
+  if constexpr(std::is_same<VT, uint16_t>::value)
+    return DT(__builtin_bit_cast(_Float16, v));
+  else
+    return DT(v);
+}
+
 // VCODE: val holds 1-byte codes into vtab (256 values, staged in LDS), else the values.
 template<typename DT, typename VT, typename CT, bool DELTA, class Epi, int UNR = kSellUnroll, bool VCODE = false>
 __global__ __launch_bounds__(kThreads) void k_sell_rows(const VT* __restrict__ val,
@@ -733,6 +743,79 @@ __global__ __launch_bounds__(kThreads) void k_dlinear_rows(int dim, int n1, cons
   grid_reduce_finish(gr, acc0, acc1);
 }
 
+// Banded plan (FEAT_FMT_BANDED): row i of an n1^DIM grid takes its neighbours (i+dx, j+dy,
+// k+dz) inside the grid, columns formed from the node position, values slot-major.  All
+// loads of a row are issued together, then summed in ascending column order (slot order),
+// which is the CSR order the plan was checked against at set-up.  The value arrays are
+// streamed once (nontemporal loads); x stays in L2.  I = int32_t: the plan requires
+// rows < 2^31.
+template<typename DT, typename VT, class Epi, int DIM>
+__global__ __launch_bounds__(kThreads) void k_band_rows(const VT* __restrict__ val, int n1,
+                                                        const DT* __restrict__ x, uint64_t rows, DT scale,
+                                                        Epi epi, GridReduce gr)
+{
+  constexpr int NS = DIM == 3 ? 27 : 9;
+  double acc0 = 0.0, acc1 = 0.0;
+  epi.begin();
+  const int32_t m = n1, mm = n1 * n1;
+  // the chunk-to-workgroup mapping of the SELL-64 launch, so grid reductions (norms, dots)
+  // take the same partial sums as on the plain plan
+  const uint64_t nch = (rows + kThreads - 1) / kThreads;
+  const XcdRange xr = xcd_range(nch);
+  for(uint64_t ch = xr.first; ch < xr.end; ch += xr.stride)
+  {
+    const uint64_t r = ch * kThreads + threadIdx.x;
+    if(r >= rows) continue;
+    typename EpiPre<Epi>::type pf{};
+    if constexpr(EpiPre<Epi>::on) pf = epi.pre(r);
+    const int32_t i = int32_t(r);
+    const int32_t fx = i % m, fy = DIM == 3 ? (i / m) % m : i / m, fz = DIM == 3 ? i / mm : 0;
+    const bool lx = fx > 0, hx = fx + 1 < m, ly = fy > 0, hy = fy + 1 < m, lz = fz > 0, hz = fz + 1 < m;
+    // Stored values stay raw until the sum.  16-bit values: every load is issued
+    // unconditionally (each slot array spans all rows, and an outside neighbour reads x[i];
+    // both are left out of the sum), since a predicated 16-bit load is compiled into a
+    // branch that waits for it before converting (FP16 plan 1.8x slower).  Wider values
+    // keep predicated loads, which measured faster for them (FP64 solve 70.4 against 78.9 ms).
+    VT v[NS];
+    DT xv[NS];
+#pragma unroll
+    for(int t = 0; t < NS; ++t)
+    {
+      const int dx = t % 3 - 1, dy = (t / 3) % 3 - 1, dz = DIM == 3 ? t / 9 - 1 : 0;
+      const bool on = (dx < 0 ? lx : dx > 0 ? hx : true) && (dy < 0 ? ly : dy > 0 ? hy : true) &&
+                      (dz < 0 ? lz : dz > 0 ? hz : true);
+      if constexpr(sizeof(VT) == 2)
+      {
+        v[t] = __builtin_nontemporal_load(val + uint64_t(t) * rows + r);
+        xv[t] = x[on ? i + dx + dy * m + dz * mm : i];
+      }
+      else
+      {
+        v[t] = on ? __builtin_nontemporal_load(val + uint64_t(t) * rows + r) : VT(0);
+        xv[t] = on ? x[i + dx + dy * m + dz * mm] : DT(0);
+      }
+    }
+    DT sum = DT(0);
+#pragma unroll
+    for(int t = 0; t < NS; ++t)
+    {
+      const int dx = t % 3 - 1, dy = (t / 3) % 3 - 1, dz = DIM == 3 ? t / 9 - 1 : 0;
+      const bool on = (dx < 0 ? lx : dx > 0 ? hx : true) && (dy < 0 ? ly : dy > 0 ? hy : true) &&
+                      (dz < 0 ? lz : dz > 0 ? hz : true);
+      if(on) sum = dfma(cvt_val<DT, VT>(v[t]), xv[t], sum);
+    }
+    if(!std::is_same<VT, DT>::value) sum = sum * scale;  // exact: a power of two
+    if constexpr(EpiPre<Epi>::on)
+      epi.row(r, sum, pf, acc0, acc1);
+    else
+      epi.row(r, sum, acc0, acc1);
+  }
+  grid_reduce_finish(gr, acc0, acc1);
+}
+
+// dimension of a banded plan: 3 where n1^3 = rows, else 2
+__host__ __device__ inline int band_dim(uint64_t rows, uint64_t n1) { return n1 * n1 * n1 == rows ? 3 : 2; }
+
 // Runs Epi over the rows of a descriptor (CSR or SELL-64).
 template<typename DT, class Epi, class M>
 int launch_mat(hipStream_t s, const M* A, const DT* x, const Epi& epi, const GridReduce& gr, const char* name)
@@ -757,6 +840,34 @@ int launch_mat(hipStream_t s, const M* A, const DT* x, const Epi& epi, const Gri
     else
       hipLaunchKernelGGL((k_dlinear_rows<DT, Epi, false, int64_t>), dim3(grid), dim3(kThreads), 0, s, A->index,
                          int(A->slice_words), x, A->rows, epi, gd);
+    finish_reduce(s, gd, uint64_t(grid));
+    return check_launch(name);
+  }
+  if(A->format == FEAT_FMT_BANDED)
+  {
+    const uint64_t nb = (A->rows + kThreads - 1) / kThreads;   // the SELL-64 default grid
+    const int grid = int(nb < uint64_t(kMaxGrid) ? nb : uint64_t(kMaxGrid));
+    const GridReduce gd = reduce_for_grid(gr, uint64_t(grid));
+    const int n1 = int(A->slice_words);
+    const bool d3 = band_dim(A->rows, A->slice_words) == 3;
+    const DT sc = DT(A->scale);
+    if(A->value_kind == FEAT_VAL_F16)
+    {
+      if constexpr(!std::is_same<DT, float>::value)
+        return fail(FEAT_ERR_UNSUPPORTED, "FP16 values need FP32 arithmetic");
+      else
+      {
+        const uint16_t* vh = (const uint16_t*)(const void*)A->val;
+        if(d3)
+          hipLaunchKernelGGL((k_band_rows<DT, uint16_t, Epi, 3>), dim3(grid), dim3(kThreads), 0, s, vh, n1, x, A->rows, sc, epi, gd);
+        else
+          hipLaunchKernelGGL((k_band_rows<DT, uint16_t, Epi, 2>), dim3(grid), dim3(kThreads), 0, s, vh, n1, x, A->rows, sc, epi, gd);
+      }
+    }
+    else if(d3)
+      hipLaunchKernelGGL((k_band_rows<DT, DT, Epi, 3>), dim3(grid), dim3(kThreads), 0, s, A->val, n1, x, A->rows, sc, epi, gd);
+    else
+      hipLaunchKernelGGL((k_band_rows<DT, DT, Epi, 2>), dim3(grid), dim3(kThreads), 0, s, A->val, n1, x, A->rows, sc, epi, gd);
     finish_reduce(s, gd, uint64_t(grid));
     return check_launch(name);
   }
@@ -966,6 +1077,53 @@ __global__ void k_sell_fill(const double* val, const uint64_t* col, const uint64
         sval[p] = DT(0);
         scol[p] = DELTA ? CT(0) : CT(r);
       }
+    }
+  }
+}
+
+// Banded-plan check: 1 in *bad where a row's CSR columns are not exactly its grid
+// neighbours inside the n1^dim grid in ascending order (set-up only)
+__global__ void k_band_check(const uint64_t* col, const uint64_t* rp, uint64_t rows, int dim, int64_t n1, int* bad)
+{
+  for(uint64_t r = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; r < rows; r += uint64_t(gridDim.x) * blockDim.x)
+  {
+    const int64_t i = int64_t(r);
+    const int64_t fx = i % n1, fy = (i / n1) % n1, fz = dim == 3 ? i / (n1 * n1) : 0;
+    uint64_t p = rp[r];
+    const uint64_t e = rp[r + 1];
+    bool ok = true;
+    for(int t = 0; t < 27 && ok; ++t)
+    {
+      const int dx = t % 3 - 1, dy = (t / 3) % 3 - 1, dz = t / 9 - 1;
+      if(dim == 2 && t >= 9) break;
+      const int64_t gz = dim == 3 ? fz + dz : 0;
+      if(fx + dx < 0 || fx + dx >= n1 || fy + dy < 0 || fy + dy >= n1 || gz < 0 || gz >= n1) continue;
+      const int64_t c = i + dx + dy * n1 + (dim == 3 ? dz * n1 * n1 : 0);
+      if(p >= e || col[p] != uint64_t(c))
+        ok = false;
+      else
+        ++p;
+    }
+    if(!ok || p != e) atomicOr(bad, 1);
+  }
+}
+
+// Banded-plan fill: slot t of row r at bval[t * rows + r], 0 where the neighbour is outside
+template<typename DT>
+__global__ void k_band_fill(const double* val, const uint64_t* rp, uint64_t rows, int dim, int64_t n1, DT* bval,
+                            float inv_scale)
+{
+  for(uint64_t r = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; r < rows; r += uint64_t(gridDim.x) * blockDim.x)
+  {
+    const int64_t i = int64_t(r);
+    const int64_t fx = i % n1, fy = (i / n1) % n1, fz = dim == 3 ? i / (n1 * n1) : 0;
+    uint64_t p = rp[r];
+    const int ns = dim == 3 ? 27 : 9;
+    for(int t = 0; t < ns; ++t)
+    {
+      const int dx = t % 3 - 1, dy = (t / 3) % 3 - 1, dz = dim == 3 ? t / 9 - 1 : 0;
+      const bool on = fx + dx >= 0 && fx + dx < n1 && fy + dy >= 0 && fy + dy < n1 && fz + dz >= 0 && fz + dz < n1;
+      bval[uint64_t(t) * rows + r] = on ? st_val<DT>(val[p++], inv_scale) : DT(0);
     }
   }
 }
@@ -2219,6 +2377,87 @@ int dict_create(feat_handle_t h, const feat_csr_f64_t* A, const std::vector<uint
   return FEAT_OK;
 }
 
+// Banded plan of a square CSR whose rows are the clipped 3^dim-point stencils of an n1^dim
+// grid (n1 >= 3; the 3D grid is tried first, then the 2D one).  Returns
+// FEAT_ERR_UNSUPPORTED, without an error record, where the pattern is anything else.
+template<typename DT, typename M>
+int band_create(feat_handle_t h, const feat_csr_f64_t* A, float scale, float inv_scale, M* out)
+{
+  const uint64_t n = A->rows;
+  uint64_t cand[2][2];
+  int nc = 0;
+  const uint64_t r3 = uint64_t(std::llround(std::cbrt(double(n)))), r2 = uint64_t(std::llround(std::sqrt(double(n))));
+  for(uint64_t q = r3 > 0 ? r3 - 1 : 0; q <= r3 + 1; ++q)
+    if(q >= 3 && q * q * q == n && A->nnz == (3 * q - 2) * (3 * q - 2) * (3 * q - 2))
+    {
+      cand[nc][0] = 3;
+      cand[nc++][1] = q;
+      break;
+    }
+  for(uint64_t q = r2 > 0 ? r2 - 1 : 0; q <= r2 + 1; ++q)
+    if(q >= 3 && q * q == n && A->nnz == (3 * q - 2) * (3 * q - 2))
+    {
+      cand[nc][0] = 2;
+      cand[nc++][1] = q;
+      break;
+    }
+  if(!nc) return FEAT_ERR_UNSUPPORTED;
+  int* bad = nullptr;
+  int rc = check_hip(hipMalloc(&bad, sizeof(int)), "banded plan flag");
+  int dim = 0;
+  uint64_t n1 = 0;
+  for(int c = 0; c < nc && !rc && !dim; ++c)
+  {
+    int hb = 1;
+    rc = check_hip(hipMemsetAsync(bad, 0, sizeof(int), h->stream), "memset");
+    if(!rc)
+    {
+      hipLaunchKernelGGL(k_band_check, dim3(grid_for(n)), dim3(kThreads), 0, h->stream, A->col, A->rowptr, n,
+                         int(cand[c][0]), int64_t(cand[c][1]), bad);
+      rc = check_launch("banded plan check");
+    }
+    if(!rc) rc = check_hip(hipMemcpyAsync(&hb, bad, sizeof(int), hipMemcpyDeviceToHost, h->stream), "copy");
+    if(!rc) rc = check_hip(hipStreamSynchronize(h->stream), "sync");
+    if(!rc && !hb)
+    {
+      dim = int(cand[c][0]);
+      n1 = cand[c][1];
+    }
+  }
+  if(bad) (void)hipFree(bad);
+  if(rc) return rc;
+  if(!dim) return FEAT_ERR_UNSUPPORTED;
+  const uint64_t ns = dim == 3 ? 27 : 9;
+  DT* bval = nullptr;
+  rc = check_hip(hipMalloc(&bval, sizeof(DT) * ns * n), "banded plan values");
+  if(!rc)
+  {
+    hipLaunchKernelGGL((k_band_fill<DT>), dim3(grid_for(n)), dim3(kThreads), 0, h->stream, A->val, A->rowptr, n, dim,
+                       int64_t(n1), bval, inv_scale);
+    rc = check_launch("banded plan fill");
+  }
+  if(!rc) rc = check_hip(hipStreamSynchronize(h->stream), "sync");
+  if(rc)
+  {
+    if(bval) (void)hipFree(bval);
+    return rc;
+  }
+  out->format = FEAT_FMT_BANDED;
+  out->index = 0;
+  out->rows = n;
+  out->cols = n;
+  out->nnz = ns * n;
+  out->val = (decltype(out->val))(void*)bval;
+  out->col = nullptr;
+  out->ptr = nullptr;
+  out->rowlen = nullptr;
+  out->value_kind = std::is_same<DT, uint16_t>::value ? FEAT_VAL_F16 : FEAT_VAL_NATIVE;
+  out->scale = scale;
+  out->tune = 0;
+  out->slice_words = uint32_t(n1);
+  return FEAT_OK;
+}
+
 template<typename DT, typename M>
 int sell_create_impl(feat_handle_t h, const feat_csr_f64_t* A, M* out, const char* name, int flags = 0)
 {
@@ -2351,6 +2590,23 @@ int sell_create_impl(feat_handle_t h, const feat_csr_f64_t* A, M* out, const cha
     }
     if(stop) vkeys.clear();
   }
+  // no pair dictionary and no value codes: the banded plan where the rows are grid stencils
+  if(vkeys.empty() && !(flags & (FEAT_SELL_NO_DICT | FEAT_SELL_NO_BANDED)) && n >= 27 && n == A->cols &&
+     n < (uint64_t(1) << 31) && A->nnz)
+  {
+    rc = band_create<DT>(h, A, scale, inv_scale, out);
+    if(rc == FEAT_OK)
+    {
+      cleanup();
+      return FEAT_OK;
+    }
+    if(rc != FEAT_ERR_UNSUPPORTED)
+    {
+      cleanup();
+      return rc;
+    }
+    rc = FEAT_OK;
+  }
   bool vcode = !vkeys.empty();
   uint64_t* dkeys = nullptr;
   const size_t vbytes = vcode ? sizeof(DT) * 256 + (total ? total : 1) : sizeof(DT) * (total ? total : 1);
@@ -2432,7 +2688,8 @@ template<typename M>
 int sell_destroy_impl(M* m)
 {
   if(!m) return FEAT_OK;
-  if(m->format != FEAT_FMT_SELL) return fail(FEAT_ERR_INVALID, "feat_sell_destroy: not a SELL plan");
+  if(m->format != FEAT_FMT_SELL && m->format != FEAT_FMT_BANDED)
+    return fail(FEAT_ERR_INVALID, "feat_sell_destroy: not a SELL-64 or banded plan");
   if(m->val) (void)hipFree(const_cast<void*>(static_cast<const void*>(m->val)));
   if(m->col) (void)hipFree(const_cast<void*>(m->col));
   if(m->ptr) (void)hipFree(const_cast<void*>(m->ptr));
@@ -2450,6 +2707,13 @@ bool mat_ok(const M* A)
     const uint64_t nf = A->index == 3 ? n1 * n1 * n1 : n1 * n1, ncn = A->index == 3 ? nc * nc * nc : nc * nc;
     return (A->index == 2 || A->index == 3) && n1 >= 3 && (n1 & 1) && (A->value_kind == 0 || A->value_kind == 1) &&
            A->rows == (A->value_kind ? ncn : nf) && A->cols == (A->value_kind ? nf : ncn) && nf <= (uint64_t(1) << 62);
+  }
+  if(A && A->format == FEAT_FMT_BANDED)
+  {
+    const uint64_t q = A->slice_words;
+    return A->val && q >= 3 && A->rows == A->cols && A->rows < (uint64_t(1) << 31) &&
+           (q * q * q == A->rows || q * q == A->rows) &&
+           (A->value_kind == FEAT_VAL_NATIVE || A->value_kind == FEAT_VAL_F16);
   }
   return A && A->ptr && (A->nnz == 0 || (A->val && A->col)) && (A->format != FEAT_FMT_SELL || A->rowlen);
 }
@@ -2607,6 +2871,11 @@ int feat_sell_create_ex_f32(feat_handle_t h, const feat_csr_f64_t* A, int flags,
 int feat_sell_create_f16(feat_handle_t h, const feat_csr_f64_t* A, feat_matrix_f32_t* out)
 {
   return sell_create_impl<uint16_t>(h, A, out, "feat_sell_create_f16");
+}
+
+int feat_sell_create_ex_f16(feat_handle_t h, const feat_csr_f64_t* A, int flags, feat_matrix_f32_t* out)
+{
+  return sell_create_impl<uint16_t>(h, A, out, "feat_sell_create_ex_f16", flags);
 }
 
 int feat_sell_destroy_f64(feat_matrix_f64_t* m) { return sell_destroy_impl(m); }

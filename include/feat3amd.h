@@ -235,6 +235,13 @@ typedef struct {            /* a CSR matrix on the device (u64 indices) */
 #define FEAT_FMT_DLINEAR 2   /* no arrays: d-linear injection between structured grids,
                               * see feat_matrix_dlinear_f64 (index = dim, slice_words =
                               * n1, value_kind = transpose)                          */
+#define FEAT_FMT_BANDED 3    /* the clipped 3^dim-point stencil rows of a structured grid
+                              * (n1 nodes per side, lexicographic, x fastest; rows = cols =
+                              * n1^dim): columns formed from the node position, values
+                              * slot-major (val[t * rows + i], slot t = (dx+1) + 3 (dy+1) +
+                              * 9 (dz+1), in ascending column order; 0 where the neighbour
+                              * lies outside the grid); slice_words = n1, the dimension is 3
+                              * where n1^3 = rows, else 2; col, ptr, rowlen unused        */
 #define FEAT_IDX_U64 0   /* CSR: u64 col/rowptr                              */
 #define FEAT_IDX_U32 1   /* CSR: u32 col/rowptr; SELL: u32 absolute columns  */
 #define FEAT_IDX_D16 2   /* SELL: int16 column minus row                      */
@@ -310,13 +317,21 @@ int feat_sell_create_f64(feat_handle_t h, const feat_csr_f64_t* A, feat_matrix_f
  * where the pair dictionary or the value codes apply. */
 #define FEAT_SELL_NO_DICT 1
 #define FEAT_SELL_NO_DIAG 2   /* keep the entry-order dictionary (FEAT_IDX_DICT) */
+/* A square matrix without a pair dictionary or value codes whose every row holds exactly the
+ * grid neighbours (i+dx, j+dy, k+dz), |d| <= 1, inside an n1^dim grid, in ascending order
+ * (Q1 matrices with variable coefficients) gets the banded plan (FEAT_FMT_BANDED: 8, 4 or
+ * 2 bytes per entry in FP64, FP32 or FP16 values against 10, 6 or 4 in the plain plan;
+ * the same row sums, bit for bit).  FEAT_SELL_NO_BANDED keeps the plain plan. */
+#define FEAT_SELL_NO_BANDED 4
 int feat_sell_create_ex_f64(feat_handle_t h, const feat_csr_f64_t* A, int flags, feat_matrix_f64_t* out);
 int feat_sell_create_ex_f32(feat_handle_t h, const feat_csr_f64_t* A, int flags, feat_matrix_f32_t* out);
+int feat_sell_create_ex_f16(feat_handle_t h, const feat_csr_f64_t* A, int flags, feat_matrix_f32_t* out);
 /* Same plan with values rounded to FP32 (ConvertPrecond, convert_precond.hpp:100-119). */
 int feat_sell_create_f32(feat_handle_t h, const feat_csr_f64_t* A, feat_matrix_f32_t* out);
 /* Same plan with FP16 values: v16 = half(float(a) * 2^-e) with 2^(e-1) <= max|a| < 2^e;
  * row sums sum(fmaf(float(v16), x, s)) are multiplied by scale = 2^e. */
 int feat_sell_create_f16(feat_handle_t h, const feat_csr_f64_t* A, feat_matrix_f32_t* out);
+/* Frees a plan made by feat_sell_create_* (SELL-64 or banded). */
 int feat_sell_destroy_f64(feat_matrix_f64_t* m);
 int feat_sell_destroy_f32(feat_matrix_f32_t* m);
 

@@ -341,7 +341,7 @@ namespace feat3amd
         if(!_plan)
         {
           auto p = std::shared_ptr<feat_matrix_f64_t>(new feat_matrix_f64_t(), [](feat_matrix_f64_t* m) {
-            if(m->format == FEAT_FMT_SELL) feat_sell_destroy_f64(m);
+            if(m->format == FEAT_FMT_SELL || m->format == FEAT_FMT_BANDED) feat_sell_destroy_f64(m);
             delete m;
           });
           const feat_csr_f64_t c = desc();
@@ -375,7 +375,7 @@ namespace feat3amd
         if(!_plan32)
         {
           auto p = std::shared_ptr<feat_matrix_f32_t>(new feat_matrix_f32_t(), [](feat_matrix_f32_t* m) {
-            if(m->format == FEAT_FMT_SELL) feat_sell_destroy_f32(m);
+            if(m->format == FEAT_FMT_SELL || m->format == FEAT_FMT_BANDED) feat_sell_destroy_f32(m);
             delete m;
           });
           const feat_csr_f64_t c = desc();
@@ -408,11 +408,11 @@ namespace feat3amd
         if(!_plan16)
         {
           auto p = std::shared_ptr<feat_matrix_f32_t>(new feat_matrix_f32_t(), [](feat_matrix_f32_t* m) {
-            if(m->format == FEAT_FMT_SELL) feat_sell_destroy_f32(m);
+            if(m->format == FEAT_FMT_SELL || m->format == FEAT_FMT_BANDED) feat_sell_destroy_f32(m);
             delete m;
           });
           const feat_csr_f64_t c = desc();
-          if(feat_sell_create_f16(Runtime::handle(), &c, p.get()) != FEAT_OK)
+          if(feat_sell_create_ex_f16(Runtime::handle(), &c, Runtime::sell_flags(), p.get()) != FEAT_OK)
           {
             _plan16 = std::shared_ptr<feat_matrix_f32_t>(std::shared_ptr<feat_matrix_f32_t>(), const_cast<feat_matrix_f32_t*>(&exec32()));
             return *_plan16;
