@@ -100,6 +100,7 @@ KERNEL_API = [
     ("feat_host_malloc", _i, [C.POINTER(_vp), C.c_size_t]),
     ("feat_host_free", _i, [_vp]),
     ("feat_copy", _i, [_vp, _vp, _vp, C.c_size_t]),
+    ("feat_copy_device", _i, [_vp, _vp, _vp, C.c_size_t]),
     ("feat_stream_read_f64", _i, [_vp, _vp, _u64, _vp]),
     ("feat_set_memory_f64", _i, [_vp, _vp, _d, _u64]),
     ("feat_set_memory_f32", _i, [_vp, _vp, _f, _u64]),

@@ -747,8 +747,8 @@ __global__ __launch_bounds__(kThreads) void k_dlinear_rows(int dim, int n1, cons
 // k+dz) inside the grid, columns formed from the node position, values slot-major.  All
 // loads of a row are issued together, then summed in ascending column order (slot order),
 // which is the CSR order the plan was checked against at set-up.  The value arrays are
-// streamed once (nontemporal loads); x stays in L2.  I = int32_t: the plan requires
-// rows < 2^31.
+// streamed once (nontemporal loads); x stays in L2.  Indices and byte offsets are 32-bit:
+// the plan requires 27 * rows * 8 B < 4 GiB (fewer than 19.8 M rows).
 template<typename DT, typename VT, class Epi, int DIM>
 __global__ __launch_bounds__(kThreads) void k_band_rows(const VT* __restrict__ val, int n1,
                                                         const DT* __restrict__ x, uint64_t rows, DT scale,
@@ -771,6 +771,7 @@ __global__ __launch_bounds__(kThreads) void k_band_rows(const VT* __restrict__ v
     const int32_t i = int32_t(r);
     const int32_t fx = i % m, fy = DIM == 3 ? (i / m) % m : i / m, fz = DIM == 3 ? i / mm : 0;
     const bool lx = fx > 0, hx = fx + 1 < m, ly = fy > 0, hy = fy + 1 < m, lz = fz > 0, hz = fz + 1 < m;
+    const uint32_t ro = uint32_t(i) * uint32_t(sizeof(VT)), st = uint32_t(rows) * uint32_t(sizeof(VT));
     // Stored values stay raw until the sum.  16-bit values: every load is issued
     // unconditionally (each slot array spans all rows, and an outside neighbour reads x[i];
     // both are left out of the sum), since a predicated 16-bit load is compiled into a
@@ -784,15 +785,18 @@ __global__ __launch_bounds__(kThreads) void k_band_rows(const VT* __restrict__ v
       const int dx = t % 3 - 1, dy = (t / 3) % 3 - 1, dz = DIM == 3 ? t / 9 - 1 : 0;
       const bool on = (dx < 0 ? lx : dx > 0 ? hx : true) && (dy < 0 ? ly : dy > 0 ? hy : true) &&
                       (dz < 0 ? lz : dz > 0 ? hz : true);
+      // 32-bit byte offsets from one base per array (the plan keeps its values below 4 GiB):
+      // one offset register per load instead of a 64-bit address
+      const VT* vp = (const VT*)((const char*)val + (ro + uint32_t(t) * st));
       if constexpr(sizeof(VT) == 2)
       {
-        v[t] = __builtin_nontemporal_load(val + uint64_t(t) * rows + r);
-        xv[t] = x[on ? i + dx + dy * m + dz * mm : i];
+        v[t] = __builtin_nontemporal_load(vp);
+        xv[t] = *(const DT*)((const char*)x + uint32_t(on ? i + dx + dy * m + dz * mm : i) * uint32_t(sizeof(DT)));
       }
       else
       {
-        v[t] = on ? __builtin_nontemporal_load(val + uint64_t(t) * rows + r) : VT(0);
-        xv[t] = on ? x[i + dx + dy * m + dz * mm] : DT(0);
+        v[t] = on ? __builtin_nontemporal_load(vp) : VT(0);
+        xv[t] = on ? *(const DT*)((const char*)x + uint32_t(i + dx + dy * m + dz * mm) * uint32_t(sizeof(DT))) : DT(0);
       }
     }
     DT sum = DT(0);
@@ -1079,6 +1083,14 @@ __global__ void k_sell_fill(const double* val, const uint64_t* col, const uint64
       }
     }
   }
+}
+
+// device-to-device copy, 16 bytes per lane (the runtime's copy kernel moves vectors at
+// about 1.4 TB/s)
+__global__ __launch_bounds__(kThreads) void k_copy16(const uint4* __restrict__ src, uint4* __restrict__ dst, uint64_t n)
+{
+  for(uint64_t i = blockIdx.x * uint64_t(kThreads) + threadIdx.x; i < n; i += uint64_t(gridDim.x) * kThreads)
+    dst[i] = src[i];
 }
 
 // Banded-plan check: 1 in *bad where a row's CSR columns are not exactly its grid
@@ -1780,6 +1792,19 @@ int feat_host_malloc(void** p, size_t bytes)
 }
 
 int feat_host_free(void* p) { return p ? check_hip(hipHostFree(p), "hipHostFree") : FEAT_OK; }
+
+int feat_copy_device(feat_handle_t h, void* dst, const void* src, size_t bytes)
+{
+  if(!h || (bytes && (!dst || !src))) return fail(FEAT_ERR_INVALID, "feat_copy_device: null argument");
+  if(!bytes || dst == src) return FEAT_OK;
+  if(((uintptr_t(dst) | uintptr_t(src) | bytes) & 15) != 0)
+    return check_hip(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, h->stream), "hipMemcpyAsync");
+  const uint64_t n16 = bytes / 16;
+  const uint64_t nb = (n16 + kThreads - 1) / kThreads;
+  const int grid = int(nb < uint64_t(kMaxGrid) ? nb : uint64_t(kMaxGrid));
+  hipLaunchKernelGGL(k_copy16, dim3(grid), dim3(kThreads), 0, h->stream, (const uint4*)src, (uint4*)dst, n16);
+  return check_launch("feat_copy_device");
+}
 
 int feat_copy(feat_handle_t h, void* dst, const void* src, size_t bytes)
 {
@@ -2592,7 +2617,7 @@ int sell_create_impl(feat_handle_t h, const feat_csr_f64_t* A, M* out, const cha
   }
   // no pair dictionary and no value codes: the banded plan where the rows are grid stencils
   if(vkeys.empty() && !(flags & (FEAT_SELL_NO_DICT | FEAT_SELL_NO_BANDED)) && n >= 27 && n == A->cols &&
-     n < (uint64_t(1) << 31) && A->nnz)
+     27 * n * sizeof(double) < (uint64_t(1) << 32) && A->nnz)
   {
     rc = band_create<DT>(h, A, scale, inv_scale, out);
     if(rc == FEAT_OK)
@@ -2711,7 +2736,7 @@ bool mat_ok(const M* A)
   if(A && A->format == FEAT_FMT_BANDED)
   {
     const uint64_t q = A->slice_words;
-    return A->val && q >= 3 && A->rows == A->cols && A->rows < (uint64_t(1) << 31) &&
+    return A->val && q >= 3 && A->rows == A->cols && 27 * A->rows * sizeof(double) < (uint64_t(1) << 32) &&
            (q * q * q == A->rows || q * q == A->rows) &&
            (A->value_kind == FEAT_VAL_NATIVE || A->value_kind == FEAT_VAL_F16);
   }

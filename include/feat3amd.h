@@ -76,6 +76,10 @@ int feat_host_malloc(void** p, size_t bytes); /* pinned host memory */
 int feat_host_free(void* p);
 /* MemoryPool::copy / set_memory (util/memory_pool.hpp:181-251) */
 int feat_copy(feat_handle_t h, void* dst, const void* src, size_t bytes);      /* any direction */
+/* Device-to-device copy by a kernel (16 bytes per lane where both pointers and the size
+ * are 16-byte aligned, else hipMemcpyAsync); both buffers must be device memory.
+ * DenseVector::copy (dense_vector.hpp) on device containers. */
+int feat_copy_device(feat_handle_t h, void* dst, const void* src, size_t bytes);
 /* Bandwidth probe (no reference counterpart): reads the n doubles of x once with 16-byte
  * nontemporal loads and writes nothing (sink is a device scalar that is never written in
  * practice).  n even, x 16-byte aligned.  The measured read ceiling for roofline fractions. */
@@ -321,7 +325,8 @@ int feat_sell_create_f64(feat_handle_t h, const feat_csr_f64_t* A, feat_matrix_f
  * grid neighbours (i+dx, j+dy, k+dz), |d| <= 1, inside an n1^dim grid, in ascending order
  * (Q1 matrices with variable coefficients) gets the banded plan (FEAT_FMT_BANDED: 8, 4 or
  * 2 bytes per entry in FP64, FP32 or FP16 values against 10, 6 or 4 in the plain plan;
- * the same row sums, bit for bit).  FEAT_SELL_NO_BANDED keeps the plain plan. */
+ * the same row sums, bit for bit; up to 19.8 M rows).  FEAT_SELL_NO_BANDED keeps the
+ * plain plan. */
 #define FEAT_SELL_NO_BANDED 4
 int feat_sell_create_ex_f64(feat_handle_t h, const feat_csr_f64_t* A, int flags, feat_matrix_f64_t* out);
 int feat_sell_create_ex_f32(feat_handle_t h, const feat_csr_f64_t* A, int flags, feat_matrix_f32_t* out);

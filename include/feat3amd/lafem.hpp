@@ -199,7 +199,7 @@ namespace feat3amd
       void copy(const DenseVector& x)
       {
         if(x.size() != size()) throw Error("DenseVector::copy: size mismatch");
-        check(feat_copy(Runtime::handle(), elements(), x.elements(), sizeof(DT_) * size()), "copy");
+        check(feat_copy_device(Runtime::handle(), elements(), x.elements(), sizeof(DT_) * size()), "copy");
       }
 
       /// this = alpha*x + y (dense_vector.hpp:807-830)

@@ -296,3 +296,17 @@ def test_diag_filters_convert(feat, handle, torch_gpu, oracle):
     feat.check(k.feat_convert_f32_f64(handle.h, P(g), P(f), 3000), "cvt")
     handle.sync()
     assert np.array_equal(host(g), x.astype(np.float32).astype(np.float64))
+
+
+@pytest.mark.parametrize("nbytes,off", [(16, 0), (8 * 1000003, 0), (4096 * 16 * 300 + 16, 0), (1000, 0), (4096, 4)])
+def test_copy_device(feat, handle, torch_gpu, nbytes, off):
+    # the kernel copy (16-byte aligned pointers and size) and its hipMemcpyAsync fallback
+    torch = torch_gpu
+    k, _ = feat.libs()
+    src = torch.randint(0, 256, (nbytes + 64,), dtype=torch.uint8, device="cuda")
+    dst = torch.zeros(nbytes + 64, dtype=torch.uint8, device="cuda")
+    feat.check(k.feat_copy_device(handle.h, dst.data_ptr() + off, src.data_ptr() + off, nbytes), "copy")
+    handle.sync()
+    s, d = src.cpu().numpy(), dst.cpu().numpy()
+    assert np.array_equal(d[off:off + nbytes], s[off:off + nbytes])
+    assert not d[:off].any() and not d[off + nbytes:].any()
