@@ -70,8 +70,10 @@ namespace feat3amd
             std::vector<float> s32(_grid[s].sten.begin(), _grid[s].sten.end());
             _grid_sten[s] = DeviceArray<float>::from_host(s32.data(), s32.size());
           }
-          // an FP16-valued cycle keeps its FP16 transfer plans
-          if(_grid[s].dlinear && !_half)
+          // the FP16-valued cycle too: its transfer plans store the d-linear weights w as w/2
+          // with the exact scale 2, and power-of-two scaling commutes with rounding, so the
+          // matrix-free rows give its row sums (outside the subnormal range)
+          if(_grid[s].dlinear)
           {
             check(feat_matrix_dlinear_f32(_grid[s].dim, _grid[s].n1, 0, &_dl_P[s]), "MixedMultiGrid d-linear prolongation");
             check(feat_matrix_dlinear_f32(_grid[s].dim, _grid[s].n1, 1, &_dl_R[s]), "MixedMultiGrid d-linear restriction");
@@ -81,11 +83,11 @@ namespace feat3amd
       /// the FP32 transfer descriptors of level i: d-linear where checked, else the plans
       const feat_matrix_f32_t& _rest(Index i) const
       {
-        return (i < _grid.size() && _grid[i].dlinear && !_half) ? _dl_R[i] : plan(_h->levels[i].transfer->get_mat_rest());
+        return (i < _grid.size() && _grid[i].dlinear) ? _dl_R[i] : plan(_h->levels[i].transfer->get_mat_rest());
       }
       const feat_matrix_f32_t& _prol(Index i) const
       {
-        return (i < _grid.size() && _grid[i].dlinear && !_half) ? _dl_P[i] : plan(_h->levels[i].transfer->get_mat_prol());
+        return (i < _grid.size() && _grid[i].dlinear) ? _dl_P[i] : plan(_h->levels[i].transfer->get_mat_prol());
       }
 
       // execution plan of a level's matrix: FP32 values or FP16 values (FP32 arithmetic)

@@ -125,3 +125,30 @@ def test_banded_mgpcg_bitwise(feat, torch_gpu, prec):
     assert res[0][2] == res[1][2] == "success"
     assert res[0][1] == res[1][1]
     assert np.array_equal(res[0][0], res[1][0])
+
+
+@pytest.mark.parametrize("dim,lf", [(2, 7), (3, 5)])
+def test_f16_cycle_dlinear_transfers_bitwise(feat, torch_gpu, dim, lf):
+    # the FP16-valued V-cycle on matrix-free d-linear transfers (grid forms on) and on its
+    # FP16 transfer plans (off): the same iterates
+    from tests.helpers import q1_levels
+    torch = torch_gpu
+    levels = q1_levels(feat, dim, lf, 1)
+    n = levels[0]["n"]
+    bh = feat.rhs_random(28054777172512, n)
+    bh[levels[0]["filt"].astype(np.int64)] = 0.0
+    b = dev(torch, bh)
+    res = []
+    try:
+        for grid in (True, False):
+            feat.runtime_grid_forms(grid)
+            s = feat.MgPcg(feat.mgpcg_cfg(coarse="direct", precision="f16"), levels=levels)
+            x = torch.zeros(n, dtype=torch.float64, device="cuda")
+            r = s.solve(x, b)
+            s.close()
+            res.append((host(x), r["iters"], r["status"]))
+    finally:
+        feat.runtime_grid_forms(True)
+    assert res[0][2] == res[1][2] == "success"
+    assert res[0][1] == res[1][1]
+    assert np.array_equal(res[0][0], res[1][0])
